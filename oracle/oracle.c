@@ -1,0 +1,1320 @@
+/*
+ * CPU oracle: scalar C restatement of viprs's resample + separable-convolution
+ * path, pinned to the reference's x86-64 + AVX2 dispatch semantics.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or, for bench.py, as the timed CPU baseline).  The product library never
+ * links or calls it.
+ *
+ * Reference files restated (paths relative to the viprs repository root):
+ *   crates/viprs-core/src/kernel.rs:50-221                     kernel weights
+ *   crates/viprs-ops-spatial/src/resample/reduce_common.rs     tables, plans, regions
+ *   crates/viprs-core/src/resample.rs:62-141                   ReduceConfig
+ *   crates/viprs-ops-spatial/src/resample/reduceh.rs:101-163   ReduceH::process_region
+ *   crates/viprs-ops-spatial/src/resample/reducev.rs:100-162   ReduceV::process_region
+ *   crates/viprs-ops-spatial/src/resample/reduce_simd.rs       scalar + AVX2 kernels
+ *   crates/viprs-ops-spatial/src/resample/sample_conv.rs       fixed-point rounding
+ *   crates/viprs-ops-spatial/src/resample/shrinkh.rs:220-471   ShrinkH (x86 dispatch)
+ *   crates/viprs-ops-spatial/src/resample/shrinkv.rs:122-339   ShrinkV
+ *   crates/viprs-ops-spatial/src/resample/thumbnail.rs:229-524 Thumbnail planner
+ *   crates/viprs-ops-spatial/src/resample/resize.rs:163-345    Resize planner (downscale)
+ *   crates/viprs-ops-spatial/src/convolution/gauss_blur.rs     integer/float Gaussian
+ *   crates/viprs-core/src/shared_ops/gauss_kernel.rs           float Gaussian weights
+ *   crates/viprs-ops-spatial/src/convolution/convsep.rs        ConvSepH/V
+ *   crates/viprs-ops-spatial/src/structural/{premultiply,unpremultiply}.rs
+ *   crates/viprs-runtime/src/scheduler/rayon_scheduler/execute.rs:413-849 tile executor
+ *   crates/viprs-runtime/src/scheduler/rayon_scheduler/planning.rs:509-601 tiles
+ *   crates/viprs-runtime/src/sources/memory.rs:149-235         clamp-to-edge source read
+ *
+ * Parity pins (tests/test_oracle_golden.py):
+ *   reduce tables, ReduceH/ReduceV u8  -> libvips fixtures in reduceh_libvips,
+ *       reducev_libvips, libvips_parity_resample (test_reduce{h,v}), in-crate
+ *       known answers (reduceh.rs:574-610, reducev.rs:510-570), invariants
+ *       (f64 phase sums = 1, i16 sums within +-taps of 4096, offset -0.5 at f=2)
+ *   ShrinkH/ShrinkV u8                  -> shrinkh/, shrinkv/ fixtures,
+ *       shrink_libvips/rgb_factor4, shrinkh.rs:1479-1506,1784-1901 known answers
+ *   Gaussian integer masks              -> gauss_blur.rs:56-65 sigma=1.5 constants,
+ *       brute-force 2-D convolution on small images
+ *   Gaussian float                      -> gauss_blur/uniform_sigma_1_5.bin (tol 1e-3)
+ *   Premultiply/unpremultiply           -> unpremultiply.rs:239-248 known answer,
+ *       exhaustive closed-form check over all 65536 (colour, alpha) pairs
+ *   Thumbnail/resize planners           -> thumbnail.rs:531-682 / resize.rs:392-536
+ *       shape tests restated in the test file
+ *   Chain executor                      -> strip-height invariance, single-op
+ *       equality with clamp-padded closed forms
+ * parity unpinned: the float reduce summation order (reduce_simd.rs:2621-2694,
+ *   2824-2888) has no golden vector in the reference; it is restated from the
+ *   AVX2 source and only checked against an FP64 closed form within tolerance.
+ * parity unpinned: ConvSep (convsep.rs:87-107) has only a vips-gated +-1 ULP
+ *   fixture; checked against an FP64 closed form within tolerance.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define REDUCE_PHASES 64
+#define MAX_REDUCE_TAPS 2000
+#define MAX_REDUCE_TAP_RADIUS ((MAX_REDUCE_TAPS - 1) / 2)
+#define PLAN_EPS 1e-9
+
+/* ------------------------------------------------------------------ helpers */
+
+/* Rust `f64 as u32` / `as i16`: saturating truncation, NaN -> 0. */
+static uint32_t sat_u32(double v) {
+    if (!(v == v)) return 0;
+    if (v <= 0.0) return 0;
+    if (v >= 4294967295.0) return 4294967295u;
+    return (uint32_t)v;
+}
+static int16_t sat_i16(double v) {
+    if (!(v == v)) return 0;
+    if (v <= -32768.0) return -32768;
+    if (v >= 32767.0) return 32767;
+    return (int16_t)v;
+}
+static int64_t sat_i64(double v) {
+    if (!(v == v)) return 0;
+    if (v <= -9.2233720368547758e18) return INT64_MIN;
+    if (v >= 9.2233720368547758e18) return INT64_MAX;
+    return (int64_t)v;
+}
+static double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+static int64_t clampi(int64_t v, int64_t lo, int64_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+typedef struct {
+    int64_t x, y;
+    int64_t w, h;
+} rgn;
+
+/* ------------------------------------------------------------- kernel.rs */
+
+/* InterpolationKernel::support (kernel.rs:66-75). */
+static double kernel_support(int k) {
+    switch (k) {
+    case ORC_NEAREST: return 0.5;
+    case ORC_BILINEAR: return 1.0;
+    case ORC_LANCZOS3: return 3.0;
+    default: return 2.0; /* Bicubic, CatmullRom, Lanczos2 */
+    }
+}
+
+/* InterpolationKernel::interpolate (kernel.rs:133-211). */
+double orc_kernel_weight(int k, double x) {
+    switch (k) {
+    case ORC_NEAREST: return fabs(x) < 0.5 ? 1.0 : 0.0;
+    case ORC_BILINEAR: {
+        double a = fabs(x);
+        return a < 1.0 ? 1.0 - a : 0.0;
+    }
+    case ORC_BICUBIC:
+    case ORC_CATMULLROM: {
+        double a = fabs(x);
+        if (a < 1.0) return fma(2.5 * a, -a, 1.5 * a * a * a) + 1.0;
+        if (a < 2.0) return fma(4.0, -a, fma(2.5 * a, a, -0.5 * a * a * a)) + 2.0;
+        return 0.0;
+    }
+    case ORC_LANCZOS2: {
+        double a = fabs(x);
+        if (a < DBL_EPSILON) return 1.0;
+        if (a < 2.0) {
+            double px = M_PI * a;
+            return 2.0 * sin(px) * sin(px / 2.0) / (px * px);
+        }
+        return 0.0;
+    }
+    case ORC_LANCZOS3: {
+        double a = fabs(x);
+        if (a < DBL_EPSILON) return 1.0;
+        if (a < 3.0) {
+            double px = M_PI * a;
+            return 3.0 * sin(px) * sin(px / 3.0) / (px * px);
+        }
+        return 0.0;
+    }
+    default: return 0.0;
+    }
+}
+
+/* ------------------------------------------------------ reduce_common.rs */
+
+/* validate_reduce_kernel (reduce_common.rs:94-124). */
+static int valid_reduce_kernel(int k) {
+    return k == ORC_NEAREST || k == ORC_BILINEAR || k == ORC_BICUBIC || k == ORC_LANCZOS2 ||
+           k == ORC_LANCZOS3 || k == ORC_CATMULLROM;
+}
+
+/* tap_count (reduce_common.rs:257-277). Returns -1 for "None". */
+int orc_reduce_taps(int kernel, double factor) {
+    if (fabs(factor - 1.0) < DBL_EPSILON) return 1;
+    if (kernel == ORC_NEAREST) return 1;
+    double r = round(kernel_support(kernel) * factor);
+    if (!isfinite(r) || r < 0.0 || r > (double)MAX_REDUCE_TAP_RADIUS) return -1;
+    long taps = (long)r * 2 + 1;
+    return taps <= MAX_REDUCE_TAPS ? (int)taps : -1;
+}
+
+/* build_coefficients + quantize_coefficients (reduce_common.rs:279-316), all
+ * 65 phases (ReduceKernel::new, reduce_common.rs:137-164). */
+int orc_reduce_tables(int kernel, double factor, double *cf, int16_t *ci) {
+    if (!isfinite(factor) || factor < 1.0) return ORC_EINVAL;
+    if (!valid_reduce_kernel(kernel)) return ORC_EKERNEL;
+    int taps = orc_reduce_taps(kernel, factor);
+    if (taps < 0) return ORC_EINVAL;
+    for (int p = 0; p <= REDUCE_PHASES; ++p) {
+        double *row = cf + (size_t)p * taps;
+        if (kernel == ORC_NEAREST) {
+            row[0] = 1.0;
+        } else {
+            double phase = (double)p / (double)REDUCE_PHASES;
+            double half = phase + (double)taps / 2.0 - 1.0;
+            double scale = 1.0 / factor;
+            double sum = 0.0;
+            for (int t = 0; t < taps; ++t) {
+                double d = ((double)t - half) * scale;
+                double wt = orc_kernel_weight(kernel, fabs(d));
+                row[t] = wt;
+                sum += wt;
+            }
+            if (sum > 0.0)
+                for (int t = 0; t < taps; ++t) row[t] /= sum;
+        }
+        for (int t = 0; t < taps; ++t) ci[(size_t)p * taps + t] = sat_i16(row[t] * 4096.0);
+    }
+    return taps;
+}
+
+/* ReduceConfig::output_len (crates/viprs-core/src/resample.rs:86-93). */
+uint32_t orc_reduce_output_len(double factor, uint32_t in) {
+    if (in == 0) return 0;
+    return sat_u32(round((double)in / factor));
+}
+
+/* ReduceKernel::bind_input_len (reduce_common.rs:171-180). */
+double orc_reduce_offset(int kernel, double factor, uint32_t in) {
+    (void)kernel;
+    if (in == 0 || fabs(factor - 1.0) < DBL_EPSILON) return 0.0;
+    uint32_t out = orc_reduce_output_len(factor, in);
+    double extra = fma((double)out, factor, -(double)in);
+    return (1.0 + extra) / 2.0 - 1.0;
+}
+
+typedef struct {
+    int kernel;
+    double factor;
+    int taps;
+    int64_t pad;
+    double offset;
+    double *cf;
+    int16_t *ci;
+} rk_t;
+
+static int rk_init(rk_t *k, int kernel, double factor, uint32_t input_len) {
+    memset(k, 0, sizeof(*k));
+    if (!isfinite(factor) || factor < 1.0) return ORC_EINVAL;
+    if (!valid_reduce_kernel(kernel)) return ORC_EKERNEL;
+    int taps = orc_reduce_taps(kernel, factor);
+    if (taps < 0) return ORC_EINVAL;
+    k->kernel = kernel;
+    k->factor = factor;
+    k->taps = taps;
+    k->pad = (int64_t)ceil((double)taps / 2.0) - 1;
+    k->cf = (double *)malloc(sizeof(double) * (size_t)taps * (REDUCE_PHASES + 1));
+    k->ci = (int16_t *)malloc(sizeof(int16_t) * (size_t)taps * (REDUCE_PHASES + 1));
+    orc_reduce_tables(kernel, factor, k->cf, k->ci);
+    k->offset = orc_reduce_offset(kernel, factor, input_len);
+    return ORC_OK;
+}
+
+/* ReduceKernel::source_position (reduce_common.rs:183-185). */
+static double rk_sp(const rk_t *k, double index) { return fma(index + 0.5, k->factor, -0.5) - k->offset; }
+
+/* plan_for_source_position (reduce_common.rs:127-135). */
+static void rk_plan(const rk_t *k, double sp, int64_t *start, int *phase) {
+    double fl = floor(sp);
+    double ph = clampd(round(fma(sp - fl, (double)REDUCE_PHASES, 0.0)), 0.0, (double)REDUCE_PHASES);
+    *phase = (int)ph;
+    *start = sat_i64(fl) - k->pad;
+}
+
+void orc_reduce_plan(int kernel, double factor, uint32_t input_len, double origin, int accumulate,
+                     uint32_t n, int64_t *starts, uint8_t *phases) {
+    rk_t k;
+    if (rk_init(&k, kernel, factor, input_len) != ORC_OK) return;
+    double sx = rk_sp(&k, origin);
+    for (uint32_t i = 0; i < n; ++i) {
+        int ph;
+        double sp = accumulate ? sx : rk_sp(&k, origin + (double)i);
+        rk_plan(&k, sp, &starts[i], &ph);
+        phases[i] = (uint8_t)ph;
+        sx += factor;
+    }
+    free(k.cf);
+    free(k.ci);
+}
+
+/* required_input_region_h / _v (reduce_common.rs:218-254). */
+static void rk_required(const rk_t *k, int64_t o, int64_t n, int64_t *start, int64_t *len) {
+    if (n == 0) {
+        *start = o;
+        *len = 0;
+        return;
+    }
+    int64_t first = sat_i64(floor(rk_sp(k, (double)o)));
+    int64_t last_i = clampi(o + (n - 1), INT32_MIN, INT32_MAX);
+    int64_t last = sat_i64(floor(rk_sp(k, (double)last_i)));
+    int64_t s = clampi(first - k->pad, INT32_MIN, INT32_MAX);
+    int64_t e = clampi(last - k->pad + k->taps - 1, INT32_MIN, INT32_MAX);
+    *start = s;
+    *len = e - s + 1;
+}
+
+/* clamp_axis (reduce_common.rs:318-322). */
+static inline int64_t clamp_axis(int64_t abs, int64_t origin, int64_t len) {
+    return clampi(abs - origin, 0, len > 0 ? len - 1 : 0);
+}
+
+/* ReduceSample::from_fixed_i64 for unsigned types (sample_conv.rs:18-35). */
+static inline int64_t from_fixed(int64_t v, int64_t maxv) { return clampi((v + 2048) >> 12, 0, maxv); }
+
+/* Gaussian integer kernel (gauss_blur.rs:80-126; gauss_kernel.rs:13-56). */
+int orc_gauss_int_kernel(float sigma_f, int16_t *coeffs, int max_taps, int32_t *scale) {
+    double sigma = (double)sigma_f;
+    if (sigma < 0.2) {
+        if (max_taps < 1) return ORC_EINVAL;
+        coeffs[0] = 1;
+        *scale = 1;
+        return 1;
+    }
+    double sig2 = 2.0 * sigma * sigma;
+    size_t max_x = (size_t)ceil(8.0 * sigma);
+    size_t x = 0;
+    while (x < max_x) {
+        double v = exp(-((double)(x * x)) / sig2);
+        if (v < 0.2) break;
+        ++x;
+    }
+    size_t radius = x > 0 ? x - 1 : 0;
+    size_t size = 2 * radius + 1;
+    if ((int)size > max_taps) return ORC_EINVAL;
+    int32_t s = 0;
+    for (size_t i = 0; i < size; ++i) {
+        int32_t off = (int32_t)i - (int32_t)radius;
+        double v = exp(-(double)(off * off) / sig2);
+        int32_t tap = (int32_t)nearbyint(20.0 * v);
+        coeffs[i] = (int16_t)tap;
+        s += tap;
+    }
+    if (s == 0) {
+        coeffs[0] = 1;
+        *scale = 1;
+        return 1;
+    }
+    *scale = s;
+    return (int)size;
+}
+
+/* ------------------------------------------------------------- pipeline */
+
+enum {
+    N_SHRINK_H = 1,
+    N_SHRINK_V = 2,
+    N_REDUCE_H = 3,
+    N_REDUCE_V = 4,
+    N_GAUSS_H = 5,
+    N_GAUSS_V = 6,
+    N_CONVSEP_H = 7,
+    N_CONVSEP_V = 8,
+    N_PREMULT = 9,
+    N_UNPREMULT = 10,
+    N_COPY = 11
+};
+
+typedef struct {
+    int type;
+    int in_fmt, out_fmt;
+    uint32_t bands;
+    uint32_t in_w, in_h, out_w, out_h;
+    /* shrink */
+    uint32_t factor;
+    uint32_t source_width;
+    /* reduce */
+    rk_t rk;
+    /* gauss / convsep */
+    int ntaps;
+    int16_t gi[512];
+    int32_t gscale;
+    double *gf;
+    double cscale, coffset;
+    double param;
+} node_t;
+
+struct orc_pipeline {
+    uint32_t src_w, src_h, bands;
+    int src_fmt;
+    uint32_t cur_w, cur_h;
+    int cur_fmt;
+    int n;
+    node_t nodes[32];
+};
+
+static size_t fmt_size(int f) { return f == ORC_U8 ? 1 : (f == ORC_U16 ? 2 : 4); }
+
+orc_pipeline *orc_pipeline_new(uint32_t w, uint32_t h, uint32_t bands, int fmt) {
+    if (bands == 0 || fmt < ORC_U8 || fmt > ORC_F32) return NULL;
+    orc_pipeline *p = (orc_pipeline *)calloc(1, sizeof(orc_pipeline));
+    p->src_w = p->cur_w = w;
+    p->src_h = p->cur_h = h;
+    p->bands = bands;
+    p->src_fmt = p->cur_fmt = fmt;
+    return p;
+}
+
+void orc_pipeline_free(orc_pipeline *p) {
+    if (!p) return;
+    for (int i = 0; i < p->n; ++i) {
+        free(p->nodes[i].rk.cf);
+        free(p->nodes[i].rk.ci);
+        free(p->nodes[i].gf);
+    }
+    free(p);
+}
+
+static node_t *push(orc_pipeline *p, int type) {
+    if (p->n >= 32) return NULL;
+    node_t *nd = &p->nodes[p->n++];
+    memset(nd, 0, sizeof(*nd));
+    nd->type = type;
+    nd->in_fmt = nd->out_fmt = p->cur_fmt;
+    nd->bands = p->bands;
+    nd->in_w = nd->out_w = p->cur_w;
+    nd->in_h = nd->out_h = p->cur_h;
+    return nd;
+}
+static void commit(orc_pipeline *p, node_t *nd) {
+    p->cur_w = nd->out_w;
+    p->cur_h = nd->out_h;
+    p->cur_fmt = nd->out_fmt;
+}
+
+/* PipelineBuilder::reduce_h -> ReduceHBridge::new(.., input_w = current width)
+ * (builder/resample.rs:62-123; reduceh.rs:40-58,187-206). */
+int orc_pipeline_reduce_h(orc_pipeline *p, double factor, int kernel) {
+    if (!valid_reduce_kernel(kernel)) return ORC_EKERNEL;
+    if (!isfinite(factor) || factor < 1.0) return ORC_EINVAL;
+    if (orc_reduce_taps(kernel, factor) < 0) return ORC_EINVAL;
+    node_t *nd = push(p, N_REDUCE_H);
+    if (!nd) return ORC_EINVAL;
+    rk_init(&nd->rk, kernel, factor, p->cur_w);
+    nd->out_w = orc_reduce_output_len(factor, p->cur_w);
+    nd->param = factor;
+    commit(p, nd);
+    return ORC_OK;
+}
+
+int orc_pipeline_reduce_v(orc_pipeline *p, double factor, int kernel) {
+    if (!valid_reduce_kernel(kernel)) return ORC_EKERNEL;
+    if (!isfinite(factor) || factor < 1.0) return ORC_EINVAL;
+    if (orc_reduce_taps(kernel, factor) < 0) return ORC_EINVAL;
+    node_t *nd = push(p, N_REDUCE_V);
+    if (!nd) return ORC_EINVAL;
+    rk_init(&nd->rk, kernel, factor, p->cur_h);
+    nd->out_h = orc_reduce_output_len(factor, p->cur_h);
+    nd->param = factor;
+    commit(p, nd);
+    return ORC_OK;
+}
+
+/* PipelineBuilder::shrink_h_with_ceil (builder/resample.rs:133-209): the
+ * bridge always carries source_width = current width (shrinkh.rs:1366-1383). */
+int orc_pipeline_shrink_h(orc_pipeline *p, uint32_t factor, int ceil_mode) {
+    if (factor == 0) return ORC_EINVAL;
+    node_t *nd = push(p, N_SHRINK_H);
+    if (!nd) return ORC_EINVAL;
+    nd->factor = factor;
+    nd->source_width = p->cur_w;
+    nd->out_w = ceil_mode ? (p->cur_w + factor - 1) / factor : p->cur_w / factor;
+    nd->param = factor;
+    commit(p, nd);
+    return ORC_OK;
+}
+
+int orc_pipeline_shrink_v(orc_pipeline *p, uint32_t factor, int ceil_mode) {
+    if (factor == 0) return ORC_EINVAL;
+    node_t *nd = push(p, N_SHRINK_V);
+    if (!nd) return ORC_EINVAL;
+    nd->factor = factor;
+    nd->out_h = ceil_mode ? (p->cur_h + factor - 1) / factor : p->cur_h / factor;
+    nd->param = factor;
+    commit(p, nd);
+    return ORC_OK;
+}
+
+/* PipelineBuilder::gauss_blur (builder/colour.rs:130-197): U8 stays U8, every
+ * other format produces F32 (gauss_blur.rs:214-403). */
+int orc_pipeline_gauss_blur(orc_pipeline *p, float sigma) {
+    int16_t c[512];
+    int32_t scale;
+    int taps = orc_gauss_int_kernel(sigma, c, 512, &scale);
+    if (taps < 0) return ORC_EINVAL;
+    for (int pass = 0; pass < 2; ++pass) {
+        node_t *nd = push(p, pass == 0 ? N_GAUSS_H : N_GAUSS_V);
+        if (!nd) return ORC_EINVAL;
+        nd->ntaps = taps;
+        memcpy(nd->gi, c, sizeof(int16_t) * (size_t)taps);
+        nd->gscale = scale;
+        nd->gf = (double *)malloc(sizeof(double) * (size_t)taps);
+        /* gaussian_kernel_1d -> normalise_integer_kernel (gauss_kernel.rs:58-65,113-117) */
+        for (int i = 0; i < taps; ++i) nd->gf[i] = (double)c[i] / (double)scale;
+        nd->out_fmt = (p->cur_fmt == ORC_U8) ? ORC_U8 : ORC_F32;
+        nd->param = sigma;
+        commit(p, nd);
+    }
+    return ORC_OK;
+}
+
+/* ConvSep::with_mask (convsep.rs:218-233): H uses (scale, offset), V uses
+ * (scale, 0). validate_kernel_1d (convolution/common.rs:11-27). */
+int orc_pipeline_convsep(orc_pipeline *p, const double *coeffs, uint32_t n, double scale,
+                         double offset) {
+    if (n == 0 || n % 2 == 0 || n > 511) return ORC_EINVAL;
+    for (uint32_t i = 0; i < n; ++i)
+        if (!isfinite(coeffs[i])) return ORC_EINVAL;
+    if (!isfinite(scale) || scale == 0.0 || !isfinite(offset)) return ORC_EINVAL;
+    for (int pass = 0; pass < 2; ++pass) {
+        node_t *nd = push(p, pass == 0 ? N_CONVSEP_H : N_CONVSEP_V);
+        if (!nd) return ORC_EINVAL;
+        nd->ntaps = (int)n;
+        nd->gf = (double *)malloc(sizeof(double) * n);
+        memcpy(nd->gf, coeffs, sizeof(double) * n);
+        nd->cscale = scale;
+        nd->coffset = pass == 0 ? offset : 0.0;
+        nd->out_fmt = ORC_F32;
+        commit(p, nd);
+    }
+    return ORC_OK;
+}
+
+int orc_pipeline_premultiply(orc_pipeline *p) {
+    if (p->cur_fmt != ORC_U8) return ORC_EFORMAT;
+    node_t *nd = push(p, N_PREMULT);
+    if (!nd) return ORC_EINVAL;
+    commit(p, nd);
+    return ORC_OK;
+}
+
+int orc_pipeline_unpremultiply(orc_pipeline *p) {
+    if (p->cur_fmt != ORC_U8) return ORC_EFORMAT;
+    node_t *nd = push(p, N_UNPREMULT);
+    if (!nd) return ORC_EINVAL;
+    commit(p, nd);
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------ thumbnail.rs planner */
+
+enum { T_PREMULT = 1, T_SHRINK_H, T_SHRINK_V, T_REDUCE_H, T_REDUCE_V, T_UNPREMULT };
+
+/* Thumbnail::target_dimensions (thumbnail.rs:458-485). */
+static void target_dims(int target, uint32_t tw, uint32_t th, uint32_t iw, uint32_t ih,
+                        uint32_t *ow, uint32_t *oh) {
+    double fw = (double)iw, fh = (double)ih;
+    switch (target) {
+    case ORC_FITBOX: {
+        double hs = (double)(tw > 1 ? tw : 1) / fw;
+        double vs = (double)(th > 1 ? th : 1) / fh;
+        double s = fmin(fmin(hs, vs), 1.0);
+        double a = round(fw * s), b = round(fh * s);
+        *ow = sat_u32(a > 1.0 ? a : 1.0);
+        *oh = sat_u32(b > 1.0 ? b : 1.0);
+        break;
+    }
+    case ORC_FORCE_EXACT:
+        *ow = tw > 1 ? tw : 1;
+        *oh = th > 1 ? th : 1;
+        break;
+    case ORC_WIDTH: {
+        uint32_t w = tw > 1 ? tw : 1;
+        if (w > iw) w = iw;
+        double b = round((fh * (double)w) / fw);
+        *ow = w;
+        *oh = sat_u32(b > 1.0 ? b : 1.0);
+        break;
+    }
+    default: {
+        uint32_t h = th > 1 ? th : 1;
+        if (h > ih) h = ih;
+        double a = round((fw * (double)h) / fh);
+        *oh = h;
+        *ow = sat_u32(a > 1.0 ? a : 1.0);
+        break;
+    }
+    }
+}
+
+/* choose_integer_shrink (thumbnail.rs:488-511). */
+static uint32_t choose_integer_shrink(uint32_t in_len, uint32_t target, double total) {
+    if (total <= 1.0 + PLAN_EPS) return 1;
+    double f0 = floor(total);
+    uint32_t f = sat_u32(f0 > 1.0 ? f0 : 1.0);
+    while (f > 1) {
+        uint32_t after = in_len / f;
+        double residual = total / (double)f;
+        uint32_t ar;
+        if (residual > 1.0 + PLAN_EPS) {
+            double v = round((double)after / residual);
+            ar = sat_u32(v > 1.0 ? v : 1.0);
+        } else {
+            ar = after;
+        }
+        if (ar >= target) break;
+        --f;
+    }
+    return f;
+}
+
+/* Thumbnail::into_pipeline_nodes_without_shrink_hint (thumbnail.rs:229-456),
+ * crop = false. Returns node count, or ORC_EUNSUPPORTED when the plan needs an
+ * Affine geometry fix-up. */
+int orc_thumbnail_plan(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint32_t tw,
+                       uint32_t th, int kernel, int *types, double *params, int max_nodes,
+                       uint32_t *out_w, uint32_t *out_h) {
+    (void)kernel;
+    if (bands == 0) return ORC_EINVAL;
+    if (iw == 0 || ih == 0) {
+        *out_w = *out_h = 0;
+        return 0;
+    }
+    int allow_upscale = target == ORC_FORCE_EXACT;
+    int single = iw == 1 && ih == 1;
+    uint32_t gw, gh;
+    target_dims(target, tw, th, iw, ih, &gw, &gh);
+    if (allow_upscale && !single) {
+        if (gw < 1) gw = 1;
+        if (gh < 1) gh = 1;
+    } else {
+        if (gw > iw) gw = iw;
+        if (gh > ih) gh = ih;
+        if (gw < 1) gw = 1;
+        if (gh < 1) gh = 1;
+    }
+    int has_alpha = bands == 2 || bands == 4;
+    *out_w = gw;
+    *out_h = gh;
+    int geo[8];
+    double gp[8];
+    int ng = 0;
+    if (iw == 1 || ih == 1) {
+        if (single) *out_w = *out_h = 1;
+        if (*out_w != iw || *out_h != ih) return ORC_EUNSUPPORTED; /* Affine-only plan */
+    } else {
+        double hs = (double)iw / (double)gw;
+        double vs = (double)ih / (double)gh;
+        uint32_t ih_ = choose_integer_shrink(iw, gw, hs);
+        uint32_t iv_ = choose_integer_shrink(ih, gh, vs);
+        uint32_t asw = iw / (ih_ > 1 ? ih_ : 1);
+        uint32_t ash = ih / (iv_ > 1 ? iv_ : 1);
+        double rh = hs > 1.0 ? hs / (double)ih_ : 1.0;
+        double rv = vs > 1.0 ? vs / (double)iv_ : 1.0;
+        uint32_t arw = asw, arh = ash;
+        if (rh > 1.0 + PLAN_EPS) {
+            double v = round((double)asw / rh);
+            arw = sat_u32(v > 1.0 ? v : 1.0);
+        }
+        if (rv > 1.0 + PLAN_EPS) {
+            double v = round((double)ash / rv);
+            arh = sat_u32(v > 1.0 ? v : 1.0);
+        }
+        if (ih_ > 1) { geo[ng] = T_SHRINK_H; gp[ng++] = ih_; }
+        if (iv_ > 1) { geo[ng] = T_SHRINK_V; gp[ng++] = iv_; }
+        if (rh > 1.0 + PLAN_EPS) { geo[ng] = T_REDUCE_H; gp[ng++] = rh; }
+        if (rv > 1.0 + PLAN_EPS) { geo[ng] = T_REDUCE_V; gp[ng++] = rv; }
+        double ax = (double)gw / (double)arw, ay = (double)gh / (double)arh;
+        if (fabs(ax - 1.0) > PLAN_EPS || fabs(ay - 1.0) > PLAN_EPS) return ORC_EUNSUPPORTED;
+    }
+    /* wrap_geometry_nodes (thumbnail.rs:438-456) */
+    int n = 0;
+    if (has_alpha && ng > 0) {
+        if (n >= max_nodes) return ORC_EINVAL;
+        types[n] = T_PREMULT;
+        params[n++] = 0.0;
+    }
+    for (int i = 0; i < ng; ++i) {
+        if (n >= max_nodes) return ORC_EINVAL;
+        types[n] = geo[i];
+        params[n++] = gp[i];
+    }
+    if (has_alpha && ng > 0) {
+        if (n >= max_nodes) return ORC_EINVAL;
+        types[n] = T_UNPREMULT;
+        params[n++] = 0.0;
+    }
+    return n;
+}
+
+/* PipelineBuilder::thumbnail (builder/resample.rs:871-979) for an in-memory
+ * source: the shrink-on-load hint is never honoured, so the plan is the
+ * "without shrink hint" one. */
+int orc_pipeline_thumbnail(orc_pipeline *p, int target, uint32_t tw, uint32_t th, int kernel) {
+    int types[16];
+    double params[16];
+    uint32_t ow, oh;
+    if (p->bands == 0) return ORC_EINVAL;
+    if (!valid_reduce_kernel(kernel)) return ORC_EKERNEL;
+    int n = orc_thumbnail_plan(p->cur_w, p->cur_h, p->bands, target, tw, th, kernel, types,
+                               params, 16, &ow, &oh);
+    if (n < 0) return n;
+    for (int i = 0; i < n; ++i) {
+        int rc = ORC_OK;
+        switch (types[i]) {
+        case T_PREMULT: rc = orc_pipeline_premultiply(p); break;
+        case T_UNPREMULT: rc = orc_pipeline_unpremultiply(p); break;
+        case T_SHRINK_H: rc = orc_pipeline_shrink_h(p, (uint32_t)params[i], 0); break;
+        case T_SHRINK_V: rc = orc_pipeline_shrink_v(p, (uint32_t)params[i], 0); break;
+        case T_REDUCE_H: rc = orc_pipeline_reduce_h(p, params[i], kernel); break;
+        case T_REDUCE_V: rc = orc_pipeline_reduce_v(p, params[i], kernel); break;
+        }
+        if (rc != ORC_OK) return rc;
+    }
+    if (n == 0) {
+        node_t *nd = push(p, N_COPY);
+        if (!nd) return ORC_EINVAL;
+        commit(p, nd);
+    }
+    return ORC_OK;
+}
+
+/* ResizeOp::into_pipeline_nodes, downscale part (resize.rs:163-284), and
+ * PipelineBuilder::resize (builder/resample.rs:637-711). */
+int orc_pipeline_resize(orc_pipeline *p, double hscale, double vscale, int kernel) {
+    if (p->bands == 0) return ORC_EINVAL;
+    if (!(isfinite(hscale) && hscale > 0.0)) hscale = 1.0;
+    if (!(isfinite(vscale) && vscale > 0.0)) vscale = 1.0;
+    if (kernel == ORC_NOHALO) kernel = ORC_BICUBIC;
+    if (kernel == ORC_LBB || kernel == ORC_VSQBS) return ORC_EUNSUPPORTED;
+    const double gap = 2.0;
+    uint32_t iw = p->cur_w, ih = p->cur_h;
+    if (iw == 0 || ih == 0) return ORC_EUNSUPPORTED;
+    double a = round((double)iw * hscale), b = round((double)ih * vscale);
+    uint32_t tw = sat_u32(a > 1.0 ? a : 1.0), th = sat_u32(b > 1.0 ? b : 1.0);
+    uint32_t cw = iw, ch = ih;
+    uint32_t hsh = 1, vsh = 1;
+    if (tw < cw && kernel != ORC_NEAREST) {
+        double v = floor((double)cw / (double)tw / gap);
+        hsh = sat_u32(v > 1.0 ? v : 1.0);
+    }
+    if (th < ch && kernel != ORC_NEAREST) {
+        double v = floor((double)ch / (double)th / gap);
+        vsh = sat_u32(v > 1.0 ? v : 1.0);
+    }
+    if (tw > cw || th > ch) return ORC_EUNSUPPORTED; /* Zoom / Affine upscale */
+    int any = 0, rc;
+    if (vsh > 1) {
+        if ((rc = orc_pipeline_shrink_v(p, vsh, 0)) != ORC_OK) return rc;
+        ch /= vsh;
+        any = 1;
+    }
+    if (hsh > 1) {
+        if ((rc = orc_pipeline_shrink_h(p, hsh, 0)) != ORC_OK) return rc;
+        cw /= hsh;
+        any = 1;
+    }
+    if (th < ch) {
+        if ((rc = orc_pipeline_reduce_v(p, (double)ch / (double)th, kernel)) != ORC_OK) return rc;
+        ch = th;
+        any = 1;
+    }
+    if (tw < cw) {
+        if ((rc = orc_pipeline_reduce_h(p, (double)cw / (double)tw, kernel)) != ORC_OK) return rc;
+        cw = tw;
+        any = 1;
+    }
+    if (fabs((double)tw / (double)cw - 1.0) > PLAN_EPS || fabs((double)th / (double)ch - 1.0) > PLAN_EPS)
+        return ORC_EUNSUPPORTED;
+    if (!any) {
+        node_t *nd = push(p, N_COPY);
+        if (!nd) return ORC_EINVAL;
+        commit(p, nd);
+    }
+    return ORC_OK;
+}
+
+int orc_pipeline_output(const orc_pipeline *p, uint32_t *w, uint32_t *h, uint32_t *bands,
+                        int *fmt) {
+    *w = p->cur_w;
+    *h = p->cur_h;
+    *bands = p->bands;
+    *fmt = p->cur_fmt;
+    return ORC_OK;
+}
+
+int orc_pipeline_node_count(const orc_pipeline *p) { return p->n; }
+
+int orc_pipeline_node_info(const orc_pipeline *p, int i, int *type, double *param) {
+    if (i < 0 || i >= p->n) return ORC_EINVAL;
+    *type = p->nodes[i].type;
+    *param = p->nodes[i].param;
+    return ORC_OK;
+}
+
+/* ------------------------------------------------- required input regions */
+
+static rgn required_input(const node_t *nd, rgn o) {
+    rgn r = o;
+    switch (nd->type) {
+    case N_SHRINK_H: { /* shrinkh.rs:220-235 */
+        int64_t need = o.w * (int64_t)nd->factor;
+        if (o.x == 0 && need <= (int64_t)nd->source_width) {
+            r.x = 0;
+            r.w = nd->source_width;
+        } else {
+            r.x = clampi(o.x * (int64_t)nd->factor, INT32_MIN, INT32_MAX);
+            r.w = need;
+        }
+        break;
+    }
+    case N_SHRINK_V: /* shrinkv.rs:122-129 */
+        r.y = clampi(o.y * (int64_t)nd->factor, INT32_MIN, INT32_MAX);
+        r.h = o.h * (int64_t)nd->factor;
+        break;
+    case N_REDUCE_H: rk_required(&nd->rk, o.x, o.w, &r.x, &r.w); break;
+    case N_REDUCE_V: rk_required(&nd->rk, o.y, o.h, &r.y, &r.h); break;
+    case N_GAUSS_H:
+    case N_CONVSEP_H: { /* gauss_blur.rs:1801-1808; convsep.rs:65-72 */
+        int64_t rad = (nd->ntaps - 1) / 2;
+        r.x = o.x - rad;
+        r.w = o.w + 2 * rad;
+        break;
+    }
+    case N_GAUSS_V:
+    case N_CONVSEP_V: {
+        int64_t rad = (nd->ntaps - 1) / 2;
+        r.y = o.y - rad;
+        r.h = o.h + 2 * rad;
+        break;
+    }
+    default: break; /* pixel-local */
+    }
+    return r;
+}
+
+/* --------------------------------------------------------- sample access */
+
+static inline double ld(const void *b, int fmt, size_t i) {
+    switch (fmt) {
+    case ORC_U8: return (double)((const uint8_t *)b)[i];
+    case ORC_U16: return (double)((const uint16_t *)b)[i];
+    default: return (double)((const float *)b)[i];
+    }
+}
+static inline int64_t ldi(const void *b, int fmt, size_t i) {
+    return fmt == ORC_U8 ? (int64_t)((const uint8_t *)b)[i] : (int64_t)((const uint16_t *)b)[i];
+}
+
+/* ------------------------------------------------------- process_region */
+
+static void process_reduce_h(const node_t *nd, const void *in, rgn ir, void *out, rgn orr) {
+    const rk_t *k = &nd->rk;
+    size_t B = nd->bands, in_w = (size_t)ir.w, out_w = (size_t)orr.w;
+    int64_t *starts = (int64_t *)malloc(sizeof(int64_t) * (out_w + 1));
+    int *phases = (int *)malloc(sizeof(int) * (out_w + 1));
+    if (nd->in_fmt == ORC_U8) {
+        /* reduceh.rs:101-120: plan from f64 accumulation starting at sp(region.x) */
+        double sx = rk_sp(k, (double)orr.x);
+        for (size_t i = 0; i < out_w; ++i) {
+            rk_plan(k, sx, &starts[i], &phases[i]);
+            sx += k->factor;
+        }
+    } else {
+        /* reduce_simd.rs:63,2565,2642: direct position per column */
+        for (size_t i = 0; i < out_w; ++i) rk_plan(k, rk_sp(k, (double)orr.x + (double)i), &starts[i], &phases[i]);
+    }
+    int T = k->taps;
+    for (int64_t y = 0; y < orr.h; ++y) {
+        for (size_t xo = 0; xo < out_w; ++xo) {
+            for (size_t b = 0; b < B; ++b) {
+                size_t oi = ((size_t)y * out_w + xo) * B + b;
+                if (nd->in_fmt != ORC_F32) {
+                    /* reduce_h_u8_scalar_planned / avx2_pixel (reduce_simd.rs:2463-2538) */
+                    const int16_t *w = k->ci + (size_t)phases[xo] * T;
+                    int64_t acc = 0;
+                    for (int t = 0; t < T; ++t) {
+                        size_t tx = (size_t)clamp_axis(starts[xo] + t, ir.x, ir.w);
+                        acc += (int64_t)w[t] * ldi(in, nd->in_fmt, ((size_t)y * in_w + tx) * B + b);
+                    }
+                    if (nd->in_fmt == ORC_U8)
+                        ((uint8_t *)out)[oi] = (uint8_t)from_fixed(acc, 255);
+                    else
+                        ((uint16_t *)out)[oi] = (uint16_t)from_fixed(acc, 65535);
+                } else {
+                    const double *w = k->cf + (size_t)phases[xo] * T;
+                    double acc = 0.0;
+                    int t = 0;
+                    if (B == 1) {
+                        /* reduce_h_f32_avx2_pixel (reduce_simd.rs:2663-2693) */
+                        for (; t + 4 <= T; t += 4) {
+                            double s = 0.0;
+                            for (int l = 0; l < 4; ++l) {
+                                size_t tx = (size_t)clamp_axis(starts[xo] + t + l, ir.x, ir.w);
+                                double prod = ld(in, ORC_F32, (size_t)y * in_w + tx) * w[t + l];
+                                s = s + prod;
+                            }
+                            acc += s;
+                        }
+                    }
+                    /* scalar FMA chain (reduce_simd.rs:78-86) / AVX2 tail */
+                    for (; t < T; ++t) {
+                        size_t tx = (size_t)clamp_axis(starts[xo] + t, ir.x, ir.w);
+                        acc = fma(ld(in, ORC_F32, ((size_t)y * in_w + tx) * B + b), w[t], acc);
+                    }
+                    ((float *)out)[oi] = (float)acc;
+                }
+            }
+        }
+    }
+    free(starts);
+    free(phases);
+}
+
+static void process_reduce_v(const node_t *nd, const void *in, rgn ir, void *out, rgn orr) {
+    const rk_t *k = &nd->rk;
+    size_t B = nd->bands, row_len = (size_t)orr.w * B, stride = (size_t)ir.w * B;
+    int T = k->taps;
+    /* reduce_v_f32_avx2 vectorises the row prefix in groups of 4 (reduce_simd.rs:2839-2887) */
+    size_t vec_len = row_len / 4 * 4;
+    for (int64_t yo = 0; yo < orr.h; ++yo) {
+        int64_t start;
+        int phase;
+        /* x86 AVX2: direct source position per row (reduce_simd.rs:2716-2717,2781,2846) */
+        rk_plan(k, rk_sp(k, (double)orr.y + (double)yo), &start, &phase);
+        for (size_t x = 0; x < row_len; ++x) {
+            size_t oi = (size_t)yo * row_len + x;
+            if (nd->in_fmt != ORC_F32) {
+                const int16_t *w = k->ci + (size_t)phase * T;
+                int64_t acc = 0;
+                for (int t = 0; t < T; ++t) {
+                    size_t ty = (size_t)clamp_axis(start + t, ir.y, ir.h);
+                    acc += (int64_t)w[t] * ldi(in, nd->in_fmt, ty * stride + x);
+                }
+                if (nd->in_fmt == ORC_U8)
+                    ((uint8_t *)out)[oi] = (uint8_t)from_fixed(acc, 255);
+                else
+                    ((uint16_t *)out)[oi] = (uint16_t)from_fixed(acc, 65535);
+            } else {
+                const double *w = k->cf + (size_t)phase * T;
+                double acc = 0.0;
+                for (int t = 0; t < T; ++t) {
+                    size_t ty = (size_t)clamp_axis(start + t, ir.y, ir.h);
+                    double xv = ld(in, ORC_F32, ty * stride + x);
+                    if (x < vec_len) {
+                        double prod = xv * w[t];
+                        acc = acc + prod;
+                    } else {
+                        acc = fma(xv, w[t], acc);
+                    }
+                }
+                ((float *)out)[oi] = (float)acc;
+            }
+        }
+    }
+}
+
+/* u8 box-average rounding of ShrinkH on x86 (shrinkh.rs:323-471,473-731). */
+static inline uint8_t shrink_h_u8_round(uint32_t sum, uint32_t f, uint32_t bands) {
+    if (f == 2) return (uint8_t)((sum + 1) >> 1);
+    if (f == 5 && (bands == 1 || bands == 3 || bands == 4)) return (uint8_t)((sum + 2) / 5);
+    uint64_t mult = (1ull << 32) / (256ull * f);
+    return (uint8_t)(((uint64_t)(sum + f / 2) * mult) >> 24);
+}
+/* ShrinkV u8 (shrinkv.rs:206-339). */
+static inline uint8_t shrink_v_u8_round(uint32_t sum, uint32_t f) {
+    if (f == 2) return (uint8_t)((sum + 1) >> 1);
+    uint32_t mult = (uint32_t)((1ull << 32) / (256ull * f));
+    return (uint8_t)(((uint64_t)(sum + f / 2) * mult) >> 24);
+}
+
+static void store_shrunk(void *out, int fmt, size_t oi, double v) {
+    /* ShrinkSample::from_f64_clamped (shrinkh.rs:49-128) */
+    if (fmt == ORC_U16) {
+        ((uint16_t *)out)[oi] = (uint16_t)clampd(round(v), 0.0, 65535.0);
+    } else {
+        ((float *)out)[oi] = (float)v;
+    }
+}
+
+static void process_shrink_h(const node_t *nd, const void *in, rgn ir, void *out, rgn orr) {
+    size_t B = nd->bands, f = nd->factor, in_w = (size_t)ir.w, out_w = (size_t)orr.w;
+    double inv = 1.0 / (double)f;
+    for (int64_t y = 0; y < orr.h; ++y) {
+        for (size_t x = 0; x < out_w; ++x) {
+            for (size_t b = 0; b < B; ++b) {
+                size_t oi = ((size_t)y * out_w + x) * B + b;
+                size_t base = ((size_t)y * in_w + x * f) * B + b;
+                if (nd->in_fmt == ORC_U8) {
+                    const uint8_t *s = (const uint8_t *)in;
+                    if (f == 1) {
+                        ((uint8_t *)out)[oi] = s[base];
+                        continue;
+                    }
+                    uint32_t sum = 0;
+                    for (size_t j = 0; j < f; ++j) sum += s[base + j * B];
+                    ((uint8_t *)out)[oi] = shrink_h_u8_round(sum, (uint32_t)f, (uint32_t)B);
+                } else {
+                    double sum = 0.0;
+                    for (size_t j = 0; j < f; ++j) sum += ld(in, nd->in_fmt, base + j * B);
+                    store_shrunk(out, nd->in_fmt, oi, sum * inv);
+                }
+            }
+        }
+    }
+}
+
+static void process_shrink_v(const node_t *nd, const void *in, rgn ir, void *out, rgn orr) {
+    (void)ir;
+    size_t B = nd->bands, f = nd->factor, row = (size_t)orr.w * B;
+    double inv = 1.0 / (double)f;
+    for (int64_t y = 0; y < orr.h; ++y) {
+        for (size_t x = 0; x < row; ++x) {
+            size_t oi = (size_t)y * row + x;
+            if (nd->in_fmt == ORC_U8) {
+                const uint8_t *s = (const uint8_t *)in;
+                if (f == 1) {
+                    ((uint8_t *)out)[oi] = s[oi];
+                    continue;
+                }
+                uint32_t sum = 0;
+                for (size_t k = 0; k < f; ++k) sum += s[((size_t)y * f + k) * row + x];
+                ((uint8_t *)out)[oi] = shrink_v_u8_round(sum, (uint32_t)f);
+            } else {
+                double sum = 0.0;
+                for (size_t k = 0; k < f; ++k) sum += ld(in, nd->in_fmt, ((size_t)y * f + k) * row + x);
+                store_shrunk(out, nd->in_fmt, oi, sum * inv);
+            }
+        }
+    }
+}
+
+static void process_conv(const node_t *nd, const void *in, rgn ir, void *out, rgn orr, int vertical) {
+    size_t B = nd->bands, in_w = (size_t)ir.w, out_w = (size_t)orr.w;
+    int T = nd->ntaps;
+    for (int64_t y = 0; y < orr.h; ++y) {
+        for (size_t x = 0; x < out_w; ++x) {
+            for (size_t b = 0; b < B; ++b) {
+                size_t oi = ((size_t)y * out_w + x) * B + b;
+#define IDX(t) (vertical ? (((size_t)y + (size_t)(t)) * in_w + x) * B + b : ((size_t)y * in_w + x + (size_t)(t)) * B + b)
+                if (nd->type == N_GAUSS_H || nd->type == N_GAUSS_V) {
+                    if (nd->in_fmt == ORC_U8) {
+                        /* process_{horizontal,vertical}_u8_scalar + clip_u8_fixed (gauss_blur.rs:405-518) */
+                        const uint8_t *s = (const uint8_t *)in;
+                        int64_t sum = 0;
+                        for (int t = 0; t < T; ++t) sum += (int64_t)nd->gi[t] * (int64_t)s[IDX(t)];
+                        int64_t v = (sum + (int64_t)(nd->gscale / 2)) / (int64_t)nd->gscale;
+                        ((uint8_t *)out)[oi] = (uint8_t)clampi(v, 0, 255);
+                    } else {
+                        /* process_{horizontal,vertical}_float (gauss_blur.rs:410-464): no FMA */
+                        double acc = 0.0;
+                        for (int t = 0; t < T; ++t) {
+                            double xv = (double)(float)ld(in, nd->in_fmt, IDX(t));
+                            double prod = xv * nd->gf[t];
+                            acc = acc + prod;
+                        }
+                        ((float *)out)[oi] = (float)acc;
+                    }
+                } else {
+                    /* ConvSepH/V::process_region (convsep.rs:87-107,184-205) */
+                    double acc = 0.0;
+                    for (int t = 0; t < T; ++t) acc = fma(ld(in, nd->in_fmt, IDX(t)), nd->gf[t], acc);
+                    ((float *)out)[oi] = (float)(acc / nd->cscale + nd->coffset);
+                }
+#undef IDX
+            }
+        }
+    }
+}
+
+static void process_alpha(const node_t *nd, const void *in, void *out, rgn orr) {
+    size_t B = nd->bands, npix = (size_t)orr.w * (size_t)orr.h;
+    const uint8_t *s = (const uint8_t *)in;
+    uint8_t *d = (uint8_t *)out;
+    for (size_t i = 0; i < npix; ++i) {
+        const uint8_t *px = s + i * B;
+        uint8_t *po = d + i * B;
+        double a = (double)px[B - 1];
+        for (size_t c = 0; c + 1 < B; ++c) {
+            double v;
+            if (nd->type == N_PREMULT) {
+                /* premultiply.rs:103-115 */
+                double na = clampd(a, 0.0, 255.0) / 255.0;
+                v = clampd(round((double)px[c] * na), 0.0, 255.0);
+            } else {
+                /* unpremultiply.rs:96-109 */
+                v = a == 0.0 ? 0.0 : clampd(round(((double)px[c] * 255.0) / a), 0.0, 255.0);
+            }
+            po[c] = (uint8_t)v;
+        }
+        po[B - 1] = px[B - 1];
+    }
+}
+
+static void process_node(const node_t *nd, const void *in, rgn ir, void *out, rgn orr) {
+    switch (nd->type) {
+    case N_REDUCE_H: process_reduce_h(nd, in, ir, out, orr); break;
+    case N_REDUCE_V: process_reduce_v(nd, in, ir, out, orr); break;
+    case N_SHRINK_H: process_shrink_h(nd, in, ir, out, orr); break;
+    case N_SHRINK_V: process_shrink_v(nd, in, ir, out, orr); break;
+    case N_GAUSS_H:
+    case N_CONVSEP_H: process_conv(nd, in, ir, out, orr, 0); break;
+    case N_GAUSS_V:
+    case N_CONVSEP_V: process_conv(nd, in, ir, out, orr, 1); break;
+    case N_PREMULT:
+    case N_UNPREMULT: process_alpha(nd, in, out, orr); break;
+    default: memcpy(out, in, (size_t)orr.w * (size_t)orr.h * nd->bands * fmt_size(nd->in_fmt)); break;
+    }
+}
+
+/* MemorySource::read_region with clamp-to-edge (sources/memory.rs:149-235). */
+static void read_source(const orc_pipeline *p, const void *src, rgn r, void *dst) {
+    size_t ps = (size_t)p->bands * fmt_size(p->src_fmt);
+    const uint8_t *s = (const uint8_t *)src;
+    uint8_t *d = (uint8_t *)dst;
+    for (int64_t row = 0; row < r.h; ++row) {
+        int64_t sy = clampi(r.y + row, 0, (int64_t)p->src_h - 1);
+        for (int64_t col = 0; col < r.w; ++col) {
+            int64_t sx = clampi(r.x + col, 0, (int64_t)p->src_w - 1);
+            memcpy(d + ((size_t)row * (size_t)r.w + (size_t)col) * ps,
+                   s + ((size_t)sy * p->src_w + (size_t)sx) * ps, ps);
+        }
+    }
+}
+
+/* execute_tile (execute.rs:174-852) for a linear chain: backward region
+ * propagation, clamp-to-edge source read, forward op calls. */
+static void run_tile(const orc_pipeline *p, const void *src, uint8_t *dst, rgn tile) {
+    int n = p->n;
+    rgn outr[32], inr[32];
+    outr[n - 1] = tile;
+    for (int i = n - 1; i >= 0; --i) {
+        inr[i] = required_input(&p->nodes[i], outr[i]);
+        if (i > 0) outr[i - 1] = inr[i];
+    }
+    size_t sz = (size_t)inr[0].w * (size_t)inr[0].h * p->bands * fmt_size(p->src_fmt);
+    void *cur = malloc(sz ? sz : 1);
+    read_source(p, src, inr[0], cur);
+    for (int i = 0; i < n; ++i) {
+        const node_t *nd = &p->nodes[i];
+        size_t osz = (size_t)outr[i].w * (size_t)outr[i].h * nd->bands * fmt_size(nd->out_fmt);
+        void *o = malloc(osz ? osz : 1);
+        process_node(nd, cur, inr[i], o, outr[i]);
+        free(cur);
+        cur = o;
+    }
+    /* sink write: tile rows into the output image */
+    size_t ps = (size_t)p->bands * fmt_size(p->cur_fmt);
+    size_t row = (size_t)tile.w * ps;
+    for (int64_t r = 0; r < tile.h; ++r)
+        memcpy(dst + ((size_t)(tile.y + r) * p->cur_w + (size_t)tile.x) * ps, (uint8_t *)cur + (size_t)r * row, row);
+    free(cur);
+}
+
+typedef struct {
+    const orc_pipeline *p;
+    const void *src;
+    uint8_t *dst;
+    uint32_t strip;
+    uint32_t ntiles;
+    int tid, nthreads;
+} job_t;
+
+static void *worker(void *arg) {
+    job_t *j = (job_t *)arg;
+    for (uint32_t t = (uint32_t)j->tid; t < j->ntiles; t += (uint32_t)j->nthreads) {
+        rgn tile;
+        tile.x = 0;
+        tile.y = (int64_t)t * j->strip;
+        tile.w = j->p->cur_w;
+        tile.h = j->strip;
+        if (tile.y + tile.h > (int64_t)j->p->cur_h) tile.h = (int64_t)j->p->cur_h - tile.y;
+        run_tile(j->p, j->src, j->dst, tile);
+    }
+    return NULL;
+}
+
+/* RayonScheduler::run over ThinStrip tiles (runtime.rs:1019-1235;
+ * planning.rs:513-601; op/demand.rs:41-85): full-width strips of 16 rows (1
+ * row beyond 10000 px). strip_height = 0 selects that default. */
+int orc_pipeline_run(const orc_pipeline *p, const void *src, void *dst, int threads,
+                     uint32_t strip_height) {
+    if (p->cur_w == 0 || p->cur_h == 0) return ORC_OK;
+    if (p->n == 0) {
+        memcpy(dst, src, (size_t)p->src_w * p->src_h * p->bands * fmt_size(p->src_fmt));
+        return ORC_OK;
+    }
+    uint32_t strip = strip_height ? strip_height : (p->cur_w > 10000 ? 1 : 16);
+    uint32_t ntiles = (p->cur_h + strip - 1) / strip;
+    if (threads < 1) threads = 1;
+    if ((uint32_t)threads > ntiles) threads = (int)ntiles;
+    pthread_t th[256];
+    job_t jobs[256];
+    if (threads > 256) threads = 256;
+    for (int i = 0; i < threads; ++i) {
+        jobs[i] = (job_t){p, src, (uint8_t *)dst, strip, ntiles, i, threads};
+        if (threads > 1)
+            pthread_create(&th[i], NULL, worker, &jobs[i]);
+        else
+            worker(&jobs[i]);
+    }
+    if (threads > 1)
+        for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+    return ORC_OK;
+}
+
+/* ---------------------------------------------------------- FP64 reference */
+
+typedef struct {
+    const float *src;
+    double *dst;
+    uint32_t w, h, bands;
+    rk_t *kh, *kv;
+    uint32_t ow, oh;
+    double *tmp;
+    int tid, nthreads;
+    int pass;
+} f64job_t;
+
+static void *f64_reduce_worker(void *arg) {
+    f64job_t *j = (f64job_t *)arg;
+    size_t B = j->bands;
+    int T;
+    if (j->pass == 0) {
+        T = j->kh->taps;
+        for (uint32_t y = (uint32_t)j->tid; y < j->h; y += (uint32_t)j->nthreads)
+            for (uint32_t x = 0; x < j->ow; ++x) {
+                int64_t st;
+                int ph;
+                rk_plan(j->kh, rk_sp(j->kh, (double)x), &st, &ph);
+                const double *wt = j->kh->cf + (size_t)ph * T;
+                for (size_t b = 0; b < B; ++b) {
+                    double acc = 0.0;
+                    for (int t = 0; t < T; ++t) {
+                        int64_t sx = clampi(st + t, 0, (int64_t)j->w - 1);
+                        acc += wt[t] * (double)j->src[((size_t)y * j->w + (size_t)sx) * B + b];
+                    }
+                    j->tmp[((size_t)y * j->ow + x) * B + b] = acc;
+                }
+            }
+    } else {
+        T = j->kv->taps;
+        for (uint32_t y = (uint32_t)j->tid; y < j->oh; y += (uint32_t)j->nthreads) {
+            int64_t st;
+            int ph;
+            rk_plan(j->kv, rk_sp(j->kv, (double)y), &st, &ph);
+            const double *wt = j->kv->cf + (size_t)ph * T;
+            for (uint32_t x = 0; x < j->ow; ++x)
+                for (size_t b = 0; b < B; ++b) {
+                    double acc = 0.0;
+                    for (int t = 0; t < T; ++t) {
+                        int64_t sy = clampi(st + t, 0, (int64_t)j->h - 1);
+                        acc += wt[t] * j->tmp[((size_t)sy * j->ow + x) * B + b];
+                    }
+                    j->dst[((size_t)y * j->ow + x) * B + b] = acc;
+                }
+        }
+    }
+    return NULL;
+}
+
+/* reduce_h then reduce_v on an f32 image, carried in double: the same
+ * 65-phase f64 weight tables, direct per-axis plans and clamp-to-edge borders as
+ * the f32 path (reduce_simd.rs:47-144), but with no f32 rounding between the
+ * passes or at the output.  This is the FP64 oracle the f32 / bf16 error study
+ * is measured against. */
+int orc_f64_reduce_hv(const float *src, uint32_t w, uint32_t h, uint32_t bands, double factor,
+                      int kernel, double *dst, uint32_t *out_w, uint32_t *out_h, int threads) {
+    if (bands == 0 || bands > 4) return ORC_EINVAL;
+    rk_t kh, kv;
+    int rc;
+    if ((rc = rk_init(&kh, kernel, factor, w)) != ORC_OK) return rc;
+    if ((rc = rk_init(&kv, kernel, factor, h)) != ORC_OK) return rc;
+    uint32_t ow = orc_reduce_output_len(factor, w), oh = orc_reduce_output_len(factor, h);
+    *out_w = ow;
+    *out_h = oh;
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)ow * h * bands);
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    for (int pass = 0; pass < 2; ++pass) {
+        pthread_t th[256];
+        f64job_t jobs[256];
+        for (int i = 0; i < threads; ++i) {
+            jobs[i] = (f64job_t){src, dst, w, h, bands, &kh, &kv, ow, oh, tmp, i, threads, pass};
+            pthread_create(&th[i], NULL, f64_reduce_worker, &jobs[i]);
+        }
+        for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+    }
+    free(tmp);
+    free(kh.cf);
+    free(kh.ci);
+    free(kv.cf);
+    free(kv.ci);
+    return ORC_OK;
+}
+
+typedef struct {
+    const float *src;
+    double *dst, *tmp;
+    uint32_t w, h, bands;
+    const double *k;
+    int taps;
+    int tid, nthreads, pass;
+} gjob_t;
+
+static void *f64_gauss_worker(void *arg) {
+    gjob_t *j = (gjob_t *)arg;
+    int r = (j->taps - 1) / 2;
+    size_t B = j->bands;
+    for (uint32_t y = (uint32_t)j->tid; y < j->h; y += (uint32_t)j->nthreads)
+        for (uint32_t x = 0; x < j->w; ++x)
+            for (size_t b = 0; b < B; ++b) {
+                double acc = 0.0;
+                for (int t = 0; t < j->taps; ++t) {
+                    if (j->pass == 0) {
+                        int64_t sx = clampi((int64_t)x + t - r, 0, (int64_t)j->w - 1);
+                        acc += j->k[t] * (double)j->src[((size_t)y * j->w + (size_t)sx) * B + b];
+                    } else {
+                        int64_t sy = clampi((int64_t)y + t - r, 0, (int64_t)j->h - 1);
+                        acc += j->k[t] * j->tmp[((size_t)sy * j->w + x) * B + b];
+                    }
+                }
+                (j->pass == 0 ? j->tmp : j->dst)[((size_t)y * j->w + x) * B + b] = acc;
+            }
+    return NULL;
+}
+
+/* Separable Gaussian with the same normalised integer mask, in double, with
+ * clamp-to-edge borders and no intermediate f32 rounding. */
+int orc_f64_gauss(const float *src, uint32_t w, uint32_t h, uint32_t bands, float sigma,
+                  double *dst, int threads) {
+    int16_t c[512];
+    int32_t scale;
+    int taps = orc_gauss_int_kernel(sigma, c, 512, &scale);
+    if (taps < 0) return ORC_EINVAL;
+    double k[512];
+    for (int i = 0; i < taps; ++i) k[i] = (double)c[i] / (double)scale;
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)w * h * bands);
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    for (int pass = 0; pass < 2; ++pass) {
+        pthread_t th[256];
+        gjob_t jobs[256];
+        for (int i = 0; i < threads; ++i) {
+            jobs[i] = (gjob_t){src, dst, tmp, w, h, bands, k, taps, i, threads, pass};
+            pthread_create(&th[i], NULL, f64_gauss_worker, &jobs[i]);
+        }
+        for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+    }
+    free(tmp);
+    return ORC_OK;
+}

@@ -1,0 +1,233 @@
+"""ctypes binding for the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, always as the checker or the timed CPU
+baseline, never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+U8, U16, F32 = 0, 1, 2
+NEAREST, BILINEAR, BICUBIC, LANCZOS2, LANCZOS3, CATMULLROM, LBB, VSQBS, NOHALO = range(9)
+FITBOX, FORCE_EXACT, WIDTH, HEIGHT = range(4)
+OK, EINVAL, EKERNEL, EFORMAT, EUNSUPPORTED = 0, -1, -2, -3, -4
+
+DTYPES = {U8: np.uint8, U16: np.uint16, F32: np.float32}
+FMT_OF = {np.dtype(np.uint8): U8, np.dtype(np.uint16): U16, np.dtype(np.float32): F32}
+
+_lib = None
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        vp, u32, i32, dbl = C.c_void_p, C.c_uint32, C.c_int, C.c_double
+        L.orc_kernel_weight.restype = dbl
+        L.orc_kernel_weight.argtypes = [i32, dbl]
+        L.orc_reduce_taps.argtypes = [i32, dbl]
+        L.orc_reduce_tables.argtypes = [i32, dbl, vp, vp]
+        L.orc_reduce_offset.restype = dbl
+        L.orc_reduce_offset.argtypes = [i32, dbl, u32]
+        L.orc_reduce_output_len.restype = u32
+        L.orc_reduce_output_len.argtypes = [dbl, u32]
+        L.orc_reduce_plan.argtypes = [i32, dbl, u32, dbl, i32, u32, vp, vp]
+        L.orc_gauss_int_kernel.argtypes = [C.c_float, vp, i32, vp]
+        L.orc_pipeline_new.restype = vp
+        L.orc_pipeline_new.argtypes = [u32, u32, u32, i32]
+        L.orc_pipeline_free.argtypes = [vp]
+        L.orc_pipeline_reduce_h.argtypes = [vp, dbl, i32]
+        L.orc_pipeline_reduce_v.argtypes = [vp, dbl, i32]
+        L.orc_pipeline_shrink_h.argtypes = [vp, u32, i32]
+        L.orc_pipeline_shrink_v.argtypes = [vp, u32, i32]
+        L.orc_pipeline_gauss_blur.argtypes = [vp, C.c_float]
+        L.orc_pipeline_convsep.argtypes = [vp, vp, u32, dbl, dbl]
+        L.orc_pipeline_premultiply.argtypes = [vp]
+        L.orc_pipeline_unpremultiply.argtypes = [vp]
+        L.orc_pipeline_thumbnail.argtypes = [vp, i32, u32, u32, i32]
+        L.orc_pipeline_resize.argtypes = [vp, dbl, dbl, i32]
+        L.orc_pipeline_output.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_pipeline_node_count.argtypes = [vp]
+        L.orc_pipeline_node_info.argtypes = [vp, i32, vp, vp]
+        L.orc_pipeline_run.argtypes = [vp, vp, vp, i32, u32]
+        L.orc_thumbnail_plan.argtypes = [u32, u32, u32, i32, u32, u32, i32, vp, vp, i32, vp, vp]
+        L.orc_f64_reduce_hv.argtypes = [vp, u32, u32, u32, dbl, i32, vp, vp, vp, i32]
+        L.orc_f64_gauss.argtypes = [vp, u32, u32, u32, C.c_float, vp, i32]
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what} failed with status {code}")
+        self.code = code
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != OK:
+        raise OracleError(rc, what)
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def reduce_tables(kernel: int, factor: float):
+    taps = lib().orc_reduce_taps(kernel, factor)
+    if taps < 0:
+        raise OracleError(EINVAL, "reduce_taps")
+    cf = np.zeros(65 * taps, np.float64)
+    ci = np.zeros(65 * taps, np.int16)
+    rc = lib().orc_reduce_tables(kernel, factor, _ptr(cf), _ptr(ci))
+    if rc < 0:
+        raise OracleError(rc, "reduce_tables")
+    return cf.reshape(65, taps), ci.reshape(65, taps)
+
+
+def reduce_plan(kernel, factor, input_len, origin, accumulate, n):
+    starts = np.zeros(n, np.int64)
+    phases = np.zeros(n, np.uint8)
+    lib().orc_reduce_plan(kernel, factor, input_len, float(origin), int(accumulate), n,
+                          _ptr(starts), _ptr(phases))
+    return starts, phases
+
+
+def gauss_int_kernel(sigma: float):
+    c = np.zeros(512, np.int16)
+    scale = np.zeros(1, np.int32)
+    n = lib().orc_gauss_int_kernel(sigma, _ptr(c), 512, _ptr(scale))
+    if n < 0:
+        raise OracleError(n, "gauss_int_kernel")
+    return c[:n].copy(), int(scale[0])
+
+
+class Pipeline:
+    """PipelineBuilder restatement over an in-memory source image."""
+
+    def __init__(self, width: int, height: int, bands: int, fmt: int = U8):
+        self._p = lib().orc_pipeline_new(width, height, bands, fmt)
+        if not self._p:
+            raise OracleError(EINVAL, "pipeline_new")
+        self.src_shape = (height, width, bands)
+        self.src_fmt = fmt
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            lib().orc_pipeline_free(self._p)
+            self._p = None
+
+    def reduce_h(self, factor, kernel=LANCZOS3):
+        _check(lib().orc_pipeline_reduce_h(self._p, factor, kernel), "reduce_h")
+        return self
+
+    def reduce_v(self, factor, kernel=LANCZOS3):
+        _check(lib().orc_pipeline_reduce_v(self._p, factor, kernel), "reduce_v")
+        return self
+
+    def shrink_h(self, factor, ceil=False):
+        _check(lib().orc_pipeline_shrink_h(self._p, factor, int(ceil)), "shrink_h")
+        return self
+
+    def shrink_v(self, factor, ceil=False):
+        _check(lib().orc_pipeline_shrink_v(self._p, factor, int(ceil)), "shrink_v")
+        return self
+
+    def gauss_blur(self, sigma):
+        _check(lib().orc_pipeline_gauss_blur(self._p, sigma), "gauss_blur")
+        return self
+
+    def convsep(self, coeffs, scale=1.0, offset=0.0):
+        c = np.ascontiguousarray(coeffs, np.float64)
+        _check(lib().orc_pipeline_convsep(self._p, _ptr(c), len(c), scale, offset), "convsep")
+        return self
+
+    def premultiply(self):
+        _check(lib().orc_pipeline_premultiply(self._p), "premultiply")
+        return self
+
+    def unpremultiply(self):
+        _check(lib().orc_pipeline_unpremultiply(self._p), "unpremultiply")
+        return self
+
+    def thumbnail(self, target, width, height=0, kernel=LANCZOS3):
+        _check(lib().orc_pipeline_thumbnail(self._p, target, width, height, kernel), "thumbnail")
+        return self
+
+    def resize(self, hscale, vscale, kernel=LANCZOS3):
+        _check(lib().orc_pipeline_resize(self._p, hscale, vscale, kernel), "resize")
+        return self
+
+    def output(self):
+        w, h, b = (C.c_uint32(), C.c_uint32(), C.c_uint32())
+        f = C.c_int()
+        lib().orc_pipeline_output(self._p, C.addressof(w), C.addressof(h), C.addressof(b),
+                                  C.addressof(f))
+        return w.value, h.value, b.value, f.value
+
+    def nodes(self):
+        out = []
+        t, p = C.c_int(), C.c_double()
+        for i in range(lib().orc_pipeline_node_count(self._p)):
+            lib().orc_pipeline_node_info(self._p, i, C.addressof(t), C.addressof(p))
+            out.append((t.value, p.value))
+        return out
+
+    def run(self, src: np.ndarray, threads: int = 0, strip_height: int = 0) -> np.ndarray:
+        h, w, b = self.src_shape
+        src = np.ascontiguousarray(src, DTYPES[self.src_fmt]).reshape(h, w, b)
+        ow, oh, ob, of = self.output()
+        dst = np.zeros((oh, ow, ob), DTYPES[of])
+        if threads <= 0:
+            threads = os.cpu_count() or 1
+        _check(lib().orc_pipeline_run(self._p, _ptr(src), _ptr(dst), threads, strip_height), "run")
+        return dst
+
+
+def thumbnail_plan(in_w, in_h, bands, target, tw, th=0, kernel=LANCZOS3):
+    types = np.zeros(16, np.int32)
+    params = np.zeros(16, np.float64)
+    ow, oh = C.c_uint32(), C.c_uint32()
+    n = lib().orc_thumbnail_plan(in_w, in_h, bands, target, tw, th, kernel, _ptr(types),
+                                 _ptr(params), 16, C.addressof(ow), C.addressof(oh))
+    if n < 0:
+        raise OracleError(n, "thumbnail_plan")
+    names = {1: "Premultiply", 2: "ShrinkH", 3: "ShrinkV", 4: "ReduceH", 5: "ReduceV",
+             6: "Unpremultiply"}
+    return [(names[int(t)], float(p)) for t, p in zip(types[:n], params[:n])], (ow.value, oh.value)
+
+
+def f64_reduce_hv(src: np.ndarray, factor: float, kernel: int = LANCZOS3, threads: int = 0):
+    h, w, b = src.shape
+    src = np.ascontiguousarray(src, np.float32)
+    ow = int(lib().orc_reduce_output_len(factor, w))
+    oh = int(lib().orc_reduce_output_len(factor, h))
+    dst = np.zeros((oh, ow, b), np.float64)
+    a, c = C.c_uint32(), C.c_uint32()
+    _check(lib().orc_f64_reduce_hv(_ptr(src), w, h, b, factor, kernel, _ptr(dst), C.addressof(a),
+                                   C.addressof(c), threads or (os.cpu_count() or 1)), "f64_reduce_hv")
+    return dst
+
+
+def f64_gauss(src: np.ndarray, sigma: float, threads: int = 0):
+    h, w, b = src.shape
+    src = np.ascontiguousarray(src, np.float32)
+    dst = np.zeros((h, w, b), np.float64)
+    _check(lib().orc_f64_gauss(_ptr(src), w, h, b, sigma, _ptr(dst), threads or (os.cpu_count() or 1)),
+           "f64_gauss")
+    return dst
