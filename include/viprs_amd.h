@@ -1,0 +1,202 @@
+/*
+ * viprs_amd: MI355X (gfx950) executor for viprs's resample and separable
+ * convolution path -- ShrinkH/ShrinkV box pre-shrink, ReduceH/ReduceV
+ * Lanczos/cubic/linear reducers (thumbnail / resize / reduce_h / reduce_v),
+ * gauss_blur and convsep.
+ *
+ * C ABI: plain pointers, sizes and status codes; no exceptions cross it.
+ * Conventions follow the reference's existing C ABI (include/viprs.h:7-20,
+ * src/ffi.rs:24-230): opaque handles created by *_new / *_create, released by
+ * the matching *_destroy, NULL-safe destroy, negative status on failure.
+ *
+ * Image memory is packed, band-interleaved, row-major, exactly as the
+ * reference's tiles (crates/viprs-core/src/image/region.rs:154-230).
+ * Unless a function says otherwise, image pointers are DEVICE pointers and the
+ * work is enqueued on the context's stream (asynchronous).
+ */
+#ifndef VIPRS_AMD_H
+#define VIPRS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* BandFormatId subset (crates/viprs-core/src/format/band.rs:19-). */
+typedef enum { VGPU_U8 = 0, VGPU_U16 = 1, VGPU_F32 = 2 } vgpu_format;
+
+/* InterpolationKernel (crates/viprs-core/src/kernel.rs:22-49). LBB, VSQBS and
+ * Nohalo are rejected by the reduce ops like the reference does
+ * (reduce_common.rs:94-124). */
+typedef enum {
+    VGPU_NEAREST = 0,
+    VGPU_BILINEAR = 1,
+    VGPU_BICUBIC = 2,
+    VGPU_LANCZOS2 = 3,
+    VGPU_LANCZOS3 = 4,
+    VGPU_CATMULLROM = 5,
+    VGPU_LBB = 6,
+    VGPU_VSQBS = 7,
+    VGPU_NOHALO = 8
+} vgpu_kernel;
+
+/* ThumbnailTarget (crates/viprs-ops-spatial/src/resample/thumbnail.rs:125-145). */
+typedef enum {
+    VGPU_FITBOX = 0,
+    VGPU_FORCE_EXACT = 1,
+    VGPU_WIDTH = 2,
+    VGPU_HEIGHT = 3
+} vgpu_thumbnail_target;
+
+/* Status codes.  BuildError::InvalidReduceParameters / SourceHint ->
+ * INVALID_ARGUMENT; BuildError::InvalidKernel -> INVALID_KERNEL;
+ * BuildError::UnsupportedFormat -> UNSUPPORTED_FORMAT
+ * (crates/viprs-core/src/error.rs). */
+typedef enum {
+    VGPU_OK = 0,
+    VGPU_E_INVALID_ARGUMENT = -1,
+    VGPU_E_INVALID_KERNEL = -2,
+    VGPU_E_UNSUPPORTED_FORMAT = -3,
+    VGPU_E_UNSUPPORTED = -4,  /* plan needs a node outside this path (Affine/Zoom/crop) */
+    VGPU_E_BUFFER_SIZE = -5,  /* a buffer length does not match the region */
+    VGPU_E_DEVICE = -6,       /* HIP runtime error (sticky on the context) */
+    VGPU_E_NO_MEMORY = -7,
+    VGPU_E_CANCELLED = -8,
+    VGPU_E_ADMISSION = -9     /* request exceeds the device memory budget */
+} vgpu_status;
+
+/* Region (crates/viprs-core/src/image/region.rs:18-27). */
+typedef struct {
+    int32_t x, y;
+    uint32_t width, height;
+} vgpu_region;
+
+typedef struct vgpu_context vgpu_context;
+typedef struct vgpu_pipeline vgpu_pipeline;
+typedef struct vgpu_op vgpu_op;
+
+const char *vgpu_status_string(int status);
+
+/* ---------------------------------------------------------------- context */
+
+/* Device executor context: one HIP device, one stream, a device buffer pool
+ * and the coefficient/plan cache.  No reference counterpart; it is the state
+ * the device executor keeps for RayonScheduler
+ * (crates/viprs-runtime/src/scheduler/rayon_scheduler.rs:53-75). */
+int vgpu_context_create(int device, vgpu_context **out);
+void vgpu_context_destroy(vgpu_context *ctx);
+/* Use a caller-owned hipStream_t (NULL = the context's own stream). */
+int vgpu_context_set_stream(vgpu_context *ctx, void *hip_stream);
+void *vgpu_context_stream(vgpu_context *ctx);
+int vgpu_synchronize(vgpu_context *ctx);
+/* Bytes held by the device buffer pool. */
+size_t vgpu_context_pool_bytes(const vgpu_context *ctx);
+/* Release every pooled buffer not in use. */
+int vgpu_context_trim(vgpu_context *ctx);
+
+/* ----------------------------------------------------- pipeline (builder) */
+
+/* PipelineBuilder::from_source(MemorySource) (crates/viprs-runtime/src/
+ * pipeline/builder/core.rs:40; sources/memory.rs:149-235): an in-memory source
+ * of width x height x bands samples of `format`, read with clamp-to-edge. */
+int vgpu_pipeline_new(vgpu_context *ctx, uint32_t width, uint32_t height, uint32_t bands,
+                      int format, vgpu_pipeline **out);
+void vgpu_pipeline_destroy(vgpu_pipeline *p);
+
+/* PipelineBuilder::reduce_h / reduce_v (builder/resample.rs:16-123,270-339). */
+int vgpu_pipeline_reduce_h(vgpu_pipeline *p, double factor, int kernel);
+int vgpu_pipeline_reduce_v(vgpu_pipeline *p, double factor, int kernel);
+/* PipelineBuilder::shrink_h_with_ceil / shrink_v_with_ceil
+ * (builder/resample.rs:133-209,392-444). */
+int vgpu_pipeline_shrink_h(vgpu_pipeline *p, uint32_t factor, int ceil_mode);
+int vgpu_pipeline_shrink_v(vgpu_pipeline *p, uint32_t factor, int ceil_mode);
+/* PipelineBuilder::gauss_blur (builder/colour.rs:130-197): U8 stays U8, other
+ * formats produce F32. */
+int vgpu_pipeline_gauss_blur(vgpu_pipeline *p, float sigma);
+/* ConvSep::with_mask via PipelineBuilder::then (convolution/convsep.rs:218-233):
+ * H pass with (scale, offset), V pass with (scale, 0); output F32. */
+int vgpu_pipeline_convsep(vgpu_pipeline *p, const double *coeffs, uint32_t n, double scale,
+                          double offset);
+/* PipelineBuilder::premultiply / unpremultiply (builder/resample.rs:721-794). */
+int vgpu_pipeline_premultiply(vgpu_pipeline *p);
+int vgpu_pipeline_unpremultiply(vgpu_pipeline *p);
+/* PipelineBuilder::thumbnail(Thumbnail::new(target, kernel))
+ * (builder/resample.rs:871-979; thumbnail.rs:229-524), in-memory source.
+ * Returns VGPU_E_UNSUPPORTED when the plan needs the Affine/crop tail. */
+int vgpu_pipeline_thumbnail(vgpu_pipeline *p, int target, uint32_t width, uint32_t height,
+                            int kernel);
+/* PipelineBuilder::resize(Resize::new(hscale, vscale, kernel))
+ * (builder/resample.rs:637-711; resize.rs:163-284), downscale only. */
+int vgpu_pipeline_resize(vgpu_pipeline *p, double hscale, double vscale, int kernel);
+
+/* CompiledPipeline output geometry (pipeline/compiled.rs:338-). */
+int vgpu_pipeline_output_info(const vgpu_pipeline *p, uint32_t *width, uint32_t *height,
+                              uint32_t *bands, int *format);
+/* Number of GPU launches one run makes after fusion (diagnostic). */
+int vgpu_pipeline_stage_count(vgpu_pipeline *p);
+
+/* TileScheduler::run (crates/viprs-ports/src/scheduler.rs:63-98) with the
+ * device region executor: runs the whole node chain over the full output on
+ * the GPU, bit-identical to the reference's tile-by-tile execution.
+ * src / dst are device pointers; enqueued on the context stream. */
+int vgpu_pipeline_run(vgpu_pipeline *p, const void *src, size_t src_bytes, void *dst,
+                      size_t dst_bytes);
+/* RayonScheduler::run_with_profile (crates/viprs-runtime/src/scheduler/
+ * rayon_scheduler/runtime.rs:424-; NodeRunProfile): runs the pipeline
+ * `iterations` times, timing every fused stage with HIP events on the context
+ * stream, and writes the mean milliseconds per stage.  Returns the stage
+ * count (>= 1) or a negative status. */
+int vgpu_pipeline_run_profiled(vgpu_pipeline *p, const void *src, size_t src_bytes, void *dst,
+                               size_t dst_bytes, int iterations, float *stage_ms,
+                               int max_stages);
+/* Same with host pointers: H2D copy, run, D2H copy, synchronize. */
+int vgpu_pipeline_run_host(vgpu_pipeline *p, const void *src, size_t src_bytes, void *dst,
+                           size_t dst_bytes);
+
+/* ------------------------------------------------- op-level (DynOperation) */
+
+/* Bridges (DynOperation, crates/viprs-core/src/op/dynamic_op.rs:16-260). */
+/* ReduceHBridge::new(factor, kernel, bands, input_w, hint) (reduceh.rs:187-206) */
+int vgpu_op_reduce_h_new(vgpu_context *ctx, double factor, int kernel, uint32_t bands,
+                         uint32_t input_w, int format, vgpu_op **out);
+/* ReduceVBridge::new(factor, kernel, bands, input_h, hint) (reducev.rs:186-205) */
+int vgpu_op_reduce_v_new(vgpu_context *ctx, double factor, int kernel, uint32_t bands,
+                         uint32_t input_h, int format, vgpu_op **out);
+/* ShrinkHBridge::new_with_ceil_and_source_width (shrinkh.rs:1354-1383);
+ * source_width = 0 means None. */
+int vgpu_op_shrink_h_new(vgpu_context *ctx, uint32_t factor, int ceil_mode, uint32_t bands,
+                         uint32_t source_width, int format, vgpu_op **out);
+/* ShrinkVBridge::new_with_ceil (shrinkv.rs:680-699) */
+int vgpu_op_shrink_v_new(vgpu_context *ctx, uint32_t factor, int ceil_mode, uint32_t bands,
+                         int format, vgpu_op **out);
+/* OperationBridge<GaussBlurH<F>> / <GaussBlurV<F>> (gauss_blur.rs:1722-1900). */
+int vgpu_op_gauss_blur_h_new(vgpu_context *ctx, float sigma, uint32_t bands, int format,
+                             vgpu_op **out);
+int vgpu_op_gauss_blur_v_new(vgpu_context *ctx, float sigma, uint32_t bands, int format,
+                             vgpu_op **out);
+/* OperationBridge<Premultiply<U8>> / <Unpremultiply<U8>>. */
+int vgpu_op_premultiply_new(vgpu_context *ctx, uint32_t bands, int format, vgpu_op **out);
+int vgpu_op_unpremultiply_new(vgpu_context *ctx, uint32_t bands, int format, vgpu_op **out);
+void vgpu_op_destroy(vgpu_op *op);
+
+/* DynOperation::required_input_region (dynamic_op.rs:16-260). */
+int vgpu_op_required_input_region(const vgpu_op *op, vgpu_region output, vgpu_region *input);
+/* DynOperation::output_width / output_height. */
+int vgpu_op_output_size(const vgpu_op *op, uint32_t in_w, uint32_t in_h, uint32_t *out_w,
+                        uint32_t *out_h);
+int vgpu_op_output_format(const vgpu_op *op, int *format);
+/* DynOperation::dyn_process_region(state, input, output, input_region,
+ * output_region) (dynamic_op.rs:222-229): device pointers, enqueued on the
+ * context stream.  Unlike OperationBridge (bridge.rs:208-236), a length that
+ * does not match the region returns VGPU_E_BUFFER_SIZE instead of silently
+ * writing nothing. */
+int vgpu_op_process_region(vgpu_op *op, const void *input, size_t input_len, void *output,
+                           size_t output_len, vgpu_region input_region,
+                           vgpu_region output_region);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,259 @@
+"""GPU parity through the C ABI: every result must equal the oracle's
+(bit-exact for u8/u16, and for f32 on the reference-exact FP64 kernels)."""
+import pathlib
+
+import numpy as np
+import pytest
+
+import viprs_amd as v
+from oracle import oracle as O
+from tests.helpers import (float_image, gradient_noise_u8, grayscale_source, noise_u8, noise_u16,
+                           rgb_source, smooth_grayscale_source, structural_source)
+
+pytestmark = pytest.mark.gpu
+FIX = pathlib.Path(__file__).resolve().parent / "golden" / "reference_fixtures"
+FMT = {np.dtype(np.uint8): v.U8, np.dtype(np.uint16): v.U16, np.dtype(np.float32): v.F32}
+
+
+def build(lib_pipeline, steps):
+    for name, args in steps:
+        getattr(lib_pipeline, name)(*args)
+    return lib_pipeline
+
+
+def run_both(ctx, src, steps, device=True):
+    """Run the same builder chain on the GPU and in the oracle."""
+    import torch
+
+    h, w, b = src.shape
+    fmt = FMT[src.dtype]
+    gp = build(v.Pipeline(ctx, w, h, b, fmt), steps)
+    op = build(O.Pipeline(w, h, b, fmt), steps)
+    exp = op.run(src)
+    if device:
+        dsrc = torch.from_numpy(np.ascontiguousarray(src)).cuda()
+        oh, ow, ob = gp.output_shape
+        ddst = torch.empty((oh, ow, ob), dtype=dsrc.dtype, device="cuda") if fmt == gp.output_info()[3] \
+            else torch.empty((oh, ow, ob), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        gp.run(dsrc, ddst)
+        ctx.synchronize()
+        got = ddst.cpu().numpy()
+    else:
+        got = gp.run_host(src)
+    return got, exp
+
+
+def assert_same(got, exp, what=""):
+    assert got.shape == exp.shape, (what, got.shape, exp.shape)
+    if not np.array_equal(got, exp):
+        bad = np.argwhere(got != exp)
+        y, x, b = bad[0]
+        raise AssertionError(f"{what}: {len(bad)} mismatches, first at (y={y}, x={x}, b={b}): "
+                             f"gpu={got[y, x, b]} oracle={exp[y, x, b]}")
+
+
+# ------------------------------------------------------------- fixtures
+
+def test_reference_fixtures_on_gpu(ctx):
+    cases = [
+        ("reduceh_libvips/rgb_factor2_linear.bin", rgb_source(8, 6), [("reduce_h", (2.0, v.BILINEAR))]),
+        ("reducev_libvips/rgb_factor2_linear.bin", rgb_source(6, 8), [("reduce_v", (2.0, v.BILINEAR))]),
+        ("reduceh_libvips/grayscale_factor1_5_cubic.bin", grayscale_source(8, 6), [("reduce_h", (1.5, v.BICUBIC))]),
+        ("libvips_parity_resample/test_reduceh_factor_1_5_lanczos3.bin", smooth_grayscale_source(8, 6),
+         [("reduce_h", (1.5, v.LANCZOS3))]),
+        ("libvips_parity_resample/test_reducev_factor_1_5_lanczos3.bin", smooth_grayscale_source(6, 8),
+         [("reduce_v", (1.5, v.LANCZOS3))]),
+        ("shrinkh/grayscale_factor_2.bin", structural_source(10, 16), [("shrink_h", (2,))]),
+        ("shrinkv/grayscale_factor_2.bin", structural_source(10, 16), [("shrink_v", (2,))]),
+        ("shrink_libvips/rgb_factor4.bin", rgb_source(16, 16), [("shrink_v", (4,)), ("shrink_h", (4,))]),
+        ("reduce_libvips/edge_4x4_factor2_linear.bin", np.arange(16, dtype=np.uint8).reshape(4, 4, 1),
+         [("reduce_v", (2.0, v.BILINEAR)), ("reduce_h", (2.0, v.BILINEAR))]),
+    ]
+    for rel, src, steps in cases:
+        h, w, b = src.shape
+        got = build(v.Pipeline(ctx, w, h, b), steps).run_host(src)
+        assert np.array_equal(got.ravel(), np.fromfile(FIX / rel, np.uint8)), rel
+
+
+def test_gauss_f32_fixture_on_gpu(ctx):
+    exp = np.fromfile(FIX / "gauss_blur/uniform_sigma_1_5.bin", np.float32)
+    got = v.Pipeline(ctx, 64, 64, 1, v.F32).gauss_blur(1.5).run_host(np.full((64, 64, 1), 37.5, np.float32))
+    assert np.max(np.abs(got.ravel() - exp)) <= 1e-3
+
+
+# ------------------------------------------------------- configurations
+
+def test_resize_1024x768_lanczos3_factor_3_7(ctx):
+    # Plain CPU-runnable resize: reduce_h(3.7) then reduce_v(3.7), Lanczos3.
+    src = noise_u8(768, 1024, 3, 20241)
+    got, exp = run_both(ctx, src, [("reduce_h", (3.7, v.LANCZOS3)), ("reduce_v", (3.7, v.LANCZOS3))])
+    assert got.shape == (208, 277, 3)
+    assert_same(got, exp, "reduce 3.7")
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_thumbnails(ctx, seed):
+    r = np.random.default_rng(1000 + seed)
+    w, h = int(r.integers(20, 1600)), int(r.integers(20, 1600))
+    b = int(r.integers(1, 5))
+    target = [v.FITBOX, v.WIDTH, v.HEIGHT][seed % 3]
+    tw = int(r.integers(8, 500))
+    src = noise_u8(h, w, b, seed)
+    steps = [("thumbnail", (target, tw, tw, v.LANCZOS3))]
+    try:
+        O.Pipeline(w, h, b).thumbnail(target, tw, tw)
+    except O.OracleError as e:
+        assert e.code == O.EUNSUPPORTED
+        with pytest.raises(v.VgpuError):
+            v.Pipeline(ctx, w, h, b).thumbnail(target, tw, tw)
+        return
+    got, exp = run_both(ctx, src, steps)
+    assert_same(got, exp, f"thumbnail {w}x{h}x{b} -> {tw}")
+
+
+@pytest.mark.parametrize("kernel", [v.NEAREST, v.BILINEAR, v.BICUBIC, v.LANCZOS2, v.LANCZOS3])
+def test_thumbnail_kernels_with_blur(ctx, kernel):
+    src = gradient_noise_u8(731, 997, 3, 77)
+    got, exp = run_both(ctx, src, [("thumbnail", (v.WIDTH, 256, 0, kernel)), ("gauss_blur", (0.8,))])
+    assert_same(got, exp, f"thumbnail+blur kernel {kernel}")
+
+
+def test_thumbnail_rgba_premultiplied_chain(ctx):
+    src = noise_u8(613, 1207, 4, 99)
+    src[..., 3] = np.where(np.random.default_rng(3).random((613, 1207)) < 0.1, 0, src[..., 3])
+    got, exp = run_both(ctx, src, [("thumbnail", (v.FITBOX, 400, 400, v.LANCZOS3))])
+    assert_same(got, exp, "rgba thumbnail")
+
+
+@pytest.mark.parametrize("hs,vs,kernel", [(0.5, 0.5, v.BILINEAR), (0.3, 0.41, v.LANCZOS3),
+                                          (0.137, 0.09, v.BICUBIC), (1.0, 0.25, v.LANCZOS2)])
+def test_resize_downscale(ctx, hs, vs, kernel):
+    src = noise_u8(555, 777, 3, 5)
+    got, exp = run_both(ctx, src, [("resize", (hs, vs, kernel))])
+    assert_same(got, exp, f"resize {hs} {vs}")
+
+
+@pytest.mark.parametrize("sigma,bands", [(0.8, 3), (2.0, 4), (1.5, 1), (8.0, 3), (0.1, 2)])
+def test_gauss_u8(ctx, sigma, bands):
+    src = noise_u8(301, 403, bands, 31)
+    got, exp = run_both(ctx, src, [("gauss_blur", (sigma,))])
+    assert_same(got, exp, f"gauss {sigma}")
+
+
+@pytest.mark.parametrize("f", [1, 2, 3, 5, 7, 19, 40])
+@pytest.mark.parametrize("bands", [1, 2, 3, 4, 5])
+def test_shrink_factors_and_bands(ctx, f, bands):
+    src = noise_u8(97, 211, bands, f * 10 + bands)
+    got, exp = run_both(ctx, src, [("shrink_h", (f,)), ("shrink_v", (f,))])
+    assert_same(got, exp, f"shrink {f} bands {bands}")
+    got, exp = run_both(ctx, src, [("shrink_v", (f,)), ("shrink_h", (f, True))])
+    assert_same(got, exp, f"shrink v/h ceil {f} bands {bands}")
+
+
+def test_u16_paths(ctx):
+    src = noise_u16(211, 307, 3, 8)
+    for steps in ([("reduce_h", (2.7, v.LANCZOS3)), ("reduce_v", (1.9, v.BICUBIC))],
+                  [("shrink_h", (3,)), ("shrink_v", (2,))],
+                  [("thumbnail", (v.WIDTH, 100, 0, v.LANCZOS3))],
+                  [("gauss_blur", (1.2,))]):
+        got, exp = run_both(ctx, src, steps)
+        assert_same(got, exp, f"u16 {steps}")
+
+
+def test_f32_reference_exact(ctx):
+    # The f32 kernels carry the reference's FP64 accumulation in its
+    # operation order: bit-identical results, for 1 band (AVX2 grouping) and
+    # 3 bands (scalar FMA chain), including the ReduceV vector/tail split.
+    for b in (1, 3):
+        src = float_image(203, 257, b, 12 + b)
+        for steps in ([("reduce_h", (2.5, v.LANCZOS3)), ("reduce_v", (2.5, v.LANCZOS3))],
+                      [("gauss_blur", (8.0,))],
+                      [("shrink_h", (3,)), ("shrink_v", (3,))],
+                      [("convsep", ([1.0, 2.0, 3.0, 2.0, 1.0], 9.0, 0.5))]):
+            got, exp = run_both(ctx, src, steps)
+            assert_same(got, exp, f"f32 bands {b} {steps}")
+
+
+@pytest.mark.parametrize("w,h,b", [(1, 1, 3), (2, 1, 1), (1, 5, 4), (7, 3, 2), (19, 19, 3), (20, 1000, 1)])
+def test_tiny_and_ragged(ctx, w, h, b):
+    src = noise_u8(h, w, b, w * 31 + h)
+    for steps in ([("reduce_h", (1.7, v.LANCZOS3))], [("reduce_v", (2.2, v.BICUBIC))],
+                  [("gauss_blur", (2.0,))], [("shrink_h", (2,)), ("shrink_v", (3, True))]):
+        try:
+            O.Pipeline(w, h, b)
+            got, exp = run_both(ctx, src, steps, device=False)
+        except O.OracleError:
+            continue
+        assert_same(got, exp, f"{w}x{h}x{b} {steps}")
+
+
+# ------------------------------------------------------------ op level
+
+def test_op_process_region_matches_oracle_reduce_h_offsets(ctx):
+    import torch
+
+    src = noise_u8(9, 300, 3, 4)
+    op = v.Op.reduce_h(ctx, 2.3, v.LANCZOS3, 3, 300)
+    for x0 in (-5, 0, 17):
+        out_r = v.Region(x0, 0, 100, 9)
+        in_r = op.required_input_region(out_r)
+        # Materialise the input tile with clamp-to-edge like MemorySource.
+        xs = np.clip(np.arange(in_r.x, in_r.x + in_r.width), 0, 299)
+        tile = np.ascontiguousarray(src[:, xs, :])
+        dst = torch.zeros((9, 100, 3), dtype=torch.uint8, device="cuda")
+        d_tile = torch.from_numpy(tile).cuda()
+        torch.cuda.synchronize()
+        op.process_region(d_tile, dst, in_r, out_r)
+        ctx.synchronize()
+        if x0 == 0:
+            exp = O.Pipeline(300, 9, 3).reduce_h(2.3).run(src)[:, :100]
+            assert np.array_equal(dst.cpu().numpy(), exp)
+
+
+def test_op_premultiply_exhaustive(ctx):
+    import torch
+
+    a = np.repeat(np.arange(256, dtype=np.uint8), 256)
+    c = np.tile(np.arange(256, dtype=np.uint8), 256)
+    img = np.stack([c, 255 - c, c // 2, a], axis=-1).reshape(256, 256, 4)
+    for kind in ("premultiply", "unpremultiply"):
+        op = getattr(v.Op, kind)(ctx, 4)
+        r = v.Region(0, 0, 256, 256)
+        dst = torch.zeros((256, 256, 4), dtype=torch.uint8, device="cuda")
+        d_img = torch.from_numpy(img).cuda()
+        torch.cuda.synchronize()
+        op.process_region(d_img, dst, r, r)
+        ctx.synchronize()
+        exp = getattr(O.Pipeline(256, 256, 4), kind)().run(img)
+        assert np.array_equal(dst.cpu().numpy(), exp), kind
+
+
+def test_op_rejects_bad_lengths_and_regions(ctx):
+    import torch
+
+    op = v.Op.gauss_blur_h(ctx, 2.0, 3)
+    out_r = v.Region(0, 0, 10, 2)
+    in_r = op.required_input_region(out_r)
+    assert (in_r.x, in_r.width) == (-3, 16)
+    src = torch.zeros((2, 16, 3), dtype=torch.uint8, device="cuda")
+    with pytest.raises(v.VgpuError) as e:
+        op.process_region(src, torch.zeros((2, 9, 3), dtype=torch.uint8, device="cuda"), in_r, out_r)
+    assert e.value.status == v.api.E_BUFFER_SIZE
+    with pytest.raises(v.VgpuError):
+        op.process_region(src, torch.zeros((2, 10, 3), dtype=torch.uint8, device="cuda"),
+                          v.Region(0, 0, 16, 2), out_r)
+
+
+def test_builder_validation_errors(ctx):
+    p = v.Pipeline(ctx, 100, 100, 3)
+    with pytest.raises(v.VgpuError) as e:
+        p.reduce_h(0.5, v.LANCZOS3)
+    assert e.value.status == v.api.E_INVALID_ARGUMENT
+    with pytest.raises(v.VgpuError) as e:
+        p.reduce_h(2.0, v.LBB)
+    assert e.value.status == v.api.E_INVALID_KERNEL
+    with pytest.raises(v.VgpuError):
+        p.reduce_v(400.0, v.LANCZOS3)  # over the 2000-tap limit
+    with pytest.raises(v.VgpuError):
+        v.Pipeline(ctx, 10, 10, 0)

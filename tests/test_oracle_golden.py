@@ -1,0 +1,337 @@
+"""Pin the CPU oracle before trusting it (CPU only).
+
+Checks the oracle against (1) the reference's committed fixture files,
+(2) the reference's in-crate known answers, (3) invariants and closed forms
+that a dropped term, a wrong sign/index or a transposed operand would break.
+"""
+from fractions import Fraction
+import pathlib
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import (grayscale_source, noise_u8, rgb_source, smooth_grayscale_source,
+                           structural_source)
+
+FIX = pathlib.Path(__file__).resolve().parent / "golden" / "reference_fixtures"
+
+
+def fixture_u8(rel):
+    return np.fromfile(FIX / rel, np.uint8)
+
+
+# ---------------------------------------------------------------- fixtures
+
+def _gray8x8_prefix(w, h):
+    # grayscale_source() is 8x8; the tests take its first w*h samples
+    # (tests/functional/golden/libvips/resample.rs:124-132, 157-165).
+    return grayscale_source(8, 8).ravel()[: w * h].reshape(h, w, 1)
+
+
+@pytest.mark.parametrize("rel,w,h,b,op,factor,kernel,src", [
+    ("reduceh_libvips/grayscale_factor1_5_lanczos3.bin", 8, 6, 1, "h", 1.5, O.LANCZOS3, "gray_prefix"),
+    ("reducev_libvips/grayscale_factor1_5_lanczos3.bin", 6, 8, 1, "v", 1.5, O.LANCZOS3, "gray_prefix"),
+    ("reduceh_libvips/grayscale_factor1_5_cubic.bin", 8, 6, 1, "h", 1.5, O.BICUBIC, "gray"),
+    ("reducev_libvips/grayscale_factor1_5_cubic.bin", 6, 8, 1, "v", 1.5, O.BICUBIC, "gray"),
+    ("reduceh_libvips/rgb_factor2_linear.bin", 8, 6, 3, "h", 2.0, O.BILINEAR, "rgb"),
+    ("reducev_libvips/rgb_factor2_linear.bin", 6, 8, 3, "v", 2.0, O.BILINEAR, "rgb"),
+    ("libvips_parity_resample/test_reduceh_factor_1_5_lanczos3.bin", 8, 6, 1, "h", 1.5, O.LANCZOS3, "smooth"),
+    ("libvips_parity_resample/test_reducev_factor_1_5_lanczos3.bin", 6, 8, 1, "v", 1.5, O.LANCZOS3, "smooth"),
+])
+def test_reduce_matches_reference_fixture(rel, w, h, b, op, factor, kernel, src):
+    img = {"gray_prefix": lambda: _gray8x8_prefix(w, h), "gray": lambda: grayscale_source(w, h),
+           "rgb": lambda: rgb_source(w, h), "smooth": lambda: smooth_grayscale_source(w, h)}[src]()
+    p = O.Pipeline(w, h, b)
+    (p.reduce_h if op == "h" else p.reduce_v)(factor, kernel)
+    assert np.array_equal(p.run(img).ravel(), fixture_u8(rel))
+
+
+def test_reduce_v_then_h_edge_fixture():
+    # reduce_4x4_edge_pixels_match_libvips (golden/libvips/resample.rs:214-248)
+    src = np.arange(16, dtype=np.uint8).reshape(4, 4, 1)
+    out = O.Pipeline(4, 4, 1).reduce_v(2.0, O.BILINEAR).reduce_h(2.0, O.BILINEAR).run(src)
+    assert np.array_equal(out.ravel(), fixture_u8("reduce_libvips/edge_4x4_factor2_linear.bin"))
+
+
+@pytest.mark.parametrize("rel,op", [("shrinkh/grayscale_factor_2.bin", "h"),
+                                    ("shrinkv/grayscale_factor_2.bin", "v")])
+def test_shrink_factor2_fixture(rel, op):
+    p = O.Pipeline(10, 16, 1)
+    (p.shrink_h if op == "h" else p.shrink_v)(2)
+    assert np.array_equal(p.run(structural_source(10, 16)).ravel(), fixture_u8(rel))
+
+
+@pytest.mark.parametrize("rel", ["shrink_libvips/rgb_factor4.bin",
+                                 "libvips_parity_resample/test_shrink_factor4_rgb.bin"])
+def test_shrink_factor4_rgb_fixture(rel):
+    out = O.Pipeline(16, 16, 3).shrink_v(4).shrink_h(4).run(rgb_source(16, 16))
+    assert np.array_equal(out.ravel(), fixture_u8(rel))
+
+
+def test_gauss_f32_uniform_fixture():
+    # gauss_blur_matches_libvips_fixture (golden/basic.rs:303-315), tolerance 1e-3
+    exp = np.fromfile(FIX / "gauss_blur/uniform_sigma_1_5.bin", np.float32)
+    src = np.full((64, 64, 1), 37.5, np.float32)
+    out = O.Pipeline(64, 64, 1, O.F32).gauss_blur(1.5).run(src).ravel()
+    assert np.max(np.abs(out - exp)) <= 1e-3
+
+
+def test_thumbnail_libvips_fixture_within_reference_tolerance():
+    # libvips_parity_upstream_resample_thumbnail_width_target
+    # (parity/resample.rs:231-275): 512^2 -> 128^2, tolerance 16 vs libvips.
+    src = smooth_grayscale_source(512, 512)
+    out = O.Pipeline(512, 512, 1).thumbnail(O.WIDTH, 128).run(src)
+    exp = fixture_u8("libvips_parity_resample/test_thumbnail_width_128_square.bin")
+    assert out.shape == (128, 128, 1)
+    assert np.max(np.abs(out.ravel().astype(int) - exp.astype(int))) <= 16
+
+
+# --------------------------------------------------------- known answers
+
+def test_reduce_known_answers():
+    # reduceh.rs:596-610, reducev.rs:532-545
+    grad = np.arange(8, dtype=np.uint8)
+    assert list(O.Pipeline(8, 1, 1).reduce_h(2.0, O.BILINEAR).run(grad.reshape(1, 8, 1)).ravel()) == [1, 3, 5, 6]
+    assert list(O.Pipeline(1, 8, 1).reduce_v(2.0, O.BILINEAR).run(grad.reshape(8, 1, 1)).ravel()) == [1, 3, 5, 6]
+    px = np.array([10, 20, 30, 40, 50, 60, 70, 80, 90, 100, 110, 120], np.uint8)
+    assert list(O.Pipeline(2, 2, 3).reduce_h(2.0, O.BILINEAR).run(px.reshape(2, 2, 3)).ravel()) == [25, 35, 45, 85, 95, 105]
+    assert list(O.Pipeline(1, 4, 3).reduce_v(2.0, O.BILINEAR).run(px.reshape(4, 1, 3)).ravel()) == [29, 39, 49, 81, 91, 101]
+    # reducev.rs:549-570: the 30-value Lanczos3 x1.5 list
+    exp = [57, 74, 25, 42, 59, 76, 98, 115, 100, 117, 51, 68, 47, 64, 89, 106, 130, 147, 126,
+           143, 77, 94, 79, 96, 118, 135, 152, 169, 120, 137]
+    assert list(O.Pipeline(6, 8, 1).reduce_v(1.5).run(_gray8x8_prefix(6, 8)).ravel()) == exp
+
+
+@pytest.mark.parametrize("kernel", [O.NEAREST, O.BILINEAR, O.BICUBIC, O.CATMULLROM, O.LANCZOS2, O.LANCZOS3])
+def test_single_pixel_edge_copy(kernel):
+    # reduceh.rs:580-594, reducev.rs:517-530
+    assert O.Pipeline(1, 1, 1).reduce_h(2.0, kernel).run(np.array([[[173]]], np.uint8)).ravel()[0] == 173
+    assert O.Pipeline(1, 1, 1).reduce_v(2.0, kernel).run(np.array([[[91]]], np.uint8)).ravel()[0] == 91
+
+
+def test_shrink_known_answers():
+    # shrinkh.rs:1479-1506
+    assert list(O.Pipeline(4, 1, 1).shrink_h(2).run(np.array([10, 20, 30, 40], np.uint8).reshape(1, 4, 1)).ravel()) == [15, 35]
+    rgb = np.array([10, 20, 30, 20, 40, 60, 30, 60, 90, 40, 80, 120], np.uint8).reshape(1, 4, 3)
+    assert list(O.Pipeline(4, 1, 3).shrink_h(2).run(rgb).ravel()) == [15, 30, 45, 35, 70, 105]
+    f5 = np.array([10, 20, 30, 20, 30, 40, 30, 40, 50, 40, 50, 60, 50, 60, 70, 100, 110, 120, 110, 120,
+                   130, 120, 130, 140, 130, 140, 150, 140, 150, 160], np.uint8).reshape(1, 10, 3)
+    assert list(O.Pipeline(10, 1, 3).shrink_h(5).run(f5).ravel()) == [30, 40, 50, 120, 130, 140]
+    # shrinkh.rs:1792-1806: factor-5 RGB with the 239 tail
+    t = np.array([10, 20, 30, 20, 30, 40, 30, 40, 50, 40, 50, 60, 50, 60, 70, 100, 110, 120, 110, 120,
+                  130, 120, 130, 140, 130, 140, 150, 140, 150, 160, 200, 210, 220, 210, 220, 230, 220,
+                  230, 240, 230, 240, 250, 240, 250, 255], np.uint8).reshape(1, 15, 3)
+    assert list(O.Pipeline(15, 1, 3).shrink_h(5).run(t).ravel()) == [30, 40, 50, 120, 130, 140, 220, 230, 239]
+    rgba = np.arange(1, 41, dtype=np.uint8).reshape(1, 10, 4)
+    assert list(O.Pipeline(10, 1, 4).shrink_h(5).run(rgba).ravel()) == [9, 10, 11, 12, 29, 30, 31, 32]
+    # shrinkh.rs:1895-1901: generic dispatch, 2 bands, factor 3
+    two = np.array([10, 100, 20, 110, 30, 120, 40, 130, 50, 140, 60, 150], np.uint8).reshape(1, 6, 2)
+    assert list(O.Pipeline(6, 1, 2).shrink_h(3).run(two).ravel()) == [20, 110, 50, 140]
+
+
+def test_multiply_shift_is_not_exact_rounding():
+    # f = 3, sum 2 -> (3 * 5592405) >> 24 = 0 although 2/3 rounds to 1: the
+    # libvips UCHAR_SHRINK rounding the reference keeps.
+    src = np.array([1, 1, 0], np.uint8).reshape(1, 3, 1)
+    assert O.Pipeline(3, 1, 1).shrink_h(3).run(src).ravel()[0] == 0
+
+
+# ------------------------------------------------ tables and invariants
+
+@pytest.mark.parametrize("kernel", [O.NEAREST, O.BILINEAR, O.BICUBIC, O.CATMULLROM, O.LANCZOS2, O.LANCZOS3])
+@pytest.mark.parametrize("factor", [1.0, 1.5, 2.0, 2.5, 4.0, 3.7, 1.0778947368421052])
+def test_reduce_table_invariants(kernel, factor):
+    # reduce_common.rs:332-372
+    cf, ci = O.reduce_tables(kernel, factor)
+    taps = cf.shape[1]
+    assert np.all(np.abs(cf.sum(axis=1) - 1.0) < 1e-12)
+    assert np.all(np.abs(ci.astype(np.int64).sum(axis=1) - 4096) <= taps)
+    # i16 = trunc(w * 4096): toward zero, never rounded
+    assert np.array_equal(ci, np.trunc(cf * 4096.0).astype(np.int16))
+
+
+def test_tap_counts_appendix():
+    for f, taps in [(3.7, 23), (2.5, 17), (2.0, 13), (1.5, 11), (1.0778947368421052, 7), (1.0, 1)]:
+        assert O.lib().orc_reduce_taps(O.LANCZOS3, f) == taps
+    assert O.lib().orc_reduce_taps(O.LANCZOS3, 400.0) == -1  # reduce_common.rs:414-421
+
+
+def test_lanczos3_weights_closed_form():
+    # Independent closed form: sinc(x) * sinc(x/3) with numpy's sinc.
+    for x in [0.1, 0.5, 1.3, 2.2, 2.9]:
+        exp = np.sinc(x) * np.sinc(x / 3.0)
+        assert abs(O.lib().orc_kernel_weight(O.LANCZOS3, x) - exp) < 1e-14
+    assert O.lib().orc_kernel_weight(O.LANCZOS3, 3.0) == 0.0
+    # Keys a = -0.5 cubic: (a+2)|x|^3 - (a+3)|x|^2 + 1 and a|x|^3 - 5a|x|^2 + 8a|x| - 4a
+    for x in [0.25, 0.75, 1.25, 1.75]:
+        a = -0.5
+        exp = ((a + 2) * x**3 - (a + 3) * x**2 + 1) if x < 1 else (a * x**3 - 5 * a * x**2 + 8 * a * x - 4 * a)
+        assert abs(O.lib().orc_kernel_weight(O.BICUBIC, x) - exp) < 1e-14
+
+
+def test_factor2_offset_and_start():
+    # reduce_common.rs:375-384: offset -0.5, sp(0) = 1, start = -1
+    assert abs(O.lib().orc_reduce_offset(O.BILINEAR, 2.0, 8) + 0.5) < 1e-12
+    starts, _ = O.reduce_plan(O.BILINEAR, 2.0, 8, 0.0, False, 1)
+    assert starts[0] == -1
+
+
+@pytest.mark.parametrize("kernel", [O.BILINEAR, O.BICUBIC, O.LANCZOS2, O.LANCZOS3])
+@pytest.mark.parametrize("factor", [1.0, 1.5, 2.0, 2.5, 3.5, 4.0, 8.0])
+def test_uniform_rows_stay_uniform(kernel, factor):
+    # reduceh.rs:806-830 (proptests restated over all values)
+    for v in [0, 1, 77, 128, 254, 255]:
+        row = np.full((1, 64, 1), v, np.uint8)
+        assert np.all(O.Pipeline(64, 1, 1).reduce_h(factor, kernel).run(row) == v)
+
+
+def test_factor1_is_identity():
+    src = noise_u8(5, 32, 3, 1)
+    for k in [O.BILINEAR, O.LANCZOS3]:
+        assert np.array_equal(O.Pipeline(32, 5, 3).reduce_h(1.0, k).run(src), src)
+
+
+def test_plan_accumulation_differs_from_direct_only_by_drift():
+    # u8 ReduceH plans by f64 accumulation (reduceh.rs:112-119); it must agree
+    # with the direct position except where the drift crosses a boundary.
+    # The quantised position start * 64 + phase may move by at most one 1/64
+    # step; at f = 3.7 several columns land on (start + 1, phase 0) instead of
+    # (start, phase 64), the same position expressed with a shifted window.
+    s1, p1 = O.reduce_plan(O.LANCZOS3, 3.7, 1024, 0.0, True, 277)
+    s2, p2 = O.reduce_plan(O.LANCZOS3, 3.7, 1024, 0.0, False, 277)
+    q1 = s1 * 64 + p1.astype(np.int64)
+    q2 = s2 * 64 + p2.astype(np.int64)
+    assert np.all(np.abs(q1 - q2) <= 1)
+    assert np.any(s1 != s2)
+
+
+# ------------------------------------------------------------ Gaussian
+
+def test_gauss_masks_appendix():
+    c, s = O.gauss_int_kernel(1.5)
+    assert c.tolist() == [8, 16, 20, 16, 8] and s == 68  # gauss_blur.rs:56-65
+    c, s = O.gauss_int_kernel(2.0)
+    assert c.tolist() == [6, 12, 18, 20, 18, 12, 6] and s == 92
+    c, s = O.gauss_int_kernel(0.8)
+    assert c.tolist() == [9, 20, 9] and s == 38
+    c, s = O.gauss_int_kernel(8.0)
+    assert len(c) == 29 and s == 372
+    c, s = O.gauss_int_kernel(0.1)
+    assert c.tolist() == [1] and s == 1
+    # taps = round_ties_even(20 exp(-x^2 / 2 sigma^2)), closed form
+    for sigma in [0.8, 1.0, 2.0, 3.3, 8.0]:
+        c, s = O.gauss_int_kernel(sigma)
+        r = (len(c) - 1) // 2
+        sig = float(np.float32(sigma))
+        exp = np.rint(20 * np.exp(-(np.arange(-r, r + 1) ** 2) / (2 * sig * sig))).astype(int)
+        assert c.tolist() == exp.tolist() and s == exp.sum()
+
+
+@pytest.mark.parametrize("sigma,bands", [(0.8, 3), (1.5, 1), (2.0, 4), (3.0, 3)])
+def test_gauss_u8_matches_bruteforce_2d(sigma, bands):
+    # naive_integer_separable_blur_u8 (gauss_blur.rs:2044-2092): clamp-padded
+    # H pass to u8, then V pass -- written here with numpy padding.
+    src = noise_u8(23, 31, bands, 7)
+    c, s = O.gauss_int_kernel(sigma)
+    r = (len(c) - 1) // 2
+    pad = np.pad(src.astype(np.int64), ((0, 0), (r, r), (0, 0)), mode="edge")
+    hz = sum(int(c[t]) * pad[:, t:t + 31, :] for t in range(len(c)))
+    hz = np.clip((hz + s // 2) // s, 0, 255)
+    pad = np.pad(hz, ((r, r), (0, 0), (0, 0)), mode="edge")
+    vt = sum(int(c[t]) * pad[t:t + 23, :, :] for t in range(len(c)))
+    exp = np.clip((vt + s // 2) // s, 0, 255).astype(np.uint8)
+    out = O.Pipeline(31, 23, bands).gauss_blur(sigma).run(src)
+    assert np.array_equal(out, exp)
+
+
+# ------------------------------------------------------------- alpha
+
+def test_premultiply_exhaustive_against_exact_rationals():
+    # premultiply.rs:103-115 in f64 vs exact round-half-away of c*a/255;
+    # unpremultiply.rs:96-109 vs exact round-half-away of c*255/a.
+    a = np.repeat(np.arange(256, dtype=np.uint8), 256)
+    c = np.tile(np.arange(256, dtype=np.uint8), 256)
+    img = np.stack([c, c, c, a], axis=-1).reshape(256, 256, 4)
+    pm = O.Pipeline(256, 256, 4).premultiply().run(img).reshape(-1, 4)
+    um = O.Pipeline(256, 256, 4).unpremultiply().run(img).reshape(-1, 4)
+
+    def rha(fr):  # round half away from zero for a non-negative Fraction
+        return int(fr + Fraction(1, 2)) if fr >= 0 else -int(-fr + Fraction(1, 2))
+
+    for i in range(0, 65536, 97):
+        ci, ai = int(c[i]), int(a[i])
+        assert pm[i, 0] == min(255, rha(Fraction(ci * ai, 255)))
+        assert um[i, 0] == (0 if ai == 0 else min(255, rha(Fraction(ci * 255, ai))))
+        assert pm[i, 3] == ai and um[i, 3] == ai
+    # unpremultiply.rs:239-248: 64 * 255 / 128 = 127.5 -> 128
+    px = np.array([64, 0, 128], np.uint8).reshape(1, 1, 3)
+    assert O.Pipeline(1, 1, 3).unpremultiply().run(px).ravel().tolist() == [128, 0, 128]
+
+
+# ------------------------------------------------------------ planners
+
+def test_thumbnail_planner_shapes():
+    nodes, dims = O.thumbnail_plan(8192, 8192, 3, O.WIDTH, 400)
+    assert [n for n, _ in nodes] == ["ShrinkH", "ShrinkV", "ReduceH", "ReduceV"]
+    assert nodes[0][1] == 19 and nodes[2][1] == 20.48 / 19 and dims == (400, 400)
+    # thumbnail.rs:588-606: 2048 -> 400 shrinks by 4 with no Affine
+    nodes, dims = O.thumbnail_plan(2048, 2048, 3, O.WIDTH, 400)
+    assert nodes[0] == ("ShrinkH", 4.0) and nodes[1] == ("ShrinkV", 4.0)
+    # thumbnail.rs:629-646: RGBA wraps in premultiply/unpremultiply
+    nodes, _ = O.thumbnail_plan(37, 19, 4, O.WIDTH, 11)
+    assert nodes[0][0] == "Premultiply" and nodes[-1][0] == "Unpremultiply"
+    # Appendix B.3
+    nodes, dims = O.thumbnail_plan(6000, 4000, 3, O.FITBOX, 400, 400)
+    assert nodes == [("ShrinkH", 15.0), ("ShrinkV", 12.0), ("ReduceV", 4000 / 267 / 12)] and dims == (400, 267)
+    # 1-pixel inputs take the Affine-only path (not on the GPU path)
+    with pytest.raises(O.OracleError):
+        O.thumbnail_plan(1, 2048, 3, O.HEIGHT, 0, 400)
+
+
+def test_integer_shrink_never_undershoots():
+    # thumbnail.rs:608-627, restated over the planner's output
+    for tw in range(1, 2049, 7):
+        try:
+            nodes, dims = O.thumbnail_plan(2048, 2048, 1, O.WIDTH, tw)
+        except O.OracleError:
+            continue  # Affine tail
+        assert dims[0] == tw
+
+
+# ------------------------------------------------------ chain executor
+
+@pytest.mark.parametrize("strip", [1, 3, 16, 1000])
+def test_chain_is_tile_height_invariant(strip):
+    src = noise_u8(197, 233, 3, 11)
+    base = O.Pipeline(233, 197, 3).thumbnail(O.WIDTH, 64).gauss_blur(0.8).run(src, strip_height=16)
+    out = O.Pipeline(233, 197, 3).thumbnail(O.WIDTH, 64).gauss_blur(0.8).run(src, strip_height=strip)
+    assert np.array_equal(base, out)
+
+
+def test_single_reduce_equals_clamp_padded_closed_form():
+    # One ReduceV on a source is a clamp-padded 1-D filter: compare with numpy
+    # edge padding and the table rows (x86 AVX2: direct per-row plan).
+    src = noise_u8(50, 7, 2, 3)
+    f = 2.3
+    _, ci = O.reduce_tables(O.LANCZOS3, f)
+    starts, phases = O.reduce_plan(O.LANCZOS3, f, 50, 0.0, False, int(O.lib().orc_reduce_output_len(f, 50)))
+    taps = ci.shape[1]
+    pad = 64
+    padded = np.pad(src.astype(np.int64), ((pad, pad), (0, 0), (0, 0)), mode="edge")
+    exp = np.stack([
+        np.clip((sum(int(ci[p, t]) * padded[s + t + pad] for t in range(taps)) + 2048) >> 12, 0, 255)
+        for s, p in zip(starts, phases)]).astype(np.uint8)
+    out = O.Pipeline(7, 50, 2).reduce_v(f).run(src)
+    assert np.array_equal(out, exp)
+
+
+def test_thumbnail_then_blur_differs_from_separate_ops_at_borders():
+    # Intermediates are evaluated on Z^2 (execute.rs:413-480): chaining
+    # thumbnail -> gauss_blur is NOT thumbnail, then blur of the clamped result.
+    src = noise_u8(300, 300, 3, 5)
+    chained = O.Pipeline(300, 300, 3).thumbnail(O.WIDTH, 70).gauss_blur(2.0).run(src)
+    thumb = O.Pipeline(300, 300, 3).thumbnail(O.WIDTH, 70).run(src)
+    separate = O.Pipeline(70, 70, 3).gauss_blur(2.0).run(thumb)
+    assert np.array_equal(chained[5:-5, 5:-5], separate[5:-5, 5:-5])
+    assert not np.array_equal(chained, separate)

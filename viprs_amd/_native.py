@@ -1,0 +1,94 @@
+"""ctypes binding of libviprs_amd.so (the C ABI declared in include/viprs_amd.h).
+
+There is no CPU fallback: if the gfx950 library is missing, importing the
+binding raises, and every entry point needs a HIP device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import re
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libviprs_amd.so"
+HEADER = HERE.parent / "include" / "viprs_amd.h"
+
+
+class Region(C.Structure):
+    """Region (crates/viprs-core/src/image/region.rs:18-27)."""
+
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+    def __repr__(self) -> str:
+        return f"Region({self.x}, {self.y}, {self.width}, {self.height})"
+
+    def as_tuple(self):
+        return (self.x, self.y, self.width, self.height)
+
+
+_lib = None
+
+
+def declared_symbols() -> list[str]:
+    """Every function name the public header declares."""
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"\b(vgpu_[a-z0-9_]+)\s*\(", text)))
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m viprs_amd.build` "
+            "(the GPU path has no CPU fallback)")
+    L = C.CDLL(str(LIB_PATH))
+    vp, u32, i32, dbl, sz, f32 = C.c_void_p, C.c_uint32, C.c_int, C.c_double, C.c_size_t, C.c_float
+    P = C.POINTER
+    sig = {
+        "vgpu_status_string": (C.c_char_p, [i32]),
+        "vgpu_context_create": (i32, [i32, P(vp)]),
+        "vgpu_context_destroy": (None, [vp]),
+        "vgpu_context_set_stream": (i32, [vp, vp]),
+        "vgpu_context_stream": (vp, [vp]),
+        "vgpu_synchronize": (i32, [vp]),
+        "vgpu_context_pool_bytes": (sz, [vp]),
+        "vgpu_context_trim": (i32, [vp]),
+        "vgpu_pipeline_new": (i32, [vp, u32, u32, u32, i32, P(vp)]),
+        "vgpu_pipeline_destroy": (None, [vp]),
+        "vgpu_pipeline_reduce_h": (i32, [vp, dbl, i32]),
+        "vgpu_pipeline_reduce_v": (i32, [vp, dbl, i32]),
+        "vgpu_pipeline_shrink_h": (i32, [vp, u32, i32]),
+        "vgpu_pipeline_shrink_v": (i32, [vp, u32, i32]),
+        "vgpu_pipeline_gauss_blur": (i32, [vp, f32]),
+        "vgpu_pipeline_convsep": (i32, [vp, P(dbl), u32, dbl, dbl]),
+        "vgpu_pipeline_premultiply": (i32, [vp]),
+        "vgpu_pipeline_unpremultiply": (i32, [vp]),
+        "vgpu_pipeline_thumbnail": (i32, [vp, i32, u32, u32, i32]),
+        "vgpu_pipeline_resize": (i32, [vp, dbl, dbl, i32]),
+        "vgpu_pipeline_output_info": (i32, [vp, P(u32), P(u32), P(u32), P(i32)]),
+        "vgpu_pipeline_stage_count": (i32, [vp]),
+        "vgpu_pipeline_run": (i32, [vp, vp, sz, vp, sz]),
+        "vgpu_pipeline_run_host": (i32, [vp, vp, sz, vp, sz]),
+        "vgpu_pipeline_run_profiled": (i32, [vp, vp, sz, vp, sz, i32, P(f32), i32]),
+        "vgpu_op_reduce_h_new": (i32, [vp, dbl, i32, u32, u32, i32, P(vp)]),
+        "vgpu_op_reduce_v_new": (i32, [vp, dbl, i32, u32, u32, i32, P(vp)]),
+        "vgpu_op_shrink_h_new": (i32, [vp, u32, i32, u32, u32, i32, P(vp)]),
+        "vgpu_op_shrink_v_new": (i32, [vp, u32, i32, u32, i32, P(vp)]),
+        "vgpu_op_gauss_blur_h_new": (i32, [vp, f32, u32, i32, P(vp)]),
+        "vgpu_op_gauss_blur_v_new": (i32, [vp, f32, u32, i32, P(vp)]),
+        "vgpu_op_premultiply_new": (i32, [vp, u32, i32, P(vp)]),
+        "vgpu_op_unpremultiply_new": (i32, [vp, u32, i32, P(vp)]),
+        "vgpu_op_destroy": (None, [vp]),
+        "vgpu_op_required_input_region": (i32, [vp, Region, P(Region)]),
+        "vgpu_op_output_size": (i32, [vp, u32, u32, P(u32), P(u32)]),
+        "vgpu_op_output_format": (i32, [vp, P(i32)]),
+        "vgpu_op_process_region": (i32, [vp, vp, sz, vp, sz, Region, Region]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L

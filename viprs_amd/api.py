@@ -1,0 +1,270 @@
+"""Python host mirror of the reference's builder / operation API over the C ABI.
+
+Names follow the reference: ``Pipeline`` mirrors ``PipelineBuilder``
+(crates/viprs-runtime/src/pipeline/builder/resample.rs, builder/colour.rs),
+``Op`` mirrors the ``DynOperation`` bridges (crates/viprs-core/src/op/
+dynamic_op.rs).  Build-time validation errors surface as ``VgpuError`` with
+the status code, the way the reference returns ``BuildError``.
+
+Device buffers are torch tensors on a HIP device (``tensor.data_ptr()``);
+``run_host`` takes and returns numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _native
+from ._native import Region
+
+U8, U16, F32 = 0, 1, 2
+NEAREST, BILINEAR, BICUBIC, LANCZOS2, LANCZOS3, CATMULLROM, LBB, VSQBS, NOHALO = range(9)
+FITBOX, FORCE_EXACT, WIDTH, HEIGHT = range(4)
+
+OK = 0
+E_INVALID_ARGUMENT = -1
+E_INVALID_KERNEL = -2
+E_UNSUPPORTED_FORMAT = -3
+E_UNSUPPORTED = -4
+E_BUFFER_SIZE = -5
+E_DEVICE = -6
+E_NO_MEMORY = -7
+
+NP_DTYPE = {U8: np.uint8, U16: np.uint16, F32: np.float32}
+SAMPLE_BYTES = {U8: 1, U16: 2, F32: 4}
+
+
+class VgpuError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        msg = _native.load().vgpu_status_string(status).decode()
+        super().__init__(f"{what}: {msg} ({status})")
+        self.status = status
+
+
+def _check(status: int, what: str) -> None:
+    if status != OK:
+        raise VgpuError(status, what)
+
+
+class Context:
+    """Device executor context: HIP device, stream, buffer pool."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _native.load()
+        h = C.c_void_p()
+        _check(self._lib.vgpu_context_create(device, C.byref(h)), "vgpu_context_create")
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, hip_stream: Optional[int]) -> None:
+        _check(self._lib.vgpu_context_set_stream(self._h, C.c_void_p(hip_stream or 0)), "set_stream")
+
+    @property
+    def stream(self) -> int:
+        return self._lib.vgpu_context_stream(self._h) or 0
+
+    def synchronize(self) -> None:
+        _check(self._lib.vgpu_synchronize(self._h), "vgpu_synchronize")
+
+    def pool_bytes(self) -> int:
+        return int(self._lib.vgpu_context_pool_bytes(self._h))
+
+    def trim(self) -> None:
+        _check(self._lib.vgpu_context_trim(self._h), "vgpu_context_trim")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.vgpu_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _nbytes(t) -> int:
+    return t.numel() * t.element_size()
+
+
+class Pipeline:
+    """PipelineBuilder over an in-memory source (MemorySource), run on the GPU."""
+
+    def __init__(self, ctx: Context, width: int, height: int, bands: int, fmt: int = U8):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        h = C.c_void_p()
+        _check(self._lib.vgpu_pipeline_new(ctx.handle, width, height, bands, fmt, C.byref(h)),
+               "vgpu_pipeline_new")
+        self._h = h
+        self.src_shape = (height, width, bands)
+        self.src_fmt = fmt
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.vgpu_pipeline_destroy(self._h)
+            self._h = None
+
+    # -- builder methods (each returns self, like PipelineBuilder's chaining)
+    def reduce_h(self, factor: float, kernel: int = LANCZOS3):
+        _check(self._lib.vgpu_pipeline_reduce_h(self._h, factor, kernel), "reduce_h")
+        return self
+
+    def reduce_v(self, factor: float, kernel: int = LANCZOS3):
+        _check(self._lib.vgpu_pipeline_reduce_v(self._h, factor, kernel), "reduce_v")
+        return self
+
+    def shrink_h(self, factor: int, ceil: bool = False):
+        _check(self._lib.vgpu_pipeline_shrink_h(self._h, factor, int(ceil)), "shrink_h")
+        return self
+
+    def shrink_v(self, factor: int, ceil: bool = False):
+        _check(self._lib.vgpu_pipeline_shrink_v(self._h, factor, int(ceil)), "shrink_v")
+        return self
+
+    def gauss_blur(self, sigma: float):
+        _check(self._lib.vgpu_pipeline_gauss_blur(self._h, sigma), "gauss_blur")
+        return self
+
+    def convsep(self, coeffs, scale: float = 1.0, offset: float = 0.0):
+        arr = (C.c_double * len(coeffs))(*[float(c) for c in coeffs])
+        _check(self._lib.vgpu_pipeline_convsep(self._h, arr, len(coeffs), scale, offset), "convsep")
+        return self
+
+    def premultiply(self):
+        _check(self._lib.vgpu_pipeline_premultiply(self._h), "premultiply")
+        return self
+
+    def unpremultiply(self):
+        _check(self._lib.vgpu_pipeline_unpremultiply(self._h), "unpremultiply")
+        return self
+
+    def thumbnail(self, target: int, width: int, height: int = 0, kernel: int = LANCZOS3):
+        _check(self._lib.vgpu_pipeline_thumbnail(self._h, target, width, height, kernel), "thumbnail")
+        return self
+
+    def resize(self, hscale: float, vscale: float, kernel: int = LANCZOS3):
+        _check(self._lib.vgpu_pipeline_resize(self._h, hscale, vscale, kernel), "resize")
+        return self
+
+    # -- compiled pipeline
+    def output_info(self):
+        w, h, b, f = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_int()
+        _check(self._lib.vgpu_pipeline_output_info(self._h, C.byref(w), C.byref(h), C.byref(b),
+                                                   C.byref(f)), "output_info")
+        return w.value, h.value, b.value, f.value
+
+    @property
+    def output_shape(self):
+        w, h, b, _ = self.output_info()
+        return (h, w, b)
+
+    def stage_count(self) -> int:
+        n = self._lib.vgpu_pipeline_stage_count(self._h)
+        if n < 0:
+            raise VgpuError(n, "stage_count")
+        return n
+
+    def run(self, src, dst) -> None:
+        """Enqueue on the context stream; src / dst are device tensors."""
+        _check(self._lib.vgpu_pipeline_run(self._h, C.c_void_p(src.data_ptr()), _nbytes(src),
+                                           C.c_void_p(dst.data_ptr()), _nbytes(dst)), "run")
+
+    def run_profiled(self, src, dst, iterations: int = 1):
+        """Mean device milliseconds per fused stage (HIP events)."""
+        buf = (C.c_float * 64)()
+        n = self._lib.vgpu_pipeline_run_profiled(self._h, C.c_void_p(src.data_ptr()), _nbytes(src),
+                                                 C.c_void_p(dst.data_ptr()), _nbytes(dst),
+                                                 iterations, buf, 64)
+        if n < 0:
+            raise VgpuError(n, "run_profiled")
+        return [buf[i] for i in range(n)]
+
+    def run_host(self, src: np.ndarray) -> np.ndarray:
+        src = np.ascontiguousarray(src, NP_DTYPE[self.src_fmt])
+        w, h, b, f = self.output_info()
+        dst = np.empty((h, w, b), NP_DTYPE[f])
+        _check(self._lib.vgpu_pipeline_run_host(self._h, C.c_void_p(src.ctypes.data), src.nbytes,
+                                                C.c_void_p(dst.ctypes.data), dst.nbytes), "run_host")
+        return dst
+
+
+class Op:
+    """One DynOperation bridge executed on the GPU (dyn_process_region)."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p, bands: int, in_fmt: int):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self._h = handle
+        self.bands = bands
+        self.in_fmt = in_fmt
+        f = C.c_int()
+        _check(self._lib.vgpu_op_output_format(self._h, C.byref(f)), "output_format")
+        self.out_fmt = f.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.vgpu_op_destroy(self._h)
+            self._h = None
+
+    @classmethod
+    def _make(cls, ctx, fn, args, bands, fmt, what):
+        h = C.c_void_p()
+        _check(getattr(ctx._lib, fn)(ctx.handle, *args, C.byref(h)), what)
+        return cls(ctx, h, bands, fmt)
+
+    @classmethod
+    def reduce_h(cls, ctx, factor, kernel, bands, input_w, fmt=U8):
+        return cls._make(ctx, "vgpu_op_reduce_h_new", (factor, kernel, bands, input_w, fmt), bands, fmt,
+                         "ReduceHBridge::new")
+
+    @classmethod
+    def reduce_v(cls, ctx, factor, kernel, bands, input_h, fmt=U8):
+        return cls._make(ctx, "vgpu_op_reduce_v_new", (factor, kernel, bands, input_h, fmt), bands, fmt,
+                         "ReduceVBridge::new")
+
+    @classmethod
+    def shrink_h(cls, ctx, factor, bands, source_width=0, fmt=U8, ceil=False):
+        return cls._make(ctx, "vgpu_op_shrink_h_new", (factor, int(ceil), bands, source_width, fmt), bands,
+                         fmt, "ShrinkHBridge::new")
+
+    @classmethod
+    def shrink_v(cls, ctx, factor, bands, fmt=U8, ceil=False):
+        return cls._make(ctx, "vgpu_op_shrink_v_new", (factor, int(ceil), bands, fmt), bands, fmt,
+                         "ShrinkVBridge::new")
+
+    @classmethod
+    def gauss_blur_h(cls, ctx, sigma, bands, fmt=U8):
+        return cls._make(ctx, "vgpu_op_gauss_blur_h_new", (sigma, bands, fmt), bands, fmt, "GaussBlurH::new")
+
+    @classmethod
+    def gauss_blur_v(cls, ctx, sigma, bands, fmt=U8):
+        return cls._make(ctx, "vgpu_op_gauss_blur_v_new", (sigma, bands, fmt), bands, fmt, "GaussBlurV::new")
+
+    @classmethod
+    def premultiply(cls, ctx, bands, fmt=U8):
+        return cls._make(ctx, "vgpu_op_premultiply_new", (bands, fmt), bands, fmt, "Premultiply::new")
+
+    @classmethod
+    def unpremultiply(cls, ctx, bands, fmt=U8):
+        return cls._make(ctx, "vgpu_op_unpremultiply_new", (bands, fmt), bands, fmt, "Unpremultiply::new")
+
+    def required_input_region(self, output: Region) -> Region:
+        r = Region()
+        _check(self._lib.vgpu_op_required_input_region(self._h, output, C.byref(r)),
+               "required_input_region")
+        return r
+
+    def output_size(self, in_w: int, in_h: int):
+        w, h = C.c_uint32(), C.c_uint32()
+        _check(self._lib.vgpu_op_output_size(self._h, in_w, in_h, C.byref(w), C.byref(h)), "output_size")
+        return w.value, h.value
+
+    def process_region(self, src, dst, input_region: Region, output_region: Region) -> None:
+        _check(self._lib.vgpu_op_process_region(self._h, C.c_void_p(src.data_ptr()), _nbytes(src),
+                                                C.c_void_p(dst.data_ptr()), _nbytes(dst),
+                                                input_region, output_region), "process_region")

@@ -1,0 +1,492 @@
+// gfx950 kernels for the resample / separable-convolution path.
+//
+// Layout everywhere: packed, band-interleaved, row-major samples
+// ((y * w + x) * bands + b), as in the reference's tiles
+// (crates/viprs-core/src/image/region.rs:154-230).
+//
+// Integer formats reproduce the reference's fixed-point sums exactly (the
+// sums are exact integers, so summation order is free).  The f32 kernels here
+// are the reference-exact ones: they carry the reference's FP64 accumulation
+// in the reference's operation order (FMA where it uses mul_add, separate
+// multiply/add where it does not), so they are bit-identical to it.  This
+// file must be compiled with -ffp-contract=off.
+#include "kernels.hpp"
+
+namespace vgpu {
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Buffer-relative column / row of an absolute coordinate: clip, then clamp to
+// the buffer.
+__device__ __forceinline__ int col_of(const InView& v, int x) {
+    return clampi(clampi(x, v.cx0, v.cx1), v.bx, v.bx + v.bw - 1) - v.bx;
+}
+__device__ __forceinline__ int row_of(const InView& v, int y) {
+    return clampi(clampi(y, v.cy0, v.cy1), v.by, v.by + v.bh - 1) - v.by;
+}
+
+template <typename T>
+__device__ __forceinline__ T load_at(const InView& v, int row, int col, int bands, int b) {
+    return static_cast<const T*>(v.ptr)[((size_t)row * v.bw + col) * bands + b];
+}
+
+// ShrinkH u8 rounding on x86 (shrinkh.rs:381-397 dispatch): f = 2 rounds a
+// pair, f = 5 with 1/3/4 bands divides exactly, every other factor uses the
+// libvips multiply-shift ((sum + f/2) * floor(2^24 / f)) >> 24.  The product
+// stays below 2^32 for u8 sums, so 32-bit arithmetic is exact.
+__device__ __forceinline__ uint32_t shrink_round_h(uint32_t sum, uint32_t f, uint32_t bands,
+                                                   uint32_t mult) {
+    if (f == 2) return (sum + 1) >> 1;
+    if (f == 5 && bands != 2 && bands <= 4) return (sum + 2) / 5;
+    return ((sum + (f >> 1)) * mult) >> 24;
+}
+// ShrinkV u8 (shrinkv.rs:206-339): pair rounding at f = 2, else multiply-shift.
+__device__ __forceinline__ uint32_t shrink_round_v(uint32_t sum, uint32_t f, uint32_t mult) {
+    if (f == 2) return (sum + 1) >> 1;
+    return ((sum + (f >> 1)) * mult) >> 24;
+}
+
+// Premultiply<U8> (premultiply.rs:103-115): round(c * (a / 255)) in f64.  For
+// integer c, a the exact quotient c*a/255 is never within 1/510 of a half, so
+// the f64 expression equals the integer rounding (2ca + 255) / 510.
+__device__ __forceinline__ uint32_t premult_u8(uint32_t c, uint32_t a) { return (2u * c * a + 255u) / 510u; }
+// Unpremultiply<U8> (unpremultiply.rs:96-109): a == 0 -> 0, else
+// clamp(round(c * 255 / a)).  The f64 quotient of two small integers is either
+// exact (ties included) or at least 1/(2a) away from a half, so it equals the
+// integer rounding (510c + a) / (2a).
+__device__ __forceinline__ uint32_t unpremult_u8(uint32_t c, uint32_t a) {
+    if (a == 0) return 0;
+    const uint32_t v = (510u * c + a) / (2u * a);
+    return v > 255u ? 255u : v;
+}
+
+// ---------------------------------------------------------------------------
+// Fused ShrinkH -> ShrinkV for u8 (the thumbnail's box pre-shrink).
+//
+// Block = TX output columns of one output row.  The block walks the fv source
+// rows of that output row RL = 256 / TX rows at a time: it stages the
+// contiguous byte span the TX box windows cover into LDS with 16-byte loads,
+// then every (column, row-lane) thread sums its fh-pixel window per band,
+// applies the ShrinkH rounding, and accumulates the rounded value.  The
+// row-lane partial sums are combined through LDS and the ShrinkV rounding is
+// applied once per output sample.  Premultiply (4-band / 2-band thumbnails)
+// is applied to each source pixel as it is read.
+// ---------------------------------------------------------------------------
+template <int B>
+__global__ __launch_bounds__(kThreads) void k_shrink_hv_u8(InView in, OutView out, int fh, int fv,
+                                                           int tx, int RL, int premult,
+                                                           uint32_t mh, uint32_t mv,
+                                                           int aligned) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lanes = kThreads / tx;  // row lanes that hold partial sums
+    const int t = threadIdx.x;
+    const int c = t % tx, r = t / tx;
+    const int col0 = blockIdx.x * tx;
+    const int ncols = min(tx, out.w - col0);
+    const int oy = out.y + (int)blockIdx.y;
+    const int cx0 = (out.x + col0) * fh;  // absolute first source column
+    const int span = ncols * fh * B;
+    const int pitch = ((tx * fh * B + 32) + 15) & ~15;
+    const unsigned char* base = static_cast<const unsigned char*>(in.ptr);
+    const size_t row_bytes = (size_t)in.bw * B;
+    const size_t total = row_bytes * (size_t)in.bh;
+    const bool inside = aligned && cx0 >= in.bx && cx0 + ncols * fh <= in.bx + in.bw;
+
+    uint32_t acc[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) acc[b] = 0;
+
+    for (int k0 = 0; k0 < fv; k0 += RL) {
+        const int nrows = min(RL, fv - k0);
+        if (inside) {
+            // Interior: one contiguous byte run per row, read as 16-byte chunks
+            // starting at the 16-byte boundary at or before the run.
+            int nchunks_max = (span + 15 + 15) / 16;
+            for (int it = t; it < nrows * nchunks_max; it += kThreads) {
+                const int rr = it / nchunks_max, i = it - rr * nchunks_max;
+                const int row = row_of(in, oy * fv + k0 + rr);
+                const size_t a = row * row_bytes + (size_t)(cx0 - in.bx) * B;
+                const size_t a16 = a & ~(size_t)15;
+                const int nchunks = (int)((a - a16 + span + 15) / 16);
+                if (i >= nchunks) continue;
+                const size_t off = a16 + (size_t)i * 16;
+                uint4 v;
+                if (off + 16 <= total) {
+                    v = *reinterpret_cast<const uint4*>(base + off);
+                } else {
+                    unsigned char tmp[16];
+                    for (int q = 0; q < 16; ++q) tmp[q] = (off + q < total) ? base[off + q] : 0;
+                    v = *reinterpret_cast<const uint4*>(tmp);
+                }
+                *reinterpret_cast<uint4*>(lds + rr * pitch + i * 16) = v;
+            }
+        } else {
+            // Border: byte gather with clamp-to-edge columns.
+            for (int it = t; it < nrows * span; it += kThreads) {
+                const int rr = it / span, j = it - rr * span;
+                const int row = row_of(in, oy * fv + k0 + rr);
+                const int col = col_of(in, cx0 + j / B);
+                lds[rr * pitch + j] = base[(size_t)row * row_bytes + (size_t)col * B + (j % B)];
+            }
+        }
+        __syncthreads();
+        if (r < nrows && c < ncols) {
+            int shift = 0;
+            if (inside) {
+                const int row = row_of(in, oy * fv + k0 + r);
+                const size_t a = row * row_bytes + (size_t)(cx0 - in.bx) * B;
+                shift = (int)(a & 15);
+            }
+            const unsigned char* w = lds + r * pitch + shift + c * fh * B;
+            uint32_t s[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) s[b] = 0;
+            if (premult && (B == 2 || B == 4)) {
+                for (int j = 0; j < fh; ++j) {
+                    const uint32_t a = w[j * B + B - 1];
+#pragma unroll
+                    for (int b = 0; b < B - 1; ++b) s[b] += premult_u8(w[j * B + b], a);
+                    s[B - 1] += a;
+                }
+            } else {
+                for (int j = 0; j < fh; ++j) {
+#pragma unroll
+                    for (int b = 0; b < B; ++b) s[b] += w[j * B + b];
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) acc[b] += shrink_round_h(s[b], (uint32_t)fh, B, mh);
+        }
+        __syncthreads();
+    }
+    uint32_t* part = reinterpret_cast<uint32_t*>(lds);
+    if (c < ncols) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) part[(r * tx + c) * B + b] = acc[b];
+    }
+    __syncthreads();
+    if (r == 0 && c < ncols) {
+        unsigned char* o = static_cast<unsigned char*>(out.ptr) + ((size_t)blockIdx.y * out.w + col0 + c) * B;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            uint32_t sum = 0;
+            for (int q = 0; q < lanes; ++q) sum += part[(q * tx + c) * B + b];
+            o[b] = (unsigned char)shrink_round_v(sum, (uint32_t)fv, mv);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Generic single-axis kernels: one thread per output sample.
+// ---------------------------------------------------------------------------
+
+// ShrinkH/ShrinkV for u16/f32 (and u8 without the fused kernel):
+// f64 sum * (1/f) then ShrinkSample::from_f64_clamped (shrinkh.rs:49-128,271-300;
+// shrinkv.rs:168-187); u8 uses the integer roundings above.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_shrink_generic(InView in, OutView out, int B, int f,
+                                                             int vertical, uint32_t mult) {
+    const size_t n = (size_t)out.w * out.h * B;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        const int b = (int)(i % B);
+        const size_t px = i / B;
+        const int x = out.x + (int)(px % out.w), y = out.y + (int)(px / out.w);
+        T* o = static_cast<T*>(out.ptr);
+        if constexpr (sizeof(T) == 1) {
+            uint32_t sum = 0;
+            for (int j = 0; j < f; ++j) {
+                const int row = row_of(in, vertical ? y * f + j : y);
+                const int col = col_of(in, vertical ? x : x * f + j);
+                sum += load_at<uint8_t>(in, row, col, B, b);
+            }
+            o[i] = (T)(vertical ? shrink_round_v(sum, f, mult) : shrink_round_h(sum, f, B, mult));
+        } else {
+            double sum = 0.0;
+            for (int j = 0; j < f; ++j) {
+                const int row = row_of(in, vertical ? y * f + j : y);
+                const int col = col_of(in, vertical ? x : x * f + j);
+                sum += (double)load_at<T>(in, row, col, B, b);
+            }
+            const double v = sum * (1.0 / (double)f);
+            if constexpr (sizeof(T) == 2) {
+                o[i] = (T)fmin(fmax(round(v), 0.0), 65535.0);
+            } else {
+                o[i] = (T)v;
+            }
+        }
+    }
+}
+
+// ReduceH (reduceh.rs:101-163; reduce_simd.rs:47-90,2463-2693): fixed-point
+// i16 taps for integer formats -> (acc + 2048) >> 12 clamped; f32 carries FP64:
+// bands == 1 follows the AVX2 4-lane grouping (products summed per group, then
+// an FMA tail), other band counts the scalar FMA chain.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_reduce_h(InView in, OutView out, int B, DevPlan p) {
+    const size_t n = (size_t)out.w * out.h * B;
+    const int T_ = p.taps;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        const int b = (int)(i % B);
+        const size_t px = i / B;
+        const int xo = (int)(px % out.w);
+        const int row = row_of(in, out.y + (int)(px / out.w));
+        const int start = p.start[xo];
+        const int ph = p.phase[xo];
+        T* o = static_cast<T*>(out.ptr);
+        if constexpr (sizeof(T) == 4) {
+            const double* w = p.wf64 + (size_t)ph * T_;
+            double acc = 0.0;
+            int t = 0;
+            if (B == 1) {
+                for (; t + 4 <= T_; t += 4) {
+                    double s = 0.0;
+                    for (int l = 0; l < 4; ++l) {
+                        const double prod = (double)load_at<float>(in, row, col_of(in, start + t + l), 1, 0) * w[t + l];
+                        s = s + prod;
+                    }
+                    acc = acc + s;
+                }
+            }
+            for (; t < T_; ++t) acc = fma((double)load_at<float>(in, row, col_of(in, start + t), B, b), w[t], acc);
+            o[i] = (float)acc;
+        } else {
+            const int16_t* w = p.wi16 + (size_t)ph * T_;
+            int64_t acc = 0;
+            for (int t = 0; t < T_; ++t)
+                acc += (int64_t)w[t] * (int64_t)load_at<T>(in, row, col_of(in, start + t), B, b);
+            const int64_t v = (acc + 2048) >> 12;
+            const int64_t mx = sizeof(T) == 1 ? 255 : 65535;
+            o[i] = (T)(v < 0 ? 0 : (v > mx ? mx : v));
+        }
+    }
+}
+
+// ReduceV (reducev.rs:100-162; reduce_simd.rs:2700-2888): direct per-row plan
+// (x86 AVX2 recomputes the source position per row); f32 rows use separate
+// multiply/add for the first (row_len / 4) * 4 samples and an FMA chain for
+// the tail, as the AVX2 kernel does.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_reduce_v(InView in, OutView out, int B, DevPlan p) {
+    const int row_len = out.w * B;
+    const int vec_len = row_len / 4 * 4;
+    const size_t n = (size_t)row_len * out.h;
+    const int T_ = p.taps;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        const int s = (int)(i % row_len);
+        const int yo = (int)(i / row_len);
+        const int b = s % B;
+        const int col = col_of(in, out.x + s / B);
+        const int start = p.start[yo];
+        const int ph = p.phase[yo];
+        T* o = static_cast<T*>(out.ptr);
+        if constexpr (sizeof(T) == 4) {
+            const double* w = p.wf64 + (size_t)ph * T_;
+            double acc = 0.0;
+            if (s < vec_len) {
+                for (int t = 0; t < T_; ++t) {
+                    const double prod = (double)load_at<float>(in, row_of(in, start + t), col, B, b) * w[t];
+                    acc = acc + prod;
+                }
+            } else {
+                for (int t = 0; t < T_; ++t)
+                    acc = fma((double)load_at<float>(in, row_of(in, start + t), col, B, b), w[t], acc);
+            }
+            o[i] = (float)acc;
+        } else {
+            const int16_t* w = p.wi16 + (size_t)ph * T_;
+            int64_t acc = 0;
+            for (int t = 0; t < T_; ++t)
+                acc += (int64_t)w[t] * (int64_t)load_at<T>(in, row_of(in, start + t), col, B, b);
+            const int64_t v = (acc + 2048) >> 12;
+            const int64_t mx = sizeof(T) == 1 ? 255 : 65535;
+            o[i] = (T)(v < 0 ? 0 : (v > mx ? mx : v));
+        }
+    }
+}
+
+// Separable convolution pass.  kind 0 = Gaussian: u8 -> u8 integer taps with
+// (sum + scale/2) / scale (gauss_blur.rs:405-518); other formats -> f32 with
+// acc += f64(f32(x)) * w, no FMA (gauss_blur.rs:410-464).  kind 1 = ConvSep:
+// FMA chain, acc / scale + offset -> f32 (convsep.rs:87-107,184-205).
+template <typename Tin, typename Tout>
+__global__ __launch_bounds__(kThreads) void k_conv(InView in, OutView out, int B, int kind,
+                                                   int vertical, const int16_t* itaps,
+                                                   int32_t iscale, const double* wts, int ntaps,
+                                                   double cscale, double coffset) {
+    const size_t n = (size_t)out.w * out.h * B;
+    const int rad = (ntaps - 1) / 2;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        const int b = (int)(i % B);
+        const size_t px = i / B;
+        const int x = out.x + (int)(px % out.w), y = out.y + (int)(px / out.w);
+        Tout* o = static_cast<Tout*>(out.ptr);
+        if (kind == 0 && sizeof(Tin) == 1 && sizeof(Tout) == 1) {
+            int32_t sum = 0;
+            for (int t = 0; t < ntaps; ++t) {
+                const int row = row_of(in, vertical ? y - rad + t : y);
+                const int col = col_of(in, vertical ? x : x - rad + t);
+                sum += (int32_t)itaps[t] * (int32_t)load_at<uint8_t>(in, row, col, B, b);
+            }
+            const int32_t v = (sum + iscale / 2) / iscale;
+            o[i] = (Tout)(v < 0 ? 0 : (v > 255 ? 255 : v));
+        } else if (kind == 0) {
+            double acc = 0.0;
+            for (int t = 0; t < ntaps; ++t) {
+                const int row = row_of(in, vertical ? y - rad + t : y);
+                const int col = col_of(in, vertical ? x : x - rad + t);
+                const double prod = (double)(float)load_at<Tin>(in, row, col, B, b) * wts[t];
+                acc = acc + prod;
+            }
+            o[i] = (Tout)(float)acc;
+        } else {
+            double acc = 0.0;
+            for (int t = 0; t < ntaps; ++t) {
+                const int row = row_of(in, vertical ? y - rad + t : y);
+                const int col = col_of(in, vertical ? x : x - rad + t);
+                acc = fma((double)load_at<Tin>(in, row, col, B, b), wts[t], acc);
+            }
+            o[i] = (Tout)(float)(acc / cscale + coffset);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_alpha_u8(InView in, OutView out, int B, int mode) {
+    const size_t n = (size_t)out.w * out.h;
+    for (size_t px = (size_t)blockIdx.x * kThreads + threadIdx.x; px < n; px += (size_t)gridDim.x * kThreads) {
+        const int x = out.x + (int)(px % out.w), y = out.y + (int)(px / out.w);
+        const int row = row_of(in, y), col = col_of(in, x);
+        const unsigned char* s = static_cast<const unsigned char*>(in.ptr) + ((size_t)row * in.bw + col) * B;
+        unsigned char* d = static_cast<unsigned char*>(out.ptr) + px * B;
+        const uint32_t a = s[B - 1];
+        for (int b = 0; b + 1 < B; ++b)
+            d[b] = (unsigned char)(mode == ALPHA_PREMULT ? premult_u8(s[b], a) : unpremult_u8(s[b], a));
+        d[B - 1] = (unsigned char)a;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_copy(InView in, OutView out, int B) {
+    const size_t n = (size_t)out.w * out.h * B;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        const int b = (int)(i % B);
+        const size_t px = i / B;
+        const int x = out.x + (int)(px % out.w), y = out.y + (int)(px / out.w);
+        static_cast<T*>(out.ptr)[i] = load_at<T>(in, row_of(in, y), col_of(in, x), B, b);
+    }
+}
+
+unsigned grid_for(size_t n) {
+    size_t g = (n + kThreads - 1) / kThreads;
+    if (g < 1) g = 1;
+    if (g > 1u << 20) g = 1u << 20;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+hipError_t launch_shrink_hv_u8(const InView& in, const OutView& out, int bands, int fh, int fv,
+                               int premult, hipStream_t s) {
+    if (out.w <= 0 || out.h <= 0) return hipSuccess;
+    // LDS budget 64 KiB: RL staged source rows of one TX-column span each.
+    const size_t budget = 64 * 1024;
+    int tx = 64;
+    auto pitch_of = [&](int cols) { return ((size_t)cols * fh * bands + 32 + 15) & ~(size_t)15; };
+    while (tx > 1 && pitch_of(tx) > budget) tx /= 2;
+    const size_t pitch = pitch_of(tx);
+    int rl = kThreads / tx;
+    if ((size_t)rl * pitch > budget) rl = (int)(budget / pitch);
+    if (rl < 1) rl = 1;
+    size_t shmem = (size_t)rl * pitch;
+    const size_t part = (size_t)kThreads * bands * 4;
+    if (shmem < part) shmem = part;
+    const uint32_t mh = (uint32_t)((1ull << 32) / (256ull * (uint64_t)fh));
+    const uint32_t mv = (uint32_t)((1ull << 32) / (256ull * (uint64_t)fv));
+    const int aligned = ((uintptr_t)in.ptr % 16) == 0;
+    dim3 grid((unsigned)((out.w + tx - 1) / tx), (unsigned)out.h);
+    switch (bands) {
+    case 1: hipLaunchKernelGGL(k_shrink_hv_u8<1>, grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, rl, premult, mh, mv, aligned); break;
+    case 2: hipLaunchKernelGGL(k_shrink_hv_u8<2>, grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, rl, premult, mh, mv, aligned); break;
+    case 3: hipLaunchKernelGGL(k_shrink_hv_u8<3>, grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, rl, premult, mh, mv, aligned); break;
+    case 4: hipLaunchKernelGGL(k_shrink_hv_u8<4>, grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, rl, premult, mh, mv, aligned); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_shrink_generic(const InView& in, const OutView& out, int bands, int fmt,
+                                 int factor, int vertical, hipStream_t s) {
+    const size_t n = (size_t)out.w * out.h * bands;
+    if (n == 0) return hipSuccess;
+    const uint32_t mult = (uint32_t)((1ull << 32) / (256ull * (uint64_t)factor));
+    const unsigned g = grid_for(n);
+    if (fmt == 0) hipLaunchKernelGGL(k_shrink_generic<uint8_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, factor, vertical, mult);
+    else if (fmt == 1) hipLaunchKernelGGL(k_shrink_generic<uint16_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, factor, vertical, mult);
+    else hipLaunchKernelGGL(k_shrink_generic<float>, dim3(g), dim3(kThreads), 0, s, in, out, bands, factor, vertical, mult);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_h(const InView& in, const OutView& out, int bands, int fmt,
+                           const DevPlan& plan, int, hipStream_t s) {
+    const size_t n = (size_t)out.w * out.h * bands;
+    if (n == 0) return hipSuccess;
+    const unsigned g = grid_for(n);
+    if (fmt == 0) hipLaunchKernelGGL(k_reduce_h<uint8_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
+    else if (fmt == 1) hipLaunchKernelGGL(k_reduce_h<uint16_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
+    else hipLaunchKernelGGL(k_reduce_h<float>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_v(const InView& in, const OutView& out, int bands, int fmt,
+                           const DevPlan& plan, int, hipStream_t s) {
+    const size_t n = (size_t)out.w * out.h * bands;
+    if (n == 0) return hipSuccess;
+    const unsigned g = grid_for(n);
+    if (fmt == 0) hipLaunchKernelGGL(k_reduce_v<uint8_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
+    else if (fmt == 1) hipLaunchKernelGGL(k_reduce_v<uint16_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
+    else hipLaunchKernelGGL(k_reduce_v<float>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv(const InView& in, const OutView& out, int bands, int in_fmt, int out_fmt,
+                       int kind, int vertical, const int16_t* itaps, int32_t iscale,
+                       const double* wts, int ntaps, double cscale, double coffset,
+                       hipStream_t s) {
+    const size_t n = (size_t)out.w * out.h * bands;
+    if (n == 0) return hipSuccess;
+    const unsigned g = grid_for(n);
+#define VGPU_CONV(TI, TO) hipLaunchKernelGGL((k_conv<TI, TO>), dim3(g), dim3(kThreads), 0, s, in, out, bands, kind, vertical, itaps, iscale, wts, ntaps, cscale, coffset)
+    if (out_fmt == 0) {
+        if (in_fmt != 0) return hipErrorInvalidValue;
+        VGPU_CONV(uint8_t, uint8_t);
+    } else if (out_fmt == 2) {
+        if (in_fmt == 0) VGPU_CONV(uint8_t, float);
+        else if (in_fmt == 1) VGPU_CONV(uint16_t, float);
+        else VGPU_CONV(float, float);
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef VGPU_CONV
+    return hipGetLastError();
+}
+
+hipError_t launch_alpha_u8(const InView& in, const OutView& out, int bands, int mode, hipStream_t s) {
+    const size_t n = (size_t)out.w * out.h;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_alpha_u8, dim3(grid_for(n)), dim3(kThreads), 0, s, in, out, bands, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const InView& in, const OutView& out, int bands, int fmt, hipStream_t s) {
+    const size_t n = (size_t)out.w * out.h * bands;
+    if (n == 0) return hipSuccess;
+    const unsigned g = grid_for(n);
+    if (fmt == 0) hipLaunchKernelGGL(k_copy<uint8_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands);
+    else if (fmt == 1) hipLaunchKernelGGL(k_copy<uint16_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands);
+    else hipLaunchKernelGGL(k_copy<float>, dim3(g), dim3(kThreads), 0, s, in, out, bands);
+    return hipGetLastError();
+}
+
+}  // namespace vgpu

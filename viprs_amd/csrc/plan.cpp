@@ -1,0 +1,372 @@
+// Host planning: coefficient tables, plans, masks, planners, regions.
+//
+// The formulas below have to be the reference's, operation for operation,
+// because their FP64 rounding decides which integer coefficients and which
+// source positions the device uses:
+//   kernel weights            crates/viprs-core/src/kernel.rs:133-211
+//   tap count / tables        crates/viprs-ops-spatial/src/resample/reduce_common.rs:137-164,257-316
+//   offset / source position  reduce_common.rs:171-185
+//   plan (start, phase)       reduce_common.rs:127-135
+//   required input regions    reduce_common.rs:218-254
+//   output length             crates/viprs-core/src/resample.rs:86-93
+//   Gaussian integer mask     crates/viprs-ops-spatial/src/convolution/gauss_blur.rs:80-126
+//   thumbnail planner         crates/viprs-ops-spatial/src/resample/thumbnail.rs:229-524
+//   resize planner            crates/viprs-ops-spatial/src/resample/resize.rs:163-345
+// glibc sin/exp/fma are the same functions Rust's f64 methods call on
+// x86_64-linux-gnu, so the tables come out bit-identical.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <limits>
+
+namespace vgpu {
+
+namespace {
+
+constexpr double kPlanEps = 1e-9;  // thumbnail.rs:27, resize.rs:9
+constexpr int kMaxTaps = 2000;     // reduce_common.rs:10 (libvips MAX_POINT)
+
+uint32_t to_u32_sat(double v) {
+    if (!(v == v) || v <= 0.0) return 0;
+    if (v >= 4294967295.0) return 0xffffffffu;
+    return (uint32_t)v;
+}
+int16_t to_i16_sat(double v) {
+    if (!(v == v)) return 0;
+    if (v <= -32768.0) return INT16_MIN;
+    if (v >= 32767.0) return INT16_MAX;
+    return (int16_t)v;
+}
+int64_t to_i64_sat(double v) {
+    if (!(v == v)) return 0;
+    if (v <= -9.2233720368547758e18) return INT64_MIN;
+    if (v >= 9.2233720368547758e18) return INT64_MAX;
+    return (int64_t)v;
+}
+int64_t clamp_i32_range(int64_t v) {
+    return std::min<int64_t>(std::max<int64_t>(v, INT32_MIN), INT32_MAX);
+}
+
+double support_of(int k) {
+    switch (k) {
+    case NEAREST: return 0.5;
+    case BILINEAR: return 1.0;
+    case LANCZOS3: return 3.0;
+    default: return 2.0;
+    }
+}
+
+}  // namespace
+
+double kernel_weight(int k, double x) {
+    const double a = std::fabs(x);
+    switch (k) {
+    case NEAREST: return a < 0.5 ? 1.0 : 0.0;
+    case BILINEAR: return a < 1.0 ? 1.0 - a : 0.0;
+    case BICUBIC:
+    case CATMULLROM:
+        // Keys a = -0.5 in mul_add form.
+        if (a < 1.0) return std::fma(2.5 * a, -a, 1.5 * a * a * a) + 1.0;
+        if (a < 2.0) return std::fma(4.0, -a, std::fma(2.5 * a, a, -0.5 * a * a * a)) + 2.0;
+        return 0.0;
+    case LANCZOS2:
+    case LANCZOS3: {
+        const double lobes = (k == LANCZOS3) ? 3.0 : 2.0;
+        if (a < DBL_EPSILON) return 1.0;
+        if (a >= lobes) return 0.0;
+        const double px = M_PI * a;
+        return lobes * std::sin(px) * std::sin(px / lobes) / (px * px);
+    }
+    default: return 0.0;
+    }
+}
+
+Status validate_reduce(int kernel, double factor) {
+    if (kernel == LBB || kernel == VSQBS || kernel == NOHALO) return E_INVALID_KERNEL;
+    if (kernel < NEAREST || kernel > CATMULLROM) return E_INVALID_KERNEL;
+    if (!std::isfinite(factor) || factor < 1.0) return E_INVALID_ARGUMENT;
+    if (reduce_tap_count(kernel, factor) < 0) return E_INVALID_ARGUMENT;
+    return OK;
+}
+
+int reduce_tap_count(int kernel, double factor) {
+    if (std::fabs(factor - 1.0) < DBL_EPSILON || kernel == NEAREST) return 1;
+    const double radius = std::round(support_of(kernel) * factor);
+    if (!std::isfinite(radius) || radius < 0.0 || radius > (double)((kMaxTaps - 1) / 2)) return -1;
+    const long taps = 2 * (long)radius + 1;
+    return taps <= kMaxTaps ? (int)taps : -1;
+}
+
+uint32_t reduce_output_len(double factor, uint32_t in) {
+    return in == 0 ? 0 : to_u32_sat(std::round((double)in / factor));
+}
+
+Status build_reduce_axis(int kernel, double factor, uint32_t input_len, ReduceAxis* ax) {
+    Status st = validate_reduce(kernel, factor);
+    if (st != OK) return st;
+    const int taps = reduce_tap_count(kernel, factor);
+    ax->kernel = kernel;
+    ax->factor = factor;
+    ax->taps = taps;
+    ax->pad = (int64_t)std::ceil((double)taps / 2.0) - 1;
+    ax->wf64.assign((size_t)(kPhases + 1) * taps, 0.0);
+    ax->wi16.assign((size_t)(kPhases + 1) * taps, 0);
+    const double inv = 1.0 / factor;
+    for (int p = 0; p <= kPhases; ++p) {
+        double* row = &ax->wf64[(size_t)p * taps];
+        if (kernel == NEAREST) {
+            row[0] = 1.0;
+        } else {
+            const double centre = (double)p / (double)kPhases + (double)taps / 2.0 - 1.0;
+            double sum = 0.0;
+            for (int t = 0; t < taps; ++t) {
+                row[t] = kernel_weight(kernel, std::fabs(((double)t - centre) * inv));
+                sum += row[t];
+            }
+            if (sum > 0.0)
+                for (int t = 0; t < taps; ++t) row[t] /= sum;
+        }
+        for (int t = 0; t < taps; ++t) ax->wi16[(size_t)p * taps + t] = to_i16_sat(row[t] * 4096.0);
+    }
+    // bind_input_len: centre the output grid like libvips.
+    if (input_len == 0 || std::fabs(factor - 1.0) < DBL_EPSILON) {
+        ax->offset = 0.0;
+    } else {
+        const double extra = std::fma((double)reduce_output_len(factor, input_len), factor,
+                                      -(double)input_len);
+        ax->offset = (1.0 + extra) / 2.0 - 1.0;
+    }
+    return OK;
+}
+
+double ReduceAxis::source_position(double index) const {
+    return std::fma(index + 0.5, factor, -0.5) - offset;
+}
+
+void ReduceAxis::plan(double sp, int64_t* start, int* phase) const {
+    const double fl = std::floor(sp);
+    double ph = std::round(std::fma(sp - fl, (double)kPhases, 0.0));
+    ph = std::min(std::max(ph, 0.0), (double)kPhases);
+    *phase = (int)ph;
+    *start = to_i64_sat(fl) - pad;
+}
+
+void ReduceAxis::required(int64_t origin, int64_t len, int64_t* start, int64_t* n) const {
+    if (len <= 0) {
+        *start = origin;
+        *n = 0;
+        return;
+    }
+    const int64_t first = to_i64_sat(std::floor(source_position((double)origin)));
+    const int64_t last_i = clamp_i32_range(origin + len - 1);
+    const int64_t last = to_i64_sat(std::floor(source_position((double)last_i)));
+    const int64_t s = clamp_i32_range(first - pad);
+    const int64_t e = clamp_i32_range(last - pad + taps - 1);
+    *start = s;
+    *n = e - s + 1;
+}
+
+GaussMask build_gauss_mask(float sigma_f) {
+    GaussMask m;
+    const double sigma = (double)sigma_f;
+    if (sigma < 0.2) {
+        m.taps = {1};
+        m.scale = 1;
+        m.weights = {1.0};
+        return m;
+    }
+    const double sig2 = 2.0 * sigma * sigma;
+    const size_t limit = (size_t)std::ceil(8.0 * sigma);
+    size_t x = 0;
+    while (x < limit && std::exp(-((double)(x * x)) / sig2) >= 0.2) ++x;
+    const size_t radius = x == 0 ? 0 : x - 1;
+    int32_t scale = 0;
+    for (size_t i = 0; i < 2 * radius + 1; ++i) {
+        const int32_t off = (int32_t)i - (int32_t)radius;
+        const int32_t tap = (int32_t)std::nearbyint(20.0 * std::exp(-(double)(off * off) / sig2));
+        m.taps.push_back((int16_t)tap);
+        scale += tap;
+    }
+    if (scale == 0) {
+        m.taps = {1};
+        scale = 1;
+    }
+    m.scale = scale;
+    for (int16_t t : m.taps) m.weights.push_back((double)t / (double)scale);
+    return m;
+}
+
+Rect Node::required_input(const Rect& o) const {
+    Rect r = o;
+    switch (kind) {
+    case N_SHRINK_H: {
+        const int64_t need = o.w * (int64_t)factor;
+        if (o.x == 0 && source_width != 0 && need <= (int64_t)source_width) {
+            r.x = 0;
+            r.w = source_width;  // full-row borrow (shrinkh.rs:220-229)
+        } else {
+            r.x = clamp_i32_range(o.x * (int64_t)factor);
+            r.w = need;
+        }
+        break;
+    }
+    case N_SHRINK_V:
+        r.y = clamp_i32_range(o.y * (int64_t)factor);
+        r.h = o.h * (int64_t)factor;
+        break;
+    case N_REDUCE_H: axis.required(o.x, o.w, &r.x, &r.w); break;
+    case N_REDUCE_V: axis.required(o.y, o.h, &r.y, &r.h); break;
+    case N_GAUSS_H:
+    case N_CONVSEP_H:
+        r.x = o.x - radius();
+        r.w = o.w + 2 * radius();
+        break;
+    case N_GAUSS_V:
+    case N_CONVSEP_V:
+        r.y = o.y - radius();
+        r.h = o.h + 2 * radius();
+        break;
+    default: break;
+    }
+    return r;
+}
+
+namespace {
+
+void thumb_target_dims(int target, uint32_t tw, uint32_t th, uint32_t iw, uint32_t ih,
+                       uint32_t* ow, uint32_t* oh) {
+    const double fw = (double)iw, fh = (double)ih;
+    switch (target) {
+    case FITBOX: {
+        const double s = std::min(std::min((double)std::max(tw, 1u) / fw,
+                                           (double)std::max(th, 1u) / fh),
+                                  1.0);
+        *ow = to_u32_sat(std::max(std::round(fw * s), 1.0));
+        *oh = to_u32_sat(std::max(std::round(fh * s), 1.0));
+        break;
+    }
+    case FORCE_EXACT:
+        *ow = std::max(tw, 1u);
+        *oh = std::max(th, 1u);
+        break;
+    case WIDTH: {
+        const uint32_t w = std::min(std::max(tw, 1u), iw);
+        *ow = w;
+        *oh = to_u32_sat(std::max(std::round((fh * (double)w) / fw), 1.0));
+        break;
+    }
+    default: {
+        const uint32_t h = std::min(std::max(th, 1u), ih);
+        *oh = h;
+        *ow = to_u32_sat(std::max(std::round((fw * (double)h) / fh), 1.0));
+        break;
+    }
+    }
+}
+
+uint32_t after_reduce(uint32_t after_shrink, double residual) {
+    if (residual > 1.0 + kPlanEps)
+        return to_u32_sat(std::max(std::round((double)after_shrink / residual), 1.0));
+    return after_shrink;
+}
+
+uint32_t integer_shrink(uint32_t in_len, uint32_t target, double total) {
+    if (total <= 1.0 + kPlanEps) return 1;
+    uint32_t f = to_u32_sat(std::max(std::floor(total), 1.0));
+    for (; f > 1; --f)
+        if (after_reduce(in_len / f, total / (double)f) >= target) break;
+    return f;
+}
+
+}  // namespace
+
+Status plan_thumbnail(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint32_t tw,
+                      uint32_t th, std::vector<PlanStep>* steps, uint32_t* out_w,
+                      uint32_t* out_h) {
+    steps->clear();
+    if (bands == 0) return E_INVALID_ARGUMENT;
+    if (target < FITBOX || target > HEIGHT) return E_INVALID_ARGUMENT;
+    if (iw == 0 || ih == 0) {
+        *out_w = *out_h = 0;
+        return OK;
+    }
+    const bool upscale = target == FORCE_EXACT;
+    const bool single = iw == 1 && ih == 1;
+    uint32_t gw, gh;
+    thumb_target_dims(target, tw, th, iw, ih, &gw, &gh);
+    if (!(upscale && !single)) {
+        gw = std::max(std::min(gw, iw), 1u);
+        gh = std::max(std::min(gh, ih), 1u);
+    }
+    std::vector<PlanStep> geo;
+    if (iw == 1 || ih == 1) {
+        if (single) gw = gh = 1;
+        if (gw != iw || gh != ih) return E_UNSUPPORTED;  // Affine-only plan
+    } else {
+        const double hs = (double)iw / (double)gw, vs = (double)ih / (double)gh;
+        const uint32_t sh = integer_shrink(iw, gw, hs), sv = integer_shrink(ih, gh, vs);
+        const double rh = hs > 1.0 ? hs / (double)sh : 1.0;
+        const double rv = vs > 1.0 ? vs / (double)sv : 1.0;
+        const uint32_t arw = after_reduce(iw / std::max(sh, 1u), rh);
+        const uint32_t arh = after_reduce(ih / std::max(sv, 1u), rv);
+        if (sh > 1) geo.push_back({N_SHRINK_H, (double)sh});
+        if (sv > 1) geo.push_back({N_SHRINK_V, (double)sv});
+        if (rh > 1.0 + kPlanEps) geo.push_back({N_REDUCE_H, rh});
+        if (rv > 1.0 + kPlanEps) geo.push_back({N_REDUCE_V, rv});
+        if (std::fabs((double)gw / (double)arw - 1.0) > kPlanEps ||
+            std::fabs((double)gh / (double)arh - 1.0) > kPlanEps)
+            return E_UNSUPPORTED;  // needs the Affine geometry fix-up
+    }
+    const bool alpha = bands == 2 || bands == 4;
+    if (alpha && !geo.empty()) steps->push_back({N_PREMULT, 0.0});
+    steps->insert(steps->end(), geo.begin(), geo.end());
+    if (alpha && !geo.empty()) steps->push_back({N_UNPREMULT, 0.0});
+    *out_w = gw;
+    *out_h = gh;
+    return OK;
+}
+
+Status plan_resize(uint32_t iw, uint32_t ih, double hscale, double vscale, int kernel,
+                   std::vector<PlanStep>* steps, int* eff_kernel) {
+    steps->clear();
+    if (!(std::isfinite(hscale) && hscale > 0.0)) hscale = 1.0;
+    if (!(std::isfinite(vscale) && vscale > 0.0)) vscale = 1.0;
+    if (kernel == NOHALO) kernel = BICUBIC;
+    if (kernel == LBB || kernel == VSQBS) return E_UNSUPPORTED;
+    *eff_kernel = kernel;
+    if (iw == 0 || ih == 0) return E_UNSUPPORTED;
+    const uint32_t tw = to_u32_sat(std::max(std::round((double)iw * hscale), 1.0));
+    const uint32_t th = to_u32_sat(std::max(std::round((double)ih * vscale), 1.0));
+    if (tw > iw || th > ih) return E_UNSUPPORTED;  // Zoom / Affine upscale
+    auto preshrink = [&](uint32_t in, uint32_t t) -> uint32_t {
+        if (t == 0 || in <= t || kernel == NEAREST) return 1;
+        return to_u32_sat(std::max(std::floor((double)in / (double)t / 2.0), 1.0));
+    };
+    uint32_t cw = iw, ch = ih;
+    const uint32_t hs = tw < cw ? preshrink(cw, tw) : 1;
+    const uint32_t vs = th < ch ? preshrink(ch, th) : 1;
+    if (vs > 1) {
+        steps->push_back({N_SHRINK_V, (double)vs});
+        ch /= vs;
+    }
+    if (hs > 1) {
+        steps->push_back({N_SHRINK_H, (double)hs});
+        cw /= hs;
+    }
+    if (th < ch) {
+        steps->push_back({N_REDUCE_V, (double)ch / (double)th});
+        ch = th;
+    }
+    if (tw < cw) {
+        steps->push_back({N_REDUCE_H, (double)cw / (double)tw});
+        cw = tw;
+    }
+    if (std::fabs((double)tw / (double)cw - 1.0) > kPlanEps ||
+        std::fabs((double)th / (double)ch - 1.0) > kPlanEps)
+        return E_UNSUPPORTED;
+    return OK;
+}
+
+}  // namespace vgpu

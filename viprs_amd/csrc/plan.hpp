@@ -1,0 +1,129 @@
+// Host-side planning for the resample / separable-convolution path.
+//
+// Everything here is FP64 host arithmetic whose rounding decides bit-exactness
+// on the device: the 65-phase reduce coefficient tables, the per-axis
+// (start, phase) plans, the integer Gaussian masks and the Thumbnail / Resize
+// planners.  Device kernels only consume the tables and plans built here.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace vgpu {
+
+enum Format : int { U8 = 0, U16 = 1, F32 = 2 };
+enum Kernel : int {
+    NEAREST = 0,
+    BILINEAR = 1,
+    BICUBIC = 2,
+    LANCZOS2 = 3,
+    LANCZOS3 = 4,
+    CATMULLROM = 5,
+    LBB = 6,
+    VSQBS = 7,
+    NOHALO = 8,
+};
+
+enum Status : int {
+    OK = 0,
+    E_INVALID_ARGUMENT = -1,
+    E_INVALID_KERNEL = -2,
+    E_UNSUPPORTED_FORMAT = -3,
+    E_UNSUPPORTED = -4,
+    E_BUFFER_SIZE = -5,
+    E_DEVICE = -6,
+    E_NO_MEMORY = -7,
+    E_CANCELLED = -8,
+    E_ADMISSION = -9,
+};
+
+inline size_t format_size(int f) { return f == U8 ? 1 : (f == U16 ? 2 : 4); }
+
+// Region on Z^2 (crates/viprs-core/src/image/region.rs:18-27), widened to 64
+// bits so halo arithmetic never wraps on the host.
+struct Rect {
+    int64_t x = 0, y = 0, w = 0, h = 0;
+    bool empty() const { return w <= 0 || h <= 0; }
+};
+
+constexpr int kPhases = 64;  // REDUCE_PHASES (reduce_common.rs:5)
+
+// ReduceKernel (reduce_common.rs:87-255) for one axis.
+struct ReduceAxis {
+    int kernel = LANCZOS3;
+    double factor = 1.0;
+    int taps = 1;
+    int64_t pad = 0;
+    double offset = 0.0;
+    std::vector<double> wf64;  // (kPhases + 1) x taps, normalised weights
+    std::vector<int16_t> wi16; // (kPhases + 1) x taps, trunc(w * 4096)
+
+    double source_position(double index) const;
+    void plan(double sp, int64_t* start, int* phase) const;
+    // required_input_region_{h,v} along this axis.
+    void required(int64_t origin, int64_t len, int64_t* start, int64_t* n) const;
+};
+
+int reduce_tap_count(int kernel, double factor);  // -1 when over the 2000-tap limit
+double kernel_weight(int kernel, double x);
+Status validate_reduce(int kernel, double factor);
+Status build_reduce_axis(int kernel, double factor, uint32_t input_len, ReduceAxis* out);
+uint32_t reduce_output_len(double factor, uint32_t input_len);
+
+// Integer Gaussian mask (gauss_blur.rs:80-126).
+struct GaussMask {
+    std::vector<int16_t> taps;
+    int32_t scale = 1;
+    std::vector<double> weights;  // taps / scale (gauss_kernel.rs:58-65)
+    int radius() const { return (int)(taps.size() - 1) / 2; }
+};
+GaussMask build_gauss_mask(float sigma);
+
+// Pipeline node kinds (the reference's bridge set on this path).
+enum NodeKind : int {
+    N_SHRINK_H = 1,
+    N_SHRINK_V = 2,
+    N_REDUCE_H = 3,
+    N_REDUCE_V = 4,
+    N_GAUSS_H = 5,
+    N_GAUSS_V = 6,
+    N_CONVSEP_H = 7,
+    N_CONVSEP_V = 8,
+    N_PREMULT = 9,
+    N_UNPREMULT = 10,
+    N_COPY = 11,
+};
+
+struct Node {
+    NodeKind kind = N_COPY;
+    int in_fmt = U8, out_fmt = U8;
+    uint32_t bands = 1;
+    uint32_t in_w = 0, in_h = 0, out_w = 0, out_h = 0;
+    uint32_t factor = 1;        // shrink
+    uint32_t source_width = 0;  // ShrinkH full-row borrow width (0 = none)
+    ReduceAxis axis;            // reduce
+    std::vector<double> conv;   // gauss float weights / convsep coefficients
+    std::vector<int16_t> itaps; // gauss integer taps
+    int32_t iscale = 1;
+    double cscale = 1.0, coffset = 0.0;
+
+    int radius() const { return conv.empty() ? 0 : (int)(conv.size() - 1) / 2; }
+    Rect required_input(const Rect& out) const;
+};
+
+// Thumbnail planner (thumbnail.rs:229-524), in-memory source (no
+// shrink-on-load), crop = false.
+enum ThumbTarget : int { FITBOX = 0, FORCE_EXACT = 1, WIDTH = 2, HEIGHT = 3 };
+struct PlanStep {
+    NodeKind kind;
+    double param;
+};
+Status plan_thumbnail(uint32_t in_w, uint32_t in_h, uint32_t bands, int target, uint32_t tw,
+                      uint32_t th, std::vector<PlanStep>* steps, uint32_t* out_w,
+                      uint32_t* out_h);
+// ResizeOp::into_pipeline_nodes, downscale subset (resize.rs:163-284).
+Status plan_resize(uint32_t in_w, uint32_t in_h, double hscale, double vscale, int kernel,
+                   std::vector<PlanStep>* steps, int* eff_kernel);
+
+}  // namespace vgpu

@@ -1,0 +1,922 @@
+// Device region executor, buffer pool and the C ABI.
+//
+// A pipeline is the reference's linear node chain (PipelineBuilder,
+// crates/viprs-runtime/src/pipeline/builder/*.rs).  Running it restates
+// execute_tile (crates/viprs-runtime/src/scheduler/rayon_scheduler/
+// execute.rs:413-849) once over the whole output instead of per 16-row strip:
+// the backward region walk gives every node the rectangle of Z^2 the
+// reference computes it on (rows unioned over strips, columns identical), the
+// source is read with clamp-to-edge, and each intermediate is evaluated on
+// that rectangle, negative and past-the-edge coordinates included.  Adjacent
+// nodes are fused into one launch where the intermediate can stay on chip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/viprs_amd.h"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+using namespace vgpu;
+
+namespace {
+
+// ------------------------------------------------------------ buffer pool
+
+// Size-bucketed cache of device allocations: hipMalloc only on a miss, never
+// on the steady-state run path.
+class BufferPool {
+  public:
+    ~BufferPool() { trim(true); }
+    void* acquire(size_t bytes) {
+        bytes = round_up(bytes);
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = free_.lower_bound(bytes);
+        if (it != free_.end() && it->first <= 2 * bytes) {
+            void* p = it->second;
+            free_.erase(it);
+            return p;
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        sizes_[p] = bytes;
+        held_ += bytes;
+        return p;
+    }
+    void release(void* p) {
+        if (!p) return;
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = sizes_.find(p);
+        if (it != sizes_.end()) free_.emplace(it->second, p);
+    }
+    void trim(bool all) {
+        std::lock_guard<std::mutex> g(mu_);
+        for (auto& kv : free_) {
+            (void)hipFree(kv.second);
+            held_ -= sizes_[kv.second];
+            sizes_.erase(kv.second);
+        }
+        free_.clear();
+        if (all) {
+            for (auto& kv : sizes_) (void)hipFree(kv.first);
+            sizes_.clear();
+            held_ = 0;
+        }
+    }
+    size_t held() const { return held_; }
+
+  private:
+    static size_t round_up(size_t b) { return (std::max<size_t>(b, 1) + 4095) & ~(size_t)4095; }
+    std::mutex mu_;
+    std::multimap<size_t, void*> free_;
+    std::map<void*, size_t> sizes_;
+    size_t held_ = 0;
+};
+
+int from_hip(hipError_t e) { return e == hipSuccess ? VGPU_OK : VGPU_E_DEVICE; }
+
+}  // namespace
+
+struct vgpu_context {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    BufferPool pool;
+    int sticky = VGPU_OK;
+};
+
+namespace {
+
+struct DeviceBlob {
+    void* ptr = nullptr;
+    vgpu_context* ctx = nullptr;
+    ~DeviceBlob() { reset(); }
+    void reset() {
+        if (ptr && ctx) ctx->pool.release(ptr);
+        ptr = nullptr;
+    }
+};
+
+// Host staging for per-node plans and tables, uploaded as one blob.
+struct Staging {
+    std::vector<unsigned char> bytes;
+    size_t put(const void* p, size_t n) {
+        size_t off = (bytes.size() + 15) & ~(size_t)15;
+        bytes.resize(off + n);
+        if (n) std::memcpy(bytes.data() + off, p, n);
+        return off;
+    }
+};
+
+enum StageKind { ST_SHRINK_HV_U8, ST_SHRINK, ST_REDUCE_H, ST_REDUCE_V, ST_CONV, ST_ALPHA, ST_COPY };
+
+struct Stage {
+    StageKind kind = ST_COPY;
+    int first = 0, last = 0;  // node index range [first, last]
+    Rect clip;                // reference tile rectangle of the first node's input
+    Rect out;                 // output rectangle of the last node
+    int in_fmt = U8, out_fmt = U8;
+    int fh = 1, fv = 1, premult = 0;
+    size_t plan_start = 0, plan_phase = 0, w16 = 0, w64 = 0;  // blob offsets
+    size_t itaps = 0, wts = 0;
+    int taps = 1;
+};
+
+Rect whole(uint32_t w, uint32_t h) { return Rect{0, 0, (int64_t)w, (int64_t)h}; }
+
+InView view_of(const void* ptr, const Rect& buf, const Rect& clip) {
+    InView v;
+    v.ptr = ptr;
+    v.bx = (int32_t)buf.x;
+    v.by = (int32_t)buf.y;
+    v.bw = (int32_t)buf.w;
+    v.bh = (int32_t)buf.h;
+    v.cx0 = (int32_t)clip.x;
+    v.cy0 = (int32_t)clip.y;
+    v.cx1 = (int32_t)(clip.x + std::max<int64_t>(clip.w, 1) - 1);
+    v.cy1 = (int32_t)(clip.y + std::max<int64_t>(clip.h, 1) - 1);
+    return v;
+}
+
+OutView outview_of(void* ptr, const Rect& r) {
+    OutView o;
+    o.ptr = ptr;
+    o.x = (int32_t)r.x;
+    o.y = (int32_t)r.y;
+    o.w = (int32_t)r.w;
+    o.h = (int32_t)r.h;
+    return o;
+}
+
+// Per-axis (start, phase) plan for a reduce node producing `n` outputs from
+// `origin`.  ReduceH on u8 accumulates the f64 source position column by
+// column from the region origin (reduceh.rs:112-119); every other case
+// evaluates the position directly (reduce_simd.rs:63,111,2716-2717).
+void build_plan(const Node& nd, int64_t origin, int64_t n, bool accumulate,
+                std::vector<int32_t>* starts, std::vector<uint8_t>* phases) {
+    starts->resize((size_t)std::max<int64_t>(n, 1));
+    phases->resize((size_t)std::max<int64_t>(n, 1));
+    double sx = nd.axis.source_position((double)origin);
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t st;
+        int ph;
+        const double sp = accumulate ? sx : nd.axis.source_position((double)origin + (double)i);
+        nd.axis.plan(sp, &st, &ph);
+        (*starts)[(size_t)i] = (int32_t)std::min<int64_t>(std::max<int64_t>(st, INT32_MIN), INT32_MAX);
+        (*phases)[(size_t)i] = (uint8_t)ph;
+        sx += nd.axis.factor;
+    }
+}
+
+void stage_reduce_params(const Node& nd, const Rect& out, Staging* st, Stage* s) {
+    std::vector<int32_t> starts;
+    std::vector<uint8_t> phases;
+    if (nd.kind == N_REDUCE_H)
+        build_plan(nd, out.x, out.w, nd.in_fmt == U8, &starts, &phases);
+    else
+        build_plan(nd, out.y, out.h, false, &starts, &phases);
+    s->plan_start = st->put(starts.data(), starts.size() * sizeof(int32_t));
+    s->plan_phase = st->put(phases.data(), phases.size());
+    s->w16 = st->put(nd.axis.wi16.data(), nd.axis.wi16.size() * sizeof(int16_t));
+    s->w64 = st->put(nd.axis.wf64.data(), nd.axis.wf64.size() * sizeof(double));
+    s->taps = nd.axis.taps;
+}
+
+void stage_conv_params(const Node& nd, Staging* st, Stage* s) {
+    s->itaps = st->put(nd.itaps.data(), nd.itaps.size() * sizeof(int16_t));
+    s->wts = st->put(nd.conv.data(), nd.conv.size() * sizeof(double));
+    s->taps = (int)nd.conv.size();
+}
+
+// Launch one stage reading `in` and writing `dst` (dense over s.out).
+int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& in, void* dst,
+                 const unsigned char* blob, hipStream_t stream) {
+    const Node& nd = nodes[(size_t)s.first];
+    const OutView out = outview_of(dst, s.out);
+    const int B = (int)nd.bands;
+    hipError_t e = hipSuccess;
+    switch (s.kind) {
+    case ST_SHRINK_HV_U8: e = launch_shrink_hv_u8(in, out, B, s.fh, s.fv, s.premult, stream); break;
+    case ST_SHRINK:
+        e = launch_shrink_generic(in, out, B, nd.in_fmt, (int)nd.factor, nd.kind == N_SHRINK_V, stream);
+        break;
+    case ST_REDUCE_H:
+    case ST_REDUCE_V: {
+        DevPlan p;
+        p.start = reinterpret_cast<const int32_t*>(blob + s.plan_start);
+        p.phase = reinterpret_cast<const uint8_t*>(blob + s.plan_phase);
+        p.wi16 = reinterpret_cast<const int16_t*>(blob + s.w16);
+        p.wf64 = reinterpret_cast<const double*>(blob + s.w64);
+        p.taps = s.taps;
+        e = s.kind == ST_REDUCE_H ? launch_reduce_h(in, out, B, nd.in_fmt, p, 0, stream)
+                                  : launch_reduce_v(in, out, B, nd.in_fmt, p, 0, stream);
+        break;
+    }
+    case ST_CONV: {
+        const bool gauss = nd.kind == N_GAUSS_H || nd.kind == N_GAUSS_V;
+        const bool vert = nd.kind == N_GAUSS_V || nd.kind == N_CONVSEP_V;
+        e = launch_conv(in, out, B, nd.in_fmt, nd.out_fmt, gauss ? 0 : 1, vert ? 1 : 0,
+                        reinterpret_cast<const int16_t*>(blob + s.itaps), nd.iscale,
+                        reinterpret_cast<const double*>(blob + s.wts), s.taps, nd.cscale,
+                        nd.coffset, stream);
+        break;
+    }
+    case ST_ALPHA:
+        e = launch_alpha_u8(in, out, B, nd.kind == N_PREMULT ? ALPHA_PREMULT : ALPHA_UNPREMULT, stream);
+        break;
+    case ST_COPY: e = launch_copy(in, out, B, nd.in_fmt, stream); break;
+    }
+    return from_hip(e);
+}
+
+}  // namespace
+
+struct vgpu_pipeline {
+    vgpu_context* ctx = nullptr;
+    uint32_t src_w = 0, src_h = 0, bands = 0;
+    int src_fmt = U8;
+    uint32_t cur_w = 0, cur_h = 0;
+    int cur_fmt = U8;
+    std::vector<Node> nodes;
+
+    bool compiled = false;
+    std::vector<Stage> stages;
+    DeviceBlob blob;
+    std::vector<void*> inter;  // stage outputs except the last
+
+    ~vgpu_pipeline() { release(); }
+    void release() {
+        for (void* p : inter) ctx->pool.release(p);
+        inter.clear();
+        blob.reset();
+        stages.clear();
+        compiled = false;
+    }
+    Node& push(NodeKind k) {
+        release();
+        Node nd;
+        nd.kind = k;
+        nd.in_fmt = nd.out_fmt = cur_fmt;
+        nd.bands = bands;
+        nd.in_w = nd.out_w = cur_w;
+        nd.in_h = nd.out_h = cur_h;
+        nodes.push_back(nd);
+        return nodes.back();
+    }
+    void commit() {
+        cur_w = nodes.back().out_w;
+        cur_h = nodes.back().out_h;
+        cur_fmt = nodes.back().out_fmt;
+    }
+    int compile();
+};
+
+int vgpu_pipeline::compile() {
+    if (compiled) return VGPU_OK;
+    const int n = (int)nodes.size();
+    if (n == 0 || cur_w == 0 || cur_h == 0) {
+        compiled = true;
+        return VGPU_OK;
+    }
+    // Backward region walk (execute.rs:413-480) from the full output.
+    std::vector<Rect> out_r((size_t)n), in_r((size_t)n);
+    out_r[(size_t)n - 1] = whole(cur_w, cur_h);
+    for (int i = n - 1; i >= 0; --i) {
+        in_r[(size_t)i] = nodes[(size_t)i].required_input(out_r[(size_t)i]);
+        if (i > 0) out_r[(size_t)i - 1] = in_r[(size_t)i];
+    }
+    Staging st;
+    for (int i = 0; i < n;) {
+        const Node& nd = nodes[(size_t)i];
+        Stage s;
+        s.first = i;
+        s.clip = in_r[(size_t)i];
+        s.in_fmt = nd.in_fmt;
+        const bool u8small = nd.in_fmt == U8 && nd.bands <= 4;
+        if (u8small && (nd.kind == N_SHRINK_H || nd.kind == N_SHRINK_V ||
+                        (nd.kind == N_PREMULT && i + 1 < n && nodes[(size_t)i + 1].kind == N_SHRINK_H))) {
+            // Fused [Premultiply] -> ShrinkH -> [ShrinkV] (or a lone ShrinkV).
+            int j = i;
+            s.kind = ST_SHRINK_HV_U8;
+            if (nodes[(size_t)j].kind == N_PREMULT) {
+                s.premult = 1;
+                ++j;
+            }
+            if (nodes[(size_t)j].kind == N_SHRINK_H) {
+                s.fh = (int)nodes[(size_t)j].factor;
+                if (j + 1 < n && nodes[(size_t)j + 1].kind == N_SHRINK_V) {
+                    ++j;
+                    s.fv = (int)nodes[(size_t)j].factor;
+                }
+            } else {
+                s.fv = (int)nodes[(size_t)j].factor;
+            }
+            s.last = j;
+        } else {
+            s.last = i;
+            switch (nd.kind) {
+            case N_SHRINK_H:
+            case N_SHRINK_V: s.kind = ST_SHRINK; break;
+            case N_REDUCE_H: s.kind = ST_REDUCE_H; break;
+            case N_REDUCE_V: s.kind = ST_REDUCE_V; break;
+            case N_GAUSS_H:
+            case N_GAUSS_V:
+            case N_CONVSEP_H:
+            case N_CONVSEP_V: s.kind = ST_CONV; break;
+            case N_PREMULT:
+            case N_UNPREMULT: s.kind = ST_ALPHA; break;
+            default: s.kind = ST_COPY; break;
+            }
+        }
+        s.out = out_r[(size_t)s.last];
+        s.out_fmt = nodes[(size_t)s.last].out_fmt;
+        if (s.kind == ST_REDUCE_H || s.kind == ST_REDUCE_V) stage_reduce_params(nd, s.out, &st, &s);
+        if (s.kind == ST_CONV) stage_conv_params(nd, &st, &s);
+        stages.push_back(s);
+        i = s.last + 1;
+    }
+    if (!st.bytes.empty()) {
+        blob.ctx = ctx;
+        blob.ptr = ctx->pool.acquire(st.bytes.size());
+        if (!blob.ptr) return VGPU_E_NO_MEMORY;
+        hipError_t e = hipMemcpyAsync(blob.ptr, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return VGPU_E_DEVICE;
+    }
+    for (size_t k = 0; k + 1 < stages.size(); ++k) {
+        const Stage& s = stages[k];
+        const size_t bytes = (size_t)s.out.w * (size_t)s.out.h * bands * format_size(s.out_fmt);
+        void* p = ctx->pool.acquire(bytes);
+        if (!p) return VGPU_E_NO_MEMORY;
+        inter.push_back(p);
+    }
+    compiled = true;
+    return VGPU_OK;
+}
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+const char* vgpu_status_string(int s) {
+    switch (s) {
+    case VGPU_OK: return "ok";
+    case VGPU_E_INVALID_ARGUMENT: return "invalid argument";
+    case VGPU_E_INVALID_KERNEL: return "invalid interpolation kernel for this op";
+    case VGPU_E_UNSUPPORTED_FORMAT: return "unsupported band format";
+    case VGPU_E_UNSUPPORTED: return "plan needs a node outside the GPU path";
+    case VGPU_E_BUFFER_SIZE: return "buffer length does not match region";
+    case VGPU_E_DEVICE: return "HIP device error";
+    case VGPU_E_NO_MEMORY: return "out of device memory";
+    case VGPU_E_CANCELLED: return "cancelled";
+    case VGPU_E_ADMISSION: return "request exceeds the device memory budget";
+    default: return "unknown status";
+    }
+}
+
+int vgpu_context_create(int device, vgpu_context** out) {
+    if (!out) return VGPU_E_INVALID_ARGUMENT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+        (void)hipGetLastError();
+        return VGPU_E_DEVICE;
+    }
+    if (hipSetDevice(device) != hipSuccess) return VGPU_E_DEVICE;
+    auto* c = new (std::nothrow) vgpu_context();
+    if (!c) return VGPU_E_NO_MEMORY;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return VGPU_E_DEVICE;
+    }
+    c->stream = c->own;
+    *out = c;
+    return VGPU_OK;
+}
+
+void vgpu_context_destroy(vgpu_context* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->pool.trim(true);
+    (void)hipStreamDestroy(c->own);
+    delete c;
+}
+
+int vgpu_context_set_stream(vgpu_context* c, void* s) {
+    if (!c) return VGPU_E_INVALID_ARGUMENT;
+    c->stream = s ? static_cast<hipStream_t>(s) : c->own;
+    return VGPU_OK;
+}
+
+void* vgpu_context_stream(vgpu_context* c) { return c ? (void*)c->stream : nullptr; }
+
+int vgpu_synchronize(vgpu_context* c) {
+    if (!c) return VGPU_E_INVALID_ARGUMENT;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) c->sticky = VGPU_E_DEVICE;
+    return c->sticky;
+}
+
+size_t vgpu_context_pool_bytes(const vgpu_context* c) { return c ? c->pool.held() : 0; }
+
+int vgpu_context_trim(vgpu_context* c) {
+    if (!c) return VGPU_E_INVALID_ARGUMENT;
+    c->pool.trim(false);
+    return VGPU_OK;
+}
+
+int vgpu_pipeline_new(vgpu_context* ctx, uint32_t w, uint32_t h, uint32_t bands, int fmt,
+                      vgpu_pipeline** out) {
+    if (!ctx || !out) return VGPU_E_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (bands == 0) return VGPU_E_INVALID_ARGUMENT;
+    if (fmt < U8 || fmt > F32) return VGPU_E_UNSUPPORTED_FORMAT;
+    if (w > INT32_MAX / 4 || h > INT32_MAX / 4) return VGPU_E_INVALID_ARGUMENT;
+    auto* p = new (std::nothrow) vgpu_pipeline();
+    if (!p) return VGPU_E_NO_MEMORY;
+    p->ctx = ctx;
+    p->src_w = p->cur_w = w;
+    p->src_h = p->cur_h = h;
+    p->bands = bands;
+    p->src_fmt = p->cur_fmt = fmt;
+    *out = p;
+    return VGPU_OK;
+}
+
+void vgpu_pipeline_destroy(vgpu_pipeline* p) { delete p; }
+
+int vgpu_pipeline_reduce_h(vgpu_pipeline* p, double factor, int kernel) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    ReduceAxis ax;
+    int st = build_reduce_axis(kernel, factor, p->cur_w, &ax);
+    if (st != OK) return st;
+    Node& nd = p->push(N_REDUCE_H);
+    nd.axis = std::move(ax);
+    nd.out_w = reduce_output_len(factor, p->cur_w);
+    p->commit();
+    return VGPU_OK;
+}
+
+int vgpu_pipeline_reduce_v(vgpu_pipeline* p, double factor, int kernel) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    ReduceAxis ax;
+    int st = build_reduce_axis(kernel, factor, p->cur_h, &ax);
+    if (st != OK) return st;
+    Node& nd = p->push(N_REDUCE_V);
+    nd.axis = std::move(ax);
+    nd.out_h = reduce_output_len(factor, p->cur_h);
+    p->commit();
+    return VGPU_OK;
+}
+
+int vgpu_pipeline_shrink_h(vgpu_pipeline* p, uint32_t factor, int ceil_mode) {
+    if (!p || factor == 0) return VGPU_E_INVALID_ARGUMENT;
+    const uint32_t w = p->cur_w;
+    Node& nd = p->push(N_SHRINK_H);
+    nd.factor = factor;
+    nd.source_width = w;  // the builder always passes Some(current width)
+    nd.out_w = ceil_mode ? (w + factor - 1) / factor : w / factor;
+    p->commit();
+    return VGPU_OK;
+}
+
+int vgpu_pipeline_shrink_v(vgpu_pipeline* p, uint32_t factor, int ceil_mode) {
+    if (!p || factor == 0) return VGPU_E_INVALID_ARGUMENT;
+    const uint32_t h = p->cur_h;
+    Node& nd = p->push(N_SHRINK_V);
+    nd.factor = factor;
+    nd.out_h = ceil_mode ? (h + factor - 1) / factor : h / factor;
+    p->commit();
+    return VGPU_OK;
+}
+
+int vgpu_pipeline_gauss_blur(vgpu_pipeline* p, float sigma) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    if (!(sigma == sigma)) return VGPU_E_INVALID_ARGUMENT;
+    GaussMask m = build_gauss_mask(sigma);
+    for (int pass = 0; pass < 2; ++pass) {
+        Node& nd = p->push(pass == 0 ? N_GAUSS_H : N_GAUSS_V);
+        nd.itaps = m.taps;
+        nd.iscale = m.scale;
+        nd.conv = m.weights;
+        nd.out_fmt = p->cur_fmt == U8 ? U8 : F32;
+        p->commit();
+    }
+    return VGPU_OK;
+}
+
+int vgpu_pipeline_convsep(vgpu_pipeline* p, const double* coeffs, uint32_t n, double scale,
+                          double offset) {
+    if (!p || !coeffs || n == 0 || n % 2 == 0) return VGPU_E_INVALID_ARGUMENT;
+    for (uint32_t i = 0; i < n; ++i)
+        if (!std::isfinite(coeffs[i])) return VGPU_E_INVALID_ARGUMENT;
+    if (!std::isfinite(scale) || scale == 0.0 || !std::isfinite(offset)) return VGPU_E_INVALID_ARGUMENT;
+    for (int pass = 0; pass < 2; ++pass) {
+        Node& nd = p->push(pass == 0 ? N_CONVSEP_H : N_CONVSEP_V);
+        nd.conv.assign(coeffs, coeffs + n);
+        nd.itaps.assign(n, 0);
+        nd.cscale = scale;
+        nd.coffset = pass == 0 ? offset : 0.0;
+        nd.out_fmt = F32;
+        p->commit();
+    }
+    return VGPU_OK;
+}
+
+int vgpu_pipeline_premultiply(vgpu_pipeline* p) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    if (p->cur_fmt != U8) return VGPU_E_UNSUPPORTED_FORMAT;
+    if (p->bands < 2) return VGPU_E_INVALID_ARGUMENT;
+    p->push(N_PREMULT);
+    p->commit();
+    return VGPU_OK;
+}
+
+int vgpu_pipeline_unpremultiply(vgpu_pipeline* p) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    if (p->cur_fmt != U8) return VGPU_E_UNSUPPORTED_FORMAT;
+    if (p->bands < 2) return VGPU_E_INVALID_ARGUMENT;
+    p->push(N_UNPREMULT);
+    p->commit();
+    return VGPU_OK;
+}
+
+static int apply_steps(vgpu_pipeline* p, const std::vector<PlanStep>& steps, int kernel) {
+    for (const PlanStep& s : steps) {
+        int rc = VGPU_OK;
+        switch (s.kind) {
+        case N_PREMULT: rc = vgpu_pipeline_premultiply(p); break;
+        case N_UNPREMULT: rc = vgpu_pipeline_unpremultiply(p); break;
+        case N_SHRINK_H: rc = vgpu_pipeline_shrink_h(p, (uint32_t)s.param, 0); break;
+        case N_SHRINK_V: rc = vgpu_pipeline_shrink_v(p, (uint32_t)s.param, 0); break;
+        case N_REDUCE_H: rc = vgpu_pipeline_reduce_h(p, s.param, kernel); break;
+        case N_REDUCE_V: rc = vgpu_pipeline_reduce_v(p, s.param, kernel); break;
+        default: rc = VGPU_E_UNSUPPORTED; break;
+        }
+        if (rc != VGPU_OK) return rc;
+    }
+    if (steps.empty()) {
+        p->push(N_COPY);
+        p->commit();
+    }
+    return VGPU_OK;
+}
+
+int vgpu_pipeline_thumbnail(vgpu_pipeline* p, int target, uint32_t w, uint32_t h, int kernel) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    if (kernel == LBB || kernel == VSQBS || kernel == NOHALO || kernel < NEAREST || kernel > CATMULLROM)
+        return VGPU_E_INVALID_KERNEL;
+    std::vector<PlanStep> steps;
+    uint32_t ow = 0, oh = 0;
+    int rc = plan_thumbnail(p->cur_w, p->cur_h, p->bands, target, w, h, &steps, &ow, &oh);
+    if (rc != OK) return rc;
+    if (p->cur_fmt != U8)
+        for (const PlanStep& s : steps)
+            if (s.kind == N_PREMULT) return VGPU_E_UNSUPPORTED_FORMAT;
+    return apply_steps(p, steps, kernel);
+}
+
+int vgpu_pipeline_resize(vgpu_pipeline* p, double hscale, double vscale, int kernel) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    std::vector<PlanStep> steps;
+    int eff = kernel;
+    int rc = plan_resize(p->cur_w, p->cur_h, hscale, vscale, kernel, &steps, &eff);
+    if (rc != OK) return rc;
+    return apply_steps(p, steps, eff);
+}
+
+int vgpu_pipeline_output_info(const vgpu_pipeline* p, uint32_t* w, uint32_t* h, uint32_t* b,
+                              int* f) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    if (w) *w = p->cur_w;
+    if (h) *h = p->cur_h;
+    if (b) *b = p->bands;
+    if (f) *f = p->cur_fmt;
+    return VGPU_OK;
+}
+
+int vgpu_pipeline_stage_count(vgpu_pipeline* p) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(p->ctx->device);
+    int rc = p->compile();
+    return rc != VGPU_OK ? rc : (int)p->stages.size();
+}
+
+int vgpu_pipeline_run(vgpu_pipeline* p, const void* src, size_t src_bytes, void* dst,
+                      size_t dst_bytes) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    vgpu_context* ctx = p->ctx;
+    if (ctx->sticky != VGPU_OK) return ctx->sticky;
+    const size_t in_bytes = (size_t)p->src_w * p->src_h * p->bands * format_size(p->src_fmt);
+    const size_t out_bytes = (size_t)p->cur_w * p->cur_h * p->bands * format_size(p->cur_fmt);
+    if (src_bytes != in_bytes || dst_bytes != out_bytes) return VGPU_E_BUFFER_SIZE;
+    if (out_bytes == 0) return VGPU_OK;
+    if (!src || !dst) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    int rc = p->compile();
+    if (rc != VGPU_OK) return rc;
+    if (p->stages.empty()) {
+        return from_hip(hipMemcpyAsync(dst, src, out_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    const unsigned char* blob = static_cast<const unsigned char*>(p->blob.ptr);
+    const void* cur = src;
+    Rect cur_rect = whole(p->src_w, p->src_h);
+    for (size_t k = 0; k < p->stages.size(); ++k) {
+        const Stage& s = p->stages[k];
+        void* o = (k + 1 == p->stages.size()) ? dst : p->inter[k];
+        rc = launch_stage(p->nodes, s, view_of(cur, cur_rect, s.clip), o, blob, ctx->stream);
+        if (rc != VGPU_OK) {
+            ctx->sticky = rc;
+            return rc;
+        }
+        cur = o;
+        cur_rect = s.out;
+    }
+    return VGPU_OK;
+}
+
+int vgpu_pipeline_run_profiled(vgpu_pipeline* p, const void* src, size_t src_bytes, void* dst,
+                               size_t dst_bytes, int iterations, float* stage_ms, int max_stages) {
+    if (!p || iterations < 1 || !stage_ms) return VGPU_E_INVALID_ARGUMENT;
+    vgpu_context* ctx = p->ctx;
+    (void)hipSetDevice(ctx->device);
+    int rc = p->compile();
+    if (rc != VGPU_OK) return rc;
+    const int ns = (int)p->stages.size();
+    if (ns == 0) return VGPU_E_UNSUPPORTED;
+    if (max_stages < ns) return VGPU_E_BUFFER_SIZE;
+    const size_t in_bytes = (size_t)p->src_w * p->src_h * p->bands * format_size(p->src_fmt);
+    const size_t out_bytes = (size_t)p->cur_w * p->cur_h * p->bands * format_size(p->cur_fmt);
+    if (src_bytes != in_bytes || dst_bytes != out_bytes) return VGPU_E_BUFFER_SIZE;
+    std::vector<hipEvent_t> ev((size_t)ns + 1);
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return VGPU_E_DEVICE;
+    std::vector<double> acc((size_t)ns, 0.0);
+    const unsigned char* blob = static_cast<const unsigned char*>(p->blob.ptr);
+    for (int it = 0; it < iterations && rc == VGPU_OK; ++it) {
+        const void* cur = src;
+        Rect cur_rect = whole(p->src_w, p->src_h);
+        (void)hipEventRecord(ev[0], ctx->stream);
+        for (int k = 0; k < ns && rc == VGPU_OK; ++k) {
+            const Stage& s = p->stages[(size_t)k];
+            void* o = (k + 1 == ns) ? dst : p->inter[(size_t)k];
+            rc = launch_stage(p->nodes, s, view_of(cur, cur_rect, s.clip), o, blob, ctx->stream);
+            (void)hipEventRecord(ev[(size_t)k + 1], ctx->stream);
+            cur = o;
+            cur_rect = s.out;
+        }
+        if (hipEventSynchronize(ev[(size_t)ns]) != hipSuccess) rc = VGPU_E_DEVICE;
+        for (int k = 0; k < ns && rc == VGPU_OK; ++k) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[(size_t)k], ev[(size_t)k + 1]);
+            acc[(size_t)k] += ms;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < ns; ++k) stage_ms[k] = (float)(acc[(size_t)k] / iterations);
+    return rc == VGPU_OK ? ns : rc;
+}
+
+int vgpu_pipeline_run_host(vgpu_pipeline* p, const void* src, size_t src_bytes, void* dst,
+                           size_t dst_bytes) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    vgpu_context* ctx = p->ctx;
+    (void)hipSetDevice(ctx->device);
+    void* dsrc = ctx->pool.acquire(src_bytes);
+    void* ddst = ctx->pool.acquire(dst_bytes);
+    if (!dsrc || !ddst) {
+        ctx->pool.release(dsrc);
+        ctx->pool.release(ddst);
+        return VGPU_E_NO_MEMORY;
+    }
+    int rc = from_hip(hipMemcpyAsync(dsrc, src, src_bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (rc == VGPU_OK) rc = vgpu_pipeline_run(p, dsrc, src_bytes, ddst, dst_bytes);
+    if (rc == VGPU_OK) rc = from_hip(hipMemcpyAsync(dst, ddst, dst_bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (rc == VGPU_OK) rc = vgpu_synchronize(ctx);
+    else (void)hipStreamSynchronize(ctx->stream);
+    ctx->pool.release(dsrc);
+    ctx->pool.release(ddst);
+    return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- op level
+
+struct vgpu_op {
+    vgpu_context* ctx = nullptr;
+    Node node;
+    DeviceBlob blob;
+    Rect blob_key;  // output region the uploaded plan was built for
+    bool have_blob = false;
+};
+
+static int make_op(vgpu_context* ctx, Node nd, vgpu_op** out) {
+    if (!ctx || !out) return VGPU_E_INVALID_ARGUMENT;
+    auto* op = new (std::nothrow) vgpu_op();
+    if (!op) return VGPU_E_NO_MEMORY;
+    op->ctx = ctx;
+    op->node = std::move(nd);
+    op->blob.ctx = ctx;
+    *out = op;
+    return VGPU_OK;
+}
+
+static Node base_node(NodeKind k, uint32_t bands, int fmt) {
+    Node nd;
+    nd.kind = k;
+    nd.bands = bands;
+    nd.in_fmt = nd.out_fmt = fmt;
+    return nd;
+}
+
+static bool valid_fmt(int f) { return f >= U8 && f <= F32; }
+
+extern "C" {
+
+int vgpu_op_reduce_h_new(vgpu_context* ctx, double factor, int kernel, uint32_t bands,
+                         uint32_t input_w, int fmt, vgpu_op** out) {
+    if (bands == 0) return VGPU_E_INVALID_ARGUMENT;
+    if (!valid_fmt(fmt)) return VGPU_E_UNSUPPORTED_FORMAT;
+    Node nd = base_node(N_REDUCE_H, bands, fmt);
+    int st = build_reduce_axis(kernel, factor, input_w, &nd.axis);
+    if (st != OK) return st;
+    return make_op(ctx, std::move(nd), out);
+}
+
+int vgpu_op_reduce_v_new(vgpu_context* ctx, double factor, int kernel, uint32_t bands,
+                         uint32_t input_h, int fmt, vgpu_op** out) {
+    if (bands == 0) return VGPU_E_INVALID_ARGUMENT;
+    if (!valid_fmt(fmt)) return VGPU_E_UNSUPPORTED_FORMAT;
+    Node nd = base_node(N_REDUCE_V, bands, fmt);
+    int st = build_reduce_axis(kernel, factor, input_h, &nd.axis);
+    if (st != OK) return st;
+    return make_op(ctx, std::move(nd), out);
+}
+
+int vgpu_op_shrink_h_new(vgpu_context* ctx, uint32_t factor, int ceil_mode, uint32_t bands,
+                         uint32_t source_width, int fmt, vgpu_op** out) {
+    (void)ceil_mode;
+    if (factor == 0 || bands == 0) return VGPU_E_INVALID_ARGUMENT;
+    if (!valid_fmt(fmt)) return VGPU_E_UNSUPPORTED_FORMAT;
+    Node nd = base_node(N_SHRINK_H, bands, fmt);
+    nd.factor = factor;
+    nd.source_width = source_width;
+    return make_op(ctx, std::move(nd), out);
+}
+
+int vgpu_op_shrink_v_new(vgpu_context* ctx, uint32_t factor, int ceil_mode, uint32_t bands,
+                         int fmt, vgpu_op** out) {
+    (void)ceil_mode;
+    if (factor == 0 || bands == 0) return VGPU_E_INVALID_ARGUMENT;
+    if (!valid_fmt(fmt)) return VGPU_E_UNSUPPORTED_FORMAT;
+    Node nd = base_node(N_SHRINK_V, bands, fmt);
+    nd.factor = factor;
+    return make_op(ctx, std::move(nd), out);
+}
+
+static int gauss_op(vgpu_context* ctx, NodeKind k, float sigma, uint32_t bands, int fmt, vgpu_op** out) {
+    if (bands == 0 || !(sigma == sigma)) return VGPU_E_INVALID_ARGUMENT;
+    if (!valid_fmt(fmt)) return VGPU_E_UNSUPPORTED_FORMAT;
+    Node nd = base_node(k, bands, fmt);
+    GaussMask m = build_gauss_mask(sigma);
+    nd.itaps = m.taps;
+    nd.iscale = m.scale;
+    nd.conv = m.weights;
+    nd.out_fmt = fmt == U8 ? U8 : F32;
+    return make_op(ctx, std::move(nd), out);
+}
+
+int vgpu_op_gauss_blur_h_new(vgpu_context* ctx, float sigma, uint32_t bands, int fmt, vgpu_op** out) {
+    return gauss_op(ctx, N_GAUSS_H, sigma, bands, fmt, out);
+}
+int vgpu_op_gauss_blur_v_new(vgpu_context* ctx, float sigma, uint32_t bands, int fmt, vgpu_op** out) {
+    return gauss_op(ctx, N_GAUSS_V, sigma, bands, fmt, out);
+}
+
+int vgpu_op_premultiply_new(vgpu_context* ctx, uint32_t bands, int fmt, vgpu_op** out) {
+    if (bands < 2) return VGPU_E_INVALID_ARGUMENT;
+    if (fmt != U8) return VGPU_E_UNSUPPORTED_FORMAT;
+    return make_op(ctx, base_node(N_PREMULT, bands, fmt), out);
+}
+int vgpu_op_unpremultiply_new(vgpu_context* ctx, uint32_t bands, int fmt, vgpu_op** out) {
+    if (bands < 2) return VGPU_E_INVALID_ARGUMENT;
+    if (fmt != U8) return VGPU_E_UNSUPPORTED_FORMAT;
+    return make_op(ctx, base_node(N_UNPREMULT, bands, fmt), out);
+}
+
+void vgpu_op_destroy(vgpu_op* op) { delete op; }
+
+static Rect rect_of(vgpu_region r) { return Rect{r.x, r.y, (int64_t)r.width, (int64_t)r.height}; }
+
+int vgpu_op_required_input_region(const vgpu_op* op, vgpu_region o, vgpu_region* in) {
+    if (!op || !in) return VGPU_E_INVALID_ARGUMENT;
+    Rect r = op->node.required_input(rect_of(o));
+    if (r.w < 0 || r.h < 0 || r.w > UINT32_MAX || r.h > UINT32_MAX) return VGPU_E_INVALID_ARGUMENT;
+    in->x = (int32_t)r.x;
+    in->y = (int32_t)r.y;
+    in->width = (uint32_t)r.w;
+    in->height = (uint32_t)r.h;
+    return VGPU_OK;
+}
+
+int vgpu_op_output_size(const vgpu_op* op, uint32_t iw, uint32_t ih, uint32_t* ow, uint32_t* oh) {
+    if (!op || !ow || !oh) return VGPU_E_INVALID_ARGUMENT;
+    const Node& nd = op->node;
+    *ow = iw;
+    *oh = ih;
+    switch (nd.kind) {
+    case N_REDUCE_H: *ow = reduce_output_len(nd.axis.factor, iw); break;
+    case N_REDUCE_V: *oh = reduce_output_len(nd.axis.factor, ih); break;
+    case N_SHRINK_H: *ow = iw / nd.factor; break;
+    case N_SHRINK_V: *oh = ih / nd.factor; break;
+    default: break;
+    }
+    return VGPU_OK;
+}
+
+int vgpu_op_output_format(const vgpu_op* op, int* f) {
+    if (!op || !f) return VGPU_E_INVALID_ARGUMENT;
+    *f = op->node.out_fmt;
+    return VGPU_OK;
+}
+
+int vgpu_op_process_region(vgpu_op* op, const void* input, size_t in_len, void* output,
+                           size_t out_len, vgpu_region ir, vgpu_region orr) {
+    if (!op) return VGPU_E_INVALID_ARGUMENT;
+    vgpu_context* ctx = op->ctx;
+    if (ctx->sticky != VGPU_OK) return ctx->sticky;
+    const Node& nd = op->node;
+    const Rect in_r = rect_of(ir), out_r = rect_of(orr);
+    if (in_len != (size_t)in_r.w * in_r.h * nd.bands * format_size(nd.in_fmt) ||
+        out_len != (size_t)out_r.w * out_r.h * nd.bands * format_size(nd.out_fmt))
+        return VGPU_E_BUFFER_SIZE;
+    if (out_r.empty()) return VGPU_OK;
+    if (!input || !output) return VGPU_E_INVALID_ARGUMENT;
+    if (nd.kind != N_REDUCE_H && nd.kind != N_REDUCE_V) {
+        // These ops index their input relative to the tile origin the
+        // scheduler derived from required_input_region; insist on it.
+        const Rect need = nd.required_input(out_r);
+        if (need.x != in_r.x || need.y != in_r.y || need.w != in_r.w || need.h != in_r.h)
+            return VGPU_E_INVALID_ARGUMENT;
+    }
+    (void)hipSetDevice(ctx->device);
+    Stage s;
+    s.first = s.last = 0;
+    s.clip = in_r;
+    s.out = out_r;
+    s.in_fmt = nd.in_fmt;
+    s.out_fmt = nd.out_fmt;
+    Staging st;
+    switch (nd.kind) {
+    case N_SHRINK_H:
+    case N_SHRINK_V:
+        if (nd.in_fmt == U8 && nd.bands <= 4) {
+            s.kind = ST_SHRINK_HV_U8;
+            (nd.kind == N_SHRINK_H ? s.fh : s.fv) = (int)nd.factor;
+        } else {
+            s.kind = ST_SHRINK;
+        }
+        break;
+    case N_REDUCE_H: s.kind = ST_REDUCE_H; stage_reduce_params(nd, out_r, &st, &s); break;
+    case N_REDUCE_V: s.kind = ST_REDUCE_V; stage_reduce_params(nd, out_r, &st, &s); break;
+    case N_GAUSS_H:
+    case N_GAUSS_V: s.kind = ST_CONV; stage_conv_params(nd, &st, &s); break;
+    case N_PREMULT:
+    case N_UNPREMULT: s.kind = ST_ALPHA; break;
+    default: s.kind = ST_COPY; break;
+    }
+    if (!st.bytes.empty()) {
+        const bool same = op->have_blob && op->blob_key.x == out_r.x && op->blob_key.y == out_r.y &&
+                          op->blob_key.w == out_r.w && op->blob_key.h == out_r.h;
+        if (!same) {
+            // The previous plan may still be read by queued work.
+            if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
+            op->blob.reset();
+            op->blob.ptr = ctx->pool.acquire(st.bytes.size());
+            if (!op->blob.ptr) return VGPU_E_NO_MEMORY;
+            hipError_t e = hipMemcpyAsync(op->blob.ptr, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
+            op->blob_key = out_r;
+            op->have_blob = true;
+        }
+    }
+    std::vector<Node> one{nd};
+    int rc = launch_stage(one, s, view_of(input, in_r, in_r), output,
+                          static_cast<const unsigned char*>(op->blob.ptr), ctx->stream);
+    if (rc != VGPU_OK) ctx->sticky = rc;
+    return rc;
+}
+
+}  // extern "C"
