@@ -47,26 +47,11 @@ def load_traffic(name):
         return None
 
 
-def time_pipeline(torch, pipe, d_src, d_dst, steps, warmup):
-    for _ in range(warmup):
-        pipe.run(d_src, d_dst)
-    torch.cuda.synchronize()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record()
-    for _ in range(steps):
-        pipe.run(d_src, d_dst)
-    end.record()
-    torch.cuda.synchronize()
-    total_ms = start.elapsed_time(end)
-    # Per-step latency distribution: events between consecutive steps.
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
-    evs[0].record()
-    for i in range(steps):
-        pipe.run(d_src, d_dst)
-        evs[i + 1].record()
-    torch.cuda.synchronize()
-    lat = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(steps)])
-    return total_ms / steps, lat
+def time_pipeline(pipe, d_src, d_dst, steps, warmup):
+    """HIP events on the executor's stream (vgpu_pipeline_benchmark): W untimed
+    steps, K back-to-back timed steps, then K steps with an event after each
+    for the latency distribution."""
+    return pipe.benchmark(d_src, d_dst, warmup, steps)
 
 
 def cpu_baseline(O, src_np, steps_fn, budget_s):
@@ -99,8 +84,6 @@ def main():
               "the path shards by image with no collective, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
     import viprs_amd as v
     from oracle import oracle as O
 
@@ -109,23 +92,21 @@ def main():
     B = 3
     src = gradient_noise(H, W, B, 20250611)
     ctx = v.Context(0)
-    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
     pipe = v.Pipeline(ctx, W, H, B).thumbnail(v.WIDTH, 400, 0, v.LANCZOS3)
     oh, ow, ob = pipe.output_shape
-    d_src = torch.from_numpy(src).cuda()
-    d_dst = torch.empty((oh, ow, ob), dtype=torch.uint8, device="cuda")
-    torch.cuda.synchronize()
+    d_src = v.DeviceBuffer.from_numpy(ctx, src)
+    d_dst = pipe.output_buffer()
 
     print("bench: parity check vs oracle", file=sys.stderr, flush=True)
     pipe.run(d_src, d_dst)
-    torch.cuda.synchronize()
+    ctx.synchronize()
     exp = O.Pipeline(W, H, B).thumbnail(O.WIDTH, 400).run(src)
-    bit_exact = bool(np.array_equal(d_dst.cpu().numpy(), exp))
+    bit_exact = bool(np.array_equal(d_dst.numpy(), exp))
 
     print("bench: timing", file=sys.stderr, flush=True)
-    ms, lat = time_pipeline(torch, pipe, d_src, d_dst, args.steps, args.warmup)
+    ms, lat = time_pipeline(pipe, d_src, d_dst, args.steps, args.warmup)
     stage_ms = pipe.run_profiled(d_src, d_dst, max(args.steps, 1))
-    torch.cuda.synchronize()
+    ctx.synchronize()
     mpix = W * H / 1e6
     value = mpix / (ms / 1e3)
     in_bytes = W * H * B
@@ -172,13 +153,12 @@ def main():
         print("bench: secondary gauss_blur", file=sys.stderr, flush=True)
         gs = np.random.default_rng(97).integers(0, 256, size=(4096, 4096, 4), dtype=np.uint8)
         gp = v.Pipeline(ctx, 4096, 4096, 4).gauss_blur(2.0)
-        gd_src = torch.from_numpy(gs).cuda()
-        gd_dst = torch.empty_like(gd_src)
-        torch.cuda.synchronize()
+        gd_src = v.DeviceBuffer.from_numpy(ctx, gs)
+        gd_dst = gp.output_buffer()
         gp.run(gd_src, gd_dst)
-        torch.cuda.synchronize()
-        gexact = bool(np.array_equal(gd_dst.cpu().numpy(), O.Pipeline(4096, 4096, 4).gauss_blur(2.0).run(gs)))
-        gms, glat = time_pipeline(torch, gp, gd_src, gd_dst, args.steps, args.warmup)
+        ctx.synchronize()
+        gexact = bool(np.array_equal(gd_dst.numpy(), O.Pipeline(4096, 4096, 4).gauss_blur(2.0).run(gs)))
+        gms, glat = time_pipeline(gp, gd_src, gd_dst, args.steps, args.warmup)
         gbytes = 2 * gs.nbytes
         result["secondary"] = {
             "workload": "gauss_blur sigma=2.0 on 4096x4096x4 uchar", "bit_exact": gexact,

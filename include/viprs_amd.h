@@ -96,6 +96,16 @@ size_t vgpu_context_pool_bytes(const vgpu_context *ctx);
 /* Release every pooled buffer not in use. */
 int vgpu_context_trim(vgpu_context *ctx);
 
+/* Device buffers from the context's pool (the device buffer pool that
+ * replaces the scheduler's per-thread ThreadBufferPool,
+ * crates/viprs-runtime/src/pipeline/compiled.rs:475-).  Allocation only
+ * calls hipMalloc on a pool miss; free returns the buffer to the pool after
+ * the context stream has drained.  The copies are synchronous. */
+int vgpu_buffer_alloc(vgpu_context *ctx, size_t bytes, void **out);
+int vgpu_buffer_free(vgpu_context *ctx, void *ptr);
+int vgpu_memcpy_h2d(vgpu_context *ctx, void *dst_device, const void *src_host, size_t bytes);
+int vgpu_memcpy_d2h(vgpu_context *ctx, void *dst_host, const void *src_device, size_t bytes);
+
 /* ----------------------------------------------------- pipeline (builder) */
 
 /* PipelineBuilder::from_source(MemorySource) (crates/viprs-runtime/src/
@@ -151,6 +161,13 @@ int vgpu_pipeline_run(vgpu_pipeline *p, const void *src, size_t src_bytes, void 
 int vgpu_pipeline_run_profiled(vgpu_pipeline *p, const void *src, size_t src_bytes, void *dst,
                                size_t dst_bytes, int iterations, float *stage_ms,
                                int max_stages);
+/* Timing harness on the context stream: `warmup` untimed runs, then
+ * `iterations` back-to-back runs bracketed by HIP events (total_ms), then
+ * `iterations` more with an event after each (step_ms[i], may be NULL).
+ * No reference counterpart (the reference's criterion benches). */
+int vgpu_pipeline_benchmark(vgpu_pipeline *p, const void *src, size_t src_bytes, void *dst,
+                            size_t dst_bytes, int warmup, int iterations, float *total_ms,
+                            float *step_ms);
 /* Same with host pointers: H2D copy, run, D2H copy, synchronize. */
 int vgpu_pipeline_run_host(vgpu_pipeline *p, const void *src, size_t src_bytes, void *dst,
                            size_t dst_bytes);

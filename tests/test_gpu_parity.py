@@ -23,22 +23,17 @@ def build(lib_pipeline, steps):
 
 def run_both(ctx, src, steps, device=True):
     """Run the same builder chain on the GPU and in the oracle."""
-    import torch
-
     h, w, b = src.shape
     fmt = FMT[src.dtype]
     gp = build(v.Pipeline(ctx, w, h, b, fmt), steps)
     op = build(O.Pipeline(w, h, b, fmt), steps)
     exp = op.run(src)
     if device:
-        dsrc = torch.from_numpy(np.ascontiguousarray(src)).cuda()
-        oh, ow, ob = gp.output_shape
-        ddst = torch.empty((oh, ow, ob), dtype=dsrc.dtype, device="cuda") if fmt == gp.output_info()[3] \
-            else torch.empty((oh, ow, ob), dtype=torch.float32, device="cuda")
-        torch.cuda.synchronize()
+        dsrc = v.DeviceBuffer.from_numpy(ctx, src)
+        ddst = gp.output_buffer()
         gp.run(dsrc, ddst)
         ctx.synchronize()
-        got = ddst.cpu().numpy()
+        got = ddst.numpy()
     else:
         got = gp.run_host(src)
     return got, exp
@@ -191,8 +186,6 @@ def test_tiny_and_ragged(ctx, w, h, b):
 # ------------------------------------------------------------ op level
 
 def test_op_process_region_matches_oracle_reduce_h_offsets(ctx):
-    import torch
-
     src = noise_u8(9, 300, 3, 4)
     op = v.Op.reduce_h(ctx, 2.3, v.LANCZOS3, 3, 300)
     for x0 in (-5, 0, 17):
@@ -201,48 +194,47 @@ def test_op_process_region_matches_oracle_reduce_h_offsets(ctx):
         # Materialise the input tile with clamp-to-edge like MemorySource.
         xs = np.clip(np.arange(in_r.x, in_r.x + in_r.width), 0, 299)
         tile = np.ascontiguousarray(src[:, xs, :])
-        dst = torch.zeros((9, 100, 3), dtype=torch.uint8, device="cuda")
-        d_tile = torch.from_numpy(tile).cuda()
-        torch.cuda.synchronize()
-        op.process_region(d_tile, dst, in_r, out_r)
+        dst = v.DeviceBuffer(ctx, (9, 100, 3), np.uint8)
+        op.process_region(v.DeviceBuffer.from_numpy(ctx, tile), dst, in_r, out_r)
         ctx.synchronize()
+        got = dst.numpy()
         if x0 == 0:
             exp = O.Pipeline(300, 9, 3).reduce_h(2.3).run(src)[:, :100]
-            assert np.array_equal(dst.cpu().numpy(), exp)
+            assert np.array_equal(got, exp)
+        # Closed form of one ReduceH tile: accumulated u8 plan from the tile
+        # origin, taps clamped into the tile (reduceh.rs:101-163).
+        starts, phases = O.reduce_plan(O.LANCZOS3, 2.3, 300, float(x0), True, 100)
+        _, ci = O.reduce_tables(O.LANCZOS3, 2.3)
+        cols = np.clip(starts[:, None] + np.arange(ci.shape[1])[None, :] - in_r.x, 0, in_r.width - 1)
+        acc = np.einsum("xt,yxtb->yxb", ci[phases].astype(np.int64), tile[:, cols, :].astype(np.int64))
+        assert np.array_equal(got, np.clip((acc + 2048) >> 12, 0, 255).astype(np.uint8))
 
 
 def test_op_premultiply_exhaustive(ctx):
-    import torch
-
     a = np.repeat(np.arange(256, dtype=np.uint8), 256)
     c = np.tile(np.arange(256, dtype=np.uint8), 256)
     img = np.stack([c, 255 - c, c // 2, a], axis=-1).reshape(256, 256, 4)
     for kind in ("premultiply", "unpremultiply"):
         op = getattr(v.Op, kind)(ctx, 4)
         r = v.Region(0, 0, 256, 256)
-        dst = torch.zeros((256, 256, 4), dtype=torch.uint8, device="cuda")
-        d_img = torch.from_numpy(img).cuda()
-        torch.cuda.synchronize()
-        op.process_region(d_img, dst, r, r)
+        dst = v.DeviceBuffer(ctx, (256, 256, 4), np.uint8)
+        op.process_region(v.DeviceBuffer.from_numpy(ctx, img), dst, r, r)
         ctx.synchronize()
         exp = getattr(O.Pipeline(256, 256, 4), kind)().run(img)
-        assert np.array_equal(dst.cpu().numpy(), exp), kind
+        assert np.array_equal(dst.numpy(), exp), kind
 
 
 def test_op_rejects_bad_lengths_and_regions(ctx):
-    import torch
-
     op = v.Op.gauss_blur_h(ctx, 2.0, 3)
     out_r = v.Region(0, 0, 10, 2)
     in_r = op.required_input_region(out_r)
     assert (in_r.x, in_r.width) == (-3, 16)
-    src = torch.zeros((2, 16, 3), dtype=torch.uint8, device="cuda")
+    src = v.DeviceBuffer(ctx, (2, 16, 3), np.uint8)
     with pytest.raises(v.VgpuError) as e:
-        op.process_region(src, torch.zeros((2, 9, 3), dtype=torch.uint8, device="cuda"), in_r, out_r)
+        op.process_region(src, v.DeviceBuffer(ctx, (2, 9, 3), np.uint8), in_r, out_r)
     assert e.value.status == v.api.E_BUFFER_SIZE
     with pytest.raises(v.VgpuError):
-        op.process_region(src, torch.zeros((2, 10, 3), dtype=torch.uint8, device="cuda"),
-                          v.Region(0, 0, 16, 2), out_r)
+        op.process_region(src, v.DeviceBuffer(ctx, (2, 10, 3), np.uint8), v.Region(0, 0, 16, 2), out_r)
 
 
 def test_builder_validation_errors(ctx):

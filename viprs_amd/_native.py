@@ -55,6 +55,10 @@ def load() -> C.CDLL:
         "vgpu_synchronize": (i32, [vp]),
         "vgpu_context_pool_bytes": (sz, [vp]),
         "vgpu_context_trim": (i32, [vp]),
+        "vgpu_buffer_alloc": (i32, [vp, sz, P(vp)]),
+        "vgpu_buffer_free": (i32, [vp, vp]),
+        "vgpu_memcpy_h2d": (i32, [vp, vp, vp, sz]),
+        "vgpu_memcpy_d2h": (i32, [vp, vp, vp, sz]),
         "vgpu_pipeline_new": (i32, [vp, u32, u32, u32, i32, P(vp)]),
         "vgpu_pipeline_destroy": (None, [vp]),
         "vgpu_pipeline_reduce_h": (i32, [vp, dbl, i32]),
@@ -72,6 +76,7 @@ def load() -> C.CDLL:
         "vgpu_pipeline_run": (i32, [vp, vp, sz, vp, sz]),
         "vgpu_pipeline_run_host": (i32, [vp, vp, sz, vp, sz]),
         "vgpu_pipeline_run_profiled": (i32, [vp, vp, sz, vp, sz, i32, P(f32), i32]),
+        "vgpu_pipeline_benchmark": (i32, [vp, vp, sz, vp, sz, i32, i32, P(f32), P(f32)]),
         "vgpu_op_reduce_h_new": (i32, [vp, dbl, i32, u32, u32, i32, P(vp)]),
         "vgpu_op_reduce_v_new": (i32, [vp, dbl, i32, u32, u32, i32, P(vp)]),
         "vgpu_op_shrink_h_new": (i32, [vp, u32, i32, u32, u32, i32, P(vp)]),

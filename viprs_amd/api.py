@@ -6,8 +6,9 @@ Names follow the reference: ``Pipeline`` mirrors ``PipelineBuilder``
 dynamic_op.rs).  Build-time validation errors surface as ``VgpuError`` with
 the status code, the way the reference returns ``BuildError``.
 
-Device buffers are torch tensors on a HIP device (``tensor.data_ptr()``);
-``run_host`` takes and returns numpy arrays.
+Device buffers are ``DeviceBuffer`` objects from the context's pool (or any
+object exposing ``data_ptr()`` and ``nbytes``); ``run_host`` takes and returns
+numpy arrays.
 """
 from __future__ import annotations
 
@@ -87,8 +88,62 @@ class Context:
         self.close()
 
 
+def _ptr(t) -> int:
+    return t.data_ptr()
+
+
 def _nbytes(t) -> int:
+    n = getattr(t, "nbytes", None)
+    if isinstance(n, int):
+        return n
     return t.numel() * t.element_size()
+
+
+class DeviceBuffer:
+    """Device memory from the context's buffer pool, shaped like an image."""
+
+    def __init__(self, ctx: Context, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(ctx._lib.vgpu_buffer_alloc(ctx.handle, self.nbytes, C.byref(p)), "vgpu_buffer_alloc")
+        self._p = p
+
+    @classmethod
+    def from_numpy(cls, ctx: Context, arr: np.ndarray) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        buf = cls(ctx, arr.shape, arr.dtype)
+        buf.upload(arr)
+        return buf
+
+    def data_ptr(self) -> int:
+        return self._p.value or 0
+
+    def upload(self, arr: np.ndarray) -> None:
+        arr = np.ascontiguousarray(arr, self.dtype)
+        if arr.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        _check(self.ctx._lib.vgpu_memcpy_h2d(self.ctx.handle, self._p, C.c_void_p(arr.ctypes.data),
+                                             self.nbytes), "vgpu_memcpy_h2d")
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        _check(self.ctx._lib.vgpu_memcpy_d2h(self.ctx.handle, C.c_void_p(out.ctypes.data), self._p,
+                                             self.nbytes), "vgpu_memcpy_d2h")
+        return out
+
+    def free(self) -> None:
+        if getattr(self, "_p", None) and self.ctx._h:
+            self.ctx._lib.vgpu_buffer_free(self.ctx.handle, self._p)
+        self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Pipeline:
@@ -171,14 +226,27 @@ class Pipeline:
 
     def run(self, src, dst) -> None:
         """Enqueue on the context stream; src / dst are device tensors."""
-        _check(self._lib.vgpu_pipeline_run(self._h, C.c_void_p(src.data_ptr()), _nbytes(src),
-                                           C.c_void_p(dst.data_ptr()), _nbytes(dst)), "run")
+        _check(self._lib.vgpu_pipeline_run(self._h, C.c_void_p(_ptr(src)), _nbytes(src),
+                                           C.c_void_p(_ptr(dst)), _nbytes(dst)), "run")
+
+    def output_buffer(self) -> "DeviceBuffer":
+        w, h, b, f = self.output_info()
+        return DeviceBuffer(self.ctx, (h, w, b), NP_DTYPE[f])
+
+    def benchmark(self, src, dst, warmup: int, iterations: int):
+        """(mean ms per step over back-to-back steps, per-step ms array)."""
+        total = C.c_float()
+        steps = (C.c_float * iterations)()
+        _check(self._lib.vgpu_pipeline_benchmark(self._h, C.c_void_p(_ptr(src)), _nbytes(src),
+                                                 C.c_void_p(_ptr(dst)), _nbytes(dst), warmup,
+                                                 iterations, C.byref(total), steps), "benchmark")
+        return total.value / iterations, np.array(steps[:], np.float64)
 
     def run_profiled(self, src, dst, iterations: int = 1):
         """Mean device milliseconds per fused stage (HIP events)."""
         buf = (C.c_float * 64)()
-        n = self._lib.vgpu_pipeline_run_profiled(self._h, C.c_void_p(src.data_ptr()), _nbytes(src),
-                                                 C.c_void_p(dst.data_ptr()), _nbytes(dst),
+        n = self._lib.vgpu_pipeline_run_profiled(self._h, C.c_void_p(_ptr(src)), _nbytes(src),
+                                                 C.c_void_p(_ptr(dst)), _nbytes(dst),
                                                  iterations, buf, 64)
         if n < 0:
             raise VgpuError(n, "run_profiled")
@@ -265,6 +333,6 @@ class Op:
         return w.value, h.value
 
     def process_region(self, src, dst, input_region: Region, output_region: Region) -> None:
-        _check(self._lib.vgpu_op_process_region(self._h, C.c_void_p(src.data_ptr()), _nbytes(src),
-                                                C.c_void_p(dst.data_ptr()), _nbytes(dst),
+        _check(self._lib.vgpu_op_process_region(self._h, C.c_void_p(_ptr(src)), _nbytes(src),
+                                                C.c_void_p(_ptr(dst)), _nbytes(dst),
                                                 input_region, output_region), "process_region")

@@ -435,6 +435,41 @@ int vgpu_context_trim(vgpu_context* c) {
     return VGPU_OK;
 }
 
+int vgpu_buffer_alloc(vgpu_context* c, size_t bytes, void** out) {
+    if (!c || !out) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    *out = c->pool.acquire(bytes);
+    return *out ? VGPU_OK : VGPU_E_NO_MEMORY;
+}
+
+int vgpu_buffer_free(vgpu_context* c, void* ptr) {
+    if (!c) return VGPU_E_INVALID_ARGUMENT;
+    if (!ptr) return VGPU_OK;
+    // Queued work may still read the buffer; the pool hands it out again only
+    // after the stream has drained.
+    if (hipStreamSynchronize(c->stream) != hipSuccess) c->sticky = VGPU_E_DEVICE;
+    c->pool.release(ptr);
+    return VGPU_OK;
+}
+
+int vgpu_memcpy_h2d(vgpu_context* c, void* dst, const void* src, size_t bytes) {
+    if (!c || (!dst && bytes) || (!src && bytes)) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    if (bytes == 0) return VGPU_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return from_hip(e);
+}
+
+int vgpu_memcpy_d2h(vgpu_context* c, void* dst, const void* src, size_t bytes) {
+    if (!c || (!dst && bytes) || (!src && bytes)) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    if (bytes == 0) return VGPU_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return from_hip(e);
+}
+
 int vgpu_pipeline_new(vgpu_context* ctx, uint32_t w, uint32_t h, uint32_t bands, int fmt,
                       vgpu_pipeline** out) {
     if (!ctx || !out) return VGPU_E_INVALID_ARGUMENT;
@@ -685,6 +720,42 @@ int vgpu_pipeline_run_profiled(vgpu_pipeline* p, const void* src, size_t src_byt
     for (auto& e : ev) (void)hipEventDestroy(e);
     for (int k = 0; k < ns; ++k) stage_ms[k] = (float)(acc[(size_t)k] / iterations);
     return rc == VGPU_OK ? ns : rc;
+}
+
+int vgpu_pipeline_benchmark(vgpu_pipeline* p, const void* src, size_t src_bytes, void* dst,
+                            size_t dst_bytes, int warmup, int iterations, float* total_ms,
+                            float* step_ms) {
+    if (!p || iterations < 1 || !total_ms) return VGPU_E_INVALID_ARGUMENT;
+    vgpu_context* ctx = p->ctx;
+    int rc = VGPU_OK;
+    for (int i = 0; i < warmup && rc == VGPU_OK; ++i) rc = vgpu_pipeline_run(p, src, src_bytes, dst, dst_bytes);
+    if (rc != VGPU_OK) return rc;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return VGPU_E_DEVICE;
+    // Back-to-back steps: whole-job throughput.
+    (void)hipEventRecord(a, ctx->stream);
+    for (int i = 0; i < iterations && rc == VGPU_OK; ++i) rc = vgpu_pipeline_run(p, src, src_bytes, dst, dst_bytes);
+    (void)hipEventRecord(b, ctx->stream);
+    if (rc == VGPU_OK && hipEventSynchronize(b) != hipSuccess) rc = VGPU_E_DEVICE;
+    if (rc == VGPU_OK) (void)hipEventElapsedTime(total_ms, a, b);
+    // Per-step latency: one event pair per step, steps still back to back.
+    if (rc == VGPU_OK && step_ms) {
+        std::vector<hipEvent_t> ev((size_t)iterations + 1);
+        for (auto& e : ev) (void)hipEventCreate(&e);
+        (void)hipEventRecord(ev[0], ctx->stream);
+        for (int i = 0; i < iterations && rc == VGPU_OK; ++i) {
+            rc = vgpu_pipeline_run(p, src, src_bytes, dst, dst_bytes);
+            (void)hipEventRecord(ev[(size_t)i + 1], ctx->stream);
+        }
+        if (rc == VGPU_OK && hipEventSynchronize(ev[(size_t)iterations]) != hipSuccess) rc = VGPU_E_DEVICE;
+        for (int i = 0; i < iterations && rc == VGPU_OK; ++i)
+            (void)hipEventElapsedTime(&step_ms[i], ev[(size_t)i], ev[(size_t)i + 1]);
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
 }
 
 int vgpu_pipeline_run_host(vgpu_pipeline* p, const void* src, size_t src_bytes, void* dst,
