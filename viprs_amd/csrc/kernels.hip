@@ -180,6 +180,144 @@ __global__ __launch_bounds__(kThreads) void k_shrink_hv_u8(InView in, OutView ou
 }
 
 // ---------------------------------------------------------------------------
+// Fused ShrinkH -> ShrinkV for u8, wave-per-output-row form.
+//
+// Each wavefront owns one output row and TX (<= 64) output columns; a block is
+// four wavefronts on four consecutive output rows, so every (row, column-span)
+// task of a thumbnail is co-resident on the chip and there is no tail round.
+// A wavefront walks its fv source rows: the clamped in-bounds byte span of the
+// row is fetched with 16-byte loads one row ahead (registers), parked in the
+// wavefront's LDS row buffer, and each lane sums its fh-pixel box window with
+// dword LDS reads realigned by alignbyte and per-band byte masks folded in by
+// udot4.  Windows that reach past the clamped span (the Z^2 halo columns of a
+// thumbnail) take a per-pixel clamped path.  Rounding is the reference's:
+// ShrinkH rounding per source row, then ShrinkV rounding of the sum.
+// ---------------------------------------------------------------------------
+constexpr int kSpanBytes = 4096;  // LDS bytes per wavefront row buffer
+constexpr int kPrefetch = kSpanBytes / (64 * 16);  // 16-byte chunks per lane
+
+template <int B>
+__device__ __forceinline__ uint32_t band_mask(int b, int kk) {
+    if constexpr (B == 1) return 0x01010101u;
+    if constexpr (B == 2) return 0x00010001u << (8 * b);
+    if constexpr (B == 4) return 1u << (8 * b);
+    // B == 3: bytes of aligned dword k carry bands (4k + j) % 3; kk = k % 3.
+    const int m = (b - kk + 3) % 3;
+    return m == 0 ? 0x01000001u : (m == 1 ? 0x00000100u : 0x00010000u);
+}
+
+template <int B>
+__global__ __launch_bounds__(kThreads) void k_shrink_hv_u8_rows(InView in, OutView out, int fh,
+                                                                int fv, int tx, int premult,
+                                                                uint32_t mh, uint32_t mv,
+                                                                int aligned) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int oyi = blockIdx.y * (kThreads / 64) + wave;
+    const bool active = oyi < out.h;
+    const int col0 = blockIdx.x * tx;
+    const int ncols = min(tx, out.w - col0);
+    const int oy = out.y + oyi;
+    const int cx0 = (out.x + col0) * fh;
+    const int cx1 = cx0 + ncols * fh;
+    const int lo = clampi(cx0, in.bx, in.bx + in.bw - 1);
+    const int hi = clampi(cx1 - 1, in.bx, in.bx + in.bw - 1) + 1;
+    const int nbytes = (hi - lo) * B;
+    const unsigned char* base = static_cast<const unsigned char*>(in.ptr);
+    const size_t row_bytes = (size_t)in.bw * B;
+    const size_t total = row_bytes * (size_t)in.bh;
+    unsigned char* rbuf = lds + wave * (kSpanBytes + 32);
+    const int wbytes = fh * B;
+
+    uint4 pre[kPrefetch];
+    int pre_shift = 0, pre_nch = 0;
+    auto fetch = [&](int k) {
+        const size_t a = (size_t)row_of(in, oy * fv + k) * row_bytes + (size_t)(lo - in.bx) * B;
+        const size_t a16 = aligned ? (a & ~(size_t)15) : a;
+        pre_shift = (int)(a - a16);
+        pre_nch = (pre_shift + nbytes + 15) >> 4;
+#pragma unroll
+        for (int j = 0; j < kPrefetch; ++j) {
+            const int i = lane + 64 * j;
+            if (i < pre_nch) {
+                const size_t off = a16 + (size_t)i * 16;
+                if (aligned && off + 16 <= total) {
+                    pre[j] = *reinterpret_cast<const uint4*>(base + off);
+                } else {
+                    unsigned char tmp[16];
+                    for (int q = 0; q < 16; ++q) tmp[q] = (off + q < total) ? base[off + q] : 0;
+                    pre[j] = *reinterpret_cast<const uint4*>(tmp);
+                }
+            }
+        }
+    };
+
+    uint32_t acc[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) acc[b] = 0;
+    if (active) fetch(0);
+    for (int k = 0; k < fv; ++k) {
+        const int shift = pre_shift;
+        if (active) {
+#pragma unroll
+            for (int j = 0; j < kPrefetch; ++j) {
+                const int i = lane + 64 * j;
+                if (i < pre_nch) *reinterpret_cast<uint4*>(rbuf + i * 16) = pre[j];
+            }
+        }
+        __syncthreads();
+        if (active && k + 1 < fv) fetch(k + 1);
+        if (active && lane < ncols) {
+            const int X0 = cx0 + lane * fh;
+            uint32_t sum[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) sum[b] = 0;
+            const bool inside = X0 >= lo && X0 + fh <= hi;
+            if (inside && !(premult && (B == 2 || B == 4))) {
+                const int sbyte = shift + (X0 - lo) * B;
+                const uint32_t* l32 = reinterpret_cast<const uint32_t*>(rbuf);
+                const int d0 = sbyte >> 2, al = sbyte & 3;
+                const int nd = (wbytes + 3) >> 2;
+                uint32_t cur = l32[d0];
+                int kk = 0;
+                for (int d = 0; d < nd; ++d) {
+                    const uint32_t nxt = l32[d0 + d + 1];
+                    uint32_t v = __builtin_amdgcn_alignbyte(nxt, cur, al);
+                    cur = nxt;
+                    const int rem = wbytes - 4 * d;
+                    if (rem < 4) v &= (1u << (8 * rem)) - 1u;
+#pragma unroll
+                    for (int b = 0; b < B; ++b) sum[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, kk), sum[b], false);
+                    kk = kk == 2 ? 0 : kk + 1;
+                }
+            } else {
+                for (int j = 0; j < fh; ++j) {
+                    const int X = clampi(X0 + j, lo, hi - 1);
+                    const unsigned char* px = rbuf + shift + (X - lo) * B;
+                    if (premult && (B == 2 || B == 4)) {
+                        const uint32_t a = px[B - 1];
+#pragma unroll
+                        for (int b = 0; b < B - 1; ++b) sum[b] += premult_u8(px[b], a);
+                        sum[B - 1] += a;
+                    } else {
+#pragma unroll
+                        for (int b = 0; b < B; ++b) sum[b] += px[b];
+                    }
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) acc[b] += shrink_round_h(sum[b], (uint32_t)fh, B, mh);
+        }
+        __syncthreads();
+    }
+    if (active && lane < ncols) {
+        unsigned char* o = static_cast<unsigned char*>(out.ptr) + ((size_t)oyi * out.w + col0 + lane) * B;
+#pragma unroll
+        for (int b = 0; b < B; ++b) o[b] = (unsigned char)shrink_round_v(acc[b], (uint32_t)fv, mv);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Generic single-axis kernels: one thread per output sample.
 // ---------------------------------------------------------------------------
 
@@ -220,6 +358,64 @@ __global__ __launch_bounds__(kThreads) void k_shrink_generic(InView in, OutView 
     }
 }
 
+// One ReduceH output sample at (row, output column xo, band b).
+template <typename T>
+__device__ __forceinline__ T reduce_h_sample(const InView& in, int row, int B, int b, int start,
+                                             int ph, const DevPlan& p) {
+    const int T_ = p.taps;
+    if constexpr (sizeof(T) == 4) {
+        const double* w = p.wf64 + (size_t)ph * T_;
+        double acc = 0.0;
+        int t = 0;
+        if (B == 1) {
+            for (; t + 4 <= T_; t += 4) {
+                double s = 0.0;
+                for (int l = 0; l < 4; ++l) {
+                    const double prod = (double)load_at<float>(in, row, col_of(in, start + t + l), 1, 0) * w[t + l];
+                    s = s + prod;
+                }
+                acc = acc + s;
+            }
+        }
+        for (; t < T_; ++t) acc = fma((double)load_at<float>(in, row, col_of(in, start + t), B, b), w[t], acc);
+        return (T)(float)acc;
+    } else {
+        const int16_t* w = p.wi16 + (size_t)ph * T_;
+        int64_t acc = 0;
+        for (int t = 0; t < T_; ++t)
+            acc += (int64_t)w[t] * (int64_t)load_at<T>(in, row, col_of(in, start + t), B, b);
+        const int64_t v = (acc + 2048) >> 12;
+        const int64_t mx = sizeof(T) == 1 ? 255 : 65535;
+        return (T)(v < 0 ? 0 : (v > mx ? mx : v));
+    }
+}
+
+// One ReduceV output sample from a column accessor get(row) -> T.
+template <typename T, typename Get>
+__device__ __forceinline__ T reduce_v_sample(Get get, int start, int ph, const DevPlan& p, bool vec) {
+    const int T_ = p.taps;
+    if constexpr (sizeof(T) == 4) {
+        const double* w = p.wf64 + (size_t)ph * T_;
+        double acc = 0.0;
+        if (vec) {
+            for (int t = 0; t < T_; ++t) {
+                const double prod = (double)get(start + t) * w[t];
+                acc = acc + prod;
+            }
+        } else {
+            for (int t = 0; t < T_; ++t) acc = fma((double)get(start + t), w[t], acc);
+        }
+        return (T)(float)acc;
+    } else {
+        const int16_t* w = p.wi16 + (size_t)ph * T_;
+        int64_t acc = 0;
+        for (int t = 0; t < T_; ++t) acc += (int64_t)w[t] * (int64_t)get(start + t);
+        const int64_t v = (acc + 2048) >> 12;
+        const int64_t mx = sizeof(T) == 1 ? 255 : 65535;
+        return (T)(v < 0 ? 0 : (v > mx ? mx : v));
+    }
+}
+
 // ReduceH (reduceh.rs:101-163; reduce_simd.rs:47-90,2463-2693): fixed-point
 // i16 taps for integer formats -> (acc + 2048) >> 12 clamped; f32 carries FP64:
 // bands == 1 follows the AVX2 4-lane grouping (products summed per group, then
@@ -227,40 +423,12 @@ __global__ __launch_bounds__(kThreads) void k_shrink_generic(InView in, OutView 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_reduce_h(InView in, OutView out, int B, DevPlan p) {
     const size_t n = (size_t)out.w * out.h * B;
-    const int T_ = p.taps;
     for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
         const int b = (int)(i % B);
         const size_t px = i / B;
         const int xo = (int)(px % out.w);
         const int row = row_of(in, out.y + (int)(px / out.w));
-        const int start = p.start[xo];
-        const int ph = p.phase[xo];
-        T* o = static_cast<T*>(out.ptr);
-        if constexpr (sizeof(T) == 4) {
-            const double* w = p.wf64 + (size_t)ph * T_;
-            double acc = 0.0;
-            int t = 0;
-            if (B == 1) {
-                for (; t + 4 <= T_; t += 4) {
-                    double s = 0.0;
-                    for (int l = 0; l < 4; ++l) {
-                        const double prod = (double)load_at<float>(in, row, col_of(in, start + t + l), 1, 0) * w[t + l];
-                        s = s + prod;
-                    }
-                    acc = acc + s;
-                }
-            }
-            for (; t < T_; ++t) acc = fma((double)load_at<float>(in, row, col_of(in, start + t), B, b), w[t], acc);
-            o[i] = (float)acc;
-        } else {
-            const int16_t* w = p.wi16 + (size_t)ph * T_;
-            int64_t acc = 0;
-            for (int t = 0; t < T_; ++t)
-                acc += (int64_t)w[t] * (int64_t)load_at<T>(in, row, col_of(in, start + t), B, b);
-            const int64_t v = (acc + 2048) >> 12;
-            const int64_t mx = sizeof(T) == 1 ? 255 : 65535;
-            o[i] = (T)(v < 0 ? 0 : (v > mx ? mx : v));
-        }
+        static_cast<T*>(out.ptr)[i] = reduce_h_sample<T>(in, row, B, b, p.start[xo], p.phase[xo], p);
     }
 }
 
@@ -273,37 +441,54 @@ __global__ __launch_bounds__(kThreads) void k_reduce_v(InView in, OutView out, i
     const int row_len = out.w * B;
     const int vec_len = row_len / 4 * 4;
     const size_t n = (size_t)row_len * out.h;
-    const int T_ = p.taps;
     for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
         const int s = (int)(i % row_len);
         const int yo = (int)(i / row_len);
         const int b = s % B;
         const int col = col_of(in, out.x + s / B);
-        const int start = p.start[yo];
-        const int ph = p.phase[yo];
-        T* o = static_cast<T*>(out.ptr);
-        if constexpr (sizeof(T) == 4) {
-            const double* w = p.wf64 + (size_t)ph * T_;
-            double acc = 0.0;
-            if (s < vec_len) {
-                for (int t = 0; t < T_; ++t) {
-                    const double prod = (double)load_at<float>(in, row_of(in, start + t), col, B, b) * w[t];
-                    acc = acc + prod;
-                }
-            } else {
-                for (int t = 0; t < T_; ++t)
-                    acc = fma((double)load_at<float>(in, row_of(in, start + t), col, B, b), w[t], acc);
-            }
-            o[i] = (float)acc;
-        } else {
-            const int16_t* w = p.wi16 + (size_t)ph * T_;
-            int64_t acc = 0;
-            for (int t = 0; t < T_; ++t)
-                acc += (int64_t)w[t] * (int64_t)load_at<T>(in, row_of(in, start + t), col, B, b);
-            const int64_t v = (acc + 2048) >> 12;
-            const int64_t mx = sizeof(T) == 1 ? 255 : 65535;
-            o[i] = (T)(v < 0 ? 0 : (v > mx ? mx : v));
-        }
+        auto get = [&](int y) -> T { return load_at<T>(in, row_of(in, y), col, B, b); };
+        static_cast<T*>(out.ptr)[i] = reduce_v_sample<T>(get, p.start[yo], p.phase[yo], p, s < vec_len);
+    }
+}
+
+// Fused ReduceH -> ReduceV.  Block = TX output columns x TY output rows.  The
+// ReduceH outputs the tile's ReduceV taps need (rows start_v[first] ..
+// start_v[last] + taps_v - 1, each evaluated on Z^2 exactly as the unfused
+// ReduceH would) are computed into LDS in the intermediate's sample type
+// (u8 / u16 / f32, i.e. rounded as the reference rounds between the passes),
+// then ReduceV reads them from LDS.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_reduce_hv(InView in, OutView out, int B, DevPlan ph,
+                                                        DevPlan pv, int TX, int TY, int nr_max,
+                                                        int vclip0, int vclip1) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* hbuf = reinterpret_cast<T*>(smem);
+    const int tx0 = blockIdx.x * TX, ty0 = blockIdx.y * TY;
+    const int ncols = min(TX, out.w - tx0), nrows = min(TY, out.h - ty0);
+    const int ys0 = pv.start[ty0];
+    const int nr = min(nr_max, pv.start[ty0 + nrows - 1] + pv.taps - ys0);
+    const int pitch = TX * B;
+    for (int it = threadIdx.x; it < nr * ncols * B; it += kThreads) {
+        const int r = it / (ncols * B), rem = it - r * (ncols * B);
+        const int x = rem / B, b = rem - x * B;
+        const int xo = tx0 + x;
+        hbuf[r * pitch + x * B + b] =
+            reduce_h_sample<T>(in, row_of(in, ys0 + r), B, b, ph.start[xo], ph.phase[xo], ph);
+    }
+    __syncthreads();
+    const int row_len = out.w * B;
+    const int vec_len = row_len / 4 * 4;
+    for (int it = threadIdx.x; it < nrows * ncols * B; it += kThreads) {
+        const int ry = it / (ncols * B), rem = it - ry * (ncols * B);
+        const int x = rem / B, b = rem - x * B;
+        const int yo = ty0 + ry;
+        const int sidx = (tx0 + x) * B + b;
+        auto get = [&](int y) -> T {
+            const int yy = clampi(clampi(y, vclip0, vclip1) - ys0, 0, nr - 1);
+            return hbuf[yy * pitch + x * B + b];
+        };
+        static_cast<T*>(out.ptr)[(size_t)yo * row_len + sidx] =
+            reduce_v_sample<T>(get, pv.start[yo], pv.phase[yo], pv, sidx < vec_len);
     }
 }
 
@@ -390,21 +575,39 @@ unsigned grid_for(size_t n) {
 hipError_t launch_shrink_hv_u8(const InView& in, const OutView& out, int bands, int fh, int fv,
                                int premult, hipStream_t s) {
     if (out.w <= 0 || out.h <= 0) return hipSuccess;
-    // LDS budget 64 KiB: RL staged source rows of one TX-column span each.
-    const size_t budget = 64 * 1024;
+    const uint32_t mh = (uint32_t)((1ull << 32) / (256ull * (uint64_t)fh));
+    const uint32_t mv = (uint32_t)((1ull << 32) / (256ull * (uint64_t)fv));
+    const int aligned = ((uintptr_t)in.ptr % 16) == 0;
+    // Wave-per-row kernel: a wavefront's span (TX windows + 16-byte skew)
+    // must fit its LDS row buffer and its prefetch registers.
+    const int wb = fh * bands;
     int tx = 64;
+    while (tx > 1 && tx * wb + 15 > kSpanBytes) tx /= 2;
+    if (tx * wb + 15 <= kSpanBytes) {
+        const size_t shmem = (size_t)(kThreads / 64) * (kSpanBytes + 32);
+        dim3 grid((unsigned)((out.w + tx - 1) / tx), (unsigned)((out.h + 3) / 4));
+        switch (bands) {
+        case 1: hipLaunchKernelGGL(k_shrink_hv_u8_rows<1>, grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, premult, mh, mv, aligned); break;
+        case 2: hipLaunchKernelGGL(k_shrink_hv_u8_rows<2>, grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, premult, mh, mv, aligned); break;
+        case 3: hipLaunchKernelGGL(k_shrink_hv_u8_rows<3>, grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, premult, mh, mv, aligned); break;
+        case 4: hipLaunchKernelGGL(k_shrink_hv_u8_rows<4>, grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, premult, mh, mv, aligned); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+    // Very wide box windows (fh * bands > 4080 bytes): block-per-row kernel.
+    const size_t budget = 64 * 1024;
+    tx = 64;
     auto pitch_of = [&](int cols) { return ((size_t)cols * fh * bands + 32 + 15) & ~(size_t)15; };
     while (tx > 1 && pitch_of(tx) > budget) tx /= 2;
     const size_t pitch = pitch_of(tx);
+    if (pitch > budget) return hipErrorInvalidValue;
     int rl = kThreads / tx;
     if ((size_t)rl * pitch > budget) rl = (int)(budget / pitch);
     if (rl < 1) rl = 1;
     size_t shmem = (size_t)rl * pitch;
     const size_t part = (size_t)kThreads * bands * 4;
     if (shmem < part) shmem = part;
-    const uint32_t mh = (uint32_t)((1ull << 32) / (256ull * (uint64_t)fh));
-    const uint32_t mv = (uint32_t)((1ull << 32) / (256ull * (uint64_t)fv));
-    const int aligned = ((uintptr_t)in.ptr % 16) == 0;
     dim3 grid((unsigned)((out.w + tx - 1) / tx), (unsigned)out.h);
     switch (bands) {
     case 1: hipLaunchKernelGGL(k_shrink_hv_u8<1>, grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, rl, premult, mh, mv, aligned); break;
@@ -447,6 +650,18 @@ hipError_t launch_reduce_v(const InView& in, const OutView& out, int bands, int 
     if (fmt == 0) hipLaunchKernelGGL(k_reduce_v<uint8_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
     else if (fmt == 1) hipLaunchKernelGGL(k_reduce_v<uint16_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
     else hipLaunchKernelGGL(k_reduce_v<float>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_hv(const InView& in, const OutView& out, int bands, int fmt,
+                            const DevPlan& ph, const DevPlan& pv, int tx, int ty, int nr_max,
+                            int vclip0, int vclip1, hipStream_t s) {
+    if (out.w <= 0 || out.h <= 0) return hipSuccess;
+    const size_t shmem = (size_t)nr_max * tx * bands * (fmt == 0 ? 1 : (fmt == 1 ? 2 : 4));
+    dim3 grid((unsigned)((out.w + tx - 1) / tx), (unsigned)((out.h + ty - 1) / ty));
+    if (fmt == 0) hipLaunchKernelGGL(k_reduce_hv<uint8_t>, grid, dim3(kThreads), shmem, s, in, out, bands, ph, pv, tx, ty, nr_max, vclip0, vclip1);
+    else if (fmt == 1) hipLaunchKernelGGL(k_reduce_hv<uint16_t>, grid, dim3(kThreads), shmem, s, in, out, bands, ph, pv, tx, ty, nr_max, vclip0, vclip1);
+    else hipLaunchKernelGGL(k_reduce_hv<float>, grid, dim3(kThreads), shmem, s, in, out, bands, ph, pv, tx, ty, nr_max, vclip0, vclip1);
     return hipGetLastError();
 }
 

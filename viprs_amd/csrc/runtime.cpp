@@ -117,7 +117,16 @@ struct Staging {
     }
 };
 
-enum StageKind { ST_SHRINK_HV_U8, ST_SHRINK, ST_REDUCE_H, ST_REDUCE_V, ST_CONV, ST_ALPHA, ST_COPY };
+enum StageKind {
+    ST_SHRINK_HV_U8,
+    ST_SHRINK,
+    ST_REDUCE_H,
+    ST_REDUCE_V,
+    ST_REDUCE_HV,
+    ST_CONV,
+    ST_ALPHA,
+    ST_COPY,
+};
 
 struct Stage {
     StageKind kind = ST_COPY;
@@ -129,6 +138,10 @@ struct Stage {
     size_t plan_start = 0, plan_phase = 0, w16 = 0, w64 = 0;  // blob offsets
     size_t itaps = 0, wts = 0;
     int taps = 1;
+    // fused ReduceH -> ReduceV: the V plan and tiling
+    size_t v_start = 0, v_phase = 0, v_w16 = 0, v_w64 = 0;
+    int v_taps = 1, tx = 64, ty = 16, nr_max = 1;
+    int vclip0 = 0, vclip1 = 0;
 };
 
 Rect whole(uint32_t w, uint32_t h) { return Rect{0, 0, (int64_t)w, (int64_t)h}; }
@@ -191,6 +204,46 @@ void stage_reduce_params(const Node& nd, const Rect& out, Staging* st, Stage* s)
     s->taps = nd.axis.taps;
 }
 
+// Fused ReduceH -> ReduceV stage: H plan over the intermediate's columns, V
+// plan over the output rows, and a tile shape whose staged ReduceH rows fit
+// 64 KiB of LDS.  Returns false when no tile fits (very large factors).
+bool stage_reduce_hv_params(const Node& h, const Node& v, const Rect& mid, const Rect& out,
+                            const Rect& vclip, Staging* st, Stage* s) {
+    std::vector<int32_t> hs, vs;
+    std::vector<uint8_t> hp, vp;
+    build_plan(h, mid.x, mid.w, h.in_fmt == U8, &hs, &hp);
+    build_plan(v, out.y, out.h, false, &vs, &vp);
+    const size_t esz = format_size(h.out_fmt);
+    int tx = 64, ty = 16, nr = 0;
+    for (;;) {
+        nr = 1;
+        for (int64_t y0 = 0; y0 < out.h; y0 += ty) {
+            const int64_t y1 = std::min<int64_t>(y0 + ty, out.h) - 1;
+            nr = std::max<int>(nr, vs[(size_t)y1] + v.axis.taps - vs[(size_t)y0]);
+        }
+        if ((size_t)nr * tx * h.bands * esz <= 64 * 1024) break;
+        if (tx > 8) tx /= 2;
+        else if (ty > 1) ty /= 2;
+        else return false;
+    }
+    s->plan_start = st->put(hs.data(), hs.size() * sizeof(int32_t));
+    s->plan_phase = st->put(hp.data(), hp.size());
+    s->w16 = st->put(h.axis.wi16.data(), h.axis.wi16.size() * sizeof(int16_t));
+    s->w64 = st->put(h.axis.wf64.data(), h.axis.wf64.size() * sizeof(double));
+    s->taps = h.axis.taps;
+    s->v_start = st->put(vs.data(), vs.size() * sizeof(int32_t));
+    s->v_phase = st->put(vp.data(), vp.size());
+    s->v_w16 = st->put(v.axis.wi16.data(), v.axis.wi16.size() * sizeof(int16_t));
+    s->v_w64 = st->put(v.axis.wf64.data(), v.axis.wf64.size() * sizeof(double));
+    s->v_taps = v.axis.taps;
+    s->tx = tx;
+    s->ty = ty;
+    s->nr_max = nr;
+    s->vclip0 = (int)vclip.y;
+    s->vclip1 = (int)(vclip.y + std::max<int64_t>(vclip.h, 1) - 1);
+    return true;
+}
+
 void stage_conv_params(const Node& nd, Staging* st, Stage* s) {
     s->itaps = st->put(nd.itaps.data(), nd.itaps.size() * sizeof(int16_t));
     s->wts = st->put(nd.conv.data(), nd.conv.size() * sizeof(double));
@@ -219,6 +272,21 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
         p.taps = s.taps;
         e = s.kind == ST_REDUCE_H ? launch_reduce_h(in, out, B, nd.in_fmt, p, 0, stream)
                                   : launch_reduce_v(in, out, B, nd.in_fmt, p, 0, stream);
+        break;
+    }
+    case ST_REDUCE_HV: {
+        DevPlan ph, pv;
+        ph.start = reinterpret_cast<const int32_t*>(blob + s.plan_start);
+        ph.phase = reinterpret_cast<const uint8_t*>(blob + s.plan_phase);
+        ph.wi16 = reinterpret_cast<const int16_t*>(blob + s.w16);
+        ph.wf64 = reinterpret_cast<const double*>(blob + s.w64);
+        ph.taps = s.taps;
+        pv.start = reinterpret_cast<const int32_t*>(blob + s.v_start);
+        pv.phase = reinterpret_cast<const uint8_t*>(blob + s.v_phase);
+        pv.wi16 = reinterpret_cast<const int16_t*>(blob + s.v_w16);
+        pv.wf64 = reinterpret_cast<const double*>(blob + s.v_w64);
+        pv.taps = s.v_taps;
+        e = launch_reduce_hv(in, out, B, nd.in_fmt, ph, pv, s.tx, s.ty, s.nr_max, s.vclip0, s.vclip1, stream);
         break;
     }
     case ST_CONV: {
@@ -321,6 +389,11 @@ int vgpu_pipeline::compile() {
                 s.fv = (int)nodes[(size_t)j].factor;
             }
             s.last = j;
+        } else if (nd.kind == N_REDUCE_H && i + 1 < n && nodes[(size_t)i + 1].kind == N_REDUCE_V &&
+                   stage_reduce_hv_params(nd, nodes[(size_t)i + 1], out_r[(size_t)i],
+                                          out_r[(size_t)i + 1], in_r[(size_t)i + 1], &st, &s)) {
+            s.kind = ST_REDUCE_HV;
+            s.last = i + 1;
         } else {
             s.last = i;
             switch (nd.kind) {
