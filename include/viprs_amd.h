@@ -122,6 +122,9 @@ int vgpu_pipeline_reduce_v(vgpu_pipeline *p, double factor, int kernel);
  * (builder/resample.rs:133-209,392-444). */
 int vgpu_pipeline_shrink_h(vgpu_pipeline *p, uint32_t factor, int ceil_mode);
 int vgpu_pipeline_shrink_v(vgpu_pipeline *p, uint32_t factor, int ceil_mode);
+/* PipelineBuilder::shrink(hshrink, vshrink) -> Shrink<F> (builder/resample.rs:
+ * 219-268; resample/shrink.rs:37-293): ShrinkH then ShrinkV inside one op. */
+int vgpu_pipeline_shrink(vgpu_pipeline *p, uint32_t hshrink, uint32_t vshrink);
 /* PipelineBuilder::gauss_blur (builder/colour.rs:130-197): U8 stays U8, other
  * formats produce F32. */
 int vgpu_pipeline_gauss_blur(vgpu_pipeline *p, float sigma);

@@ -454,6 +454,18 @@ int orc_pipeline_shrink_v(orc_pipeline *p, uint32_t factor, int ceil_mode) {
     return ORC_OK;
 }
 
+/* PipelineBuilder::shrink(h, v) -> ShrinkBridge / Shrink<F> (builder/resample.rs:
+ * 219-268; shrink.rs:37-293): ShrinkH (without the full-row borrow) composed with
+ * ShrinkV inside one op; the u8 2x2 direct path (shrink.rs:225-293) is the same
+ * two pair roundings. */
+int orc_pipeline_shrink(orc_pipeline *p, uint32_t hshrink, uint32_t vshrink) {
+    if (hshrink == 0 || vshrink == 0) return ORC_EINVAL;
+    int rc = orc_pipeline_shrink_h(p, hshrink, 0);
+    if (rc != ORC_OK) return rc;
+    p->nodes[p->n - 1].source_width = 0;
+    return orc_pipeline_shrink_v(p, vshrink, 0);
+}
+
 /* PipelineBuilder::gauss_blur (builder/colour.rs:130-197): U8 stays U8, every
  * other format produces F32 (gauss_blur.rs:214-403). */
 int orc_pipeline_gauss_blur(orc_pipeline *p, float sigma) {
@@ -768,7 +780,7 @@ static rgn required_input(const node_t *nd, rgn o) {
     switch (nd->type) {
     case N_SHRINK_H: { /* shrinkh.rs:220-235 */
         int64_t need = o.w * (int64_t)nd->factor;
-        if (o.x == 0 && need <= (int64_t)nd->source_width) {
+        if (o.x == 0 && nd->source_width != 0 && need <= (int64_t)nd->source_width) {
             r.x = 0;
             r.w = nd->source_width;
         } else {

@@ -72,6 +72,7 @@ int orc_pipeline_reduce_h(orc_pipeline *p, double factor, int kernel);
 int orc_pipeline_reduce_v(orc_pipeline *p, double factor, int kernel);
 int orc_pipeline_shrink_h(orc_pipeline *p, uint32_t factor, int ceil_mode);
 int orc_pipeline_shrink_v(orc_pipeline *p, uint32_t factor, int ceil_mode);
+int orc_pipeline_shrink(orc_pipeline *p, uint32_t hshrink, uint32_t vshrink);
 int orc_pipeline_gauss_blur(orc_pipeline *p, float sigma);
 int orc_pipeline_convsep(orc_pipeline *p, const double *coeffs, uint32_t n, double scale,
                          double offset);

@@ -56,6 +56,7 @@ def lib() -> C.CDLL:
         L.orc_pipeline_reduce_v.argtypes = [vp, dbl, i32]
         L.orc_pipeline_shrink_h.argtypes = [vp, u32, i32]
         L.orc_pipeline_shrink_v.argtypes = [vp, u32, i32]
+        L.orc_pipeline_shrink.argtypes = [vp, u32, u32]
         L.orc_pipeline_gauss_blur.argtypes = [vp, C.c_float]
         L.orc_pipeline_convsep.argtypes = [vp, vp, u32, dbl, dbl]
         L.orc_pipeline_premultiply.argtypes = [vp]
@@ -146,6 +147,10 @@ class Pipeline:
 
     def shrink_v(self, factor, ceil=False):
         _check(lib().orc_pipeline_shrink_v(self._p, factor, int(ceil)), "shrink_v")
+        return self
+
+    def shrink(self, hshrink, vshrink):
+        _check(lib().orc_pipeline_shrink(self._p, hshrink, vshrink), "shrink")
         return self
 
     def gauss_blur(self, sigma):

@@ -65,6 +65,7 @@ def load() -> C.CDLL:
         "vgpu_pipeline_reduce_v": (i32, [vp, dbl, i32]),
         "vgpu_pipeline_shrink_h": (i32, [vp, u32, i32]),
         "vgpu_pipeline_shrink_v": (i32, [vp, u32, i32]),
+        "vgpu_pipeline_shrink": (i32, [vp, u32, u32]),
         "vgpu_pipeline_gauss_blur": (i32, [vp, f32]),
         "vgpu_pipeline_convsep": (i32, [vp, P(dbl), u32, dbl, dbl]),
         "vgpu_pipeline_premultiply": (i32, [vp]),

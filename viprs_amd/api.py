@@ -181,6 +181,10 @@ class Pipeline:
         _check(self._lib.vgpu_pipeline_shrink_v(self._h, factor, int(ceil)), "shrink_v")
         return self
 
+    def shrink(self, hshrink: int, vshrink: int):
+        _check(self._lib.vgpu_pipeline_shrink(self._h, hshrink, vshrink), "shrink")
+        return self
+
     def gauss_blur(self, sigma: float):
         _check(self._lib.vgpu_pipeline_gauss_blur(self._h, sigma), "gauss_blur")
         return self

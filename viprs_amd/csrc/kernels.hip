@@ -206,11 +206,50 @@ __device__ __forceinline__ uint32_t band_mask(int b, int kk) {
     return m == 0 ? 0x01000001u : (m == 1 ? 0x00000100u : 0x00010000u);
 }
 
-template <int B>
+// Sum of one box window of wbytes bytes starting at LDS byte sbyte, per band.
+// SMALL: the window is at most 64 bytes, so its (at most 17) dwords are loaded
+// in one burst and the dword loop is unrolled with compile-time band masks.
+template <int B, bool SMALL>
+__device__ __forceinline__ void window_sums(const unsigned char* rbuf, int sbyte, int wbytes,
+                                            uint32_t (&sum)[B]) {
+    const uint32_t* l32 = reinterpret_cast<const uint32_t*>(rbuf);
+    const int d0 = sbyte >> 2, al = sbyte & 3;
+    const int nd = (wbytes + 3) >> 2;
+    const int rem_last = wbytes - 4 * (nd - 1);  // 1..4 valid bytes in the last dword
+    const uint32_t last_mask = rem_last >= 4 ? 0xffffffffu : ((1u << (8 * rem_last)) - 1u);
+    if constexpr (SMALL) {
+        constexpr int kMaxD = 17;
+        uint32_t w[kMaxD];
+#pragma unroll
+        for (int d = 0; d < kMaxD; ++d) w[d] = d <= nd ? l32[d0 + d] : 0u;
+#pragma unroll
+        for (int d = 0; d < kMaxD - 1; ++d) {
+            if (d < nd) {
+                uint32_t v = __builtin_amdgcn_alignbyte(w[d + 1], w[d], al);
+                if (d == nd - 1) v &= last_mask;
+#pragma unroll
+                for (int b = 0; b < B; ++b) sum[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, d % 3), sum[b], false);
+            }
+        }
+    } else {
+        uint32_t cur = l32[d0];
+        int kk = 0;
+        for (int d = 0; d < nd; ++d) {
+            const uint32_t nxt = l32[d0 + d + 1];
+            uint32_t v = __builtin_amdgcn_alignbyte(nxt, cur, al);
+            cur = nxt;
+            if (d == nd - 1) v &= last_mask;
+#pragma unroll
+            for (int b = 0; b < B; ++b) sum[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, kk), sum[b], false);
+            kk = kk == 2 ? 0 : kk + 1;
+        }
+    }
+}
+
+template <int B, bool SMALL>
 __global__ __launch_bounds__(kThreads) void k_shrink_hv_u8_rows(InView in, OutView out, int fh,
                                                                 int fv, int tx, int premult,
-                                                                uint32_t mh, uint32_t mv,
-                                                                int aligned) {
+                                                                uint32_t mh, uint32_t mv) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int oyi = blockIdx.y * (kThreads / 64) + wave;
@@ -229,26 +268,21 @@ __global__ __launch_bounds__(kThreads) void k_shrink_hv_u8_rows(InView in, OutVi
     unsigned char* rbuf = lds + wave * (kSpanBytes + 32);
     const int wbytes = fh * B;
 
+    // One source row ahead in registers: the 16-byte chunks of the clamped
+    // span, aligned down to 16 bytes (the buffer base is 16-byte aligned).
     uint4 pre[kPrefetch];
+    size_t pre_a16 = 0;
     int pre_shift = 0, pre_nch = 0;
     auto fetch = [&](int k) {
         const size_t a = (size_t)row_of(in, oy * fv + k) * row_bytes + (size_t)(lo - in.bx) * B;
-        const size_t a16 = aligned ? (a & ~(size_t)15) : a;
-        pre_shift = (int)(a - a16);
+        pre_a16 = a & ~(size_t)15;
+        pre_shift = (int)(a - pre_a16);
         pre_nch = (pre_shift + nbytes + 15) >> 4;
 #pragma unroll
         for (int j = 0; j < kPrefetch; ++j) {
             const int i = lane + 64 * j;
-            if (i < pre_nch) {
-                const size_t off = a16 + (size_t)i * 16;
-                if (aligned && off + 16 <= total) {
-                    pre[j] = *reinterpret_cast<const uint4*>(base + off);
-                } else {
-                    unsigned char tmp[16];
-                    for (int q = 0; q < 16; ++q) tmp[q] = (off + q < total) ? base[off + q] : 0;
-                    pre[j] = *reinterpret_cast<const uint4*>(tmp);
-                }
-            }
+            const size_t off = pre_a16 + (size_t)i * 16;
+            if (i < pre_nch && off + 16 <= total) pre[j] = *reinterpret_cast<const uint4*>(base + off);
         }
     };
 
@@ -262,7 +296,15 @@ __global__ __launch_bounds__(kThreads) void k_shrink_hv_u8_rows(InView in, OutVi
 #pragma unroll
             for (int j = 0; j < kPrefetch; ++j) {
                 const int i = lane + 64 * j;
-                if (i < pre_nch) *reinterpret_cast<uint4*>(rbuf + i * 16) = pre[j];
+                if (i < pre_nch) {
+                    const size_t off = pre_a16 + (size_t)i * 16;
+                    if (off + 16 <= total) {
+                        *reinterpret_cast<uint4*>(rbuf + i * 16) = pre[j];
+                    } else {
+                        // Last chunk of the buffer: copy only the bytes that exist.
+                        for (int q = 0; q < 16 && off + q < total; ++q) rbuf[i * 16 + q] = base[off + q];
+                    }
+                }
             }
         }
         __syncthreads();
@@ -272,29 +314,14 @@ __global__ __launch_bounds__(kThreads) void k_shrink_hv_u8_rows(InView in, OutVi
             uint32_t sum[B];
 #pragma unroll
             for (int b = 0; b < B; ++b) sum[b] = 0;
-            const bool inside = X0 >= lo && X0 + fh <= hi;
-            if (inside && !(premult && (B == 2 || B == 4))) {
-                const int sbyte = shift + (X0 - lo) * B;
-                const uint32_t* l32 = reinterpret_cast<const uint32_t*>(rbuf);
-                const int d0 = sbyte >> 2, al = sbyte & 3;
-                const int nd = (wbytes + 3) >> 2;
-                uint32_t cur = l32[d0];
-                int kk = 0;
-                for (int d = 0; d < nd; ++d) {
-                    const uint32_t nxt = l32[d0 + d + 1];
-                    uint32_t v = __builtin_amdgcn_alignbyte(nxt, cur, al);
-                    cur = nxt;
-                    const int rem = wbytes - 4 * d;
-                    if (rem < 4) v &= (1u << (8 * rem)) - 1u;
-#pragma unroll
-                    for (int b = 0; b < B; ++b) sum[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, kk), sum[b], false);
-                    kk = kk == 2 ? 0 : kk + 1;
-                }
+            const bool plain = !(premult && (B == 2 || B == 4));
+            if (plain && X0 >= lo && X0 + fh <= hi) {
+                window_sums<B, SMALL>(rbuf, shift + (X0 - lo) * B, wbytes, sum);
             } else {
                 for (int j = 0; j < fh; ++j) {
                     const int X = clampi(X0 + j, lo, hi - 1);
                     const unsigned char* px = rbuf + shift + (X - lo) * B;
-                    if (premult && (B == 2 || B == 4)) {
+                    if (!plain) {
                         const uint32_t a = px[B - 1];
 #pragma unroll
                         for (int b = 0; b < B - 1; ++b) sum[b] += premult_u8(px[b], a);
@@ -583,19 +610,23 @@ hipError_t launch_shrink_hv_u8(const InView& in, const OutView& out, int bands, 
     const int wb = fh * bands;
     int tx = 64;
     while (tx > 1 && tx * wb + 15 > kSpanBytes) tx /= 2;
-    if (tx * wb + 15 <= kSpanBytes) {
+    if (aligned && tx * wb + 15 <= kSpanBytes) {
         const size_t shmem = (size_t)(kThreads / 64) * (kSpanBytes + 32);
         dim3 grid((unsigned)((out.w + tx - 1) / tx), (unsigned)((out.h + 3) / 4));
+#define VGPU_ROWS(BB, SM) hipLaunchKernelGGL((k_shrink_hv_u8_rows<BB, SM>), grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, premult, mh, mv)
+        const bool small = wb <= 64;
         switch (bands) {
-        case 1: hipLaunchKernelGGL(k_shrink_hv_u8_rows<1>, grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, premult, mh, mv, aligned); break;
-        case 2: hipLaunchKernelGGL(k_shrink_hv_u8_rows<2>, grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, premult, mh, mv, aligned); break;
-        case 3: hipLaunchKernelGGL(k_shrink_hv_u8_rows<3>, grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, premult, mh, mv, aligned); break;
-        case 4: hipLaunchKernelGGL(k_shrink_hv_u8_rows<4>, grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, premult, mh, mv, aligned); break;
+        case 1: if (small) VGPU_ROWS(1, true); else VGPU_ROWS(1, false); break;
+        case 2: if (small) VGPU_ROWS(2, true); else VGPU_ROWS(2, false); break;
+        case 3: if (small) VGPU_ROWS(3, true); else VGPU_ROWS(3, false); break;
+        case 4: if (small) VGPU_ROWS(4, true); else VGPU_ROWS(4, false); break;
         default: return hipErrorInvalidValue;
         }
+#undef VGPU_ROWS
         return hipGetLastError();
     }
-    // Very wide box windows (fh * bands > 4080 bytes): block-per-row kernel.
+    // Very wide box windows (fh * bands > 4080 bytes) or a source that is not
+    // 16-byte aligned: block-per-row kernel with byte-exact loads.
     const size_t budget = 64 * 1024;
     tx = 64;
     auto pitch_of = [&](int cols) { return ((size_t)cols * fh * bands + 32 + 15) & ~(size_t)15; };

@@ -214,18 +214,29 @@ bool stage_reduce_hv_params(const Node& h, const Node& v, const Rect& mid, const
     build_plan(h, mid.x, mid.w, h.in_fmt == U8, &hs, &hp);
     build_plan(v, out.y, out.h, false, &vs, &vp);
     const size_t esz = format_size(h.out_fmt);
-    int tx = 64, ty = 16, nr = 0;
-    for (;;) {
-        nr = 1;
-        for (int64_t y0 = 0; y0 < out.h; y0 += ty) {
-            const int64_t y1 = std::min<int64_t>(y0 + ty, out.h) - 1;
-            nr = std::max<int>(nr, vs[(size_t)y1] + v.axis.taps - vs[(size_t)y0]);
+    // Tile candidates, largest first: take the first whose staged rows fit
+    // 64 KiB of LDS and that still gives >= 1024 blocks; otherwise the fitting
+    // candidate with the most blocks.
+    static const int cand[][2] = {{64, 16}, {32, 16}, {32, 8}, {16, 8}, {16, 4}, {8, 4}, {8, 2}, {8, 1}, {4, 1}, {1, 1}};
+    int tx = 0, ty = 0, nr = 0;
+    int64_t best_blocks = -1;
+    for (const auto& c : cand) {
+        int rows = 1;
+        for (int64_t y0 = 0; y0 < out.h; y0 += c[1]) {
+            const int64_t y1 = std::min<int64_t>(y0 + c[1], out.h) - 1;
+            rows = std::max<int>(rows, vs[(size_t)y1] + v.axis.taps - vs[(size_t)y0]);
         }
-        if ((size_t)nr * tx * h.bands * esz <= 64 * 1024) break;
-        if (tx > 8) tx /= 2;
-        else if (ty > 1) ty /= 2;
-        else return false;
+        if ((size_t)rows * c[0] * h.bands * esz > 64 * 1024) continue;
+        const int64_t blocks = ((out.w + c[0] - 1) / c[0]) * ((out.h + c[1] - 1) / c[1]);
+        if (blocks > best_blocks) {
+            best_blocks = blocks;
+            tx = c[0];
+            ty = c[1];
+            nr = rows;
+        }
+        if (blocks >= 1024) break;
     }
+    if (best_blocks < 0) return false;
     s->plan_start = st->put(hs.data(), hs.size() * sizeof(int32_t));
     s->plan_phase = st->put(hp.data(), hp.size());
     s->w16 = st->put(h.axis.wi16.data(), h.axis.wi16.size() * sizeof(int16_t));
@@ -606,6 +617,16 @@ int vgpu_pipeline_shrink_v(vgpu_pipeline* p, uint32_t factor, int ceil_mode) {
     nd.out_h = ceil_mode ? (h + factor - 1) / factor : h / factor;
     p->commit();
     return VGPU_OK;
+}
+
+int vgpu_pipeline_shrink(vgpu_pipeline* p, uint32_t hshrink, uint32_t vshrink) {
+    if (!p || hshrink == 0 || vshrink == 0) return VGPU_E_INVALID_ARGUMENT;
+    // Shrink<F> = ShrinkH (no full-row borrow) then ShrinkV inside one op
+    // (shrink.rs:37-293); the two nodes fuse into one launch.
+    int rc = vgpu_pipeline_shrink_h(p, hshrink, 0);
+    if (rc != VGPU_OK) return rc;
+    p->nodes.back().source_width = 0;
+    return vgpu_pipeline_shrink_v(p, vshrink, 0);
 }
 
 int vgpu_pipeline_gauss_blur(vgpu_pipeline* p, float sigma) {
