@@ -270,36 +270,46 @@ __global__ __launch_bounds__(kThreads) void k_shrink_hv_u8_rows(InView in, OutVi
 
     // One source row ahead in registers: the 16-byte chunks of the clamped
     // span, aligned down to 16 bytes (the buffer base is 16-byte aligned).
-    uint4 pre[kPrefetch];
+    // The loads are branch-free: a chunk that is not needed (or would run past
+    // the end of the buffer) reloads the buffer's first 16 bytes and is
+    // ignored, so all loads issue back to back and stay in flight across the
+    // compute of the current row.
+    uint4 p0 = make_uint4(0, 0, 0, 0), p1 = p0, p2 = p0, p3 = p0;
+    static_assert(kPrefetch == 4, "prefetch registers are unrolled by hand");
     size_t pre_a16 = 0;
     int pre_shift = 0, pre_nch = 0;
-    auto fetch = [&](int k) {
-        const size_t a = (size_t)row_of(in, oy * fv + k) * row_bytes + (size_t)(lo - in.bx) * B;
-        pre_a16 = a & ~(size_t)15;
-        pre_shift = (int)(a - pre_a16);
-        pre_nch = (pre_shift + nbytes + 15) >> 4;
-#pragma unroll
-        for (int j = 0; j < kPrefetch; ++j) {
-            const int i = lane + 64 * j;
-            const size_t off = pre_a16 + (size_t)i * 16;
-            if (i < pre_nch && off + 16 <= total) pre[j] = *reinterpret_cast<const uint4*>(base + off);
-        }
-    };
+#define VGPU_FETCH(k)                                                                              \
+    do {                                                                                           \
+        const size_t a_ = (size_t)row_of(in, oy * fv + (k)) * row_bytes + (size_t)(lo - in.bx) * B; \
+        pre_a16 = a_ & ~(size_t)15;                                                                \
+        pre_shift = (int)(a_ - pre_a16);                                                           \
+        pre_nch = (pre_shift + nbytes + 15) >> 4;                                                  \
+        size_t o0 = pre_a16 + (size_t)lane * 16, o1 = o0 + 1024, o2 = o0 + 2048, o3 = o0 + 3072;   \
+        o0 = (lane < pre_nch && o0 + 16 <= total) ? o0 : 0;                                        \
+        o1 = (lane + 64 < pre_nch && o1 + 16 <= total) ? o1 : 0;                                   \
+        o2 = (lane + 128 < pre_nch && o2 + 16 <= total) ? o2 : 0;                                  \
+        o3 = (lane + 192 < pre_nch && o3 + 16 <= total) ? o3 : 0;                                  \
+        p0 = *reinterpret_cast<const uint4*>(base + o0);                                           \
+        p1 = *reinterpret_cast<const uint4*>(base + o1);                                           \
+        p2 = *reinterpret_cast<const uint4*>(base + o2);                                           \
+        p3 = *reinterpret_cast<const uint4*>(base + o3);                                           \
+    } while (0)
 
     uint32_t acc[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) acc[b] = 0;
-    if (active) fetch(0);
+    if (active) VGPU_FETCH(0);
     for (int k = 0; k < fv; ++k) {
         const int shift = pre_shift;
         if (active) {
 #pragma unroll
             for (int j = 0; j < kPrefetch; ++j) {
                 const int i = lane + 64 * j;
+                const uint4 v = j == 0 ? p0 : (j == 1 ? p1 : (j == 2 ? p2 : p3));
                 if (i < pre_nch) {
                     const size_t off = pre_a16 + (size_t)i * 16;
                     if (off + 16 <= total) {
-                        *reinterpret_cast<uint4*>(rbuf + i * 16) = pre[j];
+                        *reinterpret_cast<uint4*>(rbuf + i * 16) = v;
                     } else {
                         // Last chunk of the buffer: copy only the bytes that exist.
                         for (int q = 0; q < 16 && off + q < total; ++q) rbuf[i * 16 + q] = base[off + q];
@@ -308,7 +318,7 @@ __global__ __launch_bounds__(kThreads) void k_shrink_hv_u8_rows(InView in, OutVi
             }
         }
         __syncthreads();
-        if (active && k + 1 < fv) fetch(k + 1);
+        if (active && k + 1 < fv) VGPU_FETCH(k + 1);
         if (active && lane < ncols) {
             const int X0 = cx0 + lane * fh;
             uint32_t sum[B];
@@ -342,6 +352,7 @@ __global__ __launch_bounds__(kThreads) void k_shrink_hv_u8_rows(InView in, OutVi
 #pragma unroll
         for (int b = 0; b < B; ++b) o[b] = (unsigned char)shrink_round_v(acc[b], (uint32_t)fv, mv);
     }
+#undef VGPU_FETCH
 }
 
 // ---------------------------------------------------------------------------
@@ -610,7 +621,8 @@ hipError_t launch_shrink_hv_u8(const InView& in, const OutView& out, int bands, 
     const int wb = fh * bands;
     int tx = 64;
     while (tx > 1 && tx * wb + 15 > kSpanBytes) tx /= 2;
-    if (aligned && tx * wb + 15 <= kSpanBytes) {
+    const size_t src_bytes = (size_t)in.bw * in.bh * bands;
+    if (aligned && src_bytes >= 16 && tx * wb + 15 <= kSpanBytes) {
         const size_t shmem = (size_t)(kThreads / 64) * (kSpanBytes + 32);
         dim3 grid((unsigned)((out.w + tx - 1) / tx), (unsigned)((out.h + 3) / 4));
 #define VGPU_ROWS(BB, SM) hipLaunchKernelGGL((k_shrink_hv_u8_rows<BB, SM>), grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, premult, mh, mv)
