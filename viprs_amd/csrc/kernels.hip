@@ -248,12 +248,17 @@ __device__ __forceinline__ void window_sums(const unsigned char* rbuf, int sbyte
 
 template <int B, bool SMALL>
 __global__ __launch_bounds__(kThreads) void k_shrink_hv_u8_rows(InView in, OutView out, int fh,
-                                                                int fv, int tx, int premult,
-                                                                uint32_t mh, uint32_t mv) {
+                                                                int fv, int tx, int S,
+                                                                int premult, uint32_t mh,
+                                                                uint32_t mv) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int oyi = blockIdx.y * (kThreads / 64) + wave;
+    // S wavefronts share one output row; wavefront `part` sums source rows
+    // part, part + S, ...; their integer partial sums meet in LDS at the end.
+    const int part = wave % S;
+    const int oyi = blockIdx.y * ((kThreads / 64) / S) + wave / S;
     const bool active = oyi < out.h;
+    const int nk = part < fv ? (fv - 1 - part) / S + 1 : 0;  // source rows of this wavefront
     const int col0 = blockIdx.x * tx;
     const int ncols = min(tx, out.w - col0);
     const int oy = out.y + oyi;
@@ -298,10 +303,11 @@ __global__ __launch_bounds__(kThreads) void k_shrink_hv_u8_rows(InView in, OutVi
     uint32_t acc[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) acc[b] = 0;
-    if (active) VGPU_FETCH(0);
-    for (int k = 0; k < fv; ++k) {
+    if (active && nk > 0) VGPU_FETCH(part);
+    for (int q = 0; q < (fv + S - 1) / S; ++q) {
+        const bool live = active && q < nk;
         const int shift = pre_shift;
-        if (active) {
+        if (live) {
 #pragma unroll
             for (int j = 0; j < kPrefetch; ++j) {
                 const int i = lane + 64 * j;
@@ -318,8 +324,8 @@ __global__ __launch_bounds__(kThreads) void k_shrink_hv_u8_rows(InView in, OutVi
             }
         }
         __syncthreads();
-        if (active && k + 1 < fv) VGPU_FETCH(k + 1);
-        if (active && lane < ncols) {
+        if (live && q + 1 < nk) VGPU_FETCH(part + (q + 1) * S);
+        if (live && lane < ncols) {
             const int X0 = cx0 + lane * fh;
             uint32_t sum[B];
 #pragma unroll
@@ -347,7 +353,20 @@ __global__ __launch_bounds__(kThreads) void k_shrink_hv_u8_rows(InView in, OutVi
         }
         __syncthreads();
     }
-    if (active && lane < ncols) {
+    if (S > 1) {
+        // Partial sums of the S wavefronts of each output row (row buffers are free now).
+        uint32_t* part_sums = reinterpret_cast<uint32_t*>(lds);
+#pragma unroll
+        for (int b = 0; b < B; ++b) part_sums[(wave * 64 + lane) * B + b] = acc[b];
+        __syncthreads();
+        if (part == 0) {
+            for (int p2 = 1; p2 < S; ++p2) {
+#pragma unroll
+                for (int b = 0; b < B; ++b) acc[b] += part_sums[((wave + p2) * 64 + lane) * B + b];
+            }
+        }
+    }
+    if (active && part == 0 && lane < ncols) {
         unsigned char* o = static_cast<unsigned char*>(out.ptr) + ((size_t)oyi * out.w + col0 + lane) * B;
 #pragma unroll
         for (int b = 0; b < B; ++b) o[b] = (unsigned char)shrink_round_v(acc[b], (uint32_t)fv, mv);
@@ -624,8 +643,15 @@ hipError_t launch_shrink_hv_u8(const InView& in, const OutView& out, int bands, 
     const size_t src_bytes = (size_t)in.bw * in.bh * bands;
     if (aligned && src_bytes >= 16 && tx * wb + 15 <= kSpanBytes) {
         const size_t shmem = (size_t)(kThreads / 64) * (kSpanBytes + 32);
-        dim3 grid((unsigned)((out.w + tx - 1) / tx), (unsigned)((out.h + 3) / 4));
-#define VGPU_ROWS(BB, SM) hipLaunchKernelGGL((k_shrink_hv_u8_rows<BB, SM>), grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, premult, mh, mv)
+        // Split the fv source rows of an output row over S wavefronts while
+        // the (row, span) tasks leave room on the chip: about 6 resident
+        // wavefronts per SIMD over 256 CUs.
+        const int64_t tasks = (int64_t)((out.w + tx - 1) / tx) * out.h;
+        int S = 1;
+        while (S < 4 && 2 * S <= fv && tasks * 2 * S <= 256 * 4 * 6) S *= 2;
+        const int rows_per_block = (kThreads / 64) / S;
+        dim3 grid((unsigned)((out.w + tx - 1) / tx), (unsigned)((out.h + rows_per_block - 1) / rows_per_block));
+#define VGPU_ROWS(BB, SM) hipLaunchKernelGGL((k_shrink_hv_u8_rows<BB, SM>), grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, S, premult, mh, mv)
         const bool small = wb <= 64;
         switch (bands) {
         case 1: if (small) VGPU_ROWS(1, true); else VGPU_ROWS(1, false); break;
