@@ -508,14 +508,23 @@ __global__ __launch_bounds__(kThreads) void k_reduce_v(InView in, OutView out, i
     }
 }
 
-// Fused ReduceH -> ReduceV.  Block = TX output columns x TY output rows.  The
-// ReduceH outputs the tile's ReduceV taps need (rows start_v[first] ..
-// start_v[last] + taps_v - 1, each evaluated on Z^2 exactly as the unfused
-// ReduceH would) are computed into LDS in the intermediate's sample type
-// (u8 / u16 / f32, i.e. rounded as the reference rounds between the passes),
-// then ReduceV reads them from LDS.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_reduce_hv(InView in, OutView out, int B, DevPlan ph,
+// Fixed-point accumulator: int32 when the host has proven
+// max_phase(sum |w|) * max_sample < 2^31, else int64.
+template <typename T, typename Acc>
+__device__ __forceinline__ T fixed_round(Acc acc) {
+    const Acc v = (acc + 2048) >> 12;
+    const Acc mx = sizeof(T) == 1 ? 255 : 65535;
+    return (T)(v < 0 ? 0 : (v > mx ? mx : v));
+}
+
+// Fused ReduceH -> ReduceV for B <= 4 bands.  Block = TX output columns x TY
+// output rows; thread = (column x, row lane).  Pass 1 evaluates the ReduceH
+// outputs the tile's ReduceV taps need (rows start_v[first] .. start_v[last] +
+// taps_v - 1, on Z^2 exactly as the unfused ReduceH) into LDS in the
+// intermediate's sample type (so the reference's rounding between passes is
+// kept); pass 2 applies ReduceV from LDS.
+template <typename T, int B, typename Acc>
+__global__ __launch_bounds__(kThreads) void k_reduce_hv(InView in, OutView out, DevPlan ph,
                                                         DevPlan pv, int TX, int TY, int nr_max,
                                                         int vclip0, int vclip1) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -524,28 +533,69 @@ __global__ __launch_bounds__(kThreads) void k_reduce_hv(InView in, OutView out, 
     const int ncols = min(TX, out.w - tx0), nrows = min(TY, out.h - ty0);
     const int ys0 = pv.start[ty0];
     const int nr = min(nr_max, pv.start[ty0 + nrows - 1] + pv.taps - ys0);
+    const int x = threadIdx.x % TX, rl = threadIdx.x / TX, nrl = kThreads / TX;
     const int pitch = TX * B;
-    for (int it = threadIdx.x; it < nr * ncols * B; it += kThreads) {
-        const int r = it / (ncols * B), rem = it - r * (ncols * B);
-        const int x = rem / B, b = rem - x * B;
+    if (x < ncols) {
         const int xo = tx0 + x;
-        hbuf[r * pitch + x * B + b] =
-            reduce_h_sample<T>(in, row_of(in, ys0 + r), B, b, ph.start[xo], ph.phase[xo], ph);
+        const int hs = ph.start[xo];
+        const int hph = ph.phase[xo];
+        for (int r = rl; r < nr; r += nrl) {
+            const int row = row_of(in, ys0 + r);
+            T* dst = hbuf + r * pitch + x * B;
+            if constexpr (sizeof(T) == 4) {
+#pragma unroll
+                for (int b = 0; b < B; ++b) dst[b] = reduce_h_sample<T>(in, row, B, b, hs, hph, ph);
+            } else {
+                const int16_t* w = ph.wi16 + (size_t)hph * ph.taps;
+                const T* src = static_cast<const T*>(in.ptr) + (size_t)row * in.bw * B;
+                Acc acc[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) acc[b] = 0;
+                for (int t = 0; t < ph.taps; ++t) {
+                    const T* px = src + (size_t)col_of(in, hs + t) * B;
+                    const Acc wt = w[t];
+#pragma unroll
+                    for (int b = 0; b < B; ++b) acc[b] += wt * (Acc)px[b];
+                }
+#pragma unroll
+                for (int b = 0; b < B; ++b) dst[b] = fixed_round<T, Acc>(acc[b]);
+            }
+        }
     }
     __syncthreads();
+    if (x >= ncols) return;
     const int row_len = out.w * B;
     const int vec_len = row_len / 4 * 4;
-    for (int it = threadIdx.x; it < nrows * ncols * B; it += kThreads) {
-        const int ry = it / (ncols * B), rem = it - ry * (ncols * B);
-        const int x = rem / B, b = rem - x * B;
+    for (int ry = rl; ry < nrows; ry += nrl) {
         const int yo = ty0 + ry;
-        const int sidx = (tx0 + x) * B + b;
-        auto get = [&](int y) -> T {
-            const int yy = clampi(clampi(y, vclip0, vclip1) - ys0, 0, nr - 1);
-            return hbuf[yy * pitch + x * B + b];
-        };
-        static_cast<T*>(out.ptr)[(size_t)yo * row_len + sidx] =
-            reduce_v_sample<T>(get, pv.start[yo], pv.phase[yo], pv, sidx < vec_len);
+        const int vs = pv.start[yo];
+        const int vph = pv.phase[yo];
+        T* o = static_cast<T*>(out.ptr) + (size_t)yo * row_len + (size_t)(tx0 + x) * B;
+        if constexpr (sizeof(T) == 4) {
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int sidx = (tx0 + x) * B + b;
+                auto get = [&](int y) -> T {
+                    const int yy = clampi(clampi(y, vclip0, vclip1) - ys0, 0, nr - 1);
+                    return hbuf[yy * pitch + x * B + b];
+                };
+                o[b] = reduce_v_sample<T>(get, vs, vph, pv, sidx < vec_len);
+            }
+        } else {
+            const int16_t* w = pv.wi16 + (size_t)vph * pv.taps;
+            Acc acc[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) acc[b] = 0;
+            for (int t = 0; t < pv.taps; ++t) {
+                const int yy = clampi(clampi(vs + t, vclip0, vclip1) - ys0, 0, nr - 1);
+                const T* hp = hbuf + yy * pitch + x * B;
+                const Acc wt = w[t];
+#pragma unroll
+                for (int b = 0; b < B; ++b) acc[b] += wt * (Acc)hp[b];
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) o[b] = fixed_round<T, Acc>(acc[b]);
+        }
     }
 }
 
@@ -724,13 +774,28 @@ hipError_t launch_reduce_v(const InView& in, const OutView& out, int bands, int 
 
 hipError_t launch_reduce_hv(const InView& in, const OutView& out, int bands, int fmt,
                             const DevPlan& ph, const DevPlan& pv, int tx, int ty, int nr_max,
-                            int vclip0, int vclip1, hipStream_t s) {
+                            int vclip0, int vclip1, int wide_acc, hipStream_t s) {
     if (out.w <= 0 || out.h <= 0) return hipSuccess;
+    if (bands < 1 || bands > 4) return hipErrorInvalidValue;
     const size_t shmem = (size_t)nr_max * tx * bands * (fmt == 0 ? 1 : (fmt == 1 ? 2 : 4));
     dim3 grid((unsigned)((out.w + tx - 1) / tx), (unsigned)((out.h + ty - 1) / ty));
-    if (fmt == 0) hipLaunchKernelGGL(k_reduce_hv<uint8_t>, grid, dim3(kThreads), shmem, s, in, out, bands, ph, pv, tx, ty, nr_max, vclip0, vclip1);
-    else if (fmt == 1) hipLaunchKernelGGL(k_reduce_hv<uint16_t>, grid, dim3(kThreads), shmem, s, in, out, bands, ph, pv, tx, ty, nr_max, vclip0, vclip1);
-    else hipLaunchKernelGGL(k_reduce_hv<float>, grid, dim3(kThreads), shmem, s, in, out, bands, ph, pv, tx, ty, nr_max, vclip0, vclip1);
+#define VGPU_RHV(TT, BB, AA) hipLaunchKernelGGL((k_reduce_hv<TT, BB, AA>), grid, dim3(kThreads), shmem, s, in, out, ph, pv, tx, ty, nr_max, vclip0, vclip1)
+#define VGPU_RHV_B(TT, AA)                  \
+    switch (bands) {                        \
+    case 1: VGPU_RHV(TT, 1, AA); break;     \
+    case 2: VGPU_RHV(TT, 2, AA); break;     \
+    case 3: VGPU_RHV(TT, 3, AA); break;     \
+    default: VGPU_RHV(TT, 4, AA); break;    \
+    }
+    if (fmt == 0) {
+        if (wide_acc) { VGPU_RHV_B(uint8_t, int64_t) } else { VGPU_RHV_B(uint8_t, int32_t) }
+    } else if (fmt == 1) {
+        if (wide_acc) { VGPU_RHV_B(uint16_t, int64_t) } else { VGPU_RHV_B(uint16_t, int32_t) }
+    } else {
+        VGPU_RHV_B(float, int64_t)
+    }
+#undef VGPU_RHV_B
+#undef VGPU_RHV
     return hipGetLastError();
 }
 

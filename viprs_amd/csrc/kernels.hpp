@@ -45,12 +45,13 @@ hipError_t launch_reduce_h(const InView& in, const OutView& out, int bands, int 
                            const DevPlan& plan, int post_alpha, hipStream_t s);
 hipError_t launch_reduce_v(const InView& in, const OutView& out, int bands, int fmt,
                            const DevPlan& plan, int post_alpha, hipStream_t s);
-// Fused ReduceH -> ReduceV: ph plans the ReduceH output columns (indexed from
-// out.x), pv the ReduceV output rows; [vclip0, vclip1] is ReduceV's input
-// row range (its reference tile rectangle).  nr_max bounds the staged rows.
+// Fused ReduceH -> ReduceV (bands <= 4): ph plans the ReduceH output columns
+// (indexed from out.x), pv the ReduceV output rows; [vclip0, vclip1] is
+// ReduceV's input row range (its reference tile rectangle).  nr_max bounds
+// the staged rows.  wide_acc selects int64 fixed-point sums (int32 otherwise).
 hipError_t launch_reduce_hv(const InView& in, const OutView& out, int bands, int fmt,
                             const DevPlan& ph, const DevPlan& pv, int tx, int ty, int nr_max,
-                            int vclip0, int vclip1, hipStream_t s);
+                            int vclip0, int vclip1, int wide_acc, hipStream_t s);
 hipError_t launch_conv(const InView& in, const OutView& out, int bands, int in_fmt, int out_fmt,
                        int kind, int vertical, const int16_t* itaps, int32_t iscale,
                        const double* wts, int ntaps, double cscale, double coffset,

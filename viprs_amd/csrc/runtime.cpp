@@ -142,6 +142,7 @@ struct Stage {
     size_t v_start = 0, v_phase = 0, v_w16 = 0, v_w64 = 0;
     int v_taps = 1, tx = 64, ty = 16, nr_max = 1;
     int vclip0 = 0, vclip1 = 0;
+    int wide_acc = 0;  // int64 fixed-point sums needed
 };
 
 Rect whole(uint32_t w, uint32_t h) { return Rect{0, 0, (int64_t)w, (int64_t)h}; }
@@ -252,6 +253,17 @@ bool stage_reduce_hv_params(const Node& h, const Node& v, const Rect& mid, const
     s->nr_max = nr;
     s->vclip0 = (int)vclip.y;
     s->vclip1 = (int)(vclip.y + std::max<int64_t>(vclip.h, 1) - 1);
+    // int32 sums are exact when max over phases of sum |w| times the largest
+    // sample stays below 2^31.
+    const int64_t maxv = h.in_fmt == U8 ? 255 : 65535;
+    int64_t worst = 0;
+    for (const ReduceAxis* ax : {&h.axis, &v.axis})
+        for (int p = 0; p <= kPhases; ++p) {
+            int64_t sabs = 0;
+            for (int t = 0; t < ax->taps; ++t) sabs += std::abs((int64_t)ax->wi16[(size_t)p * ax->taps + t]);
+            worst = std::max(worst, sabs * maxv);
+        }
+    s->wide_acc = worst >= (int64_t)INT32_MAX - 4096;
     return true;
 }
 
@@ -297,7 +309,8 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
         pv.wi16 = reinterpret_cast<const int16_t*>(blob + s.v_w16);
         pv.wf64 = reinterpret_cast<const double*>(blob + s.v_w64);
         pv.taps = s.v_taps;
-        e = launch_reduce_hv(in, out, B, nd.in_fmt, ph, pv, s.tx, s.ty, s.nr_max, s.vclip0, s.vclip1, stream);
+        e = launch_reduce_hv(in, out, B, nd.in_fmt, ph, pv, s.tx, s.ty, s.nr_max, s.vclip0, s.vclip1,
+                             s.wide_acc, stream);
         break;
     }
     case ST_CONV: {
@@ -400,7 +413,7 @@ int vgpu_pipeline::compile() {
                 s.fv = (int)nodes[(size_t)j].factor;
             }
             s.last = j;
-        } else if (nd.kind == N_REDUCE_H && i + 1 < n && nodes[(size_t)i + 1].kind == N_REDUCE_V &&
+        } else if (nd.kind == N_REDUCE_H && nd.bands <= 4 && i + 1 < n && nodes[(size_t)i + 1].kind == N_REDUCE_V &&
                    stage_reduce_hv_params(nd, nodes[(size_t)i + 1], out_r[(size_t)i],
                                           out_r[(size_t)i + 1], in_r[(size_t)i + 1], &st, &s)) {
             s.kind = ST_REDUCE_HV;
