@@ -136,7 +136,7 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 5),
                        "p99": round(float(np.percentile(lat, 99)), 5)},
         "stage_ms": [round(x, 5) for x in stage_ms],
-        "roofline": {"kernel": "k_shrink_hv_u8_rows<3>", "bound": "hbm", "achieved": round(achieved, 1),
+        "roofline": {"kernel": "k_shrink_job (u8, 3 bands)", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": load_traffic("shrink_hv_u8_8192x8192x3_traffic.json"),
                      "algorithmic_bytes": shrink_bytes},

@@ -170,6 +170,43 @@ def test_f32_reference_exact(ctx):
             assert_same(got, exp, f"f32 bands {b} {steps}")
 
 
+@pytest.mark.parametrize("fmt", [np.uint8, np.uint16, np.float32])
+@pytest.mark.parametrize("bands", [1, 2, 3, 4, 5])
+def test_reduce_stage_forms(ctx, fmt, bands):
+    # Lone ReduceH and lone ReduceV run through the fused reduce kernel with
+    # an identity axis for the missing pass; V-then-H order gives two such
+    # stages; five bands take the unfused per-pass kernels.
+    if fmt == np.float32:
+        src = float_image(157, 233, bands, 40 + bands)
+    elif fmt == np.uint16:
+        src = noise_u16(157, 233, bands, 40 + bands)
+    else:
+        src = noise_u8(157, 233, bands, 40 + bands)
+    for steps in ([("reduce_h", (2.3, v.LANCZOS3))],
+                  [("reduce_v", (3.1, v.LANCZOS3))],
+                  [("reduce_v", (1.6, v.BICUBIC)), ("reduce_h", (4.4, v.LANCZOS2))],
+                  [("reduce_h", (1.4, v.BILINEAR)), ("reduce_v", (5.9, v.LANCZOS3))]):
+        got, exp = run_both(ctx, src, steps)
+        assert_same(got, exp, f"{np.dtype(fmt)} bands {bands} {steps}")
+
+
+@pytest.mark.parametrize("bands", [2, 3, 4])
+def test_alpha_fusion_forms(ctx, bands):
+    # Premultiply/Unpremultiply fuse into the shrink and reduce stages for 2
+    # and 4 bands; 3 bands keep them as separate pixel stages.
+    src = noise_u8(203, 311, bands, 70 + bands)
+    src[..., -1] = np.where(np.random.default_rng(bands).random((203, 311)) < 0.15, 0, src[..., -1])
+    for steps in ([("premultiply", ()), ("shrink_h", (3,)), ("shrink_v", (2,)), ("unpremultiply", ())],
+                  [("premultiply", ()), ("shrink_v", (4,)), ("unpremultiply", ())],
+                  [("premultiply", ()), ("reduce_h", (2.2, v.LANCZOS3)), ("reduce_v", (2.2, v.LANCZOS3)),
+                   ("unpremultiply", ())],
+                  [("premultiply", ()), ("reduce_v", (1.7, v.LANCZOS3)), ("unpremultiply", ())],
+                  [("premultiply", ()), ("shrink_h", (2,)), ("shrink_v", (2,)), ("reduce_h", (1.3, v.LANCZOS3)),
+                   ("reduce_v", (1.3, v.LANCZOS3)), ("unpremultiply", ())]):
+        got, exp = run_both(ctx, src, steps)
+        assert_same(got, exp, f"bands {bands} {steps}")
+
+
 @pytest.mark.parametrize("w,h,b", [(1, 1, 3), (2, 1, 1), (1, 5, 4), (7, 3, 2), (19, 19, 3), (20, 1000, 1)])
 def test_tiny_and_ragged(ctx, w, h, b):
     src = noise_u8(h, w, b, w * 31 + h)

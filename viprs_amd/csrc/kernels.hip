@@ -247,19 +247,20 @@ __device__ __forceinline__ void window_sums(const unsigned char* rbuf, int sbyte
 }
 
 template <int B, bool SMALL>
-__global__ __launch_bounds__(kThreads) void k_shrink_hv_u8_rows(InView in, OutView out, int fh,
-                                                                int fv, int tx, int S,
-                                                                int premult, uint32_t mh,
-                                                                uint32_t mv) {
-    extern __shared__ __align__(16) unsigned char lds[];
+__device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, int by,
+                                                  unsigned char* lds) {
+    const InView& in = job.in;
+    const OutView& out = job.out;
+    const int fh = job.fh, fv = job.fv, tx = job.tx, S = job.S, premult = job.premult;
+    const uint32_t mh = job.mh, mv = job.mv;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // S wavefronts share one output row; wavefront `part` sums source rows
     // part, part + S, ...; their integer partial sums meet in LDS at the end.
     const int part = wave % S;
-    const int oyi = blockIdx.y * ((kThreads / 64) / S) + wave / S;
+    const int oyi = by * ((kThreads / 64) / S) + wave / S;
     const bool active = oyi < out.h;
     const int nk = part < fv ? (fv - 1 - part) / S + 1 : 0;  // source rows of this wavefront
-    const int col0 = blockIdx.x * tx;
+    const int col0 = bx * tx;
     const int ncols = min(tx, out.w - col0);
     const int oy = out.y + oyi;
     const int cx0 = (out.x + col0) * fh;
@@ -368,10 +369,54 @@ __global__ __launch_bounds__(kThreads) void k_shrink_hv_u8_rows(InView in, OutVi
     }
     if (active && part == 0 && lane < ncols) {
         unsigned char* o = static_cast<unsigned char*>(out.ptr) + ((size_t)oyi * out.w + col0 + lane) * B;
+        uint32_t v[B];
 #pragma unroll
-        for (int b = 0; b < B; ++b) o[b] = (unsigned char)shrink_round_v(acc[b], (uint32_t)fv, mv);
+        for (int b = 0; b < B; ++b) v[b] = shrink_round_v(acc[b], (uint32_t)fv, mv);
+        if ((B == 2 || B == 4) && job.unpremult) {
+#pragma unroll
+            for (int b = 0; b < B - 1; ++b) v[b] = unpremult_u8(v[b], v[B - 1]);
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) o[b] = (unsigned char)v[b];
     }
 #undef VGPU_FETCH
+}
+
+// One block of the shrink job per (bx, by) of its grid.
+__device__ __forceinline__ void shrink_job_block(const ShrinkJob& job, int bx, int by,
+                                                 unsigned char* lds) {
+    switch (job.bands * 2 + job.small) {
+    case 2: shrink_rows_block<1, false>(job, bx, by, lds); break;
+    case 3: shrink_rows_block<1, true>(job, bx, by, lds); break;
+    case 4: shrink_rows_block<2, false>(job, bx, by, lds); break;
+    case 5: shrink_rows_block<2, true>(job, bx, by, lds); break;
+    case 6: shrink_rows_block<3, false>(job, bx, by, lds); break;
+    case 7: shrink_rows_block<3, true>(job, bx, by, lds); break;
+    case 8: shrink_rows_block<4, false>(job, bx, by, lds); break;
+    default: shrink_rows_block<4, true>(job, bx, by, lds); break;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_shrink_job(ShrinkJob job) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    shrink_job_block(job, blockIdx.x, blockIdx.y, lds);
+}
+
+// Persistent form: the grid walks a task list of (job, bx, by) over many
+// images; jobs reading the caller's source or writing the caller's output take
+// those pointers from per-run slot tables.
+__global__ __launch_bounds__(kThreads) void k_shrink_batch(const ShrinkJob* jobs, const BatchTask* tasks,
+                                                           int ntasks, const void* const* srcs,
+                                                           void* const* dsts) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const BatchTask task = tasks[t];
+        ShrinkJob job = jobs[task.job];
+        if (job.src_slot >= 0) job.in.ptr = srcs[job.src_slot];
+        if (job.dst_slot >= 0) job.out.ptr = dsts[job.dst_slot];
+        shrink_job_block(job, task.bx, task.by, lds);
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -524,12 +569,16 @@ __device__ __forceinline__ T fixed_round(Acc acc) {
 // intermediate's sample type (so the reference's rounding between passes is
 // kept); pass 2 applies ReduceV from LDS.
 template <typename T, int B, typename Acc>
-__global__ __launch_bounds__(kThreads) void k_reduce_hv(InView in, OutView out, DevPlan ph,
-                                                        DevPlan pv, int TX, int TY, int nr_max,
-                                                        int vclip0, int vclip1) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void reduce_hv_block(const ReduceJob& job, int bx, int by,
+                                                unsigned char* smem) {
+    const InView& in = job.in;
+    const OutView& out = job.out;
+    const DevPlan& ph = job.ph;
+    const DevPlan& pv = job.pv;
+    const int TX = job.tx, TY = job.ty, nr_max = job.nr_max;
+    const int vclip0 = job.vclip0, vclip1 = job.vclip1;
     T* hbuf = reinterpret_cast<T*>(smem);
-    const int tx0 = blockIdx.x * TX, ty0 = blockIdx.y * TY;
+    const int tx0 = bx * TX, ty0 = by * TY;
     const int ncols = min(TX, out.w - tx0), nrows = min(TY, out.h - ty0);
     const int ys0 = pv.start[ty0];
     const int nr = min(nr_max, pv.start[ty0 + nrows - 1] + pv.taps - ys0);
@@ -563,10 +612,9 @@ __global__ __launch_bounds__(kThreads) void k_reduce_hv(InView in, OutView out, 
         }
     }
     __syncthreads();
-    if (x >= ncols) return;
     const int row_len = out.w * B;
     const int vec_len = row_len / 4 * 4;
-    for (int ry = rl; ry < nrows; ry += nrl) {
+    for (int ry = rl; x < ncols && ry < nrows; ry += nrl) {
         const int yo = ty0 + ry;
         const int vs = pv.start[yo];
         const int vph = pv.phase[yo];
@@ -593,9 +641,56 @@ __global__ __launch_bounds__(kThreads) void k_reduce_hv(InView in, OutView out, 
 #pragma unroll
                 for (int b = 0; b < B; ++b) acc[b] += wt * (Acc)hp[b];
             }
+            T v[B];
 #pragma unroll
-            for (int b = 0; b < B; ++b) o[b] = fixed_round<T, Acc>(acc[b]);
+            for (int b = 0; b < B; ++b) v[b] = fixed_round<T, Acc>(acc[b]);
+            if constexpr (sizeof(T) == 1 && (B == 2 || B == 4)) {
+                if (job.unpremult) {
+#pragma unroll
+                    for (int b = 0; b < B - 1; ++b) v[b] = (T)unpremult_u8(v[b], v[B - 1]);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) o[b] = v[b];
         }
+    }
+}
+
+__device__ __forceinline__ void reduce_job_block(const ReduceJob& job, int bx, int by,
+                                                 unsigned char* smem) {
+#define VGPU_RB(TT, AA)                                            \
+    switch (job.bands) {                                           \
+    case 1: reduce_hv_block<TT, 1, AA>(job, bx, by, smem); break;  \
+    case 2: reduce_hv_block<TT, 2, AA>(job, bx, by, smem); break;  \
+    case 3: reduce_hv_block<TT, 3, AA>(job, bx, by, smem); break;  \
+    default: reduce_hv_block<TT, 4, AA>(job, bx, by, smem); break; \
+    }
+    if (job.fmt == 0) {
+        if (job.wide_acc) { VGPU_RB(uint8_t, int64_t) } else { VGPU_RB(uint8_t, int32_t) }
+    } else if (job.fmt == 1) {
+        if (job.wide_acc) { VGPU_RB(uint16_t, int64_t) } else { VGPU_RB(uint16_t, int32_t) }
+    } else {
+        VGPU_RB(float, int64_t)
+    }
+#undef VGPU_RB
+}
+
+__global__ __launch_bounds__(kThreads) void k_reduce_job(ReduceJob job) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    reduce_job_block(job, blockIdx.x, blockIdx.y, smem);
+}
+
+__global__ __launch_bounds__(kThreads) void k_reduce_batch(const ReduceJob* jobs, const BatchTask* tasks,
+                                                           int ntasks, const void* const* srcs,
+                                                           void* const* dsts) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const BatchTask task = tasks[t];
+        ReduceJob job = jobs[task.job];
+        if (job.src_slot >= 0) job.in.ptr = srcs[job.src_slot];
+        if (job.dst_slot >= 0) job.out.ptr = dsts[job.dst_slot];
+        reduce_job_block(job, task.bx, task.by, smem);
+        __syncthreads();
     }
 }
 
@@ -679,44 +774,59 @@ unsigned grid_for(size_t n) {
 
 }  // namespace
 
+bool plan_shrink_job(ShrinkJob* job) {
+    const int fh = job->fh, fv = job->fv, bands = job->bands;
+    if (bands < 1 || bands > 4 || fh < 1 || fv < 1) return false;
+    const size_t src_bytes = (size_t)job->in.bw * job->in.bh * bands;
+    const int aligned = ((uintptr_t)job->in.ptr % 16) == 0 || job->src_slot >= 0;
+    const int wb = fh * bands;
+    int tx = 64;
+    while (tx > 1 && tx * wb + 15 > kSpanBytes) tx /= 2;
+    if (!aligned || src_bytes < 16 || tx * wb + 15 > kSpanBytes) return false;
+    // Split the fv source rows of an output row over S wavefronts while the
+    // (row, span) tasks leave room on the chip: about 6 resident wavefronts
+    // per SIMD over 256 CUs.
+    const int64_t tasks = (int64_t)((job->out.w + tx - 1) / tx) * job->out.h;
+    int S = 1;
+    while (S < 4 && 2 * S <= fv && tasks * 2 * S <= 256 * 4 * 6) S *= 2;
+    job->tx = tx;
+    job->S = S;
+    job->small = wb <= 64;
+    job->mh = (uint32_t)((1ull << 32) / (256ull * (uint64_t)fh));
+    job->mv = (uint32_t)((1ull << 32) / (256ull * (uint64_t)fv));
+    const int rows_per_block = (kThreads / 64) / S;
+    job->gx = (job->out.w + tx - 1) / tx;
+    job->gy = (job->out.h + rows_per_block - 1) / rows_per_block;
+    return true;
+}
+
+size_t shrink_job_lds() { return (size_t)(kThreads / 64) * (kSpanBytes + 32); }
+
+hipError_t launch_shrink_job(const ShrinkJob& job, hipStream_t s) {
+    if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_shrink_job, dim3((unsigned)job.gx, (unsigned)job.gy), dim3(kThreads),
+                       shrink_job_lds(), s, job);
+    return hipGetLastError();
+}
+
+hipError_t launch_shrink_batch(const ShrinkJob* jobs, const BatchTask* tasks, int ntasks,
+                               const void* const* srcs, void* const* dsts, int grid, hipStream_t s) {
+    if (ntasks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_shrink_batch, dim3((unsigned)grid), dim3(kThreads), shrink_job_lds(), s, jobs,
+                       tasks, ntasks, srcs, dsts);
+    return hipGetLastError();
+}
+
 hipError_t launch_shrink_hv_u8(const InView& in, const OutView& out, int bands, int fh, int fv,
                                int premult, hipStream_t s) {
     if (out.w <= 0 || out.h <= 0) return hipSuccess;
     const uint32_t mh = (uint32_t)((1ull << 32) / (256ull * (uint64_t)fh));
     const uint32_t mv = (uint32_t)((1ull << 32) / (256ull * (uint64_t)fv));
     const int aligned = ((uintptr_t)in.ptr % 16) == 0;
-    // Wave-per-row kernel: a wavefront's span (TX windows + 16-byte skew)
-    // must fit its LDS row buffer and its prefetch registers.
-    const int wb = fh * bands;
     int tx = 64;
-    while (tx > 1 && tx * wb + 15 > kSpanBytes) tx /= 2;
-    const size_t src_bytes = (size_t)in.bw * in.bh * bands;
-    if (aligned && src_bytes >= 16 && tx * wb + 15 <= kSpanBytes) {
-        const size_t shmem = (size_t)(kThreads / 64) * (kSpanBytes + 32);
-        // Split the fv source rows of an output row over S wavefronts while
-        // the (row, span) tasks leave room on the chip: about 6 resident
-        // wavefronts per SIMD over 256 CUs.
-        const int64_t tasks = (int64_t)((out.w + tx - 1) / tx) * out.h;
-        int S = 1;
-        while (S < 4 && 2 * S <= fv && tasks * 2 * S <= 256 * 4 * 6) S *= 2;
-        const int rows_per_block = (kThreads / 64) / S;
-        dim3 grid((unsigned)((out.w + tx - 1) / tx), (unsigned)((out.h + rows_per_block - 1) / rows_per_block));
-#define VGPU_ROWS(BB, SM) hipLaunchKernelGGL((k_shrink_hv_u8_rows<BB, SM>), grid, dim3(kThreads), shmem, s, in, out, fh, fv, tx, S, premult, mh, mv)
-        const bool small = wb <= 64;
-        switch (bands) {
-        case 1: if (small) VGPU_ROWS(1, true); else VGPU_ROWS(1, false); break;
-        case 2: if (small) VGPU_ROWS(2, true); else VGPU_ROWS(2, false); break;
-        case 3: if (small) VGPU_ROWS(3, true); else VGPU_ROWS(3, false); break;
-        case 4: if (small) VGPU_ROWS(4, true); else VGPU_ROWS(4, false); break;
-        default: return hipErrorInvalidValue;
-        }
-#undef VGPU_ROWS
-        return hipGetLastError();
-    }
-    // Very wide box windows (fh * bands > 4080 bytes) or a source that is not
-    // 16-byte aligned: block-per-row kernel with byte-exact loads.
+    // Block-per-row kernel for very wide box windows (fh * bands > 4080
+    // bytes) or a source that is not 16-byte aligned, with byte-exact loads.
     const size_t budget = 64 * 1024;
-    tx = 64;
     auto pitch_of = [&](int cols) { return ((size_t)cols * fh * bands + 32 + 15) & ~(size_t)15; };
     while (tx > 1 && pitch_of(tx) > budget) tx /= 2;
     const size_t pitch = pitch_of(tx);
@@ -772,30 +882,24 @@ hipError_t launch_reduce_v(const InView& in, const OutView& out, int bands, int 
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_hv(const InView& in, const OutView& out, int bands, int fmt,
-                            const DevPlan& ph, const DevPlan& pv, int tx, int ty, int nr_max,
-                            int vclip0, int vclip1, int wide_acc, hipStream_t s) {
-    if (out.w <= 0 || out.h <= 0) return hipSuccess;
-    if (bands < 1 || bands > 4) return hipErrorInvalidValue;
-    const size_t shmem = (size_t)nr_max * tx * bands * (fmt == 0 ? 1 : (fmt == 1 ? 2 : 4));
-    dim3 grid((unsigned)((out.w + tx - 1) / tx), (unsigned)((out.h + ty - 1) / ty));
-#define VGPU_RHV(TT, BB, AA) hipLaunchKernelGGL((k_reduce_hv<TT, BB, AA>), grid, dim3(kThreads), shmem, s, in, out, ph, pv, tx, ty, nr_max, vclip0, vclip1)
-#define VGPU_RHV_B(TT, AA)                  \
-    switch (bands) {                        \
-    case 1: VGPU_RHV(TT, 1, AA); break;     \
-    case 2: VGPU_RHV(TT, 2, AA); break;     \
-    case 3: VGPU_RHV(TT, 3, AA); break;     \
-    default: VGPU_RHV(TT, 4, AA); break;    \
-    }
-    if (fmt == 0) {
-        if (wide_acc) { VGPU_RHV_B(uint8_t, int64_t) } else { VGPU_RHV_B(uint8_t, int32_t) }
-    } else if (fmt == 1) {
-        if (wide_acc) { VGPU_RHV_B(uint16_t, int64_t) } else { VGPU_RHV_B(uint16_t, int32_t) }
-    } else {
-        VGPU_RHV_B(float, int64_t)
-    }
-#undef VGPU_RHV_B
-#undef VGPU_RHV
+size_t reduce_job_lds(const ReduceJob& job) {
+    return (size_t)job.nr_max * job.tx * job.bands * (job.fmt == 0 ? 1 : (job.fmt == 1 ? 2 : 4));
+}
+
+hipError_t launch_reduce_job(const ReduceJob& job, hipStream_t s) {
+    if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;
+    if (job.bands < 1 || job.bands > 4) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_reduce_job, dim3((unsigned)job.gx, (unsigned)job.gy), dim3(kThreads),
+                       reduce_job_lds(job), s, job);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_batch(const ReduceJob* jobs, const BatchTask* tasks, int ntasks,
+                               const void* const* srcs, void* const* dsts, int grid, size_t lds,
+                               hipStream_t s) {
+    if (ntasks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_batch, dim3((unsigned)grid), dim3(kThreads), lds, s, jobs, tasks,
+                       ntasks, srcs, dsts);
     return hipGetLastError();
 }
 

@@ -37,6 +37,38 @@ struct DevPlan {
 
 enum Alpha : int { ALPHA_NONE = 0, ALPHA_PREMULT = 1, ALPHA_UNPREMULT = 2 };
 
+// Fused [Premultiply] -> ShrinkH -> ShrinkV -> [Unpremultiply] on u8, one
+// image.  plan_shrink_job fills the launch shape (tx, S, small, mh, mv, gx,
+// gy); it returns false when the wave-per-row kernel cannot take the job.
+struct ShrinkJob {
+    InView in;
+    OutView out;
+    int bands = 3, fh = 1, fv = 1, premult = 0, unpremult = 0;
+    int tx = 64, S = 1, small = 1;
+    uint32_t mh = 0, mv = 0;
+    int gx = 0, gy = 0;
+    int src_slot = -1, dst_slot = -1;  // batch: pointer taken from the per-run tables
+};
+
+// Fused ReduceH -> ReduceV -> [Unpremultiply] (bands <= 4); a missing axis is
+// an identity plan (factor 1, one 4096 tap).  ph plans the intermediate's
+// columns (indexed from out.x), pv the output rows; [vclip0, vclip1] is
+// ReduceV's input row range (its reference tile rectangle); nr_max bounds the
+// staged rows; wide_acc selects int64 fixed-point sums.
+struct ReduceJob {
+    InView in;
+    OutView out;
+    DevPlan ph, pv;
+    int bands = 3, fmt = 0, tx = 16, ty = 8, nr_max = 1;
+    int vclip0 = 0, vclip1 = 0, wide_acc = 0, unpremult = 0;
+    int gx = 0, gy = 0;
+    int src_slot = -1, dst_slot = -1;
+};
+
+struct BatchTask {
+    int job, bx, by;
+};
+
 hipError_t launch_shrink_hv_u8(const InView& in, const OutView& out, int bands, int fh, int fv,
                                int premult, hipStream_t s);
 hipError_t launch_shrink_generic(const InView& in, const OutView& out, int bands, int fmt,
@@ -45,13 +77,16 @@ hipError_t launch_reduce_h(const InView& in, const OutView& out, int bands, int 
                            const DevPlan& plan, int post_alpha, hipStream_t s);
 hipError_t launch_reduce_v(const InView& in, const OutView& out, int bands, int fmt,
                            const DevPlan& plan, int post_alpha, hipStream_t s);
-// Fused ReduceH -> ReduceV (bands <= 4): ph plans the ReduceH output columns
-// (indexed from out.x), pv the ReduceV output rows; [vclip0, vclip1] is
-// ReduceV's input row range (its reference tile rectangle).  nr_max bounds
-// the staged rows.  wide_acc selects int64 fixed-point sums (int32 otherwise).
-hipError_t launch_reduce_hv(const InView& in, const OutView& out, int bands, int fmt,
-                            const DevPlan& ph, const DevPlan& pv, int tx, int ty, int nr_max,
-                            int vclip0, int vclip1, int wide_acc, hipStream_t s);
+bool plan_shrink_job(ShrinkJob* job);
+size_t shrink_job_lds();
+hipError_t launch_shrink_job(const ShrinkJob& job, hipStream_t s);
+hipError_t launch_shrink_batch(const ShrinkJob* jobs, const BatchTask* tasks, int ntasks,
+                               const void* const* srcs, void* const* dsts, int grid, hipStream_t s);
+size_t reduce_job_lds(const ReduceJob& job);
+hipError_t launch_reduce_job(const ReduceJob& job, hipStream_t s);
+hipError_t launch_reduce_batch(const ReduceJob* jobs, const BatchTask* tasks, int ntasks,
+                               const void* const* srcs, void* const* dsts, int grid, size_t lds,
+                               hipStream_t s);
 hipError_t launch_conv(const InView& in, const OutView& out, int bands, int in_fmt, int out_fmt,
                        int kind, int vertical, const int16_t* itaps, int32_t iscale,
                        const double* wts, int ntaps, double cscale, double coffset,

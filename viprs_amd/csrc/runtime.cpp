@@ -134,15 +134,14 @@ struct Stage {
     Rect clip;                // reference tile rectangle of the first node's input
     Rect out;                 // output rectangle of the last node
     int in_fmt = U8, out_fmt = U8;
-    int fh = 1, fv = 1, premult = 0;
+    int fh = 1, fv = 1, premult = 0, unpremult = 0;
     size_t plan_start = 0, plan_phase = 0, w16 = 0, w64 = 0;  // blob offsets
     size_t itaps = 0, wts = 0;
     int taps = 1;
-    // fused ReduceH -> ReduceV: the V plan and tiling
+    // fused reduce: the V plan and the job template (pointers set at launch)
     size_t v_start = 0, v_phase = 0, v_w16 = 0, v_w64 = 0;
-    int v_taps = 1, tx = 64, ty = 16, nr_max = 1;
-    int vclip0 = 0, vclip1 = 0;
-    int wide_acc = 0;  // int64 fixed-point sums needed
+    int v_taps = 1;
+    ReduceJob rjob;
 };
 
 Rect whole(uint32_t w, uint32_t h) { return Rect{0, 0, (int64_t)w, (int64_t)h}; }
@@ -175,29 +174,41 @@ OutView outview_of(void* ptr, const Rect& r) {
 // `origin`.  ReduceH on u8 accumulates the f64 source position column by
 // column from the region origin (reduceh.rs:112-119); every other case
 // evaluates the position directly (reduce_simd.rs:63,111,2716-2717).
-void build_plan(const Node& nd, int64_t origin, int64_t n, bool accumulate,
+void build_plan(const ReduceAxis& ax, int64_t origin, int64_t n, bool accumulate,
                 std::vector<int32_t>* starts, std::vector<uint8_t>* phases) {
     starts->resize((size_t)std::max<int64_t>(n, 1));
     phases->resize((size_t)std::max<int64_t>(n, 1));
-    double sx = nd.axis.source_position((double)origin);
+    double sx = ax.source_position((double)origin);
     for (int64_t i = 0; i < n; ++i) {
         int64_t st;
         int ph;
-        const double sp = accumulate ? sx : nd.axis.source_position((double)origin + (double)i);
-        nd.axis.plan(sp, &st, &ph);
+        const double sp = accumulate ? sx : ax.source_position((double)origin + (double)i);
+        ax.plan(sp, &st, &ph);
         (*starts)[(size_t)i] = (int32_t)std::min<int64_t>(std::max<int64_t>(st, INT32_MIN), INT32_MAX);
         (*phases)[(size_t)i] = (uint8_t)ph;
-        sx += nd.axis.factor;
+        sx += ax.factor;
     }
+}
+
+// The identity axis a fused reduce stage uses for a missing ReduceH or
+// ReduceV: factor 1, one tap of weight 1.0 (4096 in fixed point), so the
+// staged intermediate is an exact copy of its input.
+const ReduceAxis& identity_axis() {
+    static const ReduceAxis ax = [] {
+        ReduceAxis a;
+        build_reduce_axis(NEAREST, 1.0, 0, &a);
+        return a;
+    }();
+    return ax;
 }
 
 void stage_reduce_params(const Node& nd, const Rect& out, Staging* st, Stage* s) {
     std::vector<int32_t> starts;
     std::vector<uint8_t> phases;
     if (nd.kind == N_REDUCE_H)
-        build_plan(nd, out.x, out.w, nd.in_fmt == U8, &starts, &phases);
+        build_plan(nd.axis, out.x, out.w, nd.in_fmt == U8, &starts, &phases);
     else
-        build_plan(nd, out.y, out.h, false, &starts, &phases);
+        build_plan(nd.axis, out.y, out.h, false, &starts, &phases);
     s->plan_start = st->put(starts.data(), starts.size() * sizeof(int32_t));
     s->plan_phase = st->put(phases.data(), phases.size());
     s->w16 = st->put(nd.axis.wi16.data(), nd.axis.wi16.size() * sizeof(int16_t));
@@ -205,16 +216,18 @@ void stage_reduce_params(const Node& nd, const Rect& out, Staging* st, Stage* s)
     s->taps = nd.axis.taps;
 }
 
-// Fused ReduceH -> ReduceV stage: H plan over the intermediate's columns, V
-// plan over the output rows, and a tile shape whose staged ReduceH rows fit
-// 64 KiB of LDS.  Returns false when no tile fits (very large factors).
-bool stage_reduce_hv_params(const Node& h, const Node& v, const Rect& mid, const Rect& out,
+// Fused reduce stage: H plan over the intermediate's columns (`mid`), V plan
+// over the output rows, and a tile shape whose staged rows fit 64 KiB of LDS.
+// `accumulate_h` selects the u8 ReduceH plan; an identity axis stands in for
+// a missing pass.  Returns false when no tile fits (very large factors).
+bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const ReduceAxis& vax,
+                            int fmt, uint32_t bands, const Rect& mid, const Rect& out,
                             const Rect& vclip, Staging* st, Stage* s) {
     std::vector<int32_t> hs, vs;
     std::vector<uint8_t> hp, vp;
-    build_plan(h, mid.x, mid.w, h.in_fmt == U8, &hs, &hp);
-    build_plan(v, out.y, out.h, false, &vs, &vp);
-    const size_t esz = format_size(h.out_fmt);
+    build_plan(hax, mid.x, mid.w, accumulate_h, &hs, &hp);
+    build_plan(vax, out.y, out.h, false, &vs, &vp);
+    const size_t esz = format_size(fmt);
     // Tile candidates, largest first: take the first whose staged rows fit
     // 64 KiB of LDS and that still gives >= 1024 blocks; otherwise the fitting
     // candidate with the most blocks.
@@ -225,9 +238,9 @@ bool stage_reduce_hv_params(const Node& h, const Node& v, const Rect& mid, const
         int rows = 1;
         for (int64_t y0 = 0; y0 < out.h; y0 += c[1]) {
             const int64_t y1 = std::min<int64_t>(y0 + c[1], out.h) - 1;
-            rows = std::max<int>(rows, vs[(size_t)y1] + v.axis.taps - vs[(size_t)y0]);
+            rows = std::max<int>(rows, vs[(size_t)y1] + vax.taps - vs[(size_t)y0]);
         }
-        if ((size_t)rows * c[0] * h.bands * esz > 64 * 1024) continue;
+        if ((size_t)rows * c[0] * bands * esz > 64 * 1024) continue;
         const int64_t blocks = ((out.w + c[0] - 1) / c[0]) * ((out.h + c[1] - 1) / c[1]);
         if (blocks > best_blocks) {
             best_blocks = blocks;
@@ -240,30 +253,35 @@ bool stage_reduce_hv_params(const Node& h, const Node& v, const Rect& mid, const
     if (best_blocks < 0) return false;
     s->plan_start = st->put(hs.data(), hs.size() * sizeof(int32_t));
     s->plan_phase = st->put(hp.data(), hp.size());
-    s->w16 = st->put(h.axis.wi16.data(), h.axis.wi16.size() * sizeof(int16_t));
-    s->w64 = st->put(h.axis.wf64.data(), h.axis.wf64.size() * sizeof(double));
-    s->taps = h.axis.taps;
+    s->w16 = st->put(hax.wi16.data(), hax.wi16.size() * sizeof(int16_t));
+    s->w64 = st->put(hax.wf64.data(), hax.wf64.size() * sizeof(double));
+    s->taps = hax.taps;
     s->v_start = st->put(vs.data(), vs.size() * sizeof(int32_t));
     s->v_phase = st->put(vp.data(), vp.size());
-    s->v_w16 = st->put(v.axis.wi16.data(), v.axis.wi16.size() * sizeof(int16_t));
-    s->v_w64 = st->put(v.axis.wf64.data(), v.axis.wf64.size() * sizeof(double));
-    s->v_taps = v.axis.taps;
-    s->tx = tx;
-    s->ty = ty;
-    s->nr_max = nr;
-    s->vclip0 = (int)vclip.y;
-    s->vclip1 = (int)(vclip.y + std::max<int64_t>(vclip.h, 1) - 1);
+    s->v_w16 = st->put(vax.wi16.data(), vax.wi16.size() * sizeof(int16_t));
+    s->v_w64 = st->put(vax.wf64.data(), vax.wf64.size() * sizeof(double));
+    s->v_taps = vax.taps;
+    ReduceJob& j = s->rjob;
+    j.bands = (int)bands;
+    j.fmt = fmt;
+    j.tx = tx;
+    j.ty = ty;
+    j.nr_max = nr;
+    j.vclip0 = (int)vclip.y;
+    j.vclip1 = (int)(vclip.y + std::max<int64_t>(vclip.h, 1) - 1);
+    j.gx = (int)((out.w + tx - 1) / tx);
+    j.gy = (int)((out.h + ty - 1) / ty);
     // int32 sums are exact when max over phases of sum |w| times the largest
     // sample stays below 2^31.
-    const int64_t maxv = h.in_fmt == U8 ? 255 : 65535;
+    const int64_t maxv = fmt == U8 ? 255 : 65535;
     int64_t worst = 0;
-    for (const ReduceAxis* ax : {&h.axis, &v.axis})
+    for (const ReduceAxis* ax : {&hax, &vax})
         for (int p = 0; p <= kPhases; ++p) {
             int64_t sabs = 0;
             for (int t = 0; t < ax->taps; ++t) sabs += std::abs((int64_t)ax->wi16[(size_t)p * ax->taps + t]);
             worst = std::max(worst, sabs * maxv);
         }
-    s->wide_acc = worst >= (int64_t)INT32_MAX - 4096;
+    j.wide_acc = worst >= (int64_t)INT32_MAX - 4096;
     return true;
 }
 
@@ -271,6 +289,36 @@ void stage_conv_params(const Node& nd, Staging* st, Stage* s) {
     s->itaps = st->put(nd.itaps.data(), nd.itaps.size() * sizeof(int16_t));
     s->wts = st->put(nd.conv.data(), nd.conv.size() * sizeof(double));
     s->taps = (int)nd.conv.size();
+}
+
+ShrinkJob shrink_job_of(const Stage& s, const Node& first, const InView& in, const OutView& out) {
+    ShrinkJob j;
+    j.in = in;
+    j.out = out;
+    j.bands = (int)first.bands;
+    j.fh = s.fh;
+    j.fv = s.fv;
+    j.premult = s.premult;
+    j.unpremult = s.unpremult;
+    return j;
+}
+
+ReduceJob reduce_job_of(const Stage& s, const InView& in, const OutView& out, const unsigned char* blob) {
+    ReduceJob j = s.rjob;
+    j.in = in;
+    j.out = out;
+    j.ph.start = reinterpret_cast<const int32_t*>(blob + s.plan_start);
+    j.ph.phase = reinterpret_cast<const uint8_t*>(blob + s.plan_phase);
+    j.ph.wi16 = reinterpret_cast<const int16_t*>(blob + s.w16);
+    j.ph.wf64 = reinterpret_cast<const double*>(blob + s.w64);
+    j.ph.taps = s.taps;
+    j.pv.start = reinterpret_cast<const int32_t*>(blob + s.v_start);
+    j.pv.phase = reinterpret_cast<const uint8_t*>(blob + s.v_phase);
+    j.pv.wi16 = reinterpret_cast<const int16_t*>(blob + s.v_w16);
+    j.pv.wf64 = reinterpret_cast<const double*>(blob + s.v_w64);
+    j.pv.taps = s.v_taps;
+    j.unpremult = s.unpremult;
+    return j;
 }
 
 // Launch one stage reading `in` and writing `dst` (dense over s.out).
@@ -281,7 +329,17 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
     const int B = (int)nd.bands;
     hipError_t e = hipSuccess;
     switch (s.kind) {
-    case ST_SHRINK_HV_U8: e = launch_shrink_hv_u8(in, out, B, s.fh, s.fv, s.premult, stream); break;
+    case ST_SHRINK_HV_U8: {
+        ShrinkJob j = shrink_job_of(s, nd, in, out);
+        if (plan_shrink_job(&j)) {
+            e = launch_shrink_job(j, stream);
+        } else {
+            e = launch_shrink_hv_u8(in, out, B, s.fh, s.fv, s.premult, stream);
+            if (e == hipSuccess && s.unpremult)  // pixel-local, in place
+                e = launch_alpha_u8(view_of(dst, s.out, s.out), out, B, ALPHA_UNPREMULT, stream);
+        }
+        break;
+    }
     case ST_SHRINK:
         e = launch_shrink_generic(in, out, B, nd.in_fmt, (int)nd.factor, nd.kind == N_SHRINK_V, stream);
         break;
@@ -297,22 +355,7 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
                                   : launch_reduce_v(in, out, B, nd.in_fmt, p, 0, stream);
         break;
     }
-    case ST_REDUCE_HV: {
-        DevPlan ph, pv;
-        ph.start = reinterpret_cast<const int32_t*>(blob + s.plan_start);
-        ph.phase = reinterpret_cast<const uint8_t*>(blob + s.plan_phase);
-        ph.wi16 = reinterpret_cast<const int16_t*>(blob + s.w16);
-        ph.wf64 = reinterpret_cast<const double*>(blob + s.w64);
-        ph.taps = s.taps;
-        pv.start = reinterpret_cast<const int32_t*>(blob + s.v_start);
-        pv.phase = reinterpret_cast<const uint8_t*>(blob + s.v_phase);
-        pv.wi16 = reinterpret_cast<const int16_t*>(blob + s.v_w16);
-        pv.wf64 = reinterpret_cast<const double*>(blob + s.v_w64);
-        pv.taps = s.v_taps;
-        e = launch_reduce_hv(in, out, B, nd.in_fmt, ph, pv, s.tx, s.ty, s.nr_max, s.vclip0, s.vclip1,
-                             s.wide_acc, stream);
-        break;
-    }
+    case ST_REDUCE_HV: e = launch_reduce_job(reduce_job_of(s, in, out, blob), stream); break;
     case ST_CONV: {
         const bool gauss = nd.kind == N_GAUSS_H || nd.kind == N_GAUSS_V;
         const bool vert = nd.kind == N_GAUSS_V || nd.kind == N_CONVSEP_V;
@@ -328,6 +371,105 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
     case ST_COPY: e = launch_copy(in, out, B, nd.in_fmt, stream); break;
     }
     return from_hip(e);
+}
+
+// Group a pipeline's nodes into fused stages over the regions of the backward
+// walk; every stage's plan data goes into `st`.
+std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector<Rect>& in_r,
+                               const std::vector<Rect>& out_r, Staging* st) {
+    std::vector<Stage> stages;
+    const int n = (int)nodes.size();
+    auto kind_at = [&](int k) { return k < n ? nodes[(size_t)k].kind : N_COPY; };
+    for (int i = 0; i < n;) {
+        const Node& nd = nodes[(size_t)i];
+        Stage s;
+        s.first = i;
+        s.clip = in_r[(size_t)i];
+        s.in_fmt = nd.in_fmt;
+        const bool small_bands = nd.bands <= 4;
+        const bool u8small = nd.in_fmt == U8 && small_bands;
+        const bool alpha_ok = nd.in_fmt == U8 && (nd.bands == 2 || nd.bands == 4);
+        bool done = false;
+        if (u8small && (nd.kind == N_SHRINK_H || nd.kind == N_SHRINK_V ||
+                        (alpha_ok && nd.kind == N_PREMULT &&
+                         (kind_at(i + 1) == N_SHRINK_H || kind_at(i + 1) == N_SHRINK_V)))) {
+            // [Premultiply] -> ShrinkH -> [ShrinkV] -> [Unpremultiply], or a lone ShrinkV.
+            int j = i;
+            s.kind = ST_SHRINK_HV_U8;
+            if (nodes[(size_t)j].kind == N_PREMULT) {
+                s.premult = 1;
+                ++j;
+            }
+            if (nodes[(size_t)j].kind == N_SHRINK_H) {
+                s.fh = (int)nodes[(size_t)j].factor;
+                if (kind_at(j + 1) == N_SHRINK_V) s.fv = (int)nodes[(size_t)++j].factor;
+            } else {
+                s.fv = (int)nodes[(size_t)j].factor;
+            }
+            if (alpha_ok && kind_at(j + 1) == N_UNPREMULT) {
+                s.unpremult = 1;
+                ++j;
+            }
+            s.last = j;
+            done = true;
+        } else if (small_bands && (nd.kind == N_REDUCE_H || nd.kind == N_REDUCE_V)) {
+            // ReduceH -> ReduceV, a lone ReduceH or a lone ReduceV, -> [Unpremultiply].
+            const bool has_h = nd.kind == N_REDUCE_H;
+            const int vi = has_h ? (kind_at(i + 1) == N_REDUCE_V ? i + 1 : -1) : i;
+            const int last = vi >= 0 ? vi : i;
+            const ReduceAxis& hax = has_h ? nd.axis : identity_axis();
+            const ReduceAxis& vax = vi >= 0 ? nodes[(size_t)vi].axis : identity_axis();
+            const Rect mid = has_h ? out_r[(size_t)i] : in_r[(size_t)i];
+            const Rect out = out_r[(size_t)last];
+            const Rect vclip = vi >= 0 ? in_r[(size_t)vi] : out_r[(size_t)i];
+            if (stage_reduce_hv_params(hax, has_h && nd.in_fmt == U8, vax, nd.in_fmt, nd.bands, mid, out,
+                                       vclip, st, &s)) {
+                s.kind = ST_REDUCE_HV;
+                s.last = last;
+                if (alpha_ok && kind_at(last + 1) == N_UNPREMULT) {
+                    s.unpremult = 1;
+                    s.last = last + 1;
+                }
+                done = true;
+            }
+        }
+        if (!done) {
+            s.last = i;
+            switch (nd.kind) {
+            case N_SHRINK_H:
+            case N_SHRINK_V: s.kind = ST_SHRINK; break;
+            case N_REDUCE_H: s.kind = ST_REDUCE_H; break;
+            case N_REDUCE_V: s.kind = ST_REDUCE_V; break;
+            case N_GAUSS_H:
+            case N_GAUSS_V:
+            case N_CONVSEP_H:
+            case N_CONVSEP_V: s.kind = ST_CONV; break;
+            case N_PREMULT:
+            case N_UNPREMULT: s.kind = ST_ALPHA; break;
+            default: s.kind = ST_COPY; break;
+            }
+        }
+        s.out = out_r[(size_t)s.last];
+        s.out_fmt = nodes[(size_t)s.last].out_fmt;
+        if (s.kind == ST_REDUCE_H || s.kind == ST_REDUCE_V) stage_reduce_params(nd, s.out, st, &s);
+        if (s.kind == ST_CONV) stage_conv_params(nd, st, &s);
+        stages.push_back(s);
+        i = s.last + 1;
+    }
+    return stages;
+}
+
+void backward_regions(const std::vector<Node>& nodes, uint32_t w, uint32_t h, std::vector<Rect>* in_r,
+                      std::vector<Rect>* out_r) {
+    const int n = (int)nodes.size();
+    in_r->assign((size_t)n, Rect{});
+    out_r->assign((size_t)n, Rect{});
+    if (n == 0) return;
+    (*out_r)[(size_t)n - 1] = whole(w, h);
+    for (int i = n - 1; i >= 0; --i) {
+        (*in_r)[(size_t)i] = nodes[(size_t)i].required_input((*out_r)[(size_t)i]);
+        if (i > 0) (*out_r)[(size_t)i - 1] = (*in_r)[(size_t)i];
+    }
 }
 
 }  // namespace
@@ -380,67 +522,10 @@ int vgpu_pipeline::compile() {
         return VGPU_OK;
     }
     // Backward region walk (execute.rs:413-480) from the full output.
-    std::vector<Rect> out_r((size_t)n), in_r((size_t)n);
-    out_r[(size_t)n - 1] = whole(cur_w, cur_h);
-    for (int i = n - 1; i >= 0; --i) {
-        in_r[(size_t)i] = nodes[(size_t)i].required_input(out_r[(size_t)i]);
-        if (i > 0) out_r[(size_t)i - 1] = in_r[(size_t)i];
-    }
+    std::vector<Rect> out_r, in_r;
+    backward_regions(nodes, cur_w, cur_h, &in_r, &out_r);
     Staging st;
-    for (int i = 0; i < n;) {
-        const Node& nd = nodes[(size_t)i];
-        Stage s;
-        s.first = i;
-        s.clip = in_r[(size_t)i];
-        s.in_fmt = nd.in_fmt;
-        const bool u8small = nd.in_fmt == U8 && nd.bands <= 4;
-        if (u8small && (nd.kind == N_SHRINK_H || nd.kind == N_SHRINK_V ||
-                        (nd.kind == N_PREMULT && i + 1 < n && nodes[(size_t)i + 1].kind == N_SHRINK_H))) {
-            // Fused [Premultiply] -> ShrinkH -> [ShrinkV] (or a lone ShrinkV).
-            int j = i;
-            s.kind = ST_SHRINK_HV_U8;
-            if (nodes[(size_t)j].kind == N_PREMULT) {
-                s.premult = 1;
-                ++j;
-            }
-            if (nodes[(size_t)j].kind == N_SHRINK_H) {
-                s.fh = (int)nodes[(size_t)j].factor;
-                if (j + 1 < n && nodes[(size_t)j + 1].kind == N_SHRINK_V) {
-                    ++j;
-                    s.fv = (int)nodes[(size_t)j].factor;
-                }
-            } else {
-                s.fv = (int)nodes[(size_t)j].factor;
-            }
-            s.last = j;
-        } else if (nd.kind == N_REDUCE_H && nd.bands <= 4 && i + 1 < n && nodes[(size_t)i + 1].kind == N_REDUCE_V &&
-                   stage_reduce_hv_params(nd, nodes[(size_t)i + 1], out_r[(size_t)i],
-                                          out_r[(size_t)i + 1], in_r[(size_t)i + 1], &st, &s)) {
-            s.kind = ST_REDUCE_HV;
-            s.last = i + 1;
-        } else {
-            s.last = i;
-            switch (nd.kind) {
-            case N_SHRINK_H:
-            case N_SHRINK_V: s.kind = ST_SHRINK; break;
-            case N_REDUCE_H: s.kind = ST_REDUCE_H; break;
-            case N_REDUCE_V: s.kind = ST_REDUCE_V; break;
-            case N_GAUSS_H:
-            case N_GAUSS_V:
-            case N_CONVSEP_H:
-            case N_CONVSEP_V: s.kind = ST_CONV; break;
-            case N_PREMULT:
-            case N_UNPREMULT: s.kind = ST_ALPHA; break;
-            default: s.kind = ST_COPY; break;
-            }
-        }
-        s.out = out_r[(size_t)s.last];
-        s.out_fmt = nodes[(size_t)s.last].out_fmt;
-        if (s.kind == ST_REDUCE_H || s.kind == ST_REDUCE_V) stage_reduce_params(nd, s.out, &st, &s);
-        if (s.kind == ST_CONV) stage_conv_params(nd, &st, &s);
-        stages.push_back(s);
-        i = s.last + 1;
-    }
+    stages = plan_stages(nodes, in_r, out_r, &st);
     if (!st.bytes.empty()) {
         blob.ctx = ctx;
         blob.ptr = ctx->pool.acquire(st.bytes.size());
