@@ -86,6 +86,8 @@ const char *vgpu_status_string(int status);
  * the device executor keeps for RayonScheduler
  * (crates/viprs-runtime/src/scheduler/rayon_scheduler.rs:53-75). */
 int vgpu_context_create(int device, vgpu_context **out);
+/* Destroy every pipeline, op, batch and buffer made on the context first:
+ * they hold memory from its pool. */
 void vgpu_context_destroy(vgpu_context *ctx);
 /* Use a caller-owned hipStream_t (NULL = the context's own stream). */
 int vgpu_context_set_stream(vgpu_context *ctx, void *hip_stream);
@@ -215,6 +217,57 @@ int vgpu_op_output_format(const vgpu_op *op, int *format);
 int vgpu_op_process_region(vgpu_op *op, const void *input, size_t input_len, void *output,
                            size_t output_len, vgpu_region input_region,
                            vgpu_region output_region);
+
+/* ------------------------------------------------------ batched thumbnails */
+
+/* A batch of independent thumbnails of u8 images (1..4 bands) with one
+ * target.  Each image gets exactly the plan and output that
+ * vgpu_pipeline_thumbnail gives it alone (thumbnail.rs:229-524); the batch
+ * runs all of them as a few persistent launches over device task lists (one
+ * per plan level and kernel family), captured once into a HIP graph.  No
+ * reference counterpart: it replaces `count` separate thumbnail() pipelines
+ * run through TileScheduler::run (crates/viprs-ports/src/scheduler.rs:63-98).
+ * Status VGPU_E_UNSUPPORTED when any image's plan needs the Affine/crop tail. */
+typedef struct vgpu_batch vgpu_batch;
+
+typedef struct {
+    uint32_t images;
+    uint32_t launches;         /* persistent launches per run (batched levels) */
+    uint32_t shrink_tasks;     /* blocks of work in the fused-shrink launches */
+    uint32_t reduce_tasks;     /* blocks of work in the fused-reduce launches */
+    uint32_t fallback_images;  /* images run one stage launch at a time */
+    int32_t grid_cap;          /* persistent grid size, 0 = one block per task */
+    int32_t use_graph;
+    int32_t reduce_first_tile; /* largest reduce tile candidate (0 = 64x16) */
+    uint64_t input_bytes;
+    uint64_t output_bytes;
+    uint64_t intermediate_bytes;
+} vgpu_batch_stats;
+
+int vgpu_batch_thumbnail_new(vgpu_context *ctx, uint32_t count, const uint32_t *widths,
+                             const uint32_t *heights, const uint32_t *bands, int target,
+                             uint32_t width, uint32_t height, int kernel, vgpu_batch **out);
+void vgpu_batch_destroy(vgpu_batch *b);
+int vgpu_batch_size(const vgpu_batch *b);
+int vgpu_batch_output_info(const vgpu_batch *b, uint32_t index, uint32_t *width,
+                           uint32_t *height, uint32_t *bands);
+int vgpu_batch_info(const vgpu_batch *b, vgpu_batch_stats *stats);
+/* Launch shape: persistent grid size (0 = one block per task), HIP graph on
+ * or off, largest reduce tile candidate (re-plans the batch). */
+int vgpu_batch_configure(vgpu_batch *b, int grid_cap, int use_graph, int reduce_first_tile);
+/* Enqueue the whole batch on the context stream.  srcs[i] / dsts[i] are device
+ * pointers (sources 16-byte aligned) with byte lengths src_bytes[i] /
+ * dst_bytes[i]; runs of one batch must stay on one stream. */
+int vgpu_batch_run(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,
+                   void *const *dsts, const size_t *dst_bytes);
+/* Timing harness, as vgpu_pipeline_benchmark. */
+int vgpu_batch_benchmark(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,
+                         void *const *dsts, const size_t *dst_bytes, int warmup, int iterations,
+                         float *total_ms, float *step_ms);
+/* Autotuner: times every (reduce tile, grid size) candidate on these buffers
+ * and keeps the fastest; best_ms (may be NULL) receives its ms per run. */
+int vgpu_batch_tune(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,
+                    void *const *dsts, const size_t *dst_bytes, int iterations, float *best_ms);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,163 @@
+"""Batched thumbnails through the C ABI: every image of a batch must equal the
+oracle's thumbnail of that image alone, bit for bit, whatever the launch shape
+(one block per task or a persistent grid walking the task list, graph on or
+off, any reduce tile), on repeated graph replays and after the caller's
+buffers change."""
+import numpy as np
+import pytest
+
+import viprs_amd as v
+from oracle import oracle as O
+from tests.helpers import noise_u8
+
+pytestmark = pytest.mark.gpu
+
+
+def random_sizes(n, seed, lo=20, hi=900):
+    r = np.random.default_rng(seed)
+    return [(int(r.integers(lo, hi)), int(r.integers(lo, hi)), int(r.integers(1, 5))) for _ in range(n)]
+
+
+def sources(sizes, seed):
+    out = []
+    for i, (w, h, b) in enumerate(sizes):
+        s = noise_u8(h, w, b, seed * 1000 + i)
+        if b in (2, 4):  # some fully transparent pixels
+            s[..., -1] = np.where(np.random.default_rng(i).random((h, w)) < 0.1, 0, s[..., -1])
+        out.append(s)
+    return out
+
+
+def expected(srcs, target, tw, th, kernel=O.LANCZOS3):
+    exp = []
+    for s in srcs:
+        h, w, b = s.shape
+        exp.append(O.Pipeline(w, h, b).thumbnail(target, tw, th, kernel).run(s))
+    return exp
+
+
+def check_outputs(dsts, exp, what):
+    for i, (d, e) in enumerate(zip(dsts, exp)):
+        got = d.numpy()
+        assert got.shape == e.shape, (what, i, got.shape, e.shape)
+        if not np.array_equal(got, e):
+            bad = np.argwhere(got != e)
+            raise AssertionError(f"{what}: image {i} {e.shape}: {len(bad)} mismatches, first {bad[0]}")
+
+
+@pytest.mark.parametrize("target,tw,th", [(v.FITBOX, 150, 150), (v.WIDTH, 97, 0), (v.HEIGHT, 0, 64)])
+def test_batch_matches_oracle(ctx, target, tw, th):
+    sizes = random_sizes(24, 7 + target)
+    srcs = sources(sizes, target)
+    exp = expected(srcs, target, tw, th)
+    batch = v.Batch(ctx, sizes, target, tw, th)
+    info = batch.info()
+    # Only empty plans (images already inside the target) leave the batched launches.
+    empty = sum(1 for (w, h, b) in sizes if not O.thumbnail_plan(w, h, b, target, tw, th)[0])
+    assert info["images"] == 24 and info["fallback_images"] == empty, info
+    dsrc = [v.DeviceBuffer.from_numpy(ctx, s) for s in srcs]
+    dst = batch.output_buffers()
+    for i, d in enumerate(dst):
+        assert d.shape == exp[i].shape
+    batch.run(dsrc, dst)
+    ctx.synchronize()
+    check_outputs(dst, exp, "first run")
+
+
+def test_batch_launch_shapes_and_replays(ctx):
+    sizes = random_sizes(16, 99, 40, 1300) + [(64, 48, 3), (150, 150, 4), (2, 900, 1), (900, 3, 2)]
+    srcs = sources(sizes, 5)
+    exp = expected(srcs, v.FITBOX, 120, 120)
+    batch = v.Batch(ctx, sizes, v.FITBOX, 120, 120)
+    dsrc = [v.DeviceBuffer.from_numpy(ctx, s) for s in srcs]
+    for grid_cap, graph, tile in [(0, True, 0), (0, False, 0), (64, True, 0), (300, True, 2),
+                                  (0, True, 3), (7, False, 1)]:
+        batch.configure(grid_cap, graph, tile)
+        dst = batch.output_buffers()
+        for rep in range(3):  # graph replays, host work and syncs in between
+            for d in dst:
+                d.upload(np.zeros(d.shape, np.uint8))
+            batch.run(dsrc, dst)
+            ctx.synchronize()
+            check_outputs(dst, exp, f"grid {grid_cap} graph {graph} tile {tile} replay {rep}")
+
+
+def test_batch_outputs_equal_single_pipelines(ctx):
+    sizes = [(1031, 777, 3), (640, 1000, 4), (300, 200, 1), (1500, 97, 2)]
+    srcs = sources(sizes, 11)
+    batch = v.Batch(ctx, sizes, v.WIDTH, 200, 0)
+    dsrc = [v.DeviceBuffer.from_numpy(ctx, s) for s in srcs]
+    dst = batch.output_buffers()
+    batch.run(dsrc, dst)
+    ctx.synchronize()
+    for i, ((w, h, b), s) in enumerate(zip(sizes, srcs)):
+        alone = v.Pipeline(ctx, w, h, b).thumbnail(v.WIDTH, 200, 0).run_host(s)
+        assert np.array_equal(dst[i].numpy(), alone), i
+
+
+def test_batch_identity_and_upsize_free_plans(ctx):
+    # Images inside the box keep their size (the thumbnail never upsizes):
+    # their plan is empty and the batch copies them.
+    sizes = [(100, 80, 3), (150, 150, 4), (1, 1, 1), (149, 3, 2)]
+    srcs = sources(sizes, 3)
+    exp = expected(srcs, v.FITBOX, 150, 150)
+    batch = v.Batch(ctx, sizes, v.FITBOX, 150, 150)
+    dsrc = [v.DeviceBuffer.from_numpy(ctx, s) for s in srcs]
+    dst = batch.output_buffers()
+    batch.run(dsrc, dst)
+    ctx.synchronize()
+    check_outputs(dst, exp, "identity")
+
+
+class _View:
+    """A device pointer with an offset, for argument checks (never launched)."""
+
+    def __init__(self, base, offset, nbytes):
+        self._p = base.data_ptr() + offset
+        self.nbytes = nbytes
+
+    def data_ptr(self):
+        return self._p
+
+
+def test_batch_rejects_bad_buffers(ctx):
+    sizes = [(300, 200, 3), (50, 60, 4)]
+    batch = v.Batch(ctx, sizes, v.FITBOX, 40, 40)
+    srcs = [v.DeviceBuffer(ctx, (h, w, b), np.uint8) for (w, h, b) in sizes]
+    dst = batch.output_buffers()
+    with pytest.raises(v.VgpuError) as e:
+        batch.run(srcs, dst[:1] + [v.DeviceBuffer(ctx, (1, 1, 4), np.uint8)])
+    assert e.value.status == v.api.E_BUFFER_SIZE
+    big = v.DeviceBuffer(ctx, (200 * 300 * 3 + 16,), np.uint8)
+    with pytest.raises(v.VgpuError) as e:
+        batch.run([_View(big, 1, 200 * 300 * 3), srcs[1]], dst)
+    assert e.value.status == v.api.E_INVALID_ARGUMENT
+    with pytest.raises(v.VgpuError):
+        v.Batch(ctx, [(10, 10, 5)], v.FITBOX, 4, 4)
+    with pytest.raises(v.VgpuError):
+        v.Batch(ctx, [(10, 10, 3)], v.FITBOX, 4, 4, v.LBB)
+    # The batch still runs after the rejected calls.
+    srcs_np = sources(sizes, 1)
+    for d, s in zip(srcs, srcs_np):
+        d.upload(s)
+    batch.run(srcs, dst)
+    ctx.synchronize()
+    check_outputs(dst, expected(srcs_np, v.FITBOX, 40, 40), "after rejects")
+
+
+def test_batch_tune_keeps_results(ctx):
+    sizes = random_sizes(12, 1234, 100, 1200)
+    srcs = sources(sizes, 8)
+    exp = expected(srcs, v.FITBOX, 100, 100)
+    batch = v.Batch(ctx, sizes, v.FITBOX, 100, 100)
+    dsrc = [v.DeviceBuffer.from_numpy(ctx, s) for s in srcs]
+    dst = batch.output_buffers()
+    ms = batch.tune(dsrc, dst, iterations=2)
+    assert ms > 0
+    info = batch.info()
+    assert info["use_graph"] == 1
+    for d in dst:
+        d.upload(np.zeros(d.shape, np.uint8))
+    batch.run(dsrc, dst)
+    ctx.synchronize()
+    check_outputs(dst, exp, f"tuned {info}")

@@ -26,6 +26,18 @@ class Region(C.Structure):
         return (self.x, self.y, self.width, self.height)
 
 
+class BatchStats(C.Structure):
+    """vgpu_batch_stats."""
+
+    _fields_ = [("images", C.c_uint32), ("launches", C.c_uint32), ("shrink_tasks", C.c_uint32),
+                ("reduce_tasks", C.c_uint32), ("fallback_images", C.c_uint32), ("grid_cap", C.c_int32),
+                ("use_graph", C.c_int32), ("reduce_first_tile", C.c_int32), ("input_bytes", C.c_uint64),
+                ("output_bytes", C.c_uint64), ("intermediate_bytes", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -91,6 +103,15 @@ def load() -> C.CDLL:
         "vgpu_op_output_size": (i32, [vp, u32, u32, P(u32), P(u32)]),
         "vgpu_op_output_format": (i32, [vp, P(i32)]),
         "vgpu_op_process_region": (i32, [vp, vp, sz, vp, sz, Region, Region]),
+        "vgpu_batch_thumbnail_new": (i32, [vp, u32, P(u32), P(u32), P(u32), i32, u32, u32, i32, P(vp)]),
+        "vgpu_batch_destroy": (None, [vp]),
+        "vgpu_batch_size": (i32, [vp]),
+        "vgpu_batch_output_info": (i32, [vp, u32, P(u32), P(u32), P(u32)]),
+        "vgpu_batch_info": (i32, [vp, P(BatchStats)]),
+        "vgpu_batch_configure": (i32, [vp, i32, i32, i32]),
+        "vgpu_batch_run": (i32, [vp, P(vp), P(sz), P(vp), P(sz)]),
+        "vgpu_batch_benchmark": (i32, [vp, P(vp), P(sz), P(vp), P(sz), i32, i32, P(f32), P(f32)]),
+        "vgpu_batch_tune": (i32, [vp, P(vp), P(sz), P(vp), P(sz), i32, P(f32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

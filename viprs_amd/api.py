@@ -13,6 +13,7 @@ numpy arrays.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -58,6 +59,12 @@ class Context:
         _check(self._lib.vgpu_context_create(device, C.byref(h)), "vgpu_context_create")
         self._h = h
         self.device = device
+        # Pipelines, ops, batches and buffers made on this context: released
+        # before the context itself, whatever order Python collects them in.
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj) -> None:
+        self._children.add(obj)
 
     @property
     def handle(self):
@@ -81,6 +88,8 @@ class Context:
 
     def close(self) -> None:
         if getattr(self, "_h", None):
+            for child in list(getattr(self, "_children", ())):
+                child._release()
             self._lib.vgpu_context_destroy(self._h)
             self._h = None
 
@@ -110,6 +119,7 @@ class DeviceBuffer:
         p = C.c_void_p()
         _check(ctx._lib.vgpu_buffer_alloc(ctx.handle, self.nbytes, C.byref(p)), "vgpu_buffer_alloc")
         self._p = p
+        ctx._adopt(self)
 
     @classmethod
     def from_numpy(cls, ctx: Context, arr: np.ndarray) -> "DeviceBuffer":
@@ -139,6 +149,8 @@ class DeviceBuffer:
             self.ctx._lib.vgpu_buffer_free(self.ctx.handle, self._p)
         self._p = None
 
+    _release = free
+
     def __del__(self):
         try:
             self.free()
@@ -156,13 +168,17 @@ class Pipeline:
         _check(self._lib.vgpu_pipeline_new(ctx.handle, width, height, bands, fmt, C.byref(h)),
                "vgpu_pipeline_new")
         self._h = h
+        ctx._adopt(self)
         self.src_shape = (height, width, bands)
         self.src_fmt = fmt
 
-    def __del__(self):
+    def _release(self) -> None:
         if getattr(self, "_h", None):
             self._lib.vgpu_pipeline_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        self._release()
 
     # -- builder methods (each returns self, like PipelineBuilder's chaining)
     def reduce_h(self, factor: float, kernel: int = LANCZOS3):
@@ -265,6 +281,87 @@ class Pipeline:
         return dst
 
 
+class Batch:
+    """Batched thumbnails: independent u8 images (1..4 bands), one target.
+
+    Every output equals what ``Pipeline(...).thumbnail(...)`` gives the image
+    alone; the batch runs all images as a few persistent launches (one per plan
+    level and kernel family) captured once into a HIP graph.
+    """
+
+    def __init__(self, ctx: Context, sizes, target: int, width: int, height: int = 0,
+                 kernel: int = LANCZOS3):
+        self._lib = _native.load()
+        self.ctx = ctx
+        self._h = None
+        n = len(sizes)
+        ws = (C.c_uint32 * max(n, 1))(*[int(s[0]) for s in sizes])
+        hs = (C.c_uint32 * max(n, 1))(*[int(s[1]) for s in sizes])
+        bs = (C.c_uint32 * max(n, 1))(*[int(s[2]) for s in sizes])
+        h = C.c_void_p()
+        _check(self._lib.vgpu_batch_thumbnail_new(ctx.handle, n, ws, hs, bs, target, width, height,
+                                                  kernel, C.byref(h)), "vgpu_batch_thumbnail_new")
+        self._h = h
+        ctx._adopt(self)
+        self.sizes = [tuple(int(v) for v in s) for s in sizes]
+
+    def _release(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.vgpu_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self._release()
+
+    def __len__(self) -> int:
+        return len(self.sizes)
+
+    def output_shape(self, i: int):
+        w, h, b = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(self._lib.vgpu_batch_output_info(self._h, i, C.byref(w), C.byref(h), C.byref(b)),
+               "vgpu_batch_output_info")
+        return (h.value, w.value, b.value)
+
+    def output_buffers(self):
+        return [DeviceBuffer(self.ctx, self.output_shape(i), np.uint8) for i in range(len(self))]
+
+    def info(self) -> dict:
+        st = _native.BatchStats()
+        _check(self._lib.vgpu_batch_info(self._h, C.byref(st)), "vgpu_batch_info")
+        return st.as_dict()
+
+    def configure(self, grid_cap: int = 0, use_graph: bool = True, reduce_first_tile: int = 0) -> None:
+        _check(self._lib.vgpu_batch_configure(self._h, grid_cap, int(use_graph), reduce_first_tile),
+               "vgpu_batch_configure")
+
+    def _tables(self, srcs, dsts):
+        n = len(self)
+        if len(srcs) != n or len(dsts) != n:
+            raise ValueError(f"batch of {n} images needs {n} sources and outputs")
+        sp = (C.c_void_p * max(n, 1))(*[_ptr(s) for s in srcs])
+        sb = (C.c_size_t * max(n, 1))(*[_nbytes(s) for s in srcs])
+        dp = (C.c_void_p * max(n, 1))(*[_ptr(d) for d in dsts])
+        db = (C.c_size_t * max(n, 1))(*[_nbytes(d) for d in dsts])
+        return sp, sb, dp, db
+
+    def run(self, srcs, dsts) -> None:
+        """Enqueue the batch on the context stream (device buffers)."""
+        _check(self._lib.vgpu_batch_run(self._h, *self._tables(srcs, dsts)), "vgpu_batch_run")
+
+    def benchmark(self, srcs, dsts, warmup: int, iterations: int):
+        total = C.c_float()
+        steps = (C.c_float * iterations)()
+        _check(self._lib.vgpu_batch_benchmark(self._h, *self._tables(srcs, dsts), warmup, iterations,
+                                              C.byref(total), steps), "vgpu_batch_benchmark")
+        return total.value / iterations, np.array(steps[:], np.float64)
+
+    def tune(self, srcs, dsts, iterations: int = 3) -> float:
+        best = C.c_float()
+        _check(self._lib.vgpu_batch_tune(self._h, *self._tables(srcs, dsts), iterations, C.byref(best)),
+               "vgpu_batch_tune")
+        return best.value
+
+
 class Op:
     """One DynOperation bridge executed on the GPU (dyn_process_region)."""
 
@@ -272,16 +369,20 @@ class Op:
         self.ctx = ctx
         self._lib = ctx._lib
         self._h = handle
+        ctx._adopt(self)
         self.bands = bands
         self.in_fmt = in_fmt
         f = C.c_int()
         _check(self._lib.vgpu_op_output_format(self._h, C.byref(f)), "output_format")
         self.out_fmt = f.value
 
-    def __del__(self):
+    def _release(self) -> None:
         if getattr(self, "_h", None):
             self._lib.vgpu_op_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        self._release()
 
     @classmethod
     def _make(cls, ctx, fn, args, bands, fmt, what):

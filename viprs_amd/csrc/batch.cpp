@@ -1,0 +1,495 @@
+// Batch executor: many independent images, each with its own plan, run as a
+// handful of persistent launches over device task lists instead of one launch
+// sequence per image.
+//
+// Every image is planned exactly as a single pipeline would plan it (the
+// reference's thumbnail(), crates/viprs-ops-spatial/src/resample/
+// thumbnail.rs:229-524, one CompiledPipeline per image), so each output is the
+// one the image would get alone.  Stage k of every image forms "level" k; a
+// level runs as one persistent launch per kernel family (fused shrink, fused
+// reduce) whose blocks walk a task list of (job, bx, by) entries covering all
+// images of that level.  Images whose plan needs another stage kind run their
+// stages one launch at a time after the batched levels.
+//
+// Level-0 jobs read the caller's sources and last-stage jobs write the
+// caller's outputs through two device slot tables refreshed per run (only
+// when the pointers change), so the launch sequence is fixed for the life of
+// the batch and is captured once into a HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "runtime.hpp"
+
+using namespace vgpu;
+
+namespace {
+
+struct BatchImage {
+    uint32_t w = 0, h = 0, bands = 0;
+    uint32_t ow = 0, oh = 0;
+    std::vector<Node> nodes;
+    std::vector<Stage> stages;
+    std::vector<size_t> inter_off;  // arena offsets of every stage output but the last
+    bool batched = true;
+};
+
+// One persistent launch over the tasks of one (level, kernel family).
+struct Launch {
+    int family = 0;  // 0: fused shrink, 1: fused reduce
+    size_t jobs_off = 0, tasks_off = 0;
+    int ntasks = 0;
+    size_t lds = 0;
+};
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+struct vgpu_batch {
+    vgpu_context* ctx = nullptr;
+    std::vector<BatchImage> images;
+    StageOptions opt;
+    DeviceBlob blob;           // plans and tables of every image
+    void* arena = nullptr;     // stage intermediates
+    size_t arena_bytes = 0;
+    void* tables = nullptr;    // jobs, tasks and the two slot tables
+    size_t slots_off = 0;      // srcs[count] then dsts[count] inside `tables`
+    void* host_slots = nullptr;  // pinned staging of the slot tables
+    hipEvent_t slots_read = nullptr;
+    std::vector<const void*> cur_srcs;
+    std::vector<void*> cur_dsts;
+    bool slots_valid = false;
+    std::vector<Launch> launches;
+    std::vector<int> fallback;  // images run stage by stage
+    int n_shrink_tasks = 0, n_reduce_tasks = 0;
+    int grid_cap = 0;    // persistent grid size; 0 = one block per task
+    int use_graph = 1;
+    hipGraphExec_t graph = nullptr;
+
+    ~vgpu_batch() { release(); }
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+    }
+    void release() {
+        drop_graph();
+        if (ctx) {
+            ctx->pool.release(arena);
+            ctx->pool.release(tables);
+        }
+        arena = tables = nullptr;
+        blob.reset();
+        if (host_slots) (void)hipHostFree(host_slots);
+        host_slots = nullptr;
+        if (slots_read) (void)hipEventDestroy(slots_read);
+        slots_read = nullptr;
+        slots_valid = false;
+        launches.clear();
+        fallback.clear();
+    }
+    int plan();
+    int launch_all(hipStream_t s);
+};
+
+// Plan every image's stages with the batch options, lay out intermediates,
+// build the job arrays and task lists and upload them.
+int vgpu_batch::plan() {
+    release();
+    const size_t count = images.size();
+    Staging st;
+    size_t arena_need = 0;
+    int levels = 0;
+    for (BatchImage& im : images) {
+        im.stages.clear();
+        im.inter_off.clear();
+        im.batched = true;
+        if (im.nodes.empty()) continue;
+        std::vector<Rect> in_r, out_r;
+        backward_regions(im.nodes, im.ow, im.oh, &in_r, &out_r);
+        im.stages = plan_stages(im.nodes, in_r, out_r, &st, opt);
+        for (size_t k = 0; k + 1 < im.stages.size(); ++k) {
+            const Stage& s = im.stages[k];
+            im.inter_off.push_back(arena_need);
+            arena_need = align_up(arena_need + (size_t)s.out.w * s.out.h * im.bands * format_size(s.out_fmt), 256);
+        }
+        for (const Stage& s : im.stages)
+            if (s.kind != ST_SHRINK_HV_U8 && s.kind != ST_REDUCE_HV) im.batched = false;
+        levels = std::max(levels, (int)im.stages.size());
+    }
+    if (!st.bytes.empty()) {
+        blob.ctx = ctx;
+        blob.ptr = ctx->pool.acquire(st.bytes.size());
+        if (!blob.ptr) return VGPU_E_NO_MEMORY;
+        if (hipMemcpy(blob.ptr, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return VGPU_E_DEVICE;
+    }
+    arena_bytes = arena_need;
+    if (arena_need) {
+        arena = ctx->pool.acquire(arena_need);
+        if (!arena) return VGPU_E_NO_MEMORY;
+    }
+    const unsigned char* dblob = static_cast<const unsigned char*>(blob.ptr);
+    unsigned char* darena = static_cast<unsigned char*>(arena);
+
+    // Images in decreasing input size, so the biggest tasks start first.
+    std::vector<int> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (uint64_t)images[(size_t)a].w * images[(size_t)a].h * images[(size_t)a].bands >
+               (uint64_t)images[(size_t)b].w * images[(size_t)b].h * images[(size_t)b].bands;
+    });
+
+    Staging tbl;
+    n_shrink_tasks = n_reduce_tasks = 0;
+    for (int lv = 0; lv < levels; ++lv) {
+        std::vector<ShrinkJob> sjobs;
+        std::vector<ReduceJob> rjobs;
+        std::vector<BatchTask> stasks, rtasks;
+        size_t rlds = 0;
+        for (int idx : order) {
+            BatchImage& im = images[(size_t)idx];
+            if (!im.batched || lv >= (int)im.stages.size()) continue;
+            const Stage& s = im.stages[(size_t)lv];
+            const bool first = lv == 0, last = lv + 1 == (int)im.stages.size();
+            const Rect in_rect = first ? whole(im.w, im.h) : im.stages[(size_t)lv - 1].out;
+            const void* in_ptr = first ? nullptr : darena + im.inter_off[(size_t)lv - 1];
+            void* out_ptr = last ? nullptr : darena + im.inter_off[(size_t)lv];
+            const InView in = view_of(in_ptr, in_rect, s.clip);
+            const OutView out = outview_of(out_ptr, s.out);
+            if (s.kind == ST_SHRINK_HV_U8) {
+                ShrinkJob j = shrink_job_of(s, im.nodes[(size_t)s.first], in, out);
+                j.src_slot = first ? idx : -1;
+                j.dst_slot = last ? idx : -1;
+                if (!plan_shrink_job(&j, opt.shrink_row_split)) {
+                    im.batched = false;
+                    continue;
+                }
+                const int jid = (int)sjobs.size();
+                sjobs.push_back(j);
+                for (int by = 0; by < j.gy; ++by)
+                    for (int bx = 0; bx < j.gx; ++bx) stasks.push_back(BatchTask{jid, bx, by});
+            } else {
+                ReduceJob j = reduce_job_of(s, in, out, dblob);
+                j.src_slot = first ? idx : -1;
+                j.dst_slot = last ? idx : -1;
+                const int jid = (int)rjobs.size();
+                rjobs.push_back(j);
+                rlds = std::max(rlds, reduce_job_lds(j));
+                for (int by = 0; by < j.gy; ++by)
+                    for (int bx = 0; bx < j.gx; ++bx) rtasks.push_back(BatchTask{jid, bx, by});
+            }
+        }
+        if (!stasks.empty()) {
+            Launch l;
+            l.family = 0;
+            l.jobs_off = tbl.put(sjobs.data(), sjobs.size() * sizeof(ShrinkJob));
+            l.tasks_off = tbl.put(stasks.data(), stasks.size() * sizeof(BatchTask));
+            l.ntasks = (int)stasks.size();
+            l.lds = shrink_job_lds();
+            launches.push_back(l);
+            n_shrink_tasks += l.ntasks;
+        }
+        if (!rtasks.empty()) {
+            Launch l;
+            l.family = 1;
+            l.jobs_off = tbl.put(rjobs.data(), rjobs.size() * sizeof(ReduceJob));
+            l.tasks_off = tbl.put(rtasks.data(), rtasks.size() * sizeof(BatchTask));
+            l.ntasks = (int)rtasks.size();
+            l.lds = rlds;
+            launches.push_back(l);
+            n_reduce_tasks += l.ntasks;
+        }
+    }
+    // An image whose shrink job could not be planned runs stage by stage
+    // after the batched levels (any earlier-level job it already has only
+    // writes its own intermediate, which the stage-by-stage run rewrites).
+    for (size_t i = 0; i < count; ++i)
+        if (!images[i].batched || images[i].stages.empty()) fallback.push_back((int)i);
+
+    slots_off = tbl.put(nullptr, 0);
+    slots_off = align_up(slots_off, 16);
+    tbl.bytes.resize(slots_off + 2 * count * sizeof(void*), 0);
+    tables = ctx->pool.acquire(tbl.bytes.size());
+    if (!tables) return VGPU_E_NO_MEMORY;
+    if (hipMemcpy(tables, tbl.bytes.data(), tbl.bytes.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return VGPU_E_DEVICE;
+    if (hipHostMalloc(&host_slots, std::max<size_t>(2 * count * sizeof(void*), 16), hipHostMallocDefault) !=
+        hipSuccess) {
+        host_slots = nullptr;
+        return VGPU_E_NO_MEMORY;
+    }
+    if (hipEventCreateWithFlags(&slots_read, hipEventDisableTiming) != hipSuccess) {
+        slots_read = nullptr;
+        return VGPU_E_DEVICE;
+    }
+    return VGPU_OK;
+}
+
+int vgpu_batch::launch_all(hipStream_t s) {
+    unsigned char* t = static_cast<unsigned char*>(tables);
+    const void* const* srcs = reinterpret_cast<const void* const*>(t + slots_off);
+    void* const* dsts = reinterpret_cast<void* const*>(t + slots_off + images.size() * sizeof(void*));
+    for (const Launch& l : launches) {
+        const int grid = grid_cap > 0 ? std::min(l.ntasks, grid_cap) : l.ntasks;
+        const BatchTask* tasks = reinterpret_cast<const BatchTask*>(t + l.tasks_off);
+        hipError_t e;
+        if (l.family == 0)
+            e = launch_shrink_batch(reinterpret_cast<const ShrinkJob*>(t + l.jobs_off), tasks, l.ntasks, srcs, dsts,
+                                    grid, s);
+        else
+            e = launch_reduce_batch(reinterpret_cast<const ReduceJob*>(t + l.jobs_off), tasks, l.ntasks, srcs, dsts,
+                                    grid, l.lds, s);
+        if (e != hipSuccess) return VGPU_E_DEVICE;
+    }
+    return VGPU_OK;
+}
+
+namespace {
+
+int run_fallback(vgpu_batch* b, int idx, const void* src, void* dst, hipStream_t s) {
+    BatchImage& im = b->images[(size_t)idx];
+    const size_t out_bytes = (size_t)im.ow * im.oh * im.bands;
+    if (im.stages.empty()) return from_hip(hipMemcpyAsync(dst, src, out_bytes, hipMemcpyDeviceToDevice, s));
+    const unsigned char* dblob = static_cast<const unsigned char*>(b->blob.ptr);
+    unsigned char* darena = static_cast<unsigned char*>(b->arena);
+    const void* cur = src;
+    Rect cur_rect = whole(im.w, im.h);
+    for (size_t k = 0; k < im.stages.size(); ++k) {
+        const Stage& st = im.stages[k];
+        void* o = (k + 1 == im.stages.size()) ? dst : darena + im.inter_off[k];
+        const int rc = launch_stage(im.nodes, st, view_of(cur, cur_rect, st.clip), o, dblob, s);
+        if (rc != VGPU_OK) return rc;
+        cur = o;
+        cur_rect = st.out;
+    }
+    return VGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vgpu_batch_thumbnail_new(vgpu_context* ctx, uint32_t count, const uint32_t* widths,
+                             const uint32_t* heights, const uint32_t* bands, int target,
+                             uint32_t width, uint32_t height, int kernel, vgpu_batch** out) {
+    if (!ctx || !out || (count && (!widths || !heights || !bands))) return VGPU_E_INVALID_ARGUMENT;
+    *out = nullptr;
+    (void)hipSetDevice(ctx->device);
+    std::unique_ptr<vgpu_batch> b(new (std::nothrow) vgpu_batch());
+    if (!b) return VGPU_E_NO_MEMORY;
+    b->ctx = ctx;
+    b->opt.reduce_min_blocks = 1;  // many images: the largest tiles that fit
+    b->opt.shrink_row_split = false;
+    b->images.resize(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (bands[i] < 1 || bands[i] > 4 || widths[i] == 0 || heights[i] == 0) return VGPU_E_INVALID_ARGUMENT;
+        vgpu_pipeline* p = nullptr;
+        int rc = vgpu_pipeline_new(ctx, widths[i], heights[i], bands[i], U8, &p);
+        if (rc != VGPU_OK) return rc;
+        rc = vgpu_pipeline_thumbnail(p, target, width, height, kernel);
+        if (rc != VGPU_OK) {
+            vgpu_pipeline_destroy(p);
+            return rc;
+        }
+        BatchImage& im = b->images[i];
+        im.w = widths[i];
+        im.h = heights[i];
+        im.bands = bands[i];
+        im.ow = p->cur_w;
+        im.oh = p->cur_h;
+        im.nodes = p->nodes;
+        vgpu_pipeline_destroy(p);
+    }
+    const int rc = b->plan();
+    if (rc != VGPU_OK) return rc;
+    *out = b.release();
+    return VGPU_OK;
+}
+
+void vgpu_batch_destroy(vgpu_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+}
+
+int vgpu_batch_size(const vgpu_batch* b) { return b ? (int)b->images.size() : VGPU_E_INVALID_ARGUMENT; }
+
+int vgpu_batch_output_info(const vgpu_batch* b, uint32_t index, uint32_t* width, uint32_t* height,
+                           uint32_t* bands) {
+    if (!b || index >= b->images.size()) return VGPU_E_INVALID_ARGUMENT;
+    const BatchImage& im = b->images[index];
+    if (width) *width = im.ow;
+    if (height) *height = im.oh;
+    if (bands) *bands = im.bands;
+    return VGPU_OK;
+}
+
+int vgpu_batch_info(const vgpu_batch* b, vgpu_batch_stats* st) {
+    if (!b || !st) return VGPU_E_INVALID_ARGUMENT;
+    std::memset(st, 0, sizeof(*st));
+    st->images = (uint32_t)b->images.size();
+    st->launches = (uint32_t)b->launches.size();
+    st->shrink_tasks = (uint32_t)b->n_shrink_tasks;
+    st->reduce_tasks = (uint32_t)b->n_reduce_tasks;
+    st->fallback_images = (uint32_t)b->fallback.size();
+    st->intermediate_bytes = b->arena_bytes;
+    for (const BatchImage& im : b->images) {
+        st->input_bytes += (uint64_t)im.w * im.h * im.bands;
+        st->output_bytes += (uint64_t)im.ow * im.oh * im.bands;
+    }
+    st->grid_cap = b->grid_cap;
+    st->use_graph = b->use_graph;
+    st->reduce_first_tile = b->opt.reduce_first_tile;
+    return VGPU_OK;
+}
+
+int vgpu_batch_configure(vgpu_batch* b, int grid_cap, int use_graph, int reduce_first_tile) {
+    if (!b || grid_cap < 0 || reduce_first_tile < 0) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(b->ctx->device);
+    if (hipStreamSynchronize(b->ctx->stream) != hipSuccess) return b->ctx->sticky = VGPU_E_DEVICE;
+    b->grid_cap = grid_cap;
+    b->use_graph = use_graph ? 1 : 0;
+    b->drop_graph();
+    if (reduce_first_tile != b->opt.reduce_first_tile) {
+        b->opt.reduce_first_tile = reduce_first_tile;
+        return b->plan();
+    }
+    return VGPU_OK;
+}
+
+int vgpu_batch_run(vgpu_batch* b, const void* const* srcs, const size_t* src_bytes, void* const* dsts,
+                   const size_t* dst_bytes) {
+    if (!b) return VGPU_E_INVALID_ARGUMENT;
+    vgpu_context* ctx = b->ctx;
+    if (ctx->sticky != VGPU_OK) return ctx->sticky;
+    const size_t count = b->images.size();
+    if (count == 0) return VGPU_OK;
+    if (!srcs || !dsts || !src_bytes || !dst_bytes) return VGPU_E_INVALID_ARGUMENT;
+    for (size_t i = 0; i < count; ++i) {
+        const BatchImage& im = b->images[i];
+        if (src_bytes[i] != (size_t)im.w * im.h * im.bands || dst_bytes[i] != (size_t)im.ow * im.oh * im.bands)
+            return VGPU_E_BUFFER_SIZE;
+        // The shrink kernel's 16-byte row loads need 16-byte aligned sources.
+        if (!srcs[i] || !dsts[i] || ((uintptr_t)srcs[i] & 15)) return VGPU_E_INVALID_ARGUMENT;
+    }
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    bool same = b->slots_valid;
+    for (size_t i = 0; same && i < count; ++i) same = b->cur_srcs[i] == srcs[i] && b->cur_dsts[i] == dsts[i];
+    if (!same) {
+        // The staging may still feed the previous upload: wait for it first.
+        if (b->slots_valid && hipEventSynchronize(b->slots_read) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
+        void** hs = static_cast<void**>(b->host_slots);
+        for (size_t i = 0; i < count; ++i) {
+            hs[i] = const_cast<void*>(srcs[i]);
+            hs[count + i] = dsts[i];
+        }
+        unsigned char* t = static_cast<unsigned char*>(b->tables);
+        if (hipMemcpyAsync(t + b->slots_off, hs, 2 * count * sizeof(void*), hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipEventRecord(b->slots_read, s) != hipSuccess)
+            return ctx->sticky = VGPU_E_DEVICE;
+        b->cur_srcs.assign(srcs, srcs + count);
+        b->cur_dsts.assign(dsts, dsts + count);
+        b->slots_valid = true;
+    }
+    int rc = VGPU_OK;
+    if (b->use_graph && !b->launches.empty()) {
+        if (!b->graph) {
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
+            rc = b->launch_all(s);
+            const hipError_t e = hipStreamEndCapture(s, &g);
+            if (rc != VGPU_OK || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return ctx->sticky = VGPU_E_DEVICE;
+            }
+            const hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                b->graph = nullptr;
+                return ctx->sticky = VGPU_E_DEVICE;
+            }
+        }
+        if (hipGraphLaunch(b->graph, s) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
+    } else {
+        rc = b->launch_all(s);
+    }
+    for (int idx : b->fallback) {
+        if (rc != VGPU_OK) break;
+        rc = run_fallback(b, idx, srcs[(size_t)idx], dsts[(size_t)idx], s);
+    }
+    if (rc != VGPU_OK) ctx->sticky = rc;
+    return rc;
+}
+
+int vgpu_batch_benchmark(vgpu_batch* b, const void* const* srcs, const size_t* src_bytes, void* const* dsts,
+                         const size_t* dst_bytes, int warmup, int iterations, float* total_ms, float* step_ms) {
+    if (!b || iterations < 1 || !total_ms) return VGPU_E_INVALID_ARGUMENT;
+    vgpu_context* ctx = b->ctx;
+    int rc = VGPU_OK;
+    for (int i = 0; i < warmup && rc == VGPU_OK; ++i) rc = vgpu_batch_run(b, srcs, src_bytes, dsts, dst_bytes);
+    if (rc != VGPU_OK) return rc;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
+    hipEvent_t a, e;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&e) != hipSuccess) return VGPU_E_DEVICE;
+    (void)hipEventRecord(a, ctx->stream);
+    for (int i = 0; i < iterations && rc == VGPU_OK; ++i) rc = vgpu_batch_run(b, srcs, src_bytes, dsts, dst_bytes);
+    (void)hipEventRecord(e, ctx->stream);
+    if (rc == VGPU_OK && hipEventSynchronize(e) != hipSuccess) rc = VGPU_E_DEVICE;
+    if (rc == VGPU_OK) (void)hipEventElapsedTime(total_ms, a, e);
+    if (rc == VGPU_OK && step_ms) {
+        std::vector<hipEvent_t> ev((size_t)iterations + 1);
+        for (auto& x : ev) (void)hipEventCreate(&x);
+        (void)hipEventRecord(ev[0], ctx->stream);
+        for (int i = 0; i < iterations && rc == VGPU_OK; ++i) {
+            rc = vgpu_batch_run(b, srcs, src_bytes, dsts, dst_bytes);
+            (void)hipEventRecord(ev[(size_t)i + 1], ctx->stream);
+        }
+        if (rc == VGPU_OK && hipEventSynchronize(ev[(size_t)iterations]) != hipSuccess) rc = VGPU_E_DEVICE;
+        for (int i = 0; i < iterations && rc == VGPU_OK; ++i)
+            (void)hipEventElapsedTime(&step_ms[i], ev[(size_t)i], ev[(size_t)i + 1]);
+        for (auto& x : ev) (void)hipEventDestroy(x);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(e);
+    return rc;
+}
+
+int vgpu_batch_tune(vgpu_batch* b, const void* const* srcs, const size_t* src_bytes, void* const* dsts,
+                    const size_t* dst_bytes, int iterations, float* best_ms) {
+    if (!b || iterations < 1) return VGPU_E_INVALID_ARGUMENT;
+    // Candidates: reduce tile start x persistent grid size (one block per task,
+    // or 4/8/16 blocks per CU walking the list), all launched from a graph.
+    static const int tiles[] = {0, 1, 2, 3};
+    static const int grids[] = {0, 256 * 4, 256 * 8, 256 * 16};
+    float best = 0.f;
+    int best_tile = b->opt.reduce_first_tile, best_grid = b->grid_cap;
+    bool have = false;
+    for (int t : tiles) {
+        for (int g : grids) {
+            int rc = vgpu_batch_configure(b, g, 1, t);
+            if (rc != VGPU_OK) return rc;
+            float ms = 0.f;
+            rc = vgpu_batch_benchmark(b, srcs, src_bytes, dsts, dst_bytes, 1, iterations, &ms, nullptr);
+            if (rc != VGPU_OK) return rc;
+            if (!have || ms < best) {
+                have = true;
+                best = ms;
+                best_tile = t;
+                best_grid = g;
+            }
+        }
+    }
+    const int rc = vgpu_batch_configure(b, best_grid, 1, best_tile);
+    if (best_ms) *best_ms = best / (float)iterations;
+    return rc;
+}
+
+}  // extern "C"

@@ -600,11 +600,24 @@ __device__ __forceinline__ void reduce_hv_block(const ReduceJob& job, int bx, in
                 Acc acc[B];
 #pragma unroll
                 for (int b = 0; b < B; ++b) acc[b] = 0;
-                for (int t = 0; t < ph.taps; ++t) {
-                    const T* px = src + (size_t)col_of(in, hs + t) * B;
-                    const Acc wt = w[t];
+                if (sizeof(T) == 1 && (B == 2 || B == 4) && job.premult) {
+                    // Premultiply on load (pixel-local, so it commutes with the
+                    // clamped source read).
+                    for (int t = 0; t < ph.taps; ++t) {
+                        const T* px = src + (size_t)col_of(in, hs + t) * B;
+                        const Acc wt = w[t];
+                        const uint32_t a = px[B - 1];
 #pragma unroll
-                    for (int b = 0; b < B; ++b) acc[b] += wt * (Acc)px[b];
+                        for (int b = 0; b < B - 1; ++b) acc[b] += wt * (Acc)premult_u8(px[b], a);
+                        acc[B - 1] += wt * (Acc)a;
+                    }
+                } else {
+                    for (int t = 0; t < ph.taps; ++t) {
+                        const T* px = src + (size_t)col_of(in, hs + t) * B;
+                        const Acc wt = w[t];
+#pragma unroll
+                        for (int b = 0; b < B; ++b) acc[b] += wt * (Acc)px[b];
+                    }
                 }
 #pragma unroll
                 for (int b = 0; b < B; ++b) dst[b] = fixed_round<T, Acc>(acc[b]);
@@ -774,7 +787,7 @@ unsigned grid_for(size_t n) {
 
 }  // namespace
 
-bool plan_shrink_job(ShrinkJob* job) {
+bool plan_shrink_job(ShrinkJob* job, bool row_split) {
     const int fh = job->fh, fv = job->fv, bands = job->bands;
     if (bands < 1 || bands > 4 || fh < 1 || fv < 1) return false;
     const size_t src_bytes = (size_t)job->in.bw * job->in.bh * bands;
@@ -788,7 +801,7 @@ bool plan_shrink_job(ShrinkJob* job) {
     // per SIMD over 256 CUs.
     const int64_t tasks = (int64_t)((job->out.w + tx - 1) / tx) * job->out.h;
     int S = 1;
-    while (S < 4 && 2 * S <= fv && tasks * 2 * S <= 256 * 4 * 6) S *= 2;
+    while (row_split && S < 4 && 2 * S <= fv && tasks * 2 * S <= 256 * 4 * 6) S *= 2;
     job->tx = tx;
     job->S = S;
     job->small = wb <= 64;

@@ -50,7 +50,8 @@ struct ShrinkJob {
     int src_slot = -1, dst_slot = -1;  // batch: pointer taken from the per-run tables
 };
 
-// Fused ReduceH -> ReduceV -> [Unpremultiply] (bands <= 4); a missing axis is
+// Fused [Premultiply] -> ReduceH -> ReduceV -> [Unpremultiply] (bands <= 4;
+// the alpha steps on u8 with 2 or 4 bands only); a missing axis is
 // an identity plan (factor 1, one 4096 tap).  ph plans the intermediate's
 // columns (indexed from out.x), pv the output rows; [vclip0, vclip1] is
 // ReduceV's input row range (its reference tile rectangle); nr_max bounds the
@@ -60,7 +61,7 @@ struct ReduceJob {
     OutView out;
     DevPlan ph, pv;
     int bands = 3, fmt = 0, tx = 16, ty = 8, nr_max = 1;
-    int vclip0 = 0, vclip1 = 0, wide_acc = 0, unpremult = 0;
+    int vclip0 = 0, vclip1 = 0, wide_acc = 0, premult = 0, unpremult = 0;
     int gx = 0, gy = 0;
     int src_slot = -1, dst_slot = -1;
 };
@@ -77,7 +78,7 @@ hipError_t launch_reduce_h(const InView& in, const OutView& out, int bands, int 
                            const DevPlan& plan, int post_alpha, hipStream_t s);
 hipError_t launch_reduce_v(const InView& in, const OutView& out, int bands, int fmt,
                            const DevPlan& plan, int post_alpha, hipStream_t s);
-bool plan_shrink_job(ShrinkJob* job);
+bool plan_shrink_job(ShrinkJob* job, bool row_split = true);
 size_t shrink_job_lds();
 hipError_t launch_shrink_job(const ShrinkJob& job, hipStream_t s);
 hipError_t launch_shrink_batch(const ShrinkJob* jobs, const BatchTask* tasks, int ntasks,

@@ -19,130 +19,11 @@
 #include <new>
 #include <vector>
 
-#include "../../include/viprs_amd.h"
-#include "kernels.hpp"
-#include "plan.hpp"
+#include "runtime.hpp"
 
 using namespace vgpu;
 
-namespace {
-
-// ------------------------------------------------------------ buffer pool
-
-// Size-bucketed cache of device allocations: hipMalloc only on a miss, never
-// on the steady-state run path.
-class BufferPool {
-  public:
-    ~BufferPool() { trim(true); }
-    void* acquire(size_t bytes) {
-        bytes = round_up(bytes);
-        std::lock_guard<std::mutex> g(mu_);
-        auto it = free_.lower_bound(bytes);
-        if (it != free_.end() && it->first <= 2 * bytes) {
-            void* p = it->second;
-            free_.erase(it);
-            return p;
-        }
-        void* p = nullptr;
-        if (hipMalloc(&p, bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            return nullptr;
-        }
-        sizes_[p] = bytes;
-        held_ += bytes;
-        return p;
-    }
-    void release(void* p) {
-        if (!p) return;
-        std::lock_guard<std::mutex> g(mu_);
-        auto it = sizes_.find(p);
-        if (it != sizes_.end()) free_.emplace(it->second, p);
-    }
-    void trim(bool all) {
-        std::lock_guard<std::mutex> g(mu_);
-        for (auto& kv : free_) {
-            (void)hipFree(kv.second);
-            held_ -= sizes_[kv.second];
-            sizes_.erase(kv.second);
-        }
-        free_.clear();
-        if (all) {
-            for (auto& kv : sizes_) (void)hipFree(kv.first);
-            sizes_.clear();
-            held_ = 0;
-        }
-    }
-    size_t held() const { return held_; }
-
-  private:
-    static size_t round_up(size_t b) { return (std::max<size_t>(b, 1) + 4095) & ~(size_t)4095; }
-    std::mutex mu_;
-    std::multimap<size_t, void*> free_;
-    std::map<void*, size_t> sizes_;
-    size_t held_ = 0;
-};
-
-int from_hip(hipError_t e) { return e == hipSuccess ? VGPU_OK : VGPU_E_DEVICE; }
-
-}  // namespace
-
-struct vgpu_context {
-    int device = 0;
-    hipStream_t own = nullptr;
-    hipStream_t stream = nullptr;
-    BufferPool pool;
-    int sticky = VGPU_OK;
-};
-
-namespace {
-
-struct DeviceBlob {
-    void* ptr = nullptr;
-    vgpu_context* ctx = nullptr;
-    ~DeviceBlob() { reset(); }
-    void reset() {
-        if (ptr && ctx) ctx->pool.release(ptr);
-        ptr = nullptr;
-    }
-};
-
-// Host staging for per-node plans and tables, uploaded as one blob.
-struct Staging {
-    std::vector<unsigned char> bytes;
-    size_t put(const void* p, size_t n) {
-        size_t off = (bytes.size() + 15) & ~(size_t)15;
-        bytes.resize(off + n);
-        if (n) std::memcpy(bytes.data() + off, p, n);
-        return off;
-    }
-};
-
-enum StageKind {
-    ST_SHRINK_HV_U8,
-    ST_SHRINK,
-    ST_REDUCE_H,
-    ST_REDUCE_V,
-    ST_REDUCE_HV,
-    ST_CONV,
-    ST_ALPHA,
-    ST_COPY,
-};
-
-struct Stage {
-    StageKind kind = ST_COPY;
-    int first = 0, last = 0;  // node index range [first, last]
-    Rect clip;                // reference tile rectangle of the first node's input
-    Rect out;                 // output rectangle of the last node
-    int in_fmt = U8, out_fmt = U8;
-    int fh = 1, fv = 1, premult = 0, unpremult = 0;
-    size_t plan_start = 0, plan_phase = 0, w16 = 0, w64 = 0;  // blob offsets
-    size_t itaps = 0, wts = 0;
-    int taps = 1;
-    // fused reduce: the V plan and the job template (pointers set at launch)
-    size_t v_start = 0, v_phase = 0, v_w16 = 0, v_w64 = 0;
-    int v_taps = 1;
-    ReduceJob rjob;
-};
+namespace vgpu {
 
 Rect whole(uint32_t w, uint32_t h) { return Rect{0, 0, (int64_t)w, (int64_t)h}; }
 
@@ -169,6 +50,8 @@ OutView outview_of(void* ptr, const Rect& r) {
     o.h = (int32_t)r.h;
     return o;
 }
+
+namespace {
 
 // Per-axis (start, phase) plan for a reduce node producing `n` outputs from
 // `origin`.  ReduceH on u8 accumulates the f64 source position column by
@@ -222,19 +105,22 @@ void stage_reduce_params(const Node& nd, const Rect& out, Staging* st, Stage* s)
 // a missing pass.  Returns false when no tile fits (very large factors).
 bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const ReduceAxis& vax,
                             int fmt, uint32_t bands, const Rect& mid, const Rect& out,
-                            const Rect& vclip, Staging* st, Stage* s) {
+                            const Rect& vclip, const StageOptions& opt, Staging* st, Stage* s) {
     std::vector<int32_t> hs, vs;
     std::vector<uint8_t> hp, vp;
     build_plan(hax, mid.x, mid.w, accumulate_h, &hs, &hp);
     build_plan(vax, out.y, out.h, false, &vs, &vp);
     const size_t esz = format_size(fmt);
     // Tile candidates, largest first: take the first whose staged rows fit
-    // 64 KiB of LDS and that still gives >= 1024 blocks; otherwise the fitting
-    // candidate with the most blocks.
+    // 64 KiB of LDS and that still gives >= opt.reduce_min_blocks blocks;
+    // otherwise the fitting candidate with the most blocks.  The search starts
+    // at candidate opt.reduce_first_tile.
     static const int cand[][2] = {{64, 16}, {32, 16}, {32, 8}, {16, 8}, {16, 4}, {8, 4}, {8, 2}, {8, 1}, {4, 1}, {1, 1}};
     int tx = 0, ty = 0, nr = 0;
     int64_t best_blocks = -1;
-    for (const auto& c : cand) {
+    const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
+    for (int ci = std::min(std::max(opt.reduce_first_tile, 0), ncand - 1); ci < ncand; ++ci) {
+        const int* c = cand[ci];
         int rows = 1;
         for (int64_t y0 = 0; y0 < out.h; y0 += c[1]) {
             const int64_t y1 = std::min<int64_t>(y0 + c[1], out.h) - 1;
@@ -248,7 +134,7 @@ bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const Redu
             ty = c[1];
             nr = rows;
         }
-        if (blocks >= 1024) break;
+        if (blocks >= opt.reduce_min_blocks) break;
     }
     if (best_blocks < 0) return false;
     s->plan_start = st->put(hs.data(), hs.size() * sizeof(int32_t));
@@ -291,6 +177,8 @@ void stage_conv_params(const Node& nd, Staging* st, Stage* s) {
     s->taps = (int)nd.conv.size();
 }
 
+}  // namespace
+
 ShrinkJob shrink_job_of(const Stage& s, const Node& first, const InView& in, const OutView& out) {
     ShrinkJob j;
     j.in = in;
@@ -298,6 +186,7 @@ ShrinkJob shrink_job_of(const Stage& s, const Node& first, const InView& in, con
     j.bands = (int)first.bands;
     j.fh = s.fh;
     j.fv = s.fv;
+    j.premult = s.premult;
     j.premult = s.premult;
     j.unpremult = s.unpremult;
     return j;
@@ -317,13 +206,14 @@ ReduceJob reduce_job_of(const Stage& s, const InView& in, const OutView& out, co
     j.pv.wi16 = reinterpret_cast<const int16_t*>(blob + s.v_w16);
     j.pv.wf64 = reinterpret_cast<const double*>(blob + s.v_w64);
     j.pv.taps = s.v_taps;
+    j.premult = s.premult;
     j.unpremult = s.unpremult;
     return j;
 }
 
 // Launch one stage reading `in` and writing `dst` (dense over s.out).
 int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& in, void* dst,
-                 const unsigned char* blob, hipStream_t stream) {
+                 const unsigned char* blob, hipStream_t stream, const StageOptions& opt) {
     const Node& nd = nodes[(size_t)s.first];
     const OutView out = outview_of(dst, s.out);
     const int B = (int)nd.bands;
@@ -331,7 +221,7 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
     switch (s.kind) {
     case ST_SHRINK_HV_U8: {
         ShrinkJob j = shrink_job_of(s, nd, in, out);
-        if (plan_shrink_job(&j)) {
+        if (plan_shrink_job(&j, opt.shrink_row_split)) {
             e = launch_shrink_job(j, stream);
         } else {
             e = launch_shrink_hv_u8(in, out, B, s.fh, s.fv, s.premult, stream);
@@ -376,7 +266,7 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
 // Group a pipeline's nodes into fused stages over the regions of the backward
 // walk; every stage's plan data goes into `st`.
 std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector<Rect>& in_r,
-                               const std::vector<Rect>& out_r, Staging* st) {
+                               const std::vector<Rect>& out_r, Staging* st, const StageOptions& opt) {
     std::vector<Stage> stages;
     const int n = (int)nodes.size();
     auto kind_at = [&](int k) { return k < n ? nodes[(size_t)k].kind : N_COPY; };
@@ -412,19 +302,26 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
             }
             s.last = j;
             done = true;
-        } else if (small_bands && (nd.kind == N_REDUCE_H || nd.kind == N_REDUCE_V)) {
-            // ReduceH -> ReduceV, a lone ReduceH or a lone ReduceV, -> [Unpremultiply].
-            const bool has_h = nd.kind == N_REDUCE_H;
-            const int vi = has_h ? (kind_at(i + 1) == N_REDUCE_V ? i + 1 : -1) : i;
-            const int last = vi >= 0 ? vi : i;
-            const ReduceAxis& hax = has_h ? nd.axis : identity_axis();
+        } else if (small_bands && (nd.kind == N_REDUCE_H || nd.kind == N_REDUCE_V ||
+                                   (alpha_ok && nd.kind == N_PREMULT &&
+                                    (kind_at(i + 1) == N_REDUCE_H || kind_at(i + 1) == N_REDUCE_V)))) {
+            // [Premultiply] -> ReduceH -> ReduceV, a lone ReduceH or a lone
+            // ReduceV, -> [Unpremultiply].  Premultiply is pixel-local, so its
+            // region equals the first reduce node's input region.
+            const int r0 = nd.kind == N_PREMULT ? i + 1 : i;
+            const Node& rn = nodes[(size_t)r0];
+            const bool has_h = rn.kind == N_REDUCE_H;
+            const int vi = has_h ? (kind_at(r0 + 1) == N_REDUCE_V ? r0 + 1 : -1) : r0;
+            const int last = vi >= 0 ? vi : r0;
+            const ReduceAxis& hax = has_h ? rn.axis : identity_axis();
             const ReduceAxis& vax = vi >= 0 ? nodes[(size_t)vi].axis : identity_axis();
-            const Rect mid = has_h ? out_r[(size_t)i] : in_r[(size_t)i];
+            const Rect mid = has_h ? out_r[(size_t)r0] : in_r[(size_t)r0];
             const Rect out = out_r[(size_t)last];
-            const Rect vclip = vi >= 0 ? in_r[(size_t)vi] : out_r[(size_t)i];
-            if (stage_reduce_hv_params(hax, has_h && nd.in_fmt == U8, vax, nd.in_fmt, nd.bands, mid, out,
-                                       vclip, st, &s)) {
+            const Rect vclip = vi >= 0 ? in_r[(size_t)vi] : out_r[(size_t)r0];
+            if (stage_reduce_hv_params(hax, has_h && rn.in_fmt == U8, vax, rn.in_fmt, rn.bands, mid, out,
+                                       vclip, opt, st, &s)) {
                 s.kind = ST_REDUCE_HV;
+                s.premult = nd.kind == N_PREMULT ? 1 : 0;
                 s.last = last;
                 if (alpha_ok && kind_at(last + 1) == N_UNPREMULT) {
                     s.unpremult = 1;
@@ -472,47 +369,7 @@ void backward_regions(const std::vector<Node>& nodes, uint32_t w, uint32_t h, st
     }
 }
 
-}  // namespace
-
-struct vgpu_pipeline {
-    vgpu_context* ctx = nullptr;
-    uint32_t src_w = 0, src_h = 0, bands = 0;
-    int src_fmt = U8;
-    uint32_t cur_w = 0, cur_h = 0;
-    int cur_fmt = U8;
-    std::vector<Node> nodes;
-
-    bool compiled = false;
-    std::vector<Stage> stages;
-    DeviceBlob blob;
-    std::vector<void*> inter;  // stage outputs except the last
-
-    ~vgpu_pipeline() { release(); }
-    void release() {
-        for (void* p : inter) ctx->pool.release(p);
-        inter.clear();
-        blob.reset();
-        stages.clear();
-        compiled = false;
-    }
-    Node& push(NodeKind k) {
-        release();
-        Node nd;
-        nd.kind = k;
-        nd.in_fmt = nd.out_fmt = cur_fmt;
-        nd.bands = bands;
-        nd.in_w = nd.out_w = cur_w;
-        nd.in_h = nd.out_h = cur_h;
-        nodes.push_back(nd);
-        return nodes.back();
-    }
-    void commit() {
-        cur_w = nodes.back().out_w;
-        cur_h = nodes.back().out_h;
-        cur_fmt = nodes.back().out_fmt;
-    }
-    int compile();
-};
+}  // namespace vgpu
 
 int vgpu_pipeline::compile() {
     if (compiled) return VGPU_OK;

@@ -1,0 +1,195 @@
+// Internal declarations shared by the pipeline executor (runtime.cpp) and
+// the batch executor (batch.cpp): context, buffer pool, fused stages.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/viprs_amd.h"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace vgpu {
+
+// Size-bucketed cache of device allocations: hipMalloc only on a miss, never
+// on the steady-state run path.
+class BufferPool {
+  public:
+    ~BufferPool() { trim(true); }
+    void* acquire(size_t bytes) {
+        bytes = round_up(bytes);
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = free_.lower_bound(bytes);
+        if (it != free_.end() && it->first <= 2 * bytes) {
+            void* p = it->second;
+            free_.erase(it);
+            return p;
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        sizes_[p] = bytes;
+        held_ += bytes;
+        return p;
+    }
+    void release(void* p) {
+        if (!p) return;
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = sizes_.find(p);
+        if (it != sizes_.end()) free_.emplace(it->second, p);
+    }
+    void trim(bool all) {
+        std::lock_guard<std::mutex> g(mu_);
+        for (auto& kv : free_) {
+            (void)hipFree(kv.second);
+            held_ -= sizes_[kv.second];
+            sizes_.erase(kv.second);
+        }
+        free_.clear();
+        if (all) {
+            for (auto& kv : sizes_) (void)hipFree(kv.first);
+            sizes_.clear();
+            held_ = 0;
+        }
+    }
+    size_t held() const { return held_; }
+
+  private:
+    static size_t round_up(size_t b) { return (std::max<size_t>(b, 1) + 4095) & ~(size_t)4095; }
+    std::mutex mu_;
+    std::multimap<size_t, void*> free_;
+    std::map<void*, size_t> sizes_;
+    size_t held_ = 0;
+};
+
+inline int from_hip(hipError_t e) { return e == hipSuccess ? VGPU_OK : VGPU_E_DEVICE; }
+
+}  // namespace vgpu
+
+struct vgpu_context {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    vgpu::BufferPool pool;
+    int sticky = VGPU_OK;
+};
+
+namespace vgpu {
+
+struct DeviceBlob {
+    void* ptr = nullptr;
+    vgpu_context* ctx = nullptr;
+    ~DeviceBlob() { reset(); }
+    void reset() {
+        if (ptr && ctx) ctx->pool.release(ptr);
+        ptr = nullptr;
+    }
+};
+
+// Host staging for per-node plans and tables, uploaded as one blob.
+struct Staging {
+    std::vector<unsigned char> bytes;
+    size_t put(const void* p, size_t n) {
+        size_t off = (bytes.size() + 15) & ~(size_t)15;
+        bytes.resize(off + n);
+        if (n) std::memcpy(bytes.data() + off, p, n);
+        return off;
+    }
+};
+
+enum StageKind {
+    ST_SHRINK_HV_U8,
+    ST_SHRINK,
+    ST_REDUCE_H,
+    ST_REDUCE_V,
+    ST_REDUCE_HV,
+    ST_CONV,
+    ST_ALPHA,
+    ST_COPY,
+};
+
+struct Stage {
+    StageKind kind = ST_COPY;
+    int first = 0, last = 0;  // node index range [first, last]
+    Rect clip;                // reference tile rectangle of the first node's input
+    Rect out;                 // output rectangle of the last node
+    int in_fmt = U8, out_fmt = U8;
+    int fh = 1, fv = 1, premult = 0, unpremult = 0;
+    size_t plan_start = 0, plan_phase = 0, w16 = 0, w64 = 0;  // blob offsets
+    size_t itaps = 0, wts = 0;
+    int taps = 1;
+    // fused reduce: the V plan and the job template (pointers set at launch)
+    size_t v_start = 0, v_phase = 0, v_w16 = 0, v_w64 = 0;
+    int v_taps = 1;
+    ReduceJob rjob;
+};
+
+// Stage-planning choices that depend on how the stage is launched: a single
+// image wants enough blocks to fill the chip from one stage; a batch gets its
+// parallelism from many images and prefers the largest tiles.
+struct StageOptions {
+    int64_t reduce_min_blocks = 1024;  // stop shrinking reduce tiles once this many blocks exist
+    int reduce_first_tile = 0;         // largest reduce tile candidate considered (0 = 64x16)
+    bool shrink_row_split = true;      // split a shrink row's source rows over wavefronts
+};
+
+Rect whole(uint32_t w, uint32_t h);
+InView view_of(const void* ptr, const Rect& buf, const Rect& clip);
+OutView outview_of(void* ptr, const Rect& r);
+void backward_regions(const std::vector<Node>& nodes, uint32_t w, uint32_t h, std::vector<Rect>* in_r,
+                      std::vector<Rect>* out_r);
+std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector<Rect>& in_r,
+                               const std::vector<Rect>& out_r, Staging* st,
+                               const StageOptions& opt = StageOptions());
+ShrinkJob shrink_job_of(const Stage& s, const Node& first, const InView& in, const OutView& out);
+ReduceJob reduce_job_of(const Stage& s, const InView& in, const OutView& out, const unsigned char* blob);
+int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& in, void* dst,
+                 const unsigned char* blob, hipStream_t stream, const StageOptions& opt = StageOptions());
+
+}  // namespace vgpu
+
+struct vgpu_pipeline {
+    vgpu_context* ctx = nullptr;
+    uint32_t src_w = 0, src_h = 0, bands = 0;
+    int src_fmt = vgpu::U8;
+    uint32_t cur_w = 0, cur_h = 0;
+    int cur_fmt = vgpu::U8;
+    std::vector<vgpu::Node> nodes;
+
+    bool compiled = false;
+    std::vector<vgpu::Stage> stages;
+    vgpu::DeviceBlob blob;
+    std::vector<void*> inter;  // stage outputs except the last
+
+    ~vgpu_pipeline() { release(); }
+    void release() {
+        for (void* p : inter) ctx->pool.release(p);
+        inter.clear();
+        blob.reset();
+        stages.clear();
+        compiled = false;
+    }
+    vgpu::Node& push(vgpu::NodeKind k) {
+        release();
+        vgpu::Node nd;
+        nd.kind = k;
+        nd.in_fmt = nd.out_fmt = cur_fmt;
+        nd.bands = bands;
+        nd.in_w = nd.out_w = cur_w;
+        nd.in_h = nd.out_h = cur_h;
+        nodes.push_back(nd);
+        return nodes.back();
+    }
+    void commit() {
+        cur_w = nodes.back().out_w;
+        cur_h = nodes.back().out_h;
+        cur_fmt = nodes.back().out_fmt;
+    }
+    int compile();
+};
