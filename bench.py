@@ -1,7 +1,7 @@
 """Benchmark: Lanczos3 thumbnail of an 8192x8192 3-band uchar image to 400 px
-wide on one MI355X (input Mpix/s, % of HBM peak, p99 step latency), plus the
-4096x4096 4-band uchar gauss_blur(sigma=2) of the same configuration as a
-secondary line item.
+wide on one MI355X (input Mpix/s, % of HBM peak, p99 step latency), plus, as
+line items of the same JSON object, the 4096x4096 4-band uchar
+gauss_blur(sigma=2) and the 512-image uneven-size batched thumbnail.
 
 One step = one full pipeline run over one image already resident in HBM.
 Prints one JSON line.
@@ -70,6 +70,67 @@ def cpu_baseline(O, src_np, steps_fn, budget_s):
     return (w * h * n / 1e6) / dt, threads, n
 
 
+def bench_batch(v, O, ctx, n, steps, warmup, check_images):
+    """Uneven-size batch: n uchar images, W and H ~ U{256..6000}, 3 or 4 bands
+    (p = 1/2 each), seeded splitmix64 pixels made in HBM, FitBox(400, 400)
+    Lanczos3, all images in one batch run (a few persistent launches from a
+    HIP graph).  One step = the whole batch."""
+    r = np.random.default_rng(4004)
+    sizes = [(int(r.integers(256, 6001)), int(r.integers(256, 6001)), int(r.choice([3, 4])))
+             for _ in range(n)]
+    srcs = []
+    for i, (w, h, b) in enumerate(sizes):
+        d = v.DeviceBuffer(ctx, (h, w, b), np.uint8)
+        d.fill_random(7_000_000 + i)
+        srcs.append(d)
+    batch = v.Batch(ctx, sizes, v.FITBOX, 400, 400, v.LANCZOS3)
+    dsts = batch.output_buffers()
+    tuned_ms = batch.tune(srcs, dsts, iterations=2)
+    info = batch.info()
+    ms, lat = batch.benchmark(srcs, dsts, warmup, steps)
+    launches = batch.run_profiled(srcs, dsts, 3)
+    ctx.synchronize()
+    # Bit-exactness on a sample of images (host regenerates their pixels).
+    exact = True
+    for i in range(min(check_images, n)):
+        w, h, b = sizes[i]
+        host = O.synthetic_bytes(w * h * b, 7_000_000 + i).reshape(h, w, b)
+        exp = O.Pipeline(w, h, b).thumbnail(O.FITBOX, 400, 400).run(host)
+        exact = exact and bool(np.array_equal(dsts[i].numpy(), exp))
+    # The same images one pipeline at a time (the single-image fused kernels,
+    # one launch sequence per image): sum of per-image device times.
+    per_image_ms = 0.0
+    for i, (w, h, b) in enumerate(sizes):
+        p = v.Pipeline(ctx, w, h, b).thumbnail(v.FITBOX, 400, 400, v.LANCZOS3)
+        pm, _ = p.benchmark(srcs[i], dsts[i], 1, 3)
+        per_image_ms += pm
+        del p
+    mpix = sum(w * h for (w, h, _) in sizes) / 1e6
+    alg = info["input_bytes"] + info["output_bytes"]
+    dom = max(launches, key=lambda x: x[0])
+    return {
+        "workload": f"batch of {n} uchar images, W,H ~ U{{256..6000}}, 3|4 bands, FitBox(400,400) Lanczos3",
+        "bit_exact_sample": exact, "checked_images": min(check_images, n),
+        "ms_per_batch": round(ms, 4), "Mpix/s": round(mpix / (ms / 1e3), 1),
+        "hbm_frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "algorithmic_bytes": alg,
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                       "p99": round(float(np.percentile(lat, 99)), 4)},
+        "per_image_kernels": {"ms_sum": round(per_image_ms, 4),
+                              "Mpix/s": round(mpix / (per_image_ms / 1e3), 1),
+                              "speedup_of_batch": round(per_image_ms / ms, 3)},
+        "launches": [{"ms": round(lm, 4), "bytes": lb, "GB/s": round(lb / (lm / 1e3) / 1e9, 1)}
+                     for lm, lb in launches],
+        "roofline": {"kernel": "k_shrink_batch" if dom is launches[0] else "k_reduce_batch", "bound": "hbm",
+                     "achieved": round(dom[1] / (dom[0] / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(dom[1] / (dom[0] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "tuned": {"grid_cap": info["grid_cap"], "reduce_first_tile": info["reduce_first_tile"],
+                  "use_graph": info["use_graph"], "ms": round(tuned_ms, 4)},
+        "tasks": {"shrink": info["shrink_tasks"], "reduce": info["reduce_tasks"],
+                  "fallback_images": info["fallback_images"]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +139,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--batch-images", type=int, default=512)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -160,12 +223,16 @@ def main():
         gexact = bool(np.array_equal(gd_dst.numpy(), O.Pipeline(4096, 4096, 4).gauss_blur(2.0).run(gs)))
         gms, glat = time_pipeline(gp, gd_src, gd_dst, args.steps, args.warmup)
         gbytes = 2 * gs.nbytes
-        result["secondary"] = {
+        result["gauss_blur"] = {
             "workload": "gauss_blur sigma=2.0 on 4096x4096x4 uchar", "bit_exact": gexact,
             "ms_per_step": round(gms, 5), "Mpix/s": round(4096 * 4096 / 1e6 / (gms / 1e3), 1),
             "hbm_frac": round(gbytes / (gms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
             "stage_ms": [round(x, 5) for x in gp.run_profiled(gd_src, gd_dst, 10)],
         }
+
+    if not args.no_batch and args.batch_images > 0:
+        print("bench: uneven-size batch", file=sys.stderr, flush=True)
+        result["batch"] = bench_batch(v, O, ctx, args.batch_images, max(args.steps, 20), args.warmup, 8)
 
     ctx.synchronize()
     print(json.dumps(result), flush=True)

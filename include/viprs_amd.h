@@ -107,6 +107,12 @@ int vgpu_buffer_alloc(vgpu_context *ctx, size_t bytes, void **out);
 int vgpu_buffer_free(vgpu_context *ctx, void *ptr);
 int vgpu_memcpy_h2d(vgpu_context *ctx, void *dst_device, const void *src_host, size_t bytes);
 int vgpu_memcpy_d2h(vgpu_context *ctx, void *dst_host, const void *src_device, size_t bytes);
+/* Seeded synthetic bytes on the device, enqueued on the context stream: byte
+ * 8i+k is byte k (little-endian) of splitmix64(seed + (i+1)*0x9E3779B97F4A7C15).
+ * No reference counterpart (the reference's benches seed inputs on the host);
+ * it lets multi-GB benchmark inputs be made in HBM and regenerated on the
+ * host for checking. */
+int vgpu_fill_random_u8(vgpu_context *ctx, void *dst_device, size_t bytes, uint64_t seed);
 
 /* ----------------------------------------------------- pipeline (builder) */
 
@@ -260,6 +266,13 @@ int vgpu_batch_configure(vgpu_batch *b, int grid_cap, int use_graph, int reduce_
  * dst_bytes[i]; runs of one batch must stay on one stream. */
 int vgpu_batch_run(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,
                    void *const *dsts, const size_t *dst_bytes);
+/* Runs the batched launches (no graph, no stage-by-stage images)
+ * `iterations` times with HIP events between launches; writes the mean ms and
+ * the algorithmic bytes (input buffers + outputs of its jobs) of each launch.
+ * Returns the launch count or a negative status. */
+int vgpu_batch_run_profiled(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,
+                            void *const *dsts, const size_t *dst_bytes, int iterations,
+                            float *launch_ms, uint64_t *launch_bytes, int max_launches);
 /* Timing harness, as vgpu_pipeline_benchmark. */
 int vgpu_batch_benchmark(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,
                          void *const *dsts, const size_t *dst_bytes, int warmup, int iterations,

@@ -236,3 +236,19 @@ def f64_gauss(src: np.ndarray, sigma: float, threads: int = 0):
     _check(lib().orc_f64_gauss(_ptr(src), w, h, b, sigma, _ptr(dst), threads or (os.cpu_count() or 1)),
            "f64_gauss")
     return dst
+
+
+def synthetic_bytes(nbytes: int, seed: int) -> np.ndarray:
+    """Seeded synthetic bytes, the host restatement of the generator the
+    benchmarks fill device inputs with: 8-byte word i is
+    splitmix64(seed + (i + 1) * 0x9E3779B97F4A7C15), little-endian
+    (Steele, Lea, Flood, "Fast splittable pseudorandom number generators",
+    OOPSLA 2014)."""
+    words = (nbytes + 7) // 8
+    with np.errstate(over="ignore"):
+        i = np.arange(1, words + 1, dtype=np.uint64)
+        z = np.uint64(seed & (2**64 - 1)) + i * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z.astype("<u8").view(np.uint8)[:nbytes].copy()

@@ -66,3 +66,11 @@ def structural_source(w: int, h: int) -> np.ndarray:
     """(x*17 + y*13) % 256 (tests/functional/golden/basic.rs:148-156)."""
     y, x = np.mgrid[0:h, 0:w]
     return ((x * 17 + y * 13) % 256).astype(np.uint8)[..., None]
+
+
+
+def splitmix_bytes(nbytes: int, seed: int) -> np.ndarray:
+    """Host copy of the device's seeded synthetic bytes (DeviceBuffer.fill_random)."""
+    from oracle.oracle import synthetic_bytes
+
+    return synthetic_bytes(nbytes, seed)

@@ -8,7 +8,7 @@ import pytest
 
 import viprs_amd as v
 from oracle import oracle as O
-from tests.helpers import noise_u8
+from tests.helpers import noise_u8, splitmix_bytes
 
 pytestmark = pytest.mark.gpu
 
@@ -161,3 +161,31 @@ def test_batch_tune_keeps_results(ctx):
     batch.run(dsrc, dst)
     ctx.synchronize()
     check_outputs(dst, exp, f"tuned {info}")
+
+
+def test_device_fill_matches_host_generator(ctx):
+    for n, seed in ((1001, 5), (64, 2**63 + 11), (7, 0)):
+        d = v.DeviceBuffer(ctx, (n,), np.uint8)
+        d.fill_random(seed)
+        assert np.array_equal(d.numpy(), splitmix_bytes(n, seed)), (n, seed)
+
+
+def test_batch_uneven_sizes_fitbox_400(ctx):
+    # The uneven-size batch at reduced count: W, H ~ U{256..3000}, 3 or 4
+    # bands, seeded device-made pixels, FitBox(400, 400) Lanczos3.
+    r = np.random.default_rng(4004)
+    sizes = [(int(r.integers(256, 3001)), int(r.integers(256, 3001)), int(r.choice([3, 4]))) for _ in range(10)]
+    srcs = []
+    for i, (w, h, b) in enumerate(sizes):
+        d = v.DeviceBuffer(ctx, (h, w, b), np.uint8)
+        d.fill_random(9000 + i)
+        srcs.append(d)
+    batch = v.Batch(ctx, sizes, v.FITBOX, 400, 400, v.LANCZOS3)
+    assert batch.info()["fallback_images"] == 0
+    dst = batch.output_buffers()
+    batch.run(srcs, dst)
+    ctx.synchronize()
+    for i, (w, h, b) in enumerate(sizes):
+        host = splitmix_bytes(w * h * b, 9000 + i).reshape(h, w, b)
+        exp = O.Pipeline(w, h, b).thumbnail(O.FITBOX, 400, 400).run(host)
+        assert np.array_equal(dst[i].numpy(), exp), (i, sizes[i])

@@ -335,3 +335,11 @@ def test_thumbnail_then_blur_differs_from_separate_ops_at_borders():
     separate = O.Pipeline(70, 70, 3).gauss_blur(2.0).run(thumb)
     assert np.array_equal(chained[5:-5, 5:-5], separate[5:-5, 5:-5])
     assert not np.array_equal(chained, separate)
+
+
+def test_synthetic_bytes_known_answers():
+    # splitmix64 with seed 0: the published first outputs of the generator.
+    w = O.synthetic_bytes(24, 0).view("<u8")
+    assert [int(x) for x in w] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    # Ragged lengths are prefixes of the same stream.
+    assert np.array_equal(O.synthetic_bytes(13, 77), O.synthetic_bytes(16, 77)[:13])

@@ -138,6 +138,11 @@ class DeviceBuffer:
         _check(self.ctx._lib.vgpu_memcpy_h2d(self.ctx.handle, self._p, C.c_void_p(arr.ctypes.data),
                                              self.nbytes), "vgpu_memcpy_h2d")
 
+    def fill_random(self, seed: int) -> None:
+        """Seeded splitmix64 bytes, made on the device (enqueued on the context stream)."""
+        _check(self.ctx._lib.vgpu_fill_random_u8(self.ctx.handle, self._p, self.nbytes,
+                                                 C.c_uint64(seed & (2**64 - 1))), "vgpu_fill_random_u8")
+
     def numpy(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)
         _check(self.ctx._lib.vgpu_memcpy_d2h(self.ctx.handle, C.c_void_p(out.ctypes.data), self._p,
@@ -354,6 +359,15 @@ class Batch:
         _check(self._lib.vgpu_batch_benchmark(self._h, *self._tables(srcs, dsts), warmup, iterations,
                                               C.byref(total), steps), "vgpu_batch_benchmark")
         return total.value / iterations, np.array(steps[:], np.float64)
+
+    def run_profiled(self, srcs, dsts, iterations: int = 1):
+        """[(mean ms, algorithmic bytes)] per batched launch (HIP events)."""
+        ms = (C.c_float * 64)()
+        nb = (C.c_uint64 * 64)()
+        n = self._lib.vgpu_batch_run_profiled(self._h, *self._tables(srcs, dsts), iterations, ms, nb, 64)
+        if n < 0:
+            raise VgpuError(n, "vgpu_batch_run_profiled")
+        return [(ms[i], int(nb[i])) for i in range(n)]
 
     def tune(self, srcs, dsts, iterations: int = 3) -> float:
         best = C.c_float()

@@ -45,6 +45,7 @@ struct Launch {
     size_t jobs_off = 0, tasks_off = 0;
     int ntasks = 0;
     size_t lds = 0;
+    uint64_t alg_bytes = 0;  // input buffers read + outputs written, summed over jobs
 };
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -94,8 +95,48 @@ struct vgpu_batch {
         fallback.clear();
     }
     int plan();
-    int launch_all(hipStream_t s);
+    int launch_all(hipStream_t s, hipEvent_t* marks = nullptr);
+    int check_buffers(const void* const* srcs, const size_t* src_bytes, void* const* dsts,
+                      const size_t* dst_bytes) const;
+    int upload_slots(const void* const* srcs, void* const* dsts, hipStream_t s);
 };
+
+int vgpu_batch::check_buffers(const void* const* srcs, const size_t* src_bytes, void* const* dsts,
+                              const size_t* dst_bytes) const {
+    if (!srcs || !dsts || !src_bytes || !dst_bytes) return VGPU_E_INVALID_ARGUMENT;
+    for (size_t i = 0; i < images.size(); ++i) {
+        const BatchImage& im = images[i];
+        if (src_bytes[i] != (size_t)im.w * im.h * im.bands || dst_bytes[i] != (size_t)im.ow * im.oh * im.bands)
+            return VGPU_E_BUFFER_SIZE;
+        // The shrink kernel's 16-byte row loads need 16-byte aligned sources.
+        if (!srcs[i] || !dsts[i] || ((uintptr_t)srcs[i] & 15)) return VGPU_E_INVALID_ARGUMENT;
+    }
+    return VGPU_OK;
+}
+
+// Refresh the device slot tables when the caller's pointers differ from the
+// last run's; enqueued on `s` ahead of the launches that read them.
+int vgpu_batch::upload_slots(const void* const* srcs, void* const* dsts, hipStream_t s) {
+    const size_t count = images.size();
+    bool same = slots_valid;
+    for (size_t i = 0; same && i < count; ++i) same = cur_srcs[i] == srcs[i] && cur_dsts[i] == dsts[i];
+    if (same) return VGPU_OK;
+    // The staging may still feed the previous upload: wait for it first.
+    if (slots_valid && hipEventSynchronize(slots_read) != hipSuccess) return VGPU_E_DEVICE;
+    void** hs = static_cast<void**>(host_slots);
+    for (size_t i = 0; i < count; ++i) {
+        hs[i] = const_cast<void*>(srcs[i]);
+        hs[count + i] = dsts[i];
+    }
+    unsigned char* t = static_cast<unsigned char*>(tables);
+    if (hipMemcpyAsync(t + slots_off, hs, 2 * count * sizeof(void*), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipEventRecord(slots_read, s) != hipSuccess)
+        return VGPU_E_DEVICE;
+    cur_srcs.assign(srcs, srcs + count);
+    cur_dsts.assign(dsts, dsts + count);
+    slots_valid = true;
+    return VGPU_OK;
+}
 
 // Plan every image's stages with the batch options, lay out intermediates,
 // build the job arrays and task lists and upload them.
@@ -152,6 +193,7 @@ int vgpu_batch::plan() {
         std::vector<ReduceJob> rjobs;
         std::vector<BatchTask> stasks, rtasks;
         size_t rlds = 0;
+        uint64_t sbytes = 0, rbytes = 0;
         for (int idx : order) {
             BatchImage& im = images[(size_t)idx];
             if (!im.batched || lv >= (int)im.stages.size()) continue;
@@ -162,6 +204,7 @@ int vgpu_batch::plan() {
             void* out_ptr = last ? nullptr : darena + im.inter_off[(size_t)lv];
             const InView in = view_of(in_ptr, in_rect, s.clip);
             const OutView out = outview_of(out_ptr, s.out);
+            const uint64_t io_bytes = ((uint64_t)in_rect.w * in_rect.h + (uint64_t)s.out.w * s.out.h) * im.bands;
             if (s.kind == ST_SHRINK_HV_U8) {
                 ShrinkJob j = shrink_job_of(s, im.nodes[(size_t)s.first], in, out);
                 j.src_slot = first ? idx : -1;
@@ -172,6 +215,7 @@ int vgpu_batch::plan() {
                 }
                 const int jid = (int)sjobs.size();
                 sjobs.push_back(j);
+                sbytes += io_bytes;
                 for (int by = 0; by < j.gy; ++by)
                     for (int bx = 0; bx < j.gx; ++bx) stasks.push_back(BatchTask{jid, bx, by});
             } else {
@@ -180,6 +224,7 @@ int vgpu_batch::plan() {
                 j.dst_slot = last ? idx : -1;
                 const int jid = (int)rjobs.size();
                 rjobs.push_back(j);
+                rbytes += io_bytes;
                 rlds = std::max(rlds, reduce_job_lds(j));
                 for (int by = 0; by < j.gy; ++by)
                     for (int bx = 0; bx < j.gx; ++bx) rtasks.push_back(BatchTask{jid, bx, by});
@@ -192,6 +237,7 @@ int vgpu_batch::plan() {
             l.tasks_off = tbl.put(stasks.data(), stasks.size() * sizeof(BatchTask));
             l.ntasks = (int)stasks.size();
             l.lds = shrink_job_lds();
+            l.alg_bytes = sbytes;
             launches.push_back(l);
             n_shrink_tasks += l.ntasks;
         }
@@ -202,6 +248,7 @@ int vgpu_batch::plan() {
             l.tasks_off = tbl.put(rtasks.data(), rtasks.size() * sizeof(BatchTask));
             l.ntasks = (int)rtasks.size();
             l.lds = rlds;
+            l.alg_bytes = rbytes;
             launches.push_back(l);
             n_reduce_tasks += l.ntasks;
         }
@@ -231,11 +278,12 @@ int vgpu_batch::plan() {
     return VGPU_OK;
 }
 
-int vgpu_batch::launch_all(hipStream_t s) {
+int vgpu_batch::launch_all(hipStream_t s, hipEvent_t* marks) {
     unsigned char* t = static_cast<unsigned char*>(tables);
     const void* const* srcs = reinterpret_cast<const void* const*>(t + slots_off);
     void* const* dsts = reinterpret_cast<void* const*>(t + slots_off + images.size() * sizeof(void*));
-    for (const Launch& l : launches) {
+    for (size_t li = 0; li < launches.size(); ++li) {
+        const Launch& l = launches[li];
         const int grid = grid_cap > 0 ? std::min(l.ntasks, grid_cap) : l.ntasks;
         const BatchTask* tasks = reinterpret_cast<const BatchTask*>(t + l.tasks_off);
         hipError_t e;
@@ -246,6 +294,7 @@ int vgpu_batch::launch_all(hipStream_t s) {
             e = launch_reduce_batch(reinterpret_cast<const ReduceJob*>(t + l.jobs_off), tasks, l.ntasks, srcs, dsts,
                                     grid, l.lds, s);
         if (e != hipSuccess) return VGPU_E_DEVICE;
+        if (marks && hipEventRecord(marks[li + 1], s) != hipSuccess) return VGPU_E_DEVICE;
     }
     return VGPU_OK;
 }
@@ -371,35 +420,12 @@ int vgpu_batch_run(vgpu_batch* b, const void* const* srcs, const size_t* src_byt
     if (ctx->sticky != VGPU_OK) return ctx->sticky;
     const size_t count = b->images.size();
     if (count == 0) return VGPU_OK;
-    if (!srcs || !dsts || !src_bytes || !dst_bytes) return VGPU_E_INVALID_ARGUMENT;
-    for (size_t i = 0; i < count; ++i) {
-        const BatchImage& im = b->images[i];
-        if (src_bytes[i] != (size_t)im.w * im.h * im.bands || dst_bytes[i] != (size_t)im.ow * im.oh * im.bands)
-            return VGPU_E_BUFFER_SIZE;
-        // The shrink kernel's 16-byte row loads need 16-byte aligned sources.
-        if (!srcs[i] || !dsts[i] || ((uintptr_t)srcs[i] & 15)) return VGPU_E_INVALID_ARGUMENT;
-    }
+    int rc = b->check_buffers(srcs, src_bytes, dsts, dst_bytes);
+    if (rc != VGPU_OK) return rc;
     (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
-    bool same = b->slots_valid;
-    for (size_t i = 0; same && i < count; ++i) same = b->cur_srcs[i] == srcs[i] && b->cur_dsts[i] == dsts[i];
-    if (!same) {
-        // The staging may still feed the previous upload: wait for it first.
-        if (b->slots_valid && hipEventSynchronize(b->slots_read) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
-        void** hs = static_cast<void**>(b->host_slots);
-        for (size_t i = 0; i < count; ++i) {
-            hs[i] = const_cast<void*>(srcs[i]);
-            hs[count + i] = dsts[i];
-        }
-        unsigned char* t = static_cast<unsigned char*>(b->tables);
-        if (hipMemcpyAsync(t + b->slots_off, hs, 2 * count * sizeof(void*), hipMemcpyHostToDevice, s) != hipSuccess ||
-            hipEventRecord(b->slots_read, s) != hipSuccess)
-            return ctx->sticky = VGPU_E_DEVICE;
-        b->cur_srcs.assign(srcs, srcs + count);
-        b->cur_dsts.assign(dsts, dsts + count);
-        b->slots_valid = true;
-    }
-    int rc = VGPU_OK;
+    rc = b->upload_slots(srcs, dsts, s);
+    if (rc != VGPU_OK) return ctx->sticky = rc;
     if (b->use_graph && !b->launches.empty()) {
         if (!b->graph) {
             hipGraph_t g = nullptr;
@@ -427,6 +453,43 @@ int vgpu_batch_run(vgpu_batch* b, const void* const* srcs, const size_t* src_byt
     }
     if (rc != VGPU_OK) ctx->sticky = rc;
     return rc;
+}
+
+int vgpu_batch_run_profiled(vgpu_batch* b, const void* const* srcs, const size_t* src_bytes, void* const* dsts,
+                            const size_t* dst_bytes, int iterations, float* launch_ms, uint64_t* launch_bytes,
+                            int max_launches) {
+    if (!b || iterations < 1 || !launch_ms) return VGPU_E_INVALID_ARGUMENT;
+    vgpu_context* ctx = b->ctx;
+    if (ctx->sticky != VGPU_OK) return ctx->sticky;
+    const int nl = (int)b->launches.size();
+    if (max_launches < nl) return VGPU_E_BUFFER_SIZE;
+    int rc = b->check_buffers(srcs, src_bytes, dsts, dst_bytes);
+    if (rc != VGPU_OK) return rc;
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    rc = b->upload_slots(srcs, dsts, s);
+    if (rc != VGPU_OK) return ctx->sticky = rc;
+    std::vector<hipEvent_t> ev((size_t)nl + 1);
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return VGPU_E_DEVICE;
+    std::vector<double> acc((size_t)nl, 0.0);
+    for (int it = 0; it < iterations && rc == VGPU_OK; ++it) {
+        (void)hipEventRecord(ev[0], s);
+        rc = b->launch_all(s, ev.data());
+        if (rc == VGPU_OK && hipEventSynchronize(ev[(size_t)nl]) != hipSuccess) rc = VGPU_E_DEVICE;
+        for (int k = 0; k < nl && rc == VGPU_OK; ++k) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[(size_t)k], ev[(size_t)k + 1]);
+            acc[(size_t)k] += ms;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < nl; ++k) {
+        launch_ms[k] = (float)(acc[(size_t)k] / iterations);
+        if (launch_bytes) launch_bytes[k] = b->launches[(size_t)k].alg_bytes;
+    }
+    if (rc != VGPU_OK) return ctx->sticky = rc;
+    return nl;
 }
 
 int vgpu_batch_benchmark(vgpu_batch* b, const void* const* srcs, const size_t* src_bytes, void* const* dsts,

@@ -778,6 +778,33 @@ __global__ __launch_bounds__(kThreads) void k_copy(InView in, OutView out, int B
     }
 }
 
+// Seeded synthetic bytes: 8-byte word i is splitmix64 of seed + (i + 1) *
+// golden gamma, little-endian (a counter-based generator, so any sub-range
+// can be regenerated on the host).
+__device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill_random(unsigned char* dst, size_t bytes, uint64_t seed) {
+    const size_t words = bytes / 8;
+    const bool aligned = (reinterpret_cast<uintptr_t>(dst) & 7) == 0;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < words; i += (size_t)gridDim.x * kThreads) {
+        const uint64_t v = splitmix64_at(seed, i);
+        if (aligned) {
+            *reinterpret_cast<uint64_t*>(dst + i * 8) = v;
+        } else {
+            for (int k = 0; k < 8; ++k) dst[i * 8 + k] = (unsigned char)(v >> (8 * k));
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (bytes & 7)) {
+        const uint64_t v = splitmix64_at(seed, words);
+        for (size_t k = 0; k < (bytes & 7); ++k) dst[words * 8 + k] = (unsigned char)(v >> (8 * k));
+    }
+}
+
 unsigned grid_for(size_t n) {
     size_t g = (n + kThreads - 1) / kThreads;
     if (g < 1) g = 1;
@@ -952,6 +979,13 @@ hipError_t launch_copy(const InView& in, const OutView& out, int bands, int fmt,
     if (fmt == 0) hipLaunchKernelGGL(k_copy<uint8_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands);
     else if (fmt == 1) hipLaunchKernelGGL(k_copy<uint16_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands);
     else hipLaunchKernelGGL(k_copy<float>, dim3(g), dim3(kThreads), 0, s, in, out, bands);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_random(void* dst, size_t bytes, uint64_t seed, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_random, dim3(grid_for(bytes / 8 + 1)), dim3(kThreads), 0, s,
+                       static_cast<unsigned char*>(dst), bytes, seed);
     return hipGetLastError();
 }
 

@@ -95,5 +95,6 @@ hipError_t launch_conv(const InView& in, const OutView& out, int bands, int in_f
 hipError_t launch_alpha_u8(const InView& in, const OutView& out, int bands, int mode,
                            hipStream_t s);
 hipError_t launch_copy(const InView& in, const OutView& out, int bands, int fmt, hipStream_t s);
+hipError_t launch_fill_random(void* dst, size_t bytes, uint64_t seed, hipStream_t s);
 
 }  // namespace vgpu

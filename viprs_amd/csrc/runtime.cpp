@@ -509,6 +509,12 @@ int vgpu_memcpy_d2h(vgpu_context* c, void* dst, const void* src, size_t bytes) {
     return from_hip(e);
 }
 
+int vgpu_fill_random_u8(vgpu_context* c, void* dst, size_t bytes, uint64_t seed) {
+    if (!c || (!dst && bytes)) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    return from_hip(launch_fill_random(dst, bytes, seed, c->stream));
+}
+
 int vgpu_pipeline_new(vgpu_context* ctx, uint32_t w, uint32_t h, uint32_t bands, int fmt,
                       vgpu_pipeline** out) {
     if (!ctx || !out) return VGPU_E_INVALID_ARGUMENT;
