@@ -121,7 +121,7 @@ def bench_batch(v, O, ctx, n, steps, warmup, check_images):
                               "speedup_of_batch": round(per_image_ms / ms, 3)},
         "launches": [{"ms": round(lm, 4), "bytes": lb, "GB/s": round(lb / (lm / 1e3) / 1e9, 1)}
                      for lm, lb in launches],
-        "roofline": {"kernel": "k_shrink_batch" if dom is launches[0] else "k_reduce_batch", "bound": "hbm",
+        "roofline": {"kernel": "k_shrink_batch" if launches.index(dom) < n_shrink_launches else "k_reduce_batch", "bound": "hbm",
                      "achieved": round(dom[1] / (dom[0] / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(dom[1] / (dom[0] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
         "tuned": {"grid_cap": info["grid_cap"], "reduce_first_tile": info["reduce_first_tile"],

@@ -42,6 +42,7 @@ struct BatchImage {
 // One persistent launch over the tasks of one (level, kernel family).
 struct Launch {
     int family = 0;  // 0: fused shrink, 1: fused reduce
+    int variant = 0; // kernel variant within the family
     size_t jobs_off = 0, tasks_off = 0;
     int ntasks = 0;
     size_t lds = 0;
@@ -189,11 +190,14 @@ int vgpu_batch::plan() {
     Staging tbl;
     n_shrink_tasks = n_reduce_tasks = 0;
     for (int lv = 0; lv < levels; ++lv) {
-        std::vector<ShrinkJob> sjobs;
+        constexpr int kShrinkVariants = 4;
+        std::vector<ShrinkJob> sjobs[kShrinkVariants];
+        std::vector<BatchTask> stasks[kShrinkVariants];
+        uint64_t sbytes[kShrinkVariants] = {};
         std::vector<ReduceJob> rjobs;
-        std::vector<BatchTask> stasks, rtasks;
+        std::vector<BatchTask> rtasks;
         size_t rlds = 0;
-        uint64_t sbytes = 0, rbytes = 0;
+        uint64_t rbytes = 0;
         for (int idx : order) {
             BatchImage& im = images[(size_t)idx];
             if (!im.batched || lv >= (int)im.stages.size()) continue;
@@ -213,11 +217,12 @@ int vgpu_batch::plan() {
                     im.batched = false;
                     continue;
                 }
-                const int jid = (int)sjobs.size();
-                sjobs.push_back(j);
-                sbytes += io_bytes;
+                const int var = shrink_job_variant(j);
+                const int jid = (int)sjobs[var].size();
+                sjobs[var].push_back(j);
+                sbytes[var] += io_bytes;
                 for (int by = 0; by < j.gy; ++by)
-                    for (int bx = 0; bx < j.gx; ++bx) stasks.push_back(BatchTask{jid, bx, by});
+                    for (int bx = 0; bx < j.gx; ++bx) stasks[var].push_back(BatchTask{jid, bx, by});
             } else {
                 ReduceJob j = reduce_job_of(s, in, out, dblob);
                 j.src_slot = first ? idx : -1;
@@ -230,14 +235,16 @@ int vgpu_batch::plan() {
                     for (int bx = 0; bx < j.gx; ++bx) rtasks.push_back(BatchTask{jid, bx, by});
             }
         }
-        if (!stasks.empty()) {
+        for (int var = 0; var < kShrinkVariants; ++var) {
+            if (stasks[var].empty()) continue;
             Launch l;
             l.family = 0;
-            l.jobs_off = tbl.put(sjobs.data(), sjobs.size() * sizeof(ShrinkJob));
-            l.tasks_off = tbl.put(stasks.data(), stasks.size() * sizeof(BatchTask));
-            l.ntasks = (int)stasks.size();
+            l.variant = var;
+            l.jobs_off = tbl.put(sjobs[var].data(), sjobs[var].size() * sizeof(ShrinkJob));
+            l.tasks_off = tbl.put(stasks[var].data(), stasks[var].size() * sizeof(BatchTask));
+            l.ntasks = (int)stasks[var].size();
             l.lds = shrink_job_lds();
-            l.alg_bytes = sbytes;
+            l.alg_bytes = sbytes[var];
             launches.push_back(l);
             n_shrink_tasks += l.ntasks;
         }
@@ -288,8 +295,8 @@ int vgpu_batch::launch_all(hipStream_t s, hipEvent_t* marks) {
         const BatchTask* tasks = reinterpret_cast<const BatchTask*>(t + l.tasks_off);
         hipError_t e;
         if (l.family == 0)
-            e = launch_shrink_batch(reinterpret_cast<const ShrinkJob*>(t + l.jobs_off), tasks, l.ntasks, srcs, dsts,
-                                    grid, s);
+            e = launch_shrink_batch(l.variant, reinterpret_cast<const ShrinkJob*>(t + l.jobs_off), tasks, l.ntasks,
+                                    srcs, dsts, grid, s);
         else
             e = launch_reduce_batch(reinterpret_cast<const ReduceJob*>(t + l.jobs_off), tasks, l.ntasks, srcs, dsts,
                                     grid, l.lds, s);

@@ -12,12 +12,22 @@
 // file must be compiled with -ffp-contract=off.
 #include "kernels.hpp"
 
+#include <algorithm>
+
 namespace vgpu {
 namespace {
 
 constexpr int kThreads = 256;
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Image buffers always live in device global memory; saying so lets the
+// compiler use global (not flat) loads for pointers that come from job tables.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const void* p) {
+    return (const __attribute__((address_space(1))) T*)p;
+}
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // Buffer-relative column / row of an absolute coordinate: clip, then clamp to
 // the buffer.
@@ -180,21 +190,32 @@ __global__ __launch_bounds__(kThreads) void k_shrink_hv_u8(InView in, OutView ou
 }
 
 // ---------------------------------------------------------------------------
-// Fused ShrinkH -> ShrinkV for u8, wave-per-output-row form.
+// Fused ShrinkH -> ShrinkV for u8, wavefront-run form.
 //
-// Each wavefront owns one output row and TX (<= 64) output columns; a block is
-// four wavefronts on four consecutive output rows, so every (row, column-span)
-// task of a thumbnail is co-resident on the chip and there is no tail round.
-// A wavefront walks its fv source rows: the clamped in-bounds byte span of the
-// row is fetched with 16-byte loads one row ahead (registers), parked in the
-// wavefront's LDS row buffer, and each lane sums its fh-pixel box window with
-// dword LDS reads realigned by alignbyte and per-band byte masks folded in by
-// udot4.  Windows that reach past the clamped span (the Z^2 halo columns of a
-// thumbnail) take a per-pixel clamped path.  Rounding is the reference's:
-// ShrinkH rounding per source row, then ShrinkV rounding of the sum.
+// Each wavefront owns TX (<= 64) output columns of a run of m consecutive
+// output rows (or, with S > 1, every S-th source row of one output row, the
+// S partial sums meeting in LDS at the end).  The source rows of the run are
+// contiguous, so the wavefront streams them in fetch groups: a group is RP
+// source rows, each given a 2^lsl-chunk slot of 16 bytes (RP * 2^lsl = 256
+// chunks = 4 KiB), loaded branch-free as four 16-byte loads per lane one
+// group ahead in registers, then parked in the wavefront's own LDS buffer.
+// Small box windows therefore still keep about 4 KiB per wavefront in flight.
+// Each lane sums its fh-pixel box window with dword LDS reads realigned by
+// alignbyte and per-band byte masks folded in by udot4; windows that reach
+// past the clamped span (the Z^2 halo columns of a thumbnail) take a
+// per-pixel clamped path.  Rounding is the reference's: ShrinkH rounding per
+// source row, then ShrinkV rounding of the sum.  Wavefronts synchronise only
+// with themselves, except for the S-way partial-sum combine.
 // ---------------------------------------------------------------------------
-constexpr int kSpanBytes = 4096;  // LDS bytes per wavefront row buffer
-constexpr int kPrefetch = kSpanBytes / (64 * 16);  // 16-byte chunks per lane
+constexpr int kSpanBytes = 4096;  // LDS bytes per wavefront group buffer
+
+// LDS hand-off between the lanes of one wavefront: LDS operations of a
+// wavefront execute in issue order, so a compiler barrier suffices.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <int B>
 __device__ __forceinline__ uint32_t band_mask(int b, int kk) {
@@ -206,63 +227,73 @@ __device__ __forceinline__ uint32_t band_mask(int b, int kk) {
     return m == 0 ? 0x01000001u : (m == 1 ? 0x00000100u : 0x00010000u);
 }
 
-// Sum of one box window of wbytes bytes starting at LDS byte sbyte, per band.
-// SMALL: the window is at most 64 bytes, so its (at most 17) dwords are loaded
-// in one burst and the dword loop is unrolled with compile-time band masks.
-template <int B, bool SMALL>
+// Sum of one box window of wbytes bytes starting at LDS byte sbyte, per band;
+// rbuf must be 4-byte aligned.  Dword d of the window is realigned from two
+// LDS dwords by alignbyte, masked (bytes past the window end -> 0; the masks
+// depend only on d and the window size, so they are scalar), and folded into
+// the band sums by udot4 with per-band byte masks.  The loop is unrolled by U
+// dwords (a multiple of 3 for 3 bands, so the band pattern restarts) and may
+// read up to U dwords past the window; those bytes are masked.
+template <int B>
 __device__ __forceinline__ void window_sums(const unsigned char* rbuf, int sbyte, int wbytes,
                                             uint32_t (&sum)[B]) {
-    const uint32_t* l32 = reinterpret_cast<const uint32_t*>(rbuf);
-    const int d0 = sbyte >> 2, al = sbyte & 3;
+    const uint32_t* l32 = reinterpret_cast<const uint32_t*>(rbuf) + (sbyte >> 2);
+    const int al = sbyte & 3;
     const int nd = (wbytes + 3) >> 2;
     const int rem_last = wbytes - 4 * (nd - 1);  // 1..4 valid bytes in the last dword
     const uint32_t last_mask = rem_last >= 4 ? 0xffffffffu : ((1u << (8 * rem_last)) - 1u);
-    if constexpr (SMALL) {
-        constexpr int kMaxD = 17;
-        uint32_t w[kMaxD];
+    constexpr int U = B == 3 ? 6 : 4;
+    uint32_t cur = l32[0];
+    int d0 = 0;
+    // Blocks of U dwords that lie wholly before the last dword: no masks.
+    for (; d0 + U <= nd - 1; d0 += U) {
+        uint32_t w[U];
 #pragma unroll
-        for (int d = 0; d < kMaxD; ++d) w[d] = d <= nd ? l32[d0 + d] : 0u;
+        for (int u = 0; u < U; ++u) w[u] = l32[d0 + u + 1];
 #pragma unroll
-        for (int d = 0; d < kMaxD - 1; ++d) {
-            if (d < nd) {
-                uint32_t v = __builtin_amdgcn_alignbyte(w[d + 1], w[d], al);
-                if (d == nd - 1) v &= last_mask;
+        for (int u = 0; u < U; ++u) {
+            const uint32_t v = __builtin_amdgcn_alignbyte(w[u], cur, al);
+            cur = w[u];
 #pragma unroll
-                for (int b = 0; b < B; ++b) sum[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, d % 3), sum[b], false);
-            }
+            for (int b = 0; b < B; ++b) sum[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, u % 3), sum[b], false);
         }
-    } else {
-        uint32_t cur = l32[d0];
-        int kk = 0;
-        for (int d = 0; d < nd; ++d) {
-            const uint32_t nxt = l32[d0 + d + 1];
-            uint32_t v = __builtin_amdgcn_alignbyte(nxt, cur, al);
-            cur = nxt;
-            if (d == nd - 1) v &= last_mask;
+    }
+    // Tail block: the last 1..U dwords, masked.
+    {
+        uint32_t w[U];
 #pragma unroll
-            for (int b = 0; b < B; ++b) sum[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, kk), sum[b], false);
-            kk = kk == 2 ? 0 : kk + 1;
+        for (int u = 0; u < U; ++u) w[u] = l32[d0 + u + 1];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int d = d0 + u;
+            const uint32_t m = d < nd - 1 ? 0xffffffffu : (d == nd - 1 ? last_mask : 0u);
+            const uint32_t v = __builtin_amdgcn_alignbyte(w[u], cur, al) & m;
+            cur = w[u];
+#pragma unroll
+            for (int b = 0; b < B; ++b) sum[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, u % 3), sum[b], false);
         }
     }
 }
 
-template <int B, bool SMALL>
+template <int B>
 __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, int by,
                                                   unsigned char* lds) {
     const InView& in = job.in;
     const OutView& out = job.out;
-    const int fh = job.fh, fv = job.fv, tx = job.tx, S = job.S, premult = job.premult;
+    const int fh = job.fh, fv = job.fv, tx = job.tx, S = job.S, m = job.m, lsl = job.lsl;
+    const int RP = 256 >> lsl;  // source rows per fetch group
+    const bool premult = (B == 2 || B == 4) && job.premult;
     const uint32_t mh = job.mh, mv = job.mv;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // S wavefronts share one output row; wavefront `part` sums source rows
-    // part, part + S, ...; their integer partial sums meet in LDS at the end.
+    // The wavefront index is uniform; saying so keeps all per-wavefront
+    // geometry and row addressing in scalar registers.
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int part = wave % S;
-    const int oyi = by * ((kThreads / 64) / S) + wave / S;
-    const bool active = oyi < out.h;
-    const int nk = part < fv ? (fv - 1 - part) / S + 1 : 0;  // source rows of this wavefront
+    const int oyi0 = (by * ((kThreads / 64) / S) + wave / S) * m;  // first output row of the run
+    const int nout = max(0, min(m, out.h - oyi0));
+    const int nsrc = S > 1 ? ((nout > 0 && part < fv) ? (fv - 1 - part) / S + 1 : 0) : nout * fv;
     const int col0 = bx * tx;
     const int ncols = min(tx, out.w - col0);
-    const int oy = out.y + oyi;
+    const int ybase = (out.y + oyi0) * fv + part;  // source row of k = 0 (row k: ybase + k * S)
     const int cx0 = (out.x + col0) * fh;
     const int cx1 = cx0 + ncols * fh;
     const int lo = clampi(cx0, in.bx, in.bx + in.bw - 1);
@@ -271,150 +302,177 @@ __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, 
     const unsigned char* base = static_cast<const unsigned char*>(in.ptr);
     const size_t row_bytes = (size_t)in.bw * B;
     const size_t total = row_bytes * (size_t)in.bh;
+    const size_t col_off = (size_t)(lo - in.bx) * B;
     unsigned char* rbuf = lds + wave * (kSpanBytes + 32);
     const int wbytes = fh * B;
+    const int slot_mask = (1 << lsl) - 1;
 
-    // One source row ahead in registers: the 16-byte chunks of the clamped
-    // span, aligned down to 16 bytes (the buffer base is 16-byte aligned).
-    // The loads are branch-free: a chunk that is not needed (or would run past
-    // the end of the buffer) reloads the buffer's first 16 bytes and is
-    // ignored, so all loads issue back to back and stay in flight across the
-    // compute of the current row.
-    uint4 p0 = make_uint4(0, 0, 0, 0), p1 = p0, p2 = p0, p3 = p0;
-    static_assert(kPrefetch == 4, "prefetch registers are unrolled by hand");
-    size_t pre_a16 = 0;
-    int pre_shift = 0, pre_nch = 0;
-#define VGPU_FETCH(k)                                                                              \
-    do {                                                                                           \
-        const size_t a_ = (size_t)row_of(in, oy * fv + (k)) * row_bytes + (size_t)(lo - in.bx) * B; \
-        pre_a16 = a_ & ~(size_t)15;                                                                \
-        pre_shift = (int)(a_ - pre_a16);                                                           \
-        pre_nch = (pre_shift + nbytes + 15) >> 4;                                                  \
-        size_t o0 = pre_a16 + (size_t)lane * 16, o1 = o0 + 1024, o2 = o0 + 2048, o3 = o0 + 3072;   \
-        o0 = (lane < pre_nch && o0 + 16 <= total) ? o0 : 0;                                        \
-        o1 = (lane + 64 < pre_nch && o1 + 16 <= total) ? o1 : 0;                                   \
-        o2 = (lane + 128 < pre_nch && o2 + 16 <= total) ? o2 : 0;                                  \
-        o3 = (lane + 192 < pre_nch && o3 + 16 <= total) ? o3 : 0;                                  \
-        p0 = *reinterpret_cast<const uint4*>(base + o0);                                           \
-        p1 = *reinterpret_cast<const uint4*>(base + o1);                                           \
-        p2 = *reinterpret_cast<const uint4*>(base + o2);                                           \
-        p3 = *reinterpret_cast<const uint4*>(base + o3);                                           \
+    // Fetch group g into p0..p3; vmask holds 2 bits per chunk: 1 = full
+    // 16-byte chunk, 2 = the buffer's last, partial chunk (copied bytewise at
+    // park time), 0 = unused.  Unused or partial chunks reload the buffer's
+    // first 16 bytes, so every load is unconditional.
+    u32x4 p0 = {0u, 0u, 0u, 0u}, p1 = p0, p2 = p0, p3 = p0;
+    uint32_t vmask = 0;
+#define VGPU_CHUNK(j, g, pj, ROW)                                                                   \
+    do {                                                                                            \
+        const int k_ = (g) * RP + (ROW);                                                            \
+        const int c_ = (lane + 64 * (j)) & slot_mask;                                               \
+        const size_t a_ = (size_t)row_of(in, ybase + k_ * S) * row_bytes + col_off;                 \
+        const size_t a16_ = a_ & ~(size_t)15;                                                       \
+        const int nch_ = ((int)(a_ - a16_) + nbytes + 15) >> 4;                                     \
+        const size_t off_ = a16_ + (size_t)c_ * 16;                                                 \
+        const bool use_ = k_ < nsrc && c_ < nch_;                                                   \
+        const bool full_ = use_ && off_ + 16 <= total;                                              \
+        vmask |= (uint32_t)(full_ ? 1 : (use_ ? 2 : 0)) << (2 * (j));                               \
+        pj = gmem<u32x4>(base + (full_ ? off_ : 0))[0];                                             \
+    } while (0)
+// The row of register j's chunks is wave-uniform when a slot spans at least
+// one register (lsl >= 6): that branch keeps the address math scalar.
+#define VGPU_FETCH(g)                                                                   \
+    do {                                                                                \
+        vmask = 0;                                                                      \
+        if (lsl >= 6) {                                                                 \
+            VGPU_CHUNK(0, g, p0, 0);                                                    \
+            VGPU_CHUNK(1, g, p1, 1 >> (lsl - 6));                                       \
+            VGPU_CHUNK(2, g, p2, 2 >> (lsl - 6));                                       \
+            VGPU_CHUNK(3, g, p3, 3 >> (lsl - 6));                                       \
+        } else {                                                                        \
+            VGPU_CHUNK(0, g, p0, (lane >> lsl));                                        \
+            VGPU_CHUNK(1, g, p1, (lane >> lsl) + (64 >> lsl));                          \
+            VGPU_CHUNK(2, g, p2, (lane >> lsl) + 2 * (64 >> lsl));                      \
+            VGPU_CHUNK(3, g, p3, (lane >> lsl) + 3 * (64 >> lsl));                      \
+        }                                                                               \
+    } while (0)
+#define VGPU_PARK(j, g, pj)                                                                         \
+    do {                                                                                            \
+        const uint32_t st_ = (vmask >> (2 * (j))) & 3u;                                             \
+        const int idx_ = lane + 64 * (j);                                                           \
+        if (st_ == 1u) {                                                                            \
+            *reinterpret_cast<u32x4*>(rbuf + idx_ * 16) = pj;                                       \
+        } else if (st_ == 2u) {                                                                     \
+            const int k_ = (g) * RP + (idx_ >> lsl);                                                \
+            const size_t a_ = (size_t)row_of(in, ybase + k_ * S) * row_bytes + col_off;             \
+            const size_t off_ = (a_ & ~(size_t)15) + (size_t)(idx_ & slot_mask) * 16;               \
+            for (int q_ = 0; q_ < 16 && off_ + q_ < total; ++q_) rbuf[idx_ * 16 + q_] = gmem<unsigned char>(base)[off_ + q_]; \
+        }                                                                                           \
     } while (0)
 
     uint32_t acc[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) acc[b] = 0;
-    if (active && nk > 0) VGPU_FETCH(part);
-    for (int q = 0; q < (fv + S - 1) / S; ++q) {
-        const bool live = active && q < nk;
-        const int shift = pre_shift;
-        if (live) {
+    const int ng = (nsrc + RP - 1) / RP;
+    if (ng > 0) VGPU_FETCH(0);
+    for (int g = 0; g < ng; ++g) {
+        VGPU_PARK(0, g, p0);
+        VGPU_PARK(1, g, p1);
+        VGPU_PARK(2, g, p2);
+        VGPU_PARK(3, g, p3);
+        wave_lds_sync();
+        if (g + 1 < ng) VGPU_FETCH(g + 1);
+        const int nr = min(RP, nsrc - g * RP);
+        for (int r = 0; r < nr; ++r) {
+            const int k = g * RP + r;
+            if (lane < ncols) {
+                const size_t a = (size_t)row_of(in, ybase + k * S) * row_bytes + col_off;
+                // The slot is 16-byte aligned; the row's bytes start `shift` in.
+                const unsigned char* slot = rbuf + (r << lsl) * 16;
+                const int shift = (int)(a & 15);
+                const int X0 = cx0 + lane * fh;
+                uint32_t sum[B];
 #pragma unroll
-            for (int j = 0; j < kPrefetch; ++j) {
-                const int i = lane + 64 * j;
-                const uint4 v = j == 0 ? p0 : (j == 1 ? p1 : (j == 2 ? p2 : p3));
-                if (i < pre_nch) {
-                    const size_t off = pre_a16 + (size_t)i * 16;
-                    if (off + 16 <= total) {
-                        *reinterpret_cast<uint4*>(rbuf + i * 16) = v;
-                    } else {
-                        // Last chunk of the buffer: copy only the bytes that exist.
-                        for (int q = 0; q < 16 && off + q < total; ++q) rbuf[i * 16 + q] = base[off + q];
+                for (int b = 0; b < B; ++b) sum[b] = 0;
+                if (!premult && X0 >= lo && X0 + fh <= hi) {
+                    window_sums<B>(slot, shift + (X0 - lo) * B, wbytes, sum);
+                } else {
+                    for (int j = 0; j < fh; ++j) {
+                        const unsigned char* px = slot + shift + (clampi(X0 + j, lo, hi - 1) - lo) * B;
+                        if (premult) {
+                            const uint32_t al = px[B - 1];
+#pragma unroll
+                            for (int b = 0; b < B - 1; ++b) sum[b] += premult_u8(px[b], al);
+                            sum[B - 1] += al;
+                        } else {
+#pragma unroll
+                            for (int b = 0; b < B; ++b) sum[b] += px[b];
+                        }
                     }
                 }
+#pragma unroll
+                for (int b = 0; b < B; ++b) acc[b] += shrink_round_h(sum[b], (uint32_t)fh, B, mh);
             }
-        }
-        __syncthreads();
-        if (live && q + 1 < nk) VGPU_FETCH(part + (q + 1) * S);
-        if (live && lane < ncols) {
-            const int X0 = cx0 + lane * fh;
-            uint32_t sum[B];
+            if (S == 1 && (k + 1) % fv == 0) {
+                // Output row oyi0 + k / fv is complete.
+                if (lane < ncols) {
+                    unsigned char* o = static_cast<unsigned char*>(out.ptr) +
+                                       ((size_t)(oyi0 + k / fv) * out.w + col0 + lane) * B;
+                    uint32_t v[B];
 #pragma unroll
-            for (int b = 0; b < B; ++b) sum[b] = 0;
-            const bool plain = !(premult && (B == 2 || B == 4));
-            if (plain && X0 >= lo && X0 + fh <= hi) {
-                window_sums<B, SMALL>(rbuf, shift + (X0 - lo) * B, wbytes, sum);
-            } else {
-                for (int j = 0; j < fh; ++j) {
-                    const int X = clampi(X0 + j, lo, hi - 1);
-                    const unsigned char* px = rbuf + shift + (X - lo) * B;
-                    if (!plain) {
-                        const uint32_t a = px[B - 1];
+                    for (int b = 0; b < B; ++b) v[b] = shrink_round_v(acc[b], (uint32_t)fv, mv);
+                    if ((B == 2 || B == 4) && job.unpremult) {
 #pragma unroll
-                        for (int b = 0; b < B - 1; ++b) sum[b] += premult_u8(px[b], a);
-                        sum[B - 1] += a;
-                    } else {
-#pragma unroll
-                        for (int b = 0; b < B; ++b) sum[b] += px[b];
+                        for (int b = 0; b < B - 1; ++b) v[b] = unpremult_u8(v[b], v[B - 1]);
                     }
-                }
-            }
 #pragma unroll
-            for (int b = 0; b < B; ++b) acc[b] += shrink_round_h(sum[b], (uint32_t)fh, B, mh);
+                    for (int b = 0; b < B; ++b) o[b] = (unsigned char)v[b];
+                }
+#pragma unroll
+                for (int b = 0; b < B; ++b) acc[b] = 0;
+            }
         }
-        __syncthreads();
+        wave_lds_sync();
     }
+#undef VGPU_PARK
+#undef VGPU_FETCH
+#undef VGPU_CHUNK
     if (S > 1) {
-        // Partial sums of the S wavefronts of each output row (row buffers are free now).
+        // Partial sums of the S wavefronts of each output row.  Every
+        // wavefront of the block reaches both barriers (S is per job).
+        __syncthreads();
         uint32_t* part_sums = reinterpret_cast<uint32_t*>(lds);
 #pragma unroll
         for (int b = 0; b < B; ++b) part_sums[(wave * 64 + lane) * B + b] = acc[b];
         __syncthreads();
-        if (part == 0) {
-            for (int p2 = 1; p2 < S; ++p2) {
+        if (part == 0 && nout > 0 && lane < ncols) {
+            for (int q = 1; q < S; ++q) {
 #pragma unroll
-                for (int b = 0; b < B; ++b) acc[b] += part_sums[((wave + p2) * 64 + lane) * B + b];
+                for (int b = 0; b < B; ++b) acc[b] += part_sums[((wave + q) * 64 + lane) * B + b];
             }
+            unsigned char* o = static_cast<unsigned char*>(out.ptr) + ((size_t)oyi0 * out.w + col0 + lane) * B;
+            uint32_t v[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) v[b] = shrink_round_v(acc[b], (uint32_t)fv, mv);
+            if ((B == 2 || B == 4) && job.unpremult) {
+#pragma unroll
+                for (int b = 0; b < B - 1; ++b) v[b] = unpremult_u8(v[b], v[B - 1]);
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) o[b] = (unsigned char)v[b];
         }
-    }
-    if (active && part == 0 && lane < ncols) {
-        unsigned char* o = static_cast<unsigned char*>(out.ptr) + ((size_t)oyi * out.w + col0 + lane) * B;
-        uint32_t v[B];
-#pragma unroll
-        for (int b = 0; b < B; ++b) v[b] = shrink_round_v(acc[b], (uint32_t)fv, mv);
-        if ((B == 2 || B == 4) && job.unpremult) {
-#pragma unroll
-            for (int b = 0; b < B - 1; ++b) v[b] = unpremult_u8(v[b], v[B - 1]);
-        }
-#pragma unroll
-        for (int b = 0; b < B; ++b) o[b] = (unsigned char)v[b];
-    }
-#undef VGPU_FETCH
-}
-
-// One block of the shrink job per (bx, by) of its grid.
-__device__ __forceinline__ void shrink_job_block(const ShrinkJob& job, int bx, int by,
-                                                 unsigned char* lds) {
-    switch (job.bands * 2 + job.small) {
-    case 2: shrink_rows_block<1, false>(job, bx, by, lds); break;
-    case 3: shrink_rows_block<1, true>(job, bx, by, lds); break;
-    case 4: shrink_rows_block<2, false>(job, bx, by, lds); break;
-    case 5: shrink_rows_block<2, true>(job, bx, by, lds); break;
-    case 6: shrink_rows_block<3, false>(job, bx, by, lds); break;
-    case 7: shrink_rows_block<3, true>(job, bx, by, lds); break;
-    case 8: shrink_rows_block<4, false>(job, bx, by, lds); break;
-    default: shrink_rows_block<4, true>(job, bx, by, lds); break;
     }
 }
 
+// One kernel per band count, so each keeps only its own registers.
+template <int B>
 __global__ __launch_bounds__(kThreads) void k_shrink_job(ShrinkJob job) {
     extern __shared__ __align__(16) unsigned char lds[];
-    shrink_job_block(job, blockIdx.x, blockIdx.y, lds);
+    shrink_rows_block<B>(job, blockIdx.x, blockIdx.y, lds);
 }
 
 // Persistent form: the grid walks a task list of (job, bx, by) over many
-// images; jobs reading the caller's source or writing the caller's output take
-// those pointers from per-run slot tables.
-__global__ __launch_bounds__(kThreads) void k_shrink_batch(const ShrinkJob* jobs, const BatchTask* tasks,
-                                                           int ntasks, const void* const* srcs,
-                                                           void* const* dsts) {
+// images (all of one variant); jobs reading the caller's source or writing
+// the caller's output take those pointers from per-run slot tables.
+template <int B>
+__global__ __launch_bounds__(kThreads) void k_shrink_batch(const ShrinkJob* __restrict__ jobs,
+                                                           const BatchTask* __restrict__ tasks, int ntasks,
+                                                           const void* const* __restrict__ srcs,
+                                                           void* const* __restrict__ dsts) {
     extern __shared__ __align__(16) unsigned char lds[];
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const BatchTask task = tasks[t];
         ShrinkJob job = jobs[task.job];
         if (job.src_slot >= 0) job.in.ptr = srcs[job.src_slot];
         if (job.dst_slot >= 0) job.out.ptr = dsts[job.dst_slot];
-        shrink_job_block(job, task.bx, task.by, lds);
+        shrink_rows_block<B>(job, task.bx, task.by, lds);
         __syncthreads();
     }
 }
@@ -693,9 +751,10 @@ __global__ __launch_bounds__(kThreads) void k_reduce_job(ReduceJob job) {
     reduce_job_block(job, blockIdx.x, blockIdx.y, smem);
 }
 
-__global__ __launch_bounds__(kThreads) void k_reduce_batch(const ReduceJob* jobs, const BatchTask* tasks,
-                                                           int ntasks, const void* const* srcs,
-                                                           void* const* dsts) {
+__global__ __launch_bounds__(kThreads) void k_reduce_batch(const ReduceJob* __restrict__ jobs,
+                                                           const BatchTask* __restrict__ tasks, int ntasks,
+                                                           const void* const* __restrict__ srcs,
+                                                           void* const* __restrict__ dsts) {
     extern __shared__ __align__(16) unsigned char smem[];
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const BatchTask task = tasks[t];
@@ -823,39 +882,72 @@ bool plan_shrink_job(ShrinkJob* job, bool row_split) {
     int tx = 64;
     while (tx > 1 && tx * wb + 15 > kSpanBytes) tx /= 2;
     if (!aligned || src_bytes < 16 || tx * wb + 15 > kSpanBytes) return false;
-    // Split the fv source rows of an output row over S wavefronts while the
-    // (row, span) tasks leave room on the chip: about 6 resident wavefronts
-    // per SIMD over 256 CUs.
-    const int64_t tasks = (int64_t)((job->out.w + tx - 1) / tx) * job->out.h;
+    // Fetch-group geometry: a source row needs at most (15 + span + 15) / 16
+    // chunks; its slot is the next power of two, RP = 256 / slot rows a group.
+    const int cpr = (15 + tx * wb + 15) / 16;
+    int lsl = 0;
+    while ((1 << lsl) < cpr) ++lsl;
+    const int rp = 256 >> lsl;
+    const int gx = (job->out.w + tx - 1) / tx;
+    // Run length: enough output rows for two fetch groups per wavefront.
+    int m = std::max(1, std::min(32, (2 * rp + fv - 1) / fv));
+    auto waves = [&](int mm) { return (int64_t)gx * ((job->out.h + mm - 1) / mm); };
     int S = 1;
-    while (row_split && S < 4 && 2 * S <= fv && tasks * 2 * S <= 256 * 4 * 6) S *= 2;
+    if (row_split) {
+        // One image alone: keep about 16 wavefronts per CU busy before
+        // lengthening runs, and split the source rows of an output row over
+        // S wavefronts while the row tasks leave room on the chip.
+        while (m > 1 && waves(m) < 256 * 16) m = (m + 1) / 2;
+        if (m == 1)
+            while (S < 4 && 2 * S <= fv && waves(1) * 2 * S <= 256 * 4 * 6) S *= 2;
+    }
     job->tx = tx;
     job->S = S;
-    job->small = wb <= 64;
+    job->m = S > 1 ? 1 : m;
+    job->lsl = lsl;
     job->mh = (uint32_t)((1ull << 32) / (256ull * (uint64_t)fh));
     job->mv = (uint32_t)((1ull << 32) / (256ull * (uint64_t)fv));
-    const int rows_per_block = (kThreads / 64) / S;
-    job->gx = (job->out.w + tx - 1) / tx;
+    const int rows_per_block = ((kThreads / 64) / S) * job->m;
+    job->gx = gx;
     job->gy = (job->out.h + rows_per_block - 1) / rows_per_block;
     return true;
 }
 
+int shrink_job_variant(const ShrinkJob& job) { return job.bands - 1; }
+
 size_t shrink_job_lds() { return (size_t)(kThreads / 64) * (kSpanBytes + 32); }
+
+#define VGPU_SHRINK_VARIANTS(X) X(0, 1) X(1, 2) X(2, 3) X(3, 4)
 
 hipError_t launch_shrink_job(const ShrinkJob& job, hipStream_t s) {
     if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_shrink_job, dim3((unsigned)job.gx, (unsigned)job.gy), dim3(kThreads),
-                       shrink_job_lds(), s, job);
+    const dim3 grid((unsigned)job.gx, (unsigned)job.gy);
+    switch (shrink_job_variant(job)) {
+#define X(V, B) \
+    case V: hipLaunchKernelGGL((k_shrink_job<B>), grid, dim3(kThreads), shrink_job_lds(), s, job); break;
+        VGPU_SHRINK_VARIANTS(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_shrink_batch(const ShrinkJob* jobs, const BatchTask* tasks, int ntasks,
+hipError_t launch_shrink_batch(int variant, const ShrinkJob* jobs, const BatchTask* tasks, int ntasks,
                                const void* const* srcs, void* const* dsts, int grid, hipStream_t s) {
     if (ntasks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_shrink_batch, dim3((unsigned)grid), dim3(kThreads), shrink_job_lds(), s, jobs,
-                       tasks, ntasks, srcs, dsts);
+    switch (variant) {
+#define X(V, B)                                                                                       \
+    case V:                                                                                           \
+        hipLaunchKernelGGL((k_shrink_batch<B>), dim3((unsigned)grid), dim3(kThreads), shrink_job_lds(), s, \
+                           jobs, tasks, ntasks, srcs, dsts);                                          \
+        break;
+        VGPU_SHRINK_VARIANTS(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
+#undef VGPU_SHRINK_VARIANTS
 
 hipError_t launch_shrink_hv_u8(const InView& in, const OutView& out, int bands, int fh, int fv,
                                int premult, hipStream_t s) {

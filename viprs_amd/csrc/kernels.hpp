@@ -38,13 +38,14 @@ struct DevPlan {
 enum Alpha : int { ALPHA_NONE = 0, ALPHA_PREMULT = 1, ALPHA_UNPREMULT = 2 };
 
 // Fused [Premultiply] -> ShrinkH -> ShrinkV -> [Unpremultiply] on u8, one
-// image.  plan_shrink_job fills the launch shape (tx, S, small, mh, mv, gx,
-// gy); it returns false when the wave-per-row kernel cannot take the job.
+// image.  plan_shrink_job fills the launch shape (tx, S, m, lsl, mh, mv, gx,
+// gy); it returns false when the wavefront kernel cannot take the job.
 struct ShrinkJob {
     InView in;
     OutView out;
     int bands = 3, fh = 1, fv = 1, premult = 0, unpremult = 0;
-    int tx = 64, S = 1, small = 1;
+    int tx = 64, S = 1;
+    int m = 1, lsl = 8;  // output rows per wavefront run; log2 chunks per source-row slot
     uint32_t mh = 0, mv = 0;
     int gx = 0, gy = 0;
     int src_slot = -1, dst_slot = -1;  // batch: pointer taken from the per-run tables
@@ -81,7 +82,10 @@ hipError_t launch_reduce_v(const InView& in, const OutView& out, int bands, int 
 bool plan_shrink_job(ShrinkJob* job, bool row_split = true);
 size_t shrink_job_lds();
 hipError_t launch_shrink_job(const ShrinkJob& job, hipStream_t s);
-hipError_t launch_shrink_batch(const ShrinkJob* jobs, const BatchTask* tasks, int ntasks,
+// Kernel variant of a planned job (its band count); a batch launch runs jobs
+// of one variant.
+int shrink_job_variant(const ShrinkJob& job);
+hipError_t launch_shrink_batch(int variant, const ShrinkJob* jobs, const BatchTask* tasks, int ntasks,
                                const void* const* srcs, void* const* dsts, int grid, hipStream_t s);
 size_t reduce_job_lds(const ReduceJob& job);
 hipError_t launch_reduce_job(const ReduceJob& job, hipStream_t s);
