@@ -119,9 +119,9 @@ def bench_batch(v, O, ctx, n, steps, warmup, check_images):
         "per_image_kernels": {"ms_sum": round(per_image_ms, 4),
                               "Mpix/s": round(mpix / (per_image_ms / 1e3), 1),
                               "speedup_of_batch": round(per_image_ms / ms, 3)},
-        "launches": [{"ms": round(lm, 4), "bytes": lb, "GB/s": round(lb / (lm / 1e3) / 1e9, 1)}
-                     for lm, lb in launches],
-        "roofline": {"kernel": "k_shrink_batch" if launches.index(dom) < n_shrink_launches else "k_reduce_batch", "bound": "hbm",
+        "launches": [{"kernel": f"k_{fam}_batch", "ms": round(lm, 4), "bytes": lb,
+                      "GB/s": round(lb / (lm / 1e3) / 1e9, 1)} for lm, lb, fam in launches],
+        "roofline": {"kernel": f"k_{dom[2]}_batch", "bound": "hbm",
                      "achieved": round(dom[1] / (dom[0] / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(dom[1] / (dom[0] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
         "tuned": {"grid_cap": info["grid_cap"], "reduce_first_tile": info["reduce_first_tile"],

@@ -152,6 +152,10 @@ int vgpu_pipeline_thumbnail(vgpu_pipeline *p, int target, uint32_t width, uint32
  * (builder/resample.rs:637-711; resize.rs:163-284), downscale only. */
 int vgpu_pipeline_resize(vgpu_pipeline *p, double hscale, double vscale, int kernel);
 
+/* Kernel choice for u8 ReduceH -> ReduceV and GaussBlurH -> GaussBlurV
+ * pairs: the int8-limb MFMA kernel (default, enable = 1) or the plain integer
+ * kernels (enable = 0); both are bit-exact.  No reference counterpart. */
+int vgpu_pipeline_set_integer_mfma(vgpu_pipeline *p, int enable);
 /* CompiledPipeline output geometry (pipeline/compiled.rs:338-). */
 int vgpu_pipeline_output_info(const vgpu_pipeline *p, uint32_t *width, uint32_t *height,
                               uint32_t *bands, int *format);
@@ -240,7 +244,7 @@ typedef struct {
     uint32_t images;
     uint32_t launches;         /* persistent launches per run (batched levels) */
     uint32_t shrink_tasks;     /* blocks of work in the fused-shrink launches */
-    uint32_t reduce_tasks;     /* blocks of work in the fused-reduce launches */
+    uint32_t reduce_tasks;     /* fused-reduce blocks plus MFMA wavefront tasks */
     uint32_t fallback_images;  /* images run one stage launch at a time */
     int32_t grid_cap;          /* persistent grid size, 0 = one block per task */
     int32_t use_graph;
@@ -261,18 +265,24 @@ int vgpu_batch_info(const vgpu_batch *b, vgpu_batch_stats *stats);
 /* Launch shape: persistent grid size (0 = one block per task), HIP graph on
  * or off, largest reduce tile candidate (re-plans the batch). */
 int vgpu_batch_configure(vgpu_batch *b, int grid_cap, int use_graph, int reduce_first_tile);
+/* u8 reduce and Gaussian pairs on the int8-limb MFMA kernel (default on) or
+ * on the plain integer kernels; both are bit-exact.  Re-plans the batch. */
+int vgpu_batch_set_integer_mfma(vgpu_batch *b, int enable);
 /* Enqueue the whole batch on the context stream.  srcs[i] / dsts[i] are device
  * pointers (sources 16-byte aligned) with byte lengths src_bytes[i] /
  * dst_bytes[i]; runs of one batch must stay on one stream. */
 int vgpu_batch_run(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,
                    void *const *dsts, const size_t *dst_bytes);
 /* Runs the batched launches (no graph, no stage-by-stage images)
- * `iterations` times with HIP events between launches; writes the mean ms and
- * the algorithmic bytes (input buffers + outputs of its jobs) of each launch.
- * Returns the launch count or a negative status. */
+ * `iterations` times with HIP events between launches; writes the mean ms,
+ * the algorithmic bytes (input buffers + outputs of its jobs) and the kernel
+ * family (0 = fused shrink, 1 = fused reduce, 2 = int8-limb MFMA separable
+ * pair) of each launch; launch_bytes and
+ * launch_family may be NULL.  Returns the launch count or a negative status. */
 int vgpu_batch_run_profiled(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,
                             void *const *dsts, const size_t *dst_bytes, int iterations,
-                            float *launch_ms, uint64_t *launch_bytes, int max_launches);
+                            float *launch_ms, uint64_t *launch_bytes, int *launch_family,
+                            int max_launches);
 /* Timing harness, as vgpu_pipeline_benchmark. */
 int vgpu_batch_benchmark(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,
                          void *const *dsts, const size_t *dst_bytes, int warmup, int iterations,

@@ -207,6 +207,39 @@ def test_alpha_fusion_forms(ctx, bands):
         assert_same(got, exp, f"bands {bands} {steps}")
 
 
+U8_PAIR_CASES = [
+    [("reduce_h", (3.7, v.LANCZOS3)), ("reduce_v", (3.7, v.LANCZOS3))],
+    [("reduce_h", (1.0779, v.LANCZOS3)), ("reduce_v", (1.9, v.LANCZOS3))],
+    [("reduce_v", (2.6, v.BICUBIC)), ("reduce_h", (1.2, v.LANCZOS2))],
+    [("reduce_h", (5.5, v.BILINEAR))],
+    [("reduce_v", (4.2, v.LANCZOS3))],
+    [("gauss_blur", (0.8,))],
+    [("gauss_blur", (2.0,))],
+    [("gauss_blur", (4.5,))],
+    [("thumbnail", (v.FITBOX, 150, 150, v.LANCZOS3)), ("gauss_blur", (0.8,))],
+]
+
+
+@pytest.mark.parametrize("mfma", [True, False])
+@pytest.mark.parametrize("bands", [1, 2, 3, 4])
+def test_u8_pairs_mfma_and_plain(ctx, mfma, bands):
+    # u8 ReduceH -> ReduceV and GaussBlurH -> GaussBlurV run on the int8-limb
+    # MFMA kernel by default and on the plain integer kernels when it is
+    # switched off; both must equal the oracle bit for bit, edges included.
+    src = noise_u8(211, 389, bands, 60 + bands)
+    if bands in (2, 4):
+        src[..., -1] = np.where(np.random.default_rng(bands).random((211, 389)) < 0.1, 0, src[..., -1])
+    cases = list(U8_PAIR_CASES)
+    if bands in (2, 4):
+        cases.append([("premultiply", ()), ("reduce_h", (1.7, v.LANCZOS3)), ("reduce_v", (1.7, v.LANCZOS3)),
+                      ("unpremultiply", ())])
+    for steps in cases:
+        h, w, b = src.shape
+        gp = build(v.Pipeline(ctx, w, h, b), steps).set_integer_mfma(mfma)
+        exp = build(O.Pipeline(w, h, b), steps).run(src)
+        assert_same(gp.run_host(src), exp, f"mfma={mfma} bands {bands} {steps}")
+
+
 @pytest.mark.parametrize("w,h,b", [(1, 1, 3), (2, 1, 1), (1, 5, 4), (7, 3, 2), (19, 19, 3), (20, 1000, 1)])
 def test_tiny_and_ragged(ctx, w, h, b):
     src = noise_u8(h, w, b, w * 31 + h)

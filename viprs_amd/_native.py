@@ -103,6 +103,8 @@ def load() -> C.CDLL:
         "vgpu_op_output_size": (i32, [vp, u32, u32, P(u32), P(u32)]),
         "vgpu_op_output_format": (i32, [vp, P(i32)]),
         "vgpu_op_process_region": (i32, [vp, vp, sz, vp, sz, Region, Region]),
+        "vgpu_pipeline_set_integer_mfma": (i32, [vp, i32]),
+        "vgpu_batch_set_integer_mfma": (i32, [vp, i32]),
         "vgpu_fill_random_u8": (i32, [vp, vp, sz, C.c_uint64]),
         "vgpu_batch_thumbnail_new": (i32, [vp, u32, P(u32), P(u32), P(u32), i32, u32, u32, i32, P(vp)]),
         "vgpu_batch_destroy": (None, [vp]),
@@ -111,7 +113,7 @@ def load() -> C.CDLL:
         "vgpu_batch_info": (i32, [vp, P(BatchStats)]),
         "vgpu_batch_configure": (i32, [vp, i32, i32, i32]),
         "vgpu_batch_run": (i32, [vp, P(vp), P(sz), P(vp), P(sz)]),
-        "vgpu_batch_run_profiled": (i32, [vp, P(vp), P(sz), P(vp), P(sz), i32, P(f32), P(C.c_uint64), i32]),
+        "vgpu_batch_run_profiled": (i32, [vp, P(vp), P(sz), P(vp), P(sz), i32, P(f32), P(C.c_uint64), P(i32), i32]),
         "vgpu_batch_benchmark": (i32, [vp, P(vp), P(sz), P(vp), P(sz), i32, i32, P(f32), P(f32)]),
         "vgpu_batch_tune": (i32, [vp, P(vp), P(sz), P(vp), P(sz), i32, P(f32)]),
     }

@@ -243,6 +243,12 @@ class Pipeline:
         w, h, b, _ = self.output_info()
         return (h, w, b)
 
+    def set_integer_mfma(self, enable: bool):
+        """u8 reduce / Gaussian pairs on the int8-limb MFMA kernel (default) or
+        the plain integer kernels; both are bit-exact."""
+        _check(self._lib.vgpu_pipeline_set_integer_mfma(self._h, int(enable)), "set_integer_mfma")
+        return self
+
     def stage_count(self) -> int:
         n = self._lib.vgpu_pipeline_stage_count(self._h)
         if n < 0:
@@ -339,6 +345,9 @@ class Batch:
         _check(self._lib.vgpu_batch_configure(self._h, grid_cap, int(use_graph), reduce_first_tile),
                "vgpu_batch_configure")
 
+    def set_integer_mfma(self, enable: bool) -> None:
+        _check(self._lib.vgpu_batch_set_integer_mfma(self._h, int(enable)), "vgpu_batch_set_integer_mfma")
+
     def _tables(self, srcs, dsts):
         n = len(self)
         if len(srcs) != n or len(dsts) != n:
@@ -361,13 +370,16 @@ class Batch:
         return total.value / iterations, np.array(steps[:], np.float64)
 
     def run_profiled(self, srcs, dsts, iterations: int = 1):
-        """[(mean ms, algorithmic bytes)] per batched launch (HIP events)."""
+        """[(mean ms, algorithmic bytes, family)] per batched launch (HIP
+        events); family "shrink" or "reduce"."""
         ms = (C.c_float * 64)()
         nb = (C.c_uint64 * 64)()
-        n = self._lib.vgpu_batch_run_profiled(self._h, *self._tables(srcs, dsts), iterations, ms, nb, 64)
+        fam = (C.c_int * 64)()
+        n = self._lib.vgpu_batch_run_profiled(self._h, *self._tables(srcs, dsts), iterations, ms, nb, fam, 64)
         if n < 0:
             raise VgpuError(n, "vgpu_batch_run_profiled")
-        return [(ms[i], int(nb[i])) for i in range(n)]
+        names = {0: "shrink", 1: "reduce", 2: "sep_mfma"}
+        return [(ms[i], int(nb[i]), names[fam[i]]) for i in range(n)]
 
     def tune(self, srcs, dsts, iterations: int = 3) -> float:
         best = C.c_float()

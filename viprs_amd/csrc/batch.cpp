@@ -41,7 +41,7 @@ struct BatchImage {
 
 // One persistent launch over the tasks of one (level, kernel family).
 struct Launch {
-    int family = 0;  // 0: fused shrink, 1: fused reduce
+    int family = 0;  // 0: fused shrink, 1: fused reduce (VALU), 2: MFMA separable pair
     int variant = 0; // kernel variant within the family
     size_t jobs_off = 0, tasks_off = 0;
     int ntasks = 0;
@@ -161,7 +161,7 @@ int vgpu_batch::plan() {
             arena_need = align_up(arena_need + (size_t)s.out.w * s.out.h * im.bands * format_size(s.out_fmt), 256);
         }
         for (const Stage& s : im.stages)
-            if (s.kind != ST_SHRINK_HV_U8 && s.kind != ST_REDUCE_HV) im.batched = false;
+            if (s.kind != ST_SHRINK_HV_U8 && s.kind != ST_REDUCE_HV && s.kind != ST_SEP) im.batched = false;
         levels = std::max(levels, (int)im.stages.size());
     }
     if (!st.bytes.empty()) {
@@ -198,6 +198,11 @@ int vgpu_batch::plan() {
         std::vector<BatchTask> rtasks;
         size_t rlds = 0;
         uint64_t rbytes = 0;
+        constexpr int kSepVariants = 4;  // by band count
+        std::vector<SepJob> pjobs[kSepVariants];
+        std::vector<BatchTask> ptasks[kSepVariants];
+        size_t plds[kSepVariants] = {};
+        uint64_t pbytes[kSepVariants] = {};
         for (int idx : order) {
             BatchImage& im = images[(size_t)idx];
             if (!im.batched || lv >= (int)im.stages.size()) continue;
@@ -223,6 +228,17 @@ int vgpu_batch::plan() {
                 sbytes[var] += io_bytes;
                 for (int by = 0; by < j.gy; ++by)
                     for (int bx = 0; bx < j.gx; ++bx) stasks[var].push_back(BatchTask{jid, bx, by});
+            } else if (s.kind == ST_SEP) {
+                SepJob j = sep_job_of(s, in, out, dblob);
+                j.src_slot = first ? idx : -1;
+                j.dst_slot = last ? idx : -1;
+                const int var = j.bands - 1;
+                const int jid = (int)pjobs[var].size();
+                pjobs[var].push_back(j);
+                pbytes[var] += io_bytes;
+                plds[var] = std::max(plds[var], sep_job_wave_lds(j));
+                for (int ty = 0; ty < j.gy; ++ty)
+                    for (int jx = 0; jx < j.gx; ++jx) ptasks[var].push_back(BatchTask{jid, jx, ty});
             } else {
                 ReduceJob j = reduce_job_of(s, in, out, dblob);
                 j.src_slot = first ? idx : -1;
@@ -247,6 +263,19 @@ int vgpu_batch::plan() {
             l.alg_bytes = sbytes[var];
             launches.push_back(l);
             n_shrink_tasks += l.ntasks;
+        }
+        for (int var = 0; var < kSepVariants; ++var) {
+            if (ptasks[var].empty()) continue;
+            Launch l;
+            l.family = 2;
+            l.variant = var + 1;
+            l.jobs_off = tbl.put(pjobs[var].data(), pjobs[var].size() * sizeof(SepJob));
+            l.tasks_off = tbl.put(ptasks[var].data(), ptasks[var].size() * sizeof(BatchTask));
+            l.ntasks = (int)ptasks[var].size();
+            l.lds = plds[var];  // per wavefront
+            l.alg_bytes = pbytes[var];
+            launches.push_back(l);
+            n_reduce_tasks += l.ntasks;
         }
         if (!rtasks.empty()) {
             Launch l;
@@ -291,10 +320,15 @@ int vgpu_batch::launch_all(hipStream_t s, hipEvent_t* marks) {
     void* const* dsts = reinterpret_cast<void* const*>(t + slots_off + images.size() * sizeof(void*));
     for (size_t li = 0; li < launches.size(); ++li) {
         const Launch& l = launches[li];
-        const int grid = grid_cap > 0 ? std::min(l.ntasks, grid_cap) : l.ntasks;
+        // A block takes one task, or (MFMA family) one task per wavefront.
+        const int blocks = l.family == 2 ? (l.ntasks + 3) / 4 : l.ntasks;
+        const int grid = grid_cap > 0 ? std::min(blocks, grid_cap) : blocks;
         const BatchTask* tasks = reinterpret_cast<const BatchTask*>(t + l.tasks_off);
         hipError_t e;
-        if (l.family == 0)
+        if (l.family == 2)
+            e = launch_sep_batch(l.variant, reinterpret_cast<const SepJob*>(t + l.jobs_off), tasks, l.ntasks, srcs,
+                                 dsts, grid, l.lds, s);
+        else if (l.family == 0)
             e = launch_shrink_batch(l.variant, reinterpret_cast<const ShrinkJob*>(t + l.jobs_off), tasks, l.ntasks,
                                     srcs, dsts, grid, s);
         else
@@ -342,6 +376,7 @@ int vgpu_batch_thumbnail_new(vgpu_context* ctx, uint32_t count, const uint32_t* 
     b->ctx = ctx;
     b->opt.reduce_min_blocks = 1;  // many images: the largest tiles that fit
     b->opt.shrink_row_split = false;
+    b->opt.sep_rows_per_task = 64;
     b->images.resize(count);
     for (uint32_t i = 0; i < count; ++i) {
         if (bands[i] < 1 || bands[i] > 4 || widths[i] == 0 || heights[i] == 0) return VGPU_E_INVALID_ARGUMENT;
@@ -420,6 +455,14 @@ int vgpu_batch_configure(vgpu_batch* b, int grid_cap, int use_graph, int reduce_
     return VGPU_OK;
 }
 
+int vgpu_batch_set_integer_mfma(vgpu_batch* b, int enable) {
+    if (!b) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(b->ctx->device);
+    if (hipStreamSynchronize(b->ctx->stream) != hipSuccess) return b->ctx->sticky = VGPU_E_DEVICE;
+    b->opt.integer_mfma = enable != 0;
+    return b->plan();
+}
+
 int vgpu_batch_run(vgpu_batch* b, const void* const* srcs, const size_t* src_bytes, void* const* dsts,
                    const size_t* dst_bytes) {
     if (!b) return VGPU_E_INVALID_ARGUMENT;
@@ -464,7 +507,7 @@ int vgpu_batch_run(vgpu_batch* b, const void* const* srcs, const size_t* src_byt
 
 int vgpu_batch_run_profiled(vgpu_batch* b, const void* const* srcs, const size_t* src_bytes, void* const* dsts,
                             const size_t* dst_bytes, int iterations, float* launch_ms, uint64_t* launch_bytes,
-                            int max_launches) {
+                            int* launch_family, int max_launches) {
     if (!b || iterations < 1 || !launch_ms) return VGPU_E_INVALID_ARGUMENT;
     vgpu_context* ctx = b->ctx;
     if (ctx->sticky != VGPU_OK) return ctx->sticky;
@@ -494,6 +537,7 @@ int vgpu_batch_run_profiled(vgpu_batch* b, const void* const* srcs, const size_t
     for (int k = 0; k < nl; ++k) {
         launch_ms[k] = (float)(acc[(size_t)k] / iterations);
         if (launch_bytes) launch_bytes[k] = b->launches[(size_t)k].alg_bytes;
+        if (launch_family) launch_family[k] = b->launches[(size_t)k].family;
     }
     if (rc != VGPU_OK) return ctx->sticky = rc;
     return nl;

@@ -67,6 +67,26 @@ struct ReduceJob {
     int src_slot = -1, dst_slot = -1;
 };
 
+// Separable u8 pass pair on MFMA (int8 limbs): H then V with a u8
+// intermediate, each pass a banded contraction on v_mfma_i32_16x16x64_i8.
+// Used for fused [Premultiply] -> ReduceH -> ReduceV -> [Unpremultiply]
+// (mode 0: (sum + 2048) >> 12) and for GaussBlurH -> GaussBlurV (mode 1:
+// (sum + scale/2) / scale).  ph.start / pv.start are absolute first-tap
+// positions per output column / row (ph indexed from out.x), phases index
+// 65 x taps (or 1 x taps) i16 weight rows.  A wavefront task is one
+// 16-sample column strip (jx) x rows_per_task output rows (ty).
+struct SepJob {
+    InView in;
+    OutView out;
+    DevPlan ph, pv;
+    int bands = 3, mode = 0, scale = 1;
+    int vclip0 = 0, vclip1 = 0;
+    int kh = 1, kv = 1, ring = 128, rows_per_task = 16;
+    int premult = 0, unpremult = 0;
+    int gx = 0, gy = 0;  // column strips, row chunks
+    int src_slot = -1, dst_slot = -1;
+};
+
 struct BatchTask {
     int job, bx, by;
 };
@@ -92,6 +112,13 @@ hipError_t launch_reduce_job(const ReduceJob& job, hipStream_t s);
 hipError_t launch_reduce_batch(const ReduceJob* jobs, const BatchTask* tasks, int ntasks,
                                const void* const* srcs, void* const* dsts, int grid, size_t lds,
                                hipStream_t s);
+// LDS bytes one wavefront of the MFMA separable kernel needs for this job.
+size_t sep_job_wave_lds(const SepJob& job);
+hipError_t launch_sep_job(const SepJob& job, hipStream_t s);
+// Batch form: tasks are (job, jx, ty) per wavefront; every job has `bands`.
+hipError_t launch_sep_batch(int bands, const SepJob* jobs, const BatchTask* tasks, int ntasks,
+                            const void* const* srcs, void* const* dsts, int grid, size_t wave_lds,
+                            hipStream_t s);
 hipError_t launch_conv(const InView& in, const OutView& out, int bands, int in_fmt, int out_fmt,
                        int kind, int vertical, const int16_t* itaps, int32_t iscale,
                        const double* wts, int ntaps, double cscale, double coffset,

@@ -85,6 +85,57 @@ const ReduceAxis& identity_axis() {
     return ax;
 }
 
+// Buffer-relative clamped column / row, as the device's col_of / row_of.
+int64_t clamp_rel(int64_t x, int64_t c0, int64_t cn, int64_t b0, int64_t bn) {
+    x = std::min(std::max(x, c0), c0 + std::max<int64_t>(cn, 1) - 1);
+    x = std::min(std::max(x, b0), b0 + std::max<int64_t>(bn, 1) - 1);
+    return x - b0;
+}
+
+// Shape of the int8-limb MFMA job for a u8 H -> V pair: K steps of the H span
+// per 16-sample strip and of the V window per 16 output rows (each at most
+// two 64-wide steps), ring size and task split.  Returns false when a window
+// is too wide for the kernel.
+bool plan_sep(const std::vector<int32_t>& hs, int htaps, const std::vector<int32_t>& vs, int vtaps,
+              uint32_t bands, const Rect& out, const Rect& clip, const Rect& buf, const Rect& vclip,
+              const StageOptions& opt, SepJob* j) {
+    const int64_t nsamp = out.w * (int64_t)bands;
+    if (nsamp <= 0 || out.h <= 0 || bands < 1 || bands > 4) return false;
+    int64_t span = 0;
+    for (int64_t s0 = 0; s0 < nsamp; s0 += 16) {
+        const int64_t s1 = std::min<int64_t>(s0 + 15, nsamp - 1);
+        const int64_t c0 = clamp_rel(hs[(size_t)(s0 / bands)], clip.x, clip.w, buf.x, buf.w);
+        const int64_t c1 = clamp_rel(hs[(size_t)(s1 / bands)] + htaps - 1, clip.x, clip.w, buf.x, buf.w);
+        span = std::max(span, (c1 - c0 + 1) * (int64_t)bands);
+    }
+    const int64_t v0 = vclip.y, v1 = vclip.y + std::max<int64_t>(vclip.h, 1) - 1;
+    int64_t window = 0;
+    for (int64_t y0 = 0; y0 < out.h; y0 += 16) {
+        const int64_t y1 = std::min<int64_t>(y0 + 15, out.h - 1);
+        const int64_t lo = std::min(std::max<int64_t>(vs[(size_t)y0], v0), v1);
+        const int64_t hi = std::min(std::max<int64_t>(vs[(size_t)y1] + vtaps - 1, v0), v1);
+        window = std::max(window, hi - (lo & ~(int64_t)15) + 1);
+    }
+    const int kh = (int)((span + 63) / 64), kv = (int)((window + 63) / 64);
+    if (kh < 1 || kh > 2 || kv < 1 || kv > 2) return false;
+    j->bands = (int)bands;
+    j->kh = kh;
+    j->kv = kv;
+    j->ring = kv == 1 ? 128 : 256;
+    j->vclip0 = (int)v0;
+    j->vclip1 = (int)v1;
+    j->gx = (int)((nsamp + 15) / 16);
+    int rpt = opt.sep_rows_per_task;
+    if (rpt <= 0) {
+        // One image alone: long runs while about 2048 wavefronts remain.
+        rpt = 128;
+        while (rpt > 16 && (int64_t)j->gx * ((out.h + rpt - 1) / rpt) < 2048) rpt /= 2;
+    }
+    j->rows_per_task = rpt;
+    j->gy = (int)((out.h + rpt - 1) / rpt);
+    return true;
+}
+
 void stage_reduce_params(const Node& nd, const Rect& out, Staging* st, Stage* s) {
     std::vector<int32_t> starts;
     std::vector<uint8_t> phases;
@@ -105,11 +156,17 @@ void stage_reduce_params(const Node& nd, const Rect& out, Staging* st, Stage* s)
 // a missing pass.  Returns false when no tile fits (very large factors).
 bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const ReduceAxis& vax,
                             int fmt, uint32_t bands, const Rect& mid, const Rect& out,
-                            const Rect& vclip, const StageOptions& opt, Staging* st, Stage* s) {
+                            const Rect& vclip, const Rect& clip, const Rect& buf,
+                            const StageOptions& opt, Staging* st, Stage* s) {
     std::vector<int32_t> hs, vs;
     std::vector<uint8_t> hp, vp;
     build_plan(hax, mid.x, mid.w, accumulate_h, &hs, &hp);
     build_plan(vax, out.y, out.h, false, &vs, &vp);
+    if (fmt == U8 && opt.integer_mfma &&
+        plan_sep(hs, hax.taps, vs, vax.taps, bands, out, clip, buf, vclip, opt, &s->sep)) {
+        s->sep.mode = 0;
+        s->kind = ST_SEP;
+    }
     const size_t esz = format_size(fmt);
     // Tile candidates, largest first: take the first whose staged rows fit
     // 64 KiB of LDS and that still gives >= opt.reduce_min_blocks blocks;
@@ -136,7 +193,10 @@ bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const Redu
         }
         if (blocks >= opt.reduce_min_blocks) break;
     }
-    if (best_blocks < 0) return false;
+    if (best_blocks < 0) {
+        if (s->kind != ST_SEP) return false;
+        tx = ty = nr = 1;  // the VALU tile is not used
+    }
     s->plan_start = st->put(hs.data(), hs.size() * sizeof(int32_t));
     s->plan_phase = st->put(hp.data(), hp.size());
     s->w16 = st->put(hax.wi16.data(), hax.wi16.size() * sizeof(int16_t));
@@ -168,6 +228,33 @@ bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const Redu
             worst = std::max(worst, sabs * maxv);
         }
     j.wide_acc = worst >= (int64_t)INT32_MAX - 4096;
+    return true;
+}
+
+// GaussBlurH -> GaussBlurV on u8 as one MFMA stage: every output column /
+// row starts its taps `radius` before itself, one phase, the integer taps.
+bool stage_gauss_sep_params(const Node& gh, const Node& gv, const Rect& mid, const Rect& out,
+                            const Rect& vclip, const Rect& clip, const Rect& buf,
+                            const StageOptions& opt, Staging* st, Stage* s) {
+    const int taps = (int)gh.itaps.size();
+    const int rad = (taps - 1) / 2;
+    if (taps < 1 || gv.itaps.size() != gh.itaps.size() || gh.iscale != gv.iscale || gh.iscale < 1) return false;
+    std::vector<int32_t> hs((size_t)std::max<int64_t>(mid.w, 1)), vs((size_t)std::max<int64_t>(out.h, 1));
+    for (int64_t x = 0; x < mid.w; ++x) hs[(size_t)x] = (int32_t)(mid.x + x - rad);
+    for (int64_t y = 0; y < out.h; ++y) vs[(size_t)y] = (int32_t)(out.y + y - rad);
+    if (!plan_sep(hs, taps, vs, taps, gh.bands, out, clip, buf, vclip, opt, &s->sep)) return false;
+    const std::vector<uint8_t> hp(hs.size(), 0), vp(vs.size(), 0);
+    s->plan_start = st->put(hs.data(), hs.size() * sizeof(int32_t));
+    s->plan_phase = st->put(hp.data(), hp.size());
+    s->w16 = st->put(gh.itaps.data(), gh.itaps.size() * sizeof(int16_t));
+    s->taps = taps;
+    s->v_start = st->put(vs.data(), vs.size() * sizeof(int32_t));
+    s->v_phase = st->put(vp.data(), vp.size());
+    s->v_w16 = st->put(gv.itaps.data(), gv.itaps.size() * sizeof(int16_t));
+    s->v_taps = taps;
+    s->sep.mode = 1;
+    s->sep.scale = gh.iscale;
+    s->kind = ST_SEP;
     return true;
 }
 
@@ -211,6 +298,23 @@ ReduceJob reduce_job_of(const Stage& s, const InView& in, const OutView& out, co
     return j;
 }
 
+SepJob sep_job_of(const Stage& s, const InView& in, const OutView& out, const unsigned char* blob) {
+    SepJob j = s.sep;
+    j.in = in;
+    j.out = out;
+    j.ph.start = reinterpret_cast<const int32_t*>(blob + s.plan_start);
+    j.ph.phase = reinterpret_cast<const uint8_t*>(blob + s.plan_phase);
+    j.ph.wi16 = reinterpret_cast<const int16_t*>(blob + s.w16);
+    j.ph.taps = s.taps;
+    j.pv.start = reinterpret_cast<const int32_t*>(blob + s.v_start);
+    j.pv.phase = reinterpret_cast<const uint8_t*>(blob + s.v_phase);
+    j.pv.wi16 = reinterpret_cast<const int16_t*>(blob + s.v_w16);
+    j.pv.taps = s.v_taps;
+    j.premult = s.premult;
+    j.unpremult = s.unpremult;
+    return j;
+}
+
 // Launch one stage reading `in` and writing `dst` (dense over s.out).
 int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& in, void* dst,
                  const unsigned char* blob, hipStream_t stream, const StageOptions& opt) {
@@ -246,6 +350,7 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
         break;
     }
     case ST_REDUCE_HV: e = launch_reduce_job(reduce_job_of(s, in, out, blob), stream); break;
+    case ST_SEP: e = launch_sep_job(sep_job_of(s, in, out, blob), stream); break;
     case ST_CONV: {
         const bool gauss = nd.kind == N_GAUSS_H || nd.kind == N_GAUSS_V;
         const bool vert = nd.kind == N_GAUSS_V || nd.kind == N_CONVSEP_V;
@@ -279,6 +384,8 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
         const bool small_bands = nd.bands <= 4;
         const bool u8small = nd.in_fmt == U8 && small_bands;
         const bool alpha_ok = nd.in_fmt == U8 && (nd.bands == 2 || nd.bands == 4);
+        // The stage's input buffer: the source, or the previous stage's output.
+        const Rect buf = i == 0 ? whole(nodes[0].in_w, nodes[0].in_h) : out_r[(size_t)i - 1];
         bool done = false;
         if (u8small && (nd.kind == N_SHRINK_H || nd.kind == N_SHRINK_V ||
                         (alpha_ok && nd.kind == N_PREMULT &&
@@ -319,14 +426,23 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
             const Rect out = out_r[(size_t)last];
             const Rect vclip = vi >= 0 ? in_r[(size_t)vi] : out_r[(size_t)r0];
             if (stage_reduce_hv_params(hax, has_h && rn.in_fmt == U8, vax, rn.in_fmt, rn.bands, mid, out,
-                                       vclip, opt, st, &s)) {
-                s.kind = ST_REDUCE_HV;
+                                       vclip, s.clip, buf, opt, st, &s)) {
+                if (s.kind != ST_SEP) s.kind = ST_REDUCE_HV;
                 s.premult = nd.kind == N_PREMULT ? 1 : 0;
                 s.last = last;
                 if (alpha_ok && kind_at(last + 1) == N_UNPREMULT) {
                     s.unpremult = 1;
                     s.last = last + 1;
                 }
+                done = true;
+            }
+        }
+        if (!done && u8small && opt.integer_mfma && nd.kind == N_GAUSS_H && kind_at(i + 1) == N_GAUSS_V &&
+            nd.out_fmt == U8 && nodes[(size_t)i + 1].out_fmt == U8) {
+            // GaussBlurH -> GaussBlurV (u8 -> u8) on the MFMA kernel.
+            if (stage_gauss_sep_params(nd, nodes[(size_t)i + 1], out_r[(size_t)i], out_r[(size_t)i + 1],
+                                       in_r[(size_t)i + 1], s.clip, buf, opt, st, &s)) {
+                s.last = i + 1;
                 done = true;
             }
         }
@@ -382,7 +498,7 @@ int vgpu_pipeline::compile() {
     std::vector<Rect> out_r, in_r;
     backward_regions(nodes, cur_w, cur_h, &in_r, &out_r);
     Staging st;
-    stages = plan_stages(nodes, in_r, out_r, &st);
+    stages = plan_stages(nodes, in_r, out_r, &st, opt);
     if (!st.bytes.empty()) {
         blob.ctx = ctx;
         blob.ptr = ctx->pool.acquire(st.bytes.size());
@@ -695,6 +811,15 @@ int vgpu_pipeline_output_info(const vgpu_pipeline* p, uint32_t* w, uint32_t* h, 
     return VGPU_OK;
 }
 
+int vgpu_pipeline_set_integer_mfma(vgpu_pipeline* p, int enable) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(p->ctx->device);
+    if (hipStreamSynchronize(p->ctx->stream) != hipSuccess) return p->ctx->sticky = VGPU_E_DEVICE;
+    p->release();
+    p->opt.integer_mfma = enable != 0;
+    return VGPU_OK;
+}
+
 int vgpu_pipeline_stage_count(vgpu_pipeline* p) {
     if (!p) return VGPU_E_INVALID_ARGUMENT;
     (void)hipSetDevice(p->ctx->device);
@@ -724,7 +849,7 @@ int vgpu_pipeline_run(vgpu_pipeline* p, const void* src, size_t src_bytes, void*
     for (size_t k = 0; k < p->stages.size(); ++k) {
         const Stage& s = p->stages[k];
         void* o = (k + 1 == p->stages.size()) ? dst : p->inter[k];
-        rc = launch_stage(p->nodes, s, view_of(cur, cur_rect, s.clip), o, blob, ctx->stream);
+        rc = launch_stage(p->nodes, s, view_of(cur, cur_rect, s.clip), o, blob, ctx->stream, p->opt);
         if (rc != VGPU_OK) {
             ctx->sticky = rc;
             return rc;
@@ -760,7 +885,7 @@ int vgpu_pipeline_run_profiled(vgpu_pipeline* p, const void* src, size_t src_byt
         for (int k = 0; k < ns && rc == VGPU_OK; ++k) {
             const Stage& s = p->stages[(size_t)k];
             void* o = (k + 1 == ns) ? dst : p->inter[(size_t)k];
-            rc = launch_stage(p->nodes, s, view_of(cur, cur_rect, s.clip), o, blob, ctx->stream);
+            rc = launch_stage(p->nodes, s, view_of(cur, cur_rect, s.clip), o, blob, ctx->stream, p->opt);
             (void)hipEventRecord(ev[(size_t)k + 1], ctx->stream);
             cur = o;
             cur_rect = s.out;

@@ -104,6 +104,7 @@ struct Staging {
 };
 
 enum StageKind {
+    ST_SEP,  // fused u8 H -> V pair on MFMA (reduce or Gaussian)
     ST_SHRINK_HV_U8,
     ST_SHRINK,
     ST_REDUCE_H,
@@ -128,6 +129,7 @@ struct Stage {
     size_t v_start = 0, v_phase = 0, v_w16 = 0, v_w64 = 0;
     int v_taps = 1;
     ReduceJob rjob;
+    SepJob sep;  // ST_SEP: job template (pointers set at launch)
 };
 
 // Stage-planning choices that depend on how the stage is launched: a single
@@ -136,6 +138,8 @@ struct Stage {
 struct StageOptions {
     int64_t reduce_min_blocks = 1024;  // stop shrinking reduce tiles once this many blocks exist
     int reduce_first_tile = 0;         // largest reduce tile candidate considered (0 = 64x16)
+    bool integer_mfma = true;          // u8 reduce / Gaussian pairs on the int8-limb MFMA kernel
+    int sep_rows_per_task = 0;         // output rows per MFMA wavefront task (0 = by chip fill)
     bool shrink_row_split = true;      // split a shrink row's source rows over wavefronts
 };
 
@@ -149,6 +153,7 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
                                const StageOptions& opt = StageOptions());
 ShrinkJob shrink_job_of(const Stage& s, const Node& first, const InView& in, const OutView& out);
 ReduceJob reduce_job_of(const Stage& s, const InView& in, const OutView& out, const unsigned char* blob);
+SepJob sep_job_of(const Stage& s, const InView& in, const OutView& out, const unsigned char* blob);
 int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& in, void* dst,
                  const unsigned char* blob, hipStream_t stream, const StageOptions& opt = StageOptions());
 
@@ -163,6 +168,7 @@ struct vgpu_pipeline {
     std::vector<vgpu::Node> nodes;
 
     bool compiled = false;
+    vgpu::StageOptions opt;
     std::vector<vgpu::Stage> stages;
     vgpu::DeviceBlob blob;
     std::vector<void*> inter;  // stage outputs except the last
