@@ -55,7 +55,12 @@ def build(verbose: bool = False) -> pathlib.Path:
             cmd[1:1] = ["-x", "hip"]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True, stderr=None if verbose else subprocess.PIPE)
+        r = subprocess.run(cmd, stderr=None if verbose else subprocess.PIPE, text=True)
+        if r.returncode != 0:
+            # Show the compiler's diagnostics (warnings about the host pass not
+            # knowing -packed-fp32-ops are filtered out).
+            diag = "\n".join(line for line in (r.stderr or "").splitlines() if "packed-fp32-ops" not in line)
+            raise RuntimeError(f"hipcc failed on {src}:\n{diag}")
         objs.append(str(obj))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs]

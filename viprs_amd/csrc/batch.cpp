@@ -198,7 +198,7 @@ int vgpu_batch::plan() {
         std::vector<BatchTask> rtasks;
         size_t rlds = 0;
         uint64_t rbytes = 0;
-        constexpr int kSepVariants = 4;  // by band count
+        constexpr int kSepVariants = 32;  // sep_job_variant
         std::vector<SepJob> pjobs[kSepVariants];
         std::vector<BatchTask> ptasks[kSepVariants];
         size_t plds[kSepVariants] = {};
@@ -232,7 +232,7 @@ int vgpu_batch::plan() {
                 SepJob j = sep_job_of(s, in, out, dblob);
                 j.src_slot = first ? idx : -1;
                 j.dst_slot = last ? idx : -1;
-                const int var = j.bands - 1;
+                const int var = sep_job_variant(j);
                 const int jid = (int)pjobs[var].size();
                 pjobs[var].push_back(j);
                 pbytes[var] += io_bytes;
@@ -268,7 +268,7 @@ int vgpu_batch::plan() {
             if (ptasks[var].empty()) continue;
             Launch l;
             l.family = 2;
-            l.variant = var + 1;
+            l.variant = var;
             l.jobs_off = tbl.put(pjobs[var].data(), pjobs[var].size() * sizeof(SepJob));
             l.tasks_off = tbl.put(ptasks[var].data(), ptasks[var].size() * sizeof(BatchTask));
             l.ntasks = (int)ptasks[var].size();

@@ -79,6 +79,14 @@ struct SepJob {
     InView in;
     OutView out;
     DevPlan ph, pv;
+    // V weights pre-folded on the host per 16 output rows: kv k-steps x
+    // limbs A operands (64 lanes x 16 bytes each; {hi, lo} when limbs = 2),
+    // then 16 int32 accumulator seeds 128 * sum(w) + rounding bias; tile i
+    // starts at vtiles + i * vtile_bytes.
+    const unsigned char* vtiles = nullptr;
+    int vtile_bytes = 0;
+    uint64_t inv = 0;  // mode 1: ceil(2^40 / scale), so n / scale = (n * inv) >> 40
+    int limbs = 2;     // 1 when every folded weight fits int8 (sum |w| <= 127)
     int bands = 3, mode = 0, scale = 1;
     int vclip0 = 0, vclip1 = 0;
     int kh = 1, kv = 1, ring = 128, rows_per_task = 16;
@@ -115,8 +123,10 @@ hipError_t launch_reduce_batch(const ReduceJob* jobs, const BatchTask* tasks, in
 // LDS bytes one wavefront of the MFMA separable kernel needs for this job.
 size_t sep_job_wave_lds(const SepJob& job);
 hipError_t launch_sep_job(const SepJob& job, hipStream_t s);
-// Batch form: tasks are (job, jx, ty) per wavefront; every job has `bands`.
-hipError_t launch_sep_batch(int bands, const SepJob* jobs, const BatchTask* tasks, int ntasks,
+// Kernel variant of a job (bands, limbs, mode, premultiply); a batch launch
+// runs jobs of one variant, one (job, jx, ty) task per wavefront.
+int sep_job_variant(const SepJob& job);
+hipError_t launch_sep_batch(int variant, const SepJob* jobs, const BatchTask* tasks, int ntasks,
                             const void* const* srcs, void* const* dsts, int grid, size_t wave_lds,
                             hipStream_t s);
 hipError_t launch_conv(const InView& in, const OutView& out, int bands, int in_fmt, int out_fmt,

@@ -101,6 +101,8 @@ bool plan_sep(const std::vector<int32_t>& hs, int htaps, const std::vector<int32
               const StageOptions& opt, SepJob* j) {
     const int64_t nsamp = out.w * (int64_t)bands;
     if (nsamp <= 0 || out.h <= 0 || bands < 1 || bands > 4) return false;
+    // The kernel addresses its input with 32-bit byte offsets.
+    if (buf.w * buf.h * (int64_t)bands >= ((int64_t)1 << 31)) return false;
     int64_t span = 0;
     for (int64_t s0 = 0; s0 < nsamp; s0 += 16) {
         const int64_t s1 = std::min<int64_t>(s0 + 15, nsamp - 1);
@@ -136,6 +138,72 @@ bool plan_sep(const std::vector<int32_t>& hs, int htaps, const std::vector<int32
     return true;
 }
 
+// One int8 limb suffices when every folded weight fits int8: sum |w| <= 127
+// over every phase row of both axes.
+int sep_limbs(const int16_t* wh, int hrows, int htaps, const int16_t* wv, int vrows, int vtaps) {
+    for (int pass = 0; pass < 2; ++pass) {
+        const int16_t* w = pass == 0 ? wh : wv;
+        const int rows = pass == 0 ? hrows : vrows, taps = pass == 0 ? htaps : vtaps;
+        for (int r = 0; r < rows; ++r) {
+            int sum = 0;
+            for (int t = 0; t < taps; ++t) sum += std::abs((int)w[(size_t)r * taps + t]);
+            if (sum > 127) return 2;
+        }
+    }
+    return 1;
+}
+
+// V weights of the MFMA kernel folded per 16 output rows: for output row m of
+// the tile, every tap adds its weight at K = clamp(start + t, vclip) - base16
+// (base16: the tile's first clamped row, rounded down to 16); the folded
+// weights are split into int8 limbs (w = 128 a + c, c in [-64, 63]) and laid
+// out as the A operands of kv k-steps (lane m + 16 q holds K = 64 s + 16 q ..
+// + 15), followed by the 16 per-row weight sums.
+void build_vtiles(const std::vector<int32_t>& vs, const std::vector<uint8_t>& vp, const int16_t* wtab,
+                  int taps, int64_t out_h, const SepJob& j, std::vector<unsigned char>* tiles) {
+    const int kv = j.kv, limbs = j.limbs;
+    const int32_t bias = j.mode == 0 ? 2048 : j.scale / 2;
+    const size_t ops = (size_t)kv * limbs * 64 * 16;
+    const size_t tile_bytes = ops + 16 * sizeof(int32_t);
+    const int64_t ntiles = (out_h + 15) / 16;
+    tiles->assign((size_t)ntiles * tile_bytes, 0);
+    std::vector<int32_t> w((size_t)16 * 64 * kv);
+    for (int64_t ti = 0; ti < ntiles; ++ti) {
+        std::fill(w.begin(), w.end(), 0);
+        int32_t corr[16] = {};
+        const int64_t y0 = ti * 16;
+        const int64_t nrow = std::min<int64_t>(16, out_h - y0);
+        const int64_t lo = std::min<int64_t>(std::max<int64_t>(vs[(size_t)y0], j.vclip0), j.vclip1);
+        const int64_t base16 = lo & ~(int64_t)15;
+        for (int64_t m = 0; m < nrow; ++m) {
+            const int16_t* row = wtab + (size_t)vp[(size_t)(y0 + m)] * taps;
+            for (int t = 0; t < taps; ++t) {
+                const int64_t r = std::min<int64_t>(std::max<int64_t>(vs[(size_t)(y0 + m)] + t, j.vclip0), j.vclip1);
+                w[(size_t)(m * 64 * kv + (r - base16))] += row[t];
+                corr[m] += row[t];
+            }
+        }
+        unsigned char* dst = tiles->data() + (size_t)ti * tile_bytes;
+        for (int s = 0; s < kv; ++s)
+            for (int lane = 0; lane < 64; ++lane)
+                for (int e = 0; e < 16; ++e) {
+                    const int m = lane & 15, q = lane >> 4;
+                    const int32_t v = w[(size_t)(m * 64 * kv + 64 * s + 16 * q + e)];
+                    if (limbs == 1) {
+                        dst[((size_t)s * 64 + lane) * 16 + e] = (unsigned char)(v & 255);
+                    } else {
+                        const int32_t c = ((v + 64) & 127) - 64;
+                        const int32_t a = (v - c) >> 7;
+                        dst[((size_t)(2 * s) * 64 + lane) * 16 + e] = (unsigned char)(a & 255);
+                        dst[((size_t)(2 * s + 1) * 64 + lane) * 16 + e] = (unsigned char)(c & 255);
+                    }
+                }
+        int32_t seed[16];
+        for (int m = 0; m < 16; ++m) seed[m] = 128 * corr[m] + bias;
+        std::memcpy(dst + ops, seed, sizeof(seed));
+    }
+}
+
 void stage_reduce_params(const Node& nd, const Rect& out, Staging* st, Stage* s) {
     std::vector<int32_t> starts;
     std::vector<uint8_t> phases;
@@ -165,6 +233,11 @@ bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const Redu
     if (fmt == U8 && opt.integer_mfma &&
         plan_sep(hs, hax.taps, vs, vax.taps, bands, out, clip, buf, vclip, opt, &s->sep)) {
         s->sep.mode = 0;
+        s->sep.limbs = sep_limbs(hax.wi16.data(), kPhases + 1, hax.taps, vax.wi16.data(), kPhases + 1, vax.taps);
+        std::vector<unsigned char> tiles;
+        build_vtiles(vs, vp, vax.wi16.data(), vax.taps, out.h, s->sep, &tiles);
+        s->v_tiles = st->put(tiles.data(), tiles.size());
+        s->sep.vtile_bytes = (int)(tiles.size() / (size_t)((out.h + 15) / 16));
         s->kind = ST_SEP;
     }
     const size_t esz = format_size(fmt);
@@ -238,7 +311,9 @@ bool stage_gauss_sep_params(const Node& gh, const Node& gv, const Rect& mid, con
                             const StageOptions& opt, Staging* st, Stage* s) {
     const int taps = (int)gh.itaps.size();
     const int rad = (taps - 1) / 2;
-    if (taps < 1 || gv.itaps.size() != gh.itaps.size() || gh.iscale != gv.iscale || gh.iscale < 1) return false;
+    if (taps < 1 || gv.itaps.size() != gh.itaps.size() || gh.iscale != gv.iscale || gh.iscale < 1 ||
+        gh.iscale >= 65536)
+        return false;
     std::vector<int32_t> hs((size_t)std::max<int64_t>(mid.w, 1)), vs((size_t)std::max<int64_t>(out.h, 1));
     for (int64_t x = 0; x < mid.w; ++x) hs[(size_t)x] = (int32_t)(mid.x + x - rad);
     for (int64_t y = 0; y < out.h; ++y) vs[(size_t)y] = (int32_t)(out.y + y - rad);
@@ -254,6 +329,12 @@ bool stage_gauss_sep_params(const Node& gh, const Node& gv, const Rect& mid, con
     s->v_taps = taps;
     s->sep.mode = 1;
     s->sep.scale = gh.iscale;
+    s->sep.limbs = sep_limbs(gh.itaps.data(), 1, taps, gv.itaps.data(), 1, taps);
+    std::vector<unsigned char> tiles;
+    build_vtiles(vs, vp, gv.itaps.data(), taps, out.h, s->sep, &tiles);
+    s->v_tiles = st->put(tiles.data(), tiles.size());
+    s->sep.vtile_bytes = (int)(tiles.size() / (size_t)((out.h + 15) / 16));
+    s->sep.inv = ((uint64_t)1 << 40) / (uint64_t)gh.iscale + (((uint64_t)1 << 40) % (uint64_t)gh.iscale ? 1 : 0);
     s->kind = ST_SEP;
     return true;
 }
@@ -310,6 +391,7 @@ SepJob sep_job_of(const Stage& s, const InView& in, const OutView& out, const un
     j.pv.phase = reinterpret_cast<const uint8_t*>(blob + s.v_phase);
     j.pv.wi16 = reinterpret_cast<const int16_t*>(blob + s.v_w16);
     j.pv.taps = s.v_taps;
+    j.vtiles = blob + s.v_tiles;
     j.premult = s.premult;
     j.unpremult = s.unpremult;
     return j;

@@ -130,6 +130,7 @@ struct Stage {
     int v_taps = 1;
     ReduceJob rjob;
     SepJob sep;  // ST_SEP: job template (pointers set at launch)
+    size_t v_tiles = 0;  // ST_SEP: blob offset of the pre-folded V weight tiles
 };
 
 // Stage-planning choices that depend on how the stage is launched: a single

@@ -93,9 +93,19 @@ __device__ __forceinline__ uint32_t premult_dword(uint32_t v) {
     }
 }
 
-__device__ __forceinline__ int sep_round(int acc, int mode, int scale) {
-    int v = mode == 0 ? (acc + 2048) >> 12 : (acc + (scale >> 1)) / scale;
-    return v < 0 ? 0 : (v > 255 ? 255 : v);
+// Rounding of one exact sum whose bias (2048, or scale / 2) is already in.
+// Mode 1 divides by multiply-shift: with inv = ceil(2^40 / scale),
+// floor(n / scale) = (n * inv) >> 40 whenever n * scale <= 2^40, which holds
+// for Gaussian sums (0 <= n < 256 * scale, scale < 2^16).
+template <int MODE>
+__device__ __forceinline__ uint32_t sep_round(int acc, uint64_t inv) {
+    int v;
+    if constexpr (MODE == 0) {
+        v = acc >> 12;
+    } else {
+        v = (int)(((uint64_t)(uint32_t)max(acc, 0) * inv) >> 40);
+    }
+    return (uint32_t)min(max(v, 0), 255);
 }
 
 // Fold the taps of 16 rows j (lane j + 16 tq adds taps tq, tq + 4, ...) into
@@ -121,6 +131,22 @@ __device__ __forceinline__ void fold_weights(int* scratch, int* corr, int ks, in
     wave_lds_sync();
 }
 
+// Read lane (j, q)'s 16 folded weights of k-step s as one int8 operand
+// (every weight fits int8).
+__device__ __forceinline__ i32x4 read_single(const int* scratch, int ks, int s) {
+    const int lane = threadIdx.x & 63;
+    const int j = lane & 15, q = lane >> 4;
+    const int* p = scratch + j * ks + 64 * s + 16 * q;
+    i32x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const i32x4 x = *reinterpret_cast<const i32x4*>(p + 4 * i);
+        r[i] = (int)((uint32_t)(x[0] & 255) | ((uint32_t)(x[1] & 255) << 8) | ((uint32_t)(x[2] & 255) << 16) |
+                     ((uint32_t)x[3] << 24));
+    }
+    return r;
+}
+
 // Read lane (j, q)'s 16 folded weights of k-step s and split into limbs.
 __device__ __forceinline__ void read_limbs(const int* scratch, int ks, int s, i32x4& hi, i32x4& lo) {
     const int lane = threadIdx.x & 63;
@@ -140,11 +166,53 @@ __device__ __forceinline__ void read_limbs(const int* scratch, int ks, int s, i3
 
 constexpr int kMaxSteps = 2;
 
+// Plan tables are read-only for the kernel's life: constant-address loads
+// go through the scalar cache.
+template <typename T>
+__device__ __forceinline__ T cload(const T* p, int i) {
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
+// A operand of one 16-row M tile: lane (m, q) holds bytes 16q .. 16q + 15 of
+// k-step s of the span of intermediate row r0 + m (pixels as p ^ 0x80).
 template <int B>
+__device__ __forceinline__ void load_a(const unsigned char* base, uint32_t rb, uint32_t total, int q, int kh,
+                                       bool premult, i32x4 (&a)[kMaxSteps]) {
+#pragma unroll
+    for (int s = 0; s < kMaxSteps; ++s) {
+        if (s < kh) {
+            const uint32_t off = rb + 64 * s + 16 * q;
+            uint32_t w[4];
+            if constexpr (B == 4) {
+                // 4-band spans start on pixel (4-byte) boundaries.
+                if (off + 16 <= total) {
+                    const u32x4 v = gmem<u32x4>(base + off)[0];
+                    w[0] = v[0]; w[1] = v[1]; w[2] = v[2]; w[3] = v[3];
+                } else {
+                    load16(base, off, total, w);
+                }
+            } else {
+                load16(base, off, total, w);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                uint32_t v = w[j];
+                if constexpr (B == 2 || B == 4) {
+                    if (premult) v = premult_dword<B>(v);
+                }
+                a[s][j] = (int)(v ^ 0x80808080u);
+            }
+        }
+    }
+}
+
+template <int B, int LIMBS, int MODE, bool PM>
 __device__ void sep_task(const SepJob& job, int jx, int ty, unsigned char* lds, size_t wave_lds) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int ks = 64 * max(job.kh, job.kv);
+    const int KH = job.kh, KV = job.kv, RING = job.ring;
+    const int ks = 64 * KH;
+    const int RS = RING + 16;  // ring row stride: padded against LDS bank conflicts
     int* scratch = reinterpret_cast<int*>(lds + (size_t)wave * wave_lds);
     int* corr = scratch + 16 * ks;
     unsigned char* ring = reinterpret_cast<unsigned char*>(corr + 16);
@@ -152,17 +220,19 @@ __device__ void sep_task(const SepJob& job, int jx, int ty, unsigned char* lds, 
     const OutView& out = job.out;
     const DevPlan& ph = job.ph;
     const DevPlan& pv = job.pv;
-    const int KH = job.kh, KV = job.kv, RING = job.ring;
     const int nsamp = out.w * B;
     const int s0 = jx * 16;
     const int n = lane & 15, q = lane >> 4;
     const unsigned char* base = static_cast<const unsigned char*>(in.ptr);
-    const size_t row_bytes = (size_t)in.bw * B;
-    const size_t total = row_bytes * (size_t)in.bh;
+    const uint32_t row_bytes = (uint32_t)in.bw * B;
+    const uint32_t total = row_bytes * (uint32_t)in.bh;  // < 2^31 (checked on the host)
+    const int bias = MODE == 0 ? 2048 : (job.scale >> 1);
+    const uint64_t inv = job.inv;
+    constexpr bool premult = (B == 2 || B == 4) && PM;
 
     // ---- H weights of the strip: B operand, rows j = samples, K = span bytes.
     const int x_first = s0 / B;
-    const int span0 = col_of(in, ph.start[x_first]);  // buffer-relative first span pixel
+    const int span0 = col_of(in, cload(ph.start, x_first));  // buffer-relative first span pixel
     {
         const int smp = s0 + n;
         const bool active = smp < nsamp;
@@ -174,114 +244,136 @@ __device__ void sep_task(const SepJob& job, int jx, int ty, unsigned char* lds, 
     }
     i32x4 whi[kMaxSteps], wlo[kMaxSteps];
 #pragma unroll
-    for (int s = 0; s < kMaxSteps; ++s)
-        if (s < KH) read_limbs(scratch, ks, s, whi[s], wlo[s]);
-    const int corr_h = 128 * corr[n];
+    for (int s = 0; s < kMaxSteps; ++s) {
+        if (s < KH) {
+            if constexpr (LIMBS == 2) read_limbs(scratch, ks, s, whi[s], wlo[s]);
+            else wlo[s] = read_single(scratch, ks, s);
+        }
+    }
+    const int hseed = 128 * corr[n] + bias;
     wave_lds_sync();
 
     const int y_begin = ty * job.rows_per_task;
     const int y_end = min(out.h, y_begin + job.rows_per_task);
-    const int mode = job.mode, scale = job.scale;
+    const int vclip0 = job.vclip0, vclip1 = job.vclip1, vtaps = pv.taps;
+    const uint32_t span_off = (uint32_t)span0 * B;
+    // V window of the row tile at y0: intermediate rows [lo, hi], K from base16.
+    auto window = [&](int y0, int& hi, int& base16) {
+        const int ylast = min(y0 + 15, y_end - 1);
+        const int lo = clampi(cload(pv.start, y0), vclip0, vclip1);
+        hi = clampi(cload(pv.start, ylast) + vtaps - 1, vclip0, vclip1);
+        base16 = lo & ~15;
+    };
+    auto row_addr = [&](int r) -> uint32_t { return (uint32_t)row_of(in, r + n) * row_bytes + span_off; };
+
+    i32x4 acur[kMaxSteps], anxt[kMaxSteps];
     int have = INT32_MIN;  // intermediate rows below `have` (a multiple of 16) are in the ring
+    int pre = INT32_MIN;   // M tile whose A operand is already in acur
+    int hi, base16;
+    window(y_begin, hi, base16);
     for (int y0 = y_begin; y0 < y_end; y0 += 16) {
-        const int nrow = min(16, y_end - y0);
-        const int lo = clampi(pv.start[y0], job.vclip0, job.vclip1);
-        const int hi = clampi(pv.start[y0 + nrow - 1] + pv.taps - 1, job.vclip0, job.vclip1);
-        const int base16 = lo & ~15;
-        // ---- intermediate rows [max(have, base16), hi], 16 at a time.
-        for (int r0 = have > base16 ? have : base16; r0 <= hi; r0 += 16) {
-            const size_t rb = (size_t)row_of(in, r0 + n) * row_bytes + (size_t)span0 * B;
-            i32x4 dh = {0, 0, 0, 0}, dl = {0, 0, 0, 0};
+        // ---- intermediate rows [max(have, base16), hi], 16 at a time; the
+        // next tile's operand loads are in flight while this one is used.
+        int r0 = have > base16 ? have : base16;
+        if (r0 <= hi && pre != r0) load_a<B>(base, row_addr(r0), total, q, KH, premult, acur);
+        for (; r0 <= hi; r0 += 16) {
+            const bool more = r0 + 16 <= hi;
+            if (more) load_a<B>(base, row_addr(r0 + 16), total, q, KH, premult, anxt);
+            i32x4 dh = {0, 0, 0, 0}, dl = {hseed, hseed, hseed, hseed};
 #pragma unroll
             for (int s = 0; s < kMaxSteps; ++s) {
                 if (s < KH) {
-                    uint32_t a[4];
-                    load16(base, rb + 64 * s + 16 * q, total, a);
-                    i32x4 av;
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        uint32_t v = a[j];
-                        if constexpr (B == 2 || B == 4) {
-                            if (job.premult) v = premult_dword<B>(v);
-                        }
-                        av[j] = (int)(v ^ 0x80808080u);
-                    }
-                    dh = mfma_i8(av, whi[s], dh);
-                    dl = mfma_i8(av, wlo[s], dl);
+                    if constexpr (LIMBS == 2) dh = mfma_i8(acur[s], whi[s], dh);
+                    dl = mfma_i8(acur[s], wlo[s], dl);
                 }
             }
-            // D[m = row][n = sample]: this lane holds rows r0 + 4q .. + 3 of sample n.
+            // D[m = row][n = sample]: rows r0 + 4q .. + 3 of sample n, stored
+            // in the ring already offset (p ^ 0x80) for the V operand.
             uint32_t packed = 0;
 #pragma unroll
-            for (int e = 0; e < 4; ++e)
-                packed |= (uint32_t)sep_round(128 * dh[e] + dl[e] + corr_h, mode, scale) << (8 * e);
-            *reinterpret_cast<uint32_t*>(ring + n * RING + ((r0 + 4 * q) & (RING - 1))) = packed;
-            have = r0 + 16;
-        }
-        // ---- V weights of the 16 output rows: A operand, rows j = output rows,
-        // K = intermediate rows from base16.
-        {
-            const int m = n;
-            const bool active = m < nrow;
-            const int y = active ? y0 + m : y0;
-            const int vs = pv.start[y];
-            const int v0 = job.vclip0, v1 = job.vclip1;
-            fold_weights(scratch, corr, ks, pv.taps, pv.wi16 + (size_t)pv.phase[y] * pv.taps, active,
-                         [&](int t) { return clampi(vs + t, v0, v1) - base16; });
-        }
-        i32x4 vhi[kMaxSteps], vlo[kMaxSteps];
+            for (int e = 0; e < 4; ++e) {
+                const int acc = LIMBS == 2 ? 128 * dh[e] + dl[e] : dl[e];
+                packed |= sep_round<MODE>(acc, inv) << (8 * e);
+            }
+            *reinterpret_cast<uint32_t*>(ring + n * RS + ((r0 + 4 * q) & (RING - 1))) = packed ^ 0x80808080u;
+            if (more) {
 #pragma unroll
-        for (int s = 0; s < kMaxSteps; ++s)
-            if (s < KV) read_limbs(scratch, ks, s, vhi[s], vlo[s]);
-        const i32x4 cv = *reinterpret_cast<const i32x4*>(corr + 4 * q);
-        wave_lds_sync();
-        i32x4 dh = {0, 0, 0, 0}, dl = {0, 0, 0, 0};
+                for (int s = 0; s < kMaxSteps; ++s) acur[s] = anxt[s];
+            }
+        }
+        have = r0 > have ? r0 : have;
+        // ---- V weights of the 16 output rows (A operand), pre-folded on the
+        // host; issued before the next tile's A prefetch.
+        const unsigned char* vt = job.vtiles + (size_t)(y0 >> 4) * job.vtile_bytes;
+        i32x4 vhi[kMaxSteps], vlo[kMaxSteps];
 #pragma unroll
         for (int s = 0; s < kMaxSteps; ++s) {
             if (s < KV) {
-                u32x4 bv = *reinterpret_cast<const u32x4*>(ring + n * RING + ((base16 + 64 * s + 16 * q) & (RING - 1)));
-                i32x4 bi;
+                if constexpr (LIMBS == 2) {
+                    vhi[s] = gmem<i32x4>(vt + ((size_t)(2 * s) * 64 + lane) * 16)[0];
+                    vlo[s] = gmem<i32x4>(vt + ((size_t)(2 * s + 1) * 64 + lane) * 16)[0];
+                } else {
+                    vlo[s] = gmem<i32x4>(vt + ((size_t)s * 64 + lane) * 16)[0];
+                }
+            }
+        }
+        const i32x4 vseed = gmem<i32x4>(vt + (size_t)KV * LIMBS * 64 * 16 + 16 * q)[0];
+        // Prefetch the first new M tile of the next row tile.
+        const int cur_base16 = base16;
+        if (y0 + 16 < y_end) {
+            window(y0 + 16, hi, base16);
+            const int r1 = have > base16 ? have : base16;
+            if (r1 <= hi) {
+                load_a<B>(base, row_addr(r1), total, q, KH, premult, acur);
+                pre = r1;
+            }
+        }
+        i32x4 dh = {0, 0, 0, 0}, dl = vseed;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) bi[j] = (int)(bv[j] ^ 0x80808080u);
-                dh = mfma_i8(vhi[s], bi, dh);
-                dl = mfma_i8(vlo[s], bi, dl);
+        for (int s = 0; s < kMaxSteps; ++s) {
+            if (s < KV) {
+                const i32x4 bv =
+                    *reinterpret_cast<const i32x4*>(ring + n * RS + ((cur_base16 + 64 * s + 16 * q) & (RING - 1)));
+                if constexpr (LIMBS == 2) dh = mfma_i8(vhi[s], bv, dh);
+                dl = mfma_i8(vlo[s], bv, dl);
             }
         }
         // D[m = output row][n = sample]: rows y0 + 4q + e of sample s0 + n.
-        int v[4];
+        uint32_t v[4];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) v[e] = sep_round(128 * dh[e] + dl[e] + 128 * cv[e], mode, scale);
+        for (int e = 0; e < 4; ++e) v[e] = sep_round<MODE>(LIMBS == 2 ? 128 * dh[e] + dl[e] : dl[e], inv);
         if constexpr (B == 2 || B == 4) {
             if (job.unpremult) {
                 // The pixel's alpha sample is in lane (lane | (B - 1)) of this group.
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int a = __shfl(v[e], lane | (B - 1), 64);
-                    if ((n & (B - 1)) != B - 1) v[e] = (int)unpremult_u8((uint32_t)v[e], (uint32_t)a);
+                    const uint32_t a = (uint32_t)__shfl((int)v[e], lane | (B - 1), 64);
+                    if ((n & (B - 1)) != B - 1) v[e] = unpremult_u8(v[e], a);
                 }
             }
         }
         const int smp = s0 + n;
         if (smp < nsamp) {
-            unsigned char* o = static_cast<unsigned char*>(out.ptr) + smp;
+            unsigned char* o = static_cast<unsigned char*>(out.ptr) + (size_t)(y0 + 4 * q) * nsamp + smp;
+            const int rows_left = y_end - (y0 + 4 * q);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int yy = y0 + 4 * q + e;
-                if (yy < y_end) o[(size_t)yy * nsamp] = (unsigned char)v[e];
-            }
+            for (int e = 0; e < 4; ++e)
+                if (e < rows_left) o[(size_t)e * nsamp] = (unsigned char)v[e];
         }
     }
 }
 
-template <int B>
+// One kernel per (bands, limbs, rounding mode, premultiply) so each carries
+// only its own registers.
+template <int B, int LIMBS, int MODE, bool PM>
 __global__ __launch_bounds__(kThreads) void k_sep_job(SepJob job, int wave_lds) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int t = blockIdx.x * kSepWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (t >= job.gx * job.gy) return;
-    sep_task<B>(job, t % job.gx, t / job.gx, lds, (size_t)wave_lds);
+    sep_task<B, LIMBS, MODE, PM>(job, t % job.gx, t / job.gx, lds, (size_t)wave_lds);
 }
 
-template <int B>
+template <int B, int LIMBS, int MODE, bool PM>
 __global__ __launch_bounds__(kThreads) void k_sep_batch(const SepJob* __restrict__ jobs,
                                                         const BatchTask* __restrict__ tasks, int ntasks,
                                                         const void* const* __restrict__ srcs,
@@ -293,47 +385,63 @@ __global__ __launch_bounds__(kThreads) void k_sep_batch(const SepJob* __restrict
         SepJob job = jobs[task.job];
         if (job.src_slot >= 0) job.in.ptr = srcs[job.src_slot];
         if (job.dst_slot >= 0) job.out.ptr = dsts[job.dst_slot];
-        sep_task<B>(job, task.bx, task.by, lds, (size_t)wave_lds);
+        sep_task<B, LIMBS, MODE, PM>(job, task.bx, task.by, lds, (size_t)wave_lds);
     }
 }
 
 }  // namespace
 
 size_t sep_job_wave_lds(const SepJob& job) {
-    const size_t ks = 64 * (size_t)std::max(job.kh, job.kv);
-    return (16 * ks + 16) * sizeof(int) + 16 * (size_t)job.ring;
+    const size_t ks = 64 * (size_t)job.kh;
+    return (16 * ks + 16) * sizeof(int) + 16 * ((size_t)job.ring + 16);
 }
+
+int sep_job_variant(const SepJob& job) {
+    const int pm = (job.bands == 2 || job.bands == 4) && job.premult ? 1 : 0;
+    return (((job.bands - 1) * 2 + (job.limbs - 1)) * 2 + job.mode) * 2 + pm;
+}
+
+// Every (bands, limbs, mode, premultiply) instantiation, indexed by
+// sep_job_variant.
+#define VGPU_SEP_B(X, B)                                                                               \
+    X(B, 1, 0, false) X(B, 1, 0, true) X(B, 1, 1, false) X(B, 1, 1, true) X(B, 2, 0, false) X(B, 2, 0, true) \
+        X(B, 2, 1, false) X(B, 2, 1, true)
+#define VGPU_SEP_ALL(X) VGPU_SEP_B(X, 1) VGPU_SEP_B(X, 2) VGPU_SEP_B(X, 3) VGPU_SEP_B(X, 4)
 
 hipError_t launch_sep_job(const SepJob& job, hipStream_t s) {
     if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;
+    if (job.bands < 1 || job.bands > 4 || job.limbs < 1 || job.limbs > 2 || job.mode < 0 || job.mode > 1)
+        return hipErrorInvalidValue;
     const int wl = (int)sep_job_wave_lds(job);
     const int ntasks = job.gx * job.gy;
     const dim3 grid((unsigned)((ntasks + kSepWaves - 1) / kSepWaves));
     const size_t lds = (size_t)wl * kSepWaves;
-    switch (job.bands) {
-    case 1: hipLaunchKernelGGL(k_sep_job<1>, grid, dim3(kThreads), lds, s, job, wl); break;
-    case 2: hipLaunchKernelGGL(k_sep_job<2>, grid, dim3(kThreads), lds, s, job, wl); break;
-    case 3: hipLaunchKernelGGL(k_sep_job<3>, grid, dim3(kThreads), lds, s, job, wl); break;
-    case 4: hipLaunchKernelGGL(k_sep_job<4>, grid, dim3(kThreads), lds, s, job, wl); break;
-    default: return hipErrorInvalidValue;
-    }
+    const int var = sep_job_variant(job);
+    int idx = 0;
+#define X(B, L, M, P)                                                                                  \
+    if (idx++ == var) hipLaunchKernelGGL((k_sep_job<B, L, M, P>), grid, dim3(kThreads), lds, s, job, wl);
+    VGPU_SEP_ALL(X)
+#undef X
     return hipGetLastError();
 }
 
-hipError_t launch_sep_batch(int bands, const SepJob* jobs, const BatchTask* tasks, int ntasks,
+hipError_t launch_sep_batch(int variant, const SepJob* jobs, const BatchTask* tasks, int ntasks,
                             const void* const* srcs, void* const* dsts, int grid, size_t wave_lds,
                             hipStream_t s) {
     if (ntasks <= 0) return hipSuccess;
+    if (variant < 0 || variant >= 64) return hipErrorInvalidValue;
     const size_t lds = wave_lds * kSepWaves;
     const int wl = (int)wave_lds;
-    switch (bands) {
-    case 1: hipLaunchKernelGGL(k_sep_batch<1>, dim3((unsigned)grid), dim3(kThreads), lds, s, jobs, tasks, ntasks, srcs, dsts, wl); break;
-    case 2: hipLaunchKernelGGL(k_sep_batch<2>, dim3((unsigned)grid), dim3(kThreads), lds, s, jobs, tasks, ntasks, srcs, dsts, wl); break;
-    case 3: hipLaunchKernelGGL(k_sep_batch<3>, dim3((unsigned)grid), dim3(kThreads), lds, s, jobs, tasks, ntasks, srcs, dsts, wl); break;
-    case 4: hipLaunchKernelGGL(k_sep_batch<4>, dim3((unsigned)grid), dim3(kThreads), lds, s, jobs, tasks, ntasks, srcs, dsts, wl); break;
-    default: return hipErrorInvalidValue;
-    }
+    int idx = 0;
+#define X(B, L, M, P)                                                                                   \
+    if (idx++ == variant)                                                                               \
+        hipLaunchKernelGGL((k_sep_batch<B, L, M, P>), dim3((unsigned)grid), dim3(kThreads), lds, s, jobs, tasks, \
+                           ntasks, srcs, dsts, wl);
+    VGPU_SEP_ALL(X)
+#undef X
     return hipGetLastError();
 }
+#undef VGPU_SEP_ALL
+#undef VGPU_SEP_B
 
 }  // namespace vgpu
