@@ -152,9 +152,11 @@ int vgpu_pipeline_thumbnail(vgpu_pipeline *p, int target, uint32_t width, uint32
  * (builder/resample.rs:637-711; resize.rs:163-284), downscale only. */
 int vgpu_pipeline_resize(vgpu_pipeline *p, double hscale, double vscale, int kernel);
 
-/* Kernel choice for u8 ReduceH -> ReduceV and GaussBlurH -> GaussBlurV
- * pairs: the int8-limb MFMA kernel (default, enable = 1) or the plain integer
- * kernels (enable = 0); both are bit-exact.  No reference counterpart. */
+/* Kernel choice for u8 separable pairs, a bit mask: bit 0 puts ReduceH ->
+ * ReduceV and GaussBlurH -> GaussBlurV on the int8-limb MFMA kernel (the
+ * default, 1), bit 1 also the ShrinkH -> ShrinkV box pair (slower than the
+ * plain box kernel on MI355X; kept for comparison); 0 selects the plain
+ * integer kernels.  All choices are bit-exact.  No reference counterpart. */
 int vgpu_pipeline_set_integer_mfma(vgpu_pipeline *p, int enable);
 /* CompiledPipeline output geometry (pipeline/compiled.rs:338-). */
 int vgpu_pipeline_output_info(const vgpu_pipeline *p, uint32_t *width, uint32_t *height,
@@ -265,8 +267,7 @@ int vgpu_batch_info(const vgpu_batch *b, vgpu_batch_stats *stats);
 /* Launch shape: persistent grid size (0 = one block per task), HIP graph on
  * or off, largest reduce tile candidate (re-plans the batch). */
 int vgpu_batch_configure(vgpu_batch *b, int grid_cap, int use_graph, int reduce_first_tile);
-/* u8 reduce and Gaussian pairs on the int8-limb MFMA kernel (default on) or
- * on the plain integer kernels; both are bit-exact.  Re-plans the batch. */
+/* As vgpu_pipeline_set_integer_mfma, for every image; re-plans the batch. */
 int vgpu_batch_set_integer_mfma(vgpu_batch *b, int enable);
 /* Enqueue the whole batch on the context stream.  srcs[i] / dsts[i] are device
  * pointers (sources 16-byte aligned) with byte lengths src_bytes[i] /

@@ -240,6 +240,23 @@ def test_u8_pairs_mfma_and_plain(ctx, mfma, bands):
         assert_same(gp.run_host(src), exp, f"mfma={mfma} bands {bands} {steps}")
 
 
+@pytest.mark.parametrize("bands", [1, 2, 3, 4])
+def test_box_shrink_on_mfma(ctx, bands):
+    # The box pre-shrink as an MFMA contraction (opt-in): 0/1 weights with edge
+    # counts folded, the reference's per-factor roundings, premultiply fused.
+    src = noise_u8(173, 311, bands, 90 + bands)
+    cases = [[("shrink_h", (f,)), ("shrink_v", (f,))] for f in (2, 3, 5, 7, 19)]
+    cases += [[("shrink_h", (4,))], [("shrink_v", (6,))], [("shrink_v", (3,)), ("shrink_h", (5, True))],
+              [("thumbnail", (v.WIDTH, 40, 0, v.LANCZOS3))]]
+    if bands in (2, 4):
+        cases.append([("premultiply", ()), ("shrink_h", (3,)), ("shrink_v", (4,)), ("unpremultiply", ())])
+    for steps in cases:
+        h, w, b = src.shape
+        gp = build(v.Pipeline(ctx, w, h, b), steps).set_integer_mfma(True, shrink=True)
+        exp = build(O.Pipeline(w, h, b), steps).run(src)
+        assert_same(gp.run_host(src), exp, f"mfma shrink bands {bands} {steps}")
+
+
 @pytest.mark.parametrize("w,h,b", [(1, 1, 3), (2, 1, 1), (1, 5, 4), (7, 3, 2), (19, 19, 3), (20, 1000, 1)])
 def test_tiny_and_ragged(ctx, w, h, b):
     src = noise_u8(h, w, b, w * 31 + h)

@@ -243,10 +243,12 @@ class Pipeline:
         w, h, b, _ = self.output_info()
         return (h, w, b)
 
-    def set_integer_mfma(self, enable: bool):
+    def set_integer_mfma(self, enable, shrink: bool = False):
         """u8 reduce / Gaussian pairs on the int8-limb MFMA kernel (default) or
-        the plain integer kernels; both are bit-exact."""
-        _check(self._lib.vgpu_pipeline_set_integer_mfma(self._h, int(enable)), "set_integer_mfma")
+        the plain integer kernels; shrink=True puts the box pre-shrink on it
+        too.  All choices are bit-exact."""
+        mode = (1 if enable else 0) | (2 if enable and shrink else 0)
+        _check(self._lib.vgpu_pipeline_set_integer_mfma(self._h, mode), "set_integer_mfma")
         return self
 
     def stage_count(self) -> int:
@@ -345,8 +347,9 @@ class Batch:
         _check(self._lib.vgpu_batch_configure(self._h, grid_cap, int(use_graph), reduce_first_tile),
                "vgpu_batch_configure")
 
-    def set_integer_mfma(self, enable: bool) -> None:
-        _check(self._lib.vgpu_batch_set_integer_mfma(self._h, int(enable)), "vgpu_batch_set_integer_mfma")
+    def set_integer_mfma(self, enable, shrink: bool = False) -> None:
+        mode = (1 if enable else 0) | (2 if enable and shrink else 0)
+        _check(self._lib.vgpu_batch_set_integer_mfma(self._h, mode), "vgpu_batch_set_integer_mfma")
 
     def _tables(self, srcs, dsts):
         n = len(self)

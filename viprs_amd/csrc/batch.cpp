@@ -459,7 +459,8 @@ int vgpu_batch_set_integer_mfma(vgpu_batch* b, int enable) {
     if (!b) return VGPU_E_INVALID_ARGUMENT;
     (void)hipSetDevice(b->ctx->device);
     if (hipStreamSynchronize(b->ctx->stream) != hipSuccess) return b->ctx->sticky = VGPU_E_DEVICE;
-    b->opt.integer_mfma = enable != 0;
+    b->opt.integer_mfma = (enable & 1) != 0;
+    b->opt.mfma_shrink = (enable & 2) != 0;
     return b->plan();
 }
 

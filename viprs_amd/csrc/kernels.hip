@@ -195,6 +195,11 @@ __device__ __forceinline__ void window_sums(const unsigned char* rbuf, int sbyte
     const int rem_last = wbytes - 4 * (nd - 1);  // 1..4 valid bytes in the last dword
     const uint32_t last_mask = rem_last >= 4 ? 0xffffffffu : ((1u << (8 * rem_last)) - 1u);
     constexpr int U = B == 3 ? 6 : 4;
+    // Even and odd dwords accumulate separately: two independent udot4 chains
+    // per band.
+    uint32_t odd[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) odd[b] = 0;
     uint32_t cur = l32[0];
     int d0 = 0;
     // Blocks of U dwords that lie wholly before the last dword: no masks.
@@ -207,7 +212,10 @@ __device__ __forceinline__ void window_sums(const unsigned char* rbuf, int sbyte
             const uint32_t v = __builtin_amdgcn_alignbyte(w[u], cur, al);
             cur = w[u];
 #pragma unroll
-            for (int b = 0; b < B; ++b) sum[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, u % 3), sum[b], false);
+            for (int b = 0; b < B; ++b) {
+                if (u & 1) odd[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, u % 3), odd[b], false);
+                else sum[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, u % 3), sum[b], false);
+            }
         }
     }
     // Tail block: the last 1..U dwords, masked.
@@ -222,9 +230,14 @@ __device__ __forceinline__ void window_sums(const unsigned char* rbuf, int sbyte
             const uint32_t v = __builtin_amdgcn_alignbyte(w[u], cur, al) & m;
             cur = w[u];
 #pragma unroll
-            for (int b = 0; b < B; ++b) sum[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, u % 3), sum[b], false);
+            for (int b = 0; b < B; ++b) {
+                if (u & 1) odd[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, u % 3), odd[b], false);
+                else sum[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, u % 3), sum[b], false);
+            }
         }
     }
+#pragma unroll
+    for (int b = 0; b < B; ++b) sum[b] += odd[b];
 }
 
 template <int B>

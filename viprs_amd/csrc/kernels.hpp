@@ -69,26 +69,32 @@ struct ReduceJob {
 
 // Separable u8 pass pair on MFMA (int8 limbs): H then V with a u8
 // intermediate, each pass a banded contraction on v_mfma_i32_16x16x64_i8.
-// Used for fused [Premultiply] -> ReduceH -> ReduceV -> [Unpremultiply]
-// (mode 0: (sum + 2048) >> 12) and for GaussBlurH -> GaussBlurV (mode 1:
-// (sum + scale/2) / scale).  ph.start / pv.start are absolute first-tap
-// positions per output column / row (ph indexed from out.x), phases index
-// 65 x taps (or 1 x taps) i16 weight rows.  A wavefront task is one
-// 16-sample column strip (jx) x rows_per_task output rows (ty).
+// Family 0 (reduce): fused [Premultiply] -> ReduceH -> ReduceV ->
+// [Unpremultiply]; the H weights of a 16-sample strip are folded in the
+// kernel from ph (absolute first-tap column per output column, indexed from
+// out.x; 65 x taps i16 rows), rounding (sum + 2048) >> 12 clamped.  Family 1
+// (tabled): GaussBlurH -> GaussBlurV and [Premultiply] -> ShrinkH -> ShrinkV
+// -> [Unpremultiply]; the H operands come from host-folded patterns, rounding
+// q = mul_hi(x, mul) >> shift or (x * mul) >> shift per pass, the bias already
+// in the seeds.  Both families take the V operands from host-folded patterns
+// per 16 output rows.  A wavefront task is one 16-sample strip (jx) x
+// rows_per_task output rows (ty).
 struct SepJob {
     InView in;
     OutView out;
-    DevPlan ph, pv;
-    // V weights pre-folded on the host per 16 output rows: kv k-steps x
-    // limbs A operands (64 lanes x 16 bytes each; {hi, lo} when limbs = 2),
-    // then 16 int32 accumulator seeds 128 * sum(w) + rounding bias; tile i
-    // starts at vtiles + i * vtile_bytes.
-    const unsigned char* vtiles = nullptr;
-    int vtile_bytes = 0;
-    uint64_t inv = 0;  // mode 1: ceil(2^40 / scale), so n / scale = (n * inv) >> 40
-    int limbs = 2;     // 1 when every folded weight fits int8 (sum |w| <= 127)
-    int bands = 3, mode = 0, scale = 1;
-    int vclip0 = 0, vclip1 = 0;
+    DevPlan ph;                         // family 0: H plan
+    const unsigned char* htab = nullptr;  // family 1: H operand patterns
+    const int32_t* hstrip = nullptr;      // family 1: per strip (pattern, first span pixel)
+    int htab_bytes = 0;
+    // V operand patterns: kv k-steps x limbs A operands (64 lanes x 16
+    // bytes; {hi, lo} when limbs = 2), then 16 int32 seeds 128 * sum(w) + bias.
+    const unsigned char* vtab = nullptr;
+    const int32_t* vtile = nullptr;  // per 16-row tile: (pattern, base16, hi)
+    int vtab_bytes = 0;
+    int family = 0, limbs = 2, bands = 3;
+    uint32_t h_mul = 1, v_mul = 1;  // family 1 rounding
+    int h_shift = 0, v_shift = 0, h_hi = 0, v_hi = 0;
+    int h_bias = 0;                  // family 0: H seed bias
     int kh = 1, kv = 1, ring = 128, rows_per_task = 16;
     int premult = 0, unpremult = 0;
     int gx = 0, gy = 0;  // column strips, row chunks
@@ -120,8 +126,10 @@ hipError_t launch_reduce_job(const ReduceJob& job, hipStream_t s);
 hipError_t launch_reduce_batch(const ReduceJob* jobs, const BatchTask* tasks, int ntasks,
                                const void* const* srcs, void* const* dsts, int grid, size_t lds,
                                hipStream_t s);
-// LDS bytes one wavefront of the MFMA separable kernel needs for this job.
+// LDS bytes one wavefront of the MFMA separable kernel needs for this job;
+// whether the kernel takes the job's shape.
 size_t sep_job_wave_lds(const SepJob& job);
+bool sep_job_supported(const SepJob& job);
 hipError_t launch_sep_job(const SepJob& job, hipStream_t s);
 // Kernel variant of a job (bands, limbs, mode, premultiply); a batch launch
 // runs jobs of one variant, one (job, jx, ty) task per wavefront.

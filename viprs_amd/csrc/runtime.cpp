@@ -92,41 +92,127 @@ int64_t clamp_rel(int64_t x, int64_t c0, int64_t cn, int64_t b0, int64_t bn) {
     return x - b0;
 }
 
-// Shape of the int8-limb MFMA job for a u8 H -> V pair: K steps of the H span
-// per 16-sample strip and of the V window per 16 output rows (each at most
-// two 64-wide steps), ring size and task split.  Returns false when a window
-// is too wide for the kernel.
-bool plan_sep(const std::vector<int32_t>& hs, int htaps, const std::vector<int32_t>& vs, int vtaps,
-              uint32_t bands, const Rect& out, const Rect& clip, const Rect& buf, const Rect& vclip,
-              const StageOptions& opt, SepJob* j) {
+// One pass of the MFMA kernel as the host folds it: for output index i (a
+// column of the pass, or a row) the taps start at absolute position start[i]
+// and weigh wrow(i)[0 .. taps).
+struct SepAxis {
+    const std::vector<int32_t>* start = nullptr;
+    const std::vector<uint8_t>* phase = nullptr;  // null: one weight row
+    const int16_t* wtab = nullptr;
+    int taps = 1;
+    const int16_t* row(size_t i) const { return wtab + (phase ? (size_t)(*phase)[i] * taps : 0); }
+};
+
+// Lay out folded weights W[j][k] (16 x ksteps*64) as MFMA operand bytes: lane
+// j + 16 q holds K = 64 s + 16 q .. + 15 of step s; one int8 limb, or the
+// {hi, lo} limbs of w = 128 a + c (c in [-64, 63]); then 16 int32 seeds
+// 128 * sum_k W[j][k] + bias.
+void emit_operands(const std::vector<int32_t>& w, int ksteps, int limbs, int32_t bias,
+                   std::vector<unsigned char>* out) {
+    const size_t ops = (size_t)ksteps * limbs * 64 * 16;
+    out->assign(ops + 16 * sizeof(int32_t), 0);
+    const int kw = 64 * ksteps;
+    for (int s = 0; s < ksteps; ++s)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int e = 0; e < 16; ++e) {
+                const int j = lane & 15, q = lane >> 4;
+                const int32_t v = w[(size_t)(j * kw + 64 * s + 16 * q + e)];
+                if (limbs == 1) {
+                    (*out)[((size_t)s * 64 + lane) * 16 + e] = (unsigned char)(v & 255);
+                } else {
+                    const int32_t c = ((v + 64) & 127) - 64;
+                    const int32_t a = (v - c) >> 7;
+                    (*out)[((size_t)(2 * s) * 64 + lane) * 16 + e] = (unsigned char)(a & 255);
+                    (*out)[((size_t)(2 * s + 1) * 64 + lane) * 16 + e] = (unsigned char)(c & 255);
+                }
+            }
+    int32_t seed[16];
+    for (int j = 0; j < 16; ++j) {
+        int64_t sum = 0;
+        for (int k = 0; k < kw; ++k) sum += w[(size_t)(j * kw + k)];
+        seed[j] = (int32_t)(128 * sum + bias);
+    }
+    std::memcpy(out->data() + ops, seed, sizeof(seed));
+}
+
+// Identical operand patterns are stored once.
+struct PatternSet {
+    std::vector<std::vector<unsigned char>> pats;
+    std::map<std::vector<unsigned char>, int> index;
+    int add(std::vector<unsigned char>&& p) {
+        auto it = index.find(p);
+        if (it != index.end()) return it->second;
+        const int id = (int)pats.size();
+        index.emplace(p, id);
+        pats.push_back(std::move(p));
+        return id;
+    }
+    size_t put(Staging* st, int* bytes) const {
+        *bytes = pats.empty() ? 0 : (int)pats[0].size();
+        std::vector<unsigned char> all;
+        for (const auto& p : pats) all.insert(all.end(), p.begin(), p.end());
+        return st->put(all.data(), all.size());
+    }
+};
+
+// Largest sum |w| over the weight rows an axis uses.
+int64_t axis_abs_sum(const SepAxis& ax, size_t n) {
+    int64_t worst = 0;
+    for (size_t i = 0; i < n; ++i) {
+        int64_t sum = 0;
+        for (int t = 0; t < ax.taps; ++t) sum += std::abs((int)ax.row(i)[t]);
+        worst = std::max(worst, sum);
+    }
+    return worst;
+}
+
+// Plan the int8-limb MFMA job of a u8 H -> V pair over output rectangle `out`
+// (H input clip `clip`, input buffer `buf`, V input rows `vclip`): K steps of
+// the H span of each 16-sample strip and of the V window of each 16 output
+// rows, limbs, ring size and task split, and the V operand patterns (and, for
+// family 1, the H patterns).  Returns false when a window is too wide.
+bool plan_sep(int family, const SepAxis& hax, const SepAxis& vax, int32_t hbias, int32_t vbias, uint32_t bands,
+              const Rect& out, const Rect& clip, const Rect& buf, const Rect& vclip, const StageOptions& opt,
+              Staging* st, Stage* stg) {
+    SepJob* j = &stg->sep;
     const int64_t nsamp = out.w * (int64_t)bands;
     if (nsamp <= 0 || out.h <= 0 || bands < 1 || bands > 4) return false;
     // The kernel addresses its input with 32-bit byte offsets.
     if (buf.w * buf.h * (int64_t)bands >= ((int64_t)1 << 31)) return false;
+    const std::vector<int32_t>& hs = *hax.start;
+    const std::vector<int32_t>& vs = *vax.start;
+    auto hcol = [&](int64_t x) { return clamp_rel(x, clip.x, clip.w, buf.x, buf.w); };
+    const int64_t ntiles = (out.h + 15) / 16, nstrips = (nsamp + 15) / 16;
     int64_t span = 0;
-    for (int64_t s0 = 0; s0 < nsamp; s0 += 16) {
-        const int64_t s1 = std::min<int64_t>(s0 + 15, nsamp - 1);
-        const int64_t c0 = clamp_rel(hs[(size_t)(s0 / bands)], clip.x, clip.w, buf.x, buf.w);
-        const int64_t c1 = clamp_rel(hs[(size_t)(s1 / bands)] + htaps - 1, clip.x, clip.w, buf.x, buf.w);
+    for (int64_t jx = 0; jx < nstrips; ++jx) {
+        const int64_t s0 = 16 * jx, s1 = std::min<int64_t>(s0 + 15, nsamp - 1);
+        const int64_t c0 = hcol(hs[(size_t)(s0 / bands)]);
+        const int64_t c1 = hcol(hs[(size_t)(s1 / bands)] + hax.taps - 1);
         span = std::max(span, (c1 - c0 + 1) * (int64_t)bands);
     }
     const int64_t v0 = vclip.y, v1 = vclip.y + std::max<int64_t>(vclip.h, 1) - 1;
+    auto vrow = [&](int64_t r) { return std::min(std::max(r, v0), v1); };
     int64_t window = 0;
-    for (int64_t y0 = 0; y0 < out.h; y0 += 16) {
-        const int64_t y1 = std::min<int64_t>(y0 + 15, out.h - 1);
-        const int64_t lo = std::min(std::max<int64_t>(vs[(size_t)y0], v0), v1);
-        const int64_t hi = std::min(std::max<int64_t>(vs[(size_t)y1] + vtaps - 1, v0), v1);
+    for (int64_t ti = 0; ti < ntiles; ++ti) {
+        const int64_t y0 = 16 * ti, y1 = std::min<int64_t>(y0 + 15, out.h - 1);
+        const int64_t lo = vrow(vs[(size_t)y0]), hi = vrow(vs[(size_t)y1] + vax.taps - 1);
         window = std::max(window, hi - (lo & ~(int64_t)15) + 1);
     }
+    const int limbs = axis_abs_sum(hax, (size_t)out.w) <= 127 && axis_abs_sum(vax, (size_t)out.h) <= 127 ? 1 : 2;
     const int kh = (int)((span + 63) / 64), kv = (int)((window + 63) / 64);
-    if (kh < 1 || kh > 2 || kv < 1 || kv > 2) return false;
+    const int khmax = family == 0 ? 2 : (limbs == 1 ? 6 : 3);
+    // kv <= 7 keeps the ring (16 x 512 bytes) and four wavefronts' LDS under 64 KiB.
+    if (kh < 1 || kh > khmax || kv < 1 || kv > 7) return false;
+    if (family == 0 && limbs != 2) return false;
+    j->family = family;
+    j->limbs = limbs;
     j->bands = (int)bands;
     j->kh = kh;
     j->kv = kv;
-    j->ring = kv == 1 ? 128 : 256;
-    j->vclip0 = (int)v0;
-    j->vclip1 = (int)v1;
-    j->gx = (int)((nsamp + 15) / 16);
+    int ring = 128;
+    while (ring < 64 * kv + 16) ring *= 2;
+    j->ring = ring;
+    j->gx = (int)nstrips;
     int rpt = opt.sep_rows_per_task;
     if (rpt <= 0) {
         // One image alone: long runs while about 2048 wavefronts remain.
@@ -135,73 +221,57 @@ bool plan_sep(const std::vector<int32_t>& hs, int htaps, const std::vector<int32
     }
     j->rows_per_task = rpt;
     j->gy = (int)((out.h + rpt - 1) / rpt);
-    return true;
-}
 
-// One int8 limb suffices when every folded weight fits int8: sum |w| <= 127
-// over every phase row of both axes.
-int sep_limbs(const int16_t* wh, int hrows, int htaps, const int16_t* wv, int vrows, int vtaps) {
-    for (int pass = 0; pass < 2; ++pass) {
-        const int16_t* w = pass == 0 ? wh : wv;
-        const int rows = pass == 0 ? hrows : vrows, taps = pass == 0 ? htaps : vtaps;
-        for (int r = 0; r < rows; ++r) {
-            int sum = 0;
-            for (int t = 0; t < taps; ++t) sum += std::abs((int)w[(size_t)r * taps + t]);
-            if (sum > 127) return 2;
-        }
-    }
-    return 1;
-}
-
-// V weights of the MFMA kernel folded per 16 output rows: for output row m of
-// the tile, every tap adds its weight at K = clamp(start + t, vclip) - base16
-// (base16: the tile's first clamped row, rounded down to 16); the folded
-// weights are split into int8 limbs (w = 128 a + c, c in [-64, 63]) and laid
-// out as the A operands of kv k-steps (lane m + 16 q holds K = 64 s + 16 q ..
-// + 15), followed by the 16 per-row weight sums.
-void build_vtiles(const std::vector<int32_t>& vs, const std::vector<uint8_t>& vp, const int16_t* wtab,
-                  int taps, int64_t out_h, const SepJob& j, std::vector<unsigned char>* tiles) {
-    const int kv = j.kv, limbs = j.limbs;
-    const int32_t bias = j.mode == 0 ? 2048 : j.scale / 2;
-    const size_t ops = (size_t)kv * limbs * 64 * 16;
-    const size_t tile_bytes = ops + 16 * sizeof(int32_t);
-    const int64_t ntiles = (out_h + 15) / 16;
-    tiles->assign((size_t)ntiles * tile_bytes, 0);
-    std::vector<int32_t> w((size_t)16 * 64 * kv);
+    // V operand patterns per 16 output rows, with each tile's (base16, hi).
+    PatternSet vp;
+    std::vector<int32_t> vtile((size_t)ntiles * 3);
+    std::vector<int32_t> w;
     for (int64_t ti = 0; ti < ntiles; ++ti) {
-        std::fill(w.begin(), w.end(), 0);
-        int32_t corr[16] = {};
-        const int64_t y0 = ti * 16;
-        const int64_t nrow = std::min<int64_t>(16, out_h - y0);
-        const int64_t lo = std::min<int64_t>(std::max<int64_t>(vs[(size_t)y0], j.vclip0), j.vclip1);
+        const int64_t y0 = 16 * ti, nrow = std::min<int64_t>(16, out.h - y0);
+        const int64_t lo = vrow(vs[(size_t)y0]), hi = vrow(vs[(size_t)(y0 + nrow - 1)] + vax.taps - 1);
         const int64_t base16 = lo & ~(int64_t)15;
+        w.assign((size_t)16 * 64 * kv, 0);
         for (int64_t m = 0; m < nrow; ++m) {
-            const int16_t* row = wtab + (size_t)vp[(size_t)(y0 + m)] * taps;
-            for (int t = 0; t < taps; ++t) {
-                const int64_t r = std::min<int64_t>(std::max<int64_t>(vs[(size_t)(y0 + m)] + t, j.vclip0), j.vclip1);
-                w[(size_t)(m * 64 * kv + (r - base16))] += row[t];
-                corr[m] += row[t];
-            }
+            const int16_t* row = vax.row((size_t)(y0 + m));
+            for (int t = 0; t < vax.taps; ++t)
+                w[(size_t)(m * 64 * kv + (vrow(vs[(size_t)(y0 + m)] + t) - base16))] += row[t];
         }
-        unsigned char* dst = tiles->data() + (size_t)ti * tile_bytes;
-        for (int s = 0; s < kv; ++s)
-            for (int lane = 0; lane < 64; ++lane)
-                for (int e = 0; e < 16; ++e) {
-                    const int m = lane & 15, q = lane >> 4;
-                    const int32_t v = w[(size_t)(m * 64 * kv + 64 * s + 16 * q + e)];
-                    if (limbs == 1) {
-                        dst[((size_t)s * 64 + lane) * 16 + e] = (unsigned char)(v & 255);
-                    } else {
-                        const int32_t c = ((v + 64) & 127) - 64;
-                        const int32_t a = (v - c) >> 7;
-                        dst[((size_t)(2 * s) * 64 + lane) * 16 + e] = (unsigned char)(a & 255);
-                        dst[((size_t)(2 * s + 1) * 64 + lane) * 16 + e] = (unsigned char)(c & 255);
-                    }
-                }
-        int32_t seed[16];
-        for (int m = 0; m < 16; ++m) seed[m] = 128 * corr[m] + bias;
-        std::memcpy(dst + ops, seed, sizeof(seed));
+        std::vector<unsigned char> bytes;
+        emit_operands(w, kv, limbs, vbias, &bytes);
+        vtile[(size_t)(3 * ti)] = vp.add(std::move(bytes));
+        vtile[(size_t)(3 * ti + 1)] = (int32_t)base16;
+        vtile[(size_t)(3 * ti + 2)] = (int32_t)hi;
     }
+    stg->v_tab = vp.put(st, &j->vtab_bytes);
+    stg->v_tile = st->put(vtile.data(), vtile.size() * sizeof(int32_t));
+
+    if (family == 1) {
+        // H operand patterns per 16-sample strip, with each strip's first
+        // span pixel.
+        PatternSet hp;
+        std::vector<int32_t> hstrip((size_t)nstrips * 2);
+        for (int64_t jx = 0; jx < nstrips; ++jx) {
+            const int64_t s0 = 16 * jx;
+            const int64_t span0 = hcol(hs[(size_t)(s0 / bands)]);
+            w.assign((size_t)16 * 64 * kh, 0);
+            for (int n = 0; n < 16 && s0 + n < nsamp; ++n) {
+                const int64_t smp = s0 + n, x = smp / bands, b = smp - x * bands;
+                const int16_t* row = hax.row((size_t)x);
+                for (int t = 0; t < hax.taps; ++t)
+                    w[(size_t)(n * 64 * kh + (hcol(hs[(size_t)x] + t) - span0) * bands + b)] += row[t];
+            }
+            std::vector<unsigned char> bytes;
+            emit_operands(w, kh, limbs, hbias, &bytes);
+            hstrip[(size_t)(2 * jx)] = hp.add(std::move(bytes));
+            hstrip[(size_t)(2 * jx + 1)] = (int32_t)span0;
+        }
+        stg->h_tab = hp.put(st, &j->htab_bytes);
+        stg->h_strip = st->put(hstrip.data(), hstrip.size() * sizeof(int32_t));
+    } else {
+        j->h_bias = hbias;
+    }
+    stg->kind = ST_SEP;
+    return true;
 }
 
 void stage_reduce_params(const Node& nd, const Rect& out, Staging* st, Stage* s) {
@@ -230,15 +300,9 @@ bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const Redu
     std::vector<uint8_t> hp, vp;
     build_plan(hax, mid.x, mid.w, accumulate_h, &hs, &hp);
     build_plan(vax, out.y, out.h, false, &vs, &vp);
-    if (fmt == U8 && opt.integer_mfma &&
-        plan_sep(hs, hax.taps, vs, vax.taps, bands, out, clip, buf, vclip, opt, &s->sep)) {
-        s->sep.mode = 0;
-        s->sep.limbs = sep_limbs(hax.wi16.data(), kPhases + 1, hax.taps, vax.wi16.data(), kPhases + 1, vax.taps);
-        std::vector<unsigned char> tiles;
-        build_vtiles(vs, vp, vax.wi16.data(), vax.taps, out.h, s->sep, &tiles);
-        s->v_tiles = st->put(tiles.data(), tiles.size());
-        s->sep.vtile_bytes = (int)(tiles.size() / (size_t)((out.h + 15) / 16));
-        s->kind = ST_SEP;
+    if (fmt == U8 && opt.integer_mfma) {
+        const SepAxis ha{&hs, &hp, hax.wi16.data(), hax.taps}, va{&vs, &vp, vax.wi16.data(), vax.taps};
+        plan_sep(0, ha, va, 2048, 2048, bands, out, clip, buf, vclip, opt, st, s);  // ST_SEP on success
     }
     const size_t esz = format_size(fmt);
     // Tile candidates, largest first: take the first whose staged rows fit
@@ -311,31 +375,62 @@ bool stage_gauss_sep_params(const Node& gh, const Node& gv, const Rect& mid, con
                             const StageOptions& opt, Staging* st, Stage* s) {
     const int taps = (int)gh.itaps.size();
     const int rad = (taps - 1) / 2;
-    if (taps < 1 || gv.itaps.size() != gh.itaps.size() || gh.iscale != gv.iscale || gh.iscale < 1 ||
-        gh.iscale >= 65536)
+    // mul_hi by ceil(2^32 / scale) is an exact division for sums below
+    // 256 * scale when scale < 4096.
+    if (taps < 1 || gv.itaps != gh.itaps || gh.iscale != gv.iscale || gh.iscale < 1 || gh.iscale >= 4096)
         return false;
     std::vector<int32_t> hs((size_t)std::max<int64_t>(mid.w, 1)), vs((size_t)std::max<int64_t>(out.h, 1));
     for (int64_t x = 0; x < mid.w; ++x) hs[(size_t)x] = (int32_t)(mid.x + x - rad);
     for (int64_t y = 0; y < out.h; ++y) vs[(size_t)y] = (int32_t)(out.y + y - rad);
-    if (!plan_sep(hs, taps, vs, taps, gh.bands, out, clip, buf, vclip, opt, &s->sep)) return false;
-    const std::vector<uint8_t> hp(hs.size(), 0), vp(vs.size(), 0);
-    s->plan_start = st->put(hs.data(), hs.size() * sizeof(int32_t));
-    s->plan_phase = st->put(hp.data(), hp.size());
-    s->w16 = st->put(gh.itaps.data(), gh.itaps.size() * sizeof(int16_t));
-    s->taps = taps;
-    s->v_start = st->put(vs.data(), vs.size() * sizeof(int32_t));
-    s->v_phase = st->put(vp.data(), vp.size());
-    s->v_w16 = st->put(gv.itaps.data(), gv.itaps.size() * sizeof(int16_t));
-    s->v_taps = taps;
-    s->sep.mode = 1;
-    s->sep.scale = gh.iscale;
-    s->sep.limbs = sep_limbs(gh.itaps.data(), 1, taps, gv.itaps.data(), 1, taps);
-    std::vector<unsigned char> tiles;
-    build_vtiles(vs, vp, gv.itaps.data(), taps, out.h, s->sep, &tiles);
-    s->v_tiles = st->put(tiles.data(), tiles.size());
-    s->sep.vtile_bytes = (int)(tiles.size() / (size_t)((out.h + 15) / 16));
-    s->sep.inv = ((uint64_t)1 << 40) / (uint64_t)gh.iscale + (((uint64_t)1 << 40) % (uint64_t)gh.iscale ? 1 : 0);
-    s->kind = ST_SEP;
+    const SepAxis ha{&hs, nullptr, gh.itaps.data(), taps}, va{&vs, nullptr, gv.itaps.data(), taps};
+    const int32_t bias = gh.iscale / 2;
+    if (!plan_sep(1, ha, va, bias, bias, gh.bands, out, clip, buf, vclip, opt, st, s)) return false;
+    SepJob& j = s->sep;
+    if (gh.iscale == 1) {
+        j.h_mul = j.v_mul = 1;
+        j.h_hi = j.v_hi = 0;
+    } else {
+        const uint32_t mul = (uint32_t)((((uint64_t)1 << 32) + (uint64_t)gh.iscale - 1) / (uint64_t)gh.iscale);
+        j.h_mul = j.v_mul = mul;
+        j.h_hi = j.v_hi = 1;
+    }
+    j.h_shift = j.v_shift = 0;
+    return true;
+}
+
+// Box rounding of ShrinkH (x86 dispatch, shrinkh.rs:381-397) and ShrinkV
+// (shrinkv.rs:206-339) as (bias, mul, shift, mul_hi): pair rounding at f = 2,
+// exact /5 for ShrinkH f = 5 with 1, 3 or 4 bands, else the multiply-shift
+// ((sum + f/2) * floor(2^24 / f)) >> 24.
+void shrink_rounding(int f, int bands, bool horizontal, int32_t* bias, uint32_t* mul, int* shift, int* hi) {
+    if (f == 2) {
+        *bias = 1, *mul = 1, *shift = 1, *hi = 0;
+    } else if (horizontal && f == 5 && bands != 2 && bands <= 4) {
+        *bias = 2, *mul = 0xCCCCCCCDu, *shift = 2, *hi = 1;  // floor(x / 5) = mul_hi(x, ceil(2^34 / 5)) >> 2
+    } else {
+        *bias = f / 2, *mul = (uint32_t)((1u << 24) / (uint32_t)f), *shift = 24, *hi = 0;
+    }
+}
+
+// [Premultiply] -> ShrinkH(fh) -> ShrinkV(fv) -> [Unpremultiply] on u8 as one
+// MFMA stage: box taps of weight 1, fh = 1 / fv = 1 for a missing pass.
+bool stage_shrink_sep_params(int fh, int fv, uint32_t bands, const Rect& mid, const Rect& out, const Rect& vclip,
+                             const Rect& clip, const Rect& buf, const StageOptions& opt, Staging* st, Stage* s) {
+    if (fh < 1 || fv < 1 || fh > 127 || fv > 127) return false;  // folded counts fit int8
+    std::vector<int32_t> hs((size_t)std::max<int64_t>(mid.w, 1)), vs((size_t)std::max<int64_t>(out.h, 1));
+    for (int64_t x = 0; x < mid.w; ++x) hs[(size_t)x] = (int32_t)((mid.x + x) * fh);
+    for (int64_t y = 0; y < out.h; ++y) vs[(size_t)y] = (int32_t)((out.y + y) * fv);
+    const std::vector<int16_t> hones((size_t)fh, 1), vones((size_t)fv, 1);
+    const SepAxis ha{&hs, nullptr, hones.data(), fh}, va{&vs, nullptr, vones.data(), fv};
+    int32_t hb, vb;
+    uint32_t hm, vm;
+    int hsh, vsh, hhi, vhi;
+    shrink_rounding(fh, (int)bands, true, &hb, &hm, &hsh, &hhi);
+    shrink_rounding(fv, (int)bands, false, &vb, &vm, &vsh, &vhi);
+    if (!plan_sep(1, ha, va, hb, vb, bands, out, clip, buf, vclip, opt, st, s)) return false;
+    SepJob& j = s->sep;
+    j.h_mul = hm, j.h_shift = hsh, j.h_hi = hhi;
+    j.v_mul = vm, j.v_shift = vsh, j.v_hi = vhi;
     return true;
 }
 
@@ -387,11 +482,10 @@ SepJob sep_job_of(const Stage& s, const InView& in, const OutView& out, const un
     j.ph.phase = reinterpret_cast<const uint8_t*>(blob + s.plan_phase);
     j.ph.wi16 = reinterpret_cast<const int16_t*>(blob + s.w16);
     j.ph.taps = s.taps;
-    j.pv.start = reinterpret_cast<const int32_t*>(blob + s.v_start);
-    j.pv.phase = reinterpret_cast<const uint8_t*>(blob + s.v_phase);
-    j.pv.wi16 = reinterpret_cast<const int16_t*>(blob + s.v_w16);
-    j.pv.taps = s.v_taps;
-    j.vtiles = blob + s.v_tiles;
+    j.vtab = blob + s.v_tab;
+    j.vtile = reinterpret_cast<const int32_t*>(blob + s.v_tile);
+    j.htab = blob + s.h_tab;
+    j.hstrip = reinterpret_cast<const int32_t*>(blob + s.h_strip);
     j.premult = s.premult;
     j.unpremult = s.unpremult;
     return j;
@@ -473,24 +567,37 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
                         (alpha_ok && nd.kind == N_PREMULT &&
                          (kind_at(i + 1) == N_SHRINK_H || kind_at(i + 1) == N_SHRINK_V)))) {
             // [Premultiply] -> ShrinkH -> [ShrinkV] -> [Unpremultiply], or a lone ShrinkV.
-            int j = i;
+            int j = i, jh = -1, jv = -1;
             s.kind = ST_SHRINK_HV_U8;
             if (nodes[(size_t)j].kind == N_PREMULT) {
                 s.premult = 1;
                 ++j;
             }
             if (nodes[(size_t)j].kind == N_SHRINK_H) {
+                jh = j;
                 s.fh = (int)nodes[(size_t)j].factor;
-                if (kind_at(j + 1) == N_SHRINK_V) s.fv = (int)nodes[(size_t)++j].factor;
+                if (kind_at(j + 1) == N_SHRINK_V) {
+                    jv = ++j;
+                    s.fv = (int)nodes[(size_t)j].factor;
+                }
             } else {
+                jv = j;
                 s.fv = (int)nodes[(size_t)j].factor;
             }
+            const int jlast = j;
             if (alpha_ok && kind_at(j + 1) == N_UNPREMULT) {
                 s.unpremult = 1;
                 ++j;
             }
             s.last = j;
             done = true;
+            if (opt.integer_mfma && opt.mfma_shrink) {
+                // The box pair on the MFMA kernel (ST_SEP) when its windows fit.
+                const Rect mid = jh >= 0 ? out_r[(size_t)jh] : in_r[(size_t)jv];
+                const Rect vclip = jv >= 0 ? in_r[(size_t)jv] : out_r[(size_t)jh];
+                stage_shrink_sep_params(s.fh, s.fv, nd.bands, mid, out_r[(size_t)jlast], vclip, s.clip, buf, opt,
+                                        st, &s);
+            }
         } else if (small_bands && (nd.kind == N_REDUCE_H || nd.kind == N_REDUCE_V ||
                                    (alpha_ok && nd.kind == N_PREMULT &&
                                     (kind_at(i + 1) == N_REDUCE_H || kind_at(i + 1) == N_REDUCE_V)))) {
@@ -898,7 +1005,8 @@ int vgpu_pipeline_set_integer_mfma(vgpu_pipeline* p, int enable) {
     (void)hipSetDevice(p->ctx->device);
     if (hipStreamSynchronize(p->ctx->stream) != hipSuccess) return p->ctx->sticky = VGPU_E_DEVICE;
     p->release();
-    p->opt.integer_mfma = enable != 0;
+    p->opt.integer_mfma = (enable & 1) != 0;
+    p->opt.mfma_shrink = (enable & 2) != 0;
     return VGPU_OK;
 }
 

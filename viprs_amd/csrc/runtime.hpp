@@ -130,7 +130,9 @@ struct Stage {
     int v_taps = 1;
     ReduceJob rjob;
     SepJob sep;  // ST_SEP: job template (pointers set at launch)
-    size_t v_tiles = 0;  // ST_SEP: blob offset of the pre-folded V weight tiles
+    // ST_SEP blob offsets: V operand patterns, per-tile (pattern, base16, hi),
+    // and (family 1) H operand patterns, per-strip (pattern, first pixel).
+    size_t v_tab = 0, v_tile = 0, h_tab = 0, h_strip = 0;
 };
 
 // Stage-planning choices that depend on how the stage is launched: a single
@@ -140,6 +142,7 @@ struct StageOptions {
     int64_t reduce_min_blocks = 1024;  // stop shrinking reduce tiles once this many blocks exist
     int reduce_first_tile = 0;         // largest reduce tile candidate considered (0 = 64x16)
     bool integer_mfma = true;          // u8 reduce / Gaussian pairs on the int8-limb MFMA kernel
+    bool mfma_shrink = false;          // u8 box pre-shrink on it too (slower than the VALU kernel)
     int sep_rows_per_task = 0;         // output rows per MFMA wavefront task (0 = by chip fill)
     bool shrink_row_split = true;      // split a shrink row's source rows over wavefronts
 };

@@ -93,21 +93,6 @@ __device__ __forceinline__ uint32_t premult_dword(uint32_t v) {
     }
 }
 
-// Rounding of one exact sum whose bias (2048, or scale / 2) is already in.
-// Mode 1 divides by multiply-shift: with inv = ceil(2^40 / scale),
-// floor(n / scale) = (n * inv) >> 40 whenever n * scale <= 2^40, which holds
-// for Gaussian sums (0 <= n < 256 * scale, scale < 2^16).
-template <int MODE>
-__device__ __forceinline__ uint32_t sep_round(int acc, uint64_t inv) {
-    int v;
-    if constexpr (MODE == 0) {
-        v = acc >> 12;
-    } else {
-        v = (int)(((uint64_t)(uint32_t)max(acc, 0) * inv) >> 40);
-    }
-    return (uint32_t)min(max(v, 0), 255);
-}
-
 // Fold the taps of 16 rows j (lane j + 16 tq adds taps tq, tq + 4, ...) into
 // the wavefront's scratch [16][ks] and corr[16]; `pos` maps (j, t) to the K
 // index (or -1 for no contribution).
@@ -164,8 +149,6 @@ __device__ __forceinline__ void read_limbs(const int* scratch, int ks, int s, i3
     split_limbs(v, hi, lo);
 }
 
-constexpr int kMaxSteps = 2;
-
 // Plan tables are read-only for the kernel's life: constant-address loads
 // go through the scalar cache.
 template <typename T>
@@ -173,182 +156,206 @@ __device__ __forceinline__ T cload(const T* p, int i) {
     return ((const __attribute__((address_space(4))) T*)p)[i];
 }
 
-// A operand of one 16-row M tile: lane (m, q) holds bytes 16q .. 16q + 15 of
-// k-step s of the span of intermediate row r0 + m (pixels as p ^ 0x80).
-template <int B>
-__device__ __forceinline__ void load_a(const unsigned char* base, uint32_t rb, uint32_t total, int q, int kh,
-                                       bool premult, i32x4 (&a)[kMaxSteps]) {
-#pragma unroll
-    for (int s = 0; s < kMaxSteps; ++s) {
-        if (s < kh) {
-            const uint32_t off = rb + 64 * s + 16 * q;
-            uint32_t w[4];
-            if constexpr (B == 4) {
-                // 4-band spans start on pixel (4-byte) boundaries.
-                if (off + 16 <= total) {
-                    const u32x4 v = gmem<u32x4>(base + off)[0];
-                    w[0] = v[0]; w[1] = v[1]; w[2] = v[2]; w[3] = v[3];
-                } else {
-                    load16(base, off, total, w);
-                }
-            } else {
-                load16(base, off, total, w);
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                uint32_t v = w[j];
-                if constexpr (B == 2 || B == 4) {
-                    if (premult) v = premult_dword<B>(v);
-                }
-                a[s][j] = (int)(v ^ 0x80808080u);
-            }
+// Register budget of the H operands: up to 6 K steps with one limb, 3 with two.
+template <int LIMBS>
+constexpr int max_steps() {
+    return LIMBS == 1 ? 6 : 3;
+}
+
+// One A-operand step of an M tile: lane (m, q) holds bytes 16q .. 16q + 15
+// of K step s of the span of intermediate row r0 + m (pixels as p ^ 0x80).
+template <int B, bool PM>
+__device__ __forceinline__ i32x4 load_a(const unsigned char* base, uint32_t off, uint32_t total) {
+    uint32_t w[4];
+    if constexpr (B == 4) {
+        // 4-band spans start on pixel (4-byte) boundaries.
+        if (off + 16 <= total) {
+            const u32x4 v = gmem<u32x4>(base + off)[0];
+            w[0] = v[0];
+            w[1] = v[1];
+            w[2] = v[2];
+            w[3] = v[3];
+        } else {
+            load16(base, off, total, w);
         }
+    } else {
+        load16(base, off, total, w);
+    }
+    i32x4 a;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        uint32_t v = w[j];
+        if constexpr ((B == 2 || B == 4) && PM) v = premult_dword<B>(v);
+        a[j] = (int)(v ^ 0x80808080u);
+    }
+    return a;
+}
+
+// Reference rounding of an exact, biased sum.  Family 0: (sum + 2048) >> 12
+// clamped to 0..255.  Family 1: a quotient by multiply-shift, exact for the
+// operand ranges the host admits (Gaussian: mul = ceil(2^32 / scale), scale <
+// 4096; ShrinkH f = 5: mul_hi by ceil(2^34 / 5) >> 2; other shrinks: the
+// reference's own 32-bit ((sum + f/2) * floor(2^24 / f)) >> 24, or the pair
+// rounding (sum + 1) >> 1); these quotients lie in 0..255.
+template <int FAM>
+__device__ __forceinline__ uint32_t sep_round(int acc, uint32_t mul, int shift, int hi) {
+    if constexpr (FAM == 0) {
+        return (uint32_t)min(max(acc >> 12, 0), 255);
+    } else {
+        const uint32_t x = (uint32_t)acc;
+        return (hi ? __umulhi(x, mul) : x * mul) >> shift;
     }
 }
 
-template <int B, int LIMBS, int MODE, bool PM>
+template <int B, int LIMBS, int FAM, bool PM>
 __device__ void sep_task(const SepJob& job, int jx, int ty, unsigned char* lds, size_t wave_lds) {
+    constexpr int KMAX = max_steps<LIMBS>();
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int KH = job.kh, KV = job.kv, RING = job.ring;
-    const int ks = 64 * KH;
     const int RS = RING + 16;  // ring row stride: padded against LDS bank conflicts
+    const int ks = 64 * KH;
     int* scratch = reinterpret_cast<int*>(lds + (size_t)wave * wave_lds);
     int* corr = scratch + 16 * ks;
-    unsigned char* ring = reinterpret_cast<unsigned char*>(corr + 16);
+    unsigned char* ring = FAM == 0 ? reinterpret_cast<unsigned char*>(corr + 16)
+                                   : reinterpret_cast<unsigned char*>(scratch);
     const InView& in = job.in;
     const OutView& out = job.out;
-    const DevPlan& ph = job.ph;
-    const DevPlan& pv = job.pv;
     const int nsamp = out.w * B;
     const int s0 = jx * 16;
     const int n = lane & 15, q = lane >> 4;
     const unsigned char* base = static_cast<const unsigned char*>(in.ptr);
     const uint32_t row_bytes = (uint32_t)in.bw * B;
     const uint32_t total = row_bytes * (uint32_t)in.bh;  // < 2^31 (checked on the host)
-    const int bias = MODE == 0 ? 2048 : (job.scale >> 1);
-    const uint64_t inv = job.inv;
-    constexpr bool premult = (B == 2 || B == 4) && PM;
 
-    // ---- H weights of the strip: B operand, rows j = samples, K = span bytes.
-    const int x_first = s0 / B;
-    const int span0 = col_of(in, cload(ph.start, x_first));  // buffer-relative first span pixel
-    {
+    // ---- H operands of the strip (B operand: rows j = samples, K = span bytes).
+    i32x4 whi[KMAX], wlo[KMAX];
+    int span0, hseed;
+    if constexpr (FAM == 0) {
+        const DevPlan& ph = job.ph;
+        span0 = col_of(in, cload(ph.start, s0 / B));  // buffer-relative first span pixel
         const int smp = s0 + n;
         const bool active = smp < nsamp;
         const int x = active ? smp / B : 0;
         const int b = smp - x * B;
         const int st = ph.start[x];
+        const int sp0 = span0;
         fold_weights(scratch, corr, ks, ph.taps, ph.wi16 + (size_t)ph.phase[x] * ph.taps, active,
-                     [&](int t) { return (col_of(in, st + t) - span0) * B + b; });
-    }
-    i32x4 whi[kMaxSteps], wlo[kMaxSteps];
+                     [&](int t) { return (col_of(in, st + t) - sp0) * B + b; });
 #pragma unroll
-    for (int s = 0; s < kMaxSteps; ++s) {
-        if (s < KH) {
-            if constexpr (LIMBS == 2) read_limbs(scratch, ks, s, whi[s], wlo[s]);
-            else wlo[s] = read_single(scratch, ks, s);
+        for (int s = 0; s < KMAX; ++s) {
+            if (s < KH) {
+                if constexpr (LIMBS == 2) read_limbs(scratch, ks, s, whi[s], wlo[s]);
+                else wlo[s] = read_single(scratch, ks, s);
+            }
         }
+        hseed = 128 * corr[n] + job.h_bias;
+        wave_lds_sync();
+    } else {
+        const int pat = cload(job.hstrip, 2 * jx);
+        span0 = cload(job.hstrip, 2 * jx + 1);
+        const unsigned char* ht = job.htab + (size_t)pat * job.htab_bytes;
+#pragma unroll
+        for (int s = 0; s < KMAX; ++s) {
+            if (s < KH) {
+                if constexpr (LIMBS == 2) {
+                    whi[s] = gmem<i32x4>(ht + ((size_t)(2 * s) * 64 + lane) * 16)[0];
+                    wlo[s] = gmem<i32x4>(ht + ((size_t)(2 * s + 1) * 64 + lane) * 16)[0];
+                } else {
+                    wlo[s] = gmem<i32x4>(ht + ((size_t)s * 64 + lane) * 16)[0];
+                }
+            }
+        }
+        hseed = gmem<int>(ht + (size_t)KH * LIMBS * 64 * 16)[n];
     }
-    const int hseed = 128 * corr[n] + bias;
-    wave_lds_sync();
 
     const int y_begin = ty * job.rows_per_task;
     const int y_end = min(out.h, y_begin + job.rows_per_task);
-    const int vclip0 = job.vclip0, vclip1 = job.vclip1, vtaps = pv.taps;
     const uint32_t span_off = (uint32_t)span0 * B;
-    // V window of the row tile at y0: intermediate rows [lo, hi], K from base16.
-    auto window = [&](int y0, int& hi, int& base16) {
-        const int ylast = min(y0 + 15, y_end - 1);
-        const int lo = clampi(cload(pv.start, y0), vclip0, vclip1);
-        hi = clampi(cload(pv.start, ylast) + vtaps - 1, vclip0, vclip1);
-        base16 = lo & ~15;
-    };
-    auto row_addr = [&](int r) -> uint32_t { return (uint32_t)row_of(in, r + n) * row_bytes + span_off; };
+    const uint32_t lane_off = span_off + 16 * q;
+    auto row_addr = [&](int r) -> uint32_t { return (uint32_t)row_of(in, r + n) * row_bytes + lane_off; };
 
-    i32x4 acur[kMaxSteps], anxt[kMaxSteps];
+    i32x4 acur, anxt;
     int have = INT32_MIN;  // intermediate rows below `have` (a multiple of 16) are in the ring
-    int pre = INT32_MIN;   // M tile whose A operand is already in acur
-    int hi, base16;
-    window(y_begin, hi, base16);
-    for (int y0 = y_begin; y0 < y_end; y0 += 16) {
-        // ---- intermediate rows [max(have, base16), hi], 16 at a time; the
-        // next tile's operand loads are in flight while this one is used.
+    int pre = INT32_MIN;   // M tile whose first-step A operand is in acur
+    int ti = y_begin >> 4;
+    int base16 = cload(job.vtile, 3 * ti + 1), hi = cload(job.vtile, 3 * ti + 2);
+    for (int y0 = y_begin; y0 < y_end; y0 += 16, ++ti) {
+        // ---- V operands of the 16 output rows (A operand, K = intermediate
+        // rows from base16), host-folded; loaded ahead of the H work.
+        const unsigned char* vt = job.vtab + (size_t)cload(job.vtile, 3 * ti) * job.vtab_bytes;
+        const i32x4 vseed = gmem<i32x4>(vt + (size_t)KV * LIMBS * 64 * 16 + 16 * q)[0];
+        i32x4 vop0h = {0, 0, 0, 0}, vop0l;
+        if constexpr (LIMBS == 2) {
+            vop0h = gmem<i32x4>(vt + (size_t)lane * 16)[0];
+            vop0l = gmem<i32x4>(vt + ((size_t)64 + lane) * 16)[0];
+        } else {
+            vop0l = gmem<i32x4>(vt + (size_t)lane * 16)[0];
+        }
+        // ---- intermediate rows [max(have, base16), hi], 16 at a time; each
+        // next operand's load is in flight while the current one is used.
         int r0 = have > base16 ? have : base16;
-        if (r0 <= hi && pre != r0) load_a<B>(base, row_addr(r0), total, q, KH, premult, acur);
+        uint32_t rb = row_addr(r0);
+        if (r0 <= hi && pre != r0) acur = load_a<B, PM>(base, rb, total);
         for (; r0 <= hi; r0 += 16) {
-            const bool more = r0 + 16 <= hi;
-            if (more) load_a<B>(base, row_addr(r0 + 16), total, q, KH, premult, anxt);
             i32x4 dh = {0, 0, 0, 0}, dl = {hseed, hseed, hseed, hseed};
+            const uint32_t rb_next = r0 + 16 <= hi ? row_addr(r0 + 16) : 0;
 #pragma unroll
-            for (int s = 0; s < kMaxSteps; ++s) {
+            for (int s = 0; s < KMAX; ++s) {
                 if (s < KH) {
-                    if constexpr (LIMBS == 2) dh = mfma_i8(acur[s], whi[s], dh);
-                    dl = mfma_i8(acur[s], wlo[s], dl);
+                    if (s + 1 < KH) anxt = load_a<B, PM>(base, rb + 64 * (s + 1), total);
+                    else if (r0 + 16 <= hi) anxt = load_a<B, PM>(base, rb_next, total);
+                    if constexpr (LIMBS == 2) dh = mfma_i8(acur, whi[s], dh);
+                    dl = mfma_i8(acur, wlo[s], dl);
+                    acur = anxt;
                 }
             }
+            rb = rb_next;
             // D[m = row][n = sample]: rows r0 + 4q .. + 3 of sample n, stored
             // in the ring already offset (p ^ 0x80) for the V operand.
             uint32_t packed = 0;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int acc = LIMBS == 2 ? 128 * dh[e] + dl[e] : dl[e];
-                packed |= sep_round<MODE>(acc, inv) << (8 * e);
+                packed |= sep_round<FAM>(acc, job.h_mul, job.h_shift, job.h_hi) << (8 * e);
             }
             *reinterpret_cast<uint32_t*>(ring + n * RS + ((r0 + 4 * q) & (RING - 1))) = packed ^ 0x80808080u;
-            if (more) {
-#pragma unroll
-                for (int s = 0; s < kMaxSteps; ++s) acur[s] = anxt[s];
-            }
         }
         have = r0 > have ? r0 : have;
-        // ---- V weights of the 16 output rows (A operand), pre-folded on the
-        // host; issued before the next tile's A prefetch.
-        const unsigned char* vt = job.vtiles + (size_t)(y0 >> 4) * job.vtile_bytes;
-        i32x4 vhi[kMaxSteps], vlo[kMaxSteps];
-#pragma unroll
-        for (int s = 0; s < kMaxSteps; ++s) {
-            if (s < KV) {
-                if constexpr (LIMBS == 2) {
-                    vhi[s] = gmem<i32x4>(vt + ((size_t)(2 * s) * 64 + lane) * 16)[0];
-                    vlo[s] = gmem<i32x4>(vt + ((size_t)(2 * s + 1) * 64 + lane) * 16)[0];
-                } else {
-                    vlo[s] = gmem<i32x4>(vt + ((size_t)s * 64 + lane) * 16)[0];
-                }
-            }
-        }
-        const i32x4 vseed = gmem<i32x4>(vt + (size_t)KV * LIMBS * 64 * 16 + 16 * q)[0];
-        // Prefetch the first new M tile of the next row tile.
         const int cur_base16 = base16;
+        // Prefetch the first new M tile of the next row tile.
         if (y0 + 16 < y_end) {
-            window(y0 + 16, hi, base16);
+            base16 = cload(job.vtile, 3 * ti + 4);
+            hi = cload(job.vtile, 3 * ti + 5);
             const int r1 = have > base16 ? have : base16;
             if (r1 <= hi) {
-                load_a<B>(base, row_addr(r1), total, q, KH, premult, acur);
+                acur = load_a<B, PM>(base, row_addr(r1), total);
                 pre = r1;
             }
         }
         i32x4 dh = {0, 0, 0, 0}, dl = vseed;
-#pragma unroll
-        for (int s = 0; s < kMaxSteps; ++s) {
-            if (s < KV) {
-                const i32x4 bv =
-                    *reinterpret_cast<const i32x4*>(ring + n * RS + ((cur_base16 + 64 * s + 16 * q) & (RING - 1)));
-                if constexpr (LIMBS == 2) dh = mfma_i8(vhi[s], bv, dh);
-                dl = mfma_i8(vlo[s], bv, dl);
+        for (int s = 0; s < KV; ++s) {
+            const i32x4 bv =
+                *reinterpret_cast<const i32x4*>(ring + n * RS + ((cur_base16 + 64 * s + 16 * q) & (RING - 1)));
+            if constexpr (LIMBS == 2) {
+                dh = mfma_i8(s == 0 ? vop0h : gmem<i32x4>(vt + ((size_t)(2 * s) * 64 + lane) * 16)[0], bv, dh);
+                dl = mfma_i8(s == 0 ? vop0l : gmem<i32x4>(vt + ((size_t)(2 * s + 1) * 64 + lane) * 16)[0], bv, dl);
+            } else {
+                dl = mfma_i8(s == 0 ? vop0l : gmem<i32x4>(vt + ((size_t)s * 64 + lane) * 16)[0], bv, dl);
             }
         }
         // D[m = output row][n = sample]: rows y0 + 4q + e of sample s0 + n.
         uint32_t v[4];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) v[e] = sep_round<MODE>(LIMBS == 2 ? 128 * dh[e] + dl[e] : dl[e], inv);
+        for (int e = 0; e < 4; ++e)
+            v[e] = sep_round<FAM>(LIMBS == 2 ? 128 * dh[e] + dl[e] : dl[e], job.v_mul, job.v_shift, job.v_hi);
         if constexpr (B == 2 || B == 4) {
             if (job.unpremult) {
                 // The pixel's alpha sample is in lane (lane | (B - 1)) of this group.
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const uint32_t a = (uint32_t)__shfl((int)v[e], lane | (B - 1), 64);
-                    if ((n & (B - 1)) != B - 1) v[e] = unpremult_u8(v[e], a);
+                    const uint32_t al = (uint32_t)__shfl((int)v[e], lane | (B - 1), 64);
+                    if ((n & (B - 1)) != B - 1) v[e] = unpremult_u8(v[e], al);
                 }
             }
         }
@@ -363,17 +370,17 @@ __device__ void sep_task(const SepJob& job, int jx, int ty, unsigned char* lds, 
     }
 }
 
-// One kernel per (bands, limbs, rounding mode, premultiply) so each carries
-// only its own registers.
-template <int B, int LIMBS, int MODE, bool PM>
+// One kernel per (bands, limbs, family, premultiply) so each carries only its
+// own registers.
+template <int B, int LIMBS, int FAM, bool PM>
 __global__ __launch_bounds__(kThreads) void k_sep_job(SepJob job, int wave_lds) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int t = blockIdx.x * kSepWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (t >= job.gx * job.gy) return;
-    sep_task<B, LIMBS, MODE, PM>(job, t % job.gx, t / job.gx, lds, (size_t)wave_lds);
+    sep_task<B, LIMBS, FAM, PM>(job, t % job.gx, t / job.gx, lds, (size_t)wave_lds);
 }
 
-template <int B, int LIMBS, int MODE, bool PM>
+template <int B, int LIMBS, int FAM, bool PM>
 __global__ __launch_bounds__(kThreads) void k_sep_batch(const SepJob* __restrict__ jobs,
                                                         const BatchTask* __restrict__ tasks, int ntasks,
                                                         const void* const* __restrict__ srcs,
@@ -385,23 +392,33 @@ __global__ __launch_bounds__(kThreads) void k_sep_batch(const SepJob* __restrict
         SepJob job = jobs[task.job];
         if (job.src_slot >= 0) job.in.ptr = srcs[job.src_slot];
         if (job.dst_slot >= 0) job.out.ptr = dsts[job.dst_slot];
-        sep_task<B, LIMBS, MODE, PM>(job, task.bx, task.by, lds, (size_t)wave_lds);
+        sep_task<B, LIMBS, FAM, PM>(job, task.bx, task.by, lds, (size_t)wave_lds);
     }
 }
 
 }  // namespace
 
 size_t sep_job_wave_lds(const SepJob& job) {
-    const size_t ks = 64 * (size_t)job.kh;
-    return (16 * ks + 16) * sizeof(int) + 16 * ((size_t)job.ring + 16);
+    const size_t ring = 16 * ((size_t)job.ring + 16);
+    if (job.family != 0) return ring;
+    return (16 * 64 * (size_t)job.kh + 16) * sizeof(int) + ring;
 }
 
 int sep_job_variant(const SepJob& job) {
     const int pm = (job.bands == 2 || job.bands == 4) && job.premult ? 1 : 0;
-    return (((job.bands - 1) * 2 + (job.limbs - 1)) * 2 + job.mode) * 2 + pm;
+    return (((job.bands - 1) * 2 + (job.limbs - 1)) * 2 + job.family) * 2 + pm;
 }
 
-// Every (bands, limbs, mode, premultiply) instantiation, indexed by
+bool sep_job_supported(const SepJob& job) {
+    if (job.bands < 1 || job.bands > 4 || job.limbs < 1 || job.limbs > 2 || job.family < 0 || job.family > 1)
+        return false;
+    const int kmax = job.limbs == 1 ? 6 : 3;
+    if (job.kh < 1 || job.kh > kmax || job.kv < 1 || job.kv > 7) return false;
+    if (job.family == 0 && job.kh > 2) return false;
+    return job.ring >= 64 * job.kv + 16 && (job.ring & (job.ring - 1)) == 0 && job.rows_per_task % 16 == 0;
+}
+
+// Every (bands, limbs, family, premultiply) instantiation, indexed by
 // sep_job_variant.
 #define VGPU_SEP_B(X, B)                                                                               \
     X(B, 1, 0, false) X(B, 1, 0, true) X(B, 1, 1, false) X(B, 1, 1, true) X(B, 2, 0, false) X(B, 2, 0, true) \
@@ -410,16 +427,15 @@ int sep_job_variant(const SepJob& job) {
 
 hipError_t launch_sep_job(const SepJob& job, hipStream_t s) {
     if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;
-    if (job.bands < 1 || job.bands > 4 || job.limbs < 1 || job.limbs > 2 || job.mode < 0 || job.mode > 1)
-        return hipErrorInvalidValue;
+    if (!sep_job_supported(job)) return hipErrorInvalidValue;
     const int wl = (int)sep_job_wave_lds(job);
     const int ntasks = job.gx * job.gy;
     const dim3 grid((unsigned)((ntasks + kSepWaves - 1) / kSepWaves));
     const size_t lds = (size_t)wl * kSepWaves;
     const int var = sep_job_variant(job);
     int idx = 0;
-#define X(B, L, M, P)                                                                                  \
-    if (idx++ == var) hipLaunchKernelGGL((k_sep_job<B, L, M, P>), grid, dim3(kThreads), lds, s, job, wl);
+#define X(B, L, F, P)                                                                                  \
+    if (idx++ == var) hipLaunchKernelGGL((k_sep_job<B, L, F, P>), grid, dim3(kThreads), lds, s, job, wl);
     VGPU_SEP_ALL(X)
 #undef X
     return hipGetLastError();
@@ -433,9 +449,9 @@ hipError_t launch_sep_batch(int variant, const SepJob* jobs, const BatchTask* ta
     const size_t lds = wave_lds * kSepWaves;
     const int wl = (int)wave_lds;
     int idx = 0;
-#define X(B, L, M, P)                                                                                   \
+#define X(B, L, F, P)                                                                                   \
     if (idx++ == variant)                                                                               \
-        hipLaunchKernelGGL((k_sep_batch<B, L, M, P>), dim3((unsigned)grid), dim3(kThreads), lds, s, jobs, tasks, \
+        hipLaunchKernelGGL((k_sep_batch<B, L, F, P>), dim3((unsigned)grid), dim3(kThreads), lds, s, jobs, tasks, \
                            ntasks, srcs, dsts, wl);
     VGPU_SEP_ALL(X)
 #undef X
