@@ -158,6 +158,13 @@ int vgpu_pipeline_resize(vgpu_pipeline *p, double hscale, double vscale, int ker
  * plain box kernel on MI355X; kept for comparison); 0 selects the plain
  * integer kernels.  All choices are bit-exact.  No reference counterpart. */
 int vgpu_pipeline_set_integer_mfma(vgpu_pipeline *p, int enable);
+/* Arithmetic of f32 reduce / Gaussian pairs: 0 = the reference's FP64
+ * accumulation in its operation order (default; bit-identical), 1 = plain
+ * FP32 (f32 weights, FMA accumulation, f32 intermediate), 2 = FP16 hi/lo
+ * split on MFMA with FP32 accumulation (falls back to 1 where its windows do
+ * not fit).  Modes 1 and 2 differ from the reference in the last bits.  No
+ * reference counterpart. */
+int vgpu_pipeline_set_float_mode(vgpu_pipeline *p, int mode);
 /* CompiledPipeline output geometry (pipeline/compiled.rs:338-). */
 int vgpu_pipeline_output_info(const vgpu_pipeline *p, uint32_t *width, uint32_t *height,
                               uint32_t *bands, int *format);

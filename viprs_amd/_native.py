@@ -104,6 +104,7 @@ def load() -> C.CDLL:
         "vgpu_op_output_format": (i32, [vp, P(i32)]),
         "vgpu_op_process_region": (i32, [vp, vp, sz, vp, sz, Region, Region]),
         "vgpu_pipeline_set_integer_mfma": (i32, [vp, i32]),
+        "vgpu_pipeline_set_float_mode": (i32, [vp, i32]),
         "vgpu_batch_set_integer_mfma": (i32, [vp, i32]),
         "vgpu_fill_random_u8": (i32, [vp, vp, sz, C.c_uint64]),
         "vgpu_batch_thumbnail_new": (i32, [vp, u32, P(u32), P(u32), P(u32), i32, u32, u32, i32, P(vp)]),

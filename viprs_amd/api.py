@@ -251,6 +251,15 @@ class Pipeline:
         _check(self._lib.vgpu_pipeline_set_integer_mfma(self._h, mode), "set_integer_mfma")
         return self
 
+    FLOAT_EXACT, FLOAT_FP32, FLOAT_FP16X3 = 0, 1, 2
+
+    def set_float_mode(self, mode: int):
+        """f32 pairs: FLOAT_EXACT (the reference's FP64 order, bit-identical;
+        default), FLOAT_FP32 (plain FP32) or FLOAT_FP16X3 (FP16 hi/lo split on
+        MFMA, FP32 accumulation)."""
+        _check(self._lib.vgpu_pipeline_set_float_mode(self._h, int(mode)), "set_float_mode")
+        return self
+
     def stage_count(self) -> int:
         n = self._lib.vgpu_pipeline_stage_count(self._h)
         if n < 0:

@@ -32,6 +32,7 @@ struct DevPlan {
     const uint8_t* phase = nullptr;
     const int16_t* wi16 = nullptr;  // 65 x taps
     const double* wf64 = nullptr;   // 65 x taps
+    const float* wf32 = nullptr;    // 65 x taps (float modes)
     int taps = 1;
 };
 
@@ -137,6 +138,9 @@ int sep_job_variant(const SepJob& job);
 hipError_t launch_sep_batch(int variant, const SepJob* jobs, const BatchTask* tasks, int ntasks,
                             const void* const* srcs, void* const* dsts, int grid, size_t wave_lds,
                             hipStream_t s);
+// Float separable pair in plain FP32: f32 weights, FMA accumulation, f32
+// intermediate (ReduceJob tiling; ph / pv carry wf32).
+hipError_t launch_red_f32_job(const ReduceJob& job, hipStream_t s);
 hipError_t launch_conv(const InView& in, const OutView& out, int bands, int in_fmt, int out_fmt,
                        int kind, int vertical, const int16_t* itaps, int32_t iscale,
                        const double* wts, int ntaps, double cscale, double coffset,

@@ -288,6 +288,42 @@ void stage_reduce_params(const Node& nd, const Rect& out, Staging* st, Stage* s)
     s->taps = nd.axis.taps;
 }
 
+// Tile of the fused VALU reduce kernels.  Candidates, largest first: take the
+// first whose staged rows fit 64 KiB of LDS and that still gives >=
+// opt.reduce_min_blocks blocks; otherwise the fitting candidate with the most
+// blocks.  The search starts at candidate opt.reduce_first_tile.
+bool pick_reduce_tile(const std::vector<int32_t>& vs, int vtaps, const Rect& out, uint32_t bands, size_t esz,
+                      const StageOptions& opt, int* tx, int* ty, int* nr) {
+    static const int cand[][2] = {{64, 16}, {32, 16}, {32, 8}, {16, 8}, {16, 4}, {8, 4}, {8, 2}, {8, 1}, {4, 1}, {1, 1}};
+    int64_t best_blocks = -1;
+    const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
+    for (int ci = std::min(std::max(opt.reduce_first_tile, 0), ncand - 1); ci < ncand; ++ci) {
+        const int* c = cand[ci];
+        int rows = 1;
+        for (int64_t y0 = 0; y0 < out.h; y0 += c[1]) {
+            const int64_t y1 = std::min<int64_t>(y0 + c[1], out.h) - 1;
+            rows = std::max<int>(rows, vs[(size_t)y1] + vtaps - vs[(size_t)y0]);
+        }
+        if ((size_t)rows * c[0] * bands * esz > 64 * 1024) continue;
+        const int64_t blocks = ((out.w + c[0] - 1) / c[0]) * ((out.h + c[1] - 1) / c[1]);
+        if (blocks > best_blocks) {
+            best_blocks = blocks;
+            *tx = c[0];
+            *ty = c[1];
+            *nr = rows;
+        }
+        if (blocks >= opt.reduce_min_blocks) break;
+    }
+    return best_blocks >= 0;
+}
+
+// f32 copy of a weight table.
+size_t put_f32(const std::vector<double>& w, Staging* st) {
+    std::vector<float> f(w.size());
+    for (size_t i = 0; i < w.size(); ++i) f[i] = (float)w[i];
+    return st->put(f.data(), f.size() * sizeof(float));
+}
+
 // Fused reduce stage: H plan over the intermediate's columns (`mid`), V plan
 // over the output rows, and a tile shape whose staged rows fit 64 KiB of LDS.
 // `accumulate_h` selects the u8 ReduceH plan; an identity axis stands in for
@@ -304,33 +340,9 @@ bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const Redu
         const SepAxis ha{&hs, &hp, hax.wi16.data(), hax.taps}, va{&vs, &vp, vax.wi16.data(), vax.taps};
         plan_sep(0, ha, va, 2048, 2048, bands, out, clip, buf, vclip, opt, st, s);  // ST_SEP on success
     }
-    const size_t esz = format_size(fmt);
-    // Tile candidates, largest first: take the first whose staged rows fit
-    // 64 KiB of LDS and that still gives >= opt.reduce_min_blocks blocks;
-    // otherwise the fitting candidate with the most blocks.  The search starts
-    // at candidate opt.reduce_first_tile.
-    static const int cand[][2] = {{64, 16}, {32, 16}, {32, 8}, {16, 8}, {16, 4}, {8, 4}, {8, 2}, {8, 1}, {4, 1}, {1, 1}};
     int tx = 0, ty = 0, nr = 0;
-    int64_t best_blocks = -1;
-    const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
-    for (int ci = std::min(std::max(opt.reduce_first_tile, 0), ncand - 1); ci < ncand; ++ci) {
-        const int* c = cand[ci];
-        int rows = 1;
-        for (int64_t y0 = 0; y0 < out.h; y0 += c[1]) {
-            const int64_t y1 = std::min<int64_t>(y0 + c[1], out.h) - 1;
-            rows = std::max<int>(rows, vs[(size_t)y1] + vax.taps - vs[(size_t)y0]);
-        }
-        if ((size_t)rows * c[0] * bands * esz > 64 * 1024) continue;
-        const int64_t blocks = ((out.w + c[0] - 1) / c[0]) * ((out.h + c[1] - 1) / c[1]);
-        if (blocks > best_blocks) {
-            best_blocks = blocks;
-            tx = c[0];
-            ty = c[1];
-            nr = rows;
-        }
-        if (blocks >= opt.reduce_min_blocks) break;
-    }
-    if (best_blocks < 0) {
+    const bool tile_ok = pick_reduce_tile(vs, vax.taps, out, bands, format_size(fmt), opt, &tx, &ty, &nr);
+    if (!tile_ok) {
         if (s->kind != ST_SEP) return false;
         tx = ty = nr = 1;  // the VALU tile is not used
     }
@@ -365,6 +377,11 @@ bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const Redu
             worst = std::max(worst, sabs * maxv);
         }
     j.wide_acc = worst >= (int64_t)INT32_MAX - 4096;
+    if (fmt == F32 && opt.float_mode >= 1 && tile_ok) {
+        s->w32 = put_f32(hax.wf64, st);
+        s->v_w32 = put_f32(vax.wf64, st);
+        s->kind = ST_RED_F32;
+    }
     return true;
 }
 
@@ -395,6 +412,41 @@ bool stage_gauss_sep_params(const Node& gh, const Node& gv, const Rect& mid, con
         j.h_hi = j.v_hi = 1;
     }
     j.h_shift = j.v_shift = 0;
+    return true;
+}
+
+// GaussBlurH -> GaussBlurV on f32 images in plain FP32: the float kernel
+// taps / scale (gauss_blur.rs G1) as f32 weights, one phase.
+bool stage_gauss_float_params(const Node& gh, const Node& gv, const Rect& mid, const Rect& out, const Rect& vclip,
+                              const StageOptions& opt, Staging* st, Stage* s) {
+    const int taps = (int)gh.conv.size();
+    const int rad = (taps - 1) / 2;
+    if (taps < 1 || gv.conv != gh.conv) return false;
+    std::vector<int32_t> hs((size_t)std::max<int64_t>(mid.w, 1)), vs((size_t)std::max<int64_t>(out.h, 1));
+    for (int64_t x = 0; x < mid.w; ++x) hs[(size_t)x] = (int32_t)(mid.x + x - rad);
+    for (int64_t y = 0; y < out.h; ++y) vs[(size_t)y] = (int32_t)(out.y + y - rad);
+    int tx, ty, nr;
+    if (!pick_reduce_tile(vs, taps, out, gh.bands, sizeof(float), opt, &tx, &ty, &nr)) return false;
+    const std::vector<uint8_t> hp(hs.size(), 0), vp(vs.size(), 0);
+    s->plan_start = st->put(hs.data(), hs.size() * sizeof(int32_t));
+    s->plan_phase = st->put(hp.data(), hp.size());
+    s->w32 = put_f32(gh.conv, st);
+    s->taps = taps;
+    s->v_start = st->put(vs.data(), vs.size() * sizeof(int32_t));
+    s->v_phase = st->put(vp.data(), vp.size());
+    s->v_w32 = put_f32(gv.conv, st);
+    s->v_taps = taps;
+    ReduceJob& j = s->rjob;
+    j.bands = (int)gh.bands;
+    j.fmt = F32;
+    j.tx = tx;
+    j.ty = ty;
+    j.nr_max = nr;
+    j.vclip0 = (int)vclip.y;
+    j.vclip1 = (int)(vclip.y + std::max<int64_t>(vclip.h, 1) - 1);
+    j.gx = (int)((out.w + tx - 1) / tx);
+    j.gy = (int)((out.h + ty - 1) / ty);
+    s->kind = ST_RED_F32;
     return true;
 }
 
@@ -469,6 +521,8 @@ ReduceJob reduce_job_of(const Stage& s, const InView& in, const OutView& out, co
     j.pv.wi16 = reinterpret_cast<const int16_t*>(blob + s.v_w16);
     j.pv.wf64 = reinterpret_cast<const double*>(blob + s.v_w64);
     j.pv.taps = s.v_taps;
+    j.ph.wf32 = reinterpret_cast<const float*>(blob + s.w32);
+    j.pv.wf32 = reinterpret_cast<const float*>(blob + s.v_w32);
     j.premult = s.premult;
     j.unpremult = s.unpremult;
     return j;
@@ -527,6 +581,7 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
     }
     case ST_REDUCE_HV: e = launch_reduce_job(reduce_job_of(s, in, out, blob), stream); break;
     case ST_SEP: e = launch_sep_job(sep_job_of(s, in, out, blob), stream); break;
+    case ST_RED_F32: e = launch_red_f32_job(reduce_job_of(s, in, out, blob), stream); break;
     case ST_CONV: {
         const bool gauss = nd.kind == N_GAUSS_H || nd.kind == N_GAUSS_V;
         const bool vert = nd.kind == N_GAUSS_V || nd.kind == N_CONVSEP_V;
@@ -631,6 +686,15 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
             // GaussBlurH -> GaussBlurV (u8 -> u8) on the MFMA kernel.
             if (stage_gauss_sep_params(nd, nodes[(size_t)i + 1], out_r[(size_t)i], out_r[(size_t)i + 1],
                                        in_r[(size_t)i + 1], s.clip, buf, opt, st, &s)) {
+                s.last = i + 1;
+                done = true;
+            }
+        }
+        if (!done && small_bands && opt.float_mode >= 1 && nd.kind == N_GAUSS_H && kind_at(i + 1) == N_GAUSS_V &&
+            nd.in_fmt == F32 && nodes[(size_t)i + 1].out_fmt == F32) {
+            // GaussBlurH -> GaussBlurV on f32 in a fast float mode.
+            if (stage_gauss_float_params(nd, nodes[(size_t)i + 1], out_r[(size_t)i], out_r[(size_t)i + 1],
+                                         in_r[(size_t)i + 1], opt, st, &s)) {
                 s.last = i + 1;
                 done = true;
             }
@@ -1007,6 +1071,15 @@ int vgpu_pipeline_set_integer_mfma(vgpu_pipeline* p, int enable) {
     p->release();
     p->opt.integer_mfma = (enable & 1) != 0;
     p->opt.mfma_shrink = (enable & 2) != 0;
+    return VGPU_OK;
+}
+
+int vgpu_pipeline_set_float_mode(vgpu_pipeline* p, int mode) {
+    if (!p || mode < 0 || mode > 2) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(p->ctx->device);
+    if (hipStreamSynchronize(p->ctx->stream) != hipSuccess) return p->ctx->sticky = VGPU_E_DEVICE;
+    p->release();
+    p->opt.float_mode = mode;
     return VGPU_OK;
 }
 

@@ -104,7 +104,8 @@ struct Staging {
 };
 
 enum StageKind {
-    ST_SEP,  // fused u8 H -> V pair on MFMA (reduce or Gaussian)
+    ST_SEP,      // fused u8 H -> V pair on MFMA (reduce, Gaussian or box)
+    ST_RED_F32,  // fused f32 H -> V pair in plain FP32 (reduce or Gaussian)
     ST_SHRINK_HV_U8,
     ST_SHRINK,
     ST_REDUCE_H,
@@ -133,6 +134,7 @@ struct Stage {
     // ST_SEP blob offsets: V operand patterns, per-tile (pattern, base16, hi),
     // and (family 1) H operand patterns, per-strip (pattern, first pixel).
     size_t v_tab = 0, v_tile = 0, h_tab = 0, h_strip = 0;
+    size_t w32 = 0, v_w32 = 0;  // ST_RED_F32: f32 weight tables
 };
 
 // Stage-planning choices that depend on how the stage is launched: a single
@@ -144,6 +146,9 @@ struct StageOptions {
     bool integer_mfma = true;          // u8 reduce / Gaussian pairs on the int8-limb MFMA kernel
     bool mfma_shrink = false;          // u8 box pre-shrink on it too (slower than the VALU kernel)
     int sep_rows_per_task = 0;         // output rows per MFMA wavefront task (0 = by chip fill)
+    // f32 images: 0 = the reference's FP64 accumulation order (bit-exact),
+    // 1 = plain FP32, 2 = FP16 hi/lo split on MFMA (falls back to 1).
+    int float_mode = 0;
     bool shrink_row_split = true;      // split a shrink row's source rows over wavefronts
 };
 
