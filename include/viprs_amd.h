@@ -170,6 +170,10 @@ int vgpu_pipeline_output_info(const vgpu_pipeline *p, uint32_t *width, uint32_t 
                               uint32_t *bands, int *format);
 /* Number of GPU launches one run makes after fusion (diagnostic). */
 int vgpu_pipeline_stage_count(vgpu_pipeline *p);
+/* Name of the kernel stage `index` launches ("sep_mfma_i8", "reduce_f32",
+ * "sep_mfma_f16x3", "shrink", "reduce_hv", ...), NUL-terminated into
+ * name[0 .. cap).  Diagnostic; no reference counterpart. */
+int vgpu_pipeline_stage_kernel(vgpu_pipeline *p, int index, char *name, size_t cap);
 
 /* TileScheduler::run (crates/viprs-ports/src/scheduler.rs:63-98) with the
  * device region executor: runs the whole node chain over the full output on

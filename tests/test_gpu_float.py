@@ -18,7 +18,10 @@ from tests.helpers import float_image
 
 pytestmark = pytest.mark.gpu
 
-MODES = [v.Pipeline.FLOAT_FP32]
+MODES = [v.Pipeline.FLOAT_FP32, v.Pipeline.FLOAT_FP16X3]
+
+
+KERNEL = {0: "reduce_hv", v.Pipeline.FLOAT_FP32: "reduce_f32", v.Pipeline.FLOAT_FP16X3: "sep_mfma_f16x3"}
 
 
 def errors(got, ref):
@@ -27,11 +30,19 @@ def errors(got, ref):
     return float(err.max()), float(rel.max())
 
 
-def run_mode(ctx, src, steps, mode):
+def where(got, ref):
+    """First worst position and count of large errors, for failure messages."""
+    err = np.abs(got.astype(np.float64) - ref)
+    return np.unravel_index(int(np.argmax(err)), err.shape), int((err > 1e-4).sum())
+
+
+def run_mode(ctx, src, steps, mode, kernel=None):
     h, w, b = src.shape
     p = v.Pipeline(ctx, w, h, b, v.F32).set_float_mode(mode)
     for name, args in steps:
         getattr(p, name)(*args)
+    if kernel is not None:
+        assert p.stage_kernels() == [kernel], (mode, p.stage_kernels())
     return p.run_host(src)
 
 
@@ -42,11 +53,11 @@ def test_float_reduce_against_fp64(ctx, mode, bands):
     ref = O.f64_reduce_hv(src, 2.5, O.LANCZOS3)
     steps = [("reduce_h", (2.5, v.LANCZOS3)), ("reduce_v", (2.5, v.LANCZOS3))]
     exact = run_mode(ctx, src, steps, 0)
-    fast = run_mode(ctx, src, steps, mode)
+    fast = run_mode(ctx, src, steps, mode, KERNEL[mode])
     assert fast.shape == ref.shape
     e_abs, e_rel = errors(fast, ref)
     x_abs, _ = errors(exact, ref)
-    assert e_abs <= 2e-6, (e_abs, x_abs)
+    assert e_abs <= 2e-6, (e_abs, x_abs, where(fast, ref))
     assert e_rel <= 1e-6 * 1.6 ** 2 * 2, (e_rel, x_abs)
 
 
@@ -55,11 +66,12 @@ def test_float_reduce_against_fp64(ctx, mode, bands):
 def test_float_gauss_against_fp64(ctx, mode, sigma):
     src = float_image(211, 307, 3, 40)
     ref = O.f64_gauss(src, sigma)
-    fast = run_mode(ctx, src, [("gauss_blur", (sigma,))], mode)
+    kernel = "reduce_f32" if mode == v.Pipeline.FLOAT_FP32 else "sep_mfma_f16x3"
+    fast = run_mode(ctx, src, [("gauss_blur", (sigma,))], mode, kernel)
     exact = run_mode(ctx, src, [("gauss_blur", (sigma,))], 0)
     e_abs, e_rel = errors(fast, ref)
     x_abs, _ = errors(exact, ref)
-    assert e_abs <= 2e-6, (e_abs, x_abs)
+    assert e_abs <= 2e-6, (e_abs, x_abs, where(fast, ref))
     assert e_rel <= 2e-6, (e_rel, x_abs)
 
 

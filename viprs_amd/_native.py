@@ -86,6 +86,7 @@ def load() -> C.CDLL:
         "vgpu_pipeline_resize": (i32, [vp, dbl, dbl, i32]),
         "vgpu_pipeline_output_info": (i32, [vp, P(u32), P(u32), P(u32), P(i32)]),
         "vgpu_pipeline_stage_count": (i32, [vp]),
+        "vgpu_pipeline_stage_kernel": (i32, [vp, i32, C.c_char_p, sz]),
         "vgpu_pipeline_run": (i32, [vp, vp, sz, vp, sz]),
         "vgpu_pipeline_run_host": (i32, [vp, vp, sz, vp, sz]),
         "vgpu_pipeline_run_profiled": (i32, [vp, vp, sz, vp, sz, i32, P(f32), i32]),

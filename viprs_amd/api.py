@@ -266,6 +266,15 @@ class Pipeline:
             raise VgpuError(n, "stage_count")
         return n
 
+    def stage_kernels(self) -> list:
+        """Names of the kernels the fused stages launch, in order."""
+        out = []
+        for i in range(self.stage_count()):
+            buf = C.create_string_buffer(64)
+            _check(self._lib.vgpu_pipeline_stage_kernel(self._h, i, buf, 64), "stage_kernel")
+            out.append(buf.value.decode())
+        return out
+
     def run(self, src, dst) -> None:
         """Enqueue on the context stream; src / dst are device tensors."""
         _check(self._lib.vgpu_pipeline_run(self._h, C.c_void_p(_ptr(src)), _nbytes(src),

@@ -65,7 +65,218 @@ __global__ __launch_bounds__(kThreads) void k_red_f32(ReduceJob job) {
     }
 }
 
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4v __attribute__((ext_vector_type(4)));
+
+constexpr int kF16MaxH = 4;  // H K steps (128 samples)
+constexpr int kF16MaxV = 2;  // V K steps (64 rows)
+
+__device__ __forceinline__ f32x4 mfma_f16(u32x4v a, u32x4v b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c,
+                                                  0, 0, 0);
+}
+
+// Split 8 floats into FP16 hi and lo operands: hi = f16(x), lo = f16(x - hi),
+// both rounded to nearest; x - hi is exact in FP32.  The caller scales x by a
+// power of two first so that |x| < 2^14 (no overflow) and the lo limbs of the
+// values that matter stay normal.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split8(const float (&x)[8], u32x4v& hi, u32x4v& lo) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const f32x2 v = {x[2 * j], x[2 * j + 1]};
+        const f16x2 h = __builtin_convertvector(v, f16x2);
+        const f32x2 r = v - __builtin_convertvector(h, f32x2);
+        const f16x2 l = __builtin_convertvector(r, f16x2);
+        hi[j] = __builtin_bit_cast(uint32_t, h);
+        lo[j] = __builtin_bit_cast(uint32_t, l);
+    }
+}
+
+// Exponent s with max|x| * 2^s < 2^14 over the whole wavefront (wave-uniform).
+__device__ __forceinline__ int wave_scale_exp(float m) {
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1) m = fmaxf(m, __shfl_xor(m, k));
+    return 14 - __builtin_amdgcn_readfirstlane(__builtin_amdgcn_frexp_expf(m));
+}
+
+// 8 consecutive floats of the input buffer from element `e` (bytes past the
+// end read as 0).
+__device__ __forceinline__ void load8f(const float* base, uint32_t e, uint32_t total, float (&x)[8]) {
+    if (e + 8 <= total) {
+        // Loaded as float vectors: __builtin_bit_cast of a vector element
+        // (a[i]) reads element 0 whatever i is with this compiler.
+        const f32x4 a = gmem<f32x4>(base + e)[0];
+        const f32x4 b = gmem<f32x4>(base + e + 4)[0];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            x[i] = a[i];
+            x[4 + i] = b[i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = e + i < total ? gmem<float>(base)[e + i] : 0.f;
+    }
+}
+
+template <int B>
+__device__ void f16_task(const F16Job& job, int jx, int ty, unsigned char* lds, size_t wave_lds) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int KH = job.kh, KV = job.kv, RING = job.ring;
+    const int RS = RING + 8;  // ring row stride in floats (padded against bank conflicts)
+    float* ring = reinterpret_cast<float*>(lds + (size_t)wave * wave_lds);
+    const InView& in = job.in;
+    const OutView& out = job.out;
+    const int nsamp = out.w * B;
+    const int s0 = jx * 16;
+    const int n = lane & 15, q = lane >> 4;
+    const float* base = static_cast<const float*>(in.ptr);
+    const uint32_t row_elems = (uint32_t)in.bw * B;
+    const uint32_t total = row_elems * (uint32_t)in.bh;
+
+    // H operands of the strip (B operand: rows j = samples, K = span samples).
+    const int pat = __builtin_amdgcn_readfirstlane(job.hstrip[2 * jx]);
+    const int span0 = __builtin_amdgcn_readfirstlane(job.hstrip[2 * jx + 1]);
+    const unsigned char* ht = job.htab + (size_t)pat * job.htab_bytes;
+    u32x4v whi[kF16MaxH], wlo[kF16MaxH];
+#pragma unroll
+    for (int s = 0; s < kF16MaxH; ++s) {
+        if (s < KH) {
+            const u32x4 a = gmem<u32x4>(ht + ((size_t)(2 * s) * 64 + lane) * 16)[0];
+            const u32x4 b = gmem<u32x4>(ht + ((size_t)(2 * s + 1) * 64 + lane) * 16)[0];
+            whi[s] = u32x4v{a[0], a[1], a[2], a[3]};
+            wlo[s] = u32x4v{b[0], b[1], b[2], b[3]};
+        }
+    }
+    const uint32_t lane_off = (uint32_t)span0 * B + 8 * q;
+
+    const int y_begin = ty * job.rows_per_task;
+    const int y_end = min(out.h, y_begin + job.rows_per_task);
+    int have = INT32_MIN;
+    for (int y0 = y_begin, ti = y_begin >> 4; y0 < y_end; y0 += 16, ++ti) {
+        const int vpat = job.vtile[3 * ti];
+        const int base8 = job.vtile[3 * ti + 1], hi = job.vtile[3 * ti + 2];
+        // ---- intermediate rows from max(have, base8 rounded to 16) through hi.
+        int r0 = max(have, base8 & ~15);
+        for (; r0 <= hi; r0 += 16) {
+            const uint32_t rb = (uint32_t)row_of(in, r0 + n) * row_elems + lane_off;
+            float x[kF16MaxH][8];
+            float mx = 0.f;
+#pragma unroll
+            for (int s = 0; s < kF16MaxH; ++s) {
+                if (s < KH) {
+                    load8f(base, rb + 32 * s, total, x[s]);
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) mx = fmaxf(mx, fabsf(x[s][e]));
+                }
+            }
+            const int sx = wave_scale_exp(mx);
+            f32x4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < kF16MaxH; ++s) {
+                if (s < KH) {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) x[s][e] = __builtin_amdgcn_ldexpf(x[s][e], sx);
+                    u32x4v ah, al;
+                    split8(x[s], ah, al);
+                    d = mfma_f16(ah, whi[s], d);
+                    d = mfma_f16(ah, wlo[s], d);
+                    d = mfma_f16(al, whi[s], d);
+                }
+            }
+            const int ux = -(sx + job.h_exp);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) d[e] = __builtin_amdgcn_ldexpf(d[e], ux);
+            // D[m = row][n = sample]: rows r0 + 4q .. + 3 of sample n.
+            *reinterpret_cast<f32x4*>(ring + n * RS + ((r0 + 4 * q) & (RING - 1))) = d;
+        }
+        have = max(have, r0);
+        // ---- V: A = folded V weights (rows j = output rows, K = rows from base8).
+        // Ring rows past `hi` carry zero weight but may hold stale values:
+        // they read as 0 so that they neither reach the sum nor the scale.
+        const unsigned char* vt = job.vtab + (size_t)vpat * job.vtab_bytes;
+        float yv[kF16MaxV][8];
+        float my = 0.f;
+#pragma unroll
+        for (int s = 0; s < kF16MaxV; ++s) {
+            if (s < KV) {
+                const int rk = base8 + 32 * s + 8 * q;
+                const float* rp = ring + n * RS + (rk & (RING - 1));
+                const f32x4 r0v = *reinterpret_cast<const f32x4*>(rp);
+                const f32x4 r1v = *reinterpret_cast<const f32x4*>(rp + 4);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    yv[s][e] = rk + e <= hi ? r0v[e] : 0.f;
+                    yv[s][4 + e] = rk + 4 + e <= hi ? r1v[e] : 0.f;
+                }
+#pragma unroll
+                for (int e = 0; e < 8; ++e) my = fmaxf(my, fabsf(yv[s][e]));
+            }
+        }
+        const int sy = wave_scale_exp(my);
+        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < kF16MaxV; ++s) {
+            if (s < KV) {
+                const u32x4 a = gmem<u32x4>(vt + ((size_t)(2 * s) * 64 + lane) * 16)[0];
+                const u32x4 b = gmem<u32x4>(vt + ((size_t)(2 * s + 1) * 64 + lane) * 16)[0];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) yv[s][e] = __builtin_amdgcn_ldexpf(yv[s][e], sy);
+                u32x4v bh, bl;
+                split8(yv[s], bh, bl);
+                const u32x4v vh = u32x4v{a[0], a[1], a[2], a[3]}, vl = u32x4v{b[0], b[1], b[2], b[3]};
+                d = mfma_f16(vh, bh, d);
+                d = mfma_f16(vl, bh, d);
+                d = mfma_f16(vh, bl, d);
+            }
+        }
+        const int uy = -(sy + job.v_exp);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) d[e] = __builtin_amdgcn_ldexpf(d[e], uy);
+        // D[m = output row][n = sample]: rows y0 + 4q + e of sample s0 + n.
+        const int smp = s0 + n;
+        if (smp < nsamp) {
+            float* o = static_cast<float*>(out.ptr) + (size_t)(y0 + 4 * q) * nsamp + smp;
+            const int rows_left = y_end - (y0 + 4 * q);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (e < rows_left) o[(size_t)e * nsamp] = d[e];
+        }
+    }
+}
+
+template <int B>
+__global__ __launch_bounds__(kThreads) void k_f16_job(F16Job job, int wave_lds) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int t = blockIdx.x * (kThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (t >= job.gx * job.gy) return;
+    f16_task<B>(job, t % job.gx, t / job.gx, lds, (size_t)wave_lds);
+}
+
 }  // namespace
+
+size_t f16_job_wave_lds(const F16Job& job) { return 16 * ((size_t)job.ring + 8) * sizeof(float); }
+
+hipError_t launch_f16_job(const F16Job& job, hipStream_t s) {
+    if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;
+    if (job.kh < 1 || job.kh > kF16MaxH || job.kv < 1 || job.kv > kF16MaxV || job.ring < 32 * job.kv + 16)
+        return hipErrorInvalidValue;
+    const int wl = (int)f16_job_wave_lds(job);
+    const int ntasks = job.gx * job.gy;
+    const dim3 grid((unsigned)((ntasks + 3) / 4));
+    const size_t lds = (size_t)wl * 4;
+    switch (job.bands) {
+    case 1: hipLaunchKernelGGL(k_f16_job<1>, grid, dim3(kThreads), lds, s, job, wl); break;
+    case 2: hipLaunchKernelGGL(k_f16_job<2>, grid, dim3(kThreads), lds, s, job, wl); break;
+    case 3: hipLaunchKernelGGL(k_f16_job<3>, grid, dim3(kThreads), lds, s, job, wl); break;
+    case 4: hipLaunchKernelGGL(k_f16_job<4>, grid, dim3(kThreads), lds, s, job, wl); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_red_f32_job(const ReduceJob& job, hipStream_t s) {
     if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;

@@ -141,6 +141,27 @@ hipError_t launch_sep_batch(int variant, const SepJob* jobs, const BatchTask* ta
 // Float separable pair in plain FP32: f32 weights, FMA accumulation, f32
 // intermediate (ReduceJob tiling; ph / pv carry wf32).
 hipError_t launch_red_f32_job(const ReduceJob& job, hipStream_t s);
+
+// Float separable pair on MFMA (v_mfma_f32_16x16x32_f16) with an FP16 hi/lo
+// split of pixels and weights (x = hi + lo, products hi*hi + hi*lo + lo*hi)
+// and FP32 accumulation.  Operand patterns are host-folded like SepJob family
+// 1: per 16-sample strip (pattern, first span pixel), per 16 output rows
+// (pattern, base8, hi); K steps of 32 samples / rows.
+struct F16Job {
+    InView in;
+    OutView out;
+    const unsigned char* htab = nullptr;
+    const int32_t* hstrip = nullptr;
+    int htab_bytes = 0;
+    const unsigned char* vtab = nullptr;
+    const int32_t* vtile = nullptr;
+    int vtab_bytes = 0;
+    int bands = 3, kh = 1, kv = 1, ring = 128, rows_per_task = 16;
+    int gx = 0, gy = 0;
+    int h_exp = 0, v_exp = 0;  // the tables hold weights * 2^h_exp (H), * 2^v_exp (V)
+};
+size_t f16_job_wave_lds(const F16Job& job);
+hipError_t launch_f16_job(const F16Job& job, hipStream_t s);
 hipError_t launch_conv(const InView& in, const OutView& out, int bands, int in_fmt, int out_fmt,
                        int kind, int vertical, const int16_t* itaps, int32_t iscale,
                        const double* wts, int ntaps, double cscale, double coffset,

@@ -324,6 +324,128 @@ size_t put_f32(const std::vector<double>& w, Staging* st) {
     return st->put(f.data(), f.size() * sizeof(float));
 }
 
+// Float pair on the FP16 hi/lo MFMA kernel: fold the f64 weights of each
+// 16-sample strip and each 16 output rows (clamped taps added), split every
+// folded weight w (as f32) into FP16 hi = f16(w), lo = f16(w - hi), and lay
+// them out as operands of K steps of 32 (lane j + 16 q holds K = 32 s + 8 q
+// .. + 7), deduplicated.  Returns false when a window does not fit.
+struct F16Axis {
+    const std::vector<int32_t>* start = nullptr;
+    const std::vector<uint8_t>* phase = nullptr;  // null: one weight row
+    const double* wtab = nullptr;
+    int taps = 1;
+    const double* row(size_t i) const { return wtab + (phase ? (size_t)(*phase)[i] * taps : 0); }
+};
+
+void emit_f16_operands(const std::vector<double>& w, int ksteps, int exp2, std::vector<unsigned char>* out) {
+    out->assign((size_t)ksteps * 2 * 64 * 16, 0);
+    const int kw = 32 * ksteps;
+    for (int s = 0; s < ksteps; ++s)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int e = 0; e < 8; ++e) {
+                const int j = lane & 15, q = lane >> 4;
+                const float f = std::ldexp((float)w[(size_t)(j * kw + 32 * s + 8 * q + e)], exp2);
+                const _Float16 hi = (_Float16)f;
+                const _Float16 lo = (_Float16)(f - (float)hi);
+                std::memcpy(out->data() + ((size_t)(2 * s) * 64 + lane) * 16 + 2 * e, &hi, 2);
+                std::memcpy(out->data() + ((size_t)(2 * s + 1) * 64 + lane) * 16 + 2 * e, &lo, 2);
+            }
+}
+
+bool plan_f16(const F16Axis& hax, const F16Axis& vax, uint32_t bands, const Rect& out, const Rect& clip,
+              const Rect& buf, const Rect& vclip, const StageOptions& opt, Staging* st, Stage* stg) {
+    F16Job* j = &stg->f16;
+    const int64_t nsamp = out.w * (int64_t)bands;
+    if (nsamp <= 0 || out.h <= 0 || bands < 1 || bands > 4) return false;
+    if (buf.w * buf.h * (int64_t)bands >= ((int64_t)1 << 31)) return false;  // 32-bit element offsets
+    const std::vector<int32_t>& hs = *hax.start;
+    const std::vector<int32_t>& vs = *vax.start;
+    auto hcol = [&](int64_t x) { return clamp_rel(x, clip.x, clip.w, buf.x, buf.w); };
+    const int64_t v0 = vclip.y, v1 = vclip.y + std::max<int64_t>(vclip.h, 1) - 1;
+    auto vrow = [&](int64_t r) { return std::min(std::max(r, v0), v1); };
+    const int64_t ntiles = (out.h + 15) / 16, nstrips = (nsamp + 15) / 16;
+    int64_t span = 0, window = 0;
+    for (int64_t jx = 0; jx < nstrips; ++jx) {
+        const int64_t s0 = 16 * jx, s1 = std::min<int64_t>(s0 + 15, nsamp - 1);
+        span = std::max(span, (hcol(hs[(size_t)(s1 / bands)] + hax.taps - 1) - hcol(hs[(size_t)(s0 / bands)]) + 1) *
+                                  (int64_t)bands);
+    }
+    for (int64_t ti = 0; ti < ntiles; ++ti) {
+        const int64_t y0 = 16 * ti, y1 = std::min<int64_t>(y0 + 15, out.h - 1);
+        const int64_t lo = vrow(vs[(size_t)y0]), hi = vrow(vs[(size_t)y1] + vax.taps - 1);
+        window = std::max(window, hi - (lo & ~(int64_t)7) + 1);
+    }
+    const int kh = (int)((span + 31) / 32), kv = (int)((window + 31) / 32);
+    if (kh < 1 || kh > 4 || kv < 1 || kv > 2) return false;
+    j->bands = (int)bands;
+    j->kh = kh;
+    j->kv = kv;
+    j->ring = 128;
+    j->gx = (int)nstrips;
+    int rpt = opt.sep_rows_per_task & ~15;  // whole 16-row tiles per task
+    if (rpt <= 0) {
+        rpt = 128;
+        while (rpt > 16 && (int64_t)j->gx * ((out.h + rpt - 1) / rpt) < 2048) rpt /= 2;
+    }
+    j->rows_per_task = rpt;
+    j->gy = (int)((out.h + rpt - 1) / rpt);
+    // Weight scales: the folded sums are bounded by the sum of |w| of a row,
+    // so max|w| * 2^exp < 2^14 holds for every folded weight.
+    auto axis_exp = [](const F16Axis& a, size_t rows) {
+        double m = 0;
+        for (size_t i = 0; i < rows; ++i) {
+            double t = 0;
+            for (int k = 0; k < a.taps; ++k) t += std::fabs(a.row(i)[k]);
+            m = std::max(m, t);
+        }
+        int e = 0;
+        std::frexp(m, &e);
+        return 14 - e;
+    };
+    j->h_exp = axis_exp(hax, (size_t)out.w);
+    j->v_exp = axis_exp(vax, (size_t)out.h);
+    std::vector<double> w;
+    PatternSet vp, hp;
+    std::vector<int32_t> vtile((size_t)ntiles * 3), hstrip((size_t)nstrips * 2);
+    for (int64_t ti = 0; ti < ntiles; ++ti) {
+        const int64_t y0 = 16 * ti, nrow = std::min<int64_t>(16, out.h - y0);
+        const int64_t lo = vrow(vs[(size_t)y0]), hi = vrow(vs[(size_t)(y0 + nrow - 1)] + vax.taps - 1);
+        const int64_t base8 = lo & ~(int64_t)7;
+        w.assign((size_t)16 * 32 * kv, 0.0);
+        for (int64_t m = 0; m < nrow; ++m) {
+            const double* row = vax.row((size_t)(y0 + m));
+            for (int t = 0; t < vax.taps; ++t)
+                w[(size_t)(m * 32 * kv + (vrow(vs[(size_t)(y0 + m)] + t) - base8))] += row[t];
+        }
+        std::vector<unsigned char> bytes;
+        emit_f16_operands(w, kv, j->v_exp, &bytes);
+        vtile[(size_t)(3 * ti)] = vp.add(std::move(bytes));
+        vtile[(size_t)(3 * ti + 1)] = (int32_t)base8;
+        vtile[(size_t)(3 * ti + 2)] = (int32_t)hi;
+    }
+    for (int64_t jx = 0; jx < nstrips; ++jx) {
+        const int64_t s0 = 16 * jx;
+        const int64_t span0 = hcol(hs[(size_t)(s0 / bands)]);
+        w.assign((size_t)16 * 32 * kh, 0.0);
+        for (int n = 0; n < 16 && s0 + n < nsamp; ++n) {
+            const int64_t smp = s0 + n, x = smp / bands, b = smp - x * bands;
+            const double* row = hax.row((size_t)x);
+            for (int t = 0; t < hax.taps; ++t)
+                w[(size_t)(n * 32 * kh + (hcol(hs[(size_t)x] + t) - span0) * bands + b)] += row[t];
+        }
+        std::vector<unsigned char> bytes;
+        emit_f16_operands(w, kh, j->h_exp, &bytes);
+        hstrip[(size_t)(2 * jx)] = hp.add(std::move(bytes));
+        hstrip[(size_t)(2 * jx + 1)] = (int32_t)span0;
+    }
+    stg->v_tab = vp.put(st, &j->vtab_bytes);
+    stg->v_tile = st->put(vtile.data(), vtile.size() * sizeof(int32_t));
+    stg->h_tab = hp.put(st, &j->htab_bytes);
+    stg->h_strip = st->put(hstrip.data(), hstrip.size() * sizeof(int32_t));
+    stg->kind = ST_SEP_F16;
+    return true;
+}
+
 // Fused reduce stage: H plan over the intermediate's columns (`mid`), V plan
 // over the output rows, and a tile shape whose staged rows fit 64 KiB of LDS.
 // `accumulate_h` selects the u8 ReduceH plan; an identity axis stands in for
@@ -336,6 +458,7 @@ bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const Redu
     std::vector<uint8_t> hp, vp;
     build_plan(hax, mid.x, mid.w, accumulate_h, &hs, &hp);
     build_plan(vax, out.y, out.h, false, &vs, &vp);
+    s->kind = ST_REDUCE_HV;  // replaced below by ST_SEP, ST_RED_F32 or ST_SEP_F16
     if (fmt == U8 && opt.integer_mfma) {
         const SepAxis ha{&hs, &hp, hax.wi16.data(), hax.taps}, va{&vs, &vp, vax.wi16.data(), vax.taps};
         plan_sep(0, ha, va, 2048, 2048, bands, out, clip, buf, vclip, opt, st, s);  // ST_SEP on success
@@ -381,6 +504,10 @@ bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const Redu
         s->w32 = put_f32(hax.wf64, st);
         s->v_w32 = put_f32(vax.wf64, st);
         s->kind = ST_RED_F32;
+        if (opt.float_mode == 2) {
+            const F16Axis ha{&hs, &hp, hax.wf64.data(), hax.taps}, va{&vs, &vp, vax.wf64.data(), vax.taps};
+            plan_f16(ha, va, bands, out, clip, buf, vclip, opt, st, s);  // ST_SEP_F16 on success
+        }
     }
     return true;
 }
@@ -418,7 +545,7 @@ bool stage_gauss_sep_params(const Node& gh, const Node& gv, const Rect& mid, con
 // GaussBlurH -> GaussBlurV on f32 images in plain FP32: the float kernel
 // taps / scale (gauss_blur.rs G1) as f32 weights, one phase.
 bool stage_gauss_float_params(const Node& gh, const Node& gv, const Rect& mid, const Rect& out, const Rect& vclip,
-                              const StageOptions& opt, Staging* st, Stage* s) {
+                              const Rect& clip, const Rect& buf, const StageOptions& opt, Staging* st, Stage* s) {
     const int taps = (int)gh.conv.size();
     const int rad = (taps - 1) / 2;
     if (taps < 1 || gv.conv != gh.conv) return false;
@@ -447,6 +574,10 @@ bool stage_gauss_float_params(const Node& gh, const Node& gv, const Rect& mid, c
     j.gx = (int)((out.w + tx - 1) / tx);
     j.gy = (int)((out.h + ty - 1) / ty);
     s->kind = ST_RED_F32;
+    if (opt.float_mode == 2) {
+        const F16Axis ha{&hs, nullptr, gh.conv.data(), taps}, va{&vs, nullptr, gv.conv.data(), taps};
+        plan_f16(ha, va, gh.bands, out, clip, buf, vclip, opt, st, s);  // ST_SEP_F16 on success
+    }
     return true;
 }
 
@@ -582,6 +713,17 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
     case ST_REDUCE_HV: e = launch_reduce_job(reduce_job_of(s, in, out, blob), stream); break;
     case ST_SEP: e = launch_sep_job(sep_job_of(s, in, out, blob), stream); break;
     case ST_RED_F32: e = launch_red_f32_job(reduce_job_of(s, in, out, blob), stream); break;
+    case ST_SEP_F16: {
+        F16Job j = s.f16;
+        j.in = in;
+        j.out = out;
+        j.htab = blob + s.h_tab;
+        j.hstrip = reinterpret_cast<const int32_t*>(blob + s.h_strip);
+        j.vtab = blob + s.v_tab;
+        j.vtile = reinterpret_cast<const int32_t*>(blob + s.v_tile);
+        e = launch_f16_job(j, stream);
+        break;
+    }
     case ST_CONV: {
         const bool gauss = nd.kind == N_GAUSS_H || nd.kind == N_GAUSS_V;
         const bool vert = nd.kind == N_GAUSS_V || nd.kind == N_CONVSEP_V;
@@ -671,7 +813,6 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
             const Rect vclip = vi >= 0 ? in_r[(size_t)vi] : out_r[(size_t)r0];
             if (stage_reduce_hv_params(hax, has_h && rn.in_fmt == U8, vax, rn.in_fmt, rn.bands, mid, out,
                                        vclip, s.clip, buf, opt, st, &s)) {
-                if (s.kind != ST_SEP) s.kind = ST_REDUCE_HV;
                 s.premult = nd.kind == N_PREMULT ? 1 : 0;
                 s.last = last;
                 if (alpha_ok && kind_at(last + 1) == N_UNPREMULT) {
@@ -694,7 +835,7 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
             nd.in_fmt == F32 && nodes[(size_t)i + 1].out_fmt == F32) {
             // GaussBlurH -> GaussBlurV on f32 in a fast float mode.
             if (stage_gauss_float_params(nd, nodes[(size_t)i + 1], out_r[(size_t)i], out_r[(size_t)i + 1],
-                                         in_r[(size_t)i + 1], opt, st, &s)) {
+                                         in_r[(size_t)i + 1], s.clip, buf, opt, st, &s)) {
                 s.last = i + 1;
                 done = true;
             }
@@ -1080,6 +1221,35 @@ int vgpu_pipeline_set_float_mode(vgpu_pipeline* p, int mode) {
     if (hipStreamSynchronize(p->ctx->stream) != hipSuccess) return p->ctx->sticky = VGPU_E_DEVICE;
     p->release();
     p->opt.float_mode = mode;
+    return VGPU_OK;
+}
+
+namespace {
+const char* stage_kernel_name(const Stage& s) {
+    switch (s.kind) {
+    case ST_SEP: return "sep_mfma_i8";
+    case ST_RED_F32: return "reduce_f32";
+    case ST_SEP_F16: return "sep_mfma_f16x3";
+    case ST_SHRINK_HV_U8: return "shrink_hv";
+    case ST_SHRINK: return "shrink";
+    case ST_REDUCE_H: return "reduce_h";
+    case ST_REDUCE_V: return "reduce_v";
+    case ST_REDUCE_HV: return "reduce_hv";
+    case ST_CONV: return "conv";
+    case ST_ALPHA: return "alpha";
+    case ST_COPY: return "copy";
+    }
+    return "unknown";
+}
+}  // namespace
+
+int vgpu_pipeline_stage_kernel(vgpu_pipeline* p, int index, char* name, size_t cap) {
+    if (!p || !name || cap == 0) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(p->ctx->device);
+    const int rc = p->compile();
+    if (rc != VGPU_OK) return rc;
+    if (index < 0 || (size_t)index >= p->stages.size()) return VGPU_E_INVALID_ARGUMENT;
+    std::snprintf(name, cap, "%s", stage_kernel_name(p->stages[(size_t)index]));
     return VGPU_OK;
 }
 

@@ -106,6 +106,7 @@ struct Staging {
 enum StageKind {
     ST_SEP,      // fused u8 H -> V pair on MFMA (reduce, Gaussian or box)
     ST_RED_F32,  // fused f32 H -> V pair in plain FP32 (reduce or Gaussian)
+    ST_SEP_F16,  // fused f32 H -> V pair, FP16 hi/lo split on MFMA
     ST_SHRINK_HV_U8,
     ST_SHRINK,
     ST_REDUCE_H,
@@ -135,6 +136,7 @@ struct Stage {
     // and (family 1) H operand patterns, per-strip (pattern, first pixel).
     size_t v_tab = 0, v_tile = 0, h_tab = 0, h_strip = 0;
     size_t w32 = 0, v_w32 = 0;  // ST_RED_F32: f32 weight tables
+    F16Job f16;                 // ST_SEP_F16: job template (table offsets below)
 };
 
 // Stage-planning choices that depend on how the stage is launched: a single
