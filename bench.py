@@ -1,7 +1,8 @@
 """Benchmark: Lanczos3 thumbnail of an 8192x8192 3-band uchar image to 400 px
 wide on one MI355X (input Mpix/s, % of HBM peak, p99 step latency), plus, as
 line items of the same JSON object, the 4096x4096 4-band uchar
-gauss_blur(sigma=2) and the 512-image uneven-size batched thumbnail.
+gauss_blur(sigma=2), the 512-image uneven-size batched thumbnail and the
+4096x4096x3 float32 reduce / gauss_blur in the three float modes.
 
 One step = one full pipeline run over one image already resident in HBM.
 Prints one JSON line.
@@ -131,6 +132,54 @@ def bench_batch(v, O, ctx, n, steps, warmup, check_images):
     }
 
 
+def float_image(h, w, b, seed):
+    """clip(N(0.5, 0.2), 0, 1) + 0.05 sin(2 pi (a x + c y) + phi), float32."""
+    r = np.random.default_rng(seed)
+    base = np.clip(r.normal(0.5, 0.2, size=(h, w, b)).astype(np.float32), 0.0, 1.0)
+    a, c, phi = r.uniform(0.05, 0.45), r.uniform(0.05, 0.45), r.uniform(0, 2 * np.pi)
+    y = np.arange(h, dtype=np.float32)[:, None, None]
+    x = np.arange(w, dtype=np.float32)[None, :, None]
+    return (base + np.float32(0.05) * np.sin(np.float32(2 * np.pi) * (a * x + c * y) + phi)).astype(np.float32)
+
+
+def bench_float(v, O, ctx, steps, warmup):
+    """4096x4096x3 float32: Lanczos3 reduce by 2.5 on both axes and
+    gauss_blur(sigma=8), each in the three float modes (the reference's FP64
+    order, plain FP32, FP16 hi/lo on MFMA), with max abs / rel error of each
+    against an FP64 evaluation of the same separable math."""
+    H = W = 4096
+    B = 3
+    src = float_image(H, W, B, 31337)
+    d_src = v.DeviceBuffer.from_numpy(ctx, src)
+    cases = [("reduce", "Lanczos3 reduce_h/reduce_v by 2.5 (17 taps) -> 1638x1638x3",
+              lambda p: p.reduce_h(2.5, v.LANCZOS3).reduce_v(2.5, v.LANCZOS3),
+              lambda: O.f64_reduce_hv(src, 2.5, O.LANCZOS3), src.nbytes + 1638 * 1638 * 3 * 4),
+             ("gauss", "gauss_blur sigma=8 (29 taps) -> 4096x4096x3",
+              lambda p: p.gauss_blur(8.0), lambda: O.f64_gauss(src, 8.0), 2 * src.nbytes)]
+    out = {"input": "4096x4096x3 float32, clip(N(0.5,0.2),0,1) + 0.05 sin(2pi(ax+cy)+phi), seeded"}
+    for name, what, build, ref_fn, nbytes in cases:
+        ref = ref_fn()
+        line = {"workload": what, "algorithmic_bytes": nbytes}
+        for mode, label in ((0, "exact_fp64_order"), (1, "fp32"), (2, "fp16x3_mfma")):
+            p = build(v.Pipeline(ctx, W, H, B, v.F32).set_float_mode(mode))
+            d_dst = p.output_buffer()
+            p.run(d_src, d_dst)
+            ctx.synchronize()
+            got = d_dst.numpy().astype(np.float64)
+            err = np.abs(got - ref)
+            ms, _ = time_pipeline(p, d_src, d_dst, steps, warmup)
+            line[label] = {
+                "kernels": p.stage_kernels(), "ms": round(ms, 5),
+                "Mpix/s": round(W * H / 1e6 / (ms / 1e3), 1),
+                "hbm_frac": round(nbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                "max_abs_err_vs_fp64": float(err.max()),
+                "max_rel_err_vs_fp64": float((err / np.maximum(np.abs(ref), 1e-2)).max()),
+            }
+            del p, d_dst, got, err
+        out[name] = line
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +190,7 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--batch-images", type=int, default=512)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-float", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -233,6 +283,10 @@ def main():
     if not args.no_batch and args.batch_images > 0:
         print("bench: uneven-size batch", file=sys.stderr, flush=True)
         result["batch"] = bench_batch(v, O, ctx, args.batch_images, max(args.steps, 20), args.warmup, 8)
+
+    if not args.no_float:
+        print("bench: float modes", file=sys.stderr, flush=True)
+        result["float_modes"] = bench_float(v, O, ctx, args.steps, args.warmup)
 
     ctx.synchronize()
     print(json.dumps(result), flush=True)

@@ -95,11 +95,21 @@ __device__ __forceinline__ void split8(const float (&x)[8], u32x4v& hi, u32x4v& 
     }
 }
 
-// Exponent s with max|x| * 2^s < 2^14 over the whole wavefront (wave-uniform).
+// Exponent s with max|x| * 2^s < 2^14 over the whole wavefront (wave-uniform),
+// from m = this lane's max |x|.  Non-negative floats order like their bit
+// patterns: max within each 16-lane row by DPP (xor 1, xor 2, half-row mirror,
+// row mirror), then across the four rows with readlane.
 __device__ __forceinline__ int wave_scale_exp(float m) {
-#pragma unroll
-    for (int k = 32; k >= 1; k >>= 1) m = fmaxf(m, __shfl_xor(m, k));
-    return 14 - __builtin_amdgcn_readfirstlane(__builtin_amdgcn_frexp_expf(m));
+    int v = __float_as_int(m);
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false));  // row_mirror
+    const int r = max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+                      max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+    // frexp exponent of the max: biased exponent - 126 (zero and subnormal
+    // maxima give a large scale, harmless for such values).
+    return 14 - (((r >> 23) & 0xff) - 126);
 }
 
 // 8 consecutive floats of the input buffer from element `e` (bytes past the
@@ -121,6 +131,14 @@ __device__ __forceinline__ void load8f(const float* base, uint32_t e, uint32_t t
     }
 }
 
+// One wavefront: a 16-sample strip `jx` over rows_per_task output rows.
+// H: A = input pixels [16 rows][K = span samples], B = folded weights
+// [K][16 samples]; D rows go to a per-wave LDS ring of f32 intermediate rows.
+// V: A = folded V weights [16 output rows][K = ring rows from base8], B = the
+// ring.  Each product x * w is x_hi w_hi + x_hi w_lo + x_lo w_hi; the hi * hi
+// MFMA of each K step starts from zero and is added to the sum on the VALU
+// (round to nearest), and the two correction products share their own
+// accumulator, so no large partial sum goes through the MFMA accumulator.
 template <int B>
 __device__ void f16_task(const F16Job& job, int jx, int ty, unsigned char* lds, size_t wave_lds) {
     const int lane = threadIdx.x & 63;
@@ -174,7 +192,7 @@ __device__ void f16_task(const F16Job& job, int jx, int ty, unsigned char* lds, 
                 }
             }
             const int sx = wave_scale_exp(mx);
-            f32x4 d = {0.f, 0.f, 0.f, 0.f};
+            f32x4 d = {0.f, 0.f, 0.f, 0.f}, dc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int s = 0; s < kF16MaxH; ++s) {
                 if (s < KH) {
@@ -182,11 +200,12 @@ __device__ void f16_task(const F16Job& job, int jx, int ty, unsigned char* lds, 
                     for (int e = 0; e < 8; ++e) x[s][e] = __builtin_amdgcn_ldexpf(x[s][e], sx);
                     u32x4v ah, al;
                     split8(x[s], ah, al);
-                    d = mfma_f16(ah, whi[s], d);
-                    d = mfma_f16(ah, wlo[s], d);
-                    d = mfma_f16(al, whi[s], d);
+                    d += mfma_f16(ah, whi[s], f32x4{0.f, 0.f, 0.f, 0.f});
+                    dc = mfma_f16(ah, wlo[s], dc);
+                    dc = mfma_f16(al, whi[s], dc);
                 }
             }
+            d += dc;
             const int ux = -(sx + job.h_exp);
 #pragma unroll
             for (int e = 0; e < 4; ++e) d[e] = __builtin_amdgcn_ldexpf(d[e], ux);
@@ -194,30 +213,32 @@ __device__ void f16_task(const F16Job& job, int jx, int ty, unsigned char* lds, 
             *reinterpret_cast<f32x4*>(ring + n * RS + ((r0 + 4 * q) & (RING - 1))) = d;
         }
         have = max(have, r0);
+        // Ring rows of the V window from `have` on are not computed yet and
+        // carry zero weight: clear them so that stale LDS (possibly not a
+        // finite value) reaches neither the sum nor the scale.
+        for (int r = have; r < base8 + 32 * KV; r += 16)
+            *reinterpret_cast<f32x4*>(ring + n * RS + ((r + 4 * q) & (RING - 1))) = f32x4{0.f, 0.f, 0.f, 0.f};
         // ---- V: A = folded V weights (rows j = output rows, K = rows from base8).
-        // Ring rows past `hi` carry zero weight but may hold stale values:
-        // they read as 0 so that they neither reach the sum nor the scale.
         const unsigned char* vt = job.vtab + (size_t)vpat * job.vtab_bytes;
         float yv[kF16MaxV][8];
         float my = 0.f;
 #pragma unroll
         for (int s = 0; s < kF16MaxV; ++s) {
             if (s < KV) {
-                const int rk = base8 + 32 * s + 8 * q;
-                const float* rp = ring + n * RS + (rk & (RING - 1));
+                const float* rp = ring + n * RS + ((base8 + 32 * s + 8 * q) & (RING - 1));
                 const f32x4 r0v = *reinterpret_cast<const f32x4*>(rp);
                 const f32x4 r1v = *reinterpret_cast<const f32x4*>(rp + 4);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    yv[s][e] = rk + e <= hi ? r0v[e] : 0.f;
-                    yv[s][4 + e] = rk + 4 + e <= hi ? r1v[e] : 0.f;
+                    yv[s][e] = r0v[e];
+                    yv[s][4 + e] = r1v[e];
                 }
 #pragma unroll
                 for (int e = 0; e < 8; ++e) my = fmaxf(my, fabsf(yv[s][e]));
             }
         }
         const int sy = wave_scale_exp(my);
-        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+        f32x4 d = {0.f, 0.f, 0.f, 0.f}, dc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int s = 0; s < kF16MaxV; ++s) {
             if (s < KV) {
@@ -228,11 +249,12 @@ __device__ void f16_task(const F16Job& job, int jx, int ty, unsigned char* lds, 
                 u32x4v bh, bl;
                 split8(yv[s], bh, bl);
                 const u32x4v vh = u32x4v{a[0], a[1], a[2], a[3]}, vl = u32x4v{b[0], b[1], b[2], b[3]};
-                d = mfma_f16(vh, bh, d);
-                d = mfma_f16(vl, bh, d);
-                d = mfma_f16(vh, bl, d);
+                d += mfma_f16(vh, bh, f32x4{0.f, 0.f, 0.f, 0.f});
+                dc = mfma_f16(vl, bh, dc);
+                dc = mfma_f16(vh, bl, dc);
             }
         }
+        d += dc;
         const int uy = -(sy + job.v_exp);
 #pragma unroll
         for (int e = 0; e < 4; ++e) d[e] = __builtin_amdgcn_ldexpf(d[e], uy);
