@@ -2,7 +2,8 @@
 wide on one MI355X (input Mpix/s, % of HBM peak, p99 step latency), plus, as
 line items of the same JSON object, the 4096x4096 4-band uchar
 gauss_blur(sigma=2), the 512-image uneven-size batched thumbnail and the
-4096x4096x3 float32 reduce / gauss_blur in the three float modes.
+4096x4096x3 float32 reduce / gauss_blur in the three float modes and the
+open-loop thumbnail request trace at 500 / 2000 / 5000 requests/s.
 
 One step = one full pipeline run over one image already resident in HBM.
 Prints one JSON line.
@@ -180,6 +181,66 @@ def bench_float(v, O, ctx, steps, warmup):
     return out
 
 
+def bench_serving(v, ctx, rates, seconds, pool_images):
+    """Open-loop request trace per arrival rate: Poisson arrivals (seeded
+    exponential gaps), P ~ logU[0.1, 24] MP, aspect ~ U[0.5, 2], 3-band
+    uchar, Width(256 | 400 | 800) Lanczos3 then gauss_blur(0.8).  Sources are
+    `pool_images` pre-resident device images of such sizes (seeded pixels made
+    in HBM); requests draw from them, so host-to-device copies are not in the
+    measured path.  Latency = completion seen by the request loop - scheduled
+    arrival."""
+    r = np.random.default_rng(5005)
+    pool = []
+    for i in range(pool_images):
+        P = np.exp(r.uniform(np.log(0.1e6), np.log(24e6)))
+        a = r.uniform(0.5, 2.0)
+        w, h = int(round(np.sqrt(P * a))), int(round(np.sqrt(P / a)))
+        d = v.DeviceBuffer(ctx, (h, w, 3), np.uint8)
+        d.fill_random(9_000_000 + i)
+        pool.append((w, h, d))
+    ctx.synchronize()
+    srv = v.Server(ctx, hbm_budget=8 << 30, max_inflight=64, streams=4, planner_threads=8)
+
+    def trace(lam, n, seed):
+        g = np.random.default_rng(seed)
+        arrivals = (np.cumsum(g.exponential(1.0 / lam, size=n)) * 1e9).astype(np.int64)
+        reqs, pix, nbytes = [], 0, 0
+        for k in range(n):
+            w, h, d = pool[int(g.integers(0, len(pool)))]
+            tw = (256, 400, 800)[int(g.integers(0, 3))]
+            reqs.append(v.Server.request(d, w, h, 3, target=v.WIDTH, target_w=tw, blur_sigma=0.8))
+            pix += w * h
+            nbytes += w * h * 3
+        return reqs, arrivals, pix, nbytes
+
+    out = {"workload": "Poisson arrivals; P ~ logU[0.1,24] MP, aspect U[0.5,2], 3-band uchar; "
+                       "Width(256|400|800) Lanczos3 then gauss_blur(0.8)",
+           "sources": f"{pool_images} pre-resident device images (no host-to-device copy in the path)",
+           "server": {"hbm_budget_bytes": 8 << 30, "max_inflight": 64, "streams": 4, "planner_threads": 8},
+           "rates": []}
+    reqs, arr, _, _ = trace(2000.0, 300, 1)
+    srv.replay(reqs, arr)  # warm: pinned staging, events, device clocks
+    for lam in rates:
+        n = int(min(max(lam * seconds, 400), 20000))
+        reqs, arr, pix, nbytes = trace(float(lam), n, int(lam))
+        lat, status, wall = srv.replay(reqs, arr)
+        ok = lat[status == 0] / 1e6
+        out["rates"].append({
+            "offered_req_per_s": lam, "requests": n, "failed": int((status != 0).sum()),
+            "achieved_req_per_s": round(n / (wall / 1e9), 1),
+            "input_Mpix_per_s": round(pix / 1e6 / (wall / 1e9), 1),
+            "input_GB_per_s": round(nbytes / 1e9 / (wall / 1e9), 2),
+            "latency_ms": {"p50": round(float(np.percentile(ok, 50)), 4),
+                           "p99": round(float(np.percentile(ok, 99)), 4),
+                           "mean": round(float(ok.mean()), 4), "max": round(float(ok.max()), 4)},
+        })
+    st = srv.stats()
+    out["peak_inflight"] = st["peak_inflight"]
+    out["arena_peak_bytes"] = st["arena_peak"]
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +252,8 @@ def main():
     ap.add_argument("--batch-images", type=int, default=512)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-float", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=2.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -287,6 +350,10 @@ def main():
     if not args.no_float:
         print("bench: float modes", file=sys.stderr, flush=True)
         result["float_modes"] = bench_float(v, O, ctx, args.steps, args.warmup)
+
+    if not args.no_serving:
+        print("bench: serving trace", file=sys.stderr, flush=True)
+        result["serving"] = bench_serving(v, ctx, (500, 2000, 5000), args.serving_seconds, 96)
 
     ctx.synchronize()
     print(json.dumps(result), flush=True)

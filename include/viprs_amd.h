@@ -64,7 +64,8 @@ typedef enum {
     VGPU_E_DEVICE = -6,       /* HIP runtime error (sticky on the context) */
     VGPU_E_NO_MEMORY = -7,
     VGPU_E_CANCELLED = -8,
-    VGPU_E_ADMISSION = -9     /* request exceeds the device memory budget */
+    VGPU_E_ADMISSION = -9,    /* request exceeds the device memory budget */
+    VGPU_E_NOT_READY = -10    /* a wait timed out before the request finished */
 } vgpu_status;
 
 /* Region (crates/viprs-core/src/image/region.rs:18-27). */
@@ -303,6 +304,90 @@ int vgpu_batch_benchmark(vgpu_batch *b, const void *const *srcs, const size_t *s
  * and keeps the fastest; best_ms (may be NULL) receives its ms per run. */
 int vgpu_batch_tune(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,
                     void *const *dsts, const size_t *dst_bytes, int iterations, float *best_ms);
+
+/* ------------------------------------------------------------------------
+ * Request loop (serving).  The reference runs concurrent pipelines under a
+ * cap (ExecutionSemaphore, crates/viprs-runtime/src/limits.rs:290-360;
+ * RayonScheduler::with_max_concurrent_pipelines) and rejects requests over
+ * its resource limits (ResourceLimits::validate_output, limits.rs:238-275)
+ * with a typed error.  Here a server admits requests by device-memory
+ * footprint into a preallocated arena of `hbm_budget` bytes and a number of
+ * slots, as soon as both free up (continuous batching: requests that arrive
+ * while others run are launched next to them on several HIP streams), and
+ * reports per-request latency.  A request whose footprint exceeds the whole
+ * budget finishes with VGPU_E_ADMISSION. */
+typedef struct vgpu_server vgpu_server;
+
+typedef struct {
+    uint64_t hbm_budget;      /* arena bytes: input copies, tables, intermediates, outputs */
+    uint32_t max_inflight;    /* requests on the device at once */
+    uint32_t streams;         /* HIP streams requests are spread over (1..16) */
+    uint32_t planner_threads; /* host threads building plans (1..64) */
+} vgpu_server_config;
+
+/* One request: thumbnail(target, target_w, target_h, kernel) when
+ * thumbnail = 1 (arguments as vgpu_pipeline_thumbnail), then
+ * gauss_blur(blur_sigma) when blur_sigma > 0, of an image of
+ * width x height x bands samples of `format`.  src is a device pointer
+ * (src_on_device = 1), pinned host memory (src_pinned = 1, e.g. from
+ * vgpu_host_alloc) or pageable host memory (copied at submit).  The caller
+ * keeps device / pinned sources valid until the request finishes.
+ * arrival_ns = 0 means "now" (steady clock); latency counts from it. */
+typedef struct {
+    const void *src;
+    uint32_t width, height, bands;
+    int format;
+    int src_on_device, src_pinned;
+    int thumbnail;
+    int target;
+    uint32_t target_w, target_h;
+    int kernel;
+    float blur_sigma;
+    int64_t arrival_ns;
+} vgpu_request;
+
+typedef struct {
+    int status;                   /* VGPU_OK or the request's error */
+    uint32_t width, height, bands; /* output geometry */
+    int format;
+    uint64_t out_bytes;
+    uint64_t arena_bytes;         /* footprint held from admission to release */
+    int64_t queue_ns;             /* arrival -> admission */
+    int64_t latency_ns;           /* arrival -> completion seen by the scheduler */
+} vgpu_request_result;
+
+typedef struct {
+    uint64_t completed, rejected, inflight, peak_inflight, queued;
+    uint64_t arena_bytes, arena_used, arena_peak;
+} vgpu_server_counters;
+
+int vgpu_server_create(vgpu_context *ctx, const vgpu_server_config *cfg, vgpu_server **out);
+/* Waits for every submitted request, then frees the arena and streams. */
+void vgpu_server_destroy(vgpu_server *s);
+/* Queue a request; *id names it.  auto_release = 1 frees its memory as soon
+ * as it finishes (its output cannot be read then). */
+int vgpu_server_submit(vgpu_server *s, const vgpu_request *r, int auto_release, uint64_t *id);
+/* Block until request `id` finishes (timeout_us < 0: no limit); returns
+ * VGPU_E_NOT_READY on timeout. */
+int vgpu_server_wait(vgpu_server *s, uint64_t id, int64_t timeout_us, vgpu_request_result *res);
+/* Device pointer and length of a finished request's output (valid until release). */
+int vgpu_server_output(vgpu_server *s, uint64_t id, const void **dev_ptr, size_t *bytes);
+/* Copy a finished request's output to host memory (synchronous). */
+int vgpu_server_fetch(vgpu_server *s, uint64_t id, void *host_dst, size_t bytes);
+/* Free a finished request's memory and record. */
+int vgpu_server_release(vgpu_server *s, uint64_t id);
+/* Counters; returns the server's sticky status. */
+int vgpu_server_stats(vgpu_server *s, vgpu_server_counters *st);
+/* Open-loop load: request i is submitted at start + arrival_offset_ns[i] and
+ * released when done; latency_ns[i] and status[i] are filled in, wall_ns
+ * (may be NULL) receives start -> last completion.  No reference counterpart
+ * (measurement harness). */
+int vgpu_server_replay(vgpu_server *s, const vgpu_request *reqs, uint32_t n,
+                       const int64_t *arrival_offset_ns, int64_t *latency_ns, int32_t *status,
+                       int64_t *wall_ns);
+/* Pinned host memory for request sources (hipHostMalloc). */
+int vgpu_host_alloc(size_t bytes, void **out);
+void vgpu_host_free(void *p);
 
 #ifdef __cplusplus
 }

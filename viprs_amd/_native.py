@@ -38,6 +38,44 @@ class BatchStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerConfig(C.Structure):
+    """vgpu_server_config."""
+
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_inflight", C.c_uint32), ("streams", C.c_uint32),
+                ("planner_threads", C.c_uint32)]
+
+
+class Request(C.Structure):
+    """vgpu_request."""
+
+    _fields_ = [("src", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("bands", C.c_uint32),
+                ("format", C.c_int), ("src_on_device", C.c_int), ("src_pinned", C.c_int),
+                ("thumbnail", C.c_int), ("target", C.c_int), ("target_w", C.c_uint32),
+                ("target_h", C.c_uint32), ("kernel", C.c_int), ("blur_sigma", C.c_float),
+                ("arrival_ns", C.c_int64)]
+
+
+class RequestResult(C.Structure):
+    """vgpu_request_result."""
+
+    _fields_ = [("status", C.c_int), ("width", C.c_uint32), ("height", C.c_uint32), ("bands", C.c_uint32),
+                ("format", C.c_int), ("out_bytes", C.c_uint64), ("arena_bytes", C.c_uint64),
+                ("queue_ns", C.c_int64), ("latency_ns", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerCounters(C.Structure):
+    """vgpu_server_counters."""
+
+    _fields_ = [(k, C.c_uint64) for k in ("completed", "rejected", "inflight", "peak_inflight", "queued",
+                                           "arena_bytes", "arena_used", "arena_peak")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -118,6 +156,18 @@ def load() -> C.CDLL:
         "vgpu_batch_run_profiled": (i32, [vp, P(vp), P(sz), P(vp), P(sz), i32, P(f32), P(C.c_uint64), P(i32), i32]),
         "vgpu_batch_benchmark": (i32, [vp, P(vp), P(sz), P(vp), P(sz), i32, i32, P(f32), P(f32)]),
         "vgpu_batch_tune": (i32, [vp, P(vp), P(sz), P(vp), P(sz), i32, P(f32)]),
+        "vgpu_server_create": (i32, [vp, P(ServerConfig), P(vp)]),
+        "vgpu_server_destroy": (None, [vp]),
+        "vgpu_server_submit": (i32, [vp, P(Request), i32, P(C.c_uint64)]),
+        "vgpu_server_wait": (i32, [vp, C.c_uint64, C.c_int64, P(RequestResult)]),
+        "vgpu_server_output": (i32, [vp, C.c_uint64, P(vp), P(sz)]),
+        "vgpu_server_fetch": (i32, [vp, C.c_uint64, vp, sz]),
+        "vgpu_server_release": (i32, [vp, C.c_uint64]),
+        "vgpu_server_stats": (i32, [vp, P(ServerCounters)]),
+        "vgpu_server_replay": (i32, [vp, P(Request), u32, P(C.c_int64), P(C.c_int64), P(C.c_int32),
+                                     P(C.c_int64)]),
+        "vgpu_host_alloc": (i32, [sz, P(vp)]),
+        "vgpu_host_free": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -33,6 +33,9 @@ E_UNSUPPORTED = -4
 E_BUFFER_SIZE = -5
 E_DEVICE = -6
 E_NO_MEMORY = -7
+E_CANCELLED = -8
+E_ADMISSION = -9
+E_NOT_READY = -10
 
 NP_DTYPE = {U8: np.uint8, U16: np.uint16, F32: np.float32}
 SAMPLE_BYTES = {U8: 1, U16: 2, F32: 4}
@@ -488,3 +491,93 @@ class Op:
         _check(self._lib.vgpu_op_process_region(self._h, C.c_void_p(_ptr(src)), _nbytes(src),
                                                 C.c_void_p(_ptr(dst)), _nbytes(dst),
                                                 input_region, output_region), "process_region")
+
+
+class Server:
+    """Thumbnail request loop: requests are admitted by device-memory footprint
+    into an arena of ``hbm_budget`` bytes and ``max_inflight`` slots as soon as
+    both free up, and run next to each other on ``streams`` HIP streams."""
+
+    def __init__(self, ctx: Context, hbm_budget: int, max_inflight: int = 64, streams: int = 4,
+                 planner_threads: int = 4):
+        self._lib = _native.load()
+        self.ctx = ctx
+        self._h = None
+        cfg = _native.ServerConfig(int(hbm_budget), int(max_inflight), int(streams), int(planner_threads))
+        h = C.c_void_p()
+        _check(self._lib.vgpu_server_create(ctx.handle, C.byref(cfg), C.byref(h)), "vgpu_server_create")
+        self._h = h
+        self._keep = {}  # request id -> source kept alive until the request finishes
+        ctx._adopt(self)
+
+    @staticmethod
+    def request(src, width: int, height: int, bands: int, fmt: int = U8, *, thumbnail: bool = True,
+                target: int = WIDTH, target_w: int = 0, target_h: int = 0, kernel: int = LANCZOS3,
+                blur_sigma: float = 0.0, on_device: Optional[bool] = None, pinned: bool = False,
+                arrival_ns: int = 0) -> "_native.Request":
+        """A vgpu_request.  `src` is a DeviceBuffer, a numpy array (pageable
+        host memory, copied at submit) or an integer address with `on_device`
+        / `pinned` saying what it is."""
+        if isinstance(src, DeviceBuffer):
+            addr, dev = src.data_ptr(), 1
+        elif isinstance(src, np.ndarray):
+            addr, dev = src.ctypes.data, 0
+        else:
+            addr, dev = int(src), int(bool(on_device))
+        return _native.Request(addr, width, height, bands, fmt, dev, int(pinned), int(thumbnail), target,
+                               target_w, target_h, kernel, float(blur_sigma), int(arrival_ns))
+
+    def submit(self, req: "_native.Request", keep=None, auto_release: bool = False) -> int:
+        rid = C.c_uint64()
+        _check(self._lib.vgpu_server_submit(self._h, C.byref(req), int(auto_release), C.byref(rid)),
+               "vgpu_server_submit")
+        if keep is not None:
+            self._keep[rid.value] = keep
+        return rid.value
+
+    def wait(self, rid: int, timeout_us: int = -1) -> dict:
+        res = _native.RequestResult()
+        _check(self._lib.vgpu_server_wait(self._h, rid, timeout_us, C.byref(res)), "vgpu_server_wait")
+        self._keep.pop(rid, None)
+        return res.as_dict()
+
+    def fetch(self, rid: int) -> np.ndarray:
+        r = self.wait(rid)
+        _check(r["status"], "request")
+        out = np.empty((r["height"], r["width"], r["bands"]), NP_DTYPE[r["format"]])
+        _check(self._lib.vgpu_server_fetch(self._h, rid, C.c_void_p(out.ctypes.data), out.nbytes),
+               "vgpu_server_fetch")
+        return out
+
+    def release(self, rid: int) -> None:
+        _check(self._lib.vgpu_server_release(self._h, rid), "vgpu_server_release")
+
+    def stats(self) -> dict:
+        st = _native.ServerCounters()
+        _check(self._lib.vgpu_server_stats(self._h, C.byref(st)), "vgpu_server_stats")
+        return st.as_dict()
+
+    def replay(self, reqs, arrival_offsets_ns):
+        """Open-loop trace: request i submitted at start + arrival_offsets_ns[i];
+        returns (latency_ns array, status array, wall_ns)."""
+        n = len(reqs)
+        arr = (_native.Request * max(n, 1))(*reqs)
+        offs = (C.c_int64 * max(n, 1))(*[int(x) for x in arrival_offsets_ns])
+        lat = (C.c_int64 * max(n, 1))()
+        sts = (C.c_int32 * max(n, 1))()
+        wall = C.c_int64()
+        _check(self._lib.vgpu_server_replay(self._h, arr, n, offs, lat, sts, C.byref(wall)),
+               "vgpu_server_replay")
+        return np.array(lat[:n], np.int64), np.array(sts[:n], np.int32), int(wall.value)
+
+    def _release(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.vgpu_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def close(self) -> None:
+        self._release()
+
+    def __del__(self):
+        self._release()

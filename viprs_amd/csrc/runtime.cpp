@@ -928,6 +928,7 @@ const char* vgpu_status_string(int s) {
     case VGPU_E_NO_MEMORY: return "out of device memory";
     case VGPU_E_CANCELLED: return "cancelled";
     case VGPU_E_ADMISSION: return "request exceeds the device memory budget";
+    case VGPU_E_NOT_READY: return "request not finished";
     default: return "unknown status";
     }
 }
