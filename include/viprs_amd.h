@@ -352,7 +352,9 @@ typedef struct {
     int format;
     uint64_t out_bytes;
     uint64_t arena_bytes;         /* footprint held from admission to release */
-    int64_t queue_ns;             /* arrival -> admission */
+    int64_t plan_ns;              /* arrival -> plan built */
+    int64_t queue_ns;             /* arrival -> admission (launch) */
+    int64_t run_ns;               /* admission -> completion */
     int64_t latency_ns;           /* arrival -> completion seen by the scheduler */
 } vgpu_request_result;
 
@@ -379,12 +381,13 @@ int vgpu_server_release(vgpu_server *s, uint64_t id);
 /* Counters; returns the server's sticky status. */
 int vgpu_server_stats(vgpu_server *s, vgpu_server_counters *st);
 /* Open-loop load: request i is submitted at start + arrival_offset_ns[i] and
- * released when done; latency_ns[i] and status[i] are filled in, wall_ns
- * (may be NULL) receives start -> last completion.  No reference counterpart
- * (measurement harness). */
+ * released when done; latency_ns[i] and status[i] are filled in; wall_ns,
+ * plan_ns[i] and queue_ns[i] (each may be NULL) receive start -> last
+ * completion and the request's arrival -> plan built / -> launch.  No
+ * reference counterpart (measurement harness). */
 int vgpu_server_replay(vgpu_server *s, const vgpu_request *reqs, uint32_t n,
                        const int64_t *arrival_offset_ns, int64_t *latency_ns, int32_t *status,
-                       int64_t *wall_ns);
+                       int64_t *wall_ns, int64_t *plan_ns, int64_t *queue_ns);
 /* Pinned host memory for request sources (hipHostMalloc). */
 int vgpu_host_alloc(size_t bytes, void **out);
 void vgpu_host_free(void *p);

@@ -60,7 +60,8 @@ class RequestResult(C.Structure):
 
     _fields_ = [("status", C.c_int), ("width", C.c_uint32), ("height", C.c_uint32), ("bands", C.c_uint32),
                 ("format", C.c_int), ("out_bytes", C.c_uint64), ("arena_bytes", C.c_uint64),
-                ("queue_ns", C.c_int64), ("latency_ns", C.c_int64)]
+                ("plan_ns", C.c_int64), ("queue_ns", C.c_int64), ("run_ns", C.c_int64),
+                ("latency_ns", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -165,7 +166,7 @@ def load() -> C.CDLL:
         "vgpu_server_release": (i32, [vp, C.c_uint64]),
         "vgpu_server_stats": (i32, [vp, P(ServerCounters)]),
         "vgpu_server_replay": (i32, [vp, P(Request), u32, P(C.c_int64), P(C.c_int64), P(C.c_int32),
-                                     P(C.c_int64)]),
+                                     P(C.c_int64), P(C.c_int64), P(C.c_int64)]),
         "vgpu_host_alloc": (i32, [sz, P(vp)]),
         "vgpu_host_free": (None, [vp]),
     }

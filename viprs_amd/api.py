@@ -557,18 +557,24 @@ class Server:
         _check(self._lib.vgpu_server_stats(self._h, C.byref(st)), "vgpu_server_stats")
         return st.as_dict()
 
-    def replay(self, reqs, arrival_offsets_ns):
+    def replay(self, reqs, arrival_offsets_ns, breakdown: bool = False):
         """Open-loop trace: request i submitted at start + arrival_offsets_ns[i];
-        returns (latency_ns array, status array, wall_ns)."""
+        returns (latency_ns, status, wall_ns) and, with breakdown, also the
+        arrival -> plan built and arrival -> launch times (ns)."""
         n = len(reqs)
         arr = (_native.Request * max(n, 1))(*reqs)
         offs = (C.c_int64 * max(n, 1))(*[int(x) for x in arrival_offsets_ns])
         lat = (C.c_int64 * max(n, 1))()
         sts = (C.c_int32 * max(n, 1))()
+        pln = (C.c_int64 * max(n, 1))()
+        que = (C.c_int64 * max(n, 1))()
         wall = C.c_int64()
-        _check(self._lib.vgpu_server_replay(self._h, arr, n, offs, lat, sts, C.byref(wall)),
+        _check(self._lib.vgpu_server_replay(self._h, arr, n, offs, lat, sts, C.byref(wall), pln, que),
                "vgpu_server_replay")
-        return np.array(lat[:n], np.int64), np.array(sts[:n], np.int32), int(wall.value)
+        out = (np.array(lat[:n], np.int64), np.array(sts[:n], np.int32), int(wall.value))
+        if breakdown:
+            out += (np.array(pln[:n], np.int64), np.array(que[:n], np.int64))
+        return out
 
     def _release(self) -> None:
         if getattr(self, "_h", None):

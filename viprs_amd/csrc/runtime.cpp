@@ -17,6 +17,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <unordered_map>
 #include <vector>
 
 #include "runtime.hpp"
@@ -106,52 +107,94 @@ struct SepAxis {
 // Lay out folded weights W[j][k] (16 x ksteps*64) as MFMA operand bytes: lane
 // j + 16 q holds K = 64 s + 16 q .. + 15 of step s; one int8 limb, or the
 // {hi, lo} limbs of w = 128 a + c (c in [-64, 63]); then 16 int32 seeds
-// 128 * sum_k W[j][k] + bias.
-void emit_operands(const std::vector<int32_t>& w, int ksteps, int limbs, int32_t bias,
-                   std::vector<unsigned char>* out) {
+// 128 * sum_k W[j][k] + bias.  Only row j's window lo[j] .. hi[j] can be
+// nonzero; the rest of the operand bytes stay zero.
+void emit_operands(const std::vector<int32_t>& w, int ksteps, int limbs, int32_t bias, const int64_t (&lo)[16],
+                   const int64_t (&hi)[16], std::vector<unsigned char>* out) {
     const size_t ops = (size_t)ksteps * limbs * 64 * 16;
     out->assign(ops + 16 * sizeof(int32_t), 0);
+    unsigned char* o = out->data();
     const int kw = 64 * ksteps;
-    for (int s = 0; s < ksteps; ++s)
-        for (int lane = 0; lane < 64; ++lane)
-            for (int e = 0; e < 16; ++e) {
-                const int j = lane & 15, q = lane >> 4;
-                const int32_t v = w[(size_t)(j * kw + 64 * s + 16 * q + e)];
-                if (limbs == 1) {
-                    (*out)[((size_t)s * 64 + lane) * 16 + e] = (unsigned char)(v & 255);
-                } else {
-                    const int32_t c = ((v + 64) & 127) - 64;
-                    const int32_t a = (v - c) >> 7;
-                    (*out)[((size_t)(2 * s) * 64 + lane) * 16 + e] = (unsigned char)(a & 255);
-                    (*out)[((size_t)(2 * s + 1) * 64 + lane) * 16 + e] = (unsigned char)(c & 255);
-                }
-            }
     int32_t seed[16];
     for (int j = 0; j < 16; ++j) {
         int64_t sum = 0;
-        for (int k = 0; k < kw; ++k) sum += w[(size_t)(j * kw + k)];
+        for (int64_t k = lo[j]; k <= hi[j]; ++k) {
+            const int32_t v = w[(size_t)(j * kw + k)];
+            sum += v;
+            const int s = (int)(k >> 6), lane = j + 16 * (int)((k >> 4) & 3), e = (int)(k & 15);
+            if (limbs == 1) {
+                o[((size_t)s * 64 + lane) * 16 + e] = (unsigned char)(v & 255);
+            } else {
+                const int32_t c = ((v + 64) & 127) - 64;
+                const int32_t a = (v - c) >> 7;
+                o[((size_t)(2 * s) * 64 + lane) * 16 + e] = (unsigned char)(a & 255);
+                o[((size_t)(2 * s + 1) * 64 + lane) * 16 + e] = (unsigned char)(c & 255);
+            }
+        }
         seed[j] = (int32_t)(128 * sum + bias);
     }
-    std::memcpy(out->data() + ops, seed, sizeof(seed));
+    std::memcpy(o + ops, seed, sizeof(seed));
 }
 
-// Identical operand patterns are stored once.
+// Zero the touched K windows of a folded-weight buffer (rows of kw entries).
+template <typename T>
+void clear_rows(std::vector<T>* w, int kw, const int64_t (&lo)[16], const int64_t (&hi)[16]) {
+    for (int n = 0; n < 16; ++n)
+        for (int64_t k = lo[n]; k <= hi[n]; ++k) (*w)[(size_t)(n * kw + k)] = T(0);
+}
+
+// Identical operand patterns are stored once.  A pattern is named by a key
+// that determines its folded weights: for each of its 16 rows (or samples),
+// the unclamped position of tap 0, the clamped positions of the first and the
+// last tap (clamping is monotone, so these fix where every tap lands) and the
+// weight row, all relative to the window start.  Only a new key folds and lays
+// out its operand bytes.
+struct PatternKey {
+    int64_t k[16 * 4];
+    void empty(int n) {
+        k[4 * n] = INT64_MIN;
+        k[4 * n + 1] = 0;
+        k[4 * n + 2] = -1;
+        k[4 * n + 3] = -1;
+    }
+    void set(int n, int64_t start, int64_t lo, int64_t hi, int64_t phase) {
+        k[4 * n] = start;
+        k[4 * n + 1] = lo;
+        k[4 * n + 2] = hi;
+        k[4 * n + 3] = phase;
+    }
+};
+
 struct PatternSet {
     std::vector<std::vector<unsigned char>> pats;
-    std::map<std::vector<unsigned char>, int> index;
-    int add(std::vector<unsigned char>&& p) {
-        auto it = index.find(p);
-        if (it != index.end()) return it->second;
+    std::vector<PatternKey> keys;
+    std::unordered_map<uint64_t, std::vector<int>> by_hash;
+    template <typename Emit>
+    int add(const PatternKey& key, Emit emit) {
+        uint64_t h = 1469598103934665603ull;
+        for (int64_t v : key.k) {
+            h = (h ^ (uint64_t)v) * 1099511628211ull;
+            h ^= h >> 29;
+        }
+        std::vector<int>& ids = by_hash[h];
+        for (int id : ids)
+            if (std::memcmp(keys[(size_t)id].k, key.k, sizeof(key.k)) == 0) return id;
         const int id = (int)pats.size();
-        index.emplace(p, id);
-        pats.push_back(std::move(p));
+        pats.emplace_back();
+        emit(&pats.back());
+        keys.push_back(key);
+        ids.push_back(id);
         return id;
     }
     size_t put(Staging* st, int* bytes) const {
         *bytes = pats.empty() ? 0 : (int)pats[0].size();
-        std::vector<unsigned char> all;
-        for (const auto& p : pats) all.insert(all.end(), p.begin(), p.end());
-        return st->put(all.data(), all.size());
+        const size_t off = st->put(nullptr, pats.size() * (size_t)*bytes);
+        unsigned char* dst = st->bytes.data() + off;
+        for (const auto& p : pats) {
+            std::memcpy(dst, p.data(), p.size());
+            dst += p.size();
+        }
+        return off;
     }
 };
 
@@ -230,15 +273,31 @@ bool plan_sep(int family, const SepAxis& hax, const SepAxis& vax, int32_t hbias,
         const int64_t y0 = 16 * ti, nrow = std::min<int64_t>(16, out.h - y0);
         const int64_t lo = vrow(vs[(size_t)y0]), hi = vrow(vs[(size_t)(y0 + nrow - 1)] + vax.taps - 1);
         const int64_t base16 = lo & ~(int64_t)15;
-        w.assign((size_t)16 * 64 * kv, 0);
-        for (int64_t m = 0; m < nrow; ++m) {
-            const int16_t* row = vax.row((size_t)(y0 + m));
-            for (int t = 0; t < vax.taps; ++t)
-                w[(size_t)(m * 64 * kv + (vrow(vs[(size_t)(y0 + m)] + t) - base16))] += row[t];
+        if (ti == 0) w.assign((size_t)16 * 64 * kv, 0);
+        PatternKey key;
+        for (int m = 0; m < 16; ++m) {
+            if (m >= nrow) {
+                key.empty(m);
+                continue;
+            }
+            const int64_t st0 = vs[(size_t)(y0 + m)];
+            key.set(m, st0 - base16, vrow(st0) - base16, vrow(st0 + vax.taps - 1) - base16,
+                    vax.phase ? (*vax.phase)[(size_t)(y0 + m)] : 0);
         }
-        std::vector<unsigned char> bytes;
-        emit_operands(w, kv, limbs, vbias, &bytes);
-        vtile[(size_t)(3 * ti)] = vp.add(std::move(bytes));
+        vtile[(size_t)(3 * ti)] = vp.add(key, [&](std::vector<unsigned char>* b) {
+            int64_t klo[16], khi[16];
+            for (int m = 0; m < 16; ++m) {
+                klo[m] = key.k[4 * m + 1];
+                khi[m] = key.k[4 * m + 2];
+            }
+            for (int64_t m = 0; m < nrow; ++m) {
+                const int16_t* row = vax.row((size_t)(y0 + m));
+                for (int t = 0; t < vax.taps; ++t)
+                    w[(size_t)(m * 64 * kv + (vrow(vs[(size_t)(y0 + m)] + t) - base16))] += row[t];
+            }
+            emit_operands(w, kv, limbs, vbias, klo, khi, b);
+            clear_rows(&w, 64 * kv, klo, khi);
+        });
         vtile[(size_t)(3 * ti + 1)] = (int32_t)base16;
         vtile[(size_t)(3 * ti + 2)] = (int32_t)hi;
     }
@@ -253,16 +312,33 @@ bool plan_sep(int family, const SepAxis& hax, const SepAxis& vax, int32_t hbias,
         for (int64_t jx = 0; jx < nstrips; ++jx) {
             const int64_t s0 = 16 * jx;
             const int64_t span0 = hcol(hs[(size_t)(s0 / bands)]);
-            w.assign((size_t)16 * 64 * kh, 0);
-            for (int n = 0; n < 16 && s0 + n < nsamp; ++n) {
+            if (jx == 0) w.assign((size_t)16 * 64 * kh, 0);
+            PatternKey key;
+            for (int n = 0; n < 16; ++n) {
+                if (s0 + n >= nsamp) {
+                    key.empty(n);
+                    continue;
+                }
                 const int64_t smp = s0 + n, x = smp / bands, b = smp - x * bands;
-                const int16_t* row = hax.row((size_t)x);
-                for (int t = 0; t < hax.taps; ++t)
-                    w[(size_t)(n * 64 * kh + (hcol(hs[(size_t)x] + t) - span0) * bands + b)] += row[t];
+                const int64_t st0 = hs[(size_t)x];
+                key.set(n, (st0 - span0) * bands + b, (hcol(st0) - span0) * bands + b,
+                        (hcol(st0 + hax.taps - 1) - span0) * bands + b, hax.phase ? (*hax.phase)[(size_t)x] : 0);
             }
-            std::vector<unsigned char> bytes;
-            emit_operands(w, kh, limbs, hbias, &bytes);
-            hstrip[(size_t)(2 * jx)] = hp.add(std::move(bytes));
+            hstrip[(size_t)(2 * jx)] = hp.add(key, [&](std::vector<unsigned char>* bytes) {
+                int64_t klo[16], khi[16];
+                for (int n = 0; n < 16; ++n) {
+                    klo[n] = key.k[4 * n + 1];
+                    khi[n] = key.k[4 * n + 2];
+                }
+                for (int n = 0; n < 16 && s0 + n < nsamp; ++n) {
+                    const int64_t smp = s0 + n, x = smp / bands, b = smp - x * bands;
+                    const int16_t* row = hax.row((size_t)x);
+                    for (int t = 0; t < hax.taps; ++t)
+                        w[(size_t)(n * 64 * kh + (hcol(hs[(size_t)x] + t) - span0) * bands + b)] += row[t];
+                }
+                emit_operands(w, kh, limbs, hbias, klo, khi, bytes);
+                clear_rows(&w, 64 * kh, klo, khi);
+            });
             hstrip[(size_t)(2 * jx + 1)] = (int32_t)span0;
         }
         stg->h_tab = hp.put(st, &j->htab_bytes);
@@ -337,19 +413,20 @@ struct F16Axis {
     const double* row(size_t i) const { return wtab + (phase ? (size_t)(*phase)[i] * taps : 0); }
 };
 
-void emit_f16_operands(const std::vector<double>& w, int ksteps, int exp2, std::vector<unsigned char>* out) {
+void emit_f16_operands(const std::vector<double>& w, int ksteps, int exp2, const int64_t (&lo)[16],
+                       const int64_t (&hi)[16], std::vector<unsigned char>* out) {
     out->assign((size_t)ksteps * 2 * 64 * 16, 0);
+    unsigned char* o = out->data();
     const int kw = 32 * ksteps;
-    for (int s = 0; s < ksteps; ++s)
-        for (int lane = 0; lane < 64; ++lane)
-            for (int e = 0; e < 8; ++e) {
-                const int j = lane & 15, q = lane >> 4;
-                const float f = std::ldexp((float)w[(size_t)(j * kw + 32 * s + 8 * q + e)], exp2);
-                const _Float16 hi = (_Float16)f;
-                const _Float16 lo = (_Float16)(f - (float)hi);
-                std::memcpy(out->data() + ((size_t)(2 * s) * 64 + lane) * 16 + 2 * e, &hi, 2);
-                std::memcpy(out->data() + ((size_t)(2 * s + 1) * 64 + lane) * 16 + 2 * e, &lo, 2);
-            }
+    for (int j = 0; j < 16; ++j)
+        for (int64_t k = lo[j]; k <= hi[j]; ++k) {
+            const float f = std::ldexp((float)w[(size_t)(j * kw + k)], exp2);
+            const _Float16 h = (_Float16)f;
+            const _Float16 l = (_Float16)(f - (float)h);
+            const int s = (int)(k >> 5), lane = j + 16 * (int)((k >> 3) & 3), e = (int)(k & 7);
+            std::memcpy(o + ((size_t)(2 * s) * 64 + lane) * 16 + 2 * e, &h, 2);
+            std::memcpy(o + ((size_t)(2 * s + 1) * 64 + lane) * 16 + 2 * e, &l, 2);
+        }
 }
 
 bool plan_f16(const F16Axis& hax, const F16Axis& vax, uint32_t bands, const Rect& out, const Rect& clip,
@@ -411,31 +488,64 @@ bool plan_f16(const F16Axis& hax, const F16Axis& vax, uint32_t bands, const Rect
         const int64_t y0 = 16 * ti, nrow = std::min<int64_t>(16, out.h - y0);
         const int64_t lo = vrow(vs[(size_t)y0]), hi = vrow(vs[(size_t)(y0 + nrow - 1)] + vax.taps - 1);
         const int64_t base8 = lo & ~(int64_t)7;
-        w.assign((size_t)16 * 32 * kv, 0.0);
-        for (int64_t m = 0; m < nrow; ++m) {
-            const double* row = vax.row((size_t)(y0 + m));
-            for (int t = 0; t < vax.taps; ++t)
-                w[(size_t)(m * 32 * kv + (vrow(vs[(size_t)(y0 + m)] + t) - base8))] += row[t];
+        if (ti == 0) w.assign((size_t)16 * 32 * kv, 0.0);
+        PatternKey key;
+        for (int m = 0; m < 16; ++m) {
+            if (m >= nrow) {
+                key.empty(m);
+                continue;
+            }
+            const int64_t st0 = vs[(size_t)(y0 + m)];
+            key.set(m, st0 - base8, vrow(st0) - base8, vrow(st0 + vax.taps - 1) - base8,
+                    vax.phase ? (*vax.phase)[(size_t)(y0 + m)] : 0);
         }
-        std::vector<unsigned char> bytes;
-        emit_f16_operands(w, kv, j->v_exp, &bytes);
-        vtile[(size_t)(3 * ti)] = vp.add(std::move(bytes));
+        vtile[(size_t)(3 * ti)] = vp.add(key, [&](std::vector<unsigned char>* b) {
+            int64_t klo[16], khi[16];
+            for (int m = 0; m < 16; ++m) {
+                klo[m] = key.k[4 * m + 1];
+                khi[m] = key.k[4 * m + 2];
+            }
+            for (int64_t m = 0; m < nrow; ++m) {
+                const double* row = vax.row((size_t)(y0 + m));
+                for (int t = 0; t < vax.taps; ++t)
+                    w[(size_t)(m * 32 * kv + (vrow(vs[(size_t)(y0 + m)] + t) - base8))] += row[t];
+            }
+            emit_f16_operands(w, kv, j->v_exp, klo, khi, b);
+            clear_rows(&w, 32 * kv, klo, khi);
+        });
         vtile[(size_t)(3 * ti + 1)] = (int32_t)base8;
         vtile[(size_t)(3 * ti + 2)] = (int32_t)hi;
     }
     for (int64_t jx = 0; jx < nstrips; ++jx) {
         const int64_t s0 = 16 * jx;
         const int64_t span0 = hcol(hs[(size_t)(s0 / bands)]);
-        w.assign((size_t)16 * 32 * kh, 0.0);
-        for (int n = 0; n < 16 && s0 + n < nsamp; ++n) {
+        if (jx == 0) w.assign((size_t)16 * 32 * kh, 0.0);
+        PatternKey key;
+        for (int n = 0; n < 16; ++n) {
+            if (s0 + n >= nsamp) {
+                key.empty(n);
+                continue;
+            }
             const int64_t smp = s0 + n, x = smp / bands, b = smp - x * bands;
-            const double* row = hax.row((size_t)x);
-            for (int t = 0; t < hax.taps; ++t)
-                w[(size_t)(n * 32 * kh + (hcol(hs[(size_t)x] + t) - span0) * bands + b)] += row[t];
+            const int64_t st0 = hs[(size_t)x];
+            key.set(n, (st0 - span0) * bands + b, (hcol(st0) - span0) * bands + b,
+                    (hcol(st0 + hax.taps - 1) - span0) * bands + b, hax.phase ? (*hax.phase)[(size_t)x] : 0);
         }
-        std::vector<unsigned char> bytes;
-        emit_f16_operands(w, kh, j->h_exp, &bytes);
-        hstrip[(size_t)(2 * jx)] = hp.add(std::move(bytes));
+        hstrip[(size_t)(2 * jx)] = hp.add(key, [&](std::vector<unsigned char>* bytes) {
+            int64_t klo[16], khi[16];
+            for (int n = 0; n < 16; ++n) {
+                klo[n] = key.k[4 * n + 1];
+                khi[n] = key.k[4 * n + 2];
+            }
+            for (int n = 0; n < 16 && s0 + n < nsamp; ++n) {
+                const int64_t smp = s0 + n, x = smp / bands, b = smp - x * bands;
+                const double* row = hax.row((size_t)x);
+                for (int t = 0; t < hax.taps; ++t)
+                    w[(size_t)(n * 32 * kh + (hcol(hs[(size_t)x] + t) - span0) * bands + b)] += row[t];
+            }
+            emit_f16_operands(w, kh, j->h_exp, klo, khi, bytes);
+            clear_rows(&w, 32 * kh, klo, khi);
+        });
         hstrip[(size_t)(2 * jx + 1)] = (int32_t)span0;
     }
     stg->v_tab = vp.put(st, &j->vtab_bytes);

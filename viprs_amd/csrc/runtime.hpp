@@ -95,10 +95,12 @@ struct DeviceBlob {
 // Host staging for per-node plans and tables, uploaded as one blob.
 struct Staging {
     std::vector<unsigned char> bytes;
+    // Appends n bytes (16-byte aligned) and returns their offset; p == nullptr
+    // reserves them for the caller to fill.
     size_t put(const void* p, size_t n) {
         size_t off = (bytes.size() + 15) & ~(size_t)15;
         bytes.resize(off + n);
-        if (n) std::memcpy(bytes.data() + off, p, n);
+        if (n && p) std::memcpy(bytes.data() + off, p, n);
         return off;
     }
 };

@@ -171,7 +171,7 @@ struct Job {
     // state
     JobState state = J_PLANNING;
     int status = VGPU_OK;
-    int64_t t_arrival = 0, t_admit = 0, t_done = 0;
+    int64_t t_arrival = 0, t_planned = 0, t_admit = 0, t_done = 0;
 };
 
 }  // namespace
@@ -203,7 +203,7 @@ struct vgpu_server {
     bool launch_job(Job& j);
     void finish_job(Job& j);
     void free_chunk(Job& j);
-    void planner_loop();
+    void planner_loop(int index);
     void scheduler_loop();
 };
 
@@ -257,23 +257,40 @@ void vgpu_server::plan_job(Job& j) {
     }
 }
 
-void vgpu_server::planner_loop() {
+// Idle threads keep polling this long after their last piece of work before
+// they block: a thread woken through a condition variable costs tens of
+// microseconds of request latency.
+constexpr int64_t kSpinNs = 2000000;
+
+void vgpu_server::planner_loop(int index) {
     (void)hipSetDevice(ctx->device);  // pinned staging allocations
+    int64_t last = now_ns();
     for (;;) {
         std::shared_ptr<Job> j;
         {
             std::unique_lock<std::mutex> g(mu);
-            cv_plan.wait(g, [&] { return stop || !to_plan.empty(); });
-            if (stop && to_plan.empty()) return;
+            for (;;) {
+                if (stop && to_plan.empty()) return;
+                if (!to_plan.empty()) break;
+                if (index < 2 && now_ns() - last < kSpinNs) {
+                    g.unlock();
+                    std::this_thread::yield();
+                    g.lock();
+                } else {
+                    cv_plan.wait(g);
+                }
+            }
             j = to_plan.front();
             to_plan.pop_front();
         }
         plan_job(*j);
         {
             std::lock_guard<std::mutex> g(mu);
+            j->t_planned = now_ns();
             j->state = J_READY;
         }
         cv_sched.notify_one();
+        last = now_ns();
     }
 }
 
@@ -346,6 +363,7 @@ void vgpu_server::finish_job(Job& j) {
 void vgpu_server::scheduler_loop() {
     (void)hipSetDevice(ctx->device);
     std::unique_lock<std::mutex> g(mu);
+    int64_t last = now_ns();
     for (;;) {
         // Retire finished requests.
         bool retired = false;
@@ -365,7 +383,10 @@ void vgpu_server::scheduler_loop() {
             running.pop_back();
             retired = true;
         }
-        if (retired) cv_done.notify_all();
+        if (retired) {
+            cv_done.notify_all();
+            last = now_ns();
+        }
         // Admit planned requests in arrival order while slots and memory allow.
         bool blocked = false;
         while (!arrival.empty() && !blocked) {
@@ -417,13 +438,15 @@ void vgpu_server::scheduler_loop() {
             running.push_back(arrival.front());
             peak_inflight = std::max<uint64_t>(peak_inflight, running.size());
             arrival.pop_front();
+            last = now_ns();
         }
         if (stop && arrival.empty() && running.empty() && to_plan.empty()) return;
-        if (running.empty()) {
-            // Nothing on the device: sleep until a request is planned or released.
+        if (running.empty() && now_ns() - last >= kSpinNs) {
+            // Idle for a while: sleep until a request is planned or released.
             cv_sched.wait(g);
+            last = now_ns();
         } else {
-            // Requests on the device: poll their events without holding the lock long.
+            // Requests on the device, or recently: poll without holding the lock long.
             g.unlock();
             std::this_thread::yield();
             g.lock();
@@ -445,7 +468,9 @@ void fill_result(const Job& j, vgpu_request_result* r) {
     r->format = j.ofmt;
     r->out_bytes = j.out_bytes;
     r->arena_bytes = j.chunk;
+    r->plan_ns = j.t_planned ? j.t_planned - j.t_arrival : 0;
     r->queue_ns = j.t_admit ? j.t_admit - j.t_arrival : 0;
+    r->run_ns = j.t_admit ? j.t_done - j.t_admit : 0;
     r->latency_ns = j.t_done - j.t_arrival;
 }
 }  // namespace
@@ -477,7 +502,7 @@ int vgpu_server_create(vgpu_context* ctx, const vgpu_server_config* cfg, vgpu_se
         }
         s->streams.push_back(st);
     }
-    for (uint32_t i = 0; i < cfg->planner_threads; ++i) s->planners.emplace_back([s] { s->planner_loop(); });
+    for (uint32_t i = 0; i < cfg->planner_threads; ++i) s->planners.emplace_back([s, i] { s->planner_loop((int)i); });
     s->scheduler = std::thread([s] { s->scheduler_loop(); });
     *out = s;
     return VGPU_OK;
@@ -607,7 +632,8 @@ int vgpu_server_stats(vgpu_server* s, vgpu_server_counters* st) {
 // (busy-waiting the last stretch), outputs are released on completion, and
 // latency_ns[i] = completion time - scheduled arrival time.
 int vgpu_server_replay(vgpu_server* s, const vgpu_request* reqs, uint32_t n, const int64_t* arrival_offset_ns,
-                       int64_t* latency_ns, int32_t* status, int64_t* wall_ns) {
+                       int64_t* latency_ns, int32_t* status, int64_t* wall_ns, int64_t* plan_ns,
+                       int64_t* queue_ns) {
     if (!s || (!reqs && n) || (!arrival_offset_ns && n) || (!latency_ns && n) || (!status && n))
         return VGPU_E_INVALID_ARGUMENT;
     std::vector<uint64_t> ids(n);
@@ -631,6 +657,8 @@ int vgpu_server_replay(vgpu_server* s, const vgpu_request* reqs, uint32_t n, con
         if (rc != VGPU_OK) return rc;
         latency_ns[i] = res.latency_ns;
         status[i] = res.status;
+        if (plan_ns) plan_ns[i] = res.plan_ns;
+        if (queue_ns) queue_ns[i] = res.queue_ns;
         last = std::max(last, (int64_t)(arrival_offset_ns[i] + t0 + res.latency_ns));
         std::lock_guard<std::mutex> g(s->mu);
         s->jobs.erase(ids[i]);
