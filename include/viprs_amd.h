@@ -190,6 +190,11 @@ int vgpu_pipeline_run(vgpu_pipeline *p, const void *src, size_t src_bytes, void 
 int vgpu_pipeline_run_profiled(vgpu_pipeline *p, const void *src, size_t src_bytes, void *dst,
                                size_t dst_bytes, int iterations, float *stage_ms,
                                int max_stages);
+/* Mean duration of fused stage `stage` alone: one full run (fills the
+ * intermediates), then `iterations` back-to-back launches of that stage
+ * between two HIP events on the context stream.  No reference counterpart. */
+int vgpu_pipeline_time_stage(vgpu_pipeline *p, const void *src, size_t src_bytes, void *dst,
+                             size_t dst_bytes, int stage, int iterations, float *mean_ms);
 /* Timing harness on the context stream: `warmup` untimed runs, then
  * `iterations` back-to-back runs bracketed by HIP events (total_ms), then
  * `iterations` more with an event after each (step_ms[i], may be NULL).

@@ -146,6 +146,25 @@ def test_shrink_factors_and_bands(ctx, f, bands):
     assert_same(got, exp, f"shrink v/h ceil {f} bands {bands}")
 
 
+@pytest.mark.parametrize("hs,vs", [(2, 2), (3, 2), (4, 4), (1, 5), (7, 1)])
+@pytest.mark.parametrize("bands", [1, 3, 4])
+def test_builder_shrink_both_axes(ctx, hs, vs, bands):
+    # PipelineBuilder::shrink(h, v): Shrink<F> (shrink.rs:37-293) = ShrinkH then
+    # ShrinkV in one op.  For u8 2x2 the reference's direct form is
+    # ((floor((a+b+1)/2) + floor((c+d+1)/2) + 1) >> 1) (shrink.rs:225-293).
+    src = noise_u8(101, 203, bands, hs * 10 + vs + bands)
+    got, exp = run_both(ctx, src, [("shrink", (hs, vs))])
+    assert_same(got, exp, f"shrink({hs}, {vs}) bands {bands}")
+    if (hs, vs) == (2, 2):
+        s = src.astype(np.int32)
+        a, b_, c, d = s[0:100:2, 0:202:2], s[0:100:2, 1:202:2], s[1:100:2, 0:202:2], s[1:100:2, 1:202:2]
+        direct = ((((a + b_ + 1) >> 1) + ((c + d + 1) >> 1) + 1) >> 1).astype(np.uint8)
+        assert np.array_equal(got[:50, :101], direct)
+    for src16 in (noise_u16(64, 96, bands, 3),):
+        got, exp = run_both(ctx, src16, [("shrink", (hs, vs))])
+        assert_same(got, exp, f"u16 shrink({hs}, {vs}) bands {bands}")
+
+
 def test_u16_paths(ctx):
     src = noise_u16(211, 307, 3, 8)
     for steps in ([("reduce_h", (2.7, v.LANCZOS3)), ("reduce_v", (1.9, v.BICUBIC))],

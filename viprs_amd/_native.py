@@ -129,6 +129,7 @@ def load() -> C.CDLL:
         "vgpu_pipeline_run": (i32, [vp, vp, sz, vp, sz]),
         "vgpu_pipeline_run_host": (i32, [vp, vp, sz, vp, sz]),
         "vgpu_pipeline_run_profiled": (i32, [vp, vp, sz, vp, sz, i32, P(f32), i32]),
+        "vgpu_pipeline_time_stage": (i32, [vp, vp, sz, vp, sz, i32, i32, P(f32)]),
         "vgpu_pipeline_benchmark": (i32, [vp, vp, sz, vp, sz, i32, i32, P(f32), P(f32)]),
         "vgpu_op_reduce_h_new": (i32, [vp, dbl, i32, u32, u32, i32, P(vp)]),
         "vgpu_op_reduce_v_new": (i32, [vp, dbl, i32, u32, u32, i32, P(vp)]),

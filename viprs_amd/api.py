@@ -306,6 +306,15 @@ class Pipeline:
             raise VgpuError(n, "run_profiled")
         return [buf[i] for i in range(n)]
 
+    def time_stage(self, src, dst, stage: int, iterations: int = 50) -> float:
+        """Mean device milliseconds of fused stage `stage` over back-to-back
+        launches between two HIP events."""
+        ms = C.c_float()
+        _check(self._lib.vgpu_pipeline_time_stage(self._h, C.c_void_p(_ptr(src)), _nbytes(src),
+                                                  C.c_void_p(_ptr(dst)), _nbytes(dst), stage, iterations,
+                                                  C.byref(ms)), "time_stage")
+        return ms.value
+
     def run_host(self, src: np.ndarray) -> np.ndarray:
         src = np.ascontiguousarray(src, NP_DTYPE[self.src_fmt])
         w, h, b, f = self.output_info()

@@ -1446,6 +1446,42 @@ int vgpu_pipeline_run_profiled(vgpu_pipeline* p, const void* src, size_t src_byt
     return rc == VGPU_OK ? ns : rc;
 }
 
+int vgpu_pipeline_time_stage(vgpu_pipeline* p, const void* src, size_t src_bytes, void* dst, size_t dst_bytes,
+                             int stage, int iterations, float* mean_ms) {
+    if (!p || iterations < 1 || !mean_ms) return VGPU_E_INVALID_ARGUMENT;
+    vgpu_context* ctx = p->ctx;
+    (void)hipSetDevice(ctx->device);
+    int rc = vgpu_pipeline_run(p, src, src_bytes, dst, dst_bytes);  // compiles, fills the intermediates
+    if (rc != VGPU_OK) return rc;
+    const int ns = (int)p->stages.size();
+    if (stage < 0 || stage >= ns) return VGPU_E_INVALID_ARGUMENT;
+    // Input view of the stage: the source or the previous stage's output.
+    const void* cur = src;
+    Rect cur_rect = whole(p->src_w, p->src_h);
+    for (int k = 0; k < stage; ++k) {
+        cur = p->inter[(size_t)k];
+        cur_rect = p->stages[(size_t)k].out;
+    }
+    const Stage& s = p->stages[(size_t)stage];
+    void* o = (stage + 1 == ns) ? dst : p->inter[(size_t)stage];
+    const unsigned char* blob = static_cast<const unsigned char*>(p->blob.ptr);
+    const InView in = view_of(cur, cur_rect, s.clip);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return VGPU_E_DEVICE;
+    for (int i = 0; i < 3 && rc == VGPU_OK; ++i) rc = launch_stage(p->nodes, s, in, o, blob, ctx->stream, p->opt);
+    (void)hipEventRecord(e0, ctx->stream);
+    for (int i = 0; i < iterations && rc == VGPU_OK; ++i)
+        rc = launch_stage(p->nodes, s, in, o, blob, ctx->stream, p->opt);
+    (void)hipEventRecord(e1, ctx->stream);
+    float ms = 0.f;
+    if (rc == VGPU_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = VGPU_E_DEVICE;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *mean_ms = ms / (float)iterations;
+    return rc;
+}
+
 int vgpu_pipeline_benchmark(vgpu_pipeline* p, const void* src, size_t src_bytes, void* dst,
                             size_t dst_bytes, int warmup, int iterations, float* total_ms,
                             float* step_ms) {
