@@ -166,6 +166,12 @@ int vgpu_pipeline_set_integer_mfma(vgpu_pipeline *p, int enable);
  * not fit).  Modes 1 and 2 differ from the reference in the last bits.  No
  * reference counterpart. */
 int vgpu_pipeline_set_float_mode(vgpu_pipeline *p, int mode);
+/* Kernel tuning knobs (results do not change).  No reference counterpart. */
+typedef enum {
+    VGPU_OPT_SEP_ROWS_PER_TASK = 2, /* output rows per MFMA wavefront task: 0 = auto, else a multiple of 16 */
+    VGPU_OPT_SHRINK_RUN = 3         /* box-shrink output rows per wavefront run: 0 = auto, 1..64 */
+} vgpu_option;
+int vgpu_pipeline_set_option(vgpu_pipeline *p, int option, int value);
 /* CompiledPipeline output geometry (pipeline/compiled.rs:338-). */
 int vgpu_pipeline_output_info(const vgpu_pipeline *p, uint32_t *width, uint32_t *height,
                               uint32_t *bands, int *format);
@@ -284,6 +290,8 @@ int vgpu_batch_info(const vgpu_batch *b, vgpu_batch_stats *stats);
 /* Launch shape: persistent grid size (0 = one block per task), HIP graph on
  * or off, largest reduce tile candidate (re-plans the batch). */
 int vgpu_batch_configure(vgpu_batch *b, int grid_cap, int use_graph, int reduce_first_tile);
+/* As vgpu_pipeline_set_option, for every image; re-plans the batch. */
+int vgpu_batch_set_option(vgpu_batch *b, int option, int value);
 /* As vgpu_pipeline_set_integer_mfma, for every image; re-plans the batch. */
 int vgpu_batch_set_integer_mfma(vgpu_batch *b, int enable);
 /* Enqueue the whole batch on the context stream.  srcs[i] / dsts[i] are device

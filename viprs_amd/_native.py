@@ -125,6 +125,8 @@ def load() -> C.CDLL:
         "vgpu_pipeline_resize": (i32, [vp, dbl, dbl, i32]),
         "vgpu_pipeline_output_info": (i32, [vp, P(u32), P(u32), P(u32), P(i32)]),
         "vgpu_pipeline_stage_count": (i32, [vp]),
+        "vgpu_pipeline_set_option": (i32, [vp, i32, i32]),
+        "vgpu_batch_set_option": (i32, [vp, i32, i32]),
         "vgpu_pipeline_stage_kernel": (i32, [vp, i32, C.c_char_p, sz]),
         "vgpu_pipeline_run": (i32, [vp, vp, sz, vp, sz]),
         "vgpu_pipeline_run_host": (i32, [vp, vp, sz, vp, sz]),

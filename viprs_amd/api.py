@@ -36,6 +36,7 @@ E_NO_MEMORY = -7
 E_CANCELLED = -8
 E_ADMISSION = -9
 E_NOT_READY = -10
+OPT_SEP_ROWS_PER_TASK, OPT_SHRINK_RUN = 2, 3
 
 NP_DTYPE = {U8: np.uint8, U16: np.uint16, F32: np.float32}
 SAMPLE_BYTES = {U8: 1, U16: 2, F32: 4}
@@ -263,6 +264,11 @@ class Pipeline:
         _check(self._lib.vgpu_pipeline_set_float_mode(self._h, int(mode)), "set_float_mode")
         return self
 
+    def set_option(self, option: int, value: int):
+        """Kernel tuning knob (OPT_*); results do not change."""
+        _check(self._lib.vgpu_pipeline_set_option(self._h, option, value), "set_option")
+        return self
+
     def stage_count(self) -> int:
         n = self._lib.vgpu_pipeline_stage_count(self._h)
         if n < 0:
@@ -380,6 +386,10 @@ class Batch:
     def set_integer_mfma(self, enable, shrink: bool = False) -> None:
         mode = (1 if enable else 0) | (2 if enable and shrink else 0)
         _check(self._lib.vgpu_batch_set_integer_mfma(self._h, mode), "vgpu_batch_set_integer_mfma")
+
+    def set_option(self, option: int, value: int) -> None:
+        """Kernel tuning knob (OPT_*) for every image; results do not change."""
+        _check(self._lib.vgpu_batch_set_option(self._h, option, value), "vgpu_batch_set_option")
 
     def _tables(self, srcs, dsts):
         n = len(self)

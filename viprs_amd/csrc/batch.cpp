@@ -218,7 +218,7 @@ int vgpu_batch::plan() {
                 ShrinkJob j = shrink_job_of(s, im.nodes[(size_t)s.first], in, out);
                 j.src_slot = first ? idx : -1;
                 j.dst_slot = last ? idx : -1;
-                if (!plan_shrink_job(&j, opt.shrink_row_split)) {
+                if (!plan_shrink_job(&j, opt.shrink_row_split, opt.shrink_run)) {
                     im.batched = false;
                     continue;
                 }
@@ -453,6 +453,25 @@ int vgpu_batch_configure(vgpu_batch* b, int grid_cap, int use_graph, int reduce_
         return b->plan();
     }
     return VGPU_OK;
+}
+
+int vgpu_batch_set_option(vgpu_batch* b, int option, int value) {
+    if (!b) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(b->ctx->device);
+    if (hipStreamSynchronize(b->ctx->stream) != hipSuccess) return b->ctx->sticky = VGPU_E_DEVICE;
+    switch (option) {
+    case VGPU_OPT_SEP_ROWS_PER_TASK:
+        if (value < 0 || value % 16) return VGPU_E_INVALID_ARGUMENT;
+        b->opt.sep_rows_per_task = value;
+        break;
+    case VGPU_OPT_SHRINK_RUN:
+        if (value < 0 || value > 64) return VGPU_E_INVALID_ARGUMENT;
+        b->opt.shrink_run = value;
+        break;
+    default: return VGPU_E_INVALID_ARGUMENT;
+    }
+    b->drop_graph();
+    return b->plan();
 }
 
 int vgpu_batch_set_integer_mfma(vgpu_batch* b, int enable) {

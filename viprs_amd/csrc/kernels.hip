@@ -181,11 +181,12 @@ __device__ __forceinline__ uint32_t band_mask(int b, int kk) {
 
 // Sum of one box window of wbytes bytes starting at LDS byte sbyte, per band;
 // rbuf must be 4-byte aligned.  Dword d of the window is realigned from two
-// LDS dwords by alignbyte, masked (bytes past the window end -> 0; the masks
-// depend only on d and the window size, so they are scalar), and folded into
+// LDS dwords by alignbyte, masked (bytes past the window end -> 0; the mask
+// depends only on d and the window size, so it is scalar), and folded into
 // the band sums by udot4 with per-band byte masks.  The loop is unrolled by U
-// dwords (a multiple of 3 for 3 bands, so the band pattern restarts) and may
-// read up to U dwords past the window; those bytes are masked.
+// dwords (a multiple of 3 for 3 bands, so the band pattern restarts) with two
+// accumulator chains, and may read up to U dwords past the window; those
+// bytes are masked.
 template <int B>
 __device__ __forceinline__ void window_sums(const unsigned char* rbuf, int sbyte, int wbytes,
                                             uint32_t (&sum)[B]) {
@@ -218,7 +219,9 @@ __device__ __forceinline__ void window_sums(const unsigned char* rbuf, int sbyte
             }
         }
     }
-    // Tail block: the last 1..U dwords, masked.
+    // Tail block: the last 1..U dwords, masked.  Unrolled, so that all its
+    // LDS reads are in flight together (a dword-at-a-time loop exposes the LDS
+    // latency on every step and is slower even for small windows).
     {
         uint32_t w[U];
 #pragma unroll
@@ -272,13 +275,13 @@ __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, 
     const int wbytes = fh * B;
     const int slot_mask = (1 << lsl) - 1;
 
-    // Fetch group g into p0..p3; vmask holds 2 bits per chunk: 1 = full
-    // 16-byte chunk, 2 = the buffer's last, partial chunk (copied bytewise at
-    // park time), 0 = unused.  Unused or partial chunks reload the buffer's
-    // first 16 bytes, so every load is unconditional.
-    u32x4 p0 = {0u, 0u, 0u, 0u}, p1 = p0, p2 = p0, p3 = p0;
-    uint32_t vmask = 0;
-#define VGPU_CHUNK(j, g, pj, ROW)                                                                   \
+    // Fetch group g into p0..p3; vm holds 2 bits per chunk: 1 = full 16-byte
+    // chunk, 2 = the buffer's last, partial chunk (copied bytewise at park
+    // time), 0 = unused.  Unused or partial chunks reload the buffer's first
+    // 16 bytes, so every load is unconditional.  One group is in flight in
+    // registers while the parked one is summed (a second register set costs
+    // more occupancy than it buys).
+#define VGPU_CHUNK(j, g, pj, ROW, VM)                                                               \
     do {                                                                                            \
         const int k_ = (g) * RP + (ROW);                                                            \
         const int c_ = (lane + 64 * (j)) & slot_mask;                                               \
@@ -288,29 +291,29 @@ __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, 
         const size_t off_ = a16_ + (size_t)c_ * 16;                                                 \
         const bool use_ = k_ < nsrc && c_ < nch_;                                                   \
         const bool full_ = use_ && off_ + 16 <= total;                                              \
-        vmask |= (uint32_t)(full_ ? 1 : (use_ ? 2 : 0)) << (2 * (j));                               \
+        VM |= (uint32_t)(full_ ? 1 : (use_ ? 2 : 0)) << (2 * (j));                                  \
         pj = gmem<u32x4>(base + (full_ ? off_ : 0))[0];                                             \
     } while (0)
 // The row of register j's chunks is wave-uniform when a slot spans at least
 // one register (lsl >= 6): that branch keeps the address math scalar.
-#define VGPU_FETCH(g)                                                                   \
+#define VGPU_FETCH(g, P0, P1, P2, P3, VM)                                               \
     do {                                                                                \
-        vmask = 0;                                                                      \
+        VM = 0;                                                                         \
         if (lsl >= 6) {                                                                 \
-            VGPU_CHUNK(0, g, p0, 0);                                                    \
-            VGPU_CHUNK(1, g, p1, 1 >> (lsl - 6));                                       \
-            VGPU_CHUNK(2, g, p2, 2 >> (lsl - 6));                                       \
-            VGPU_CHUNK(3, g, p3, 3 >> (lsl - 6));                                       \
+            VGPU_CHUNK(0, g, P0, 0, VM);                                                \
+            VGPU_CHUNK(1, g, P1, 1 >> (lsl - 6), VM);                                   \
+            VGPU_CHUNK(2, g, P2, 2 >> (lsl - 6), VM);                                   \
+            VGPU_CHUNK(3, g, P3, 3 >> (lsl - 6), VM);                                   \
         } else {                                                                        \
-            VGPU_CHUNK(0, g, p0, (lane >> lsl));                                        \
-            VGPU_CHUNK(1, g, p1, (lane >> lsl) + (64 >> lsl));                          \
-            VGPU_CHUNK(2, g, p2, (lane >> lsl) + 2 * (64 >> lsl));                      \
-            VGPU_CHUNK(3, g, p3, (lane >> lsl) + 3 * (64 >> lsl));                      \
+            VGPU_CHUNK(0, g, P0, (lane >> lsl), VM);                                    \
+            VGPU_CHUNK(1, g, P1, (lane >> lsl) + (64 >> lsl), VM);                      \
+            VGPU_CHUNK(2, g, P2, (lane >> lsl) + 2 * (64 >> lsl), VM);                  \
+            VGPU_CHUNK(3, g, P3, (lane >> lsl) + 3 * (64 >> lsl), VM);                  \
         }                                                                               \
     } while (0)
-#define VGPU_PARK(j, g, pj)                                                                         \
+#define VGPU_PARK1(j, g, pj, VM)                                                                    \
     do {                                                                                            \
-        const uint32_t st_ = (vmask >> (2 * (j))) & 3u;                                             \
+        const uint32_t st_ = (VM >> (2 * (j))) & 3u;                                                \
         const int idx_ = lane + 64 * (j);                                                           \
         if (st_ == 1u) {                                                                            \
             *reinterpret_cast<u32x4*>(rbuf + idx_ * 16) = pj;                                       \
@@ -321,19 +324,19 @@ __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, 
             for (int q_ = 0; q_ < 16 && off_ + q_ < total; ++q_) rbuf[idx_ * 16 + q_] = gmem<unsigned char>(base)[off_ + q_]; \
         }                                                                                           \
     } while (0)
+#define VGPU_PARK(g, P0, P1, P2, P3, VM)  \
+    do {                                  \
+        VGPU_PARK1(0, g, P0, VM);         \
+        VGPU_PARK1(1, g, P1, VM);         \
+        VGPU_PARK1(2, g, P2, VM);         \
+        VGPU_PARK1(3, g, P3, VM);         \
+    } while (0)
 
     uint32_t acc[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) acc[b] = 0;
-    const int ng = (nsrc + RP - 1) / RP;
-    if (ng > 0) VGPU_FETCH(0);
-    for (int g = 0; g < ng; ++g) {
-        VGPU_PARK(0, g, p0);
-        VGPU_PARK(1, g, p1);
-        VGPU_PARK(2, g, p2);
-        VGPU_PARK(3, g, p3);
-        wave_lds_sync();
-        if (g + 1 < ng) VGPU_FETCH(g + 1);
+    // Box sums of the parked group g's source rows.
+    auto compute = [&](int g) {
         const int nr = min(RP, nsrc - g * RP);
         for (int r = 0; r < nr; ++r) {
             const int k = g * RP + r;
@@ -384,9 +387,20 @@ __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, 
                 for (int b = 0; b < B; ++b) acc[b] = 0;
             }
         }
+    };
+    const int ng = (nsrc + RP - 1) / RP;
+    u32x4 p0 = {0u, 0u, 0u, 0u}, p1 = p0, p2 = p0, p3 = p0;
+    uint32_t vma = 0;
+    if (ng > 0) VGPU_FETCH(0, p0, p1, p2, p3, vma);
+    for (int g = 0; g < ng; ++g) {
+        VGPU_PARK(g, p0, p1, p2, p3, vma);
+        wave_lds_sync();
+        if (g + 1 < ng) VGPU_FETCH(g + 1, p0, p1, p2, p3, vma);
+        compute(g);
         wave_lds_sync();
     }
 #undef VGPU_PARK
+#undef VGPU_PARK1
 #undef VGPU_FETCH
 #undef VGPU_CHUNK
     if (S > 1) {
@@ -416,6 +430,8 @@ __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, 
     }
 }
 
+// One kernel per band count and fetch depth, so each keeps only its own
+// registers.
 // One kernel per band count, so each keeps only its own registers.
 template <int B>
 __global__ __launch_bounds__(kThreads) void k_shrink_job(ShrinkJob job) {
@@ -838,7 +854,7 @@ unsigned grid_for(size_t n) {
 
 }  // namespace
 
-bool plan_shrink_job(ShrinkJob* job, bool row_split) {
+bool plan_shrink_job(ShrinkJob* job, bool row_split, int run) {
     const int fh = job->fh, fv = job->fv, bands = job->bands;
     if (bands < 1 || bands > 4 || fh < 1 || fv < 1) return false;
     const size_t src_bytes = (size_t)job->in.bw * job->in.bh * bands;
@@ -865,6 +881,10 @@ bool plan_shrink_job(ShrinkJob* job, bool row_split) {
         while (m > 1 && waves(m) < 256 * 16) m = (m + 1) / 2;
         if (m == 1)
             while (S < 4 && 2 * S <= fv && waves(1) * 2 * S <= 256 * 4 * 6) S *= 2;
+    }
+    if (run > 0) {  // caller's run length: no row split
+        m = std::min(run, 64);
+        S = 1;
     }
     job->tx = tx;
     job->S = S;

@@ -114,7 +114,7 @@ hipError_t launch_reduce_h(const InView& in, const OutView& out, int bands, int 
                            const DevPlan& plan, int post_alpha, hipStream_t s);
 hipError_t launch_reduce_v(const InView& in, const OutView& out, int bands, int fmt,
                            const DevPlan& plan, int post_alpha, hipStream_t s);
-bool plan_shrink_job(ShrinkJob* job, bool row_split = true);
+bool plan_shrink_job(ShrinkJob* job, bool row_split = true, int run = 0);
 size_t shrink_job_lds();
 hipError_t launch_shrink_job(const ShrinkJob& job, hipStream_t s);
 // Kernel variant of a planned job (its band count); a batch launch runs jobs

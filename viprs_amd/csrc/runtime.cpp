@@ -796,7 +796,7 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
     switch (s.kind) {
     case ST_SHRINK_HV_U8: {
         ShrinkJob j = shrink_job_of(s, nd, in, out);
-        if (plan_shrink_job(&j, opt.shrink_row_split)) {
+        if (plan_shrink_job(&j, opt.shrink_row_split, opt.shrink_run)) {
             e = launch_shrink_job(j, stream);
         } else {
             e = launch_shrink_hv_u8(in, out, B, s.fh, s.fv, s.premult, stream);
@@ -1324,6 +1324,25 @@ int vgpu_pipeline_set_integer_mfma(vgpu_pipeline* p, int enable) {
     p->opt.integer_mfma = (enable & 1) != 0;
     p->opt.mfma_shrink = (enable & 2) != 0;
     return VGPU_OK;
+}
+
+int vgpu_pipeline_set_option(vgpu_pipeline* p, int option, int value) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(p->ctx->device);
+    if (hipStreamSynchronize(p->ctx->stream) != hipSuccess) return p->ctx->sticky = VGPU_E_DEVICE;
+    switch (option) {
+    case VGPU_OPT_SEP_ROWS_PER_TASK:
+        if (value < 0 || value % 16) return VGPU_E_INVALID_ARGUMENT;
+        p->release();
+        p->opt.sep_rows_per_task = value;
+        return VGPU_OK;
+    case VGPU_OPT_SHRINK_RUN:
+        if (value < 0 || value > 64) return VGPU_E_INVALID_ARGUMENT;
+        p->release();
+        p->opt.shrink_run = value;
+        return VGPU_OK;
+    default: return VGPU_E_INVALID_ARGUMENT;
+    }
 }
 
 int vgpu_pipeline_set_float_mode(vgpu_pipeline* p, int mode) {

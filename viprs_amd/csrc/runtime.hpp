@@ -154,6 +154,7 @@ struct StageOptions {
     // 1 = plain FP32, 2 = FP16 hi/lo split on MFMA (falls back to 1).
     int float_mode = 0;
     bool shrink_row_split = true;      // split a shrink row's source rows over wavefronts
+    int shrink_run = 0;                // shrink output rows per wavefront run (0 = planner's choice)
 };
 
 Rect whole(uint32_t w, uint32_t h);
