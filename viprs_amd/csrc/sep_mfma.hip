@@ -197,6 +197,56 @@ __device__ __forceinline__ i32x4 load_a(const unsigned char* base, uint32_t off,
 // 4096; ShrinkH f = 5: mul_hi by ceil(2^34 / 5) >> 2; other shrinks: the
 // reference's own 32-bit ((sum + f/2) * floor(2^24 / f)) >> 24, or the pair
 // rounding (sum + 1) >> 1); these quotients lie in 0..255.
+// H A operands stream through a per-wavefront LDS ring of slots by
+// LDS-direct loads (global_load_lds_dwordx4: lane L's 16 bytes land at
+// slot + 16 L; global_load_lds_dword: lane L's dword at slot + 1024 + 4 L),
+// so several M-tile steps are in flight without holding registers.  A slot
+// carries the 16 bytes from the lane's dword-aligned start (4 bands: rows
+// start on dwords) plus, for other band counts, the next dword, realigned by
+// alignbyte.
+template <int B>
+constexpr int a_dwords() {
+    return B == 4 ? 4 : 5;
+}
+template <int FAM>
+constexpr int sep_slots() {
+    return FAM == 0 ? 3 : 4;  // deeper rings measured slower (LDS occupancy)
+}
+// A slot holds an H operand (1 KiB, plus 256 bytes when realigned) or a V
+// tile's first K-step operand (1 KiB per limb).
+template <int B, int LIMBS>
+constexpr size_t a_slot_bytes() {
+    return (size_t)(a_dwords<B>() > 4 * LIMBS ? a_dwords<B>() : 4 * LIMBS) * 256;
+}
+// Bytes of the per-wavefront region before the ring: family 0's
+// weight-folding scratch, which doubles as the slot ring once the H weights
+// are in registers, or the slot ring alone.
+template <int B, int LIMBS, int FAM>
+__host__ __device__ constexpr size_t sep_slot_region(int kh) {
+    const size_t slots = sep_slots<FAM>() * a_slot_bytes<B, LIMBS>();
+    const size_t fold = FAM == 0 ? (16 * 64 * (size_t)kh + 16) * sizeof(int) : 0;
+    return fold > slots ? fold : slots;
+}
+
+// LDS-direct loads: every lane's 16 (4) bytes from src land at dst + 16 lane
+// (dst + 4 lane); dst is wavefront-uniform.
+__device__ __forceinline__ void lds_load16(const unsigned char* src, unsigned char* dst) {
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                     (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+}
+__device__ __forceinline__ void lds_load4(const unsigned char* src, unsigned char* dst) {
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                     (__attribute__((address_space(3))) void*)dst, 4, 0, 0);
+}
+
+// Wait until at most N vector-memory operations are outstanding (they retire
+// in issue order); the memory clobber keeps LDS reads after it.
+template <int N>
+__device__ __forceinline__ void wait_vmem() {
+    static_assert(N >= 0 && N < 16, "vmcnt immediate");
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
 template <int FAM>
 __device__ __forceinline__ uint32_t sep_round(int acc, uint32_t mul, int shift, int hi) {
     if constexpr (FAM == 0) {
@@ -217,8 +267,7 @@ __device__ void sep_task(const SepJob& job, int jx, int ty, unsigned char* lds, 
     const int ks = 64 * KH;
     int* scratch = reinterpret_cast<int*>(lds + (size_t)wave * wave_lds);
     int* corr = scratch + 16 * ks;
-    unsigned char* ring = FAM == 0 ? reinterpret_cast<unsigned char*>(corr + 16)
-                                   : reinterpret_cast<unsigned char*>(scratch);
+    unsigned char* ring = lds + (size_t)wave * wave_lds + sep_slot_region<B, LIMBS, FAM>(KH);
     const InView& in = job.in;
     const OutView& out = job.out;
     const int nsamp = out.w * B;
@@ -275,42 +324,132 @@ __device__ void sep_task(const SepJob& job, int jx, int ty, unsigned char* lds, 
     const uint32_t lane_off = span_off + 16 * q;
     auto row_addr = [&](int r) -> uint32_t { return (uint32_t)row_of(in, r + n) * row_bytes + lane_off; };
 
-    i32x4 acur, anxt;
+    // ---- Operand stream, in the order the loop consumes it: for each 16-row
+    // output tile, its new H (M tile, K step) operands, then its first V
+    // operand; NS slots ahead, through LDS-direct loads.  Waiting for a slot
+    // needs an immediate vmcnt: every element is at least one load, so while
+    // NS - 1 newer elements are in flight, NS - 1 outstanding loads leave the
+    // slot's own landed.
+    constexpr int ND = a_dwords<B>(), NS = sep_slots<FAM>();
+    constexpr size_t SLOT = a_slot_bytes<B, LIMBS>();
+    unsigned char* aring = lds + (size_t)wave * wave_lds;
+    const int last_ti = (y_end - 1) >> 4;
+    int pf_ti = y_begin >> 4;
+    int pf_r0 = cload(job.vtile, 3 * pf_ti + 1), pf_hi = cload(job.vtile, 3 * pf_ti + 2), pf_s = 0;
+    bool pf_v = false;  // the cursor is on tile pf_ti's V operand
+    bool pf_done = y_begin >= y_end;
+    int issued = 0, consumed = 0;
+
+    auto pf_issue = [&]() {
+        if (pf_done) return;
+        unsigned char* slot = aring + (size_t)(issued % NS) * SLOT;
+        ++issued;
+        if (pf_v) {
+            const unsigned char* vt = job.vtab + (size_t)cload(job.vtile, 3 * pf_ti) * job.vtab_bytes;
+#pragma unroll
+            for (int l = 0; l < LIMBS; ++l) lds_load16(vt + (size_t)l * 1024 + lane * 16, slot + 1024 * l);
+            pf_v = false;
+            if (++pf_ti > last_ti) {
+                pf_done = true;
+                return;
+            }
+            pf_r0 = max(pf_r0, cload(job.vtile, 3 * pf_ti + 1));
+            pf_hi = cload(job.vtile, 3 * pf_ti + 2);
+            pf_v = pf_r0 > pf_hi;
+            return;
+        }
+        const uint32_t a4 = (row_addr(pf_r0) + 64 * pf_s) & ~3u;
+        const bool whole = a4 + 4 * ND <= total;
+        const unsigned char* src = base + (whole ? a4 : 0u);
+        lds_load16(src, slot);
+        if constexpr (ND == 5) lds_load4(src + 16, slot + 1024);
+        if (!whole) {
+            // The buffer's last bytes (a lane whose dwords run past its end
+            // loaded from its start above): read bytewise, zero past the end,
+            // and written over the slot once those loads (and so the slot's,
+            // issued before them) have landed.
+            uint32_t d[ND];
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+                uint32_t v = 0;
+                for (int e = 0; e < 4; ++e) {
+                    const uint32_t off = a4 + 4 * j + e;
+                    if (off < total) v |= (uint32_t)gmem<unsigned char>(base)[off] << (8 * e);
+                }
+                d[j] = v;
+            }
+            wait_vmem<0>();
+#pragma unroll
+            for (int j = 0; j < 4; ++j) *reinterpret_cast<uint32_t*>(slot + 16 * lane + 4 * j) = d[j];
+            if constexpr (ND == 5) *reinterpret_cast<uint32_t*>(slot + 1024 + 4 * lane) = d[4];
+        }
+        if (++pf_s == KH) {
+            pf_s = 0;
+            pf_r0 += 16;
+            pf_v = pf_r0 > pf_hi;
+        }
+    };
+    // Wait for the next element's slot; returns it.  next() then refills it
+    // with the element NS ahead once its reads are done.
+    auto slot_in = [&]() -> const unsigned char* {
+        if (issued - consumed >= NS) wait_vmem<NS - 1>();
+        else wait_vmem<0>();
+        return aring + (size_t)(consumed % NS) * SLOT;
+    };
+    auto next = [&]() {
+        ++consumed;
+        // (The slot's reads are in registers before the load that overwrites it.)
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        pf_issue();
+    };
+    auto take_h = [&](int r0, int s) -> i32x4 {
+        const unsigned char* slot = slot_in();
+        const u32x4 d4 = *reinterpret_cast<const u32x4*>(slot + 16 * lane);
+        uint32_t d[ND];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) d[j] = d4[j];
+        if constexpr (ND == 5) d[4] = *reinterpret_cast<const uint32_t*>(slot + 1024 + 4 * lane);
+        uint32_t w[4];
+        if constexpr (B == 4) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w[k] = d[k];
+        } else {
+            const int al = (int)((row_addr(r0) + 64 * s) & 3);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w[k] = __builtin_amdgcn_alignbyte(d[k + 1], d[k], al);
+        }
+        i32x4 av;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t v = w[k];
+            if constexpr ((B == 2 || B == 4) && PM) v = premult_dword<B>(v);
+            av[k] = (int)(v ^ 0x80808080u);
+        }
+        next();
+        return av;
+    };
+    // (When a step's first element is consumed NS elements after the
+    // previous ones were issued, slot_in's wait count stays valid.)
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // family 0's fold reads are done
+    for (int i = 0; i < NS; ++i) pf_issue();
+
     int have = INT32_MIN;  // intermediate rows below `have` (a multiple of 16) are in the ring
-    int pre = INT32_MIN;   // M tile whose first-step A operand is in acur
     int ti = y_begin >> 4;
     int base16 = cload(job.vtile, 3 * ti + 1), hi = cload(job.vtile, 3 * ti + 2);
     for (int y0 = y_begin; y0 < y_end; y0 += 16, ++ti) {
-        // ---- V operands of the 16 output rows (A operand, K = intermediate
-        // rows from base16), host-folded; loaded ahead of the H work.
         const unsigned char* vt = job.vtab + (size_t)cload(job.vtile, 3 * ti) * job.vtab_bytes;
-        const i32x4 vseed = gmem<i32x4>(vt + (size_t)KV * LIMBS * 64 * 16 + 16 * q)[0];
-        i32x4 vop0h = {0, 0, 0, 0}, vop0l;
-        if constexpr (LIMBS == 2) {
-            vop0h = gmem<i32x4>(vt + (size_t)lane * 16)[0];
-            vop0l = gmem<i32x4>(vt + ((size_t)64 + lane) * 16)[0];
-        } else {
-            vop0l = gmem<i32x4>(vt + (size_t)lane * 16)[0];
-        }
-        // ---- intermediate rows [max(have, base16), hi], 16 at a time; each
-        // next operand's load is in flight while the current one is used.
+        // ---- intermediate rows [max(have, base16), hi], 16 at a time.
         int r0 = have > base16 ? have : base16;
-        uint32_t rb = row_addr(r0);
-        if (r0 <= hi && pre != r0) acur = load_a<B, PM>(base, rb, total);
         for (; r0 <= hi; r0 += 16) {
             i32x4 dh = {0, 0, 0, 0}, dl = {hseed, hseed, hseed, hseed};
-            const uint32_t rb_next = r0 + 16 <= hi ? row_addr(r0 + 16) : 0;
 #pragma unroll
             for (int s = 0; s < KMAX; ++s) {
                 if (s < KH) {
-                    if (s + 1 < KH) anxt = load_a<B, PM>(base, rb + 64 * (s + 1), total);
-                    else if (r0 + 16 <= hi) anxt = load_a<B, PM>(base, rb_next, total);
+                    const i32x4 acur = take_h(r0, s);
                     if constexpr (LIMBS == 2) dh = mfma_i8(acur, whi[s], dh);
                     dl = mfma_i8(acur, wlo[s], dl);
-                    acur = anxt;
                 }
             }
-            rb = rb_next;
             // D[m = row][n = sample]: rows r0 + 4q .. + 3 of sample n, stored
             // in the ring already offset (p ^ 0x80) for the V operand.
             uint32_t packed = 0;
@@ -323,15 +462,32 @@ __device__ void sep_task(const SepJob& job, int jx, int ty, unsigned char* lds, 
         }
         have = r0 > have ? r0 : have;
         const int cur_base16 = base16;
-        // Prefetch the first new M tile of the next row tile.
         if (y0 + 16 < y_end) {
             base16 = cload(job.vtile, 3 * ti + 4);
             hi = cload(job.vtile, 3 * ti + 5);
-            const int r1 = have > base16 ? have : base16;
-            if (r1 <= hi) {
-                acur = load_a<B, PM>(base, row_addr(r1), total);
-                pre = r1;
+        }
+        // ---- V operands of the 16 output rows (A operand, K = intermediate
+        // rows from base16), host-folded: the first K step from the stream,
+        // the seeds (128 * sum of the row's weights + bias, per output row
+        // 4q + e) through the scalar cache.
+        i32x4 vop0h = {0, 0, 0, 0}, vop0l;
+        {
+            const unsigned char* slot = slot_in();
+            if constexpr (LIMBS == 2) {
+                vop0h = *reinterpret_cast<const i32x4*>(slot + 16 * lane);
+                vop0l = *reinterpret_cast<const i32x4*>(slot + 1024 + 16 * lane);
+            } else {
+                vop0l = *reinterpret_cast<const i32x4*>(slot + 16 * lane);
             }
+            next();
+        }
+        const int* seeds = reinterpret_cast<const int*>(vt + (size_t)KV * LIMBS * 64 * 16);
+        i32x4 vseed;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int s0v = cload(seeds, e), s1v = cload(seeds, 4 + e), s2v = cload(seeds, 8 + e),
+                      s3v = cload(seeds, 12 + e);
+            vseed[e] = q == 0 ? s0v : (q == 1 ? s1v : (q == 2 ? s2v : s3v));
         }
         i32x4 dh = {0, 0, 0, 0}, dl = vseed;
         for (int s = 0; s < KV; ++s) {
@@ -400,8 +556,10 @@ __global__ __launch_bounds__(kThreads) void k_sep_batch(const SepJob* __restrict
 
 size_t sep_job_wave_lds(const SepJob& job) {
     const size_t ring = 16 * ((size_t)job.ring + 16);
-    if (job.family != 0) return ring;
-    return (16 * 64 * (size_t)job.kh + 16) * sizeof(int) + ring;
+    const int nd = job.bands == 4 ? 4 : 5, vd = 4 * job.limbs;
+    const size_t slots = (size_t)(nd > vd ? nd : vd) * 256 * (job.family == 0 ? 3 : 4);
+    const size_t fold = job.family == 0 ? (16 * 64 * (size_t)job.kh + 16) * sizeof(int) : 0;
+    return std::max(fold, slots) + ring;
 }
 
 int sep_job_variant(const SepJob& job) {
