@@ -101,6 +101,10 @@ struct SepAxis {
     const std::vector<uint8_t>* phase = nullptr;  // null: one weight row
     const int16_t* wtab = nullptr;
     int taps = 1;
+    // 1 when start[i] = start[0] + i (and no phases): a window clear of the
+    // clamps then folds to the same pattern as any other at the same
+    // alignment, which the planner reuses without building its key.
+    int unit_step = 0;
     const int16_t* row(size_t i) const { return wtab + (phase ? (size_t)(*phase)[i] * taps : 0); }
 };
 
@@ -269,11 +273,21 @@ bool plan_sep(int family, const SepAxis& hax, const SepAxis& vax, int32_t hbias,
     PatternSet vp;
     std::vector<int32_t> vtile((size_t)ntiles * 3);
     std::vector<int32_t> w;
+    int vmemo[16];  // unit-step axis: pattern of a full, unclamped tile by its offset in base16
+    std::fill(vmemo, vmemo + 16, -1);
     for (int64_t ti = 0; ti < ntiles; ++ti) {
         const int64_t y0 = 16 * ti, nrow = std::min<int64_t>(16, out.h - y0);
         const int64_t lo = vrow(vs[(size_t)y0]), hi = vrow(vs[(size_t)(y0 + nrow - 1)] + vax.taps - 1);
         const int64_t base16 = lo & ~(int64_t)15;
         if (ti == 0) w.assign((size_t)16 * 64 * kv, 0);
+        vtile[(size_t)(3 * ti + 1)] = (int32_t)base16;
+        vtile[(size_t)(3 * ti + 2)] = (int32_t)hi;
+        const bool clear = vax.unit_step && !vax.phase && nrow == 16 && lo == vs[(size_t)y0] &&
+                           hi == vs[(size_t)(y0 + 15)] + vax.taps - 1;
+        if (clear && vmemo[lo - base16] >= 0) {
+            vtile[(size_t)(3 * ti)] = vmemo[lo - base16];
+            continue;
+        }
         PatternKey key;
         for (int m = 0; m < 16; ++m) {
             if (m >= nrow) {
@@ -298,8 +312,7 @@ bool plan_sep(int family, const SepAxis& hax, const SepAxis& vax, int32_t hbias,
             emit_operands(w, kv, limbs, vbias, klo, khi, b);
             clear_rows(&w, 64 * kv, klo, khi);
         });
-        vtile[(size_t)(3 * ti + 1)] = (int32_t)base16;
-        vtile[(size_t)(3 * ti + 2)] = (int32_t)hi;
+        if (clear) vmemo[lo - base16] = vtile[(size_t)(3 * ti)];
     }
     stg->v_tab = vp.put(st, &j->vtab_bytes);
     stg->v_tile = st->put(vtile.data(), vtile.size() * sizeof(int32_t));
@@ -309,10 +322,20 @@ bool plan_sep(int family, const SepAxis& hax, const SepAxis& vax, int32_t hbias,
         // span pixel.
         PatternSet hp;
         std::vector<int32_t> hstrip((size_t)nstrips * 2);
+        int hmemo[4] = {-1, -1, -1, -1};  // unit-step axis: full, unclamped strip by s0 % bands
         for (int64_t jx = 0; jx < nstrips; ++jx) {
             const int64_t s0 = 16 * jx;
             const int64_t span0 = hcol(hs[(size_t)(s0 / bands)]);
             if (jx == 0) w.assign((size_t)16 * 64 * kh, 0);
+            hstrip[(size_t)(2 * jx + 1)] = (int32_t)span0;
+            const int64_t xl = (s0 + 15) / bands;
+            const bool clear = hax.unit_step && !hax.phase && s0 + 15 < nsamp &&
+                               span0 == hs[(size_t)(s0 / bands)] - buf.x &&
+                               hcol(hs[(size_t)xl] + hax.taps - 1) == hs[(size_t)xl] + hax.taps - 1 - buf.x;
+            if (clear && hmemo[s0 % bands] >= 0) {
+                hstrip[(size_t)(2 * jx)] = hmemo[s0 % bands];
+                continue;
+            }
             PatternKey key;
             for (int n = 0; n < 16; ++n) {
                 if (s0 + n >= nsamp) {
@@ -339,7 +362,7 @@ bool plan_sep(int family, const SepAxis& hax, const SepAxis& vax, int32_t hbias,
                 emit_operands(w, kh, limbs, hbias, klo, khi, bytes);
                 clear_rows(&w, 64 * kh, klo, khi);
             });
-            hstrip[(size_t)(2 * jx + 1)] = (int32_t)span0;
+            if (clear) hmemo[s0 % bands] = hstrip[(size_t)(2 * jx)];
         }
         stg->h_tab = hp.put(st, &j->htab_bytes);
         stg->h_strip = st->put(hstrip.data(), hstrip.size() * sizeof(int32_t));
@@ -636,7 +659,7 @@ bool stage_gauss_sep_params(const Node& gh, const Node& gv, const Rect& mid, con
     std::vector<int32_t> hs((size_t)std::max<int64_t>(mid.w, 1)), vs((size_t)std::max<int64_t>(out.h, 1));
     for (int64_t x = 0; x < mid.w; ++x) hs[(size_t)x] = (int32_t)(mid.x + x - rad);
     for (int64_t y = 0; y < out.h; ++y) vs[(size_t)y] = (int32_t)(out.y + y - rad);
-    const SepAxis ha{&hs, nullptr, gh.itaps.data(), taps}, va{&vs, nullptr, gv.itaps.data(), taps};
+    const SepAxis ha{&hs, nullptr, gh.itaps.data(), taps, 1}, va{&vs, nullptr, gv.itaps.data(), taps, 1};
     const int32_t bias = gh.iscale / 2;
     if (!plan_sep(1, ha, va, bias, bias, gh.bands, out, clip, buf, vclip, opt, st, s)) return false;
     SepJob& j = s->sep;

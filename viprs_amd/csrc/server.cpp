@@ -259,8 +259,9 @@ void vgpu_server::plan_job(Job& j) {
 
 // Idle threads keep polling this long after their last piece of work before
 // they block: a thread woken through a condition variable costs tens of
-// microseconds of request latency.
-constexpr int64_t kSpinNs = 2000000;
+// microseconds of request latency (the scheduler and two planner threads
+// poll; the other planners block at once).
+constexpr int64_t kSpinNs = 10000000;
 
 void vgpu_server::planner_loop(int index) {
     (void)hipSetDevice(ctx->device);  // pinned staging allocations
