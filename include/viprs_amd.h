@@ -127,6 +127,10 @@ void vgpu_pipeline_destroy(vgpu_pipeline *p);
 /* PipelineBuilder::reduce_h / reduce_v (builder/resample.rs:16-123,270-339). */
 int vgpu_pipeline_reduce_h(vgpu_pipeline *p, double factor, int kernel);
 int vgpu_pipeline_reduce_v(vgpu_pipeline *p, double factor, int kernel);
+/* PipelineBuilder::reduce (builder/resample.rs:349-382): the fused f64
+ * ReduceOp (resample/reduce.rs:23-262), filters never bound to the input
+ * length (offset 0), H results rounded to the sample type before the V pass. */
+int vgpu_pipeline_reduce(vgpu_pipeline *p, double h_factor, double v_factor, int kernel);
 /* PipelineBuilder::shrink_h_with_ceil / shrink_v_with_ceil
  * (builder/resample.rs:133-209,392-444). */
 int vgpu_pipeline_shrink_h(vgpu_pipeline *p, uint32_t factor, int ceil_mode);

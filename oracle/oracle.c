@@ -331,7 +331,9 @@ enum {
     N_CONVSEP_V = 8,
     N_PREMULT = 9,
     N_UNPREMULT = 10,
-    N_COPY = 11
+    N_COPY = 11,
+    N_REDUCE_OP_H = 12, /* ReduceOp's horizontal pass (reduce.rs:104-133) */
+    N_REDUCE_OP_V = 13  /* ReduceOp's vertical pass (reduce.rs:136-166) */
 };
 
 typedef struct {
@@ -426,6 +428,32 @@ int orc_pipeline_reduce_v(orc_pipeline *p, double factor, int kernel) {
     nd->out_h = orc_reduce_output_len(factor, p->cur_h);
     nd->param = factor;
     commit(p, nd);
+    return ORC_OK;
+}
+
+/* PipelineBuilder::reduce (builder/resample.rs:349-382) -> ReduceOp::new
+ * (reduce.rs:51-67): validate_reduce_factors over both factors, then the
+ * kernel, then both tap limits (reduce_common.rs:26-44,60-67).  The filters
+ * are ReduceKernel::new without bind_input_len, so both axes have offset 0.
+ * The op is held as two nodes, H then V: the backward walk then gives the V
+ * node's input region = required_input_region_v(out) = ReduceOp's scratch
+ * region and the H node's input region = required_input_region_h(that),
+ * exactly ReduceOp::required_input_region (reduce.rs:181-184,224-250). */
+int orc_pipeline_reduce(orc_pipeline *p, double hfactor, double vfactor, int kernel) {
+    if (!isfinite(hfactor) || !isfinite(vfactor) || hfactor < 1.0 || vfactor < 1.0) return ORC_EINVAL;
+    if (!valid_reduce_kernel(kernel)) return ORC_EKERNEL;
+    if (orc_reduce_taps(kernel, hfactor) < 0 || orc_reduce_taps(kernel, vfactor) < 0) return ORC_EINVAL;
+    if (p->n + 2 > 32) return ORC_EINVAL;
+    node_t *nh = push(p, N_REDUCE_OP_H);
+    rk_init(&nh->rk, kernel, hfactor, 0);
+    nh->out_w = orc_reduce_output_len(hfactor, p->cur_w);
+    nh->param = hfactor;
+    commit(p, nh);
+    node_t *nv = push(p, N_REDUCE_OP_V);
+    rk_init(&nv->rk, kernel, vfactor, 0);
+    nv->out_h = orc_reduce_output_len(vfactor, p->cur_h);
+    nv->param = vfactor;
+    commit(p, nv);
     return ORC_OK;
 }
 
@@ -793,8 +821,10 @@ static rgn required_input(const node_t *nd, rgn o) {
         r.y = clampi(o.y * (int64_t)nd->factor, INT32_MIN, INT32_MAX);
         r.h = o.h * (int64_t)nd->factor;
         break;
-    case N_REDUCE_H: rk_required(&nd->rk, o.x, o.w, &r.x, &r.w); break;
-    case N_REDUCE_V: rk_required(&nd->rk, o.y, o.h, &r.y, &r.h); break;
+    case N_REDUCE_H:
+    case N_REDUCE_OP_H: rk_required(&nd->rk, o.x, o.w, &r.x, &r.w); break;
+    case N_REDUCE_V:
+    case N_REDUCE_OP_V: rk_required(&nd->rk, o.y, o.h, &r.y, &r.h); break;
     case N_GAUSS_H:
     case N_CONVSEP_H: { /* gauss_blur.rs:1801-1808; convsep.rs:65-72 */
         int64_t rad = (nd->ntaps - 1) / 2;
@@ -930,6 +960,47 @@ static void process_reduce_v(const node_t *nd, const void *in, rgn ir, void *out
                     }
                 }
                 ((float *)out)[oi] = (float)acc;
+            }
+        }
+    }
+}
+
+/* F::Sample::from_f64 (sample_conv.rs:58-93): integers clamp to the type's
+ * range, then round half away from zero; f32 is a plain cast. */
+static void store_from_f64(void *out, int fmt, size_t oi, double v) {
+    if (fmt == ORC_U8)
+        ((uint8_t *)out)[oi] = (uint8_t)round(clampd(v, 0.0, 255.0));
+    else if (fmt == ORC_U16)
+        ((uint16_t *)out)[oi] = (uint16_t)round(clampd(v, 0.0, 65535.0));
+    else
+        ((float *)out)[oi] = (float)v;
+}
+
+/* ReduceOp passes (reduce.rs:104-166): per output position the direct
+ * source position (offset 0), taps_for_f64 (reduce_common.rs:188-191), an
+ * f64 mul_add chain in tap order over clamp_axis'd input samples, then
+ * from_f64.  Same loop for H (vertical = 0) and V. */
+static void process_reduce_op(const node_t *nd, const void *in, rgn ir, void *out, rgn orr, int vertical) {
+    const rk_t *k = &nd->rk;
+    size_t B = nd->bands;
+    int T = k->taps;
+    for (int64_t y = 0; y < orr.h; ++y) {
+        for (int64_t x = 0; x < orr.w; ++x) {
+            int64_t start;
+            int phase;
+            rk_plan(k, rk_sp(k, vertical ? (double)orr.y + (double)y : (double)orr.x + (double)x), &start, &phase);
+            const double *w = k->cf + (size_t)phase * T;
+            for (size_t b = 0; b < B; ++b) {
+                double acc = 0.0;
+                for (int t = 0; t < T; ++t) {
+                    size_t idx;
+                    if (vertical)
+                        idx = ((size_t)clamp_axis(start + t, ir.y, ir.h) * (size_t)ir.w + (size_t)x) * B + b;
+                    else
+                        idx = ((size_t)y * (size_t)ir.w + (size_t)clamp_axis(start + t, ir.x, ir.w)) * B + b;
+                    acc = fma(ld(in, nd->in_fmt, idx), w[t], acc);
+                }
+                store_from_f64(out, nd->out_fmt, ((size_t)y * (size_t)orr.w + (size_t)x) * B + b, acc);
             }
         }
     }
@@ -1076,6 +1147,8 @@ static void process_node(const node_t *nd, const void *in, rgn ir, void *out, rg
     switch (nd->type) {
     case N_REDUCE_H: process_reduce_h(nd, in, ir, out, orr); break;
     case N_REDUCE_V: process_reduce_v(nd, in, ir, out, orr); break;
+    case N_REDUCE_OP_H: process_reduce_op(nd, in, ir, out, orr, 0); break;
+    case N_REDUCE_OP_V: process_reduce_op(nd, in, ir, out, orr, 1); break;
     case N_SHRINK_H: process_shrink_h(nd, in, ir, out, orr); break;
     case N_SHRINK_V: process_shrink_v(nd, in, ir, out, orr); break;
     case N_GAUSS_H:

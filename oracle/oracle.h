@@ -70,6 +70,7 @@ orc_pipeline *orc_pipeline_new(uint32_t width, uint32_t height, uint32_t bands, 
 void orc_pipeline_free(orc_pipeline *p);
 int orc_pipeline_reduce_h(orc_pipeline *p, double factor, int kernel);
 int orc_pipeline_reduce_v(orc_pipeline *p, double factor, int kernel);
+int orc_pipeline_reduce(orc_pipeline *p, double hfactor, double vfactor, int kernel);
 int orc_pipeline_shrink_h(orc_pipeline *p, uint32_t factor, int ceil_mode);
 int orc_pipeline_shrink_v(orc_pipeline *p, uint32_t factor, int ceil_mode);
 int orc_pipeline_shrink(orc_pipeline *p, uint32_t hshrink, uint32_t vshrink);

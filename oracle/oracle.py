@@ -54,6 +54,7 @@ def lib() -> C.CDLL:
         L.orc_pipeline_free.argtypes = [vp]
         L.orc_pipeline_reduce_h.argtypes = [vp, dbl, i32]
         L.orc_pipeline_reduce_v.argtypes = [vp, dbl, i32]
+        L.orc_pipeline_reduce.argtypes = [vp, dbl, dbl, i32]
         L.orc_pipeline_shrink_h.argtypes = [vp, u32, i32]
         L.orc_pipeline_shrink_v.argtypes = [vp, u32, i32]
         L.orc_pipeline_shrink.argtypes = [vp, u32, u32]
@@ -139,6 +140,11 @@ class Pipeline:
 
     def reduce_v(self, factor, kernel=LANCZOS3):
         _check(lib().orc_pipeline_reduce_v(self._p, factor, kernel), "reduce_v")
+        return self
+
+    def reduce(self, hfactor, vfactor, kernel=LANCZOS3):
+        """PipelineBuilder::reduce: the fused f64 ReduceOp (reduce.rs:104-166)."""
+        _check(lib().orc_pipeline_reduce(self._p, hfactor, vfactor, kernel), "reduce")
         return self
 
     def shrink_h(self, factor, ceil=False):

@@ -209,6 +209,71 @@ def test_reduce_stage_forms(ctx, fmt, bands):
         assert_same(got, exp, f"{np.dtype(fmt)} bands {bands} {steps}")
 
 
+def reduce_op_source(fmt, h, w, b, seed):
+    if fmt == np.float32:
+        return float_image(h, w, b, seed)
+    if fmt == np.uint16:
+        return noise_u16(h, w, b, seed)
+    return noise_u8(h, w, b, seed)
+
+
+@pytest.mark.parametrize("fmt", [np.uint8, np.uint16, np.float32])
+@pytest.mark.parametrize("bands", [1, 2, 3, 4, 5])
+def test_reduce_op(ctx, fmt, bands):
+    # PipelineBuilder::reduce: one fused f64 stage for <= 4 bands, the two
+    # per-pass kernels for 5.
+    src = reduce_op_source(fmt, 141, 203, bands, 60 + bands)
+    for hf, vf, k in ((1.5, 2.5, v.LANCZOS3), (2.0, 2.0, v.BILINEAR), (1.0, 3.3, v.BICUBIC),
+                      (4.7, 1.0, v.LANCZOS2), (3.0, 3.0, v.NEAREST), (7.9, 6.1, v.CATMULLROM)):
+        steps = [("reduce", (hf, vf, k))]
+        got, exp = run_both(ctx, src, steps)
+        assert_same(got, exp, f"{np.dtype(fmt)} bands {bands} {steps}")
+    h, w, b = src.shape
+    p = v.Pipeline(ctx, w, h, b, FMT[src.dtype]).reduce(1.5, 2.5, v.LANCZOS3)
+    assert p.stage_kernels() == (["reduce_hv"] if bands <= 4 else ["reduce_h", "reduce_v"])
+
+
+def test_reduce_op_reference_integration_case(ctx):
+    # spatial_integration.rs:199-232: 777x333x3, sample (i * 31 + 11) % 256,
+    # reduce(1.5, 2.5, Lanczos3).
+    src = ((np.arange(777 * 333 * 3) * 31 + 11) % 256).astype(np.uint8).reshape(333, 777, 3)
+    got, exp = run_both(ctx, src, [("reduce", (1.5, 2.5, v.LANCZOS3))])
+    assert got.shape == (133, 518, 3)
+    assert_same(got, exp, "integration case")
+
+
+def test_reduce_op_in_chains_and_fixtures(ctx):
+    rgb4 = np.array([0, 10, 20, 30, 45, 60, 90, 60, 30, 120, 90, 40, 15, 35, 55, 45, 75, 105, 95, 65,
+                     35, 125, 95, 45, 25, 50, 75, 55, 85, 115, 105, 70, 40, 135, 100, 50, 35, 60, 85,
+                     65, 95, 125, 115, 80, 45, 145, 105, 55], np.uint8).reshape(4, 4, 3)
+    got, exp = run_both(ctx, rgb4, [("reduce", (2.0, 2.0, v.BILINEAR))])
+    assert_same(got, exp, "reduce.rs 4x4x3 fixture")
+    src = noise_u8(300, 401, 3, 77)
+    for steps in ([("shrink", (2, 2)), ("reduce", (1.7, 1.3, v.LANCZOS3)), ("gauss_blur", (0.8,))],
+                  [("reduce", (2.2, 2.2, v.LANCZOS3)), ("reduce_h", (1.5, v.BICUBIC))],
+                  [("gauss_blur", (1.5,)), ("reduce", (3.0, 1.5, v.BILINEAR))],
+                  [("reduce", (1.5, 1.5, v.LANCZOS3)), ("reduce", (1.5, 1.5, v.LANCZOS3))]):
+        got, exp = run_both(ctx, src, steps)
+        assert_same(got, exp, str(steps))
+    got, exp = run_both(ctx, noise_u8(1, 1, 4, 3), [("reduce", (2.0, 2.0, v.LANCZOS3))])
+    assert_same(got, exp, "1x1")
+
+
+def test_reduce_op_validation(ctx):
+    with pytest.raises(v.VgpuError) as e:
+        v.Pipeline(ctx, 8, 8, 1).reduce(0.5, 2.0, v.BILINEAR)
+    assert e.value.status == v.api.E_INVALID_ARGUMENT
+    with pytest.raises(v.VgpuError) as e:
+        v.Pipeline(ctx, 8, 8, 1).reduce(0.5, 2.0, v.LBB)  # factors are checked first
+    assert e.value.status == v.api.E_INVALID_ARGUMENT
+    with pytest.raises(v.VgpuError) as e:
+        v.Pipeline(ctx, 8, 8, 1).reduce(2.0, 2.0, v.NOHALO)
+    assert e.value.status == v.api.E_INVALID_KERNEL
+    with pytest.raises(v.VgpuError) as e:
+        v.Pipeline(ctx, 8, 8, 1).reduce(2.0, 400.0, v.LANCZOS3)
+    assert e.value.status == v.api.E_INVALID_ARGUMENT
+
+
 @pytest.mark.parametrize("bands", [2, 3, 4])
 def test_alpha_fusion_forms(ctx, bands):
     # Premultiply/Unpremultiply fuse into the shrink and reduce stages for 2

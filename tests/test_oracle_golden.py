@@ -343,3 +343,146 @@ def test_synthetic_bytes_known_answers():
     assert [int(x) for x in w] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
     # Ragged lengths are prefixes of the same stream.
     assert np.array_equal(O.synthetic_bytes(13, 77), O.synthetic_bytes(16, 77)[:13])
+
+
+# ------------------------------------------------------------- ReduceOp
+
+# The reference's ReduceOp fixtures (reduce.rs:535-578): 4x4x3 at factor 2 and
+# 5x4x1 at 1.5 x 2.0, both Bilinear.
+REDUCE_OP_RGB4 = np.array([
+    0, 10, 20, 30, 45, 60, 90, 60, 30, 120, 90, 40, 15, 35, 55, 45, 75, 105, 95, 65,
+    35, 125, 95, 45, 25, 50, 75, 55, 85, 115, 105, 70, 40, 135, 100, 50, 35, 60, 85,
+    65, 95, 125, 115, 80, 45, 145, 105, 55], np.uint8).reshape(4, 4, 3)
+REDUCE_OP_GRAY5x4 = np.array([
+    0, 20, 80, 120, 200, 15, 35, 95, 135, 215, 25, 55, 110, 150, 230, 40, 70, 125, 165,
+    245], np.uint8).reshape(4, 5, 1)
+
+# Bilinear weights with offset 0 (no bind_input_len), derived by hand from the
+# table convention of reduce_common.rs:279-316: output i reads source
+# positions p -> weight.  Factor 2: sp = 2i + 0.5, phase 32, a [1,2,1]/4 filter
+# centred on 2i.  Factor 1.5: sp = 1.5i + 0.25, phases 16/48 alternate:
+# {-1: 1/3, 0: 5/9, 1: 1/9} around floor(sp) for phase 16, {-1: 1/9, 0: 5/9,
+# 1: 1/3} for phase 48.
+def _bilinear_taps(factor, i):
+    F = Fraction
+    if factor == 2.0:
+        return {2 * i - 1: F(1, 4), 2 * i: F(1, 2), 2 * i + 1: F(1, 4)}
+    assert factor == 1.5
+    fl = (3 * i) // 2
+    w = (F(1, 3), F(5, 9), F(1, 9)) if i % 2 == 0 else (F(1, 9), F(5, 9), F(1, 3))
+    return {fl - 1: w[0], fl: w[1], fl + 1: w[2]}
+
+
+def _round_half_away(q):
+    # from_f64 on u8: clamp, then f64::round (half away from zero).
+    q = min(max(q, Fraction(0)), Fraction(255))
+    return int(q + Fraction(1, 2)) if q >= 0 else -int(-q + Fraction(1, 2))
+
+
+def _reduce_op_closed_form(img, hf, vf):
+    h, w, b = img.shape
+    ow, oh = int(np.floor(w / hf + 0.5)), int(np.floor(h / vf + 0.5))
+    # H pass over every source row the V pass reads (clamp to edge), rounded
+    # to u8 as ReduceOp's scratch is.
+    rows = {}
+    def hrow(y):
+        yy = min(max(y, 0), h - 1)
+        if yy not in rows:
+            rows[yy] = [[_round_half_away(sum(wt * int(img[yy, min(max(p, 0), w - 1), c])
+                                              for p, wt in _bilinear_taps(hf, x).items()))
+                         for c in range(b)] for x in range(ow)]
+        return rows[yy]
+    out = np.zeros((oh, ow, b), np.uint8)
+    for y in range(oh):
+        for x in range(ow):
+            for c in range(b):
+                out[y, x, c] = _round_half_away(sum(wt * hrow(p)[x][c] for p, wt in _bilinear_taps(vf, y).items()))
+    return out
+
+
+def test_reduce_op_identity_and_dims():
+    # factor_one_is_identity, factor_two_halves_dimensions (reduce.rs:483-496)
+    img = np.arange(12, dtype=np.uint8).reshape(3, 4, 1)
+    assert np.array_equal(O.Pipeline(4, 3, 1).reduce(1.0, 1.0, O.BILINEAR).run(img), img)
+    z = np.zeros((6, 8, 1), np.uint8)
+    assert O.Pipeline(8, 6, 1).reduce(2.0, 2.0, O.BILINEAR).run(z).shape == (3, 4, 1)
+
+
+@pytest.mark.parametrize("img,hf,vf,shape", [
+    (REDUCE_OP_RGB4, 2.0, 2.0, (2, 2, 3)),
+    (REDUCE_OP_GRAY5x4, 1.5, 2.0, (2, 3, 1)),
+])
+def test_reduce_op_reference_fixtures_closed_form(img, hf, vf, shape):
+    h, w, b = img.shape
+    got = O.Pipeline(w, h, b).reduce(hf, vf, O.BILINEAR).run(img)
+    assert got.shape == shape
+    assert np.array_equal(got, _reduce_op_closed_form(img, hf, vf))
+
+
+@pytest.mark.parametrize("w,h,b,hf,vf,seed", [
+    (2, 2, 1, 1.5, 2.0, 1), (5, 6, 3, 2.0, 1.5, 2), (6, 3, 2, 1.5, 1.5, 3), (37, 23, 3, 2.0, 2.0, 4),
+    (40, 41, 1, 1.5, 2.0, 5),
+])
+def test_reduce_op_bilinear_closed_form_random(w, h, b, hf, vf, seed):
+    # The proptest of reduce.rs:603-619 draws 2..6 x 2..6 x 1..3 images at
+    # factors {1.5, 2, 3}; the exact-rational closed form pins borders, the
+    # offset-0 phase, u8 scratch rounding and half-away rounding.
+    img = noise_u8(h, w, b, seed)
+    got = O.Pipeline(w, h, b).reduce(hf, vf, O.BILINEAR).run(img)
+    assert np.array_equal(got, _reduce_op_closed_form(img, hf, vf))
+
+
+def test_reduce_op_ties_round_half_away():
+    # Two-pixel rows at factor 2 Bilinear: out = (x0 + 2 x0 + x1) / 4 with the
+    # left edge clamped; x1 = x0 + 2 makes every H output an exact .5 tie.
+    img = np.array([[[10], [12]], [[10], [12]]], np.uint8)  # 2x2
+    got = O.Pipeline(2, 2, 1).reduce(2.0, 2.0, O.BILINEAR).run(img)
+    assert got.shape == (1, 1, 1) and int(got[0, 0, 0]) == 11  # 10.5 -> 11
+
+
+@pytest.mark.parametrize("kernel", [O.LANCZOS3, O.BICUBIC, O.BILINEAR])
+@pytest.mark.parametrize("fmt", [O.U8, O.U16])
+def test_reduce_op_uniform_stays_uniform(kernel, fmt):
+    dt = np.uint8 if fmt == O.U8 else np.uint16
+    v = 200 if fmt == O.U8 else 51234
+    img = np.full((19, 23, 3), v, dt)
+    got = O.Pipeline(23, 19, 3, fmt).reduce(1.7, 2.3, kernel).run(img)
+    assert got.shape == (8, 14, 3) and (got == v).all()
+
+
+def test_reduce_op_f32_matches_exact_separable_sum():
+    # f32: no rounding between the passes beyond the f32 scratch; against an
+    # FP64 separable sum with the oracle's own (pinned) tables, within f32
+    # rounding of the scratch and the output.
+    r = np.random.default_rng(9)
+    img = r.random((17, 21, 2), dtype=np.float32)
+    got = O.Pipeline(21, 17, 2, O.F32).reduce(1.5, 2.0, O.BILINEAR).run(img)
+    exp = _reduce_op_closed_form_f(img.astype(np.float64), 1.5, 2.0)
+    assert got.shape == exp.shape
+    assert np.max(np.abs(got - exp)) < 2e-7
+
+
+def _reduce_op_closed_form_f(img, hf, vf):
+    h, w, b = img.shape
+    ow, oh = int(np.floor(w / hf + 0.5)), int(np.floor(h / vf + 0.5))
+    mid = np.zeros((h, ow, b))
+    for x in range(ow):
+        for p, wt in _bilinear_taps(hf, x).items():
+            mid[:, x] += float(wt) * img[:, min(max(p, 0), w - 1)]
+    mid = mid.astype(np.float32).astype(np.float64)
+    out = np.zeros((oh, ow, b))
+    for y in range(oh):
+        for p, wt in _bilinear_taps(vf, y).items():
+            out[y] += float(wt) * mid[min(max(p, 0), h - 1)]
+    return out
+
+
+def test_reduce_op_rejections():
+    p = O.Pipeline(8, 8, 1)
+    with pytest.raises(O.OracleError):
+        p.reduce(0.5, 2.0, O.BILINEAR)
+    with pytest.raises(O.OracleError):
+        O.Pipeline(8, 8, 1).reduce(2.0, float("inf"), O.BILINEAR)
+    with pytest.raises(O.OracleError) as e:
+        O.Pipeline(8, 8, 1).reduce(2.0, 2.0, 99)
+    assert e.value.code == O.EKERNEL

@@ -114,6 +114,7 @@ def load() -> C.CDLL:
         "vgpu_pipeline_destroy": (None, [vp]),
         "vgpu_pipeline_reduce_h": (i32, [vp, dbl, i32]),
         "vgpu_pipeline_reduce_v": (i32, [vp, dbl, i32]),
+        "vgpu_pipeline_reduce": (i32, [vp, dbl, dbl, i32]),
         "vgpu_pipeline_shrink_h": (i32, [vp, u32, i32]),
         "vgpu_pipeline_shrink_v": (i32, [vp, u32, i32]),
         "vgpu_pipeline_shrink": (i32, [vp, u32, u32]),

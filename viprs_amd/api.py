@@ -198,6 +198,12 @@ class Pipeline:
         _check(self._lib.vgpu_pipeline_reduce_v(self._h, factor, kernel), "reduce_v")
         return self
 
+    def reduce(self, h_factor: float, v_factor: float, kernel: int = LANCZOS3):
+        """PipelineBuilder::reduce (builder/resample.rs:349-382): the fused
+        f64 ReduceOp, filters at offset 0."""
+        _check(self._lib.vgpu_pipeline_reduce(self._h, h_factor, v_factor, kernel), "reduce")
+        return self
+
     def shrink_h(self, factor: int, ceil: bool = False):
         _check(self._lib.vgpu_pipeline_shrink_h(self._h, factor, int(ceil)), "shrink_h")
         return self

@@ -557,11 +557,34 @@ __device__ __forceinline__ T reduce_v_sample(Get get, int start, int ph, const D
     }
 }
 
+// F::Sample::from_f64 (sample_conv.rs:58-93): integer samples clamp to the
+// type's range, then round half away from zero; f32 is a cast.
+template <typename T>
+__device__ __forceinline__ T op_from_f64(double v) {
+    if constexpr (sizeof(T) == 4) {
+        return (T)(float)v;
+    } else {
+        const double mx = sizeof(T) == 1 ? 255.0 : 65535.0;
+        return (T)round(fmin(fmax(v, 0.0), mx));
+    }
+}
+
+// One ReduceOp pass sample (reduce.rs:104-166): acc = fma(x_t, w_t, acc) in
+// tap order over the f64 phase row, from acc = 0.
+template <typename T, typename Get>
+__device__ __forceinline__ T reduce_op_sample(Get get, int start, int ph, const DevPlan& p) {
+    const double* w = p.wf64 + (size_t)ph * p.taps;
+    double acc = 0.0;
+    for (int t = 0; t < p.taps; ++t) acc = fma((double)get(start + t), w[t], acc);
+    return op_from_f64<T>(acc);
+}
+
 // ReduceH (reduceh.rs:101-163; reduce_simd.rs:47-90,2463-2693): fixed-point
 // i16 taps for integer formats -> (acc + 2048) >> 12 clamped; f32 carries FP64:
 // bands == 1 follows the AVX2 4-lane grouping (products summed per group, then
 // an FMA tail), other band counts the scalar FMA chain.
-template <typename T>
+// OP = true: ReduceOp's horizontal pass instead (reduce.rs:104-133).
+template <typename T, bool OP>
 __global__ __launch_bounds__(kThreads) void k_reduce_h(InView in, OutView out, int B, DevPlan p) {
     const size_t n = (size_t)out.w * out.h * B;
     for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
@@ -569,7 +592,12 @@ __global__ __launch_bounds__(kThreads) void k_reduce_h(InView in, OutView out, i
         const size_t px = i / B;
         const int xo = (int)(px % out.w);
         const int row = row_of(in, out.y + (int)(px / out.w));
-        static_cast<T*>(out.ptr)[i] = reduce_h_sample<T>(in, row, B, b, p.start[xo], p.phase[xo], p);
+        if constexpr (OP) {
+            auto get = [&](int x) -> T { return load_at<T>(in, row, col_of(in, x), B, b); };
+            static_cast<T*>(out.ptr)[i] = reduce_op_sample<T>(get, p.start[xo], p.phase[xo], p);
+        } else {
+            static_cast<T*>(out.ptr)[i] = reduce_h_sample<T>(in, row, B, b, p.start[xo], p.phase[xo], p);
+        }
     }
 }
 
@@ -577,7 +605,8 @@ __global__ __launch_bounds__(kThreads) void k_reduce_h(InView in, OutView out, i
 // (x86 AVX2 recomputes the source position per row); f32 rows use separate
 // multiply/add for the first (row_len / 4) * 4 samples and an FMA chain for
 // the tail, as the AVX2 kernel does.
-template <typename T>
+// OP = true: ReduceOp's vertical pass instead (reduce.rs:136-166).
+template <typename T, bool OP>
 __global__ __launch_bounds__(kThreads) void k_reduce_v(InView in, OutView out, int B, DevPlan p) {
     const int row_len = out.w * B;
     const int vec_len = row_len / 4 * 4;
@@ -588,7 +617,10 @@ __global__ __launch_bounds__(kThreads) void k_reduce_v(InView in, OutView out, i
         const int b = s % B;
         const int col = col_of(in, out.x + s / B);
         auto get = [&](int y) -> T { return load_at<T>(in, row_of(in, y), col, B, b); };
-        static_cast<T*>(out.ptr)[i] = reduce_v_sample<T>(get, p.start[yo], p.phase[yo], p, s < vec_len);
+        if constexpr (OP)
+            static_cast<T*>(out.ptr)[i] = reduce_op_sample<T>(get, p.start[yo], p.phase[yo], p);
+        else
+            static_cast<T*>(out.ptr)[i] = reduce_v_sample<T>(get, p.start[yo], p.phase[yo], p, s < vec_len);
     }
 }
 
@@ -708,8 +740,79 @@ __device__ __forceinline__ void reduce_hv_block(const ReduceJob& job, int bx, in
     }
 }
 
+// ReduceOp (reduce.rs:224-250) on the fused tile: the H pass into LDS over
+// the rows the tile's V taps need, rounded by from_f64 as the reference's
+// scratch is, then the V pass from LDS; both f64 mul_add chains.
+template <typename T, int B>
+__device__ __forceinline__ void reduce_op_block(const ReduceJob& job, int bx, int by, unsigned char* smem) {
+    const InView& in = job.in;
+    const OutView& out = job.out;
+    const DevPlan& ph = job.ph;
+    const DevPlan& pv = job.pv;
+    const int TX = job.tx, TY = job.ty;
+    T* hbuf = reinterpret_cast<T*>(smem);
+    const int tx0 = bx * TX, ty0 = by * TY;
+    const int ncols = min(TX, out.w - tx0), nrows = min(TY, out.h - ty0);
+    const int ys0 = pv.start[ty0];
+    const int nr = min(job.nr_max, pv.start[ty0 + nrows - 1] + pv.taps - ys0);
+    const int x = threadIdx.x % TX, rl = threadIdx.x / TX, nrl = kThreads / TX;
+    const int pitch = TX * B;
+    if (x < ncols) {
+        const int xo = tx0 + x;
+        const int hs = ph.start[xo];
+        const double* w = ph.wf64 + (size_t)ph.phase[xo] * ph.taps;
+        for (int r = rl; r < nr; r += nrl) {
+            const T* src = static_cast<const T*>(in.ptr) + (size_t)row_of(in, ys0 + r) * in.bw * B;
+            double acc[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) acc[b] = 0.0;
+            for (int t = 0; t < ph.taps; ++t) {
+                const T* px = src + (size_t)col_of(in, hs + t) * B;
+                const double wt = w[t];
+#pragma unroll
+                for (int b = 0; b < B; ++b) acc[b] = fma((double)px[b], wt, acc[b]);
+            }
+            T* dst = hbuf + r * pitch + x * B;
+#pragma unroll
+            for (int b = 0; b < B; ++b) dst[b] = op_from_f64<T>(acc[b]);
+        }
+    }
+    __syncthreads();
+    const int vclip0 = job.vclip0, vclip1 = job.vclip1;
+    for (int ry = rl; x < ncols && ry < nrows; ry += nrl) {
+        const int yo = ty0 + ry;
+        const int vs = pv.start[yo];
+        const double* w = pv.wf64 + (size_t)pv.phase[yo] * pv.taps;
+        double acc[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) acc[b] = 0.0;
+        for (int t = 0; t < pv.taps; ++t) {
+            const int yy = clampi(clampi(vs + t, vclip0, vclip1) - ys0, 0, nr - 1);
+            const T* hp = hbuf + yy * pitch + x * B;
+            const double wt = w[t];
+#pragma unroll
+            for (int b = 0; b < B; ++b) acc[b] = fma((double)hp[b], wt, acc[b]);
+        }
+        T* o = static_cast<T*>(out.ptr) + (size_t)yo * out.w * B + (size_t)(tx0 + x) * B;
+#pragma unroll
+        for (int b = 0; b < B; ++b) o[b] = op_from_f64<T>(acc[b]);
+    }
+}
+
 __device__ __forceinline__ void reduce_job_block(const ReduceJob& job, int bx, int by,
                                                  unsigned char* smem) {
+    if (job.op) {
+#define VGPU_OB(TT)                                            \
+    switch (job.bands) {                                       \
+    case 1: reduce_op_block<TT, 1>(job, bx, by, smem); break;  \
+    case 2: reduce_op_block<TT, 2>(job, bx, by, smem); break;  \
+    case 3: reduce_op_block<TT, 3>(job, bx, by, smem); break;  \
+    default: reduce_op_block<TT, 4>(job, bx, by, smem); break; \
+    }
+        if (job.fmt == 0) { VGPU_OB(uint8_t) } else if (job.fmt == 1) { VGPU_OB(uint16_t) } else { VGPU_OB(float) }
+#undef VGPU_OB
+        return;
+    }
 #define VGPU_RB(TT, AA)                                            \
     switch (job.bands) {                                           \
     case 1: reduce_hv_block<TT, 1, AA>(job, bx, by, smem); break;  \
@@ -977,25 +1080,30 @@ hipError_t launch_shrink_generic(const InView& in, const OutView& out, int bands
     return hipGetLastError();
 }
 
+// op = 1: ReduceOp's passes (f64 mul_add chains, from_f64 rounding).
 hipError_t launch_reduce_h(const InView& in, const OutView& out, int bands, int fmt,
-                           const DevPlan& plan, int, hipStream_t s) {
+                           const DevPlan& plan, int op, hipStream_t s) {
     const size_t n = (size_t)out.w * out.h * bands;
     if (n == 0) return hipSuccess;
     const unsigned g = grid_for(n);
-    if (fmt == 0) hipLaunchKernelGGL(k_reduce_h<uint8_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
-    else if (fmt == 1) hipLaunchKernelGGL(k_reduce_h<uint16_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
-    else hipLaunchKernelGGL(k_reduce_h<float>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
+#define VGPU_RP(TT)                                                                                          \
+    if (op) hipLaunchKernelGGL((k_reduce_h<TT, true>), dim3(g), dim3(kThreads), 0, s, in, out, bands, plan); \
+    else hipLaunchKernelGGL((k_reduce_h<TT, false>), dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
+    if (fmt == 0) { VGPU_RP(uint8_t) } else if (fmt == 1) { VGPU_RP(uint16_t) } else { VGPU_RP(float) }
+#undef VGPU_RP
     return hipGetLastError();
 }
 
 hipError_t launch_reduce_v(const InView& in, const OutView& out, int bands, int fmt,
-                           const DevPlan& plan, int, hipStream_t s) {
+                           const DevPlan& plan, int op, hipStream_t s) {
     const size_t n = (size_t)out.w * out.h * bands;
     if (n == 0) return hipSuccess;
     const unsigned g = grid_for(n);
-    if (fmt == 0) hipLaunchKernelGGL(k_reduce_v<uint8_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
-    else if (fmt == 1) hipLaunchKernelGGL(k_reduce_v<uint16_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
-    else hipLaunchKernelGGL(k_reduce_v<float>, dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
+#define VGPU_RP(TT)                                                                                          \
+    if (op) hipLaunchKernelGGL((k_reduce_v<TT, true>), dim3(g), dim3(kThreads), 0, s, in, out, bands, plan); \
+    else hipLaunchKernelGGL((k_reduce_v<TT, false>), dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
+    if (fmt == 0) { VGPU_RP(uint8_t) } else if (fmt == 1) { VGPU_RP(uint16_t) } else { VGPU_RP(float) }
+#undef VGPU_RP
     return hipGetLastError();
 }
 

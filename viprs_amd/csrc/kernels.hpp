@@ -66,6 +66,7 @@ struct ReduceJob {
     int vclip0 = 0, vclip1 = 0, wide_acc = 0, premult = 0, unpremult = 0;
     int gx = 0, gy = 0;
     int src_slot = -1, dst_slot = -1;
+    int op = 0;  // 1: ReduceOp arithmetic (f64 mul_add chains over wf64, from_f64 rounding)
 };
 
 // Separable u8 pass pair on MFMA (int8 limbs): H then V with a u8
@@ -110,10 +111,11 @@ hipError_t launch_shrink_hv_u8(const InView& in, const OutView& out, int bands, 
                                int premult, hipStream_t s);
 hipError_t launch_shrink_generic(const InView& in, const OutView& out, int bands, int fmt,
                                  int factor, int vertical, hipStream_t s);
+// op = 1: ReduceOp's passes instead of ReduceH / ReduceV.
 hipError_t launch_reduce_h(const InView& in, const OutView& out, int bands, int fmt,
-                           const DevPlan& plan, int post_alpha, hipStream_t s);
+                           const DevPlan& plan, int op, hipStream_t s);
 hipError_t launch_reduce_v(const InView& in, const OutView& out, int bands, int fmt,
-                           const DevPlan& plan, int post_alpha, hipStream_t s);
+                           const DevPlan& plan, int op, hipStream_t s);
 bool plan_shrink_job(ShrinkJob* job, bool row_split = true, int run = 0);
 size_t shrink_job_lds();
 hipError_t launch_shrink_job(const ShrinkJob& job, hipStream_t s);

@@ -216,8 +216,10 @@ Rect Node::required_input(const Rect& o) const {
         r.y = clamp_i32_range(o.y * (int64_t)factor);
         r.h = o.h * (int64_t)factor;
         break;
-    case N_REDUCE_H: axis.required(o.x, o.w, &r.x, &r.w); break;
-    case N_REDUCE_V: axis.required(o.y, o.h, &r.y, &r.h); break;
+    case N_REDUCE_H:
+    case N_REDUCE_OP_H: axis.required(o.x, o.w, &r.x, &r.w); break;
+    case N_REDUCE_V:
+    case N_REDUCE_OP_V: axis.required(o.y, o.h, &r.y, &r.h); break;
     case N_GAUSS_H:
     case N_CONVSEP_H:
         r.x = o.x - radius();

@@ -93,6 +93,11 @@ enum NodeKind : int {
     N_PREMULT = 9,
     N_UNPREMULT = 10,
     N_COPY = 11,
+    // ReduceOp (reduce.rs:23-262) as two nodes, H then V, both axes with
+    // offset 0: the backward walk then yields ReduceOp's scratch region
+    // (required_input_region_v of the output) as the V node's input region.
+    N_REDUCE_OP_H = 12,
+    N_REDUCE_OP_V = 13,
 };
 
 struct Node {

@@ -376,8 +376,8 @@ bool plan_sep(int family, const SepAxis& hax, const SepAxis& vax, int32_t hbias,
 void stage_reduce_params(const Node& nd, const Rect& out, Staging* st, Stage* s) {
     std::vector<int32_t> starts;
     std::vector<uint8_t> phases;
-    if (nd.kind == N_REDUCE_H)
-        build_plan(nd.axis, out.x, out.w, nd.in_fmt == U8, &starts, &phases);
+    if (nd.kind == N_REDUCE_H || nd.kind == N_REDUCE_OP_H)
+        build_plan(nd.axis, out.x, out.w, nd.kind == N_REDUCE_H && nd.in_fmt == U8, &starts, &phases);
     else
         build_plan(nd.axis, out.y, out.h, false, &starts, &phases);
     s->plan_start = st->put(starts.data(), starts.size() * sizeof(int32_t));
@@ -586,13 +586,13 @@ bool plan_f16(const F16Axis& hax, const F16Axis& vax, uint32_t bands, const Rect
 bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const ReduceAxis& vax,
                             int fmt, uint32_t bands, const Rect& mid, const Rect& out,
                             const Rect& vclip, const Rect& clip, const Rect& buf,
-                            const StageOptions& opt, Staging* st, Stage* s) {
+                            const StageOptions& opt, Staging* st, Stage* s, bool reduce_op = false) {
     std::vector<int32_t> hs, vs;
     std::vector<uint8_t> hp, vp;
     build_plan(hax, mid.x, mid.w, accumulate_h, &hs, &hp);
     build_plan(vax, out.y, out.h, false, &vs, &vp);
     s->kind = ST_REDUCE_HV;  // replaced below by ST_SEP, ST_RED_F32 or ST_SEP_F16
-    if (fmt == U8 && opt.integer_mfma) {
+    if (fmt == U8 && opt.integer_mfma && !reduce_op) {
         const SepAxis ha{&hs, &hp, hax.wi16.data(), hax.taps}, va{&vs, &vp, vax.wi16.data(), vax.taps};
         plan_sep(0, ha, va, 2048, 2048, bands, out, clip, buf, vclip, opt, st, s);  // ST_SEP on success
     }
@@ -633,7 +633,8 @@ bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const Redu
             worst = std::max(worst, sabs * maxv);
         }
     j.wide_acc = worst >= (int64_t)INT32_MAX - 4096;
-    if (fmt == F32 && opt.float_mode >= 1 && tile_ok) {
+    j.op = reduce_op ? 1 : 0;
+    if (fmt == F32 && opt.float_mode >= 1 && tile_ok && !reduce_op) {
         s->w32 = put_f32(hax.wf64, st);
         s->v_w32 = put_f32(vax.wf64, st);
         s->kind = ST_RED_F32;
@@ -839,8 +840,8 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
         p.wi16 = reinterpret_cast<const int16_t*>(blob + s.w16);
         p.wf64 = reinterpret_cast<const double*>(blob + s.w64);
         p.taps = s.taps;
-        e = s.kind == ST_REDUCE_H ? launch_reduce_h(in, out, B, nd.in_fmt, p, 0, stream)
-                                  : launch_reduce_v(in, out, B, nd.in_fmt, p, 0, stream);
+        e = s.kind == ST_REDUCE_H ? launch_reduce_h(in, out, B, nd.in_fmt, p, s.reduce_op, stream)
+                                  : launch_reduce_v(in, out, B, nd.in_fmt, p, s.reduce_op, stream);
         break;
     }
     case ST_REDUCE_HV: e = launch_reduce_job(reduce_job_of(s, in, out, blob), stream); break;
@@ -955,6 +956,16 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
                 done = true;
             }
         }
+        if (!done && small_bands && nd.kind == N_REDUCE_OP_H && kind_at(i + 1) == N_REDUCE_OP_V) {
+            // ReduceOp: both f64 passes on one tile, the H results kept in
+            // LDS in the sample type as the reference's scratch holds them.
+            if (stage_reduce_hv_params(nd.axis, false, nodes[(size_t)i + 1].axis, nd.in_fmt, nd.bands,
+                                       out_r[(size_t)i], out_r[(size_t)i + 1], in_r[(size_t)i + 1], s.clip, buf,
+                                       opt, st, &s, true)) {
+                s.last = i + 1;
+                done = true;
+            }
+        }
         if (!done && u8small && opt.integer_mfma && nd.kind == N_GAUSS_H && kind_at(i + 1) == N_GAUSS_V &&
             nd.out_fmt == U8 && nodes[(size_t)i + 1].out_fmt == U8) {
             // GaussBlurH -> GaussBlurV (u8 -> u8) on the MFMA kernel.
@@ -980,6 +991,14 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
             case N_SHRINK_V: s.kind = ST_SHRINK; break;
             case N_REDUCE_H: s.kind = ST_REDUCE_H; break;
             case N_REDUCE_V: s.kind = ST_REDUCE_V; break;
+            case N_REDUCE_OP_H:
+                s.kind = ST_REDUCE_H;
+                s.reduce_op = 1;
+                break;
+            case N_REDUCE_OP_V:
+                s.kind = ST_REDUCE_V;
+                s.reduce_op = 1;
+                break;
             case N_GAUSS_H:
             case N_GAUSS_V:
             case N_CONVSEP_H:
@@ -1199,6 +1218,28 @@ int vgpu_pipeline_reduce_v(vgpu_pipeline* p, double factor, int kernel) {
     Node& nd = p->push(N_REDUCE_V);
     nd.axis = std::move(ax);
     nd.out_h = reduce_output_len(factor, p->cur_h);
+    p->commit();
+    return VGPU_OK;
+}
+
+// PipelineBuilder::reduce (builder/resample.rs:349-382) -> ReduceOp::new
+// (reduce.rs:51-67): both factors validated first, then the kernel, then each
+// axis's tap limit; the filters are never bound to the input length.
+int vgpu_pipeline_reduce(vgpu_pipeline* p, double hfactor, double vfactor, int kernel) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    if (!std::isfinite(hfactor) || !std::isfinite(vfactor) || hfactor < 1.0 || vfactor < 1.0)
+        return VGPU_E_INVALID_ARGUMENT;
+    ReduceAxis hax, vax;
+    int st = build_reduce_axis(kernel, hfactor, 0, &hax);
+    if (st == OK) st = build_reduce_axis(kernel, vfactor, 0, &vax);
+    if (st != OK) return st;
+    Node& nh = p->push(N_REDUCE_OP_H);
+    nh.axis = std::move(hax);
+    nh.out_w = reduce_output_len(hfactor, p->cur_w);
+    p->commit();
+    Node& nv = p->push(N_REDUCE_OP_V);
+    nv.axis = std::move(vax);
+    nv.out_h = reduce_output_len(vfactor, p->cur_h);
     p->commit();
     return VGPU_OK;
 }

@@ -129,6 +129,7 @@ struct Stage {
     size_t plan_start = 0, plan_phase = 0, w16 = 0, w64 = 0;  // blob offsets
     size_t itaps = 0, wts = 0;
     int taps = 1;
+    int reduce_op = 0;  // ST_REDUCE_H / ST_REDUCE_V: ReduceOp's pass arithmetic
     // fused reduce: the V plan and the job template (pointers set at launch)
     size_t v_start = 0, v_phase = 0, v_w16 = 0, v_w64 = 0;
     int v_taps = 1;
