@@ -225,6 +225,11 @@ int vgpu_op_reduce_h_new(vgpu_context *ctx, double factor, int kernel, uint32_t 
 /* ReduceVBridge::new(factor, kernel, bands, input_h, hint) (reducev.rs:186-205) */
 int vgpu_op_reduce_v_new(vgpu_context *ctx, double factor, int kernel, uint32_t bands,
                          uint32_t input_h, int format, vgpu_op **out);
+/* ReduceBridge::new(h_factor, v_factor, kernel, bands) (reduce.rs:273-287):
+ * ReduceOp, both passes on one call; the input region's rows must be
+ * required_input_region_v(output)'s rows, as the scheduler derives them. */
+int vgpu_op_reduce_new(vgpu_context *ctx, double h_factor, double v_factor, int kernel,
+                       uint32_t bands, int format, vgpu_op **out);
 /* ShrinkHBridge::new_with_ceil_and_source_width (shrinkh.rs:1354-1383);
  * source_width = 0 means None. */
 int vgpu_op_shrink_h_new(vgpu_context *ctx, uint32_t factor, int ceil_mode, uint32_t bands,

@@ -381,6 +381,38 @@ def test_op_process_region_matches_oracle_reduce_h_offsets(ctx):
         assert np.array_equal(got, np.clip((acc + 2048) >> 12, 0, 255).astype(np.uint8))
 
 
+@pytest.mark.parametrize("fmt,bands", [(np.uint8, 3), (np.uint16, 1), (np.float32, 4), (np.uint8, 5)])
+def test_op_reduce_bridge_tiles_match_whole_image(ctx, fmt, bands):
+    # ReduceBridge::dyn_process_region on strips and an interior tile, each
+    # input tile read from the source with clamp-to-edge as MemorySource
+    # does: every tile equals the oracle's whole-image ReduceOp over it.
+    src = reduce_op_source(fmt, 120, 170, bands, 90 + bands)
+    h, w, b = src.shape
+    hf, vf, k = 1.5, 2.5, v.LANCZOS3
+    exp = O.Pipeline(w, h, b, FMT[src.dtype]).reduce(hf, vf, k).run(src)
+    op = v.Op.reduce(ctx, hf, vf, k, b, FMT[src.dtype])
+    ow, oh = op.output_size(w, h)
+    assert (oh, ow) == exp.shape[:2]
+    for out_r in (v.Region(0, 0, ow, oh), v.Region(0, 0, ow, 16), v.Region(0, 32, ow, 16),
+                  v.Region(7, 40, 60, 8), v.Region(0, oh - 3, ow, 3)):
+        in_r = op.required_input_region(out_r)
+        ys = np.clip(np.arange(in_r.y, in_r.y + in_r.height), 0, h - 1)
+        xs = np.clip(np.arange(in_r.x, in_r.x + in_r.width), 0, w - 1)
+        tile = np.ascontiguousarray(src[ys][:, xs, :])
+        dst = v.DeviceBuffer(ctx, (out_r.height, out_r.width, b), src.dtype)
+        op.process_region(v.DeviceBuffer.from_numpy(ctx, tile), dst, in_r, out_r)
+        ctx.synchronize()
+        want = exp[out_r.y:out_r.y + out_r.height, out_r.x:out_r.x + out_r.width]
+        assert_same(dst.numpy(), want, f"tile {out_r.x},{out_r.y},{out_r.width}x{out_r.height}")
+    # The input tile's rows must be the scratch rows.
+    out_r = v.Region(0, 0, ow, 16)
+    in_r = op.required_input_region(out_r)
+    bad = v.Region(in_r.x, in_r.y + 1, in_r.width, in_r.height)
+    tile = v.DeviceBuffer(ctx, (in_r.height, in_r.width, b), src.dtype)
+    with pytest.raises(v.VgpuError):
+        op.process_region(tile, v.DeviceBuffer(ctx, (16, ow, b), src.dtype), bad, out_r)
+
+
 def test_op_premultiply_exhaustive(ctx):
     a = np.repeat(np.arange(256, dtype=np.uint8), 256)
     c = np.tile(np.arange(256, dtype=np.uint8), 256)

@@ -136,6 +136,7 @@ def load() -> C.CDLL:
         "vgpu_pipeline_benchmark": (i32, [vp, vp, sz, vp, sz, i32, i32, P(f32), P(f32)]),
         "vgpu_op_reduce_h_new": (i32, [vp, dbl, i32, u32, u32, i32, P(vp)]),
         "vgpu_op_reduce_v_new": (i32, [vp, dbl, i32, u32, u32, i32, P(vp)]),
+        "vgpu_op_reduce_new": (i32, [vp, dbl, dbl, i32, u32, i32, P(vp)]),
         "vgpu_op_shrink_h_new": (i32, [vp, u32, i32, u32, u32, i32, P(vp)]),
         "vgpu_op_shrink_v_new": (i32, [vp, u32, i32, u32, i32, P(vp)]),
         "vgpu_op_gauss_blur_h_new": (i32, [vp, f32, u32, i32, P(vp)]),

@@ -476,6 +476,11 @@ class Op:
                          "ReduceVBridge::new")
 
     @classmethod
+    def reduce(cls, ctx, h_factor, v_factor, kernel, bands, fmt=U8):
+        return cls._make(ctx, "vgpu_op_reduce_new", (h_factor, v_factor, kernel, bands, fmt), bands, fmt,
+                         "ReduceBridge::new")
+
+    @classmethod
     def shrink_h(cls, ctx, factor, bands, source_width=0, fmt=U8, ceil=False):
         return cls._make(ctx, "vgpu_op_shrink_h_new", (factor, int(ceil), bands, source_width, fmt), bands,
                          fmt, "ShrinkHBridge::new")

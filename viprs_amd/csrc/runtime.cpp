@@ -1630,9 +1630,13 @@ int vgpu_pipeline_run_host(vgpu_pipeline* p, const void* src, size_t src_bytes, 
 struct vgpu_op {
     vgpu_context* ctx = nullptr;
     Node node;
+    Node vnode;          // ReduceOp: the vertical pass (node holds the horizontal one)
+    bool pair = false;   // ReduceOp
     DeviceBlob blob;
     Rect blob_key;  // output region the uploaded plan was built for
     bool have_blob = false;
+    DeviceBlob scratch;  // ReduceOp's intermediate when its passes run as two kernels
+    size_t scratch_bytes = 0;
 };
 
 static int make_op(vgpu_context* ctx, Node nd, vgpu_op** out) {
@@ -1644,6 +1648,86 @@ static int make_op(vgpu_context* ctx, Node nd, vgpu_op** out) {
     op->blob.ctx = ctx;
     *out = op;
     return VGPU_OK;
+}
+
+// Uploads an op's plan tables unless the blob already holds the plan for
+// this output region.
+static int upload_op_blob(vgpu_op* op, const Staging& st, const Rect& out_r) {
+    if (st.bytes.empty()) return VGPU_OK;
+    vgpu_context* ctx = op->ctx;
+    const bool same = op->have_blob && op->blob_key.x == out_r.x && op->blob_key.y == out_r.y &&
+                      op->blob_key.w == out_r.w && op->blob_key.h == out_r.h;
+    if (same) return VGPU_OK;
+    // The previous plan may still be read by queued work.
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
+    op->blob.reset();
+    op->blob.ptr = ctx->pool.acquire(st.bytes.size());
+    if (!op->blob.ptr) return VGPU_E_NO_MEMORY;
+    hipError_t e = hipMemcpyAsync(op->blob.ptr, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
+    op->blob_key = out_r;
+    op->have_blob = true;
+    return VGPU_OK;
+}
+
+// ReduceOp::process_region (reduce.rs:224-250): the H pass fills the scratch
+// region required_input_region_v(out), clamping columns to the input tile;
+// its rows are the scratch rows themselves, so the input tile must carry
+// exactly those rows.  One fused stage for <= 4 bands, else two kernels
+// through a device scratch owned by the op.
+static int process_reduce_op(vgpu_op* op, const void* input, void* output, const Rect& in_r, const Rect& out_r) {
+    vgpu_context* ctx = op->ctx;
+    const Node& nh = op->node;
+    const Node& nv = op->vnode;
+    const Rect mid = nv.required_input(out_r);
+    if (in_r.y != mid.y || in_r.h != mid.h) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    std::vector<Node> nodes{nh, nv};
+    Staging st;
+    Stage s;
+    s.first = 0;
+    s.clip = in_r;
+    s.in_fmt = s.out_fmt = nh.in_fmt;
+    const InView in = view_of(input, in_r, in_r);
+    if (nh.bands <= 4 && stage_reduce_hv_params(nh.axis, false, nv.axis, nh.in_fmt, nh.bands, mid, out_r, mid, in_r,
+                                                in_r, StageOptions(), &st, &s, true)) {
+        s.last = 1;
+        s.out = out_r;
+        int rc = upload_op_blob(op, st, out_r);
+        if (rc == VGPU_OK)
+            rc = launch_stage(nodes, s, in, output, static_cast<const unsigned char*>(op->blob.ptr), ctx->stream);
+        if (rc != VGPU_OK) ctx->sticky = rc;
+        return rc;
+    }
+    Stage sv = s;
+    s.last = 0;
+    s.kind = ST_REDUCE_H;
+    s.reduce_op = 1;
+    s.out = mid;
+    stage_reduce_params(nh, mid, &st, &s);
+    sv.first = sv.last = 1;
+    sv.kind = ST_REDUCE_V;
+    sv.reduce_op = 1;
+    sv.clip = mid;
+    sv.out = out_r;
+    stage_reduce_params(nv, out_r, &st, &sv);
+    int rc = upload_op_blob(op, st, out_r);
+    if (rc != VGPU_OK) return rc;
+    const size_t bytes = (size_t)mid.w * mid.h * nh.bands * format_size(nh.in_fmt);
+    if (bytes > op->scratch_bytes) {
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
+        op->scratch.reset();
+        op->scratch.ctx = ctx;
+        op->scratch.ptr = ctx->pool.acquire(bytes);
+        op->scratch_bytes = op->scratch.ptr ? bytes : 0;
+        if (!op->scratch.ptr) return VGPU_E_NO_MEMORY;
+    }
+    const unsigned char* blob = static_cast<const unsigned char*>(op->blob.ptr);
+    rc = launch_stage(nodes, s, in, op->scratch.ptr, blob, ctx->stream);
+    if (rc == VGPU_OK) rc = launch_stage(nodes, sv, view_of(op->scratch.ptr, mid, mid), output, blob, ctx->stream);
+    if (rc != VGPU_OK) ctx->sticky = rc;
+    return rc;
 }
 
 static Node base_node(NodeKind k, uint32_t bands, int fmt) {
@@ -1676,6 +1760,23 @@ int vgpu_op_reduce_v_new(vgpu_context* ctx, double factor, int kernel, uint32_t 
     int st = build_reduce_axis(kernel, factor, input_h, &nd.axis);
     if (st != OK) return st;
     return make_op(ctx, std::move(nd), out);
+}
+
+int vgpu_op_reduce_new(vgpu_context* ctx, double hfactor, double vfactor, int kernel, uint32_t bands, int fmt,
+                       vgpu_op** out) {
+    if (bands == 0) return VGPU_E_INVALID_ARGUMENT;
+    if (!valid_fmt(fmt)) return VGPU_E_UNSUPPORTED_FORMAT;
+    if (!std::isfinite(hfactor) || !std::isfinite(vfactor) || hfactor < 1.0 || vfactor < 1.0)
+        return VGPU_E_INVALID_ARGUMENT;
+    Node nh = base_node(N_REDUCE_OP_H, bands, fmt), nv = base_node(N_REDUCE_OP_V, bands, fmt);
+    int st = build_reduce_axis(kernel, hfactor, 0, &nh.axis);
+    if (st == OK) st = build_reduce_axis(kernel, vfactor, 0, &nv.axis);
+    if (st != OK) return st;
+    st = make_op(ctx, std::move(nh), out);
+    if (st != VGPU_OK) return st;
+    (*out)->vnode = std::move(nv);
+    (*out)->pair = true;
+    return VGPU_OK;
 }
 
 int vgpu_op_shrink_h_new(vgpu_context* ctx, uint32_t factor, int ceil_mode, uint32_t bands,
@@ -1735,7 +1836,8 @@ static Rect rect_of(vgpu_region r) { return Rect{r.x, r.y, (int64_t)r.width, (in
 
 int vgpu_op_required_input_region(const vgpu_op* op, vgpu_region o, vgpu_region* in) {
     if (!op || !in) return VGPU_E_INVALID_ARGUMENT;
-    Rect r = op->node.required_input(rect_of(o));
+    // ReduceOp::required_input_region = h(v(out)) (reduce.rs:181-184).
+    Rect r = op->node.required_input(op->pair ? op->vnode.required_input(rect_of(o)) : rect_of(o));
     if (r.w < 0 || r.h < 0 || r.w > UINT32_MAX || r.h > UINT32_MAX) return VGPU_E_INVALID_ARGUMENT;
     in->x = (int32_t)r.x;
     in->y = (int32_t)r.y;
@@ -1752,6 +1854,10 @@ int vgpu_op_output_size(const vgpu_op* op, uint32_t iw, uint32_t ih, uint32_t* o
     switch (nd.kind) {
     case N_REDUCE_H: *ow = reduce_output_len(nd.axis.factor, iw); break;
     case N_REDUCE_V: *oh = reduce_output_len(nd.axis.factor, ih); break;
+    case N_REDUCE_OP_H:  // reduce.rs:257-263
+        *ow = reduce_output_len(nd.axis.factor, iw);
+        *oh = reduce_output_len(op->vnode.axis.factor, ih);
+        break;
     case N_SHRINK_H: *ow = iw / nd.factor; break;
     case N_SHRINK_V: *oh = ih / nd.factor; break;
     default: break;
@@ -1777,6 +1883,7 @@ int vgpu_op_process_region(vgpu_op* op, const void* input, size_t in_len, void* 
         return VGPU_E_BUFFER_SIZE;
     if (out_r.empty()) return VGPU_OK;
     if (!input || !output) return VGPU_E_INVALID_ARGUMENT;
+    if (op->pair) return process_reduce_op(op, input, output, in_r, out_r);
     if (nd.kind != N_REDUCE_H && nd.kind != N_REDUCE_V) {
         // These ops index their input relative to the tile origin the
         // scheduler derived from required_input_region; insist on it.
@@ -1810,25 +1917,11 @@ int vgpu_op_process_region(vgpu_op* op, const void* input, size_t in_len, void* 
     case N_UNPREMULT: s.kind = ST_ALPHA; break;
     default: s.kind = ST_COPY; break;
     }
-    if (!st.bytes.empty()) {
-        const bool same = op->have_blob && op->blob_key.x == out_r.x && op->blob_key.y == out_r.y &&
-                          op->blob_key.w == out_r.w && op->blob_key.h == out_r.h;
-        if (!same) {
-            // The previous plan may still be read by queued work.
-            if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
-            op->blob.reset();
-            op->blob.ptr = ctx->pool.acquire(st.bytes.size());
-            if (!op->blob.ptr) return VGPU_E_NO_MEMORY;
-            hipError_t e = hipMemcpyAsync(op->blob.ptr, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice, ctx->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-            if (e != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
-            op->blob_key = out_r;
-            op->have_blob = true;
-        }
-    }
+    int rc = upload_op_blob(op, st, out_r);
+    if (rc != VGPU_OK) return rc;
     std::vector<Node> one{nd};
-    int rc = launch_stage(one, s, view_of(input, in_r, in_r), output,
-                          static_cast<const unsigned char*>(op->blob.ptr), ctx->stream);
+    rc = launch_stage(one, s, view_of(input, in_r, in_r), output,
+                      static_cast<const unsigned char*>(op->blob.ptr), ctx->stream);
     if (rc != VGPU_OK) ctx->sticky = rc;
     return rc;
 }
