@@ -190,7 +190,7 @@ int vgpu_batch::plan() {
     Staging tbl;
     n_shrink_tasks = n_reduce_tasks = 0;
     for (int lv = 0; lv < levels; ++lv) {
-        constexpr int kShrinkVariants = 4;
+        constexpr int kShrinkVariants = 8;  // shrink_job_variant
         std::vector<ShrinkJob> sjobs[kShrinkVariants];
         std::vector<BatchTask> stasks[kShrinkVariants];
         uint64_t sbytes[kShrinkVariants] = {};

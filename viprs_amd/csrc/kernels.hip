@@ -430,19 +430,223 @@ __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, 
     }
 }
 
-// One kernel per band count and fetch depth, so each keeps only its own
-// registers.
-// One kernel per band count, so each keeps only its own registers.
+// Lean form of the same kernel for spans of at least 128 chunks (lsl >= 7):
+// a fetch group is then one or two source rows and each 64-lane register
+// holds part of one of them (register j: row j >> (lsl - 6)), so a row's
+// address, chunk limit and 16-byte shift are wave-uniform and computed once
+// per row in scalar registers.  Loads take the row's 16-byte-aligned start
+// plus a per-lane 32-bit offset; lanes past the row's whole chunks re-read
+// its last one.  Parking is four unconditional 16-byte LDS stores; a chunk
+// cut by the end of the buffer (scalar flag) is then copied bytewise.
+// Output rows complete on a counter instead of a division per row.
 template <int B>
+__device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, int by, unsigned char* lds) {
+    const InView& in = job.in;
+    const OutView& out = job.out;
+    const int fh = job.fh, fv = job.fv, tx = job.tx, S = job.S, m = job.m, lsl = job.lsl;
+    const bool two = lsl == 7;  // two source rows per fetch group (registers 0-1, 2-3)
+    const int RP = two ? 2 : 1;
+    const bool premult = (B == 2 || B == 4) && job.premult;
+    const uint32_t mh = job.mh, mv = job.mv;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int part = wave % S;
+    const int oyi0 = (by * ((kThreads / 64) / S) + wave / S) * m;
+    const int nout = max(0, min(m, out.h - oyi0));
+    const int nsrc = S > 1 ? ((nout > 0 && part < fv) ? (fv - 1 - part) / S + 1 : 0) : nout * fv;
+    const int col0 = bx * tx;
+    const int ncols = min(tx, out.w - col0);
+    const int ybase = (out.y + oyi0) * fv + part;
+    const int cx0 = (out.x + col0) * fh;
+    const int cx1 = cx0 + ncols * fh;
+    const int lo = clampi(cx0, in.bx, in.bx + in.bw - 1);
+    const int hi = clampi(cx1 - 1, in.bx, in.bx + in.bw - 1) + 1;
+    const int nbytes = (hi - lo) * B;
+    const unsigned char* base = static_cast<const unsigned char*>(in.ptr);
+    const size_t row_bytes = (size_t)in.bw * B;
+    const size_t total = row_bytes * (size_t)in.bh;
+    const size_t col_off = (size_t)(lo - in.bx) * B;
+    unsigned char* rbuf = lds + wave * (kSpanBytes + 32);
+    const int wbytes = fh * B;
+    const int X0 = cx0 + lane * fh;
+    const bool fast = !premult && lane < ncols && X0 >= lo && X0 + fh <= hi;
+    const int lane_byte = (X0 - lo) * B;
+
+    // Row k of the wavefront's run: a16 = its span's 16-byte-aligned buffer
+    // offset, lb = where its chunks are loaded from (a16, or 0 -- the
+    // buffer's first chunk -- for a row past the run or with no whole chunk),
+    // lim = chunks to load (>= 1), cut = the chunk cut by the buffer end or
+    // -1, shift = the span's offset past a16.
+    auto row_fetch = [&](int k, size_t& a16, size_t& lb, int& lim, int& cut, int& shift) {
+        a16 = 0;
+        lb = 0;
+        lim = 1;
+        cut = -1;
+        shift = 0;
+        if (k < nsrc) {
+            const size_t a = (size_t)row_of(in, ybase + k * S) * row_bytes + col_off;
+            a16 = a & ~(size_t)15;
+            shift = (int)(a - a16);
+            const int nch = (shift + nbytes + 15) >> 4;
+            const size_t whole = (total - a16) >> 4;
+            const int w = (size_t)nch <= whole ? nch : (int)whole;
+            cut = w < nch ? w : -1;
+            if (w > 0) {
+                lb = a16;
+                lim = w;
+            }
+        }
+    };
+
+    size_t na0 = 0, na1 = 0, nb0 = 0, nb1 = 0;
+    int nl0 = 1, nl1 = 1, nc0 = -1, nc1 = -1, ns0 = 0, ns1 = 0;  // rows of the group in flight
+    u32x4 p0, p1, p2, p3;
+    auto fetch = [&](int g) {
+        const int k0 = g * RP;
+        row_fetch(k0, na0, nb0, nl0, nc0, ns0);
+        if (two) row_fetch(k0 + 1, na1, nb1, nl1, nc1, ns1);
+        const unsigned char* r0 = base + nb0;
+        const unsigned char* r1 = base + nb1;
+        if (two) {
+            p0 = gmem<u32x4>(r0 + (uint32_t)min(lane, nl0 - 1) * 16u)[0];
+            p1 = gmem<u32x4>(r0 + (uint32_t)min(lane + 64, nl0 - 1) * 16u)[0];
+            p2 = gmem<u32x4>(r1 + (uint32_t)min(lane, nl1 - 1) * 16u)[0];
+            p3 = gmem<u32x4>(r1 + (uint32_t)min(lane + 64, nl1 - 1) * 16u)[0];
+        } else {
+            p0 = gmem<u32x4>(r0 + (uint32_t)min(lane, nl0 - 1) * 16u)[0];
+            p1 = gmem<u32x4>(r0 + (uint32_t)min(lane + 64, nl0 - 1) * 16u)[0];
+            p2 = gmem<u32x4>(r0 + (uint32_t)min(lane + 128, nl0 - 1) * 16u)[0];
+            p3 = gmem<u32x4>(r0 + (uint32_t)min(lane + 192, nl0 - 1) * 16u)[0];
+        }
+    };
+    // The cut chunk c of a row whose slot starts at LDS chunk slot0: its
+    // bytes before the end of the buffer.
+    auto fix_cut = [&](size_t a16, int c, int slot0) {
+        if (lane != (c & 63)) return;  // the lane that stored that slot
+        const size_t off = a16 + (size_t)c * 16;
+        unsigned char* d = rbuf + (slot0 + c) * 16;
+        for (int q = 0; q < 16; ++q) d[q] = off + q < total ? gmem<unsigned char>(base)[off + q] : 0;
+    };
+    int cs0 = 0, cs1 = 0;  // shifts of the parked group's rows
+    auto park = [&]() {
+        u32x4* dst = reinterpret_cast<u32x4*>(rbuf) + lane;
+        dst[0] = p0;
+        dst[64] = p1;
+        dst[128] = p2;
+        dst[192] = p3;
+        cs0 = ns0;
+        cs1 = ns1;
+        if (nc0 >= 0 || (two && nc1 >= 0)) {
+            // Lanes whose chunk index is the cut one rewrite its slot (the
+            // same lane stored it, so LDS order holds).
+            if (nc0 >= 0) fix_cut(na0, nc0, 0);
+            if (two && nc1 >= 0) fix_cut(na1, nc1, 128);
+        }
+    };
+
+    uint32_t acc[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) acc[b] = 0;
+    int krow = 0, orow = 0;  // source rows into the current output row; output rows done
+    auto row_sum = [&](int r, int shift) {
+        const unsigned char* slot = rbuf + (r << lsl) * 16;
+        uint32_t sum[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) sum[b] = 0;
+        if (fast) {
+            window_sums<B>(slot, shift + lane_byte, wbytes, sum);
+        } else if (lane < ncols) {
+            for (int j = 0; j < fh; ++j) {
+                const unsigned char* px = slot + shift + (clampi(X0 + j, lo, hi - 1) - lo) * B;
+                if (premult) {
+                    const uint32_t al = px[B - 1];
+#pragma unroll
+                    for (int b = 0; b < B - 1; ++b) sum[b] += premult_u8(px[b], al);
+                    sum[B - 1] += al;
+                } else {
+#pragma unroll
+                    for (int b = 0; b < B; ++b) sum[b] += px[b];
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) acc[b] += shrink_round_h(sum[b], (uint32_t)fh, B, mh);
+        if (S == 1 && ++krow == fv) {
+            if (lane < ncols) {
+                unsigned char* o = static_cast<unsigned char*>(out.ptr) +
+                                   ((size_t)(oyi0 + orow) * out.w + col0 + lane) * B;
+                uint32_t v[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) v[b] = shrink_round_v(acc[b], (uint32_t)fv, mv);
+                if ((B == 2 || B == 4) && job.unpremult) {
+#pragma unroll
+                    for (int b = 0; b < B - 1; ++b) v[b] = unpremult_u8(v[b], v[B - 1]);
+                }
+#pragma unroll
+                for (int b = 0; b < B; ++b) o[b] = (unsigned char)v[b];
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) acc[b] = 0;
+            krow = 0;
+            ++orow;
+        }
+    };
+    const int ng = (nsrc + RP - 1) / RP;
+    if (ng > 0) fetch(0);
+    for (int g = 0; g < ng; ++g) {
+        park();
+        wave_lds_sync();
+        const int k0 = g * RP;
+        const int s0 = cs0, s1 = cs1;
+        if (g + 1 < ng) fetch(g + 1);
+        row_sum(0, s0);
+        if (two && k0 + 1 < nsrc) row_sum(1, s1);
+        wave_lds_sync();
+    }
+    if (S > 1) {
+        // Partial sums of the S wavefronts of each output row; every
+        // wavefront of the block reaches both barriers (S is per job).
+        __syncthreads();
+        uint32_t* part_sums = reinterpret_cast<uint32_t*>(lds);
+#pragma unroll
+        for (int b = 0; b < B; ++b) part_sums[(wave * 64 + lane) * B + b] = acc[b];
+        __syncthreads();
+        if (part == 0 && nout > 0 && lane < ncols) {
+            for (int q = 1; q < S; ++q) {
+#pragma unroll
+                for (int b = 0; b < B; ++b) acc[b] += part_sums[((wave + q) * 64 + lane) * B + b];
+            }
+            unsigned char* o = static_cast<unsigned char*>(out.ptr) + ((size_t)oyi0 * out.w + col0 + lane) * B;
+            uint32_t v[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) v[b] = shrink_round_v(acc[b], (uint32_t)fv, mv);
+            if ((B == 2 || B == 4) && job.unpremult) {
+#pragma unroll
+                for (int b = 0; b < B - 1; ++b) v[b] = unpremult_u8(v[b], v[B - 1]);
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) o[b] = (unsigned char)v[b];
+        }
+    }
+}
+
+// One kernel per band count and form (LEAN: spans of >= 128 chunks), so each
+// keeps only its own registers.
+template <int B, bool LEAN>
+__device__ __forceinline__ void shrink_block(const ShrinkJob& job, int bx, int by, unsigned char* lds) {
+    if constexpr (LEAN) shrink_rows_lean<B>(job, bx, by, lds);
+    else shrink_rows_block<B>(job, bx, by, lds);
+}
+
+template <int B, bool LEAN>
 __global__ __launch_bounds__(kThreads) void k_shrink_job(ShrinkJob job) {
     extern __shared__ __align__(16) unsigned char lds[];
-    shrink_rows_block<B>(job, blockIdx.x, blockIdx.y, lds);
+    shrink_block<B, LEAN>(job, blockIdx.x, blockIdx.y, lds);
 }
 
 // Persistent form: the grid walks a task list of (job, bx, by) over many
 // images (all of one variant); jobs reading the caller's source or writing
 // the caller's output take those pointers from per-run slot tables.
-template <int B>
+template <int B, bool LEAN>
 __global__ __launch_bounds__(kThreads) void k_shrink_batch(const ShrinkJob* __restrict__ jobs,
                                                            const BatchTask* __restrict__ tasks, int ntasks,
                                                            const void* const* __restrict__ srcs,
@@ -453,7 +657,7 @@ __global__ __launch_bounds__(kThreads) void k_shrink_batch(const ShrinkJob* __re
         ShrinkJob job = jobs[task.job];
         if (job.src_slot >= 0) job.in.ptr = srcs[job.src_slot];
         if (job.dst_slot >= 0) job.out.ptr = dsts[job.dst_slot];
-        shrink_rows_block<B>(job, task.bx, task.by, lds);
+        shrink_block<B, LEAN>(job, task.bx, task.by, lds);
         __syncthreads();
     }
 }
@@ -1001,18 +1205,19 @@ bool plan_shrink_job(ShrinkJob* job, bool row_split, int run) {
     return true;
 }
 
-int shrink_job_variant(const ShrinkJob& job) { return job.bands - 1; }
+int shrink_job_variant(const ShrinkJob& job) { return job.bands - 1 + (job.lsl >= 7 ? 4 : 0); }
 
 size_t shrink_job_lds() { return (size_t)(kThreads / 64) * (kSpanBytes + 32); }
 
-#define VGPU_SHRINK_VARIANTS(X) X(0, 1) X(1, 2) X(2, 3) X(3, 4)
+#define VGPU_SHRINK_VARIANTS(X) \
+    X(0, 1, false) X(1, 2, false) X(2, 3, false) X(3, 4, false) X(4, 1, true) X(5, 2, true) X(6, 3, true) X(7, 4, true)
 
 hipError_t launch_shrink_job(const ShrinkJob& job, hipStream_t s) {
     if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;
     const dim3 grid((unsigned)job.gx, (unsigned)job.gy);
     switch (shrink_job_variant(job)) {
-#define X(V, B) \
-    case V: hipLaunchKernelGGL((k_shrink_job<B>), grid, dim3(kThreads), shrink_job_lds(), s, job); break;
+#define X(V, B, L) \
+    case V: hipLaunchKernelGGL((k_shrink_job<B, L>), grid, dim3(kThreads), shrink_job_lds(), s, job); break;
         VGPU_SHRINK_VARIANTS(X)
 #undef X
     default: return hipErrorInvalidValue;
@@ -1024,9 +1229,9 @@ hipError_t launch_shrink_batch(int variant, const ShrinkJob* jobs, const BatchTa
                                const void* const* srcs, void* const* dsts, int grid, hipStream_t s) {
     if (ntasks <= 0) return hipSuccess;
     switch (variant) {
-#define X(V, B)                                                                                       \
+#define X(V, B, L)                                                                                    \
     case V:                                                                                           \
-        hipLaunchKernelGGL((k_shrink_batch<B>), dim3((unsigned)grid), dim3(kThreads), shrink_job_lds(), s, \
+        hipLaunchKernelGGL((k_shrink_batch<B, L>), dim3((unsigned)grid), dim3(kThreads), shrink_job_lds(), s, \
                            jobs, tasks, ntasks, srcs, dsts);                                          \
         break;
         VGPU_SHRINK_VARIANTS(X)
