@@ -497,26 +497,37 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
         }
     };
 
-    size_t na0 = 0, na1 = 0, nb0 = 0, nb1 = 0;
-    int nl0 = 1, nl1 = 1, nc0 = -1, nc1 = -1, ns0 = 0, ns1 = 0;  // rows of the group in flight
-    u32x4 p0, p1, p2, p3;
+    // A fetch group in flight: its four 16-byte registers and, per row, the
+    // span offset, cut chunk and shift.
+    struct Group {
+        u32x4 p0, p1, p2, p3;
+        size_t a0, a1;
+        int c0, c1, s0, s1;
+    };
     auto fetch = [&](int g) {
+        Group G;
         const int k0 = g * RP;
-        row_fetch(k0, na0, nb0, nl0, nc0, ns0);
-        if (two) row_fetch(k0 + 1, na1, nb1, nl1, nc1, ns1);
-        const unsigned char* r0 = base + nb0;
-        const unsigned char* r1 = base + nb1;
+        size_t b0 = 0, b1 = 0;
+        int l0 = 1, l1 = 1;
+        G.a1 = 0;
+        G.c1 = -1;
+        G.s1 = 0;
+        row_fetch(k0, G.a0, b0, l0, G.c0, G.s0);
+        if (two) row_fetch(k0 + 1, G.a1, b1, l1, G.c1, G.s1);
+        const unsigned char* r0 = base + b0;
+        const unsigned char* r1 = base + b1;
         if (two) {
-            p0 = gmem<u32x4>(r0 + (uint32_t)min(lane, nl0 - 1) * 16u)[0];
-            p1 = gmem<u32x4>(r0 + (uint32_t)min(lane + 64, nl0 - 1) * 16u)[0];
-            p2 = gmem<u32x4>(r1 + (uint32_t)min(lane, nl1 - 1) * 16u)[0];
-            p3 = gmem<u32x4>(r1 + (uint32_t)min(lane + 64, nl1 - 1) * 16u)[0];
+            G.p0 = gmem<u32x4>(r0 + (uint32_t)min(lane, l0 - 1) * 16u)[0];
+            G.p1 = gmem<u32x4>(r0 + (uint32_t)min(lane + 64, l0 - 1) * 16u)[0];
+            G.p2 = gmem<u32x4>(r1 + (uint32_t)min(lane, l1 - 1) * 16u)[0];
+            G.p3 = gmem<u32x4>(r1 + (uint32_t)min(lane + 64, l1 - 1) * 16u)[0];
         } else {
-            p0 = gmem<u32x4>(r0 + (uint32_t)min(lane, nl0 - 1) * 16u)[0];
-            p1 = gmem<u32x4>(r0 + (uint32_t)min(lane + 64, nl0 - 1) * 16u)[0];
-            p2 = gmem<u32x4>(r0 + (uint32_t)min(lane + 128, nl0 - 1) * 16u)[0];
-            p3 = gmem<u32x4>(r0 + (uint32_t)min(lane + 192, nl0 - 1) * 16u)[0];
+            G.p0 = gmem<u32x4>(r0 + (uint32_t)min(lane, l0 - 1) * 16u)[0];
+            G.p1 = gmem<u32x4>(r0 + (uint32_t)min(lane + 64, l0 - 1) * 16u)[0];
+            G.p2 = gmem<u32x4>(r0 + (uint32_t)min(lane + 128, l0 - 1) * 16u)[0];
+            G.p3 = gmem<u32x4>(r0 + (uint32_t)min(lane + 192, l0 - 1) * 16u)[0];
         }
+        return G;
     };
     // The cut chunk c of a row whose slot starts at LDS chunk slot0: its
     // bytes before the end of the buffer.
@@ -526,20 +537,17 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
         unsigned char* d = rbuf + (slot0 + c) * 16;
         for (int q = 0; q < 16; ++q) d[q] = off + q < total ? gmem<unsigned char>(base)[off + q] : 0;
     };
-    int cs0 = 0, cs1 = 0;  // shifts of the parked group's rows
-    auto park = [&]() {
+    auto park = [&](const Group& G) {
         u32x4* dst = reinterpret_cast<u32x4*>(rbuf) + lane;
-        dst[0] = p0;
-        dst[64] = p1;
-        dst[128] = p2;
-        dst[192] = p3;
-        cs0 = ns0;
-        cs1 = ns1;
-        if (nc0 >= 0 || (two && nc1 >= 0)) {
+        dst[0] = G.p0;
+        dst[64] = G.p1;
+        dst[128] = G.p2;
+        dst[192] = G.p3;
+        if (G.c0 >= 0 || (two && G.c1 >= 0)) {
             // Lanes whose chunk index is the cut one rewrite its slot (the
             // same lane stored it, so LDS order holds).
-            if (nc0 >= 0) fix_cut(na0, nc0, 0);
-            if (two && nc1 >= 0) fix_cut(na1, nc1, 128);
+            if (G.c0 >= 0) fix_cut(G.a0, G.c0, 0);
+            if (two && G.c1 >= 0) fix_cut(G.a1, G.c1, 128);
         }
     };
 
@@ -591,15 +599,18 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
         }
     };
     const int ng = (nsrc + RP - 1) / RP;
-    if (ng > 0) fetch(0);
-    for (int g = 0; g < ng; ++g) {
-        park();
-        wave_lds_sync();
-        const int k0 = g * RP;
-        const int s0 = cs0, s1 = cs1;
-        if (g + 1 < ng) fetch(g + 1);
+    auto rows = [&](int g, int s0, int s1) {
         row_sum(0, s0);
-        if (two && k0 + 1 < nsrc) row_sum(1, s1);
+        if (two && g * RP + 1 < nsrc) row_sum(1, s1);
+    };
+    Group GA;
+    if (ng > 0) GA = fetch(0);
+    for (int g = 0; g < ng; ++g) {
+        park(GA);
+        wave_lds_sync();
+        const int a0 = GA.s0, a1 = GA.s1;
+        if (g + 1 < ng) GA = fetch(g + 1);
+        rows(g, a0, a1);
         wave_lds_sync();
     }
     if (S > 1) {
