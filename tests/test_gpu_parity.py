@@ -291,6 +291,27 @@ def test_alpha_fusion_forms(ctx, bands):
         assert_same(got, exp, f"bands {bands} {steps}")
 
 
+@pytest.mark.parametrize("bands", [2, 4])
+@pytest.mark.parametrize("f", [4, 7, 19, 40])
+def test_premultiplied_shrink_large_factors(ctx, bands, f):
+    # Wide box windows take the lean shrink form; premultiplied 4-band windows
+    # inside the span sum on packed 16-bit lanes, the rest per pixel.  Ragged
+    # sizes put a chunk cut by the end of the buffer into the last row.
+    src = noise_u8(3 * f + 5, 64 * f + 7, bands, 300 + f + bands)
+    src[..., -1] = np.where(np.random.default_rng(f).random(src.shape[:2]) < 0.15, 0, src[..., -1])
+    src[: f, : f, -1] = 255
+    for steps in ([("premultiply", ()), ("shrink_h", (f,)), ("shrink_v", (f,)), ("unpremultiply", ())],
+                  [("premultiply", ()), ("shrink_h", (f, True)), ("shrink_v", (max(f // 3, 1), True)),
+                   ("unpremultiply", ())],
+                  [("premultiply", ()), ("shrink_h", (f,)), ("shrink_v", (f,))]):
+        got, exp = run_both(ctx, src, steps)
+        assert_same(got, exp, f"bands {bands} f {f} {steps}")
+    big = noise_u8(1531, 2049, bands, 7 + f)
+    big[..., -1] = np.where(np.random.default_rng(f + 1).random(big.shape[:2]) < 0.2, 0, big[..., -1])
+    got, exp = run_both(ctx, big, [("thumbnail", (v.FITBOX, 2049 // f, 2049 // f, v.LANCZOS3))])
+    assert_same(got, exp, f"rgba thumbnail f {f}")
+
+
 U8_PAIR_CASES = [
     [("reduce_h", (3.7, v.LANCZOS3)), ("reduce_v", (3.7, v.LANCZOS3))],
     [("reduce_h", (1.0779, v.LANCZOS3)), ("reduce_v", (1.9, v.LANCZOS3))],

@@ -47,6 +47,7 @@ struct Launch {
     int ntasks = 0;
     size_t lds = 0;
     uint64_t alg_bytes = 0;  // input buffers read + outputs written, summed over jobs
+    int chain = -1;          // band count - 1 of every image it serves, or -1 for mixed
 };
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -73,8 +74,20 @@ struct vgpu_batch {
     int grid_cap = 0;    // persistent grid size; 0 = one block per task
     int use_graph = 1;
     hipGraphExec_t graph = nullptr;
+    // Launches that serve images of one band count form a chain; chains run on
+    // their own streams, forked from and joined back into the context stream.
+    hipStream_t side[4] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[4] = {};
+    unsigned chains = 0;  // bit c: some launch runs on chain c
 
-    ~vgpu_batch() { release(); }
+    ~vgpu_batch() {
+        release();
+        for (int c = 0; c < 4; ++c) {
+            if (side[c]) (void)hipStreamDestroy(side[c]);
+            if (join_ev[c]) (void)hipEventDestroy(join_ev[c]);
+        }
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+    }
     void drop_graph() {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
@@ -256,6 +269,7 @@ int vgpu_batch::plan() {
             Launch l;
             l.family = 0;
             l.variant = var;
+            l.chain = var % 4;  // shrink_job_variant: bands - 1 (+ 4 for the lean form)
             l.jobs_off = tbl.put(sjobs[var].data(), sjobs[var].size() * sizeof(ShrinkJob));
             l.tasks_off = tbl.put(stasks[var].data(), stasks[var].size() * sizeof(BatchTask));
             l.ntasks = (int)stasks[var].size();
@@ -269,6 +283,7 @@ int vgpu_batch::plan() {
             Launch l;
             l.family = 2;
             l.variant = var;
+            l.chain = var >> 3;  // sep_job_variant: bands - 1 in the high bits
             l.jobs_off = tbl.put(pjobs[var].data(), pjobs[var].size() * sizeof(SepJob));
             l.tasks_off = tbl.put(ptasks[var].data(), ptasks[var].size() * sizeof(BatchTask));
             l.ntasks = (int)ptasks[var].size();
@@ -289,6 +304,17 @@ int vgpu_batch::plan() {
             n_reduce_tasks += l.ntasks;
         }
     }
+    chains = 0;
+    for (const Launch& l : launches)
+        if (l.chain >= 0) chains |= 1u << l.chain;
+    for (int c = 0; c < 4; ++c) {
+        if (!(chains & (1u << c)) || side[c]) continue;
+        // Default priority, as the context stream.
+        if (hipStreamCreateWithFlags(&side[c], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&join_ev[c], hipEventDisableTiming) != hipSuccess)
+            return VGPU_E_DEVICE;
+    }
+    if (!fork_ev && hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming) != hipSuccess) return VGPU_E_DEVICE;
     // An image whose shrink job could not be planned runs stage by stage
     // after the batched levels (any earlier-level job it already has only
     // writes its own intermediate, which the stage-by-stage run rewrites).
@@ -318,8 +344,32 @@ int vgpu_batch::launch_all(hipStream_t s, hipEvent_t* marks) {
     unsigned char* t = static_cast<unsigned char*>(tables);
     const void* const* srcs = reinterpret_cast<const void* const*>(t + slots_off);
     void* const* dsts = reinterpret_cast<void* const*>(t + slots_off + images.size() * sizeof(void*));
+    // Per-launch timing (marks) keeps every launch on `s`, in order.
+    const bool par = !marks && __builtin_popcount(chains) >= 2;
+    auto fork = [&]() -> bool {
+        if (hipEventRecord(fork_ev, s) != hipSuccess) return false;
+        for (int c = 0; c < 4; ++c)
+            if ((chains & (1u << c)) && hipStreamWaitEvent(side[c], fork_ev, 0) != hipSuccess) return false;
+        return true;
+    };
+    auto join = [&]() -> bool {
+        for (int c = 0; c < 4; ++c)
+            if ((chains & (1u << c)) &&
+                (hipEventRecord(join_ev[c], side[c]) != hipSuccess || hipStreamWaitEvent(s, join_ev[c], 0) != hipSuccess))
+                return false;
+        return true;
+    };
+    if (par && !fork()) return VGPU_E_DEVICE;
     for (size_t li = 0; li < launches.size(); ++li) {
         const Launch& l = launches[li];
+        hipStream_t ls = s;
+        if (par) {
+            if (l.chain >= 0) {
+                ls = side[l.chain];
+            } else if (!join()) {  // a launch over mixed band counts waits for every chain
+                return VGPU_E_DEVICE;
+            }
+        }
         // A block takes one task, or (MFMA family) one task per wavefront.
         const int blocks = l.family == 2 ? (l.ntasks + 3) / 4 : l.ntasks;
         const int grid = grid_cap > 0 ? std::min(blocks, grid_cap) : blocks;
@@ -327,16 +377,18 @@ int vgpu_batch::launch_all(hipStream_t s, hipEvent_t* marks) {
         hipError_t e;
         if (l.family == 2)
             e = launch_sep_batch(l.variant, reinterpret_cast<const SepJob*>(t + l.jobs_off), tasks, l.ntasks, srcs,
-                                 dsts, grid, l.lds, s);
+                                 dsts, grid, l.lds, ls);
         else if (l.family == 0)
             e = launch_shrink_batch(l.variant, reinterpret_cast<const ShrinkJob*>(t + l.jobs_off), tasks, l.ntasks,
-                                    srcs, dsts, grid, s);
+                                    srcs, dsts, grid, ls);
         else
             e = launch_reduce_batch(reinterpret_cast<const ReduceJob*>(t + l.jobs_off), tasks, l.ntasks, srcs, dsts,
-                                    grid, l.lds, s);
+                                    grid, l.lds, ls);
         if (e != hipSuccess) return VGPU_E_DEVICE;
+        if (par && l.chain < 0 && !fork()) return VGPU_E_DEVICE;
         if (marks && hipEventRecord(marks[li + 1], s) != hipSuccess) return VGPU_E_DEVICE;
     }
+    if (par && !join()) return VGPU_E_DEVICE;
     return VGPU_OK;
 }
 

@@ -243,6 +243,45 @@ __device__ __forceinline__ void window_sums(const unsigned char* rbuf, int sbyte
     for (int b = 0; b < B; ++b) sum[b] += odd[b];
 }
 
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// Premultiplied box sums of a 4-band window of fh pixels starting at the
+// 4-byte-aligned LDS byte sbyte (fh <= 257, so each packed 16-bit lane sum
+// stays below 2^16).  Per pixel, (r, b) and (g, a) travel as 16-bit pairs:
+// premultiply = round(c * a / 255) = (t + (t >> 8)) >> 8 with t = c * a + 128,
+// exact for all 8-bit c, a and equal to the (2ca + 255) / 510 of premult_u8;
+// alpha goes through the same form with the multiplier 255, which returns it
+// unchanged.
+__device__ __forceinline__ void premult_window_sums4(const unsigned char* rbuf, int sbyte, int fh,
+                                                     uint32_t (&sum)[4]) {
+    const uint32_t* l32 = reinterpret_cast<const uint32_t*>(rbuf + sbyte);
+    const u16x2 half = {128, 128};
+    u16x2 acc_rb = {0, 0}, acc_ga = {0, 0};
+    auto pixel = [&](uint32_t v) {
+        const uint32_t a = v >> 24;
+        const u16x2 aa = __builtin_bit_cast(u16x2, a | (a << 16));
+        const u16x2 a255 = __builtin_bit_cast(u16x2, a | (255u << 16));
+        u16x2 rb = __builtin_bit_cast(u16x2, v & 0x00ff00ffu) * aa + half;
+        u16x2 ga = __builtin_bit_cast(u16x2, (v >> 8) & 0x00ff00ffu) * a255 + half;
+        rb = (rb + (rb >> (u16x2){8, 8})) >> (u16x2){8, 8};
+        ga = (ga + (ga >> (u16x2){8, 8})) >> (u16x2){8, 8};
+        acc_rb += rb;
+        acc_ga += ga;
+    };
+    int j = 0;
+    for (; j + 2 <= fh; j += 2) {
+        const uint32_t v0 = l32[j], v1 = l32[j + 1];
+        pixel(v0);
+        pixel(v1);
+    }
+    if (j < fh) pixel(l32[j]);
+    const uint32_t rb = __builtin_bit_cast(uint32_t, acc_rb), ga = __builtin_bit_cast(uint32_t, acc_ga);
+    sum[0] = rb & 0xffffu;
+    sum[1] = ga & 0xffffu;
+    sum[2] = rb >> 16;
+    sum[3] = ga >> 16;
+}
+
 template <int B>
 __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, int by,
                                                   unsigned char* lds) {
@@ -469,6 +508,8 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
     const int wbytes = fh * B;
     const int X0 = cx0 + lane * fh;
     const bool fast = !premult && lane < ncols && X0 >= lo && X0 + fh <= hi;
+    // 4-band premultiplied windows inside the span: packed 16-bit path.
+    const bool fast_pm4 = B == 4 && premult && fh <= 257 && lane < ncols && X0 >= lo && X0 + fh <= hi;
     const int lane_byte = (X0 - lo) * B;
 
     // Row k of the wavefront's run: a16 = its span's 16-byte-aligned buffer
@@ -562,6 +603,8 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
         for (int b = 0; b < B; ++b) sum[b] = 0;
         if (fast) {
             window_sums<B>(slot, shift + lane_byte, wbytes, sum);
+        } else if (fast_pm4) {
+            if constexpr (B == 4) premult_window_sums4(slot, shift + lane_byte, fh, sum);
         } else if (lane < ncols) {
             for (int j = 0; j < fh; ++j) {
                 const unsigned char* px = slot + shift + (clampi(X0 + j, lo, hi - 1) - lo) * B;
