@@ -1,7 +1,9 @@
 """Benchmark: Lanczos3 thumbnail of an 8192x8192 3-band uchar image to 400 px
 wide on one MI355X (input Mpix/s, % of HBM peak, p99 step latency), plus, as
 line items of the same JSON object, the 4096x4096 4-band uchar
-gauss_blur(sigma=2), the 512-image uneven-size batched thumbnail and the
+gauss_blur(sigma=2), the 1024x768x3 uchar reduce_h -> reduce_v Lanczos3 by 3.7
+and the fused f64 reduce (ReduceOp) of the same image, the 512-image
+uneven-size batched thumbnail and the
 4096x4096x3 float32 reduce / gauss_blur in the three float modes and the
 open-loop thumbnail request trace at 500 / 2000 / 5000 requests/s.
 
@@ -92,6 +94,25 @@ def cpu_baseline(O, src_np, steps_fn, budget_s):
             break
     dt = time.perf_counter() - t0
     return (w * h * n / 1e6) / dt, threads, n
+
+
+def bench_line(v, O, ctx, src, steps_fn, steps, warmup, workload):
+    """One secondary pipeline on a resident image: parity against the oracle,
+    then timed like the headline (HIP events, back-to-back steps)."""
+    h, w, b = src.shape
+    gp = steps_fn(v.Pipeline(ctx, w, h, b))
+    d_src = v.DeviceBuffer.from_numpy(ctx, src)
+    d_dst = gp.output_buffer()
+    gp.run(d_src, d_dst)
+    ctx.synchronize()
+    exact = bool(np.array_equal(d_dst.numpy(), steps_fn(O.Pipeline(w, h, b)).run(src)))
+    ms, _ = time_pipeline(gp, d_src, d_dst, steps, warmup)
+    nbytes = src.nbytes + int(np.prod(gp.output_shape))
+    return {"workload": workload, "bit_exact": exact, "ms_per_step": round(ms, 5),
+            "Mpix/s": round(w * h / 1e6 / (ms / 1e3), 1), "algorithmic_bytes": nbytes,
+            "hbm_frac": round(nbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "stages": gp.stage_kernels(),
+            "stage_ms": [round(x, 5) for x in gp.run_profiled(d_src, d_dst, 10)]}
 
 
 def bench_batch(v, O, ctx, n, steps, warmup, check_images):
@@ -360,20 +381,16 @@ def main():
     if not args.no_secondary:
         print("bench: secondary gauss_blur", file=sys.stderr, flush=True)
         gs = np.random.default_rng(97).integers(0, 256, size=(4096, 4096, 4), dtype=np.uint8)
-        gp = v.Pipeline(ctx, 4096, 4096, 4).gauss_blur(2.0)
-        gd_src = v.DeviceBuffer.from_numpy(ctx, gs)
-        gd_dst = gp.output_buffer()
-        gp.run(gd_src, gd_dst)
-        ctx.synchronize()
-        gexact = bool(np.array_equal(gd_dst.numpy(), O.Pipeline(4096, 4096, 4).gauss_blur(2.0).run(gs)))
-        gms, glat = time_pipeline(gp, gd_src, gd_dst, args.steps, args.warmup)
-        gbytes = 2 * gs.nbytes
-        result["gauss_blur"] = {
-            "workload": "gauss_blur sigma=2.0 on 4096x4096x4 uchar", "bit_exact": gexact,
-            "ms_per_step": round(gms, 5), "Mpix/s": round(4096 * 4096 / 1e6 / (gms / 1e3), 1),
-            "hbm_frac": round(gbytes / (gms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-            "stage_ms": [round(x, 5) for x in gp.run_profiled(gd_src, gd_dst, 10)],
-        }
+        result["gauss_blur"] = bench_line(v, O, ctx, gs, lambda p: p.gauss_blur(2.0), args.steps, args.warmup,
+                                          "gauss_blur sigma=2.0 on 4096x4096x4 uchar")
+        print("bench: secondary reduce 1024x768", file=sys.stderr, flush=True)
+        rs = np.random.default_rng(31).integers(0, 256, size=(768, 1024, 3), dtype=np.uint8)
+        result["reduce_hv"] = bench_line(
+            v, O, ctx, rs, lambda p: p.reduce_h(3.7, v.LANCZOS3).reduce_v(3.7, v.LANCZOS3), args.steps,
+            args.warmup, "reduce_h(3.7) -> reduce_v(3.7) Lanczos3 on 1024x768x3 uchar -> 277x208")
+        result["reduce_op"] = bench_line(
+            v, O, ctx, rs, lambda p: p.reduce(3.7, 3.7, v.LANCZOS3), args.steps, args.warmup,
+            "reduce(3.7, 3.7) Lanczos3 (fused f64 ReduceOp) on 1024x768x3 uchar -> 277x208")
 
     if not args.no_batch and args.batch_images > 0:
         print("bench: uneven-size batch", file=sys.stderr, flush=True)
