@@ -173,7 +173,8 @@ int vgpu_pipeline_set_float_mode(vgpu_pipeline *p, int mode);
 /* Kernel tuning knobs (results do not change).  No reference counterpart. */
 typedef enum {
     VGPU_OPT_SEP_ROWS_PER_TASK = 2, /* output rows per MFMA wavefront task: 0 = auto, else a multiple of 16 */
-    VGPU_OPT_SHRINK_RUN = 3         /* box-shrink output rows per wavefront run: 0 = auto, 1..64 */
+    VGPU_OPT_SHRINK_RUN = 3,        /* box-shrink output rows per wavefront run: 0 = auto, 1..64 */
+    VGPU_OPT_GAUSS_ROWS = 4         /* u8 3/4-band Gaussian VALU tile rows: 0 = auto, 16 or 32; -1 = the MFMA kernel */
 } vgpu_option;
 int vgpu_pipeline_set_option(vgpu_pipeline *p, int option, int value);
 /* CompiledPipeline output geometry (pipeline/compiled.rs:338-). */

@@ -136,6 +136,28 @@ def test_gauss_u8(ctx, sigma, bands):
     assert_same(got, exp, f"gauss {sigma}")
 
 
+@pytest.mark.parametrize("rows", [0, 16, 32, -1])
+@pytest.mark.parametrize("bands", [3, 4])
+def test_gauss_u8_forms(ctx, bands, rows):
+    # 3/4-band u8 GaussBlurH -> GaussBlurV: VALU tiles of 16 or 32 rows
+    # (0 = chosen by the planner; radius <= 6 and scale <= 256), else
+    # (rows = -1, or a wider mask) the int8-limb MFMA kernel.  Ragged and tiny sizes, and a Gaussian after a
+    # thumbnail (its input is an intermediate with Z^2 halo rows).
+    for h, w in ((301, 403), (70, 64), (1, 1), (2, 3), (5, 130), (129, 1)):
+        src = noise_u8(h, w, bands, 31 + h + w)
+        for sigma in (0.8, 1.5, 2.0, 3.0, 4.0):
+            gp = v.Pipeline(ctx, w, h, bands).gauss_blur(sigma).set_option(v.api.OPT_GAUSS_ROWS, rows)
+            exp = O.Pipeline(w, h, bands).gauss_blur(sigma).run(src)
+            assert_same(gp.run_host(src), exp, f"{w}x{h}x{bands} sigma {sigma} rows {rows}")
+            if rows >= 0 and sigma <= 3.0:
+                assert gp.stage_kernels() == ["gauss_u8"], gp.stage_kernels()
+    src = noise_u8(517, 777, bands, 5)
+    steps = [("thumbnail", (v.WIDTH, 300, 0, v.LANCZOS3)), ("gauss_blur", (0.8,))]
+    gp = build(v.Pipeline(ctx, 777, 517, bands), steps).set_option(v.api.OPT_GAUSS_ROWS, rows)
+    exp = build(O.Pipeline(777, 517, bands), steps).run(src)
+    assert_same(gp.run_host(src), exp, f"thumbnail + blur rows {rows}")
+
+
 @pytest.mark.parametrize("f", [1, 2, 3, 5, 7, 19, 40])
 @pytest.mark.parametrize("bands", [1, 2, 3, 4, 5])
 def test_shrink_factors_and_bands(ctx, f, bands):

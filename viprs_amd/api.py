@@ -36,7 +36,7 @@ E_NO_MEMORY = -7
 E_CANCELLED = -8
 E_ADMISSION = -9
 E_NOT_READY = -10
-OPT_SEP_ROWS_PER_TASK, OPT_SHRINK_RUN = 2, 3
+OPT_SEP_ROWS_PER_TASK, OPT_SHRINK_RUN, OPT_GAUSS_ROWS = 2, 3, 4
 
 NP_DTYPE = {U8: np.uint8, U16: np.uint16, F32: np.float32}
 SAMPLE_BYTES = {U8: 1, U16: 2, F32: 4}

@@ -520,6 +520,10 @@ int vgpu_batch_set_option(vgpu_batch* b, int option, int value) {
         if (value < 0 || value > 64) return VGPU_E_INVALID_ARGUMENT;
         b->opt.shrink_run = value;
         break;
+    case VGPU_OPT_GAUSS_ROWS:
+        if (value != -1 && value != 0 && value != 16 && value != 32) return VGPU_E_INVALID_ARGUMENT;
+        b->opt.gauss_rows = value;
+        break;
     default: return VGPU_E_INVALID_ARGUMENT;
     }
     b->drop_graph();

@@ -1,0 +1,208 @@
+// GaussBlurH -> GaussBlurV for 3- and 4-band u8 images on the VALU, one
+// tile per block (gauss_blur.rs:405-518 integer path: sum of c_t * p, then
+// (sum + scale / 2) / scale, H rounded to u8 before V).
+//
+// A pixel travels as two packed 16-bit pairs, (b0, b2) and (b1, b3), so one
+// v_pk_mad_u16 applies a tap to two bands.  The sums stay below 2^16 when
+// 255 * scale + scale / 2 < 2^16 (scale <= 256), which the host checks.  The
+// division by scale is an exact multiply-high: with mul = ceil(2^32 / scale),
+// floor(x * mul / 2^32) = floor(x / scale) for every x < 256 * scale.
+//
+// Block = 64 output columns x TH output rows.  The input tile (TH + 2R rows x
+// 64 + 2R columns, read through the clamped view, i.e. the reference's
+// clamp-to-edge source and its Z^2 intermediate rows) is split into LDS once;
+// the H pass writes the TH + 2R intermediate rows the V taps need, rounded,
+// to LDS as pairs; the V pass gives each lane one column and each wavefront
+// TH / 4 consecutive rows, with the 2R + TH / 4 intermediate rows it reads
+// held in registers.
+#include "device.hpp"
+
+namespace vgpu {
+namespace {
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ u16x2 as_pair(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// (x + scale/2) / scale for the two 16-bit halves of a pair (the half-scale
+// bias is already in x); mul == 0 means scale == 1.
+__device__ __forceinline__ uint32_t div_pair(uint32_t x, uint32_t mul) {
+    const uint32_t lo = x & 0xffffu, hi = x >> 16;
+    if (mul == 0) return lo | (hi << 16);
+    return __umulhi(lo, mul) | (__umulhi(hi, mul) << 16);
+}
+
+template <int B, int R, int TH>
+__global__ __launch_bounds__(kThreads) void k_gauss_u8(GaussJob job) {
+    constexpr int TW = 64;
+    constexpr int IW = TW + 2 * R;  // input tile columns (<= 128: two per lane)
+    constexpr int NR = TH + 2 * R;  // input / intermediate tile rows
+    constexpr int VR = TH / 4;      // output rows per wavefront
+    constexpr int KR = (NR + 3) / 4;  // input rows per wavefront
+    constexpr int IH = IW / 2;        // input columns of one parity
+    // (b0, b2) and (b1, b3) pairs side by side: one 8-byte LDS access a pixel.
+    // The input tile keeps even and odd columns apart, so a lane that makes
+    // two adjacent H outputs reads 2R + 2 consecutive entries, conflict-free.
+    __shared__ uint2 in_e[NR * IH], in_o[NR * IH];
+    __shared__ __align__(16) uint2 mid_px[NR * TW];
+    const InView& in = job.in;
+    const OutView& out = job.out;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ox0 = (int)blockIdx.x * TW;
+    // This block's tiles: rows [t0, t1) of the column of tiles.
+    const int t0 = (int)blockIdx.y * job.tiles_per_block;
+    const int t1 = min(t0 + job.tiles_per_block, (out.h + TH - 1) / TH);
+    const unsigned char* src = static_cast<const unsigned char*>(in.ptr);
+    const size_t pitch = (size_t)in.bw * B;
+
+    // Input tile of tile row tr: rows out.y + tr * TH - R + rr (row
+    // addresses wave-uniform), columns out.x + ox0 - R + cc for cc = lane and
+    // lane + 64 (byte offsets per lane, computed once).
+    const int cc1 = lane + 64;
+    const uint32_t cb0 = (uint32_t)col_of(in, out.x + ox0 - R + lane) * B;
+    const uint32_t cb1 = (uint32_t)col_of(in, out.x + ox0 - R + min(cc1, IW - 1)) * B;
+    auto load_px = [&](const unsigned char* rowp, uint32_t cb) -> uint32_t {
+        if constexpr (B == 4) {
+            return gmem<uint32_t>(rowp + cb)[0];
+        } else {
+            const auto* g = gmem<unsigned char>(rowp + cb);
+            return (uint32_t)g[0] | ((uint32_t)g[1] << 8) | ((uint32_t)g[2] << 16);
+        }
+    };
+    auto split = [](uint32_t v) { return make_uint2(v & 0x00ff00ffu, (v >> 8) & 0x00ff00ffu); };
+    // A wavefront's loads of one tile, all issued before any is used; the
+    // next tile's are in flight while the current one is computed.
+    uint32_t v0[KR], v1[KR];
+    auto fetch = [&](int tr) {
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int rr = wave + 4 * k;
+            if (rr < NR) {
+                const unsigned char* rowp = src + (size_t)row_of(in, out.y + tr * TH - R + rr) * pitch;
+                v0[k] = load_px(rowp, cb0);
+                v1[k] = load_px(rowp, cb1);
+            }
+        }
+    };
+    const u16x2 half = as_pair(job.half2);
+    unsigned char* dst = static_cast<unsigned char*>(out.ptr);
+    const int ox = ox0 + lane;
+    const int r0 = wave * VR;
+    if (t0 < t1) fetch(t0);
+    for (int tr = t0; tr < t1; ++tr) {
+        uint2* in0 = (lane & 1) ? in_o : in_e;  // column lane and lane + 64 share its parity
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int rr = wave + 4 * k;
+            if (rr < NR) {
+                in0[rr * IH + (lane >> 1)] = split(v0[k]);
+                if (cc1 < IW) in0[rr * IH + (cc1 >> 1)] = split(v1[k]);
+            }
+        }
+        __syncthreads();  // the tile is in LDS; the previous V pass is done with mid_px
+        if (tr + 1 < t1) fetch(tr + 1);
+
+        // H pass: intermediate row 2 * rp + (lane >> 5), columns 2m and
+        // 2m + 1 (m = lane & 31): even input columns 2m .. 2m + 2R and odd
+        // ones 2m + 1 .. 2m + 2R + 1 are in_e[m .. m + R], in_o[m .. m + R].
+        const int m = lane & 31;
+        for (int rp = wave; rp < (NR + 1) / 2; rp += 4) {
+            const int rr = 2 * rp + (lane >> 5);
+            if (rr < NR) {
+                uint2 e[R + 1], o[R + 1];
+#pragma unroll
+                for (int k = 0; k <= R; ++k) {
+                    e[k] = in_e[rr * IH + m + k];
+                    o[k] = in_o[rr * IH + m + k];
+                }
+                u16x2 a0 = half, b0 = half, a1 = half, b1 = half;
+#pragma unroll
+                for (int t = 0; t <= 2 * R; ++t) {
+                    const u16x2 w = as_pair(job.taps2[t]);
+                    // column 2m + t, and 2m + 1 + t
+                    const uint2 x0 = (t & 1) ? o[t >> 1] : e[t >> 1];
+                    const uint2 x1 = (t & 1) ? e[(t + 1) >> 1] : o[t >> 1];
+                    a0 = as_pair(x0.x) * w + a0;
+                    b0 = as_pair(x0.y) * w + b0;
+                    a1 = as_pair(x1.x) * w + a1;
+                    b1 = as_pair(x1.y) * w + b1;
+                }
+                uint4 q;
+                q.x = div_pair(as_u32(a0), job.mul);
+                q.y = div_pair(as_u32(b0), job.mul);
+                q.z = div_pair(as_u32(a1), job.mul);
+                q.w = div_pair(as_u32(b1), job.mul);
+                *reinterpret_cast<uint4*>(mid_px + rr * TW + 2 * m) = q;
+            }
+        }
+        __syncthreads();  // mid_px is complete; the input tile may be overwritten
+
+        // V pass: output rows tr * TH + wave * VR .. + VR - 1, column ox.
+        uint2 win[VR + 2 * R];
+#pragma unroll
+        for (int j = 0; j < VR + 2 * R; ++j) win[j] = mid_px[(r0 + j) * TW + lane];
+        if (ox < out.w) {
+#pragma unroll
+            for (int o = 0; o < VR; ++o) {
+                const int oy = tr * TH + r0 + o;
+                if (oy >= out.h) break;
+                u16x2 a = half, b = half;
+#pragma unroll
+                for (int t = 0; t <= 2 * R; ++t) {
+                    const u16x2 w = as_pair(job.taps2[t]);
+                    a = as_pair(win[o + t].x) * w + a;
+                    b = as_pair(win[o + t].y) * w + b;
+                }
+                const uint32_t qrb = div_pair(as_u32(a), job.mul), qga = div_pair(as_u32(b), job.mul);
+                const uint32_t v = qrb | (qga << 8);  // bytes b0 b1 b2 b3
+                unsigned char* o8 = dst + ((size_t)oy * out.w + ox) * B;
+                if constexpr (B == 4) {
+                    *reinterpret_cast<uint32_t*>(o8) = v;
+                } else {
+                    o8[0] = (unsigned char)v;
+                    o8[1] = (unsigned char)(v >> 8);
+                    o8[2] = (unsigned char)(v >> 16);
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+bool gauss_u8_supported(int bands, int taps, int32_t scale) {
+    const int r = (taps - 1) / 2;
+    return (bands == 3 || bands == 4) && taps == 2 * r + 1 && r >= 1 && r <= kGaussU8MaxRadius && scale >= 1 &&
+           scale <= 256;
+}
+
+hipError_t launch_gauss_u8(const GaussJob& job, hipStream_t s) {
+    if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;
+    const dim3 grid((unsigned)job.gx, (unsigned)job.gy);
+#define VGPU_G(BB, RR, TT) \
+    hipLaunchKernelGGL((k_gauss_u8<BB, RR, TT>), grid, dim3(kThreads), 0, s, job)
+#define VGPU_GR(BB, TT)                 \
+    switch (job.radius) {               \
+    case 1: VGPU_G(BB, 1, TT); break;   \
+    case 2: VGPU_G(BB, 2, TT); break;   \
+    case 3: VGPU_G(BB, 3, TT); break;   \
+    case 4: VGPU_G(BB, 4, TT); break;   \
+    case 5: VGPU_G(BB, 5, TT); break;   \
+    case 6: VGPU_G(BB, 6, TT); break;   \
+    default: return hipErrorInvalidValue; \
+    }
+    if (job.bands == 4) {
+        if (job.th == 32) { VGPU_GR(4, 32) } else { VGPU_GR(4, 16) }
+    } else if (job.bands == 3) {
+        if (job.th == 32) { VGPU_GR(3, 32) } else { VGPU_GR(3, 16) }
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef VGPU_GR
+#undef VGPU_G
+    return hipGetLastError();
+}
+
+}  // namespace vgpu

@@ -164,6 +164,23 @@ struct F16Job {
 };
 size_t f16_job_wave_lds(const F16Job& job);
 hipError_t launch_f16_job(const F16Job& job, hipStream_t s);
+
+// GaussBlurH -> GaussBlurV on 3- or 4-band u8 on the VALU (packed 16-bit
+// sums): a block walks tiles_per_block 64 x th output tiles down a column.  taps2[t] holds tap t in both
+// 16-bit halves, half2 = (scale / 2) in both halves, mul = ceil(2^32 / scale)
+// (0 when scale == 1).
+constexpr int kGaussU8MaxRadius = 6;
+struct GaussJob {
+    InView in;
+    OutView out;
+    int bands = 4, radius = 1, th = 16;
+    uint32_t taps2[2 * kGaussU8MaxRadius + 1] = {};
+    uint32_t half2 = 0, mul = 0;
+    int tiles_per_block = 1;  // consecutive tile rows one block walks down
+    int gx = 0, gy = 0;
+};
+bool gauss_u8_supported(int bands, int taps, int32_t scale);
+hipError_t launch_gauss_u8(const GaussJob& job, hipStream_t s);
 hipError_t launch_conv(const InView& in, const OutView& out, int bands, int in_fmt, int out_fmt,
                        int kind, int vertical, const int16_t* itaps, int32_t iscale,
                        const double* wts, int ntaps, double cscale, double coffset,

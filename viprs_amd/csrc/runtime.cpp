@@ -676,6 +676,37 @@ bool stage_gauss_sep_params(const Node& gh, const Node& gv, const Rect& mid, con
     return true;
 }
 
+// GaussBlurH -> GaussBlurV on 3- or 4-band u8 as one VALU stage (packed
+// 16-bit sums, the integer taps, (sum + scale / 2) / scale per pass).
+bool stage_gauss_u8_params(const Node& gh, const Node& gv, const Rect& out, const StageOptions& opt, Stage* s) {
+    const int taps = (int)gh.itaps.size();
+    if (opt.gauss_rows < 0) return false;
+    if (gv.itaps != gh.itaps || gh.iscale != gv.iscale || !gauss_u8_supported((int)gh.bands, taps, gh.iscale))
+        return false;
+    GaussJob& j = s->gauss;
+    j.bands = (int)gh.bands;
+    j.radius = (taps - 1) / 2;
+    // 32-row tiles pay off once the mask is 5 taps or wider (fewer halo rows
+    // per output row); 16 for 3 taps (measured on 4096x4096 images).
+    j.th = opt.gauss_rows ? opt.gauss_rows : (j.radius >= 2 ? 32 : 16);
+    for (int t = 0; t < taps; ++t) {
+        const uint32_t w = (uint32_t)(uint16_t)gh.itaps[(size_t)t];
+        j.taps2[t] = w | (w << 16);
+    }
+    const uint32_t half = (uint32_t)(gh.iscale / 2);
+    j.half2 = half | (half << 16);
+    j.mul = gh.iscale == 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + (uint64_t)gh.iscale - 1) / (uint64_t)gh.iscale);
+    j.gx = (int)((out.w + 63) / 64);
+    // Blocks walk down their column of tiles (the next tile's loads in flight
+    // while one is computed); enough blocks for about 8 a CU.
+    const int64_t rows = (out.h + j.th - 1) / j.th;
+    const int64_t want = std::max<int64_t>(1, (int64_t)256 * 8 / std::max(j.gx, 1));
+    j.tiles_per_block = (int)std::max<int64_t>(1, (rows + want - 1) / want);
+    j.gy = (int)((rows + j.tiles_per_block - 1) / j.tiles_per_block);
+    s->kind = ST_GAUSS_U8;
+    return true;
+}
+
 // GaussBlurH -> GaussBlurV on f32 images in plain FP32: the float kernel
 // taps / scale (gauss_blur.rs G1) as f32 weights, one phase.
 bool stage_gauss_float_params(const Node& gh, const Node& gv, const Rect& mid, const Rect& out, const Rect& vclip,
@@ -847,6 +878,13 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
     case ST_REDUCE_HV: e = launch_reduce_job(reduce_job_of(s, in, out, blob), stream); break;
     case ST_SEP: e = launch_sep_job(sep_job_of(s, in, out, blob), stream); break;
     case ST_RED_F32: e = launch_red_f32_job(reduce_job_of(s, in, out, blob), stream); break;
+    case ST_GAUSS_U8: {
+        GaussJob j = s.gauss;
+        j.in = in;
+        j.out = out;
+        e = launch_gauss_u8(j, stream);
+        break;
+    }
     case ST_SEP_F16: {
         F16Job j = s.f16;
         j.in = in;
@@ -965,6 +1003,12 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
                 s.last = i + 1;
                 done = true;
             }
+        }
+        if (!done && u8small && nd.kind == N_GAUSS_H && kind_at(i + 1) == N_GAUSS_V && nd.out_fmt == U8 &&
+            nodes[(size_t)i + 1].out_fmt == U8 && stage_gauss_u8_params(nd, nodes[(size_t)i + 1], out_r[(size_t)i + 1], opt, &s)) {
+            // GaussBlurH -> GaussBlurV (u8 -> u8, 3 or 4 bands) on VALU tiles.
+            s.last = i + 1;
+            done = true;
         }
         if (!done && u8small && opt.integer_mfma && nd.kind == N_GAUSS_H && kind_at(i + 1) == N_GAUSS_V &&
             nd.out_fmt == U8 && nodes[(size_t)i + 1].out_fmt == U8) {
@@ -1405,6 +1449,11 @@ int vgpu_pipeline_set_option(vgpu_pipeline* p, int option, int value) {
         p->release();
         p->opt.shrink_run = value;
         return VGPU_OK;
+    case VGPU_OPT_GAUSS_ROWS:
+        if (value != -1 && value != 0 && value != 16 && value != 32) return VGPU_E_INVALID_ARGUMENT;
+        p->release();
+        p->opt.gauss_rows = value;
+        return VGPU_OK;
     default: return VGPU_E_INVALID_ARGUMENT;
     }
 }
@@ -1422,6 +1471,7 @@ namespace {
 const char* stage_kernel_name(const Stage& s) {
     switch (s.kind) {
     case ST_SEP: return "sep_mfma_i8";
+    case ST_GAUSS_U8: return "gauss_u8";
     case ST_RED_F32: return "reduce_f32";
     case ST_SEP_F16: return "sep_mfma_f16x3";
     case ST_SHRINK_HV_U8: return "shrink_hv";

@@ -109,6 +109,7 @@ enum StageKind {
     ST_SEP,      // fused u8 H -> V pair on MFMA (reduce, Gaussian or box)
     ST_RED_F32,  // fused f32 H -> V pair in plain FP32 (reduce or Gaussian)
     ST_SEP_F16,  // fused f32 H -> V pair, FP16 hi/lo split on MFMA
+    ST_GAUSS_U8, // fused u8 GaussBlurH -> GaussBlurV, 3 or 4 bands, VALU tiles
     ST_SHRINK_HV_U8,
     ST_SHRINK,
     ST_REDUCE_H,
@@ -140,6 +141,7 @@ struct Stage {
     size_t v_tab = 0, v_tile = 0, h_tab = 0, h_strip = 0;
     size_t w32 = 0, v_w32 = 0;  // ST_RED_F32: f32 weight tables
     F16Job f16;                 // ST_SEP_F16: job template (table offsets below)
+    GaussJob gauss;             // ST_GAUSS_U8: job template (views set at launch)
 };
 
 // Stage-planning choices that depend on how the stage is launched: a single
@@ -156,6 +158,7 @@ struct StageOptions {
     int float_mode = 0;
     bool shrink_row_split = true;      // split a shrink row's source rows over wavefronts
     int shrink_run = 0;                // shrink output rows per wavefront run (0 = planner's choice)
+    int gauss_rows = 0;                // u8 3/4-band Gaussian pair: VALU tile rows (0 = auto, 16, 32), -1 = MFMA kernel
 };
 
 Rect whole(uint32_t w, uint32_t h);
