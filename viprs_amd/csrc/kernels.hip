@@ -282,6 +282,49 @@ __device__ __forceinline__ void premult_window_sums4(const unsigned char* rbuf, 
     sum[3] = ga >> 16;
 }
 
+// window_sums for a window of exactly ND dwords (ND = ceil(wbytes / 4),
+// known per job): fully unrolled, the ND + 1 dwords it spans read at once,
+// only the last dword masked (last_mask keeps its valid bytes).
+template <int B, int ND>
+__device__ __forceinline__ void window_sums_n(const unsigned char* rbuf, int sbyte, uint32_t last_mask,
+                                              uint32_t (&sum)[B]) {
+    const uint32_t* l32 = reinterpret_cast<const uint32_t*>(rbuf) + (sbyte >> 2);
+    const int al = sbyte & 3;
+    uint32_t w[ND + 1];
+#pragma unroll
+    for (int d = 0; d <= ND; ++d) w[d] = l32[d];
+    uint32_t odd[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) odd[b] = 0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        uint32_t v = __builtin_amdgcn_alignbyte(w[d + 1], w[d], al);
+        if (d == ND - 1) v &= last_mask;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            if (d & 1) odd[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, d % 3), odd[b], false);
+            else sum[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, d % 3), sum[b], false);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) sum[b] += odd[b];
+}
+
+// Window sums with the window length (in dwords) dispatched once per call
+// to an unrolled form; longer windows take the looped form.
+template <int B>
+__device__ __forceinline__ void window_sums_any(const unsigned char* rbuf, int sbyte, int wbytes, int nd,
+                                                uint32_t last_mask, uint32_t (&sum)[B]) {
+    switch (nd) {
+#define VGPU_WN(N) \
+    case N: window_sums_n<B, N>(rbuf, sbyte, last_mask, sum); return;
+        VGPU_WN(1) VGPU_WN(2) VGPU_WN(3) VGPU_WN(4) VGPU_WN(5) VGPU_WN(6) VGPU_WN(7) VGPU_WN(8)
+        VGPU_WN(9) VGPU_WN(10) VGPU_WN(11) VGPU_WN(12) VGPU_WN(13) VGPU_WN(14) VGPU_WN(15) VGPU_WN(16)
+#undef VGPU_WN
+    default: window_sums<B>(rbuf, sbyte, wbytes, sum); return;
+    }
+}
+
 template <int B>
 __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, int by,
                                                   unsigned char* lds) {
@@ -511,6 +554,9 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
     // 4-band premultiplied windows inside the span: packed 16-bit path.
     const bool fast_pm4 = B == 4 && premult && fh <= 257 && lane < ncols && X0 >= lo && X0 + fh <= hi;
     const int lane_byte = (X0 - lo) * B;
+    const int wnd = (wbytes + 3) >> 2;  // window dwords
+    const int wrem = wbytes - 4 * (wnd - 1);
+    const uint32_t wlast = wrem >= 4 ? 0xffffffffu : ((1u << (8 * wrem)) - 1u);
 
     // Row k of the wavefront's run: a16 = its span's 16-byte-aligned buffer
     // offset, lb = where its chunks are loaded from (a16, or 0 -- the
@@ -602,7 +648,7 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
 #pragma unroll
         for (int b = 0; b < B; ++b) sum[b] = 0;
         if (fast) {
-            window_sums<B>(slot, shift + lane_byte, wbytes, sum);
+            window_sums_any<B>(slot, shift + lane_byte, wbytes, wnd, wlast, sum);
         } else if (fast_pm4) {
             if constexpr (B == 4) premult_window_sums4(slot, shift + lane_byte, fh, sum);
         } else if (lane < ncols) {
