@@ -47,8 +47,13 @@ struct Launch {
     int ntasks = 0;
     size_t lds = 0;
     uint64_t alg_bytes = 0;  // input buffers read + outputs written, summed over jobs
-    int chain = -1;          // band count - 1 of every image it serves, or -1 for mixed
+    int chain = -1;          // (band count - 1) * kGroups + size group of its images, or -1 for mixed
 };
+
+// Images are dealt round-robin, largest first, into this many groups per band
+// count; each (band count, group) is a chain of launches on its own stream.
+constexpr int kGroups = 2;
+constexpr int kChains = 4 * kGroups;
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -74,15 +79,16 @@ struct vgpu_batch {
     int grid_cap = 0;    // persistent grid size; 0 = one block per task
     int use_graph = 1;
     hipGraphExec_t graph = nullptr;
-    // Launches that serve images of one band count form a chain; chains run on
-    // their own streams, forked from and joined back into the context stream.
-    hipStream_t side[4] = {};
-    hipEvent_t fork_ev = nullptr, join_ev[4] = {};
+    // Launches that serve images of one band count and size group form a
+    // chain; chains run on their own streams, forked from and joined back into
+    // the context stream.
+    hipStream_t side[kChains] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kChains] = {};
     unsigned chains = 0;  // bit c: some launch runs on chain c
 
     ~vgpu_batch() {
         release();
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < kChains; ++c) {
             if (side[c]) (void)hipStreamDestroy(side[c]);
             if (join_ev[c]) (void)hipEventDestroy(join_ev[c]);
         }
@@ -204,19 +210,23 @@ int vgpu_batch::plan() {
     n_shrink_tasks = n_reduce_tasks = 0;
     for (int lv = 0; lv < levels; ++lv) {
         constexpr int kShrinkVariants = 8;  // shrink_job_variant
-        std::vector<ShrinkJob> sjobs[kShrinkVariants];
-        std::vector<BatchTask> stasks[kShrinkVariants];
-        uint64_t sbytes[kShrinkVariants] = {};
+        constexpr int kS = kShrinkVariants * kGroups;  // (variant, group)
+        std::vector<ShrinkJob> sjobs[kS];
+        std::vector<BatchTask> stasks[kS];
+        uint64_t sbytes[kS] = {};
         std::vector<ReduceJob> rjobs;
         std::vector<BatchTask> rtasks;
         size_t rlds = 0;
         uint64_t rbytes = 0;
         constexpr int kSepVariants = 32;  // sep_job_variant
-        std::vector<SepJob> pjobs[kSepVariants];
-        std::vector<BatchTask> ptasks[kSepVariants];
-        size_t plds[kSepVariants] = {};
-        uint64_t pbytes[kSepVariants] = {};
-        for (int idx : order) {
+        constexpr int kP = kSepVariants * kGroups;
+        std::vector<SepJob> pjobs[kP];
+        std::vector<BatchTask> ptasks[kP];
+        size_t plds[kP] = {};
+        uint64_t pbytes[kP] = {};
+        for (size_t rank = 0; rank < order.size(); ++rank) {
+            const int idx = order[rank];
+            const int grp = (int)(rank % kGroups);
             BatchImage& im = images[(size_t)idx];
             if (!im.batched || lv >= (int)im.stages.size()) continue;
             const Stage& s = im.stages[(size_t)lv];
@@ -235,7 +245,7 @@ int vgpu_batch::plan() {
                     im.batched = false;
                     continue;
                 }
-                const int var = shrink_job_variant(j);
+                const int var = shrink_job_variant(j) * kGroups + grp;
                 const int jid = (int)sjobs[var].size();
                 sjobs[var].push_back(j);
                 sbytes[var] += io_bytes;
@@ -245,7 +255,7 @@ int vgpu_batch::plan() {
                 SepJob j = sep_job_of(s, in, out, dblob);
                 j.src_slot = first ? idx : -1;
                 j.dst_slot = last ? idx : -1;
-                const int var = sep_job_variant(j);
+                const int var = sep_job_variant(j) * kGroups + grp;
                 const int jid = (int)pjobs[var].size();
                 pjobs[var].push_back(j);
                 pbytes[var] += io_bytes;
@@ -264,31 +274,33 @@ int vgpu_batch::plan() {
                     for (int bx = 0; bx < j.gx; ++bx) rtasks.push_back(BatchTask{jid, bx, by});
             }
         }
-        for (int var = 0; var < kShrinkVariants; ++var) {
-            if (stasks[var].empty()) continue;
+        for (int vg = 0; vg < kS; ++vg) {
+            if (stasks[vg].empty()) continue;
+            const int var = vg / kGroups;
             Launch l;
             l.family = 0;
             l.variant = var;
-            l.chain = var % 4;  // shrink_job_variant: bands - 1 (+ 4 for the lean form)
-            l.jobs_off = tbl.put(sjobs[var].data(), sjobs[var].size() * sizeof(ShrinkJob));
-            l.tasks_off = tbl.put(stasks[var].data(), stasks[var].size() * sizeof(BatchTask));
-            l.ntasks = (int)stasks[var].size();
+            l.chain = (var % 4) * kGroups + vg % kGroups;  // shrink_job_variant: bands - 1 (+ 4 lean)
+            l.jobs_off = tbl.put(sjobs[vg].data(), sjobs[vg].size() * sizeof(ShrinkJob));
+            l.tasks_off = tbl.put(stasks[vg].data(), stasks[vg].size() * sizeof(BatchTask));
+            l.ntasks = (int)stasks[vg].size();
             l.lds = shrink_job_lds();
-            l.alg_bytes = sbytes[var];
+            l.alg_bytes = sbytes[vg];
             launches.push_back(l);
             n_shrink_tasks += l.ntasks;
         }
-        for (int var = 0; var < kSepVariants; ++var) {
-            if (ptasks[var].empty()) continue;
+        for (int vg = 0; vg < kP; ++vg) {
+            if (ptasks[vg].empty()) continue;
+            const int var = vg / kGroups;
             Launch l;
             l.family = 2;
             l.variant = var;
-            l.chain = var >> 3;  // sep_job_variant: bands - 1 in the high bits
-            l.jobs_off = tbl.put(pjobs[var].data(), pjobs[var].size() * sizeof(SepJob));
-            l.tasks_off = tbl.put(ptasks[var].data(), ptasks[var].size() * sizeof(BatchTask));
-            l.ntasks = (int)ptasks[var].size();
-            l.lds = plds[var];  // per wavefront
-            l.alg_bytes = pbytes[var];
+            l.chain = (var >> 3) * kGroups + vg % kGroups;  // sep_job_variant: bands - 1 in the high bits
+            l.jobs_off = tbl.put(pjobs[vg].data(), pjobs[vg].size() * sizeof(SepJob));
+            l.tasks_off = tbl.put(ptasks[vg].data(), ptasks[vg].size() * sizeof(BatchTask));
+            l.ntasks = (int)ptasks[vg].size();
+            l.lds = plds[vg];  // per wavefront
+            l.alg_bytes = pbytes[vg];
             launches.push_back(l);
             n_reduce_tasks += l.ntasks;
         }
@@ -307,7 +319,7 @@ int vgpu_batch::plan() {
     chains = 0;
     for (const Launch& l : launches)
         if (l.chain >= 0) chains |= 1u << l.chain;
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < kChains; ++c) {
         if (!(chains & (1u << c)) || side[c]) continue;
         // Default priority, as the context stream.
         if (hipStreamCreateWithFlags(&side[c], hipStreamNonBlocking) != hipSuccess ||
@@ -348,12 +360,12 @@ int vgpu_batch::launch_all(hipStream_t s, hipEvent_t* marks) {
     const bool par = !marks && __builtin_popcount(chains) >= 2;
     auto fork = [&]() -> bool {
         if (hipEventRecord(fork_ev, s) != hipSuccess) return false;
-        for (int c = 0; c < 4; ++c)
+        for (int c = 0; c < kChains; ++c)
             if ((chains & (1u << c)) && hipStreamWaitEvent(side[c], fork_ev, 0) != hipSuccess) return false;
         return true;
     };
     auto join = [&]() -> bool {
-        for (int c = 0; c < 4; ++c)
+        for (int c = 0; c < kChains; ++c)
             if ((chains & (1u << c)) &&
                 (hipEventRecord(join_ev[c], side[c]) != hipSuccess || hipStreamWaitEvent(s, join_ev[c], 0) != hipSuccess))
                 return false;
