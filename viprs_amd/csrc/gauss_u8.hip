@@ -63,12 +63,18 @@ __global__ __launch_bounds__(kThreads) void k_gauss_u8(GaussJob job) {
     const int cc1 = lane + 64;
     const uint32_t cb0 = (uint32_t)col_of(in, out.x + ox0 - R + lane) * B;
     const uint32_t cb1 = (uint32_t)col_of(in, out.x + ox0 - R + min(cc1, IW - 1)) * B;
+    // A 3-band pixel is read as the two aligned dwords holding its bytes,
+    // then realigned and masked.  For the buffer's last pixel the second
+    // dword may run up to 3 bytes past the buffer's end; an aligned dword
+    // never crosses a page, so those bytes are mapped, and they are dropped.
     auto load_px = [&](const unsigned char* rowp, uint32_t cb) -> uint32_t {
         if constexpr (B == 4) {
             return gmem<uint32_t>(rowp + cb)[0];
         } else {
-            const auto* g = gmem<unsigned char>(rowp + cb);
-            return (uint32_t)g[0] | ((uint32_t)g[1] << 8) | ((uint32_t)g[2] << 16);
+            const uintptr_t a = (uintptr_t)(rowp + cb);
+            const uint32_t lo = gmem<uint32_t>((const void*)(a & ~(uintptr_t)3))[0];
+            const uint32_t hi = gmem<uint32_t>((const void*)((a + 2) & ~(uintptr_t)3))[0];
+            return __builtin_amdgcn_alignbyte(hi, lo, (uint32_t)(a & 3)) & 0x00ffffffu;
         }
     };
     auto split = [](uint32_t v) { return make_uint2(v & 0x00ff00ffu, (v >> 8) & 0x00ff00ffu); };
