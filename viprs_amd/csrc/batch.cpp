@@ -209,7 +209,7 @@ int vgpu_batch::plan() {
     Staging tbl;
     n_shrink_tasks = n_reduce_tasks = 0;
     for (int lv = 0; lv < levels; ++lv) {
-        constexpr int kShrinkVariants = 8;  // shrink_job_variant
+        constexpr int kShrinkVariants = 12;  // shrink_job_variant
         constexpr int kS = kShrinkVariants * kGroups;  // (variant, group)
         std::vector<ShrinkJob> sjobs[kS];
         std::vector<BatchTask> stasks[kS];
@@ -280,7 +280,7 @@ int vgpu_batch::plan() {
             Launch l;
             l.family = 0;
             l.variant = var;
-            l.chain = (var % 4) * kGroups + vg % kGroups;  // shrink_job_variant: bands - 1 (+ 4 lean)
+            l.chain = (var % 4) * kGroups + vg % kGroups;  // shrink_job_variant: bands - 1 (+ 4 / 8 lean)
             l.jobs_off = tbl.put(sjobs[vg].data(), sjobs[vg].size() * sizeof(ShrinkJob));
             l.tasks_off = tbl.put(stasks[vg].data(), stasks[vg].size() * sizeof(BatchTask));
             l.ntasks = (int)stasks[vg].size();

@@ -521,13 +521,13 @@ __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, 
 // its last one.  Parking is four unconditional 16-byte LDS stores; a chunk
 // cut by the end of the buffer (scalar flag) is then copied bytewise.
 // Output rows complete on a counter instead of a division per row.
-template <int B>
+template <int B, bool TWO>
 __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, int by, unsigned char* lds) {
     const InView& in = job.in;
     const OutView& out = job.out;
     const int fh = job.fh, fv = job.fv, tx = job.tx, S = job.S, m = job.m, lsl = job.lsl;
-    const bool two = lsl == 7;  // two source rows per fetch group (registers 0-1, 2-3)
-    const int RP = two ? 2 : 1;
+    constexpr bool two = TWO;  // lsl == 7: two source rows per fetch group (registers 0-1, 2-3)
+    constexpr int RP = two ? 2 : 1;
     const bool premult = (B == 2 || B == 4) && job.premult;
     const uint32_t mh = job.mh, mv = job.mv;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -729,24 +729,26 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
     }
 }
 
-// One kernel per band count and form (LEAN: spans of >= 128 chunks), so each
-// keeps only its own registers.
-template <int B, bool LEAN>
+// One kernel per band count and form (FORM 0: general; 1: lean, one source
+// row per fetch group (lsl = 8); 2: lean, two rows (lsl = 7)), so each keeps
+// only its own registers.
+template <int B, int FORM>
 __device__ __forceinline__ void shrink_block(const ShrinkJob& job, int bx, int by, unsigned char* lds) {
-    if constexpr (LEAN) shrink_rows_lean<B>(job, bx, by, lds);
+    if constexpr (FORM == 1) shrink_rows_lean<B, false>(job, bx, by, lds);
+    else if constexpr (FORM == 2) shrink_rows_lean<B, true>(job, bx, by, lds);
     else shrink_rows_block<B>(job, bx, by, lds);
 }
 
-template <int B, bool LEAN>
+template <int B, int FORM>
 __global__ __launch_bounds__(kThreads) void k_shrink_job(ShrinkJob job) {
     extern __shared__ __align__(16) unsigned char lds[];
-    shrink_block<B, LEAN>(job, blockIdx.x, blockIdx.y, lds);
+    shrink_block<B, FORM>(job, blockIdx.x, blockIdx.y, lds);
 }
 
 // Persistent form: the grid walks a task list of (job, bx, by) over many
 // images (all of one variant); jobs reading the caller's source or writing
 // the caller's output take those pointers from per-run slot tables.
-template <int B, bool LEAN>
+template <int B, int FORM>
 __global__ __launch_bounds__(kThreads) void k_shrink_batch(const ShrinkJob* __restrict__ jobs,
                                                            const BatchTask* __restrict__ tasks, int ntasks,
                                                            const void* const* __restrict__ srcs,
@@ -757,7 +759,7 @@ __global__ __launch_bounds__(kThreads) void k_shrink_batch(const ShrinkJob* __re
         ShrinkJob job = jobs[task.job];
         if (job.src_slot >= 0) job.in.ptr = srcs[job.src_slot];
         if (job.dst_slot >= 0) job.out.ptr = dsts[job.dst_slot];
-        shrink_block<B, LEAN>(job, task.bx, task.by, lds);
+        shrink_block<B, FORM>(job, task.bx, task.by, lds);
         __syncthreads();
     }
 }
@@ -1305,12 +1307,15 @@ bool plan_shrink_job(ShrinkJob* job, bool row_split, int run) {
     return true;
 }
 
-int shrink_job_variant(const ShrinkJob& job) { return job.bands - 1 + (job.lsl >= 7 ? 4 : 0); }
+int shrink_job_variant(const ShrinkJob& job) {
+    return job.bands - 1 + (job.lsl >= 8 ? 4 : (job.lsl == 7 ? 8 : 0));
+}
 
 size_t shrink_job_lds() { return (size_t)(kThreads / 64) * (kSpanBytes + 32); }
 
-#define VGPU_SHRINK_VARIANTS(X) \
-    X(0, 1, false) X(1, 2, false) X(2, 3, false) X(3, 4, false) X(4, 1, true) X(5, 2, true) X(6, 3, true) X(7, 4, true)
+#define VGPU_SHRINK_VARIANTS(X)                                                                    \
+    X(0, 1, 0) X(1, 2, 0) X(2, 3, 0) X(3, 4, 0) X(4, 1, 1) X(5, 2, 1) X(6, 3, 1) X(7, 4, 1) X(8, 1, 2) X(9, 2, 2) \
+    X(10, 3, 2) X(11, 4, 2)
 
 hipError_t launch_shrink_job(const ShrinkJob& job, hipStream_t s) {
     if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;
