@@ -131,6 +131,9 @@ int vgpu_pipeline_reduce_v(vgpu_pipeline *p, double factor, int kernel);
  * ReduceOp (resample/reduce.rs:23-262), filters never bound to the input
  * length (offset 0), H results rounded to the sample type before the V pass. */
 int vgpu_pipeline_reduce(vgpu_pipeline *p, double h_factor, double v_factor, int kernel);
+/* PipelineBuilder::extract_area (builder/geometry.rs:40-80): a width x height
+ * rectangle at (x, y), inside the current image. */
+int vgpu_pipeline_extract_area(vgpu_pipeline *p, uint32_t x, uint32_t y, uint32_t width, uint32_t height);
 /* PipelineBuilder::shrink_h_with_ceil / shrink_v_with_ceil
  * (builder/resample.rs:133-209,392-444). */
 int vgpu_pipeline_shrink_h(vgpu_pipeline *p, uint32_t factor, int ceil_mode);
@@ -150,9 +153,14 @@ int vgpu_pipeline_premultiply(vgpu_pipeline *p);
 int vgpu_pipeline_unpremultiply(vgpu_pipeline *p);
 /* PipelineBuilder::thumbnail(Thumbnail::new(target, kernel))
  * (builder/resample.rs:871-979; thumbnail.rs:229-524), in-memory source.
- * Returns VGPU_E_UNSUPPORTED when the plan needs the Affine/crop tail. */
+ * Returns VGPU_E_UNSUPPORTED when the plan needs the Affine tail. */
 int vgpu_pipeline_thumbnail(vgpu_pipeline *p, int target, uint32_t width, uint32_t height,
                             int kernel);
+/* The same with Thumbnail::with_crop (thumbnail.rs:185-190) when crop != 0:
+ * the image is reduced to cover the target box, then a centred ExtractArea
+ * of the box's size is taken (thumbnail.rs:246-285, 421-428). */
+int vgpu_pipeline_thumbnail_crop(vgpu_pipeline *p, int target, uint32_t width, uint32_t height,
+                                 int kernel, int crop);
 /* PipelineBuilder::resize(Resize::new(hscale, vscale, kernel))
  * (builder/resample.rs:637-711; resize.rs:163-284), downscale only. */
 int vgpu_pipeline_resize(vgpu_pipeline *p, double hscale, double vscale, int kernel);

@@ -333,7 +333,8 @@ enum {
     N_UNPREMULT = 10,
     N_COPY = 11,
     N_REDUCE_OP_H = 12, /* ReduceOp's horizontal pass (reduce.rs:104-133) */
-    N_REDUCE_OP_V = 13  /* ReduceOp's vertical pass (reduce.rs:136-166) */
+    N_REDUCE_OP_V = 13, /* ReduceOp's vertical pass (reduce.rs:136-166) */
+    N_EXTRACT = 14      /* ExtractArea (structural/extract_area.rs:23-60) */
 };
 
 typedef struct {
@@ -353,6 +354,8 @@ typedef struct {
     double *gf;
     double cscale, coffset;
     double param;
+    /* extract area: offset of the output in the input */
+    int64_t ex, ey;
 } node_t;
 
 struct orc_pipeline {
@@ -457,6 +460,21 @@ int orc_pipeline_reduce(orc_pipeline *p, double hfactor, double vfactor, int ker
     return ORC_OK;
 }
 
+/* PipelineBuilder::extract_area (builder/geometry.rs:40-80): bounds as
+ * validate_extract_area_bounds (builder/state.rs:238-270): a non-empty
+ * rectangle inside the current image. */
+int orc_pipeline_extract_area(orc_pipeline *p, uint32_t x, uint32_t y, uint32_t w, uint32_t h) {
+    if (w == 0 || h == 0 || (uint64_t)x + w > p->cur_w || (uint64_t)y + h > p->cur_h) return ORC_EINVAL;
+    node_t *nd = push(p, N_EXTRACT);
+    if (!nd) return ORC_EINVAL;
+    nd->ex = x;
+    nd->ey = y;
+    nd->out_w = w;
+    nd->out_h = h;
+    commit(p, nd);
+    return ORC_OK;
+}
+
 /* PipelineBuilder::shrink_h_with_ceil (builder/resample.rs:133-209): the
  * bridge always carries source_width = current width (shrinkh.rs:1366-1383). */
 int orc_pipeline_shrink_h(orc_pipeline *p, uint32_t factor, int ceil_mode) {
@@ -557,7 +575,7 @@ int orc_pipeline_unpremultiply(orc_pipeline *p) {
 
 /* ------------------------------------------------------ thumbnail.rs planner */
 
-enum { T_PREMULT = 1, T_SHRINK_H, T_SHRINK_V, T_REDUCE_H, T_REDUCE_V, T_UNPREMULT };
+enum { T_PREMULT = 1, T_SHRINK_H, T_SHRINK_V, T_REDUCE_H, T_REDUCE_V, T_UNPREMULT, T_EXTRACT };
 
 /* Thumbnail::target_dimensions (thumbnail.rs:458-485). */
 static void target_dims(int target, uint32_t tw, uint32_t th, uint32_t iw, uint32_t ih,
@@ -620,9 +638,15 @@ static uint32_t choose_integer_shrink(uint32_t in_len, uint32_t target, double t
 /* Thumbnail::into_pipeline_nodes_without_shrink_hint (thumbnail.rs:229-456),
  * crop = false. Returns node count, or ORC_EUNSUPPORTED when the plan needs an
  * Affine geometry fix-up. */
-int orc_thumbnail_plan(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint32_t tw,
-                       uint32_t th, int kernel, int *types, double *params, int max_nodes,
-                       uint32_t *out_w, uint32_t *out_h) {
+/* Thumbnail::into_pipeline_nodes_internal (thumbnail.rs:229-436) with the
+ * crop option (thumbnail.rs:185-190): FitBox / ForceExact crops keep the box
+ * as the target; the geometry is the covering size round(in * max(scale))
+ * (at least the target), and a centred ExtractArea ((geometry - target) / 2)
+ * follows the reduce nodes when the two differ.  extract[4] receives that
+ * rectangle (x, y, width, height) for the T_EXTRACT node. */
+int orc_thumbnail_plan_crop(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint32_t tw,
+                            uint32_t th, int kernel, int crop, int *types, double *params, int max_nodes,
+                            uint32_t *out_w, uint32_t *out_h, uint32_t *extract) {
     (void)kernel;
     if (bands == 0) return ORC_EINVAL;
     if (iw == 0 || ih == 0) {
@@ -631,8 +655,31 @@ int orc_thumbnail_plan(uint32_t iw, uint32_t ih, uint32_t bands, int target, uin
     }
     int allow_upscale = target == ORC_FORCE_EXACT;
     int single = iw == 1 && ih == 1;
-    uint32_t gw, gh;
-    target_dims(target, tw, th, iw, ih, &gw, &gh);
+    uint32_t tgw, tgh;
+    if (crop && (target == ORC_FITBOX || target == ORC_FORCE_EXACT)) {
+        tgw = tw > 1 ? tw : 1;
+        tgh = th > 1 ? th : 1;
+    } else {
+        target_dims(target, tw, th, iw, ih, &tgw, &tgh);
+    }
+    if (allow_upscale && !single) {
+        if (tgw < 1) tgw = 1;
+        if (tgh < 1) tgh = 1;
+    } else {
+        if (tgw > iw) tgw = iw;
+        if (tgh > ih) tgh = ih;
+        if (tgw < 1) tgw = 1;
+        if (tgh < 1) tgh = 1;
+    }
+    uint32_t gw = tgw, gh = tgh;
+    if (crop) {
+        double hsc = (double)(tgw > 1 ? tgw : 1) / (double)iw;
+        double vsc = (double)(tgh > 1 ? tgh : 1) / (double)ih;
+        double cover = hsc > vsc ? hsc : vsc;
+        double cw = round((double)iw * cover), ch = round((double)ih * cover);
+        gw = sat_u32(cw > (double)tgw ? cw : (double)tgw);
+        gh = sat_u32(ch > (double)tgh ? ch : (double)tgh);
+    }
     if (allow_upscale && !single) {
         if (gw < 1) gw = 1;
         if (gh < 1) gh = 1;
@@ -643,8 +690,8 @@ int orc_thumbnail_plan(uint32_t iw, uint32_t ih, uint32_t bands, int target, uin
         if (gh < 1) gh = 1;
     }
     int has_alpha = bands == 2 || bands == 4;
-    *out_w = gw;
-    *out_h = gh;
+    *out_w = tgw;
+    *out_h = tgh;
     int geo[8];
     double gp[8];
     int ng = 0;
@@ -675,6 +722,14 @@ int orc_thumbnail_plan(uint32_t iw, uint32_t ih, uint32_t bands, int target, uin
         if (rv > 1.0 + PLAN_EPS) { geo[ng] = T_REDUCE_V; gp[ng++] = rv; }
         double ax = (double)gw / (double)arw, ay = (double)gh / (double)arh;
         if (fabs(ax - 1.0) > PLAN_EPS || fabs(ay - 1.0) > PLAN_EPS) return ORC_EUNSUPPORTED;
+        if (crop && (gw != tgw || gh != tgh)) { /* thumbnail.rs:421-428 */
+            extract[0] = (gw - tgw) / 2;
+            extract[1] = (gh - tgh) / 2;
+            extract[2] = tgw;
+            extract[3] = tgh;
+            geo[ng] = T_EXTRACT;
+            gp[ng++] = 0.0;
+        }
     }
     /* wrap_geometry_nodes (thumbnail.rs:438-456) */
     int n = 0;
@@ -696,17 +751,25 @@ int orc_thumbnail_plan(uint32_t iw, uint32_t ih, uint32_t bands, int target, uin
     return n;
 }
 
+int orc_thumbnail_plan(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint32_t tw,
+                       uint32_t th, int kernel, int *types, double *params, int max_nodes,
+                       uint32_t *out_w, uint32_t *out_h) {
+    uint32_t extract[4];
+    return orc_thumbnail_plan_crop(iw, ih, bands, target, tw, th, kernel, 0, types, params, max_nodes, out_w,
+                                   out_h, extract);
+}
+
 /* PipelineBuilder::thumbnail (builder/resample.rs:871-979) for an in-memory
  * source: the shrink-on-load hint is never honoured, so the plan is the
  * "without shrink hint" one. */
-int orc_pipeline_thumbnail(orc_pipeline *p, int target, uint32_t tw, uint32_t th, int kernel) {
+int orc_pipeline_thumbnail_crop(orc_pipeline *p, int target, uint32_t tw, uint32_t th, int kernel, int crop) {
     int types[16];
     double params[16];
-    uint32_t ow, oh;
+    uint32_t ow, oh, extract[4] = {0, 0, 0, 0};
     if (p->bands == 0) return ORC_EINVAL;
     if (!valid_reduce_kernel(kernel)) return ORC_EKERNEL;
-    int n = orc_thumbnail_plan(p->cur_w, p->cur_h, p->bands, target, tw, th, kernel, types,
-                               params, 16, &ow, &oh);
+    int n = orc_thumbnail_plan_crop(p->cur_w, p->cur_h, p->bands, target, tw, th, kernel, crop, types, params, 16,
+                                    &ow, &oh, extract);
     if (n < 0) return n;
     for (int i = 0; i < n; ++i) {
         int rc = ORC_OK;
@@ -717,6 +780,7 @@ int orc_pipeline_thumbnail(orc_pipeline *p, int target, uint32_t tw, uint32_t th
         case T_SHRINK_V: rc = orc_pipeline_shrink_v(p, (uint32_t)params[i], 0); break;
         case T_REDUCE_H: rc = orc_pipeline_reduce_h(p, params[i], kernel); break;
         case T_REDUCE_V: rc = orc_pipeline_reduce_v(p, params[i], kernel); break;
+        case T_EXTRACT: rc = orc_pipeline_extract_area(p, extract[0], extract[1], extract[2], extract[3]); break;
         }
         if (rc != ORC_OK) return rc;
     }
@@ -726,6 +790,10 @@ int orc_pipeline_thumbnail(orc_pipeline *p, int target, uint32_t tw, uint32_t th
         commit(p, nd);
     }
     return ORC_OK;
+}
+
+int orc_pipeline_thumbnail(orc_pipeline *p, int target, uint32_t tw, uint32_t th, int kernel) {
+    return orc_pipeline_thumbnail_crop(p, target, tw, th, kernel, 0);
 }
 
 /* ResizeOp::into_pipeline_nodes, downscale part (resize.rs:163-284), and
@@ -825,6 +893,10 @@ static rgn required_input(const node_t *nd, rgn o) {
     case N_REDUCE_OP_H: rk_required(&nd->rk, o.x, o.w, &r.x, &r.w); break;
     case N_REDUCE_V:
     case N_REDUCE_OP_V: rk_required(&nd->rk, o.y, o.h, &r.y, &r.h); break;
+    case N_EXTRACT: /* extract_area.rs:49-56 */
+        r.x = clampi(o.x + nd->ex, INT32_MIN, INT32_MAX);
+        r.y = clampi(o.y + nd->ey, INT32_MIN, INT32_MAX);
+        break;
     case N_GAUSS_H:
     case N_CONVSEP_H: { /* gauss_blur.rs:1801-1808; convsep.rs:65-72 */
         int64_t rad = (nd->ntaps - 1) / 2;

@@ -80,6 +80,8 @@ int orc_pipeline_convsep(orc_pipeline *p, const double *coeffs, uint32_t n, doub
 int orc_pipeline_premultiply(orc_pipeline *p);
 int orc_pipeline_unpremultiply(orc_pipeline *p);
 int orc_pipeline_thumbnail(orc_pipeline *p, int target, uint32_t tw, uint32_t th, int kernel);
+int orc_pipeline_thumbnail_crop(orc_pipeline *p, int target, uint32_t tw, uint32_t th, int kernel, int crop);
+int orc_pipeline_extract_area(orc_pipeline *p, uint32_t x, uint32_t y, uint32_t w, uint32_t h);
 int orc_pipeline_resize(orc_pipeline *p, double hscale, double vscale, int kernel);
 int orc_pipeline_output(const orc_pipeline *p, uint32_t *w, uint32_t *h, uint32_t *bands,
                         int *format);
@@ -92,6 +94,10 @@ int orc_pipeline_run(const orc_pipeline *p, const void *src, void *dst, int thre
 int orc_thumbnail_plan(uint32_t in_w, uint32_t in_h, uint32_t bands, int target, uint32_t tw,
                        uint32_t th, int kernel, int *types, double *params, int max_nodes,
                        uint32_t *out_w, uint32_t *out_h);
+/* The same with the crop option; extract[4] gets the ExtractArea rectangle. */
+int orc_thumbnail_plan_crop(uint32_t in_w, uint32_t in_h, uint32_t bands, int target, uint32_t tw,
+                            uint32_t th, int kernel, int crop, int *types, double *params, int max_nodes,
+                            uint32_t *out_w, uint32_t *out_h, uint32_t *extract);
 
 /* FP64 reference for floating-point error studies: the same separable
  * Lanczos/Gaussian math as the f32 path, carried entirely in double with no

@@ -63,6 +63,9 @@ def lib() -> C.CDLL:
         L.orc_pipeline_premultiply.argtypes = [vp]
         L.orc_pipeline_unpremultiply.argtypes = [vp]
         L.orc_pipeline_thumbnail.argtypes = [vp, i32, u32, u32, i32]
+        L.orc_pipeline_thumbnail_crop.argtypes = [vp, i32, u32, u32, i32, i32]
+        L.orc_pipeline_extract_area.argtypes = [vp, u32, u32, u32, u32]
+        L.orc_thumbnail_plan_crop.argtypes = [u32, u32, u32, i32, u32, u32, i32, i32, vp, vp, i32, vp, vp, vp]
         L.orc_pipeline_resize.argtypes = [vp, dbl, dbl, i32]
         L.orc_pipeline_output.argtypes = [vp, vp, vp, vp, vp]
         L.orc_pipeline_node_count.argtypes = [vp]
@@ -176,8 +179,13 @@ class Pipeline:
         _check(lib().orc_pipeline_unpremultiply(self._p), "unpremultiply")
         return self
 
-    def thumbnail(self, target, width, height=0, kernel=LANCZOS3):
-        _check(lib().orc_pipeline_thumbnail(self._p, target, width, height, kernel), "thumbnail")
+    def thumbnail(self, target, width, height=0, kernel=LANCZOS3, crop=False):
+        _check(lib().orc_pipeline_thumbnail_crop(self._p, target, width, height, kernel, int(crop)), "thumbnail")
+        return self
+
+    def extract_area(self, x, y, width, height):
+        """PipelineBuilder::extract_area (builder/geometry.rs:40-80)."""
+        _check(lib().orc_pipeline_extract_area(self._p, x, y, width, height), "extract_area")
         return self
 
     def resize(self, hscale, vscale, kernel=LANCZOS3):
@@ -210,17 +218,23 @@ class Pipeline:
         return dst
 
 
-def thumbnail_plan(in_w, in_h, bands, target, tw, th=0, kernel=LANCZOS3):
+def thumbnail_plan(in_w, in_h, bands, target, tw, th=0, kernel=LANCZOS3, crop=False):
+    """Node list [(name, param)] and output size; an ExtractArea node's
+    param is its (x, y, width, height)."""
     types = np.zeros(16, np.int32)
     params = np.zeros(16, np.float64)
+    extract = np.zeros(4, np.uint32)
     ow, oh = C.c_uint32(), C.c_uint32()
-    n = lib().orc_thumbnail_plan(in_w, in_h, bands, target, tw, th, kernel, _ptr(types),
-                                 _ptr(params), 16, C.addressof(ow), C.addressof(oh))
+    n = lib().orc_thumbnail_plan_crop(in_w, in_h, bands, target, tw, th, kernel, int(crop), _ptr(types),
+                                      _ptr(params), 16, C.addressof(ow), C.addressof(oh), _ptr(extract))
     if n < 0:
         raise OracleError(n, "thumbnail_plan")
     names = {1: "Premultiply", 2: "ShrinkH", 3: "ShrinkV", 4: "ReduceH", 5: "ReduceV",
-             6: "Unpremultiply"}
-    return [(names[int(t)], float(p)) for t, p in zip(types[:n], params[:n])], (ow.value, oh.value)
+             6: "Unpremultiply", 7: "ExtractArea"}
+    out = []
+    for t, p in zip(types[:n], params[:n]):
+        out.append((names[int(t)], tuple(int(v) for v in extract) if int(t) == 7 else float(p)))
+    return out, (ow.value, oh.value)
 
 
 def f64_reduce_hv(src: np.ndarray, factor: float, kernel: int = LANCZOS3, threads: int = 0):

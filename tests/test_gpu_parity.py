@@ -121,6 +121,61 @@ def test_thumbnail_rgba_premultiplied_chain(ctx):
     assert_same(got, exp, "rgba thumbnail")
 
 
+def test_thumbnail_crop_reference_case(ctx):
+    # thumbnail.rs:553-575: 400x200 -> FitBox 120x120 with crop
+    src = gradient_noise_u8(200, 400, 3, 5)
+    got, exp = run_both(ctx, src, [("thumbnail", (v.FITBOX, 120, 120, v.LANCZOS3, True))])
+    assert got.shape == (120, 120, 3)
+    assert_same(got, exp, "crop 400x200 -> 120x120")
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_thumbnail_crops(ctx, seed):
+    r = np.random.default_rng(2000 + seed)
+    w, h = int(r.integers(20, 2400)), int(r.integers(20, 2400))
+    b = int(r.integers(1, 5))
+    target = [v.FITBOX, v.FORCE_EXACT, v.WIDTH, v.HEIGHT, v.FITBOX][seed % 5]
+    tw, th = int(r.integers(4, min(w, h))), int(r.integers(4, min(w, h)))
+    src = noise_u8(h, w, b, seed)
+    if b == 4:
+        src[..., 3] = np.where(r.random((h, w)) < 0.1, 0, src[..., 3])
+    steps = [("thumbnail", (target, tw, th, v.LANCZOS3, True))]
+    try:
+        O.Pipeline(w, h, b).thumbnail(target, tw, th, crop=True)
+    except O.OracleError as e:
+        assert e.code == O.EUNSUPPORTED
+        with pytest.raises(v.VgpuError):
+            v.Pipeline(ctx, w, h, b).thumbnail(target, tw, th, crop=True)
+        return
+    got, exp = run_both(ctx, src, steps)
+    assert_same(got, exp, f"crop {w}x{h}x{b} target {target} -> {tw}x{th}")
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32])
+def test_extract_area_chains(ctx, dtype):
+    h, w, b = 301, 517, 3
+    src = {np.uint8: noise_u8, np.uint16: noise_u16, np.float32: float_image}[dtype](h, w, b, 9)
+    chains = [
+        [("extract_area", (13, 7, 200, 100))],
+        [("extract_area", (0, 0, w, h))],
+        [("gauss_blur", (1.5,)), ("extract_area", (w - 40, h - 30, 40, 30))],
+        [("extract_area", (100, 50, 300, 201)), ("reduce_h", (2.5, v.LANCZOS3)), ("reduce_v", (1.7, v.BICUBIC))],
+        [("reduce_h", (1.3, v.LANCZOS3)), ("extract_area", (5, 0, 60, h)), ("shrink_v", (3, 0))],
+    ]
+    for steps in chains:
+        got, exp = run_both(ctx, src, steps)
+        assert_same(got, exp, f"{np.dtype(dtype)} {steps}")
+        got, exp = run_both(ctx, src, steps, device=False)
+        assert_same(got, exp, f"{np.dtype(dtype)} host {steps}")
+
+
+def test_extract_area_validation(ctx):
+    for args in [(0, 0, 0, 5), (0, 0, 5, 0), (97, 0, 4, 5), (0, 96, 5, 5)]:
+        with pytest.raises(v.VgpuError) as e:
+            v.Pipeline(ctx, 100, 100, 3).extract_area(*args)
+        assert e.value.status == v.api.E_INVALID_ARGUMENT
+
+
 @pytest.mark.parametrize("hs,vs,kernel", [(0.5, 0.5, v.BILINEAR), (0.3, 0.41, v.LANCZOS3),
                                           (0.137, 0.09, v.BICUBIC), (1.0, 0.25, v.LANCZOS2)])
 def test_resize_downscale(ctx, hs, vs, kernel):

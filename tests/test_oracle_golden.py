@@ -486,3 +486,57 @@ def test_reduce_op_rejections():
     with pytest.raises(O.OracleError) as e:
         O.Pipeline(8, 8, 1).reduce(2.0, 2.0, 99)
     assert e.value.code == O.EKERNEL
+
+
+# ------------------------------------------------ thumbnail crop / extract
+
+def test_thumbnail_crop_plan_reference_case():
+    # thumbnail.rs:553-575: 400x200 FitBox 120x120 with crop reduces to the
+    # covering 240x120 and takes the centred 120x120 at (60, 0).
+    nodes, dims = O.thumbnail_plan(400, 200, 3, O.FITBOX, 120, 120, crop=True)
+    assert [n for n, _ in nodes][-1] == "ExtractArea"
+    assert nodes[-1][1] == (60, 0, 120, 120) and dims == (120, 120)
+    # without crop the same box is fitted, and nothing is extracted
+    nodes, dims = O.thumbnail_plan(400, 200, 3, O.FITBOX, 120, 120)
+    assert "ExtractArea" not in [n for n, _ in nodes] and dims == (120, 60)
+    # alpha: the extract sits inside the premultiplied section
+    nodes, _ = O.thumbnail_plan(400, 200, 4, O.FITBOX, 120, 120, crop=True)
+    assert [n for n, _ in nodes][-2:] == ["ExtractArea", "Unpremultiply"]
+
+
+def _cover_dims(iw, ih, tw, th):
+    # The smallest image of the input's aspect ratio that covers tw x th:
+    # scale by the larger of the two ratios (never below the box).
+    s = max(tw / iw, th / ih)
+    return max(int(np.floor(iw * s + 0.5)), tw), max(int(np.floor(ih * s + 0.5)), th)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_thumbnail_crop_is_centre_of_covering_thumbnail(seed):
+    r = np.random.default_rng(500 + seed)
+    iw, ih = int(r.integers(40, 300)), int(r.integers(40, 300))
+    b = int(r.integers(1, 5))
+    tw, th = int(r.integers(4, min(iw, ih))), int(r.integers(4, min(iw, ih)))
+    src = noise_u8(ih, iw, b, seed)
+    gw, gh = _cover_dims(iw, ih, tw, th)
+    try:
+        got = O.Pipeline(iw, ih, b).thumbnail(O.FITBOX, tw, th, crop=True).run(src)
+    except O.OracleError as e:
+        assert e.code == O.EUNSUPPORTED  # the plan needs the Affine tail
+        return
+    full = O.Pipeline(iw, ih, b).thumbnail(O.FORCE_EXACT, gw, gh).run(src)
+    x0, y0 = (gw - tw) // 2, (gh - th) // 2
+    assert got.shape == (th, tw, b)
+    assert np.array_equal(got, full[y0:y0 + th, x0:x0 + tw])
+
+
+def test_extract_area_is_a_slice_and_checks_bounds():
+    src = noise_u8(37, 53, 3, 4)
+    got = O.Pipeline(53, 37, 3).extract_area(7, 5, 20, 11).run(src)
+    assert np.array_equal(got, src[5:16, 7:27])
+    blurred = O.Pipeline(53, 37, 3).gauss_blur(1.0).run(src)
+    got = O.Pipeline(53, 37, 3).gauss_blur(1.0).extract_area(0, 30, 53, 7).run(src)
+    assert np.array_equal(got, blurred[30:37])
+    for args in [(0, 0, 0, 5), (0, 0, 5, 0), (50, 0, 4, 5), (0, 33, 5, 5)]:
+        with pytest.raises(O.OracleError):
+            O.Pipeline(53, 37, 3).extract_area(*args)

@@ -233,8 +233,18 @@ class Pipeline:
         _check(self._lib.vgpu_pipeline_unpremultiply(self._h), "unpremultiply")
         return self
 
-    def thumbnail(self, target: int, width: int, height: int = 0, kernel: int = LANCZOS3):
-        _check(self._lib.vgpu_pipeline_thumbnail(self._h, target, width, height, kernel), "thumbnail")
+    def thumbnail(self, target: int, width: int, height: int = 0, kernel: int = LANCZOS3, crop: bool = False):
+        """Thumbnail::new(target, kernel), with_crop() when crop is true
+        (thumbnail.rs:185-190): cover the box, then a centred ExtractArea."""
+        _check(
+            self._lib.vgpu_pipeline_thumbnail_crop(self._h, target, width, height, kernel, int(bool(crop))),
+            "thumbnail",
+        )
+        return self
+
+    def extract_area(self, x: int, y: int, width: int, height: int):
+        """PipelineBuilder::extract_area (builder/geometry.rs:40-80)."""
+        _check(self._lib.vgpu_pipeline_extract_area(self._h, x, y, width, height), "extract_area")
         return self
 
     def resize(self, hscale: float, vscale: float, kernel: int = LANCZOS3):

@@ -220,6 +220,10 @@ Rect Node::required_input(const Rect& o) const {
     case N_REDUCE_OP_H: axis.required(o.x, o.w, &r.x, &r.w); break;
     case N_REDUCE_V:
     case N_REDUCE_OP_V: axis.required(o.y, o.h, &r.y, &r.h); break;
+    case N_EXTRACT:  // extract_area.rs:49-56
+        r.x = clamp_i32_range(o.x + ex);
+        r.y = clamp_i32_range(o.y + ey);
+        break;
     case N_GAUSS_H:
     case N_CONVSEP_H:
         r.x = o.x - radius();
@@ -286,7 +290,7 @@ uint32_t integer_shrink(uint32_t in_len, uint32_t target, double total) {
 
 Status plan_thumbnail(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint32_t tw,
                       uint32_t th, std::vector<PlanStep>* steps, uint32_t* out_w,
-                      uint32_t* out_h) {
+                      uint32_t* out_h, bool crop) {
     steps->clear();
     if (bands == 0) return E_INVALID_ARGUMENT;
     if (target < FITBOX || target > HEIGHT) return E_INVALID_ARGUMENT;
@@ -296,16 +300,36 @@ Status plan_thumbnail(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint
     }
     const bool upscale = target == FORCE_EXACT;
     const bool single = iw == 1 && ih == 1;
-    uint32_t gw, gh;
-    thumb_target_dims(target, tw, th, iw, ih, &gw, &gh);
-    if (!(upscale && !single)) {
-        gw = std::max(std::min(gw, iw), 1u);
-        gh = std::max(std::min(gh, ih), 1u);
+    auto clamp_dims = [&](uint32_t* w, uint32_t* h) {
+        if (!(upscale && !single)) {
+            *w = std::max(std::min(*w, iw), 1u);
+            *h = std::max(std::min(*h, ih), 1u);
+        }
+    };
+    // Target (the output) and geometry (what shrink + reduce produce): equal
+    // without crop; with crop a FitBox / ForceExact box is the target and the
+    // geometry covers it (thumbnail.rs:246-285).
+    uint32_t tgw, tgh;
+    if (crop && (target == FITBOX || target == FORCE_EXACT)) {
+        tgw = std::max(tw, 1u);
+        tgh = std::max(th, 1u);
+    } else {
+        thumb_target_dims(target, tw, th, iw, ih, &tgw, &tgh);
     }
+    clamp_dims(&tgw, &tgh);
+    uint32_t gw = tgw, gh = tgh;
+    if (crop) {
+        const double cover = std::max((double)std::max(tgw, 1u) / (double)iw, (double)std::max(tgh, 1u) / (double)ih);
+        gw = to_u32_sat(std::max(std::round((double)iw * cover), (double)tgw));
+        gh = to_u32_sat(std::max(std::round((double)ih * cover), (double)tgh));
+    }
+    clamp_dims(&gw, &gh);
     std::vector<PlanStep> geo;
     if (iw == 1 || ih == 1) {
-        if (single) gw = gh = 1;
-        if (gw != iw || gh != ih) return E_UNSUPPORTED;  // Affine-only plan
+        if (single) tgw = tgh = 1;
+        if (tgw != iw || tgh != ih) return E_UNSUPPORTED;  // Affine-only plan
+        gw = tgw;
+        gh = tgh;
     } else {
         const double hs = (double)iw / (double)gw, vs = (double)ih / (double)gh;
         const uint32_t sh = integer_shrink(iw, gw, hs), sv = integer_shrink(ih, gh, vs);
@@ -320,13 +344,21 @@ Status plan_thumbnail(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint
         if (std::fabs((double)gw / (double)arw - 1.0) > kPlanEps ||
             std::fabs((double)gh / (double)arh - 1.0) > kPlanEps)
             return E_UNSUPPORTED;  // needs the Affine geometry fix-up
+        if (crop && (gw != tgw || gh != tgh)) {  // thumbnail.rs:421-428
+            PlanStep e{N_EXTRACT, 0.0};
+            e.ex = (gw - tgw) / 2;
+            e.ey = (gh - tgh) / 2;
+            e.ew = tgw;
+            e.eh = tgh;
+            geo.push_back(e);
+        }
     }
     const bool alpha = bands == 2 || bands == 4;
     if (alpha && !geo.empty()) steps->push_back({N_PREMULT, 0.0});
     steps->insert(steps->end(), geo.begin(), geo.end());
     if (alpha && !geo.empty()) steps->push_back({N_UNPREMULT, 0.0});
-    *out_w = gw;
-    *out_h = gh;
+    *out_w = tgw;
+    *out_h = tgh;
     return OK;
 }
 

@@ -98,6 +98,7 @@ enum NodeKind : int {
     // (required_input_region_v of the output) as the V node's input region.
     N_REDUCE_OP_H = 12,
     N_REDUCE_OP_V = 13,
+    N_EXTRACT = 14,  // ExtractArea (structural/extract_area.rs:23-60)
 };
 
 struct Node {
@@ -112,6 +113,7 @@ struct Node {
     std::vector<int16_t> itaps; // gauss integer taps
     int32_t iscale = 1;
     double cscale = 1.0, coffset = 0.0;
+    int32_t ex = 0, ey = 0;     // extract area: offset of the output in the input
 
     int radius() const { return conv.empty() ? 0 : (int)(conv.size() - 1) / 2; }
     Rect required_input(const Rect& out) const;
@@ -123,10 +125,13 @@ enum ThumbTarget : int { FITBOX = 0, FORCE_EXACT = 1, WIDTH = 2, HEIGHT = 3 };
 struct PlanStep {
     NodeKind kind;
     double param;
+    uint32_t ex = 0, ey = 0, ew = 0, eh = 0;  // N_EXTRACT rectangle
 };
+// crop: Thumbnail::with_crop (thumbnail.rs:185-190), a centred ExtractArea
+// of the covering geometry.
 Status plan_thumbnail(uint32_t in_w, uint32_t in_h, uint32_t bands, int target, uint32_t tw,
                       uint32_t th, std::vector<PlanStep>* steps, uint32_t* out_w,
-                      uint32_t* out_h);
+                      uint32_t* out_h, bool crop = false);
 // ResizeOp::into_pipeline_nodes, downscale subset (resize.rs:163-284).
 Status plan_resize(uint32_t in_w, uint32_t in_h, double hscale, double vscale, int kernel,
                    std::vector<PlanStep>* steps, int* eff_kernel);

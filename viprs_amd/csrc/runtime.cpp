@@ -908,7 +908,16 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
     case ST_ALPHA:
         e = launch_alpha_u8(in, out, B, nd.kind == N_PREMULT ? ALPHA_PREMULT : ALPHA_UNPREMULT, stream);
         break;
-    case ST_COPY: e = launch_copy(in, out, B, nd.in_fmt, stream); break;
+    case ST_COPY: {
+        // ExtractArea reads its input at the output's coordinates plus its offset.
+        OutView o = out;
+        if (nd.kind == N_EXTRACT) {
+            o.x += nd.ex;
+            o.y += nd.ey;
+        }
+        e = launch_copy(in, o, B, nd.in_fmt, stream);
+        break;
+    }
     }
     return from_hip(e);
 }
@@ -1266,6 +1275,21 @@ int vgpu_pipeline_reduce_v(vgpu_pipeline* p, double factor, int kernel) {
     return VGPU_OK;
 }
 
+// PipelineBuilder::extract_area (builder/geometry.rs:40-80): a non-empty
+// rectangle inside the current image (validate_extract_area_bounds,
+// builder/state.rs:238-270).
+int vgpu_pipeline_extract_area(vgpu_pipeline* p, uint32_t x, uint32_t y, uint32_t w, uint32_t h) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    if (w == 0 || h == 0 || (uint64_t)x + w > p->cur_w || (uint64_t)y + h > p->cur_h) return VGPU_E_INVALID_ARGUMENT;
+    Node& nd = p->push(N_EXTRACT);
+    nd.ex = (int32_t)x;
+    nd.ey = (int32_t)y;
+    nd.out_w = w;
+    nd.out_h = h;
+    p->commit();
+    return VGPU_OK;
+}
+
 // PipelineBuilder::reduce (builder/resample.rs:349-382) -> ReduceOp::new
 // (reduce.rs:51-67): both factors validated first, then the kernel, then each
 // axis's tap limit; the filters are never bound to the input length.
@@ -1380,6 +1404,7 @@ static int apply_steps(vgpu_pipeline* p, const std::vector<PlanStep>& steps, int
         case N_SHRINK_V: rc = vgpu_pipeline_shrink_v(p, (uint32_t)s.param, 0); break;
         case N_REDUCE_H: rc = vgpu_pipeline_reduce_h(p, s.param, kernel); break;
         case N_REDUCE_V: rc = vgpu_pipeline_reduce_v(p, s.param, kernel); break;
+        case N_EXTRACT: rc = vgpu_pipeline_extract_area(p, s.ex, s.ey, s.ew, s.eh); break;
         default: rc = VGPU_E_UNSUPPORTED; break;
         }
         if (rc != VGPU_OK) return rc;
@@ -1392,12 +1417,16 @@ static int apply_steps(vgpu_pipeline* p, const std::vector<PlanStep>& steps, int
 }
 
 int vgpu_pipeline_thumbnail(vgpu_pipeline* p, int target, uint32_t w, uint32_t h, int kernel) {
+    return vgpu_pipeline_thumbnail_crop(p, target, w, h, kernel, 0);
+}
+
+int vgpu_pipeline_thumbnail_crop(vgpu_pipeline* p, int target, uint32_t w, uint32_t h, int kernel, int crop) {
     if (!p) return VGPU_E_INVALID_ARGUMENT;
     if (kernel == LBB || kernel == VSQBS || kernel == NOHALO || kernel < NEAREST || kernel > CATMULLROM)
         return VGPU_E_INVALID_KERNEL;
     std::vector<PlanStep> steps;
     uint32_t ow = 0, oh = 0;
-    int rc = plan_thumbnail(p->cur_w, p->cur_h, p->bands, target, w, h, &steps, &ow, &oh);
+    int rc = plan_thumbnail(p->cur_w, p->cur_h, p->bands, target, w, h, &steps, &ow, &oh, crop != 0);
     if (rc != OK) return rc;
     if (p->cur_fmt != U8)
         for (const PlanStep& s : steps)
