@@ -334,7 +334,9 @@ enum {
     N_COPY = 11,
     N_REDUCE_OP_H = 12, /* ReduceOp's horizontal pass (reduce.rs:104-133) */
     N_REDUCE_OP_V = 13, /* ReduceOp's vertical pass (reduce.rs:136-166) */
-    N_EXTRACT = 14      /* ExtractArea (structural/extract_area.rs:23-60) */
+    N_EXTRACT = 14,     /* ExtractArea (structural/extract_area.rs:23-60) */
+    N_AFFINE = 15       /* Affine, ExtendMode::Copy, source bounds = its input
+                         * (affine/core.rs; builder/state.rs:298-341) */
 };
 
 typedef struct {
@@ -356,6 +358,9 @@ typedef struct {
     double param;
     /* extract area: offset of the output in the input */
     int64_t ex, ey;
+    /* affine: [a, b, c, d], translation, kernel, SmallTile demand */
+    double am[4], atx, aty;
+    int akern, asmall;
 } node_t;
 
 struct orc_pipeline {
@@ -365,6 +370,7 @@ struct orc_pipeline {
     int cur_fmt;
     int n;
     node_t nodes[32];
+    int thin_override; /* the ThinStrip override of large thumbnails (builder/resample.rs:908-917) */
 };
 
 static size_t fmt_size(int f) { return f == ORC_U8 ? 1 : (f == ORC_U16 ? 2 : 4); }
@@ -573,9 +579,35 @@ int orc_pipeline_unpremultiply(orc_pipeline *p) {
     return ORC_OK;
 }
 
+/* PipelineBuilder::affine_backward_with_extend_and_hint (builder/resample.rs:
+ * 478-560) -> AffineBridge::new_with_extend (builder/state.rs:298-341) ->
+ * Affine::try_new (affine/interpolation.rs:13-41): finite coefficients and a
+ * non-singular matrix; the diagonal matrix [a, 0, 0, d] of the thumbnail,
+ * ExtendMode::Copy, source bounds = the whole input. */
+int orc_pipeline_affine(orc_pipeline *p, double a, double d, double tx, double ty, uint32_t ow, uint32_t oh,
+                        int kernel, int small_tile) {
+    const double m[4] = {a, 0.0, 0.0, d};
+    for (int i = 0; i < 4; ++i)
+        if (!isfinite(m[i])) return ORC_EINVAL;
+    if (!isfinite(tx) || !isfinite(ty)) return ORC_EINVAL;
+    if (fabs(fma(m[1], -m[2], m[0] * m[3])) <= DBL_EPSILON) return ORC_EINVAL;
+    if (kernel != ORC_NEAREST && kernel != ORC_BILINEAR && kernel != ORC_BICUBIC) return ORC_EUNSUPPORTED;
+    node_t *nd = push(p, N_AFFINE);
+    if (!nd) return ORC_EINVAL;
+    memcpy(nd->am, m, sizeof(m));
+    nd->atx = tx;
+    nd->aty = ty;
+    nd->akern = kernel;
+    nd->asmall = small_tile;
+    nd->out_w = ow;
+    nd->out_h = oh;
+    commit(p, nd);
+    return ORC_OK;
+}
+
 /* ------------------------------------------------------ thumbnail.rs planner */
 
-enum { T_PREMULT = 1, T_SHRINK_H, T_SHRINK_V, T_REDUCE_H, T_REDUCE_V, T_UNPREMULT, T_EXTRACT };
+enum { T_PREMULT = 1, T_SHRINK_H, T_SHRINK_V, T_REDUCE_H, T_REDUCE_V, T_UNPREMULT, T_EXTRACT, T_AFFINE };
 
 /* Thumbnail::target_dimensions (thumbnail.rs:458-485). */
 static void target_dims(int target, uint32_t tw, uint32_t th, uint32_t iw, uint32_t ih,
@@ -643,11 +675,24 @@ static uint32_t choose_integer_shrink(uint32_t in_len, uint32_t target, double t
  * as the target; the geometry is the covering size round(in * max(scale))
  * (at least the target), and a centred ExtractArea ((geometry - target) / 2)
  * follows the reduce nodes when the two differ.  extract[4] receives that
- * rectangle (x, y, width, height) for the T_EXTRACT node. */
+ * rectangle (x, y, width, height) for the T_EXTRACT node, and affine[7] the
+ * T_AFFINE node's (a, d, tx, ty, output width, output height, kernel): the
+ * 1-pixel-input branch (thumbnail.rs:286-323) and the geometry fix-up after
+ * the reduce nodes (thumbnail.rs:396-419), both with the kernel mapped by
+ * affine_kernel_for_resize_kernel (thumbnail.rs:31-37). */
+static void thumb_affine(double inv_x, double inv_y, uint32_t ow, uint32_t oh, int kernel, double *affine) {
+    affine[0] = inv_x;
+    affine[1] = inv_y;
+    affine[2] = kernel == ORC_NEAREST ? 0.0 : 0.5 * (1.0 - inv_x);
+    affine[3] = kernel == ORC_NEAREST ? 0.0 : 0.5 * (1.0 - inv_y);
+    affine[4] = ow;
+    affine[5] = oh;
+    affine[6] = kernel == ORC_NEAREST ? ORC_NEAREST : (kernel == ORC_BILINEAR ? ORC_BILINEAR : ORC_BICUBIC);
+}
+
 int orc_thumbnail_plan_crop(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint32_t tw,
                             uint32_t th, int kernel, int crop, int *types, double *params, int max_nodes,
-                            uint32_t *out_w, uint32_t *out_h, uint32_t *extract) {
-    (void)kernel;
+                            uint32_t *out_w, uint32_t *out_h, uint32_t *extract, double *affine) {
     if (bands == 0) return ORC_EINVAL;
     if (iw == 0 || ih == 0) {
         *out_w = *out_h = 0;
@@ -697,7 +742,11 @@ int orc_thumbnail_plan_crop(uint32_t iw, uint32_t ih, uint32_t bands, int target
     int ng = 0;
     if (iw == 1 || ih == 1) {
         if (single) *out_w = *out_h = 1;
-        if (*out_w != iw || *out_h != ih) return ORC_EUNSUPPORTED; /* Affine-only plan */
+        if (*out_w != iw || *out_h != ih) {
+            thumb_affine((double)iw / (double)*out_w, (double)ih / (double)*out_h, *out_w, *out_h, kernel, affine);
+            geo[ng] = T_AFFINE;
+            gp[ng++] = 0.0;
+        }
     } else {
         double hs = (double)iw / (double)gw;
         double vs = (double)ih / (double)gh;
@@ -721,7 +770,11 @@ int orc_thumbnail_plan_crop(uint32_t iw, uint32_t ih, uint32_t bands, int target
         if (rh > 1.0 + PLAN_EPS) { geo[ng] = T_REDUCE_H; gp[ng++] = rh; }
         if (rv > 1.0 + PLAN_EPS) { geo[ng] = T_REDUCE_V; gp[ng++] = rv; }
         double ax = (double)gw / (double)arw, ay = (double)gh / (double)arh;
-        if (fabs(ax - 1.0) > PLAN_EPS || fabs(ay - 1.0) > PLAN_EPS) return ORC_EUNSUPPORTED;
+        if (fabs(ax - 1.0) > PLAN_EPS || fabs(ay - 1.0) > PLAN_EPS) {
+            thumb_affine(1.0 / ax, 1.0 / ay, gw, gh, kernel, affine);
+            geo[ng] = T_AFFINE;
+            gp[ng++] = 0.0;
+        }
         if (crop && (gw != tgw || gh != tgh)) { /* thumbnail.rs:421-428 */
             extract[0] = (gw - tgw) / 2;
             extract[1] = (gh - tgh) / 2;
@@ -755,8 +808,9 @@ int orc_thumbnail_plan(uint32_t iw, uint32_t ih, uint32_t bands, int target, uin
                        uint32_t th, int kernel, int *types, double *params, int max_nodes,
                        uint32_t *out_w, uint32_t *out_h) {
     uint32_t extract[4];
+    double affine[7];
     return orc_thumbnail_plan_crop(iw, ih, bands, target, tw, th, kernel, 0, types, params, max_nodes, out_w,
-                                   out_h, extract);
+                                   out_h, extract, affine);
 }
 
 /* PipelineBuilder::thumbnail (builder/resample.rs:871-979) for an in-memory
@@ -766,11 +820,18 @@ int orc_pipeline_thumbnail_crop(orc_pipeline *p, int target, uint32_t tw, uint32
     int types[16];
     double params[16];
     uint32_t ow, oh, extract[4] = {0, 0, 0, 0};
+    double affine[7] = {0, 0, 0, 0, 0, 0, 0};
     if (p->bands == 0) return ORC_EINVAL;
     if (!valid_reduce_kernel(kernel)) return ORC_EKERNEL;
-    int n = orc_thumbnail_plan_crop(p->cur_w, p->cur_h, p->bands, target, tw, th, kernel, crop, types, params, 16,
-                                    &ow, &oh, extract);
+    const uint32_t iw = p->cur_w, ih = p->cur_h;
+    int n = orc_thumbnail_plan_crop(iw, ih, p->bands, target, tw, th, kernel, crop, types, params, 16,
+                                    &ow, &oh, extract, affine);
     if (n < 0) return n;
+    /* builder/resample.rs:908-917,939-943: inputs above 512 px run as
+     * ThinStrip (forced when the thumbnail is the first node); smaller ones
+     * give the Affine the SmallTile demand. */
+    const int large = iw > 512 || ih > 512;
+    if (p->n == 0 && large) p->thin_override = 1;
     for (int i = 0; i < n; ++i) {
         int rc = ORC_OK;
         switch (types[i]) {
@@ -781,6 +842,10 @@ int orc_pipeline_thumbnail_crop(orc_pipeline *p, int target, uint32_t tw, uint32
         case T_REDUCE_H: rc = orc_pipeline_reduce_h(p, params[i], kernel); break;
         case T_REDUCE_V: rc = orc_pipeline_reduce_v(p, params[i], kernel); break;
         case T_EXTRACT: rc = orc_pipeline_extract_area(p, extract[0], extract[1], extract[2], extract[3]); break;
+        case T_AFFINE:
+            rc = orc_pipeline_affine(p, affine[0], affine[1], affine[2], affine[3], (uint32_t)affine[4],
+                                 (uint32_t)affine[5], (int)affine[6], !large);
+            break;
         }
         if (rc != ORC_OK) return rc;
     }
@@ -871,6 +936,39 @@ int orc_pipeline_node_info(const orc_pipeline *p, int i, int *type, double *para
 
 /* ------------------------------------------------- required input regions */
 
+/* InterpolationKernel::affine_padding (kernel.rs:80-117) for the affine
+ * kernels of the thumbnail. */
+static void affine_pad(int k, int64_t *left, int64_t *right) {
+    *left = k == ORC_BICUBIC ? 1 : 0;
+    *right = k == ORC_NEAREST ? 0 : (k == ORC_BILINEAR ? 1 : 2);
+}
+
+/* Affine::map_to_input (affine/core.rs:78-84). */
+static void affine_map(const node_t *nd, double x, double y, double *mx, double *my) {
+    *mx = fma(nd->am[1], y, nd->am[0] * x) + nd->atx;
+    *my = fma(nd->am[3], y, nd->am[2] * x) + nd->aty;
+}
+
+/* Affine::required_input_axis_bounds with source bounds (0, len)
+ * (affine/core.rs:373-405). */
+static void affine_axis(int64_t start, int64_t end, uint32_t len, int64_t *o, int64_t *n) {
+    if (len == 0) {
+        *o = 0;
+        *n = 0;
+        return;
+    }
+    const int64_t be = (int64_t)len - 1;
+    const int64_t cs = start > 0 ? start : 0, ce = end < be ? end : be;
+    if (ce < cs) {
+        *o = clampi(end < 0 ? 0 : be + 1, INT32_MIN, INT32_MAX);
+        *n = 0;
+        return;
+    }
+    const int64_t rs = clampi(cs, INT32_MIN, INT32_MAX), re = clampi(ce, INT32_MIN, INT32_MAX);
+    *o = rs;
+    *n = re - rs + 1;
+}
+
 static rgn required_input(const node_t *nd, rgn o) {
     rgn r = o;
     switch (nd->type) {
@@ -897,6 +995,25 @@ static rgn required_input(const node_t *nd, rgn o) {
         r.x = clampi(o.x + nd->ex, INT32_MIN, INT32_MAX);
         r.y = clampi(o.y + nd->ey, INT32_MIN, INT32_MAX);
         break;
+    case N_AFFINE: { /* affine/core.rs:344-366,1531-1560: mapped corner pixels, floor, padding */
+        const double ox0 = (double)o.x, oy0 = (double)o.y;
+        const double ox1 = (double)o.x + (double)(o.w > 0 ? o.w - 1 : 0);
+        const double oy1 = (double)o.y + (double)(o.h > 0 ? o.h - 1 : 0);
+        double cx[4], cy[4];
+        affine_map(nd, ox0, oy0, &cx[0], &cy[0]);
+        affine_map(nd, ox1, oy0, &cx[1], &cy[1]);
+        affine_map(nd, ox0, oy1, &cx[2], &cy[2]);
+        affine_map(nd, ox1, oy1, &cx[3], &cy[3]);
+        double minx = fmin(fmin(fmin(cx[0], cx[1]), cx[2]), cx[3]);
+        double maxx = fmax(fmax(fmax(cx[0], cx[1]), cx[2]), cx[3]);
+        double miny = fmin(fmin(fmin(cy[0], cy[1]), cy[2]), cy[3]);
+        double maxy = fmax(fmax(fmax(cy[0], cy[1]), cy[2]), cy[3]);
+        int64_t lp, rp;
+        affine_pad(nd->akern, &lp, &rp);
+        affine_axis((int64_t)floor(minx) - lp, (int64_t)floor(maxx) + rp, nd->in_w, &r.x, &r.w);
+        affine_axis((int64_t)floor(miny) - lp, (int64_t)floor(maxy) + rp, nd->in_h, &r.y, &r.h);
+        break;
+    }
     case N_GAUSS_H:
     case N_CONVSEP_H: { /* gauss_blur.rs:1801-1808; convsep.rs:65-72 */
         int64_t rad = (nd->ntaps - 1) / 2;
@@ -1215,6 +1332,100 @@ static void process_alpha(const node_t *nd, const void *in, void *out, rgn orr) 
     }
 }
 
+/* Affine::sample_band (affine/core.rs:442-461) under ExtendMode::Copy and
+ * source bounds (0, 0, in_w, in_h): coordinates clamp to the image
+ * (resolve_source_coord, affine/interpolation.rs:93-125); outside the tile
+ * the fill value 0 is read. */
+static double affine_sample(const node_t *nd, const void *in, rgn ir, int64_t ix, int64_t iy, int band) {
+    const int64_t rx = clampi(ix, 0, (int64_t)nd->in_w - 1), ry = clampi(iy, 0, (int64_t)nd->in_h - 1);
+    const int64_t tx = rx - ir.x, ty = ry - ir.y;
+    if (tx < 0 || tx >= ir.w || ty < 0 || ty >= ir.h) return 0.0;
+    return ld(in, nd->in_fmt, ((size_t)ty * (size_t)ir.w + (size_t)tx) * nd->bands + (size_t)band);
+}
+
+/* sample_band_premultiplied (affine/core.rs:463-482); alpha = band 3. */
+static double affine_sample_pm(const node_t *nd, const void *in, rgn ir, int64_t ix, int64_t iy, int band,
+                               int pm, double amax) {
+    const double v = affine_sample(nd, in, ir, ix, iy, band);
+    if (!pm || band == 3) return v;
+    return v * (affine_sample(nd, in, ir, ix, iy, 3) / amax);
+}
+
+/* bilinear_mix (affine/core.rs:197-201). */
+static double bilinear_mix(double p00, double p01, double p10, double p11, double fx, double fy) {
+    const double top = fma(p01, fx, p00 * (1.0 - fx));
+    const double bottom = fma(p11, fx, p10 * (1.0 - fx));
+    return top * (1.0 - fy) + bottom * fy;
+}
+
+/* One band at (x, y): interp_nearest / interp_bilinear / interp_separable<2>
+ * for Bicubic (affine/core.rs:485-514,648-697), and their premultiplied
+ * forms (sample_at_premultiplied, affine/core.rs:751-935,937-986) when pm. */
+static double affine_band(const node_t *nd, const void *in, rgn ir, double x, double y, int band, int pm,
+                          double amax) {
+    if (nd->akern == ORC_NEAREST) return affine_sample_pm(nd, in, ir, (int64_t)x, (int64_t)y, band, pm, amax);
+    if (nd->akern == ORC_BILINEAR) {
+        const int64_t x0 = (int64_t)x, y0 = (int64_t)y; /* truncation, as `as i64` */
+        const double fx = x - (double)x0, fy = y - (double)y0;
+        return bilinear_mix(affine_sample_pm(nd, in, ir, x0, y0, band, pm, amax),
+                            affine_sample_pm(nd, in, ir, x0 + 1, y0, band, pm, amax),
+                            affine_sample_pm(nd, in, ir, x0, y0 + 1, band, pm, amax),
+                            affine_sample_pm(nd, in, ir, x0 + 1, y0 + 1, band, pm, amax), fx, fy);
+    }
+    const int64_t cx = (int64_t)floor(x), cy = (int64_t)floor(y);
+    double acc = 0.0, wsum = 0.0;
+    for (int64_t ky = -1; ky <= 2; ++ky) {
+        const int64_t iy = cy + ky;
+        const double wy = orc_kernel_weight(ORC_BICUBIC, fabs((double)iy - y));
+        if (wy == 0.0) continue;
+        for (int64_t kx = -1; kx <= 2; ++kx) {
+            const int64_t ix = cx + kx;
+            const double wx = orc_kernel_weight(ORC_BICUBIC, fabs((double)ix - x));
+            if (wx == 0.0) continue;
+            const double w = wx * wy;
+            wsum += w;
+            acc = fma(affine_sample_pm(nd, in, ir, ix, iy, band, pm, amax), w, acc);
+        }
+    }
+    return wsum > 0.0 ? acc / wsum : 0.0;
+}
+
+/* Affine::process_region, the generic path (affine/core.rs:1593-1632; the
+ * fast paths need a constant-fill extend): the source position starts from
+ * the region origin and is stepped by one matrix column per pixel and one
+ * per row, so it depends on the tile origin.  Four-band images go through
+ * sample_pixel_at_with_alpha (affine/core.rs:988-1020): the Affine's own
+ * premultiplied flag is never set by the builder. */
+static void process_affine(const node_t *nd, const void *in, rgn ir, void *out, rgn orr) {
+    const size_t B = nd->bands;
+    const int fmt = nd->out_fmt;
+    const double amax = fmt == ORC_U8 ? 255.0 : (fmt == ORC_U16 ? 65535.0 : 1.0);
+    double row_x = fma(nd->am[1], (double)orr.y, nd->am[0] * (double)orr.x) + nd->atx;
+    double row_y = fma(nd->am[3], (double)orr.y, nd->am[2] * (double)orr.x) + nd->aty;
+    for (int64_t yy = 0; yy < orr.h; ++yy) {
+        double x = row_x, y = row_y;
+        for (int64_t xx = 0; xx < orr.w; ++xx) {
+            const size_t o = ((size_t)yy * (size_t)orr.w + (size_t)xx) * B;
+            if (B == 4) {
+                const double alpha = clampd(affine_band(nd, in, ir, x, y, 3, 1, amax), 0.0, amax);
+                const double na = alpha / amax;
+                for (int b = 0; b < 3; ++b) {
+                    const double pmv = affine_band(nd, in, ir, x, y, b, 1, amax);
+                    store_from_f64(out, fmt, o + (size_t)b, na > 0.0 ? pmv / na : 0.0);
+                }
+                store_from_f64(out, fmt, o + 3, alpha);
+            } else {
+                for (size_t b = 0; b < B; ++b)
+                    store_from_f64(out, fmt, o + b, affine_band(nd, in, ir, x, y, (int)b, 0, amax));
+            }
+            x += nd->am[0];
+            y += nd->am[2];
+        }
+        row_x += nd->am[1];
+        row_y += nd->am[3];
+    }
+}
+
 static void process_node(const node_t *nd, const void *in, rgn ir, void *out, rgn orr) {
     switch (nd->type) {
     case N_REDUCE_H: process_reduce_h(nd, in, ir, out, orr); break;
@@ -1229,6 +1440,7 @@ static void process_node(const node_t *nd, const void *in, rgn ir, void *out, rg
     case N_CONVSEP_V: process_conv(nd, in, ir, out, orr, 1); break;
     case N_PREMULT:
     case N_UNPREMULT: process_alpha(nd, in, out, orr); break;
+    case N_AFFINE: process_affine(nd, in, ir, out, orr); break;
     default: memcpy(out, in, (size_t)orr.w * (size_t)orr.h * nd->bands * fmt_size(nd->in_fmt)); break;
     }
 }
@@ -1281,7 +1493,7 @@ typedef struct {
     const orc_pipeline *p;
     const void *src;
     uint8_t *dst;
-    uint32_t strip;
+    uint32_t tw, th, cols;
     uint32_t ntiles;
     int tid, nthreads;
 } job_t;
@@ -1290,19 +1502,24 @@ static void *worker(void *arg) {
     job_t *j = (job_t *)arg;
     for (uint32_t t = (uint32_t)j->tid; t < j->ntiles; t += (uint32_t)j->nthreads) {
         rgn tile;
-        tile.x = 0;
-        tile.y = (int64_t)t * j->strip;
-        tile.w = j->p->cur_w;
-        tile.h = j->strip;
+        tile.x = (int64_t)(t % j->cols) * j->tw;
+        tile.y = (int64_t)(t / j->cols) * j->th;
+        tile.w = j->tw;
+        tile.h = j->th;
+        if (tile.x + tile.w > (int64_t)j->p->cur_w) tile.w = (int64_t)j->p->cur_w - tile.x;
         if (tile.y + tile.h > (int64_t)j->p->cur_h) tile.h = (int64_t)j->p->cur_h - tile.y;
         run_tile(j->p, j->src, j->dst, tile);
     }
     return NULL;
 }
 
-/* RayonScheduler::run over ThinStrip tiles (runtime.rs:1019-1235;
- * planning.rs:513-601; op/demand.rs:41-85): full-width strips of 16 rows (1
- * row beyond 10000 px). strip_height = 0 selects that default. */
+/* RayonScheduler::run (runtime.rs:1019-1235) over the tiles of
+ * generate_tiles_with_l2_budget (planning.rs:513-601; op/demand.rs:41-85).
+ * The pipeline's demand is the ThinStrip override or the largest node demand
+ * (arena.rs:411-420): every node here is ThinStrip except an Affine after a
+ * small thumbnail, which is SmallTile.  ThinStrip: full-width strips of 16
+ * rows (1 row beyond 10000 px); SmallTile: 128 x 128 tiles.
+ * strip_height != 0 instead runs full-width strips of that height. */
 int orc_pipeline_run(const orc_pipeline *p, const void *src, void *dst, int threads,
                      uint32_t strip_height) {
     if (p->cur_w == 0 || p->cur_h == 0) return ORC_OK;
@@ -1310,15 +1527,29 @@ int orc_pipeline_run(const orc_pipeline *p, const void *src, void *dst, int thre
         memcpy(dst, src, (size_t)p->src_w * p->src_h * p->bands * fmt_size(p->src_fmt));
         return ORC_OK;
     }
-    uint32_t strip = strip_height ? strip_height : (p->cur_w > 10000 ? 1 : 16);
-    uint32_t ntiles = (p->cur_h + strip - 1) / strip;
+    int small = 0;
+    for (int i = 0; i < p->n; ++i)
+        if (p->nodes[i].type == N_AFFINE && p->nodes[i].asmall) small = 1;
+    if (p->thin_override) small = 0;
+    uint32_t tile_w = p->cur_w, tile_h;
+    if (strip_height) {
+        tile_h = strip_height;
+    } else if (small) {
+        tile_w = 128;
+        tile_h = 128;
+    } else {
+        tile_h = p->cur_w > 10000 ? 1 : 16;
+    }
+    if (tile_h > p->cur_h) tile_h = p->cur_h;
+    uint32_t cols = (p->cur_w + tile_w - 1) / tile_w;
+    uint32_t ntiles = cols * ((p->cur_h + tile_h - 1) / tile_h);
     if (threads < 1) threads = 1;
     if ((uint32_t)threads > ntiles) threads = (int)ntiles;
     pthread_t th[256];
     job_t jobs[256];
     if (threads > 256) threads = 256;
     for (int i = 0; i < threads; ++i) {
-        jobs[i] = (job_t){p, src, (uint8_t *)dst, strip, ntiles, i, threads};
+        jobs[i] = (job_t){p, src, (uint8_t *)dst, tile_w, tile_h, cols, ntiles, i, threads};
         if (threads > 1)
             pthread_create(&th[i], NULL, worker, &jobs[i]);
         else

@@ -81,6 +81,11 @@ int orc_pipeline_premultiply(orc_pipeline *p);
 int orc_pipeline_unpremultiply(orc_pipeline *p);
 int orc_pipeline_thumbnail(orc_pipeline *p, int target, uint32_t tw, uint32_t th, int kernel);
 int orc_pipeline_thumbnail_crop(orc_pipeline *p, int target, uint32_t tw, uint32_t th, int kernel, int crop);
+/* The thumbnail's Affine node: matrix [a, 0, 0, d], translation (tx, ty),
+ * ExtendMode::Copy, kernel Nearest / Bilinear / Bicubic; small_tile gives it
+ * the SmallTile demand (128 x 128 scheduler tiles) instead of ThinStrip. */
+int orc_pipeline_affine(orc_pipeline *p, double a, double d, double tx, double ty, uint32_t width,
+                        uint32_t height, int kernel, int small_tile);
 int orc_pipeline_extract_area(orc_pipeline *p, uint32_t x, uint32_t y, uint32_t w, uint32_t h);
 int orc_pipeline_resize(orc_pipeline *p, double hscale, double vscale, int kernel);
 int orc_pipeline_output(const orc_pipeline *p, uint32_t *w, uint32_t *h, uint32_t *bands,
@@ -94,10 +99,11 @@ int orc_pipeline_run(const orc_pipeline *p, const void *src, void *dst, int thre
 int orc_thumbnail_plan(uint32_t in_w, uint32_t in_h, uint32_t bands, int target, uint32_t tw,
                        uint32_t th, int kernel, int *types, double *params, int max_nodes,
                        uint32_t *out_w, uint32_t *out_h);
-/* The same with the crop option; extract[4] gets the ExtractArea rectangle. */
+/* The same with the crop option; extract[4] gets the ExtractArea rectangle,
+ * affine[7] the Affine node's (a, d, tx, ty, width, height, kernel). */
 int orc_thumbnail_plan_crop(uint32_t in_w, uint32_t in_h, uint32_t bands, int target, uint32_t tw,
                             uint32_t th, int kernel, int crop, int *types, double *params, int max_nodes,
-                            uint32_t *out_w, uint32_t *out_h, uint32_t *extract);
+                            uint32_t *out_w, uint32_t *out_h, uint32_t *extract, double *affine);
 
 /* FP64 reference for floating-point error studies: the same separable
  * Lanczos/Gaussian math as the f32 path, carried entirely in double with no

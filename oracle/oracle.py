@@ -65,7 +65,8 @@ def lib() -> C.CDLL:
         L.orc_pipeline_thumbnail.argtypes = [vp, i32, u32, u32, i32]
         L.orc_pipeline_thumbnail_crop.argtypes = [vp, i32, u32, u32, i32, i32]
         L.orc_pipeline_extract_area.argtypes = [vp, u32, u32, u32, u32]
-        L.orc_thumbnail_plan_crop.argtypes = [u32, u32, u32, i32, u32, u32, i32, i32, vp, vp, i32, vp, vp, vp]
+        L.orc_pipeline_affine.argtypes = [vp, dbl, dbl, dbl, dbl, u32, u32, i32, i32]
+        L.orc_thumbnail_plan_crop.argtypes = [u32, u32, u32, i32, u32, u32, i32, i32, vp, vp, i32, vp, vp, vp, vp]
         L.orc_pipeline_resize.argtypes = [vp, dbl, dbl, i32]
         L.orc_pipeline_output.argtypes = [vp, vp, vp, vp, vp]
         L.orc_pipeline_node_count.argtypes = [vp]
@@ -188,6 +189,11 @@ class Pipeline:
         _check(lib().orc_pipeline_extract_area(self._p, x, y, width, height), "extract_area")
         return self
 
+    def affine(self, a, d, tx, ty, width, height, kernel=BICUBIC, small_tile=True):
+        """The thumbnail's Affine node (matrix [a, 0, 0, d], ExtendMode::Copy)."""
+        _check(lib().orc_pipeline_affine(self._p, a, d, tx, ty, width, height, kernel, int(small_tile)), "affine")
+        return self
+
     def resize(self, hscale, vscale, kernel=LANCZOS3):
         _check(lib().orc_pipeline_resize(self._p, hscale, vscale, kernel), "resize")
         return self
@@ -220,20 +226,29 @@ class Pipeline:
 
 def thumbnail_plan(in_w, in_h, bands, target, tw, th=0, kernel=LANCZOS3, crop=False):
     """Node list [(name, param)] and output size; an ExtractArea node's
-    param is its (x, y, width, height)."""
+    param is its (x, y, width, height), an Affine node's its
+    (a, d, tx, ty, width, height, kernel) for the matrix [a, 0, 0, d]."""
     types = np.zeros(16, np.int32)
     params = np.zeros(16, np.float64)
     extract = np.zeros(4, np.uint32)
+    affine = np.zeros(7, np.float64)
     ow, oh = C.c_uint32(), C.c_uint32()
     n = lib().orc_thumbnail_plan_crop(in_w, in_h, bands, target, tw, th, kernel, int(crop), _ptr(types),
-                                      _ptr(params), 16, C.addressof(ow), C.addressof(oh), _ptr(extract))
+                                      _ptr(params), 16, C.addressof(ow), C.addressof(oh), _ptr(extract),
+                                      _ptr(affine))
     if n < 0:
         raise OracleError(n, "thumbnail_plan")
     names = {1: "Premultiply", 2: "ShrinkH", 3: "ShrinkV", 4: "ReduceH", 5: "ReduceV",
-             6: "Unpremultiply", 7: "ExtractArea"}
+             6: "Unpremultiply", 7: "ExtractArea", 8: "Affine"}
     out = []
     for t, p in zip(types[:n], params[:n]):
-        out.append((names[int(t)], tuple(int(v) for v in extract) if int(t) == 7 else float(p)))
+        if int(t) == 7:
+            out.append((names[7], tuple(int(v) for v in extract)))
+        elif int(t) == 8:
+            a = [float(v) for v in affine]
+            out.append((names[8], (a[0], a[1], a[2], a[3], int(a[4]), int(a[5]), int(a[6]))))
+        else:
+            out.append((names[int(t)], float(p)))
     return out, (ow.value, oh.value)
 
 

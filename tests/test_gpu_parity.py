@@ -176,6 +176,76 @@ def test_extract_area_validation(ctx):
         assert e.value.status == v.api.E_INVALID_ARGUMENT
 
 
+def _small_tile_u8_reduce_h(w, h, b, target, tw, th, kernel, crop, dtype):
+    """The one Affine-tail case the device declines: a u8 ReduceH before an
+    Affine that runs on 128 x 128 tiles (thumbnail input up to 512 px)."""
+    nodes, _ = O.thumbnail_plan(w, h, b, target, tw, th, kernel, crop)
+    names = [n for n, _ in nodes]
+    return dtype == np.uint8 and w <= 512 and h <= 512 and "Affine" in names and "ReduceH" in names
+
+
+AFFINE_CASES = [
+    # (w, h, bands, dtype, target, tw, th, kernel, crop)
+    (1, 2048, 3, np.uint8, v.HEIGHT, 0, 400, v.LANCZOS3, False),     # thumbnail.rs:663-677
+    (2048, 1, 4, np.uint8, v.WIDTH, 300, 0, v.BICUBIC, False),
+    (1, 300, 4, np.uint8, v.HEIGHT, 0, 120, v.BILINEAR, False),      # SmallTile, alpha
+    (1, 1, 3, np.uint8, v.FORCE_EXACT, 5, 7, v.LANCZOS3, False),
+    (200, 1000, 3, np.uint8, v.FORCE_EXACT, 300, 300, v.LANCZOS3, False),
+    (300, 200, 4, np.uint8, v.FORCE_EXACT, 700, 500, v.LANCZOS3, False),  # several 128 x 128 tiles
+    (300, 200, 2, np.uint8, v.FORCE_EXACT, 333, 271, v.NEAREST, False),
+    (257, 190, 3, np.uint16, v.FORCE_EXACT, 600, 401, v.BICUBIC, False),
+    (257, 190, 1, np.float32, v.FORCE_EXACT, 600, 401, v.BILINEAR, False),
+    (700, 90, 3, np.uint8, v.FORCE_EXACT, 500, 333, v.LANCZOS3, False),   # ThinStrip, ReduceH + Affine
+    (300, 200, 3, np.uint8, v.FORCE_EXACT, 500, 500, v.LANCZOS3, True),   # crop of an upscale
+    (300, 200, 4, np.uint8, v.FITBOX, 100, 150, v.LANCZOS3, True),
+    (3000, 2, 3, np.uint8, v.FORCE_EXACT, 12000, 4, v.BICUBIC, False),  # 1-row strips beyond 10000 px
+]
+
+
+@pytest.mark.parametrize("case", range(len(AFFINE_CASES)))
+def test_thumbnail_affine_tail(ctx, case):
+    w, h, b, dtype, target, tw, th, kernel, crop = AFFINE_CASES[case]
+    src = {np.uint8: noise_u8, np.uint16: noise_u16, np.float32: float_image}[dtype](h, w, b, 40 + case)
+    if b in (2, 4) and dtype == np.uint8:
+        src[..., -1] = np.where(np.random.default_rng(case).random((h, w)) < 0.2, 0, src[..., -1])
+    got, exp = run_both(ctx, src, [("thumbnail", (target, tw, th, kernel, crop))])
+    assert_same(got, exp, f"affine tail {AFFINE_CASES[case]}")
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_affine_tails(ctx, seed):
+    r = np.random.default_rng(3000 + seed)
+    w, h = int(r.integers(2, 1200)), int(r.integers(2, 1200))
+    b = int(r.integers(1, 5))
+    tw, th = int(r.integers(2, 1600)), int(r.integers(2, 1600))
+    kernel = [v.LANCZOS3, v.BILINEAR, v.NEAREST, v.BICUBIC][seed % 4]
+    crop = seed % 3 == 0
+    src = noise_u8(h, w, b, seed)
+    if _small_tile_u8_reduce_h(w, h, b, v.FORCE_EXACT, tw, th, kernel, crop, np.uint8):
+        with pytest.raises(v.VgpuError) as e:
+            v.Pipeline(ctx, w, h, b).thumbnail(v.FORCE_EXACT, tw, th, kernel, crop)
+        assert e.value.status == v.api.E_UNSUPPORTED
+        return
+    got, exp = run_both(ctx, src, [("thumbnail", (v.FORCE_EXACT, tw, th, kernel, crop))])
+    assert_same(got, exp, f"ForceExact {w}x{h}x{b} -> {tw}x{th} kernel {kernel} crop {crop}")
+
+
+def test_affine_tail_limits(ctx):
+    # a u8 ReduceH ahead of a SmallTile Affine is declined, and the builder unchanged
+    p = v.Pipeline(ctx, 400, 300, 1)
+    assert _small_tile_u8_reduce_h(400, 300, 1, v.FORCE_EXACT, 150, 900, v.LANCZOS3, False, np.uint8)
+    with pytest.raises(v.VgpuError) as e:
+        p.thumbnail(v.FORCE_EXACT, 150, 900)
+    assert e.value.status == v.api.E_UNSUPPORTED
+    assert p.output_info()[:2] == (400, 300)
+    # an op after the Affine: declined when the pipeline runs
+    p = v.Pipeline(ctx, 1, 700, 3).thumbnail(v.HEIGHT, 0, 300).gauss_blur(1.0)
+    src = v.DeviceBuffer.from_numpy(ctx, noise_u8(700, 1, 3, 1))
+    with pytest.raises(v.VgpuError) as e:
+        p.run(src, p.output_buffer())
+    assert e.value.status == v.api.E_UNSUPPORTED
+
+
 @pytest.mark.parametrize("hs,vs,kernel", [(0.5, 0.5, v.BILINEAR), (0.3, 0.41, v.LANCZOS3),
                                           (0.137, 0.09, v.BICUBIC), (1.0, 0.25, v.LANCZOS2)])
 def test_resize_downscale(ctx, hs, vs, kernel):

@@ -284,9 +284,8 @@ def test_thumbnail_planner_shapes():
     # Appendix B.3
     nodes, dims = O.thumbnail_plan(6000, 4000, 3, O.FITBOX, 400, 400)
     assert nodes == [("ShrinkH", 15.0), ("ShrinkV", 12.0), ("ReduceV", 4000 / 267 / 12)] and dims == (400, 267)
-    # 1-pixel inputs take the Affine-only path (not on the GPU path)
-    with pytest.raises(O.OracleError):
-        O.thumbnail_plan(1, 2048, 3, O.HEIGHT, 0, 400)
+    # 1-pixel inputs take the Affine-only path
+    assert [n for n, _ in O.thumbnail_plan(1, 2048, 3, O.HEIGHT, 0, 400)[0]] == ["Affine"]
 
 
 def test_integer_shrink_never_undershoots():
@@ -540,3 +539,160 @@ def test_extract_area_is_a_slice_and_checks_bounds():
     for args in [(0, 0, 0, 5), (0, 0, 5, 0), (50, 0, 4, 5), (0, 33, 5, 5)]:
         with pytest.raises(O.OracleError):
             O.Pipeline(53, 37, 3).extract_area(*args)
+
+
+# ------------------------------------------------------ thumbnail Affine tail
+
+def test_affine_reference_cases():
+    # affine/tests.rs:428-546, the ExtendMode::Copy row: nearest, tx = -2
+    src = np.array([10, 11, 12, 13, 20, 21, 22, 23, 30, 31, 32, 33], np.uint8).reshape(1, 3, 4)
+    got = O.Pipeline(3, 1, 4).affine(1.0, 1.0, -2.0, 0.0, 7, 1, O.NEAREST).run(src)
+    assert got.ravel().tolist() == [10, 11, 12, 13] * 3 + [20, 21, 22, 23] + [30, 31, 32, 33] * 3
+    # affine/tests.rs:840-857: RGBA bilinear premultiplies before interpolating
+    src = np.array([255, 0, 0, 0, 0, 0, 255, 255], np.uint8).reshape(1, 2, 4)
+    got = O.Pipeline(2, 1, 4).affine(1.0, 1.0, 0.5, 0.0, 1, 1, O.BILINEAR).run(src)
+    assert got.ravel().tolist() == [0, 0, 255, 128]
+    # affine/tests.rs:225-360: identity transforms return the input
+    for b in (1, 3, 4):
+        src = noise_u8(23, 31, b, b)
+        if b == 4:
+            src[..., 3] = np.maximum(src[..., 3], 1)
+        for k in (O.NEAREST, O.BILINEAR, O.BICUBIC):
+            got = O.Pipeline(31, 23, b).affine(1.0, 1.0, 0.0, 0.0, 31, 23, k).run(src)
+            if b == 4 and k != O.NEAREST:
+                # premultiply / divide round trips: within one level
+                assert np.abs(got.astype(int) - src).max() <= 1
+            else:
+                assert np.array_equal(got, src), (b, k)
+
+
+def test_thumbnail_affine_plan_reference_case():
+    # thumbnail.rs:663-677: 1x2048 -> Height(400), Lanczos3 tail mapped to Bicubic
+    nodes, dims = O.thumbnail_plan(1, 2048, 3, O.HEIGHT, 0, 400)
+    assert dims == (1, 400) and [n for n, _ in nodes] == ["Affine"]
+    a, d, tx, ty, w, h, k = nodes[0][1]
+    assert (w, h, k) == (1, 400, O.BICUBIC) and a == 1.0 and d == 2048 / 400
+    assert tx == 0.0 and ty == 0.5 * (1.0 - 2048 / 400)
+    # thumbnail.rs:31-37: Nearest keeps a zero translation, Bilinear stays Bilinear
+    nodes, _ = O.thumbnail_plan(1, 2048, 3, O.HEIGHT, 0, 400, kernel=O.NEAREST)
+    assert nodes[0][1][2:4] == (0.0, 0.0) and nodes[0][1][6] == O.NEAREST
+    nodes, _ = O.thumbnail_plan(1, 2048, 3, O.HEIGHT, 0, 400, kernel=O.BILINEAR)
+    assert nodes[0][1][6] == O.BILINEAR
+    # ForceExact upscale on one axis: reduce the other, then the Affine fix-up
+    nodes, dims = O.thumbnail_plan(200, 1000, 4, O.FORCE_EXACT, 300, 300)
+    assert [n for n, _ in nodes] == ["Premultiply", "ShrinkV", "ReduceV", "Affine", "Unpremultiply"]
+    assert dims == (300, 300) and nodes[3][1][4:6] == (300, 300) and nodes[3][1][0] == 1.0 / (300 / 200)
+
+
+def _catmull_rom(t):
+    t = np.abs(t)
+    return np.where(t < 1, 1.5 * t ** 3 - 2.5 * t ** 2 + 1,
+                    np.where(t < 2, -0.5 * t ** 3 + 2.5 * t ** 2 - 4 * t + 2, 0.0))
+
+
+def _affine_model(src, a, d, tx, ty, ow, oh, kernel, tile=(128, 128)):
+    """Independent numpy statement of the axis-aligned Affine with the Copy
+    extend: source positions start at each scheduler tile's origin and step by
+    a (columns) and d (rows); interpolation in float64, before rounding."""
+    h, w, b = src.shape
+    amax = {np.uint8: 255.0, np.uint16: 65535.0}.get(src.dtype.type, 1.0)
+
+    def coords(n, step, off, t):
+        out = np.empty(n)
+        for t0 in range(0, n, t):
+            v = step * t0 + off
+            for i in range(t0, min(t0 + t, n)):
+                out[i] = v
+                v += step
+        return out
+
+    xs, ys = coords(ow, a, tx, tile[0]), coords(oh, d, ty, tile[1])
+    img = src.astype(np.float64)
+    if b == 4:  # the Affine premultiplies four-band images itself
+        img[..., :3] *= img[..., 3:4] / amax
+
+    def px(iy, ix):  # (oh, ow) integer grids -> (oh, ow, b)
+        return img[np.clip(iy, 0, h - 1), np.clip(ix, 0, w - 1)]
+
+    Y, X = np.meshgrid(ys, xs, indexing="ij")
+    if kernel == O.NEAREST:
+        v = px(np.trunc(Y).astype(int), np.trunc(X).astype(int))
+    elif kernel == O.BILINEAR:
+        x0, y0 = np.trunc(X).astype(int), np.trunc(Y).astype(int)
+        fx, fy = (X - x0)[..., None], (Y - y0)[..., None]
+        top = px(y0, x0) * (1 - fx) + px(y0, x0 + 1) * fx
+        bot = px(y0 + 1, x0) * (1 - fx) + px(y0 + 1, x0 + 1) * fx
+        v = top * (1 - fy) + bot * fy
+    else:
+        cx, cy = np.floor(X).astype(int), np.floor(Y).astype(int)
+        acc = np.zeros((oh, ow, b))
+        wsum = np.zeros((oh, ow, 1))
+        for ky in range(-1, 3):
+            wy = _catmull_rom(cy + ky - Y)[..., None]
+            for kx in range(-1, 3):
+                wgt = _catmull_rom(cx + kx - X)[..., None] * wy
+                acc += px(cy + ky, cx + kx) * wgt
+                wsum += wgt
+        v = acc / wsum
+    if b == 4:
+        alpha = np.clip(v[..., 3:4], 0, amax)
+        na = alpha / amax
+        v = np.concatenate([np.where(na > 0, v[..., :3] / np.where(na > 0, na, 1), 0.0), alpha], axis=-1)
+    return v
+
+
+def _assert_rounds_to(got, v, what):
+    """got == round-half-away(clamp(v)), except within 1e-6 of a tie."""
+    if got.dtype == np.float32:
+        np.testing.assert_allclose(got, v, rtol=1e-6, atol=1e-6, err_msg=what)
+        return
+    hi = 255.0 if got.dtype == np.uint8 else 65535.0
+    c = np.clip(v, 0, hi)
+    exp = np.floor(c + 0.5)
+    near_tie = np.abs(c - np.floor(c) - 0.5) < 1e-6
+    ok = (got == exp) | (near_tie & (np.abs(got.astype(np.float64) - exp) <= 1))
+    assert ok.all(), f"{what}: {np.count_nonzero(~ok)} of {ok.size} differ"
+
+
+@pytest.mark.parametrize("kernel", [O.NEAREST, O.BILINEAR, O.BICUBIC])
+@pytest.mark.parametrize("bands", [1, 3, 4])
+@pytest.mark.parametrize("fmt", [np.uint8, np.uint16, np.float32])
+def test_affine_matches_numpy_model(kernel, bands, fmt):
+    r = np.random.default_rng(kernel * 100 + bands * 10 + np.dtype(fmt).itemsize)
+    h, w = 37, 53
+    if fmt == np.float32:
+        src = r.random((h, w, bands)).astype(np.float32)
+    else:
+        src = r.integers(0, np.iinfo(fmt).max + 1, size=(h, w, bands)).astype(fmt)
+    if bands == 4:
+        src[..., 3] = np.where(r.random((h, w)) < 0.15, 0, src[..., 3])
+    for (ow, oh) in [(97, 71), (40, 30), (150, 140)]:
+        a, d = w / ow, h / oh
+        tx = 0.0 if kernel == O.NEAREST else 0.5 * (1 - a)
+        ty = 0.0 if kernel == O.NEAREST else 0.5 * (1 - d)
+        for small, tile in [(True, (128, 128)), (False, (ow, 16))]:
+            got = O.Pipeline(w, h, bands, {np.uint8: O.U8, np.uint16: O.U16, np.float32: O.F32}[fmt]) \
+                .affine(a, d, tx, ty, ow, oh, kernel, small).run(src)
+            v = _affine_model(src, a, d, tx, ty, ow, oh, kernel, tile)
+            _assert_rounds_to(got, v, f"{kernel} {bands} {np.dtype(fmt)} {ow}x{oh} small={small}")
+
+
+@pytest.mark.parametrize("kernel", [O.NEAREST, O.BILINEAR, O.BICUBIC])
+def test_affine_uniform_stays_uniform(kernel):
+    for b in (1, 3, 4):
+        src = np.full((19, 23, b), 77, np.uint8)
+        got = O.Pipeline(23, 19, b).affine(23 / 61, 19 / 45, 0.3, -0.2, 61, 45, kernel).run(src)
+        assert (got == 77).all()
+
+
+def test_thumbnail_affine_tail_runs():
+    # ForceExact upscale of one axis, and a 1-pixel-wide input (Affine only)
+    src = noise_u8(1000, 200, 3, 3)
+    got = O.Pipeline(200, 1000, 3).thumbnail(O.FORCE_EXACT, 300, 300).run(src)
+    assert got.shape == (300, 300, 3)
+    col = noise_u8(2048, 1, 3, 4)
+    got = O.Pipeline(1, 2048, 3).thumbnail(O.HEIGHT, 0, 400).run(col)
+    assert got.shape == (400, 1, 3)
+    # the Affine alone is the numpy model on the 1-pixel column
+    v = _affine_model(col, 1.0, 2048 / 400, 0.0, 0.5 * (1 - 2048 / 400), 1, 400, O.BICUBIC, (1, 16))
+    _assert_rounds_to(got, v, "1x2048 -> 1x400")

@@ -1,0 +1,115 @@
+// The thumbnail's Affine tail (thumbnail.rs:292-316,396-419): an
+// axis-aligned matrix [a, 0, 0, d] with ExtendMode::Copy over its whole input
+// image, Nearest / Bilinear / Bicubic, in the reference's generic per-pixel
+// path (affine/core.rs:1593-1632 -> sample_pixel_at, 1022-1055).  The source
+// positions and, for Bicubic, the per-axis weights are tabulated on the host
+// (AffineTap), so a thread does the f64 sampling of one output pixel in the
+// reference's operation order:
+//  * Nearest: the sample at (trunc x, trunc y);
+//  * Bilinear: bilinear_mix (affine/core.rs:197-201) of the 2 x 2 samples
+//    from (trunc x, trunc y);
+//  * Bicubic: interp_separable<2> (affine/core.rs:648-697): rows outer,
+//    columns inner, taps with a zero weight skipped, acc = fma(p, wx * wy,
+//    acc), then acc / sum of weights;
+//  * four bands: sample_pixel_at_with_alpha (affine/core.rs:988-1020), every
+//    sample of a colour band multiplied by (its alpha / alpha_max) first, the
+//    result divided by the interpolated alpha / alpha_max.
+// Sources clamp to the input image (resolve_source_coord,
+// affine/interpolation.rs:93-125); from_f64 stores (sample_conv.rs:58-93).
+#include "device.hpp"
+
+namespace vgpu {
+namespace {
+
+template <typename T, int K, bool ALPHA>
+__global__ __launch_bounds__(kThreads) void k_affine(AffineJob j) {
+    const InView& in = j.in;
+    const OutView& out = j.out;
+    const int B = ALPHA ? 4 : j.bands;
+    const double amax = sizeof(T) == 1 ? 255.0 : (sizeof(T) == 2 ? 65535.0 : 1.0);
+    const size_t n = (size_t)out.w * out.h;
+    for (size_t px = (size_t)blockIdx.x * kThreads + threadIdx.x; px < n; px += (size_t)gridDim.x * kThreads) {
+        const int xo = (int)(px % out.w), yo = (int)(px / out.w);
+        const AffineTap tx = j.xs[xo], ty = j.ys[yo];
+        auto sample = [&](int ix, int iy, int b) -> double {
+            const int cx = clampi(ix, 0, j.in_w - 1), cy = clampi(iy, 0, j.in_h - 1);
+            return (double)load_at<T>(in, row_of(in, cy), col_of(in, cx), B, b);
+        };
+        // sample_band_premultiplied (affine/core.rs:463-482) when ALPHA
+        auto pm = [&](int ix, int iy, int b) -> double {
+            const double v = sample(ix, iy, b);
+            if (!ALPHA || b == 3) return v;
+            return v * (sample(ix, iy, 3) / amax);
+        };
+        auto band = [&](int b) -> double {
+            if constexpr (K == AFF_NEAREST) {
+                return pm(tx.i0, ty.i0, b);
+            } else if constexpr (K == AFF_BILINEAR) {
+                const double fx = tx.f, fy = ty.f;
+                const double top = fma(pm(tx.i0 + 1, ty.i0, b), fx, pm(tx.i0, ty.i0, b) * (1.0 - fx));
+                const double bottom =
+                    fma(pm(tx.i0 + 1, ty.i0 + 1, b), fx, pm(tx.i0, ty.i0 + 1, b) * (1.0 - fx));
+                return top * (1.0 - fy) + bottom * fy;
+            } else {
+                double acc = 0.0, wsum = 0.0;
+                for (int ky = 0; ky < 4; ++ky) {
+                    const double wy = ty.w[ky];
+                    if (wy == 0.0) continue;
+                    for (int kx = 0; kx < 4; ++kx) {
+                        const double wx = tx.w[kx];
+                        if (wx == 0.0) continue;
+                        const double w = wx * wy;
+                        wsum += w;
+                        acc = fma(pm(tx.i0 + kx, ty.i0 + ky, b), w, acc);
+                    }
+                }
+                return wsum > 0.0 ? acc / wsum : 0.0;
+            }
+        };
+        T* o = static_cast<T*>(out.ptr) + px * B;
+        if constexpr (ALPHA) {
+            const double alpha = fmin(fmax(band(3), 0.0), amax);
+            const double na = alpha / amax;
+            for (int b = 0; b < 3; ++b) {
+                const double v = band(b);
+                o[b] = op_from_f64<T>(na > 0.0 ? v / na : 0.0);
+            }
+            o[3] = op_from_f64<T>(alpha);
+        } else {
+            for (int b = 0; b < B; ++b) o[b] = op_from_f64<T>(band(b));
+        }
+    }
+}
+
+template <typename T, int K>
+void launch_affine_t(const AffineJob& j, unsigned g, hipStream_t s) {
+    if (j.bands == 4)
+        hipLaunchKernelGGL((k_affine<T, K, true>), dim3(g), dim3(kThreads), 0, s, j);
+    else
+        hipLaunchKernelGGL((k_affine<T, K, false>), dim3(g), dim3(kThreads), 0, s, j);
+}
+
+template <typename T>
+hipError_t launch_affine_k(const AffineJob& j, unsigned g, hipStream_t s) {
+    switch (j.kernel) {
+    case AFF_NEAREST: launch_affine_t<T, AFF_NEAREST>(j, g, s); break;
+    case AFF_BILINEAR: launch_affine_t<T, AFF_BILINEAR>(j, g, s); break;
+    case AFF_BICUBIC: launch_affine_t<T, AFF_BICUBIC>(j, g, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_affine(const AffineJob& j, hipStream_t s) {
+    if (j.out.w <= 0 || j.out.h <= 0) return hipSuccess;
+    if (j.in_w <= 0 || j.in_h <= 0 || !j.xs || !j.ys) return hipErrorInvalidValue;
+    const size_t n = (size_t)j.out.w * j.out.h;
+    const unsigned g = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 65536);
+    if (j.fmt == 0) return launch_affine_k<uint8_t>(j, g, s);
+    if (j.fmt == 1) return launch_affine_k<uint16_t>(j, g, s);
+    return launch_affine_k<float>(j, g, s);
+}
+
+}  // namespace vgpu

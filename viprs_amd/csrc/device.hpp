@@ -31,6 +31,18 @@ __device__ __forceinline__ int row_of(const InView& v, int y) {
     return clampi(clampi(y, v.cy0, v.cy1), v.by, v.by + v.bh - 1) - v.by;
 }
 
+// F::Sample::from_f64 (sample_conv.rs:58-93): integer samples clamp to the
+// type's range, then round half away from zero; f32 is a cast.
+template <typename T>
+__device__ __forceinline__ T op_from_f64(double v) {
+    if constexpr (sizeof(T) == 4) {
+        return (T)(float)v;
+    } else {
+        const double mx = sizeof(T) == 1 ? 255.0 : 65535.0;
+        return (T)round(fmin(fmax(v, 0.0), mx));
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T load_at(const InView& v, int row, int col, int bands, int b) {
     return static_cast<const T*>(v.ptr)[((size_t)row * v.bw + col) * bands + b];

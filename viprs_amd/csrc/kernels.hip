@@ -863,17 +863,6 @@ __device__ __forceinline__ T reduce_v_sample(Get get, int start, int ph, const D
     }
 }
 
-// F::Sample::from_f64 (sample_conv.rs:58-93): integer samples clamp to the
-// type's range, then round half away from zero; f32 is a cast.
-template <typename T>
-__device__ __forceinline__ T op_from_f64(double v) {
-    if constexpr (sizeof(T) == 4) {
-        return (T)(float)v;
-    } else {
-        const double mx = sizeof(T) == 1 ? 255.0 : 65535.0;
-        return (T)round(fmin(fmax(v, 0.0), mx));
-    }
-}
 
 // One ReduceOp pass sample (reduce.rs:104-166): acc = fma(x_t, w_t, acc) in
 // tap order over the f64 phase row, from acc = 0.

@@ -38,6 +38,25 @@ struct DevPlan {
 
 enum Alpha : int { ALPHA_NONE = 0, ALPHA_PREMULT = 1, ALPHA_UNPREMULT = 2 };
 
+// One output column (or row) of the thumbnail's axis-aligned Affine: the
+// first source index read (trunc(x) for Nearest and Bilinear, floor(x) - 1
+// for Bicubic), the Bilinear phase x - trunc(x), and the four Bicubic weights
+// of source indices i0 .. i0 + 3.
+enum AffineKernel : int { AFF_NEAREST = 0, AFF_BILINEAR = 1, AFF_BICUBIC = 2 };  // InterpolationKernel ids
+struct AffineTap {
+    int32_t i0 = 0, pad = 0;
+    double f = 0.0;
+    double w[4] = {0.0, 0.0, 0.0, 0.0};
+};
+struct AffineJob {
+    InView in;
+    OutView out;
+    int bands = 3, fmt = 0, kernel = 2;
+    int32_t in_w = 0, in_h = 0;  // the Affine's input image: ExtendMode::Copy clamps to it
+    const AffineTap* xs = nullptr;  // out.w entries
+    const AffineTap* ys = nullptr;  // out.h entries
+};
+
 // Fused [Premultiply] -> ShrinkH -> ShrinkV -> [Unpremultiply] on u8, one
 // image.  plan_shrink_job fills the launch shape (tx, S, m, lsl, mh, mv, gx,
 // gy); it returns false when the wavefront kernel cannot take the job.
@@ -188,6 +207,7 @@ hipError_t launch_conv(const InView& in, const OutView& out, int bands, int in_f
 hipError_t launch_alpha_u8(const InView& in, const OutView& out, int bands, int mode,
                            hipStream_t s);
 hipError_t launch_copy(const InView& in, const OutView& out, int bands, int fmt, hipStream_t s);
+hipError_t launch_affine(const AffineJob& job, hipStream_t s);
 hipError_t launch_fill_random(void* dst, size_t bytes, uint64_t seed, hipStream_t s);
 
 }  // namespace vgpu

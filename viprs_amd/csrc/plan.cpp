@@ -198,6 +198,45 @@ GaussMask build_gauss_mask(float sigma_f) {
     return m;
 }
 
+// Affine::required_input_axis_bounds with source bounds (0, len)
+// (affine/core.rs:373-405).
+static void affine_axis(int64_t start, int64_t end, uint32_t len, int64_t* o, int64_t* n) {
+    if (len == 0) {
+        *o = 0;
+        *n = 0;
+        return;
+    }
+    const int64_t be = (int64_t)len - 1;
+    const int64_t cs = std::max<int64_t>(start, 0), ce = std::min<int64_t>(end, be);
+    if (ce < cs) {
+        *o = clamp_i32_range(end < 0 ? 0 : be + 1);
+        *n = 0;
+        return;
+    }
+    *o = clamp_i32_range(cs);
+    *n = clamp_i32_range(ce) - *o + 1;
+}
+
+void affine_positions(const Node& nd, bool columns, int64_t first, int64_t n, int64_t tile, int64_t shift,
+                      std::vector<double>* pos) {
+    pos->resize((size_t)std::max<int64_t>(n, 0));
+    const double step = columns ? nd.am[0] : nd.am[3];
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t p = first + i;
+        const int64_t t0 = (p - shift) / tile * tile + shift;  // origin of p's tile
+        if (i > 0 && (p - shift) % tile != 0) {
+            (*pos)[(size_t)i] = (*pos)[(size_t)i - 1] + step;
+            continue;
+        }
+        // row_x = fma(b, oy, a * ox) + tx and row_y = fma(d, oy, c * ox) + ty
+        // with b = c = 0: the other coordinate's term is an exact zero.
+        double v = columns ? std::fma(nd.am[1], 0.0, nd.am[0] * (double)t0) + nd.atx
+                           : std::fma(nd.am[3], (double)t0, nd.am[2] * 0.0) + nd.aty;
+        for (int64_t q = t0; q < p; ++q) v += step;
+        (*pos)[(size_t)i] = v;
+    }
+}
+
 Rect Node::required_input(const Rect& o) const {
     Rect r = o;
     switch (kind) {
@@ -224,6 +263,23 @@ Rect Node::required_input(const Rect& o) const {
         r.x = clamp_i32_range(o.x + ex);
         r.y = clamp_i32_range(o.y + ey);
         break;
+    case N_AFFINE: {  // affine/core.rs:344-366,373-405,1531-1560
+        const double ox0 = (double)o.x, oy0 = (double)o.y;
+        const double ox1 = (double)o.x + (double)(o.w > 0 ? o.w - 1 : 0);
+        const double oy1 = (double)o.y + (double)(o.h > 0 ? o.h - 1 : 0);
+        double cx[4], cy[4];
+        const double px[4] = {ox0, ox1, ox0, ox1}, py[4] = {oy0, oy0, oy1, oy1};
+        for (int i = 0; i < 4; ++i) affine_map(px[i], py[i], &cx[i], &cy[i]);
+        const double minx = std::fmin(std::fmin(std::fmin(cx[0], cx[1]), cx[2]), cx[3]);
+        const double maxx = std::fmax(std::fmax(std::fmax(cx[0], cx[1]), cx[2]), cx[3]);
+        const double miny = std::fmin(std::fmin(std::fmin(cy[0], cy[1]), cy[2]), cy[3]);
+        const double maxy = std::fmax(std::fmax(std::fmax(cy[0], cy[1]), cy[2]), cy[3]);
+        const int64_t lp = akern == BICUBIC ? 1 : 0;
+        const int64_t rp = akern == NEAREST ? 0 : (akern == BILINEAR ? 1 : 2);
+        affine_axis((int64_t)std::floor(minx) - lp, (int64_t)std::floor(maxx) + rp, in_w, &r.x, &r.w);
+        affine_axis((int64_t)std::floor(miny) - lp, (int64_t)std::floor(maxy) + rp, in_h, &r.y, &r.h);
+        break;
+    }
     case N_GAUSS_H:
     case N_CONVSEP_H:
         r.x = o.x - radius();
@@ -288,9 +344,24 @@ uint32_t integer_shrink(uint32_t in_len, uint32_t target, double total) {
 
 }  // namespace
 
+// The thumbnail's Affine step: matrix [inv_x, 0, 0, inv_y], translation
+// 0.5 (1 - inv) except for Nearest, kernel mapped by
+// affine_kernel_for_resize_kernel (thumbnail.rs:31-37).
+static PlanStep thumb_affine(double inv_x, double inv_y, uint32_t ow, uint32_t oh, int kernel) {
+    PlanStep s{N_AFFINE, 0.0};
+    s.a = inv_x;
+    s.d = inv_y;
+    s.tx = kernel == NEAREST ? 0.0 : 0.5 * (1.0 - inv_x);
+    s.ty = kernel == NEAREST ? 0.0 : 0.5 * (1.0 - inv_y);
+    s.ew = ow;
+    s.eh = oh;
+    s.akern = kernel == NEAREST ? NEAREST : (kernel == BILINEAR ? BILINEAR : BICUBIC);
+    return s;
+}
+
 Status plan_thumbnail(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint32_t tw,
                       uint32_t th, std::vector<PlanStep>* steps, uint32_t* out_w,
-                      uint32_t* out_h, bool crop) {
+                      uint32_t* out_h, bool crop, int kernel) {
     steps->clear();
     if (bands == 0) return E_INVALID_ARGUMENT;
     if (target < FITBOX || target > HEIGHT) return E_INVALID_ARGUMENT;
@@ -327,9 +398,10 @@ Status plan_thumbnail(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint
     std::vector<PlanStep> geo;
     if (iw == 1 || ih == 1) {
         if (single) tgw = tgh = 1;
-        if (tgw != iw || tgh != ih) return E_UNSUPPORTED;  // Affine-only plan
         gw = tgw;
         gh = tgh;
+        if (tgw != iw || tgh != ih)  // thumbnail.rs:292-316
+            geo.push_back(thumb_affine((double)iw / (double)tgw, (double)ih / (double)tgh, tgw, tgh, kernel));
     } else {
         const double hs = (double)iw / (double)gw, vs = (double)ih / (double)gh;
         const uint32_t sh = integer_shrink(iw, gw, hs), sv = integer_shrink(ih, gh, vs);
@@ -341,9 +413,9 @@ Status plan_thumbnail(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint
         if (sv > 1) geo.push_back({N_SHRINK_V, (double)sv});
         if (rh > 1.0 + kPlanEps) geo.push_back({N_REDUCE_H, rh});
         if (rv > 1.0 + kPlanEps) geo.push_back({N_REDUCE_V, rv});
-        if (std::fabs((double)gw / (double)arw - 1.0) > kPlanEps ||
-            std::fabs((double)gh / (double)arh - 1.0) > kPlanEps)
-            return E_UNSUPPORTED;  // needs the Affine geometry fix-up
+        const double ax = (double)gw / (double)arw, ay = (double)gh / (double)arh;
+        if (std::fabs(ax - 1.0) > kPlanEps || std::fabs(ay - 1.0) > kPlanEps)  // thumbnail.rs:396-419
+            geo.push_back(thumb_affine(1.0 / ax, 1.0 / ay, gw, gh, kernel));
         if (crop && (gw != tgw || gh != tgh)) {  // thumbnail.rs:421-428
             PlanStep e{N_EXTRACT, 0.0};
             e.ex = (gw - tgw) / 2;

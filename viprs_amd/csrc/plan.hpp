@@ -8,6 +8,7 @@
 
 #include <cstdint>
 #include <string>
+#include <cmath>
 #include <vector>
 
 namespace vgpu {
@@ -99,6 +100,7 @@ enum NodeKind : int {
     N_REDUCE_OP_H = 12,
     N_REDUCE_OP_V = 13,
     N_EXTRACT = 14,  // ExtractArea (structural/extract_area.rs:23-60)
+    N_AFFINE = 15,   // the thumbnail's Affine: [a, 0, 0, d], ExtendMode::Copy (affine/core.rs)
 };
 
 struct Node {
@@ -114,10 +116,30 @@ struct Node {
     int32_t iscale = 1;
     double cscale = 1.0, coffset = 0.0;
     int32_t ex = 0, ey = 0;     // extract area: offset of the output in the input
+    // affine: matrix [a, b, c, d], translation, kernel (NEAREST / BILINEAR /
+    // BICUBIC), and whether the scheduler runs it on 128 x 128 tiles
+    // (SmallTile) rather than full-width strips
+    double am[4] = {1.0, 0.0, 0.0, 1.0}, atx = 0.0, aty = 0.0;
+    int akern = 0;
+    bool asmall = false;
 
     int radius() const { return conv.empty() ? 0 : (int)(conv.size() - 1) / 2; }
     Rect required_input(const Rect& out) const;
+    // Affine::map_to_input (affine/core.rs:78-84)
+    void affine_map(double x, double y, double* mx, double* my) const {
+        *mx = std::fma(am[1], y, am[0] * x) + atx;
+        *my = std::fma(am[3], y, am[2] * x) + aty;
+    }
 };
+
+// Per-column (or per-row) source positions of an Affine node over
+// [first, first + n) of its output: Affine::process_region (affine/core.rs:
+// 1593-1632) starts each scheduler tile at its origin and steps one matrix
+// entry per pixel, so a position depends on where its tile starts.  Tiles
+// are `tile` wide in the pipeline's output, which sits at `shift` in the
+// Affine's output.  Matrix diagonal: columns use a / tx, rows d / ty.
+void affine_positions(const Node& nd, bool columns, int64_t first, int64_t n, int64_t tile, int64_t shift,
+                      std::vector<double>* pos);
 
 // Thumbnail planner (thumbnail.rs:229-524), in-memory source (no
 // shrink-on-load), crop = false.
@@ -125,13 +147,15 @@ enum ThumbTarget : int { FITBOX = 0, FORCE_EXACT = 1, WIDTH = 2, HEIGHT = 3 };
 struct PlanStep {
     NodeKind kind;
     double param;
-    uint32_t ex = 0, ey = 0, ew = 0, eh = 0;  // N_EXTRACT rectangle
+    uint32_t ex = 0, ey = 0, ew = 0, eh = 0;  // N_EXTRACT rectangle; N_AFFINE output size (ew x eh)
+    double a = 1.0, d = 1.0, tx = 0.0, ty = 0.0;  // N_AFFINE
+    int akern = 0;
 };
 // crop: Thumbnail::with_crop (thumbnail.rs:185-190), a centred ExtractArea
 // of the covering geometry.
 Status plan_thumbnail(uint32_t in_w, uint32_t in_h, uint32_t bands, int target, uint32_t tw,
                       uint32_t th, std::vector<PlanStep>* steps, uint32_t* out_w,
-                      uint32_t* out_h, bool crop = false);
+                      uint32_t* out_h, bool crop = false, int kernel = 4);
 // ResizeOp::into_pipeline_nodes, downscale subset (resize.rs:163-284).
 Status plan_resize(uint32_t in_w, uint32_t in_h, double hscale, double vscale, int kernel,
                    std::vector<PlanStep>* steps, int* eff_kernel);

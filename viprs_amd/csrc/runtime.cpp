@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -371,6 +373,49 @@ bool plan_sep(int family, const SepAxis& hax, const SepAxis& vax, int32_t hbias,
     }
     stg->kind = ST_SEP;
     return true;
+}
+
+// Tap tables of an Affine stage (node i): the source position of every
+// output column and row as the reference's tile walk computes it, then its
+// AffineTap.  The scheduler's tiles cover the pipeline's output (the last
+// rectangle of out_r): 128 x 128 for SmallTile, else full-width strips of 16
+// rows (1 row beyond 10000 px) (op/demand.rs:41-85); the ExtractArea nodes
+// after the Affine place that output at their summed offset.
+void stage_affine_params(const std::vector<Node>& nodes, int i, const std::vector<Rect>& out_r, Staging* st,
+                         Stage* s) {
+    const Node& nd = nodes[(size_t)i];
+    const Rect& fin = out_r.back();
+    int64_t sx = 0, sy = 0;
+    for (size_t k = (size_t)i + 1; k < nodes.size(); ++k)
+        if (nodes[k].kind == N_EXTRACT) {
+            sx += nodes[k].ex;
+            sy += nodes[k].ey;
+        }
+    const int64_t tw = nd.asmall ? 128 : std::max<int64_t>(fin.w, 1);
+    const int64_t th = std::max<int64_t>(std::min<int64_t>(nd.asmall ? 128 : (fin.w > 10000 ? 1 : 16), fin.h), 1);
+    auto taps = [&](bool columns, int64_t first, int64_t n, int64_t tile, int64_t shift) {
+        std::vector<double> pos;
+        affine_positions(nd, columns, first, n, tile, shift, &pos);
+        std::vector<AffineTap> t(pos.size());
+        for (size_t k = 0; k < pos.size(); ++k) {
+            const double x = pos[k];
+            AffineTap& a = t[k];
+            if (nd.akern == BICUBIC) {  // interp_separable<2>: taps floor(x) - 1 .. floor(x) + 2
+                const int64_t c = (int64_t)std::floor(x);
+                a.i0 = (int32_t)(c - 1);
+                for (int q = 0; q < 4; ++q) a.w[q] = kernel_weight(BICUBIC, std::fabs((double)(c - 1 + q) - x));
+            } else {  // `x as i64` truncates toward zero
+                const int64_t c = (int64_t)x;
+                a.i0 = (int32_t)c;
+                a.f = x - (double)c;
+            }
+        }
+        return t;
+    };
+    const std::vector<AffineTap> xs = taps(true, s->out.x, s->out.w, tw, sx);
+    const std::vector<AffineTap> ys = taps(false, s->out.y, s->out.h, th, sy);
+    s->aff_x = st->put(xs.data(), xs.size() * sizeof(AffineTap));
+    s->aff_y = st->put(ys.data(), ys.size() * sizeof(AffineTap));
 }
 
 void stage_reduce_params(const Node& nd, const Rect& out, Staging* st, Stage* s) {
@@ -908,6 +953,20 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
     case ST_ALPHA:
         e = launch_alpha_u8(in, out, B, nd.kind == N_PREMULT ? ALPHA_PREMULT : ALPHA_UNPREMULT, stream);
         break;
+    case ST_AFFINE: {
+        AffineJob j;
+        j.in = in;
+        j.out = out;
+        j.bands = B;
+        j.fmt = nd.in_fmt;
+        j.kernel = nd.akern;
+        j.in_w = (int32_t)nd.in_w;
+        j.in_h = (int32_t)nd.in_h;
+        j.xs = reinterpret_cast<const AffineTap*>(blob + s.aff_x);
+        j.ys = reinterpret_cast<const AffineTap*>(blob + s.aff_y);
+        e = launch_affine(j, stream);
+        break;
+    }
     case ST_COPY: {
         // ExtractArea reads its input at the output's coordinates plus its offset.
         OutView o = out;
@@ -924,6 +983,19 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
 
 // Group a pipeline's nodes into fused stages over the regions of the backward
 // walk; every stage's plan data goes into `st`.
+bool affine_tail_supported(const std::vector<Node>& nodes) {
+    bool seen = false, small = false, u8_reduce_h = false;
+    for (const Node& nd : nodes) {
+        if (seen && nd.kind != N_EXTRACT && nd.kind != N_UNPREMULT) return false;
+        if (nd.kind == N_AFFINE) {
+            seen = true;
+            small = nd.asmall;
+        }
+        if (nd.kind == N_REDUCE_H && nd.in_fmt == U8) u8_reduce_h = true;
+    }
+    return !(small && u8_reduce_h);
+}
+
 std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector<Rect>& in_r,
                                const std::vector<Rect>& out_r, Staging* st, const StageOptions& opt) {
     std::vector<Stage> stages;
@@ -1058,11 +1130,13 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
             case N_CONVSEP_V: s.kind = ST_CONV; break;
             case N_PREMULT:
             case N_UNPREMULT: s.kind = ST_ALPHA; break;
+            case N_AFFINE: s.kind = ST_AFFINE; break;
             default: s.kind = ST_COPY; break;
             }
         }
         s.out = out_r[(size_t)s.last];
         s.out_fmt = nodes[(size_t)s.last].out_fmt;
+        if (s.kind == ST_AFFINE) stage_affine_params(nodes, i, out_r, st, &s);
         if (s.kind == ST_REDUCE_H || s.kind == ST_REDUCE_V) stage_reduce_params(nd, s.out, st, &s);
         if (s.kind == ST_CONV) stage_conv_params(nd, st, &s);
         stages.push_back(s);
@@ -1088,6 +1162,7 @@ void backward_regions(const std::vector<Node>& nodes, uint32_t w, uint32_t h, st
 
 int vgpu_pipeline::compile() {
     if (compiled) return VGPU_OK;
+    if (!affine_tail_supported(nodes)) return VGPU_E_UNSUPPORTED;
     const int n = (int)nodes.size();
     if (n == 0 || cur_w == 0 || cur_h == 0) {
         compiled = true;
@@ -1394,7 +1469,32 @@ int vgpu_pipeline_unpremultiply(vgpu_pipeline* p) {
     return VGPU_OK;
 }
 
-static int apply_steps(vgpu_pipeline* p, const std::vector<PlanStep>& steps, int kernel) {
+// Thumbnail::Affine -> PipelineBuilder::affine_backward_with_extend_and_hint
+// (builder/resample.rs:478-560, 932-955) -> AffineBridge::new_with_extend
+// (builder/state.rs:298-341) -> Affine::try_new (affine/interpolation.rs:
+// 13-41): finite coefficients, a non-singular matrix, ExtendMode::Copy over
+// the whole input.  small: the SmallTile demand of a thumbnail of an input no
+// larger than 512 x 512.
+static int push_affine(vgpu_pipeline* p, const PlanStep& s, bool small) {
+    const double m[4] = {s.a, 0.0, 0.0, s.d};
+    for (double v : m)
+        if (!std::isfinite(v)) return VGPU_E_INVALID_ARGUMENT;
+    if (!std::isfinite(s.tx) || !std::isfinite(s.ty)) return VGPU_E_INVALID_ARGUMENT;
+    if (std::fabs(std::fma(m[1], -m[2], m[0] * m[3])) <= DBL_EPSILON) return VGPU_E_INVALID_ARGUMENT;
+    if (s.ew == 0 || s.eh == 0) return VGPU_E_INVALID_ARGUMENT;
+    Node& nd = p->push(N_AFFINE);
+    for (int k = 0; k < 4; ++k) nd.am[k] = m[k];
+    nd.atx = s.tx;
+    nd.aty = s.ty;
+    nd.akern = s.akern;
+    nd.asmall = small;
+    nd.out_w = s.ew;
+    nd.out_h = s.eh;
+    p->commit();
+    return VGPU_OK;
+}
+
+static int apply_steps(vgpu_pipeline* p, const std::vector<PlanStep>& steps, int kernel, bool affine_small = false) {
     for (const PlanStep& s : steps) {
         int rc = VGPU_OK;
         switch (s.kind) {
@@ -1405,6 +1505,7 @@ static int apply_steps(vgpu_pipeline* p, const std::vector<PlanStep>& steps, int
         case N_REDUCE_H: rc = vgpu_pipeline_reduce_h(p, s.param, kernel); break;
         case N_REDUCE_V: rc = vgpu_pipeline_reduce_v(p, s.param, kernel); break;
         case N_EXTRACT: rc = vgpu_pipeline_extract_area(p, s.ex, s.ey, s.ew, s.eh); break;
+        case N_AFFINE: rc = push_affine(p, s, affine_small); break;
         default: rc = VGPU_E_UNSUPPORTED; break;
         }
         if (rc != VGPU_OK) return rc;
@@ -1426,12 +1527,27 @@ int vgpu_pipeline_thumbnail_crop(vgpu_pipeline* p, int target, uint32_t w, uint3
         return VGPU_E_INVALID_KERNEL;
     std::vector<PlanStep> steps;
     uint32_t ow = 0, oh = 0;
-    int rc = plan_thumbnail(p->cur_w, p->cur_h, p->bands, target, w, h, &steps, &ow, &oh, crop != 0);
+    int rc = plan_thumbnail(p->cur_w, p->cur_h, p->bands, target, w, h, &steps, &ow, &oh, crop != 0, kernel);
     if (rc != OK) return rc;
     if (p->cur_fmt != U8)
         for (const PlanStep& s : steps)
             if (s.kind == N_PREMULT) return VGPU_E_UNSUPPORTED_FORMAT;
-    return apply_steps(p, steps, kernel);
+    // builder/resample.rs:908-917,939-943: the Affine of a thumbnail of an
+    // input up to 512 x 512 asks for SmallTile tiles.
+    const bool small = p->cur_w <= 512 && p->cur_h <= 512;
+    const size_t n0 = p->nodes.size();
+    const uint32_t w0 = p->cur_w, h0 = p->cur_h;
+    const int f0 = p->cur_fmt;
+    rc = apply_steps(p, steps, kernel, small);
+    if (rc == VGPU_OK && !affine_tail_supported(p->nodes)) rc = VGPU_E_UNSUPPORTED;
+    if (rc != VGPU_OK) {  // leave the pipeline as it was
+        p->release();
+        p->nodes.resize(n0);
+        p->cur_w = w0;
+        p->cur_h = h0;
+        p->cur_fmt = f0;
+    }
+    return rc;
 }
 
 int vgpu_pipeline_resize(vgpu_pipeline* p, double hscale, double vscale, int kernel) {

@@ -118,6 +118,7 @@ enum StageKind {
     ST_CONV,
     ST_ALPHA,
     ST_COPY,
+    ST_AFFINE,   // the thumbnail's Affine tail
 };
 
 struct Stage {
@@ -140,6 +141,7 @@ struct Stage {
     // and (family 1) H operand patterns, per-strip (pattern, first pixel).
     size_t v_tab = 0, v_tile = 0, h_tab = 0, h_strip = 0;
     size_t w32 = 0, v_w32 = 0;  // ST_RED_F32: f32 weight tables
+    size_t aff_x = 0, aff_y = 0;  // ST_AFFINE: per-column / per-row AffineTap tables
     F16Job f16;                 // ST_SEP_F16: job template (table offsets below)
     GaussJob gauss;             // ST_GAUSS_U8: job template (views set at launch)
 };
@@ -166,6 +168,11 @@ InView view_of(const void* ptr, const Rect& buf, const Rect& clip);
 OutView outview_of(void* ptr, const Rect& r);
 void backward_regions(const std::vector<Node>& nodes, uint32_t w, uint32_t h, std::vector<Rect>* in_r,
                       std::vector<Rect>* out_r);
+// Whether the device reproduces a chain holding the thumbnail's Affine: the
+// nodes after it may only move regions (ExtractArea, Unpremultiply), and an
+// Affine on SmallTile (128 x 128) tiles may not follow a u8 ReduceH, whose
+// fixed-point positions accumulate from each tile's origin.
+bool affine_tail_supported(const std::vector<Node>& nodes);
 std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector<Rect>& in_r,
                                const std::vector<Rect>& out_r, Staging* st,
                                const StageOptions& opt = StageOptions());

@@ -215,6 +215,7 @@ void vgpu_server::plan_job(Job& j) {
     if (rc == VGPU_OK && r.thumbnail) rc = vgpu_pipeline_thumbnail(p, r.target, r.target_w, r.target_h, r.kernel);
     if (rc == VGPU_OK && r.blur_sigma > 0.f) rc = vgpu_pipeline_gauss_blur(p, r.blur_sigma);
     if (rc == VGPU_OK) rc = vgpu_pipeline_output_info(p, &j.ow, &j.oh, &j.ob, &j.ofmt);
+    if (rc == VGPU_OK && !affine_tail_supported(p->nodes)) rc = VGPU_E_UNSUPPORTED;
     if (rc != VGPU_OK) {
         j.status = rc;
         return;
