@@ -335,8 +335,9 @@ enum {
     N_REDUCE_OP_H = 12, /* ReduceOp's horizontal pass (reduce.rs:104-133) */
     N_REDUCE_OP_V = 13, /* ReduceOp's vertical pass (reduce.rs:136-166) */
     N_EXTRACT = 14,     /* ExtractArea (structural/extract_area.rs:23-60) */
-    N_AFFINE = 15       /* Affine, ExtendMode::Copy, source bounds = its input
+    N_AFFINE = 15,      /* Affine, ExtendMode::Copy, source bounds = its input
                          * (affine/core.rs; builder/state.rs:298-341) */
+    N_ZOOM = 16         /* Zoom (structural/zoom.rs:339-400) */
 };
 
 typedef struct {
@@ -361,6 +362,8 @@ typedef struct {
     /* affine: [a, b, c, d], translation, kernel, SmallTile demand */
     double am[4], atx, aty;
     int akern, asmall;
+    /* zoom factors */
+    uint32_t zx, zy;
 } node_t;
 
 struct orc_pipeline {
@@ -591,7 +594,9 @@ int orc_pipeline_affine(orc_pipeline *p, double a, double d, double tx, double t
         if (!isfinite(m[i])) return ORC_EINVAL;
     if (!isfinite(tx) || !isfinite(ty)) return ORC_EINVAL;
     if (fabs(fma(m[1], -m[2], m[0] * m[3])) <= DBL_EPSILON) return ORC_EINVAL;
-    if (kernel != ORC_NEAREST && kernel != ORC_BILINEAR && kernel != ORC_BICUBIC) return ORC_EUNSUPPORTED;
+    if (kernel != ORC_NEAREST && kernel != ORC_BILINEAR && kernel != ORC_BICUBIC && kernel != ORC_CATMULLROM &&
+        kernel != ORC_LANCZOS2 && kernel != ORC_LANCZOS3)
+        return ORC_EUNSUPPORTED;
     node_t *nd = push(p, N_AFFINE);
     if (!nd) return ORC_EINVAL;
     memcpy(nd->am, m, sizeof(m));
@@ -861,8 +866,8 @@ int orc_pipeline_thumbnail(orc_pipeline *p, int target, uint32_t tw, uint32_t th
     return orc_pipeline_thumbnail_crop(p, target, tw, th, kernel, 0);
 }
 
-/* ResizeOp::into_pipeline_nodes, downscale part (resize.rs:163-284), and
- * PipelineBuilder::resize (builder/resample.rs:637-711). */
+/* ResizeOp::into_pipeline_nodes (resize.rs:163-284) without the LBB / VSQBS
+ * interpolators, and PipelineBuilder::resize (builder/resample.rs:637-711). */
 int orc_pipeline_resize(orc_pipeline *p, double hscale, double vscale, int kernel) {
     if (p->bands == 0) return ORC_EINVAL;
     if (!(isfinite(hscale) && hscale > 0.0)) hscale = 1.0;
@@ -884,7 +889,6 @@ int orc_pipeline_resize(orc_pipeline *p, double hscale, double vscale, int kerne
         double v = floor((double)ch / (double)th / gap);
         vsh = sat_u32(v > 1.0 ? v : 1.0);
     }
-    if (tw > cw || th > ch) return ORC_EUNSUPPORTED; /* Zoom / Affine upscale */
     int any = 0, rc;
     if (vsh > 1) {
         if ((rc = orc_pipeline_shrink_v(p, vsh, 0)) != ORC_OK) return rc;
@@ -906,8 +910,28 @@ int orc_pipeline_resize(orc_pipeline *p, double hscale, double vscale, int kerne
         cw = tw;
         any = 1;
     }
-    if (fabs((double)tw / (double)cw - 1.0) > PLAN_EPS || fabs((double)th / (double)ch - 1.0) > PLAN_EPS)
-        return ORC_EUNSUPPORTED;
+    /* resize.rs:240-280: integer Zoom of what remains to grow, then the
+     * Affine fix-up (SmallTile demand, builder/resample.rs:690-706). */
+    const uint32_t zx = tw > cw ? tw / cw : 1, zy = th > ch ? th / ch : 1;
+    if (zx > 1 || zy > 1) {
+        node_t *nd = push(p, N_ZOOM);
+        if (!nd) return ORC_EINVAL;
+        nd->zx = zx;
+        nd->zy = zy;
+        nd->out_w = cw * zx;
+        nd->out_h = ch * zy;
+        commit(p, nd);
+        cw *= zx;
+        ch *= zy;
+        any = 1;
+    }
+    const double ax = (double)tw / (double)cw, ay = (double)th / (double)ch;
+    if (fabs(ax - 1.0) > PLAN_EPS || fabs(ay - 1.0) > PLAN_EPS) {
+        const double ox = kernel == ORC_NEAREST ? 0.0 : 0.5 * (1.0 - 1.0 / ax);
+        const double oy = kernel == ORC_NEAREST ? 0.0 : 0.5 * (1.0 - 1.0 / ay);
+        if ((rc = orc_pipeline_affine(p, 1.0 / ax, 1.0 / ay, ox, oy, tw, th, kernel, 1)) != ORC_OK) return rc;
+        any = 1;
+    }
     if (!any) {
         node_t *nd = push(p, N_COPY);
         if (!nd) return ORC_EINVAL;
@@ -936,11 +960,12 @@ int orc_pipeline_node_info(const orc_pipeline *p, int i, int *type, double *para
 
 /* ------------------------------------------------- required input regions */
 
-/* InterpolationKernel::affine_padding (kernel.rs:80-117) for the affine
- * kernels of the thumbnail. */
+/* InterpolationKernel::affine_padding (kernel.rs:80-117): window_offset
+ * and window_size - offset - 1. */
 static void affine_pad(int k, int64_t *left, int64_t *right) {
-    *left = k == ORC_BICUBIC ? 1 : 0;
-    *right = k == ORC_NEAREST ? 0 : (k == ORC_BILINEAR ? 1 : 2);
+    const int64_t size = k == ORC_NEAREST ? 1 : (k == ORC_BILINEAR ? 2 : (k == ORC_LANCZOS3 ? 6 : 4));
+    *left = (k == ORC_NEAREST || k == ORC_BILINEAR) ? 0 : (k == ORC_LANCZOS3 ? 2 : 1);
+    *right = size - *left - 1;
 }
 
 /* Affine::map_to_input (affine/core.rs:78-84). */
@@ -995,6 +1020,15 @@ static rgn required_input(const node_t *nd, rgn o) {
         r.x = clampi(o.x + nd->ex, INT32_MIN, INT32_MAX);
         r.y = clampi(o.y + nd->ey, INT32_MIN, INT32_MAX);
         break;
+    case N_ZOOM: { /* zoom.rs:343-349 */
+        const int64_t left = o.x / nd->zx, top = o.y / nd->zy;
+        const int64_t right = (o.x + o.w - 1) / nd->zx, bottom = (o.y + o.h - 1) / nd->zy;
+        r.x = left;
+        r.y = top;
+        r.w = right - left + 1;
+        r.h = bottom - top + 1;
+        break;
+    }
     case N_AFFINE: { /* affine/core.rs:344-366,1531-1560: mapped corner pixels, floor, padding */
         const double ox0 = (double)o.x, oy0 = (double)o.y;
         const double ox1 = (double)o.x + (double)(o.w > 0 ? o.w - 1 : 0);
@@ -1358,8 +1392,8 @@ static double bilinear_mix(double p00, double p01, double p10, double p11, doubl
     return top * (1.0 - fy) + bottom * fy;
 }
 
-/* One band at (x, y): interp_nearest / interp_bilinear / interp_separable<2>
- * for Bicubic (affine/core.rs:485-514,648-697), and their premultiplied
+/* One band at (x, y): interp_nearest / interp_bilinear / interp_separable
+ * for the cubic and Lanczos kernels (affine/core.rs:485-514,648-697,728-749), and their premultiplied
  * forms (sample_at_premultiplied, affine/core.rs:751-935,937-986) when pm. */
 static double affine_band(const node_t *nd, const void *in, rgn ir, double x, double y, int band, int pm,
                           double amax) {
@@ -1372,15 +1406,18 @@ static double affine_band(const node_t *nd, const void *in, rgn ir, double x, do
                             affine_sample_pm(nd, in, ir, x0, y0 + 1, band, pm, amax),
                             affine_sample_pm(nd, in, ir, x0 + 1, y0 + 1, band, pm, amax), fx, fy);
     }
+    /* interp_separable<SUPPORT>: SUPPORT 3 for Lanczos3, else 2; taps
+     * floor - SUPPORT + 1 .. floor + SUPPORT */
+    const int64_t S = nd->akern == ORC_LANCZOS3 ? 3 : 2;
     const int64_t cx = (int64_t)floor(x), cy = (int64_t)floor(y);
     double acc = 0.0, wsum = 0.0;
-    for (int64_t ky = -1; ky <= 2; ++ky) {
+    for (int64_t ky = 1 - S; ky <= S; ++ky) {
         const int64_t iy = cy + ky;
-        const double wy = orc_kernel_weight(ORC_BICUBIC, fabs((double)iy - y));
+        const double wy = orc_kernel_weight(nd->akern, fabs((double)iy - y));
         if (wy == 0.0) continue;
-        for (int64_t kx = -1; kx <= 2; ++kx) {
+        for (int64_t kx = 1 - S; kx <= S; ++kx) {
             const int64_t ix = cx + kx;
-            const double wx = orc_kernel_weight(ORC_BICUBIC, fabs((double)ix - x));
+            const double wx = orc_kernel_weight(nd->akern, fabs((double)ix - x));
             if (wx == 0.0) continue;
             const double w = wx * wy;
             wsum += w;
@@ -1426,6 +1463,18 @@ static void process_affine(const node_t *nd, const void *in, rgn ir, void *out, 
     }
 }
 
+/* Zoom::process_region (zoom.rs:364-400): every output pixel repeats the
+ * input pixel (x / xfac, y / yfac). */
+static void process_zoom(const node_t *nd, const void *in, rgn ir, void *out, rgn orr) {
+    const size_t ps = (size_t)nd->bands * fmt_size(nd->in_fmt);
+    for (int64_t y = 0; y < orr.h; ++y)
+        for (int64_t x = 0; x < orr.w; ++x) {
+            const int64_t sx = (orr.x + x) / nd->zx - ir.x, sy = (orr.y + y) / nd->zy - ir.y;
+            memcpy((uint8_t *)out + ((size_t)y * (size_t)orr.w + (size_t)x) * ps,
+                   (const uint8_t *)in + ((size_t)sy * (size_t)ir.w + (size_t)sx) * ps, ps);
+        }
+}
+
 static void process_node(const node_t *nd, const void *in, rgn ir, void *out, rgn orr) {
     switch (nd->type) {
     case N_REDUCE_H: process_reduce_h(nd, in, ir, out, orr); break;
@@ -1441,6 +1490,7 @@ static void process_node(const node_t *nd, const void *in, rgn ir, void *out, rg
     case N_PREMULT:
     case N_UNPREMULT: process_alpha(nd, in, out, orr); break;
     case N_AFFINE: process_affine(nd, in, ir, out, orr); break;
+    case N_ZOOM: process_zoom(nd, in, ir, out, orr); break;
     default: memcpy(out, in, (size_t)orr.w * (size_t)orr.h * nd->bands * fmt_size(nd->in_fmt)); break;
     }
 }

@@ -620,3 +620,43 @@ def test_builder_validation_errors(ctx):
         p.reduce_v(400.0, v.LANCZOS3)  # over the 2000-tap limit
     with pytest.raises(v.VgpuError):
         v.Pipeline(ctx, 10, 10, 0)
+
+
+RESIZE_UP = [
+    # (w, h, bands, dtype, hscale, vscale, kernel)
+    (100, 80, 3, np.uint8, 2.0, 3.0, v.LANCZOS3),      # Zoom alone
+    (100, 80, 3, np.uint8, 2.5, 1.3, v.LANCZOS3),      # Zoom + Affine, 6 taps
+    (64, 64, 1, np.uint8, 1.7, 1.7, v.NEAREST),
+    (77, 51, 4, np.uint8, 3.3, 2.2, v.BILINEAR),       # alpha sampled premultiplied
+    (90, 70, 3, np.uint8, 1.9, 2.6, v.BICUBIC),
+    (90, 70, 2, np.uint8, 1.45, 1.45, v.LANCZOS2),
+    (90, 70, 3, np.uint8, 2.2, 1.6, v.CATMULLROM),
+    (300, 200, 3, np.uint8, 1.0, 2.4, v.LANCZOS3),      # one axis only
+    (300, 200, 3, np.uint8, 1.3, 0.5, v.LANCZOS3),      # ReduceV then the Affine
+    (120, 90, 3, np.uint16, 2.3, 1.7, v.LANCZOS3),
+    (120, 90, 1, np.float32, 2.3, 1.7, v.BICUBIC),
+    (120, 90, 4, np.float32, 1.6, 1.6, v.LANCZOS3),
+]
+
+
+@pytest.mark.parametrize("case", range(len(RESIZE_UP)))
+def test_resize_upscale(ctx, case):
+    w, h, b, dtype, hs, vs, kernel = RESIZE_UP[case]
+    src = {np.uint8: noise_u8, np.uint16: noise_u16, np.float32: float_image}[dtype](h, w, b, 60 + case)
+    if b == 4:
+        src[..., 3] = np.where(np.random.default_rng(case).random((h, w)) < 0.2, 0, src[..., 3])
+    got, exp = run_both(ctx, src, [("resize", (hs, vs, kernel))])
+    assert_same(got, exp, f"resize up {RESIZE_UP[case]}")
+
+
+def test_resize_mixed_u8_declined(ctx):
+    # resize.rs:441-457 shape: a u8 ReduceH before a SmallTile Affine
+    p = v.Pipeline(ctx, 100, 100, 3)
+    with pytest.raises(v.VgpuError) as e:
+        p.resize(0.5, 2.5, v.BILINEAR)
+    assert e.value.status == v.api.E_UNSUPPORTED
+    assert p.output_info()[:2] == (100, 100)
+    # the same plan on u16 runs (its ReduceH positions do not accumulate)
+    src = noise_u16(100, 100, 3, 1)
+    got, exp = run_both(ctx, src, [("resize", (0.5, 2.5, v.BILINEAR))])
+    assert_same(got, exp, "mixed resize u16")

@@ -696,3 +696,75 @@ def test_thumbnail_affine_tail_runs():
     # the Affine alone is the numpy model on the 1-pixel column
     v = _affine_model(col, 1.0, 2048 / 400, 0.0, 0.5 * (1 - 2048 / 400), 1, 400, O.BICUBIC, (1, 16))
     _assert_rounds_to(got, v, "1x2048 -> 1x400")
+
+
+# ------------------------------------------------------------ resize upscale
+
+N_REDUCE_H, N_AFFINE, N_ZOOM = 3, 15, 16
+
+
+def test_resize_upscale_plans_reference_cases():
+    # resize.rs:441-457: 0.5 x 2.5 Bilinear on 100x100 -> ReduceH(2), Zoom(1, 2), Affine; 50x250
+    p = O.Pipeline(100, 100, 3).resize(0.5, 2.5, O.BILINEAR)
+    nodes = p.nodes()
+    assert [t for t, _ in nodes] == [N_REDUCE_H, N_ZOOM, N_AFFINE] and nodes[0][1] == 2.0
+    assert p.output()[:2] == (50, 250)
+    # resize.rs:485-496: Nearest upscale keeps a zero offset (1.5x of 100 -> Affine only)
+    p = O.Pipeline(100, 100, 1).resize(1.5, 1.5, O.NEAREST)
+    assert [t for t, _ in p.nodes()] == [N_AFFINE] and p.output()[:2] == (150, 150)
+    # resize.rs:518-530: output dimensions are round(in * scale), at least 1
+    for w, h, hs, vs in [(1, 1, 3.9, 0.05), (256, 7, 0.37, 3.3), (33, 200, 2.0, 1.01)]:
+        p = O.Pipeline(w, h, 1).resize(hs, vs, O.LANCZOS3)
+        assert p.output()[:2] == (max(1, int(np.floor(w * hs + 0.5))), max(1, int(np.floor(h * vs + 0.5))))
+
+
+def test_resize_integer_upscale_is_replication():
+    # An integer factor is a Zoom alone: each pixel repeated (zoom.rs:364-400)
+    src = noise_u8(40, 50, 3, 8)
+    got = O.Pipeline(50, 40, 3).resize(2.0, 3.0, O.LANCZOS3).run(src)
+    assert np.array_equal(got, np.repeat(np.repeat(src, 3, axis=0), 2, axis=1))
+
+
+@pytest.mark.parametrize("kernel", [O.LANCZOS2, O.LANCZOS3, O.CATMULLROM])
+def test_affine_lanczos_matches_numpy_model(kernel):
+    # the separable Affine with the resize kernels: Lanczos2 / CatmullRom over
+    # 4 taps, Lanczos3 over 6 (affine/core.rs:648-697,728-749)
+    def weight(t):
+        t = np.abs(t)
+        if kernel == O.CATMULLROM:
+            return _catmull_rom(t)
+        a = 3.0 if kernel == O.LANCZOS3 else 2.0
+        with np.errstate(invalid="ignore", divide="ignore"):
+            v = a * np.sin(np.pi * t) * np.sin(np.pi * t / a) / (np.pi * t) ** 2
+        return np.where(t < 1e-12, 1.0, np.where(t < a, v, 0.0))
+
+    S = 3 if kernel == O.LANCZOS3 else 2
+    src = noise_u8(29, 41, 3, kernel)
+    ow, oh = 100, 77
+    a, d = 41 / ow, 29 / oh
+    tx, ty = 0.5 * (1 - a), 0.5 * (1 - d)
+    got = O.Pipeline(41, 29, 3).affine(a, d, tx, ty, ow, oh, kernel).run(src)
+    # single 128 x 128 tile: positions step from the origin
+    X = np.empty(ow)
+    v_ = tx
+    for i in range(ow):
+        X[i] = v_
+        v_ += a
+    Y = np.empty(oh)
+    v_ = ty
+    for i in range(oh):
+        Y[i] = v_
+        v_ += d
+    img = src.astype(np.float64)
+    acc = np.zeros((oh, ow, 3))
+    wsum = np.zeros((oh, ow, 1))
+    cx, cy = np.floor(X).astype(int), np.floor(Y).astype(int)
+    for ky in range(1 - S, S + 1):
+        wy = weight(cy + ky - Y)[:, None, None]
+        for kx in range(1 - S, S + 1):
+            wx = weight(cx + kx - X)[None, :, None]
+            iy = np.clip(cy + ky, 0, 28)[:, None]
+            ix = np.clip(cx + kx, 0, 40)[None, :]
+            acc += img[iy, ix] * (wx * wy)
+            wsum += wx * wy
+    _assert_rounds_to(got, acc / wsum, f"kernel {kernel}")

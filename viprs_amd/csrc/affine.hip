@@ -1,16 +1,17 @@
-// The thumbnail's Affine tail (thumbnail.rs:292-316,396-419): an
-// axis-aligned matrix [a, 0, 0, d] with ExtendMode::Copy over its whole input
-// image, Nearest / Bilinear / Bicubic, in the reference's generic per-pixel
-// path (affine/core.rs:1593-1632 -> sample_pixel_at, 1022-1055).  The source
+// The Affine tails of thumbnail (thumbnail.rs:292-316,396-419) and resize
+// (resize.rs:258-280): an axis-aligned matrix [a, 0, 0, d] with
+// ExtendMode::Copy over its whole input image, in the reference's generic
+// per-pixel path (affine/core.rs:1593-1632 -> sample_pixel_at, 1022-1055).  The source
 // positions and, for Bicubic, the per-axis weights are tabulated on the host
 // (AffineTap), so a thread does the f64 sampling of one output pixel in the
 // reference's operation order:
 //  * Nearest: the sample at (trunc x, trunc y);
 //  * Bilinear: bilinear_mix (affine/core.rs:197-201) of the 2 x 2 samples
 //    from (trunc x, trunc y);
-//  * Bicubic: interp_separable<2> (affine/core.rs:648-697): rows outer,
-//    columns inner, taps with a zero weight skipped, acc = fma(p, wx * wy,
-//    acc), then acc / sum of weights;
+//  * Bicubic, CatmullRom, Lanczos2 (4 taps) and Lanczos3 (6 taps):
+//    interp_separable (affine/core.rs:648-697): rows outer, columns inner,
+//    taps with a zero weight skipped, acc = fma(p, wx * wy, acc), then
+//    acc / sum of weights;
 //  * four bands: sample_pixel_at_with_alpha (affine/core.rs:988-1020), every
 //    sample of a colour band multiplied by (its alpha / alpha_max) first, the
 //    result divided by the interpolated alpha / alpha_max.
@@ -51,11 +52,12 @@ __global__ __launch_bounds__(kThreads) void k_affine(AffineJob j) {
                     fma(pm(tx.i0 + 1, ty.i0 + 1, b), fx, pm(tx.i0, ty.i0 + 1, b) * (1.0 - fx));
                 return top * (1.0 - fy) + bottom * fy;
             } else {
+                constexpr int NT = K == AFF_SEP6 ? 6 : 4;
                 double acc = 0.0, wsum = 0.0;
-                for (int ky = 0; ky < 4; ++ky) {
+                for (int ky = 0; ky < NT; ++ky) {
                     const double wy = ty.w[ky];
                     if (wy == 0.0) continue;
-                    for (int kx = 0; kx < 4; ++kx) {
+                    for (int kx = 0; kx < NT; ++kx) {
                         const double wx = tx.w[kx];
                         if (wx == 0.0) continue;
                         const double w = wx * wy;
@@ -94,13 +96,36 @@ hipError_t launch_affine_k(const AffineJob& j, unsigned g, hipStream_t s) {
     switch (j.kernel) {
     case AFF_NEAREST: launch_affine_t<T, AFF_NEAREST>(j, g, s); break;
     case AFF_BILINEAR: launch_affine_t<T, AFF_BILINEAR>(j, g, s); break;
-    case AFF_BICUBIC: launch_affine_t<T, AFF_BICUBIC>(j, g, s); break;
+    case AFF_SEP4: launch_affine_t<T, AFF_SEP4>(j, g, s); break;
+    case AFF_SEP6: launch_affine_t<T, AFF_SEP6>(j, g, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_zoom(InView in, OutView out, int B, int zx, int zy) {
+    const size_t n = (size_t)out.w * out.h * B;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        const int b = (int)(i % B);
+        const size_t px = i / B;
+        const int x = out.x + (int)(px % out.w), y = out.y + (int)(px / out.w);
+        static_cast<T*>(out.ptr)[i] = load_at<T>(in, row_of(in, y / zy), col_of(in, x / zx), B, b);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_zoom(const InView& in, const OutView& out, int bands, int fmt, int zx, int zy, hipStream_t s) {
+    if (out.w <= 0 || out.h <= 0) return hipSuccess;
+    if (zx < 1 || zy < 1 || out.x < 0 || out.y < 0) return hipErrorInvalidValue;
+    const size_t n = (size_t)out.w * out.h * bands;
+    const unsigned g = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 65536);
+    if (fmt == 0) hipLaunchKernelGGL(k_zoom<uint8_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, zx, zy);
+    else if (fmt == 1) hipLaunchKernelGGL(k_zoom<uint16_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, zx, zy);
+    else hipLaunchKernelGGL(k_zoom<float>, dim3(g), dim3(kThreads), 0, s, in, out, bands, zx, zy);
+    return hipGetLastError();
+}
 
 hipError_t launch_affine(const AffineJob& j, hipStream_t s) {
     if (j.out.w <= 0 || j.out.h <= 0) return hipSuccess;

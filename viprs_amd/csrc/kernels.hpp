@@ -38,15 +38,20 @@ struct DevPlan {
 
 enum Alpha : int { ALPHA_NONE = 0, ALPHA_PREMULT = 1, ALPHA_UNPREMULT = 2 };
 
-// One output column (or row) of the thumbnail's axis-aligned Affine: the
-// first source index read (trunc(x) for Nearest and Bilinear, floor(x) - 1
-// for Bicubic), the Bilinear phase x - trunc(x), and the four Bicubic weights
-// of source indices i0 .. i0 + 3.
-enum AffineKernel : int { AFF_NEAREST = 0, AFF_BILINEAR = 1, AFF_BICUBIC = 2 };  // InterpolationKernel ids
+// One output column (or row) of an axis-aligned Affine: the first source
+// index read (trunc(x) for Nearest and Bilinear, floor(x) - S + 1 for the
+// separable kernels of half-width S), the Bilinear phase x - trunc(x), and
+// the separable weights of source indices i0 .. i0 + 2S - 1.
+enum AffineKernel : int {
+    AFF_NEAREST = 0,
+    AFF_BILINEAR = 1,
+    AFF_SEP4 = 2,  // Bicubic, CatmullRom, Lanczos2: interp_separable<2>
+    AFF_SEP6 = 3,  // Lanczos3: interp_separable<3>
+};
 struct AffineTap {
     int32_t i0 = 0, pad = 0;
     double f = 0.0;
-    double w[4] = {0.0, 0.0, 0.0, 0.0};
+    double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 };
 struct AffineJob {
     InView in;
@@ -208,6 +213,8 @@ hipError_t launch_alpha_u8(const InView& in, const OutView& out, int bands, int 
                            hipStream_t s);
 hipError_t launch_copy(const InView& in, const OutView& out, int bands, int fmt, hipStream_t s);
 hipError_t launch_affine(const AffineJob& job, hipStream_t s);
+// Zoom (structural/zoom.rs:364-400): output (x, y) repeats input (x / zx, y / zy).
+hipError_t launch_zoom(const InView& in, const OutView& out, int bands, int fmt, int zx, int zy, hipStream_t s);
 hipError_t launch_fill_random(void* dst, size_t bytes, uint64_t seed, hipStream_t s);
 
 }  // namespace vgpu

@@ -263,6 +263,15 @@ Rect Node::required_input(const Rect& o) const {
         r.x = clamp_i32_range(o.x + ex);
         r.y = clamp_i32_range(o.y + ey);
         break;
+    case N_ZOOM: {  // zoom.rs:343-349
+        const int64_t left = o.x / zx, top = o.y / zy;
+        const int64_t right = (o.x + o.w - 1) / zx, bottom = (o.y + o.h - 1) / zy;
+        r.x = left;
+        r.y = top;
+        r.w = right - left + 1;
+        r.h = bottom - top + 1;
+        break;
+    }
     case N_AFFINE: {  // affine/core.rs:344-366,373-405,1531-1560
         const double ox0 = (double)o.x, oy0 = (double)o.y;
         const double ox1 = (double)o.x + (double)(o.w > 0 ? o.w - 1 : 0);
@@ -274,8 +283,10 @@ Rect Node::required_input(const Rect& o) const {
         const double maxx = std::fmax(std::fmax(std::fmax(cx[0], cx[1]), cx[2]), cx[3]);
         const double miny = std::fmin(std::fmin(std::fmin(cy[0], cy[1]), cy[2]), cy[3]);
         const double maxy = std::fmax(std::fmax(std::fmax(cy[0], cy[1]), cy[2]), cy[3]);
-        const int64_t lp = akern == BICUBIC ? 1 : 0;
-        const int64_t rp = akern == NEAREST ? 0 : (akern == BILINEAR ? 1 : 2);
+        // InterpolationKernel::affine_padding (kernel.rs:80-117)
+        const int64_t size = akern == NEAREST ? 1 : (akern == BILINEAR ? 2 : (akern == LANCZOS3 ? 6 : 4));
+        const int64_t lp = (akern == NEAREST || akern == BILINEAR) ? 0 : (akern == LANCZOS3 ? 2 : 1);
+        const int64_t rp = size - lp - 1;
         affine_axis((int64_t)std::floor(minx) - lp, (int64_t)std::floor(maxx) + rp, in_w, &r.x, &r.w);
         affine_axis((int64_t)std::floor(miny) - lp, (int64_t)std::floor(maxy) + rp, in_h, &r.y, &r.h);
         break;
@@ -445,7 +456,6 @@ Status plan_resize(uint32_t iw, uint32_t ih, double hscale, double vscale, int k
     if (iw == 0 || ih == 0) return E_UNSUPPORTED;
     const uint32_t tw = to_u32_sat(std::max(std::round((double)iw * hscale), 1.0));
     const uint32_t th = to_u32_sat(std::max(std::round((double)ih * vscale), 1.0));
-    if (tw > iw || th > ih) return E_UNSUPPORTED;  // Zoom / Affine upscale
     auto preshrink = [&](uint32_t in, uint32_t t) -> uint32_t {
         if (t == 0 || in <= t || kernel == NEAREST) return 1;
         return to_u32_sat(std::max(std::floor((double)in / (double)t / 2.0), 1.0));
@@ -469,9 +479,29 @@ Status plan_resize(uint32_t iw, uint32_t ih, double hscale, double vscale, int k
         steps->push_back({N_REDUCE_H, (double)cw / (double)tw});
         cw = tw;
     }
-    if (std::fabs((double)tw / (double)cw - 1.0) > kPlanEps ||
-        std::fabs((double)th / (double)ch - 1.0) > kPlanEps)
-        return E_UNSUPPORTED;
+    // resize.rs:240-280: integer Zoom of what remains to grow, then the Affine
+    // fix-up with the resize kernel.
+    const uint32_t zx = tw > cw ? tw / cw : 1, zy = th > ch ? th / ch : 1;
+    if (zx > 1 || zy > 1) {
+        PlanStep z{N_ZOOM, 0.0};
+        z.zx = zx;
+        z.zy = zy;
+        steps->push_back(z);
+        cw *= zx;
+        ch *= zy;
+    }
+    const double ax = (double)tw / (double)cw, ay = (double)th / (double)ch;
+    if (std::fabs(ax - 1.0) > kPlanEps || std::fabs(ay - 1.0) > kPlanEps) {
+        PlanStep s{N_AFFINE, 0.0};
+        s.a = 1.0 / ax;
+        s.d = 1.0 / ay;
+        s.tx = kernel == NEAREST ? 0.0 : 0.5 * (1.0 - 1.0 / ax);
+        s.ty = kernel == NEAREST ? 0.0 : 0.5 * (1.0 - 1.0 / ay);
+        s.ew = tw;
+        s.eh = th;
+        s.akern = kernel;
+        steps->push_back(s);
+    }
     return OK;
 }
 

@@ -100,7 +100,8 @@ enum NodeKind : int {
     N_REDUCE_OP_H = 12,
     N_REDUCE_OP_V = 13,
     N_EXTRACT = 14,  // ExtractArea (structural/extract_area.rs:23-60)
-    N_AFFINE = 15,   // the thumbnail's Affine: [a, 0, 0, d], ExtendMode::Copy (affine/core.rs)
+    N_AFFINE = 15,   // Affine: [a, 0, 0, d], ExtendMode::Copy (affine/core.rs)
+    N_ZOOM = 16,     // Zoom (structural/zoom.rs:339-400)
 };
 
 struct Node {
@@ -122,6 +123,7 @@ struct Node {
     double am[4] = {1.0, 0.0, 0.0, 1.0}, atx = 0.0, aty = 0.0;
     int akern = 0;
     bool asmall = false;
+    uint32_t zx = 1, zy = 1;  // zoom factors
 
     int radius() const { return conv.empty() ? 0 : (int)(conv.size() - 1) / 2; }
     Rect required_input(const Rect& out) const;
@@ -150,6 +152,7 @@ struct PlanStep {
     uint32_t ex = 0, ey = 0, ew = 0, eh = 0;  // N_EXTRACT rectangle; N_AFFINE output size (ew x eh)
     double a = 1.0, d = 1.0, tx = 0.0, ty = 0.0;  // N_AFFINE
     int akern = 0;
+    uint32_t zx = 1, zy = 1;  // N_ZOOM
 };
 // crop: Thumbnail::with_crop (thumbnail.rs:185-190), a centred ExtractArea
 // of the covering geometry.

@@ -400,10 +400,13 @@ void stage_affine_params(const std::vector<Node>& nodes, int i, const std::vecto
         for (size_t k = 0; k < pos.size(); ++k) {
             const double x = pos[k];
             AffineTap& a = t[k];
-            if (nd.akern == BICUBIC) {  // interp_separable<2>: taps floor(x) - 1 .. floor(x) + 2
+            if (nd.akern != NEAREST && nd.akern != BILINEAR) {
+                // interp_separable<S>: taps floor(x) - S + 1 .. floor(x) + S
+                const int S = nd.akern == LANCZOS3 ? 3 : 2;
                 const int64_t c = (int64_t)std::floor(x);
-                a.i0 = (int32_t)(c - 1);
-                for (int q = 0; q < 4; ++q) a.w[q] = kernel_weight(BICUBIC, std::fabs((double)(c - 1 + q) - x));
+                a.i0 = (int32_t)(c - S + 1);
+                for (int q = 0; q < 2 * S; ++q)
+                    a.w[q] = kernel_weight(nd.akern, std::fabs((double)(c - S + 1 + q) - x));
             } else {  // `x as i64` truncates toward zero
                 const int64_t c = (int64_t)x;
                 a.i0 = (int32_t)c;
@@ -953,13 +956,17 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
     case ST_ALPHA:
         e = launch_alpha_u8(in, out, B, nd.kind == N_PREMULT ? ALPHA_PREMULT : ALPHA_UNPREMULT, stream);
         break;
+    case ST_ZOOM: e = launch_zoom(in, out, B, nd.in_fmt, (int)nd.zx, (int)nd.zy, stream); break;
     case ST_AFFINE: {
         AffineJob j;
         j.in = in;
         j.out = out;
         j.bands = B;
         j.fmt = nd.in_fmt;
-        j.kernel = nd.akern;
+        j.kernel = nd.akern == NEAREST    ? AFF_NEAREST
+                   : nd.akern == BILINEAR ? AFF_BILINEAR
+                   : nd.akern == LANCZOS3 ? AFF_SEP6
+                                          : AFF_SEP4;
         j.in_w = (int32_t)nd.in_w;
         j.in_h = (int32_t)nd.in_h;
         j.xs = reinterpret_cast<const AffineTap*>(blob + s.aff_x);
@@ -1131,6 +1138,7 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
             case N_PREMULT:
             case N_UNPREMULT: s.kind = ST_ALPHA; break;
             case N_AFFINE: s.kind = ST_AFFINE; break;
+            case N_ZOOM: s.kind = ST_ZOOM; break;
             default: s.kind = ST_COPY; break;
             }
         }
@@ -1469,12 +1477,13 @@ int vgpu_pipeline_unpremultiply(vgpu_pipeline* p) {
     return VGPU_OK;
 }
 
-// Thumbnail::Affine -> PipelineBuilder::affine_backward_with_extend_and_hint
-// (builder/resample.rs:478-560, 932-955) -> AffineBridge::new_with_extend
+// ThumbnailNode::Affine / ResizeNode::Affine ->
+// PipelineBuilder::affine_backward_with_extend_and_hint (builder/resample.rs:
+// 478-560, 690-706, 932-955) -> AffineBridge::new_with_extend
 // (builder/state.rs:298-341) -> Affine::try_new (affine/interpolation.rs:
 // 13-41): finite coefficients, a non-singular matrix, ExtendMode::Copy over
-// the whole input.  small: the SmallTile demand of a thumbnail of an input no
-// larger than 512 x 512.
+// the whole input.  small: the SmallTile demand (resize, and a thumbnail of
+// an input no larger than 512 x 512).
 static int push_affine(vgpu_pipeline* p, const PlanStep& s, bool small) {
     const double m[4] = {s.a, 0.0, 0.0, s.d};
     for (double v : m)
@@ -1506,6 +1515,15 @@ static int apply_steps(vgpu_pipeline* p, const std::vector<PlanStep>& steps, int
         case N_REDUCE_V: rc = vgpu_pipeline_reduce_v(p, s.param, kernel); break;
         case N_EXTRACT: rc = vgpu_pipeline_extract_area(p, s.ex, s.ey, s.ew, s.eh); break;
         case N_AFFINE: rc = push_affine(p, s, affine_small); break;
+        case N_ZOOM: {  // ResizeNode::Zoom -> PipelineBuilder::zoom (builder/resample.rs:683-689)
+            Node& nd = p->push(N_ZOOM);
+            nd.zx = s.zx;
+            nd.zy = s.zy;
+            nd.out_w = p->cur_w * s.zx;
+            nd.out_h = p->cur_h * s.zy;
+            p->commit();
+            break;
+        }
         default: rc = VGPU_E_UNSUPPORTED; break;
         }
         if (rc != VGPU_OK) return rc;
@@ -1556,7 +1574,20 @@ int vgpu_pipeline_resize(vgpu_pipeline* p, double hscale, double vscale, int ker
     int eff = kernel;
     int rc = plan_resize(p->cur_w, p->cur_h, hscale, vscale, kernel, &steps, &eff);
     if (rc != OK) return rc;
-    return apply_steps(p, steps, eff);
+    const size_t n0 = p->nodes.size();
+    const uint32_t w0 = p->cur_w, h0 = p->cur_h;
+    const int f0 = p->cur_fmt;
+    // builder/resample.rs:690-706: resize's Affine always asks for SmallTile tiles.
+    rc = apply_steps(p, steps, eff, true);
+    if (rc == VGPU_OK && !affine_tail_supported(p->nodes)) rc = VGPU_E_UNSUPPORTED;
+    if (rc != VGPU_OK) {  // leave the pipeline as it was
+        p->release();
+        p->nodes.resize(n0);
+        p->cur_w = w0;
+        p->cur_h = h0;
+        p->cur_fmt = f0;
+    }
+    return rc;
 }
 
 int vgpu_pipeline_output_info(const vgpu_pipeline* p, uint32_t* w, uint32_t* h, uint32_t* b,

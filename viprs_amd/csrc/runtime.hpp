@@ -118,7 +118,8 @@ enum StageKind {
     ST_CONV,
     ST_ALPHA,
     ST_COPY,
-    ST_AFFINE,   // the thumbnail's Affine tail
+    ST_AFFINE,   // the thumbnail / resize Affine tail
+    ST_ZOOM,
 };
 
 struct Stage {
