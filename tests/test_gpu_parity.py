@@ -176,14 +176,6 @@ def test_extract_area_validation(ctx):
         assert e.value.status == v.api.E_INVALID_ARGUMENT
 
 
-def _small_tile_u8_reduce_h(w, h, b, target, tw, th, kernel, crop, dtype):
-    """The one Affine-tail case the device declines: a u8 ReduceH before an
-    Affine that runs on 128 x 128 tiles (thumbnail input up to 512 px)."""
-    nodes, _ = O.thumbnail_plan(w, h, b, target, tw, th, kernel, crop)
-    names = [n for n, _ in nodes]
-    return dtype == np.uint8 and w <= 512 and h <= 512 and "Affine" in names and "ReduceH" in names
-
-
 AFFINE_CASES = [
     # (w, h, bands, dtype, target, tw, th, kernel, crop)
     (1, 2048, 3, np.uint8, v.HEIGHT, 0, 400, v.LANCZOS3, False),     # thumbnail.rs:663-677
@@ -199,6 +191,10 @@ AFFINE_CASES = [
     (300, 200, 3, np.uint8, v.FORCE_EXACT, 500, 500, v.LANCZOS3, True),   # crop of an upscale
     (300, 200, 4, np.uint8, v.FITBOX, 100, 150, v.LANCZOS3, True),
     (3000, 2, 3, np.uint8, v.FORCE_EXACT, 12000, 4, v.BICUBIC, False),  # 1-row strips beyond 10000 px
+    # u8 ReduceH ahead of a SmallTile Affine: one run per 128-column tile column
+    (400, 300, 1, np.uint8, v.FORCE_EXACT, 150, 900, v.LANCZOS3, False),
+    (500, 120, 4, np.uint8, v.FORCE_EXACT, 300, 260, v.BICUBIC, False),
+    (511, 97, 2, np.uint8, v.FORCE_EXACT, 200, 300, v.BILINEAR, False),
 ]
 
 
@@ -221,23 +217,11 @@ def test_random_affine_tails(ctx, seed):
     kernel = [v.LANCZOS3, v.BILINEAR, v.NEAREST, v.BICUBIC][seed % 4]
     crop = seed % 3 == 0
     src = noise_u8(h, w, b, seed)
-    if _small_tile_u8_reduce_h(w, h, b, v.FORCE_EXACT, tw, th, kernel, crop, np.uint8):
-        with pytest.raises(v.VgpuError) as e:
-            v.Pipeline(ctx, w, h, b).thumbnail(v.FORCE_EXACT, tw, th, kernel, crop)
-        assert e.value.status == v.api.E_UNSUPPORTED
-        return
     got, exp = run_both(ctx, src, [("thumbnail", (v.FORCE_EXACT, tw, th, kernel, crop))])
     assert_same(got, exp, f"ForceExact {w}x{h}x{b} -> {tw}x{th} kernel {kernel} crop {crop}")
 
 
 def test_affine_tail_limits(ctx):
-    # a u8 ReduceH ahead of a SmallTile Affine is declined, and the builder unchanged
-    p = v.Pipeline(ctx, 400, 300, 1)
-    assert _small_tile_u8_reduce_h(400, 300, 1, v.FORCE_EXACT, 150, 900, v.LANCZOS3, False, np.uint8)
-    with pytest.raises(v.VgpuError) as e:
-        p.thumbnail(v.FORCE_EXACT, 150, 900)
-    assert e.value.status == v.api.E_UNSUPPORTED
-    assert p.output_info()[:2] == (400, 300)
     # an op after the Affine: declined when the pipeline runs
     p = v.Pipeline(ctx, 1, 700, 3).thumbnail(v.HEIGHT, 0, 300).gauss_blur(1.0)
     src = v.DeviceBuffer.from_numpy(ctx, noise_u8(700, 1, 3, 1))
@@ -649,14 +633,16 @@ def test_resize_upscale(ctx, case):
     assert_same(got, exp, f"resize up {RESIZE_UP[case]}")
 
 
-def test_resize_mixed_u8_declined(ctx):
-    # resize.rs:441-457 shape: a u8 ReduceH before a SmallTile Affine
-    p = v.Pipeline(ctx, 100, 100, 3)
-    with pytest.raises(v.VgpuError) as e:
-        p.resize(0.5, 2.5, v.BILINEAR)
-    assert e.value.status == v.api.E_UNSUPPORTED
-    assert p.output_info()[:2] == (100, 100)
-    # the same plan on u16 runs (its ReduceH positions do not accumulate)
-    src = noise_u16(100, 100, 3, 1)
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_resize_mixed_down_and_up(ctx, dtype):
+    # resize.rs:441-457: ReduceH(2), Zoom(1, 2), Affine on 100x100 -> 50x250;
+    # u8 runs one 128-column tile column at a time
+    src = (noise_u8 if dtype == np.uint8 else noise_u16)(100, 100, 3, 1)
     got, exp = run_both(ctx, src, [("resize", (0.5, 2.5, v.BILINEAR))])
-    assert_same(got, exp, "mixed resize u16")
+    assert got.shape == (250, 50, 3)
+    assert_same(got, exp, f"mixed resize {np.dtype(dtype)}")
+    src = (noise_u8 if dtype == np.uint8 else noise_u16)(300, 450, 4, 2)
+    got, exp = run_both(ctx, src, [("resize", (0.61, 1.9, v.LANCZOS3))])
+    assert_same(got, exp, f"mixed resize 450x300 {np.dtype(dtype)}")
+    got, exp = run_both(ctx, src, [("resize", (0.61, 1.9, v.LANCZOS3))], device=False)
+    assert_same(got, exp, f"mixed resize host {np.dtype(dtype)}")

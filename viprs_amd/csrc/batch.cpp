@@ -171,9 +171,7 @@ int vgpu_batch::plan() {
         im.inter_off.clear();
         im.batched = true;
         if (im.nodes.empty()) continue;
-        std::vector<Rect> in_r, out_r;
-        backward_regions(im.nodes, im.ow, im.oh, &in_r, &out_r);
-        im.stages = plan_stages(im.nodes, in_r, out_r, &st, opt);
+        im.stages = plan_pipeline(im.nodes, im.ow, im.oh, &st, opt);
         for (size_t k = 0; k + 1 < im.stages.size(); ++k) {
             const Stage& s = im.stages[k];
             im.inter_off.push_back(arena_need);
@@ -416,7 +414,11 @@ int run_fallback(vgpu_batch* b, int idx, const void* src, void* dst, hipStream_t
     Rect cur_rect = whole(im.w, im.h);
     for (size_t k = 0; k < im.stages.size(); ++k) {
         const Stage& st = im.stages[k];
-        void* o = (k + 1 == im.stages.size()) ? dst : darena + im.inter_off[k];
+        if (stage_reads_source(im.stages, k)) {
+            cur = src;
+            cur_rect = whole(im.w, im.h);
+        }
+        void* o = stage_writes_dst(im.stages, k) ? dst : darena + im.inter_off[k];
         const int rc = launch_stage(im.nodes, st, view_of(cur, cur_rect, st.clip), o, dblob, s);
         if (rc != VGPU_OK) return rc;
         cur = o;

@@ -956,6 +956,16 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
     case ST_ALPHA:
         e = launch_alpha_u8(in, out, B, nd.kind == N_PREMULT ? ALPHA_PREMULT : ALPHA_UNPREMULT, stream);
         break;
+    case ST_PLACE: {  // dst is the whole destination: rows of place_w pixels
+        const size_t ps = (size_t)B * format_size(s.out_fmt);
+        const size_t row = (size_t)s.out.w * ps;
+        unsigned char* d = static_cast<unsigned char*>(dst) + ((size_t)s.out.y * (size_t)s.place_w + (size_t)s.out.x) * ps;
+        const unsigned char* sp = static_cast<const unsigned char*>(in.ptr) +
+                                  ((size_t)(s.out.y - in.by) * (size_t)in.bw + (size_t)(s.out.x - in.bx)) * ps;
+        e = hipMemcpy2DAsync(d, (size_t)s.place_w * ps, sp, (size_t)in.bw * ps, row, (size_t)s.out.h,
+                             hipMemcpyDeviceToDevice, stream);
+        break;
+    }
     case ST_ZOOM: e = launch_zoom(in, out, B, nd.in_fmt, (int)nd.zx, (int)nd.zy, stream); break;
     case ST_AFFINE: {
         AffineJob j;
@@ -991,16 +1001,12 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
 // Group a pipeline's nodes into fused stages over the regions of the backward
 // walk; every stage's plan data goes into `st`.
 bool affine_tail_supported(const std::vector<Node>& nodes) {
-    bool seen = false, small = false, u8_reduce_h = false;
+    bool seen = false;
     for (const Node& nd : nodes) {
         if (seen && nd.kind != N_EXTRACT && nd.kind != N_UNPREMULT) return false;
-        if (nd.kind == N_AFFINE) {
-            seen = true;
-            small = nd.asmall;
-        }
-        if (nd.kind == N_REDUCE_H && nd.in_fmt == U8) u8_reduce_h = true;
+        if (nd.kind == N_AFFINE) seen = true;
     }
-    return !(small && u8_reduce_h);
+    return true;
 }
 
 std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector<Rect>& in_r,
@@ -1153,17 +1159,55 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
     return stages;
 }
 
-void backward_regions(const std::vector<Node>& nodes, uint32_t w, uint32_t h, std::vector<Rect>* in_r,
-                      std::vector<Rect>* out_r) {
+static void backward_regions_from(const std::vector<Node>& nodes, const Rect& fin, std::vector<Rect>* in_r,
+                                  std::vector<Rect>* out_r) {
     const int n = (int)nodes.size();
     in_r->assign((size_t)n, Rect{});
     out_r->assign((size_t)n, Rect{});
     if (n == 0) return;
-    (*out_r)[(size_t)n - 1] = whole(w, h);
+    (*out_r)[(size_t)n - 1] = fin;
     for (int i = n - 1; i >= 0; --i) {
         (*in_r)[(size_t)i] = nodes[(size_t)i].required_input((*out_r)[(size_t)i]);
         if (i > 0) (*out_r)[(size_t)i - 1] = (*in_r)[(size_t)i];
     }
+}
+
+void backward_regions(const std::vector<Node>& nodes, uint32_t w, uint32_t h, std::vector<Rect>* in_r,
+                      std::vector<Rect>* out_r) {
+    backward_regions_from(nodes, whole(w, h), in_r, out_r);
+}
+
+std::vector<Stage> plan_pipeline(const std::vector<Node>& nodes, uint32_t w, uint32_t h, Staging* st,
+                                 const StageOptions& opt) {
+    bool small = false, u8_reduce_h = false;
+    for (const Node& nd : nodes) {
+        if (nd.kind == N_AFFINE && nd.asmall) small = true;
+        if (nd.kind == N_REDUCE_H && nd.in_fmt == U8) u8_reduce_h = true;
+    }
+    std::vector<Rect> in_r, out_r;
+    if (!(small && u8_reduce_h) || nodes.empty()) {
+        backward_regions(nodes, w, h, &in_r, &out_r);
+        return plan_stages(nodes, in_r, out_r, st, opt);
+    }
+    std::vector<Stage> all;
+    for (int64_t x0 = 0; x0 < (int64_t)w; x0 += 128) {
+        const Rect fin{x0, 0, std::min<int64_t>(128, (int64_t)w - x0), (int64_t)h};
+        backward_regions_from(nodes, fin, &in_r, &out_r);
+        std::vector<Stage> run = plan_stages(nodes, in_r, out_r, st, opt);
+        if (run.empty()) continue;
+        run.front().from_source = true;
+        Stage pl;
+        pl.kind = ST_PLACE;
+        pl.first = pl.last = (int)nodes.size() - 1;
+        pl.clip = fin;
+        pl.out = fin;
+        pl.in_fmt = pl.out_fmt = nodes.back().out_fmt;
+        pl.to_dst = true;
+        pl.place_w = w;
+        run.push_back(pl);
+        all.insert(all.end(), run.begin(), run.end());
+    }
+    return all;
 }
 
 }  // namespace vgpu
@@ -1176,11 +1220,10 @@ int vgpu_pipeline::compile() {
         compiled = true;
         return VGPU_OK;
     }
-    // Backward region walk (execute.rs:413-480) from the full output.
-    std::vector<Rect> out_r, in_r;
-    backward_regions(nodes, cur_w, cur_h, &in_r, &out_r);
+    // Backward region walk (execute.rs:413-480) from the full output (or
+    // from each tile column, see plan_pipeline).
     Staging st;
-    stages = plan_stages(nodes, in_r, out_r, &st, opt);
+    stages = plan_pipeline(nodes, cur_w, cur_h, &st, opt);
     if (!st.bytes.empty()) {
         blob.ctx = ctx;
         blob.ptr = ctx->pool.acquire(st.bytes.size());
@@ -1191,6 +1234,10 @@ int vgpu_pipeline::compile() {
     }
     for (size_t k = 0; k + 1 < stages.size(); ++k) {
         const Stage& s = stages[k];
+        if (stage_writes_dst(stages, k)) {
+            inter.push_back(nullptr);
+            continue;
+        }
         const size_t bytes = (size_t)s.out.w * (size_t)s.out.h * bands * format_size(s.out_fmt);
         void* p = ctx->pool.acquire(bytes);
         if (!p) return VGPU_E_NO_MEMORY;
@@ -1658,6 +1705,9 @@ const char* stage_kernel_name(const Stage& s) {
     case ST_CONV: return "conv";
     case ST_ALPHA: return "alpha";
     case ST_COPY: return "copy";
+    case ST_AFFINE: return "affine";
+    case ST_ZOOM: return "zoom";
+    case ST_PLACE: return "place";
     }
     return "unknown";
 }
@@ -1701,7 +1751,11 @@ int vgpu_pipeline_run(vgpu_pipeline* p, const void* src, size_t src_bytes, void*
     Rect cur_rect = whole(p->src_w, p->src_h);
     for (size_t k = 0; k < p->stages.size(); ++k) {
         const Stage& s = p->stages[k];
-        void* o = (k + 1 == p->stages.size()) ? dst : p->inter[k];
+        if (stage_reads_source(p->stages, k)) {
+            cur = src;
+            cur_rect = whole(p->src_w, p->src_h);
+        }
+        void* o = stage_writes_dst(p->stages, k) ? dst : p->inter[k];
         rc = launch_stage(p->nodes, s, view_of(cur, cur_rect, s.clip), o, blob, ctx->stream, p->opt);
         if (rc != VGPU_OK) {
             ctx->sticky = rc;
@@ -1737,7 +1791,11 @@ int vgpu_pipeline_run_profiled(vgpu_pipeline* p, const void* src, size_t src_byt
         (void)hipEventRecord(ev[0], ctx->stream);
         for (int k = 0; k < ns && rc == VGPU_OK; ++k) {
             const Stage& s = p->stages[(size_t)k];
-            void* o = (k + 1 == ns) ? dst : p->inter[(size_t)k];
+            if (stage_reads_source(p->stages, (size_t)k)) {
+                cur = src;
+                cur_rect = whole(p->src_w, p->src_h);
+            }
+            void* o = stage_writes_dst(p->stages, (size_t)k) ? dst : p->inter[(size_t)k];
             rc = launch_stage(p->nodes, s, view_of(cur, cur_rect, s.clip), o, blob, ctx->stream, p->opt);
             (void)hipEventRecord(ev[(size_t)k + 1], ctx->stream);
             cur = o;
@@ -1767,12 +1825,12 @@ int vgpu_pipeline_time_stage(vgpu_pipeline* p, const void* src, size_t src_bytes
     // Input view of the stage: the source or the previous stage's output.
     const void* cur = src;
     Rect cur_rect = whole(p->src_w, p->src_h);
-    for (int k = 0; k < stage; ++k) {
-        cur = p->inter[(size_t)k];
-        cur_rect = p->stages[(size_t)k].out;
+    if (!stage_reads_source(p->stages, (size_t)stage)) {
+        cur = p->inter[(size_t)stage - 1];
+        cur_rect = p->stages[(size_t)stage - 1].out;
     }
     const Stage& s = p->stages[(size_t)stage];
-    void* o = (stage + 1 == ns) ? dst : p->inter[(size_t)stage];
+    void* o = stage_writes_dst(p->stages, (size_t)stage) ? dst : p->inter[(size_t)stage];
     const unsigned char* blob = static_cast<const unsigned char*>(p->blob.ptr);
     const InView in = view_of(cur, cur_rect, s.clip);
     hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -120,6 +120,7 @@ enum StageKind {
     ST_COPY,
     ST_AFFINE,   // the thumbnail / resize Affine tail
     ST_ZOOM,
+    ST_PLACE,    // copy a tile-column run's output into its columns of the destination
 };
 
 struct Stage {
@@ -143,6 +144,11 @@ struct Stage {
     size_t v_tab = 0, v_tile = 0, h_tab = 0, h_strip = 0;
     size_t w32 = 0, v_w32 = 0;  // ST_RED_F32: f32 weight tables
     size_t aff_x = 0, aff_y = 0;  // ST_AFFINE: per-column / per-row AffineTap tables
+    // Tile-column runs: the stage reads the pipeline's source instead of the
+    // previous stage's output / writes the destination (ST_PLACE, whose
+    // destination rows are place_w pixels wide).
+    bool from_source = false, to_dst = false;
+    int64_t place_w = 0;
     F16Job f16;                 // ST_SEP_F16: job template (table offsets below)
     GaussJob gauss;             // ST_GAUSS_U8: job template (views set at launch)
 };
@@ -169,11 +175,24 @@ InView view_of(const void* ptr, const Rect& buf, const Rect& clip);
 OutView outview_of(void* ptr, const Rect& r);
 void backward_regions(const std::vector<Node>& nodes, uint32_t w, uint32_t h, std::vector<Rect>* in_r,
                       std::vector<Rect>* out_r);
-// Whether the device reproduces a chain holding the thumbnail's Affine: the
-// nodes after it may only move regions (ExtractArea, Unpremultiply), and an
-// Affine on SmallTile (128 x 128) tiles may not follow a u8 ReduceH, whose
-// fixed-point positions accumulate from each tile's origin.
+// Whether the device reproduces a chain holding an Affine tail: the nodes
+// after it may only move regions (ExtractArea, Unpremultiply).
 bool affine_tail_supported(const std::vector<Node>& nodes);
+// Stages of the whole pipeline (w x h output).  When an Affine runs on
+// SmallTile (128 x 128) tiles after a u8 ReduceH, whose fixed-point source
+// positions accumulate from each tile's left edge, every 128-column tile
+// column is planned as its own run from the source (its stages computing
+// exactly the regions the reference's tiles of that column compute), closed
+// by an ST_PLACE stage.
+std::vector<Stage> plan_pipeline(const std::vector<Node>& nodes, uint32_t w, uint32_t h, Staging* st,
+                                 const StageOptions& opt);
+// The input of stage k: the previous stage's output, or the source.
+inline bool stage_reads_source(const std::vector<Stage>& stages, size_t k) {
+    return k == 0 || stages[k].from_source;
+}
+inline bool stage_writes_dst(const std::vector<Stage>& stages, size_t k) {
+    return k + 1 == stages.size() || stages[k].to_dst;
+}
 std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector<Rect>& in_r,
                                const std::vector<Rect>& out_r, Staging* st,
                                const StageOptions& opt = StageOptions());

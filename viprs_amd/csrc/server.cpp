@@ -224,9 +224,7 @@ void vgpu_server::plan_job(Job& j) {
     j.out_bytes = (size_t)j.ow * j.oh * j.ob * format_size(j.ofmt);
     Staging st;
     if (!p->nodes.empty() && j.out_bytes) {
-        std::vector<Rect> out_r, in_r;
-        backward_regions(p->nodes, p->cur_w, p->cur_h, &in_r, &out_r);
-        j.stages = plan_stages(p->nodes, in_r, out_r, &st, p->opt);
+        j.stages = plan_pipeline(p->nodes, p->cur_w, p->cur_h, &st, p->opt);
         for (size_t k = 0; k + 1 < j.stages.size(); ++k) {
             const Stage& s = j.stages[k];
             j.inter_bytes.push_back((size_t)s.out.w * (size_t)s.out.h * r.bands * format_size(s.out_fmt));
@@ -328,7 +326,11 @@ bool vgpu_server::launch_job(Job& j) {
             Rect cur_rect = whole(j.req.width, j.req.height);
             for (size_t k = 0; k < j.stages.size(); ++k) {
                 const Stage& st = j.stages[k];
-                void* dst = (k + 1 == j.stages.size()) ? out : inter[k];
+                if (stage_reads_source(j.stages, k)) {
+                    cur = src;
+                    cur_rect = whole(j.req.width, j.req.height);
+                }
+                void* dst = stage_writes_dst(j.stages, k) ? out : inter[k];
                 if (launch_stage(j.pipe->nodes, st, view_of(cur, cur_rect, st.clip), dst, blob, s, j.pipe->opt) !=
                     VGPU_OK)
                     return false;
