@@ -391,6 +391,14 @@ def main():
         result["reduce_op"] = bench_line(
             v, O, ctx, rs, lambda p: p.reduce(3.7, 3.7, v.LANCZOS3), args.steps, args.warmup,
             "reduce(3.7, 3.7) Lanczos3 (fused f64 ReduceOp) on 1024x768x3 uchar -> 277x208")
+        print("bench: secondary upscales", file=sys.stderr, flush=True)
+        result["resize_up"] = bench_line(
+            v, O, ctx, rs, lambda p: p.resize(2.5, 2.5, v.LANCZOS3), args.steps, args.warmup,
+            "resize(2.5, 2.5) Lanczos3 on 1024x768x3 uchar -> 2560x1920 (Zoom 2 + Affine, 6x6 f64 taps)")
+        ra = np.random.default_rng(32).integers(0, 256, size=(768, 1024, 4), dtype=np.uint8)
+        result["thumbnail_up"] = bench_line(
+            v, O, ctx, ra, lambda p: p.thumbnail(v.FORCE_EXACT, 1600, 1200, v.LANCZOS3), args.steps, args.warmup,
+            "thumbnail ForceExact(1600, 1200) Lanczos3 on 1024x768x4 uchar (premultiplied Bicubic Affine tail)")
 
     if not args.no_batch and args.batch_images > 0:
         print("bench: uneven-size batch", file=sys.stderr, flush=True)

@@ -22,73 +22,143 @@
 namespace vgpu {
 namespace {
 
-template <typename T, int K, bool ALPHA>
+// NB = band count (1 .. 4), or 0 for any count (read from the job).  The
+// weight product wx * wy and a tap's alpha / alpha_max are computed once per
+// tap and shared by the bands: the same values the reference recomputes per
+// band, so every band's chain of operations is unchanged.
+template <typename T, int K, bool ALPHA, int NB>
 __global__ __launch_bounds__(kThreads) void k_affine(AffineJob j) {
     const InView& in = j.in;
     const OutView& out = j.out;
-    const int B = ALPHA ? 4 : j.bands;
+    constexpr int CAP = NB ? NB : 1;
+    const int B = NB ? NB : j.bands;
     const double amax = sizeof(T) == 1 ? 255.0 : (sizeof(T) == 2 ? 65535.0 : 1.0);
+    const T* src = static_cast<const T*>(in.ptr);
     const size_t n = (size_t)out.w * out.h;
     for (size_t px = (size_t)blockIdx.x * kThreads + threadIdx.x; px < n; px += (size_t)gridDim.x * kThreads) {
         const int xo = (int)(px % out.w), yo = (int)(px / out.w);
         const AffineTap tx = j.xs[xo], ty = j.ys[yo];
-        auto sample = [&](int ix, int iy, int b) -> double {
-            const int cx = clampi(ix, 0, j.in_w - 1), cy = clampi(iy, 0, j.in_h - 1);
-            return (double)load_at<T>(in, row_of(in, cy), col_of(in, cx), B, b);
-        };
-        // sample_band_premultiplied (affine/core.rs:463-482) when ALPHA
-        auto pm = [&](int ix, int iy, int b) -> double {
-            const double v = sample(ix, iy, b);
-            if (!ALPHA || b == 3) return v;
-            return v * (sample(ix, iy, 3) / amax);
-        };
-        auto band = [&](int b) -> double {
+        // Copy extend: clamp to the input image, then (through a folded Zoom,
+        // which repeats pixel (x / zx, y / zy)) to the buffered rectangle.
+        auto col = [&](int ix) { return col_of(in, clampi(ix, 0, j.in_w - 1) / j.zx); };
+        auto row = [&](int iy) { return (size_t)row_of(in, clampi(iy, 0, j.in_h - 1) / j.zy) * (size_t)in.bw; };
+        T* o = static_cast<T*>(out.ptr) + px * B;
+        double v[CAP];
+        if constexpr (NB == 0) {
+            // any band count: one band at a time, as sample_at does
+            for (int b = 0; b < B; ++b) {
+                double r;
+                if constexpr (K == AFF_NEAREST) {
+                    r = (double)src[(row(ty.i0) + col(tx.i0)) * B + b];
+                } else if constexpr (K == AFF_BILINEAR) {
+                    const size_t r0 = row(ty.i0), r1 = row(ty.i0 + 1);
+                    const int c0 = col(tx.i0), c1 = col(tx.i0 + 1);
+                    const double fx = tx.f, fy = ty.f;
+                    const double top = fma((double)src[(r0 + c1) * B + b], fx, (double)src[(r0 + c0) * B + b] * (1.0 - fx));
+                    const double bot = fma((double)src[(r1 + c1) * B + b], fx, (double)src[(r1 + c0) * B + b] * (1.0 - fx));
+                    r = top * (1.0 - fy) + bot * fy;
+                } else {
+                    constexpr int NT = K == AFF_SEP6 ? 6 : 4;
+                    double acc = 0.0, wsum = 0.0;
+                    for (int ky = 0; ky < NT; ++ky) {
+                        const double wy = ty.w[ky];
+                        if (wy == 0.0) continue;
+                        const size_t rr = row(ty.i0 + ky);
+                        for (int kx = 0; kx < NT; ++kx) {
+                            const double wx = tx.w[kx];
+                            if (wx == 0.0) continue;
+                            const double w = wx * wy;
+                            wsum += w;
+                            acc = fma((double)src[(rr + col(tx.i0 + kx)) * B + b], w, acc);
+                        }
+                    }
+                    r = wsum > 0.0 ? acc / wsum : 0.0;
+                }
+                o[b] = op_from_f64<T>(r);
+            }
+            continue;
+        } else {
+            // The premultiplied samples of one pixel (sample_band_premultiplied).
+            auto load = [&](size_t r, int c, double* p) {
+                const T* q = src + (r + c) * NB;
+                if constexpr (ALPHA) {
+                    const double a = (double)q[3];
+                    const double f = a / amax;
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) p[b] = (double)q[b] * f;
+                    p[3] = a;
+                } else {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) p[b] = (double)q[b];
+                }
+            };
             if constexpr (K == AFF_NEAREST) {
-                return pm(tx.i0, ty.i0, b);
+                load(row(ty.i0), col(tx.i0), v);
             } else if constexpr (K == AFF_BILINEAR) {
+                const size_t r0 = row(ty.i0), r1 = row(ty.i0 + 1);
+                const int c0 = col(tx.i0), c1 = col(tx.i0 + 1);
+                double p00[NB], p01[NB], p10[NB], p11[NB];
+                load(r0, c0, p00);
+                load(r0, c1, p01);
+                load(r1, c0, p10);
+                load(r1, c1, p11);
                 const double fx = tx.f, fy = ty.f;
-                const double top = fma(pm(tx.i0 + 1, ty.i0, b), fx, pm(tx.i0, ty.i0, b) * (1.0 - fx));
-                const double bottom =
-                    fma(pm(tx.i0 + 1, ty.i0 + 1, b), fx, pm(tx.i0, ty.i0 + 1, b) * (1.0 - fx));
-                return top * (1.0 - fy) + bottom * fy;
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    const double top = fma(p01[b], fx, p00[b] * (1.0 - fx));
+                    const double bot = fma(p11[b], fx, p10[b] * (1.0 - fx));
+                    v[b] = top * (1.0 - fy) + bot * fy;
+                }
             } else {
                 constexpr int NT = K == AFF_SEP6 ? 6 : 4;
-                double acc = 0.0, wsum = 0.0;
+                int cols[NT];
+#pragma unroll
+                for (int kx = 0; kx < NT; ++kx) cols[kx] = col(tx.i0 + kx);
+                double acc[NB], wsum = 0.0;
+#pragma unroll
+                for (int b = 0; b < NB; ++b) acc[b] = 0.0;
                 for (int ky = 0; ky < NT; ++ky) {
                     const double wy = ty.w[ky];
                     if (wy == 0.0) continue;
+                    const size_t rr = row(ty.i0 + ky);
+#pragma unroll
                     for (int kx = 0; kx < NT; ++kx) {
                         const double wx = tx.w[kx];
                         if (wx == 0.0) continue;
                         const double w = wx * wy;
                         wsum += w;
-                        acc = fma(pm(tx.i0 + kx, ty.i0 + ky, b), w, acc);
+                        double p[NB];
+                        load(rr, cols[kx], p);
+#pragma unroll
+                        for (int b = 0; b < NB; ++b) acc[b] = fma(p[b], w, acc[b]);
                     }
                 }
-                return wsum > 0.0 ? acc / wsum : 0.0;
+#pragma unroll
+                for (int b = 0; b < NB; ++b) v[b] = wsum > 0.0 ? acc[b] / wsum : 0.0;
             }
-        };
-        T* o = static_cast<T*>(out.ptr) + px * B;
-        if constexpr (ALPHA) {
-            const double alpha = fmin(fmax(band(3), 0.0), amax);
-            const double na = alpha / amax;
-            for (int b = 0; b < 3; ++b) {
-                const double v = band(b);
-                o[b] = op_from_f64<T>(na > 0.0 ? v / na : 0.0);
+            if constexpr (ALPHA) {
+                const double alpha = fmin(fmax(v[3], 0.0), amax);
+                const double na = alpha / amax;
+#pragma unroll
+                for (int b = 0; b < 3; ++b) o[b] = op_from_f64<T>(na > 0.0 ? v[b] / na : 0.0);
+                o[3] = op_from_f64<T>(alpha);
+            } else {
+#pragma unroll
+                for (int b = 0; b < NB; ++b) o[b] = op_from_f64<T>(v[b]);
             }
-            o[3] = op_from_f64<T>(alpha);
-        } else {
-            for (int b = 0; b < B; ++b) o[b] = op_from_f64<T>(band(b));
         }
     }
 }
 
 template <typename T, int K>
 void launch_affine_t(const AffineJob& j, unsigned g, hipStream_t s) {
-    if (j.bands == 4)
-        hipLaunchKernelGGL((k_affine<T, K, true>), dim3(g), dim3(kThreads), 0, s, j);
-    else
-        hipLaunchKernelGGL((k_affine<T, K, false>), dim3(g), dim3(kThreads), 0, s, j);
+    switch (j.bands) {
+    case 1: hipLaunchKernelGGL((k_affine<T, K, false, 1>), dim3(g), dim3(kThreads), 0, s, j); break;
+    case 2: hipLaunchKernelGGL((k_affine<T, K, false, 2>), dim3(g), dim3(kThreads), 0, s, j); break;
+    case 3: hipLaunchKernelGGL((k_affine<T, K, false, 3>), dim3(g), dim3(kThreads), 0, s, j); break;
+    case 4: hipLaunchKernelGGL((k_affine<T, K, true, 4>), dim3(g), dim3(kThreads), 0, s, j); break;
+    default: hipLaunchKernelGGL((k_affine<T, K, false, 0>), dim3(g), dim3(kThreads), 0, s, j); break;
+    }
 }
 
 template <typename T>
@@ -129,7 +199,7 @@ hipError_t launch_zoom(const InView& in, const OutView& out, int bands, int fmt,
 
 hipError_t launch_affine(const AffineJob& j, hipStream_t s) {
     if (j.out.w <= 0 || j.out.h <= 0) return hipSuccess;
-    if (j.in_w <= 0 || j.in_h <= 0 || !j.xs || !j.ys) return hipErrorInvalidValue;
+    if (j.in_w <= 0 || j.in_h <= 0 || j.zx < 1 || j.zy < 1 || !j.xs || !j.ys) return hipErrorInvalidValue;
     const size_t n = (size_t)j.out.w * j.out.h;
     const unsigned g = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 65536);
     if (j.fmt == 0) return launch_affine_k<uint8_t>(j, g, s);

@@ -58,6 +58,7 @@ struct AffineJob {
     OutView out;
     int bands = 3, fmt = 0, kernel = 2;
     int32_t in_w = 0, in_h = 0;  // the Affine's input image: ExtendMode::Copy clamps to it
+    int32_t zx = 1, zy = 1;      // a Zoom folded in: `in` holds the image before it
     const AffineTap* xs = nullptr;  // out.w entries
     const AffineTap* ys = nullptr;  // out.h entries
 };

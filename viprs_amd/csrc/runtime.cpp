@@ -968,17 +968,22 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
     }
     case ST_ZOOM: e = launch_zoom(in, out, B, nd.in_fmt, (int)nd.zx, (int)nd.zy, stream); break;
     case ST_AFFINE: {
+        const Node& an = nodes[(size_t)s.last];  // nd is a folded Zoom when first != last
         AffineJob j;
         j.in = in;
         j.out = out;
         j.bands = B;
-        j.fmt = nd.in_fmt;
-        j.kernel = nd.akern == NEAREST    ? AFF_NEAREST
-                   : nd.akern == BILINEAR ? AFF_BILINEAR
-                   : nd.akern == LANCZOS3 ? AFF_SEP6
+        j.fmt = an.in_fmt;
+        if (s.first != s.last) {
+            j.zx = (int32_t)nd.zx;
+            j.zy = (int32_t)nd.zy;
+        }
+        j.kernel = an.akern == NEAREST    ? AFF_NEAREST
+                   : an.akern == BILINEAR ? AFF_BILINEAR
+                   : an.akern == LANCZOS3 ? AFF_SEP6
                                           : AFF_SEP4;
-        j.in_w = (int32_t)nd.in_w;
-        j.in_h = (int32_t)nd.in_h;
+        j.in_w = (int32_t)an.in_w;
+        j.in_h = (int32_t)an.in_h;
         j.xs = reinterpret_cast<const AffineTap*>(blob + s.aff_x);
         j.ys = reinterpret_cast<const AffineTap*>(blob + s.aff_y);
         e = launch_affine(j, stream);
@@ -1122,6 +1127,12 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
                 done = true;
             }
         }
+        if (!done && nd.kind == N_ZOOM && kind_at(i + 1) == N_AFFINE) {
+            // Zoom folded into the Affine's sample reads
+            s.kind = ST_AFFINE;
+            s.last = i + 1;
+            done = true;
+        }
         if (!done) {
             s.last = i;
             switch (nd.kind) {
@@ -1150,7 +1161,7 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
         }
         s.out = out_r[(size_t)s.last];
         s.out_fmt = nodes[(size_t)s.last].out_fmt;
-        if (s.kind == ST_AFFINE) stage_affine_params(nodes, i, out_r, st, &s);
+        if (s.kind == ST_AFFINE) stage_affine_params(nodes, s.last, out_r, st, &s);
         if (s.kind == ST_REDUCE_H || s.kind == ST_REDUCE_V) stage_reduce_params(nd, s.out, st, &s);
         if (s.kind == ST_CONV) stage_conv_params(nd, st, &s);
         stages.push_back(s);
