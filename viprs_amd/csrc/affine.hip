@@ -78,18 +78,31 @@ __global__ __launch_bounds__(kThreads) void k_affine(AffineJob j) {
             }
             continue;
         } else {
-            // The premultiplied samples of one pixel (sample_band_premultiplied).
+            // The samples of one pixel: a folded Premultiply node's output
+            // (premultiply.rs:103-115), then the Affine's own premultiplied
+            // sampling (sample_band_premultiplied) on four bands.
             auto load = [&](size_t r, int c, double* p) {
                 const T* q = src + (r + c) * NB;
-                if constexpr (ALPHA) {
-                    const double a = (double)q[3];
-                    const double f = a / amax;
+                uint32_t u[NB];
 #pragma unroll
-                    for (int b = 0; b < 3; ++b) p[b] = (double)q[b] * f;
-                    p[3] = a;
-                } else {
+                for (int b = 0; b < NB; ++b) u[b] = (uint32_t)q[b];
+                if constexpr (sizeof(T) == 1 && (NB == 2 || NB == 4)) {
+                    if (j.premult) {
+#pragma unroll
+                        for (int b = 0; b + 1 < NB; ++b) u[b] = premult_u8(u[b], u[NB - 1]);
+                    }
+                }
+                if constexpr (sizeof(T) == 4) {
 #pragma unroll
                     for (int b = 0; b < NB; ++b) p[b] = (double)q[b];
+                } else {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) p[b] = (double)u[b];
+                }
+                if constexpr (ALPHA) {
+                    const double f = p[3] / amax;
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) p[b] = p[b] * f;
                 }
             };
             if constexpr (K == AFF_NEAREST) {
@@ -136,16 +149,25 @@ __global__ __launch_bounds__(kThreads) void k_affine(AffineJob j) {
 #pragma unroll
                 for (int b = 0; b < NB; ++b) v[b] = wsum > 0.0 ? acc[b] / wsum : 0.0;
             }
+            T r[NB];
             if constexpr (ALPHA) {
                 const double alpha = fmin(fmax(v[3], 0.0), amax);
                 const double na = alpha / amax;
 #pragma unroll
-                for (int b = 0; b < 3; ++b) o[b] = op_from_f64<T>(na > 0.0 ? v[b] / na : 0.0);
-                o[3] = op_from_f64<T>(alpha);
+                for (int b = 0; b < 3; ++b) r[b] = op_from_f64<T>(na > 0.0 ? v[b] / na : 0.0);
+                r[3] = op_from_f64<T>(alpha);
             } else {
 #pragma unroll
-                for (int b = 0; b < NB; ++b) o[b] = op_from_f64<T>(v[b]);
+                for (int b = 0; b < NB; ++b) r[b] = op_from_f64<T>(v[b]);
             }
+            if constexpr (sizeof(T) == 1 && (NB == 2 || NB == 4)) {
+                if (j.unpremult) {  // unpremultiply.rs:96-109
+#pragma unroll
+                    for (int b = 0; b + 1 < NB; ++b) r[b] = (T)unpremult_u8(r[b], r[NB - 1]);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) o[b] = r[b];
         }
     }
 }

@@ -59,6 +59,9 @@ struct AffineJob {
     int bands = 3, fmt = 0, kernel = 2;
     int32_t in_w = 0, in_h = 0;  // the Affine's input image: ExtendMode::Copy clamps to it
     int32_t zx = 1, zy = 1;      // a Zoom folded in: `in` holds the image before it
+    // u8 with 2 or 4 bands: a Premultiply before the Affine (and its Zoom)
+    // and an Unpremultiply after it folded into the sample reads and stores
+    int premult = 0, unpremult = 0;
     const AffineTap* xs = nullptr;  // out.w entries
     const AffineTap* ys = nullptr;  // out.h entries
 };

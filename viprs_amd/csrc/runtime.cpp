@@ -968,16 +968,18 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
     }
     case ST_ZOOM: e = launch_zoom(in, out, B, nd.in_fmt, (int)nd.zx, (int)nd.zy, stream); break;
     case ST_AFFINE: {
-        const Node& an = nodes[(size_t)s.last];  // nd is a folded Zoom when first != last
+        const Node& an = nodes[(size_t)s.aff_node];
         AffineJob j;
         j.in = in;
         j.out = out;
         j.bands = B;
         j.fmt = an.in_fmt;
-        if (s.first != s.last) {
-            j.zx = (int32_t)nd.zx;
-            j.zy = (int32_t)nd.zy;
+        if (s.aff_node > s.first && nodes[(size_t)s.aff_node - 1].kind == N_ZOOM) {
+            j.zx = (int32_t)nodes[(size_t)s.aff_node - 1].zx;
+            j.zy = (int32_t)nodes[(size_t)s.aff_node - 1].zy;
         }
+        j.premult = s.premult;
+        j.unpremult = s.unpremult;
         j.kernel = an.akern == NEAREST    ? AFF_NEAREST
                    : an.akern == BILINEAR ? AFF_BILINEAR
                    : an.akern == LANCZOS3 ? AFF_SEP6
@@ -1127,11 +1129,25 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
                 done = true;
             }
         }
-        if (!done && nd.kind == N_ZOOM && kind_at(i + 1) == N_AFFINE) {
-            // Zoom folded into the Affine's sample reads
-            s.kind = ST_AFFINE;
-            s.last = i + 1;
-            done = true;
+        if (!done) {
+            // [Premultiply] -> [Zoom] -> Affine -> [Unpremultiply]: the Zoom
+            // folds into the Affine's sample reads, the alpha steps (u8, 2 or 4
+            // bands) into its loads and stores.
+            int j = i;
+            const bool pm = alpha_ok && nd.kind == N_PREMULT;
+            if (pm) ++j;
+            if (kind_at(j) == N_ZOOM) ++j;
+            if (kind_at(j) == N_AFFINE) {
+                s.kind = ST_AFFINE;
+                s.premult = pm ? 1 : 0;
+                s.aff_node = j;
+                s.last = j;
+                if (alpha_ok && kind_at(j + 1) == N_UNPREMULT) {
+                    s.unpremult = 1;
+                    s.last = j + 1;
+                }
+                done = true;
+            }
         }
         if (!done) {
             s.last = i;
@@ -1161,7 +1177,7 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
         }
         s.out = out_r[(size_t)s.last];
         s.out_fmt = nodes[(size_t)s.last].out_fmt;
-        if (s.kind == ST_AFFINE) stage_affine_params(nodes, s.last, out_r, st, &s);
+        if (s.kind == ST_AFFINE) stage_affine_params(nodes, s.aff_node, out_r, st, &s);
         if (s.kind == ST_REDUCE_H || s.kind == ST_REDUCE_V) stage_reduce_params(nd, s.out, st, &s);
         if (s.kind == ST_CONV) stage_conv_params(nd, st, &s);
         stages.push_back(s);

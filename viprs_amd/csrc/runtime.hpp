@@ -144,6 +144,7 @@ struct Stage {
     size_t v_tab = 0, v_tile = 0, h_tab = 0, h_strip = 0;
     size_t w32 = 0, v_w32 = 0;  // ST_RED_F32: f32 weight tables
     size_t aff_x = 0, aff_y = 0;  // ST_AFFINE: per-column / per-row AffineTap tables
+    int aff_node = -1;            // ST_AFFINE: index of the Affine node
     // Tile-column runs: the stage reads the pipeline's source instead of the
     // previous stage's output / writes the destination (ST_PLACE, whose
     // destination rows are place_w pixels wide).
