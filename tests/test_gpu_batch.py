@@ -189,3 +189,22 @@ def test_batch_uneven_sizes_fitbox_400(ctx):
         host = splitmix_bytes(w * h * b, 9000 + i).reshape(h, w, b)
         exp = O.Pipeline(w, h, b).thumbnail(O.FITBOX, 400, 400).run(host)
         assert np.array_equal(dst[i].numpy(), exp), (i, sizes[i])
+
+
+def test_batch_affine_tails_run_stage_by_stage(ctx):
+    # ForceExact plans with an Affine tail (upscales, 1-pixel inputs, and u8
+    # ReduceH ahead of a SmallTile Affine, planned per 128-column tile column)
+    # leave the batched launches and run stage by stage, bit-exact.
+    sizes = [(300, 200, 3), (400, 300, 1), (500, 120, 4), (1, 700, 3), (900, 40, 2), (2000, 1500, 3)]
+    srcs = sources(sizes, 21)
+    exp = expected(srcs, v.FORCE_EXACT, 333, 555)
+    batch = v.Batch(ctx, sizes, v.FORCE_EXACT, 333, 555)
+    tails = sum(1 for (w, h, b) in sizes
+                if "Affine" in [n for n, _ in O.thumbnail_plan(w, h, b, O.FORCE_EXACT, 333, 555)[0]])
+    assert tails == 5 and batch.info()["fallback_images"] == tails
+    dsrc = [v.DeviceBuffer.from_numpy(ctx, s) for s in srcs]
+    dst = batch.output_buffers()
+    for rep in range(2):
+        batch.run(dsrc, dst)
+        ctx.synchronize()
+        check_outputs(dst, exp, f"affine tails, run {rep}")
