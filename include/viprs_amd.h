@@ -2,7 +2,8 @@
  * viprs_amd: MI355X (gfx950) executor for viprs's resample and separable
  * convolution path -- ShrinkH/ShrinkV box pre-shrink, ReduceH/ReduceV
  * Lanczos/cubic/linear reducers (thumbnail / resize / reduce_h / reduce_v),
- * gauss_blur and convsep.
+ * gauss_blur and convsep, and the thumbnail / resize tails around them
+ * (Affine, Zoom, ExtractArea).
  *
  * C ABI: plain pointers, sizes and status codes; no exceptions cross it.
  * Conventions follow the reference's existing C ABI (include/viprs.h:7-20,
@@ -59,7 +60,7 @@ typedef enum {
     VGPU_E_INVALID_ARGUMENT = -1,
     VGPU_E_INVALID_KERNEL = -2,
     VGPU_E_UNSUPPORTED_FORMAT = -3,
-    VGPU_E_UNSUPPORTED = -4,  /* plan needs a node outside this path (Affine/Zoom/crop) */
+    VGPU_E_UNSUPPORTED = -4,  /* plan needs a node outside this path (LBB/VSQBS resize, an op after an Affine tail) */
     VGPU_E_BUFFER_SIZE = -5,  /* a buffer length does not match the region */
     VGPU_E_DEVICE = -6,       /* HIP runtime error (sticky on the context) */
     VGPU_E_NO_MEMORY = -7,
@@ -152,8 +153,10 @@ int vgpu_pipeline_convsep(vgpu_pipeline *p, const double *coeffs, uint32_t n, do
 int vgpu_pipeline_premultiply(vgpu_pipeline *p);
 int vgpu_pipeline_unpremultiply(vgpu_pipeline *p);
 /* PipelineBuilder::thumbnail(Thumbnail::new(target, kernel))
- * (builder/resample.rs:871-979; thumbnail.rs:229-524), in-memory source.
- * Returns VGPU_E_UNSUPPORTED when the plan needs the Affine tail. */
+ * (builder/resample.rs:871-979; thumbnail.rs:229-524), in-memory source,
+ * every plan including the Affine tail (affine/core.rs).  Ops added after an
+ * Affine tail other than extract_area / unpremultiply make the pipeline's
+ * run return VGPU_E_UNSUPPORTED. */
 int vgpu_pipeline_thumbnail(vgpu_pipeline *p, int target, uint32_t width, uint32_t height,
                             int kernel);
 /* The same with Thumbnail::with_crop (thumbnail.rs:185-190) when crop != 0:
@@ -162,7 +165,9 @@ int vgpu_pipeline_thumbnail(vgpu_pipeline *p, int target, uint32_t width, uint32
 int vgpu_pipeline_thumbnail_crop(vgpu_pipeline *p, int target, uint32_t width, uint32_t height,
                                  int kernel, int crop);
 /* PipelineBuilder::resize(Resize::new(hscale, vscale, kernel))
- * (builder/resample.rs:637-711; resize.rs:163-284), downscale only. */
+ * (builder/resample.rs:637-711; resize.rs:163-284): shrink and reduce for
+ * downscales, Zoom and the Affine fix-up for upscales.  The LBB and VSQBS
+ * interpolators return VGPU_E_UNSUPPORTED. */
 int vgpu_pipeline_resize(vgpu_pipeline *p, double hscale, double vscale, int kernel);
 
 /* Kernel choice for u8 separable pairs, a bit mask: bit 0 puts ReduceH ->
@@ -280,7 +285,8 @@ int vgpu_op_process_region(vgpu_op *op, const void *input, size_t input_len, voi
  * per plan level and kernel family), captured once into a HIP graph.  No
  * reference counterpart: it replaces `count` separate thumbnail() pipelines
  * run through TileScheduler::run (crates/viprs-ports/src/scheduler.rs:63-98).
- * Status VGPU_E_UNSUPPORTED when any image's plan needs the Affine/crop tail. */
+ * Images whose plan has stages the batched kernels do not take (an Affine
+ * tail, a lone copy) run stage by stage after the batched launches. */
 typedef struct vgpu_batch vgpu_batch;
 
 typedef struct {
