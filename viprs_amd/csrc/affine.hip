@@ -22,6 +22,8 @@
 namespace vgpu {
 namespace {
 
+constexpr int kAffineRows = 16;
+
 // NB = band count (1 .. 4), or 0 for any count (read from the job).  The
 // weight product wx * wy and a tap's alpha / alpha_max are computed once per
 // tap and shared by the bands: the same values the reference recomputes per
@@ -34,10 +36,16 @@ __global__ __launch_bounds__(kThreads) void k_affine(AffineJob j) {
     const int B = NB ? NB : j.bands;
     const double amax = sizeof(T) == 1 ? 255.0 : (sizeof(T) == 2 ? 65535.0 : 1.0);
     const T* src = static_cast<const T*>(in.ptr);
-    const size_t n = (size_t)out.w * out.h;
-    for (size_t px = (size_t)blockIdx.x * kThreads + threadIdx.x; px < n; px += (size_t)gridDim.x * kThreads) {
-        const int xo = (int)(px % out.w), yo = (int)(px / out.w);
-        const AffineTap tx = j.xs[xo], ty = j.ys[yo];
+    // Block = 64 columns x kAffineRows rows: a lane keeps its column's tap
+    // record for all its rows; a row's record is the same for the wavefront.
+    const int xo = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
+    if (xo >= out.w) return;
+    const int wv = (int)(threadIdx.x >> 6);
+    const AffineTap tx = j.xs[xo];
+    for (int yb = (int)blockIdx.y * kAffineRows; yb < out.h; yb += (int)gridDim.y * kAffineRows)
+    for (int yo = yb + wv; yo < min(yb + kAffineRows, out.h); yo += kThreads / 64) {
+        const size_t px = (size_t)yo * out.w + xo;
+        const AffineTap ty = j.ys[yo];
         // Copy extend: clamp to the input image, then (through a folded Zoom,
         // which repeats pixel (x / zx, y / zy)) to the buffered rectangle.
         auto col = [&](int ix) { return col_of(in, clampi(ix, 0, j.in_w - 1) / j.zx); };
@@ -132,18 +140,20 @@ __global__ __launch_bounds__(kThreads) void k_affine(AffineJob j) {
                 for (int b = 0; b < NB; ++b) acc[b] = 0.0;
                 for (int ky = 0; ky < NT; ++ky) {
                     const double wy = ty.w[ky];
-                    if (wy == 0.0) continue;
                     const size_t rr = row(ty.i0 + ky);
+                    // the row's samples first (clamped, always in the
+                    // buffer), then the taps whose weights are non-zero
+                    double p[NT][NB];
+#pragma unroll
+                    for (int kx = 0; kx < NT; ++kx) load(rr, cols[kx], p[kx]);
 #pragma unroll
                     for (int kx = 0; kx < NT; ++kx) {
                         const double wx = tx.w[kx];
-                        if (wx == 0.0) continue;
+                        if (wy == 0.0 || wx == 0.0) continue;
                         const double w = wx * wy;
                         wsum += w;
-                        double p[NB];
-                        load(rr, cols[kx], p);
 #pragma unroll
-                        for (int b = 0; b < NB; ++b) acc[b] = fma(p[b], w, acc[b]);
+                        for (int b = 0; b < NB; ++b) acc[b] = fma(p[kx][b], w, acc[b]);
                     }
                 }
 #pragma unroll
@@ -173,18 +183,18 @@ __global__ __launch_bounds__(kThreads) void k_affine(AffineJob j) {
 }
 
 template <typename T, int K>
-void launch_affine_t(const AffineJob& j, unsigned g, hipStream_t s) {
+void launch_affine_t(const AffineJob& j, dim3 g, hipStream_t s) {
     switch (j.bands) {
-    case 1: hipLaunchKernelGGL((k_affine<T, K, false, 1>), dim3(g), dim3(kThreads), 0, s, j); break;
-    case 2: hipLaunchKernelGGL((k_affine<T, K, false, 2>), dim3(g), dim3(kThreads), 0, s, j); break;
-    case 3: hipLaunchKernelGGL((k_affine<T, K, false, 3>), dim3(g), dim3(kThreads), 0, s, j); break;
-    case 4: hipLaunchKernelGGL((k_affine<T, K, true, 4>), dim3(g), dim3(kThreads), 0, s, j); break;
-    default: hipLaunchKernelGGL((k_affine<T, K, false, 0>), dim3(g), dim3(kThreads), 0, s, j); break;
+    case 1: hipLaunchKernelGGL((k_affine<T, K, false, 1>), g, dim3(kThreads), 0, s, j); break;
+    case 2: hipLaunchKernelGGL((k_affine<T, K, false, 2>), g, dim3(kThreads), 0, s, j); break;
+    case 3: hipLaunchKernelGGL((k_affine<T, K, false, 3>), g, dim3(kThreads), 0, s, j); break;
+    case 4: hipLaunchKernelGGL((k_affine<T, K, true, 4>), g, dim3(kThreads), 0, s, j); break;
+    default: hipLaunchKernelGGL((k_affine<T, K, false, 0>), g, dim3(kThreads), 0, s, j); break;
     }
 }
 
 template <typename T>
-hipError_t launch_affine_k(const AffineJob& j, unsigned g, hipStream_t s) {
+hipError_t launch_affine_k(const AffineJob& j, dim3 g, hipStream_t s) {
     switch (j.kernel) {
     case AFF_NEAREST: launch_affine_t<T, AFF_NEAREST>(j, g, s); break;
     case AFF_BILINEAR: launch_affine_t<T, AFF_BILINEAR>(j, g, s); break;
@@ -222,8 +232,8 @@ hipError_t launch_zoom(const InView& in, const OutView& out, int bands, int fmt,
 hipError_t launch_affine(const AffineJob& j, hipStream_t s) {
     if (j.out.w <= 0 || j.out.h <= 0) return hipSuccess;
     if (j.in_w <= 0 || j.in_h <= 0 || j.zx < 1 || j.zy < 1 || !j.xs || !j.ys) return hipErrorInvalidValue;
-    const size_t n = (size_t)j.out.w * j.out.h;
-    const unsigned g = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 65536);
+    const dim3 g((unsigned)((j.out.w + 63) / 64),
+                 (unsigned)std::min<int64_t>(((int64_t)j.out.h + kAffineRows - 1) / kAffineRows, 65535));
     if (j.fmt == 0) return launch_affine_k<uint8_t>(j, g, s);
     if (j.fmt == 1) return launch_affine_k<uint16_t>(j, g, s);
     return launch_affine_k<float>(j, g, s);
