@@ -164,6 +164,16 @@ int vgpu_pipeline_thumbnail(vgpu_pipeline *p, int target, uint32_t width, uint32
  * of the box's size is taken (thumbnail.rs:246-285, 421-428). */
 int vgpu_pipeline_thumbnail_crop(vgpu_pipeline *p, int target, uint32_t width, uint32_t height,
                                  int kernel, int crop);
+/* Thumbnail::into_pipeline_nodes(...).shrink_factor (thumbnail.rs:326-330,
+ * 514-524): the shrink-on-load factor (2, 4, 16; 0 = none) a decoder source
+ * would be asked for (ImageSource::set_thumbnail_shrink_on_load,
+ * builder/resample.rs:878-906).  A caller that decodes at 1/factor builds
+ * the pipeline over the decoded image and calls vgpu_pipeline_thumbnail on
+ * it, which plans without the hint from those dimensions, as the builder
+ * does.  Host-only (no device work). */
+int vgpu_thumbnail_shrink_on_load_factor(uint32_t width, uint32_t height, uint32_t bands, int target,
+                                         uint32_t target_width, uint32_t target_height, int crop,
+                                         uint32_t *factor);
 /* PipelineBuilder::resize(Resize::new(hscale, vscale, kernel))
  * (builder/resample.rs:637-711; resize.rs:163-284): shrink and reduce for
  * downscales, Zoom and the Affine fix-up for upscales.  The LBB and VSQBS

@@ -809,6 +809,43 @@ int orc_thumbnail_plan_crop(uint32_t iw, uint32_t ih, uint32_t bands, int target
     return n;
 }
 
+/* shrink_on_load_factor (thumbnail.rs:514-524) of the plan's common shrink
+ * min(in / geometry) over the axes (thumbnail.rs:326-330); 0 = None, also for
+ * 1-pixel inputs (thumbnail.rs:286-323) and empty ones. */
+uint32_t orc_shrink_on_load_factor(double common_shrink) {
+    return common_shrink >= 16.0 ? 16 : (common_shrink >= 8.0 ? 4 : (common_shrink >= 4.0 ? 2 : 0));
+}
+
+uint32_t orc_thumbnail_shrink_on_load(uint32_t iw, uint32_t ih, int target, uint32_t tw, uint32_t th, int crop) {
+    if (iw <= 1 || ih <= 1) return 0;
+    int allow_upscale = target == ORC_FORCE_EXACT;
+    uint32_t tgw, tgh;
+    if (crop && (target == ORC_FITBOX || target == ORC_FORCE_EXACT)) {
+        tgw = tw > 1 ? tw : 1;
+        tgh = th > 1 ? th : 1;
+    } else {
+        target_dims(target, tw, th, iw, ih, &tgw, &tgh);
+    }
+    if (!allow_upscale) {
+        if (tgw > iw) tgw = iw;
+        if (tgh > ih) tgh = ih;
+    }
+    if (tgw < 1) tgw = 1;
+    if (tgh < 1) tgh = 1;
+    uint32_t gw = tgw, gh = tgh;
+    if (crop) {
+        double cover = fmax((double)tgw / (double)iw, (double)tgh / (double)ih);
+        double cw = round((double)iw * cover), ch = round((double)ih * cover);
+        gw = sat_u32(fmax(cw, (double)tgw));
+        gh = sat_u32(fmax(ch, (double)tgh));
+        if (!allow_upscale) {
+            if (gw > iw) gw = iw;
+            if (gh > ih) gh = ih;
+        }
+    }
+    return orc_shrink_on_load_factor(fmin((double)iw / (double)gw, (double)ih / (double)gh));
+}
+
 int orc_thumbnail_plan(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint32_t tw,
                        uint32_t th, int kernel, int *types, double *params, int max_nodes,
                        uint32_t *out_w, uint32_t *out_h) {

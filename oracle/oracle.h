@@ -101,6 +101,9 @@ int orc_thumbnail_plan(uint32_t in_w, uint32_t in_h, uint32_t bands, int target,
                        uint32_t *out_w, uint32_t *out_h);
 /* The same with the crop option; extract[4] gets the ExtractArea rectangle,
  * affine[7] the Affine node's (a, d, tx, ty, width, height, kernel). */
+/* The shrink-on-load hint of the thumbnail plan (thumbnail.rs:326-330,514-524). */
+uint32_t orc_shrink_on_load_factor(double common_shrink);
+uint32_t orc_thumbnail_shrink_on_load(uint32_t in_w, uint32_t in_h, int target, uint32_t tw, uint32_t th, int crop);
 int orc_thumbnail_plan_crop(uint32_t in_w, uint32_t in_h, uint32_t bands, int target, uint32_t tw,
                             uint32_t th, int kernel, int crop, int *types, double *params, int max_nodes,
                             uint32_t *out_w, uint32_t *out_h, uint32_t *extract, double *affine);

@@ -66,6 +66,10 @@ def lib() -> C.CDLL:
         L.orc_pipeline_thumbnail_crop.argtypes = [vp, i32, u32, u32, i32, i32]
         L.orc_pipeline_extract_area.argtypes = [vp, u32, u32, u32, u32]
         L.orc_pipeline_affine.argtypes = [vp, dbl, dbl, dbl, dbl, u32, u32, i32, i32]
+        L.orc_shrink_on_load_factor.argtypes = [dbl]
+        L.orc_shrink_on_load_factor.restype = u32
+        L.orc_thumbnail_shrink_on_load.argtypes = [u32, u32, i32, u32, u32, i32]
+        L.orc_thumbnail_shrink_on_load.restype = u32
         L.orc_thumbnail_plan_crop.argtypes = [u32, u32, u32, i32, u32, u32, i32, i32, vp, vp, i32, vp, vp, vp, vp]
         L.orc_pipeline_resize.argtypes = [vp, dbl, dbl, i32]
         L.orc_pipeline_output.argtypes = [vp, vp, vp, vp, vp]

@@ -768,3 +768,33 @@ def test_affine_lanczos_matches_numpy_model(kernel):
             acc += img[iy, ix] * (wx * wy)
             wsum += wx * wy
     _assert_rounds_to(got, acc / wsum, f"kernel {kernel}")
+
+
+# ------------------------------------------------------- shrink-on-load hint
+
+def test_shrink_on_load_reference_cases():
+    lib = O.lib()
+    # thumbnail.rs:577-585: thresholds leave at least 2 for the final resize
+    for common, f in [(3.99, 0), (4.0, 2), (7.99, 2), (8.0, 4), (15.99, 4), (16.0, 16)]:
+        assert lib.orc_shrink_on_load_factor(common) == f
+    # thumbnail.rs:532-536: 4000x3000 -> Width(99) shrinks 16 on load
+    assert lib.orc_thumbnail_shrink_on_load(4000, 3000, O.WIDTH, 99, 0, 0) == 16
+    # thumbnail.rs:553-564: the crop case asks for none
+    assert lib.orc_thumbnail_shrink_on_load(400, 200, O.FITBOX, 120, 120, 1) == 0
+    # thumbnail.rs:664-680: ForceExact at the input size asks for none
+    assert lib.orc_thumbnail_shrink_on_load(1024, 777, O.FORCE_EXACT, 1024, 777, 0) == 0
+    assert lib.orc_thumbnail_shrink_on_load(1, 5000, O.HEIGHT, 0, 10, 0) == 0  # 1-pixel branch
+
+
+def test_library_shrink_on_load_matches_oracle():
+    # host-only planner call of the library (no device work)
+    import viprs_amd as v
+    lib = O.lib()
+    r = np.random.default_rng(77)
+    for i in range(400):
+        w, h = int(r.integers(1, 20000)), int(r.integers(1, 20000))
+        target = int(r.integers(0, 4))
+        tw, th = int(r.integers(1, 3000)), int(r.integers(1, 3000))
+        crop = bool(i % 3 == 0)
+        exp = lib.orc_thumbnail_shrink_on_load(w, h, target, tw, th, int(crop))
+        assert v.thumbnail_shrink_on_load_factor(w, h, 3, target, tw, th, crop) == exp, (w, h, target, tw, th, crop)

@@ -5,5 +5,5 @@ reference's builder and operation API."""
 from .api import (  # noqa: F401
     BICUBIC, BILINEAR, CATMULLROM, F32, FITBOX, FORCE_EXACT, HEIGHT, LANCZOS2, LANCZOS3, LBB,
     NEAREST, NOHALO, U8, U16, VSQBS, WIDTH, Batch, Context, DeviceBuffer, Op, Pipeline, Region,
-    Server, VgpuError,
+    Server, VgpuError, thumbnail_shrink_on_load_factor,
 )

@@ -101,6 +101,18 @@ class Context:
         self.close()
 
 
+def thumbnail_shrink_on_load_factor(width: int, height: int, bands: int, target: int, target_width: int,
+                                    target_height: int = 0, crop: bool = False) -> int:
+    """Thumbnail::into_pipeline_nodes(...).shrink_factor (thumbnail.rs:326-330,
+    514-524): the factor (2, 4 or 16; 0 for none) a decoder would shrink on
+    load.  Host planning only."""
+    f = C.c_uint32()
+    _check(_native.load().vgpu_thumbnail_shrink_on_load_factor(width, height, bands, target, target_width,
+                                                                target_height, int(bool(crop)), C.byref(f)),
+           "thumbnail_shrink_on_load_factor")
+    return f.value
+
+
 def _ptr(t) -> int:
     return t.data_ptr()
 

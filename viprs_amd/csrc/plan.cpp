@@ -372,8 +372,9 @@ static PlanStep thumb_affine(double inv_x, double inv_y, uint32_t ow, uint32_t o
 
 Status plan_thumbnail(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint32_t tw,
                       uint32_t th, std::vector<PlanStep>* steps, uint32_t* out_w,
-                      uint32_t* out_h, bool crop, int kernel) {
+                      uint32_t* out_h, bool crop, int kernel, uint32_t* shrink_on_load) {
     steps->clear();
+    if (shrink_on_load) *shrink_on_load = 0;
     if (bands == 0) return E_INVALID_ARGUMENT;
     if (target < FITBOX || target > HEIGHT) return E_INVALID_ARGUMENT;
     if (iw == 0 || ih == 0) {
@@ -415,6 +416,10 @@ Status plan_thumbnail(uint32_t iw, uint32_t ih, uint32_t bands, int target, uint
             geo.push_back(thumb_affine((double)iw / (double)tgw, (double)ih / (double)tgh, tgw, tgh, kernel));
     } else {
         const double hs = (double)iw / (double)gw, vs = (double)ih / (double)gh;
+        if (shrink_on_load) {  // shrink_on_load_factor (thumbnail.rs:326-330,514-524)
+            const double common = std::min(hs, vs);
+            *shrink_on_load = common >= 16.0 ? 16 : (common >= 8.0 ? 4 : (common >= 4.0 ? 2 : 0));
+        }
         const uint32_t sh = integer_shrink(iw, gw, hs), sv = integer_shrink(ih, gh, vs);
         const double rh = hs > 1.0 ? hs / (double)sh : 1.0;
         const double rv = vs > 1.0 ? vs / (double)sv : 1.0;

@@ -158,7 +158,7 @@ struct PlanStep {
 // of the covering geometry.
 Status plan_thumbnail(uint32_t in_w, uint32_t in_h, uint32_t bands, int target, uint32_t tw,
                       uint32_t th, std::vector<PlanStep>* steps, uint32_t* out_w,
-                      uint32_t* out_h, bool crop = false, int kernel = 4);
+                      uint32_t* out_h, bool crop = false, int kernel = 4, uint32_t* shrink_on_load = nullptr);
 // ResizeOp::into_pipeline_nodes, downscale subset (resize.rs:163-284).
 Status plan_resize(uint32_t in_w, uint32_t in_h, double hscale, double vscale, int kernel,
                    std::vector<PlanStep>* steps, int* eff_kernel);

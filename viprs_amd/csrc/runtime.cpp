@@ -1609,6 +1609,15 @@ static int apply_steps(vgpu_pipeline* p, const std::vector<PlanStep>& steps, int
     return VGPU_OK;
 }
 
+int vgpu_thumbnail_shrink_on_load_factor(uint32_t width, uint32_t height, uint32_t bands, int target, uint32_t tw,
+                                         uint32_t th, int crop, uint32_t* factor) {
+    if (!factor) return VGPU_E_INVALID_ARGUMENT;
+    std::vector<PlanStep> steps;
+    uint32_t ow = 0, oh = 0;
+    const int rc = plan_thumbnail(width, height, bands, target, tw, th, &steps, &ow, &oh, crop != 0, LANCZOS3, factor);
+    return rc == OK ? VGPU_OK : rc;
+}
+
 int vgpu_pipeline_thumbnail(vgpu_pipeline* p, int target, uint32_t w, uint32_t h, int kernel) {
     return vgpu_pipeline_thumbnail_crop(p, target, w, h, kernel, 0);
 }
