@@ -267,6 +267,13 @@ int vgpu_op_gauss_blur_h_new(vgpu_context *ctx, float sigma, uint32_t bands, int
 int vgpu_op_gauss_blur_v_new(vgpu_context *ctx, float sigma, uint32_t bands, int format,
                              vgpu_op **out);
 /* OperationBridge<Premultiply<U8>> / <Unpremultiply<U8>>. */
+/* ConvSepH::with_mask / ConvSepV::with_mask (convolution/convsep.rs:23-52,
+ * 120-149; process_region 53-110, 150-207): acc = fma chain over the mask in
+ * FP64, then (acc / scale + offset) as f32; any input format, F32 output. */
+int vgpu_op_convsep_h_new(vgpu_context *ctx, const double *coeffs, uint32_t n, double scale, double offset,
+                          uint32_t bands, int format, vgpu_op **out);
+int vgpu_op_convsep_v_new(vgpu_context *ctx, const double *coeffs, uint32_t n, double scale, double offset,
+                          uint32_t bands, int format, vgpu_op **out);
 int vgpu_op_premultiply_new(vgpu_context *ctx, uint32_t bands, int format, vgpu_op **out);
 int vgpu_op_unpremultiply_new(vgpu_context *ctx, uint32_t bands, int format, vgpu_op **out);
 void vgpu_op_destroy(vgpu_op *op);

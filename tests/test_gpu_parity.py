@@ -646,3 +646,29 @@ def test_resize_mixed_down_and_up(ctx, dtype):
     assert_same(got, exp, f"mixed resize 450x300 {np.dtype(dtype)}")
     got, exp = run_both(ctx, src, [("resize", (0.61, 1.9, v.LANCZOS3))], device=False)
     assert_same(got, exp, f"mixed resize host {np.dtype(dtype)}")
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.float32])
+def test_op_convsep_pair_matches_pipeline(ctx, dtype):
+    # ConvSepH then ConvSepV as operations over the regions the backward walk
+    # gives them (convsep.rs:53-207), equal to the oracle's builder convsep
+    h, w, b = 45, 61, 3
+    src = noise_u8(h, w, b, 12) if dtype == np.uint8 else float_image(h, w, b, 12)
+    coeffs, scale, offset = [1.0, -2.0, 5.0, -2.0, 1.0], 3.0, 0.25
+    r = len(coeffs) // 2
+    exp = O.Pipeline(w, h, b, FMT[np.dtype(dtype)]).convsep(coeffs, scale, offset).run(src)
+    opH = v.Op.convsep_h(ctx, coeffs, scale, offset, b, FMT[np.dtype(dtype)])
+    opV = v.Op.convsep_v(ctx, coeffs, scale, 0.0, b, v.F32)
+    out_r = v.Region(0, 0, w, h)
+    mid_r = opV.required_input_region(out_r)
+    in_r = opH.required_input_region(mid_r)
+    assert (in_r.x, in_r.y, in_r.width, in_r.height) == (-r, -r, w + 2 * r, h + 2 * r)
+    # the source read of that region clamps to the image (sources/memory.rs:149-235)
+    tile = np.pad(src, ((r, r), (r, r), (0, 0)), mode="edge")
+    d_in = v.DeviceBuffer.from_numpy(ctx, tile)
+    d_mid = v.DeviceBuffer(ctx, (mid_r.height, mid_r.width, b), np.float32)
+    d_out = v.DeviceBuffer(ctx, (h, w, b), np.float32)
+    opH.process_region(d_in, d_mid, in_r, mid_r)
+    opV.process_region(d_mid, d_out, mid_r, out_r)
+    ctx.synchronize()
+    assert_same(d_out.numpy(), exp, f"op convsep {np.dtype(dtype)}")

@@ -143,6 +143,8 @@ def load() -> C.CDLL:
         "vgpu_op_shrink_h_new": (i32, [vp, u32, i32, u32, u32, i32, P(vp)]),
         "vgpu_op_shrink_v_new": (i32, [vp, u32, i32, u32, i32, P(vp)]),
         "vgpu_op_gauss_blur_h_new": (i32, [vp, f32, u32, i32, P(vp)]),
+        "vgpu_op_convsep_h_new": (i32, [vp, P(dbl), u32, dbl, dbl, u32, i32, P(vp)]),
+        "vgpu_op_convsep_v_new": (i32, [vp, P(dbl), u32, dbl, dbl, u32, i32, P(vp)]),
         "vgpu_op_gauss_blur_v_new": (i32, [vp, f32, u32, i32, P(vp)]),
         "vgpu_op_premultiply_new": (i32, [vp, u32, i32, P(vp)]),
         "vgpu_op_unpremultiply_new": (i32, [vp, u32, i32, P(vp)]),

@@ -521,6 +521,18 @@ class Op:
         return cls._make(ctx, "vgpu_op_gauss_blur_v_new", (sigma, bands, fmt), bands, fmt, "GaussBlurV::new")
 
     @classmethod
+    def convsep_h(cls, ctx, coeffs, scale, offset, bands, fmt=F32):
+        arr = (C.c_double * len(coeffs))(*[float(c) for c in coeffs])
+        return cls._make(ctx, "vgpu_op_convsep_h_new", (arr, len(coeffs), scale, offset, bands, fmt), bands, fmt,
+                         "ConvSepH::with_mask")
+
+    @classmethod
+    def convsep_v(cls, ctx, coeffs, scale, offset, bands, fmt=F32):
+        arr = (C.c_double * len(coeffs))(*[float(c) for c in coeffs])
+        return cls._make(ctx, "vgpu_op_convsep_v_new", (arr, len(coeffs), scale, offset, bands, fmt), bands, fmt,
+                         "ConvSepV::with_mask")
+
+    @classmethod
     def premultiply(cls, ctx, bands, fmt=U8):
         return cls._make(ctx, "vgpu_op_premultiply_new", (bands, fmt), bands, fmt, "Premultiply::new")
 

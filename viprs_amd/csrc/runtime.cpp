@@ -2139,6 +2139,34 @@ int vgpu_op_gauss_blur_v_new(vgpu_context* ctx, float sigma, uint32_t bands, int
     return gauss_op(ctx, N_GAUSS_V, sigma, bands, fmt, out);
 }
 
+// ConvSepH::with_mask / ConvSepV::with_mask (convolution/convsep.rs:23-52,
+// 120-149): an odd-length finite mask with its scale and offset; any input
+// format, F32 output.
+static int convsep_op(vgpu_context* ctx, NodeKind k, const double* coeffs, uint32_t n, double scale, double offset,
+                      uint32_t bands, int fmt, vgpu_op** out) {
+    if (bands == 0 || !coeffs || n == 0 || n % 2 == 0) return VGPU_E_INVALID_ARGUMENT;
+    for (uint32_t i = 0; i < n; ++i)
+        if (!std::isfinite(coeffs[i])) return VGPU_E_INVALID_ARGUMENT;
+    if (!std::isfinite(scale) || scale == 0.0 || !std::isfinite(offset)) return VGPU_E_INVALID_ARGUMENT;
+    if (!valid_fmt(fmt)) return VGPU_E_UNSUPPORTED_FORMAT;
+    Node nd = base_node(k, bands, fmt);
+    nd.conv.assign(coeffs, coeffs + n);
+    nd.itaps.assign(n, 0);
+    nd.cscale = scale;
+    nd.coffset = offset;
+    nd.out_fmt = F32;
+    return make_op(ctx, std::move(nd), out);
+}
+
+int vgpu_op_convsep_h_new(vgpu_context* ctx, const double* coeffs, uint32_t n, double scale, double offset,
+                          uint32_t bands, int fmt, vgpu_op** out) {
+    return convsep_op(ctx, N_CONVSEP_H, coeffs, n, scale, offset, bands, fmt, out);
+}
+int vgpu_op_convsep_v_new(vgpu_context* ctx, const double* coeffs, uint32_t n, double scale, double offset,
+                          uint32_t bands, int fmt, vgpu_op** out) {
+    return convsep_op(ctx, N_CONVSEP_V, coeffs, n, scale, offset, bands, fmt, out);
+}
+
 int vgpu_op_premultiply_new(vgpu_context* ctx, uint32_t bands, int fmt, vgpu_op** out) {
     if (bands < 2) return VGPU_E_INVALID_ARGUMENT;
     if (fmt != U8) return VGPU_E_UNSUPPORTED_FORMAT;
@@ -2232,7 +2260,9 @@ int vgpu_op_process_region(vgpu_op* op, const void* input, size_t in_len, void* 
     case N_REDUCE_H: s.kind = ST_REDUCE_H; stage_reduce_params(nd, out_r, &st, &s); break;
     case N_REDUCE_V: s.kind = ST_REDUCE_V; stage_reduce_params(nd, out_r, &st, &s); break;
     case N_GAUSS_H:
-    case N_GAUSS_V: s.kind = ST_CONV; stage_conv_params(nd, &st, &s); break;
+    case N_GAUSS_V:
+    case N_CONVSEP_H:
+    case N_CONVSEP_V: s.kind = ST_CONV; stage_conv_params(nd, &st, &s); break;
     case N_PREMULT:
     case N_UNPREMULT: s.kind = ST_ALPHA; break;
     default: s.kind = ST_COPY; break;
