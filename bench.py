@@ -302,7 +302,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the path shards by image with no collective, see DESIGN.md", file=sys.stderr)
+              "more GPUs would each take whole images (or row bands of one image) with no collective",
+              file=sys.stderr)
         sys.exit(2)
 
     import viprs_amd as v
