@@ -27,27 +27,42 @@
  *   crates/viprs-runtime/src/scheduler/rayon_scheduler/planning.rs:509-601 tiles
  *   crates/viprs-runtime/src/sources/memory.rs:149-235         clamp-to-edge source read
  *
- * Parity pins (tests/test_oracle_golden.py):
+ * Parity pins (tests/test_oracle_golden.py, tests/test_oracle_pins.py):
  *   reduce tables, ReduceH/ReduceV u8  -> libvips fixtures in reduceh_libvips,
  *       reducev_libvips, libvips_parity_resample (test_reduce{h,v}), in-crate
  *       known answers (reduceh.rs:574-610, reducev.rs:510-570), invariants
  *       (f64 phase sums = 1, i16 sums within +-taps of 4096, offset -0.5 at f=2)
+ *   ReduceH/ReduceV u16                -> exact integer closed form (i16 table
+ *       row times clamped samples, (acc + 2048) >> 12)
+ *   ReduceH/ReduceV f32                -> exact rational value of each output's
+ *       weighted sum with a rigorous FP64 error bound: bit-exact wherever that
+ *       interval decides the f32 (order-independent)
  *   ShrinkH/ShrinkV u8                  -> shrinkh/, shrinkv/ fixtures,
  *       shrink_libvips/rgb_factor4, shrinkh.rs:1479-1506,1784-1901 known answers
+ *   ShrinkH/ShrinkV u16 / f32          -> exact closed forms (f64 box sum,
+ *       times 1/f, from_f64_clamped)
  *   Gaussian integer masks              -> gauss_blur.rs:56-65 sigma=1.5 constants,
- *       brute-force 2-D convolution on small images
- *   Gaussian float                      -> gauss_blur/uniform_sigma_1_5.bin (tol 1e-3)
+ *       brute-force 2-D convolution on small images; u8 libvips fixtures (+-1)
+ *   Gaussian float                      -> gauss_blur_libvips/nonuniform_5x3 and
+ *       uniform_field, gauss_blur/fractional_ramp_sigma_1_0 (<= 1 ULP),
+ *       libvips_parity_convolution (1e-6), uniform_sigma_1_5 (1e-3); exact
+ *       rational closed form per pass (f32 and u16 inputs)
+ *   ConvSep                             -> convsep_libvips/gaussian_sigma_1_0
+ *       (<= 1 ULP); exact rational closed form with scale and offset (f32, u8,
+ *       u16 inputs)
+ *   Resize                              -> libvips_parity_resample resize fixtures
+ *       (exact)
  *   Premultiply/unpremultiply           -> unpremultiply.rs:239-248 known answer,
  *       exhaustive closed-form check over all 65536 (colour, alpha) pairs
  *   Thumbnail/resize planners           -> thumbnail.rs:531-682 / resize.rs:392-536
- *       shape tests restated in the test file
+ *       shape tests restated in the test file; thumbnail libvips fixtures
  *   Chain executor                      -> strip-height invariance, single-op
  *       equality with clamp-padded closed forms
- * parity unpinned: the float reduce summation order (reduce_simd.rs:2621-2694,
- *   2824-2888) has no golden vector in the reference; it is restated from the
- *   AVX2 source and only checked against an FP64 closed form within tolerance.
- * parity unpinned: ConvSep (convsep.rs:87-107) has only a vips-gated +-1 ULP
- *   fixture; checked against an FP64 closed form within tolerance.
+ * The float summation ORDER (reduce_simd.rs:2621-2694,2824-2888; FMA or not)
+ * is restated from the AVX2 source; it decides only the outputs whose exact
+ * value lies within a few FP64 ulps of an f32 rounding boundary, which the
+ * closed forms leave open (about 1 in 2 x denominator samples for masks with
+ * small-denominator weights, far fewer otherwise).
  */
 #include "oracle.h"
 

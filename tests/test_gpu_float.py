@@ -59,6 +59,8 @@ def test_float_reduce_against_fp64(ctx, mode, bands):
     x_abs, _ = errors(exact, ref)
     assert e_abs <= 2e-6, (e_abs, x_abs, where(fast, ref))
     assert e_rel <= 1e-6 * 1.6 ** 2 * 2, (e_rel, x_abs)
+    # The reference's order (mode 0) rounds to f32 twice: <= ~2 half-ulps of 1.0.
+    assert x_abs <= 2.5e-7, x_abs
 
 
 @pytest.mark.parametrize("mode", MODES)
@@ -73,6 +75,7 @@ def test_float_gauss_against_fp64(ctx, mode, sigma):
     x_abs, _ = errors(exact, ref)
     assert e_abs <= 2e-6, (e_abs, x_abs, where(fast, ref))
     assert e_rel <= 2e-6, (e_rel, x_abs)
+    assert x_abs <= 2.5e-7, x_abs
 
 
 def test_float_mode_keeps_integer_images_exact(ctx):
@@ -81,3 +84,20 @@ def test_float_mode_keeps_integer_images_exact(ctx):
     p = v.Pipeline(ctx, 131, 97, 3).set_float_mode(v.Pipeline.FLOAT_FP32).reduce_h(1.7).reduce_v(1.7)
     exp = O.Pipeline(131, 97, 3).reduce_h(1.7).reduce_v(1.7).run(src)
     assert np.array_equal(p.run_host(src), exp)
+
+
+@pytest.mark.parametrize("what", ["reduce", "gauss"])
+def test_fp16x3_no_less_accurate_than_fp32(ctx, what):
+    # The FP16 hi/lo MFMA path stays within a max relative error of 1e-6
+    # against FP64, and within 10 % of the plain FP32 kernel's errors.
+    src = float_image(256, 256, 3, 99)
+    if what == "reduce":
+        ref = O.f64_reduce_hv(src, 2.5, O.LANCZOS3)
+        steps = [("reduce_h", (2.5, v.LANCZOS3)), ("reduce_v", (2.5, v.LANCZOS3))]
+    else:
+        ref = O.f64_gauss(src, 8.0)
+        steps = [("gauss_blur", (8.0,))]
+    e32 = errors(run_mode(ctx, src, steps, v.Pipeline.FLOAT_FP32), ref)
+    e16 = errors(run_mode(ctx, src, steps, v.Pipeline.FLOAT_FP16X3), ref)
+    assert e16[1] <= 1e-6, e16
+    assert e16[0] <= 1.1 * e32[0] and e16[1] <= 1.1 * e32[1], (e16, e32)

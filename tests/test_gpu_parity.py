@@ -77,6 +77,39 @@ def test_gauss_f32_fixture_on_gpu(ctx):
     assert np.max(np.abs(got.ravel() - exp)) <= 1e-3
 
 
+def test_float_and_resize_fixtures_on_gpu(ctx):
+    # The fixtures that pin the oracle's ConvSep, float Gaussian and resize
+    # (tests/test_oracle_pins.py), run through the device path at the
+    # reference's tolerance, and equal to the oracle bit for bit.
+    from tests.test_oracle_pins import (convsep_source, fractional_ramp, max_ulp, nonuniform_source,
+                                        sigma1_mask)
+
+    f32_cases = [
+        ("convsep_libvips/gaussian_sigma_1_0.bin", convsep_source(9, 7),
+         [("convsep", (list(sigma1_mask()), 1.0, 0.0))]),
+        ("gauss_blur_libvips/nonuniform_5x3.bin", nonuniform_source(5, 3), [("gauss_blur", (1.0,))]),
+        ("gauss_blur/fractional_ramp_sigma_1_0.bin", fractional_ramp(), [("gauss_blur", (1.0,))]),
+        ("gauss_blur_libvips/uniform_field.bin", np.full((8, 8, 1), 37.5, np.float32), [("gauss_blur", (1.0,))]),
+    ]
+    for rel, src, steps in f32_cases:
+        got, exp = run_both(ctx, src, steps)
+        assert max_ulp(got, np.fromfile(FIX / rel, "<f4")) <= 1, rel
+        assert_same(got, exp, rel)
+    u8_cases = [
+        ("libvips_parity_resample/test_resize_half_bilinear_rgb.bin", rgb_source(16, 12), 0),
+        ("libvips_parity_resample/test_resize_height_one_rounding.bin", smooth_grayscale_source(100, 1), 0),
+    ]
+    for rel, src, tol in u8_cases:
+        got, exp = run_both(ctx, src, [("resize", (0.5, 0.5, v.BILINEAR))])
+        assert np.array_equal(got.ravel(), np.fromfile(FIX / rel, np.uint8)), rel
+        assert_same(got, exp, rel)
+    for rel, src in [("libvips_parity_convolution/test_gaussblur_u8_nonuniform_5x3.bin", grayscale_source(5, 3)),
+                     ("libvips_parity_convolution/test_gaussblur_u8_rgb_8x8.bin", rgb_source(8, 8))]:
+        got, exp = run_both(ctx, src, [("gauss_blur", (1.0,))])
+        assert np.max(np.abs(got.ravel().astype(int) - np.fromfile(FIX / rel, np.uint8).astype(int))) <= 1, rel
+        assert_same(got, exp, rel)
+
+
 # ------------------------------------------------------- configurations
 
 def test_resize_1024x768_lanczos3_factor_3_7(ctx):

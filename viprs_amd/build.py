@@ -44,25 +44,33 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the gfx950 library cannot be built")
 
 
-def build(verbose: bool = False) -> pathlib.Path:
+def _compile(cc: str, src: str, objdir: pathlib.Path, verbose: bool) -> str:
+    obj = objdir / (src + ".o")
+    cmd = [cc, *COMMON, "-c", str(CSRC / src), "-o", str(obj)]
+    if src.endswith(".cpp"):
+        cmd[1:1] = ["-x", "hip"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stderr=None if verbose else subprocess.PIPE, text=True)
+    if r.returncode != 0:
+        # Show the compiler's diagnostics (warnings about the host pass not
+        # knowing -packed-fp32-ops are filtered out).
+        diag = "\n".join(line for line in (r.stderr or "").splitlines() if "packed-fp32-ops" not in line)
+        raise RuntimeError(f"hipcc failed on {src}:\n{diag}")
+    return str(obj)
+
+
+def build(verbose: bool = False, jobs: int = 0) -> pathlib.Path:
+    """Compile every source (in parallel, `jobs` processes; 0 = up to 8) and
+    link the library.  Any failed compile raises."""
+    from concurrent.futures import ThreadPoolExecutor
+
     cc = hipcc()
     objdir = HERE / "_build"
     objdir.mkdir(exist_ok=True)
-    objs = []
-    for src in SOURCES:
-        obj = objdir / (src + ".o")
-        cmd = [cc, *COMMON, "-c", str(CSRC / src), "-o", str(obj)]
-        if src.endswith(".cpp"):
-            cmd[1:1] = ["-x", "hip"]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, stderr=None if verbose else subprocess.PIPE, text=True)
-        if r.returncode != 0:
-            # Show the compiler's diagnostics (warnings about the host pass not
-            # knowing -packed-fp32-ops are filtered out).
-            diag = "\n".join(line for line in (r.stderr or "").splitlines() if "packed-fp32-ops" not in line)
-            raise RuntimeError(f"hipcc failed on {src}:\n{diag}")
-        objs.append(str(obj))
+    jobs = jobs or min(8, os.cpu_count() or 1, len(SOURCES))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        objs = list(pool.map(lambda s: _compile(cc, s, objdir, verbose), SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs]
     subprocess.run(cmd, check=True)
