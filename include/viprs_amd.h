@@ -66,7 +66,8 @@ typedef enum {
     VGPU_E_NO_MEMORY = -7,
     VGPU_E_CANCELLED = -8,
     VGPU_E_ADMISSION = -9,    /* request exceeds the device memory budget */
-    VGPU_E_NOT_READY = -10    /* a wait timed out before the request finished */
+    VGPU_E_NOT_READY = -10,   /* a wait timed out before the request finished */
+    VGPU_E_SCRATCH_LIMIT = -11 /* plan tables + intermediates exceed VGPU_NOALLOC_SCRATCH_BYTES */
 } vgpu_status;
 
 /* Region (crates/viprs-core/src/image/region.rs:18-27). */
@@ -236,6 +237,25 @@ int vgpu_pipeline_time_stage(vgpu_pipeline *p, const void *src, size_t src_bytes
 int vgpu_pipeline_benchmark(vgpu_pipeline *p, const void *src, size_t src_bytes, void *dst,
                             size_t dst_bytes, int warmup, int iterations, float *total_ms,
                             float *step_ms);
+/* Capacity of the fixed device scratch array used by
+ * vgpu_pipeline_run_noalloc (one copy per GPU the library runs on, held until
+ * the process ends). */
+#define VGPU_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+/* vgpu_pipeline_run without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The plan tables and every intermediate live in the library's fixed scratch
+ * array of VGPU_NOALLOC_SCRATCH_BYTES per GPU instead of the context's pool;
+ * the caller passes the source and destination device buffers as for
+ * vgpu_pipeline_run, and gets bit-identical results.  A pipeline whose tables
+ * plus intermediates (each rounded up to 256 bytes) do not fit returns
+ * VGPU_E_SCRATCH_LIMIT before enqueuing anything.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other threads, or the caller's own queued calls on other streams) race
+ * and must be serialised by the caller.  Calls on one stream are ordered and
+ * safe.  The pipeline must stay unmodified until the call's work has run. */
+int vgpu_pipeline_run_noalloc(vgpu_pipeline *p, const void *src, size_t src_bytes, void *dst,
+                              size_t dst_bytes);
 /* Same with host pointers: H2D copy, run, D2H copy, synchronize. */
 int vgpu_pipeline_run_host(vgpu_pipeline *p, const void *src, size_t src_bytes, void *dst,
                            size_t dst_bytes);

@@ -134,6 +134,7 @@ def load() -> C.CDLL:
         "vgpu_pipeline_stage_kernel": (i32, [vp, i32, C.c_char_p, sz]),
         "vgpu_pipeline_run": (i32, [vp, vp, sz, vp, sz]),
         "vgpu_pipeline_run_host": (i32, [vp, vp, sz, vp, sz]),
+        "vgpu_pipeline_run_noalloc": (i32, [vp, vp, sz, vp, sz]),
         "vgpu_pipeline_run_profiled": (i32, [vp, vp, sz, vp, sz, i32, P(f32), i32]),
         "vgpu_pipeline_time_stage": (i32, [vp, vp, sz, vp, sz, i32, i32, P(f32)]),
         "vgpu_pipeline_benchmark": (i32, [vp, vp, sz, vp, sz, i32, i32, P(f32), P(f32)]),

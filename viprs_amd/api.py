@@ -36,6 +36,8 @@ E_NO_MEMORY = -7
 E_CANCELLED = -8
 E_ADMISSION = -9
 E_NOT_READY = -10
+E_SCRATCH_LIMIT = -11
+NOALLOC_SCRATCH_BYTES = 64 << 20
 OPT_SEP_ROWS_PER_TASK, OPT_SHRINK_RUN, OPT_GAUSS_ROWS = 2, 3, 4
 
 NP_DTYPE = {U8: np.uint8, U16: np.uint16, F32: np.float32}
@@ -316,6 +318,13 @@ class Pipeline:
         """Enqueue on the context stream; src / dst are device tensors."""
         _check(self._lib.vgpu_pipeline_run(self._h, C.c_void_p(_ptr(src)), _nbytes(src),
                                            C.c_void_p(_ptr(dst)), _nbytes(dst)), "run")
+
+    def run_noalloc(self, src, dst) -> None:
+        """As run(), with the plan tables and intermediates in the library's
+        fixed scratch array instead of pooled allocations
+        (vgpu_pipeline_run_noalloc)."""
+        _check(self._lib.vgpu_pipeline_run_noalloc(self._h, C.c_void_p(_ptr(src)), _nbytes(src),
+                                                   C.c_void_p(_ptr(dst)), _nbytes(dst)), "run_noalloc")
 
     def output_buffer(self) -> "DeviceBuffer":
         w, h, b, f = self.output_info()

@@ -1239,23 +1239,32 @@ std::vector<Stage> plan_pipeline(const std::vector<Node>& nodes, uint32_t w, uin
 
 }  // namespace vgpu
 
-int vgpu_pipeline::compile() {
-    if (compiled) return VGPU_OK;
+int vgpu_pipeline::plan() {
+    if (planned) return VGPU_OK;
     if (!affine_tail_supported(nodes)) return VGPU_E_UNSUPPORTED;
     const int n = (int)nodes.size();
     if (n == 0 || cur_w == 0 || cur_h == 0) {
-        compiled = true;
+        planned = true;
         return VGPU_OK;
     }
     // Backward region walk (execute.rs:413-480) from the full output (or
     // from each tile column, see plan_pipeline).
     Staging st;
     stages = plan_pipeline(nodes, cur_w, cur_h, &st, opt);
-    if (!st.bytes.empty()) {
+    host_blob = std::move(st.bytes);
+    planned = true;
+    return VGPU_OK;
+}
+
+int vgpu_pipeline::compile() {
+    if (compiled) return VGPU_OK;
+    int rc = plan();
+    if (rc != VGPU_OK) return rc;
+    if (!host_blob.empty()) {
         blob.ctx = ctx;
-        blob.ptr = ctx->pool.acquire(st.bytes.size());
+        blob.ptr = ctx->pool.acquire(host_blob.size());
         if (!blob.ptr) return VGPU_E_NO_MEMORY;
-        hipError_t e = hipMemcpyAsync(blob.ptr, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice, ctx->stream);
+        hipError_t e = hipMemcpyAsync(blob.ptr, host_blob.data(), host_blob.size(), hipMemcpyHostToDevice, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) return VGPU_E_DEVICE;
     }
@@ -1274,6 +1283,35 @@ int vgpu_pipeline::compile() {
     return VGPU_OK;
 }
 
+// ------------------------------------------------- fixed scratch (no-alloc)
+
+// The no-alloc run's plan tables and intermediates.  In host code `scratch`
+// names a host-side stand-in, not device memory: only the address
+// hipGetSymbolAddress returns for the current device may be passed to kernels
+// or copies.
+static __device__ __align__(256) unsigned char scratch[VGPU_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_by_device;  // per device id, looked up once
+
+// This device's copy of `scratch`; the device must be current.
+int scratch_address(int device, void** out) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if ((size_t)device >= g_scratch_by_device.size()) g_scratch_by_device.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_by_device[device]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return VGPU_E_DEVICE;
+        }
+        g_scratch_by_device[device] = ptr;
+    }
+    *out = g_scratch_by_device[device];
+    return VGPU_OK;
+}
+}  // namespace
+
 // ------------------------------------------------------------------ C ABI
 
 extern "C" {
@@ -1291,6 +1329,7 @@ const char* vgpu_status_string(int s) {
     case VGPU_E_CANCELLED: return "cancelled";
     case VGPU_E_ADMISSION: return "request exceeds the device memory budget";
     case VGPU_E_NOT_READY: return "request not finished";
+    case VGPU_E_SCRATCH_LIMIT: return "plan tables and intermediates exceed the fixed scratch array";
     default: return "unknown status";
     }
 }
@@ -1304,9 +1343,13 @@ int vgpu_context_create(int device, vgpu_context** out) {
         return VGPU_E_DEVICE;
     }
     if (hipSetDevice(device) != hipSuccess) return VGPU_E_DEVICE;
+    void* scratch_ptr = nullptr;
+    int rc = scratch_address(device, &scratch_ptr);
+    if (rc != VGPU_OK) return rc;
     auto* c = new (std::nothrow) vgpu_context();
     if (!c) return VGPU_E_NO_MEMORY;
     c->device = device;
+    c->scratch = scratch_ptr;
     if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return VGPU_E_DEVICE;
@@ -1797,6 +1840,58 @@ int vgpu_pipeline_run(vgpu_pipeline* p, const void* src, size_t src_bytes, void*
             ctx->sticky = rc;
             return rc;
         }
+        cur = o;
+        cur_rect = s.out;
+    }
+    return VGPU_OK;
+}
+
+int vgpu_pipeline_run_noalloc(vgpu_pipeline* p, const void* src, size_t src_bytes, void* dst,
+                              size_t dst_bytes) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    vgpu_context* ctx = p->ctx;
+    if (ctx->sticky != VGPU_OK) return ctx->sticky;
+    const size_t in_bytes = (size_t)p->src_w * p->src_h * p->bands * format_size(p->src_fmt);
+    const size_t out_bytes = (size_t)p->cur_w * p->cur_h * p->bands * format_size(p->cur_fmt);
+    if (src_bytes != in_bytes || dst_bytes != out_bytes) return VGPU_E_BUFFER_SIZE;
+    if (out_bytes == 0) return VGPU_OK;
+    if (!src || !dst) return VGPU_E_INVALID_ARGUMENT;
+    if (!ctx->scratch) return VGPU_E_DEVICE;
+    (void)hipSetDevice(ctx->device);
+    int rc = p->plan();  // host memory only
+    if (rc != VGPU_OK) return rc;
+    if (p->stages.empty()) {
+        return from_hip(hipMemcpyAsync(dst, src, out_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    // Layout in the scratch array: plan tables, then each intermediate.
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t used = up(p->host_blob.size());
+    std::vector<size_t> off(p->stages.size(), 0);
+    for (size_t k = 0; k < p->stages.size(); ++k) {
+        if (stage_writes_dst(p->stages, k)) continue;
+        const Stage& s = p->stages[k];
+        off[k] = used;
+        used += up((size_t)s.out.w * (size_t)s.out.h * p->bands * format_size(s.out_fmt));
+        if (used > VGPU_NOALLOC_SCRATCH_BYTES) return VGPU_E_SCRATCH_LIMIT;
+    }
+    if (used > VGPU_NOALLOC_SCRATCH_BYTES) return VGPU_E_SCRATCH_LIMIT;
+    unsigned char* base = static_cast<unsigned char*>(ctx->scratch);
+    if (!p->host_blob.empty()) {
+        rc = from_hip(hipMemcpyAsync(base, p->host_blob.data(), p->host_blob.size(), hipMemcpyHostToDevice,
+                                     ctx->stream));
+        if (rc != VGPU_OK) return ctx->sticky = rc;
+    }
+    const void* cur = src;
+    Rect cur_rect = whole(p->src_w, p->src_h);
+    for (size_t k = 0; k < p->stages.size(); ++k) {
+        const Stage& s = p->stages[k];
+        if (stage_reads_source(p->stages, k)) {
+            cur = src;
+            cur_rect = whole(p->src_w, p->src_h);
+        }
+        void* o = stage_writes_dst(p->stages, k) ? dst : static_cast<void*>(base + off[k]);
+        rc = launch_stage(p->nodes, s, view_of(cur, cur_rect, s.clip), o, base, ctx->stream, p->opt);
+        if (rc != VGPU_OK) return ctx->sticky = rc;
         cur = o;
         cur_rect = s.out;
     }

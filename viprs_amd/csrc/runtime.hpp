@@ -78,6 +78,9 @@ struct vgpu_context {
     hipStream_t stream = nullptr;
     vgpu::BufferPool pool;
     int sticky = VGPU_OK;
+    // Device address of this device's copy of the library's fixed scratch
+    // array (vgpu_pipeline_run_noalloc), looked up once at context creation.
+    void* scratch = nullptr;
 };
 
 namespace vgpu {
@@ -213,9 +216,11 @@ struct vgpu_pipeline {
     int cur_fmt = vgpu::U8;
     std::vector<vgpu::Node> nodes;
 
-    bool compiled = false;
+    bool planned = false;   // stages and host_blob built (host only)
+    bool compiled = false;  // plus the blob uploaded and intermediates allocated
     vgpu::StageOptions opt;
     std::vector<vgpu::Stage> stages;
+    std::vector<unsigned char> host_blob;  // plan tables as uploaded
     vgpu::DeviceBlob blob;
     std::vector<void*> inter;  // stage outputs except the last
 
@@ -225,6 +230,8 @@ struct vgpu_pipeline {
         inter.clear();
         blob.reset();
         stages.clear();
+        host_blob.clear();
+        planned = false;
         compiled = false;
     }
     vgpu::Node& push(vgpu::NodeKind k) {
@@ -243,5 +250,6 @@ struct vgpu_pipeline {
         cur_h = nodes.back().out_h;
         cur_fmt = nodes.back().out_fmt;
     }
-    int compile();
+    int plan();     // host-side planning only: stages and host_blob
+    int compile();  // plan(), then the blob and intermediates in pooled HBM
 };
