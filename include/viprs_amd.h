@@ -217,6 +217,21 @@ int vgpu_pipeline_stage_kernel(vgpu_pipeline *p, int index, char *name, size_t c
  * src / dst are device pointers; enqueued on the context stream. */
 int vgpu_pipeline_run(vgpu_pipeline *p, const void *src, size_t src_bytes, void *dst,
                       size_t dst_bytes);
+/* CancellationToken (crates/viprs-core/src/cancel.rs:19-72): a flag shared
+ * between the caller and a run; cancel() may be called from any thread. */
+typedef struct vgpu_cancel_token vgpu_cancel_token;
+int vgpu_cancel_token_new(vgpu_cancel_token **out);
+void vgpu_cancel_token_destroy(vgpu_cancel_token *t);
+void vgpu_cancel_token_cancel(vgpu_cancel_token *t);
+int vgpu_cancel_token_is_cancelled(const vgpu_cancel_token *t);
+/* TileScheduler::run_cancellable (crates/viprs-ports/src/scheduler.rs:132-
+ * 142; RayonScheduler checks between tiles, scheduler/rayon_scheduler/
+ * runtime.rs:1237): as vgpu_pipeline_run, polling `token` (may be NULL)
+ * before the run and before each fused stage launch; returns
+ * VGPU_E_CANCELLED when it is set (stages already enqueued still run, so the
+ * destination may be partly written, as the reference's sink is). */
+int vgpu_pipeline_run_cancellable(vgpu_pipeline *p, const void *src, size_t src_bytes, void *dst,
+                                  size_t dst_bytes, const vgpu_cancel_token *token);
 /* RayonScheduler::run_with_profile (crates/viprs-runtime/src/scheduler/
  * rayon_scheduler/runtime.rs:424-; NodeRunProfile): runs the pipeline
  * `iterations` times, timing every fused stage with HIP events on the context
@@ -305,13 +320,23 @@ int vgpu_op_output_size(const vgpu_op *op, uint32_t in_w, uint32_t in_h, uint32_
                         uint32_t *out_h);
 int vgpu_op_output_format(const vgpu_op *op, int *format);
 /* DynOperation::dyn_process_region(state, input, output, input_region,
- * output_region) (dynamic_op.rs:222-229): device pointers, enqueued on the
- * context stream.  Unlike OperationBridge (bridge.rs:208-236), a length that
- * does not match the region returns VGPU_E_BUFFER_SIZE instead of silently
- * writing nothing. */
+ * output_region) (dynamic_op.rs:222-229) for a device scheduler: DEVICE
+ * pointers (host pointers are rejected with VGPU_E_INVALID_ARGUMENT),
+ * enqueued on the context stream.  Unlike OperationBridge
+ * (bridge.rs:208-236), a length that does not match the region returns
+ * VGPU_E_BUFFER_SIZE instead of silently writing nothing. */
 int vgpu_op_process_region(vgpu_op *op, const void *input, size_t input_len, void *output,
                            size_t output_len, vgpu_region input_region,
                            vgpu_region output_region);
+/* The same for HOST buffers, as the unchanged RayonScheduler hands them to
+ * dyn_process_region (dynamic_op.rs:222-229; execute.rs:323-325,675-677):
+ * the input tile is copied to pooled device memory, processed, and the
+ * output copied back before the call returns (synchronous, like the
+ * reference's process_region).  Concurrent calls on one op are serialised;
+ * distinct ops may be called from different threads. */
+int vgpu_op_process_region_host(vgpu_op *op, const void *input, size_t input_len, void *output,
+                                size_t output_len, vgpu_region input_region,
+                                vgpu_region output_region);
 
 /* ------------------------------------------------------ batched thumbnails */
 
@@ -450,6 +475,12 @@ int vgpu_server_wait(vgpu_server *s, uint64_t id, int64_t timeout_us, vgpu_reque
 int vgpu_server_output(vgpu_server *s, uint64_t id, const void **dev_ptr, size_t *bytes);
 /* Copy a finished request's output to host memory (synchronous). */
 int vgpu_server_fetch(vgpu_server *s, uint64_t id, void *host_dst, size_t bytes);
+/* Cancel a request (CancellationToken per request, cancel.rs:6-10): a
+ * request not yet on the device is dropped without running; one already
+ * launched runs to the end on the device but its result is discarded.  Either
+ * way it finishes with status VGPU_E_CANCELLED.  Returns VGPU_OK, or
+ * VGPU_E_INVALID_ARGUMENT for an unknown or already finished id. */
+int vgpu_server_cancel(vgpu_server *s, uint64_t id);
 /* Free a finished request's memory and record. */
 int vgpu_server_release(vgpu_server *s, uint64_t id);
 /* Counters; returns the server's sticky status. */

@@ -3,7 +3,7 @@ convolution path (shrink / reduce / thumbnail / resize / gauss_blur /
 convsep), behind a C ABI (include/viprs_amd.h) with this Python mirror of the
 reference's builder and operation API."""
 from .api import (  # noqa: F401
-    BICUBIC, BILINEAR, CATMULLROM, F32, FITBOX, FORCE_EXACT, HEIGHT, LANCZOS2, LANCZOS3, LBB,
+    BICUBIC, BILINEAR, CATMULLROM, CancelToken, F32, FITBOX, FORCE_EXACT, HEIGHT, LANCZOS2, LANCZOS3, LBB,
     NEAREST, NOHALO, U8, U16, VSQBS, WIDTH, Batch, Context, DeviceBuffer, Op, Pipeline, Region,
     Server, VgpuError, thumbnail_shrink_on_load_factor,
 )

@@ -135,6 +135,11 @@ def load() -> C.CDLL:
         "vgpu_pipeline_run": (i32, [vp, vp, sz, vp, sz]),
         "vgpu_pipeline_run_host": (i32, [vp, vp, sz, vp, sz]),
         "vgpu_pipeline_run_noalloc": (i32, [vp, vp, sz, vp, sz]),
+        "vgpu_pipeline_run_cancellable": (i32, [vp, vp, sz, vp, sz, vp]),
+        "vgpu_cancel_token_new": (i32, [vp]),
+        "vgpu_cancel_token_destroy": (None, [vp]),
+        "vgpu_cancel_token_cancel": (None, [vp]),
+        "vgpu_cancel_token_is_cancelled": (i32, [vp]),
         "vgpu_pipeline_run_profiled": (i32, [vp, vp, sz, vp, sz, i32, P(f32), i32]),
         "vgpu_pipeline_time_stage": (i32, [vp, vp, sz, vp, sz, i32, i32, P(f32)]),
         "vgpu_pipeline_benchmark": (i32, [vp, vp, sz, vp, sz, i32, i32, P(f32), P(f32)]),
@@ -154,6 +159,7 @@ def load() -> C.CDLL:
         "vgpu_op_output_size": (i32, [vp, u32, u32, P(u32), P(u32)]),
         "vgpu_op_output_format": (i32, [vp, P(i32)]),
         "vgpu_op_process_region": (i32, [vp, vp, sz, vp, sz, Region, Region]),
+        "vgpu_op_process_region_host": (i32, [vp, vp, sz, vp, sz, Region, Region]),
         "vgpu_pipeline_set_integer_mfma": (i32, [vp, i32]),
         "vgpu_pipeline_set_float_mode": (i32, [vp, i32]),
         "vgpu_batch_set_integer_mfma": (i32, [vp, i32]),
@@ -175,6 +181,7 @@ def load() -> C.CDLL:
         "vgpu_server_output": (i32, [vp, C.c_uint64, P(vp), P(sz)]),
         "vgpu_server_fetch": (i32, [vp, C.c_uint64, vp, sz]),
         "vgpu_server_release": (i32, [vp, C.c_uint64]),
+        "vgpu_server_cancel": (i32, [vp, C.c_uint64]),
         "vgpu_server_stats": (i32, [vp, P(ServerCounters)]),
         "vgpu_server_replay": (i32, [vp, P(Request), u32, P(C.c_int64), P(C.c_int64), P(C.c_int32),
                                      P(C.c_int64), P(C.c_int64), P(C.c_int64)]),

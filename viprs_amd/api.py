@@ -319,6 +319,13 @@ class Pipeline:
         _check(self._lib.vgpu_pipeline_run(self._h, C.c_void_p(_ptr(src)), _nbytes(src),
                                            C.c_void_p(_ptr(dst)), _nbytes(dst)), "run")
 
+    def run_cancellable(self, src, dst, token: "CancelToken") -> None:
+        """TileScheduler::run_cancellable: raises VgpuError(E_CANCELLED) when
+        the token is set before the run or between its stage launches."""
+        _check(self._lib.vgpu_pipeline_run_cancellable(self._h, C.c_void_p(_ptr(src)), _nbytes(src),
+                                                       C.c_void_p(_ptr(dst)), _nbytes(dst), token._h),
+               "run_cancellable")
+
     def run_noalloc(self, src, dst) -> None:
         """As run(), with the plan tables and intermediates in the library's
         fixed scratch array instead of pooled allocations
@@ -565,6 +572,36 @@ class Op:
                                                 C.c_void_p(_ptr(dst)), _nbytes(dst),
                                                 input_region, output_region), "process_region")
 
+    def process_region_host(self, src: np.ndarray, dst: np.ndarray, input_region: Region,
+                            output_region: Region) -> None:
+        """dyn_process_region on host tiles (vgpu_op_process_region_host):
+        synchronous, `dst` is written when it returns."""
+        assert src.flags.c_contiguous and dst.flags.c_contiguous
+        _check(self._lib.vgpu_op_process_region_host(self._h, C.c_void_p(src.ctypes.data), src.nbytes,
+                                                     C.c_void_p(dst.ctypes.data), dst.nbytes,
+                                                     input_region, output_region), "process_region_host")
+
+
+class CancelToken:
+    """CancellationToken (vgpu_cancel_token)."""
+
+    def __init__(self):
+        self._lib = _native.load()
+        h = C.c_void_p()
+        _check(self._lib.vgpu_cancel_token_new(C.byref(h)), "vgpu_cancel_token_new")
+        self._h = h
+
+    def cancel(self) -> None:
+        self._lib.vgpu_cancel_token_cancel(self._h)
+
+    def is_cancelled(self) -> bool:
+        return bool(self._lib.vgpu_cancel_token_is_cancelled(self._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.vgpu_cancel_token_destroy(self._h)
+            self._h = None
+
 
 class Server:
     """Thumbnail request loop: requests are admitted by device-memory footprint
@@ -624,6 +661,10 @@ class Server:
 
     def release(self, rid: int) -> None:
         _check(self._lib.vgpu_server_release(self._h, rid), "vgpu_server_release")
+
+    def cancel(self, rid: int) -> None:
+        """The request finishes with E_CANCELLED (vgpu_server_cancel)."""
+        _check(self._lib.vgpu_server_cancel(self._h, rid), "vgpu_server_cancel")
 
     def stats(self) -> dict:
         st = _native.ServerCounters()

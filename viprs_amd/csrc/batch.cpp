@@ -175,6 +175,7 @@ int vgpu_batch::plan() {
         for (size_t k = 0; k + 1 < im.stages.size(); ++k) {
             const Stage& s = im.stages[k];
             im.inter_off.push_back(arena_need);
+            if (stage_writes_dst(im.stages, k)) continue;  // a tile column's ST_PLACE writes the output
             arena_need = align_up(arena_need + (size_t)s.out.w * s.out.h * im.bands * format_size(s.out_fmt), 256);
         }
         for (const Stage& s : im.stages)
@@ -642,7 +643,7 @@ int vgpu_batch_benchmark(vgpu_batch* b, const void* const* srcs, const size_t* s
     if (rc != VGPU_OK) return rc;
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
     hipEvent_t a, e;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&e) != hipSuccess) return VGPU_E_DEVICE;
+    if (!create_event_pair(&a, &e)) return VGPU_E_DEVICE;
     (void)hipEventRecord(a, ctx->stream);
     for (int i = 0; i < iterations && rc == VGPU_OK; ++i) rc = vgpu_batch_run(b, srcs, src_bytes, dsts, dst_bytes);
     (void)hipEventRecord(e, ctx->stream);

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -1809,9 +1810,32 @@ int vgpu_pipeline_stage_count(vgpu_pipeline* p) {
     return rc != VGPU_OK ? rc : (int)p->stages.size();
 }
 
+struct vgpu_cancel_token {
+    std::atomic<bool> cancelled{false};
+};
+
+int vgpu_cancel_token_new(vgpu_cancel_token** out) {
+    if (!out) return VGPU_E_INVALID_ARGUMENT;
+    *out = new (std::nothrow) vgpu_cancel_token();
+    return *out ? VGPU_OK : VGPU_E_NO_MEMORY;
+}
+void vgpu_cancel_token_destroy(vgpu_cancel_token* t) { delete t; }
+void vgpu_cancel_token_cancel(vgpu_cancel_token* t) {
+    if (t) t->cancelled.store(true, std::memory_order_relaxed);
+}
+int vgpu_cancel_token_is_cancelled(const vgpu_cancel_token* t) {
+    return t && t->cancelled.load(std::memory_order_relaxed) ? 1 : 0;
+}
+
 int vgpu_pipeline_run(vgpu_pipeline* p, const void* src, size_t src_bytes, void* dst,
                       size_t dst_bytes) {
+    return vgpu_pipeline_run_cancellable(p, src, src_bytes, dst, dst_bytes, nullptr);
+}
+
+int vgpu_pipeline_run_cancellable(vgpu_pipeline* p, const void* src, size_t src_bytes, void* dst,
+                                  size_t dst_bytes, const vgpu_cancel_token* token) {
     if (!p) return VGPU_E_INVALID_ARGUMENT;
+    if (vgpu_cancel_token_is_cancelled(token)) return VGPU_E_CANCELLED;
     vgpu_context* ctx = p->ctx;
     if (ctx->sticky != VGPU_OK) return ctx->sticky;
     const size_t in_bytes = (size_t)p->src_w * p->src_h * p->bands * format_size(p->src_fmt);
@@ -1829,6 +1853,9 @@ int vgpu_pipeline_run(vgpu_pipeline* p, const void* src, size_t src_bytes, void*
     const void* cur = src;
     Rect cur_rect = whole(p->src_w, p->src_h);
     for (size_t k = 0; k < p->stages.size(); ++k) {
+        // The scheduler polls the token between tiles (runtime.rs:1237); the
+        // units here are the fused stage launches (and tile-column runs).
+        if (vgpu_cancel_token_is_cancelled(token)) return VGPU_E_CANCELLED;
         const Stage& s = p->stages[k];
         if (stage_reads_source(p->stages, k)) {
             cur = src;
@@ -1965,7 +1992,7 @@ int vgpu_pipeline_time_stage(vgpu_pipeline* p, const void* src, size_t src_bytes
     const unsigned char* blob = static_cast<const unsigned char*>(p->blob.ptr);
     const InView in = view_of(cur, cur_rect, s.clip);
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return VGPU_E_DEVICE;
+    if (!create_event_pair(&e0, &e1)) return VGPU_E_DEVICE;
     for (int i = 0; i < 3 && rc == VGPU_OK; ++i) rc = launch_stage(p->nodes, s, in, o, blob, ctx->stream, p->opt);
     (void)hipEventRecord(e0, ctx->stream);
     for (int i = 0; i < iterations && rc == VGPU_OK; ++i)
@@ -1990,7 +2017,7 @@ int vgpu_pipeline_benchmark(vgpu_pipeline* p, const void* src, size_t src_bytes,
     if (rc != VGPU_OK) return rc;
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
     hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return VGPU_E_DEVICE;
+    if (!create_event_pair(&a, &b)) return VGPU_E_DEVICE;
     // Back-to-back steps: whole-job throughput.
     (void)hipEventRecord(a, ctx->stream);
     for (int i = 0; i < iterations && rc == VGPU_OK; ++i) rc = vgpu_pipeline_run(p, src, src_bytes, dst, dst_bytes);
@@ -2042,17 +2069,63 @@ int vgpu_pipeline_run_host(vgpu_pipeline* p, const void* src, size_t src_bytes, 
 
 // ------------------------------------------------------------- op level
 
+// A device buffer that queued work may still read: returned to the pool once
+// the event recorded after that work has completed.
+struct Retired {
+    void* ptr = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<unsigned char> host;  // host source of a copy still in flight
+};
+
 struct vgpu_op {
     vgpu_context* ctx = nullptr;
     Node node;
     Node vnode;          // ReduceOp: the vertical pass (node holds the horizontal one)
     bool pair = false;   // ReduceOp
+    // Serialises calls on one op: the blob, the scratch and the staging below
+    // are per-op state (the reference gives each rayon worker its own State;
+    // here the state is device memory shared by the op's calls).
+    std::mutex mu;
     DeviceBlob blob;
-    Rect blob_key;  // output region the uploaded plan was built for
+    std::vector<unsigned char> blob_host;  // bytes last uploaded into blob
+    Rect blob_key;  // output region the uploaded plan was built for (region-dependent plans)
     bool have_blob = false;
+    std::vector<Retired> retired;
     DeviceBlob scratch;  // ReduceOp's intermediate when its passes run as two kernels
     size_t scratch_bytes = 0;
+    ~vgpu_op() {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (Retired& r : retired) {
+            ctx->pool.release(r.ptr);
+            if (r.done) (void)hipEventDestroy(r.done);
+        }
+    }
 };
+
+// Return retired buffers whose readers have finished to the pool.
+static void sweep_retired(vgpu_op* op) {
+    auto& v = op->retired;
+    for (size_t i = 0; i < v.size();) {
+        if (!v[i].done || hipEventQuery(v[i].done) == hipSuccess) {
+            op->ctx->pool.release(v[i].ptr);
+            if (v[i].done) (void)hipEventDestroy(v[i].done);
+            v[i] = std::move(v.back());
+            v.pop_back();
+        } else {
+            ++i;
+        }
+    }
+    (void)hipGetLastError();
+}
+
+// Plans of ReduceH / ReduceV / ReduceOp depend on the output region (tile
+// origins decide the accumulated u8 positions and the tap windows); the
+// others (Gaussian, ConvSep, shrink, alpha) do not.
+static bool plan_depends_on_region(const vgpu_op* op) {
+    const NodeKind k = op->node.kind;
+    return op->pair || k == N_REDUCE_H || k == N_REDUCE_V;
+}
 
 static int make_op(vgpu_context* ctx, Node nd, vgpu_op** out) {
     if (!ctx || !out) return VGPU_E_INVALID_ARGUMENT;
@@ -2065,22 +2138,44 @@ static int make_op(vgpu_context* ctx, Node nd, vgpu_op** out) {
     return VGPU_OK;
 }
 
-// Uploads an op's plan tables unless the blob already holds the plan for
-// this output region.
-static int upload_op_blob(vgpu_op* op, const Staging& st, const Rect& out_r) {
+// Uploads an op's plan tables unless the blob already holds them (the same
+// bytes; for region-dependent plans the same output region).  The new blob
+// replaces the old one only after its allocation and copy have been
+// enqueued; the old one is retired until the work already queued on the
+// stream has finished with it, so no call drains the stream.
+static int upload_op_blob(vgpu_op* op, Staging& st, const Rect& out_r) {
     if (st.bytes.empty()) return VGPU_OK;
     vgpu_context* ctx = op->ctx;
-    const bool same = op->have_blob && op->blob_key.x == out_r.x && op->blob_key.y == out_r.y &&
-                      op->blob_key.w == out_r.w && op->blob_key.h == out_r.h;
-    if (same) return VGPU_OK;
-    // The previous plan may still be read by queued work.
-    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
-    op->blob.reset();
-    op->blob.ptr = ctx->pool.acquire(st.bytes.size());
-    if (!op->blob.ptr) return VGPU_E_NO_MEMORY;
-    hipError_t e = hipMemcpyAsync(op->blob.ptr, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
+    sweep_retired(op);
+    if (op->have_blob) {
+        const bool same_region = op->blob_key.x == out_r.x && op->blob_key.y == out_r.y &&
+                                 op->blob_key.w == out_r.w && op->blob_key.h == out_r.h;
+        if ((!plan_depends_on_region(op) || same_region) && op->blob_host == st.bytes) return VGPU_OK;
+    }
+    void* fresh = ctx->pool.acquire(st.bytes.size());
+    if (!fresh) return VGPU_E_NO_MEMORY;
+    std::vector<unsigned char> host = std::move(st.bytes);
+    if (hipMemcpyAsync(fresh, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->pool.release(fresh);
+        return ctx->sticky = VGPU_E_DEVICE;
+    }
+    if (op->blob.ptr) {
+        Retired r;
+        r.ptr = op->blob.ptr;
+        r.host = std::move(op->blob_host);
+        if (hipEventCreateWithFlags(&r.done, hipEventDisableTiming) != hipSuccess ||
+            hipEventRecord(r.done, ctx->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            if (r.done) (void)hipEventDestroy(r.done);
+            r.done = nullptr;
+            if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
+        }
+        op->retired.push_back(std::move(r));
+        op->blob.ptr = nullptr;
+    }
+    op->blob.ptr = fresh;
+    op->blob_host = std::move(host);
     op->blob_key = out_r;
     op->have_blob = true;
     return VGPU_OK;
@@ -2314,18 +2409,93 @@ int vgpu_op_output_format(const vgpu_op* op, int* f) {
     return VGPU_OK;
 }
 
+}  // extern "C"
+
+// Checks the byte lengths against the regions (OperationBridge casts the
+// slices and writes nothing on a mismatch, bridge.rs:208-236; here that is
+// VGPU_E_BUFFER_SIZE).
+static int check_region_lengths(const vgpu_op* op, size_t in_len, size_t out_len, const Rect& in_r,
+                                const Rect& out_r) {
+    const Node& nd = op->node;
+    if (in_len != (size_t)in_r.w * in_r.h * nd.bands * format_size(nd.in_fmt) ||
+        out_len != (size_t)out_r.w * out_r.h * nd.bands * format_size(nd.out_fmt))
+        return VGPU_E_BUFFER_SIZE;
+    return VGPU_OK;
+}
+
+// Whether kernels on this context's device may dereference `p`: device or
+// managed memory (host memory, pinned or not, is rejected: the op-level
+// device entry takes device buffers, the host entry takes host buffers).
+static bool device_pointer(const void* p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+static int process_region_locked(vgpu_op* op, const void* input, void* output, const Rect& in_r,
+                                 const Rect& out_r);
+
+extern "C" {
+
 int vgpu_op_process_region(vgpu_op* op, const void* input, size_t in_len, void* output,
                            size_t out_len, vgpu_region ir, vgpu_region orr) {
     if (!op) return VGPU_E_INVALID_ARGUMENT;
     vgpu_context* ctx = op->ctx;
     if (ctx->sticky != VGPU_OK) return ctx->sticky;
-    const Node& nd = op->node;
     const Rect in_r = rect_of(ir), out_r = rect_of(orr);
-    if (in_len != (size_t)in_r.w * in_r.h * nd.bands * format_size(nd.in_fmt) ||
-        out_len != (size_t)out_r.w * out_r.h * nd.bands * format_size(nd.out_fmt))
-        return VGPU_E_BUFFER_SIZE;
+    int rc = check_region_lengths(op, in_len, out_len, in_r, out_r);
+    if (rc != VGPU_OK) return rc;
     if (out_r.empty()) return VGPU_OK;
     if (!input || !output) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    if (!device_pointer(input) || !device_pointer(output)) return VGPU_E_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> g(op->mu);
+    return process_region_locked(op, input, output, in_r, out_r);
+}
+
+int vgpu_op_process_region_host(vgpu_op* op, const void* input, size_t in_len, void* output,
+                                size_t out_len, vgpu_region ir, vgpu_region orr) {
+    if (!op) return VGPU_E_INVALID_ARGUMENT;
+    vgpu_context* ctx = op->ctx;
+    if (ctx->sticky != VGPU_OK) return ctx->sticky;
+    const Rect in_r = rect_of(ir), out_r = rect_of(orr);
+    int rc = check_region_lengths(op, in_len, out_len, in_r, out_r);
+    if (rc != VGPU_OK) return rc;
+    if (out_r.empty()) return VGPU_OK;
+    if (!input || !output) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    std::lock_guard<std::mutex> g(op->mu);
+    // Staging through pooled device buffers: H2D, the device op, D2H, then
+    // wait, so the call returns with `output` written, as dyn_process_region.
+    void* d_in = ctx->pool.acquire(in_len);
+    void* d_out = ctx->pool.acquire(out_len);
+    if (!d_in || !d_out) {
+        ctx->pool.release(d_in);
+        ctx->pool.release(d_out);
+        return VGPU_E_NO_MEMORY;
+    }
+    rc = from_hip(hipMemcpyAsync(d_in, input, in_len, hipMemcpyHostToDevice, ctx->stream));
+    if (rc == VGPU_OK) rc = process_region_locked(op, d_in, d_out, in_r, out_r);
+    if (rc == VGPU_OK) rc = from_hip(hipMemcpyAsync(output, d_out, out_len, hipMemcpyDeviceToHost, ctx->stream));
+    const int sync = from_hip(hipStreamSynchronize(ctx->stream));
+    if (rc == VGPU_OK) rc = sync;
+    if (sync == VGPU_OK) {
+        ctx->pool.release(d_in);
+        ctx->pool.release(d_out);
+    }  // else the buffers may still be in use by a faulted stream: leak them
+    if (rc == VGPU_E_DEVICE) ctx->sticky = rc;
+    return rc;
+}
+
+}  // extern "C"
+
+static int process_region_locked(vgpu_op* op, const void* input, void* output, const Rect& in_r,
+                                 const Rect& out_r) {
+    vgpu_context* ctx = op->ctx;
+    const Node& nd = op->node;
     if (op->pair) return process_reduce_op(op, input, output, in_r, out_r);
     if (nd.kind != N_REDUCE_H && nd.kind != N_REDUCE_V) {
         // These ops index their input relative to the tile origin the
@@ -2334,7 +2504,6 @@ int vgpu_op_process_region(vgpu_op* op, const void* input, size_t in_len, void* 
         if (need.x != in_r.x || need.y != in_r.y || need.w != in_r.w || need.h != in_r.h)
             return VGPU_E_INVALID_ARGUMENT;
     }
-    (void)hipSetDevice(ctx->device);
     Stage s;
     s.first = s.last = 0;
     s.clip = in_r;
@@ -2370,5 +2539,3 @@ int vgpu_op_process_region(vgpu_op* op, const void* input, size_t in_len, void* 
     if (rc != VGPU_OK) ctx->sticky = rc;
     return rc;
 }
-
-}  // extern "C"

@@ -70,6 +70,20 @@ class BufferPool {
 
 inline int from_hip(hipError_t e) { return e == hipSuccess ? VGPU_OK : VGPU_E_DEVICE; }
 
+// Two timing events, or none: the first is destroyed if the second fails.
+inline bool create_event_pair(hipEvent_t* a, hipEvent_t* b) {
+    if (hipEventCreate(a) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (hipEventCreate(b) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipEventDestroy(*a);
+        return false;
+    }
+    return true;
+}
+
 }  // namespace vgpu
 
 struct vgpu_context {

@@ -171,6 +171,7 @@ struct Job {
     // state
     JobState state = J_PLANNING;
     int status = VGPU_OK;
+    bool cancelled = false;  // vgpu_server_cancel: finish with VGPU_E_CANCELLED
     int64_t t_arrival = 0, t_planned = 0, t_admit = 0, t_done = 0;
 };
 
@@ -227,7 +228,10 @@ void vgpu_server::plan_job(Job& j) {
         j.stages = plan_pipeline(p->nodes, p->cur_w, p->cur_h, &st, p->opt);
         for (size_t k = 0; k + 1 < j.stages.size(); ++k) {
             const Stage& s = j.stages[k];
-            j.inter_bytes.push_back((size_t)s.out.w * (size_t)s.out.h * r.bands * format_size(s.out_fmt));
+            // Stages that write the output (a tile column's ST_PLACE) need no space.
+            j.inter_bytes.push_back(stage_writes_dst(j.stages, k)
+                                        ? 0
+                                        : (size_t)s.out.w * (size_t)s.out.h * r.bands * format_size(s.out_fmt));
         }
     }
     j.blob_bytes = st.bytes.size();
@@ -283,9 +287,15 @@ void vgpu_server::planner_loop(int index) {
             j = to_plan.front();
             to_plan.pop_front();
         }
-        plan_job(*j);
+        bool skip;
         {
             std::lock_guard<std::mutex> g(mu);
+            skip = j->cancelled;
+        }
+        if (!skip) plan_job(*j);
+        {
+            std::lock_guard<std::mutex> g(mu);
+            if (j->cancelled) j->status = VGPU_E_CANCELLED;
             j->t_planned = now_ns();
             j->state = J_READY;
         }
@@ -353,6 +363,7 @@ void vgpu_server::free_chunk(Job& j) {
 void vgpu_server::finish_job(Job& j) {
     j.t_done = now_ns();
     j.state = J_DONE;
+    if (j.cancelled && j.status == VGPU_OK) j.status = VGPU_E_CANCELLED;
     ++completed;
     pinned.release(j.pin_blob, j.pin_blob_cls);
     pinned.release(j.pin_in, j.pin_in_cls);
@@ -396,6 +407,7 @@ void vgpu_server::scheduler_loop() {
         while (!arrival.empty() && !blocked) {
             Job& j = *arrival.front();
             if (j.state == J_PLANNING) break;  // keep arrival order
+            if (j.cancelled && j.status == VGPU_OK) j.status = VGPU_E_CANCELLED;
             if (j.status != VGPU_OK || sticky != VGPU_OK) {
                 if (j.status == VGPU_OK) j.status = sticky;
                 if (j.status == VGPU_E_ADMISSION) ++rejected;
@@ -603,6 +615,18 @@ int vgpu_server_fetch(vgpu_server* s, uint64_t id, void* host_dst, size_t bytes)
     if (!host_dst || bytes != n) return VGPU_E_BUFFER_SIZE;
     (void)hipSetDevice(s->ctx->device);
     return from_hip(hipMemcpy(host_dst, p, n, hipMemcpyDeviceToHost));
+}
+
+int vgpu_server_cancel(vgpu_server* s, uint64_t id) {
+    if (!s) return VGPU_E_INVALID_ARGUMENT;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        std::shared_ptr<Job> j = find_job(s, id);
+        if (!j || j->state == J_DONE) return VGPU_E_INVALID_ARGUMENT;
+        j->cancelled = true;
+    }
+    s->cv_sched.notify_one();
+    return VGPU_OK;
 }
 
 int vgpu_server_release(vgpu_server* s, uint64_t id) {
