@@ -1808,3 +1808,48 @@ int orc_f64_gauss(const float *src, uint32_t w, uint32_t h, uint32_t bands, floa
     free(tmp);
     return ORC_OK;
 }
+
+/* ------------------------------------------------------- synthetic inputs */
+
+/* Seeded synthetic bytes (test input generator, no reference counterpart):
+ * 8-byte word i (little-endian) is splitmix64(seed + (i + 1) *
+ * 0x9E3779B97F4A7C15) (Steele, Lea, Flood, "Fast splittable pseudorandom
+ * number generators", OOPSLA 2014); the same stream the library's device fill
+ * writes, so large device-made inputs can be regenerated here for checking. */
+static uint64_t splitmix_word(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+typedef struct {
+    uint8_t *dst;
+    size_t n, w0, w1;
+    uint64_t seed;
+} synth_job;
+
+static void *synth_worker(void *arg) {
+    const synth_job *j = (const synth_job *)arg;
+    for (size_t w = j->w0; w < j->w1; ++w) {
+        const uint64_t z = splitmix_word(j->seed, w);
+        const size_t at = w * 8;
+        memcpy(j->dst + at, &z, j->n - at < 8 ? j->n - at : 8);
+    }
+    return NULL;
+}
+
+void orc_synthetic_bytes(uint8_t *dst, size_t n, uint64_t seed) {
+    enum { MAXT = 16 };
+    const size_t words = (n + 7) / 8;
+    int t = words < (1u << 20) ? 1 : MAXT;
+    pthread_t th[MAXT];
+    synth_job jobs[MAXT];
+    for (int k = 0; k < t; ++k) {
+        jobs[k] = (synth_job){dst, n, words * k / t, words * (k + 1) / t, seed};
+        if (t == 1) synth_worker(&jobs[k]);
+        else pthread_create(&th[k], NULL, synth_worker, &jobs[k]);
+    }
+    if (t > 1)
+        for (int k = 0; k < t; ++k) pthread_join(th[k], NULL);
+}

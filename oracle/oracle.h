@@ -116,6 +116,9 @@ int orc_f64_reduce_hv(const float *src, uint32_t w, uint32_t h, uint32_t bands, 
 int orc_f64_gauss(const float *src, uint32_t w, uint32_t h, uint32_t bands, float sigma,
                   double *dst, int threads);
 
+/* Seeded synthetic bytes: splitmix64 words, the device fill's stream. */
+void orc_synthetic_bytes(uint8_t *dst, size_t n, uint64_t seed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,8 @@ def lib() -> C.CDLL:
         L.orc_thumbnail_plan.argtypes = [u32, u32, u32, i32, u32, u32, i32, vp, vp, i32, vp, vp]
         L.orc_f64_reduce_hv.argtypes = [vp, u32, u32, u32, dbl, i32, vp, vp, vp, i32]
         L.orc_f64_gauss.argtypes = [vp, u32, u32, u32, C.c_float, vp, i32]
+        L.orc_synthetic_bytes.argtypes = [vp, C.c_size_t, C.c_uint64]
+        L.orc_synthetic_bytes.restype = None
         _lib = L
     return _lib
 
@@ -275,6 +277,13 @@ def f64_gauss(src: np.ndarray, sigma: float, threads: int = 0):
     _check(lib().orc_f64_gauss(_ptr(src), w, h, b, sigma, _ptr(dst), threads or (os.cpu_count() or 1)),
            "f64_gauss")
     return dst
+
+
+def synthetic_bytes_fast(nbytes: int, seed: int) -> np.ndarray:
+    """synthetic_bytes() computed by the C oracle (for multi-GB inputs)."""
+    out = np.empty(nbytes, np.uint8)
+    lib().orc_synthetic_bytes(_ptr(out), nbytes, seed & (2**64 - 1))
+    return out
 
 
 def synthetic_bytes(nbytes: int, seed: int) -> np.ndarray:

@@ -342,6 +342,9 @@ def test_synthetic_bytes_known_answers():
     assert [int(x) for x in w] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
     # Ragged lengths are prefixes of the same stream.
     assert np.array_equal(O.synthetic_bytes(13, 77), O.synthetic_bytes(16, 77)[:13])
+    # The C generator writes the same stream.
+    for n, seed in ((0, 1), (13, 77), (4099, 2**63 + 5), (1 << 16, 0)):
+        assert np.array_equal(O.synthetic_bytes_fast(n, seed), O.synthetic_bytes(n, seed))
 
 
 # ------------------------------------------------------------- ReduceOp
