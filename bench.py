@@ -96,11 +96,14 @@ def cpu_baseline(O, src_np, steps_fn, budget_s):
     return (w * h * n / 1e6) / dt, threads, n
 
 
-def bench_line(v, O, ctx, src, steps_fn, steps, warmup, workload):
+def bench_line(v, O, ctx, src, steps_fn, steps, warmup, workload, tune_fn=None):
     """One secondary pipeline on a resident image: parity against the oracle,
-    then timed like the headline (HIP events, back-to-back steps)."""
+    then timed like the headline (HIP events, back-to-back steps).  tune_fn
+    picks a kernel form on the GPU pipeline (results do not change)."""
     h, w, b = src.shape
     gp = steps_fn(v.Pipeline(ctx, w, h, b))
+    if tune_fn is not None:
+        tune_fn(gp)
     d_src = v.DeviceBuffer.from_numpy(ctx, src)
     d_dst = gp.output_buffer()
     gp.run(d_src, d_dst)
@@ -113,6 +116,38 @@ def bench_line(v, O, ctx, src, steps_fn, steps, warmup, workload):
             "hbm_frac": round(nbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
             "stages": gp.stage_kernels(),
             "stage_ms": [round(x, 5) for x in gp.run_profiled(d_src, d_dst, 10)]}
+
+
+def bench_integer_forms(v, O, ctx, gauss_src, thumb_src, d_thumb_src, steps, warmup):
+    """Plain integer kernels vs the int8-limb MFMA form (int32 accumulation),
+    both bit-exact: gauss_blur(2.0) on 4096x4096x4 uchar in the VALU tile
+    kernel, the MFMA pair and the generic one-axis integer kernels; and the
+    8192^2 thumbnail's ReduceH -> ReduceV stage (stage 1) on MFMA and on the
+    plain fused reduce kernel, timed alone between HIP events."""
+    gauss = lambda p: p.gauss_blur(2.0)
+    what = "gauss_blur sigma=2.0 on 4096x4096x4 uchar"
+    forms = {
+        "valu_tiles": lambda p: p.set_option(v.api.OPT_GAUSS_ROWS, 0),
+        "int8_mfma": lambda p: p.set_option(v.api.OPT_GAUSS_ROWS, -1),
+        "plain_generic": lambda p: p.set_integer_mfma(False).set_option(v.api.OPT_GAUSS_ROWS, -1),
+    }
+    out = {"gauss": {}}
+    for name, fn in forms.items():
+        out["gauss"][name] = bench_line(v, O, ctx, gauss_src, gauss, steps, warmup, what, tune_fn=fn)
+    H, W, B = thumb_src.shape
+    exp = O.Pipeline(W, H, B).thumbnail(O.WIDTH, 400).run(thumb_src)
+    red = {}
+    for name, mfma in (("int8_mfma", True), ("plain_integer", False)):
+        p = v.Pipeline(ctx, W, H, B).thumbnail(v.WIDTH, 400, 0, v.LANCZOS3).set_integer_mfma(mfma)
+        d_dst = p.output_buffer()
+        p.run(d_thumb_src, d_dst)
+        ctx.synchronize()
+        stage_ms = p.time_stage(d_thumb_src, d_dst, 1, max(steps, 20))
+        red[name] = {"kernel": p.stage_kernels()[1], "stage_ms": round(stage_ms, 5),
+                     "bit_exact": bool(np.array_equal(d_dst.numpy(), exp))}
+    out["thumbnail_reduce_stage"] = {"workload": "ReduceH -> ReduceV 1.0779 Lanczos3 (7 taps), 431x431x3 -> "
+                                                 "400x400x3, stage 1 of the 8192^2 thumbnail", **red}
+    return out
 
 
 def bench_batch(v, O, ctx, n, steps, warmup, check_images):
@@ -384,6 +419,8 @@ def main():
         gs = np.random.default_rng(97).integers(0, 256, size=(4096, 4096, 4), dtype=np.uint8)
         result["gauss_blur"] = bench_line(v, O, ctx, gs, lambda p: p.gauss_blur(2.0), args.steps, args.warmup,
                                           "gauss_blur sigma=2.0 on 4096x4096x4 uchar")
+        print("bench: integer kernel forms", file=sys.stderr, flush=True)
+        result["integer_forms"] = bench_integer_forms(v, O, ctx, gs, src, d_src, args.steps, args.warmup)
         print("bench: secondary reduce 1024x768", file=sys.stderr, flush=True)
         rs = np.random.default_rng(31).integers(0, 256, size=(768, 1024, 3), dtype=np.uint8)
         result["reduce_hv"] = bench_line(
