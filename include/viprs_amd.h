@@ -245,6 +245,18 @@ int vgpu_pipeline_run_profiled(vgpu_pipeline *p, const void *src, size_t src_byt
  * between two HIP events on the context stream.  No reference counterpart. */
 int vgpu_pipeline_time_stage(vgpu_pipeline *p, const void *src, size_t src_bytes, void *dst,
                              size_t dst_bytes, int stage, int iterations, float *mean_ms);
+/* As vgpu_pipeline_time_stage, the stage reading srcs[i % nsrc] on launch i
+ * (all of src_bytes): with sources that together exceed the last-level
+ * cache, the timing shows the kernel streaming from HBM rather than
+ * re-reading a cached image.  No reference counterpart. */
+int vgpu_pipeline_time_stage_sources(vgpu_pipeline *p, const void *const *srcs, int nsrc,
+                                     size_t src_bytes, void *dst, size_t dst_bytes, int stage,
+                                     int iterations, float *mean_ms);
+/* As vgpu_pipeline_benchmark, step i reading srcs[i % nsrc].  No reference
+ * counterpart. */
+int vgpu_pipeline_benchmark_sources(vgpu_pipeline *p, const void *const *srcs, int nsrc,
+                                    size_t src_bytes, void *dst, size_t dst_bytes, int warmup,
+                                    int iterations, float *total_ms, float *step_ms);
 /* Timing harness on the context stream: `warmup` untimed runs, then
  * `iterations` back-to-back runs bracketed by HIP events (total_ms), then
  * `iterations` more with an event after each (step_ms[i], may be NULL).

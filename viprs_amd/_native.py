@@ -135,6 +135,8 @@ def load() -> C.CDLL:
         "vgpu_pipeline_run": (i32, [vp, vp, sz, vp, sz]),
         "vgpu_pipeline_run_host": (i32, [vp, vp, sz, vp, sz]),
         "vgpu_pipeline_run_noalloc": (i32, [vp, vp, sz, vp, sz]),
+        "vgpu_pipeline_time_stage_sources": (i32, [vp, P(vp), i32, sz, vp, sz, i32, i32, P(f32)]),
+        "vgpu_pipeline_benchmark_sources": (i32, [vp, P(vp), i32, sz, vp, sz, i32, i32, P(f32), P(f32)]),
         "vgpu_pipeline_run_cancellable": (i32, [vp, vp, sz, vp, sz, vp]),
         "vgpu_cancel_token_new": (i32, [vp]),
         "vgpu_cancel_token_destroy": (None, [vp]),

@@ -365,6 +365,25 @@ class Pipeline:
                                                   C.byref(ms)), "time_stage")
         return ms.value
 
+    def time_stage_sources(self, srcs, dst, stage: int, iterations: int = 50) -> float:
+        """time_stage with launch i reading srcs[i % len(srcs)]."""
+        arr = (C.c_void_p * len(srcs))(*[_ptr(x) for x in srcs])
+        ms = C.c_float()
+        _check(self._lib.vgpu_pipeline_time_stage_sources(self._h, arr, len(srcs), _nbytes(srcs[0]),
+                                                          C.c_void_p(_ptr(dst)), _nbytes(dst), stage,
+                                                          iterations, C.byref(ms)), "time_stage_sources")
+        return ms.value
+
+    def benchmark_sources(self, srcs, dst, warmup: int, iterations: int):
+        """benchmark() with step i reading srcs[i % len(srcs)]."""
+        arr = (C.c_void_p * len(srcs))(*[_ptr(x) for x in srcs])
+        total = C.c_float()
+        steps = (C.c_float * iterations)()
+        _check(self._lib.vgpu_pipeline_benchmark_sources(self._h, arr, len(srcs), _nbytes(srcs[0]),
+                                                         C.c_void_p(_ptr(dst)), _nbytes(dst), warmup,
+                                                         iterations, C.byref(total), steps), "benchmark_sources")
+        return total.value / iterations, np.array(steps[:], np.float64)
+
     def run_host(self, src: np.ndarray) -> np.ndarray:
         src = np.ascontiguousarray(src, NP_DTYPE[self.src_fmt])
         w, h, b, f = self.output_info()

@@ -246,40 +246,47 @@ __device__ __forceinline__ void window_sums(const unsigned char* rbuf, int sbyte
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 // Premultiplied box sums of a 4-band window of fh pixels starting at the
-// 4-byte-aligned LDS byte sbyte (fh <= 257, so each packed 16-bit lane sum
-// stays below 2^16).  Per pixel, (r, b) and (g, a) travel as 16-bit pairs:
-// premultiply = round(c * a / 255) = (t + (t >> 8)) >> 8 with t = c * a + 128,
-// exact for all 8-bit c, a and equal to the (2ca + 255) / 510 of premult_u8;
-// alpha goes through the same form with the multiplier 255, which returns it
-// unchanged.
+// 4-byte-aligned LDS byte sbyte.  Pixels go in pairs: v_perm gathers each
+// colour channel of the two pixels into one packed 16-bit pair [c0, c1] and
+// their alphas into [a0, a1]; premultiply = round(c * a / 255) =
+// (t + (t >> 8)) >> 8 with t = c * a + 128, exact for all 8-bit c, a and equal
+// to the (2ca + 255) / 510 of premult_u8; the >> 8 of both halves is their
+// high bytes, which udot4 adds into the channel's 32-bit sum.  Alpha passes
+// through premultiply unchanged, so its sum is the plain byte sum.  An odd
+// last pixel pairs with a zero pixel, which adds 0 to every sum.
 __device__ __forceinline__ void premult_window_sums4(const unsigned char* rbuf, int sbyte, int fh,
                                                      uint32_t (&sum)[4]) {
     const uint32_t* l32 = reinterpret_cast<const uint32_t*>(rbuf + sbyte);
     const u16x2 half = {128, 128};
-    u16x2 acc_rb = {0, 0}, acc_ga = {0, 0};
-    auto pixel = [&](uint32_t v) {
-        const uint32_t a = v >> 24;
-        const u16x2 aa = __builtin_bit_cast(u16x2, a | (a << 16));
-        const u16x2 a255 = __builtin_bit_cast(u16x2, a | (255u << 16));
-        u16x2 rb = __builtin_bit_cast(u16x2, v & 0x00ff00ffu) * aa + half;
-        u16x2 ga = __builtin_bit_cast(u16x2, (v >> 8) & 0x00ff00ffu) * a255 + half;
-        rb = (rb + (rb >> (u16x2){8, 8})) >> (u16x2){8, 8};
-        ga = (ga + (ga >> (u16x2){8, 8})) >> (u16x2){8, 8};
-        acc_rb += rb;
-        acc_ga += ga;
+    const u16x2 eight = {8, 8};
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    auto pair = [&](uint32_t v0, uint32_t v1) {
+        // perm(hi, lo, sel): selector bytes 0-3 pick lo's bytes, 4-7 hi's, 0x0c gives 0.
+        const u16x2 aa = __builtin_bit_cast(u16x2, __builtin_amdgcn_perm(v1, v0, 0x0c070c03u));
+        const u16x2 r = __builtin_bit_cast(u16x2, __builtin_amdgcn_perm(v1, v0, 0x0c040c00u));
+        const u16x2 g = __builtin_bit_cast(u16x2, __builtin_amdgcn_perm(v1, v0, 0x0c050c01u));
+        const u16x2 b = __builtin_bit_cast(u16x2, __builtin_amdgcn_perm(v1, v0, 0x0c060c02u));
+        const u16x2 tr = r * aa + half, tg = g * aa + half, tb = b * aa + half;
+        const uint32_t ur = __builtin_bit_cast(uint32_t, (u16x2)(tr + (tr >> eight)));
+        const uint32_t ug = __builtin_bit_cast(uint32_t, (u16x2)(tg + (tg >> eight)));
+        const uint32_t ub = __builtin_bit_cast(uint32_t, (u16x2)(tb + (tb >> eight)));
+        s0 = __builtin_amdgcn_udot4(ur, 0x01000100u, s0, false);
+        s1 = __builtin_amdgcn_udot4(ug, 0x01000100u, s1, false);
+        s2 = __builtin_amdgcn_udot4(ub, 0x01000100u, s2, false);
+        s3 = __builtin_amdgcn_udot4(__builtin_bit_cast(uint32_t, aa), 0x00010001u, s3, false);
     };
     int j = 0;
-    for (; j + 2 <= fh; j += 2) {
-        const uint32_t v0 = l32[j], v1 = l32[j + 1];
-        pixel(v0);
-        pixel(v1);
+    for (; j + 4 <= fh; j += 4) {
+        const uint32_t v0 = l32[j], v1 = l32[j + 1], v2 = l32[j + 2], v3 = l32[j + 3];
+        pair(v0, v1);
+        pair(v2, v3);
     }
-    if (j < fh) pixel(l32[j]);
-    const uint32_t rb = __builtin_bit_cast(uint32_t, acc_rb), ga = __builtin_bit_cast(uint32_t, acc_ga);
-    sum[0] = rb & 0xffffu;
-    sum[1] = ga & 0xffffu;
-    sum[2] = rb >> 16;
-    sum[3] = ga >> 16;
+    for (; j + 2 <= fh; j += 2) pair(l32[j], l32[j + 1]);
+    if (j < fh) pair(l32[j], 0u);
+    sum[0] = s0;
+    sum[1] = s1;
+    sum[2] = s2;
+    sum[3] = s3;
 }
 
 // window_sums for a window of exactly ND dwords (ND = ceil(wbytes / 4),
@@ -552,7 +559,7 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
     const int X0 = cx0 + lane * fh;
     const bool fast = !premult && lane < ncols && X0 >= lo && X0 + fh <= hi;
     // 4-band premultiplied windows inside the span: packed 16-bit path.
-    const bool fast_pm4 = B == 4 && premult && fh <= 257 && lane < ncols && X0 >= lo && X0 + fh <= hi;
+    const bool fast_pm4 = B == 4 && premult && lane < ncols && X0 >= lo && X0 + fh <= hi;
     const int lane_byte = (X0 - lo) * B;
     const int wnd = (wbytes + 3) >> 2;  // window dwords
     const int wrem = wbytes - 4 * (wnd - 1);

@@ -1973,30 +1973,36 @@ int vgpu_pipeline_run_profiled(vgpu_pipeline* p, const void* src, size_t src_byt
 
 int vgpu_pipeline_time_stage(vgpu_pipeline* p, const void* src, size_t src_bytes, void* dst, size_t dst_bytes,
                              int stage, int iterations, float* mean_ms) {
-    if (!p || iterations < 1 || !mean_ms) return VGPU_E_INVALID_ARGUMENT;
+    return vgpu_pipeline_time_stage_sources(p, &src, 1, src_bytes, dst, dst_bytes, stage, iterations, mean_ms);
+}
+
+int vgpu_pipeline_time_stage_sources(vgpu_pipeline* p, const void* const* srcs, int nsrc, size_t src_bytes,
+                                     void* dst, size_t dst_bytes, int stage, int iterations, float* mean_ms) {
+    if (!p || !srcs || nsrc < 1 || iterations < 1 || !mean_ms) return VGPU_E_INVALID_ARGUMENT;
     vgpu_context* ctx = p->ctx;
     (void)hipSetDevice(ctx->device);
-    int rc = vgpu_pipeline_run(p, src, src_bytes, dst, dst_bytes);  // compiles, fills the intermediates
+    int rc = vgpu_pipeline_run(p, srcs[0], src_bytes, dst, dst_bytes);  // compiles, fills the intermediates
     if (rc != VGPU_OK) return rc;
     const int ns = (int)p->stages.size();
     if (stage < 0 || stage >= ns) return VGPU_E_INVALID_ARGUMENT;
-    // Input view of the stage: the source or the previous stage's output.
-    const void* cur = src;
-    Rect cur_rect = whole(p->src_w, p->src_h);
-    if (!stage_reads_source(p->stages, (size_t)stage)) {
-        cur = p->inter[(size_t)stage - 1];
-        cur_rect = p->stages[(size_t)stage - 1].out;
-    }
+    for (int i = 0; i < nsrc; ++i)
+        if (!srcs[i]) return VGPU_E_INVALID_ARGUMENT;
+    // Input view of the stage: the source (cycled over srcs) or the previous
+    // stage's output.
+    const bool from_src = stage_reads_source(p->stages, (size_t)stage);
     const Stage& s = p->stages[(size_t)stage];
     void* o = stage_writes_dst(p->stages, (size_t)stage) ? dst : p->inter[(size_t)stage];
     const unsigned char* blob = static_cast<const unsigned char*>(p->blob.ptr);
-    const InView in = view_of(cur, cur_rect, s.clip);
+    auto view = [&](int i) {
+        if (from_src) return view_of(srcs[i % nsrc], whole(p->src_w, p->src_h), s.clip);
+        return view_of(p->inter[(size_t)stage - 1], p->stages[(size_t)stage - 1].out, s.clip);
+    };
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!create_event_pair(&e0, &e1)) return VGPU_E_DEVICE;
-    for (int i = 0; i < 3 && rc == VGPU_OK; ++i) rc = launch_stage(p->nodes, s, in, o, blob, ctx->stream, p->opt);
+    for (int i = 0; i < 3 && rc == VGPU_OK; ++i) rc = launch_stage(p->nodes, s, view(i), o, blob, ctx->stream, p->opt);
     (void)hipEventRecord(e0, ctx->stream);
     for (int i = 0; i < iterations && rc == VGPU_OK; ++i)
-        rc = launch_stage(p->nodes, s, in, o, blob, ctx->stream, p->opt);
+        rc = launch_stage(p->nodes, s, view(i), o, blob, ctx->stream, p->opt);
     (void)hipEventRecord(e1, ctx->stream);
     float ms = 0.f;
     if (rc == VGPU_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
@@ -2010,17 +2016,25 @@ int vgpu_pipeline_time_stage(vgpu_pipeline* p, const void* src, size_t src_bytes
 int vgpu_pipeline_benchmark(vgpu_pipeline* p, const void* src, size_t src_bytes, void* dst,
                             size_t dst_bytes, int warmup, int iterations, float* total_ms,
                             float* step_ms) {
-    if (!p || iterations < 1 || !total_ms) return VGPU_E_INVALID_ARGUMENT;
+    return vgpu_pipeline_benchmark_sources(p, &src, 1, src_bytes, dst, dst_bytes, warmup, iterations, total_ms,
+                                           step_ms);
+}
+
+int vgpu_pipeline_benchmark_sources(vgpu_pipeline* p, const void* const* srcs, int nsrc, size_t src_bytes,
+                                    void* dst, size_t dst_bytes, int warmup, int iterations, float* total_ms,
+                                    float* step_ms) {
+    if (!p || !srcs || nsrc < 1 || iterations < 1 || !total_ms) return VGPU_E_INVALID_ARGUMENT;
     vgpu_context* ctx = p->ctx;
     int rc = VGPU_OK;
-    for (int i = 0; i < warmup && rc == VGPU_OK; ++i) rc = vgpu_pipeline_run(p, src, src_bytes, dst, dst_bytes);
+    auto step = [&](int i) { return vgpu_pipeline_run(p, srcs[i % nsrc], src_bytes, dst, dst_bytes); };
+    for (int i = 0; i < warmup && rc == VGPU_OK; ++i) rc = step(i);
     if (rc != VGPU_OK) return rc;
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->sticky = VGPU_E_DEVICE;
     hipEvent_t a, b;
     if (!create_event_pair(&a, &b)) return VGPU_E_DEVICE;
     // Back-to-back steps: whole-job throughput.
     (void)hipEventRecord(a, ctx->stream);
-    for (int i = 0; i < iterations && rc == VGPU_OK; ++i) rc = vgpu_pipeline_run(p, src, src_bytes, dst, dst_bytes);
+    for (int i = 0; i < iterations && rc == VGPU_OK; ++i) rc = step(i);
     (void)hipEventRecord(b, ctx->stream);
     if (rc == VGPU_OK && hipEventSynchronize(b) != hipSuccess) rc = VGPU_E_DEVICE;
     if (rc == VGPU_OK) (void)hipEventElapsedTime(total_ms, a, b);
@@ -2030,7 +2044,7 @@ int vgpu_pipeline_benchmark(vgpu_pipeline* p, const void* src, size_t src_bytes,
         for (auto& e : ev) (void)hipEventCreate(&e);
         (void)hipEventRecord(ev[0], ctx->stream);
         for (int i = 0; i < iterations && rc == VGPU_OK; ++i) {
-            rc = vgpu_pipeline_run(p, src, src_bytes, dst, dst_bytes);
+            rc = step(i);
             (void)hipEventRecord(ev[(size_t)i + 1], ctx->stream);
         }
         if (rc == VGPU_OK && hipEventSynchronize(ev[(size_t)iterations]) != hipSuccess) rc = VGPU_E_DEVICE;
