@@ -519,24 +519,65 @@ __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, 
     }
 }
 
+// Window sums of a lean-form window of nd dwords (ND - 3 <= nd <= ND, ND a
+// multiple of 4 fixed per kernel path): the first ND - 4 dwords are whole,
+// the last four are masked by tail[0..3] (per wavefront, computed once), so
+// no per-row dispatch on the window length is needed.  Reads ND + 1 dwords
+// from the 4-byte-aligned position below sbyte (the LDS group buffer is
+// padded for the read past the last window).
+template <int B, int ND>
+__device__ __forceinline__ void window_sums_masked(const unsigned char* rbuf, int sbyte, const uint32_t (&tail)[4],
+                                                   uint32_t (&sum)[B]) {
+    const uint32_t* l32 = reinterpret_cast<const uint32_t*>(rbuf) + (sbyte >> 2);
+    const int al = sbyte & 3;
+    uint32_t w[ND + 1];
+#pragma unroll
+    for (int d = 0; d <= ND; ++d) w[d] = l32[d];
+    uint32_t odd[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) odd[b] = 0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        uint32_t v = __builtin_amdgcn_alignbyte(w[d + 1], w[d], al);
+        if (d >= ND - 4) v &= tail[d - (ND - 4)];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            if (d & 1) odd[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, d % 3), odd[b], false);
+            else sum[b] = __builtin_amdgcn_udot4(v, band_mask<B>(b, d % 3), sum[b], false);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) sum[b] += odd[b];
+}
+
+// __umul24 returns the low 32 bits of the 48-bit product as an int: shift it
+// as unsigned (products reach 2^32 - 1).
+__device__ __forceinline__ uint32_t round_mul(uint32_t sum, uint32_t add, uint32_t mul, uint32_t shift) {
+    return (uint32_t)__umul24(sum + add, mul) >> shift;
+}
+
 // Lean form of the same kernel for spans of at least 128 chunks (lsl >= 7):
 // a fetch group is then one or two source rows and each 64-lane register
 // holds part of one of them (register j: row j >> (lsl - 6)), so a row's
-// address, chunk limit and 16-byte shift are wave-uniform and computed once
-// per row in scalar registers.  Loads take the row's 16-byte-aligned start
-// plus a per-lane 32-bit offset; lanes past the row's whole chunks re-read
-// its last one.  Parking is four unconditional 16-byte LDS stores; a chunk
-// cut by the end of the buffer (scalar flag) is then copied bytewise.
-// Output rows complete on a counter instead of a division per row.
-template <int B, bool TWO>
-__device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, int by, unsigned char* lds) {
+// address, chunk limit and 16-byte shift are wave-uniform.  The per-row
+// scalar work is kept small, since the scalar unit is shared by the CU's
+// four SIMDs: 32-bit buffer offsets (the planner sends images of 4 GiB and
+// more to the general form), the window length fixed per kernel path (ND),
+// both roundings as one multiply-shift with per-job constants.  Loads take
+// the row's 16-byte-aligned start plus a per-lane 32-bit offset; lanes past
+// the row's whole chunks re-read its last one.  Parking is four
+// unconditional 16-byte LDS stores; a chunk cut by the end of the buffer
+// (scalar flag) is then copied bytewise.  Output rows complete on a counter.
+template <int B, bool TWO, int ND>
+__device__ __forceinline__ void shrink_rows_lean_nd(const ShrinkJob& job, int bx, int by, unsigned char* lds) {
     const InView& in = job.in;
     const OutView& out = job.out;
     const int fh = job.fh, fv = job.fv, tx = job.tx, S = job.S, m = job.m, lsl = job.lsl;
     constexpr bool two = TWO;  // lsl == 7: two source rows per fetch group (registers 0-1, 2-3)
     constexpr int RP = two ? 2 : 1;
     const bool premult = (B == 2 || B == 4) && job.premult;
-    const uint32_t mh = job.mh, mv = job.mv;
+    const uint32_t rh_add = job.rh_add, rh_mul = job.rh_mul, rh_shift = job.rh_shift;
+    const uint32_t rv_add = job.rv_add, rv_mul = job.rv_mul, rv_shift = job.rv_shift;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int part = wave % S;
     const int oyi0 = (by * ((kThreads / 64) / S) + wave / S) * m;
@@ -549,11 +590,11 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
     const int cx1 = cx0 + ncols * fh;
     const int lo = clampi(cx0, in.bx, in.bx + in.bw - 1);
     const int hi = clampi(cx1 - 1, in.bx, in.bx + in.bw - 1) + 1;
-    const int nbytes = (hi - lo) * B;
+    const uint32_t nbytes = (uint32_t)(hi - lo) * B;
     const unsigned char* base = static_cast<const unsigned char*>(in.ptr);
-    const size_t row_bytes = (size_t)in.bw * B;
-    const size_t total = row_bytes * (size_t)in.bh;
-    const size_t col_off = (size_t)(lo - in.bx) * B;
+    const uint32_t row_bytes = (uint32_t)in.bw * B;
+    const uint32_t total = row_bytes * (uint32_t)in.bh;  // < 2^32 (planner)
+    const uint32_t col_off = (uint32_t)(lo - in.bx) * B;
     unsigned char* rbuf = lds + wave * (kSpanBytes + 32);
     const int wbytes = fh * B;
     const int X0 = cx0 + lane * fh;
@@ -561,29 +602,37 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
     // 4-band premultiplied windows inside the span: packed 16-bit path.
     const bool fast_pm4 = B == 4 && premult && lane < ncols && X0 >= lo && X0 + fh <= hi;
     const int lane_byte = (X0 - lo) * B;
-    const int wnd = (wbytes + 3) >> 2;  // window dwords
-    const int wrem = wbytes - 4 * (wnd - 1);
-    const uint32_t wlast = wrem >= 4 ? 0xffffffffu : ((1u << (8 * wrem)) - 1u);
+    // Masks of the window's last four dwords ND - 4 .. ND - 1 (ND = 0: the
+    // looped window sums of windows longer than 16 dwords).
+    uint32_t tail[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int valid = wbytes - 4 * (ND - 4 + t);
+        tail[t] = valid >= 4 ? 0xffffffffu : (valid <= 0 ? 0u : ((1u << (8 * valid)) - 1u));
+    }
 
     // Row k of the wavefront's run: a16 = its span's 16-byte-aligned buffer
     // offset, lb = where its chunks are loaded from (a16, or 0 -- the
     // buffer's first chunk -- for a row past the run or with no whole chunk),
     // lim = chunks to load (>= 1), cut = the chunk cut by the buffer end or
     // -1, shift = the span's offset past a16.
-    auto row_fetch = [&](int k, size_t& a16, size_t& lb, int& lim, int& cut, int& shift) {
+    auto row_fetch = [&](int k, uint32_t& a16, uint32_t& lb, int& lim, int& cut, int& shift) {
         a16 = 0;
         lb = 0;
         lim = 1;
         cut = -1;
         shift = 0;
         if (k < nsrc) {
-            const size_t a = (size_t)row_of(in, ybase + k * S) * row_bytes + col_off;
-            a16 = a & ~(size_t)15;
-            shift = (int)(a - a16);
-            const int nch = (shift + nbytes + 15) >> 4;
-            const size_t whole = (total - a16) >> 4;
-            const int w = (size_t)nch <= whole ? nch : (int)whole;
-            cut = w < nch ? w : -1;
+            const int y = row_of(in, ybase + k * S);
+            const uint32_t a = (uint32_t)y * row_bytes + col_off;
+            a16 = a & ~15u;
+            shift = (int)(a & 15u);
+            const int nch = (int)((a & 15u) + nbytes + 15u) >> 4;
+            int w = nch;
+            if (a16 + (uint32_t)nch * 16u > total) {
+                w = (int)((total - a16) >> 4);
+                cut = w;
+            }
             if (w > 0) {
                 lb = a16;
                 lim = w;
@@ -595,39 +644,37 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
     // span offset, cut chunk and shift.
     struct Group {
         u32x4 p0, p1, p2, p3;
-        size_t a0, a1;
+        uint32_t a0, a1;
         int c0, c1, s0, s1;
     };
     auto fetch = [&](int g) {
         Group G;
         const int k0 = g * RP;
-        size_t b0 = 0, b1 = 0;
+        uint32_t b0 = 0, b1 = 0;
         int l0 = 1, l1 = 1;
         G.a1 = 0;
         G.c1 = -1;
         G.s1 = 0;
         row_fetch(k0, G.a0, b0, l0, G.c0, G.s0);
         if (two) row_fetch(k0 + 1, G.a1, b1, l1, G.c1, G.s1);
-        const unsigned char* r0 = base + b0;
-        const unsigned char* r1 = base + b1;
         if (two) {
-            G.p0 = gmem<u32x4>(r0 + (uint32_t)min(lane, l0 - 1) * 16u)[0];
-            G.p1 = gmem<u32x4>(r0 + (uint32_t)min(lane + 64, l0 - 1) * 16u)[0];
-            G.p2 = gmem<u32x4>(r1 + (uint32_t)min(lane, l1 - 1) * 16u)[0];
-            G.p3 = gmem<u32x4>(r1 + (uint32_t)min(lane + 64, l1 - 1) * 16u)[0];
+            G.p0 = gmem<u32x4>(base + b0 + (uint32_t)min(lane, l0 - 1) * 16u)[0];
+            G.p1 = gmem<u32x4>(base + b0 + (uint32_t)min(lane + 64, l0 - 1) * 16u)[0];
+            G.p2 = gmem<u32x4>(base + b1 + (uint32_t)min(lane, l1 - 1) * 16u)[0];
+            G.p3 = gmem<u32x4>(base + b1 + (uint32_t)min(lane + 64, l1 - 1) * 16u)[0];
         } else {
-            G.p0 = gmem<u32x4>(r0 + (uint32_t)min(lane, l0 - 1) * 16u)[0];
-            G.p1 = gmem<u32x4>(r0 + (uint32_t)min(lane + 64, l0 - 1) * 16u)[0];
-            G.p2 = gmem<u32x4>(r0 + (uint32_t)min(lane + 128, l0 - 1) * 16u)[0];
-            G.p3 = gmem<u32x4>(r0 + (uint32_t)min(lane + 192, l0 - 1) * 16u)[0];
+            G.p0 = gmem<u32x4>(base + b0 + (uint32_t)min(lane, l0 - 1) * 16u)[0];
+            G.p1 = gmem<u32x4>(base + b0 + (uint32_t)min(lane + 64, l0 - 1) * 16u)[0];
+            G.p2 = gmem<u32x4>(base + b0 + (uint32_t)min(lane + 128, l0 - 1) * 16u)[0];
+            G.p3 = gmem<u32x4>(base + b0 + (uint32_t)min(lane + 192, l0 - 1) * 16u)[0];
         }
         return G;
     };
     // The cut chunk c of a row whose slot starts at LDS chunk slot0: its
     // bytes before the end of the buffer.
-    auto fix_cut = [&](size_t a16, int c, int slot0) {
+    auto fix_cut = [&](uint32_t a16, int c, int slot0) {
         if (lane != (c & 63)) return;  // the lane that stored that slot
-        const size_t off = a16 + (size_t)c * 16;
+        const uint32_t off = a16 + (uint32_t)c * 16u;
         unsigned char* d = rbuf + (slot0 + c) * 16;
         for (int q = 0; q < 16; ++q) d[q] = off + q < total ? gmem<unsigned char>(base)[off + q] : 0;
     };
@@ -649,13 +696,26 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
 #pragma unroll
     for (int b = 0; b < B; ++b) acc[b] = 0;
     int krow = 0, orow = 0;  // source rows into the current output row; output rows done
+    auto store = [&](int oy) {
+        unsigned char* o = static_cast<unsigned char*>(out.ptr) + ((size_t)oy * out.w + col0 + lane) * B;
+        uint32_t v[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) v[b] = round_mul(acc[b], rv_add, rv_mul, rv_shift);
+        if ((B == 2 || B == 4) && job.unpremult) {
+#pragma unroll
+            for (int b = 0; b < B - 1; ++b) v[b] = unpremult_u8(v[b], v[B - 1]);
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) o[b] = (unsigned char)v[b];
+    };
     auto row_sum = [&](int r, int shift) {
         const unsigned char* slot = rbuf + (r << lsl) * 16;
         uint32_t sum[B];
 #pragma unroll
         for (int b = 0; b < B; ++b) sum[b] = 0;
         if (fast) {
-            window_sums_any<B>(slot, shift + lane_byte, wbytes, wnd, wlast, sum);
+            if constexpr (ND == 0) window_sums<B>(slot, shift + lane_byte, wbytes, sum);
+            else window_sums_masked<B, ND>(slot, shift + lane_byte, tail, sum);
         } else if (fast_pm4) {
             if constexpr (B == 4) premult_window_sums4(slot, shift + lane_byte, fh, sum);
         } else if (lane < ncols) {
@@ -673,21 +733,9 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
             }
         }
 #pragma unroll
-        for (int b = 0; b < B; ++b) acc[b] += shrink_round_h(sum[b], (uint32_t)fh, B, mh);
+        for (int b = 0; b < B; ++b) acc[b] += round_mul(sum[b], rh_add, rh_mul, rh_shift);
         if (S == 1 && ++krow == fv) {
-            if (lane < ncols) {
-                unsigned char* o = static_cast<unsigned char*>(out.ptr) +
-                                   ((size_t)(oyi0 + orow) * out.w + col0 + lane) * B;
-                uint32_t v[B];
-#pragma unroll
-                for (int b = 0; b < B; ++b) v[b] = shrink_round_v(acc[b], (uint32_t)fv, mv);
-                if ((B == 2 || B == 4) && job.unpremult) {
-#pragma unroll
-                    for (int b = 0; b < B - 1; ++b) v[b] = unpremult_u8(v[b], v[B - 1]);
-                }
-#pragma unroll
-                for (int b = 0; b < B; ++b) o[b] = (unsigned char)v[b];
-            }
+            if (lane < ncols) store(oyi0 + orow);
 #pragma unroll
             for (int b = 0; b < B; ++b) acc[b] = 0;
             krow = 0;
@@ -695,10 +743,6 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
         }
     };
     const int ng = (nsrc + RP - 1) / RP;
-    auto rows = [&](int g, int s0, int s1) {
-        row_sum(0, s0);
-        if (two && g * RP + 1 < nsrc) row_sum(1, s1);
-    };
     Group GA;
     if (ng > 0) GA = fetch(0);
     for (int g = 0; g < ng; ++g) {
@@ -706,7 +750,8 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
         wave_lds_sync();
         const int a0 = GA.s0, a1 = GA.s1;
         if (g + 1 < ng) GA = fetch(g + 1);
-        rows(g, a0, a1);
+        row_sum(0, a0);
+        if (two && g * RP + 1 < nsrc) row_sum(1, a1);
         wave_lds_sync();
     }
     if (S > 1) {
@@ -722,17 +767,31 @@ __device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, i
 #pragma unroll
                 for (int b = 0; b < B; ++b) acc[b] += part_sums[((wave + q) * 64 + lane) * B + b];
             }
-            unsigned char* o = static_cast<unsigned char*>(out.ptr) + ((size_t)oyi0 * out.w + col0 + lane) * B;
-            uint32_t v[B];
-#pragma unroll
-            for (int b = 0; b < B; ++b) v[b] = shrink_round_v(acc[b], (uint32_t)fv, mv);
-            if ((B == 2 || B == 4) && job.unpremult) {
-#pragma unroll
-                for (int b = 0; b < B - 1; ++b) v[b] = unpremult_u8(v[b], v[B - 1]);
-            }
-#pragma unroll
-            for (int b = 0; b < B; ++b) o[b] = (unsigned char)v[b];
+            store(oyi0);
         }
+    }
+}
+
+// The lean form with its window length fixed once per wavefront: two-row
+// groups (lsl = 7, 64 columns) have windows of 4..8 dwords, one-row groups
+// (lsl = 8) of 8..16 dwords with 64 columns and more with fewer columns
+// (wnd_t = 0: looped sums); 4-band premultiplied windows take their own path.
+template <int B, bool TWO>
+__device__ __forceinline__ void shrink_rows_lean(const ShrinkJob& job, int bx, int by, unsigned char* lds) {
+    if constexpr (TWO) {
+        if (job.wnd_t == 4) shrink_rows_lean_nd<B, true, 4>(job, bx, by, lds);
+        else if (job.wnd_t == 8) shrink_rows_lean_nd<B, true, 8>(job, bx, by, lds);
+        else shrink_rows_lean_nd<B, true, 0>(job, bx, by, lds);
+    } else if constexpr (B == 4) {
+        if (job.premult || job.wnd_t == 0) shrink_rows_lean_nd<B, false, 0>(job, bx, by, lds);
+        else if (job.wnd_t == 16) shrink_rows_lean_nd<B, false, 16>(job, bx, by, lds);
+        else if (job.wnd_t == 12) shrink_rows_lean_nd<B, false, 12>(job, bx, by, lds);
+        else shrink_rows_lean_nd<B, false, 8>(job, bx, by, lds);
+    } else {
+        if (job.wnd_t == 16) shrink_rows_lean_nd<B, false, 16>(job, bx, by, lds);
+        else if (job.wnd_t == 12) shrink_rows_lean_nd<B, false, 12>(job, bx, by, lds);
+        else if (job.wnd_t == 8) shrink_rows_lean_nd<B, false, 8>(job, bx, by, lds);
+        else shrink_rows_lean_nd<B, false, 0>(job, bx, by, lds);
     }
 }
 
@@ -1297,6 +1356,28 @@ bool plan_shrink_job(ShrinkJob* job, bool row_split, int run) {
     job->lsl = lsl;
     job->mh = (uint32_t)((1ull << 32) / (256ull * (uint64_t)fh));
     job->mv = (uint32_t)((1ull << 32) / (256ull * (uint64_t)fv));
+    // ((sum + add) * mul) >> shift forms of the roundings (shrinkh.rs:381-471,
+    // shrinkv.rs:206-339): identity at f = 1, the pair rounding at f = 2, the
+    // exact x86 division by 5 for 1, 3 or 4 bands (ceil(2^24 / 5) is exact
+    // for every sum + 2 <= 1277), else the libvips multiply-shift.
+    auto rounding = [&](int f, bool horizontal, uint32_t* add, uint32_t* mul, uint32_t* shift) {
+        if (f == 1) {
+            *add = 0, *mul = 1, *shift = 0;
+        } else if (f == 2) {
+            *add = 1, *mul = 1, *shift = 1;
+        } else if (horizontal && f == 5 && bands != 2) {
+            *add = 2, *mul = (uint32_t)(((1u << 24) + 4) / 5), *shift = 24;
+        } else {
+            *add = (uint32_t)(f >> 1), *mul = (uint32_t)((1u << 24) / (uint32_t)f), *shift = 24;
+        }
+    };
+    rounding(fh, true, &job->rh_add, &job->rh_mul, &job->rh_shift);
+    rounding(fv, false, &job->rv_add, &job->rv_mul, &job->rv_shift);
+    // Lean forms: window length in dwords rounded up to a multiple of 4
+    // (0 beyond 16: looped sums); 32-bit buffer offsets need < 4 GiB.
+    const int wnd = (wb + 3) >> 2;
+    job->wnd_t = wnd <= 16 ? (wnd + 3) / 4 * 4 : 0;
+    job->lean = src_bytes + 8192 < (1ull << 32) ? 1 : 0;
     const int rows_per_block = ((kThreads / 64) / S) * job->m;
     job->gx = gx;
     job->gy = (job->out.h + rows_per_block - 1) / rows_per_block;
@@ -1304,6 +1385,7 @@ bool plan_shrink_job(ShrinkJob* job, bool row_split, int run) {
 }
 
 int shrink_job_variant(const ShrinkJob& job) {
+    if (!job.lean) return job.bands - 1;
     return job.bands - 1 + (job.lsl >= 8 ? 4 : (job.lsl == 7 ? 8 : 0));
 }
 

@@ -76,6 +76,12 @@ struct ShrinkJob {
     int tx = 64, S = 1;
     int m = 1, lsl = 8;  // output rows per wavefront run; log2 chunks per source-row slot
     uint32_t mh = 0, mv = 0;
+    // ShrinkH / ShrinkV roundings as ((sum + add) * mul) >> shift (every
+    // u8 rounding form of the reference fits: pair, exact /5, libvips
+    // multiply-shift; mul < 2^24 and the product < 2^32).
+    uint32_t rh_add = 0, rh_mul = 1, rh_shift = 0, rv_add = 0, rv_mul = 1, rv_shift = 0;
+    int wnd_t = 16;  // lean forms: window dwords rounded up to 4, 8, 12 or 16 (0: longer)
+    int lean = 1;    // lean forms allowed (32-bit buffer offsets: source < 4 GiB)
     int gx = 0, gy = 0;
     int src_slot = -1, dst_slot = -1;  // batch: pointer taken from the per-run tables
 };
