@@ -68,7 +68,7 @@ def test_float_reduce_against_fp64(ctx, mode, bands):
 def test_float_gauss_against_fp64(ctx, mode, sigma):
     src = float_image(211, 307, 3, 40)
     ref = O.f64_gauss(src, sigma)
-    kernel = "reduce_f32" if mode == v.Pipeline.FLOAT_FP32 else "sep_mfma_f16x3"
+    kernel = "conv_pair_f32" if mode == v.Pipeline.FLOAT_FP32 else "sep_mfma_f16x3"
     fast = run_mode(ctx, src, [("gauss_blur", (sigma,))], mode, kernel)
     exact = run_mode(ctx, src, [("gauss_blur", (sigma,))], 0)
     e_abs, e_rel = errors(fast, ref)

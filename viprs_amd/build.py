@@ -18,8 +18,8 @@ CSRC = HERE / "csrc"
 LIB = HERE / "libviprs_amd.so"
 INCLUDE = HERE.parent / "include"
 
-SOURCES = ["kernels.hip", "sep_mfma.hip", "float_paths.hip", "gauss_u8.hip", "affine.hip", "plan.cpp", "runtime.cpp", "batch.cpp",
-           "server.cpp"]
+SOURCES = ["kernels.hip", "sep_mfma.hip", "float_paths.hip", "gauss_u8.hip", "affine.hip", "conv_pair.hip", "plan.cpp",
+           "runtime.cpp", "batch.cpp", "server.cpp"]
 
 # -ffp-contract=off: the reference-exact f32 kernels and the host planner rely
 # on every a*b+c the reference writes without mul_add staying two roundings.

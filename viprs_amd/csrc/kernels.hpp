@@ -215,6 +215,24 @@ struct GaussJob {
 };
 bool gauss_u8_supported(int bands, int taps, int32_t scale);
 hipError_t launch_gauss_u8(const GaussJob& job, hipStream_t s);
+// Fused float H -> V convolution pair (conv_pair.hip): a strip of tw output
+// columns (tw * bands <= 256) walking down runs of `run` output rows; the
+// masks have 2 * rad + 1 taps; kind 0 = Gaussian (FP64, no FMA), 1 = ConvSep
+// (FP64 FMA chains, / scale + offset), 2 = Gaussian in plain FP32.
+constexpr int kConvPairThreads = 256;  // threads per block = samples per strip row, at most
+struct ConvPairJob {
+    InView in;
+    OutView out;
+    int bands = 1, tw = 64, rad = 0, run = 64;
+    const double* wh = nullptr;
+    const double* wv = nullptr;
+    const float* wh32 = nullptr;
+    const float* wv32 = nullptr;
+    double hscale = 1.0, hoffset = 0.0, vscale = 1.0, voffset = 0.0;
+};
+bool conv_pair_supported(int bands, int rad);
+size_t conv_pair_lds(const ConvPairJob& job);
+hipError_t launch_conv_pair(const ConvPairJob& job, int in_fmt, int kind, hipStream_t s);
 hipError_t launch_conv(const InView& in, const OutView& out, int bands, int in_fmt, int out_fmt,
                        int kind, int vertical, const int16_t* itaps, int32_t iscale,
                        const double* wts, int ntaps, double cscale, double coffset,

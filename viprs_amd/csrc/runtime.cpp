@@ -831,6 +831,43 @@ bool stage_shrink_sep_params(int fh, int fv, uint32_t bands, const Rect& mid, co
     return true;
 }
 
+// The fused float pair: both masks odd and of one length, the strip and its
+// H ring within the kernel's limits; rows per block run chosen so that the
+// launch has about 1000 blocks or more (each run recomputes 2r halo H rows).
+bool stage_conv_pair_params(const Node& nh, const Node& nv, int kind, const Rect& out, const StageOptions& opt,
+                            Staging* st, Stage* s) {
+    (void)opt;
+    const size_t n = nh.conv.size();
+    if (n == 0 || n % 2 == 0 || nv.conv.size() != n) return false;
+    const int rad = (int)(n / 2);
+    const int B = (int)nh.bands;
+    if (!conv_pair_supported(B, rad)) return false;
+    ConvPairJob& j = s->cpair;
+    j.bands = B;
+    j.tw = kConvPairThreads / B;
+    j.rad = rad;
+    const int64_t strips = (out.w + j.tw - 1) / j.tw;
+    int run = 256;
+    while (run > 32 && strips * ((out.h + run - 1) / run) < 1024) run /= 2;
+    j.run = run;
+    j.hscale = nh.cscale;
+    j.hoffset = nh.coffset;
+    j.vscale = nv.cscale;
+    j.voffset = nv.coffset;
+    s->kind = ST_CONV_PAIR;
+    s->cpair_kind = kind;
+    s->cp_wh = st->put(nh.conv.data(), n * sizeof(double));
+    s->cp_wv = st->put(nv.conv.data(), n * sizeof(double));
+    std::vector<float> h32(n), v32(n);
+    for (size_t t = 0; t < n; ++t) {
+        h32[t] = (float)nh.conv[t];
+        v32[t] = (float)nv.conv[t];
+    }
+    s->cp_wh32 = st->put(h32.data(), n * sizeof(float));
+    s->cp_wv32 = st->put(v32.data(), n * sizeof(float));
+    return true;
+}
+
 void stage_conv_params(const Node& nd, Staging* st, Stage* s) {
     s->itaps = st->put(nd.itaps.data(), nd.itaps.size() * sizeof(int16_t));
     s->wts = st->put(nd.conv.data(), nd.conv.size() * sizeof(double));
@@ -952,6 +989,17 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
                         reinterpret_cast<const int16_t*>(blob + s.itaps), nd.iscale,
                         reinterpret_cast<const double*>(blob + s.wts), s.taps, nd.cscale,
                         nd.coffset, stream);
+        break;
+    }
+    case ST_CONV_PAIR: {
+        ConvPairJob j = s.cpair;
+        j.in = in;
+        j.out = out;
+        j.wh = reinterpret_cast<const double*>(blob + s.cp_wh);
+        j.wv = reinterpret_cast<const double*>(blob + s.cp_wv);
+        j.wh32 = reinterpret_cast<const float*>(blob + s.cp_wh32);
+        j.wv32 = reinterpret_cast<const float*>(blob + s.cp_wv32);
+        e = launch_conv_pair(j, s.in_fmt, s.cpair_kind, stream);
         break;
     }
     case ST_ALPHA:
@@ -1117,6 +1165,21 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
             // GaussBlurH -> GaussBlurV (u8 -> u8) on the MFMA kernel.
             if (stage_gauss_sep_params(nd, nodes[(size_t)i + 1], out_r[(size_t)i], out_r[(size_t)i + 1],
                                        in_r[(size_t)i + 1], s.clip, buf, opt, st, &s)) {
+                s.last = i + 1;
+                done = true;
+            }
+        }
+        if (!done && small_bands && i + 1 < n) {
+            // GaussBlurH -> GaussBlurV on the float path (reference order, or
+            // plain FP32 in float mode 1) or ConvSepH -> ConvSepV: one fused
+            // strip kernel.
+            const Node& nv = nodes[(size_t)i + 1];
+            int kind = -1;
+            if (nd.kind == N_GAUSS_H && nv.kind == N_GAUSS_V && nd.in_fmt != U8 && nd.out_fmt == F32 &&
+                nv.out_fmt == F32 && opt.float_mode <= 1)
+                kind = opt.float_mode == 1 ? 2 : 0;
+            if (nd.kind == N_CONVSEP_H && nv.kind == N_CONVSEP_V) kind = 1;
+            if (kind >= 0 && stage_conv_pair_params(nd, nv, kind, out_r[(size_t)i + 1], opt, st, &s)) {
                 s.last = i + 1;
                 done = true;
             }
@@ -1788,6 +1851,7 @@ const char* stage_kernel_name(const Stage& s) {
     case ST_AFFINE: return "affine";
     case ST_ZOOM: return "zoom";
     case ST_PLACE: return "place";
+    case ST_CONV_PAIR: return s.cpair_kind == 2 ? "conv_pair_f32" : "conv_pair_f64";
     }
     return "unknown";
 }

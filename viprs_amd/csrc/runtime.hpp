@@ -138,6 +138,7 @@ enum StageKind {
     ST_AFFINE,   // the thumbnail / resize Affine tail
     ST_ZOOM,
     ST_PLACE,    // copy a tile-column run's output into its columns of the destination
+    ST_CONV_PAIR, // fused float GaussBlurH -> GaussBlurV or ConvSepH -> ConvSepV
 };
 
 struct Stage {
@@ -167,6 +168,9 @@ struct Stage {
     // destination rows are place_w pixels wide).
     bool from_source = false, to_dst = false;
     int64_t place_w = 0;
+    ConvPairJob cpair;          // ST_CONV_PAIR: job template (views and weights set at launch)
+    int cpair_kind = 0;         // 0 Gaussian FP64, 1 ConvSep FP64, 2 Gaussian FP32
+    size_t cp_wh = 0, cp_wv = 0, cp_wh32 = 0, cp_wv32 = 0;  // ST_CONV_PAIR blob offsets
     F16Job f16;                 // ST_SEP_F16: job template (table offsets below)
     GaussJob gauss;             // ST_GAUSS_U8: job template (views set at launch)
 };
