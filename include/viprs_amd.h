@@ -175,6 +175,24 @@ int vgpu_pipeline_thumbnail_crop(vgpu_pipeline *p, int target, uint32_t width, u
 int vgpu_thumbnail_shrink_on_load_factor(uint32_t width, uint32_t height, uint32_t bands, int target,
                                          uint32_t target_width, uint32_t target_height, int crop,
                                          uint32_t *factor);
+/* The PNG source's box pre-shrink on the device, as the pipeline's first
+ * step (the decoded image stays in HBM; the chain reads its shrunk copy):
+ * ceil_mode == 0 is software_box_shrink_generic (sources/decoder_source/
+ * shrink.rs:47-117), what DecoderSource::set_thumbnail_shrink_on_load does to
+ * an eager backing (source_impl.rs:245-275): output max(src / f, 1) per axis,
+ * u8 rounds (sum + n/2) / n, other formats from_f64_clamped(sum / n);
+ * ceil_mode == 1 (u8) is decode_png_with_box_shrink_u8 (codecs/src/png/
+ * decode_full.rs:320-452): output ceil(src / f), the last column and row
+ * divided by their partial windows' pixel counts.  Only before any other
+ * node (VGPU_E_INVALID_ARGUMENT otherwise). */
+int vgpu_pipeline_box_shrink(vgpu_pipeline *p, uint32_t factor, int ceil_mode);
+/* PipelineBuilder::thumbnail over a PNG source with an eager backing
+ * (builder/resample.rs:871-906): when the thumbnail is the first node and
+ * the plan's shrink-on-load hint is set, vgpu_pipeline_box_shrink(hint, 0)
+ * replaces the source, and the thumbnail is planned without the hint over
+ * the shrunk dimensions (and is "large", for its tiling, by those). */
+int vgpu_pipeline_thumbnail_shrink_on_load(vgpu_pipeline *p, int target, uint32_t width, uint32_t height,
+                                           int kernel, int crop);
 /* PipelineBuilder::resize(Resize::new(hscale, vscale, kernel))
  * (builder/resample.rs:637-711; resize.rs:163-284): shrink and reduce for
  * downscales, Zoom and the Affine fix-up for upscales.  The LBB and VSQBS

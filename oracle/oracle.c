@@ -389,6 +389,12 @@ struct orc_pipeline {
     int n;
     node_t nodes[32];
     int thin_override; /* the ThinStrip override of large thumbnails (builder/resample.rs:908-917) */
+    /* A box pre-shrink of the source (orc_pipeline_box_shrink): the decoded
+     * image is orig_w x orig_h, the node chain reads its shrunk copy, whose
+     * size is src_w x src_h. */
+    uint32_t box_f;
+    int box_ceil;
+    uint32_t orig_w, orig_h;
 };
 
 static size_t fmt_size(int f) { return f == ORC_U8 ? 1 : (f == ORC_U16 ? 2 : 4); }
@@ -396,8 +402,8 @@ static size_t fmt_size(int f) { return f == ORC_U8 ? 1 : (f == ORC_U16 ? 2 : 4);
 orc_pipeline *orc_pipeline_new(uint32_t w, uint32_t h, uint32_t bands, int fmt) {
     if (bands == 0 || fmt < ORC_U8 || fmt > ORC_F32) return NULL;
     orc_pipeline *p = (orc_pipeline *)calloc(1, sizeof(orc_pipeline));
-    p->src_w = p->cur_w = w;
-    p->src_h = p->cur_h = h;
+    p->src_w = p->cur_w = p->orig_w = w;
+    p->src_h = p->cur_h = p->orig_h = h;
     p->bands = bands;
     p->src_fmt = p->cur_fmt = fmt;
     return p;
@@ -916,6 +922,103 @@ int orc_pipeline_thumbnail_crop(orc_pipeline *p, int target, uint32_t tw, uint32
 
 int orc_pipeline_thumbnail(orc_pipeline *p, int target, uint32_t tw, uint32_t th, int kernel) {
     return orc_pipeline_thumbnail_crop(p, target, tw, th, kernel, 0);
+}
+
+/* The source-side box shrinks of the PNG thumbnail path.
+ *
+ * ceil_mode == 0: software_box_shrink_generic (sources/decoder_source/
+ * shrink.rs:47-117), the shrink applied to an eager decoded backing:
+ * dst = max(src / f, 1) per axis; window [d*f, min(d*f + f, src)) (cut short
+ * only when the source is smaller than f); u8 sums in integers and rounds
+ * (sum + n/2) / n with n = the window's pixel count; the other formats sum in
+ * f64 in row order (sy outer, sx inner) and store from_f64_clamped(sum / n)
+ * (shrinkh.rs:49-128: u16 rounds half away from zero, f32 casts).
+ *
+ * ceil_mode == 1 (u8 only): decode_png_with_box_shrink_u8 (codecs/src/png/
+ * decode_full.rs:320-452), the shrink applied row by row while a PNG file
+ * decodes: dst = ceil(src / f); the last column and row take the partial
+ * windows, each divided by its own pixel count with the same rounding. */
+int orc_box_shrink(const void *src, uint32_t w, uint32_t h, uint32_t bands, int fmt, uint32_t f, int ceil_mode,
+                   void *dst, uint32_t *out_w, uint32_t *out_h) {
+    if (f == 0 || bands == 0 || fmt < ORC_U8 || fmt > ORC_F32) return ORC_EINVAL;
+    if (ceil_mode && fmt != ORC_U8) return ORC_EFORMAT;
+    uint32_t dw, dh;
+    if (ceil_mode) {
+        dw = (w + f - 1) / f;
+        dh = (h + f - 1) / f;
+    } else {
+        dw = w / f > 1 ? w / f : 1;
+        dh = h / f > 1 ? h / f : 1;
+    }
+    if (w == 0 || h == 0) dw = dh = 0;
+    if (out_w) *out_w = dw;
+    if (out_h) *out_h = dh;
+    if (!dst) return ORC_OK;
+    const size_t B = bands;
+    for (uint32_t dy = 0; dy < dh; ++dy)
+        for (uint32_t dx = 0; dx < dw; ++dx) {
+            const size_t sy0 = (size_t)dy * f, sx0 = (size_t)dx * f;
+            const size_t sy1 = sy0 + f < h ? sy0 + f : h, sx1 = sx0 + f < w ? sx0 + f : w;
+            size_t n = (sy1 - sy0) * (sx1 - sx0);
+            if (n == 0) n = 1;
+            const size_t o = ((size_t)dy * dw + dx) * B;
+            for (size_t b = 0; b < B; ++b) {
+                if (fmt == ORC_U8) {
+                    const uint8_t *s = (const uint8_t *)src;
+                    uint32_t sum = 0;
+                    for (size_t sy = sy0; sy < sy1; ++sy)
+                        for (size_t sx = sx0; sx < sx1; ++sx) sum += s[(sy * w + sx) * B + b];
+                    ((uint8_t *)dst)[o + b] = (uint8_t)((sum + (uint32_t)n / 2) / (uint32_t)n);
+                } else {
+                    double sum = 0.0;
+                    for (size_t sy = sy0; sy < sy1; ++sy)
+                        for (size_t sx = sx0; sx < sx1; ++sx)
+                            sum += fmt == ORC_U16 ? (double)((const uint16_t *)src)[(sy * w + sx) * B + b]
+                                                  : (double)((const float *)src)[(sy * w + sx) * B + b];
+                    const double v = sum / (double)n;
+                    if (fmt == ORC_U16)
+                        ((uint16_t *)dst)[o + b] = (uint16_t)fmin(fmax(round(v), 0.0), 65535.0);
+                    else
+                        ((float *)dst)[o + b] = (float)v;
+                }
+            }
+        }
+    return ORC_OK;
+}
+
+/* The source replaced by its box-shrunk copy before any node is added
+ * (DecoderSource::set_thumbnail_shrink_on_load, sources/decoder_source/
+ * source_impl.rs:245-275): the pipeline then sees the shrunk dimensions. */
+int orc_pipeline_box_shrink(orc_pipeline *p, uint32_t factor, int ceil_mode) {
+    if (!p || factor == 0) return ORC_EINVAL;
+    if (p->n != 0 || p->box_f != 0) return ORC_EINVAL;
+    uint32_t dw, dh;
+    int rc = orc_box_shrink(NULL, p->src_w, p->src_h, p->bands, p->src_fmt, factor, ceil_mode, NULL, &dw, &dh);
+    if (rc != ORC_OK) return rc;
+    p->box_f = factor;
+    p->box_ceil = ceil_mode;
+    p->src_w = p->cur_w = dw;
+    p->src_h = p->cur_h = dh;
+    return ORC_OK;
+}
+
+/* PipelineBuilder::thumbnail over a source with the PNG SoftwareBoxShrink
+ * pre-shrink (builder/resample.rs:871-906; decoder_source/shrink.rs:5-30,
+ * 119-125): when the thumbnail is the first node and the plan's shrink-on-load
+ * hint is set, the eager backing is box-shrunk by the hint (2, 4 or 16: all
+ * kept by normalize_shrink_factor) and the thumbnail is planned again, without
+ * the hint, over the shrunk dimensions. */
+int orc_pipeline_thumbnail_shrink_on_load(orc_pipeline *p, int target, uint32_t tw, uint32_t th, int kernel,
+                                          int crop) {
+    if (!p) return ORC_EINVAL;
+    if (p->n == 0 && p->box_f == 0) {
+        const uint32_t f = orc_thumbnail_shrink_on_load(p->cur_w, p->cur_h, target, tw, th, crop);
+        if (f == 2 || f == 4 || f == 8 || f == 16) {
+            const int rc = orc_pipeline_box_shrink(p, f, 0);
+            if (rc != ORC_OK) return rc;
+        }
+    }
+    return orc_pipeline_thumbnail_crop(p, target, tw, th, kernel, crop);
 }
 
 /* ResizeOp::into_pipeline_nodes (resize.rs:163-284) without the LBB / VSQBS
@@ -1625,6 +1728,17 @@ static void *worker(void *arg) {
 int orc_pipeline_run(const orc_pipeline *p, const void *src, void *dst, int threads,
                      uint32_t strip_height) {
     if (p->cur_w == 0 || p->cur_h == 0) return ORC_OK;
+    if (p->box_f) {
+        /* The node chain runs over the box-shrunk source. */
+        void *shrunk = malloc((size_t)p->src_w * p->src_h * p->bands * fmt_size(p->src_fmt));
+        if (!shrunk) return ORC_EINVAL;
+        orc_box_shrink(src, p->orig_w, p->orig_h, p->bands, p->src_fmt, p->box_f, p->box_ceil, shrunk, NULL, NULL);
+        orc_pipeline bare = *p;
+        bare.box_f = 0;
+        const int rc = orc_pipeline_run(&bare, shrunk, dst, threads, strip_height);
+        free(shrunk);
+        return rc;
+    }
     if (p->n == 0) {
         memcpy(dst, src, (size_t)p->src_w * p->src_h * p->bands * fmt_size(p->src_fmt));
         return ORC_OK;

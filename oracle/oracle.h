@@ -81,6 +81,14 @@ int orc_pipeline_premultiply(orc_pipeline *p);
 int orc_pipeline_unpremultiply(orc_pipeline *p);
 int orc_pipeline_thumbnail(orc_pipeline *p, int target, uint32_t tw, uint32_t th, int kernel);
 int orc_pipeline_thumbnail_crop(orc_pipeline *p, int target, uint32_t tw, uint32_t th, int kernel, int crop);
+/* PNG box pre-shrinks: software_box_shrink_generic (ceil_mode 0) and
+ * decode_png_with_box_shrink_u8 (ceil_mode 1); dst may be NULL to ask for the
+ * output size only. */
+int orc_box_shrink(const void *src, uint32_t w, uint32_t h, uint32_t bands, int fmt, uint32_t factor,
+                   int ceil_mode, void *dst, uint32_t *out_w, uint32_t *out_h);
+int orc_pipeline_box_shrink(orc_pipeline *p, uint32_t factor, int ceil_mode);
+int orc_pipeline_thumbnail_shrink_on_load(orc_pipeline *p, int target, uint32_t tw, uint32_t th, int kernel,
+                                          int crop);
 /* The thumbnail's Affine node: matrix [a, 0, 0, d], translation (tx, ty),
  * ExtendMode::Copy, kernel Nearest / Bilinear / Bicubic; small_tile gives it
  * the SmallTile demand (128 x 128 scheduler tiles) instead of ThinStrip. */

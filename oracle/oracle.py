@@ -64,6 +64,9 @@ def lib() -> C.CDLL:
         L.orc_pipeline_unpremultiply.argtypes = [vp]
         L.orc_pipeline_thumbnail.argtypes = [vp, i32, u32, u32, i32]
         L.orc_pipeline_thumbnail_crop.argtypes = [vp, i32, u32, u32, i32, i32]
+        L.orc_box_shrink.argtypes = [vp, u32, u32, u32, i32, u32, i32, vp, vp, vp]
+        L.orc_pipeline_box_shrink.argtypes = [vp, u32, i32]
+        L.orc_pipeline_thumbnail_shrink_on_load.argtypes = [vp, i32, u32, u32, i32, i32]
         L.orc_pipeline_extract_area.argtypes = [vp, u32, u32, u32, u32]
         L.orc_pipeline_affine.argtypes = [vp, dbl, dbl, dbl, dbl, u32, u32, i32, i32]
         L.orc_shrink_on_load_factor.argtypes = [dbl]
@@ -190,6 +193,19 @@ class Pipeline:
         _check(lib().orc_pipeline_thumbnail_crop(self._p, target, width, height, kernel, int(crop)), "thumbnail")
         return self
 
+    def box_shrink(self, factor, ceil=False):
+        """The source replaced by its box-shrunk copy (the PNG pre-shrinks,
+        decoder_source/shrink.rs:47-117 and png/decode_full.rs:320-452)."""
+        _check(lib().orc_pipeline_box_shrink(self._p, factor, int(ceil)), "box_shrink")
+        return self
+
+    def thumbnail_shrink_on_load(self, target, width, height=0, kernel=LANCZOS3, crop=False):
+        """thumbnail() over a PNG source that box-shrinks on load
+        (builder/resample.rs:871-906)."""
+        _check(lib().orc_pipeline_thumbnail_shrink_on_load(self._p, target, width, height, kernel, int(crop)),
+               "thumbnail_shrink_on_load")
+        return self
+
     def extract_area(self, x, y, width, height):
         """PipelineBuilder::extract_area (builder/geometry.rs:40-80)."""
         _check(lib().orc_pipeline_extract_area(self._p, x, y, width, height), "extract_area")
@@ -228,6 +244,19 @@ class Pipeline:
             threads = os.cpu_count() or 1
         _check(lib().orc_pipeline_run(self._p, _ptr(src), _ptr(dst), threads, strip_height), "run")
         return dst
+
+
+def box_shrink(src: np.ndarray, factor: int, ceil: bool = False) -> np.ndarray:
+    """The PNG box pre-shrink of one image (orc_box_shrink)."""
+    h, w, b = src.shape
+    fmt = FMT_OF[src.dtype]
+    src = np.ascontiguousarray(src)
+    ow, oh = C.c_uint32(), C.c_uint32()
+    _check(lib().orc_box_shrink(None, w, h, b, fmt, factor, int(ceil), None, C.addressof(ow), C.addressof(oh)),
+           "box_shrink")
+    dst = np.zeros((oh.value, ow.value, b), src.dtype)
+    _check(lib().orc_box_shrink(_ptr(src), w, h, b, fmt, factor, int(ceil), _ptr(dst), None, None), "box_shrink")
+    return dst
 
 
 def thumbnail_plan(in_w, in_h, bands, target, tw, th=0, kernel=LANCZOS3, crop=False):

@@ -3,7 +3,8 @@ oracle (integer images bit for bit, float images bit for bit in the
 reference's FP64 order and within stated bounds of an FP64 evaluation in the
 fast modes):
 
-  * 8192x8192x3 uchar thumbnail Width(400), Lanczos3 (the headline);
+  * 8192x8192x3 uchar thumbnail Width(400), Lanczos3 (the headline), and the
+    same from a PNG source that box-shrinks on load;
   * 4096x4096x4 uchar gauss_blur(2.0), in both the VALU and the int8-limb
     MFMA kernel;
   * 4096x4096x3 float32 Lanczos3 reduce by 2.5 and gauss_blur(8.0), in the
@@ -50,6 +51,22 @@ def test_headline_thumbnail_8192x8192x3(ctx):
     p.run_noalloc(d_src, out2)
     ctx.synchronize()
     _same(out2.numpy(), exp, "thumbnail 8192^2 no-alloc")
+
+
+def test_png_thumbnail_box_shrink_8192x8192x3(ctx):
+    # The PNG source: box pre-shrink by the hint (16) on the device, then the
+    # thumbnail over the 512 x 512 image.
+    _say("PNG thumbnail 8192x8192x3 -> Width(400), box pre-shrink")
+    src = gradient_noise_u8(8192, 8192, 3, 77)
+    p = v.Pipeline(ctx, 8192, 8192, 3).thumbnail_shrink_on_load(v.WIDTH, 400)
+    assert p.stage_kernels()[0] == "box_shrink"
+    d_src = v.DeviceBuffer.from_numpy(ctx, src)
+    out = p.output_buffer()
+    p.run(d_src, out)
+    ctx.synchronize()
+    exp = O.Pipeline(8192, 8192, 3).thumbnail_shrink_on_load(O.WIDTH, 400).run(src)
+    assert exp.shape == (400, 400, 3)
+    _same(out.numpy(), exp, "PNG thumbnail 8192^2")
 
 
 @pytest.mark.parametrize("form", ["valu", "mfma"])

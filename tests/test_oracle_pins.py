@@ -438,3 +438,138 @@ def test_exact_band_catches_a_dropped_tap():
     vt = _sep_pass_exact(hz, w_bad, axis=0)
     with pytest.raises(AssertionError):
         _check_f32(out, list(vt.ravel()), "dropped tap")
+
+
+# --- PNG box pre-shrinks (decoder_source/shrink.rs:47-117; png/decode_full.rs:320-452)
+
+def _box_exact(src, f, ceil):
+    """Exact rational mean of every window, brute force: u8 / u16 rounded
+    half up (the means are non-negative), f32 the correctly rounded double of
+    the mean cast to f32 (what sum / n gives when every partial sum is a
+    double).  Windows are cut at the image edge; floor mode keeps at least
+    one output pixel per axis."""
+    h, w, b = src.shape
+    dw = -(-w // f) if ceil else max(w // f, 1)
+    dh = -(-h // f) if ceil else max(h // f, 1)
+    out = np.zeros((dh, dw, b), src.dtype)
+    for dy in range(dh):
+        for dx in range(dw):
+            win = src[dy * f:min(dy * f + f, h), dx * f:min(dx * f + f, w)]
+            n = win.shape[0] * win.shape[1]
+            for c in range(b):
+                m = sum((Fraction(float(v)) for v in win[:, :, c].ravel()), Fraction(0)) / n
+                if src.dtype == np.float32:
+                    out[dy, dx, c] = np.float32(float(m))
+                else:
+                    out[dy, dx, c] = int(m + Fraction(1, 2))  # floor(m + 1/2)
+    return out
+
+
+def test_box_shrink_png_tracking_decoder():
+    # decoder_source/tests.rs:389-453: an 8x8 one-band image of 128 shrinks
+    # to 2x2 at factor 4.
+    src = np.full((8, 8, 1), 128, np.uint8)
+    got = O.box_shrink(src, 4)
+    assert got.shape == (2, 2, 1) and (got == 128).all()
+    p = O.Pipeline(8, 8, 1).box_shrink(4)
+    assert p.output()[:2] == (2, 2)
+
+
+def _png_tail_pixels(w, h):
+    # png/tests.rs:606-647: the partial-tail test image
+    y, x = np.mgrid[0:h, 0:w].astype(np.uint32)
+    return np.stack([(x * 13 + y) & 255, (y * 7 + x * 3) & 255, x ^ ((y * 11) & 255)], -1).astype(np.uint8)
+
+
+def _png_ga_pixels(w, h):
+    # png/tests.rs:650-680: grey + alpha pairs
+    y, x = np.mgrid[0:h, 0:w].astype(np.int64)
+    return np.stack([(x * 17 + y * 9) & 255, (255 - ((x * 21 + y * 5) & 255)) & 255], -1).astype(np.uint8)
+
+
+def test_box_shrink_png_decode_fixtures():
+    # decode_path_with_shrink_keeps_partial_tail_blocks (17x18x3, f=16: 2x2
+    # output, partial last column and row) and ..._preserves_grayscale_alpha_pairs
+    # (4x4x2, f=2), both in the row decoder's ceil form.
+    tail = _png_tail_pixels(17, 18)
+    got = O.box_shrink(tail, 16, ceil=True)
+    assert got.shape == (2, 2, 3)
+    assert np.array_equal(got, _box_exact(tail, 16, True))
+    ga = _png_ga_pixels(4, 4)
+    assert np.array_equal(O.box_shrink(ga, 2, ceil=True), _box_exact(ga, 2, True))
+
+
+@pytest.mark.parametrize("f", [2, 3, 4, 8])
+def test_box_shrink_u8_ramp_closed_form(f):
+    # v = x + 2y + 3: a window's mean is v at its centre, x0 + (f-1)/2 and
+    # y0 + (f-1)/2, so an even f lands on a half (3/2 per axis step of 1/2):
+    # rounding half up gives ceil, a truncating divide would give floor.
+    w, h = 10 * f + f // 2, 7 * f + 1
+    y, x = np.mgrid[0:h, 0:w]
+    src = (x + 2 * y + 3).astype(np.uint8)[..., None]
+    got = O.box_shrink(src, f)
+    dy, dx = np.mgrid[0:h // f, 0:w // f]
+    centre = Fraction(f - 1, 2)
+    exp = np.vectorize(lambda a, b: int(a * f + centre + 2 * (b * f + centre) + 3 + Fraction(1, 2)))(dx, dy)
+    assert got.shape == (h // f, w // f, 1)
+    assert np.array_equal(got[..., 0], exp)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32])
+@pytest.mark.parametrize("f,ceil", [(2, False), (3, False), (5, False), (4, True), (3, True)])
+def test_box_shrink_brute_force(dtype, f, ceil):
+    if ceil and dtype != np.uint8:
+        pytest.skip("the row decoder's form is u8 only")
+    r = np.random.default_rng(f * 7 + int(ceil))
+    h, w, b = 13, 17, 3
+    if dtype == np.float32:
+        src = (r.integers(-512, 2048, (h, w, b)) / 1024).astype(np.float32)
+    else:
+        src = r.integers(0, np.iinfo(dtype).max + 1, (h, w, b), dtype=np.int64).astype(dtype)
+    assert np.array_equal(O.box_shrink(src, f, ceil), _box_exact(src, f, ceil))
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32])
+def test_box_shrink_window_cut_when_smaller_than_factor(dtype):
+    # shrink.rs:58-59,70-73: a 5x3 image at f = 4 gives one pixel, the mean
+    # of all 15.
+    src = (np.arange(30).reshape(3, 5, 2) * 7).astype(dtype)
+    got = O.box_shrink(src, 4)
+    assert got.shape == (1, 1, 2)
+    assert np.array_equal(got, _box_exact(src, 4, False))
+
+
+def test_box_shrink_catches_a_truncating_divide():
+    # A plausible mistake (sum / n without the + n/2) disagrees with the
+    # closed form on the ramp.
+    src = (np.arange(16, dtype=np.uint8) * 3).reshape(4, 4, 1)
+    good = _box_exact(src, 2, False)
+    bad = src.reshape(2, 2, 2, 2).astype(np.uint32).sum((1, 3)) // 4
+    assert not np.array_equal(good[..., 0], bad)
+    assert np.array_equal(O.box_shrink(src, 2), good)
+
+
+@pytest.mark.parametrize("w,h,target,tw,th,crop", [
+    (8192, 8192, O.WIDTH, 400, 0, False),    # hint 16: 512 x 512, then reduce 1.28
+    (4000, 3000, O.FITBOX, 300, 300, False),  # hint 4
+    (1200, 900, O.WIDTH, 250, 0, False),     # hint 2
+    (900, 700, O.WIDTH, 300, 0, False),      # no hint: the plain thumbnail
+    (3000, 2000, O.FITBOX, 200, 200, True),  # crop
+])
+def test_thumbnail_shrink_on_load_composes(w, h, target, tw, th, crop):
+    # builder/resample.rs:878-906: a box pre-shrink by the hint, then the
+    # thumbnail planned over the shrunk image.
+    bands = 3
+    f = O.lib().orc_thumbnail_shrink_on_load(w, h, target, tw, th, int(crop))
+    p = O.Pipeline(w, h, bands).thumbnail_shrink_on_load(target, tw, th, crop=crop)
+    if f:
+        sw, sh = max(w // f, 1), max(h // f, 1)
+    else:
+        sw, sh = w, h
+    q = O.Pipeline(sw, sh, bands).thumbnail(target, tw, th, crop=crop)
+    assert p.output() == q.output()
+    assert p.nodes() == q.nodes()
+    if w * h <= 4_000_000:
+        src = O.synthetic_bytes_fast(w * h * bands, w + h).reshape(h, w, bands)
+        shrunk = O.box_shrink(src, f) if f else src
+        assert np.array_equal(p.run(src), q.run(shrunk))

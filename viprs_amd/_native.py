@@ -124,6 +124,8 @@ def load() -> C.CDLL:
         "vgpu_pipeline_unpremultiply": (i32, [vp]),
         "vgpu_pipeline_thumbnail": (i32, [vp, i32, u32, u32, i32]),
         "vgpu_pipeline_thumbnail_crop": (i32, [vp, i32, u32, u32, i32, i32]),
+        "vgpu_pipeline_box_shrink": (i32, [vp, u32, i32]),
+        "vgpu_pipeline_thumbnail_shrink_on_load": (i32, [vp, i32, u32, u32, i32, i32]),
         "vgpu_thumbnail_shrink_on_load_factor": (i32, [u32, u32, u32, i32, u32, u32, i32, P(u32)]),
         "vgpu_pipeline_extract_area": (i32, [vp, u32, u32, u32, u32]),
         "vgpu_pipeline_resize": (i32, [vp, dbl, dbl, i32]),

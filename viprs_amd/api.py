@@ -247,6 +247,21 @@ class Pipeline:
         _check(self._lib.vgpu_pipeline_unpremultiply(self._h), "unpremultiply")
         return self
 
+    def box_shrink(self, factor: int, ceil: bool = False):
+        """The PNG source's box pre-shrink as the first step: ceil=False is
+        software_box_shrink_generic (decoder_source/shrink.rs:47-117), ceil=True
+        the u8 row decoder's form (png/decode_full.rs:320-452)."""
+        _check(self._lib.vgpu_pipeline_box_shrink(self._h, factor, int(bool(ceil))), "box_shrink")
+        return self
+
+    def thumbnail_shrink_on_load(self, target: int, width: int, height: int = 0, kernel: int = LANCZOS3,
+                                 crop: bool = False):
+        """thumbnail() over a PNG source that box-shrinks its decoded image by
+        the plan's shrink-on-load hint (builder/resample.rs:871-906)."""
+        _check(self._lib.vgpu_pipeline_thumbnail_shrink_on_load(self._h, target, width, height, kernel,
+                                                                 int(bool(crop))), "thumbnail_shrink_on_load")
+        return self
+
     def thumbnail(self, target: int, width: int, height: int = 0, kernel: int = LANCZOS3, crop: bool = False):
         """Thumbnail::new(target, kernel), with_crop() when crop is true
         (thumbnail.rs:185-190): cover the box, then a centred ExtractArea."""

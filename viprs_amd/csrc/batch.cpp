@@ -411,19 +411,14 @@ int run_fallback(vgpu_batch* b, int idx, const void* src, void* dst, hipStream_t
     if (im.stages.empty()) return from_hip(hipMemcpyAsync(dst, src, out_bytes, hipMemcpyDeviceToDevice, s));
     const unsigned char* dblob = static_cast<const unsigned char*>(b->blob.ptr);
     unsigned char* darena = static_cast<unsigned char*>(b->arena);
-    const void* cur = src;
-    Rect cur_rect = whole(im.w, im.h);
     for (size_t k = 0; k < im.stages.size(); ++k) {
         const Stage& st = im.stages[k];
-        if (stage_reads_source(im.stages, k)) {
-            cur = src;
-            cur_rect = whole(im.w, im.h);
-        }
+        const int si = stage_input(im.stages, k);
+        const InView in = si < 0 ? view_of(src, whole(im.w, im.h), st.clip)
+                                 : view_of(darena + im.inter_off[(size_t)si], im.stages[(size_t)si].out, st.clip);
         void* o = stage_writes_dst(im.stages, k) ? dst : darena + im.inter_off[k];
-        const int rc = launch_stage(im.nodes, st, view_of(cur, cur_rect, st.clip), o, dblob, s);
+        const int rc = launch_stage(im.nodes, st, in, o, dblob, s);
         if (rc != VGPU_OK) return rc;
-        cur = o;
-        cur_rect = st.out;
     }
     return VGPU_OK;
 }

@@ -332,6 +332,12 @@ __device__ __forceinline__ void window_sums_any(const unsigned char* rbuf, int s
     }
 }
 
+// A rounding ((sum + add) * mul) >> shift with the planner's per-job
+// constants (sum + add and mul below 2^24, product below 2^32).
+__device__ __forceinline__ uint32_t round_mul(uint32_t sum, uint32_t add, uint32_t mul, uint32_t shift) {
+    return (uint32_t)__umul24(sum + add, mul) >> shift;
+}
+
 template <int B>
 __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, int by,
                                                   unsigned char* lds) {
@@ -340,7 +346,9 @@ __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, 
     const int fh = job.fh, fv = job.fv, tx = job.tx, S = job.S, m = job.m, lsl = job.lsl;
     const int RP = 256 >> lsl;  // source rows per fetch group
     const bool premult = (B == 2 || B == 4) && job.premult;
-    const uint32_t mh = job.mh, mv = job.mv;
+    const uint32_t rh_add = job.rh_add, rh_mul = job.rh_mul, rh_shift = job.rh_shift;
+    const uint32_t rv_add = job.rv_add, rv_mul = job.rv_mul, rv_shift = job.rv_shift;
+    const int ostride = job.ostride ? job.ostride : out.w;
     // The wavefront index is uniform; saying so keeps all per-wavefront
     // geometry and row addressing in scalar registers.
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -455,16 +463,16 @@ __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, 
                     }
                 }
 #pragma unroll
-                for (int b = 0; b < B; ++b) acc[b] += shrink_round_h(sum[b], (uint32_t)fh, B, mh);
+                for (int b = 0; b < B; ++b) acc[b] += round_mul(sum[b], rh_add, rh_mul, rh_shift);
             }
             if (S == 1 && (k + 1) % fv == 0) {
                 // Output row oyi0 + k / fv is complete.
                 if (lane < ncols) {
                     unsigned char* o = static_cast<unsigned char*>(out.ptr) +
-                                       ((size_t)(oyi0 + k / fv) * out.w + col0 + lane) * B;
+                                       ((size_t)(oyi0 + k / fv) * ostride + col0 + lane) * B;
                     uint32_t v[B];
 #pragma unroll
-                    for (int b = 0; b < B; ++b) v[b] = shrink_round_v(acc[b], (uint32_t)fv, mv);
+                    for (int b = 0; b < B; ++b) v[b] = round_mul(acc[b], rv_add, rv_mul, rv_shift);
                     if ((B == 2 || B == 4) && job.unpremult) {
 #pragma unroll
                         for (int b = 0; b < B - 1; ++b) v[b] = unpremult_u8(v[b], v[B - 1]);
@@ -505,10 +513,10 @@ __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, 
 #pragma unroll
                 for (int b = 0; b < B; ++b) acc[b] += part_sums[((wave + q) * 64 + lane) * B + b];
             }
-            unsigned char* o = static_cast<unsigned char*>(out.ptr) + ((size_t)oyi0 * out.w + col0 + lane) * B;
+            unsigned char* o = static_cast<unsigned char*>(out.ptr) + ((size_t)oyi0 * ostride + col0 + lane) * B;
             uint32_t v[B];
 #pragma unroll
-            for (int b = 0; b < B; ++b) v[b] = shrink_round_v(acc[b], (uint32_t)fv, mv);
+            for (int b = 0; b < B; ++b) v[b] = round_mul(acc[b], rv_add, rv_mul, rv_shift);
             if ((B == 2 || B == 4) && job.unpremult) {
 #pragma unroll
                 for (int b = 0; b < B - 1; ++b) v[b] = unpremult_u8(v[b], v[B - 1]);
@@ -552,9 +560,6 @@ __device__ __forceinline__ void window_sums_masked(const unsigned char* rbuf, in
 
 // __umul24 returns the low 32 bits of the 48-bit product as an int: shift it
 // as unsigned (products reach 2^32 - 1).
-__device__ __forceinline__ uint32_t round_mul(uint32_t sum, uint32_t add, uint32_t mul, uint32_t shift) {
-    return (uint32_t)__umul24(sum + add, mul) >> shift;
-}
 
 // Lean form of the same kernel for spans of at least 128 chunks (lsl >= 7):
 // a fetch group is then one or two source rows and each 64-lane register
@@ -578,6 +583,7 @@ __device__ __forceinline__ void shrink_rows_lean_nd(const ShrinkJob& job, int bx
     const bool premult = (B == 2 || B == 4) && job.premult;
     const uint32_t rh_add = job.rh_add, rh_mul = job.rh_mul, rh_shift = job.rh_shift;
     const uint32_t rv_add = job.rv_add, rv_mul = job.rv_mul, rv_shift = job.rv_shift;
+    const int ostride = job.ostride ? job.ostride : out.w;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int part = wave % S;
     const int oyi0 = (by * ((kThreads / 64) / S) + wave / S) * m;
@@ -697,7 +703,7 @@ __device__ __forceinline__ void shrink_rows_lean_nd(const ShrinkJob& job, int bx
     for (int b = 0; b < B; ++b) acc[b] = 0;
     int krow = 0, orow = 0;  // source rows into the current output row; output rows done
     auto store = [&](int oy) {
-        unsigned char* o = static_cast<unsigned char*>(out.ptr) + ((size_t)oy * out.w + col0 + lane) * B;
+        unsigned char* o = static_cast<unsigned char*>(out.ptr) + ((size_t)oy * ostride + col0 + lane) * B;
         uint32_t v[B];
 #pragma unroll
         for (int b = 0; b < B; ++b) v[b] = round_mul(acc[b], rv_add, rv_mul, rv_shift);
@@ -867,6 +873,40 @@ __global__ __launch_bounds__(kThreads) void k_shrink_generic(InView in, OutView 
             } else {
                 o[i] = (T)v;
             }
+        }
+    }
+}
+
+// The PNG box pre-shrinks, one thread per output pixel of the rectangle
+// [x0, x0 + rw) x [y0, y0 + rh) of a dense dw-wide output: the window
+// [d*f, min(d*f + f, src)) per axis, divided by its own pixel count n
+// (software_box_shrink_generic, decoder_source/shrink.rs:47-117, and its
+// partial-window form in decode_png_with_box_shrink_u8, png/decode_full.rs:
+// 320-452).  u8: (sum + n/2) / n; u16: round(sum / n) half away from zero,
+// which for a ratio of integers below 2^32 is (2 sum + n) / (2 n); f32: the
+// f64 sum in row order, divided, cast.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_box_generic(const T* __restrict__ src, int sw, int sh, int B, int f,
+                                                          T* __restrict__ dst, int dw, int x0, int y0, int rw,
+                                                          int rh) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= (int64_t)rw * rh) return;
+    const int dx = x0 + (int)(i % rw), dy = y0 + (int)(i / rw);
+    const int64_t sx0 = (int64_t)dx * f, sy0 = (int64_t)dy * f;
+    const int64_t sx1 = min(sx0 + f, (int64_t)sw), sy1 = min(sy0 + f, (int64_t)sh);
+    const uint64_t n = max<int64_t>((sy1 - sy0) * (sx1 - sx0), 1);
+    T* o = dst + ((size_t)dy * dw + dx) * B;
+    for (int b = 0; b < B; ++b) {
+        if constexpr (sizeof(T) == 4) {
+            double sum = 0.0;
+            for (int64_t y = sy0; y < sy1; ++y)
+                for (int64_t x = sx0; x < sx1; ++x) sum += (double)src[((size_t)y * sw + x) * B + b];
+            o[b] = (T)(sum / (double)n);
+        } else {
+            uint64_t sum = 0;
+            for (int64_t y = sy0; y < sy1; ++y)
+                for (int64_t x = sx0; x < sx1; ++x) sum += src[((size_t)y * sw + x) * B + b];
+            o[b] = sizeof(T) == 1 ? (T)((sum + n / 2) / n) : (T)((2 * sum + n) / (2 * n));
         }
     }
 }
@@ -1384,6 +1424,34 @@ bool plan_shrink_job(ShrinkJob* job, bool row_split, int run) {
     return true;
 }
 
+bool plan_box_shrink_job(ShrinkJob* job, bool row_split) {
+    if (job->fh != job->fv || job->premult || job->unpremult) return false;
+    if (!plan_shrink_job(job, row_split, 0)) return false;
+    // Rows of raw window sums (identity ShrinkH rounding), one rounding of the
+    // f x f total: (sum + d/2) / d as a multiply-shift, exact for every
+    // sum <= 255 d (a power-of-two d shifts; others are checked here).
+    const uint32_t d = (uint32_t)job->fh * (uint32_t)job->fv, add = d / 2;
+    const uint64_t xmax = 255ull * d + add;
+    if (xmax >= (1u << 24)) return false;
+    uint32_t mul = 0, shift = 0;
+    if ((d & (d - 1)) == 0) {
+        mul = 1;
+        while ((1u << shift) < d) ++shift;
+    } else {
+        for (uint32_t sh = 24; sh < 40 && !mul; ++sh) {
+            const uint64_t m = ((1ull << sh) + d - 1) / d;
+            if (m >= (1u << 24) || xmax * m >= (1ull << 32)) break;
+            bool ok = true;
+            for (uint64_t x = 0; x <= xmax && ok; ++x) ok = ((x * m) >> sh) == x / d;
+            if (ok) mul = (uint32_t)m, shift = sh;
+        }
+        if (!mul) return false;
+    }
+    job->rh_add = 0, job->rh_mul = 1, job->rh_shift = 0;
+    job->rv_add = add, job->rv_mul = mul, job->rv_shift = shift;
+    return true;
+}
+
 int shrink_job_variant(const ShrinkJob& job) {
     if (!job.lean) return job.bands - 1;
     return job.bands - 1 + (job.lsl >= 8 ? 4 : (job.lsl == 7 ? 8 : 0));
@@ -1465,6 +1533,21 @@ hipError_t launch_shrink_generic(const InView& in, const OutView& out, int bands
     if (fmt == 0) hipLaunchKernelGGL(k_shrink_generic<uint8_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, factor, vertical, mult);
     else if (fmt == 1) hipLaunchKernelGGL(k_shrink_generic<uint16_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, factor, vertical, mult);
     else hipLaunchKernelGGL(k_shrink_generic<float>, dim3(g), dim3(kThreads), 0, s, in, out, bands, factor, vertical, mult);
+    return hipGetLastError();
+}
+
+hipError_t launch_box_generic(const void* src, int sw, int sh, int bands, int fmt, int f, void* dst, int dw,
+                              int x0, int y0, int rw, int rh, hipStream_t s) {
+    const int64_t n = (int64_t)rw * rh;
+    if (n <= 0) return hipSuccess;
+    const unsigned g = (unsigned)((n + kThreads - 1) / kThreads);
+#define VGPU_BOX(T)                                                                                          \
+    hipLaunchKernelGGL(k_box_generic<T>, dim3(g), dim3(kThreads), 0, s, static_cast<const T*>(src), sw, sh, \
+                       bands, f, static_cast<T*>(dst), dw, x0, y0, rw, rh)
+    if (fmt == 0) VGPU_BOX(uint8_t);
+    else if (fmt == 1) VGPU_BOX(uint16_t);
+    else VGPU_BOX(float);
+#undef VGPU_BOX
     return hipGetLastError();
 }
 

@@ -82,6 +82,7 @@ struct ShrinkJob {
     uint32_t rh_add = 0, rh_mul = 1, rh_shift = 0, rv_add = 0, rv_mul = 1, rv_shift = 0;
     int wnd_t = 16;  // lean forms: window dwords rounded up to 4, 8, 12 or 16 (0: longer)
     int lean = 1;    // lean forms allowed (32-bit buffer offsets: source < 4 GiB)
+    int ostride = 0; // output row pitch in pixels (0: out.w)
     int gx = 0, gy = 0;
     int src_slot = -1, dst_slot = -1;  // batch: pointer taken from the per-run tables
 };
@@ -143,6 +144,11 @@ struct BatchTask {
 
 hipError_t launch_shrink_hv_u8(const InView& in, const OutView& out, int bands, int fh, int fv,
                                int premult, hipStream_t s);
+// The PNG box pre-shrinks over the output rectangle [x0, x0 + rw) x
+// [y0, y0 + rh) of a dense dw-wide output (any format, window cut at the
+// source's edges).
+hipError_t launch_box_generic(const void* src, int sw, int sh, int bands, int fmt, int f, void* dst, int dw,
+                              int x0, int y0, int rw, int rh, hipStream_t s);
 hipError_t launch_shrink_generic(const InView& in, const OutView& out, int bands, int fmt,
                                  int factor, int vertical, hipStream_t s);
 // op = 1: ReduceOp's passes instead of ReduceH / ReduceV.
@@ -151,6 +157,11 @@ hipError_t launch_reduce_h(const InView& in, const OutView& out, int bands, int 
 hipError_t launch_reduce_v(const InView& in, const OutView& out, int bands, int fmt,
                            const DevPlan& plan, int op, hipStream_t s);
 bool plan_shrink_job(ShrinkJob* job, bool row_split = true, int run = 0);
+// The same kernel as the u8 PNG box pre-shrink (fh == fv == f): window sums
+// of each row unrounded, one rounding (sum + f*f/2) / (f*f) of the total.
+// False when the wavefront kernel cannot take the job or f*f has no exact
+// multiply-shift.
+bool plan_box_shrink_job(ShrinkJob* job, bool row_split = true);
 size_t shrink_job_lds();
 hipError_t launch_shrink_job(const ShrinkJob& job, hipStream_t s);
 // Kernel variant of a planned job (its band count); a batch launch runs jobs

@@ -255,6 +255,9 @@ Rect Node::required_input(const Rect& o) const {
         r.y = clamp_i32_range(o.y * (int64_t)factor);
         r.h = o.h * (int64_t)factor;
         break;
+    case N_BOX_SHRINK:  // the whole decoded image, whatever is asked of the shrunk one
+        r = Rect{0, 0, (int64_t)in_w, (int64_t)in_h};
+        break;
     case N_REDUCE_H:
     case N_REDUCE_OP_H: axis.required(o.x, o.w, &r.x, &r.w); break;
     case N_REDUCE_V:

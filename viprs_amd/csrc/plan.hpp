@@ -102,6 +102,10 @@ enum NodeKind : int {
     N_EXTRACT = 14,  // ExtractArea (structural/extract_area.rs:23-60)
     N_AFFINE = 15,   // Affine: [a, 0, 0, d], ExtendMode::Copy (affine/core.rs)
     N_ZOOM = 16,     // Zoom (structural/zoom.rs:339-400)
+    // The PNG box pre-shrink that replaces the source before any node
+    // (decoder_source/shrink.rs:47-117; png/decode_full.rs:320-452): only
+    // ever node 0; its output is read as the chain's source.
+    N_BOX_SHRINK = 17,
 };
 
 struct Node {
@@ -124,6 +128,7 @@ struct Node {
     int akern = 0;
     bool asmall = false;
     uint32_t zx = 1, zy = 1;  // zoom factors
+    int box_ceil = 0;         // box pre-shrink: ceil output size, partial edge windows
 
     int radius() const { return conv.empty() ? 0 : (int)(conv.size() - 1) / 2; }
     Rect required_input(const Rect& out) const;

@@ -884,7 +884,6 @@ ShrinkJob shrink_job_of(const Stage& s, const Node& first, const InView& in, con
     j.fh = s.fh;
     j.fv = s.fv;
     j.premult = s.premult;
-    j.premult = s.premult;
     j.unpremult = s.unpremult;
     return j;
 }
@@ -949,6 +948,33 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
     case ST_SHRINK:
         e = launch_shrink_generic(in, out, B, nd.in_fmt, (int)nd.factor, nd.kind == N_SHRINK_V, stream);
         break;
+    case ST_BOX: {
+        // The u8 windows that lie inside the source on the box-sum form of
+        // the shrink kernel; the partial windows of the last column and row
+        // (and every other format) one thread per output pixel.
+        const int f = (int)nd.factor, sw = (int)nd.in_w, sh = (int)nd.in_h;
+        const int dw = (int)nd.out_w, dh = (int)nd.out_h;
+        const int fw = std::min(dw, sw / f), fhh = std::min(dh, sh / f);
+        bool fast = false;
+        if (nd.in_fmt == U8 && B <= 4 && fw > 0 && fhh > 0) {
+            ShrinkJob j;
+            j.in = in;
+            j.out = OutView{dst, 0, 0, fw, fhh};
+            j.ostride = dw;
+            j.bands = B;
+            j.fh = j.fv = f;
+            if (plan_box_shrink_job(&j, opt.shrink_row_split)) {
+                fast = true;
+                e = launch_shrink_job(j, stream);
+                if (e == hipSuccess && dw > fw)
+                    e = launch_box_generic(in.ptr, sw, sh, B, U8, f, dst, dw, fw, 0, dw - fw, fhh, stream);
+                if (e == hipSuccess && dh > fhh)
+                    e = launch_box_generic(in.ptr, sw, sh, B, U8, f, dst, dw, 0, fhh, dw, dh - fhh, stream);
+            }
+        }
+        if (!fast) e = launch_box_generic(in.ptr, sw, sh, B, nd.in_fmt, f, dst, dw, 0, 0, dw, dh, stream);
+        break;
+    }
     case ST_REDUCE_H:
     case ST_REDUCE_V: {
         DevPlan p;
@@ -1066,11 +1092,12 @@ bool affine_tail_supported(const std::vector<Node>& nodes) {
 }
 
 std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector<Rect>& in_r,
-                               const std::vector<Rect>& out_r, Staging* st, const StageOptions& opt) {
+                               const std::vector<Rect>& out_r, Staging* st, const StageOptions& opt,
+                               int first_node) {
     std::vector<Stage> stages;
     const int n = (int)nodes.size();
     auto kind_at = [&](int k) { return k < n ? nodes[(size_t)k].kind : N_COPY; };
-    for (int i = 0; i < n;) {
+    for (int i = first_node; i < n;) {
         const Node& nd = nodes[(size_t)i];
         Stage s;
         s.first = i;
@@ -1236,6 +1263,7 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
             case N_UNPREMULT: s.kind = ST_ALPHA; break;
             case N_AFFINE: s.kind = ST_AFFINE; break;
             case N_ZOOM: s.kind = ST_ZOOM; break;
+            case N_BOX_SHRINK: s.kind = ST_BOX; break;
             default: s.kind = ST_COPY; break;
             }
         }
@@ -1261,6 +1289,11 @@ static void backward_regions_from(const std::vector<Node>& nodes, const Rect& fi
         (*in_r)[(size_t)i] = nodes[(size_t)i].required_input((*out_r)[(size_t)i]);
         if (i > 0) (*out_r)[(size_t)i - 1] = (*in_r)[(size_t)i];
     }
+    if (nodes[0].kind == N_BOX_SHRINK && n > 1) {
+        // The pre-shrunk image is the chain's source: computed whole, read
+        // with clamp-to-edge over whatever region node 1 asks for.
+        (*out_r)[0] = whole(nodes[0].out_w, nodes[0].out_h);
+    }
 }
 
 void backward_regions(const std::vector<Node>& nodes, uint32_t w, uint32_t h, std::vector<Rect>* in_r,
@@ -1276,17 +1309,32 @@ std::vector<Stage> plan_pipeline(const std::vector<Node>& nodes, uint32_t w, uin
         if (nd.kind == N_REDUCE_H && nd.in_fmt == U8) u8_reduce_h = true;
     }
     std::vector<Rect> in_r, out_r;
+    // A box pre-shrink (node 0) is one stage over the whole image; the other
+    // nodes read its output as their source.
+    const bool box = !nodes.empty() && nodes[0].kind == N_BOX_SHRINK;
+    std::vector<Stage> all;
+    if (box) {
+        Stage b;
+        b.kind = ST_BOX;
+        b.first = b.last = 0;
+        b.clip = whole(nodes[0].in_w, nodes[0].in_h);
+        b.out = whole(nodes[0].out_w, nodes[0].out_h);
+        b.in_fmt = b.out_fmt = nodes[0].in_fmt;
+        all.push_back(b);
+    }
     if (!(small && u8_reduce_h) || nodes.empty()) {
         backward_regions(nodes, w, h, &in_r, &out_r);
-        return plan_stages(nodes, in_r, out_r, st, opt);
+        std::vector<Stage> rest = plan_stages(nodes, in_r, out_r, st, opt, box ? 1 : 0);
+        all.insert(all.end(), rest.begin(), rest.end());
+        return all;
     }
-    std::vector<Stage> all;
     for (int64_t x0 = 0; x0 < (int64_t)w; x0 += 128) {
         const Rect fin{x0, 0, std::min<int64_t>(128, (int64_t)w - x0), (int64_t)h};
         backward_regions_from(nodes, fin, &in_r, &out_r);
-        std::vector<Stage> run = plan_stages(nodes, in_r, out_r, st, opt);
+        std::vector<Stage> run = plan_stages(nodes, in_r, out_r, st, opt, box ? 1 : 0);
         if (run.empty()) continue;
         run.front().from_source = true;
+        run.front().src_stage = box ? 0 : -1;
         Stage pl;
         pl.kind = ST_PLACE;
         pl.first = pl.last = (int)nodes.size() - 1;
@@ -1607,6 +1655,27 @@ int vgpu_pipeline_shrink(vgpu_pipeline* p, uint32_t hshrink, uint32_t vshrink) {
     return vgpu_pipeline_shrink_v(p, vshrink, 0);
 }
 
+int vgpu_pipeline_box_shrink(vgpu_pipeline* p, uint32_t factor, int ceil_mode) {
+    if (!p || factor == 0 || factor > (uint32_t)INT32_MAX) return VGPU_E_INVALID_ARGUMENT;
+    if (!p->nodes.empty()) return VGPU_E_INVALID_ARGUMENT;  // it replaces the source
+    if (ceil_mode && p->cur_fmt != U8) return VGPU_E_UNSUPPORTED_FORMAT;
+    const uint32_t w = p->cur_w, h = p->cur_h;
+    Node& nd = p->push(N_BOX_SHRINK);
+    nd.factor = factor;
+    nd.box_ceil = ceil_mode ? 1 : 0;
+    if (w == 0 || h == 0) {
+        nd.out_w = nd.out_h = 0;
+    } else if (ceil_mode) {  // div_ceil (decode_full.rs:343-344)
+        nd.out_w = (uint32_t)(((uint64_t)w + factor - 1) / factor);
+        nd.out_h = (uint32_t)(((uint64_t)h + factor - 1) / factor);
+    } else {  // (src / f).max(1) (shrink.rs:58-59)
+        nd.out_w = std::max(w / factor, 1u);
+        nd.out_h = std::max(h / factor, 1u);
+    }
+    p->commit();
+    return VGPU_OK;
+}
+
 int vgpu_pipeline_gauss_blur(vgpu_pipeline* p, float sigma) {
     if (!p) return VGPU_E_INVALID_ARGUMENT;
     if (!(sigma == sigma)) return VGPU_E_INVALID_ARGUMENT;
@@ -1758,6 +1827,36 @@ int vgpu_pipeline_thumbnail_crop(vgpu_pipeline* p, int target, uint32_t w, uint3
     return rc;
 }
 
+int vgpu_pipeline_thumbnail_shrink_on_load(vgpu_pipeline* p, int target, uint32_t w, uint32_t h, int kernel,
+                                           int crop) {
+    if (!p) return VGPU_E_INVALID_ARGUMENT;
+    if (kernel == LBB || kernel == VSQBS || kernel == NOHALO || kernel < NEAREST || kernel > CATMULLROM)
+        return VGPU_E_INVALID_KERNEL;
+    uint32_t f = 0;
+    if (p->nodes.empty()) {  // the hint is only taken by the source (builder/resample.rs:878-881)
+        std::vector<PlanStep> steps;
+        uint32_t ow = 0, oh = 0;
+        const int rc = plan_thumbnail(p->cur_w, p->cur_h, p->bands, target, w, h, &steps, &ow, &oh, crop != 0,
+                                      kernel, &f);
+        if (rc != OK) return rc;
+    }
+    // normalize_shrink_factor (shrink.rs:119-125) keeps 2, 4, 8 and 16.
+    const bool box = f == 2 || f == 4 || f == 8 || f == 16;
+    if (box) {
+        const int rc = vgpu_pipeline_box_shrink(p, f, 0);
+        if (rc != VGPU_OK) return rc;
+    }
+    const int rc = vgpu_pipeline_thumbnail_crop(p, target, w, h, kernel, crop);
+    if (rc != VGPU_OK && box) {  // leave the pipeline as it was
+        p->release();
+        p->nodes.clear();
+        p->cur_w = p->src_w;
+        p->cur_h = p->src_h;
+        p->cur_fmt = p->src_fmt;
+    }
+    return rc;
+}
+
 int vgpu_pipeline_resize(vgpu_pipeline* p, double hscale, double vscale, int kernel) {
     if (!p) return VGPU_E_INVALID_ARGUMENT;
     std::vector<PlanStep> steps;
@@ -1851,6 +1950,7 @@ const char* stage_kernel_name(const Stage& s) {
     case ST_AFFINE: return "affine";
     case ST_ZOOM: return "zoom";
     case ST_PLACE: return "place";
+    case ST_BOX: return "box_shrink";
     case ST_CONV_PAIR: return s.cpair_kind == 2 ? "conv_pair_f32" : "conv_pair_f64";
     }
     return "unknown";
@@ -1914,25 +2014,20 @@ int vgpu_pipeline_run_cancellable(vgpu_pipeline* p, const void* src, size_t src_
         return from_hip(hipMemcpyAsync(dst, src, out_bytes, hipMemcpyDeviceToDevice, ctx->stream));
     }
     const unsigned char* blob = static_cast<const unsigned char*>(p->blob.ptr);
-    const void* cur = src;
-    Rect cur_rect = whole(p->src_w, p->src_h);
     for (size_t k = 0; k < p->stages.size(); ++k) {
         // The scheduler polls the token between tiles (runtime.rs:1237); the
         // units here are the fused stage launches (and tile-column runs).
         if (vgpu_cancel_token_is_cancelled(token)) return VGPU_E_CANCELLED;
         const Stage& s = p->stages[k];
-        if (stage_reads_source(p->stages, k)) {
-            cur = src;
-            cur_rect = whole(p->src_w, p->src_h);
-        }
+        const int si = stage_input(p->stages, k);
+        const InView in = si < 0 ? view_of(src, whole(p->src_w, p->src_h), s.clip)
+                                 : view_of(p->inter[(size_t)si], p->stages[(size_t)si].out, s.clip);
         void* o = stage_writes_dst(p->stages, k) ? dst : p->inter[k];
-        rc = launch_stage(p->nodes, s, view_of(cur, cur_rect, s.clip), o, blob, ctx->stream, p->opt);
+        rc = launch_stage(p->nodes, s, in, o, blob, ctx->stream, p->opt);
         if (rc != VGPU_OK) {
             ctx->sticky = rc;
             return rc;
         }
-        cur = o;
-        cur_rect = s.out;
     }
     return VGPU_OK;
 }
@@ -1972,19 +2067,14 @@ int vgpu_pipeline_run_noalloc(vgpu_pipeline* p, const void* src, size_t src_byte
                                      ctx->stream));
         if (rc != VGPU_OK) return ctx->sticky = rc;
     }
-    const void* cur = src;
-    Rect cur_rect = whole(p->src_w, p->src_h);
     for (size_t k = 0; k < p->stages.size(); ++k) {
         const Stage& s = p->stages[k];
-        if (stage_reads_source(p->stages, k)) {
-            cur = src;
-            cur_rect = whole(p->src_w, p->src_h);
-        }
+        const int si = stage_input(p->stages, k);
+        const InView in = si < 0 ? view_of(src, whole(p->src_w, p->src_h), s.clip)
+                                 : view_of(base + off[(size_t)si], p->stages[(size_t)si].out, s.clip);
         void* o = stage_writes_dst(p->stages, k) ? dst : static_cast<void*>(base + off[k]);
-        rc = launch_stage(p->nodes, s, view_of(cur, cur_rect, s.clip), o, base, ctx->stream, p->opt);
+        rc = launch_stage(p->nodes, s, in, o, base, ctx->stream, p->opt);
         if (rc != VGPU_OK) return ctx->sticky = rc;
-        cur = o;
-        cur_rect = s.out;
     }
     return VGPU_OK;
 }
@@ -2008,20 +2098,15 @@ int vgpu_pipeline_run_profiled(vgpu_pipeline* p, const void* src, size_t src_byt
     std::vector<double> acc((size_t)ns, 0.0);
     const unsigned char* blob = static_cast<const unsigned char*>(p->blob.ptr);
     for (int it = 0; it < iterations && rc == VGPU_OK; ++it) {
-        const void* cur = src;
-        Rect cur_rect = whole(p->src_w, p->src_h);
         (void)hipEventRecord(ev[0], ctx->stream);
         for (int k = 0; k < ns && rc == VGPU_OK; ++k) {
             const Stage& s = p->stages[(size_t)k];
-            if (stage_reads_source(p->stages, (size_t)k)) {
-                cur = src;
-                cur_rect = whole(p->src_w, p->src_h);
-            }
+            const int si = stage_input(p->stages, (size_t)k);
+            const InView in = si < 0 ? view_of(src, whole(p->src_w, p->src_h), s.clip)
+                                     : view_of(p->inter[(size_t)si], p->stages[(size_t)si].out, s.clip);
             void* o = stage_writes_dst(p->stages, (size_t)k) ? dst : p->inter[(size_t)k];
-            rc = launch_stage(p->nodes, s, view_of(cur, cur_rect, s.clip), o, blob, ctx->stream, p->opt);
+            rc = launch_stage(p->nodes, s, in, o, blob, ctx->stream, p->opt);
             (void)hipEventRecord(ev[(size_t)k + 1], ctx->stream);
-            cur = o;
-            cur_rect = s.out;
         }
         if (hipEventSynchronize(ev[(size_t)ns]) != hipSuccess) rc = VGPU_E_DEVICE;
         for (int k = 0; k < ns && rc == VGPU_OK; ++k) {
@@ -2051,15 +2136,15 @@ int vgpu_pipeline_time_stage_sources(vgpu_pipeline* p, const void* const* srcs, 
     if (stage < 0 || stage >= ns) return VGPU_E_INVALID_ARGUMENT;
     for (int i = 0; i < nsrc; ++i)
         if (!srcs[i]) return VGPU_E_INVALID_ARGUMENT;
-    // Input view of the stage: the source (cycled over srcs) or the previous
-    // stage's output.
-    const bool from_src = stage_reads_source(p->stages, (size_t)stage);
+    // Input view of the stage: the source (cycled over srcs) or the output
+    // of the stage it reads.
+    const int si = stage_input(p->stages, (size_t)stage);
     const Stage& s = p->stages[(size_t)stage];
     void* o = stage_writes_dst(p->stages, (size_t)stage) ? dst : p->inter[(size_t)stage];
     const unsigned char* blob = static_cast<const unsigned char*>(p->blob.ptr);
     auto view = [&](int i) {
-        if (from_src) return view_of(srcs[i % nsrc], whole(p->src_w, p->src_h), s.clip);
-        return view_of(p->inter[(size_t)stage - 1], p->stages[(size_t)stage - 1].out, s.clip);
+        if (si < 0) return view_of(srcs[i % nsrc], whole(p->src_w, p->src_h), s.clip);
+        return view_of(p->inter[(size_t)si], p->stages[(size_t)si].out, s.clip);
     };
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!create_event_pair(&e0, &e1)) return VGPU_E_DEVICE;

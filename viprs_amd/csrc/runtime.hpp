@@ -139,6 +139,7 @@ enum StageKind {
     ST_ZOOM,
     ST_PLACE,    // copy a tile-column run's output into its columns of the destination
     ST_CONV_PAIR, // fused float GaussBlurH -> GaussBlurV or ConvSepH -> ConvSepV
+    ST_BOX,       // the PNG box pre-shrink of the source
 };
 
 struct Stage {
@@ -163,10 +164,12 @@ struct Stage {
     size_t w32 = 0, v_w32 = 0;  // ST_RED_F32: f32 weight tables
     size_t aff_x = 0, aff_y = 0;  // ST_AFFINE: per-column / per-row AffineTap tables
     int aff_node = -1;            // ST_AFFINE: index of the Affine node
-    // Tile-column runs: the stage reads the pipeline's source instead of the
-    // previous stage's output / writes the destination (ST_PLACE, whose
-    // destination rows are place_w pixels wide).
+    // Tile-column runs: the stage reads the pipeline's source (or, after a
+    // box pre-shrink, src_stage's output) instead of the previous stage's
+    // output / writes the destination (ST_PLACE, whose destination rows are
+    // place_w pixels wide).
     bool from_source = false, to_dst = false;
+    int src_stage = -1;
     int64_t place_w = 0;
     ConvPairJob cpair;          // ST_CONV_PAIR: job template (views and weights set at launch)
     int cpair_kind = 0;         // 0 Gaussian FP64, 1 ConvSep FP64, 2 Gaussian FP32
@@ -208,16 +211,21 @@ bool affine_tail_supported(const std::vector<Node>& nodes);
 // by an ST_PLACE stage.
 std::vector<Stage> plan_pipeline(const std::vector<Node>& nodes, uint32_t w, uint32_t h, Staging* st,
                                  const StageOptions& opt);
-// The input of stage k: the previous stage's output, or the source.
-inline bool stage_reads_source(const std::vector<Stage>& stages, size_t k) {
-    return k == 0 || stages[k].from_source;
+// The input of stage k: -1 for the pipeline's source, else the index of the
+// stage whose output it reads (the previous one, or the box pre-shrink for a
+// tile-column run that starts after it).
+inline int stage_input(const std::vector<Stage>& stages, size_t k) {
+    if (k == 0) return -1;
+    if (stages[k].from_source) return stages[k].src_stage;
+    return (int)k - 1;
 }
 inline bool stage_writes_dst(const std::vector<Stage>& stages, size_t k) {
     return k + 1 == stages.size() || stages[k].to_dst;
 }
+// Stages of nodes [first_node, n) over the regions of the backward walk.
 std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector<Rect>& in_r,
                                const std::vector<Rect>& out_r, Staging* st,
-                               const StageOptions& opt = StageOptions());
+                               const StageOptions& opt = StageOptions(), int first_node = 0);
 ShrinkJob shrink_job_of(const Stage& s, const Node& first, const InView& in, const OutView& out);
 ReduceJob reduce_job_of(const Stage& s, const InView& in, const OutView& out, const unsigned char* blob);
 SepJob sep_job_of(const Stage& s, const InView& in, const OutView& out, const unsigned char* blob);

@@ -332,20 +332,13 @@ bool vgpu_server::launch_job(Job& j) {
         if (j.stages.empty()) {
             if (hipMemcpyAsync(out, src, j.out_bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) return false;
         } else {
-            const void* cur = src;
-            Rect cur_rect = whole(j.req.width, j.req.height);
             for (size_t k = 0; k < j.stages.size(); ++k) {
                 const Stage& st = j.stages[k];
-                if (stage_reads_source(j.stages, k)) {
-                    cur = src;
-                    cur_rect = whole(j.req.width, j.req.height);
-                }
+                const int si = stage_input(j.stages, k);
+                const InView in = si < 0 ? view_of(src, whole(j.req.width, j.req.height), st.clip)
+                                         : view_of(inter[(size_t)si], j.stages[(size_t)si].out, st.clip);
                 void* dst = stage_writes_dst(j.stages, k) ? out : inter[k];
-                if (launch_stage(j.pipe->nodes, st, view_of(cur, cur_rect, st.clip), dst, blob, s, j.pipe->opt) !=
-                    VGPU_OK)
-                    return false;
-                cur = dst;
-                cur_rect = st.out;
+                if (launch_stage(j.pipe->nodes, st, in, dst, blob, s, j.pipe->opt) != VGPU_OK) return false;
             }
         }
     }
