@@ -25,8 +25,17 @@
 extern "C" {
 #endif
 
-/* BandFormatId subset (crates/viprs-core/src/format/band.rs:19-). */
-typedef enum { VGPU_U8 = 0, VGPU_U16 = 1, VGPU_F32 = 2 } vgpu_format;
+/* BandFormatId (crates/viprs-core/src/format/band.rs:19-35), numbered by this
+ * library (U8, U16, F32 first). */
+typedef enum {
+    VGPU_U8 = 0,
+    VGPU_U16 = 1,
+    VGPU_F32 = 2,
+    VGPU_I16 = 3,
+    VGPU_U32 = 4,
+    VGPU_I32 = 5,
+    VGPU_F64 = 6
+} vgpu_format;
 
 /* InterpolationKernel (crates/viprs-core/src/kernel.rs:22-49). LBB, VSQBS and
  * Nohalo are rejected by the reduce ops like the reference does

@@ -293,9 +293,6 @@ static inline int64_t clamp_axis(int64_t abs, int64_t origin, int64_t len) {
     return clampi(abs - origin, 0, len > 0 ? len - 1 : 0);
 }
 
-/* ReduceSample::from_fixed_i64 for unsigned types (sample_conv.rs:18-35). */
-static inline int64_t from_fixed(int64_t v, int64_t maxv) { return clampi((v + 2048) >> 12, 0, maxv); }
-
 /* Gaussian integer kernel (gauss_blur.rs:80-126; gauss_kernel.rs:13-56). */
 int orc_gauss_int_kernel(float sigma_f, int16_t *coeffs, int max_taps, int32_t *scale) {
     double sigma = (double)sigma_f;
@@ -397,10 +394,28 @@ struct orc_pipeline {
     uint32_t orig_w, orig_h;
 };
 
-static size_t fmt_size(int f) { return f == ORC_U8 ? 1 : (f == ORC_U16 ? 2 : 4); }
+static size_t fmt_size(int f) {
+    return f == ORC_U8 ? 1 : (f == ORC_U16 || f == ORC_I16) ? 2 : f == ORC_F64 ? 8 : 4;
+}
+static int fmt_is_int(int f) { return f != ORC_F32 && f != ORC_F64; }
+static inline double ld(const void *b, int fmt, size_t i);
+static void store_shrunk(void *out, int fmt, size_t oi, double v);
+/* The integer formats' ranges (band.rs), as doubles and as integers. */
+static double fmt_lo(int f) {
+    return f == ORC_I16 ? -32768.0 : f == ORC_I32 ? -2147483648.0 : 0.0;
+}
+static double fmt_hi(int f) {
+    switch (f) {
+    case ORC_U8: return 255.0;
+    case ORC_U16: return 65535.0;
+    case ORC_I16: return 32767.0;
+    case ORC_U32: return 4294967295.0;
+    default: return 2147483647.0;
+    }
+}
 
 orc_pipeline *orc_pipeline_new(uint32_t w, uint32_t h, uint32_t bands, int fmt) {
-    if (bands == 0 || fmt < ORC_U8 || fmt > ORC_F32) return NULL;
+    if (bands == 0 || fmt < ORC_U8 || fmt > ORC_F64) return NULL;
     orc_pipeline *p = (orc_pipeline *)calloc(1, sizeof(orc_pipeline));
     p->src_w = p->cur_w = p->orig_w = w;
     p->src_h = p->cur_h = p->orig_h = h;
@@ -932,7 +947,7 @@ int orc_pipeline_thumbnail(orc_pipeline *p, int target, uint32_t tw, uint32_t th
  * only when the source is smaller than f); u8 sums in integers and rounds
  * (sum + n/2) / n with n = the window's pixel count; the other formats sum in
  * f64 in row order (sy outer, sx inner) and store from_f64_clamped(sum / n)
- * (shrinkh.rs:49-128: u16 rounds half away from zero, f32 casts).
+ * (shrinkh.rs:44-128).
  *
  * ceil_mode == 1 (u8 only): decode_png_with_box_shrink_u8 (codecs/src/png/
  * decode_full.rs:320-452), the shrink applied row by row while a PNG file
@@ -940,7 +955,7 @@ int orc_pipeline_thumbnail(orc_pipeline *p, int target, uint32_t tw, uint32_t th
  * windows, each divided by its own pixel count with the same rounding. */
 int orc_box_shrink(const void *src, uint32_t w, uint32_t h, uint32_t bands, int fmt, uint32_t f, int ceil_mode,
                    void *dst, uint32_t *out_w, uint32_t *out_h) {
-    if (f == 0 || bands == 0 || fmt < ORC_U8 || fmt > ORC_F32) return ORC_EINVAL;
+    if (f == 0 || bands == 0 || fmt < ORC_U8 || fmt > ORC_F64) return ORC_EINVAL;
     if (ceil_mode && fmt != ORC_U8) return ORC_EFORMAT;
     uint32_t dw, dh;
     if (ceil_mode) {
@@ -972,14 +987,8 @@ int orc_box_shrink(const void *src, uint32_t w, uint32_t h, uint32_t bands, int 
                 } else {
                     double sum = 0.0;
                     for (size_t sy = sy0; sy < sy1; ++sy)
-                        for (size_t sx = sx0; sx < sx1; ++sx)
-                            sum += fmt == ORC_U16 ? (double)((const uint16_t *)src)[(sy * w + sx) * B + b]
-                                                  : (double)((const float *)src)[(sy * w + sx) * B + b];
-                    const double v = sum / (double)n;
-                    if (fmt == ORC_U16)
-                        ((uint16_t *)dst)[o + b] = (uint16_t)fmin(fmax(round(v), 0.0), 65535.0);
-                    else
-                        ((float *)dst)[o + b] = (float)v;
+                        for (size_t sx = sx0; sx < sx1; ++sx) sum += ld(src, fmt, (sy * w + sx) * B + b);
+                    store_shrunk(dst, fmt, o + b, sum / (double)n);
                 }
             }
         }
@@ -1224,15 +1233,41 @@ static rgn required_input(const node_t *nd, rgn o) {
 
 /* --------------------------------------------------------- sample access */
 
+/* ToF64 (viprs-core shared_ops/sample_conv.rs:1-50): every sample exactly. */
 static inline double ld(const void *b, int fmt, size_t i) {
     switch (fmt) {
     case ORC_U8: return (double)((const uint8_t *)b)[i];
     case ORC_U16: return (double)((const uint16_t *)b)[i];
+    case ORC_I16: return (double)((const int16_t *)b)[i];
+    case ORC_U32: return (double)((const uint32_t *)b)[i];
+    case ORC_I32: return (double)((const int32_t *)b)[i];
+    case ORC_F64: return ((const double *)b)[i];
     default: return (double)((const float *)b)[i];
     }
 }
+/* ReduceSample::to_i64 (resample/sample_conv.rs:18-62). */
 static inline int64_t ldi(const void *b, int fmt, size_t i) {
-    return fmt == ORC_U8 ? (int64_t)((const uint8_t *)b)[i] : (int64_t)((const uint16_t *)b)[i];
+    switch (fmt) {
+    case ORC_U8: return (int64_t)((const uint8_t *)b)[i];
+    case ORC_U16: return (int64_t)((const uint16_t *)b)[i];
+    case ORC_I16: return (int64_t)((const int16_t *)b)[i];
+    case ORC_U32: return (int64_t)((const uint32_t *)b)[i];
+    default: return (int64_t)((const int32_t *)b)[i];
+    }
+}
+/* ReduceSample::from_fixed_i64 (resample/sample_conv.rs:18-62): unsigned
+ * (v + 2048) >> 12, signed (v - 2048) >> 12 for a negative v (arithmetic
+ * shifts), then the type's range. */
+static void store_fixed(void *out, int fmt, size_t oi, int64_t v) {
+    const int64_t r = (fmt == ORC_I16 || fmt == ORC_I32) && v < 0 ? (v - 2048) >> 12 : (v + 2048) >> 12;
+    const int64_t q = clampi(r, (int64_t)fmt_lo(fmt), (int64_t)fmt_hi(fmt));
+    switch (fmt) {
+    case ORC_U8: ((uint8_t *)out)[oi] = (uint8_t)q; break;
+    case ORC_U16: ((uint16_t *)out)[oi] = (uint16_t)q; break;
+    case ORC_I16: ((int16_t *)out)[oi] = (int16_t)q; break;
+    case ORC_U32: ((uint32_t *)out)[oi] = (uint32_t)q; break;
+    default: ((int32_t *)out)[oi] = (int32_t)q; break;
+    }
 }
 
 /* ------------------------------------------------------- process_region */
@@ -1258,18 +1293,25 @@ static void process_reduce_h(const node_t *nd, const void *in, rgn ir, void *out
         for (size_t xo = 0; xo < out_w; ++xo) {
             for (size_t b = 0; b < B; ++b) {
                 size_t oi = ((size_t)y * out_w + xo) * B + b;
-                if (nd->in_fmt != ORC_F32) {
-                    /* reduce_h_u8_scalar_planned / avx2_pixel (reduce_simd.rs:2463-2538) */
+                if (fmt_is_int(nd->in_fmt)) {
+                    /* reduce_h_u8_scalar_planned / avx2_pixel (reduce_simd.rs:2463-2538);
+                     * reduce_h_scalar's fixed-point loop for I16/U32/I32 (:47-75) */
                     const int16_t *w = k->ci + (size_t)phases[xo] * T;
                     int64_t acc = 0;
                     for (int t = 0; t < T; ++t) {
                         size_t tx = (size_t)clamp_axis(starts[xo] + t, ir.x, ir.w);
                         acc += (int64_t)w[t] * ldi(in, nd->in_fmt, ((size_t)y * in_w + tx) * B + b);
                     }
-                    if (nd->in_fmt == ORC_U8)
-                        ((uint8_t *)out)[oi] = (uint8_t)from_fixed(acc, 255);
-                    else
-                        ((uint16_t *)out)[oi] = (uint16_t)from_fixed(acc, 65535);
+                    store_fixed(out, nd->in_fmt, oi, acc);
+                } else if (nd->in_fmt == ORC_F64) {
+                    /* reduce_h_scalar's f64 loop (reduce_simd.rs:76-88): mul_add in tap order */
+                    const double *w = k->cf + (size_t)phases[xo] * T;
+                    double acc = 0.0;
+                    for (int t = 0; t < T; ++t) {
+                        size_t tx = (size_t)clamp_axis(starts[xo] + t, ir.x, ir.w);
+                        acc = fma(ld(in, ORC_F64, ((size_t)y * in_w + tx) * B + b), w[t], acc);
+                    }
+                    ((double *)out)[oi] = acc;
                 } else {
                     const double *w = k->cf + (size_t)phases[xo] * T;
                     double acc = 0.0;
@@ -1313,17 +1355,23 @@ static void process_reduce_v(const node_t *nd, const void *in, rgn ir, void *out
         rk_plan(k, rk_sp(k, (double)orr.y + (double)yo), &start, &phase);
         for (size_t x = 0; x < row_len; ++x) {
             size_t oi = (size_t)yo * row_len + x;
-            if (nd->in_fmt != ORC_F32) {
+            if (fmt_is_int(nd->in_fmt)) {
                 const int16_t *w = k->ci + (size_t)phase * T;
                 int64_t acc = 0;
                 for (int t = 0; t < T; ++t) {
                     size_t ty = (size_t)clamp_axis(start + t, ir.y, ir.h);
                     acc += (int64_t)w[t] * ldi(in, nd->in_fmt, ty * stride + x);
                 }
-                if (nd->in_fmt == ORC_U8)
-                    ((uint8_t *)out)[oi] = (uint8_t)from_fixed(acc, 255);
-                else
-                    ((uint16_t *)out)[oi] = (uint16_t)from_fixed(acc, 65535);
+                store_fixed(out, nd->in_fmt, oi, acc);
+            } else if (nd->in_fmt == ORC_F64) {
+                /* reduce_v_scalar's f64 loop (reduce_simd.rs:127-141) */
+                const double *w = k->cf + (size_t)phase * T;
+                double acc = 0.0;
+                for (int t = 0; t < T; ++t) {
+                    size_t ty = (size_t)clamp_axis(start + t, ir.y, ir.h);
+                    acc = fma(ld(in, ORC_F64, ty * stride + x), w[t], acc);
+                }
+                ((double *)out)[oi] = acc;
             } else {
                 const double *w = k->cf + (size_t)phase * T;
                 double acc = 0.0;
@@ -1343,16 +1391,33 @@ static void process_reduce_v(const node_t *nd, const void *in, rgn ir, void *out
     }
 }
 
-/* F::Sample::from_f64 (sample_conv.rs:58-93): integers clamp to the type's
- * range, then round half away from zero; f32 is a plain cast. */
-static void store_from_f64(void *out, int fmt, size_t oi, double v) {
-    if (fmt == ORC_U8)
-        ((uint8_t *)out)[oi] = (uint8_t)round(clampd(v, 0.0, 255.0));
-    else if (fmt == ORC_U16)
-        ((uint16_t *)out)[oi] = (uint16_t)round(clampd(v, 0.0, 65535.0));
-    else
-        ((float *)out)[oi] = (float)v;
+static void store_int(void *out, int fmt, size_t oi, double q) {
+    switch (fmt) {
+    case ORC_U8: ((uint8_t *)out)[oi] = (uint8_t)q; break;
+    case ORC_U16: ((uint16_t *)out)[oi] = (uint16_t)q; break;
+    case ORC_I16: ((int16_t *)out)[oi] = (int16_t)q; break;
+    case ORC_U32: ((uint32_t *)out)[oi] = (uint32_t)q; break;
+    default: ((int32_t *)out)[oi] = (int32_t)q; break;
+    }
 }
+
+/* F::Sample::from_f64 (viprs-core shared_ops/sample_conv.rs:53-99): integers
+ * clamp to the type's range, then round half away from zero; f32 is a plain
+ * cast, f64 the value. */
+static void store_from_f64(void *out, int fmt, size_t oi, double v) {
+    if (fmt == ORC_F32)
+        ((float *)out)[oi] = (float)v;
+    else if (fmt == ORC_F64)
+        ((double *)out)[oi] = v;
+    else
+        store_int(out, fmt, oi, round(clampd(v, fmt_lo(fmt), fmt_hi(fmt))));
+}
+
+/* The three sample conversions, one value at a time, for the known-answer
+ * tests: from_fixed_i64, FromF64::from_f64, ShrinkSample::from_f64_clamped. */
+void orc_convert_fixed(int64_t v, int fmt, void *out) { store_fixed(out, fmt, 0, v); }
+void orc_convert_from_f64(double v, int fmt, void *out) { store_from_f64(out, fmt, 0, v); }
+void orc_convert_shrunk(double v, int fmt, void *out) { store_shrunk(out, fmt, 0, v); }
 
 /* ReduceOp passes (reduce.rs:104-166): per output position the direct
  * source position (offset 0), taps_for_f64 (reduce_common.rs:188-191), an
@@ -1399,12 +1464,16 @@ static inline uint8_t shrink_v_u8_round(uint32_t sum, uint32_t f) {
 }
 
 static void store_shrunk(void *out, int fmt, size_t oi, double v) {
-    /* ShrinkSample::from_f64_clamped (shrinkh.rs:49-128) */
-    if (fmt == ORC_U16) {
-        ((uint16_t *)out)[oi] = (uint16_t)clampd(round(v), 0.0, 65535.0);
-    } else {
+    /* ShrinkSample::from_f64_clamped (shrinkh.rs:44-128): unsigned round then
+     * clamp; signed clamp(v + 0.5) then truncate; f32 cast; f64 as is. */
+    if (fmt == ORC_F32)
         ((float *)out)[oi] = (float)v;
-    }
+    else if (fmt == ORC_F64)
+        ((double *)out)[oi] = v;
+    else if (fmt == ORC_I16 || fmt == ORC_I32)
+        store_int(out, fmt, oi, trunc(clampd(v + 0.5, fmt_lo(fmt), fmt_hi(fmt))));
+    else
+        store_int(out, fmt, oi, clampd(round(v), fmt_lo(fmt), fmt_hi(fmt)));
 }
 
 static void process_shrink_h(const node_t *nd, const void *in, rgn ir, void *out, rgn orr) {
@@ -1591,7 +1660,7 @@ static double affine_band(const node_t *nd, const void *in, rgn ir, double x, do
 static void process_affine(const node_t *nd, const void *in, rgn ir, void *out, rgn orr) {
     const size_t B = nd->bands;
     const int fmt = nd->out_fmt;
-    const double amax = fmt == ORC_U8 ? 255.0 : (fmt == ORC_U16 ? 65535.0 : 1.0);
+    const double amax = fmt_is_int(fmt) ? fmt_hi(fmt) : 1.0; /* Affine::alpha_max (affine/core.rs:101-109) */
     double row_x = fma(nd->am[1], (double)orr.y, nd->am[0] * (double)orr.x) + nd->atx;
     double row_y = fma(nd->am[3], (double)orr.y, nd->am[2] * (double)orr.x) + nd->aty;
     for (int64_t yy = 0; yy < orr.h; ++yy) {

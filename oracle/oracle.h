@@ -26,9 +26,9 @@
 extern "C" {
 #endif
 
-/* Band formats (crates/viprs-core/src/format/band.rs:19-). Only the three the
- * hot path is measured on are restated. */
-enum { ORC_U8 = 0, ORC_U16 = 1, ORC_F32 = 2 };
+/* Band formats (crates/viprs-core/src/format/band.rs:19-35), numbered as the
+ * library numbers them. */
+enum { ORC_U8 = 0, ORC_U16 = 1, ORC_F32 = 2, ORC_I16 = 3, ORC_U32 = 4, ORC_I32 = 5, ORC_F64 = 6 };
 
 /* InterpolationKernel (crates/viprs-core/src/kernel.rs:22-49). */
 enum {
@@ -126,6 +126,12 @@ int orc_f64_gauss(const float *src, uint32_t w, uint32_t h, uint32_t bands, floa
 
 /* Seeded synthetic bytes: splitmix64 words, the device fill's stream. */
 void orc_synthetic_bytes(uint8_t *dst, size_t n, uint64_t seed);
+/* Sample conversions of one value into *out (a sample of format fmt):
+ * ReduceSample::from_fixed_i64, FromF64::from_f64 and
+ * ShrinkSample::from_f64_clamped. */
+void orc_convert_fixed(int64_t v, int fmt, void *out);
+void orc_convert_from_f64(double v, int fmt, void *out);
+void orc_convert_shrunk(double v, int fmt, void *out);
 
 #ifdef __cplusplus
 }

@@ -16,13 +16,14 @@ import numpy as np
 HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
-U8, U16, F32 = 0, 1, 2
+U8, U16, F32, I16, U32, I32, F64 = 0, 1, 2, 3, 4, 5, 6
 NEAREST, BILINEAR, BICUBIC, LANCZOS2, LANCZOS3, CATMULLROM, LBB, VSQBS, NOHALO = range(9)
 FITBOX, FORCE_EXACT, WIDTH, HEIGHT = range(4)
 OK, EINVAL, EKERNEL, EFORMAT, EUNSUPPORTED = 0, -1, -2, -3, -4
 
-DTYPES = {U8: np.uint8, U16: np.uint16, F32: np.float32}
-FMT_OF = {np.dtype(np.uint8): U8, np.dtype(np.uint16): U16, np.dtype(np.float32): F32}
+DTYPES = {U8: np.uint8, U16: np.uint16, F32: np.float32, I16: np.int16, U32: np.uint32, I32: np.int32,
+          F64: np.float64}
+FMT_OF = {np.dtype(t): f for f, t in DTYPES.items()}
 
 _lib = None
 
@@ -84,6 +85,10 @@ def lib() -> C.CDLL:
         L.orc_f64_gauss.argtypes = [vp, u32, u32, u32, C.c_float, vp, i32]
         L.orc_synthetic_bytes.argtypes = [vp, C.c_size_t, C.c_uint64]
         L.orc_synthetic_bytes.restype = None
+        for name, arg in (("orc_convert_fixed", C.c_int64), ("orc_convert_from_f64", dbl),
+                          ("orc_convert_shrunk", dbl)):
+            getattr(L, name).argtypes = [arg, i32, vp]
+            getattr(L, name).restype = None
         _lib = L
     return _lib
 
@@ -244,6 +249,16 @@ class Pipeline:
             threads = os.cpu_count() or 1
         _check(lib().orc_pipeline_run(self._p, _ptr(src), _ptr(dst), threads, strip_height), "run")
         return dst
+
+
+def convert(kind: str, value, fmt: int):
+    """One sample conversion: kind "fixed" (from_fixed_i64 of an int), "f64"
+    (FromF64::from_f64) or "shrink" (ShrinkSample::from_f64_clamped)."""
+    out = np.zeros(1, DTYPES[fmt])
+    fn = {"fixed": lib().orc_convert_fixed, "f64": lib().orc_convert_from_f64,
+          "shrink": lib().orc_convert_shrunk}[kind]
+    fn(value, fmt, _ptr(out))
+    return out[0].item()
 
 
 def box_shrink(src: np.ndarray, factor: int, ceil: bool = False) -> np.ndarray:

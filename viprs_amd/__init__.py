@@ -7,4 +7,4 @@ from .api import (  # noqa: F401
     NEAREST, NOHALO, U8, U16, VSQBS, WIDTH, Batch, Context, DeviceBuffer, Op, Pipeline, Region,
     Server, VgpuError, thumbnail_shrink_on_load_factor,
 )
-from .api import E_SCRATCH_LIMIT, NOALLOC_SCRATCH_BYTES  # noqa: F401
+from .api import E_SCRATCH_LIMIT, NOALLOC_SCRATCH_BYTES, F64, I16, I32, U32  # noqa: F401

@@ -21,7 +21,7 @@ import numpy as np
 from . import _native
 from ._native import Region
 
-U8, U16, F32 = 0, 1, 2
+U8, U16, F32, I16, U32, I32, F64 = 0, 1, 2, 3, 4, 5, 6
 NEAREST, BILINEAR, BICUBIC, LANCZOS2, LANCZOS3, CATMULLROM, LBB, VSQBS, NOHALO = range(9)
 FITBOX, FORCE_EXACT, WIDTH, HEIGHT = range(4)
 
@@ -40,7 +40,8 @@ E_SCRATCH_LIMIT = -11
 NOALLOC_SCRATCH_BYTES = 64 << 20
 OPT_SEP_ROWS_PER_TASK, OPT_SHRINK_RUN, OPT_GAUSS_ROWS = 2, 3, 4
 
-NP_DTYPE = {U8: np.uint8, U16: np.uint16, F32: np.float32}
+NP_DTYPE = {U8: np.uint8, U16: np.uint16, F32: np.float32, I16: np.int16, U32: np.uint32, I32: np.int32,
+            F64: np.float64}
 SAMPLE_BYTES = {U8: 1, U16: 2, F32: 4}
 
 

@@ -34,7 +34,9 @@ __global__ __launch_bounds__(kThreads) void k_affine(AffineJob j) {
     const OutView& out = j.out;
     constexpr int CAP = NB ? NB : 1;
     const int B = NB ? NB : j.bands;
-    const double amax = sizeof(T) == 1 ? 255.0 : (sizeof(T) == 2 ? 65535.0 : 1.0);
+    // Affine::alpha_max (affine/core.rs:101-109): the type's maximum for
+    // integer samples, 1 for f32 and f64.
+    const double amax = is_float_sample<T> ? 1.0 : (double)std::numeric_limits<T>::max();
     const T* src = static_cast<const T*>(in.ptr);
     // Block = 64 columns x kAffineRows rows: a lane keeps its column's tap
     // record for all its rows; a row's record is the same for the wavefront.
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(kThreads) void k_affine(AffineJob j) {
                         for (int b = 0; b + 1 < NB; ++b) u[b] = premult_u8(u[b], u[NB - 1]);
                     }
                 }
-                if constexpr (sizeof(T) == 4) {
+                if constexpr (!std::is_same<T, uint8_t>::value) {
 #pragma unroll
                     for (int b = 0; b < NB; ++b) p[b] = (double)q[b];
                 } else {
@@ -223,9 +225,9 @@ hipError_t launch_zoom(const InView& in, const OutView& out, int bands, int fmt,
     if (zx < 1 || zy < 1 || out.x < 0 || out.y < 0) return hipErrorInvalidValue;
     const size_t n = (size_t)out.w * out.h * bands;
     const unsigned g = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 65536);
-    if (fmt == 0) hipLaunchKernelGGL(k_zoom<uint8_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, zx, zy);
-    else if (fmt == 1) hipLaunchKernelGGL(k_zoom<uint16_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, zx, zy);
-    else hipLaunchKernelGGL(k_zoom<float>, dim3(g), dim3(kThreads), 0, s, in, out, bands, zx, zy);
+#define VGPU_ZOOM(T) hipLaunchKernelGGL(k_zoom<T>, dim3(g), dim3(kThreads), 0, s, in, out, bands, zx, zy)
+    VGPU_FOR_FORMAT(fmt, VGPU_ZOOM)
+#undef VGPU_ZOOM
     return hipGetLastError();
 }
 
@@ -234,9 +236,9 @@ hipError_t launch_affine(const AffineJob& j, hipStream_t s) {
     if (j.in_w <= 0 || j.in_h <= 0 || j.zx < 1 || j.zy < 1 || !j.xs || !j.ys) return hipErrorInvalidValue;
     const dim3 g((unsigned)((j.out.w + 63) / 64),
                  (unsigned)std::min<int64_t>(((int64_t)j.out.h + kAffineRows - 1) / kAffineRows, 65535));
-    if (j.fmt == 0) return launch_affine_k<uint8_t>(j, g, s);
-    if (j.fmt == 1) return launch_affine_k<uint16_t>(j, g, s);
-    return launch_affine_k<float>(j, g, s);
+#define VGPU_AFF(T) return launch_affine_k<T>(j, g, s)
+    VGPU_FOR_FORMAT(j.fmt, VGPU_AFF)
+#undef VGPU_AFF
 }
 
 }  // namespace vgpu

@@ -251,17 +251,19 @@ hipError_t launch_conv_pair(const ConvPairJob& job, int in_fmt, int kind, hipStr
     if (job.tw != kThreads / job.bands) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((job.out.w + job.tw - 1) / job.tw), (unsigned)((job.out.h + job.run - 1) / job.run));
     const size_t lds = conv_pair_lds(job);
-    if (kind == 0) {
-        if (in_fmt == 1) return launch_cp<uint16_t, 0>(job, grid, lds, s);
-        if (in_fmt == 2) return launch_cp<float, 0>(job, grid, lds, s);
-    } else if (kind == 1) {
+    // The Gaussian reads every sample as f32 (ToF32, gauss_blur.rs:248-293),
+    // so any non-u8 format stages exactly; ConvSep reads f64, which the f32
+    // staging holds exactly for u8, u16 and f32 only.
+    if (kind == 1) {
         if (in_fmt == 0) return launch_cp<uint8_t, 1>(job, grid, lds, s);
         if (in_fmt == 1) return launch_cp<uint16_t, 1>(job, grid, lds, s);
         if (in_fmt == 2) return launch_cp<float, 1>(job, grid, lds, s);
-    } else {
-        if (in_fmt == 1) return launch_cp<uint16_t, 2>(job, grid, lds, s);
-        if (in_fmt == 2) return launch_cp<float, 2>(job, grid, lds, s);
+        return hipErrorInvalidValue;
     }
+    if (in_fmt == 0 || (kind != 0 && kind != 2)) return hipErrorInvalidValue;
+#define VGPU_CPK(T) return kind == 0 ? launch_cp<T, 0>(job, grid, lds, s) : launch_cp<T, 2>(job, grid, lds, s)
+    VGPU_FOR_FORMAT(in_fmt, VGPU_CPK)
+#undef VGPU_CPK
     return hipErrorInvalidValue;
 }
 

@@ -4,6 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <limits>
+#include <type_traits>
+
 #include <cstdint>
 
 #include "kernels.hpp"
@@ -31,16 +34,73 @@ __device__ __forceinline__ int row_of(const InView& v, int y) {
     return clampi(clampi(y, v.cy0, v.cy1), v.by, v.by + v.bh - 1) - v.by;
 }
 
-// F::Sample::from_f64 (sample_conv.rs:58-93): integer samples clamp to the
-// type's range, then round half away from zero; f32 is a cast.
+// Range of an integer sample type (band.rs: U8, U16, I16, U32, I32).
+template <typename T>
+struct SampleRange {
+    static constexpr double lo = (double)std::numeric_limits<T>::min();
+    static constexpr double hi = (double)std::numeric_limits<T>::max();
+    static constexpr int64_t ilo = (int64_t)std::numeric_limits<T>::min();
+    static constexpr int64_t ihi = (int64_t)std::numeric_limits<T>::max();
+};
+template <typename T>
+constexpr bool is_float_sample = std::is_same<T, float>::value || std::is_same<T, double>::value;
+
+// Launch-time dispatch over the band formats (U8, U16, F32, I16, U32, I32,
+// F64 = 0..6): MACRO(T) with T the sample type; an unknown format returns
+// hipErrorInvalidValue from the enclosing launcher.
+#define VGPU_FOR_FORMAT(fmt, MACRO)                  \
+    switch (fmt) {                                   \
+    case 0: MACRO(uint8_t); break;                   \
+    case 1: MACRO(uint16_t); break;                  \
+    case 2: MACRO(float); break;                     \
+    case 3: MACRO(int16_t); break;                   \
+    case 4: MACRO(uint32_t); break;                  \
+    case 5: MACRO(int32_t); break;                   \
+    case 6: MACRO(double); break;                    \
+    default: return hipErrorInvalidValue;            \
+    }
+
+// F::Sample::from_f64 (viprs-core shared_ops/sample_conv.rs:53-99): integer
+// samples clamp to the type's range, then round half away from zero; f32 is a
+// cast, f64 the value itself.
 template <typename T>
 __device__ __forceinline__ T op_from_f64(double v) {
-    if constexpr (sizeof(T) == 4) {
-        return (T)(float)v;
+    if constexpr (std::is_same<T, float>::value) {
+        return (float)v;
+    } else if constexpr (std::is_same<T, double>::value) {
+        return v;
     } else {
-        const double mx = sizeof(T) == 1 ? 255.0 : 65535.0;
-        return (T)round(fmin(fmax(v, 0.0), mx));
+        return (T)round(fmin(fmax(v, SampleRange<T>::lo), SampleRange<T>::hi));
     }
+}
+
+// ShrinkSample::from_f64_clamped (shrinkh.rs:44-128): unsigned samples round
+// half away from zero, then clamp; signed ones clamp v + 0.5, then truncate
+// toward zero; f32 casts, f64 is kept.
+template <typename T>
+__device__ __forceinline__ T shrink_from_f64(double v) {
+    if constexpr (std::is_same<T, float>::value) {
+        return (float)v;
+    } else if constexpr (std::is_same<T, double>::value) {
+        return v;
+    } else if constexpr (std::is_signed<T>::value) {
+        return (T)trunc(fmin(fmax(v + 0.5, SampleRange<T>::lo), SampleRange<T>::hi));
+    } else {
+        return (T)fmin(fmax(round(v), SampleRange<T>::lo), SampleRange<T>::hi);
+    }
+}
+
+// ReduceSample::from_fixed_i64 (resample/sample_conv.rs:18-62) of a 12-bit
+// fixed-point sum: unsigned (acc + 2048) >> 12; signed adds 2048 to a
+// non-negative sum and subtracts it from a negative one before the
+// arithmetic shift; then the type's range.
+template <typename T, typename Acc>
+__device__ __forceinline__ T fixed_from_acc(Acc acc) {
+    Acc v;
+    if constexpr (std::is_signed<T>::value) v = acc >= 0 ? (acc + 2048) >> 12 : (acc - 2048) >> 12;
+    else v = (acc + 2048) >> 12;
+    const Acc lo = (Acc)SampleRange<T>::ilo, hi = (Acc)SampleRange<T>::ihi;
+    return (T)(v < lo ? lo : (v > hi ? hi : v));
 }
 
 template <typename T>

@@ -840,9 +840,10 @@ __global__ __launch_bounds__(kThreads) void k_shrink_batch(const ShrinkJob* __re
 // Generic single-axis kernels: one thread per output sample.
 // ---------------------------------------------------------------------------
 
-// ShrinkH/ShrinkV for u16/f32 (and u8 without the fused kernel):
-// f64 sum * (1/f) then ShrinkSample::from_f64_clamped (shrinkh.rs:49-128,271-300;
-// shrinkv.rs:168-187); u8 uses the integer roundings above.
+// ShrinkH/ShrinkV for every format other than u8 (and u8 without the fused
+// kernel): f64 sum * (1/f) then ShrinkSample::from_f64_clamped
+// (shrinkh.rs:44-128,271-300; shrinkv.rs:168-187); u8 uses the integer
+// roundings above.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_shrink_generic(InView in, OutView out, int B, int f,
                                                              int vertical, uint32_t mult) {
@@ -867,12 +868,7 @@ __global__ __launch_bounds__(kThreads) void k_shrink_generic(InView in, OutView 
                 const int col = col_of(in, vertical ? x : x * f + j);
                 sum += (double)load_at<T>(in, row, col, B, b);
             }
-            const double v = sum * (1.0 / (double)f);
-            if constexpr (sizeof(T) == 2) {
-                o[i] = (T)fmin(fmax(round(v), 0.0), 65535.0);
-            } else {
-                o[i] = (T)v;
-            }
+            o[i] = shrink_from_f64<T>(sum * (1.0 / (double)f));
         }
     }
 }
@@ -882,9 +878,8 @@ __global__ __launch_bounds__(kThreads) void k_shrink_generic(InView in, OutView 
 // [d*f, min(d*f + f, src)) per axis, divided by its own pixel count n
 // (software_box_shrink_generic, decoder_source/shrink.rs:47-117, and its
 // partial-window form in decode_png_with_box_shrink_u8, png/decode_full.rs:
-// 320-452).  u8: (sum + n/2) / n; u16: round(sum / n) half away from zero,
-// which for a ratio of integers below 2^32 is (2 sum + n) / (2 n); f32: the
-// f64 sum in row order, divided, cast.
+// 320-452).  u8: (sum + n/2) / n in integers; every other format: the f64
+// sum in row order, divided, ShrinkSample::from_f64_clamped.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_box_generic(const T* __restrict__ src, int sw, int sh, int B, int f,
                                                           T* __restrict__ dst, int dw, int x0, int y0, int rw,
@@ -897,16 +892,16 @@ __global__ __launch_bounds__(kThreads) void k_box_generic(const T* __restrict__ 
     const uint64_t n = max<int64_t>((sy1 - sy0) * (sx1 - sx0), 1);
     T* o = dst + ((size_t)dy * dw + dx) * B;
     for (int b = 0; b < B; ++b) {
-        if constexpr (sizeof(T) == 4) {
-            double sum = 0.0;
-            for (int64_t y = sy0; y < sy1; ++y)
-                for (int64_t x = sx0; x < sx1; ++x) sum += (double)src[((size_t)y * sw + x) * B + b];
-            o[b] = (T)(sum / (double)n);
-        } else {
+        if constexpr (sizeof(T) == 1) {
             uint64_t sum = 0;
             for (int64_t y = sy0; y < sy1; ++y)
                 for (int64_t x = sx0; x < sx1; ++x) sum += src[((size_t)y * sw + x) * B + b];
-            o[b] = sizeof(T) == 1 ? (T)((sum + n / 2) / n) : (T)((2 * sum + n) / (2 * n));
+            o[b] = (T)((sum + n / 2) / n);
+        } else {
+            double sum = 0.0;
+            for (int64_t y = sy0; y < sy1; ++y)
+                for (int64_t x = sx0; x < sx1; ++x) sum += (double)src[((size_t)y * sw + x) * B + b];
+            o[b] = shrink_from_f64<T>(sum / (double)n);
         }
     }
 }
@@ -916,7 +911,13 @@ template <typename T>
 __device__ __forceinline__ T reduce_h_sample(const InView& in, int row, int B, int b, int start,
                                              int ph, const DevPlan& p) {
     const int T_ = p.taps;
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (std::is_same<T, double>::value) {
+        // reduce_h_scalar (reduce_simd.rs:47-90): the FMA chain in tap order.
+        const double* w = p.wf64 + (size_t)ph * T_;
+        double acc = 0.0;
+        for (int t = 0; t < T_; ++t) acc = fma(load_at<double>(in, row, col_of(in, start + t), B, b), w[t], acc);
+        return acc;
+    } else if constexpr (std::is_same<T, float>::value) {
         const double* w = p.wf64 + (size_t)ph * T_;
         double acc = 0.0;
         int t = 0;
@@ -937,9 +938,7 @@ __device__ __forceinline__ T reduce_h_sample(const InView& in, int row, int B, i
         int64_t acc = 0;
         for (int t = 0; t < T_; ++t)
             acc += (int64_t)w[t] * (int64_t)load_at<T>(in, row, col_of(in, start + t), B, b);
-        const int64_t v = (acc + 2048) >> 12;
-        const int64_t mx = sizeof(T) == 1 ? 255 : 65535;
-        return (T)(v < 0 ? 0 : (v > mx ? mx : v));
+        return fixed_from_acc<T, int64_t>(acc);
     }
 }
 
@@ -947,7 +946,13 @@ __device__ __forceinline__ T reduce_h_sample(const InView& in, int row, int B, i
 template <typename T, typename Get>
 __device__ __forceinline__ T reduce_v_sample(Get get, int start, int ph, const DevPlan& p, bool vec) {
     const int T_ = p.taps;
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (std::is_same<T, double>::value) {
+        // reduce_v_scalar (reduce_simd.rs:93-144): the FMA chain in tap order.
+        const double* w = p.wf64 + (size_t)ph * T_;
+        double acc = 0.0;
+        for (int t = 0; t < T_; ++t) acc = fma(get(start + t), w[t], acc);
+        return acc;
+    } else if constexpr (std::is_same<T, float>::value) {
         const double* w = p.wf64 + (size_t)ph * T_;
         double acc = 0.0;
         if (vec) {
@@ -963,9 +968,7 @@ __device__ __forceinline__ T reduce_v_sample(Get get, int start, int ph, const D
         const int16_t* w = p.wi16 + (size_t)ph * T_;
         int64_t acc = 0;
         for (int t = 0; t < T_; ++t) acc += (int64_t)w[t] * (int64_t)get(start + t);
-        const int64_t v = (acc + 2048) >> 12;
-        const int64_t mx = sizeof(T) == 1 ? 255 : 65535;
-        return (T)(v < 0 ? 0 : (v > mx ? mx : v));
+        return fixed_from_acc<T, int64_t>(acc);
     }
 }
 
@@ -981,7 +984,8 @@ __device__ __forceinline__ T reduce_op_sample(Get get, int start, int ph, const 
 }
 
 // ReduceH (reduceh.rs:101-163; reduce_simd.rs:47-90,2463-2693): fixed-point
-// i16 taps for integer formats -> (acc + 2048) >> 12 clamped; f32 carries FP64:
+// i16 taps for integer formats -> from_fixed_i64; f64 the scalar FMA chain;
+// f32 carries FP64:
 // bands == 1 follows the AVX2 4-lane grouping (products summed per group, then
 // an FMA tail), other band counts the scalar FMA chain.
 // OP = true: ReduceOp's horizontal pass instead (reduce.rs:104-133).
@@ -1029,9 +1033,8 @@ __global__ __launch_bounds__(kThreads) void k_reduce_v(InView in, OutView out, i
 // max_phase(sum |w|) * max_sample < 2^31, else int64.
 template <typename T, typename Acc>
 __device__ __forceinline__ T fixed_round(Acc acc) {
-    const Acc v = (acc + 2048) >> 12;
-    const Acc mx = sizeof(T) == 1 ? 255 : 65535;
-    return (T)(v < 0 ? 0 : (v > mx ? mx : v));
+    static_assert(sizeof(T) <= 2, "the fused reduce takes u8 and u16 fixed-point sums");
+    return fixed_from_acc<T, Acc>(acc);
 }
 
 // Fused ReduceH -> ReduceV for B <= 4 bands.  Block = TX output columns x TY
@@ -1267,7 +1270,7 @@ __global__ __launch_bounds__(kThreads) void k_conv(InView in, OutView out, int B
         const size_t px = i / B;
         const int x = out.x + (int)(px % out.w), y = out.y + (int)(px / out.w);
         Tout* o = static_cast<Tout*>(out.ptr);
-        if (kind == 0 && sizeof(Tin) == 1 && sizeof(Tout) == 1) {
+        if (kind == 0 && std::is_same<Tin, uint8_t>::value && std::is_same<Tout, uint8_t>::value) {
             int32_t sum = 0;
             for (int t = 0; t < ntaps; ++t) {
                 const int row = row_of(in, vertical ? y - rad + t : y);
@@ -1530,9 +1533,10 @@ hipError_t launch_shrink_generic(const InView& in, const OutView& out, int bands
     if (n == 0) return hipSuccess;
     const uint32_t mult = (uint32_t)((1ull << 32) / (256ull * (uint64_t)factor));
     const unsigned g = grid_for(n);
-    if (fmt == 0) hipLaunchKernelGGL(k_shrink_generic<uint8_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, factor, vertical, mult);
-    else if (fmt == 1) hipLaunchKernelGGL(k_shrink_generic<uint16_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands, factor, vertical, mult);
-    else hipLaunchKernelGGL(k_shrink_generic<float>, dim3(g), dim3(kThreads), 0, s, in, out, bands, factor, vertical, mult);
+#define VGPU_SG(T) \
+    hipLaunchKernelGGL(k_shrink_generic<T>, dim3(g), dim3(kThreads), 0, s, in, out, bands, factor, vertical, mult)
+    VGPU_FOR_FORMAT(fmt, VGPU_SG)
+#undef VGPU_SG
     return hipGetLastError();
 }
 
@@ -1544,9 +1548,7 @@ hipError_t launch_box_generic(const void* src, int sw, int sh, int bands, int fm
 #define VGPU_BOX(T)                                                                                          \
     hipLaunchKernelGGL(k_box_generic<T>, dim3(g), dim3(kThreads), 0, s, static_cast<const T*>(src), sw, sh, \
                        bands, f, static_cast<T*>(dst), dw, x0, y0, rw, rh)
-    if (fmt == 0) VGPU_BOX(uint8_t);
-    else if (fmt == 1) VGPU_BOX(uint16_t);
-    else VGPU_BOX(float);
+    VGPU_FOR_FORMAT(fmt, VGPU_BOX)
 #undef VGPU_BOX
     return hipGetLastError();
 }
@@ -1560,7 +1562,7 @@ hipError_t launch_reduce_h(const InView& in, const OutView& out, int bands, int 
 #define VGPU_RP(TT)                                                                                          \
     if (op) hipLaunchKernelGGL((k_reduce_h<TT, true>), dim3(g), dim3(kThreads), 0, s, in, out, bands, plan); \
     else hipLaunchKernelGGL((k_reduce_h<TT, false>), dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
-    if (fmt == 0) { VGPU_RP(uint8_t) } else if (fmt == 1) { VGPU_RP(uint16_t) } else { VGPU_RP(float) }
+    VGPU_FOR_FORMAT(fmt, VGPU_RP)
 #undef VGPU_RP
     return hipGetLastError();
 }
@@ -1573,7 +1575,7 @@ hipError_t launch_reduce_v(const InView& in, const OutView& out, int bands, int 
 #define VGPU_RP(TT)                                                                                          \
     if (op) hipLaunchKernelGGL((k_reduce_v<TT, true>), dim3(g), dim3(kThreads), 0, s, in, out, bands, plan); \
     else hipLaunchKernelGGL((k_reduce_v<TT, false>), dim3(g), dim3(kThreads), 0, s, in, out, bands, plan);
-    if (fmt == 0) { VGPU_RP(uint8_t) } else if (fmt == 1) { VGPU_RP(uint16_t) } else { VGPU_RP(float) }
+    VGPU_FOR_FORMAT(fmt, VGPU_RP)
 #undef VGPU_RP
     return hipGetLastError();
 }
@@ -1611,9 +1613,9 @@ hipError_t launch_conv(const InView& in, const OutView& out, int bands, int in_f
         if (in_fmt != 0) return hipErrorInvalidValue;
         VGPU_CONV(uint8_t, uint8_t);
     } else if (out_fmt == 2) {
-        if (in_fmt == 0) VGPU_CONV(uint8_t, float);
-        else if (in_fmt == 1) VGPU_CONV(uint16_t, float);
-        else VGPU_CONV(float, float);
+#define VGPU_CONV_F32(TI) VGPU_CONV(TI, float)
+        VGPU_FOR_FORMAT(in_fmt, VGPU_CONV_F32)
+#undef VGPU_CONV_F32
     } else {
         return hipErrorInvalidValue;
     }
@@ -1632,9 +1634,14 @@ hipError_t launch_copy(const InView& in, const OutView& out, int bands, int fmt,
     const size_t n = (size_t)out.w * out.h * bands;
     if (n == 0) return hipSuccess;
     const unsigned g = grid_for(n);
-    if (fmt == 0) hipLaunchKernelGGL(k_copy<uint8_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands);
-    else if (fmt == 1) hipLaunchKernelGGL(k_copy<uint16_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands);
-    else hipLaunchKernelGGL(k_copy<float>, dim3(g), dim3(kThreads), 0, s, in, out, bands);
+    // By sample size: 1 (U8), 2 (U16, I16), 8 (F64) or 4 bytes.
+    switch (fmt == 0 ? 1 : (fmt == 1 || fmt == 3) ? 2 : fmt == 6 ? 8 : 4) {
+    case 1: hipLaunchKernelGGL(k_copy<uint8_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands); break;
+    case 2: hipLaunchKernelGGL(k_copy<uint16_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands); break;
+    case 4: hipLaunchKernelGGL(k_copy<uint32_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands); break;
+    case 8: hipLaunchKernelGGL(k_copy<uint64_t>, dim3(g), dim3(kThreads), 0, s, in, out, bands); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -13,7 +13,11 @@
 
 namespace vgpu {
 
-enum Format : int { U8 = 0, U16 = 1, F32 = 2 };
+// Band formats (crates/viprs-core/src/format/band.rs:19-35); the numbers are
+// this library's own (the three first ones predate the others).
+enum Format : int { U8 = 0, U16 = 1, F32 = 2, I16 = 3, U32 = 4, I32 = 5, F64 = 6 };
+// The formats the fused kernels take; the others run on the per-node kernels.
+inline bool base_format(int f) { return f == U8 || f == U16 || f == F32; }
 enum Kernel : int {
     NEAREST = 0,
     BILINEAR = 1,
@@ -39,7 +43,15 @@ enum Status : int {
     E_ADMISSION = -9,
 };
 
-inline size_t format_size(int f) { return f == U8 ? 1 : (f == U16 ? 2 : 4); }
+inline size_t format_size(int f) {
+    switch (f) {
+    case U8: return 1;
+    case U16:
+    case I16: return 2;
+    case F64: return 8;
+    default: return 4;
+    }
+}
 
 // Region on Z^2 (crates/viprs-core/src/image/region.rs:18-27), widened to 64
 // bits so halo arithmetic never wraps on the host.

@@ -636,6 +636,7 @@ bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const Redu
                             int fmt, uint32_t bands, const Rect& mid, const Rect& out,
                             const Rect& vclip, const Rect& clip, const Rect& buf,
                             const StageOptions& opt, Staging* st, Stage* s, bool reduce_op = false) {
+    if (!base_format(fmt)) return false;  // I16, U32, I32, F64: the per-node kernels
     std::vector<int32_t> hs, vs;
     std::vector<uint8_t> hp, vp;
     build_plan(hax, mid.x, mid.w, accumulate_h, &hs, &hp);
@@ -1205,7 +1206,7 @@ std::vector<Stage> plan_stages(const std::vector<Node>& nodes, const std::vector
             if (nd.kind == N_GAUSS_H && nv.kind == N_GAUSS_V && nd.in_fmt != U8 && nd.out_fmt == F32 &&
                 nv.out_fmt == F32 && opt.float_mode <= 1)
                 kind = opt.float_mode == 1 ? 2 : 0;
-            if (nd.kind == N_CONVSEP_H && nv.kind == N_CONVSEP_V) kind = 1;
+            if (nd.kind == N_CONVSEP_H && nv.kind == N_CONVSEP_V && base_format(nd.in_fmt)) kind = 1;
             if (kind >= 0 && stage_conv_pair_params(nd, nv, kind, out_r[(size_t)i + 1], opt, st, &s)) {
                 s.last = i + 1;
                 done = true;
@@ -1548,7 +1549,7 @@ int vgpu_pipeline_new(vgpu_context* ctx, uint32_t w, uint32_t h, uint32_t bands,
     if (!ctx || !out) return VGPU_E_INVALID_ARGUMENT;
     *out = nullptr;
     if (bands == 0) return VGPU_E_INVALID_ARGUMENT;
-    if (fmt < U8 || fmt > F32) return VGPU_E_UNSUPPORTED_FORMAT;
+    if (fmt < U8 || fmt > F64) return VGPU_E_UNSUPPORTED_FORMAT;
     if (w > INT32_MAX / 4 || h > INT32_MAX / 4) return VGPU_E_INVALID_ARGUMENT;
     auto* p = new (std::nothrow) vgpu_pipeline();
     if (!p) return VGPU_E_NO_MEMORY;
@@ -2411,7 +2412,7 @@ static Node base_node(NodeKind k, uint32_t bands, int fmt) {
     return nd;
 }
 
-static bool valid_fmt(int f) { return f >= U8 && f <= F32; }
+static bool valid_fmt(int f) { return f >= U8 && f <= F64; }
 
 extern "C" {
 
