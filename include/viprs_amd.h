@@ -69,7 +69,7 @@ typedef enum {
     VGPU_E_INVALID_ARGUMENT = -1,
     VGPU_E_INVALID_KERNEL = -2,
     VGPU_E_UNSUPPORTED_FORMAT = -3,
-    VGPU_E_UNSUPPORTED = -4,  /* plan needs a node outside this path (LBB/VSQBS resize, an op after an Affine tail) */
+    VGPU_E_UNSUPPORTED = -4,  /* plan needs a node outside this path (an op after an Affine tail) */
     VGPU_E_BUFFER_SIZE = -5,  /* a buffer length does not match the region */
     VGPU_E_DEVICE = -6,       /* HIP runtime error (sticky on the context) */
     VGPU_E_NO_MEMORY = -7,
@@ -204,8 +204,12 @@ int vgpu_pipeline_thumbnail_shrink_on_load(vgpu_pipeline *p, int target, uint32_
                                            int kernel, int crop);
 /* PipelineBuilder::resize(Resize::new(hscale, vscale, kernel))
  * (builder/resample.rs:637-711; resize.rs:163-284): shrink and reduce for
- * downscales, Zoom and the Affine fix-up for upscales.  The LBB and VSQBS
- * interpolators return VGPU_E_UNSUPPORTED. */
+ * downscales, Zoom and the Affine fix-up for upscales.  LBB is one Affine
+ * with the locally bounded bicubic stencil over the whole image
+ * (resize.rs:176-199); VSQBS never zooms and its Affine takes the 4 x 4
+ * quadratic B-spline stencil, while a VSQBS downscale that needs a reduce
+ * step fails with VGPU_E_INVALID_KERNEL, as ReduceH / ReduceV refuse the
+ * kernel (reduce_common.rs:107-113). */
 int vgpu_pipeline_resize(vgpu_pipeline *p, double hscale, double vscale, int kernel);
 
 /* Kernel choice for u8 separable pairs, a bit mask: bit 0 puts ReduceH ->

@@ -630,9 +630,7 @@ int orc_pipeline_affine(orc_pipeline *p, double a, double d, double tx, double t
         if (!isfinite(m[i])) return ORC_EINVAL;
     if (!isfinite(tx) || !isfinite(ty)) return ORC_EINVAL;
     if (fabs(fma(m[1], -m[2], m[0] * m[3])) <= DBL_EPSILON) return ORC_EINVAL;
-    if (kernel != ORC_NEAREST && kernel != ORC_BILINEAR && kernel != ORC_BICUBIC && kernel != ORC_CATMULLROM &&
-        kernel != ORC_LANCZOS2 && kernel != ORC_LANCZOS3)
-        return ORC_EUNSUPPORTED;
+    if (kernel < ORC_NEAREST || kernel > ORC_VSQBS) return ORC_EUNSUPPORTED; /* Nohalo is not restated */
     node_t *nd = push(p, N_AFFINE);
     if (!nd) return ORC_EINVAL;
     memcpy(nd->am, m, sizeof(m));
@@ -1030,19 +1028,33 @@ int orc_pipeline_thumbnail_shrink_on_load(orc_pipeline *p, int target, uint32_t 
     return orc_pipeline_thumbnail_crop(p, target, tw, th, kernel, crop);
 }
 
-/* ResizeOp::into_pipeline_nodes (resize.rs:163-284) without the LBB / VSQBS
- * interpolators, and PipelineBuilder::resize (builder/resample.rs:637-711). */
+/* ResizeOp::into_pipeline_nodes (resize.rs:163-284) and PipelineBuilder::resize
+ * (builder/resample.rs:637-711). */
 int orc_pipeline_resize(orc_pipeline *p, double hscale, double vscale, int kernel) {
     if (p->bands == 0) return ORC_EINVAL;
     if (!(isfinite(hscale) && hscale > 0.0)) hscale = 1.0;
     if (!(isfinite(vscale) && vscale > 0.0)) vscale = 1.0;
     if (kernel == ORC_NOHALO) kernel = ORC_BICUBIC;
-    if (kernel == ORC_LBB || kernel == ORC_VSQBS) return ORC_EUNSUPPORTED;
     const double gap = 2.0;
     uint32_t iw = p->cur_w, ih = p->cur_h;
     if (iw == 0 || ih == 0) return ORC_EUNSUPPORTED;
     double a = round((double)iw * hscale), b = round((double)ih * vscale);
     uint32_t tw = sat_u32(a > 1.0 ? a : 1.0), th = sat_u32(b > 1.0 ? b : 1.0);
+    int rc;
+    if (kernel == ORC_LBB) {
+        /* resize.rs:176-199: the whole resize is one LBB Affine */
+        if (tw != iw || th != ih) {
+            const double ax = (double)tw / (double)iw, ay = (double)th / (double)ih;
+            rc = orc_pipeline_affine(p, 1.0 / ax, 1.0 / ay, 0.5 * (1.0 - 1.0 / ax), 0.5 * (1.0 - 1.0 / ay), tw, th,
+                                     kernel, 1);
+            if (rc != ORC_OK) return rc;
+        } else {
+            node_t *nd = push(p, N_COPY);
+            if (!nd) return ORC_EINVAL;
+            commit(p, nd);
+        }
+        return ORC_OK;
+    }
     uint32_t cw = iw, ch = ih;
     uint32_t hsh = 1, vsh = 1;
     if (tw < cw && kernel != ORC_NEAREST) {
@@ -1053,7 +1065,7 @@ int orc_pipeline_resize(orc_pipeline *p, double hscale, double vscale, int kerne
         double v = floor((double)ch / (double)th / gap);
         vsh = sat_u32(v > 1.0 ? v : 1.0);
     }
-    int any = 0, rc;
+    int any = 0;
     if (vsh > 1) {
         if ((rc = orc_pipeline_shrink_v(p, vsh, 0)) != ORC_OK) return rc;
         ch /= vsh;
@@ -1074,9 +1086,12 @@ int orc_pipeline_resize(orc_pipeline *p, double hscale, double vscale, int kerne
         cw = tw;
         any = 1;
     }
-    /* resize.rs:240-280: integer Zoom of what remains to grow, then the
-     * Affine fix-up (SmallTile demand, builder/resample.rs:690-706). */
-    const uint32_t zx = tw > cw ? tw / cw : 1, zy = th > ch ? th / ch : 1;
+    /* resize.rs:240-280: integer Zoom of what remains to grow (not for
+     * VSQBS, resize.rs:241-243), then the Affine fix-up (SmallTile demand,
+     * builder/resample.rs:690-706).  A VSQBS reduce above fails with
+     * ORC_EKERNEL (reduce_common.rs:107-113). */
+    const int zoom_ok = kernel != ORC_VSQBS;
+    const uint32_t zx = zoom_ok && tw > cw ? tw / cw : 1, zy = zoom_ok && th > ch ? th / ch : 1;
     if (zx > 1 || zy > 1) {
         node_t *nd = push(p, N_ZOOM);
         if (!nd) return ORC_EINVAL;
@@ -1619,6 +1634,97 @@ static double bilinear_mix(double p00, double p01, double p10, double p11, doubl
 /* One band at (x, y): interp_nearest / interp_bilinear / interp_separable
  * for the cubic and Lanczos kernels (affine/core.rs:485-514,648-697,728-749), and their premultiplied
  * forms (sample_at_premultiplied, affine/core.rs:751-935,937-986) when pm. */
+/* interpolate_vsqbs_2d (viprs-core kernel.rs:863-923) over n[row][col]. */
+static double vsqbs_eval(double xph, double yph, double n[4][4]) {
+    int ax, sx, ay, sy;
+    double x0, y0;
+    if (xph < 0.5) { ax = 1; sx = 1; x0 = xph; } else { ax = 2; sx = -1; x0 = xph - 1.0; }
+    if (yph < 0.5) { ay = 1; sy = 1; y0 = yph; } else { ay = 2; sy = -1; y0 = yph - 1.0; }
+    const double x = (double)(2 * sx) * x0 - 0.5, y = (double)(2 * sy) * y0 - 0.5;
+    const double cent = 0.75 - x * x, mid = 0.75 - y * y;
+    const double left = fma(-0.5, x + cent, 0.5), top = fma(-0.5, y + mid, 0.5);
+    const double lc = left + cent, tm = top + mid, cr = 1.0 - left, mb = 1.0 - top;
+    const double rite = 1.0 - lc, bot = 1.0 - tm;
+    const double k_t_m = lc * top, k_m_l = left * tm, k_m_m = lc + tm, k_m_r = cr * tm + rite;
+    const double k_b_m = mb * lc + bot, k_b_r = mb * rite + cr * bot;
+    const double k_t_r = top - k_t_m, k_b_l = left - k_m_l;
+    const int T = ay - sy, M = ay, Bo = ay + sy, L = ax - sx, C = ax, R = ax + sx;
+    return (((k_t_m * n[T][C] + k_m_l * n[M][L]) + (k_m_m * n[M][C] + k_m_r * n[M][R])) +
+            ((k_b_m * n[Bo][C] + k_b_r * n[Bo][R]) + (k_t_r * n[T][R] + k_b_l * n[Bo][L]))) *
+           0.25;
+}
+
+static inline double lmin(double a, double b) { return a <= b ? a : b; }
+static inline double lmax(double a, double b) { return a >= b ? a : b; }
+static inline double lsgn(double a) { return a >= 0.0 ? 1.0 : -1.0; }
+
+/* One corner's limited slopes and cross derivative of lbbicubic
+ * (affine/interpolation.rs:170-378): z its value, lo / hi its local bounds,
+ * gx / gy the raw central differences, xy the raw cross difference. */
+static void lbb_corner(double z, double lo, double hi, double gx, double gy, double xy, double *dx, double *dy,
+                       double *q, double *u, double *v) {
+    *u = z - lo;
+    *v = hi - z;
+    const double lim = 6.0 * lmin(*u, *v);
+    *dx = lsgn(gx) * gx <= lim ? gx : lsgn(gx) * lim;
+    *dy = lsgn(gy) * gy <= lim ? gy : lsgn(gy) * lim;
+    const double as = fabs(6.0 * (*dx + *dy)), ad = fabs(6.0 * (*dx - *dy));
+    const double u36 = 36.0 * *u, v36 = 36.0 * *v;
+    double t = lmax(xy, as - u36);
+    t = lmin(t, v36 - as);
+    t = lmax(t, ad - v36);
+    *q = lmin(t, u36 - ad);
+}
+
+/* lbbicubic (affine/interpolation.rs:170-441) over n[row][col], [1][1] at
+ * (floor x, floor y), phases (rx, ry). */
+static double lbb_eval(double n[4][4], double rx, double ry) {
+    /* ordered central pairs of the two middle rows, and the arm pairs */
+    const double a1 = n[1][1] <= n[1][2] ? n[1][1] : n[1][2], b1 = n[1][1] <= n[1][2] ? n[1][2] : n[1][1];
+    const double a2 = n[2][1] <= n[2][2] ? n[2][1] : n[2][2], b2 = n[2][1] <= n[2][2] ? n[2][2] : n[2][1];
+    const double p3 = n[0][1] <= n[1][0] ? n[0][1] : n[1][0], P3 = n[0][1] <= n[1][0] ? n[1][0] : n[0][1];
+    const double p4 = n[0][2] <= n[1][3] ? n[0][2] : n[1][3], P4 = n[0][2] <= n[1][3] ? n[1][3] : n[0][2];
+    const double p5 = n[2][0] <= n[3][1] ? n[2][0] : n[3][1], P5 = n[2][0] <= n[3][1] ? n[3][1] : n[2][0];
+    const double p6 = n[2][3] <= n[3][2] ? n[2][3] : n[3][2], P6 = n[2][3] <= n[3][2] ? n[3][2] : n[2][3];
+    const double lo00 = lmin(lmin(a1, n[2][1]), p3), hi00 = lmax(lmax(b1, n[2][1]), P3);
+    const double lo10 = lmin(lmin(a1, n[2][2]), p4), hi10 = lmax(lmax(b1, n[2][2]), P4);
+    const double lo01 = lmin(lmin(a2, n[1][1]), p5), hi01 = lmax(lmax(b2, n[1][1]), P5);
+    const double lo11 = lmin(lmin(a2, n[1][2]), p6), hi11 = lmax(lmax(b2, n[1][2]), P6);
+    const double gx00 = n[1][2] - n[1][0], gx10 = n[1][3] - n[1][1];
+    const double gx01 = n[2][2] - n[2][0], gx11 = n[2][3] - n[2][1];
+    const double gy00 = n[2][1] - n[0][1], gy10 = n[2][2] - n[0][2];
+    const double gy01 = n[3][1] - n[1][1], gy11 = n[3][2] - n[1][2];
+    const double xy00 = n[0][0] - n[0][2] + gx01, xy10 = n[0][1] - n[0][3] + gx11;
+    const double xy01 = n[3][2] - n[3][0] - gx00, xy11 = n[3][3] - n[3][1] - gx10;
+    double dx[4], dy[4], q[4], u, v;
+    lbb_corner(n[1][1], lo00, hi00, gx00, gy00, xy00, &dx[0], &dy[0], &q[0], &u, &v);
+    lbb_corner(n[1][2], lo10, hi10, gx10, gy10, xy10, &dx[1], &dy[1], &q[1], &u, &v);
+    lbb_corner(n[2][1], lo01, hi01, gx01, gy01, xy01, &dx[2], &dy[2], &q[2], &u, &v);
+    lbb_corner(n[2][2], lo11, hi11, gx11, gy11, xy11, &dx[3], &dy[3], &q[3], &u, &v);
+    /* Hermite basis */
+    const double xa = rx, xb = rx - 1.0, ya = ry, yb = ry - 1.0;
+    const double ta = (0.5 + xa) + (0.5 + xa), tb = (1.5 - xa) + (1.5 - xa);
+    const double sa = (0.5 + ya) + (0.5 + ya), sb = (1.5 - ya) + (1.5 - ya);
+    const double xa2 = xa * xa, xb2 = xb * xb, ya2 = ya * ya, yb2 = yb * yb;
+    const double e00 = xb2 * yb2, e10 = xa2 * yb2, e01 = xb2 * ya2, e11 = xa2 * ya2;
+    const double v1 = (ta * sa) * e00 * n[1][1] + (tb * sa) * e10 * n[1][2] + (ta * sb) * e01 * n[2][1] +
+                      (tb * sb) * e11 * n[2][2];
+    const double v2 = (xa * sa) * e00 * dx[0] + (xb * sa) * e10 * dx[1] + (xa * sb) * e01 * dx[2] +
+                      (xb * sb) * e11 * dx[3] + (ta * ya) * e00 * dy[0] + (tb * ya) * e10 * dy[1] +
+                      (ta * yb) * e01 * dy[2] + (tb * yb) * e11 * dy[3];
+    const double v3 = (xa * ya) * e00 * q[0] + (xb * ya) * e10 * q[1] + (xa * yb) * e01 * q[2] +
+                      (xb * yb) * e11 * q[3];
+    return fma(0.25, v3, fma(0.5, v2, v1));
+}
+
+/* The two 2-D stencil interpolators alone, for the known-answer tests:
+ * n16 is the 4 x 4 neighbourhood row by row. */
+double orc_interp_2d(int kernel, const double *n16, double xph, double yph) {
+    double n[4][4];
+    for (int i = 0; i < 16; ++i) n[i / 4][i % 4] = n16[i];
+    return kernel == ORC_VSQBS ? vsqbs_eval(xph, yph, n) : lbb_eval(n, xph, yph);
+}
+
 static double affine_band(const node_t *nd, const void *in, rgn ir, double x, double y, int band, int pm,
                           double amax) {
     if (nd->akern == ORC_NEAREST) return affine_sample_pm(nd, in, ir, (int64_t)x, (int64_t)y, band, pm, amax);
@@ -1629,6 +1735,15 @@ static double affine_band(const node_t *nd, const void *in, rgn ir, double x, do
                             affine_sample_pm(nd, in, ir, x0 + 1, y0, band, pm, amax),
                             affine_sample_pm(nd, in, ir, x0, y0 + 1, band, pm, amax),
                             affine_sample_pm(nd, in, ir, x0 + 1, y0 + 1, band, pm, amax), fx, fy);
+    }
+    if (nd->akern == ORC_VSQBS || nd->akern == ORC_LBB) {
+        /* interp_vsqbs / interp_lbb (affine/core.rs:588-610,699-725) */
+        const double fx = floor(x), fy = floor(y);
+        const int64_t ix = (int64_t)fx, iy = (int64_t)fy;
+        double n[4][4];
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) n[r][c] = affine_sample_pm(nd, in, ir, ix + c - 1, iy + r - 1, band, pm, amax);
+        return nd->akern == ORC_VSQBS ? vsqbs_eval(x - fx, y - fy, n) : lbb_eval(n, x - fx, y - fy);
     }
     /* interp_separable<SUPPORT>: SUPPORT 3 for Lanczos3, else 2; taps
      * floor - SUPPORT + 1 .. floor + SUPPORT */

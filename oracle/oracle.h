@@ -132,6 +132,10 @@ void orc_synthetic_bytes(uint8_t *dst, size_t n, uint64_t seed);
 void orc_convert_fixed(int64_t v, int fmt, void *out);
 void orc_convert_from_f64(double v, int fmt, void *out);
 void orc_convert_shrunk(double v, int fmt, void *out);
+/* interpolate_vsqbs_2d (kernel == ORC_VSQBS) or lbbicubic (ORC_LBB) over a
+ * 4 x 4 neighbourhood given row by row, [1][1] at the floor of the position,
+ * with phases (xph, yph). */
+double orc_interp_2d(int kernel, const double *n16, double xph, double yph);
 
 #ifdef __cplusplus
 }

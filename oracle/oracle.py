@@ -85,6 +85,8 @@ def lib() -> C.CDLL:
         L.orc_f64_gauss.argtypes = [vp, u32, u32, u32, C.c_float, vp, i32]
         L.orc_synthetic_bytes.argtypes = [vp, C.c_size_t, C.c_uint64]
         L.orc_synthetic_bytes.restype = None
+        L.orc_interp_2d.argtypes = [i32, vp, dbl, dbl]
+        L.orc_interp_2d.restype = dbl
         for name, arg in (("orc_convert_fixed", C.c_int64), ("orc_convert_from_f64", dbl),
                           ("orc_convert_shrunk", dbl)):
             getattr(L, name).argtypes = [arg, i32, vp]
@@ -249,6 +251,12 @@ class Pipeline:
             threads = os.cpu_count() or 1
         _check(lib().orc_pipeline_run(self._p, _ptr(src), _ptr(dst), threads, strip_height), "run")
         return dst
+
+
+def interp_2d(kernel: int, neighbourhood, x_phase: float, y_phase: float) -> float:
+    """VSQBS or LBB over a 4 x 4 neighbourhood (rows first)."""
+    n = np.ascontiguousarray(neighbourhood, np.float64).reshape(16)
+    return lib().orc_interp_2d(kernel, _ptr(n), x_phase, y_phase)
 
 
 def convert(kind: str, value, fmt: int):

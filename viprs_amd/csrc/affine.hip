@@ -12,6 +12,9 @@
 //    interp_separable (affine/core.rs:648-697): rows outer, columns inner,
 //    taps with a zero weight skipped, acc = fma(p, wx * wy, acc), then
 //    acc / sum of weights;
+//  * VSQBS and LBB: the 4 x 4 stencil from (floor x - 1, floor y - 1) and
+//    the phases x - floor x, y - floor y (interp_vsqbs, interp_lbb,
+//    affine/core.rs:588-610,699-725);
 //  * four bands: sample_pixel_at_with_alpha (affine/core.rs:988-1020), every
 //    sample of a colour band multiplied by (its alpha / alpha_max) first, the
 //    result divided by the interpolated alpha / alpha_max.
@@ -23,6 +26,119 @@ namespace vgpu {
 namespace {
 
 constexpr int kAffineRows = 16;
+
+// VSQBS over a 4 x 4 neighbourhood nb[row][col] whose [1][1] sample sits at
+// (floor x, floor y) (interpolate_vsqbs_2d, kernel.rs:863-923): per axis the
+// phase picks the anchor (column / row 1 below one half, else 2 with the
+// stencil mirrored) and the distance x0 to it; from the vertex-split
+// quadratic B-spline weights of 2|x0| - 1/2 eight taps around the anchor are
+// weighted and summed in pairs, times 1/4.
+__device__ __forceinline__ double vsqbs_2d(double xp, double yp, const double (&nb)[4][4]) {
+    const bool xlo = xp < 0.5, ylo = yp < 0.5;
+    const int ax = xlo ? 1 : 2, sx = xlo ? 1 : -1;
+    const int ay = ylo ? 1 : 2, sy = ylo ? 1 : -1;
+    const double x0 = xlo ? xp : xp - 1.0, y0 = ylo ? yp : yp - 1.0;
+    const double x = (double)(2 * sx) * x0 - 0.5;
+    const double y = (double)(2 * sy) * y0 - 0.5;
+    const double cent = 0.75 - x * x;
+    const double mid = 0.75 - y * y;
+    const double left = fma(-0.5, x + cent, 0.5);
+    const double top = fma(-0.5, y + mid, 0.5);
+    const double left_cent = left + cent, top_mid = top + mid;
+    const double cent_right = 1.0 - left, mid_bottom = 1.0 - top;
+    const double right = 1.0 - left_cent, bottom = 1.0 - top_mid;
+    // four times the weights of: (top, mid) (mid, left) (mid, mid)
+    // (mid, right) (bottom, mid) (bottom, right) (top, right) (bottom, left)
+    const double w_tm = left_cent * top;
+    const double w_ml = left * top_mid;
+    const double w_mm = left_cent + top_mid;
+    const double w_mr = cent_right * top_mid + right;
+    const double w_bm = mid_bottom * left_cent + bottom;
+    const double w_br = mid_bottom * right + cent_right * bottom;
+    const double w_tr = top - w_tm;
+    const double w_bl = left - w_ml;
+    const int rt = ay - sy, rm = ay, rb = ay + sy;
+    const int cl = ax - sx, cm = ax, cr = ax + sx;
+    return (((w_tm * nb[rt][cm] + w_ml * nb[rm][cl]) + (w_mm * nb[rm][cm] + w_mr * nb[rm][cr])) +
+            ((w_bm * nb[rb][cm] + w_br * nb[rb][cr]) + (w_tr * nb[rt][cr] + w_bl * nb[rb][cl]))) *
+           0.25;
+}
+
+// LBB, the locally bounded bicubic Hermite interpolator over s[row][col]
+// with [1][1] at (floor x, floor y) and phases (rx, ry) (lbbicubic,
+// affine/interpolation.rs:170-441).  Per corner c of the central 2 x 2 cell
+// (00, 10, 01, 11: x then y): the local min / max over the corner, its
+// neighbours and two arm samples; central-difference slopes limited to six
+// times the distance to those bounds; the cross derivative clamped by four
+// limits; then the Hermite basis of (rx, ry) over value, slopes and cross
+// derivative, each of the three sums in corner order.
+__device__ __forceinline__ double lbb_min(double a, double b) { return a <= b ? a : b; }
+__device__ __forceinline__ double lbb_max(double a, double b) { return a >= b ? a : b; }
+__device__ __forceinline__ double lbb_sign(double a) { return a >= 0.0 ? 1.0 : -1.0; }
+
+__device__ __forceinline__ double lbb_2d(const double (&s)[4][4], double rx, double ry) {
+    // the central pair of each middle row, ordered
+    const bool r1 = s[1][1] <= s[1][2], r2 = s[2][1] <= s[2][2];
+    const double lo1 = r1 ? s[1][1] : s[1][2], hi1 = r1 ? s[1][2] : s[1][1];
+    const double lo2 = r2 ? s[2][1] : s[2][2], hi2 = r2 ? s[2][2] : s[2][1];
+    double z[4], u[4], v[4], dxi[4], dyi[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int cx = c & 1, cy = c >> 1;
+        // the row pair of the corner's row, the middle-row sample below or
+        // above it, and the corner's two arm samples (a <= b picks a)
+        const double rlo = cy ? lo2 : lo1, rhi = cy ? hi2 : hi1;
+        const double other = s[2 - cy][1 + cx];
+        const double a = cy ? s[2][cx ? 3 : 0] : s[0][1 + cx];
+        const double b = cy ? s[3][1 + cx] : s[1][cx ? 3 : 0];
+        const double alo = a <= b ? a : b, ahi = a <= b ? b : a;
+        z[c] = s[1 + cy][1 + cx];
+        u[c] = z[c] - lbb_min(lbb_min(rlo, other), alo);
+        v[c] = lbb_max(lbb_max(rhi, other), ahi) - z[c];
+        dxi[c] = s[1 + cy][2 + cx] - s[1 + cy][cx];
+        dyi[c] = s[2 + cy][1 + cx] - s[cy][1 + cx];
+    }
+    double dx[4], dy[4], q[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int cx = c & 1, cy = c >> 1;
+        const double qi = cy ? (s[3][cx + 2] - s[3][cx]) - dxi[cx] : (s[0][cx] - s[0][cx + 2]) + dxi[2 + cx];
+        const double lim = 6.0 * lbb_min(u[c], v[c]);
+        const double sgx = lbb_sign(dxi[c]), sgy = lbb_sign(dyi[c]);
+        dx[c] = sgx * dxi[c] <= lim ? dxi[c] : sgx * lim;
+        dy[c] = sgy * dyi[c] <= lim ? dyi[c] : sgy * lim;
+        const double ts = fabs(6.0 * (dx[c] + dy[c]));
+        const double td = fabs(6.0 * (dx[c] - dy[c]));
+        const double u36 = 36.0 * u[c], v36 = 36.0 * v[c];
+        double qq = lbb_max(qi, ts - u36);
+        qq = lbb_min(qq, v36 - ts);
+        qq = lbb_max(qq, td - v36);
+        q[c] = lbb_min(qq, u36 - td);
+    }
+    // Hermite basis: per axis the value weight 2(1/2 + x) or 2(3/2 - x), the
+    // slope weight x or x - 1, and the squared distance to the far side.
+    const double xp = rx, xm = xp - 1.0, yp = ry, ym = yp - 1.0;
+    const double opx = 0.5 + xp, omx = 1.5 - xp, opy = 0.5 + yp, omy = 1.5 - yp;
+    const double xpsq = xp * xp, ypsq = yp * yp, xmsq = xm * xm, ymsq = ym * ym;
+    const double tpx = opx + opx, tpy = opy + opy, tmx = omx + omx, tmy = omy + omy;
+    double cv[4], cdx[4], cdy[4], cxy[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int cx = c & 1, cy = c >> 1;
+        const double tX = cx ? tmx : tpx, sX = cx ? xm : xp, qX = cx ? xpsq : xmsq;
+        const double tY = cy ? tmy : tpy, sY = cy ? ym : yp, qY = cy ? ypsq : ymsq;
+        const double sq = qX * qY;
+        cv[c] = (tX * tY) * sq;
+        cdx[c] = (sX * tY) * sq;
+        cdy[c] = (tX * sY) * sq;
+        cxy[c] = (sX * sY) * sq;
+    }
+    const double n1 = cv[0] * z[0] + cv[1] * z[1] + cv[2] * z[2] + cv[3] * z[3];
+    const double n2 = cdx[0] * dx[0] + cdx[1] * dx[1] + cdx[2] * dx[2] + cdx[3] * dx[3] + cdy[0] * dy[0] +
+                      cdy[1] * dy[1] + cdy[2] * dy[2] + cdy[3] * dy[3];
+    const double n3 = cxy[0] * q[0] + cxy[1] * q[1] + cxy[2] * q[2] + cxy[3] * q[3];
+    return fma(0.25, n3, fma(0.5, n2, n1));
+}
 
 // NB = band count (1 .. 4), or 0 for any count (read from the job).  The
 // weight product wx * wy and a tap's alpha / alpha_max are computed once per
@@ -67,6 +183,13 @@ __global__ __launch_bounds__(kThreads) void k_affine(AffineJob j) {
                     const double top = fma((double)src[(r0 + c1) * B + b], fx, (double)src[(r0 + c0) * B + b] * (1.0 - fx));
                     const double bot = fma((double)src[(r1 + c1) * B + b], fx, (double)src[(r1 + c0) * B + b] * (1.0 - fx));
                     r = top * (1.0 - fy) + bot * fy;
+                } else if constexpr (K == AFF_VSQBS || K == AFF_LBB) {
+                    double nb[4][4];
+                    for (int ky = 0; ky < 4; ++ky) {
+                        const size_t rr = row(ty.i0 + ky);
+                        for (int kx = 0; kx < 4; ++kx) nb[ky][kx] = (double)src[(rr + col(tx.i0 + kx)) * B + b];
+                    }
+                    r = K == AFF_VSQBS ? vsqbs_2d(tx.f, ty.f, nb) : lbb_2d(nb, tx.f, ty.f);
                 } else {
                     constexpr int NT = K == AFF_SEP6 ? 6 : 4;
                     double acc = 0.0, wsum = 0.0;
@@ -131,6 +254,25 @@ __global__ __launch_bounds__(kThreads) void k_affine(AffineJob j) {
                     const double top = fma(p01[b], fx, p00[b] * (1.0 - fx));
                     const double bot = fma(p11[b], fx, p10[b] * (1.0 - fx));
                     v[b] = top * (1.0 - fy) + bot * fy;
+                }
+            } else if constexpr (K == AFF_VSQBS || K == AFF_LBB) {
+                // the 4 x 4 stencil of every band (premultiplied samples
+                // for four bands), then each band's interpolation
+                double p[4][4][NB];
+#pragma unroll
+                for (int ky = 0; ky < 4; ++ky) {
+                    const size_t rr = row(ty.i0 + ky);
+#pragma unroll
+                    for (int kx = 0; kx < 4; ++kx) load(rr, col(tx.i0 + kx), p[ky][kx]);
+                }
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    double nb[4][4];
+#pragma unroll
+                    for (int ky = 0; ky < 4; ++ky)
+#pragma unroll
+                        for (int kx = 0; kx < 4; ++kx) nb[ky][kx] = p[ky][kx][b];
+                    v[b] = K == AFF_VSQBS ? vsqbs_2d(tx.f, ty.f, nb) : lbb_2d(nb, tx.f, ty.f);
                 }
             } else {
                 constexpr int NT = K == AFF_SEP6 ? 6 : 4;
@@ -202,6 +344,8 @@ hipError_t launch_affine_k(const AffineJob& j, dim3 g, hipStream_t s) {
     case AFF_BILINEAR: launch_affine_t<T, AFF_BILINEAR>(j, g, s); break;
     case AFF_SEP4: launch_affine_t<T, AFF_SEP4>(j, g, s); break;
     case AFF_SEP6: launch_affine_t<T, AFF_SEP6>(j, g, s); break;
+    case AFF_VSQBS: launch_affine_t<T, AFF_VSQBS>(j, g, s); break;
+    case AFF_LBB: launch_affine_t<T, AFF_LBB>(j, g, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -40,13 +40,16 @@ enum Alpha : int { ALPHA_NONE = 0, ALPHA_PREMULT = 1, ALPHA_UNPREMULT = 2 };
 
 // One output column (or row) of an axis-aligned Affine: the first source
 // index read (trunc(x) for Nearest and Bilinear, floor(x) - S + 1 for the
-// separable kernels of half-width S), the Bilinear phase x - trunc(x), and
+// separable kernels of half-width S, floor(x) - 1 for the 4 x 4 stencils),
+// the phase (x - trunc(x) for Bilinear, x - floor(x) for the stencils), and
 // the separable weights of source indices i0 .. i0 + 2S - 1.
 enum AffineKernel : int {
     AFF_NEAREST = 0,
     AFF_BILINEAR = 1,
-    AFF_SEP4 = 2,  // Bicubic, CatmullRom, Lanczos2: interp_separable<2>
-    AFF_SEP6 = 3,  // Lanczos3: interp_separable<3>
+    AFF_SEP4 = 2,   // Bicubic, CatmullRom, Lanczos2: interp_separable<2>
+    AFF_SEP6 = 3,   // Lanczos3: interp_separable<3>
+    AFF_VSQBS = 4,  // interp_vsqbs: 4 x 4 stencil, quadratic B-spline (vertex split)
+    AFF_LBB = 5,    // interp_lbb: 4 x 4 stencil, locally bounded bicubic
 };
 struct AffineTap {
     int32_t i0 = 0, pad = 0;

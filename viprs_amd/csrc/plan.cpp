@@ -459,11 +459,28 @@ Status plan_resize(uint32_t iw, uint32_t ih, double hscale, double vscale, int k
     if (!(std::isfinite(hscale) && hscale > 0.0)) hscale = 1.0;
     if (!(std::isfinite(vscale) && vscale > 0.0)) vscale = 1.0;
     if (kernel == NOHALO) kernel = BICUBIC;
-    if (kernel == LBB || kernel == VSQBS) return E_UNSUPPORTED;
+    if (kernel < NEAREST || kernel > NOHALO) return E_INVALID_KERNEL;
     *eff_kernel = kernel;
     if (iw == 0 || ih == 0) return E_UNSUPPORTED;
     const uint32_t tw = to_u32_sat(std::max(std::round((double)iw * hscale), 1.0));
     const uint32_t th = to_u32_sat(std::max(std::round((double)ih * vscale), 1.0));
+    auto affine = [&](double ax, double ay) {
+        PlanStep s{N_AFFINE, 0.0};
+        s.a = 1.0 / ax;
+        s.d = 1.0 / ay;
+        s.tx = kernel == NEAREST ? 0.0 : 0.5 * (1.0 - 1.0 / ax);
+        s.ty = kernel == NEAREST ? 0.0 : 0.5 * (1.0 - 1.0 / ay);
+        s.ew = tw;
+        s.eh = th;
+        s.akern = kernel;
+        steps->push_back(s);
+    };
+    if (kernel == LBB) {
+        // resize.rs:176-199: LBB exists only as the 2-D affine interpolator,
+        // one Affine over the whole image.
+        if (tw != iw || th != ih) affine((double)tw / (double)iw, (double)th / (double)ih);
+        return OK;
+    }
     auto preshrink = [&](uint32_t in, uint32_t t) -> uint32_t {
         if (t == 0 || in <= t || kernel == NEAREST) return 1;
         return to_u32_sat(std::max(std::floor((double)in / (double)t / 2.0), 1.0));
@@ -487,9 +504,13 @@ Status plan_resize(uint32_t iw, uint32_t ih, double hscale, double vscale, int k
         steps->push_back({N_REDUCE_H, (double)cw / (double)tw});
         cw = tw;
     }
-    // resize.rs:240-280: integer Zoom of what remains to grow, then the Affine
-    // fix-up with the resize kernel.
-    const uint32_t zx = tw > cw ? tw / cw : 1, zy = th > ch ? th / ch : 1;
+    // resize.rs:240-280: integer Zoom of what remains to grow (never for
+    // VSQBS, whose 4 x 4 stencil the Zoom would bypass), then the Affine
+    // fix-up with the resize kernel.  A VSQBS reduce step is refused by the
+    // reduce builder (reduce_common.rs:107-113), so a VSQBS downscale fails
+    // as in the reference.
+    const bool can_zoom = kernel != VSQBS;
+    const uint32_t zx = can_zoom && tw > cw ? tw / cw : 1, zy = can_zoom && th > ch ? th / ch : 1;
     if (zx > 1 || zy > 1) {
         PlanStep z{N_ZOOM, 0.0};
         z.zx = zx;
@@ -499,17 +520,7 @@ Status plan_resize(uint32_t iw, uint32_t ih, double hscale, double vscale, int k
         ch *= zy;
     }
     const double ax = (double)tw / (double)cw, ay = (double)th / (double)ch;
-    if (std::fabs(ax - 1.0) > kPlanEps || std::fabs(ay - 1.0) > kPlanEps) {
-        PlanStep s{N_AFFINE, 0.0};
-        s.a = 1.0 / ax;
-        s.d = 1.0 / ay;
-        s.tx = kernel == NEAREST ? 0.0 : 0.5 * (1.0 - 1.0 / ax);
-        s.ty = kernel == NEAREST ? 0.0 : 0.5 * (1.0 - 1.0 / ay);
-        s.ew = tw;
-        s.eh = th;
-        s.akern = kernel;
-        steps->push_back(s);
-    }
+    if (std::fabs(ax - 1.0) > kPlanEps || std::fabs(ay - 1.0) > kPlanEps) affine(ax, ay);
     return OK;
 }
 

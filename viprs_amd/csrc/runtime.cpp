@@ -401,7 +401,12 @@ void stage_affine_params(const std::vector<Node>& nodes, int i, const std::vecto
         for (size_t k = 0; k < pos.size(); ++k) {
             const double x = pos[k];
             AffineTap& a = t[k];
-            if (nd.akern != NEAREST && nd.akern != BILINEAR) {
+            if (nd.akern == LBB || nd.akern == VSQBS) {
+                // the 4 x 4 stencil from floor(x) - 1; phase x - floor(x)
+                const double c = std::floor(x);
+                a.i0 = (int32_t)((int64_t)c - 1);
+                a.f = x - c;
+            } else if (nd.akern != NEAREST && nd.akern != BILINEAR) {
                 // interp_separable<S>: taps floor(x) - S + 1 .. floor(x) + S
                 const int S = nd.akern == LANCZOS3 ? 3 : 2;
                 const int64_t c = (int64_t)std::floor(x);
@@ -1059,6 +1064,8 @@ int launch_stage(const std::vector<Node>& nodes, const Stage& s, const InView& i
         j.kernel = an.akern == NEAREST    ? AFF_NEAREST
                    : an.akern == BILINEAR ? AFF_BILINEAR
                    : an.akern == LANCZOS3 ? AFF_SEP6
+                   : an.akern == VSQBS    ? AFF_VSQBS
+                   : an.akern == LBB      ? AFF_LBB
                                           : AFF_SEP4;
         j.in_w = (int32_t)an.in_w;
         j.in_h = (int32_t)an.in_h;
