@@ -3,9 +3,10 @@ wide on one MI355X (input Mpix/s, % of HBM peak, p99 step latency), plus, as
 line items of the same JSON object, the 4096x4096 4-band uchar
 gauss_blur(sigma=2), the 1024x768x3 uchar reduce_h -> reduce_v Lanczos3 by 3.7
 and the fused f64 reduce (ReduceOp) of the same image, the 512-image
-uneven-size batched thumbnail and the
-4096x4096x3 float32 reduce / gauss_blur in the three float modes and the
-open-loop thumbnail request trace at 500 / 2000 / 5000 requests/s.
+uneven-size batched thumbnail, the same thumbnail from a PNG source that
+box-shrinks on load, the 4096x4096x3 float32 reduce / gauss_blur in the three
+float modes and the open-loop thumbnail request trace at 500 / 2000 / 5000
+requests/s.  --only runs one of the secondary workloads alone.
 
 One step = one full pipeline run over one image already resident in HBM;
 steps cycle over four such images (805 MB together, more than the 256 MB
@@ -261,6 +262,59 @@ def bench_float(v, O, ctx, steps, warmup):
     return out
 
 
+def bench_png(v, O, ctx, d_srcs, host_src, steps, warmup):
+    """The headline thumbnail from a PNG source that box-shrinks on load
+    (decoder_source/shrink.rs:47-117; builder/resample.rs:871-906): the
+    decoded 8192x8192x3 image is resident in HBM, the device box pre-shrink
+    by the hint (16) gives 512x512, then the thumbnail is planned over it.
+    Timed over the same cycled sources as the headline."""
+    W = H = 8192
+    B = 3
+    p = v.Pipeline(ctx, W, H, B).thumbnail_shrink_on_load(v.WIDTH, 400)
+    d_dst = p.output_buffer()
+    exact = None
+    if host_src is not None:
+        p.run(d_srcs[0], d_dst)
+        ctx.synchronize()
+        exact = bool(np.array_equal(d_dst.numpy(), O.Pipeline(W, H, B).thumbnail_shrink_on_load(O.WIDTH, 400)
+                                    .run(host_src)))
+    ms, lat = p.benchmark_sources(d_srcs, d_dst, warmup, steps)
+    box_ms = p.time_stage_sources(d_srcs, d_dst, 0, max(steps, 20))
+    in_bytes = W * H * B
+    box_bytes = in_bytes + 512 * 512 * B
+    return {"workload": "PNG source: box pre-shrink 16 (8192x8192x3 -> 512x512) then thumbnail Width(400) "
+                        "Lanczos3, uchar",
+            "bit_exact": exact, "ms_per_step": round(ms, 5), "Mpix/s": round(W * H / 1e6 / (ms / 1e3), 1),
+            "p99_ms": round(float(np.percentile(lat, 99)), 5), "stages": p.stage_kernels(),
+            "stage_ms": [round(x, 5) for x in p.run_profiled(d_srcs[0], d_dst, 10)],
+            "box_kernel": {"ms": round(box_ms, 5), "algorithmic_bytes": box_bytes,
+                           "hbm_frac": round(box_bytes / (box_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
+
+
+def only_workload(v, O, args):
+    """One secondary workload alone (--only), for counter and trace runs:
+    prints its JSON object."""
+    ctx = v.Context(0)
+    if args.only == "float":
+        out = bench_float(v, O, ctx, args.steps, args.warmup)
+    elif args.only == "gauss":
+        gs = np.random.default_rng(97).integers(0, 256, size=(4096, 4096, 4), dtype=np.uint8)
+        out = bench_line(v, O, ctx, gs, lambda p: p.gauss_blur(2.0), args.steps, args.warmup,
+                         "gauss_blur sigma=2.0 on 4096x4096x4 uchar")
+    elif args.only == "batch":
+        out = bench_batch(v, O, ctx, args.batch_images, max(args.steps, 20), args.warmup, 8)
+    else:  # png: four device-filled sources, parity on the first
+        d_srcs = []
+        for i in range(4):
+            d = v.DeviceBuffer(ctx, (8192, 8192, 3), np.uint8)
+            d.fill_random(777 + i)
+            d_srcs.append(d)
+        host = O.synthetic_bytes_fast(8192 * 8192 * 3, 777).reshape(8192, 8192, 3)
+        out = bench_png(v, O, ctx, d_srcs, host, args.steps, args.warmup)
+    ctx.synchronize()
+    print(json.dumps({args.only: out}), flush=True)
+
+
 def bench_serving(v, ctx, rates, seconds):
     """Open-loop request trace per arrival rate: Poisson arrivals (seeded
     exponential gaps), every request a new size P ~ logU[0.1, 24] MP, aspect
@@ -334,6 +388,8 @@ def main():
     ap.add_argument("--no-float", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=2.0)
+    ap.add_argument("--only", choices=["float", "gauss", "batch", "png"],
+                    help="run one secondary workload alone (counter and trace runs) and print its line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -350,6 +406,10 @@ def main():
         return
 
     from oracle import oracle as O
+
+    if args.only:
+        only_workload(v, O, args)
+        return
 
     print("bench: building inputs", file=sys.stderr, flush=True)
     W = H = 8192
@@ -456,6 +516,9 @@ def main():
         result["thumbnail_up"] = bench_line(
             v, O, ctx, ra, lambda p: p.thumbnail(v.FORCE_EXACT, 1600, 1200, v.LANCZOS3), args.steps, args.warmup,
             "thumbnail ForceExact(1600, 1200) Lanczos3 on 1024x768x4 uchar (premultiplied Bicubic Affine tail)")
+
+    print("bench: PNG-source thumbnail", file=sys.stderr, flush=True)
+    result["png_thumbnail"] = bench_png(v, O, ctx, d_srcs, src, args.steps, args.warmup)
 
     if not args.no_batch and args.batch_images > 0:
         print("bench: uneven-size batch", file=sys.stderr, flush=True)
