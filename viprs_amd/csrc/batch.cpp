@@ -47,13 +47,16 @@ struct Launch {
     int ntasks = 0;
     size_t lds = 0;
     uint64_t alg_bytes = 0;  // input buffers read + outputs written, summed over jobs
-    int chain = -1;          // (band count - 1) * kGroups + size group of its images, or -1 for mixed
+    int chain = -1;          // the group of its images (one stream each), or -1 for mixed
 };
 
-// Images are dealt round-robin, largest first, into this many groups per band
-// count; each (band count, group) is a chain of launches on its own stream.
-constexpr int kGroups = 2;
-constexpr int kChains = 4 * kGroups;
+// Images are dealt, largest first, to the group with the fewest bytes so far;
+// each group is a chain of launches (one per kernel variant and level) on its
+// own stream (group 0 on the caller's).  Four groups, as many as the
+// device's hardware queues: chains of equal size end their box shrinks
+// together and run their reduce tails side by side.
+constexpr int kGroups = 4;
+constexpr int kChains = kGroups;
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -205,6 +208,25 @@ int vgpu_batch::plan() {
                (uint64_t)images[(size_t)b].w * images[(size_t)b].h * images[(size_t)b].bands;
     });
 
+    // Greedy balance of the groups by the bytes each image's stages move.
+    std::vector<int> group_of(count, 0);
+    {
+        uint64_t load[kGroups] = {};
+        for (int idx : order) {
+            const BatchImage& im = images[(size_t)idx];
+            uint64_t bytes = 0;
+            for (size_t k = 0; k < im.stages.size(); ++k) {
+                const Rect in_rect = k == 0 ? whole(im.w, im.h) : im.stages[k - 1].out;
+                bytes += ((uint64_t)in_rect.w * in_rect.h + (uint64_t)im.stages[k].out.w * im.stages[k].out.h) *
+                         im.bands;
+            }
+            int g = 0;
+            for (int c = 1; c < kGroups; ++c)
+                if (load[c] < load[g]) g = c;
+            group_of[(size_t)idx] = g;
+            load[g] += bytes;
+        }
+    }
     Staging tbl;
     n_shrink_tasks = n_reduce_tasks = 0;
     for (int lv = 0; lv < levels; ++lv) {
@@ -225,7 +247,7 @@ int vgpu_batch::plan() {
         uint64_t pbytes[kP] = {};
         for (size_t rank = 0; rank < order.size(); ++rank) {
             const int idx = order[rank];
-            const int grp = (int)(rank % kGroups);
+            const int grp = group_of[(size_t)idx];
             BatchImage& im = images[(size_t)idx];
             if (!im.batched || lv >= (int)im.stages.size()) continue;
             const Stage& s = im.stages[(size_t)lv];
@@ -279,7 +301,7 @@ int vgpu_batch::plan() {
             Launch l;
             l.family = 0;
             l.variant = var;
-            l.chain = (var % 4) * kGroups + vg % kGroups;  // shrink_job_variant: bands - 1 (+ 4 / 8 lean)
+            l.chain = vg % kGroups;
             l.jobs_off = tbl.put(sjobs[vg].data(), sjobs[vg].size() * sizeof(ShrinkJob));
             l.tasks_off = tbl.put(stasks[vg].data(), stasks[vg].size() * sizeof(BatchTask));
             l.ntasks = (int)stasks[vg].size();
@@ -294,7 +316,7 @@ int vgpu_batch::plan() {
             Launch l;
             l.family = 2;
             l.variant = var;
-            l.chain = (var >> 3) * kGroups + vg % kGroups;  // sep_job_variant: bands - 1 in the high bits
+            l.chain = vg % kGroups;
             l.jobs_off = tbl.put(pjobs[vg].data(), pjobs[vg].size() * sizeof(SepJob));
             l.tasks_off = tbl.put(ptasks[vg].data(), ptasks[vg].size() * sizeof(BatchTask));
             l.ntasks = (int)ptasks[vg].size();
@@ -319,7 +341,7 @@ int vgpu_batch::plan() {
     for (const Launch& l : launches)
         if (l.chain >= 0) chains |= 1u << l.chain;
     for (int c = 0; c < kChains; ++c) {
-        if (!(chains & (1u << c)) || side[c]) continue;
+        if (c == 0 || !(chains & (1u << c)) || side[c]) continue;  // chain 0 uses the caller's stream
         // Default priority, as the context stream.
         if (hipStreamCreateWithFlags(&side[c], hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&join_ev[c], hipEventDisableTiming) != hipSuccess)
@@ -359,12 +381,12 @@ int vgpu_batch::launch_all(hipStream_t s, hipEvent_t* marks) {
     const bool par = !marks && __builtin_popcount(chains) >= 2;
     auto fork = [&]() -> bool {
         if (hipEventRecord(fork_ev, s) != hipSuccess) return false;
-        for (int c = 0; c < kChains; ++c)
+        for (int c = 1; c < kChains; ++c)
             if ((chains & (1u << c)) && hipStreamWaitEvent(side[c], fork_ev, 0) != hipSuccess) return false;
         return true;
     };
     auto join = [&]() -> bool {
-        for (int c = 0; c < kChains; ++c)
+        for (int c = 1; c < kChains; ++c)
             if ((chains & (1u << c)) &&
                 (hipEventRecord(join_ev[c], side[c]) != hipSuccess || hipStreamWaitEvent(s, join_ev[c], 0) != hipSuccess))
                 return false;
@@ -376,7 +398,7 @@ int vgpu_batch::launch_all(hipStream_t s, hipEvent_t* marks) {
         hipStream_t ls = s;
         if (par) {
             if (l.chain >= 0) {
-                ls = side[l.chain];
+                ls = l.chain == 0 ? s : side[l.chain];  // chain 0 on the caller's stream
             } else if (!join()) {  // a launch over mixed band counts waits for every chain
                 return VGPU_E_DEVICE;
             }

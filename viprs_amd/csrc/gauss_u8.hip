@@ -10,11 +10,11 @@
 //
 // Block = 64 output columns x TH output rows.  The input tile (TH + 2R rows x
 // 64 + 2R columns, read through the clamped view, i.e. the reference's
-// clamp-to-edge source and its Z^2 intermediate rows) is split into LDS once;
-// the H pass writes the TH + 2R intermediate rows the V taps need, rounded,
-// to LDS as pairs; the V pass gives each lane one column and each wavefront
-// TH / 4 consecutive rows, with the 2R + TH / 4 intermediate rows it reads
-// held in registers.
+// clamp-to-edge source and its Z^2 intermediate rows) goes to LDS as packed
+// pixels; the H pass (lane = column) writes the TH + 2R intermediate rows the
+// V taps need, rounded to bytes, to LDS; the V pass gives each lane one
+// column and each wavefront TH / 4 consecutive rows, with the 2R + TH / 4
+// intermediate rows it reads held in registers as pairs.
 #include "device.hpp"
 
 namespace vgpu {
@@ -36,23 +36,36 @@ __device__ __forceinline__ uint32_t div_pair(uint32_t x, uint32_t mul) {
 template <int B, int R, int TH>
 __global__ __launch_bounds__(kThreads) void k_gauss_u8(GaussJob job) {
     constexpr int TW = 64;
-    constexpr int IW = TW + 2 * R;  // input tile columns (<= 128: two per lane)
-    constexpr int NR = TH + 2 * R;  // input / intermediate tile rows
-    constexpr int VR = TH / 4;      // output rows per wavefront
+    constexpr int IW = TW + 2 * R;    // input tile columns (<= 128: two per lane)
+    constexpr int NR = TH + 2 * R;    // input / intermediate tile rows
+    constexpr int VR = TH / 4;        // output rows per wavefront
     constexpr int KR = (NR + 3) / 4;  // input rows per wavefront
-    constexpr int IH = IW / 2;        // input columns of one parity
-    // (b0, b2) and (b1, b3) pairs side by side: one 8-byte LDS access a pixel.
-    // The input tile keeps even and odd columns apart, so a lane that makes
-    // two adjacent H outputs reads 2R + 2 consecutive entries, conflict-free.
-    __shared__ uint2 in_e[NR * IH], in_o[NR * IH];
-    __shared__ __align__(16) uint2 mid_px[NR * TW];
+    // Pixels stay packed (one dword each) in LDS: the input tile row by row,
+    // the H results as rounded bytes; a lane reads its column's dwords, which
+    // are consecutive across the wavefront (no bank conflicts), and splits
+    // them into the (b0, b2), (b1, b3) pairs in registers.
+    // A lane of the H pass makes 4 adjacent columns from NRD 16-byte reads
+    // (4 + 2R pixels); tile rows are IWP >= IW dwords, a multiple of 4.
+    constexpr int NRD = (2 * R + 7) / 4;
+    constexpr int IWP = 60 + 4 * NRD;
+    static_assert(IWP >= IW && IWP % 4 == 0, "tile pitch");
+    __shared__ __align__(16) uint32_t tile[NR * IWP];
+    __shared__ __align__(16) uint32_t mid[NR * TW];
     const InView& in = job.in;
     const OutView& out = job.out;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int ox0 = (int)blockIdx.x * TW;
+    // XCD-aware order: workgroups go to the eight XCDs round robin, so
+    // workgroup p takes logical block (p % 8) * chunk + p / 8 -- each XCD a
+    // contiguous run of tile columns, whose shared halo lines then hit in
+    // that XCD's L2.  Logical blocks past gx * gy exit (the whole block).
+    const int nblk = job.gx * job.gy, chunk = (nblk + 7) / 8;
+    const int lb = ((int)blockIdx.x % 8) * chunk + (int)blockIdx.x / 8;
+    if (lb >= nblk) return;
+    const int bx = lb % job.gx, by = lb / job.gx;
+    const int ox0 = bx * TW;
     // This block's tiles: rows [t0, t1) of the column of tiles.
-    const int t0 = (int)blockIdx.y * job.tiles_per_block;
+    const int t0 = by * job.tiles_per_block;
     const int t1 = min(t0 + job.tiles_per_block, (out.h + TH - 1) / TH);
     const unsigned char* src = static_cast<const unsigned char*>(in.ptr);
     const size_t pitch = (size_t)in.bw * B;
@@ -77,7 +90,6 @@ __global__ __launch_bounds__(kThreads) void k_gauss_u8(GaussJob job) {
             return __builtin_amdgcn_alignbyte(hi, lo, (uint32_t)(a & 3)) & 0x00ffffffu;
         }
     };
-    auto split = [](uint32_t v) { return make_uint2(v & 0x00ff00ffu, (v >> 8) & 0x00ff00ffu); };
     // A wavefront's loads of one tile, all issued before any is used; the
     // next tile's are in flight while the current one is computed.
     uint32_t v0[KR], v1[KR];
@@ -98,57 +110,65 @@ __global__ __launch_bounds__(kThreads) void k_gauss_u8(GaussJob job) {
     const int r0 = wave * VR;
     if (t0 < t1) fetch(t0);
     for (int tr = t0; tr < t1; ++tr) {
-        uint2* in0 = (lane & 1) ? in_o : in_e;  // column lane and lane + 64 share its parity
 #pragma unroll
         for (int k = 0; k < KR; ++k) {
             const int rr = wave + 4 * k;
             if (rr < NR) {
-                in0[rr * IH + (lane >> 1)] = split(v0[k]);
-                if (cc1 < IW) in0[rr * IH + (cc1 >> 1)] = split(v1[k]);
+                tile[rr * IWP + lane] = v0[k];
+                if (cc1 < IW) tile[rr * IWP + cc1] = v1[k];
             }
         }
-        __syncthreads();  // the tile is in LDS; the previous V pass is done with mid_px
+        __syncthreads();  // the tile is in LDS; the previous V pass is done with mid
         if (tr + 1 < t1) fetch(tr + 1);
 
-        // H pass: intermediate row 2 * rp + (lane >> 5), columns 2m and
-        // 2m + 1 (m = lane & 31): even input columns 2m .. 2m + 2R and odd
-        // ones 2m + 1 .. 2m + 2R + 1 are in_e[m .. m + R], in_o[m .. m + R].
-        const int m = lane & 31;
-        for (int rp = wave; rp < (NR + 1) / 2; rp += 4) {
-            const int rr = 2 * rp + (lane >> 5);
+        // H pass: a wavefront takes 4 rows at a time, 16 lanes a row; lane q
+        // makes intermediate columns 4q .. 4q + 3 of its row from input
+        // columns 4q .. 4q + 3 + 2R.
+        const int q = lane & 15;
+        for (int rb = wave * 4; rb < NR; rb += 16) {
+            const int rr = rb + (lane >> 4);
             if (rr < NR) {
-                uint2 e[R + 1], o[R + 1];
+                uint32_t pa[4 * NRD], pb[4 * NRD];
+                const uint4* rp = reinterpret_cast<const uint4*>(tile + rr * IWP + 4 * q);
 #pragma unroll
-                for (int k = 0; k <= R; ++k) {
-                    e[k] = in_e[rr * IH + m + k];
-                    o[k] = in_o[rr * IH + m + k];
+                for (int k = 0; k < NRD; ++k) {
+                    const uint4 v = rp[k];
+                    const uint32_t px[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        pa[4 * k + e] = px[e] & 0x00ff00ffu;
+                        pb[4 * k + e] = (px[e] >> 8) & 0x00ff00ffu;
+                    }
                 }
-                u16x2 a0 = half, b0 = half, a1 = half, b1 = half;
+                u16x2 a[4], b[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) a[j] = b[j] = half;
 #pragma unroll
                 for (int t = 0; t <= 2 * R; ++t) {
                     const u16x2 w = as_pair(job.taps2[t]);
-                    // column 2m + t, and 2m + 1 + t
-                    const uint2 x0 = (t & 1) ? o[t >> 1] : e[t >> 1];
-                    const uint2 x1 = (t & 1) ? e[(t + 1) >> 1] : o[t >> 1];
-                    a0 = as_pair(x0.x) * w + a0;
-                    b0 = as_pair(x0.y) * w + b0;
-                    a1 = as_pair(x1.x) * w + a1;
-                    b1 = as_pair(x1.y) * w + b1;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        a[j] = as_pair(pa[j + t]) * w + a[j];
+                        b[j] = as_pair(pb[j + t]) * w + b[j];
+                    }
                 }
-                uint4 q;
-                q.x = div_pair(as_u32(a0), job.mul);
-                q.y = div_pair(as_u32(b0), job.mul);
-                q.z = div_pair(as_u32(a1), job.mul);
-                q.w = div_pair(as_u32(b1), job.mul);
-                *reinterpret_cast<uint4*>(mid_px + rr * TW + 2 * m) = q;
+                uint32_t o[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    o[j] = div_pair(as_u32(a[j]), job.mul) | (div_pair(as_u32(b[j]), job.mul) << 8);
+                *reinterpret_cast<uint4*>(mid + rr * TW + 4 * q) = make_uint4(o[0], o[1], o[2], o[3]);
             }
         }
-        __syncthreads();  // mid_px is complete; the input tile may be overwritten
+        __syncthreads();  // mid is complete; the input tile may be overwritten
 
         // V pass: output rows tr * TH + wave * VR .. + VR - 1, column ox.
-        uint2 win[VR + 2 * R];
+        uint32_t wa[VR + 2 * R], wb[VR + 2 * R];
 #pragma unroll
-        for (int j = 0; j < VR + 2 * R; ++j) win[j] = mid_px[(r0 + j) * TW + lane];
+        for (int j = 0; j < VR + 2 * R; ++j) {
+            const uint32_t px = mid[(r0 + j) * TW + lane];
+            wa[j] = px & 0x00ff00ffu;
+            wb[j] = (px >> 8) & 0x00ff00ffu;
+        }
         if (ox < out.w) {
 #pragma unroll
             for (int o = 0; o < VR; ++o) {
@@ -158,8 +178,8 @@ __global__ __launch_bounds__(kThreads) void k_gauss_u8(GaussJob job) {
 #pragma unroll
                 for (int t = 0; t <= 2 * R; ++t) {
                     const u16x2 w = as_pair(job.taps2[t]);
-                    a = as_pair(win[o + t].x) * w + a;
-                    b = as_pair(win[o + t].y) * w + b;
+                    a = as_pair(wa[o + t]) * w + a;
+                    b = as_pair(wb[o + t]) * w + b;
                 }
                 const uint32_t qrb = div_pair(as_u32(a), job.mul), qga = div_pair(as_u32(b), job.mul);
                 const uint32_t v = qrb | (qga << 8);  // bytes b0 b1 b2 b3
@@ -186,7 +206,7 @@ bool gauss_u8_supported(int bands, int taps, int32_t scale) {
 
 hipError_t launch_gauss_u8(const GaussJob& job, hipStream_t s) {
     if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;
-    const dim3 grid((unsigned)job.gx, (unsigned)job.gy);
+    const dim3 grid((unsigned)(8 * ((job.gx * job.gy + 7) / 8)));
 #define VGPU_G(BB, RR, TT) \
     hipLaunchKernelGGL((k_gauss_u8<BB, RR, TT>), grid, dim3(kThreads), 0, s, job)
 #define VGPU_GR(BB, TT)                 \

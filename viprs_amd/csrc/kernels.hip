@@ -604,9 +604,17 @@ __device__ __forceinline__ void shrink_rows_lean_nd(const ShrinkJob& job, int bx
     unsigned char* rbuf = lds + wave * (kSpanBytes + 32);
     const int wbytes = fh * B;
     const int X0 = cx0 + lane * fh;
-    const bool fast = !premult && lane < ncols && X0 >= lo && X0 + fh <= hi;
+    // Windows that run past the buffer's right end read its last pixel
+    // repeated (the clamped read): park() writes those copies after the
+    // span, so such windows take the fast paths too.  Windows wholly left of
+    // the buffer (the Z^2 halo columns left of a source) are fh copies of its
+    // first pixel.  Only a window cut by the left end takes the per-pixel
+    // loop.
+    const int over = max(0, cx1 - hi);  // pixels past the end (wave-uniform)
+    const bool fast = !premult && lane < ncols && X0 >= lo;
     // 4-band premultiplied windows inside the span: packed 16-bit path.
-    const bool fast_pm4 = B == 4 && premult && lane < ncols && X0 >= lo && X0 + fh <= hi;
+    const bool fast_pm4 = B == 4 && premult && lane < ncols && X0 >= lo;
+    const bool left_out = lane < ncols && X0 + fh <= lo;
     const int lane_byte = (X0 - lo) * B;
     // Masks of the window's last four dwords ND - 4 .. ND - 1 (ND = 0: the
     // looped window sums of windows longer than 16 dwords).
@@ -684,6 +692,12 @@ __device__ __forceinline__ void shrink_rows_lean_nd(const ShrinkJob& job, int bx
         unsigned char* d = rbuf + (slot0 + c) * 16;
         for (int q = 0; q < 16; ++q) d[q] = off + q < total ? gmem<unsigned char>(base)[off + q] : 0;
     };
+    // The over * B bytes after a parked row's span: its last pixel again.
+    auto replicate = [&](int shift, int slot0) {
+        unsigned char* row = rbuf + slot0 * 16 + shift;
+        const int e0 = (hi - lo) * B;
+        for (int k = lane; k < over * B; k += 64) row[e0 + k] = row[e0 - B + k % B];
+    };
     auto park = [&](const Group& G) {
         u32x4* dst = reinterpret_cast<u32x4*>(rbuf) + lane;
         dst[0] = G.p0;
@@ -695,6 +709,11 @@ __device__ __forceinline__ void shrink_rows_lean_nd(const ShrinkJob& job, int bx
             // same lane stored it, so LDS order holds).
             if (G.c0 >= 0) fix_cut(G.a0, G.c0, 0);
             if (two && G.c1 >= 0) fix_cut(G.a1, G.c1, 128);
+        }
+        if (over > 0) {
+            wave_lds_sync();  // the span's last pixel is in LDS
+            replicate(G.s0, 0);
+            if (two) replicate(G.s1, 128);
         }
     };
 
@@ -724,6 +743,17 @@ __device__ __forceinline__ void shrink_rows_lean_nd(const ShrinkJob& job, int bx
             else window_sums_masked<B, ND>(slot, shift + lane_byte, tail, sum);
         } else if (fast_pm4) {
             if constexpr (B == 4) premult_window_sums4(slot, shift + lane_byte, fh, sum);
+        } else if (left_out) {
+            const unsigned char* px = slot + shift;
+            if (premult) {
+                const uint32_t al = px[B - 1];
+#pragma unroll
+                for (int b = 0; b < B - 1; ++b) sum[b] = premult_u8(px[b], al) * (uint32_t)fh;
+                sum[B - 1] = al * (uint32_t)fh;
+            } else {
+#pragma unroll
+                for (int b = 0; b < B; ++b) sum[b] = px[b] * (uint32_t)fh;
+            }
         } else if (lane < ncols) {
             for (int j = 0; j < fh; ++j) {
                 const unsigned char* px = slot + shift + (clampi(X0 + j, lo, hi - 1) - lo) * B;
@@ -832,7 +862,11 @@ __global__ __launch_bounds__(kThreads) void k_shrink_batch(const ShrinkJob* __re
         if (job.src_slot >= 0) job.in.ptr = srcs[job.src_slot];
         if (job.dst_slot >= 0) job.out.ptr = dsts[job.dst_slot];
         shrink_block<B, FORM>(job, task.bx, task.by, lds);
-        __syncthreads();
+        // With one wavefront per output row (S == 1, every batch job) each
+        // wavefront parks only in its own LDS span, so the wavefronts of a
+        // block go on to the next task without waiting for each other; split
+        // rows share LDS and wait.
+        if (job.S > 1) __syncthreads();
     }
 }
 
@@ -1387,7 +1421,7 @@ bool plan_shrink_job(ShrinkJob* job, bool row_split, int run) {
         // S wavefronts while the row tasks leave room on the chip.
         while (m > 1 && waves(m) < 256 * 16) m = (m + 1) / 2;
         if (m == 1)
-            while (S < 4 && 2 * S <= fv && waves(1) * 2 * S <= 256 * 4 * 6) S *= 2;
+            while (S < 4 && 2 * S <= fv && waves(1) * 2 * S <= 256 * 4 * 4) S *= 2;
     }
     if (run > 0) {  // caller's run length: no row split
         m = std::min(run, 64);
