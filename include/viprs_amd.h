@@ -536,6 +536,66 @@ int vgpu_server_stats(vgpu_server *s, vgpu_server_counters *st);
 int vgpu_server_replay(vgpu_server *s, const vgpu_request *reqs, uint32_t n,
                        const int64_t *arrival_offset_ns, int64_t *latency_ns, int32_t *status,
                        int64_t *wall_ns, int64_t *plan_ns, int64_t *queue_ns);
+/* ---------------------------------------------------------------------
+ * Encode after the path and the bytes-in / bytes-out helper
+ * (crates/viprs-runtime/src/process.rs:147-260; jpeg/encode.rs through
+ * turbojpeg; png via zlib).
+ * ------------------------------------------------------------------- */
+/* JpegSubsampling (viprs-core codec_options): Auto is 4:2:0 below quality
+ * 90, else none (jpeg/common.rs:115-131). */
+typedef enum {
+    VGPU_JPEG_SUBSAMPLE_AUTO = 0,
+    VGPU_JPEG_SUBSAMPLE_OFF = 1,
+    VGPU_JPEG_SUBSAMPLE_420 = 2
+} vgpu_jpeg_subsampling;
+/* Baseline JFIF JPEG of a u8 image in device memory (1 band: grey; 3 bands:
+ * YCbCr; 4 bands: YCbCr of the first three, alpha dropped), quality 0..100
+ * (0 encodes as 1), the standard tables and no Huffman optimisation -- the
+ * bytes libjpeg-turbo's encoder writes for the same settings
+ * (JpegCodec::encode_with_options, jpeg/encode.rs:169-258).  The
+ * per-block colour conversion, downsampling, DCT and quantisation run on the
+ * device, the entropy coding on the host.  *out_len receives the stream's
+ * size; VGPU_E_BUFFER_SIZE when out_cap is smaller (out may be NULL to ask). */
+int vgpu_jpeg_encode(vgpu_context *ctx, const void *pixels, uint32_t width, uint32_t height, uint32_t bands,
+                     int quality, int subsampling, void *out, size_t out_cap, size_t *out_len);
+/* PNG decode of host bytes (8-bit grey, grey + alpha, RGB, RGBA, palette ->
+ * RGB; not interlaced): the size into *width, *height, *bands, and, when
+ * pixels is not NULL, the samples (VGPU_E_BUFFER_SIZE if cap is short). */
+int vgpu_png_decode(const void *data, size_t len, uint32_t *width, uint32_t *height, uint32_t *bands, void *pixels,
+                    size_t cap);
+/* PNG encode of host u8 pixels (1..4 bands) at zlib level 0..9, lossless. */
+int vgpu_png_encode(const void *pixels, uint32_t width, uint32_t height, uint32_t bands, int compression, void *out,
+                    size_t out_cap, size_t *out_len);
+/* EncodeOptions (process.rs:52-111): JPEG (quality, subsampling) or PNG
+ * (compression). */
+typedef enum { VGPU_ENCODE_JPEG = 0, VGPU_ENCODE_PNG = 1 } vgpu_encode_format;
+typedef struct {
+    int format;
+    int quality;      /* JPEG: 1..100 */
+    int subsampling;  /* JPEG: vgpu_jpeg_subsampling */
+    int compression;  /* PNG: 0..9 */
+} vgpu_encode_options;
+/* ProcessResult (process.rs:125-136). */
+typedef struct {
+    char input_format[8];
+    uint32_t input_width, input_height;
+    char output_format[8];
+    uint64_t bytes_written;
+} vgpu_process_result;
+/* The `ops` closure of process(): adds nodes to a pipeline over the decoded
+ * image (U8, its size and bands) and returns VGPU_OK, or an error that
+ * process returns. */
+typedef int (*vgpu_process_build_fn)(vgpu_pipeline *p, void *user);
+/* process() / process_pipeline() (process.rs:147-260): validate the encode
+ * options, decode the input bytes (PNG), run the caller's pipeline on the
+ * device, encode (JPEG on the device + host, PNG on the host), copy the
+ * stream to out (*out_len = its size; VGPU_E_BUFFER_SIZE when short).  The
+ * token is checked between the steps and by the pipeline run
+ * (VGPU_E_CANCELLED). */
+int vgpu_process(vgpu_context *ctx, const void *input, size_t input_len, vgpu_process_build_fn build, void *user,
+                 const vgpu_encode_options *enc, const vgpu_cancel_token *token, void *out, size_t out_cap,
+                 size_t *out_len, vgpu_process_result *result);
+
 /* Pinned host memory for request sources (hipHostMalloc). */
 int vgpu_host_alloc(size_t bytes, void **out);
 void vgpu_host_free(void *p);

@@ -137,6 +137,13 @@ void orc_convert_shrunk(double v, int fmt, void *out);
  * with phases (xph, yph). */
 double orc_interp_2d(int kernel, const double *n16, double xph, double yph);
 
+/* Baseline JFIF JPEG of host u8 pixels (jpeg.c): the bytes libjpeg-turbo
+ * writes through tj3Compress8 with only quality and subsampling set
+ * (jpeg/encode.rs:169-258).  subsampling: 0 auto (4:2:0 below quality 90),
+ * 1 off, 2 4:2:0.  *len = the stream's size; out may be NULL. */
+int orc_jpeg_encode(const uint8_t *px, uint32_t w, uint32_t h, uint32_t bands, int quality, int subsampling,
+                    uint8_t *out, size_t cap, size_t *len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,7 @@ def lib() -> C.CDLL:
         L.orc_synthetic_bytes.restype = None
         L.orc_interp_2d.argtypes = [i32, vp, dbl, dbl]
         L.orc_interp_2d.restype = dbl
+        L.orc_jpeg_encode.argtypes = [vp, u32, u32, u32, i32, i32, vp, C.c_size_t, C.POINTER(C.c_size_t)]
         for name, arg in (("orc_convert_fixed", C.c_int64), ("orc_convert_from_f64", dbl),
                           ("orc_convert_shrunk", dbl)):
             getattr(L, name).argtypes = [arg, i32, vp]
@@ -280,6 +281,20 @@ def box_shrink(src: np.ndarray, factor: int, ceil: bool = False) -> np.ndarray:
     dst = np.zeros((oh.value, ow.value, b), src.dtype)
     _check(lib().orc_box_shrink(_ptr(src), w, h, b, fmt, factor, int(ceil), _ptr(dst), None, None), "box_shrink")
     return dst
+
+
+def jpeg_encode(pixels: np.ndarray, quality: int = 75, subsampling: int = 0) -> bytes:
+    """Baseline JFIF JPEG of (h, w, 1|3|4) u8 pixels (jpeg.c): subsampling 0
+    auto (4:2:0 below quality 90), 1 off, 2 4:2:0."""
+    a = np.ascontiguousarray(pixels, np.uint8)
+    if a.ndim == 2:
+        a = a[..., None]
+    h, w, b = a.shape
+    n = C.c_size_t()
+    _check(lib().orc_jpeg_encode(_ptr(a), w, h, b, quality, subsampling, None, 0, C.byref(n)), "jpeg_encode")
+    out = C.create_string_buffer(n.value)
+    _check(lib().orc_jpeg_encode(_ptr(a), w, h, b, quality, subsampling, out, n.value, C.byref(n)), "jpeg_encode")
+    return out.raw[:n.value]
 
 
 def thumbnail_plan(in_w, in_h, bands, target, tw, th=0, kernel=LANCZOS3, crop=False):

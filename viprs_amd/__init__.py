@@ -8,3 +8,7 @@ from .api import (  # noqa: F401
     Server, VgpuError, thumbnail_shrink_on_load_factor,
 )
 from .api import E_SCRATCH_LIMIT, NOALLOC_SCRATCH_BYTES, F64, I16, I32, U32  # noqa: F401
+from .api import (  # noqa: F401
+    ENCODE_JPEG, ENCODE_PNG, JPEG_SUBSAMPLE_420, JPEG_SUBSAMPLE_AUTO, JPEG_SUBSAMPLE_OFF, jpeg_encode, png_decode,
+    png_encode, process,
+)

@@ -77,6 +77,25 @@ class ServerCounters(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class EncodeOptions(C.Structure):
+    """vgpu_encode_options."""
+
+    _fields_ = [("format", C.c_int), ("quality", C.c_int), ("subsampling", C.c_int), ("compression", C.c_int)]
+
+
+class ProcessResult(C.Structure):
+    """vgpu_process_result."""
+
+    _fields_ = [("input_format", C.c_char * 8), ("input_width", C.c_uint32), ("input_height", C.c_uint32),
+                ("output_format", C.c_char * 8), ("bytes_written", C.c_uint64)]
+
+    def as_dict(self):
+        return {"input_format": self.input_format.decode(), "input_dimensions": (self.input_width, self.input_height),
+                "output_format": self.output_format.decode(), "bytes_written": self.bytes_written}
+
+
+BUILD_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)
+
 _lib = None
 
 
@@ -189,6 +208,10 @@ def load() -> C.CDLL:
         "vgpu_server_stats": (i32, [vp, P(ServerCounters)]),
         "vgpu_server_replay": (i32, [vp, P(Request), u32, P(C.c_int64), P(C.c_int64), P(C.c_int32),
                                      P(C.c_int64), P(C.c_int64), P(C.c_int64)]),
+        "vgpu_jpeg_encode": (i32, [vp, vp, u32, u32, u32, i32, i32, vp, sz, P(sz)]),
+        "vgpu_png_decode": (i32, [vp, sz, P(u32), P(u32), P(u32), vp, sz]),
+        "vgpu_png_encode": (i32, [vp, u32, u32, u32, i32, vp, sz, P(sz)]),
+        "vgpu_process": (i32, [vp, vp, sz, BUILD_FN, vp, P(EncodeOptions), vp, vp, sz, P(sz), P(ProcessResult)]),
         "vgpu_host_alloc": (i32, [sz, P(vp)]),
         "vgpu_host_free": (None, [vp]),
     }

@@ -196,10 +196,19 @@ class Pipeline:
         self.src_shape = (height, width, bands)
         self.src_fmt = fmt
 
+    @classmethod
+    def _borrow(cls, ctx: Context, handle, width: int, height: int, bands: int, fmt: int = U8) -> "Pipeline":
+        """A Pipeline object over a pipeline the library owns (process()'s
+        build callback): its calls build nodes, it never destroys it."""
+        p = cls.__new__(cls)
+        p.ctx, p._lib, p._h = ctx, ctx._lib, C.c_void_p(handle)
+        p.src_shape, p.src_fmt, p._borrowed = (height, width, bands), fmt, True
+        return p
+
     def _release(self) -> None:
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and not getattr(self, "_borrowed", False):
             self._lib.vgpu_pipeline_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         self._release()
@@ -736,3 +745,89 @@ class Server:
 
     def __del__(self):
         self._release()
+
+
+# ------------------------------------------------------------ encode / process
+
+JPEG_SUBSAMPLE_AUTO, JPEG_SUBSAMPLE_OFF, JPEG_SUBSAMPLE_420 = 0, 1, 2
+ENCODE_JPEG, ENCODE_PNG = 0, 1
+
+
+def _bytes_call(fn, *args):
+    """Two-pass call of an entry point that writes a byte stream: size, then data."""
+    n = C.c_size_t()
+    rc = fn(*args, None, 0, C.byref(n))
+    if rc not in (OK, E_BUFFER_SIZE):
+        _check(rc, fn.__name__)
+    buf = C.create_string_buffer(max(n.value, 1))
+    _check(fn(*args, buf, n.value, C.byref(n)), fn.__name__)
+    return buf.raw[:n.value]
+
+
+def jpeg_encode(ctx: Context, pixels: "DeviceBuffer", quality: int = 75,
+                subsampling: int = JPEG_SUBSAMPLE_AUTO) -> bytes:
+    """Baseline JPEG of a u8 (h, w, 1|3) device image (vgpu_jpeg_encode)."""
+    h, w, b = pixels.shape
+    return _bytes_call(ctx._lib.vgpu_jpeg_encode, ctx.handle, C.c_void_p(pixels.data_ptr()), w, h, b, quality,
+                       subsampling)
+
+
+def png_decode(data: bytes) -> np.ndarray:
+    """PNG bytes -> (h, w, bands) uint8 (vgpu_png_decode)."""
+    L = _native.load()
+    w, h, b = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    _check(L.vgpu_png_decode(data, len(data), C.byref(w), C.byref(h), C.byref(b), None, 0), "vgpu_png_decode")
+    out = np.empty((h.value, w.value, b.value), np.uint8)
+    _check(L.vgpu_png_decode(data, len(data), C.byref(w), C.byref(h), C.byref(b), out.ctypes.data, out.nbytes),
+           "vgpu_png_decode")
+    return out
+
+
+def png_encode(pixels: np.ndarray, compression: int = 6) -> bytes:
+    """(h, w, bands) uint8 -> PNG bytes (vgpu_png_encode)."""
+    a = np.ascontiguousarray(pixels, np.uint8)
+    if a.ndim == 2:
+        a = a[..., None]
+    h, w, b = a.shape
+    return _bytes_call(_native.load().vgpu_png_encode, a.ctypes.data, w, h, b, compression)
+
+
+def process(ctx: Context, data: bytes, build=None, fmt: int = ENCODE_JPEG, quality: int = 75,
+            subsampling: int = JPEG_SUBSAMPLE_AUTO, compression: int = 6, token: "CancelToken" = None):
+    """process() (process.rs:147-260): decode `data`, let `build(pipeline)`
+    add the operations, run them on the device, encode.  Returns (bytes,
+    result dict)."""
+    L = ctx._lib
+    errors = []
+
+    def cb(handle, _user):
+        try:
+            w, h, b = C.c_uint32(), C.c_uint32(), C.c_uint32()
+            f = C.c_int()
+            L.vgpu_pipeline_output_info(C.c_void_p(handle), C.byref(w), C.byref(h), C.byref(b), C.byref(f))
+            if build is not None:
+                build(Pipeline._borrow(ctx, handle, w.value, h.value, b.value, f.value))
+            return OK
+        except VgpuError as e:
+            return e.status
+        except Exception as e:  # pragma: no cover - reported below
+            errors.append(e)
+            return E_INVALID_ARGUMENT
+
+    fn = _native.BUILD_FN(cb)
+    enc = _native.EncodeOptions(fmt, quality, subsampling, compression)
+    res = _native.ProcessResult()
+    tok = token._h if token is not None else None
+    n = C.c_size_t()
+    rc = L.vgpu_process(ctx.handle, data, len(data), fn, None, C.byref(enc), tok, None, 0, C.byref(n), C.byref(res))
+    if errors:
+        raise errors[0]
+    if rc not in (OK, E_BUFFER_SIZE):
+        _check(rc, "vgpu_process")
+    buf = C.create_string_buffer(max(n.value, 1))
+    rc = L.vgpu_process(ctx.handle, data, len(data), fn, None, C.byref(enc), tok, buf, n.value, C.byref(n),
+                        C.byref(res))
+    if errors:
+        raise errors[0]
+    _check(rc, "vgpu_process")
+    return buf.raw[:n.value], res.as_dict()

@@ -18,8 +18,8 @@ CSRC = HERE / "csrc"
 LIB = HERE / "libviprs_amd.so"
 INCLUDE = HERE.parent / "include"
 
-SOURCES = ["kernels.hip", "sep_mfma.hip", "float_paths.hip", "gauss_u8.hip", "affine.hip", "conv_pair.hip", "plan.cpp",
-           "runtime.cpp", "batch.cpp", "server.cpp"]
+SOURCES = ["kernels.hip", "sep_mfma.hip", "float_paths.hip", "gauss_u8.hip", "affine.hip", "conv_pair.hip", "jpeg.hip",
+           "plan.cpp", "runtime.cpp", "batch.cpp", "server.cpp", "codec.cpp"]
 
 # -ffp-contract=off: the reference-exact f32 kernels and the host planner rely
 # on every a*b+c the reference writes without mul_add staying two roundings.
@@ -72,7 +72,7 @@ def build(verbose: bool = False, jobs: int = 0) -> pathlib.Path:
     with ThreadPoolExecutor(max_workers=jobs) as pool:
         objs = list(pool.map(lambda s: _compile(cc, s, objdir, verbose), SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs]
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lz"]
     subprocess.run(cmd, check=True)
     tmp.replace(LIB)
     return LIB

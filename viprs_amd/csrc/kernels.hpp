@@ -227,6 +227,19 @@ struct GaussJob {
     int tiles_per_block = 1;  // consecutive tile rows one block walks down
     int gx = 0, gy = 0;
 };
+// Baseline JPEG block stage (jpeg.hip): the quantised DCT coefficients of
+// every 8 x 8 block of every component, zigzag order, components one after
+// another (component c's blocks at [off[c], off[c + 1]), raster order within).
+// mode 0: one grey component; 1: YCbCr 4:4:4; 2: YCbCr 4:2:0.
+struct JpegJob {
+    const unsigned char* px = nullptr;  // u8 pixels, w x h x bands (1 or 3)
+    int w = 0, h = 0, bands = 1, mode = 0;
+    int bw[3] = {}, bh[3] = {};  // blocks per row / column of each component
+    int off[4] = {};             // first block of each component; off[ncomp] = total
+    uint16_t div[2][64] = {};    // luma / chroma divisors (8 x table entry), natural order
+    int16_t* out = nullptr;
+};
+hipError_t launch_jpeg_blocks(const JpegJob& j, hipStream_t s);
 bool gauss_u8_supported(int bands, int taps, int32_t scale);
 hipError_t launch_gauss_u8(const GaussJob& job, hipStream_t s);
 // Fused float H -> V convolution pair (conv_pair.hip): a strip of tw output
