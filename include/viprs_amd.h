@@ -400,9 +400,10 @@ typedef struct {
     uint32_t shrink_tasks;     /* blocks of work in the fused-shrink launches */
     uint32_t reduce_tasks;     /* fused-reduce blocks plus MFMA wavefront tasks */
     uint32_t fallback_images;  /* images run one stage launch at a time */
-    int32_t grid_cap;          /* persistent grid size, 0 = one block per task */
+    int32_t grid_cap;          /* persistent grid size of the shrink launches, 0 = one block per task */
     int32_t use_graph;
     int32_t reduce_first_tile; /* largest reduce tile candidate (0 = 64x16) */
+    int32_t reduce_grid_cap;   /* the same for the reduce and MFMA separable launches */
     uint64_t input_bytes;
     uint64_t output_bytes;
     uint64_t intermediate_bytes;
@@ -416,9 +417,10 @@ int vgpu_batch_size(const vgpu_batch *b);
 int vgpu_batch_output_info(const vgpu_batch *b, uint32_t index, uint32_t *width,
                            uint32_t *height, uint32_t *bands);
 int vgpu_batch_info(const vgpu_batch *b, vgpu_batch_stats *stats);
-/* Launch shape: persistent grid size (0 = one block per task), HIP graph on
- * or off, largest reduce tile candidate (re-plans the batch). */
-int vgpu_batch_configure(vgpu_batch *b, int grid_cap, int use_graph, int reduce_first_tile);
+/* Launch shape: persistent grid size of the shrink launches and of the
+ * reduce / MFMA separable launches (0 = one block per task), HIP graph on or
+ * off, largest reduce tile candidate (re-plans the batch). */
+int vgpu_batch_configure(vgpu_batch *b, int grid_cap, int reduce_grid_cap, int use_graph, int reduce_first_tile);
 /* As vgpu_pipeline_set_option, for every image; re-plans the batch. */
 int vgpu_batch_set_option(vgpu_batch *b, int option, int value);
 /* As vgpu_pipeline_set_integer_mfma, for every image; re-plans the batch. */

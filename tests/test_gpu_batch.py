@@ -70,16 +70,17 @@ def test_batch_launch_shapes_and_replays(ctx):
     exp = expected(srcs, v.FITBOX, 120, 120)
     batch = v.Batch(ctx, sizes, v.FITBOX, 120, 120)
     dsrc = [v.DeviceBuffer.from_numpy(ctx, s) for s in srcs]
-    for grid_cap, graph, tile in [(0, True, 0), (0, False, 0), (64, True, 0), (300, True, 2),
-                                  (0, True, 3), (7, False, 1)]:
-        batch.configure(grid_cap, graph, tile)
+    for grid_cap, graph, tile, rgrid in [(0, True, 0, None), (0, False, 0, None), (64, True, 0, None),
+                                         (300, True, 2, None), (0, True, 3, None), (7, False, 1, None),
+                                         (0, True, 0, 5), (9, True, 0, 0)]:
+        batch.configure(grid_cap, graph, tile, rgrid)
         dst = batch.output_buffers()
         for rep in range(3):  # graph replays, host work and syncs in between
             for d in dst:
                 d.upload(np.zeros(d.shape, np.uint8))
             batch.run(dsrc, dst)
             ctx.synchronize()
-            check_outputs(dst, exp, f"grid {grid_cap} graph {graph} tile {tile} replay {rep}")
+            check_outputs(dst, exp, f"grid {grid_cap}/{rgrid} graph {graph} tile {tile} replay {rep}")
 
 
 def test_batch_outputs_equal_single_pipelines(ctx):

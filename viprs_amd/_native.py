@@ -31,7 +31,8 @@ class BatchStats(C.Structure):
 
     _fields_ = [("images", C.c_uint32), ("launches", C.c_uint32), ("shrink_tasks", C.c_uint32),
                 ("reduce_tasks", C.c_uint32), ("fallback_images", C.c_uint32), ("grid_cap", C.c_int32),
-                ("use_graph", C.c_int32), ("reduce_first_tile", C.c_int32), ("input_bytes", C.c_uint64),
+                ("use_graph", C.c_int32), ("reduce_first_tile", C.c_int32), ("reduce_grid_cap", C.c_int32),
+                ("input_bytes", C.c_uint64),
                 ("output_bytes", C.c_uint64), ("intermediate_bytes", C.c_uint64)]
 
     def as_dict(self):
@@ -192,7 +193,7 @@ def load() -> C.CDLL:
         "vgpu_batch_size": (i32, [vp]),
         "vgpu_batch_output_info": (i32, [vp, u32, P(u32), P(u32), P(u32)]),
         "vgpu_batch_info": (i32, [vp, P(BatchStats)]),
-        "vgpu_batch_configure": (i32, [vp, i32, i32, i32]),
+        "vgpu_batch_configure": (i32, [vp, i32, i32, i32, i32]),
         "vgpu_batch_run": (i32, [vp, P(vp), P(sz), P(vp), P(sz)]),
         "vgpu_batch_run_profiled": (i32, [vp, P(vp), P(sz), P(vp), P(sz), i32, P(f32), P(C.c_uint64), P(i32), i32]),
         "vgpu_batch_benchmark": (i32, [vp, P(vp), P(sz), P(vp), P(sz), i32, i32, P(f32), P(f32)]),

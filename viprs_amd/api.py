@@ -467,8 +467,12 @@ class Batch:
         _check(self._lib.vgpu_batch_info(self._h, C.byref(st)), "vgpu_batch_info")
         return st.as_dict()
 
-    def configure(self, grid_cap: int = 0, use_graph: bool = True, reduce_first_tile: int = 0) -> None:
-        _check(self._lib.vgpu_batch_configure(self._h, grid_cap, int(use_graph), reduce_first_tile),
+    def configure(self, grid_cap: int = 0, use_graph: bool = True, reduce_first_tile: int = 0,
+                  reduce_grid_cap: Optional[int] = None) -> None:
+        """Persistent grid sizes (shrink launches; reduce / MFMA separable
+        launches, by default the same), HIP graph on or off, reduce tile."""
+        rg = grid_cap if reduce_grid_cap is None else reduce_grid_cap
+        _check(self._lib.vgpu_batch_configure(self._h, grid_cap, rg, int(use_graph), reduce_first_tile),
                "vgpu_batch_configure")
 
     def set_integer_mfma(self, enable, shrink: bool = False) -> None:

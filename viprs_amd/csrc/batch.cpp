@@ -79,7 +79,8 @@ struct vgpu_batch {
     std::vector<Launch> launches;
     std::vector<int> fallback;  // images run stage by stage
     int n_shrink_tasks = 0, n_reduce_tasks = 0;
-    int grid_cap = 0;    // persistent grid size; 0 = one block per task
+    int grid_cap = 0;         // persistent grid size of the shrink launches; 0 = one block per task
+    int reduce_grid_cap = 0;  // the same for the reduce and MFMA separable launches
     int use_graph = 1;
     hipGraphExec_t graph = nullptr;
     // Launches that serve images of one band count and size group form a
@@ -405,7 +406,8 @@ int vgpu_batch::launch_all(hipStream_t s, hipEvent_t* marks) {
         }
         // A block takes one task, or (MFMA family) one task per wavefront.
         const int blocks = l.family == 2 ? (l.ntasks + 3) / 4 : l.ntasks;
-        const int grid = grid_cap > 0 ? std::min(blocks, grid_cap) : blocks;
+        const int cap = l.family == 0 ? grid_cap : reduce_grid_cap;
+        const int grid = cap > 0 ? std::min(blocks, cap) : blocks;
         const BatchTask* tasks = reinterpret_cast<const BatchTask*>(t + l.tasks_off);
         hipError_t e;
         if (l.family == 2)
@@ -520,16 +522,18 @@ int vgpu_batch_info(const vgpu_batch* b, vgpu_batch_stats* st) {
         st->output_bytes += (uint64_t)im.ow * im.oh * im.bands;
     }
     st->grid_cap = b->grid_cap;
+    st->reduce_grid_cap = b->reduce_grid_cap;
     st->use_graph = b->use_graph;
     st->reduce_first_tile = b->opt.reduce_first_tile;
     return VGPU_OK;
 }
 
-int vgpu_batch_configure(vgpu_batch* b, int grid_cap, int use_graph, int reduce_first_tile) {
-    if (!b || grid_cap < 0 || reduce_first_tile < 0) return VGPU_E_INVALID_ARGUMENT;
+int vgpu_batch_configure(vgpu_batch* b, int grid_cap, int reduce_grid_cap, int use_graph, int reduce_first_tile) {
+    if (!b || grid_cap < 0 || reduce_grid_cap < 0 || reduce_first_tile < 0) return VGPU_E_INVALID_ARGUMENT;
     (void)hipSetDevice(b->ctx->device);
     if (hipStreamSynchronize(b->ctx->stream) != hipSuccess) return b->ctx->sticky = VGPU_E_DEVICE;
     b->grid_cap = grid_cap;
+    b->reduce_grid_cap = reduce_grid_cap;
     b->use_graph = use_graph ? 1 : 0;
     b->drop_graph();
     if (reduce_first_tile != b->opt.reduce_first_tile) {
@@ -687,29 +691,38 @@ int vgpu_batch_benchmark(vgpu_batch* b, const void* const* srcs, const size_t* s
 int vgpu_batch_tune(vgpu_batch* b, const void* const* srcs, const size_t* src_bytes, void* const* dsts,
                     const size_t* dst_bytes, int iterations, float* best_ms) {
     if (!b || iterations < 1) return VGPU_E_INVALID_ARGUMENT;
-    // Candidates: reduce tile start x persistent grid size (one block per task,
-    // or 4/8/16 blocks per CU walking the list), all launched from a graph.
+    // Candidates: reduce tile start x persistent grid size of the shrink
+    // launches x that of the reduce launches (one block per task, or 4/8/16
+    // blocks per CU walking the list), all launched from a graph.  The
+    // VALU reduce tile only matters when the batch has VALU reduce launches.
     static const int tiles[] = {0, 1, 2, 3};
-    static const int grids[] = {0, 256 * 4, 256 * 8, 256 * 16};
+    static const int sgrids[] = {0, 256 * 8, 256 * 16};
+    static const int rgrids[] = {0, 256 * 4, 256 * 8, 256 * 16};
+    bool valu_reduce = false;
+    for (const Launch& l : b->launches) valu_reduce = valu_reduce || l.family == 1;
     float best = 0.f;
-    int best_tile = b->opt.reduce_first_tile, best_grid = b->grid_cap;
+    int best_tile = b->opt.reduce_first_tile, best_grid = b->grid_cap, best_rgrid = b->reduce_grid_cap;
     bool have = false;
     for (int t : tiles) {
-        for (int g : grids) {
-            int rc = vgpu_batch_configure(b, g, 1, t);
-            if (rc != VGPU_OK) return rc;
-            float ms = 0.f;
-            rc = vgpu_batch_benchmark(b, srcs, src_bytes, dsts, dst_bytes, 1, iterations, &ms, nullptr);
-            if (rc != VGPU_OK) return rc;
-            if (!have || ms < best) {
-                have = true;
-                best = ms;
-                best_tile = t;
-                best_grid = g;
+        if (!valu_reduce && t != tiles[0]) break;
+        for (int g : sgrids) {
+            for (int rg : rgrids) {
+                int rc = vgpu_batch_configure(b, g, rg, 1, t);
+                if (rc != VGPU_OK) return rc;
+                float ms = 0.f;
+                rc = vgpu_batch_benchmark(b, srcs, src_bytes, dsts, dst_bytes, 1, iterations, &ms, nullptr);
+                if (rc != VGPU_OK) return rc;
+                if (!have || ms < best) {
+                    have = true;
+                    best = ms;
+                    best_tile = t;
+                    best_grid = g;
+                    best_rgrid = rg;
+                }
             }
         }
     }
-    const int rc = vgpu_batch_configure(b, best_grid, 1, best_tile);
+    const int rc = vgpu_batch_configure(b, best_grid, best_rgrid, 1, best_tile);
     if (best_ms) *best_ms = best / (float)iterations;
     return rc;
 }

@@ -25,6 +25,12 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const
 }
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+// 16 bytes of a stream read once (nontemporal: the load is not kept in the
+// caches for reuse, which raises the sustained HBM read rate).
+__device__ __forceinline__ u32x4 ld_stream16(const void* p) {
+    return __builtin_nontemporal_load(gmem<u32x4>(p));
+}
+
 // Buffer-relative column / row of an absolute coordinate: clip, then clamp to
 // the buffer.
 __device__ __forceinline__ int col_of(const InView& v, int x) {

@@ -389,7 +389,7 @@ __device__ __forceinline__ void shrink_rows_block(const ShrinkJob& job, int bx, 
         const bool use_ = k_ < nsrc && c_ < nch_;                                                   \
         const bool full_ = use_ && off_ + 16 <= total;                                              \
         VM |= (uint32_t)(full_ ? 1 : (use_ ? 2 : 0)) << (2 * (j));                                  \
-        pj = gmem<u32x4>(base + (full_ ? off_ : 0))[0];                                             \
+        pj = ld_stream16(base + (full_ ? off_ : 0));                                                \
     } while (0)
 // The row of register j's chunks is wave-uniform when a slot spans at least
 // one register (lsl >= 6): that branch keeps the address math scalar.
@@ -672,15 +672,15 @@ __device__ __forceinline__ void shrink_rows_lean_nd(const ShrinkJob& job, int bx
         row_fetch(k0, G.a0, b0, l0, G.c0, G.s0);
         if (two) row_fetch(k0 + 1, G.a1, b1, l1, G.c1, G.s1);
         if (two) {
-            G.p0 = gmem<u32x4>(base + b0 + (uint32_t)min(lane, l0 - 1) * 16u)[0];
-            G.p1 = gmem<u32x4>(base + b0 + (uint32_t)min(lane + 64, l0 - 1) * 16u)[0];
-            G.p2 = gmem<u32x4>(base + b1 + (uint32_t)min(lane, l1 - 1) * 16u)[0];
-            G.p3 = gmem<u32x4>(base + b1 + (uint32_t)min(lane + 64, l1 - 1) * 16u)[0];
+            G.p0 = ld_stream16(base + b0 + (uint32_t)min(lane, l0 - 1) * 16u);
+            G.p1 = ld_stream16(base + b0 + (uint32_t)min(lane + 64, l0 - 1) * 16u);
+            G.p2 = ld_stream16(base + b1 + (uint32_t)min(lane, l1 - 1) * 16u);
+            G.p3 = ld_stream16(base + b1 + (uint32_t)min(lane + 64, l1 - 1) * 16u);
         } else {
-            G.p0 = gmem<u32x4>(base + b0 + (uint32_t)min(lane, l0 - 1) * 16u)[0];
-            G.p1 = gmem<u32x4>(base + b0 + (uint32_t)min(lane + 64, l0 - 1) * 16u)[0];
-            G.p2 = gmem<u32x4>(base + b0 + (uint32_t)min(lane + 128, l0 - 1) * 16u)[0];
-            G.p3 = gmem<u32x4>(base + b0 + (uint32_t)min(lane + 192, l0 - 1) * 16u)[0];
+            G.p0 = ld_stream16(base + b0 + (uint32_t)min(lane, l0 - 1) * 16u);
+            G.p1 = ld_stream16(base + b0 + (uint32_t)min(lane + 64, l0 - 1) * 16u);
+            G.p2 = ld_stream16(base + b0 + (uint32_t)min(lane + 128, l0 - 1) * 16u);
+            G.p3 = ld_stream16(base + b0 + (uint32_t)min(lane + 192, l0 - 1) * 16u);
         }
         return G;
     };
