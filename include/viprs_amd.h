@@ -560,8 +560,10 @@ typedef enum {
  * size; VGPU_E_BUFFER_SIZE when out_cap is smaller (out may be NULL to ask). */
 int vgpu_jpeg_encode(vgpu_context *ctx, const void *pixels, uint32_t width, uint32_t height, uint32_t bands,
                      int quality, int subsampling, void *out, size_t out_cap, size_t *out_len);
-/* PNG decode of host bytes (8-bit grey, grey + alpha, RGB, RGBA, palette ->
- * RGB; not interlaced): the size into *width, *height, *bands, and, when
+/* PNG decode of host bytes to u8 (png/decode_full.rs:101-108, 470-520:
+ * grey, grey + alpha, RGB, RGBA, palette -> RGB; bit depths 1, 2, 4 expanded
+ * to 8; tRNS -> an alpha band; Adam7 interlacing undone; 16-bit files return
+ * VGPU_E_UNSUPPORTED_FORMAT): the size into *width, *height, *bands, and, when
  * pixels is not NULL, the samples (VGPU_E_BUFFER_SIZE if cap is short). */
 int vgpu_png_decode(const void *data, size_t len, uint32_t *width, uint32_t *height, uint32_t *bands, void *pixels,
                     size_t cap);

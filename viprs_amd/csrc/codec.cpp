@@ -6,8 +6,9 @@
 //    tables), markers, the sequential Huffman pass with the DC predictors and
 //    the libjpeg-turbo encoder's dummy blocks at the right and bottom edges
 //    of 4:2:0 images (DC of the neighbour the encoder copies, AC zero);
-//  * PNG: decode (8-bit grey, grey + alpha, RGB, RGBA, palette; not
-//    interlaced) and encode with zlib, lossless;
+//  * PNG: decode (grey, grey + alpha, RGB, RGBA, palette; bit depths 1, 2,
+//    4 and 8 expanded to u8, tRNS to alpha, Adam7 interlacing undone) and
+//    encode with zlib, lossless;
 //  * vgpu_process: decode, the caller's pipeline on the device, encode.
 #include <zlib.h>
 
