@@ -229,7 +229,12 @@ int vgpu_pipeline_set_float_mode(vgpu_pipeline *p, int mode);
 typedef enum {
     VGPU_OPT_SEP_ROWS_PER_TASK = 2, /* output rows per MFMA wavefront task: 0 = auto, else a multiple of 16 */
     VGPU_OPT_SHRINK_RUN = 3,        /* box-shrink output rows per wavefront run: 0 = auto, 1..64 */
-    VGPU_OPT_GAUSS_ROWS = 4         /* u8 3/4-band Gaussian VALU tile rows: 0 = auto, 16 or 32; -1 = the MFMA kernel */
+    VGPU_OPT_GAUSS_ROWS = 4,        /* u8 3/4-band Gaussian VALU tile rows: 0 = auto, 16 or 32; -1 = the MFMA kernel */
+    VGPU_OPT_BATCH_LAST_SHARE = 5,  /* batch only: percent (0..90) of the box-shrink bytes, largest images first,
+                                       given to the last launch chain so the other chains' reduce tails run
+                                       under its shrinks; 0 = chains balanced by bytes */
+    VGPU_OPT_SEP_FORM = 6           /* int8 MFMA pairs: 0 = block tiles staged in LDS where they fit (default),
+                                       1 = one wavefront per strip task streaming its operands */
 } vgpu_option;
 int vgpu_pipeline_set_option(vgpu_pipeline *p, int option, int value);
 /* CompiledPipeline output geometry (pipeline/compiled.rs:338-). */
@@ -407,6 +412,7 @@ typedef struct {
     uint64_t input_bytes;
     uint64_t output_bytes;
     uint64_t intermediate_bytes;
+    int32_t last_share;        /* VGPU_OPT_BATCH_LAST_SHARE in effect */
 } vgpu_batch_stats;
 
 int vgpu_batch_thumbnail_new(vgpu_context *ctx, uint32_t count, const uint32_t *widths,
@@ -434,7 +440,7 @@ int vgpu_batch_run(vgpu_batch *b, const void *const *srcs, const size_t *src_byt
  * `iterations` times with HIP events between launches; writes the mean ms,
  * the algorithmic bytes (input buffers + outputs of its jobs) and the kernel
  * family (0 = fused shrink, 1 = fused reduce, 2 = int8-limb MFMA separable
- * pair) of each launch; launch_bytes and
+ * pair, 3 = its block-tile form) of each launch; launch_bytes and
  * launch_family may be NULL.  Returns the launch count or a negative status. */
 int vgpu_batch_run_profiled(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,
                             void *const *dsts, const size_t *dst_bytes, int iterations,
@@ -444,8 +450,9 @@ int vgpu_batch_run_profiled(vgpu_batch *b, const void *const *srcs, const size_t
 int vgpu_batch_benchmark(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,
                          void *const *dsts, const size_t *dst_bytes, int warmup, int iterations,
                          float *total_ms, float *step_ms);
-/* Autotuner: times every (reduce tile, grid size) candidate on these buffers
- * and keeps the fastest; best_ms (may be NULL) receives its ms per run. */
+/* Autotuner: times every (reduce tile, grid sizes, last-chain share)
+ * candidate on these buffers and keeps the fastest; best_ms (may be NULL)
+ * receives its ms per run. */
 int vgpu_batch_tune(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,
                     void *const *dsts, const size_t *dst_bytes, int iterations, float *best_ms);
 

@@ -83,6 +83,28 @@ def test_batch_launch_shapes_and_replays(ctx):
             check_outputs(dst, exp, f"grid {grid_cap}/{rgrid} graph {graph} tile {tile} replay {rep}")
 
 
+@pytest.mark.parametrize("share", [25, 45, 90])
+def test_batch_staggered_chains_keep_results(ctx, share):
+    # The last launch chain takes the largest images up to `share` percent
+    # of the box-shrink bytes; the grouping moves work between streams only.
+    sizes = random_sizes(20, 321, 200, 2600) + [(64, 48, 3), (150, 150, 4)]
+    srcs = sources(sizes, 17)
+    exp = expected(srcs, v.FITBOX, 160, 160)
+    batch = v.Batch(ctx, sizes, v.FITBOX, 160, 160)
+    batch.set_option(v.api.OPT_BATCH_LAST_SHARE, share)
+    assert batch.info()["last_share"] == share
+    dsrc = [v.DeviceBuffer.from_numpy(ctx, s) for s in srcs]
+    dst = batch.output_buffers()
+    for rep in range(3):
+        for d in dst:
+            d.upload(np.zeros(d.shape, np.uint8))
+        batch.run(dsrc, dst)
+        ctx.synchronize()
+        check_outputs(dst, exp, f"last share {share} replay {rep}")
+    with pytest.raises(v.VgpuError):
+        batch.set_option(v.api.OPT_BATCH_LAST_SHARE, 91)
+
+
 def test_batch_outputs_equal_single_pipelines(ctx):
     sizes = [(1031, 777, 3), (640, 1000, 4), (300, 200, 1), (1500, 97, 2)]
     srcs = sources(sizes, 11)

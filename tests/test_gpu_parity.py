@@ -489,12 +489,13 @@ U8_PAIR_CASES = [
 ]
 
 
-@pytest.mark.parametrize("mfma", [True, False])
+@pytest.mark.parametrize("form", ["tile", "wave", "plain"])
 @pytest.mark.parametrize("bands", [1, 2, 3, 4])
-def test_u8_pairs_mfma_and_plain(ctx, mfma, bands):
+def test_u8_pairs_mfma_and_plain(ctx, form, bands):
     # u8 ReduceH -> ReduceV and GaussBlurH -> GaussBlurV run on the int8-limb
-    # MFMA kernel by default and on the plain integer kernels when it is
-    # switched off; both must equal the oracle bit for bit, edges included.
+    # MFMA kernel by default (block tiles staged in LDS, or one wavefront per
+    # strip task) and on the plain integer kernels when it is switched off;
+    # every form must equal the oracle bit for bit, edges included.
     src = noise_u8(211, 389, bands, 60 + bands)
     if bands in (2, 4):
         src[..., -1] = np.where(np.random.default_rng(bands).random((211, 389)) < 0.1, 0, src[..., -1])
@@ -504,13 +505,16 @@ def test_u8_pairs_mfma_and_plain(ctx, mfma, bands):
                       ("unpremultiply", ())])
     for steps in cases:
         h, w, b = src.shape
-        gp = build(v.Pipeline(ctx, w, h, b), steps).set_integer_mfma(mfma)
+        gp = build(v.Pipeline(ctx, w, h, b), steps).set_integer_mfma(form != "plain")
+        if form == "wave":
+            gp.set_option(v.api.OPT_SEP_FORM, 1)
         exp = build(O.Pipeline(w, h, b), steps).run(src)
-        assert_same(gp.run_host(src), exp, f"mfma={mfma} bands {bands} {steps}")
+        assert_same(gp.run_host(src), exp, f"form={form} bands {bands} {steps}")
 
 
+@pytest.mark.parametrize("form", [0, 1])
 @pytest.mark.parametrize("bands", [1, 2, 3, 4])
-def test_box_shrink_on_mfma(ctx, bands):
+def test_box_shrink_on_mfma(ctx, bands, form):
     # The box pre-shrink as an MFMA contraction (opt-in): 0/1 weights with edge
     # counts folded, the reference's per-factor roundings, premultiply fused.
     src = noise_u8(173, 311, bands, 90 + bands)
@@ -522,6 +526,7 @@ def test_box_shrink_on_mfma(ctx, bands):
     for steps in cases:
         h, w, b = src.shape
         gp = build(v.Pipeline(ctx, w, h, b), steps).set_integer_mfma(True, shrink=True)
+        gp.set_option(v.api.OPT_SEP_FORM, form)
         exp = build(O.Pipeline(w, h, b), steps).run(src)
         assert_same(gp.run_host(src), exp, f"mfma shrink bands {bands} {steps}")
 

@@ -38,7 +38,7 @@ E_ADMISSION = -9
 E_NOT_READY = -10
 E_SCRATCH_LIMIT = -11
 NOALLOC_SCRATCH_BYTES = 64 << 20
-OPT_SEP_ROWS_PER_TASK, OPT_SHRINK_RUN, OPT_GAUSS_ROWS = 2, 3, 4
+OPT_SEP_ROWS_PER_TASK, OPT_SHRINK_RUN, OPT_GAUSS_ROWS, OPT_BATCH_LAST_SHARE, OPT_SEP_FORM = 2, 3, 4, 5, 6
 
 NP_DTYPE = {U8: np.uint8, U16: np.uint16, F32: np.float32, I16: np.int16, U32: np.uint32, I32: np.int32,
             F64: np.float64}
@@ -513,7 +513,7 @@ class Batch:
         n = self._lib.vgpu_batch_run_profiled(self._h, *self._tables(srcs, dsts), iterations, ms, nb, fam, 64)
         if n < 0:
             raise VgpuError(n, "vgpu_batch_run_profiled")
-        names = {0: "shrink", 1: "reduce", 2: "sep_mfma"}
+        names = {0: "shrink", 1: "reduce", 2: "sep_mfma", 3: "sep_tile"}
         return [(ms[i], int(nb[i]), names[fam[i]]) for i in range(n)]
 
     def tune(self, srcs, dsts, iterations: int = 3) -> float:

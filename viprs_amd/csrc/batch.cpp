@@ -41,7 +41,7 @@ struct BatchImage {
 
 // One persistent launch over the tasks of one (level, kernel family).
 struct Launch {
-    int family = 0;  // 0: fused shrink, 1: fused reduce (VALU), 2: MFMA separable pair
+    int family = 0;  // 0: fused shrink, 1: fused reduce (VALU), 2: MFMA separable pair, 3: its block-tile form
     int variant = 0; // kernel variant within the family
     size_t jobs_off = 0, tasks_off = 0;
     int ntasks = 0;
@@ -79,6 +79,7 @@ struct vgpu_batch {
     std::vector<Launch> launches;
     std::vector<int> fallback;  // images run stage by stage
     int n_shrink_tasks = 0, n_reduce_tasks = 0;
+    int last_share = 0;       // percent of the shrink bytes the last chain takes (largest images); 0 = balanced
     int grid_cap = 0;         // persistent grid size of the shrink launches; 0 = one block per task
     int reduce_grid_cap = 0;  // the same for the reduce and MFMA separable launches
     int use_graph = 1;
@@ -209,23 +210,46 @@ int vgpu_batch::plan() {
                (uint64_t)images[(size_t)b].w * images[(size_t)b].h * images[(size_t)b].bands;
     });
 
-    // Greedy balance of the groups by the bytes each image's stages move.
+    // Groups.  Balanced: images dealt largest first to the group that moves
+    // the fewest bytes so far.  Staggered (last_share > 0): the largest
+    // images go to the last group until it holds last_share percent of the
+    // box-shrink bytes, the rest are balanced over the other groups with the
+    // reduce stages weighted by their lower rate, so those groups finish
+    // their reduce tails while the last group still streams its shrinks and
+    // only the last group's (few images') reduce work trails.
     std::vector<int> group_of(count, 0);
     {
-        uint64_t load[kGroups] = {};
-        for (int idx : order) {
-            const BatchImage& im = images[(size_t)idx];
-            uint64_t bytes = 0;
+        std::vector<uint64_t> shrink_b(count, 0), other_b(count, 0);
+        uint64_t shrink_total = 0;
+        for (size_t i = 0; i < count; ++i) {
+            const BatchImage& im = images[i];
             for (size_t k = 0; k < im.stages.size(); ++k) {
                 const Rect in_rect = k == 0 ? whole(im.w, im.h) : im.stages[k - 1].out;
-                bytes += ((uint64_t)in_rect.w * in_rect.h + (uint64_t)im.stages[k].out.w * im.stages[k].out.h) *
-                         im.bands;
+                const uint64_t b =
+                    ((uint64_t)in_rect.w * in_rect.h + (uint64_t)im.stages[k].out.w * im.stages[k].out.h) * im.bands;
+                (im.stages[k].kind == ST_SHRINK_HV_U8 ? shrink_b[i] : other_b[i]) += b;
             }
+            shrink_total += shrink_b[i];
+        }
+        // The MFMA reduce tails move their bytes about ten times slower than
+        // the box shrinks stream theirs.
+        constexpr uint64_t kReduceWeight = 10;
+        const int spread = last_share > 0 ? kGroups - 1 : kGroups;
+        uint64_t load[kGroups] = {};
+        uint64_t last_bytes = 0;
+        for (int idx : order) {
+            const size_t i = (size_t)idx;
+            if (last_share > 0 && last_bytes * 100 < shrink_total * (uint64_t)last_share) {
+                group_of[i] = kGroups - 1;
+                last_bytes += shrink_b[i];
+                continue;
+            }
+            const uint64_t cost = last_share > 0 ? shrink_b[i] + kReduceWeight * other_b[i] : shrink_b[i] + other_b[i];
             int g = 0;
-            for (int c = 1; c < kGroups; ++c)
+            for (int c = 1; c < spread; ++c)
                 if (load[c] < load[g]) g = c;
-            group_of[(size_t)idx] = g;
-            load[g] += bytes;
+            group_of[i] = g;
+            load[g] += cost;
         }
     }
     Staging tbl;
@@ -246,6 +270,10 @@ int vgpu_batch::plan() {
         std::vector<BatchTask> ptasks[kP];
         size_t plds[kP] = {};
         uint64_t pbytes[kP] = {};
+        std::vector<SepJob> tjobs[kP];
+        std::vector<BatchTask> ttasks[kP];
+        size_t tlds[kP] = {};
+        uint64_t tbytes[kP] = {};
         for (size_t rank = 0; rank < order.size(); ++rank) {
             const int idx = order[rank];
             const int grp = group_of[(size_t)idx];
@@ -273,6 +301,19 @@ int vgpu_batch::plan() {
                 sbytes[var] += io_bytes;
                 for (int by = 0; by < j.gy; ++by)
                     for (int bx = 0; bx < j.gx; ++bx) stasks[var].push_back(BatchTask{jid, bx, by});
+            } else if (s.kind == ST_SEP && s.sep.tile) {
+                // Block-tile form: one task per block (four strips x R rows).
+                SepJob j = sep_job_of(s, in, out, dblob);
+                j.src_slot = first ? idx : -1;
+                j.dst_slot = last ? idx : -1;
+                const int var = sep_job_variant(j) * kGroups + grp;
+                const int jid = (int)tjobs[var].size();
+                tjobs[var].push_back(j);
+                tbytes[var] += io_bytes;
+                tlds[var] = std::max(tlds[var], (size_t)j.t_lds);
+                const int gxb = (j.gx + 3) / 4;
+                for (int by = 0; by < j.gy; ++by)
+                    for (int bx = 0; bx < gxb; ++bx) ttasks[var].push_back(BatchTask{jid, bx, by});
             } else if (s.kind == ST_SEP) {
                 SepJob j = sep_job_of(s, in, out, dblob);
                 j.src_slot = first ? idx : -1;
@@ -323,6 +364,20 @@ int vgpu_batch::plan() {
             l.ntasks = (int)ptasks[vg].size();
             l.lds = plds[vg];  // per wavefront
             l.alg_bytes = pbytes[vg];
+            launches.push_back(l);
+            n_reduce_tasks += l.ntasks;
+        }
+        for (int vg = 0; vg < kP; ++vg) {
+            if (ttasks[vg].empty()) continue;
+            Launch l;
+            l.family = 3;
+            l.variant = vg / kGroups;
+            l.chain = vg % kGroups;
+            l.jobs_off = tbl.put(tjobs[vg].data(), tjobs[vg].size() * sizeof(SepJob));
+            l.tasks_off = tbl.put(ttasks[vg].data(), ttasks[vg].size() * sizeof(BatchTask));
+            l.ntasks = (int)ttasks[vg].size();
+            l.lds = tlds[vg];  // per block
+            l.alg_bytes = tbytes[vg];
             launches.push_back(l);
             n_reduce_tasks += l.ntasks;
         }
@@ -410,7 +465,10 @@ int vgpu_batch::launch_all(hipStream_t s, hipEvent_t* marks) {
         const int grid = cap > 0 ? std::min(blocks, cap) : blocks;
         const BatchTask* tasks = reinterpret_cast<const BatchTask*>(t + l.tasks_off);
         hipError_t e;
-        if (l.family == 2)
+        if (l.family == 3)
+            e = launch_sep_tile_batch(l.variant, reinterpret_cast<const SepJob*>(t + l.jobs_off), tasks, l.ntasks,
+                                      srcs, dsts, grid, l.lds, ls);
+        else if (l.family == 2)
             e = launch_sep_batch(l.variant, reinterpret_cast<const SepJob*>(t + l.jobs_off), tasks, l.ntasks, srcs,
                                  dsts, grid, l.lds, ls);
         else if (l.family == 0)
@@ -525,6 +583,7 @@ int vgpu_batch_info(const vgpu_batch* b, vgpu_batch_stats* st) {
     st->reduce_grid_cap = b->reduce_grid_cap;
     st->use_graph = b->use_graph;
     st->reduce_first_tile = b->opt.reduce_first_tile;
+    st->last_share = b->last_share;
     return VGPU_OK;
 }
 
@@ -559,6 +618,14 @@ int vgpu_batch_set_option(vgpu_batch* b, int option, int value) {
     case VGPU_OPT_GAUSS_ROWS:
         if (value != -1 && value != 0 && value != 16 && value != 32) return VGPU_E_INVALID_ARGUMENT;
         b->opt.gauss_rows = value;
+        break;
+    case VGPU_OPT_BATCH_LAST_SHARE:
+        if (value < 0 || value > 90) return VGPU_E_INVALID_ARGUMENT;
+        b->last_share = value;
+        break;
+    case VGPU_OPT_SEP_FORM:
+        if (value != 0 && value != 1) return VGPU_E_INVALID_ARGUMENT;
+        b->opt.sep_form = value;
         break;
     default: return VGPU_E_INVALID_ARGUMENT;
     }
@@ -691,36 +758,56 @@ int vgpu_batch_benchmark(vgpu_batch* b, const void* const* srcs, const size_t* s
 int vgpu_batch_tune(vgpu_batch* b, const void* const* srcs, const size_t* src_bytes, void* const* dsts,
                     const size_t* dst_bytes, int iterations, float* best_ms) {
     if (!b || iterations < 1) return VGPU_E_INVALID_ARGUMENT;
-    // Candidates: reduce tile start x persistent grid size of the shrink
-    // launches x that of the reduce launches (one block per task, or 4/8/16
-    // blocks per CU walking the list), all launched from a graph.  The
-    // VALU reduce tile only matters when the batch has VALU reduce launches.
+    // Candidates: last-chain share x reduce tile start x persistent grid
+    // size of the shrink launches x that of the reduce launches (one block
+    // per task, or 4/8/16 blocks per CU walking the list), all launched from
+    // a graph.  The VALU reduce tile only matters when the batch has VALU
+    // reduce launches.
+    static const int shares[] = {0, 25, 35, 45};
     static const int tiles[] = {0, 1, 2, 3};
     static const int sgrids[] = {0, 256 * 8, 256 * 16};
     static const int rgrids[] = {0, 256 * 4, 256 * 8, 256 * 16};
-    bool valu_reduce = false;
-    for (const Launch& l : b->launches) valu_reduce = valu_reduce || l.family == 1;
     float best = 0.f;
     int best_tile = b->opt.reduce_first_tile, best_grid = b->grid_cap, best_rgrid = b->reduce_grid_cap;
+    int best_share = b->last_share;
     bool have = false;
-    for (int t : tiles) {
-        if (!valu_reduce && t != tiles[0]) break;
-        for (int g : sgrids) {
-            for (int rg : rgrids) {
-                int rc = vgpu_batch_configure(b, g, rg, 1, t);
-                if (rc != VGPU_OK) return rc;
-                float ms = 0.f;
-                rc = vgpu_batch_benchmark(b, srcs, src_bytes, dsts, dst_bytes, 1, iterations, &ms, nullptr);
-                if (rc != VGPU_OK) return rc;
-                if (!have || ms < best) {
-                    have = true;
-                    best = ms;
-                    best_tile = t;
-                    best_grid = g;
-                    best_rgrid = rg;
+    for (int sh : shares) {
+        if (sh != b->last_share) {
+            (void)hipStreamSynchronize(b->ctx->stream);
+            b->last_share = sh;
+            b->drop_graph();
+            const int rc = b->plan();
+            if (rc != VGPU_OK) return rc;
+        }
+        bool valu_reduce = false;
+        for (const Launch& l : b->launches) valu_reduce = valu_reduce || l.family == 1;
+        for (int t : tiles) {
+            if (!valu_reduce && t != tiles[0]) break;
+            for (int g : sgrids) {
+                for (int rg : rgrids) {
+                    int rc = vgpu_batch_configure(b, g, rg, 1, t);
+                    if (rc != VGPU_OK) return rc;
+                    float ms = 0.f;
+                    rc = vgpu_batch_benchmark(b, srcs, src_bytes, dsts, dst_bytes, 1, iterations, &ms, nullptr);
+                    if (rc != VGPU_OK) return rc;
+                    if (!have || ms < best) {
+                        have = true;
+                        best = ms;
+                        best_tile = t;
+                        best_grid = g;
+                        best_rgrid = rg;
+                        best_share = sh;
+                    }
                 }
             }
         }
+    }
+    if (best_share != b->last_share) {
+        (void)hipStreamSynchronize(b->ctx->stream);
+        b->last_share = best_share;
+        b->drop_graph();
+        const int rc = b->plan();
+        if (rc != VGPU_OK) return rc;
     }
     const int rc = vgpu_batch_configure(b, best_grid, best_rgrid, 1, best_tile);
     if (best_ms) *best_ms = best / (float)iterations;

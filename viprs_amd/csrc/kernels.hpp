@@ -139,6 +139,15 @@ struct SepJob {
     int premult = 0, unpremult = 0;
     int gx = 0, gy = 0;  // column strips, row chunks
     int src_slot = -1, dst_slot = -1;
+    // Block-tile form (tile != 0): a block of four wavefronts takes four
+    // adjacent strips x rows_per_task output rows, with its input rows (each
+    // t_row_stride bytes, at most t_rows of them), the intermediate columns
+    // (t_inter_stride bytes per sample) and the V operands of its 16-row
+    // tiles staged in t_lds bytes of LDS; H operands come from htab / hstrip
+    // for both families.  t_chunk_magic = ceil(2^32 / (t_row_stride / 16)).
+    int tile = 0;
+    int t_row_stride = 0, t_rows = 0, t_inter_stride = 0, t_lds = 0;
+    uint32_t t_chunk_magic = 0;
 };
 
 struct BatchTask {
@@ -188,6 +197,11 @@ int sep_job_variant(const SepJob& job);
 hipError_t launch_sep_batch(int variant, const SepJob* jobs, const BatchTask* tasks, int ntasks,
                             const void* const* srcs, void* const* dsts, int grid, size_t wave_lds,
                             hipStream_t s);
+// The block-tile form over a task list: one (job, bx, by) task per block,
+// bx = strip group of four, by = row chunk; lds = the largest t_lds.
+hipError_t launch_sep_tile_batch(int variant, const SepJob* jobs, const BatchTask* tasks, int ntasks,
+                                 const void* const* srcs, void* const* dsts, int grid, size_t lds,
+                                 hipStream_t s);
 // Float separable pair in plain FP32: f32 weights, FMA accumulation, f32
 // intermediate (ReduceJob tiling; ph / pv carry wf32).
 hipError_t launch_red_f32_job(const ReduceJob& job, hipStream_t s);

@@ -320,7 +320,70 @@ bool plan_sep(int family, const SepAxis& hax, const SepAxis& vax, int32_t hbias,
     stg->v_tab = vp.put(st, &j->vtab_bytes);
     stg->v_tile = st->put(vtile.data(), vtile.size() * sizeof(int32_t));
 
-    if (family == 1) {
+    // Block-tile form: four strips x R output rows per block with its input
+    // rows, intermediate columns and V operands staged in LDS (H operands
+    // from host-folded patterns for both families).  R: opt.sep_rows_per_task
+    // if set, else the largest of 64 / 32 / 16 that still gives 1024 blocks
+    // (16 otherwise); the form is used when that R needs at most 48 KiB.
+    j->tile = 0;
+    if (opt.sep_form == 0 && kh <= (limbs == 1 ? 6 : 3)) {
+        std::vector<int64_t> span0s((size_t)nstrips);
+        for (int64_t jx = 0; jx < nstrips; ++jx) span0s[(size_t)jx] = hcol(hs[(size_t)(16 * jx / bands)]);
+        const int64_t gxb = (nstrips + 3) / 4;
+        int64_t rsb = 0;
+        for (int64_t bx = 0; bx < gxb; ++bx) {
+            const int64_t f = span0s[(size_t)(4 * bx)], l = span0s[(size_t)std::min(4 * bx + 3, nstrips - 1)];
+            rsb = std::max(rsb, (l - f) * (int64_t)bands + 64 * kh + 8);
+        }
+        rsb = (rsb + 15) / 16 * 16;
+        struct Geo {
+            int64_t rows = 0, rs = 0, lds = 0, blocks = 0;
+        };
+        auto geo = [&](int64_t R) {
+            Geo g;
+            const int64_t gyb = (out.h + R - 1) / R;
+            for (int64_t by = 0; by < gyb; ++by) {
+                const int64_t t0 = by * R / 16, t1 = (std::min(out.h, by * R + R) - 1) / 16;
+                const int64_t r_lo = vtile[(size_t)(3 * t0 + 1)], r_hi = vtile[(size_t)(3 * t1 + 2)];
+                g.rows = std::max(g.rows, r_hi - r_lo + 1);
+                g.rs = std::max(g.rs, vtile[(size_t)(3 * t1 + 1)] - r_lo + 64 * (int64_t)kv);
+            }
+            g.rs = (g.rs + 15) / 16 * 16;
+            if ((g.rs / 16) % 2 == 0) g.rs += 16;  // odd 16-byte stride: sample columns spread over the banks
+            g.lds = g.rows * rsb + 64 * g.rs + (R / 16) * (int64_t)kv * limbs * 1024;
+            g.blocks = gxb * gyb;
+            return g;
+        };
+        int64_t R = 0;
+        Geo g;
+        if (opt.sep_rows_per_task > 0) {
+            for (int64_t r = opt.sep_rows_per_task & ~15; r >= 16 && !R; r = (r / 2) & ~15) {
+                g = geo(r);
+                if (g.lds <= 48 * 1024) R = r;
+            }
+        } else {
+            for (int64_t r : {64, 32, 16}) {
+                const Geo c = geo(r);
+                if (c.lds > 48 * 1024) continue;
+                R = r;
+                g = c;
+                if (c.blocks >= 1024) break;
+            }
+        }
+        if (R > 0) {
+            j->tile = 1;
+            j->rows_per_task = (int)R;
+            j->gy = (int)((out.h + R - 1) / R);
+            j->t_row_stride = (int)rsb;
+            j->t_rows = (int)g.rows;
+            j->t_inter_stride = (int)g.rs;
+            j->t_lds = (int)g.lds;
+            const uint64_t chunks = (uint64_t)rsb / 16;
+            j->t_chunk_magic = (uint32_t)(((1ull << 32) + chunks - 1) / chunks);
+        }
+    }
+
+    if (family == 1 || j->tile) {
         // H operand patterns per 16-sample strip, with each strip's first
         // span pixel.
         PatternSet hp;
@@ -369,9 +432,8 @@ bool plan_sep(int family, const SepAxis& hax, const SepAxis& vax, int32_t hbias,
         }
         stg->h_tab = hp.put(st, &j->htab_bytes);
         stg->h_strip = st->put(hstrip.data(), hstrip.size() * sizeof(int32_t));
-    } else {
-        j->h_bias = hbias;
     }
+    if (family == 0) j->h_bias = hbias;
     stg->kind = ST_SEP;
     return true;
 }
@@ -1926,6 +1988,11 @@ int vgpu_pipeline_set_option(vgpu_pipeline* p, int option, int value) {
         if (value != -1 && value != 0 && value != 16 && value != 32) return VGPU_E_INVALID_ARGUMENT;
         p->release();
         p->opt.gauss_rows = value;
+        return VGPU_OK;
+    case VGPU_OPT_SEP_FORM:
+        if (value != 0 && value != 1) return VGPU_E_INVALID_ARGUMENT;
+        p->release();
+        p->opt.sep_form = value;
         return VGPU_OK;
     default: return VGPU_E_INVALID_ARGUMENT;
     }

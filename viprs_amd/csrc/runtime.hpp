@@ -193,6 +193,7 @@ struct StageOptions {
     bool shrink_row_split = true;      // split a shrink row's source rows over wavefronts
     int shrink_run = 0;                // shrink output rows per wavefront run (0 = planner's choice)
     int gauss_rows = 0;                // u8 3/4-band Gaussian pair: VALU tile rows (0 = auto, 16, 32), -1 = MFMA kernel
+    int sep_form = 0;                  // int8 MFMA pairs: 0 = block-tile form where it fits, 1 = wavefront tasks
 };
 
 Rect whole(uint32_t w, uint32_t h);

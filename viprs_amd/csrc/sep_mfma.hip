@@ -552,6 +552,237 @@ __global__ __launch_bounds__(kThreads) void k_sep_batch(const SepJob* __restrict
     }
 }
 
+// ---- Block-tile form ----------------------------------------------------
+// A block of four wavefronts owns four adjacent 16-sample strips x R output
+// rows (R = rows_per_task).  Everything the block reads -- the input rows its
+// V windows need (intermediate rows r_lo .. r_hi, each row of input
+// row_of(r)) across the four strips' H spans, and the V operand patterns of
+// its 16-row tiles -- goes into LDS first by LDS-direct 16-byte loads, all in
+// flight at once, so the block waits one memory latency.  Each wavefront
+// then runs its strip from LDS: the H M tiles into its own intermediate
+// columns (16 samples x the block's rows, column-major, already offset for
+// the V operand), then the V tiles.  No wavefront reads another's columns,
+// so the only block barrier is the one after the loads.
+//
+// Staged row i (intermediate row r_lo + i) holds the bytes of input row
+// row_of(r_lo + i) from the dword below the tile's first span byte, so the
+// row's byte j sits at offset mis(i) + j, mis(i) = (row address + tile byte
+// 0) & 3.  Chunks past the end of the buffer are read bytewise, zero beyond.
+template <int B, int LIMBS, int FAM, bool PM>
+__device__ void sep_tile_task(const SepJob& job, int bx, int by, unsigned char* lds) {
+    constexpr int KMAX = max_steps<LIMBS>();
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const InView& in = job.in;
+    const OutView& out = job.out;
+    const int KH = job.kh, KV = job.kv;
+    const int nsamp = out.w * B;
+    const int jx0 = 4 * bx;
+    const int nst = min(4, job.gx - jx0);
+    const int R = job.rows_per_task;
+    const int y_begin = by * R, y_end = min(out.h, y_begin + R);
+    const int t0 = y_begin >> 4, t1 = (y_end - 1) >> 4;
+    const int r_lo = cload(job.vtile, 3 * t0 + 1);
+    const int r_hi = cload(job.vtile, 3 * t1 + 2);
+    const int nrows = r_hi - r_lo + 1;
+    const int c_first = cload(job.hstrip, 2 * jx0 + 1);
+    const unsigned char* base = static_cast<const unsigned char*>(in.ptr);
+    const uint32_t row_bytes = (uint32_t)in.bw * B;
+    const uint32_t total = row_bytes * (uint32_t)in.bh;  // < 2^31 (checked on the host)
+    const uint32_t byte0 = (uint32_t)c_first * B;
+    const int RSB = job.t_row_stride, RS = job.t_inter_stride;
+    const int nchunk_row = RSB >> 4;
+    const int VSZ = KV * LIMBS * 1024;
+    unsigned char* tile = lds;
+    unsigned char* inter = lds + (size_t)job.t_rows * RSB;
+    unsigned char* vstage = inter + (size_t)64 * RS;
+
+    // ---- Stage the input rows: chunk e = (row e / nchunk_row, 16-byte
+    // column e % nchunk_row); a wavefront's 64 chunks land contiguously.
+    const int nchunks = nrows * nchunk_row;
+    auto chunk_addr = [&](int e) -> uint32_t {
+        const int i = (int)__umulhi((uint32_t)e, job.t_chunk_magic);
+        const int c = e - i * nchunk_row;
+        const uint32_t ra = (uint32_t)row_of(in, r_lo + min(i, nrows - 1)) * row_bytes + byte0;
+        return (ra & ~3u) + 16u * (uint32_t)c;
+    };
+    for (int e0 = wave * 64; e0 < nchunks; e0 += 256) {
+        const int e = e0 + lane;
+        const uint32_t a = chunk_addr(e);
+        if (e < nchunks) lds_load16(base + (a + 16 <= total ? a : 0u), tile + (size_t)e0 * 16);
+    }
+    // Whether some chunk of the block runs past the buffer end (rows are
+    // clamped, so the last staged row has the highest address).
+    const bool edge =
+        ((((uint32_t)row_of(in, r_hi) * row_bytes + byte0) & ~3u) + (uint32_t)RSB) > total;
+    // ---- Stage the V operands of the block's 16-row tiles.
+    for (int t = t0; t <= t1; ++t) {
+        const unsigned char* vt = job.vtab + (size_t)cload(job.vtile, 3 * t) * job.vtab_bytes;
+        for (int k = wave; k < KV * LIMBS; k += 4)
+            lds_load16(vt + (size_t)k * 1024 + lane * 16, vstage + (size_t)(t - t0) * VSZ + (size_t)k * 1024);
+    }
+    // ---- This wavefront's H operands (host-folded pattern of its strip).
+    const bool have = wave < nst;
+    const int jx = jx0 + (have ? wave : 0);
+    i32x4 whi[KMAX], wlo[KMAX];
+    const int pat = cload(job.hstrip, 2 * jx);
+    const int span0 = cload(job.hstrip, 2 * jx + 1);
+    const unsigned char* ht = job.htab + (size_t)pat * job.htab_bytes;
+#pragma unroll
+    for (int s = 0; s < KMAX; ++s) {
+        if (s < KH) {
+            if constexpr (LIMBS == 2) {
+                whi[s] = gmem<i32x4>(ht + ((size_t)(2 * s) * 64 + lane) * 16)[0];
+                wlo[s] = gmem<i32x4>(ht + ((size_t)(2 * s + 1) * 64 + lane) * 16)[0];
+            } else {
+                wlo[s] = gmem<i32x4>(ht + ((size_t)s * 64 + lane) * 16)[0];
+            }
+        }
+    }
+    const int n = lane & 15, q = lane >> 4;
+    const int hseed = gmem<int>(ht + (size_t)KH * LIMBS * 64 * 16)[n];
+    wait_vmem<0>();
+    if (edge) {
+        // Chunks that cross the buffer end were loaded from its start: write
+        // the buffer's last bytes over them, zero past its end.
+        for (int e = wave * 64 + lane; e < nchunks; e += 256) {
+            const uint32_t a = chunk_addr(e);
+            if (a + 16 <= total) continue;
+            uint32_t d[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                uint32_t v = 0;
+                for (int b = 0; b < 4; ++b) {
+                    const uint32_t off = a + 4 * j + b;
+                    if (off < total) v |= (uint32_t)gmem<unsigned char>(base)[off] << (8 * b);
+                }
+                d[j] = v;
+            }
+            *reinterpret_cast<u32x4*>(tile + (size_t)e * 16) = u32x4{d[0], d[1], d[2], d[3]};
+        }
+    }
+    __syncthreads();
+    if (!have) return;
+
+    // ---- H: intermediate rows r_lo .. r_hi, 16 at a time, into this
+    // wavefront's columns (sample w * 16 + n at inter + (w * 16 + n) * RS).
+    unsigned char* col = inter + (size_t)(wave * 16 + n) * RS;
+    const uint32_t soff = (uint32_t)(span0 - c_first) * B + 16u * q;
+    for (int r0 = r_lo; r0 <= r_hi; r0 += 16) {
+        const int i = min(r0 - r_lo + n, nrows - 1);  // lane (m = n, q): staged row of M row m
+        const uint32_t mis = ((uint32_t)row_of(in, r_lo + i) * row_bytes + byte0) & 3u;
+        const unsigned char* rowp = tile + (size_t)i * RSB;
+        i32x4 dh = {0, 0, 0, 0}, dl = {hseed, hseed, hseed, hseed};
+#pragma unroll
+        for (int s = 0; s < KMAX; ++s) {
+            if (s < KH) {
+                const uint32_t bo = soff + 64u * s + mis;
+                const uint32_t* p = reinterpret_cast<const uint32_t*>(rowp + (bo & ~3u));
+                uint32_t w[4];
+                if constexpr (B == 4) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) w[k] = p[k];
+                } else {
+                    uint32_t d[5];
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) d[k] = p[k];
+                    const int al = (int)(bo & 3u);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) w[k] = __builtin_amdgcn_alignbyte(d[k + 1], d[k], al);
+                }
+                i32x4 av;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    uint32_t v = w[k];
+                    if constexpr ((B == 2 || B == 4) && PM) v = premult_dword<B>(v);
+                    av[k] = (int)(v ^ 0x80808080u);
+                }
+                if constexpr (LIMBS == 2) dh = mfma_i8(av, whi[s], dh);
+                dl = mfma_i8(av, wlo[s], dl);
+            }
+        }
+        uint32_t packed = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int acc = LIMBS == 2 ? 128 * dh[e] + dl[e] : dl[e];
+            packed |= sep_round<FAM>(acc, job.h_mul, job.h_shift, job.h_hi) << (8 * e);
+        }
+        *reinterpret_cast<uint32_t*>(col + (r0 - r_lo + 4 * q)) = packed ^ 0x80808080u;
+    }
+
+    // ---- V: each 16-row output tile from the staged operands.
+    const unsigned char* ncol = inter + (size_t)(wave * 16 + n) * RS;
+    const int s0 = jx * 16;
+    for (int t = t0; t <= t1; ++t) {
+        const int y0 = 16 * t;
+        const int base16 = cload(job.vtile, 3 * t + 1);
+        const unsigned char* vt = job.vtab + (size_t)cload(job.vtile, 3 * t) * job.vtab_bytes;
+        const int* seeds = reinterpret_cast<const int*>(vt + (size_t)KV * LIMBS * 64 * 16);
+        i32x4 vseed;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int s0v = cload(seeds, e), s1v = cload(seeds, 4 + e), s2v = cload(seeds, 8 + e),
+                      s3v = cload(seeds, 12 + e);
+            vseed[e] = q == 0 ? s0v : (q == 1 ? s1v : (q == 2 ? s2v : s3v));
+        }
+        const unsigned char* vs = vstage + (size_t)(t - t0) * VSZ;
+        i32x4 dh = {0, 0, 0, 0}, dl = vseed;
+        for (int s = 0; s < KV; ++s) {
+            const i32x4 bv = *reinterpret_cast<const i32x4*>(ncol + (base16 - r_lo + 64 * s + 16 * q));
+            if constexpr (LIMBS == 2) {
+                dh = mfma_i8(*reinterpret_cast<const i32x4*>(vs + (size_t)(2 * s) * 1024 + lane * 16), bv, dh);
+                dl = mfma_i8(*reinterpret_cast<const i32x4*>(vs + (size_t)(2 * s + 1) * 1024 + lane * 16), bv, dl);
+            } else {
+                dl = mfma_i8(*reinterpret_cast<const i32x4*>(vs + (size_t)s * 1024 + lane * 16), bv, dl);
+            }
+        }
+        uint32_t v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            v[e] = sep_round<FAM>(LIMBS == 2 ? 128 * dh[e] + dl[e] : dl[e], job.v_mul, job.v_shift, job.v_hi);
+        if constexpr (B == 2 || B == 4) {
+            if (job.unpremult) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const uint32_t al = (uint32_t)__shfl((int)v[e], lane | (B - 1), 64);
+                    if ((n & (B - 1)) != B - 1) v[e] = unpremult_u8(v[e], al);
+                }
+            }
+        }
+        const int smp = s0 + n;
+        if (smp < nsamp) {
+            unsigned char* o = static_cast<unsigned char*>(out.ptr) + (size_t)(y0 + 4 * q) * nsamp + smp;
+            const int rows_left = y_end - (y0 + 4 * q);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (e < rows_left) o[(size_t)e * nsamp] = (unsigned char)v[e];
+        }
+    }
+}
+
+template <int B, int LIMBS, int FAM, bool PM>
+__global__ __launch_bounds__(kThreads) void k_sep_tile(SepJob job) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int gxb = (job.gx + 3) >> 2;
+    sep_tile_task<B, LIMBS, FAM, PM>(job, (int)blockIdx.x % gxb, (int)blockIdx.x / gxb, lds);
+}
+
+template <int B, int LIMBS, int FAM, bool PM>
+__global__ __launch_bounds__(kThreads) void k_sep_tile_batch(const SepJob* __restrict__ jobs,
+                                                             const BatchTask* __restrict__ tasks, int ntasks,
+                                                             const void* const* __restrict__ srcs,
+                                                             void* const* __restrict__ dsts) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const BatchTask task = tasks[t];
+        SepJob job = jobs[task.job];
+        if (job.src_slot >= 0) job.in.ptr = srcs[job.src_slot];
+        if (job.dst_slot >= 0) job.out.ptr = dsts[job.dst_slot];
+        sep_tile_task<B, LIMBS, FAM, PM>(job, task.bx, task.by, lds);
+        __syncthreads();  // the next task's staging overwrites this one's LDS
+    }
+}
+
 }  // namespace
 
 size_t sep_job_wave_lds(const SepJob& job) {
@@ -572,6 +803,10 @@ bool sep_job_supported(const SepJob& job) {
         return false;
     const int kmax = job.limbs == 1 ? 6 : 3;
     if (job.kh < 1 || job.kh > kmax || job.kv < 1 || job.kv > 7) return false;
+    if (job.tile)
+        return job.rows_per_task % 16 == 0 && job.rows_per_task > 0 && job.t_row_stride % 16 == 0 &&
+               job.t_row_stride > 0 && job.t_inter_stride % 16 == 0 && job.t_lds > 0 && job.t_lds <= 64 * 1024 &&
+               job.htab && job.hstrip;
     if (job.family == 0 && job.kh > 2) return false;
     return job.ring >= 64 * job.kv + 16 && (job.ring & (job.ring - 1)) == 0 && job.rows_per_task % 16 == 0;
 }
@@ -586,6 +821,17 @@ bool sep_job_supported(const SepJob& job) {
 hipError_t launch_sep_job(const SepJob& job, hipStream_t s) {
     if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;
     if (!sep_job_supported(job)) return hipErrorInvalidValue;
+    if (job.tile) {
+        const int blocks = ((job.gx + 3) / 4) * job.gy;
+        const int var = sep_job_variant(job);
+        int idx = 0;
+#define X(B, L, F, P)                                                                                   \
+    if (idx++ == var)                                                                                   \
+        hipLaunchKernelGGL((k_sep_tile<B, L, F, P>), dim3((unsigned)blocks), dim3(kThreads), (size_t)job.t_lds, s, job);
+        VGPU_SEP_ALL(X)
+#undef X
+        return hipGetLastError();
+    }
     const int wl = (int)sep_job_wave_lds(job);
     const int ntasks = job.gx * job.gy;
     const dim3 grid((unsigned)((ntasks + kSepWaves - 1) / kSepWaves));
@@ -611,6 +857,20 @@ hipError_t launch_sep_batch(int variant, const SepJob* jobs, const BatchTask* ta
     if (idx++ == variant)                                                                               \
         hipLaunchKernelGGL((k_sep_batch<B, L, F, P>), dim3((unsigned)grid), dim3(kThreads), lds, s, jobs, tasks, \
                            ntasks, srcs, dsts, wl);
+    VGPU_SEP_ALL(X)
+#undef X
+    return hipGetLastError();
+}
+hipError_t launch_sep_tile_batch(int variant, const SepJob* jobs, const BatchTask* tasks, int ntasks,
+                                 const void* const* srcs, void* const* dsts, int grid, size_t lds,
+                                 hipStream_t s) {
+    if (ntasks <= 0) return hipSuccess;
+    if (variant < 0 || variant >= 64) return hipErrorInvalidValue;
+    int idx = 0;
+#define X(B, L, F, P)                                                                                   \
+    if (idx++ == variant)                                                                               \
+        hipLaunchKernelGGL((k_sep_tile_batch<B, L, F, P>), dim3((unsigned)grid), dim3(kThreads), lds, s, jobs, tasks, \
+                           ntasks, srcs, dsts);
     VGPU_SEP_ALL(X)
 #undef X
     return hipGetLastError();
