@@ -208,7 +208,7 @@ def bench_batch(v, O, ctx, n, steps, warmup, check_images):
                      "achieved": round(dom[1] / (dom[0] / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(dom[1] / (dom[0] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
         "tuned": {"grid_cap": info["grid_cap"], "reduce_grid_cap": info["reduce_grid_cap"],
-                  "reduce_first_tile": info["reduce_first_tile"], "last_share": info["last_share"],
+                  "reduce_first_tile": info["reduce_first_tile"], "reduce_chain": info["reduce_chain"],
                   "use_graph": info["use_graph"], "ms": round(tuned_ms, 4)},
         "tasks": {"shrink": info["shrink_tasks"], "reduce": info["reduce_tasks"],
                   "fallback_images": info["fallback_images"]},

@@ -230,9 +230,9 @@ typedef enum {
     VGPU_OPT_SEP_ROWS_PER_TASK = 2, /* output rows per MFMA wavefront task: 0 = auto, else a multiple of 16 */
     VGPU_OPT_SHRINK_RUN = 3,        /* box-shrink output rows per wavefront run: 0 = auto, 1..64 */
     VGPU_OPT_GAUSS_ROWS = 4,        /* u8 3/4-band Gaussian VALU tile rows: 0 = auto, 16 or 32; -1 = the MFMA kernel */
-    VGPU_OPT_BATCH_LAST_SHARE = 5,  /* batch only: percent (0..90) of the box-shrink bytes, largest images first,
-                                       given to the last launch chain so the other chains' reduce tails run
-                                       under its shrinks; 0 = chains balanced by bytes */
+    VGPU_OPT_BATCH_REDUCE_CHAIN = 5, /* batch only: 1 (default) = reduce stages on a stream of their own, each
+                                        launch after the shrink launch that wrote its inputs; 0 = every image's
+                                        stages on its own group's stream */
     VGPU_OPT_SEP_FORM = 6           /* int8 MFMA pairs: 0 = block tiles staged in LDS where they fit (default),
                                        1 = one wavefront per strip task streaming its operands */
 } vgpu_option;
@@ -412,7 +412,7 @@ typedef struct {
     uint64_t input_bytes;
     uint64_t output_bytes;
     uint64_t intermediate_bytes;
-    int32_t last_share;        /* VGPU_OPT_BATCH_LAST_SHARE in effect */
+    int32_t reduce_chain;      /* VGPU_OPT_BATCH_REDUCE_CHAIN in effect */
 } vgpu_batch_stats;
 
 int vgpu_batch_thumbnail_new(vgpu_context *ctx, uint32_t count, const uint32_t *widths,
@@ -450,7 +450,7 @@ int vgpu_batch_run_profiled(vgpu_batch *b, const void *const *srcs, const size_t
 int vgpu_batch_benchmark(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,
                          void *const *dsts, const size_t *dst_bytes, int warmup, int iterations,
                          float *total_ms, float *step_ms);
-/* Autotuner: times every (reduce tile, grid sizes, last-chain share)
+/* Autotuner: times every (reduce tile, grid sizes, reduce chain on / off)
  * candidate on these buffers and keeps the fastest; best_ms (may be NULL)
  * receives its ms per run. */
 int vgpu_batch_tune(vgpu_batch *b, const void *const *srcs, const size_t *src_bytes,

@@ -83,16 +83,19 @@ def test_batch_launch_shapes_and_replays(ctx):
             check_outputs(dst, exp, f"grid {grid_cap}/{rgrid} graph {graph} tile {tile} replay {rep}")
 
 
-@pytest.mark.parametrize("share", [25, 45, 90])
-def test_batch_staggered_chains_keep_results(ctx, share):
-    # The last launch chain takes the largest images up to `share` percent
-    # of the box-shrink bytes; the grouping moves work between streams only.
-    sizes = random_sizes(20, 321, 200, 2600) + [(64, 48, 3), (150, 150, 4)]
+@pytest.mark.parametrize("reduce_chain", [1, 0])
+@pytest.mark.parametrize("form", [0, 1])
+def test_batch_chain_layouts_keep_results(ctx, reduce_chain, form):
+    # Reduce stages on a chain of their own (each launch after the shrink
+    # launch that wrote its inputs) or on their images' chains, on either
+    # form of the MFMA pair: the layout moves work between streams only.
+    sizes = random_sizes(20, 321, 200, 2600) + [(64, 48, 3), (150, 150, 4), (300, 2000, 2)]
     srcs = sources(sizes, 17)
     exp = expected(srcs, v.FITBOX, 160, 160)
     batch = v.Batch(ctx, sizes, v.FITBOX, 160, 160)
-    batch.set_option(v.api.OPT_BATCH_LAST_SHARE, share)
-    assert batch.info()["last_share"] == share
+    batch.set_option(v.api.OPT_BATCH_REDUCE_CHAIN, reduce_chain)
+    batch.set_option(v.api.OPT_SEP_FORM, form)
+    assert batch.info()["reduce_chain"] == reduce_chain
     dsrc = [v.DeviceBuffer.from_numpy(ctx, s) for s in srcs]
     dst = batch.output_buffers()
     for rep in range(3):
@@ -100,9 +103,9 @@ def test_batch_staggered_chains_keep_results(ctx, share):
             d.upload(np.zeros(d.shape, np.uint8))
         batch.run(dsrc, dst)
         ctx.synchronize()
-        check_outputs(dst, exp, f"last share {share} replay {rep}")
+        check_outputs(dst, exp, f"reduce chain {reduce_chain} form {form} replay {rep}")
     with pytest.raises(v.VgpuError):
-        batch.set_option(v.api.OPT_BATCH_LAST_SHARE, 91)
+        batch.set_option(v.api.OPT_BATCH_REDUCE_CHAIN, 2)
 
 
 def test_batch_outputs_equal_single_pipelines(ctx):

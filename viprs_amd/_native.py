@@ -33,7 +33,7 @@ class BatchStats(C.Structure):
                 ("reduce_tasks", C.c_uint32), ("fallback_images", C.c_uint32), ("grid_cap", C.c_int32),
                 ("use_graph", C.c_int32), ("reduce_first_tile", C.c_int32), ("reduce_grid_cap", C.c_int32),
                 ("input_bytes", C.c_uint64),
-                ("output_bytes", C.c_uint64), ("intermediate_bytes", C.c_uint64), ("last_share", C.c_int32)]
+                ("output_bytes", C.c_uint64), ("intermediate_bytes", C.c_uint64), ("reduce_chain", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

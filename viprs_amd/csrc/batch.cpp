@@ -19,9 +19,11 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <new>
 #include <numeric>
+#include <tuple>
 #include <vector>
 
 #include "runtime.hpp"
@@ -48,13 +50,12 @@ struct Launch {
     size_t lds = 0;
     uint64_t alg_bytes = 0;  // input buffers read + outputs written, summed over jobs
     int chain = -1;          // the group of its images (one stream each), or -1 for mixed
+    int wait_for = -1;       // launch (on another chain) whose outputs it reads, or -1
+    bool signal = false;     // a later launch waits for this one
 };
 
-// Images are dealt, largest first, to the group with the fewest bytes so far;
-// each group is a chain of launches (one per kernel variant and level) on its
-// own stream (group 0 on the caller's).  Four groups, as many as the
-// device's hardware queues: chains of equal size end their box shrinks
-// together and run their reduce tails side by side.
+// Four launch chains, as many as the device's hardware queues, each on its
+// own stream (chain 0 on the caller's).
 constexpr int kGroups = 4;
 constexpr int kChains = kGroups;
 
@@ -79,7 +80,7 @@ struct vgpu_batch {
     std::vector<Launch> launches;
     std::vector<int> fallback;  // images run stage by stage
     int n_shrink_tasks = 0, n_reduce_tasks = 0;
-    int last_share = 0;       // percent of the shrink bytes the last chain takes (largest images); 0 = balanced
+    int reduce_chain = 1;     // 1: reduce stages on their own chain after their producers; 0: on the image's chain
     int grid_cap = 0;         // persistent grid size of the shrink launches; 0 = one block per task
     int reduce_grid_cap = 0;  // the same for the reduce and MFMA separable launches
     int use_graph = 1;
@@ -89,6 +90,7 @@ struct vgpu_batch {
     // the context stream.
     hipStream_t side[kChains] = {};
     hipEvent_t fork_ev = nullptr, join_ev[kChains] = {};
+    std::vector<hipEvent_t> done_ev;  // per launch that a launch on another chain waits for
     unsigned chains = 0;  // bit c: some launch runs on chain c
 
     ~vgpu_batch() {
@@ -118,6 +120,9 @@ struct vgpu_batch {
         slots_valid = false;
         launches.clear();
         fallback.clear();
+        for (hipEvent_t e : done_ev)
+            if (e) (void)hipEventDestroy(e);
+        done_ev.clear();
     }
     int plan();
     int launch_all(hipStream_t s, hipEvent_t* marks = nullptr);
@@ -210,70 +215,53 @@ int vgpu_batch::plan() {
                (uint64_t)images[(size_t)b].w * images[(size_t)b].h * images[(size_t)b].bands;
     });
 
-    // Groups.  Balanced: images dealt largest first to the group that moves
-    // the fewest bytes so far.  Staggered (last_share > 0): the largest
-    // images go to the last group until it holds last_share percent of the
-    // box-shrink bytes, the rest are balanced over the other groups with the
-    // reduce stages weighted by their lower rate, so those groups finish
-    // their reduce tails while the last group still streams its shrinks and
-    // only the last group's (few images') reduce work trails.
+    // Shrink groups: images dealt, largest first, to the group that moves
+    // the fewest bytes so far; each group's box shrinks form a chain of
+    // launches on its own stream.  With a reduce chain (the default) the
+    // reduce stages of every level after the first go to the last stream
+    // instead, one launch per (kernel variant, producing launch), each
+    // waiting only for the shrink launch that wrote its inputs: the shrink
+    // chains never wait for a reduce, and the reduce work of all but the last
+    // shrink launches runs under the remaining shrinks.
+    const int shrink_groups = reduce_chain ? kGroups - 1 : kGroups;
+    const int rchain = reduce_chain ? kGroups - 1 : -1;
     std::vector<int> group_of(count, 0);
     {
-        std::vector<uint64_t> shrink_b(count, 0), other_b(count, 0);
-        uint64_t shrink_total = 0;
-        for (size_t i = 0; i < count; ++i) {
-            const BatchImage& im = images[i];
+        uint64_t load[kGroups] = {};
+        for (int idx : order) {
+            const BatchImage& im = images[(size_t)idx];
+            uint64_t bytes = 0;
             for (size_t k = 0; k < im.stages.size(); ++k) {
                 const Rect in_rect = k == 0 ? whole(im.w, im.h) : im.stages[k - 1].out;
-                const uint64_t b =
-                    ((uint64_t)in_rect.w * in_rect.h + (uint64_t)im.stages[k].out.w * im.stages[k].out.h) * im.bands;
-                (im.stages[k].kind == ST_SHRINK_HV_U8 ? shrink_b[i] : other_b[i]) += b;
+                bytes += ((uint64_t)in_rect.w * in_rect.h + (uint64_t)im.stages[k].out.w * im.stages[k].out.h) *
+                         im.bands;
             }
-            shrink_total += shrink_b[i];
-        }
-        // The MFMA reduce tails move their bytes about ten times slower than
-        // the box shrinks stream theirs.
-        constexpr uint64_t kReduceWeight = 10;
-        const int spread = last_share > 0 ? kGroups - 1 : kGroups;
-        uint64_t load[kGroups] = {};
-        uint64_t last_bytes = 0;
-        for (int idx : order) {
-            const size_t i = (size_t)idx;
-            if (last_share > 0 && last_bytes * 100 < shrink_total * (uint64_t)last_share) {
-                group_of[i] = kGroups - 1;
-                last_bytes += shrink_b[i];
-                continue;
-            }
-            const uint64_t cost = last_share > 0 ? shrink_b[i] + kReduceWeight * other_b[i] : shrink_b[i] + other_b[i];
             int g = 0;
-            for (int c = 1; c < spread; ++c)
+            for (int c = 1; c < shrink_groups; ++c)
                 if (load[c] < load[g]) g = c;
-            group_of[i] = g;
-            load[g] += cost;
+            group_of[(size_t)idx] = g;
+            load[g] += bytes;
         }
     }
+    // The launch that ran each image's previous level (-1: none yet).
+    std::vector<int> producer(count, -1);
     Staging tbl;
     n_shrink_tasks = n_reduce_tasks = 0;
-    for (int lv = 0; lv < levels; ++lv) {
-        constexpr int kShrinkVariants = 12;  // shrink_job_variant
-        constexpr int kS = kShrinkVariants * kGroups;  // (variant, group)
-        std::vector<ShrinkJob> sjobs[kS];
-        std::vector<BatchTask> stasks[kS];
-        uint64_t sbytes[kS] = {};
+    // Jobs of one future launch: kernel family and variant, its chain, and
+    // the launch it waits for (-1: none).
+    struct Bucket {
+        std::vector<ShrinkJob> sjobs;
         std::vector<ReduceJob> rjobs;
-        std::vector<BatchTask> rtasks;
-        size_t rlds = 0;
-        uint64_t rbytes = 0;
-        constexpr int kSepVariants = 32;  // sep_job_variant
-        constexpr int kP = kSepVariants * kGroups;
-        std::vector<SepJob> pjobs[kP];
-        std::vector<BatchTask> ptasks[kP];
-        size_t plds[kP] = {};
-        uint64_t pbytes[kP] = {};
-        std::vector<SepJob> tjobs[kP];
-        std::vector<BatchTask> ttasks[kP];
-        size_t tlds[kP] = {};
-        uint64_t tbytes[kP] = {};
+        std::vector<SepJob> pjobs;
+        std::vector<BatchTask> tasks;
+        std::vector<int> images;
+        size_t lds = 0;
+        uint64_t bytes = 0;
+    };
+    for (int lv = 0; lv < levels; ++lv) {
+        // key: (order, family, variant, chain, wait_for); std::map keeps the
+        // launches of a level in producer order, then family and variant.
+        std::map<std::tuple<int, int, int, int, int>, Bucket> buckets;
         for (size_t rank = 0; rank < order.size(); ++rank) {
             const int idx = order[rank];
             const int grp = group_of[(size_t)idx];
@@ -287,6 +275,12 @@ int vgpu_batch::plan() {
             const InView in = view_of(in_ptr, in_rect, s.clip);
             const OutView out = outview_of(out_ptr, s.out);
             const uint64_t io_bytes = ((uint64_t)in_rect.w * in_rect.h + (uint64_t)s.out.w * s.out.h) * im.bands;
+            // Reduce stages go to the reduce chain (after the producing
+            // launch) or stay on the image's group.
+            const int wait = producer[(size_t)idx];
+            const int rkey_chain = rchain >= 0 ? rchain : grp;
+            const int rkey_wait = rchain >= 0 ? wait : -1;
+            const int rkey_order = rchain >= 0 ? wait + 1 : 0;
             if (s.kind == ST_SHRINK_HV_U8) {
                 ShrinkJob j = shrink_job_of(s, im.nodes[(size_t)s.first], in, out);
                 j.src_slot = first ? idx : -1;
@@ -295,102 +289,72 @@ int vgpu_batch::plan() {
                     im.batched = false;
                     continue;
                 }
-                const int var = shrink_job_variant(j) * kGroups + grp;
-                const int jid = (int)sjobs[var].size();
-                sjobs[var].push_back(j);
-                sbytes[var] += io_bytes;
+                Bucket& bk = buckets[std::make_tuple(0, 0, shrink_job_variant(j), grp, -1)];
+                const int jid = (int)bk.sjobs.size();
+                bk.sjobs.push_back(j);
+                bk.bytes += io_bytes;
+                bk.lds = shrink_job_lds();
+                bk.images.push_back(idx);
                 for (int by = 0; by < j.gy; ++by)
-                    for (int bx = 0; bx < j.gx; ++bx) stasks[var].push_back(BatchTask{jid, bx, by});
-            } else if (s.kind == ST_SEP && s.sep.tile) {
-                // Block-tile form: one task per block (four strips x R rows).
-                SepJob j = sep_job_of(s, in, out, dblob);
-                j.src_slot = first ? idx : -1;
-                j.dst_slot = last ? idx : -1;
-                const int var = sep_job_variant(j) * kGroups + grp;
-                const int jid = (int)tjobs[var].size();
-                tjobs[var].push_back(j);
-                tbytes[var] += io_bytes;
-                tlds[var] = std::max(tlds[var], (size_t)j.t_lds);
-                const int gxb = (j.gx + 3) / 4;
-                for (int by = 0; by < j.gy; ++by)
-                    for (int bx = 0; bx < gxb; ++bx) ttasks[var].push_back(BatchTask{jid, bx, by});
+                    for (int bx = 0; bx < j.gx; ++bx) bk.tasks.push_back(BatchTask{jid, bx, by});
             } else if (s.kind == ST_SEP) {
                 SepJob j = sep_job_of(s, in, out, dblob);
                 j.src_slot = first ? idx : -1;
                 j.dst_slot = last ? idx : -1;
-                const int var = sep_job_variant(j) * kGroups + grp;
-                const int jid = (int)pjobs[var].size();
-                pjobs[var].push_back(j);
-                pbytes[var] += io_bytes;
-                plds[var] = std::max(plds[var], sep_job_wave_lds(j));
-                for (int ty = 0; ty < j.gy; ++ty)
-                    for (int jx = 0; jx < j.gx; ++jx) ptasks[var].push_back(BatchTask{jid, jx, ty});
+                // Family 3: the block-tile form (one task per block of four
+                // strips x R rows); family 2: one task per wavefront.
+                const int fam = j.tile ? 3 : 2;
+                Bucket& bk = buckets[std::make_tuple(rkey_order, fam, sep_job_variant(j), rkey_chain, rkey_wait)];
+                const int jid = (int)bk.pjobs.size();
+                bk.pjobs.push_back(j);
+                bk.bytes += io_bytes;
+                bk.images.push_back(idx);
+                if (j.tile) {
+                    bk.lds = std::max(bk.lds, (size_t)j.t_lds);  // per block
+                    const int gxb = (j.gx + 3) / 4;
+                    for (int by = 0; by < j.gy; ++by)
+                        for (int bx = 0; bx < gxb; ++bx) bk.tasks.push_back(BatchTask{jid, bx, by});
+                } else {
+                    bk.lds = std::max(bk.lds, sep_job_wave_lds(j));  // per wavefront
+                    for (int ty = 0; ty < j.gy; ++ty)
+                        for (int jx = 0; jx < j.gx; ++jx) bk.tasks.push_back(BatchTask{jid, jx, ty});
+                }
             } else {
                 ReduceJob j = reduce_job_of(s, in, out, dblob);
                 j.src_slot = first ? idx : -1;
                 j.dst_slot = last ? idx : -1;
-                const int jid = (int)rjobs.size();
-                rjobs.push_back(j);
-                rbytes += io_bytes;
-                rlds = std::max(rlds, reduce_job_lds(j));
+                // The VALU reduce takes every band count in one launch, after
+                // every chain (chain -1 joins them).
+                Bucket& bk = buckets[std::make_tuple(1 << 30, 1, 0, -1, -1)];
+                const int jid = (int)bk.rjobs.size();
+                bk.rjobs.push_back(j);
+                bk.bytes += io_bytes;
+                bk.lds = std::max(bk.lds, reduce_job_lds(j));
+                bk.images.push_back(idx);
                 for (int by = 0; by < j.gy; ++by)
-                    for (int bx = 0; bx < j.gx; ++bx) rtasks.push_back(BatchTask{jid, bx, by});
+                    for (int bx = 0; bx < j.gx; ++bx) bk.tasks.push_back(BatchTask{jid, bx, by});
             }
         }
-        for (int vg = 0; vg < kS; ++vg) {
-            if (stasks[vg].empty()) continue;
-            const int var = vg / kGroups;
+        for (auto& kv : buckets) {
+            Bucket& bk = kv.second;
+            if (bk.tasks.empty()) continue;
             Launch l;
-            l.family = 0;
-            l.variant = var;
-            l.chain = vg % kGroups;
-            l.jobs_off = tbl.put(sjobs[vg].data(), sjobs[vg].size() * sizeof(ShrinkJob));
-            l.tasks_off = tbl.put(stasks[vg].data(), stasks[vg].size() * sizeof(BatchTask));
-            l.ntasks = (int)stasks[vg].size();
-            l.lds = shrink_job_lds();
-            l.alg_bytes = sbytes[vg];
+            l.family = std::get<1>(kv.first);
+            l.variant = std::get<2>(kv.first);
+            l.chain = std::get<3>(kv.first);
+            l.wait_for = std::get<4>(kv.first);
+            if (l.family == 0) l.jobs_off = tbl.put(bk.sjobs.data(), bk.sjobs.size() * sizeof(ShrinkJob));
+            else if (l.family == 1) l.jobs_off = tbl.put(bk.rjobs.data(), bk.rjobs.size() * sizeof(ReduceJob));
+            else l.jobs_off = tbl.put(bk.pjobs.data(), bk.pjobs.size() * sizeof(SepJob));
+            l.tasks_off = tbl.put(bk.tasks.data(), bk.tasks.size() * sizeof(BatchTask));
+            l.ntasks = (int)bk.tasks.size();
+            l.lds = bk.lds;
+            l.alg_bytes = bk.bytes;
+            const int li = (int)launches.size();
+            for (int idx : bk.images) producer[(size_t)idx] = li;
+            if (l.wait_for >= 0) launches[(size_t)l.wait_for].signal = true;
             launches.push_back(l);
-            n_shrink_tasks += l.ntasks;
-        }
-        for (int vg = 0; vg < kP; ++vg) {
-            if (ptasks[vg].empty()) continue;
-            const int var = vg / kGroups;
-            Launch l;
-            l.family = 2;
-            l.variant = var;
-            l.chain = vg % kGroups;
-            l.jobs_off = tbl.put(pjobs[vg].data(), pjobs[vg].size() * sizeof(SepJob));
-            l.tasks_off = tbl.put(ptasks[vg].data(), ptasks[vg].size() * sizeof(BatchTask));
-            l.ntasks = (int)ptasks[vg].size();
-            l.lds = plds[vg];  // per wavefront
-            l.alg_bytes = pbytes[vg];
-            launches.push_back(l);
-            n_reduce_tasks += l.ntasks;
-        }
-        for (int vg = 0; vg < kP; ++vg) {
-            if (ttasks[vg].empty()) continue;
-            Launch l;
-            l.family = 3;
-            l.variant = vg / kGroups;
-            l.chain = vg % kGroups;
-            l.jobs_off = tbl.put(tjobs[vg].data(), tjobs[vg].size() * sizeof(SepJob));
-            l.tasks_off = tbl.put(ttasks[vg].data(), ttasks[vg].size() * sizeof(BatchTask));
-            l.ntasks = (int)ttasks[vg].size();
-            l.lds = tlds[vg];  // per block
-            l.alg_bytes = tbytes[vg];
-            launches.push_back(l);
-            n_reduce_tasks += l.ntasks;
-        }
-        if (!rtasks.empty()) {
-            Launch l;
-            l.family = 1;
-            l.jobs_off = tbl.put(rjobs.data(), rjobs.size() * sizeof(ReduceJob));
-            l.tasks_off = tbl.put(rtasks.data(), rtasks.size() * sizeof(BatchTask));
-            l.ntasks = (int)rtasks.size();
-            l.lds = rlds;
-            l.alg_bytes = rbytes;
-            launches.push_back(l);
-            n_reduce_tasks += l.ntasks;
+            (l.family == 0 ? n_shrink_tasks : n_reduce_tasks) += l.ntasks;
         }
     }
     chains = 0;
@@ -404,6 +368,12 @@ int vgpu_batch::plan() {
             return VGPU_E_DEVICE;
     }
     if (!fork_ev && hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming) != hipSuccess) return VGPU_E_DEVICE;
+    done_ev.assign(launches.size(), nullptr);
+    for (size_t li = 0; li < launches.size(); ++li)
+        if (launches[li].signal && hipEventCreateWithFlags(&done_ev[li], hipEventDisableTiming) != hipSuccess) {
+            done_ev[li] = nullptr;
+            return VGPU_E_DEVICE;
+        }
     // An image whose shrink job could not be planned runs stage by stage
     // after the batched levels (any earlier-level job it already has only
     // writes its own intermediate, which the stage-by-stage run rewrites).
@@ -455,11 +425,14 @@ int vgpu_batch::launch_all(hipStream_t s, hipEvent_t* marks) {
         if (par) {
             if (l.chain >= 0) {
                 ls = l.chain == 0 ? s : side[l.chain];  // chain 0 on the caller's stream
+                if (l.wait_for >= 0 && launches[(size_t)l.wait_for].chain != l.chain &&
+                    hipStreamWaitEvent(ls, done_ev[(size_t)l.wait_for], 0) != hipSuccess)
+                    return VGPU_E_DEVICE;
             } else if (!join()) {  // a launch over mixed band counts waits for every chain
                 return VGPU_E_DEVICE;
             }
         }
-        // A block takes one task, or (MFMA family) one task per wavefront.
+        // A block takes one task, or (MFMA wavefront family) one task per wavefront.
         const int blocks = l.family == 2 ? (l.ntasks + 3) / 4 : l.ntasks;
         const int cap = l.family == 0 ? grid_cap : reduce_grid_cap;
         const int grid = cap > 0 ? std::min(blocks, cap) : blocks;
@@ -478,6 +451,7 @@ int vgpu_batch::launch_all(hipStream_t s, hipEvent_t* marks) {
             e = launch_reduce_batch(reinterpret_cast<const ReduceJob*>(t + l.jobs_off), tasks, l.ntasks, srcs, dsts,
                                     grid, l.lds, ls);
         if (e != hipSuccess) return VGPU_E_DEVICE;
+        if (par && l.signal && l.chain >= 0 && hipEventRecord(done_ev[li], ls) != hipSuccess) return VGPU_E_DEVICE;
         if (par && l.chain < 0 && !fork()) return VGPU_E_DEVICE;
         if (marks && hipEventRecord(marks[li + 1], s) != hipSuccess) return VGPU_E_DEVICE;
     }
@@ -583,7 +557,7 @@ int vgpu_batch_info(const vgpu_batch* b, vgpu_batch_stats* st) {
     st->reduce_grid_cap = b->reduce_grid_cap;
     st->use_graph = b->use_graph;
     st->reduce_first_tile = b->opt.reduce_first_tile;
-    st->last_share = b->last_share;
+    st->reduce_chain = b->reduce_chain;
     return VGPU_OK;
 }
 
@@ -619,9 +593,9 @@ int vgpu_batch_set_option(vgpu_batch* b, int option, int value) {
         if (value != -1 && value != 0 && value != 16 && value != 32) return VGPU_E_INVALID_ARGUMENT;
         b->opt.gauss_rows = value;
         break;
-    case VGPU_OPT_BATCH_LAST_SHARE:
-        if (value < 0 || value > 90) return VGPU_E_INVALID_ARGUMENT;
-        b->last_share = value;
+    case VGPU_OPT_BATCH_REDUCE_CHAIN:
+        if (value != 0 && value != 1) return VGPU_E_INVALID_ARGUMENT;
+        b->reduce_chain = value;
         break;
     case VGPU_OPT_SEP_FORM:
         if (value != 0 && value != 1) return VGPU_E_INVALID_ARGUMENT;
@@ -758,23 +732,23 @@ int vgpu_batch_benchmark(vgpu_batch* b, const void* const* srcs, const size_t* s
 int vgpu_batch_tune(vgpu_batch* b, const void* const* srcs, const size_t* src_bytes, void* const* dsts,
                     const size_t* dst_bytes, int iterations, float* best_ms) {
     if (!b || iterations < 1) return VGPU_E_INVALID_ARGUMENT;
-    // Candidates: last-chain share x reduce tile start x persistent grid
+    // Candidates: reduce chain on / off x reduce tile start x persistent grid
     // size of the shrink launches x that of the reduce launches (one block
     // per task, or 4/8/16 blocks per CU walking the list), all launched from
     // a graph.  The VALU reduce tile only matters when the batch has VALU
     // reduce launches.
-    static const int shares[] = {0, 25, 35, 45};
+    static const int shares[] = {1, 0};  // reduce chain on / off
     static const int tiles[] = {0, 1, 2, 3};
     static const int sgrids[] = {0, 256 * 8, 256 * 16};
     static const int rgrids[] = {0, 256 * 4, 256 * 8, 256 * 16};
     float best = 0.f;
     int best_tile = b->opt.reduce_first_tile, best_grid = b->grid_cap, best_rgrid = b->reduce_grid_cap;
-    int best_share = b->last_share;
+    int best_share = b->reduce_chain;
     bool have = false;
     for (int sh : shares) {
-        if (sh != b->last_share) {
+        if (sh != b->reduce_chain) {
             (void)hipStreamSynchronize(b->ctx->stream);
-            b->last_share = sh;
+            b->reduce_chain = sh;
             b->drop_graph();
             const int rc = b->plan();
             if (rc != VGPU_OK) return rc;
@@ -802,9 +776,9 @@ int vgpu_batch_tune(vgpu_batch* b, const void* const* srcs, const size_t* src_by
             }
         }
     }
-    if (best_share != b->last_share) {
+    if (best_share != b->reduce_chain) {
         (void)hipStreamSynchronize(b->ctx->stream);
-        b->last_share = best_share;
+        b->reduce_chain = best_share;
         b->drop_graph();
         const int rc = b->plan();
         if (rc != VGPU_OK) return rc;

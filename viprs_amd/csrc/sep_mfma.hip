@@ -557,17 +557,19 @@ __global__ __launch_bounds__(kThreads) void k_sep_batch(const SepJob* __restrict
 // rows (R = rows_per_task).  Everything the block reads -- the input rows its
 // V windows need (intermediate rows r_lo .. r_hi, each row of input
 // row_of(r)) across the four strips' H spans, and the V operand patterns of
-// its 16-row tiles -- goes into LDS first by LDS-direct 16-byte loads, all in
-// flight at once, so the block waits one memory latency.  Each wavefront
-// then runs its strip from LDS: the H M tiles into its own intermediate
-// columns (16 samples x the block's rows, column-major, already offset for
-// the V operand), then the V tiles.  No wavefront reads another's columns,
-// so the only block barrier is the one after the loads.
+// its 16-row tiles -- goes into LDS first by LDS-direct loads, all in flight
+// at once, so the block waits one memory latency.  Each wavefront then runs
+// its strip from LDS: the H M tiles into its own intermediate columns (16
+// samples x the block's rows, column-major, already offset for the V
+// operand), then the V tiles; both two tiles per step so that one tile's LDS
+// reads and MFMAs overlap the other's.  No wavefront reads another's
+// columns, so the only block barrier is the one after the loads.
 //
 // Staged row i (intermediate row r_lo + i) holds the bytes of input row
 // row_of(r_lo + i) from the dword below the tile's first span byte, so the
 // row's byte j sits at offset mis(i) + j, mis(i) = (row address + tile byte
-// 0) & 3.  Chunks past the end of the buffer are read bytewise, zero beyond.
+// 0) & 3.  Chunks past the end of the buffer are read bytewise, zero beyond
+// it.
 template <int B, int LIMBS, int FAM, bool PM>
 __device__ void sep_tile_task(const SepJob& job, int bx, int by, unsigned char* lds) {
     constexpr int KMAX = max_steps<LIMBS>();
@@ -591,7 +593,6 @@ __device__ void sep_tile_task(const SepJob& job, int bx, int by, unsigned char* 
     const uint32_t total = row_bytes * (uint32_t)in.bh;  // < 2^31 (checked on the host)
     const uint32_t byte0 = (uint32_t)c_first * B;
     const int RSB = job.t_row_stride, RS = job.t_inter_stride;
-    const int nchunk_row = RSB >> 4;
     const int VSZ = KV * LIMBS * 1024;
     unsigned char* tile = lds;
     unsigned char* inter = lds + (size_t)job.t_rows * RSB;
@@ -599,6 +600,7 @@ __device__ void sep_tile_task(const SepJob& job, int bx, int by, unsigned char* 
 
     // ---- Stage the input rows: chunk e = (row e / nchunk_row, 16-byte
     // column e % nchunk_row); a wavefront's 64 chunks land contiguously.
+    const int nchunk_row = RSB >> 4;
     const int nchunks = nrows * nchunk_row;
     auto chunk_addr = [&](int e) -> uint32_t {
         const int i = (int)__umulhi((uint32_t)e, job.t_chunk_magic);
@@ -613,8 +615,7 @@ __device__ void sep_tile_task(const SepJob& job, int bx, int by, unsigned char* 
     }
     // Whether some chunk of the block runs past the buffer end (rows are
     // clamped, so the last staged row has the highest address).
-    const bool edge =
-        ((((uint32_t)row_of(in, r_hi) * row_bytes + byte0) & ~3u) + (uint32_t)RSB) > total;
+    const bool edge = ((((uint32_t)row_of(in, r_hi) * row_bytes + byte0) & ~3u) + (uint32_t)RSB) > total;
     // ---- Stage the V operands of the block's 16-row tiles.
     for (int t = t0; t <= t1; ++t) {
         const unsigned char* vt = job.vtab + (size_t)cload(job.vtile, 3 * t) * job.vtab_bytes;
@@ -664,71 +665,85 @@ __device__ void sep_tile_task(const SepJob& job, int bx, int by, unsigned char* 
     __syncthreads();
     if (!have) return;
 
-    // ---- H: intermediate rows r_lo .. r_hi, 16 at a time, into this
-    // wavefront's columns (sample w * 16 + n at inter + (w * 16 + n) * RS).
+    // ---- H: intermediate rows r_lo .. r_hi, 32 at a time (two M tiles),
+    // into this wavefront's columns (sample w * 16 + n at
+    // inter + (w * 16 + n) * RS).
     unsigned char* col = inter + (size_t)(wave * 16 + n) * RS;
     const uint32_t soff = (uint32_t)(span0 - c_first) * B + 16u * q;
-    for (int r0 = r_lo; r0 <= r_hi; r0 += 16) {
+    auto a_operand = [&](int r0, int s) -> i32x4 {
         const int i = min(r0 - r_lo + n, nrows - 1);  // lane (m = n, q): staged row of M row m
         const uint32_t mis = ((uint32_t)row_of(in, r_lo + i) * row_bytes + byte0) & 3u;
-        const unsigned char* rowp = tile + (size_t)i * RSB;
-        i32x4 dh = {0, 0, 0, 0}, dl = {hseed, hseed, hseed, hseed};
+        const uint32_t bo = soff + 64u * s + mis;
+        const uint32_t* p = reinterpret_cast<const uint32_t*>(tile + (size_t)i * RSB + (bo & ~3u));
+        uint32_t w[4];
+        if constexpr (B == 4) {
 #pragma unroll
-        for (int s = 0; s < KMAX; ++s) {
-            if (s < KH) {
-                const uint32_t bo = soff + 64u * s + mis;
-                const uint32_t* p = reinterpret_cast<const uint32_t*>(rowp + (bo & ~3u));
-                uint32_t w[4];
-                if constexpr (B == 4) {
+            for (int k = 0; k < 4; ++k) w[k] = p[k];
+        } else {
+            uint32_t d[5];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) w[k] = p[k];
-                } else {
-                    uint32_t d[5];
+            for (int k = 0; k < 5; ++k) d[k] = p[k];
+            const int al = (int)(bo & 3u);
 #pragma unroll
-                    for (int k = 0; k < 5; ++k) d[k] = p[k];
-                    const int al = (int)(bo & 3u);
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) w[k] = __builtin_amdgcn_alignbyte(d[k + 1], d[k], al);
-                }
-                i32x4 av;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    uint32_t v = w[k];
-                    if constexpr ((B == 2 || B == 4) && PM) v = premult_dword<B>(v);
-                    av[k] = (int)(v ^ 0x80808080u);
-                }
-                if constexpr (LIMBS == 2) dh = mfma_i8(av, whi[s], dh);
-                dl = mfma_i8(av, wlo[s], dl);
-            }
+            for (int k = 0; k < 4; ++k) w[k] = __builtin_amdgcn_alignbyte(d[k + 1], d[k], al);
         }
+        i32x4 av;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t v = w[k];
+            if constexpr ((B == 2 || B == 4) && PM) v = premult_dword<B>(v);
+            av[k] = (int)(v ^ 0x80808080u);
+        }
+        return av;
+    };
+    auto h_pack = [&](const i32x4& dh, const i32x4& dl) -> uint32_t {
         uint32_t packed = 0;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int acc = LIMBS == 2 ? 128 * dh[e] + dl[e] : dl[e];
             packed |= sep_round<FAM>(acc, job.h_mul, job.h_shift, job.h_hi) << (8 * e);
         }
-        *reinterpret_cast<uint32_t*>(col + (r0 - r_lo + 4 * q)) = packed ^ 0x80808080u;
+        return packed ^ 0x80808080u;
+    };
+    for (int r0 = r_lo; r0 <= r_hi; r0 += 32) {
+        const bool two = r0 + 16 <= r_hi;
+        const int r1 = two ? r0 + 16 : r0;
+        i32x4 dh0 = {0, 0, 0, 0}, dl0 = {hseed, hseed, hseed, hseed};
+        i32x4 dh1 = {0, 0, 0, 0}, dl1 = {hseed, hseed, hseed, hseed};
+#pragma unroll
+        for (int s = 0; s < KMAX; ++s) {
+            if (s < KH) {
+                const i32x4 a0 = a_operand(r0, s);
+                const i32x4 a1 = a_operand(r1, s);
+                if constexpr (LIMBS == 2) {
+                    dh0 = mfma_i8(a0, whi[s], dh0);
+                    dh1 = mfma_i8(a1, whi[s], dh1);
+                }
+                dl0 = mfma_i8(a0, wlo[s], dl0);
+                dl1 = mfma_i8(a1, wlo[s], dl1);
+            }
+        }
+        *reinterpret_cast<uint32_t*>(col + (r0 - r_lo + 4 * q)) = h_pack(dh0, dl0);
+        if (two) *reinterpret_cast<uint32_t*>(col + (r1 - r_lo + 4 * q)) = h_pack(dh1, dl1);
     }
 
-    // ---- V: each 16-row output tile from the staged operands.
-    const unsigned char* ncol = inter + (size_t)(wave * 16 + n) * RS;
+    // ---- V: the 16-row output tiles from the staged operands, two per step.
     const int s0 = jx * 16;
-    for (int t = t0; t <= t1; ++t) {
-        const int y0 = 16 * t;
+    const int smp = s0 + n;
+    auto v_tile = [&](int t, i32x4& dh, i32x4& dl) {
         const int base16 = cload(job.vtile, 3 * t + 1);
         const unsigned char* vt = job.vtab + (size_t)cload(job.vtile, 3 * t) * job.vtab_bytes;
         const int* seeds = reinterpret_cast<const int*>(vt + (size_t)KV * LIMBS * 64 * 16);
-        i32x4 vseed;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int s0v = cload(seeds, e), s1v = cload(seeds, 4 + e), s2v = cload(seeds, 8 + e),
                       s3v = cload(seeds, 12 + e);
-            vseed[e] = q == 0 ? s0v : (q == 1 ? s1v : (q == 2 ? s2v : s3v));
+            dl[e] = q == 0 ? s0v : (q == 1 ? s1v : (q == 2 ? s2v : s3v));
+            dh[e] = 0;
         }
         const unsigned char* vs = vstage + (size_t)(t - t0) * VSZ;
-        i32x4 dh = {0, 0, 0, 0}, dl = vseed;
         for (int s = 0; s < KV; ++s) {
-            const i32x4 bv = *reinterpret_cast<const i32x4*>(ncol + (base16 - r_lo + 64 * s + 16 * q));
+            const i32x4 bv = *reinterpret_cast<const i32x4*>(col + (base16 - r_lo + 64 * s + 16 * q));
             if constexpr (LIMBS == 2) {
                 dh = mfma_i8(*reinterpret_cast<const i32x4*>(vs + (size_t)(2 * s) * 1024 + lane * 16), bv, dh);
                 dl = mfma_i8(*reinterpret_cast<const i32x4*>(vs + (size_t)(2 * s + 1) * 1024 + lane * 16), bv, dl);
@@ -736,6 +751,9 @@ __device__ void sep_tile_task(const SepJob& job, int bx, int by, unsigned char* 
                 dl = mfma_i8(*reinterpret_cast<const i32x4*>(vs + (size_t)s * 1024 + lane * 16), bv, dl);
             }
         }
+    };
+    auto v_store = [&](int t, const i32x4& dh, const i32x4& dl) {
+        const int y0 = 16 * t;
         uint32_t v[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e)
@@ -749,7 +767,6 @@ __device__ void sep_tile_task(const SepJob& job, int bx, int by, unsigned char* 
                 }
             }
         }
-        const int smp = s0 + n;
         if (smp < nsamp) {
             unsigned char* o = static_cast<unsigned char*>(out.ptr) + (size_t)(y0 + 4 * q) * nsamp + smp;
             const int rows_left = y_end - (y0 + 4 * q);
@@ -757,6 +774,13 @@ __device__ void sep_tile_task(const SepJob& job, int bx, int by, unsigned char* 
             for (int e = 0; e < 4; ++e)
                 if (e < rows_left) o[(size_t)e * nsamp] = (unsigned char)v[e];
         }
+    };
+    for (int t = t0; t <= t1; t += 2) {
+        i32x4 dh0, dl0, dh1, dl1;
+        v_tile(t, dh0, dl0);
+        if (t + 1 <= t1) v_tile(t + 1, dh1, dl1);
+        v_store(t, dh0, dl0);
+        if (t + 1 <= t1) v_store(t + 1, dh1, dl1);
     }
 }
 
