@@ -80,7 +80,7 @@ struct vgpu_batch {
     std::vector<Launch> launches;
     std::vector<int> fallback;  // images run stage by stage
     int n_shrink_tasks = 0, n_reduce_tasks = 0;
-    int reduce_chain = 1;     // 1: reduce stages on their own chain after their producers; 0: on the image's chain
+    int reduce_chain = 0;     // 1: reduce stages on their own chain after their producers; 0: on the image's chain
     int grid_cap = 0;         // persistent grid size of the shrink launches; 0 = one block per task
     int reduce_grid_cap = 0;  // the same for the reduce and MFMA separable launches
     int use_graph = 1;
@@ -217,12 +217,14 @@ int vgpu_batch::plan() {
 
     // Shrink groups: images dealt, largest first, to the group that moves
     // the fewest bytes so far; each group's box shrinks form a chain of
-    // launches on its own stream.  With a reduce chain (the default) the
+    // launches on its own stream.  With a reduce chain (an option) the
     // reduce stages of every level after the first go to the last stream
     // instead, one launch per (kernel variant, producing launch), each
     // waiting only for the shrink launch that wrote its inputs: the shrink
-    // chains never wait for a reduce, and the reduce work of all but the last
-    // shrink launches runs under the remaining shrinks.
+    // chains never wait for a reduce.  (Measured: the hardware dispatches
+    // those reduce launches only once the shrink launches, whose persistent
+    // grids hold every CU slot, have drained, so the option is off unless
+    // the tuner finds it faster.)
     const int shrink_groups = reduce_chain ? kGroups - 1 : kGroups;
     const int rchain = reduce_chain ? kGroups - 1 : -1;
     std::vector<int> group_of(count, 0);
@@ -737,7 +739,7 @@ int vgpu_batch_tune(vgpu_batch* b, const void* const* srcs, const size_t* src_by
     // per task, or 4/8/16 blocks per CU walking the list), all launched from
     // a graph.  The VALU reduce tile only matters when the batch has VALU
     // reduce launches.
-    static const int shares[] = {1, 0};  // reduce chain on / off
+    static const int shares[] = {0, 1};  // reduce chain off / on
     static const int tiles[] = {0, 1, 2, 3};
     static const int sgrids[] = {0, 256 * 8, 256 * 16};
     static const int rgrids[] = {0, 256 * 4, 256 * 8, 256 * 16};
