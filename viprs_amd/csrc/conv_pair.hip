@@ -114,8 +114,11 @@ __global__ __launch_bounds__(kThreads) void k_conv_pair(ConvPairJob job) {
     double* wv = wh + n;
     float* wh32 = reinterpret_cast<float*>(wv + n);
     float* wv32 = wh32 + n;
-    float* stage[2] = {wv32 + n, wv32 + n + RB * span};
-    float* ring = stage[1] + RB * span;
+    // (The two staging groups are addressed arithmetically from the shared
+    // base: a pointer picked from an array loses the LDS address space and
+    // turns every tap read into a flat access.)
+    float* const stage0 = wv32 + n;
+    float* ring = stage0 + 2 * RB * span;
     for (int t = s; t < n; t += kThreads) {
         wh[t] = job.wh[t];
         wv[t] = job.wv[t];
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(kThreads) void k_conv_pair(ConvPairJob job) {
 
     load_group(0);
     for (int g = 0; g < ngroups; ++g) {
-        float* cur = stage[g & 1];
+        float* cur = stage0 + (g & 1) * (RB * span);
         park_group(cur);
         __syncthreads();
         if (g + 1 < ngroups) load_group(g + 1);

@@ -128,6 +128,46 @@ __device__ __forceinline__ uint32_t unpremult_u8(uint32_t c, uint32_t a) {
     return v > 255u ? 255u : v;
 }
 
+// Stage source rows ys .. ys + hr - 1 of the columns c0 .. c0 + spx - 1
+// (each through the clamped read) into LDS, spx * B samples per row: the H
+// taps of a fused tile then read LDS instead of global memory.  A span that
+// no clamp touches is a contiguous run of each source row: 4-byte samples
+// then come in by LDS-direct dword loads (one wavefront instruction per 64
+// samples, no registers); other spans sample by sample.  Ends with the
+// loads landed (the caller's barrier then publishes them).
+template <typename T, int B>
+__device__ __forceinline__ void stage_h_rows(const InView& in, int ys, int hr, int c0, int spx, T* stage) {
+    const int srow = spx * B;
+    const int lo = max(in.cx0, in.bx), hi = min(in.cx1, in.bx + in.bw - 1);
+    if (sizeof(T) == 4 && c0 >= lo && c0 + spx - 1 <= hi) {
+        const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        for (int k = wave; k < hr; k += kThreads / 64) {
+            const T* src = static_cast<const T*>(in.ptr) + ((size_t)row_of(in, ys + k) * in.bw + (c0 - in.bx)) * B;
+            for (int d0 = 0; d0 < srow; d0 += 64) {
+                if (d0 + lane < srow)
+                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + d0 + lane),
+                                                     (__attribute__((address_space(3))) void*)(stage + k * srow + d0),
+                                                     4, 0, 0);
+            }
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        return;
+    }
+    for (int k = 0; k < hr; ++k) {
+        const T* src = static_cast<const T*>(in.ptr) + (size_t)row_of(in, ys + k) * in.bw * B;
+        for (int w = threadIdx.x; w < srow; w += kThreads) {
+            const int cc = w / B, b = w - cc * B;
+            stage[k * srow + w] = src[(size_t)col_of(in, c0 + cc) * B + b];
+        }
+    }
+}
+
+// Bytes the fused reduce tile keeps before its staged source rows (the H
+// results, 16-byte aligned).
+__host__ __device__ inline size_t reduce_hbuf_bytes(const ReduceJob& job, size_t esz) {
+    return ((size_t)job.nr_max * job.tx * job.bands * esz + 15) & ~(size_t)15;
+}
+
 // LDS hand-off between the lanes of one wavefront: LDS operations of a
 // wavefront execute in issue order, so a compiler barrier suffices.
 __device__ __forceinline__ void wave_lds_sync() {

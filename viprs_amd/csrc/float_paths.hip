@@ -24,27 +24,35 @@ __global__ __launch_bounds__(kThreads) void k_red_f32(ReduceJob job) {
     const int nr = min(job.nr_max, pv.start[ty0 + nrows - 1] + pv.taps - ys0);
     const int x = threadIdx.x % TX, rl = threadIdx.x / TX, nrl = kThreads / TX;
     const int pitch = TX * B;
-    const float* src = static_cast<const float*>(in.ptr);
-    if (x < ncols) {
-        const int xo = tx0 + x;
-        const int hs = ph.start[xo];
-        const float* w = ph.wf32 + (size_t)ph.phase[xo] * ph.taps;
-        for (int r = rl; r < nr; r += nrl) {
-            const float* row = src + (size_t)row_of(in, ys0 + r) * in.bw * B;
+    // H over staged chunks of job.hr source rows (the tile's source span in
+    // LDS, clamped reads), then V from the H rows.
+    float* stage = hbuf + reduce_hbuf_bytes(job, sizeof(float)) / sizeof(float);
+    const int c0 = ph.start[tx0];
+    const int spx = ph.start[tx0 + ncols - 1] + ph.taps - c0;
+    const int srow = spx * B;
+    const int xo = tx0 + min(x, ncols - 1);
+    const int hoff = (ph.start[xo] - c0) * B;
+    const float* w = ph.wf32 + (size_t)ph.phase[xo] * ph.taps;
+    const int hstep = max(job.hr, 1);
+    for (int rc = 0; rc < nr; rc += hstep) {
+        const int hr = min(hstep, nr - rc);
+        stage_h_rows<float, B>(in, ys0 + rc, hr, c0, spx, stage);
+        __syncthreads();
+        for (int r = rl; x < ncols && r < hr; r += nrl) {
+            const float* row = stage + r * srow + hoff;
             float acc[B];
 #pragma unroll
             for (int b = 0; b < B; ++b) acc[b] = 0.f;
             for (int t = 0; t < ph.taps; ++t) {
-                const float* px = row + (size_t)col_of(in, hs + t) * B;
                 const float wt = w[t];
 #pragma unroll
-                for (int b = 0; b < B; ++b) acc[b] = fmaf(wt, px[b], acc[b]);
+                for (int b = 0; b < B; ++b) acc[b] = fmaf(wt, row[t * B + b], acc[b]);
             }
 #pragma unroll
-            for (int b = 0; b < B; ++b) hbuf[r * pitch + x * B + b] = acc[b];
+            for (int b = 0; b < B; ++b) hbuf[(rc + r) * pitch + x * B + b] = acc[b];
         }
+        __syncthreads();
     }
-    __syncthreads();
     for (int ry = rl; x < ncols && ry < nrows; ry += nrl) {
         const int yo = ty0 + ry;
         const int vs = pv.start[yo];
@@ -303,7 +311,7 @@ hipError_t launch_f16_job(const F16Job& job, hipStream_t s) {
 hipError_t launch_red_f32_job(const ReduceJob& job, hipStream_t s) {
     if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;
     const dim3 grid((unsigned)job.gx, (unsigned)job.gy);
-    const size_t lds = (size_t)job.nr_max * job.tx * job.bands * sizeof(float);
+    const size_t lds = reduce_hbuf_bytes(job, sizeof(float)) + (size_t)job.hr * job.span_max * job.bands * sizeof(float);
     switch (job.bands) {
     case 1: hipLaunchKernelGGL(k_red_f32<1>, grid, dim3(kThreads), lds, s, job); break;
     case 2: hipLaunchKernelGGL(k_red_f32<2>, grid, dim3(kThreads), lds, s, job); break;

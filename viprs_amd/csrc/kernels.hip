@@ -976,6 +976,36 @@ __device__ __forceinline__ T reduce_h_sample(const InView& in, int row, int B, i
     }
 }
 
+// reduce_h_sample's arithmetic over an accessor get(t) -> the sample under
+// tap t (already clamped), for the staged fused kernels.
+template <typename T, typename Get>
+__device__ __forceinline__ T reduce_h_from(Get get, int B, int ph, const DevPlan& p) {
+    const int T_ = p.taps;
+    if constexpr (std::is_same<T, float>::value) {
+        const double* w = p.wf64 + (size_t)ph * T_;
+        double acc = 0.0;
+        int t = 0;
+        if (B == 1) {
+            for (; t + 4 <= T_; t += 4) {
+                double s = 0.0;
+                for (int l = 0; l < 4; ++l) {
+                    const double prod = (double)get(t + l) * w[t + l];
+                    s = s + prod;
+                }
+                acc = acc + s;
+            }
+        }
+        for (; t < T_; ++t) acc = fma((double)get(t), w[t], acc);
+        return (T)(float)acc;
+    } else {
+        static_assert(std::is_same<T, double>::value, "integer H passes sum inline");
+        const double* w = p.wf64 + (size_t)ph * T_;
+        double acc = 0.0;
+        for (int t = 0; t < T_; ++t) acc = fma(get(t), w[t], acc);
+        return acc;
+    }
+}
+
 // One ReduceV output sample from a column accessor get(row) -> T.
 template <typename T, typename Get>
 __device__ __forceinline__ T reduce_v_sample(Get get, int start, int ph, const DevPlan& p, bool vec) {
@@ -1063,6 +1093,75 @@ __global__ __launch_bounds__(kThreads) void k_reduce_v(InView in, OutView out, i
     }
 }
 
+// Reference-order f32 ReduceH of RB staged rows at once (rows[k] at the
+// thread's first tap): per row and band the f64 chain of reduce_h_from --
+// for one band the AVX2 4-lane grouping (each group's products summed, then
+// added), else the FMA chain -- with the RB x B chains interleaved.
+template <int B, int RB>
+__device__ __forceinline__ void reduce_h_f32_rows(const float* const (&rows)[RB], int ph, const DevPlan& p,
+                                                  double (&acc)[RB][B]) {
+    const int T_ = p.taps;
+    const double* w = p.wf64 + (size_t)ph * T_;
+#pragma unroll
+    for (int k = 0; k < RB; ++k)
+#pragma unroll
+        for (int b = 0; b < B; ++b) acc[k][b] = 0.0;
+    int t = 0;
+    if constexpr (B == 1) {
+        for (; t + 4 <= T_; t += 4) {
+            const double w0 = w[t], w1 = w[t + 1], w2 = w[t + 2], w3 = w[t + 3];
+#pragma unroll
+            for (int k = 0; k < RB; ++k) {
+                double g = 0.0;
+                g = g + (double)rows[k][t] * w0;
+                g = g + (double)rows[k][t + 1] * w1;
+                g = g + (double)rows[k][t + 2] * w2;
+                g = g + (double)rows[k][t + 3] * w3;
+                acc[k][0] = acc[k][0] + g;
+            }
+        }
+    }
+    for (; t < T_; ++t) {
+        const double wt = w[t];
+#pragma unroll
+        for (int k = 0; k < RB; ++k)
+#pragma unroll
+            for (int b = 0; b < B; ++b) acc[k][b] = fma((double)rows[k][t * B + b], wt, acc[k][b]);
+    }
+}
+
+// Reference-order f32 ReduceV of RB output rows at once from the H rows in
+// LDS (col = the thread's first sample, rows `pitch` floats apart): per row
+// its own start and phase; samples before vec_len (a multiple of 4) take
+// separate multiply and add, the rest the FMA chain (reduce_v_sample).
+template <int B, int RB>
+__device__ __forceinline__ void reduce_v_f32_rows(const float* col, int pitch, const int (&vs)[RB],
+                                                  const int (&vph)[RB], const DevPlan& p, int vclip0, int vclip1,
+                                                  int ys0, int nr, int s0, int vec_len, double (&acc)[RB][B]) {
+    const int T_ = p.taps;
+    const double* w[RB];
+#pragma unroll
+    for (int k = 0; k < RB; ++k) {
+        w[k] = p.wf64 + (size_t)vph[k] * T_;
+#pragma unroll
+        for (int b = 0; b < B; ++b) acc[k][b] = 0.0;
+    }
+    for (int t = 0; t < T_; ++t) {
+#pragma unroll
+        for (int k = 0; k < RB; ++k) {
+            const int yy = clampi(clampi(vs[k] + t, vclip0, vclip1) - ys0, 0, nr - 1);
+            const float* hp = col + yy * pitch;
+            const double wt = w[k][t];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const double xv = (double)hp[b];
+                if (s0 + b < vec_len) acc[k][b] = acc[k][b] + xv * wt;
+                else acc[k][b] = fma(xv, wt, acc[k][b]);
+            }
+        }
+    }
+}
+
 // Fixed-point accumulator: int32 when the host has proven
 // max_phase(sum |w|) * max_sample < 2^31, else int64.
 template <typename T, typename Acc>
@@ -1093,65 +1192,122 @@ __device__ __forceinline__ void reduce_hv_block(const ReduceJob& job, int bx, in
     const int nr = min(nr_max, pv.start[ty0 + nrows - 1] + pv.taps - ys0);
     const int x = threadIdx.x % TX, rl = threadIdx.x / TX, nrl = kThreads / TX;
     const int pitch = TX * B;
-    if (x < ncols) {
-        const int xo = tx0 + x;
-        const int hs = ph.start[xo];
-        const int hph = ph.phase[xo];
-        for (int r = rl; r < nr; r += nrl) {
-            const int row = row_of(in, ys0 + r);
-            T* dst = hbuf + r * pitch + x * B;
-            if constexpr (sizeof(T) == 4) {
+    // Pass 1 (f32) in chunks of job.hr source rows: the chunk's span
+    // (absolute columns c0 .. c0 + spx - 1, clamped reads) staged in LDS,
+    // then every thread's H samples of those rows from it.
+    T* stage = reinterpret_cast<T*>(smem + reduce_hbuf_bytes(job, sizeof(T)));
+    const int c0 = ph.start[tx0];
+    const int spx = ph.start[tx0 + ncols - 1] + ph.taps - c0;
+    const int srow = spx * B;
+    const int xo = tx0 + min(x, ncols - 1);
+    const int hoff = (ph.start[xo] - c0) * B;
+    const int hph = ph.phase[xo];
+    const int hstep = max(job.hr, 1);  // rows per staged chunk (>= 1 on every staged path)
+    for (int rc = 0; sizeof(T) == 4 && rc < nr; rc += hstep) {
+        const int hr = min(hstep, nr - rc);
+        stage_h_rows<T, B>(in, ys0 + rc, hr, c0, spx, stage);
+        __syncthreads();
+        if constexpr (sizeof(T) == 4) {
+            // job.rb rows per thread at a time (a chunk holds rb * nrl
+            // rows): rb x B independent FP64 chains interleaved tap by tap,
+            // each in the reference's order.
+            auto h_rows = [&](auto rbc) {
+                constexpr int RB = decltype(rbc)::value;
+                for (int r = rl; x < ncols && r < hr; r += RB * nrl) {
+                    const T* rows[RB];
 #pragma unroll
-                for (int b = 0; b < B; ++b) dst[b] = reduce_h_sample<T>(in, row, B, b, hs, hph, ph);
-            } else {
-                const int16_t* w = ph.wi16 + (size_t)hph * ph.taps;
-                const T* src = static_cast<const T*>(in.ptr) + (size_t)row * in.bw * B;
-                Acc acc[B];
+                    for (int k = 0; k < RB; ++k) rows[k] = stage + min(r + k * nrl, hr - 1) * srow + hoff;
+                    double acc[RB][B];
+                    reduce_h_f32_rows<B, RB>(rows, hph, ph, acc);
 #pragma unroll
-                for (int b = 0; b < B; ++b) acc[b] = 0;
-                if (sizeof(T) == 1 && (B == 2 || B == 4) && job.premult) {
-                    // Premultiply on load (pixel-local, so it commutes with the
-                    // clamped source read).
-                    for (int t = 0; t < ph.taps; ++t) {
-                        const T* px = src + (size_t)col_of(in, hs + t) * B;
-                        const Acc wt = w[t];
-                        const uint32_t a = px[B - 1];
+                    for (int k = 0; k < RB; ++k) {
+                        if (r + k * nrl >= hr) break;
+                        T* dst = hbuf + (rc + r + k * nrl) * pitch + x * B;
 #pragma unroll
-                        for (int b = 0; b < B - 1; ++b) acc[b] += wt * (Acc)premult_u8(px[b], a);
-                        acc[B - 1] += wt * (Acc)a;
-                    }
-                } else {
-                    for (int t = 0; t < ph.taps; ++t) {
-                        const T* px = src + (size_t)col_of(in, hs + t) * B;
-                        const Acc wt = w[t];
-#pragma unroll
-                        for (int b = 0; b < B; ++b) acc[b] += wt * (Acc)px[b];
+                        for (int b = 0; b < B; ++b) dst[b] = (T)(float)acc[k][b];
                     }
                 }
-#pragma unroll
-                for (int b = 0; b < B; ++b) dst[b] = fixed_round<T, Acc>(acc[b]);
-            }
+            };
+            if (job.rb >= 4) h_rows(std::integral_constant<int, 4>());
+            else if (job.rb == 2) h_rows(std::integral_constant<int, 2>());
+            else h_rows(std::integral_constant<int, 1>());
         }
+        __syncthreads();
     }
-    __syncthreads();
+    if constexpr (sizeof(T) != 4) {
+        // Integer samples read the clamped source directly (job.hr == 0):
+        // the fixed-point sums are cheap, and a staging round trip would
+        // lengthen the small, latency-bound tiles these jobs run.
+        const int16_t* w = ph.wi16 + (size_t)hph * ph.taps;
+        const int hs = ph.start[xo];
+        for (int r = rl; x < ncols && r < nr; r += nrl) {
+            const T* src = static_cast<const T*>(in.ptr) + (size_t)row_of(in, ys0 + r) * in.bw * B;
+            T* dst = hbuf + r * pitch + x * B;
+            Acc acc[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) acc[b] = 0;
+            if (sizeof(T) == 1 && (B == 2 || B == 4) && job.premult) {
+                // Premultiply on load (pixel-local, so it commutes with the
+                // clamped source read).
+                for (int t = 0; t < ph.taps; ++t) {
+                    const T* px = src + (size_t)col_of(in, hs + t) * B;
+                    const Acc wt = w[t];
+                    const uint32_t a = px[B - 1];
+#pragma unroll
+                    for (int b = 0; b < B - 1; ++b) acc[b] += wt * (Acc)premult_u8(px[b], a);
+                    acc[B - 1] += wt * (Acc)a;
+                }
+            } else {
+                for (int t = 0; t < ph.taps; ++t) {
+                    const T* px = src + (size_t)col_of(in, hs + t) * B;
+                    const Acc wt = w[t];
+#pragma unroll
+                    for (int b = 0; b < B; ++b) acc[b] += wt * (Acc)px[b];
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) dst[b] = fixed_round<T, Acc>(acc[b]);
+        }
+        __syncthreads();
+    }
     const int row_len = out.w * B;
     const int vec_len = row_len / 4 * 4;
+    if constexpr (sizeof(T) == 4) {
+        // Up to four output rows per thread at a time (RB x B FP64 chains).
+        auto v_rows = [&](auto rbc) {
+            constexpr int RB = decltype(rbc)::value;
+            for (int ry = rl; x < ncols && ry < nrows; ry += RB * nrl) {
+                int vs[RB], vph[RB];
+#pragma unroll
+                for (int k = 0; k < RB; ++k) {
+                    const int yo = ty0 + min(ry + k * nrl, nrows - 1);
+                    vs[k] = pv.start[yo];
+                    vph[k] = pv.phase[yo];
+                }
+                double acc[RB][B];
+                reduce_v_f32_rows<B, RB>(hbuf + x * B, pitch, vs, vph, pv, vclip0, vclip1, ys0, nr,
+                                         (tx0 + x) * B, vec_len, acc);
+#pragma unroll
+                for (int k = 0; k < RB; ++k) {
+                    if (ry + k * nrl >= nrows) break;
+                    T* o = static_cast<T*>(out.ptr) + (size_t)(ty0 + ry + k * nrl) * row_len + (size_t)(tx0 + x) * B;
+#pragma unroll
+                    for (int b = 0; b < B; ++b) o[b] = (T)(float)acc[k][b];
+                }
+            }
+        };
+        const int rows_per_thread = (TY + nrl - 1) / nrl;
+        if (rows_per_thread >= 4) v_rows(std::integral_constant<int, 4>());
+        else if (rows_per_thread >= 2) v_rows(std::integral_constant<int, 2>());
+        else v_rows(std::integral_constant<int, 1>());
+    }
+    if constexpr (sizeof(T) != 4) {
     for (int ry = rl; x < ncols && ry < nrows; ry += nrl) {
         const int yo = ty0 + ry;
         const int vs = pv.start[yo];
         const int vph = pv.phase[yo];
         T* o = static_cast<T*>(out.ptr) + (size_t)yo * row_len + (size_t)(tx0 + x) * B;
-        if constexpr (sizeof(T) == 4) {
-#pragma unroll
-            for (int b = 0; b < B; ++b) {
-                const int sidx = (tx0 + x) * B + b;
-                auto get = [&](int y) -> T {
-                    const int yy = clampi(clampi(y, vclip0, vclip1) - ys0, 0, nr - 1);
-                    return hbuf[yy * pitch + x * B + b];
-                };
-                o[b] = reduce_v_sample<T>(get, vs, vph, pv, sidx < vec_len);
-            }
-        } else {
+        {
             const int16_t* w = pv.wi16 + (size_t)vph * pv.taps;
             Acc acc[B];
 #pragma unroll
@@ -1176,6 +1332,7 @@ __device__ __forceinline__ void reduce_hv_block(const ReduceJob& job, int bx, in
             for (int b = 0; b < B; ++b) o[b] = v[b];
         }
     }
+    }
 }
 
 // ReduceOp (reduce.rs:224-250) on the fused tile: the H pass into LDS over
@@ -1195,27 +1352,36 @@ __device__ __forceinline__ void reduce_op_block(const ReduceJob& job, int bx, in
     const int nr = min(job.nr_max, pv.start[ty0 + nrows - 1] + pv.taps - ys0);
     const int x = threadIdx.x % TX, rl = threadIdx.x / TX, nrl = kThreads / TX;
     const int pitch = TX * B;
-    if (x < ncols) {
-        const int xo = tx0 + x;
-        const int hs = ph.start[xo];
-        const double* w = ph.wf64 + (size_t)ph.phase[xo] * ph.taps;
-        for (int r = rl; r < nr; r += nrl) {
-            const T* src = static_cast<const T*>(in.ptr) + (size_t)row_of(in, ys0 + r) * in.bw * B;
+    // H over staged chunks of job.hr source rows, as in reduce_hv_block.
+    T* stage = reinterpret_cast<T*>(smem + reduce_hbuf_bytes(job, sizeof(T)));
+    const int c0 = ph.start[tx0];
+    const int spx = ph.start[tx0 + ncols - 1] + ph.taps - c0;
+    const int srow = spx * B;
+    const int xo = tx0 + min(x, ncols - 1);
+    const int hoff = (ph.start[xo] - c0) * B;
+    const double* w = ph.wf64 + (size_t)ph.phase[xo] * ph.taps;
+    const int hstep = max(job.hr, 1);
+    for (int rc = 0; rc < nr; rc += hstep) {
+        const int hr = min(hstep, nr - rc);
+        stage_h_rows<T, B>(in, ys0 + rc, hr, c0, spx, stage);
+        __syncthreads();
+        for (int r = rl; x < ncols && r < hr; r += nrl) {
+            const T* row = stage + r * srow + hoff;
             double acc[B];
 #pragma unroll
             for (int b = 0; b < B; ++b) acc[b] = 0.0;
             for (int t = 0; t < ph.taps; ++t) {
-                const T* px = src + (size_t)col_of(in, hs + t) * B;
+                const T* px = row + t * B;
                 const double wt = w[t];
 #pragma unroll
                 for (int b = 0; b < B; ++b) acc[b] = fma((double)px[b], wt, acc[b]);
             }
-            T* dst = hbuf + r * pitch + x * B;
+            T* dst = hbuf + (rc + r) * pitch + x * B;
 #pragma unroll
             for (int b = 0; b < B; ++b) dst[b] = op_from_f64<T>(acc[b]);
         }
+        __syncthreads();
     }
-    __syncthreads();
     const int vclip0 = job.vclip0, vclip1 = job.vclip1;
     for (int ry = rl; x < ncols && ry < nrows; ry += nrl) {
         const int yo = ty0 + ry;
@@ -1615,7 +1781,8 @@ hipError_t launch_reduce_v(const InView& in, const OutView& out, int bands, int 
 }
 
 size_t reduce_job_lds(const ReduceJob& job) {
-    return (size_t)job.nr_max * job.tx * job.bands * (job.fmt == 0 ? 1 : (job.fmt == 1 ? 2 : 4));
+    const size_t esz = job.fmt == 0 ? 1 : (job.fmt == 1 ? 2 : 4);
+    return reduce_hbuf_bytes(job, esz) + (size_t)job.hr * job.span_max * job.bands * esz;
 }
 
 hipError_t launch_reduce_job(const ReduceJob& job, hipStream_t s) {

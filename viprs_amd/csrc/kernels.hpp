@@ -105,6 +105,10 @@ struct ReduceJob {
     int gx = 0, gy = 0;
     int src_slot = -1, dst_slot = -1;
     int op = 0;  // 1: ReduceOp arithmetic (f64 mul_add chains over wf64, from_f64 rounding)
+    // H source staging: span_max = the widest span of source columns a tile's
+    // H taps read, hr = source rows staged in LDS per chunk, rb = H rows a
+    // thread of the f32 path evaluates at once (hr = rb * 256 / tx).
+    int span_max = 1, hr = 1, rb = 1;
 };
 
 // Separable u8 pass pair on MFMA (int8 limbs): H then V with a u8

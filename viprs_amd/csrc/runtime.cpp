@@ -503,12 +503,24 @@ void stage_reduce_params(const Node& nd, const Rect& out, Staging* st, Stage* s)
     s->taps = nd.axis.taps;
 }
 
+// Widest span of source columns the H taps of one tx-column tile read.
+int64_t reduce_span(const std::vector<int32_t>& hs, int htaps, int64_t outw, int tx) {
+    int64_t span = 1;
+    for (int64_t x0 = 0; x0 < outw; x0 += tx) {
+        const int64_t x1 = std::min<int64_t>(x0 + tx, outw) - 1;
+        span = std::max<int64_t>(span, (int64_t)hs[(size_t)x1] + htaps - hs[(size_t)x0]);
+    }
+    return span;
+}
+
 // Tile of the fused VALU reduce kernels.  Candidates, largest first: take the
-// first whose staged rows fit 64 KiB of LDS and that still gives >=
-// opt.reduce_min_blocks blocks; otherwise the fitting candidate with the most
-// blocks.  The search starts at candidate opt.reduce_first_tile.
-bool pick_reduce_tile(const std::vector<int32_t>& vs, int vtaps, const Rect& out, uint32_t bands, size_t esz,
-                      const StageOptions& opt, int* tx, int* ty, int* nr) {
+// first whose staged H rows plus two staged source rows fit 64 KiB of LDS and
+// that still gives >= opt.reduce_min_blocks blocks; otherwise the fitting
+// candidate with the most blocks.  The search starts at candidate
+// opt.reduce_first_tile.
+bool pick_reduce_tile(const std::vector<int32_t>& vs, int vtaps, const std::vector<int32_t>& hs, int htaps,
+                      const Rect& out, uint32_t bands, size_t esz, bool stage, const StageOptions& opt, int* tx,
+                      int* ty, int* nr) {
     static const int cand[][2] = {{64, 16}, {32, 16}, {32, 8}, {16, 8}, {16, 4}, {8, 4}, {8, 2}, {8, 1}, {4, 1}, {1, 1}};
     int64_t best_blocks = -1;
     const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
@@ -519,7 +531,12 @@ bool pick_reduce_tile(const std::vector<int32_t>& vs, int vtaps, const Rect& out
             const int64_t y1 = std::min<int64_t>(y0 + c[1], out.h) - 1;
             rows = std::max<int>(rows, vs[(size_t)y1] + vtaps - vs[(size_t)y0]);
         }
-        if ((size_t)rows * c[0] * bands * esz > 64 * 1024) continue;
+        // With staged source rows (stage): the H rows plus one source row
+        // per row lane of the block (a chunk of 256 / tx rows) within 40 KiB;
+        // else the H rows within 64 KiB.
+        const size_t hbuf = ((size_t)rows * c[0] * bands * esz + 15) & ~(size_t)15;
+        const size_t srow = (size_t)reduce_span(hs, htaps, out.w, c[0]) * bands * esz;
+        if (stage ? hbuf + (size_t)(256 / c[0]) * srow > 40 * 1024 : hbuf > 64 * 1024) continue;
         const int64_t blocks = ((out.w + c[0] - 1) / c[0]) * ((out.h + c[1] - 1) / c[1]);
         if (blocks > best_blocks) {
             best_blocks = blocks;
@@ -530,6 +547,34 @@ bool pick_reduce_tile(const std::vector<int32_t>& vs, int vtaps, const Rect& out
         if (blocks >= opt.reduce_min_blocks) break;
     }
     return best_blocks >= 0;
+}
+
+// H source staging of a planned fused VALU reduce job (f32 and ReduceOp;
+// integer fused reduces read the source directly, hr = 0): the widest tile
+// span and the rows per staged chunk, rb (1, 2 or 4) rows per row lane of
+// the block, as many as keep the tile's LDS within 40 KiB (pick_reduce_tile
+// guarantees one).
+void set_reduce_staging(ReduceJob* j, const std::vector<int32_t>& hs, int htaps, int64_t outw, size_t esz) {
+    if (esz < 4 && !j->op) {
+        j->span_max = 1;
+        j->hr = 0;
+        j->rb = 1;
+        return;
+    }
+    j->span_max = (int)reduce_span(hs, htaps, outw, j->tx);
+    const size_t hbuf = ((size_t)j->nr_max * j->tx * j->bands * esz + 15) & ~(size_t)15;
+    const size_t srow = (size_t)j->span_max * j->bands * esz;
+    const int lanes = std::max(1, 256 / std::max(j->tx, 1));
+    int rb = 4;
+    while (rb > 1 && hbuf + (size_t)(rb * lanes) * srow > 40 * 1024) rb /= 2;
+    j->rb = rb;
+    j->hr = rb * lanes;
+    if (esz < 4) {
+        // ReduceOp on integers sums one row per thread iteration: stage as
+        // many rows per chunk as fit (fewer chunks, fewer barriers).
+        const size_t fit = hbuf < 40 * 1024 ? (40 * 1024 - hbuf) / srow : 0;
+        j->hr = (int)std::max<size_t>((size_t)lanes, std::min<size_t>(fit, (size_t)j->nr_max));
+    }
 }
 
 // f32 copy of a weight table.
@@ -714,7 +759,8 @@ bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const Redu
         plan_sep(0, ha, va, 2048, 2048, bands, out, clip, buf, vclip, opt, st, s);  // ST_SEP on success
     }
     int tx = 0, ty = 0, nr = 0;
-    const bool tile_ok = pick_reduce_tile(vs, vax.taps, out, bands, format_size(fmt), opt, &tx, &ty, &nr);
+    const bool tile_ok = pick_reduce_tile(vs, vax.taps, hs, hax.taps, out, bands, format_size(fmt),
+                                          fmt == F32 || reduce_op, opt, &tx, &ty, &nr);
     if (!tile_ok) {
         if (s->kind != ST_SEP) return false;
         tx = ty = nr = 1;  // the VALU tile is not used
@@ -739,6 +785,8 @@ bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const Redu
     j.vclip1 = (int)(vclip.y + std::max<int64_t>(vclip.h, 1) - 1);
     j.gx = (int)((out.w + tx - 1) / tx);
     j.gy = (int)((out.h + ty - 1) / ty);
+    j.op = reduce_op ? 1 : 0;
+    set_reduce_staging(&j, hs, hax.taps, out.w, format_size(fmt));
     // int32 sums are exact when max over phases of sum |w| times the largest
     // sample stays below 2^31.
     const int64_t maxv = fmt == U8 ? 255 : 65535;
@@ -835,7 +883,7 @@ bool stage_gauss_float_params(const Node& gh, const Node& gv, const Rect& mid, c
     for (int64_t x = 0; x < mid.w; ++x) hs[(size_t)x] = (int32_t)(mid.x + x - rad);
     for (int64_t y = 0; y < out.h; ++y) vs[(size_t)y] = (int32_t)(out.y + y - rad);
     int tx, ty, nr;
-    if (!pick_reduce_tile(vs, taps, out, gh.bands, sizeof(float), opt, &tx, &ty, &nr)) return false;
+    if (!pick_reduce_tile(vs, taps, hs, taps, out, gh.bands, sizeof(float), true, opt, &tx, &ty, &nr)) return false;
     const std::vector<uint8_t> hp(hs.size(), 0), vp(vs.size(), 0);
     s->plan_start = st->put(hs.data(), hs.size() * sizeof(int32_t));
     s->plan_phase = st->put(hp.data(), hp.size());
@@ -855,6 +903,7 @@ bool stage_gauss_float_params(const Node& gh, const Node& gv, const Rect& mid, c
     j.vclip1 = (int)(vclip.y + std::max<int64_t>(vclip.h, 1) - 1);
     j.gx = (int)((out.w + tx - 1) / tx);
     j.gy = (int)((out.h + ty - 1) / ty);
+    set_reduce_staging(&j, hs, taps, out.w, sizeof(float));
     s->kind = ST_RED_F32;
     if (opt.float_mode == 2) {
         const F16Axis ha{&hs, nullptr, gh.conv.data(), taps}, va{&vs, nullptr, gv.conv.data(), taps};
