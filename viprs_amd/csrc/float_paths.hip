@@ -147,7 +147,10 @@ __device__ __forceinline__ void load8f(const float* base, uint32_t e, uint32_t t
 // MFMA of each K step starts from zero and is added to the sum on the VALU
 // (round to nearest), and the two correction products share their own
 // accumulator, so no large partial sum goes through the MFMA accumulator.
-template <int B>
+// MAXH: H K steps the registers hold (KH <= MAXH); PF: the next M tile's
+// samples are loaded while the current one is worked (up to three K steps:
+// with four the extra registers cost more occupancy than the overlap gains).
+template <int B, int MAXH, bool PF>
 __device__ void f16_task(const F16Job& job, int jx, int ty, unsigned char* lds, size_t wave_lds) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -167,9 +170,9 @@ __device__ void f16_task(const F16Job& job, int jx, int ty, unsigned char* lds, 
     const int pat = __builtin_amdgcn_readfirstlane(job.hstrip[2 * jx]);
     const int span0 = __builtin_amdgcn_readfirstlane(job.hstrip[2 * jx + 1]);
     const unsigned char* ht = job.htab + (size_t)pat * job.htab_bytes;
-    u32x4v whi[kF16MaxH], wlo[kF16MaxH];
+    u32x4v whi[MAXH], wlo[MAXH];
 #pragma unroll
-    for (int s = 0; s < kF16MaxH; ++s) {
+    for (int s = 0; s < MAXH; ++s) {
         if (s < KH) {
             const u32x4 a = gmem<u32x4>(ht + ((size_t)(2 * s) * 64 + lane) * 16)[0];
             const u32x4 b = gmem<u32x4>(ht + ((size_t)(2 * s + 1) * 64 + lane) * 16)[0];
@@ -182,19 +185,56 @@ __device__ void f16_task(const F16Job& job, int jx, int ty, unsigned char* lds, 
     const int y_begin = ty * job.rows_per_task;
     const int y_end = min(out.h, y_begin + job.rows_per_task);
     int have = INT32_MIN;
+    // The next M tile's samples are loaded while the current one is split
+    // and multiplied (xn, for row r_next).
+    float xn[MAXH][8];
+    int r_next = INT32_MIN;
+    auto load_m = [&](int r) {
+        const uint32_t rb = (uint32_t)row_of(in, r + n) * row_elems + lane_off;
+#pragma unroll
+        for (int s = 0; s < MAXH; ++s)
+            if (s < KH) load8f(base, rb + 32 * s, total, xn[s]);
+        r_next = r;
+    };
     for (int y0 = y_begin, ti = y_begin >> 4; y0 < y_end; y0 += 16, ++ti) {
         const int vpat = job.vtile[3 * ti];
         const int base8 = job.vtile[3 * ti + 1], hi = job.vtile[3 * ti + 2];
+        // V operands of this output tile, loaded ahead of its H work.
+        const unsigned char* vt = job.vtab + (size_t)vpat * job.vtab_bytes;
+        u32x4 vop[kF16MaxV][2];
+#pragma unroll
+        for (int s = 0; s < kF16MaxV; ++s) {
+            if (s < KV) {
+                vop[s][0] = gmem<u32x4>(vt + ((size_t)(2 * s) * 64 + lane) * 16)[0];
+                vop[s][1] = gmem<u32x4>(vt + ((size_t)(2 * s + 1) * 64 + lane) * 16)[0];
+            }
+        }
         // ---- intermediate rows from max(have, base8 rounded to 16) through hi.
         int r0 = max(have, base8 & ~15);
+        if (PF && r0 <= hi && r_next != r0) load_m(r0);
         for (; r0 <= hi; r0 += 16) {
-            const uint32_t rb = (uint32_t)row_of(in, r0 + n) * row_elems + lane_off;
-            float x[kF16MaxH][8];
+            float x[MAXH][8];
             float mx = 0.f;
+            if constexpr (PF) {
 #pragma unroll
-            for (int s = 0; s < kF16MaxH; ++s) {
+                for (int s = 0; s < MAXH; ++s) {
+                    if (s < KH) {
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) x[s][e] = xn[s][e];
+                    }
+                }
+                // Prefetch the following M tile (this output tile's next
+                // one, or the first one the next output tile computes).
+                if (r0 + 16 <= hi || y0 + 16 < y_end) load_m(r0 + 16);
+            } else {
+                const uint32_t rb = (uint32_t)row_of(in, r0 + n) * row_elems + lane_off;
+#pragma unroll
+                for (int s = 0; s < MAXH; ++s)
+                    if (s < KH) load8f(base, rb + 32 * s, total, x[s]);
+            }
+#pragma unroll
+            for (int s = 0; s < MAXH; ++s) {
                 if (s < KH) {
-                    load8f(base, rb + 32 * s, total, x[s]);
 #pragma unroll
                     for (int e = 0; e < 8; ++e) mx = fmaxf(mx, fabsf(x[s][e]));
                 }
@@ -202,7 +242,7 @@ __device__ void f16_task(const F16Job& job, int jx, int ty, unsigned char* lds, 
             const int sx = wave_scale_exp(mx);
             f32x4 d = {0.f, 0.f, 0.f, 0.f}, dc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int s = 0; s < kF16MaxH; ++s) {
+            for (int s = 0; s < MAXH; ++s) {
                 if (s < KH) {
 #pragma unroll
                     for (int e = 0; e < 8; ++e) x[s][e] = __builtin_amdgcn_ldexpf(x[s][e], sx);
@@ -227,7 +267,6 @@ __device__ void f16_task(const F16Job& job, int jx, int ty, unsigned char* lds, 
         for (int r = have; r < base8 + 32 * KV; r += 16)
             *reinterpret_cast<f32x4*>(ring + n * RS + ((r + 4 * q) & (RING - 1))) = f32x4{0.f, 0.f, 0.f, 0.f};
         // ---- V: A = folded V weights (rows j = output rows, K = rows from base8).
-        const unsigned char* vt = job.vtab + (size_t)vpat * job.vtab_bytes;
         float yv[kF16MaxV][8];
         float my = 0.f;
 #pragma unroll
@@ -250,8 +289,7 @@ __device__ void f16_task(const F16Job& job, int jx, int ty, unsigned char* lds, 
 #pragma unroll
         for (int s = 0; s < kF16MaxV; ++s) {
             if (s < KV) {
-                const u32x4 a = gmem<u32x4>(vt + ((size_t)(2 * s) * 64 + lane) * 16)[0];
-                const u32x4 b = gmem<u32x4>(vt + ((size_t)(2 * s + 1) * 64 + lane) * 16)[0];
+                const u32x4 a = vop[s][0], b = vop[s][1];
 #pragma unroll
                 for (int e = 0; e < 8; ++e) yv[s][e] = __builtin_amdgcn_ldexpf(yv[s][e], sy);
                 u32x4v bh, bl;
@@ -278,12 +316,12 @@ __device__ void f16_task(const F16Job& job, int jx, int ty, unsigned char* lds, 
     }
 }
 
-template <int B>
+template <int B, int MAXH, bool PF>
 __global__ __launch_bounds__(kThreads) void k_f16_job(F16Job job, int wave_lds) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int t = blockIdx.x * (kThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (t >= job.gx * job.gy) return;
-    f16_task<B>(job, t % job.gx, t / job.gx, lds, (size_t)wave_lds);
+    f16_task<B, MAXH, PF>(job, t % job.gx, t / job.gx, lds, (size_t)wave_lds);
 }
 
 }  // namespace
@@ -298,13 +336,18 @@ hipError_t launch_f16_job(const F16Job& job, hipStream_t s) {
     const int ntasks = job.gx * job.gy;
     const dim3 grid((unsigned)((ntasks + 3) / 4));
     const size_t lds = (size_t)wl * 4;
+    const bool pf = job.kh <= 3;  // H spans up to 3 K steps: prefetch the next M tile
+#define VGPU_F16(BB)                                                                                   \
+    if (pf) hipLaunchKernelGGL((k_f16_job<BB, 3, true>), grid, dim3(kThreads), lds, s, job, wl);       \
+    else hipLaunchKernelGGL((k_f16_job<BB, kF16MaxH, false>), grid, dim3(kThreads), lds, s, job, wl);
     switch (job.bands) {
-    case 1: hipLaunchKernelGGL(k_f16_job<1>, grid, dim3(kThreads), lds, s, job, wl); break;
-    case 2: hipLaunchKernelGGL(k_f16_job<2>, grid, dim3(kThreads), lds, s, job, wl); break;
-    case 3: hipLaunchKernelGGL(k_f16_job<3>, grid, dim3(kThreads), lds, s, job, wl); break;
-    case 4: hipLaunchKernelGGL(k_f16_job<4>, grid, dim3(kThreads), lds, s, job, wl); break;
+    case 1: VGPU_F16(1) break;
+    case 2: VGPU_F16(2) break;
+    case 3: VGPU_F16(3) break;
+    case 4: VGPU_F16(4) break;
     default: return hipErrorInvalidValue;
     }
+#undef VGPU_F16
     return hipGetLastError();
 }
 
