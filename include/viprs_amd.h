@@ -125,6 +125,12 @@ int vgpu_memcpy_d2h(vgpu_context *ctx, void *dst_host, const void *src_device, s
  * it lets multi-GB benchmark inputs be made in HBM and regenerated on the
  * host for checking. */
 int vgpu_fill_random_u8(vgpu_context *ctx, void *dst_device, size_t bytes, uint64_t seed);
+/* Measured streaming peaks of this device, for the roofline reports: mean ms
+ * of a nontemporal 16-byte streaming read of `bytes` (the box shrink's load
+ * form) and of a device-to-device copy of `bytes`, each over `iterations`
+ * back-to-back launches between HIP events (buffers from the context pool).
+ * No reference counterpart (measurement helper). */
+int vgpu_measure_stream(vgpu_context *ctx, size_t bytes, int iterations, float *read_ms, float *copy_ms);
 
 /* ----------------------------------------------------- pipeline (builder) */
 

@@ -188,6 +188,7 @@ def load() -> C.CDLL:
         "vgpu_pipeline_set_float_mode": (i32, [vp, i32]),
         "vgpu_batch_set_integer_mfma": (i32, [vp, i32]),
         "vgpu_fill_random_u8": (i32, [vp, vp, sz, C.c_uint64]),
+        "vgpu_measure_stream": (i32, [vp, sz, i32, P(C.c_float), P(C.c_float)]),
         "vgpu_batch_thumbnail_new": (i32, [vp, u32, P(u32), P(u32), P(u32), i32, u32, u32, i32, P(vp)]),
         "vgpu_batch_destroy": (None, [vp]),
         "vgpu_batch_size": (i32, [vp]),

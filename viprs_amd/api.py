@@ -87,6 +87,15 @@ class Context:
     def synchronize(self) -> None:
         _check(self._lib.vgpu_synchronize(self._h), "vgpu_synchronize")
 
+    def measure_stream(self, nbytes: int, iterations: int = 20):
+        """(read GB/s, copy GB/s) of this device: a nontemporal streaming
+        read of nbytes and a device copy of nbytes (read + written bytes)."""
+        r, c = C.c_float(), C.c_float()
+        _check(self._lib.vgpu_measure_stream(self.handle, nbytes, iterations, C.byref(r), C.byref(c)),
+               "vgpu_measure_stream")
+        nb = nbytes // 16 * 16
+        return nb / (r.value / 1e3) / 1e9, 2 * nb / (c.value / 1e3) / 1e9
+
     def pool_bytes(self) -> int:
         return int(self._lib.vgpu_context_pool_bytes(self._h))
 

@@ -1552,6 +1552,22 @@ __global__ __launch_bounds__(kThreads) void k_fill_random(unsigned char* dst, si
     }
 }
 
+// Streaming read of n16 16-byte words (nontemporal, as the box shrink
+// reads its source); each block leaves the xor of what it read in sink[block]
+// so the loads stay live.
+__global__ __launch_bounds__(kThreads) void k_stream_read(const unsigned char* __restrict__ src, size_t n16,
+                                                         uint32_t* __restrict__ sink) {
+    u32x4 acc = {0u, 0u, 0u, 0u};
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n16; i += (size_t)gridDim.x * kThreads)
+        acc ^= ld_stream16(src + i * 16);
+    uint32_t v = acc[0] ^ acc[1] ^ acc[2] ^ acc[3];
+    for (int o = 32; o > 0; o >>= 1) v ^= (uint32_t)__shfl_xor((int)v, o, 64);
+    __shared__ uint32_t part[kThreads / 64];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) sink[blockIdx.x] = part[0] ^ part[1] ^ part[2] ^ part[3];
+}
+
 unsigned grid_for(size_t n) {
     size_t g = (n + kThreads - 1) / kThreads;
     if (g < 1) g = 1;
@@ -1850,6 +1866,12 @@ hipError_t launch_fill_random(void* dst, size_t bytes, uint64_t seed, hipStream_
     if (bytes == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_random, dim3(grid_for(bytes / 8 + 1)), dim3(kThreads), 0, s,
                        static_cast<unsigned char*>(dst), bytes, seed);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_read(const void* src, size_t bytes, uint32_t* sink, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_stream_read, dim3((unsigned)blocks), dim3(kThreads), 0, s,
+                       static_cast<const unsigned char*>(src), bytes / 16, sink);
     return hipGetLastError();
 }
 

@@ -289,5 +289,8 @@ hipError_t launch_affine(const AffineJob& job, hipStream_t s);
 // Zoom (structural/zoom.rs:364-400): output (x, y) repeats input (x / zx, y / zy).
 hipError_t launch_zoom(const InView& in, const OutView& out, int bands, int fmt, int zx, int zy, hipStream_t s);
 hipError_t launch_fill_random(void* dst, size_t bytes, uint64_t seed, hipStream_t s);
+// Streaming read of bytes (a multiple of 16) by `blocks` blocks; sink holds
+// one word per block.
+hipError_t launch_stream_read(const void* src, size_t bytes, uint32_t* sink, int blocks, hipStream_t s);
 
 }  // namespace vgpu

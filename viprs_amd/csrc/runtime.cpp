@@ -1662,6 +1662,42 @@ int vgpu_fill_random_u8(vgpu_context* c, void* dst, size_t bytes, uint64_t seed)
     return from_hip(launch_fill_random(dst, bytes, seed, c->stream));
 }
 
+int vgpu_measure_stream(vgpu_context* c, size_t bytes, int iterations, float* read_ms, float* copy_ms) {
+    if (!c || bytes < 16 || iterations < 1 || !read_ms || !copy_ms) return VGPU_E_INVALID_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    bytes &= ~(size_t)15;
+    constexpr int kBlocks = 256 * 16;  // a persistent grid of 16 blocks per CU
+    unsigned char* a = static_cast<unsigned char*>(c->pool.acquire(bytes));
+    unsigned char* b = static_cast<unsigned char*>(c->pool.acquire(bytes));
+    uint32_t* sink = static_cast<uint32_t*>(c->pool.acquire(kBlocks * sizeof(uint32_t)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = VGPU_OK;
+    if (!a || !b || !sink) rc = VGPU_E_NO_MEMORY;
+    else if (!create_event_pair(&e0, &e1)) rc = VGPU_E_DEVICE;
+    auto time = [&](auto launch, float* ms) {
+        if (rc != VGPU_OK) return;
+        hipError_t e = launch();  // warm
+        e = e == hipSuccess ? hipEventRecord(e0, c->stream) : e;
+        for (int i = 0; i < iterations && e == hipSuccess; ++i) e = launch();
+        e = e == hipSuccess ? hipEventRecord(e1, c->stream) : e;
+        e = e == hipSuccess ? hipEventSynchronize(e1) : e;
+        e = e == hipSuccess ? hipEventElapsedTime(ms, e0, e1) : e;
+        if (e != hipSuccess) rc = VGPU_E_DEVICE;
+        else *ms /= (float)iterations;
+    };
+    if (rc == VGPU_OK) rc = from_hip(launch_fill_random(a, bytes, 1, c->stream));
+    time([&]() { return launch_stream_read(a, bytes, sink, kBlocks, c->stream); }, read_ms);
+    time([&]() { return hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, c->stream); }, copy_ms);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(c->stream);
+    c->pool.release(a);
+    c->pool.release(b);
+    c->pool.release(sink);
+    if (rc == VGPU_E_DEVICE) c->sticky = rc;
+    return rc;
+}
+
 int vgpu_pipeline_new(vgpu_context* ctx, uint32_t w, uint32_t h, uint32_t bands, int fmt,
                       vgpu_pipeline** out) {
     if (!ctx || !out) return VGPU_E_INVALID_ARGUMENT;
