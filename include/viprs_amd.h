@@ -236,9 +236,10 @@ typedef enum {
     VGPU_OPT_SEP_ROWS_PER_TASK = 2, /* output rows per MFMA wavefront task: 0 = auto, else a multiple of 16 */
     VGPU_OPT_SHRINK_RUN = 3,        /* box-shrink output rows per wavefront run: 0 = auto, 1..64 */
     VGPU_OPT_GAUSS_ROWS = 4,        /* u8 3/4-band Gaussian VALU tile rows: 0 = auto, 16 or 32; -1 = the MFMA kernel */
-    VGPU_OPT_BATCH_REDUCE_CHAIN = 5, /* batch only: 0 (default) = every image's stages on its own group's
-                                        stream; 1 = reduce stages on a stream of their own, each launch after
-                                        the shrink launch that wrote its inputs */
+    VGPU_OPT_BATCH_REDUCE_CHAIN = 5, /* batch only, the launch-chain layout: 2 (default) = each reduce launch
+                                        right after the shrink launch that wrote its inputs on the same stream,
+                                        shrink launches with the most images first; 0 = level by level on each
+                                        group's stream; 1 = reduce stages on a stream of their own */
     VGPU_OPT_SEP_FORM = 6           /* int8 MFMA pairs: 0 = block tiles staged in LDS where they fit (default),
                                        1 = one wavefront per strip task streaming its operands */
 } vgpu_option;

@@ -83,7 +83,7 @@ def test_batch_launch_shapes_and_replays(ctx):
             check_outputs(dst, exp, f"grid {grid_cap}/{rgrid} graph {graph} tile {tile} replay {rep}")
 
 
-@pytest.mark.parametrize("reduce_chain", [1, 0])
+@pytest.mark.parametrize("reduce_chain", [2, 1, 0])
 @pytest.mark.parametrize("form", [0, 1])
 def test_batch_chain_layouts_keep_results(ctx, reduce_chain, form):
     # Reduce stages on a chain of their own (each launch after the shrink
@@ -105,7 +105,7 @@ def test_batch_chain_layouts_keep_results(ctx, reduce_chain, form):
         ctx.synchronize()
         check_outputs(dst, exp, f"reduce chain {reduce_chain} form {form} replay {rep}")
     with pytest.raises(v.VgpuError):
-        batch.set_option(v.api.OPT_BATCH_REDUCE_CHAIN, 2)
+        batch.set_option(v.api.OPT_BATCH_REDUCE_CHAIN, 3)
 
 
 def test_batch_outputs_equal_single_pipelines(ctx):

@@ -52,6 +52,8 @@ struct Launch {
     int chain = -1;          // the group of its images (one stream each), or -1 for mixed
     int wait_for = -1;       // launch (on another chain) whose outputs it reads, or -1
     bool signal = false;     // a later launch waits for this one
+    int after = -1;          // interleaved layout: the launch whose outputs it reads (planning only)
+    int images = 0;          // images its jobs belong to
 };
 
 // Four launch chains, as many as the device's hardware queues, each on its
@@ -80,7 +82,7 @@ struct vgpu_batch {
     std::vector<Launch> launches;
     std::vector<int> fallback;  // images run stage by stage
     int n_shrink_tasks = 0, n_reduce_tasks = 0;
-    int reduce_chain = 0;     // 1: reduce stages on their own chain after their producers; 0: on the image's chain
+    int reduce_chain = 2;     // chain layout: 0 level by level, 1 a reduce chain, 2 interleaved (see plan())
     int grid_cap = 0;         // persistent grid size of the shrink launches; 0 = one block per task
     int reduce_grid_cap = 0;  // the same for the reduce and MFMA separable launches
     int use_graph = 1;
@@ -225,8 +227,14 @@ int vgpu_batch::plan() {
     // those reduce launches only once the shrink launches, whose persistent
     // grids hold every CU slot, have drained, so the option is off unless
     // the tuner finds it faster.)
-    const int shrink_groups = reduce_chain ? kGroups - 1 : kGroups;
-    const int rchain = reduce_chain ? kGroups - 1 : -1;
+    const int shrink_groups = reduce_chain == 1 ? kGroups - 1 : kGroups;
+    const int rchain = reduce_chain == 1 ? kGroups - 1 : -1;
+    // Interleaved (reduce_chain == 2): each reduce launch follows, on the
+    // same stream, the shrink launch that wrote its inputs, and a chain's
+    // shrink launches run those with the most images first, so the reduce
+    // work of all but a chain's last shrink launch runs while the other
+    // chains still stream their shrinks.
+    const bool interleave = reduce_chain == 2;
     std::vector<int> group_of(count, 0);
     {
         uint64_t load[kGroups] = {};
@@ -282,7 +290,7 @@ int vgpu_batch::plan() {
             const int wait = producer[(size_t)idx];
             const int rkey_chain = rchain >= 0 ? rchain : grp;
             const int rkey_wait = rchain >= 0 ? wait : -1;
-            const int rkey_order = rchain >= 0 ? wait + 1 : 0;
+            const int rkey_order = rchain >= 0 || interleave ? wait + 1 : 0;
             if (s.kind == ST_SHRINK_HV_U8) {
                 ShrinkJob j = shrink_job_of(s, im.nodes[(size_t)s.first], in, out);
                 j.src_slot = first ? idx : -1;
@@ -352,12 +360,63 @@ int vgpu_batch::plan() {
             l.ntasks = (int)bk.tasks.size();
             l.lds = bk.lds;
             l.alg_bytes = bk.bytes;
+            l.after = interleave && l.chain >= 0 ? std::get<0>(kv.first) - 1 : -1;
+            l.images = (int)bk.images.size();
             const int li = (int)launches.size();
             for (int idx : bk.images) producer[(size_t)idx] = li;
             if (l.wait_for >= 0) launches[(size_t)l.wait_for].signal = true;
             launches.push_back(l);
             (l.family == 0 ? n_shrink_tasks : n_reduce_tasks) += l.ntasks;
         }
+    }
+    if (interleave) {
+        // Launch order: level-0 launches by chain, most images first, each
+        // followed by the launches that read its outputs (recursively).  No
+        // launch of this layout waits on an event, so only the order within
+        // a chain matters.
+        std::vector<std::vector<int>> children(launches.size());
+        std::vector<int> roots;
+        for (int li = 0; li < (int)launches.size(); ++li) {
+            const int a = launches[(size_t)li].after;
+            if (a >= 0) children[(size_t)a].push_back(li);
+            else roots.push_back(li);
+        }
+        // Launches over every chain (chain -1: joins them) stay last, in
+        // their planned order.
+        std::stable_sort(roots.begin(), roots.end(), [&](int a, int b) {
+            const Launch& x = launches[(size_t)a];
+            const Launch& y = launches[(size_t)b];
+            if ((x.chain < 0) != (y.chain < 0)) return y.chain < 0;
+            if (x.chain < 0) return false;
+            if (x.chain != y.chain) return x.chain < y.chain;
+            if ((x.family == 0) != (y.family == 0)) return x.family != 0;  // lone reduces (no shrink) first
+            return x.images > y.images;
+        });
+        std::vector<Launch> ordered;
+        std::vector<int> stack;
+        for (auto it = roots.rbegin(); it != roots.rend(); ++it) stack.push_back(*it);
+        while (!stack.empty()) {
+            const int li = stack.back();
+            stack.pop_back();
+            ordered.push_back(launches[(size_t)li]);
+            for (auto it = children[(size_t)li].rbegin(); it != children[(size_t)li].rend(); ++it) stack.push_back(*it);
+        }
+        // Issue the chains' launches round robin (each chain keeps its own
+        // order): the captured graph then starts every chain at once.
+        std::vector<std::vector<Launch>> per(kChains);
+        std::vector<Launch> joins;
+        for (const Launch& l : ordered) (l.chain >= 0 ? per[(size_t)l.chain] : joins).push_back(l);
+        launches.clear();
+        for (size_t k = 0;; ++k) {
+            bool any = false;
+            for (int c = 0; c < kChains; ++c)
+                if (k < per[(size_t)c].size()) {
+                    launches.push_back(per[(size_t)c][k]);
+                    any = true;
+                }
+            if (!any) break;
+        }
+        for (const Launch& l : joins) launches.push_back(l);
     }
     chains = 0;
     for (const Launch& l : launches)
@@ -596,7 +655,7 @@ int vgpu_batch_set_option(vgpu_batch* b, int option, int value) {
         b->opt.gauss_rows = value;
         break;
     case VGPU_OPT_BATCH_REDUCE_CHAIN:
-        if (value != 0 && value != 1) return VGPU_E_INVALID_ARGUMENT;
+        if (value < 0 || value > 2) return VGPU_E_INVALID_ARGUMENT;
         b->reduce_chain = value;
         break;
     case VGPU_OPT_SEP_FORM:
@@ -739,7 +798,7 @@ int vgpu_batch_tune(vgpu_batch* b, const void* const* srcs, const size_t* src_by
     // per task, or 4/8/16 blocks per CU walking the list), all launched from
     // a graph.  The VALU reduce tile only matters when the batch has VALU
     // reduce launches.
-    static const int shares[] = {0, 1};  // reduce chain off / on
+    static const int shares[] = {2, 0};  // chain layouts: interleaved, level by level
     static const int tiles[] = {0, 1, 2, 3};
     static const int sgrids[] = {0, 256 * 8, 256 * 16};
     static const int rgrids[] = {0, 256 * 4, 256 * 8, 256 * 16};
