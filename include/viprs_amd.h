@@ -240,8 +240,11 @@ typedef enum {
                                         right after the shrink launch that wrote its inputs on the same stream,
                                         shrink launches with the most images first; 0 = level by level on each
                                         group's stream; 1 = reduce stages on a stream of their own */
-    VGPU_OPT_SEP_FORM = 6           /* int8 MFMA pairs: 0 = block tiles staged in LDS where they fit (default),
+    VGPU_OPT_SEP_FORM = 6,          /* int8 MFMA pairs: 0 = block tiles staged in LDS where they fit (default),
                                        1 = one wavefront per strip task streaming its operands */
+    VGPU_OPT_REDUCE_FORM = 7        /* f32 ReduceH -> ReduceV pairs (FP64 order and FP32): 0 = blocks walking
+                                       down strips of output columns where the strip fits (default), 1 = one
+                                       block per output tile */
 } vgpu_option;
 int vgpu_pipeline_set_option(vgpu_pipeline *p, int option, int value);
 /* CompiledPipeline output geometry (pipeline/compiled.rs:338-). */

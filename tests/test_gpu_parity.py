@@ -353,6 +353,30 @@ def test_f32_reference_exact(ctx):
             assert_same(got, exp, f"f32 bands {b} {steps}")
 
 
+@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("bands", [1, 2, 3, 4])
+def test_f32_reduce_forms_exact(ctx, form, bands):
+    # Both forms of the fused f32 reduce -- blocks walking down strips of
+    # output columns (0) and one block per output tile (1) -- are
+    # bit-identical to the oracle in the reference's order: ragged strips and
+    # row runs, windows clamped at the top and bottom, one phase (2.0), two
+    # (2.5) and many; the plain FP32 mode gives the same bits in both forms.
+    from viprs_amd.api import OPT_REDUCE_FORM
+    for (w, h), steps in (((613, 389), [("reduce_h", (2.5, v.LANCZOS3)), ("reduce_v", (2.5, v.LANCZOS3))]),
+                          ((613, 389), [("reduce_h", (3.7, v.LANCZOS3)), ("reduce_v", (1.3, v.BICUBIC))]),
+                          ((47, 1201), [("reduce_h", (1.1, v.BILINEAR)), ("reduce_v", (6.2, v.LANCZOS2))]),
+                          ((1733, 29), [("reduce_h", (2.0, v.LANCZOS3)), ("reduce_v", (2.0, v.LANCZOS3))]),
+                          ((9, 7), [("reduce_h", (1.5, v.LANCZOS3)), ("reduce_v", (1.5, v.LANCZOS3))])):
+        src = float_image(w, h, bands, 70 + bands + w)
+        exp = build(O.Pipeline(w, h, bands, v.F32), steps).run(src)
+        got = build(v.Pipeline(ctx, w, h, bands, v.F32), steps).set_option(OPT_REDUCE_FORM, form).run_host(src)
+        assert_same(got, exp, f"form {form} bands {bands} {w}x{h} {steps}")
+        f32 = [build(v.Pipeline(ctx, w, h, bands, v.F32), steps).set_float_mode(v.Pipeline.FLOAT_FP32)
+               .set_option(OPT_REDUCE_FORM, f).run_host(src) for f in (0, 1)]
+        assert_same(f32[0], f32[1], f"fp32 forms bands {bands} {w}x{h} {steps}")
+        assert np.abs(f32[0].astype(np.float64) - exp).max() <= 2e-6
+
+
 @pytest.mark.parametrize("fmt", [np.uint8, np.uint16, np.float32])
 @pytest.mark.parametrize("bands", [1, 2, 3, 4, 5])
 def test_reduce_stage_forms(ctx, fmt, bands):

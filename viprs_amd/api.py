@@ -39,6 +39,7 @@ E_NOT_READY = -10
 E_SCRATCH_LIMIT = -11
 NOALLOC_SCRATCH_BYTES = 64 << 20
 OPT_SEP_ROWS_PER_TASK, OPT_SHRINK_RUN, OPT_GAUSS_ROWS, OPT_BATCH_REDUCE_CHAIN, OPT_SEP_FORM = 2, 3, 4, 5, 6
+OPT_REDUCE_FORM = 7
 
 NP_DTYPE = {U8: np.uint8, U16: np.uint16, F32: np.float32, I16: np.int16, U32: np.uint32, I32: np.int32,
             F64: np.float64}

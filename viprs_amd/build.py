@@ -18,7 +18,7 @@ CSRC = HERE / "csrc"
 LIB = HERE / "libviprs_amd.so"
 INCLUDE = HERE.parent / "include"
 
-SOURCES = ["kernels.hip", "sep_mfma.hip", "float_paths.hip", "gauss_u8.hip", "affine.hip", "conv_pair.hip", "jpeg.hip",
+SOURCES = ["kernels.hip", "sep_mfma.hip", "float_paths.hip", "gauss_u8.hip", "affine.hip", "conv_pair.hip", "jpeg.hip", "reduce_stream.hip",
            "plan.cpp", "runtime.cpp", "batch.cpp", "server.cpp", "codec.cpp"]
 
 # -ffp-contract=off: the reference-exact f32 kernels and the host planner rely

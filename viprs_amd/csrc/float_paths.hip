@@ -353,6 +353,7 @@ hipError_t launch_f16_job(const F16Job& job, hipStream_t s) {
 
 hipError_t launch_red_f32_job(const ReduceJob& job, hipStream_t s) {
     if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;
+    if (job.stream) return launch_reduce_stream(job, 2, s);
     const dim3 grid((unsigned)job.gx, (unsigned)job.gy);
     const size_t lds = reduce_hbuf_bytes(job, sizeof(float)) + (size_t)job.hr * job.span_max * job.bands * sizeof(float);
     switch (job.bands) {

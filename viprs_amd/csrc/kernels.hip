@@ -1804,6 +1804,7 @@ size_t reduce_job_lds(const ReduceJob& job) {
 hipError_t launch_reduce_job(const ReduceJob& job, hipStream_t s) {
     if (job.out.w <= 0 || job.out.h <= 0) return hipSuccess;
     if (job.bands < 1 || job.bands > 4) return hipErrorInvalidValue;
+    if (job.stream && job.fmt == 2 && !job.op) return launch_reduce_stream(job, 0, s);
     hipLaunchKernelGGL(k_reduce_job, dim3((unsigned)job.gx, (unsigned)job.gy), dim3(kThreads),
                        reduce_job_lds(job), s, job);
     return hipGetLastError();

@@ -109,6 +109,12 @@ struct ReduceJob {
     // H taps read, hr = source rows staged in LDS per chunk, rb = H rows a
     // thread of the f32 path evaluates at once (hr = rb * 256 / tx).
     int span_max = 1, hr = 1, rb = 1;
+    // Streaming form of the f32 pair (stream = 1): strips of st_tx output
+    // columns whose H taps read at most st_span source columns, runs of
+    // st_run output rows, at most st_open output rows whose V windows are
+    // open at once, st_gx x st_gy blocks, st_pad >= the most intermediate
+    // rows a run reads.
+    int stream = 0, st_tx = 0, st_span = 0, st_run = 0, st_open = 0, st_gx = 0, st_gy = 0, st_pad = 0;
 };
 
 // Separable u8 pass pair on MFMA (int8 limbs): H then V with a u8
@@ -209,6 +215,10 @@ hipError_t launch_sep_tile_batch(int variant, const SepJob* jobs, const BatchTas
 // Float separable pair in plain FP32: f32 weights, FMA accumulation, f32
 // intermediate (ReduceJob tiling; ph / pv carry wf32).
 hipError_t launch_red_f32_job(const ReduceJob& job, hipStream_t s);
+// Streaming f32 reduce pair (job.stream): mode 0 = the reference's FP64
+// order, 2 = plain FP32.  reduce_stream_lds = its dynamic LDS bytes.
+hipError_t launch_reduce_stream(const ReduceJob& job, int mode, hipStream_t s);
+size_t reduce_stream_lds(const ReduceJob& job, int mode);
 
 // Float separable pair on MFMA (v_mfma_f32_16x16x32_f16) with an FP16 hi/lo
 // split of pixels and weights (x = hi + lo, products hi*hi + hi*lo + lo*hi)

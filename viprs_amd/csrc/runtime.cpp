@@ -577,6 +577,62 @@ void set_reduce_staging(ReduceJob* j, const std::vector<int32_t>& hs, int htaps,
     }
 }
 
+// Streaming form of an f32 reduce pair: the widest strip (at most 256 / B
+// output columns, at least 128 samples) whose H span stays within 1024
+// staged samples a row, and runs of output rows sized so that the blocks fill
+// the chip once (each run re-reads the taps_v - stride halo rows).  Leaves
+// job.stream = 0 where no strip fits.
+void plan_reduce_stream(ReduceJob* j, const std::vector<int32_t>& hs, const std::vector<int32_t>& vs, int htaps,
+                        int vtaps, const Rect& out) {
+    j->stream = 0;
+    const int B = j->bands;
+    if (B < 1 || B > 4 || out.w < 1 || out.h < 1) return;
+    int tx = (int)std::min<int64_t>(256 / B, out.w);
+    int64_t span = 0;
+    for (; tx >= 1; --tx) {
+        span = reduce_span(hs, htaps, out.w, tx);
+        if (span * B <= 1024) break;
+    }
+    if (tx < 1 || ((int64_t)tx * B < 128 && tx < out.w)) return;
+    // Output rows whose clamped V windows hold a row at once (the kernel
+    // keeps one accumulator per open output: 8 or 16).
+    const int64_t c0 = j->vclip0, c1 = j->vclip1;
+    int open = 0;
+    for (int64_t y = 0, y1 = 0; y < out.h; ++y) {
+        const int64_t lo = std::min(std::max<int64_t>(vs[(size_t)y], c0), c1);
+        while (y1 < y && std::min(std::max<int64_t>((int64_t)vs[(size_t)y1] + vtaps - 1, c0), c1) < lo) ++y1;
+        open = std::max(open, (int)(y - y1 + 1));
+    }
+    if (open > 16) return;
+    ReduceJob t = *j;
+    t.st_tx = tx;
+    t.st_span = (int)span;
+    t.st_open = open;
+    t.st_gx = (int)((out.w + tx - 1) / tx);
+    // One round of resident blocks (LDS-limited, at most 4 a CU) where the
+    // runs stay within 128 rows: a second, partly filled round would idle
+    // most of the chip for a whole block lifetime.
+    t.stream = 0;
+    for (int64_t per_cu = 4; per_cu >= 1 && !t.stream; --per_cu) {
+        const int64_t gy = std::max<int64_t>(1, 256 * per_cu / t.st_gx);
+        t.st_run = (int)std::min<int64_t>(128, (out.h + gy - 1) / gy);
+        // Intermediate rows of the widest run, plus one.
+        int64_t rows = 1;
+        for (int64_t y0 = 0; y0 < out.h; y0 += t.st_run) {
+            const int64_t y1 = std::min<int64_t>(y0 + t.st_run, out.h) - 1;
+            const int64_t lo = std::min(std::max<int64_t>(vs[(size_t)y0], c0), c1);
+            const int64_t hi = std::min(std::max<int64_t>((int64_t)vs[(size_t)y1] + vtaps - 1, c0), c1);
+            rows = std::max(rows, hi - lo + 1);
+        }
+        t.st_pad = (int)rows + 1;
+        const size_t lds = reduce_stream_lds(t, 0);
+        if (lds <= 64 * 1024 && lds * (size_t)per_cu <= 160 * 1024) t.stream = 1;
+    }
+    if (!t.stream) return;
+    t.st_gy = (int)((out.h + t.st_run - 1) / t.st_run);
+    *j = t;
+}
+
 // f32 copy of a weight table.
 size_t put_f32(const std::vector<double>& w, Staging* st) {
     std::vector<float> f(w.size());
@@ -799,6 +855,11 @@ bool stage_reduce_hv_params(const ReduceAxis& hax, bool accumulate_h, const Redu
         }
     j.wide_acc = worst >= (int64_t)INT32_MAX - 4096;
     j.op = reduce_op ? 1 : 0;
+    if (fmt == F32 && !reduce_op && opt.reduce_form == 0) {
+        j.ph.taps = hax.taps;  // the planner's LDS estimate reads the tap counts
+        j.pv.taps = vax.taps;
+        plan_reduce_stream(&j, hs, vs, hax.taps, vax.taps, out);
+    }
     if (fmt == F32 && opt.float_mode >= 1 && tile_ok && !reduce_op) {
         s->w32 = put_f32(hax.wf64, st);
         s->v_w32 = put_f32(vax.wf64, st);
@@ -2078,6 +2139,11 @@ int vgpu_pipeline_set_option(vgpu_pipeline* p, int option, int value) {
         if (value != 0 && value != 1) return VGPU_E_INVALID_ARGUMENT;
         p->release();
         p->opt.sep_form = value;
+        return VGPU_OK;
+    case VGPU_OPT_REDUCE_FORM:
+        if (value != 0 && value != 1) return VGPU_E_INVALID_ARGUMENT;
+        p->release();
+        p->opt.reduce_form = value;
         return VGPU_OK;
     default: return VGPU_E_INVALID_ARGUMENT;
     }

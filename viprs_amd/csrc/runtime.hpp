@@ -194,6 +194,7 @@ struct StageOptions {
     int shrink_run = 0;                // shrink output rows per wavefront run (0 = planner's choice)
     int gauss_rows = 0;                // u8 3/4-band Gaussian pair: VALU tile rows (0 = auto, 16, 32), -1 = MFMA kernel
     int sep_form = 0;                  // int8 MFMA pairs: 0 = block-tile form where it fits, 1 = wavefront tasks
+    int reduce_form = 0;               // f32 reduce pairs: 0 = streaming strips where they fit, 1 = tiles
 };
 
 Rect whole(uint32_t w, uint32_t h);
