@@ -360,12 +360,14 @@ def test_f32_reduce_forms_exact(ctx, form, bands):
     # output columns (0) and one block per output tile (1) -- are
     # bit-identical to the oracle in the reference's order: ragged strips and
     # row runs, windows clamped at the top and bottom, one phase (2.0), two
-    # (2.5) and many; the plain FP32 mode gives the same bits in both forms.
+    # (2.5) and many, H taps held in registers (<= 24) and in the LDS table
+    # (31 at 5.0); the plain FP32 mode gives the same bits in both forms.
     from viprs_amd.api import OPT_REDUCE_FORM
     for (w, h), steps in (((613, 389), [("reduce_h", (2.5, v.LANCZOS3)), ("reduce_v", (2.5, v.LANCZOS3))]),
                           ((613, 389), [("reduce_h", (3.7, v.LANCZOS3)), ("reduce_v", (1.3, v.BICUBIC))]),
                           ((47, 1201), [("reduce_h", (1.1, v.BILINEAR)), ("reduce_v", (6.2, v.LANCZOS2))]),
                           ((1733, 29), [("reduce_h", (2.0, v.LANCZOS3)), ("reduce_v", (2.0, v.LANCZOS3))]),
+                          ((1211, 301), [("reduce_h", (5.0, v.LANCZOS3)), ("reduce_v", (2.2, v.LANCZOS3))]),
                           ((9, 7), [("reduce_h", (1.5, v.LANCZOS3)), ("reduce_v", (1.5, v.LANCZOS3))])):
         src = float_image(w, h, bands, 70 + bands + w)
         exp = build(O.Pipeline(w, h, bands, v.F32), steps).run(src)

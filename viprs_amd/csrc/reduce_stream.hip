@@ -45,6 +45,7 @@ namespace {
 constexpr int kStreamRows = 4;  // intermediate rows per block step
 constexpr int kStreamQ = 4;     // staged samples per row at most kStreamQ * 256 (span * B <= 1024)
 constexpr int kPhaseRows = 65;  // rows of a reduce weight table (phases 0 .. 64)
+constexpr int kHRegTaps = 24;   // H taps held in registers (more: the 65-phase table in LDS)
 
 template <int MODE>
 using StreamAcc = typename std::conditional<MODE == 2, float, double>::type;
@@ -53,11 +54,12 @@ using StreamAcc = typename std::conditional<MODE == 2, float, double>::type;
 // rounded to 16.
 __host__ __device__ inline size_t stream_stage_offset(const ReduceJob& job, int mode) {
     const size_t wsz = mode == 2 ? sizeof(float) : sizeof(double);
-    return ((size_t)kPhaseRows * job.ph.taps * wsz + ((size_t)job.st_run * job.pv.taps + job.st_pad) * wsz +
+    const size_t htab = job.ph.taps <= kHRegTaps ? 0 : (size_t)kPhaseRows * job.ph.taps * wsz;
+    return (htab + ((size_t)job.st_run * job.pv.taps + job.st_pad) * wsz +
             (size_t)job.st_run * 3 * sizeof(int) + 15) & ~(size_t)15;
 }
 
-template <int B, int MODE, int P>
+template <int B, int MODE, int P, int HREG>
 __global__ __launch_bounds__(kThreads) void k_reduce_stream(ReduceJob job) {
     using W = StreamAcc<MODE>;
     constexpr int G = kStreamRows;
@@ -78,8 +80,8 @@ __global__ __launch_bounds__(kThreads) void k_reduce_stream(ReduceJob job) {
     // st_pad zeros), the run's V starts and clamped first / last rows, the
     // two staging buffers.
     const int vtab = nrows * vT;
-    W* wl_h = reinterpret_cast<W*>(smem);
-    W* wl_v = wl_h + kPhaseRows * hT;
+    W* wl_h = reinterpret_cast<W*>(smem);  // HREG: no table (the thread's H weights live in registers)
+    W* wl_v = wl_h + (HREG ? 0 : kPhaseRows * hT);
     int* vstart = reinterpret_cast<int*>(wl_v + run * vT + job.st_pad);
     int* vlo = vstart + run;
     int* vhi = vlo + run;
@@ -94,7 +96,8 @@ __global__ __launch_bounds__(kThreads) void k_reduce_stream(ReduceJob job) {
         tab_v = pv.wf64;
     }
     const int vclip0 = job.vclip0, vclip1 = job.vclip1;
-    for (int i = threadIdx.x; i < kPhaseRows * hT; i += kThreads) wl_h[i] = tab_h[i];
+    if constexpr (!HREG)
+        for (int i = threadIdx.x; i < kPhaseRows * hT; i += kThreads) wl_h[i] = tab_h[i];
     for (int i = threadIdx.x; i < vtab + job.st_pad; i += kThreads) {
         W w = 0;
         if (i < vtab) {
@@ -119,7 +122,13 @@ __global__ __launch_bounds__(kThreads) void k_reduce_stream(ReduceJob job) {
     const bool active = s < ncols * B;
     const int xs = min(s / B, ncols - 1);
     const int hoff = (ph.start[tx0 + xs] - c0) * B + (s - (s / B) * B);
-    const W* wh = wl_h + (int)ph.phase[tx0 + xs] * hT;
+    const int hph = ph.phase[tx0 + xs];
+    const W* wh = wl_h + hph * hT;
+    W whr[HREG > 0 ? HREG : 1];  // HREG: this thread's H weights (taps < hT)
+    if constexpr (HREG > 0) {
+#pragma unroll
+        for (int t = 0; t < HREG; ++t) whr[t] = t < hT ? tab_h[(size_t)hph * hT + t] : W(0);
+    }
     const size_t row_samples = (size_t)in.bw * B;
     const int flat = tx0 * B + s;  // the thread's sample index within an output row
     const int vec_len = out.w * B / 4 * 4;
@@ -263,7 +272,51 @@ __global__ __launch_bounds__(kThreads) void k_reduce_stream(ReduceJob job) {
 #pragma unroll
         for (int k = 0; k < G; ++k) acc[k] = 0;
         const float* px = cur + (active ? hoff : 0);
-        int t = 0;
+        if constexpr (HREG > 0) {
+            // Groups of four taps (all their LDS reads first), the weights
+            // from registers; a last partial group tap by tap.
+#pragma unroll
+            for (int t4 = 0; t4 < HREG; t4 += 4) {
+                if (t4 + 4 <= hT) {
+                    float xv[4][G];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int k = 0; k < G; ++k) xv[u][k] = px[k * srow + (t4 + u) * B];
+                    if constexpr (MODE == 2) {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+#pragma unroll
+                            for (int k = 0; k < G; ++k) acc[k] = fmaf(whr[t4 + u], xv[u][k], acc[k]);
+                    } else if constexpr (B == 1) {
+#pragma unroll
+                        for (int k = 0; k < G; ++k) {
+                            double gsum = 0.0;
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) gsum = gsum + (double)xv[u][k] * whr[t4 + u];
+                            acc[k] = acc[k] + gsum;
+                        }
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+#pragma unroll
+                            for (int k = 0; k < G; ++k) acc[k] = fma((double)xv[u][k], whr[t4 + u], acc[k]);
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (t4 + u < hT) {
+#pragma unroll
+                            for (int k = 0; k < G; ++k) {
+                                const float xx = px[k * srow + (t4 + u) * B];
+                                if constexpr (MODE == 2) acc[k] = fmaf(whr[t4 + u], xx, acc[k]);
+                                else acc[k] = fma((double)xx, whr[t4 + u], acc[k]);
+                            }
+                        }
+                }
+            }
+        }
+        int t = HREG > 0 ? hT : 0;
         for (; t + 8 <= hT; t += 8) {
             float xv[8][G];
             W wv[8];
@@ -295,7 +348,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce_stream(ReduceJob job) {
                     for (int k = 0; k < G; ++k) acc[k] = fma((double)xv[u][k], wv[u], acc[k]);
             }
         }
-        if constexpr (MODE == 0 && B == 1) {
+        if constexpr (MODE == 0 && B == 1 && HREG == 0) {
             // A last group of four (AVX2 grouping) before the FMA tail.
             if (t + 4 <= hT) {
 #pragma unroll
@@ -336,9 +389,12 @@ hipError_t launch_reduce_stream(const ReduceJob& job, int mode, hipStream_t s) {
     const dim3 grid((unsigned)job.st_gx, (unsigned)job.st_gy);
     const size_t lds = reduce_stream_lds(job, mode);
     const bool p8 = job.st_open <= 8;
-#define VGPU_RSQ(BB, MM)                                                                                \
-    if (p8) hipLaunchKernelGGL((k_reduce_stream<BB, MM, 8>), grid, dim3(kThreads), lds, s, job);       \
-    else hipLaunchKernelGGL((k_reduce_stream<BB, MM, 16>), grid, dim3(kThreads), lds, s, job);
+    const bool hreg = job.ph.taps <= kHRegTaps;
+#define VGPU_RSQ(BB, MM)                                                                                       \
+    if (p8 && hreg) hipLaunchKernelGGL((k_reduce_stream<BB, MM, 8, kHRegTaps>), grid, dim3(kThreads), lds, s, job); \
+    else if (p8) hipLaunchKernelGGL((k_reduce_stream<BB, MM, 8, 0>), grid, dim3(kThreads), lds, s, job);            \
+    else if (hreg) hipLaunchKernelGGL((k_reduce_stream<BB, MM, 16, kHRegTaps>), grid, dim3(kThreads), lds, s, job); \
+    else hipLaunchKernelGGL((k_reduce_stream<BB, MM, 16, 0>), grid, dim3(kThreads), lds, s, job);
 #define VGPU_RS(BB)          \
     if (mode == 2) {         \
         VGPU_RSQ(BB, 2)      \
