@@ -168,6 +168,16 @@ __host__ __device__ inline size_t reduce_hbuf_bytes(const ReduceJob& job, size_t
     return ((size_t)job.nr_max * job.tx * job.bands * esz + 15) & ~(size_t)15;
 }
 
+// XCD-aware block order: the dispatcher places block b on XCD b % 8, each
+// XCD with its own L2.  Renumbering so that every XCD runs one contiguous
+// range of logical blocks keeps blocks that share source lines (neighbouring
+// strips) on one L2.  A bijection on 0 .. nb - 1.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    constexpr int kXcds = 8;
+    const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
 // LDS hand-off between the lanes of one wavefront: LDS operations of a
 // wavefront execute in issue order, so a compiler barrier suffices.
 __device__ __forceinline__ void wave_lds_sync() {

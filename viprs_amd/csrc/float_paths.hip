@@ -319,7 +319,8 @@ __device__ void f16_task(const F16Job& job, int jx, int ty, unsigned char* lds, 
 template <int B, int MAXH, bool PF>
 __global__ __launch_bounds__(kThreads) void k_f16_job(F16Job job, int wave_lds) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int t = blockIdx.x * (kThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // Neighbouring strips overlap by their tap halo: keep them on one XCD.
+    const int t = xcd_block(blockIdx.x, gridDim.x) * (kThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (t >= job.gx * job.gy) return;
     f16_task<B, MAXH, PF>(job, t % job.gx, t / job.gx, lds, (size_t)wave_lds);
 }
