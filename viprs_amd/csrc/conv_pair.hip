@@ -101,10 +101,14 @@ __global__ __launch_bounds__(kThreads) void k_conv_pair(ConvPairJob job) {
     const int RS = n + RB - 1;
     const int span = (TW + 2 * r) * B;      // staged source samples per row
     const int s = threadIdx.x;              // this thread's sample in the strip
-    const int x0 = out.x + blockIdx.x * TW;  // first output column of the strip
+    // Neighbouring strips share their 2r halo columns: XCD-aware order keeps
+    // them on one L2 (a contiguous range of strips per XCD).
+    const int lb = xcd_block((int)(blockIdx.y * gridDim.x + blockIdx.x), (int)(gridDim.x * gridDim.y));
+    const int bx = lb % (int)gridDim.x, by = lb / (int)gridDim.x;
+    const int x0 = out.x + bx * TW;  // first output column of the strip
     const int ncols = min(TW, out.x + out.w - x0);
     const bool active = s < ncols * B;
-    const int yo0 = out.y + blockIdx.y * job.run;
+    const int yo0 = out.y + by * job.run;
     const int nrows = min(job.run, out.y + out.h - yo0);
     if (nrows <= 0) return;
     // LDS: the H and V weights (read per tap as broadcasts; from global
