@@ -53,6 +53,22 @@ def test_headline_thumbnail_8192x8192x3(ctx):
     _same(out2.numpy(), exp, "thumbnail 8192^2 no-alloc")
 
 
+@pytest.mark.parametrize("w,h,b", [(5845, 5551, 4), (5846, 5551, 4), (4321, 4355, 3), (2734, 4686, 3)])
+def test_thumbnail_unaligned_row_pitch(ctx, w, h, b):
+    # Row pitches that are not a multiple of 16 bytes: the single-image box
+    # shrink splits each output row's source rows over up to four wavefronts
+    # (16 wavefronts per SIMD) and sums their parts in LDS.
+    _say(f"thumbnail {w}x{h}x{b} (pitch % 16 = {w * b % 16}) -> FitBox(400, 400)")
+    src = gradient_noise_u8(w, h, b, 5000 + w)
+    p = v.Pipeline(ctx, w, h, b).thumbnail(v.FITBOX, 400, 400)
+    d_src = v.DeviceBuffer.from_numpy(ctx, src)
+    out = p.output_buffer()
+    p.run(d_src, out)
+    ctx.synchronize()
+    exp = O.Pipeline(w, h, b).thumbnail(O.FITBOX, 400, 400).run(src)
+    _same(out.numpy(), exp, f"thumbnail {w}x{h}x{b}")
+
+
 def test_png_thumbnail_box_shrink_8192x8192x3(ctx):
     # The PNG source: box pre-shrink by the hint (16) on the device, then the
     # thumbnail over the 512 x 512 image.

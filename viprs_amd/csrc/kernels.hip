@@ -1600,10 +1600,17 @@ bool plan_shrink_job(ShrinkJob* job, bool row_split, int run) {
     if (row_split) {
         // One image alone: keep about 16 wavefronts per CU busy before
         // lengthening runs, and split the source rows of an output row over
-        // S wavefronts while the row tasks leave room on the chip.
+        // S wavefronts while the row tasks leave room on the chip.  A source
+        // whose row pitch is not a multiple of 16 bytes gets up to 16
+        // wavefronts per SIMD instead of 4: its waves run the same
+        // instructions and fetch the same bytes, yet each row's loads take
+        // longer (measured: 5845x5551x4 at shrink 14 went from 39 to 27 us,
+        // the 16-byte-aligned 5844 wide image stays at 24 us either way), so
+        // it needs more rows in flight.
         while (m > 1 && waves(m) < 256 * 16) m = (m + 1) / 2;
+        const int64_t wave_cap = ((uint64_t)job->in.bw * bands) % 16 ? 256 * 4 * 16 : 256 * 4 * 4;
         if (m == 1)
-            while (S < 4 && 2 * S <= fv && waves(1) * 2 * S <= 256 * 4 * 4) S *= 2;
+            while (S < 4 && 2 * S <= fv && waves(1) * 2 * S <= wave_cap) S *= 2;
     }
     if (run > 0) {  // caller's run length: no row split
         m = std::min(run, 64);
