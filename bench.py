@@ -292,6 +292,39 @@ def bench_png(v, O, ctx, d_srcs, host_src, steps, warmup):
                            "hbm_frac": round(box_bytes / (box_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
+def bench_pitch(v, O, ctx, steps):
+    """Single-image thumbnails FitBox(400, 400) of sources whose row pitch is,
+    or is not, a multiple of 16 bytes (images of arbitrary width): the box
+    shrink stage timed over three device-filled sources cycled step by step
+    (beyond the last-level cache), its input bytes per second, and the whole
+    output of the first unaligned shape checked against the oracle."""
+    shapes = [(5844, 5551, 4), (5845, 5551, 4), (5846, 5551, 4), (4320, 4355, 3), (4321, 4355, 3),
+              (2734, 4686, 3)]
+    lines = []
+    exact = None
+    for (w, h, b) in shapes:
+        srcs = []
+        for k in range(3):
+            d = v.DeviceBuffer(ctx, (h, w, b), np.uint8)
+            d.fill_random(9100 + k)
+            srcs.append(d)
+        p = v.Pipeline(ctx, w, h, b).thumbnail(v.FITBOX, 400, 400, v.LANCZOS3)
+        dst = p.output_buffer()
+        t = p.time_stage_sources(srcs, dst, 0, max(steps, 20))
+        if exact is None and (w * b) % 16:
+            p.run(srcs[0], dst)
+            ctx.synchronize()
+            host = O.synthetic_bytes_fast(w * h * b, 9100).reshape(h, w, b)
+            exact = bool(np.array_equal(dst.numpy(), O.Pipeline(w, h, b).thumbnail(O.FITBOX, 400, 400).run(host)))
+        lines.append({"input": [w, h, b], "pitch_mod_16": (w * b) % 16, "shrink_ms": round(t, 5),
+                      "input_GB/s": round(w * h * b / (t / 1e3) / 1e9, 1),
+                      "hbm_frac": round(w * h * b / (t / 1e3) / 1e9 / HBM_PEAK_GBS, 4)})
+        del p, srcs
+    return {"workload": "thumbnail FitBox(400, 400) Lanczos3, uchar, sources of arbitrary width "
+                        "(box shrink stage, three cycled sources)",
+            "bit_exact_first_unaligned": exact, "shapes": lines}
+
+
 def only_workload(v, O, args):
     """One secondary workload alone (--only), for counter and trace runs:
     prints its JSON object."""
@@ -304,6 +337,8 @@ def only_workload(v, O, args):
                          "gauss_blur sigma=2.0 on 4096x4096x4 uchar")
     elif args.only == "batch":
         out = bench_batch(v, O, ctx, args.batch_images, max(args.steps, 20), args.warmup, 8)
+    elif args.only == "pitch":
+        out = bench_pitch(v, O, ctx, args.steps)
     else:  # png: four device-filled sources, parity on the first
         d_srcs = []
         for i in range(4):
@@ -389,7 +424,7 @@ def main():
     ap.add_argument("--no-float", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=2.0)
-    ap.add_argument("--only", choices=["float", "gauss", "batch", "png"],
+    ap.add_argument("--only", choices=["float", "gauss", "batch", "png", "pitch"],
                     help="run one secondary workload alone (counter and trace runs) and print its line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (counter calibration) and exit")
@@ -528,6 +563,9 @@ def main():
 
     print("bench: PNG-source thumbnail", file=sys.stderr, flush=True)
     result["png_thumbnail"] = bench_png(v, O, ctx, d_srcs, src, args.steps, args.warmup)
+    if not args.no_secondary:
+        print("bench: arbitrary-width sources", file=sys.stderr, flush=True)
+        result["row_pitch"] = bench_pitch(v, O, ctx, args.steps)
 
     if not args.no_batch and args.batch_images > 0:
         print("bench: uneven-size batch", file=sys.stderr, flush=True)
